@@ -1,0 +1,135 @@
+/*
+ * CPU ORACLE -- test infrastructure only.
+ *
+ * A plain-C restatement of HSSPACK's tridiagonal divide-and-conquer merge path
+ * (SRC/mdstedc.f90, mdlaed0.f90, mdlaed1.f90, mdlaed2.f90, mdlaed3.f90,
+ * CauchyHssEig.f90, aux_hss.f90, ConstructHssd.f90) together with the LAPACK
+ * routines that path calls (DLAED4, DLAED3, DLAMRG, DSTEQR).  It is the checker
+ * for the HIP product library and the CPU baseline timed next to it.  Nothing in
+ * the product links, loads or calls it; only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg do.
+ *
+ * Index arrays (INDXQ, INDX, INDXC, INDXP, COLTYP) hold 1-based values exactly as
+ * the Fortran routines do, so they compare directly with LAPACK's outputs.
+ * Matrices are column-major.
+ *
+ * Pinning (how each function is checked, see tests/test_oracle_*.py):
+ *   orc_dlamrg   -- LAPACK dlamrg on random / tied inputs (bit-exact index sets)
+ *   orc_laed4    -- LAPACK dlaed4 (roots and DELTA to a few ulps), K=1 and K=2 closed
+ *                   forms, interlacing, mpmath 50-digit roots
+ *   orc_steqr    -- LAPACK dsteqr (eigenvalues), orthogonality and residual
+ *   orc_laed2    -- LAPACK dlaed2 (K, DLAMDA, W, INDXP, INDXC, COLTYP, CTOT bit-exact;
+ *                   Q and Q2 to the rotation rounding)
+ *   orc_laed3    -- LAPACK dlaed3 (eigenvalues and vectors within 1e-13)
+ *   orc_mdlaed3  -- dense eig of diag(d)+rho*z*z' (the HSS product of the Cauchy
+ *                   matrix against the explicitly formed one)
+ *   orc_hss_*    -- dense expansion of the HSS form against the dense Cauchy-like
+ *                   matrix, ranks monotone in tol, identity rows at skeletons
+ *   orc_stedc    -- LAPACK dstedc (eigenvalues <= 4 n eps |T|; |Q'Q-I| and
+ *                   |TQ-QL|/|T| <= 1e-12)
+ */
+#ifndef HSDC_ORACLE_H
+#define HSDC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int smlsiz;   /* largest leaf of the DC tree, ILAENV(9,'DSTEDC') = 25 */
+    int bsmlz;    /* a merge with K >= bsmlz takes the HSS branch (mdlaed2.f90:175) */
+    int ni;       /* HSS leaf block size (mdlaed3.f90:253) */
+    double tol;   /* HSS compression tolerance (CauchyHssEig.f90:2523) */
+    int maxit;    /* secular iteration cap */
+} orc_params;
+
+/* per-merge trace filled by orc_stedc when non-NULL */
+typedef struct {
+    int cap, nmerge;
+    int *level, *n, *k, *hss, *rank_max;
+    long long roots, iters;
+} orc_trace;
+
+void orc_default_params(orc_params *p);
+
+/* LAPACK DLAMRG: stable merge permutation of two sorted runs (strides +-1). */
+void orc_dlamrg(int n1, int n2, const double *a, int s1, int s2, int *index);
+
+/* Secular root J (1-based) of 1 + rho*sum z_i^2/(d_i - lambda).  delta(i) = d(i)-lambda
+ * formed as (d(i)-d(org))-tau.  For n==2 delta holds the normalised eigenvector, as
+ * LAPACK's DLAED5 does.  Returns 0, or 1 if the iteration did not converge. */
+int orc_laed4(int n, int j, const double *d, const double *z, double *delta, double rho,
+              double *dlam, int *org, double *tau, int *niter, int maxit);
+
+/* Implicit-shift QL on a symmetric tridiagonal of order n, eigenvalues ascending and
+ * eigenvectors in z (ldz), z is overwritten with the eigenvector matrix. */
+int orc_steqr(int n, double *d, double *e, double *z, int ldz);
+/* Eigenvalues only, same iteration. */
+int orc_sterf(int n, double *d, double *e);
+
+/* MDLAED2 (mdlaed2.f90:149) == LAPACK DLAED2 when K < bsmlz.  *hflag=1 dense branch,
+ * 0 HSS branch.  Optional rotation log (may be NULL). */
+int orc_laed2(int *k, int n, int n1, double *d, double *q, int ldq, int *indxq, double *rho,
+              double *z, double *dlamda, double *w, double *q2, int *indx, int *indxc,
+              int *indxp, int *coltyp, int bsmlz, int *hflag,
+              int *nrot, int *rot_pj, int *rot_nj, double *rot_c, double *rot_s);
+
+/* LAPACK DLAED3 (dense branch called at mdlaed1.f90:199). iters (may be NULL) receives
+ * the total secular iterations. */
+int orc_laed3(int k, int n, int n1, double *d, double *q, int ldq, double rho,
+              double *dlamda, const double *q2, const int *indx, const int *ctot,
+              double *w, double *s, int maxit, long long *iters);
+
+/* MDLAED3 (mdlaed3.f90:124), HSS branch: secular roots, Loewner z, alpha, DHSSEVC. */
+int orc_mdlaed3(int k, int n, double *d, double *q, int ldq, double rho, double *dlamda,
+                double *q2, double *z, double *difl, double *difr, double *alpha,
+                const orc_params *p, int *rank_max, long long *iters);
+
+/* The secular / Loewner / alpha half of MDLAED3 alone (mdlaed3.f90:201-249). */
+int orc_mdlaed3_vectors(int k, double *d, double rho, double *dlamda, double *z, double *difl,
+                        double *difr, double *alpha, int maxit, long long *iters);
+
+/* MDLAED1 (mdlaed1.f90:109) */
+int orc_laed1(int n, double *d, double *q, int ldq, int *indxq, double rho, int cutpnt,
+              double *work, int *iwork, const orc_params *p, int *kout, int *hssout,
+              int *rank_max, long long *iters);
+
+/* MDLAED0 with ICOMPQ=2 (mdlaed0.f90:121) */
+int orc_laed0(int n, double *d, double *e, double *q, int ldq, double *work, int *iwork,
+              const orc_params *p, orc_trace *tr);
+
+/* MDSTEDC (mdstedc.f90:139).  compz 'N','I' or 'V'.  lwork/liwork == -1 is a query. */
+int orc_stedc(char compz, int n, double *d, double *e, double *z, int ldz, double *work,
+              int lwork, int *iwork, int liwork, const orc_params *p, orc_trace *tr);
+
+/* ---- HSS (CauchyHssEig.f90 / aux_hss.f90 / ConstructHssd.f90) ---- */
+typedef struct {
+    int k, ltr, lm, lvl;
+    int *tr, *m;            /* post-ordered parent array (ltr), leaf sizes (lm) */
+    int *ch;                /* children, 2*ltr (0 = none) */
+    int *lentr, *ttr;       /* level lists: lentr[lvl], ttr[(LL-1)*ltr + i] */
+    int *dsz, *usz, *vsz, *bsz;  /* D size, U/V/B dims (2 per node) */
+    long *pu, *pv, *pb, *pd;     /* offsets into h and dd (0-based) */
+    double *h, *dd;
+    long nh, ndd;
+    long long nflops, nswap;
+} orc_hss;
+
+/* DHSSEVC construction half: HSS approximation of A(i,j)=u(i)v(j)/(f(i)-d(j)). */
+int orc_hss_build(orc_hss *H, int k, int ni, const double *f, const double *d0,
+                  const double *u, const double *v, const double *difl, const double *difr,
+                  double tol);
+/* fastHssmmL: X(ldx x K, rows nrow) := X * H */
+int orc_hss_mmL(const orc_hss *H, double *x, int ldx, int nrow);
+/* Dense expansion of H into A (k x k, lda) for checks. */
+void orc_hss_dense(const orc_hss *H, double *a, int lda);
+void orc_hss_free(orc_hss *H);
+
+/* Cauchy fill with accurate differences (CauchylikeEig, CauchyHssEig.f90:1688). */
+void orc_cauchy_dense(int k, const double *f, const double *u, const double *v,
+                      const double *difl, const double *difr, double *a, int lda);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,872 @@
+/*
+ * CPU ORACLE -- test infrastructure only (see orc.h).
+ *
+ * Restatement of the eigenvector Cauchy->HSS construction and the HSS x dense
+ * multiply of HSSPACK:
+ *   tree        npart/n2tree/btree/child/hsslevel/GetTreeInfo   aux_hss.f90:238-827
+ *   DHSSEVC     construction half                                CauchyHssEig.f90:2444-2583
+ *   Cauchy2hssEig_omp (run sequentially, node order = level list) CauchyHssEig.f90:1945-2250
+ *   ComprcauchyEig, rrluCauchyEigrow/col, CauchyPivtEig(m)(_CP),
+ *   searchMax2Eig(m), CauchyMaxEig*, computdd/computfd,
+ *   CauchylikeEig(Q)(Q2), invp                                   CauchyHssEig.f90:7-1942, 2253-2440
+ *   fastHssmmL_omp                                               ConstructHssd.f90:1394-1630
+ *
+ * Arrays are addressed 1-based through the A1() macro to keep the statement
+ * order of the Fortran.  One deliberate difference: CauchyPivtEig reads the
+ * current pivot as F(PL(k)) with the node-local pole array F indexed by a global
+ * index (CauchyHssEig.f90:665), which reads outside the node's poles; the global
+ * pole array FF(PL(k)) is used here, as in the column variant (:821-825).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include "orc.h"
+
+#define A1(a, i) (a)[(i) - 1]
+
+/* ---------------- tree ---------------- */
+static void btree(int n, int *tr)
+{
+    if (n > 3) {
+        int m = (n - 1) / 2;
+        btree(m, tr);
+        for (int i = 1; i <= m; i++) A1(tr, m + i) = A1(tr, i) + m;
+        A1(tr, m) = n;
+        A1(tr, m + m) = n;
+        A1(tr, n) = 0;
+    } else {
+        A1(tr, 1) = 3; A1(tr, 2) = 3; A1(tr, 3) = 0;
+    }
+}
+
+static void n2tree(int n, int *tr)
+{
+    if (n == 1) { A1(tr, 1) = 0; return; }
+    int n1 = (1 << (int)floorf(logf((float)n) / logf(2.0f))) - 1;
+    btree(n1, tr);
+    n2tree(n - n1 - 1, tr + n1);
+    for (int i = n1 + 1; i <= n - 1; i++) A1(tr, i) += n1;
+    A1(tr, n1) = n;
+    A1(tr, n - 1) = n;
+    A1(tr, n) = 0;
+}
+
+static void child(const int *tr, int *ch, int n)
+{
+    for (int i = 1; i <= n; i++) { ch[2 * (i - 1)] = 0; ch[2 * (i - 1) + 1] = 0; }
+    for (int i = 1; i <= n - 1; i++) {
+        int p = A1(tr, i);
+        if (ch[2 * (p - 1)] == 0) ch[2 * (p - 1)] = i;
+        else ch[2 * (p - 1) + 1] = i;
+    }
+}
+#define CH1(ch, i) (ch)[2 * ((i) - 1)]
+#define CH2(ch, i) (ch)[2 * ((i) - 1) + 1]
+
+/* npart (aux_hss.f90:803) with ltr = 0 on entry */
+static void npart(int n, int ni, int *tr, int *ltr, int *m, int *lm)
+{
+    int l = n / ni;
+    for (int i = 1; i <= l; i++) A1(m, i) = ni;
+    int mr = n % ni;
+    if (mr >= ni / 2) { l++; A1(m, l) = mr; }
+    else A1(m, l) += mr;
+    *lm = l;
+    *ltr = 2 * l - 1;
+    n2tree(*ltr, tr);
+}
+
+/* ---------------- accurate differences ---------------- */
+/* F(j)-D(k) = f_{PL(j)} - lambda_{PU(k)}  (computfd, CauchyHssEig.f90:519) */
+static inline double computfd(const double *FF, int j, int k, const int *PL, const int *PU,
+                              const double *DIFL, const double *DIFR)
+{
+    int pd = A1(PL, j), pw = A1(PU, k);
+    if (pd <= pw) return (A1(FF, pd) - A1(FF, pw)) - A1(DIFL, pw);
+    return (A1(FF, pd) - A1(FF, pw + 1)) - A1(DIFR, pw);
+}
+/* D(j)-D(k) = lambda_{PU(j)} - lambda_{PU(k)}  (computdd, CauchyHssEig.f90:470) */
+static inline double computdd(const double *FF, int j, int k, const int *PU,
+                              const double *DIFL, const double *DIFR)
+{
+    int pj = A1(PU, j), pk = A1(PU, k);
+    if (pj < pk) return ((A1(FF, pj + 1) - A1(FF, pk)) + A1(DIFR, pj)) - A1(DIFL, pk);
+    return ((A1(FF, pj) - A1(FF, pk + 1)) + A1(DIFL, pj)) - A1(DIFR, pk);
+}
+
+static inline void dswp(double *a, int i, int j) { double t = A1(a, i); A1(a, i) = A1(a, j); A1(a, j) = t; }
+static inline void iswp(int *a, int i, int j) { int t = A1(a, i); A1(a, i) = A1(a, j); A1(a, j) = t; }
+
+/* max |.| and first location (MAXVAL/MAXLOC); n == 0 gives -huge, 0 */
+#define MAXLOC_UPDATE(val, idx)                                   \
+    do {                                                          \
+        double _v = fabs(val);                                    \
+        if (first || _v > best) { best = _v; loc = (idx); first = 0; } \
+    } while (0)
+
+/* CauchyMaxEigmc: column entries U(i)/(f_{PL(i)} - lambda_{PU(1)}) */
+static void maxEigmc(const double *FF, const int *PL, const int *PU, const double *DIFL,
+                     const double *DIFR, const double *U, int n, double *junk, int *jj)
+{
+    double best = -DBL_MAX; int loc = 0, first = 1;
+    int dk = A1(PU, 1);
+    for (int i = 1; i <= n; i++) {
+        int pli = A1(PL, i);
+        double v;
+        if (pli <= dk) v = A1(U, i) / ((A1(FF, pli) - A1(FF, dk)) - A1(DIFL, dk));
+        else v = A1(U, i) / ((A1(FF, pli) - A1(FF, dk + 1)) - A1(DIFR, dk));
+        MAXLOC_UPDATE(v, i);
+    }
+    *junk = best; *jj = loc;
+}
+/* CauchyMaxEigm: row entries V(i)/(f_{Fk} - lambda_{PU(i)}) */
+static void maxEigm(const double *FF, int fk, const int *PU, const double *DIFL,
+                    const double *DIFR, const double *V, int n, double *junk, int *jj)
+{
+    double best = -DBL_MAX; int loc = 0, first = 1;
+    double f = A1(FF, fk);
+    for (int i = 1; i <= n; i++) {
+        int pui = A1(PU, i);
+        double v;
+        if (fk <= pui) v = A1(V, i) / ((f - A1(FF, pui)) - A1(DIFL, pui));
+        else v = A1(V, i) / ((f - A1(FF, pui + 1)) - A1(DIFR, pui));
+        MAXLOC_UPDATE(v, i);
+    }
+    *junk = best; *jj = loc;
+}
+/* CauchyMaxEig: U(i)/(D(i)-F) */
+static void maxEig(const double *D, double f, const double *U, int n, double *junk, int *jj)
+{
+    double best = -DBL_MAX; int loc = 0, first = 1;
+    for (int i = 1; i <= n; i++) {
+        double v = A1(U, i) / (A1(D, i) - f);
+        MAXLOC_UPDATE(v, i);
+    }
+    *junk = best; *jj = loc;
+}
+/* CauchyMaxEigmm: U(i)/(D(i+N)-D(k)) via computdd */
+static void maxEigmm(const double *FF, const int *PL, int k, const double *DIFL,
+                     const double *DIFR, const double *U, int m, int n, double *junk, int *jj)
+{
+    double best = -DBL_MAX; int loc = 0, first = 1;
+    for (int i = 1; i <= m; i++) {
+        double tt = computdd(FF, i + n, k, PL, DIFL, DIFR);
+        double v = A1(U, i) / tt;
+        MAXLOC_UPDATE(v, i);
+    }
+    *junk = best; *jj = loc;
+}
+
+/* CauchyPivtEig (CauchyHssEig.f90:571): rook-style pivot for the row RRLU */
+static void pivtEig(double *F, double *D, double *U, double *V, int *PL, int *PU, int k,
+                    double *A, int M, int N, int *PQ, const double *FF, const double *DIFL,
+                    const double *DIFR)
+{
+    double junkL, junkU, piv;
+    int jjL, jjU;
+    maxEigmc(FF, PL + k - 1, PU + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &junkL, &jjL);
+    maxEigm(FF, A1(PL, k), PU + k - 1, DIFL, DIFR, V + k - 1, N - k + 1, &junkU, &jjU);
+    junkL = junkL * fabs(A1(V, k));
+    junkU = junkU * fabs(A1(U, k));
+    if (A1(PL, k) <= A1(PU, k))
+        piv = fabs(A1(U, k) * A1(V, k) / ((A1(FF, A1(PL, k)) - A1(FF, A1(PU, k))) - A1(DIFL, A1(PU, k))));
+    else
+        piv = fabs(A1(U, k) * A1(V, k) / ((A1(FF, A1(PL, k)) - A1(FF, A1(PU, k) + 1)) - A1(DIFR, A1(PU, k))));
+    if (junkL <= piv && junkU <= piv) return;
+    int flg = (junkL > junkU) ? 1 : 0;
+    for (;;) {
+        if (flg == 1) {
+            int jj = jjL;
+            dswp(F, k, jj + k - 1); dswp(U, k, jj + k - 1); dswp(A, k, jj + k - 1);
+            iswp(PL, k, jj + k - 1); iswp(PQ, k, jj + k - 1);
+            maxEigm(FF, A1(PL, k), PU + k - 1, DIFL, DIFR, V + k - 1, N - k + 1, &junkU, &jjU);
+            if (jjU == 1) return;
+            flg = 0;
+        }
+        int jj = jjU;
+        dswp(D, k, jj + k - 1); dswp(V, k, jj + k - 1); iswp(PU, k, jj + k - 1);
+        maxEigmc(FF, PL + k - 1, PU + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &junkL, &jjL);
+        if (jjL == 1) return;
+        flg = 1;
+    }
+}
+
+/* CauchyPivtEig_CP (CauchyHssEig.f90:1049): complete pivoting on the Schur complement */
+static void pivtEigCP(double *F, double *D, double *U, double *V, int *PL, int *PU, int k,
+                      double *A, int M, int N, int *PQ, const double *FF, const double *DIFL,
+                      const double *DIFR)
+{
+    double ju, junk;
+    int jju, ii, jj = 1;
+    maxEigmc(FF, PL + k - 1, PU + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &ju, &jju);
+    junk = fabs(ju * A1(V, k));
+    ii = jju;
+    for (int j = 2; j <= N - k + 1; j++) {
+        maxEigmc(FF, PL + k - 1, PU + k + j - 2, DIFL, DIFR, U + k - 1, M - k + 1, &ju, &jju);
+        double jt = fabs(ju * A1(V, k + j - 1));
+        if (jt > junk) { junk = jt; ii = jju; jj = j; }
+    }
+    dswp(F, k, ii + k - 1); dswp(U, k, ii + k - 1); dswp(A, k, ii + k - 1);
+    iswp(PL, k, ii + k - 1); iswp(PQ, k, ii + k - 1);
+    dswp(D, k, jj + k - 1); dswp(V, k, jj + k - 1); iswp(PU, k, jj + k - 1);
+}
+
+/* CauchyPivtEigm (CauchyHssEig.f90:737): rook-style pivot for the column RRLU */
+static void pivtEigm(double *D, double *F, double *U, double *V, int *PL, int *PU, int k,
+                     double *A, int M, int N, int *PQ, const double *FF, const double *DIFL,
+                     const double *DIFR)
+{
+    double junkL, junkU, piv;
+    int jjL, jjU;
+    maxEigm(FF, A1(PU, k), PL + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &junkL, &jjL);
+    maxEigmc(FF, PU + k - 1, PL + k - 1, DIFL, DIFR, V + k - 1, N - k + 1, &junkU, &jjU);
+    junkL = junkL * fabs(A1(V, k));
+    junkU = junkU * fabs(A1(U, k));
+    if (A1(PL, k) < A1(PU, k))
+        piv = fabs(A1(U, k) * A1(V, k) / ((A1(FF, A1(PL, k) + 1) - A1(FF, A1(PU, k))) + A1(DIFR, A1(PL, k))));
+    else
+        piv = fabs(A1(U, k) * A1(V, k) / ((A1(FF, A1(PL, k)) - A1(FF, A1(PU, k))) + A1(DIFL, A1(PL, k))));
+    if (junkL <= piv && junkU <= piv) return;
+    int flg = (junkL > junkU) ? 1 : 0;
+    for (;;) {
+        if (flg == 1) {
+            int jj = jjL;
+            dswp(D, k, jj + k - 1); dswp(U, k, jj + k - 1); dswp(A, k, jj + k - 1);
+            iswp(PL, k, jj + k - 1); iswp(PQ, k, jj + k - 1);
+            maxEigmc(FF, PU + k - 1, PL + k - 1, DIFL, DIFR, V + k - 1, N - k + 1, &junkU, &jjU);
+            if (jjU == 1) return;
+            flg = 0;
+        }
+        int jj = jjU;
+        dswp(F, k, jj + k - 1); dswp(V, k, jj + k - 1); iswp(PU, k, jj + k - 1);
+        maxEigm(FF, A1(PU, k), PL + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &junkL, &jjL);
+        if (jjL == 1) return;
+        flg = 1;
+    }
+}
+
+/* CauchyPivtEigm_CP (CauchyHssEig.f90:861) */
+static void pivtEigmCP(double *D, double *F, double *U, double *V, int *PL, int *PU, int k,
+                       double *A, int M, int N, int *PQ, const double *FF, const double *DIFL,
+                       const double *DIFR)
+{
+    double ju, junk;
+    int jju, ii, jj = 1;
+    maxEigm(FF, A1(PU, k), PL + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &ju, &jju);
+    junk = fabs(ju * A1(V, k));
+    ii = jju;
+    for (int j = 2; j <= N - k + 1; j++) {
+        maxEigm(FF, A1(PU, k + j - 1), PL + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &ju, &jju);
+        double jt = fabs(ju * A1(V, k + j - 1));
+        if (jt > junk) { junk = jt; ii = jju; jj = j; }
+    }
+    dswp(D, k, ii + k - 1); dswp(U, k, ii + k - 1); dswp(A, k, ii + k - 1);
+    iswp(PL, k, ii + k - 1); iswp(PQ, k, ii + k - 1);
+    dswp(F, k, jj + k - 1); dswp(V, k, jj + k - 1); iswp(PU, k, jj + k - 1);
+}
+
+/* rrluCauchyEigrow (CauchyHssEig.f90:7-212) */
+static void rrluRow(double *F, double *D, double *A, double *V, const double *DIFL,
+                    const double *DIFR, double tol, double *U, double *W, const double *FF,
+                    int *PL, int *PU, int M, int N, int *Rk, int *PQ, long long *nflops,
+                    long long *nswap)
+{
+    int mn = M < N ? M : N, k, j, ii, jj;
+    double amax = 0.0, rho = 1.5, ukk, temp, temp2, nmax;
+    const int prd = 10;
+    for (j = 1; j <= M; j++) { A1(U, j) = A1(A, j); A1(PQ, j) = j; }
+    *Rk = mn;
+    for (k = 1; k <= mn; k++) {
+        pivtEig(F, D, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR);
+        temp = computfd(FF, k, k, PL, PU, DIFL, DIFR);
+        ukk = A1(U, k) * A1(V, k) / temp;
+        amax = fmax(amax, fabs(ukk));
+        if (fabs(ukk) < tol * amax) {
+            pivtEigCP(F, D, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR);
+            temp = computfd(FF, k, k, PL, PU, DIFL, DIFR);
+            ukk = A1(U, k) * A1(V, k) / temp;
+            amax = fmax(amax, fabs(ukk));
+            if (fabs(ukk) < tol * amax) { *Rk = k - 1; break; }
+        }
+        for (j = k + 1; j <= M; j++) {
+            temp = computfd(FF, j, k, PL, PU, DIFL, DIFR);
+            temp = (A1(F, j) - A1(F, k)) / temp;
+            A1(U, j) = A1(U, j) * temp;
+        }
+        for (j = k + 1; j <= N; j++) {
+            temp = computdd(FF, j, k, PU, DIFL, DIFR);
+            temp2 = -1.0 * computfd(FF, k, j, PL, PU, DIFL, DIFR);
+            temp = temp / temp2;
+            A1(V, j) = A1(V, j) * temp;
+        }
+        for (j = 1; j <= k - 1; j++) {
+            temp = -1.0 * computfd(FF, j, k, PL, PU, DIFL, DIFR);
+            A1(W, j) = A1(W, j) * (temp / (A1(F, k) - A1(F, j)));
+        }
+        temp = computfd(FF, k, k, PL, PU, DIFL, DIFR);
+        A1(W, k) = temp / A1(A, k);
+        for (j = 1; j <= k - 1; j++) {
+            temp = -1.0 * computfd(FF, k, j, PL, PU, DIFL, DIFR);
+            A1(W, k) = A1(W, k) * (temp / (A1(F, j) - A1(F, k)));
+        }
+        *nflops += 5 * (M - k) + 7 * (N - k) + 2 * 6 * (k - 1);
+        int flgl = k % prd;
+        while (flgl < 1) {
+            flgl = 1;
+            /* searchMax2Eig(U(k+1),W,F(k+1),F,M-k,k,...) */
+            double junk; int jjL;
+            maxEig(F + k, A1(F, 1), U + k, M - k, &junk, &jjL);
+            junk = junk * fabs(A1(W, 1));
+            nmax = junk; ii = 1; jj = jjL;
+            for (j = 2; j <= k; j++) {
+                maxEig(F + k, A1(F, j), U + k, M - k, &junk, &jjL);
+                junk = junk * fabs(A1(W, j));
+                if (junk > nmax) { nmax = junk; ii = j; jj = jjL; }
+            }
+            if (nmax > rho) {
+                (*nswap)++;
+                flgl = 0;
+                jj = jj + k;
+                for (j = k + 1; j <= N; j++) {
+                    temp = -1.0 * computfd(FF, ii, j, PL, PU, DIFL, DIFR);
+                    temp2 = -1.0 * computfd(FF, jj, j, PL, PU, DIFL, DIFR);
+                    temp = temp / temp2;
+                    A1(V, j) = A1(V, j) * temp;
+                }
+                for (j = k + 1; j <= jj - 1; j++)
+                    A1(U, j) = A1(U, j) * ((A1(F, j) - A1(F, jj)) / (A1(F, j) - A1(F, ii)));
+                for (j = jj + 1; j <= M; j++)
+                    A1(U, j) = A1(U, j) * ((A1(F, j) - A1(F, jj)) / (A1(F, j) - A1(F, ii)));
+                for (j = 1; j <= ii - 1; j++)
+                    A1(W, j) = A1(W, j) * ((A1(F, j) - A1(F, ii)) / (A1(F, j) - A1(F, jj)));
+                for (j = ii + 1; j <= k; j++)
+                    A1(W, j) = A1(W, j) * ((A1(F, j) - A1(F, ii)) / (A1(F, j) - A1(F, jj)));
+                temp = computfd(FF, ii, ii, PL, PU, DIFL, DIFR);
+                A1(U, jj) = A1(A, ii) * ((A1(F, ii) - A1(F, jj)) / temp);
+                temp = computfd(FF, jj, ii, PL, PU, DIFL, DIFR);
+                A1(W, ii) = temp / A1(A, jj);
+                for (j = 1; j <= k; j++) {
+                    if (j == ii) continue;
+                    temp = computfd(FF, ii, j, PL, PU, DIFL, DIFR);
+                    A1(U, jj) = A1(U, jj) * ((A1(F, ii) - A1(F, j)) / temp);
+                    temp = -1.0 * computfd(FF, jj, j, PL, PU, DIFL, DIFR);
+                    A1(W, ii) = A1(W, ii) * (temp / (A1(F, j) - A1(F, jj)));
+                }
+                iswp(PL, ii, jj); iswp(PQ, ii, jj);
+                dswp(F, ii, jj); dswp(A, ii, jj);
+                *nflops += 10 * (N - k) + 8 * M + 18 * k;
+            }
+        }
+    }
+}
+
+/* rrluCauchyEigcol (CauchyHssEig.f90:215-467) */
+static void rrluCol(double *D, double *F, double *A, double *V, const double *DIFL,
+                    const double *DIFR, double tol, double *U, double *W, const double *FF,
+                    int *PL, int *PU, int M, int N, int *Rk, int *PQ, long long *nflops,
+                    long long *nswap)
+{
+    int mn = M < N ? M : N, k, j, ii, jj;
+    double amax = 0.0, rho = 1.5, ukk, temp, temp2, nmax;
+    const int prd = 10;
+    for (j = 1; j <= M; j++) { A1(U, j) = A1(A, j); A1(PQ, j) = j; }
+    *Rk = mn;
+    for (k = 1; k <= mn; k++) {
+        pivtEigm(D, F, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR);
+        temp = -1.0 * computfd(FF, k, k, PU, PL, DIFL, DIFR);
+        ukk = A1(U, k) * A1(V, k) / temp;
+        amax = fmax(amax, fabs(ukk));
+        if (fabs(ukk) < tol * amax) {
+            pivtEigmCP(D, F, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR);
+            temp = -1.0 * computfd(FF, k, k, PU, PL, DIFL, DIFR);
+            ukk = A1(U, k) * A1(V, k) / temp;
+            amax = fmax(amax, fabs(ukk));
+            if (fabs(ukk) < tol * amax) { *Rk = k - 1; break; }
+        }
+        for (j = k + 1; j <= M; j++) {
+            temp = computdd(FF, j, k, PL, DIFL, DIFR);
+            temp2 = -1.0 * computfd(FF, k, j, PU, PL, DIFL, DIFR);
+            temp = temp / temp2;
+            A1(U, j) = A1(U, j) * temp;
+        }
+        for (j = k + 1; j <= N; j++) {
+            temp = computfd(FF, j, k, PU, PL, DIFL, DIFR);
+            A1(V, j) = A1(V, j) * ((A1(F, j) - A1(F, k)) / temp);
+        }
+        for (j = 1; j <= k - 1; j++) {
+            temp = computfd(FF, k, j, PU, PL, DIFL, DIFR);
+            temp2 = computdd(FF, k, j, PL, DIFL, DIFR);
+            temp = temp / temp2;
+            A1(W, j) = A1(W, j) * temp;
+        }
+        temp = -1.0 * computfd(FF, k, k, PU, PL, DIFL, DIFR);
+        A1(W, k) = temp / A1(A, k);
+        for (j = 1; j <= k - 1; j++) {
+            temp = computfd(FF, j, k, PU, PL, DIFL, DIFR);
+            temp2 = computdd(FF, j, k, PL, DIFL, DIFR);
+            temp = temp / temp2;
+            A1(W, k) = A1(W, k) * temp;
+        }
+        *nflops += 6 * (M - k) + 5 * (N - k) + 2 * 7 * (k - 1);
+        int flgl = k % prd;
+        while (flgl < 1) {
+            flgl = 1;
+            double junk; int jjL;
+            maxEigmm(FF, PL, 1, DIFL, DIFR, U + k, M - k, k, &junk, &jjL);
+            junk = junk * fabs(A1(W, 1));
+            nmax = junk; ii = 1; jj = jjL;
+            for (j = 2; j <= k; j++) {
+                maxEigmm(FF, PL, j, DIFL, DIFR, U + k, M - k, k, &junk, &jjL);
+                junk = junk * fabs(A1(W, j));
+                if (junk > nmax) { nmax = junk; ii = j; jj = jjL; }
+            }
+            if (nmax > rho) {
+                (*nswap)++;
+                flgl = 0;
+                jj = jj + k;
+                for (j = k + 1; j <= N; j++) {
+                    temp = computfd(FF, j, ii, PU, PL, DIFL, DIFR);
+                    temp2 = computfd(FF, j, jj, PU, PL, DIFL, DIFR);
+                    temp = temp / temp2;
+                    A1(V, j) = A1(V, j) * temp;
+                }
+                for (j = k + 1; j <= M; j++) {
+                    if (j == jj) continue;
+                    temp = computdd(FF, j, jj, PL, DIFL, DIFR);
+                    temp2 = computdd(FF, j, ii, PL, DIFL, DIFR);
+                    temp = temp / temp2;
+                    A1(U, j) = A1(U, j) * temp;
+                }
+                for (j = 1; j <= k; j++) {
+                    if (j == ii) continue;
+                    temp = computdd(FF, j, ii, PL, DIFL, DIFR);
+                    temp2 = computdd(FF, j, jj, PL, DIFL, DIFR);
+                    temp = temp / temp2;
+                    A1(W, j) = A1(W, j) * temp;
+                }
+                temp = computdd(FF, ii, jj, PL, DIFL, DIFR);
+                temp2 = -1.0 * computfd(FF, ii, ii, PU, PL, DIFL, DIFR);
+                temp = temp / temp2;
+                A1(U, jj) = A1(A, ii) * temp;
+                temp = -1.0 * computfd(FF, ii, jj, PU, PL, DIFL, DIFR);
+                A1(W, ii) = temp / A1(A, jj);
+                for (j = 1; j <= k; j++) {
+                    if (j == ii) continue;
+                    temp = computdd(FF, ii, j, PL, DIFL, DIFR);
+                    temp2 = -1.0 * computfd(FF, j, ii, PU, PL, DIFL, DIFR);
+                    temp = temp / temp2;
+                    A1(U, jj) = A1(U, jj) * temp;
+                    temp = computfd(FF, j, jj, PU, PL, DIFL, DIFR);
+                    temp2 = computdd(FF, j, jj, PL, DIFL, DIFR);
+                    temp = temp / temp2;
+                    A1(W, ii) = A1(W, ii) * temp;
+                }
+                iswp(PL, ii, jj); iswp(PQ, ii, jj);
+                dswp(D, ii, jj); dswp(A, ii, jj);
+                *nflops += 10 * (N - k) + 12 * M + 22 * k;
+            }
+        }
+    }
+}
+
+static void invp(int *P, int n)
+{
+    int *ip = (int *)calloc(n, sizeof(int));
+    for (int i = 1; i <= n; i++) A1(ip, A1(P, i)) = i;
+    memcpy(P, ip, sizeof(int) * n);
+    free(ip);
+}
+
+/* CauchylikeEig (CauchyHssEig.f90:1688): A(i,j) = U(i)V(j)/(f_{PL(i)} - lambda_{PU(j)}) */
+static void cauchylikeEig(double *a, int lda, const double *F, const double *U, const double *V,
+                          const double *DIFL, const double *DIFR, const int *PL, const int *PU,
+                          int M, int N)
+{
+    for (int j = 1; j <= N; j++) {
+        int pdjj = A1(PU, j);
+        double fj = A1(F, pdjj), vj = A1(V, j);
+        for (int i = 1; i <= M; i++) {
+            int pfii = A1(PL, i);
+            double den;
+            if (pfii <= pdjj) den = (A1(F, pfii) - fj) - A1(DIFL, pdjj);
+            else den = (A1(F, pfii) - A1(F, pdjj + 1)) - A1(DIFR, pdjj);
+            a[(i - 1) + (long)(j - 1) * lda] = A1(U, i) * vj / den;
+        }
+    }
+}
+
+/* ---------------- HSS storage ---------------- */
+static long push(orc_hss *H, long *cap, long n)
+{
+    long off = H->nh;
+    if (H->nh + n > *cap) {
+        while (H->nh + n > *cap) *cap = *cap * 2 + 1024;
+        H->h = (double *)realloc(H->h, sizeof(double) * (*cap));
+    }
+    H->nh += n;
+    return off;
+}
+
+/* ComprcauchyEig (CauchyHssEig.f90:2253-2440) */
+static void comprEig(orc_hss *H, long *cap, char rc, double *F, double *D, double *U, double *V,
+                     const double *DIFL, const double *DIFR, const double *FF, int *PL, int *PU,
+                     double tol, int M, int N, int nodi, int *Rk)
+{
+    int mn = M < N ? M : N, i, j;
+    if (rc == 'r') {
+        double *Z = (double *)calloc(M + 1, sizeof(double)), *W = (double *)calloc(mn + 1, sizeof(double));
+        int *PQ = (int *)malloc(sizeof(int) * (M + 1));
+        rrluRow(F, D, U, V, DIFL, DIFR, tol, Z, W, FF, PL, PU, M, N, Rk, PQ, &H->nflops, &H->nswap);
+        invp(PQ, M);
+        int r = *Rk;
+        if (r < mn) {
+            double *Q = (double *)calloc((long)M * (r > 0 ? r : 1), sizeof(double));
+            /* CauchylikeEigQ(Q, F(Rk+1), F, Z(Rk+1), W, M, Rk) */
+            for (j = 1; j <= r; j++) {
+                Q[(j - 1) + (long)(j - 1) * M] = 1.0;
+                for (i = r + 1; i <= M; i++)
+                    Q[(i - 1) + (long)(j - 1) * M] = A1(Z, i) * A1(W, j) / (A1(F, i) - A1(F, j));
+            }
+            long off = push(H, cap, (long)M * r);
+            for (j = 0; j < r; j++)
+                for (i = 1; i <= M; i++) H->h[off + (i - 1) + (long)j * M] = Q[(A1(PQ, i) - 1) + (long)j * M];
+            H->usz[2 * (nodi - 1)] = M; H->usz[2 * (nodi - 1) + 1] = r; H->pu[nodi - 1] = off;
+            free(Q);
+        } else {
+            *Rk = M;
+            long off = push(H, cap, (long)M * M);
+            for (j = 0; j < M; j++)
+                for (i = 1; i <= M; i++) H->h[off + (i - 1) + (long)j * M] = (A1(PQ, i) - 1 == j) ? 1.0 : 0.0;
+            H->usz[2 * (nodi - 1)] = M; H->usz[2 * (nodi - 1) + 1] = M; H->pu[nodi - 1] = off;
+        }
+        free(Z); free(W); free(PQ);
+    } else {
+        /* column compression: the rows of the RRLU are this node's N columns */
+        double *Z = (double *)calloc(N + 1, sizeof(double)), *W = (double *)calloc(mn + 1, sizeof(double));
+        int *PQ = (int *)malloc(sizeof(int) * (N + 1));
+        rrluCol(D, F, V, U, DIFL, DIFR, tol, Z, W, FF, PU, PL, N, M, Rk, PQ, &H->nflops, &H->nswap);
+        invp(PQ, N);
+        int r = *Rk;
+        if (r < mn) {
+            double *Q = (double *)calloc((long)(r > 0 ? r : 1) * N, sizeof(double));
+            /* CauchylikeEigQ2(Q, D, D(Rk+1), W, Z(Rk+1), Rk, N, FF, DIFL, DIFR, PU) */
+            for (j = 1; j <= r; j++) Q[(j - 1) + (long)(j - 1) * r] = 1.0;
+            for (j = r + 1; j <= N; j++) {
+                int pdjj = A1(PU, j);
+                for (i = 1; i <= r; i++) {
+                    int pdii = A1(PU, i);
+                    double den;
+                    if (pdii < pdjj) den = (A1(FF, pdii + 1) - A1(FF, pdjj)) + (A1(DIFR, pdii) - A1(DIFL, pdjj));
+                    else den = (A1(FF, pdii) - A1(FF, pdjj + 1)) + (A1(DIFL, pdii) - A1(DIFR, pdjj));
+                    Q[(i - 1) + (long)(j - 1) * r] = -1.0 * A1(W, i) * A1(Z, j) / den;
+                }
+            }
+            long off = push(H, cap, (long)r * N);
+            for (j = 1; j <= N; j++)
+                for (i = 0; i < r; i++) H->h[off + i + (long)(j - 1) * r] = Q[i + (long)(A1(PQ, j) - 1) * r];
+            H->vsz[2 * (nodi - 1)] = r; H->vsz[2 * (nodi - 1) + 1] = N; H->pv[nodi - 1] = off;
+            free(Q);
+        } else {
+            *Rk = N;
+            long off = push(H, cap, (long)N * N);
+            for (j = 1; j <= N; j++)
+                for (i = 0; i < N; i++) H->h[off + i + (long)(j - 1) * N] = (A1(PQ, j) - 1 == i) ? 1.0 : 0.0;
+            H->vsz[2 * (nodi - 1)] = N; H->vsz[2 * (nodi - 1) + 1] = N; H->pv[nodi - 1] = off;
+        }
+        free(Z); free(W); free(PQ);
+    }
+}
+
+void orc_hss_free(orc_hss *H)
+{
+    free(H->tr); free(H->m); free(H->ch); free(H->lentr); free(H->ttr);
+    free(H->dsz); free(H->usz); free(H->vsz); free(H->bsz);
+    free(H->pu); free(H->pv); free(H->pb); free(H->pd);
+    free(H->h); free(H->dd);
+    memset(H, 0, sizeof(*H));
+}
+
+#define TTR(H, LL, i) (H)->ttr[((LL) - 1) * (H)->ltr + (i) - 1]
+
+/* DHSSEVC construction + Cauchy2hssEig_omp, sequential */
+int orc_hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, const double *u0,
+                  const double *v0, const double *DIFL, const double *DIFR, double tol)
+{
+    memset(H, 0, sizeof(*H));
+    H->k = K;
+    int lm0 = K / ni + 1;
+    H->m = (int *)calloc(lm0 + 2, sizeof(int));
+    H->tr = (int *)calloc(2 * lm0 + 8, sizeof(int));
+    int ltr, lm;
+    npart(K, ni, H->tr, &ltr, H->m, &lm);
+    H->ltr = ltr; H->lm = lm;
+    H->ch = (int *)calloc(2 * ltr, sizeof(int));
+    child(H->tr, H->ch, ltr);
+    /* node depths (hsslevel) and the number of levels */
+    int *lv = (int *)calloc(ltr + 1, sizeof(int));
+    int lvl = 0;
+    A1(lv, ltr) = 0;
+    for (int i = ltr - 1; i >= 1; i--) { A1(lv, i) = A1(lv, A1(H->tr, i)) + 1; if (A1(lv, i) > lvl) lvl = A1(lv, i); }
+    H->lvl = lvl;
+    if (lvl < 1) { free(lv); return -1; }
+    H->lentr = (int *)calloc(lvl + 1, sizeof(int));
+    H->ttr = (int *)calloc((long)lvl * ltr, sizeof(int));
+    for (int LL = lvl; LL >= 1; LL--) {
+        int it = 0;
+        for (int j = 1; j <= ltr - 1; j++) if (A1(lv, j) == LL) { it++; TTR(H, LL, it) = j; }
+        A1(H->lentr, LL) = it;
+    }
+    free(lv);
+    H->dsz = (int *)calloc(ltr, sizeof(int));
+    H->usz = (int *)calloc(2 * ltr, sizeof(int));
+    H->vsz = (int *)calloc(2 * ltr, sizeof(int));
+    H->bsz = (int *)calloc(2 * ltr, sizeof(int));
+    H->pu = (long *)calloc(ltr, sizeof(long));
+    H->pv = (long *)calloc(ltr, sizeof(long));
+    H->pb = (long *)calloc(ltr, sizeof(long));
+    H->pd = (long *)calloc(ltr, sizeof(long));
+    long tnk = 0;
+    for (int i = 1; i <= lm; i++) tnk += (long)A1(H->m, i) * A1(H->m, i);
+    H->dd = (double *)malloc(sizeof(double) * tnk);
+    H->ndd = tnk;
+    long cap = 10L * ni * K;
+    H->h = (double *)malloc(sizeof(double) * cap);
+    H->nh = 0;
+
+    int *ch = H->ch;
+    double *D = (double *)malloc(sizeof(double) * K), *U = (double *)malloc(sizeof(double) * K),
+           *V = (double *)malloc(sizeof(double) * K);
+    int *PL = (int *)malloc(sizeof(int) * K), *PU = (int *)malloc(sizeof(int) * K);
+    int *lr = (int *)malloc(sizeof(int) * 2 * ltr), *lc = (int *)malloc(sizeof(int) * 2 * ltr);
+    int *indx = (int *)malloc(sizeof(int) * K);
+    double *Di = (double *)malloc(sizeof(double) * K), *Fi = (double *)malloc(sizeof(double) * K),
+           *Ui = (double *)malloc(sizeof(double) * K), *Vi = (double *)malloc(sizeof(double) * K);
+    int *PLi = (int *)malloc(sizeof(int) * K), *PUi = (int *)malloc(sizeof(int) * K);
+    memcpy(D, d0, sizeof(double) * K);
+    memcpy(U, u0, sizeof(double) * K);
+    memcpy(V, v0, sizeof(double) * K);
+    for (int j = 1; j <= K; j++) { A1(PL, j) = j; A1(PU, j) = j; }
+#define LR(i, c) lr[2 * ((i) - 1) + (c) - 1]
+#define LC(i, c) lc[2 * ((i) - 1) + (c) - 1]
+    int lt = 1, itl = 1;
+    long pdt = 0;
+    for (int i = 1; i <= ltr; i++) {
+        if (CH1(ch, i) == 0 && CH2(ch, i) == 0) {
+            LR(i, 1) = lt; LR(i, 2) = lt + A1(H->m, itl) - 1;
+            lt = LR(i, 2) + 1;
+            H->pd[i - 1] = pdt;
+            pdt += (long)A1(H->m, itl) * A1(H->m, itl);
+            itl++;
+        } else {
+            LR(i, 1) = LR(CH1(ch, i), 1); LR(i, 2) = LR(CH2(ch, i), 2);
+        }
+    }
+    memcpy(lc, lr, sizeof(int) * 2 * ltr);
+    for (int i = 1; i <= ltr; i++) {
+        if (CH1(ch, i) == 0 && CH2(ch, i) == 0) {
+            int mi = LR(i, 2) - LR(i, 1) + 1;
+            cauchylikeEig(H->dd + H->pd[i - 1], mi, f, U + LR(i, 1) - 1, V + LR(i, 1) - 1, DIFL, DIFR,
+                          PL + LR(i, 1) - 1, PU + LR(i, 1) - 1, mi, mi);
+            H->dsz[i - 1] = mi;
+        }
+    }
+    int lda = K, lca = K;
+    for (int LL = lvl; LL >= 1; LL--) {
+        int n1 = A1(H->lentr, LL);
+        /* row compression */
+        for (int i = 1; i <= n1; i++) {
+            int nodi = TTR(H, LL, i), Rk;
+            if (CH1(ch, nodi) != 0) {
+                LR(nodi, 1) = LR(CH1(ch, nodi), 1); LR(nodi, 2) = LR(CH2(ch, nodi), 2);
+                LC(nodi, 1) = LC(CH1(ch, nodi), 1); LC(nodi, 2) = LC(CH2(ch, nodi), 2);
+            }
+            int mi = LR(nodi, 2) - LR(nodi, 1) + 1;
+            int nic = LC(nodi, 2) - LC(nodi, 1) + 1;
+            int nt = lca - nic, q = 0;
+            for (int j = 1; j <= LC(nodi, 1) - 1; j++) indx[q++] = j;
+            for (int j = LC(nodi, 2) + 1; j <= lca; j++) indx[q++] = j;
+            for (int j = 0; j < nt; j++) {
+                PUi[j] = A1(PU, indx[j]); Vi[j] = A1(V, indx[j]); Di[j] = A1(D, indx[j]);
+            }
+            for (int j = 0; j < mi; j++) Fi[j] = A1(f, A1(PL, LR(nodi, 1) + j));
+            comprEig(H, &cap, 'r', Fi, Di, U + LR(nodi, 1) - 1, Vi, DIFL, DIFR, f,
+                     PL + LR(nodi, 1) - 1, PUi, tol, mi, nt, nodi, &Rk);
+        }
+        /* compact surviving row generators */
+        lt = 1;
+        for (int i = 1; i <= n1; i++) {
+            int nodi = TTR(H, LL, i);
+            int Rk = H->usz[2 * (nodi - 1) + 1];
+            for (int j = 0; j < Rk; j++) { Ui[lt - 1 + j] = A1(U, LR(nodi, 1) + j); PLi[lt - 1 + j] = A1(PL, LR(nodi, 1) + j); }
+            LR(nodi, 1) = lt; LR(nodi, 2) = lt + Rk - 1; lt += Rk;
+        }
+        for (int LLt = LL - 1; LLt >= 1; LLt--) {
+            for (int i = 1; i <= A1(H->lentr, LLt); i++) {
+                int nodi = TTR(H, LLt, i);
+                if (CH1(ch, nodi) == 0 && CH2(ch, nodi) == 0) {
+                    int mi = LR(nodi, 2) - LR(nodi, 1) + 1;
+                    for (int j = 0; j < mi; j++) { Ui[lt - 1 + j] = A1(U, LR(nodi, 1) + j); PLi[lt - 1 + j] = A1(PL, LR(nodi, 1) + j); }
+                    LR(nodi, 1) = lt; LR(nodi, 2) = lt + mi - 1; lt += mi;
+                }
+            }
+        }
+        memcpy(U, Ui, sizeof(double) * (lt - 1));
+        memcpy(PL, PLi, sizeof(int) * (lt - 1));
+        lda = lt - 1;
+        /* column compression */
+        for (int i = 1; i <= n1; i++) {
+            int nodi = TTR(H, LL, i), Rk;
+            int mi = LC(nodi, 2) - LC(nodi, 1) + 1;
+            int nir = LR(nodi, 2) - LR(nodi, 1) + 1;
+            int nt = lda - nir, q = 0;
+            for (int j = 1; j <= LR(nodi, 1) - 1; j++) indx[q++] = j;
+            for (int j = LR(nodi, 2) + 1; j <= lda; j++) indx[q++] = j;
+            for (int j = 0; j < nt; j++) {
+                Ui[j] = A1(U, indx[j]); PLi[j] = A1(PL, indx[j]); Fi[j] = A1(f, A1(PL, indx[j]));
+            }
+            comprEig(H, &cap, 'c', Fi, D + LC(nodi, 1) - 1, Ui, V + LC(nodi, 1) - 1, DIFL, DIFR, f,
+                     PLi, PU + LC(nodi, 1) - 1, tol, nt, mi, nodi, &Rk);
+        }
+        /* compact surviving column generators */
+        lt = 1;
+        for (int i = 1; i <= n1; i++) {
+            int nodi = TTR(H, LL, i);
+            int Rk = H->vsz[2 * (nodi - 1)];
+            for (int j = 0; j < Rk; j++) {
+                Vi[lt - 1 + j] = A1(V, LC(nodi, 1) + j); Di[lt - 1 + j] = A1(D, LC(nodi, 1) + j);
+                PUi[lt - 1 + j] = A1(PU, LC(nodi, 1) + j);
+            }
+            LC(nodi, 1) = lt; LC(nodi, 2) = lt + Rk - 1; lt += Rk;
+        }
+        for (int LLt = LL - 1; LLt >= 1; LLt--) {
+            for (int i = 1; i <= A1(H->lentr, LLt); i++) {
+                int nodi = TTR(H, LLt, i);
+                if (CH1(ch, nodi) == 0 && CH2(ch, nodi) == 0) {
+                    int mi = LC(nodi, 2) - LC(nodi, 1) + 1;
+                    for (int j = 0; j < mi; j++) {
+                        Vi[lt - 1 + j] = A1(V, LC(nodi, 1) + j); Di[lt - 1 + j] = A1(D, LC(nodi, 1) + j);
+                        PUi[lt - 1 + j] = A1(PU, LC(nodi, 1) + j);
+                    }
+                    LC(nodi, 1) = lt; LC(nodi, 2) = lt + mi - 1; lt += mi;
+                }
+            }
+        }
+        memcpy(V, Vi, sizeof(double) * (lt - 1));
+        memcpy(D, Di, sizeof(double) * (lt - 1));
+        memcpy(PU, PUi, sizeof(int) * (lt - 1));
+        lca = lt - 1;
+        /* B generators */
+        for (int i = 1; i <= n1; i++) {
+            int nodi = TTR(H, LL, i);
+            int par = A1(H->tr, nodi);
+            int nodj = (nodi == CH1(ch, par)) ? CH2(ch, par) : CH1(ch, par);
+            int mi = LR(nodi, 2) - LR(nodi, 1) + 1;
+            int nj = LC(nodj, 2) - LC(nodj, 1) + 1;
+            long off = push(H, &cap, (long)mi * nj);
+            cauchylikeEig(H->h + off, mi, f, U + LR(nodi, 1) - 1, V + LC(nodj, 1) - 1, DIFL, DIFR,
+                          PL + LR(nodi, 1) - 1, PU + LC(nodj, 1) - 1, mi, nj);
+            H->bsz[2 * (nodi - 1)] = mi; H->bsz[2 * (nodi - 1) + 1] = nj; H->pb[nodi - 1] = off;
+            H->nflops += 6L * mi * nj;
+        }
+    }
+#undef LR
+#undef LC
+    free(D); free(U); free(V); free(PL); free(PU); free(lr); free(lc); free(indx);
+    free(Di); free(Fi); free(Ui); free(Vi); free(PLi); free(PUi);
+    return 0;
+}
+
+static void gemm_acc(int m, int n, int k, const double *a, int lda, const double *b, int ldb,
+                     double beta, double *c, int ldc)
+{
+    for (int j = 0; j < n; j++) {
+        double *cj = c + (long)j * ldc;
+        if (beta == 0.0) for (int i = 0; i < m; i++) cj[i] = 0.0;
+        for (int l = 0; l < k; l++) {
+            double blj = b[l + (long)j * ldb];
+            const double *al = a + (long)l * lda;
+            for (int i = 0; i < m; i++) cj[i] += al[i] * blj;
+        }
+    }
+}
+
+/* fastHssmmL_omp (ConstructHssd.f90:1394-1630): X := X * H.  No early return for
+ * shallow trees: the sweeps below are valid for any depth >= 1. */
+int orc_hss_mmL(const orc_hss *H, double *X, int ldx, int nrow)
+{
+    int ltr = H->ltr, lvl = H->lvl;
+    const int *ch = H->ch, *tr = H->tr;
+    int *lx = (int *)calloc(ltr + 1, sizeof(int)), *lg = (int *)calloc(ltr + 1, sizeof(int)),
+        *lf = (int *)calloc(ltr + 1, sizeof(int));
+    int lt = 1, i = 1;
+    for (int nd = 1; nd <= ltr; nd++)
+        if (CH1(ch, nd) == 0) { A1(lx, nd) = lt; lt += A1(H->m, i); i++; }
+    int it = 1;
+    for (int LL = lvl; LL >= 1; LL--)
+        for (i = 1; i <= A1(H->lentr, LL); i++) { int nd = TTR(H, LL, i); A1(lg, nd) = it; it += H->usz[2 * (nd - 1) + 1]; }
+    int lcg = it;
+    it = 1;
+    for (int LL = 1; LL <= lvl; LL++)
+        for (i = 1; i <= A1(H->lentr, LL); i++) {
+            int nd = TTR(H, LL, i), par = A1(tr, nd);
+            int nj = (nd == CH1(ch, par)) ? CH2(ch, par) : CH1(ch, par);
+            A1(lf, nd) = it; it += H->bsz[2 * (nj - 1) + 1];
+        }
+    int lcf = it;
+    double *G = (double *)calloc((long)nrow * (lcg > 1 ? lcg : 1), sizeof(double));
+    double *Fm = (double *)calloc((long)nrow * (lcf > 1 ? lcf : 1), sizeof(double));
+    /* upsweep */
+    for (int LL = lvl; LL >= 1; LL--)
+        for (i = 1; i <= A1(H->lentr, LL); i++) {
+            int nd = TTR(H, LL, i), c1 = CH1(ch, nd);
+            int ur = H->usz[2 * (nd - 1)], uc = H->usz[2 * (nd - 1) + 1];
+            const double *src = (c1 != 0) ? G + (long)(A1(lg, c1) - 1) * nrow : X + (long)(A1(lx, nd) - 1) * ldx;
+            int lds = (c1 != 0) ? nrow : ldx;
+            gemm_acc(nrow, uc, ur, src, lds, H->h + H->pu[nd - 1], ur, 0.0, G + (long)(A1(lg, nd) - 1) * nrow, nrow);
+        }
+    /* sibling products and downsweep */
+    for (int LL = 1; LL <= lvl; LL++)
+        for (i = 1; i <= A1(H->lentr, LL); i++) {
+            int nd = TTR(H, LL, i), par = A1(tr, nd);
+            int nj = (nd == CH1(ch, par)) ? CH2(ch, par) : CH1(ch, par);
+            int br = H->bsz[2 * (nj - 1)], bc = H->bsz[2 * (nj - 1) + 1];
+            gemm_acc(nrow, bc, br, G + (long)(A1(lg, nj) - 1) * nrow, nrow, H->h + H->pb[nj - 1], br,
+                     LL == 1 ? 0.0 : 1.0, Fm + (long)(A1(lf, nd) - 1) * nrow, nrow);
+            int c1 = CH1(ch, nd);
+            if (c1 != 0) {
+                int vr = H->vsz[2 * (nd - 1)], vc = H->vsz[2 * (nd - 1) + 1];
+                gemm_acc(nrow, vc, vr, Fm + (long)(A1(lf, nd) - 1) * nrow, nrow, H->h + H->pv[nd - 1], vr,
+                         0.0, Fm + (long)(A1(lf, c1) - 1) * nrow, nrow);
+            }
+        }
+    /* leaves: X_i := F_i V_i + X_i D_i */
+    double *T2 = (double *)malloc(sizeof(double) * (long)nrow * (H->k > 0 ? H->k : 1));
+    for (int nd = 1; nd <= ltr - 1; nd++) {
+        if (CH1(ch, nd) != 0) continue;
+        int vr = H->vsz[2 * (nd - 1)], vc = H->vsz[2 * (nd - 1) + 1], md = H->dsz[nd - 1];
+        gemm_acc(nrow, vc, vr, Fm + (long)(A1(lf, nd) - 1) * nrow, nrow, H->h + H->pv[nd - 1], vr, 0.0, T2, nrow);
+        gemm_acc(nrow, md, md, X + (long)(A1(lx, nd) - 1) * ldx, ldx, H->dd + H->pd[nd - 1], md, 1.0, T2, nrow);
+        for (int j = 0; j < md; j++)
+            memcpy(X + (long)(A1(lx, nd) - 1 + j) * ldx, T2 + (long)j * nrow, sizeof(double) * nrow);
+    }
+    free(T2); free(G); free(Fm); free(lx); free(lg); free(lf);
+    return 0;
+}
+
+void orc_hss_dense(const orc_hss *H, double *a, int lda)
+{
+    int K = H->k;
+    for (int j = 0; j < K; j++)
+        for (int i = 0; i < K; i++) a[i + (long)j * lda] = (i == j) ? 1.0 : 0.0;
+    orc_hss_mmL(H, a, lda, K);
+}
+
+void orc_cauchy_dense(int k, const double *f, const double *u, const double *v,
+                      const double *difl, const double *difr, double *a, int lda)
+{
+    int *p = (int *)malloc(sizeof(int) * k);
+    for (int i = 0; i < k; i++) p[i] = i + 1;
+    cauchylikeEig(a, lda, f, u, v, difl, difr, p, p, k, k);
+    free(p);
+}

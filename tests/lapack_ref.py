@@ -1,0 +1,131 @@
+"""ctypes bindings to the LAPACK routines shipped inside scipy's OpenBLAS.
+
+These are the third-party routines the reference path calls (DLAED4 at
+mdlaed3.f90:208, DLAED2/DLAED3 behind mdlaed1.f90:184-199, DLAMRG at
+mdlaed1.f90:221, DSTEQR at mdlaed0.f90:263) plus DSTEDC, the LAPACK driver
+MDSTEDC replaces.  They serve as golden references for the oracle.  Test-only.
+"""
+import ctypes
+import glob
+import os
+
+import numpy as np
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        import scipy  # noqa: F401  (locates scipy.libs)
+        base = os.path.join(os.path.dirname(scipy.__file__), os.pardir, "scipy.libs")
+        cand = sorted(glob.glob(os.path.join(base, "libscipy_openblas*.so")))
+        if not cand:
+            raise RuntimeError("scipy OpenBLAS LAPACK not found")
+        _LIB = ctypes.CDLL(cand[0])
+    return _LIB
+
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+
+def _ip(x):
+    return ctypes.byref(_i(x))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ipa(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def dlaed4(d, z, rho, j):
+    """Root j (1-based); returns (lam, delta, info)."""
+    n = len(d)
+    d = np.ascontiguousarray(d, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    delta = np.zeros(n)
+    lam = _d(0)
+    info = _i(0)
+    lib().scipy_dlaed4_(_ip(n), _ip(j), _dp(d), _dp(z), _dp(delta), ctypes.byref(_d(rho)),
+                        ctypes.byref(lam), ctypes.byref(info))
+    return lam.value, delta, info.value
+
+
+def dlamrg(a, n1, n2, s1, s2):
+    a = np.ascontiguousarray(a, np.float64)
+    n1, n2, s1, s2 = int(n1), int(n2), int(s1), int(s2)
+    idx = np.zeros(n1 + n2, np.int32)
+    lib().scipy_dlamrg_(_ip(n1), _ip(n2), _dp(a), _ip(s1), _ip(s2), _ipa(idx))
+    return idx
+
+
+def dsteqr(d, e):
+    n = len(d)
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64)
+    z = np.zeros((n, n), order="F")
+    work = np.zeros(max(1, 2 * n - 2))
+    info = _i(0)
+    lib().scipy_dsteqr_(ctypes.c_char_p(b"I"), _ip(n), _dp(d), _dp(e), _dp(z), _ip(max(1, n)),
+                        _dp(work), ctypes.byref(info), ctypes.c_size_t(1))
+    return d, z, info.value
+
+
+def dstedc(d, e, compz=b"I"):
+    n = len(d)
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64)
+    z = np.zeros((n, n), order="F")
+    lwork = 1 + 4 * n + n * n
+    liwork = 3 + 5 * n
+    work = np.zeros(lwork)
+    iwork = np.zeros(liwork, np.int32)
+    info = _i(0)
+    lib().scipy_dstedc_(ctypes.c_char_p(compz), _ip(n), _dp(d), _dp(e), _dp(z), _ip(max(1, n)),
+                        _dp(work), _ip(lwork), _ipa(iwork), _ip(liwork), ctypes.byref(info),
+                        ctypes.c_size_t(1))
+    return d, z, info.value
+
+
+def dlaed2(d, q, indxq, rho, z, n1):
+    """LAPACK DLAED2 on copies; returns a dict of every output."""
+    n = len(d)
+    d = np.array(d, np.float64)
+    q = np.array(q, np.float64, order="F")
+    indxq = np.array(indxq, np.int32)
+    z = np.array(z, np.float64)
+    dlamda = np.zeros(n)
+    w = np.zeros(n)
+    q2 = np.zeros(n * n + n)
+    indx = np.zeros(n, np.int32)
+    indxc = np.zeros(n, np.int32)
+    indxp = np.zeros(n, np.int32)
+    coltyp = np.zeros(n, np.int32)
+    k = _i(0)
+    rho_c = _d(rho)
+    info = _i(0)
+    lib().scipy_dlaed2_(ctypes.byref(k), _ip(n), _ip(n1), _dp(d), _dp(q), _ip(n), _ipa(indxq),
+                        ctypes.byref(rho_c), _dp(z), _dp(dlamda), _dp(w), _dp(q2), _ipa(indx),
+                        _ipa(indxc), _ipa(indxp), _ipa(coltyp), ctypes.byref(info))
+    return dict(k=k.value, d=d, q=q, indxq=indxq, rho=rho_c.value, z=z, dlamda=dlamda, w=w,
+                q2=q2, indx=indx, indxc=indxc, indxp=indxp, coltyp=coltyp, info=info.value)
+
+
+def dlaed3(k, n, n1, q, rho, dlamda, q2, indx, ctot, w):
+    d = np.zeros(n)
+    q = np.array(q, np.float64, order="F")
+    dlamda = np.array(dlamda, np.float64)
+    q2 = np.array(q2, np.float64)
+    indx = np.array(indx, np.int32)
+    ctot = np.array(ctot, np.int32)
+    w = np.array(w, np.float64)
+    s = np.zeros(n * n + n)
+    info = _i(0)
+    lib().scipy_dlaed3_(_ip(k), _ip(n), _ip(n1), _dp(d), _dp(q), _ip(n), ctypes.byref(_d(rho)),
+                        _dp(dlamda), _dp(q2), _ipa(indx), _ipa(ctot), _dp(w), _dp(s),
+                        ctypes.byref(info))
+    return d, q, info.value

@@ -1,0 +1,244 @@
+"""ctypes wrapper around the C oracle (oracle/liborc.so).  Test infrastructure."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_SO = os.path.join(ROOT, "oracle", "liborc.so")
+_LIB = None
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_P = ctypes.POINTER
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("smlsiz", _i), ("bsmlz", _i), ("ni", _i), ("tol", _d), ("maxit", _i)]
+
+
+class Trace(ctypes.Structure):
+    _fields_ = [("cap", _i), ("nmerge", _i), ("level", _P(_i)), ("n", _P(_i)), ("k", _P(_i)),
+                ("hss", _P(_i)), ("rank_max", _P(_i)), ("roots", ctypes.c_longlong),
+                ("iters", ctypes.c_longlong)]
+
+
+class Hss(ctypes.Structure):
+    _fields_ = [("k", _i), ("ltr", _i), ("lm", _i), ("lvl", _i),
+                ("tr", _P(_i)), ("m", _P(_i)), ("ch", _P(_i)), ("lentr", _P(_i)), ("ttr", _P(_i)),
+                ("dsz", _P(_i)), ("usz", _P(_i)), ("vsz", _P(_i)), ("bsz", _P(_i)),
+                ("pu", _P(ctypes.c_long)), ("pv", _P(ctypes.c_long)), ("pb", _P(ctypes.c_long)),
+                ("pd", _P(ctypes.c_long)), ("h", _P(_d)), ("dd", _P(_d)),
+                ("nh", ctypes.c_long), ("ndd", ctypes.c_long),
+                ("nflops", ctypes.c_longlong), ("nswap", ctypes.c_longlong)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_SO):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        _LIB = ctypes.CDLL(_SO)
+    return _LIB
+
+
+def dp(a):
+    return a.ctypes.data_as(_P(_d))
+
+
+def ip(a):
+    return a.ctypes.data_as(_P(_i))
+
+
+def params(smlsiz=25, bsmlz=1500, ni=200, tol=None, maxit=100):
+    p = Params()
+    lib().orc_default_params(ctypes.byref(p))
+    p.smlsiz, p.bsmlz, p.ni, p.maxit = smlsiz, bsmlz, ni, maxit
+    if tol is not None:
+        p.tol = tol
+    return p
+
+
+def laed4(d, z, rho, j, maxit=100):
+    n = len(d)
+    d = np.ascontiguousarray(d, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    delta = np.zeros(n)
+    lam, tau = _d(0), _d(0)
+    org, nit = _i(0), _i(0)
+    info = lib().orc_laed4(n, j, dp(d), dp(z), dp(delta), _d(rho), ctypes.byref(lam),
+                           ctypes.byref(org), ctypes.byref(tau), ctypes.byref(nit), maxit)
+    return dict(lam=lam.value, delta=delta, org=org.value, tau=tau.value, niter=nit.value,
+                info=info)
+
+
+def dlamrg(a, n1, n2, s1, s2):
+    a = np.ascontiguousarray(a, np.float64)
+    idx = np.zeros(n1 + n2, np.int32)
+    lib().orc_dlamrg(int(n1), int(n2), dp(a), int(s1), int(s2), ip(idx))
+    return idx
+
+
+def steqr(d, e):
+    n = len(d)
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64) if n > 1 else np.zeros(1)
+    z = np.zeros((n, n), order="F")
+    info = lib().orc_steqr(n, dp(d), dp(e), dp(z), max(1, n))
+    return d, z, info
+
+
+def laed2(d, q, indxq, rho, z, n1, bsmlz=1 << 30):
+    n = len(d)
+    d = np.array(d, np.float64)
+    q = np.array(q, np.float64, order="F")
+    indxq = np.array(indxq, np.int32)
+    z = np.array(z, np.float64)
+    dlamda = np.zeros(n)
+    w = np.zeros(n)
+    q2 = np.zeros(n * n + n)
+    indx = np.zeros(n, np.int32)
+    indxc = np.zeros(n, np.int32)
+    indxp = np.zeros(n, np.int32)
+    coltyp = np.zeros(n, np.int32)
+    k = _i(0)
+    hflag = _i(0)
+    nrot = _i(0)
+    rot_pj = np.zeros(n, np.int32)
+    rot_nj = np.zeros(n, np.int32)
+    rot_c = np.zeros(n)
+    rot_s = np.zeros(n)
+    rho_c = _d(rho)
+    info = lib().orc_laed2(ctypes.byref(k), n, n1, dp(d), dp(q), n, ip(indxq), ctypes.byref(rho_c),
+                           dp(z), dp(dlamda), dp(w), dp(q2), ip(indx), ip(indxc), ip(indxp),
+                           ip(coltyp), bsmlz, ctypes.byref(hflag), ctypes.byref(nrot), ip(rot_pj),
+                           ip(rot_nj), dp(rot_c), dp(rot_s))
+    r = nrot.value
+    return dict(k=k.value, d=d, q=q, indxq=indxq, rho=rho_c.value, z=z, dlamda=dlamda, w=w,
+                q2=q2, indx=indx, indxc=indxc, indxp=indxp, coltyp=coltyp, info=info,
+                hflag=hflag.value, rot=(rot_pj[:r], rot_nj[:r], rot_c[:r], rot_s[:r]))
+
+
+def laed3(k, n, n1, q, rho, dlamda, q2, indx, ctot, w, maxit=100):
+    d = np.zeros(n)
+    q = np.array(q, np.float64, order="F")
+    dlamda = np.array(dlamda, np.float64)
+    q2 = np.array(q2, np.float64)
+    indx = np.array(indx, np.int32)
+    ctot = np.array(ctot, np.int32)
+    w = np.array(w, np.float64)
+    s = np.zeros(n * n + n)
+    it = ctypes.c_longlong(0)
+    info = lib().orc_laed3(k, n, n1, dp(d), dp(q), n, _d(rho), dp(dlamda), dp(q2), ip(indx),
+                           ip(ctot), dp(w), dp(s), maxit, ctypes.byref(it))
+    return d, q, info
+
+
+def stedc(d, e, compz=b"I", prm=None, trace_cap=0, z=None):
+    n = len(d)
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64) if n > 1 else np.zeros(1)
+    if z is None:
+        z = np.zeros((n, n), order="F")
+    else:
+        z = np.array(z, np.float64, order="F")
+    prm = prm or params()
+    lgn = max(1, int(np.ceil(np.log2(max(n, 2)))) + 1)
+    if compz in (b"V", "V"):
+        lwork = 1 + 3 * n + 2 * n * lgn + 4 * n * n
+        liwork = 6 + 6 * n + 5 * n * lgn
+    else:
+        lwork = 1 + 4 * n + n * n
+        liwork = 3 + 5 * n
+    work = np.zeros(max(1, lwork))
+    iwork = np.zeros(max(1, liwork), np.int32)
+    tr = None
+    bufs = None
+    if trace_cap:
+        tr = Trace()
+        bufs = [np.zeros(trace_cap, np.int32) for _ in range(5)]
+        tr.cap = trace_cap
+        tr.nmerge = 0
+        tr.level, tr.n, tr.k, tr.hss, tr.rank_max = [ip(b) for b in bufs]
+        tr.roots = 0
+        tr.iters = 0
+    c = compz if isinstance(compz, bytes) else compz.encode()
+    info = lib().orc_stedc(ctypes.c_char(c), n, dp(d), dp(e), dp(z), max(1, n), dp(work), lwork,
+                           ip(iwork), liwork, ctypes.byref(prm),
+                           ctypes.byref(tr) if tr is not None else None)
+    out = dict(d=d, z=z, info=info, e=e)
+    if tr is not None:
+        m = min(tr.nmerge, trace_cap)
+        out["trace"] = dict(level=bufs[0][:m].copy(), n=bufs[1][:m].copy(), k=bufs[2][:m].copy(),
+                            hss=bufs[3][:m].copy(), rank_max=bufs[4][:m].copy(),
+                            roots=tr.roots, iters=tr.iters)
+    return out
+
+
+def hss_build(f, d0, u, v, difl, difr, ni, tol):
+    k = len(f)
+    H = Hss()
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (f, d0, u, v, difl, difr)]
+    info = lib().orc_hss_build(ctypes.byref(H), k, ni, *[dp(a) for a in arrs], _d(tol))
+    if info != 0:
+        raise RuntimeError("orc_hss_build failed %d" % info)
+    return H
+
+
+def hss_dense(H):
+    k = H.k
+    a = np.zeros((k, k), order="F")
+    lib().orc_hss_dense(ctypes.byref(H), dp(a), k)
+    return a
+
+
+def hss_mmL(H, x):
+    x = np.array(x, np.float64, order="F")
+    lib().orc_hss_mmL(ctypes.byref(H), dp(x), x.shape[0], x.shape[0])
+    return x
+
+
+def hss_ranks(H):
+    usz = np.ctypeslib.as_array(H.usz, (2 * H.ltr,)).reshape(-1, 2)
+    vsz = np.ctypeslib.as_array(H.vsz, (2 * H.ltr,)).reshape(-1, 2)
+    return usz.copy(), vsz.copy()
+
+
+def hss_free(H):
+    lib().orc_hss_free(ctypes.byref(H))
+
+
+def cauchy_dense(f, u, v, difl, difr):
+    k = len(f)
+    a = np.zeros((k, k), order="F")
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (f, u, v, difl, difr)]
+    lib().orc_cauchy_dense(k, *[dp(x) for x in arrs], dp(a), k)
+    return a
+
+
+def _setup():
+    L = lib()
+    L.orc_laed4.restype = _i
+    L.orc_stedc.restype = _i
+    L.orc_laed2.restype = _i
+    L.orc_laed3.restype = _i
+    L.orc_steqr.restype = _i
+    L.orc_hss_build.restype = _i
+
+
+def mdlaed3_vectors(dlamda, z, rho, maxit=100):
+    """Roots, updated z, DIFL, DIFR, ALPHA of MDLAED3 (no eigenvector update)."""
+    k = len(dlamda)
+    dl = np.array(dlamda, np.float64)
+    zz = np.array(z, np.float64)
+    d = np.zeros(k)
+    difl = np.zeros(k)
+    difr = np.zeros(k)
+    alpha = np.zeros(k)
+    it = ctypes.c_longlong(0)
+    info = lib().orc_mdlaed3_vectors(k, dp(d), _d(rho), dp(dl), dp(zz), dp(difl), dp(difr),
+                                     dp(alpha), maxit, ctypes.byref(it))
+    if info != 0:
+        raise RuntimeError("mdlaed3_vectors info=%d" % info)
+    return dict(d=d, dlamda=dl, z=zz, difl=difl, difr=difr, alpha=alpha, iters=it.value)

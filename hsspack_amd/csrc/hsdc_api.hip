@@ -1,0 +1,798 @@
+// Host driver and C ABI of the MI355X divide-and-conquer eigensolver.
+// Boundary: include/hsdc.h.  The schedule follows MDSTEDC/MDLAED0
+// (mdstedc.f90:139-436, mdlaed0.f90:121-380): split on negligible off-diagonals,
+// scale, tear into leaves, solve the leaves, then merge level by level.  All
+// merges of a level (of every problem in a batch) run in the same launches;
+// the per-merge schedule (params.batched = 0) launches them one at a time.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+
+#include "hsdc_internal.h"
+#include "../../include/hsdc.h"
+
+// kernels (hsdc_kern.hip)
+struct GTile { int m, half, r0, c0; };
+__global__ void k_leaf(const Leaf *, int, const Prob *, const double *, const double *, Work, int);
+__global__ void k_prep(const Merge *, const int *, int, const Prob *, Work, int);
+__global__ void k_sortmerge(const Merge *, Work);
+__global__ void k_deflate(const Merge *, Work, int, MergeOut *, int);
+__global__ void k_rot_prep(const Merge *, const Prob *, Work, int, const MergeOut *);
+__global__ void k_rotate(const Merge *, const int *, int, const Prob *, Work, int, const MergeOut *);
+__global__ void k_secular(const Merge *, const int *, int, const MergeOut *, Work, int, int *);
+__global__ void k_loewner(const Merge *, const int *, int, const MergeOut *, Work);
+__global__ void k_sbuild(const Merge *, const int *, int, const MergeOut *, const long *, Work, double *);
+__global__ void k_alpha(const Merge *, const int *, int, const MergeOut *, Work);
+__global__ void k_gemm(const Merge *, const GTile *, const Prob *, const MergeOut *, const long *,
+                       const long *, Work, int, const double *, double *);
+__global__ void k_newcols(const Merge *, const int *, int, const MergeOut *, Work, int);
+__global__ void k_scatter(const Merge *, const int *, int, const MergeOut *, const long *, const Prob *,
+                          Work, int, const double *);
+__global__ void k_lamrg(const Merge *, const MergeOut *, Work, int);
+__global__ void k_final(const Prob *, const int *, Work, int, const int *, double *, const long *,
+                        const long *, int, double *, const double *);
+__global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, long, double *, long);
+
+#define WGS 256
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+int ensure(DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return 0;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 8;
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        if (hipMalloc(&b.p, bytes) != hipSuccess) { (void)hipGetLastError(); b.p = nullptr; return HSDC_ERR_NOMEM; }
+        want = bytes;
+    }
+    b.cap = want;
+    return 0;
+}
+
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+int ensure_pin(PinBuf &b, size_t bytes)
+{
+    if (b.cap >= bytes) return 0;
+    if (b.p) hipHostFree(b.p);
+    b.p = nullptr;
+    size_t want = bytes * 2 + 4096;
+    if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) { b.p = nullptr; b.cap = 0; return HSDC_ERR_NOMEM; }
+    b.cap = want;
+    return 0;
+}
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+struct hsdc_ctx {
+    hsdc_params prm;
+    hsdc_stats st;
+    hipStream_t stream = 0;
+    // device storage
+    DevBuf q, ct, sb, din, ein, wsd, wsi, tabs, lvl, mo, probs, misc, zout, dout;
+    PinBuf pin, pin2;
+    Work W;
+    size_t wcap = 0;
+};
+
+// ----------------------------------------------------------------------------
+// host plan of one solve
+// ----------------------------------------------------------------------------
+struct HostProb {
+    int off, n;        // global offset and order
+    double scale;      // eigenvalues are multiplied back by this (mdstedc.f90:358)
+    int levels;        // DC merge levels
+};
+
+struct Plan {
+    int ntot = 0;
+    std::vector<HostProb> probs;
+    std::vector<Leaf> leaves;
+    std::vector<std::vector<Merge>> lv;  // merges per level (level 0 = first merge level)
+};
+
+// DC tree of one problem (mdlaed0.f90:192-219): halve every block until the last
+// one is <= smlsiz, tear with |E| at each cut.
+static void plan_problem(Plan &pl, int pi, double *d, const double *e, int smlsiz)
+{
+    HostProb &P = pl.probs[pi];
+    int n = P.n, off = P.off;
+    std::vector<int> sz(1, n);
+    int tl = 0;
+    if (n > smlsiz) {
+        while (sz.back() > smlsiz) {
+            std::vector<int> nx(sz.size() * 2);
+            for (size_t j = 0; j < sz.size(); j++) {
+                nx[2 * j] = sz[j] / 2;
+                nx[2 * j + 1] = (sz[j] + 1) / 2;
+            }
+            sz.swap(nx);
+            tl++;
+        }
+    }
+    P.levels = tl;
+    std::vector<int> ends(sz.size());
+    int acc = 0;
+    for (size_t j = 0; j < sz.size(); j++) { acc += sz[j]; ends[j] = acc; }
+    for (size_t i = 0; i + 1 < sz.size(); i++) {
+        int sub = ends[i];  // 0-based index of first element of block i+1
+        double ae = fabs(e[off + sub - 1]);
+        d[off + sub - 1] -= ae;
+        d[off + sub] -= ae;
+    }
+    int st = 0;
+    for (size_t i = 0; i < sz.size(); i++) {
+        pl.leaves.push_back(Leaf{off + st, sz[i], pi});
+        st = ends[i];
+    }
+    // merges level by level (mdlaed0.f90:294-341)
+    std::vector<int> bend = ends;
+    for (int L = 0; L < tl; L++) {
+        if ((int)pl.lv.size() <= L) pl.lv.resize(L + 1);
+        std::vector<int> nb;
+        for (size_t i = 0; i + 1 < bend.size(); i += 2) {
+            int s = (i == 0) ? 0 : bend[i - 1];
+            int matsiz = bend[i + 1] - s;
+            int msd2 = (i == 0) ? bend[0] : matsiz / 2;
+            Merge M;
+            M.s = off + s; M.n = matsiz; M.n1 = msd2; M.prob = pi;
+            M.rho = e[off + s + msd2 - 1];
+            pl.lv[L].push_back(M);
+            nb.push_back(bend[i + 1]);
+        }
+        bend.swap(nb);
+    }
+}
+
+#define CK(x) do { int _r = (x); if (_r) return _r; } while (0)
+#define HK(x) do { if ((x) != hipSuccess) { (void)hipGetLastError(); return HSDC_ERR_HIP; } } while (0)
+
+static int have_device()
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { (void)hipGetLastError(); return 0; }
+    return 1;
+}
+
+static int alloc_work(hsdc_ctx *c, int ntot)
+{
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    if ((size_t)ntot <= c->wcap && c->wcap) return 0;
+    size_t nd = 17, ni = 16;
+    CK(ensure(c->wsd, sizeof(double) * nd * (size_t)ntot));
+    CK(ensure(c->wsi, sizeof(int) * ni * (size_t)ntot));
+    double *D = (double *)c->wsd.p;
+    int *I = (int *)c->wsi.p;
+    Work &W = c->W;
+    size_t n = ntot;
+    W.Dv[0] = D + 0 * n; W.Dv[1] = D + 1 * n; W.z = D + 2 * n; W.dlamda = D + 3 * n; W.w = D + 4 * n;
+    W.ddef = D + 5 * n; W.zs = D + 6 * n; W.ds = D + 7 * n; W.rot_c = D + 8 * n; W.rot_s = D + 9 * n;
+    W.lam = D + 10 * n; W.tau = D + 11 * n; W.wnew = D + 12 * n; W.alpha = D + 13 * n;
+    W.difl = D + 14 * n; W.difr = D + 15 * n;
+    W.vm[0] = I + 0 * n; W.vm[1] = I + 1 * n; W.indxq = I + 2 * n; W.supp_lo = I + 3 * n;
+    W.supp_hi = I + 4 * n; W.indxq2 = I + 5 * n; W.indx = I + 6 * n; W.indxc = I + 7 * n;
+    W.indxp = I + 8 * n; W.coltyp = I + 9 * n; W.rot_pj = I + 10 * n; W.rot_nj = I + 11 * n;
+    W.org = I + 12 * n; W.nit = I + 13 * n;
+    c->wcap = ntot;
+    return 0;
+}
+
+// one level (or one merge of it): merges [m0, m1) of table dm (device) / hm (host);
+// dchunkN is the device chunk table of exactly this slice (first entry 0)
+static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm, int m0, int m1,
+                      const int *dchunkN, int totN, Prob *dprobs, int &cur, int *dinfo)
+{
+    const int nm = m1 - m0;
+    hipStream_t s = c->stream;
+    const Merge *dmg = dm + m0;
+    size_t need_mo = sizeof(MergeOut) * nm;
+    CK(ensure(c->mo, need_mo));
+    MergeOut *dmo = (MergeOut *)c->mo.p;
+    Work W = c->W;
+    hipLaunchKernelGGL(k_prep, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur);
+    hipLaunchKernelGGL(k_sortmerge, dim3(nm), dim3(WGS), 0, s, dmg, W);
+    hipLaunchKernelGGL(k_deflate, dim3(nm), dim3(WGS), 0, s, dmg, W, cur, dmo, c->prm.hss_min_k);
+    // K of every merge back to the host
+    CK(ensure_pin(c->pin2, need_mo));
+    HK(hipMemcpyAsync(c->pin2.p, dmo, need_mo, hipMemcpyDeviceToHost, s));
+    HK(hipStreamSynchronize(s));
+    std::vector<MergeOut> ho(nm);
+    memcpy(ho.data(), c->pin2.p, need_mo);
+    // derived tables
+    std::vector<int> chunkK(nm + 1), blockK(nm + 1), colK(nm + 1);
+    std::vector<long> soff(nm), coff(nm);
+    std::vector<GTile> tiles;
+    long stot = 0, ctot = 0;
+    int anyrot = 0, anyk = 0;
+    chunkK[0] = blockK[0] = colK[0] = 0;
+    for (int i = 0; i < nm; i++) {
+        const MergeOut &o = ho[i];
+        const Merge &M = hm[m0 + i];
+        int k = o.flags ? 0 : o.k;
+        chunkK[i + 1] = chunkK[i] + cdiv(k, WGS);
+        blockK[i + 1] = blockK[i] + cdiv(k, 64);
+        colK[i + 1] = colK[i] + k;
+        anyrot |= (o.nrot > 0 && !o.flags);
+        anyk |= (k > 0);
+        soff[i] = stot;
+        coff[i] = ctot;
+        if (k > 0 && !o.hss) stot += (long)k * k;
+        ctot += (long)M.n * k;
+        c->st.roots += k;
+        c->st.deflated += M.n - k;
+        if (o.hss) c->st.hss_merges++;
+        if (!o.hss && k > 0) {
+            int n1 = M.n1, n2 = M.n - M.n1;
+            for (int r0 = 0; r0 < n1; r0 += 64)
+                for (int c0 = 0; c0 < k; c0 += 64) tiles.push_back(GTile{i, 0, r0, c0});
+            for (int r0 = 0; r0 < n2; r0 += 64)
+                for (int c0 = 0; c0 < k; c0 += 64) tiles.push_back(GTile{i, 1, r0, c0});
+        }
+        c->st.merges++;
+        c->st.k_top = k;
+    }
+    // upload derived tables in one copy
+    size_t o1 = 0, o2 = o1 + sizeof(int) * (nm + 1), o3 = o2 + sizeof(int) * (nm + 1),
+           o4 = o3 + sizeof(int) * (nm + 1), o5 = o4 + sizeof(long) * nm, o6 = o5 + sizeof(long) * nm,
+           o7 = o6 + sizeof(GTile) * tiles.size();
+    CK(ensure_pin(c->pin, o7));
+    char *hp = (char *)c->pin.p;
+    memcpy(hp + o1, chunkK.data(), sizeof(int) * (nm + 1));
+    memcpy(hp + o2, blockK.data(), sizeof(int) * (nm + 1));
+    memcpy(hp + o3, colK.data(), sizeof(int) * (nm + 1));
+    memcpy(hp + o4, soff.data(), sizeof(long) * nm);
+    memcpy(hp + o5, coff.data(), sizeof(long) * nm);
+    if (!tiles.empty()) memcpy(hp + o6, tiles.data(), sizeof(GTile) * tiles.size());
+    CK(ensure(c->lvl, o7));
+    char *dp = (char *)c->lvl.p;
+    HK(hipMemcpyAsync(dp, hp, o7, hipMemcpyHostToDevice, s));
+    int *dchunkK = (int *)(dp + o1), *dblockK = (int *)(dp + o2), *dcolK = (int *)(dp + o3);
+    long *dsoff = (long *)(dp + o4), *dcoff = (long *)(dp + o5);
+    GTile *dtiles = (GTile *)(dp + o6);
+    CK(ensure(c->sb, sizeof(double) * (size_t)std::max(stot, 1L)));
+    CK(ensure(c->ct, sizeof(double) * (size_t)std::max(ctot, 1L)));
+    if (anyrot) {
+        hipLaunchKernelGGL(k_rot_prep, dim3(nm), dim3(WGS), 0, s, dmg, dprobs, W, cur, dmo);
+        hipLaunchKernelGGL(k_rotate, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur, dmo);
+    }
+    if (anyk) {
+        hipLaunchKernelGGL(k_secular, dim3(blockK[nm]), dim3(64), 0, s, dmg, dblockK, nm, dmo, W,
+                           c->prm.maxit, dinfo);
+        hipLaunchKernelGGL(k_loewner, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, W);
+        hipLaunchKernelGGL(k_sbuild, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, dsoff, W,
+                           (double *)c->sb.p);
+        if (!tiles.empty())
+            hipLaunchKernelGGL(k_gemm, dim3(tiles.size()), dim3(WGS), 0, s, dmg, dtiles, dprobs, dmo, dsoff,
+                               dcoff, W, cur, (const double *)c->sb.p, (double *)c->ct.p);
+    }
+    hipLaunchKernelGGL(k_newcols, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dmo, W, cur);
+    if (anyk)
+        hipLaunchKernelGGL(k_scatter, dim3(colK[nm]), dim3(WGS), 0, s, dmg, dcolK, nm, dmo, dcoff, dprobs, W,
+                           cur ^ 1, (const double *)c->ct.p);
+    hipLaunchKernelGGL(k_lamrg, dim3(nm), dim3(WGS), 0, s, dmg, dmo, W, cur ^ 1);
+    HK(hipGetLastError());
+    cur ^= 1;
+    return 0;
+}
+
+// Solve every problem of the plan; d_in/e_in are the torn, scaled host arrays.
+// Output: either one global Z (zrows_global = ntot, ldz) or per-problem blocks.
+static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, double *dZ,
+                 const std::vector<long> &zoff, const std::vector<long> &ldz, int zglobal,
+                 double *d_out_dev, const std::vector<int> *outcol)
+{
+    hipStream_t s = c->stream;
+    const int ntot = pl.ntot;
+    hipEvent_t e0, e1;
+    HK(hipEventCreate(&e0));
+    HK(hipEventCreate(&e1));
+    HK(hipEventRecord(e0, s));
+    CK(alloc_work(c, ntot));
+    // Q storage: one n x n block per problem
+    size_t qtot = 0;
+    std::vector<Prob> hp(pl.probs.size());
+    for (size_t i = 0; i < pl.probs.size(); i++) qtot += (size_t)pl.probs[i].n * pl.probs[i].n;
+    CK(ensure(c->q, sizeof(double) * qtot));
+    size_t qo = 0;
+    for (size_t i = 0; i < pl.probs.size(); i++) {
+        hp[i].q = (double *)c->q.p + qo;
+        hp[i].ld = pl.probs[i].n;
+        hp[i].off = pl.probs[i].off;
+        hp[i].n = pl.probs[i].n;
+        hp[i].par = pl.probs[i].levels & 1;
+        qo += (size_t)pl.probs[i].n * pl.probs[i].n;
+    }
+    // static tables: probs | leaves | merges of all levels | d | e | scale | colprob | zoff | ldz | outcol | info
+    size_t nl = pl.leaves.size(), nmtot = 0;
+    for (auto &L : pl.lv) nmtot += L.size();
+    size_t np = pl.probs.size();
+    size_t a0 = 0, a1 = a0 + sizeof(Prob) * np, a2 = a1 + sizeof(Leaf) * nl, a3 = a2 + sizeof(Merge) * nmtot,
+           a4 = a3 + sizeof(double) * ntot, a5 = a4 + sizeof(double) * ntot, a6 = a5 + sizeof(double) * np,
+           a7 = a6 + sizeof(int) * ntot, a8 = a7 + sizeof(long) * np, a9 = a8 + sizeof(long) * np,
+           a10 = a9 + sizeof(int) * ntot, a11 = a10 + 64;
+    // chunk tables: per level (nm+1 entries) followed by per merge {0, chunks}
+    std::vector<int> chunks;
+    std::vector<size_t> lvchunk, mchunk;
+    for (auto &L : pl.lv) {
+        lvchunk.push_back(chunks.size());
+        int acc = 0;
+        chunks.push_back(0);
+        for (auto &M : L) { acc += cdiv(M.n, WGS); chunks.push_back(acc); }
+        for (auto &M : L) { mchunk.push_back(chunks.size()); chunks.push_back(0); chunks.push_back(cdiv(M.n, WGS)); }
+    }
+    size_t a12 = a11 + sizeof(int) * chunks.size();
+    CK(ensure(c->probs, a12));
+    CK(ensure_pin(c->pin, a12));
+    char *h = (char *)c->pin.p;
+    memcpy(h + a0, hp.data(), sizeof(Prob) * np);
+    memcpy(h + a1, pl.leaves.data(), sizeof(Leaf) * nl);
+    size_t mo = 0;
+    std::vector<size_t> lvoff;
+    for (auto &L : pl.lv) {
+        lvoff.push_back(mo);
+        memcpy(h + a2 + sizeof(Merge) * mo, L.data(), sizeof(Merge) * L.size());
+        mo += L.size();
+    }
+    memcpy(h + a3, hd, sizeof(double) * ntot);
+    memcpy(h + a4, he, sizeof(double) * ntot);
+    std::vector<double> scl(np);
+    std::vector<int> colprob(ntot);
+    for (size_t i = 0; i < np; i++) {
+        scl[i] = pl.probs[i].scale;
+        for (int j = 0; j < pl.probs[i].n; j++) colprob[pl.probs[i].off + j] = (int)i;
+    }
+    memcpy(h + a5, scl.data(), sizeof(double) * np);
+    memcpy(h + a6, colprob.data(), sizeof(int) * ntot);
+    std::vector<long> zo(np, 0), lz(np, 1);
+    for (size_t i = 0; i < np && i < zoff.size(); i++) zo[i] = zoff[i];
+    for (size_t i = 0; i < np && i < ldz.size(); i++) lz[i] = ldz[i];
+    memcpy(h + a7, zo.data(), sizeof(long) * np);
+    memcpy(h + a8, lz.data(), sizeof(long) * np);
+    if (outcol) memcpy(h + a9, outcol->data(), sizeof(int) * ntot);
+    memset(h + a10, 0, 64);
+    if (!chunks.empty()) memcpy(h + a11, chunks.data(), sizeof(int) * chunks.size());
+    char *dp = (char *)c->probs.p;
+    HK(hipMemcpyAsync(dp, h, a12, hipMemcpyHostToDevice, s));
+    const int *dchunks = (const int *)(dp + a11);
+    Prob *dprobs = (Prob *)(dp + a0);
+    Leaf *dleaves = (Leaf *)(dp + a1);
+    Merge *dmerges = (Merge *)(dp + a2);
+    double *dd = (double *)(dp + a3), *de = (double *)(dp + a4), *dscale = (double *)(dp + a5);
+    int *dcolprob = (int *)(dp + a6);
+    long *dzoff = (long *)(dp + a7), *dldz = (long *)(dp + a8);
+    int *doutcol = outcol ? (int *)(dp + a9) : nullptr;
+    int *dinfo = (int *)(dp + a10);
+    Work W = c->W;
+    int cur = 0;
+    hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(64), 0, s, dleaves, (int)nl, dprobs, dd, de, W, cur);
+    HK(hipGetLastError());
+    // the host tables above live in c->pin, which run_merges reuses: wait for the upload
+    HK(hipStreamSynchronize(s));
+    size_t mi = 0;
+    for (size_t L = 0; L < pl.lv.size(); L++) {
+        const std::vector<Merge> &hm = pl.lv[L];
+        const Merge *dm = dmerges + lvoff[L];
+        const int nm = (int)hm.size();
+        if (c->prm.batched) {
+            int totN = chunks[lvchunk[L] + nm];
+            CK(run_merges(c, dm, hm, 0, nm, dchunks + lvchunk[L], totN, dprobs, cur, dinfo));
+            mi += nm;
+        } else {
+            // one merge at a time, every merge keeping the same ping-pong parity
+            int base = cur;
+            for (int m = 0; m < nm; m++, mi++) {
+                int cc = base;
+                CK(run_merges(c, dm, hm, m, m + 1, dchunks + mchunk[mi], chunks[mchunk[mi] + 1], dprobs, cc,
+                              dinfo));
+            }
+            cur = base ^ 1;
+        }
+        c->st.levels++;
+    }
+    int zrows = zglobal ? ntot : 0;
+    hipLaunchKernelGGL(k_final, dim3(ntot), dim3(WGS), 0, s, dprobs, dcolprob, W, cur, doutcol, dZ, dzoff,
+                       dldz, zrows, d_out_dev, dscale);
+    HK(hipGetLastError());
+    HK(hipEventRecord(e1, s));
+    int hinfo = 0;
+    HK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    HK(hipStreamSynchronize(s));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    c->st.ms_total = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return hinfo ? 1 : 0;
+}
+
+// Splitting and scaling of MDSTEDC (mdstedc.f90:305-359) into a plan of problems.
+static void make_plan_split(Plan &pl, int n, double *d, double *e, int smlsiz)
+{
+    int start = 0;
+    while (start < n) {
+        int finish = start;
+        while (finish < n - 1) {
+            double tiny = HSDC_EPS * sqrt(fabs(d[finish])) * sqrt(fabs(d[finish + 1]));
+            if (fabs(e[finish]) > tiny) finish++;
+            else break;
+        }
+        int m = finish - start + 1;
+        HostProb P;
+        P.off = start; P.n = m; P.scale = 1.0; P.levels = 0;
+        if (m > smlsiz) {
+            double onrm = 0.0;
+            for (int i = 0; i < m; i++) onrm = fmax(onrm, fabs(d[start + i]));
+            for (int i = 0; i < m - 1; i++) onrm = fmax(onrm, fabs(e[start + i]));
+            double mul = 1.0 / onrm;
+            for (int i = 0; i < m; i++) d[start + i] *= mul;
+            for (int i = 0; i < m - 1; i++) e[start + i] *= mul;
+            P.scale = onrm;
+        }
+        pl.probs.push_back(P);
+        start = finish + 1;
+    }
+    pl.ntot = n;
+    for (size_t i = 0; i < pl.probs.size(); i++) plan_problem(pl, (int)i, d, e, smlsiz);
+}
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+void hsdc_default_params(hsdc_params *p)
+{
+    p->smlsiz = 25;
+    p->hss_min_k = 1500;
+    p->hss_leaf = 200;
+    p->hss_tol = (double)1.0e-17f;
+    p->maxit = 100;
+    p->batched = 1;
+    p->learned = 0;
+}
+
+// The context touches no device until the first solve, so argument checking and
+// workspace queries work on a host without a GPU.
+int hsdc_ctx_create(hsdc_ctx **ctx)
+{
+    if (!ctx) return -1;
+    hsdc_ctx *c = new hsdc_ctx();
+    hsdc_default_params(&c->prm);
+    memset(&c->st, 0, sizeof(c->st));
+    *ctx = c;
+    return 0;
+}
+
+int hsdc_ctx_destroy(hsdc_ctx *c)
+{
+    if (!c) return 0;
+    DevBuf *bufs[] = {&c->q, &c->ct, &c->sb, &c->din, &c->ein, &c->wsd, &c->wsi, &c->tabs, &c->lvl,
+                      &c->mo, &c->probs, &c->misc, &c->zout, &c->dout};
+    for (DevBuf *b : bufs) if (b->p) hipFree(b->p);
+    if (c->pin.p) hipHostFree(c->pin.p);
+    if (c->pin2.p) hipHostFree(c->pin2.p);
+    delete c;
+    return 0;
+}
+
+int hsdc_set_params(hsdc_ctx *c, const hsdc_params *p)
+{
+    if (!c || !p) return -1;
+    if (p->smlsiz < 1 || p->smlsiz > 64) return -2;
+    if (p->hss_leaf < 8 || p->hss_tol < 0 || p->maxit < 0) return -2;
+    c->prm = *p;
+    if (c->prm.hss_min_k < 4) c->prm.hss_min_k = 4;
+    return 0;
+}
+
+int hsdc_get_stats(hsdc_ctx *c, hsdc_stats *s)
+{
+    if (!c || !s) return -1;
+    *s = c->st;
+    return 0;
+}
+
+int hsdc_set_stream(hsdc_ctx *c, void *stream)
+{
+    if (!c) return -1;
+    c->stream = (hipStream_t)stream;
+    return 0;
+}
+
+// Device-resident COMPZ='I'
+int hsdc_stedc_dev(hsdc_ctx *c, int n, double *d_d, double *d_e, double *d_z, int ldz)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (n < 0) return -2;
+    if (ldz < (n > 1 ? n : 1)) return -6;
+    memset(&c->st, 0, sizeof(c->st));
+    if (n == 0) return 0;
+    std::vector<double> hd(n), he(n, 0.0);
+    HK(hipMemcpyAsync(hd.data(), d_d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (n > 1) HK(hipMemcpyAsync(he.data(), d_e, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, c->stream));
+    HK(hipStreamSynchronize(c->stream));
+    double orgnrm = 0.0;
+    for (int i = 0; i < n; i++) orgnrm = fmax(orgnrm, fabs(hd[i]));
+    for (int i = 0; i < n - 1; i++) orgnrm = fmax(orgnrm, fabs(he[i]));
+    if (n == 1 || orgnrm == 0.0) {
+        // Z = I, D unchanged
+        HK(hipMemset2DAsync(d_z, sizeof(double) * ldz, 0, sizeof(double) * n, n, c->stream));
+        std::vector<double> one(1, 1.0);
+        for (int j = 0; j < n; j++)
+            HK(hipMemcpyAsync(d_z + (long)j * ldz + j, one.data(), sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HK(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    Plan pl;
+    make_plan_split(pl, n, hd.data(), he.data(), c->prm.smlsiz);
+    // splits: eigenvalues need a global selection sort (mdstedc.f90:397-425), done
+    // after the solve by permuting columns; first solve into the plan order.
+    std::vector<long> zoff(1, 0), lz(1, ldz);
+    int info;
+    if (pl.probs.size() == 1) {
+        info = solve(c, pl, hd.data(), he.data(), d_z, zoff, lz, 1, d_d, nullptr);
+    } else {
+        // solve once to learn the eigenvalues, then once more writing sorted columns
+        CK(ensure(c->dout, sizeof(double) * n));
+        info = solve(c, pl, hd.data(), he.data(), d_z, zoff, lz, 1, (double *)c->dout.p, nullptr);
+        if (info) return info;
+        std::vector<double> lam(n);
+        HK(hipMemcpy(lam.data(), c->dout.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        // selection sort as MDSTEDC: track where each column goes
+        std::vector<int> pos(n);
+        for (int i = 0; i < n; i++) pos[i] = i;
+        for (int ii = 1; ii < n; ii++) {
+            int i = ii - 1, k = i;
+            double p = lam[i];
+            for (int j = ii; j < n; j++) if (lam[j] < p) { k = j; p = lam[j]; }
+            if (k != i) { lam[k] = lam[i]; lam[i] = p; std::swap(pos[i], pos[k]); }
+        }
+        std::vector<int> outcol(n);
+        for (int i = 0; i < n; i++) outcol[pos[i]] = i;
+        // re-run with the same plan (deterministic) writing columns to their place
+        Plan pl3;
+        std::vector<double> hd3(n), he3(n, 0.0);
+        HK(hipMemcpy(hd3.data(), d_d, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (n > 1) HK(hipMemcpy(he3.data(), d_e, sizeof(double) * (n - 1), hipMemcpyDeviceToHost));
+        make_plan_split(pl3, n, hd3.data(), he3.data(), c->prm.smlsiz);
+        info = solve(c, pl3, hd3.data(), he3.data(), d_z, zoff, lz, 1, d_d, &outcol);
+    }
+    return info;
+}
+
+int hsdc_stedc_batched_dev(hsdc_ctx *c, int nprob, const int *ns, const long *offs, double *d_d,
+                           double *d_e, double *d_z, const long *zoffs)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (nprob < 0) return -2;
+    memset(&c->st, 0, sizeof(c->st));
+    long tot = 0;
+    for (int p = 0; p < nprob; p++) {
+        if (ns[p] < 1) return -3;
+        tot = std::max(tot, offs[p] + ns[p]);
+    }
+    std::vector<double> hd(tot), he(tot, 0.0);
+    HK(hipMemcpy(hd.data(), d_d, sizeof(double) * tot, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(he.data(), d_e, sizeof(double) * tot, hipMemcpyDeviceToHost));
+    // concatenate problems in a compact global index space
+    Plan pl;
+    int g = 0;
+    std::vector<double> gd, ge;
+    std::vector<long> zo, lz;
+    for (int p = 0; p < nprob; p++) {
+        int n = ns[p];
+        HostProb P;
+        P.off = g; P.n = n; P.scale = 1.0; P.levels = 0;
+        double onrm = 0.0;
+        for (int i = 0; i < n; i++) onrm = fmax(onrm, fabs(hd[offs[p] + i]));
+        for (int i = 0; i < n - 1; i++) onrm = fmax(onrm, fabs(he[offs[p] + i]));
+        double mul = (n > c->prm.smlsiz && onrm > 0) ? 1.0 / onrm : 1.0;
+        for (int i = 0; i < n; i++) gd.push_back(hd[offs[p] + i] * mul);
+        for (int i = 0; i < n - 1; i++) ge.push_back(he[offs[p] + i] * mul);
+        ge.push_back(0.0);
+        if (mul != 1.0) P.scale = onrm;
+        pl.probs.push_back(P);
+        zo.push_back(zoffs[p]);
+        lz.push_back(n);
+        g += n;
+    }
+    pl.ntot = g;
+    for (int p = 0; p < nprob; p++) plan_problem(pl, p, gd.data(), ge.data(), c->prm.smlsiz);
+    // eigenvalues back into d_d at each problem's offset: solve into a compact
+    // buffer, then scatter
+    CK(ensure(c->dout, sizeof(double) * g));
+    int info = solve(c, pl, gd.data(), ge.data(), d_z, zo, lz, 0, (double *)c->dout.p, nullptr);
+    for (int p = 0; p < nprob; p++)
+        HK(hipMemcpyAsync(d_d + offs[p], (double *)c->dout.p + pl.probs[p].off, sizeof(double) * ns[p],
+                          hipMemcpyDeviceToDevice, c->stream));
+    HK(hipStreamSynchronize(c->stream));
+    return info;
+}
+
+// one merge of order n held at global index 0 of the work arrays
+static int single_merge_setup(hsdc_ctx *c, int n, int n1, double rho, Merge **dm, int **dchunk)
+{
+    CK(alloc_work(c, std::max(n, 1)));
+    CK(ensure(c->probs, sizeof(Merge) + sizeof(int) * 2 + 64));
+    Merge M;
+    M.s = 0; M.n = n; M.n1 = n1; M.prob = 0; M.rho = rho;
+    int ch[2] = {0, cdiv(n, WGS)};
+    char *dp = (char *)c->probs.p;
+    HK(hipMemcpy(dp, &M, sizeof(Merge), hipMemcpyHostToDevice));
+    HK(hipMemcpy(dp + sizeof(Merge), ch, sizeof(ch), hipMemcpyHostToDevice));
+    *dm = (Merge *)dp;
+    *dchunk = (int *)(dp + sizeof(Merge));
+    return 0;
+}
+
+int hsdc_deflate(hsdc_ctx *c, int n, int n1, double *d, double *z, int *indxq, double rho, int hss_min_k,
+                 int *k, double *rho_out, double *dlamda, double *w, int *indxp, int *indx, int *indxc,
+                 int *coltyp, int *nrot, int *rot_pj, int *rot_nj, double *rot_c, double *rot_s)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (n < 0) return -2;
+    if ((1 < n / 2 ? 1 : n / 2) > n1 || n / 2 < n1) return -3;
+    if (n == 0) { *k = 0; return 0; }
+    Merge *dm;
+    int *dch;
+    CK(single_merge_setup(c, n, n1, rho, &dm, &dch));
+    Work W = c->W;
+    CK(ensure(c->mo, sizeof(MergeOut)));
+    MergeOut *dmo = (MergeOut *)c->mo.p;
+    HK(hipMemcpy(W.Dv[0], d, sizeof(double) * n, hipMemcpyHostToDevice));
+    HK(hipMemcpy(W.z, z, sizeof(double) * n, hipMemcpyHostToDevice));
+    HK(hipMemcpy(W.indxq, indxq, sizeof(int) * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_prep, dim3(cdiv(n, WGS)), dim3(WGS), 0, c->stream, (const Merge *)dm, (const int *)dch, 1,
+                       (const Prob *)nullptr, W, 0);
+    hipLaunchKernelGGL(k_sortmerge, dim3(1), dim3(WGS), 0, c->stream, (const Merge *)dm, W);
+    hipLaunchKernelGGL(k_deflate, dim3(1), dim3(WGS), 0, c->stream, (const Merge *)dm, W, 0, dmo, hss_min_k);
+    HK(hipGetLastError());
+    MergeOut o;
+    HK(hipMemcpy(&o, dmo, sizeof(MergeOut), hipMemcpyDeviceToHost));
+    *k = o.k;
+    *rho_out = o.rho;
+    *nrot = o.nrot;
+    HK(hipMemcpy(indx, W.indx, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<int> iq2(n);
+    HK(hipMemcpy(iq2.data(), W.indxq2, sizeof(int) * n, hipMemcpyDeviceToHost));
+    memcpy(indxq, iq2.data(), sizeof(int) * n);
+    if (o.flags) {
+        // everything deflated: DLAMDA = D in sorted order (mdlaed2.f90:273-285)
+        HK(hipMemcpy(dlamda, W.dlamda, sizeof(double) * n, hipMemcpyDeviceToHost));
+        std::vector<double> dl(n);
+        for (int j = 0; j < n; j++) dl[j] = d[indx[j] - 1];
+        memcpy(dlamda, dl.data(), sizeof(double) * n);
+        memcpy(d, dl.data(), sizeof(double) * n);
+        return 0;
+    }
+    HK(hipMemcpy(dlamda, W.dlamda, sizeof(double) * o.k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(w, W.w, sizeof(double) * o.k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(indxp, W.indxp, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(indxc, W.indxc, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(coltyp, W.coltyp, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int t = 0; t < 4; t++) coltyp[t] = o.ctot[t];
+    if (o.nrot) {
+        HK(hipMemcpy(rot_pj, W.rot_pj, sizeof(int) * o.nrot, hipMemcpyDeviceToHost));
+        HK(hipMemcpy(rot_nj, W.rot_nj, sizeof(int) * o.nrot, hipMemcpyDeviceToHost));
+        HK(hipMemcpy(rot_c, W.rot_c, sizeof(double) * o.nrot, hipMemcpyDeviceToHost));
+        HK(hipMemcpy(rot_s, W.rot_s, sizeof(double) * o.nrot, hipMemcpyDeviceToHost));
+    }
+    if (o.k < n) HK(hipMemcpy(d + o.k, W.ddef + o.k, sizeof(double) * (n - o.k), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int hsdc_secular(hsdc_ctx *c, int k, const double *dlamda, const double *w, double rho, double *lam, int *org,
+                 double *tau, int *niter)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (k < 0) return -2;
+    if (k == 0) return 0;
+    Merge *dm;
+    int *dch;
+    CK(single_merge_setup(c, k, k / 2, rho, &dm, &dch));
+    Work W = c->W;
+    CK(ensure(c->mo, sizeof(MergeOut) + 64));
+    MergeOut o;
+    memset(&o, 0, sizeof(o));
+    o.k = k; o.rho = rho; o.hss = 0; o.flags = 0;
+    MergeOut *dmo = (MergeOut *)c->mo.p;
+    int *dinfo = (int *)((char *)c->mo.p + sizeof(MergeOut));
+    int zero = 0;
+    int bs[2] = {0, cdiv(k, 64)};
+    HK(hipMemcpy(dmo, &o, sizeof(o), hipMemcpyHostToDevice));
+    HK(hipMemcpy(dinfo, &zero, sizeof(int), hipMemcpyHostToDevice));
+    HK(hipMemcpy(dch, bs, sizeof(bs), hipMemcpyHostToDevice));
+    HK(hipMemcpy(W.dlamda, dlamda, sizeof(double) * k, hipMemcpyHostToDevice));
+    HK(hipMemcpy(W.w, w, sizeof(double) * k, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_secular, dim3(bs[1]), dim3(64), 0, c->stream, (const Merge *)dm, (const int *)dch, 1,
+                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo);
+    HK(hipGetLastError());
+    int info = 0;
+    HK(hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost));
+    HK(hipMemcpy(lam, W.lam, sizeof(double) * k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(tau, W.tau, sizeof(double) * k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(org, W.org, sizeof(int) * k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(niter, W.nit, sizeof(int) * k, hipMemcpyDeviceToHost));
+    return info ? 1 : 0;
+}
+
+int hsdc_stedc(hsdc_ctx *c, char compz, int n, double *d, double *e, double *z, int ldz, double *work,
+               int lwork, int *iwork, int liwork)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    int icompz, info = 0, lwmin, liwmin, smlsiz = c->prm.smlsiz;
+    int lquery = (lwork == -1 || liwork == -1);
+    if (compz == 'N' || compz == 'n') icompz = 0;
+    else if (compz == 'V' || compz == 'v') icompz = 1;
+    else if (compz == 'I' || compz == 'i') icompz = 2;
+    else icompz = -1;
+    if (icompz < 0) info = -1;
+    else if (n < 0) info = -2;
+    else if (ldz < 1 || (icompz > 0 && ldz < (n > 1 ? n : 1))) info = -6;
+    if (info == 0) {
+        if (n <= 1 || icompz == 0) { liwmin = 1; lwmin = 1; }
+        else if (n <= smlsiz) { liwmin = 1; lwmin = 2 * (n - 1); }
+        else {
+            int lgn = (int)(log((double)n) / log(2.0));
+            if ((1 << lgn) < n) lgn++;
+            if ((1 << lgn) < n) lgn++;
+            if (icompz == 1) { lwmin = 1 + 3 * n + 2 * n * lgn + 4 * n * n; liwmin = 6 + 6 * n + 5 * n * lgn; }
+            else { lwmin = 1 + 4 * n + n * n; liwmin = 3 + 5 * n; }
+        }
+        if (work) work[0] = lwmin;
+        if (iwork) iwork[0] = liwmin;
+        if (lwork < lwmin && !lquery) info = -8;
+        else if (liwork < liwmin && !lquery) info = -10;
+    }
+    if (info != 0 || lquery) return info;
+    if (n == 0) return 0;
+    if (n == 1) { if (icompz != 0) z[0] = 1.0; return 0; }
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    // stage through device memory
+    CK(ensure(c->din, sizeof(double) * 2 * n));
+    double *dd = (double *)c->din.p, *de = dd + n;
+    CK(ensure(c->zout, sizeof(double) * (size_t)n * n));
+    double *dz = (double *)c->zout.p;
+    HK(hipMemcpy(dd, d, sizeof(double) * n, hipMemcpyHostToDevice));
+    HK(hipMemcpy(de, e, sizeof(double) * (n - 1), hipMemcpyHostToDevice));
+    info = hsdc_stedc_dev(c, n, dd, de, dz, n);
+    if (info) return info;
+    HK(hipMemcpy(d, dd, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (icompz == 2) {
+        HK(hipMemcpy2D(z, sizeof(double) * ldz, dz, sizeof(double) * n, sizeof(double) * n, n,
+                       hipMemcpyDeviceToHost));
+    } else if (icompz == 1) {
+        // COMPZ = 'V': Z := Zin * Q (the back-transformation of DSTEDC) on the device
+        CK(ensure(c->misc, sizeof(double) * 2 * (size_t)n * n));
+        double *za = (double *)c->misc.p, *zc = za + (size_t)n * n;
+        HK(hipMemcpy2D(za, sizeof(double) * n, z, sizeof(double) * ldz, sizeof(double) * n, n,
+                       hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_dgemm_nn, dim3(cdiv(n, 64), cdiv(n, 64)), dim3(WGS), 0, c->stream, n, n, n,
+                           (const double *)za, (long)n, (const double *)dz, (long)n, zc, (long)n);
+        HK(hipGetLastError());
+        HK(hipMemcpy2D(z, sizeof(double) * ldz, zc, sizeof(double) * n, sizeof(double) * n, n,
+                       hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// Internal types shared by the hsdc HIP kernels and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HSDC_EPS 0x1p-53 /* DLAMCH('Epsilon') */
+
+// One eigenproblem of the batch; its eigenvector block Q is n x n, column-major.
+struct Prob {
+    double *q;  // device base of Q
+    long ld;    // leading dimension (n)
+    int off;    // first global index of the problem
+    int n;
+    int par;    // ping-pong buffer holding its final state (= DC levels mod 2)
+};
+
+// One rank-one merge of one DC level (mdlaed0.f90:298-337): block [s, s+n) with
+// cut n1 and coupling rho = E(s+n1-1) (signed, before MDLAED2 rescales it).
+struct Merge {
+    int s, n, n1, prob;
+    double rho;
+};
+
+// A leaf of the DC tree (mdlaed0.f90:253-287).
+struct Leaf {
+    int s, m, prob;
+};
+
+// per-merge scalars written by the deflation kernel
+struct MergeOut {
+    int k;        // non-deflated count
+    int ctot[4];  // column types 1..4 (mdlaed2.f90:393-400)
+    int hss;      // 1 = HSS branch (k >= hss_min_k)
+    int nrot;     // Givens rotations recorded
+    int flags;    // bit0: rho*max|z| <= tol (K = 0 path)
+    double rho;   // |2 rho|
+    double tol;   // deflation tolerance
+};
+
+// workspace arrays indexed by global index (size ntot)
+struct Work {
+    double *Dv[2];   // eigenvalue of each virtual column (ping-pong)
+    int *vm[2];      // physical slot of each virtual column (ping-pong)
+    int *indxq;      // per block: sorted position -> local virtual column (1-based)
+    int *supp_lo, *supp_hi;  // row support [lo, hi) of each physical slot
+    double *z, *dlamda, *w, *ddef, *zs, *ds;
+    int *indxq2, *indx, *indxc, *indxp, *coltyp;
+    int *rot_pj, *rot_nj;
+    double *rot_c, *rot_s;
+    double *lam, *tau, *wnew, *alpha, *difl, *difr;
+    int *org, *nit;
+};

@@ -1,0 +1,889 @@
+// HIP kernels of the divide-and-conquer merge path for gfx950.
+//
+// Every kernel works on all merges of one DC level at once (a table of Merge
+// descriptors); launching it with a one-entry table gives the merge-at-a-time
+// schedule of the reference (mdlaed0.f90:298-337).
+//
+// Arithmetic that decides integer outcomes (deflation, pivoting) or feeds the
+// next level (roots, Loewner z) is written operation for operation like the CPU
+// oracle and compiled with -ffp-contract=off, so a merge given identical inputs
+// returns bit-identical roots, index sets and rotations.
+#include "hsdc_internal.h"
+
+#define WG 256
+
+__device__ __forceinline__ double lapy2_d(double x, double y)
+{
+    double xa = fabs(x), ya = fabs(y);
+    double w = xa > ya ? xa : ya, z = xa < ya ? xa : ya;
+    if (z == 0.0) return w;
+    double r = z / w;
+    return w * sqrt(1.0 + r * r);
+}
+
+__device__ __forceinline__ double qload(const Prob &P, const int *lo, const int *hi, int row, int slot)
+{
+    if (row < lo[slot] || row >= hi[slot]) return 0.0;
+    return P.q[(long)(row - P.off) + (long)(slot - P.off) * P.ld];
+}
+
+__device__ __forceinline__ int find_seg(const int *start, int n, int b)
+{
+    // largest m with start[m] <= b (start has n+1 entries, start[n] = total)
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Leaves: implicit-shift QL (the tql2 iteration) on each leaf of the DC tree
+// with eigenvectors, then selection sort (DSTEQR at mdlaed0.f90:263).  One wave
+// per leaf; the scalar recurrence is evaluated redundantly by every lane, lane r
+// owns row r of the leaf's eigenvector block in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_leaf(const Leaf *leaves, int nleaf, const Prob *probs,
+                                             const double *din, const double *ein, Work W, int cur)
+{
+    __shared__ double sd[64], se[64], sz[64 * 64];
+    int li = blockIdx.x;
+    if (li >= nleaf) return;
+    Leaf L = leaves[li];
+    Prob P = probs[L.prob];
+    const int n = L.m, r = threadIdx.x;
+    for (int i = r; i < n * n; i += 64) sz[i] = ((i % n) == (i / n)) ? 1.0 : 0.0;
+    if (r < n) {
+        sd[r] = din[L.s + r];
+        se[r] = (r < n - 1) ? ein[L.s + r] : 0.0;
+    }
+    __syncthreads();
+    int fail = 0;
+    for (int l = 0; l < n && !fail; l++) {
+        int iter = 0;
+        for (;;) {
+            int m;
+            for (m = l; m < n - 1; m++) {
+                double dd = fabs(sd[m]) + fabs(sd[m + 1]);
+                if (fabs(se[m]) <= HSDC_EPS * dd) break;
+            }
+            if (m == l) break;
+            if (++iter > 30 * n) { fail = 1; break; }
+            double g = (sd[l + 1] - sd[l]) / (2.0 * se[l]);
+            double rr = lapy2_d(g, 1.0);
+            g = sd[m] - sd[l] + se[l] / (g + (g >= 0.0 ? fabs(rr) : -fabs(rr)));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int underflow = 0;
+            for (int i = m - 1; i >= l; i--) {
+                double f = s * se[i], b = c * se[i];
+                rr = lapy2_d(f, g);
+                __syncthreads();
+                se[i + 1] = rr;
+                __syncthreads();
+                if (rr == 0.0) {
+                    double t = sd[i + 1] - p;
+                    __syncthreads();
+                    sd[i + 1] = t;
+                    se[m] = 0.0;
+                    __syncthreads();
+                    underflow = 1;
+                    break;
+                }
+                s = f / rr;
+                c = g / rr;
+                g = sd[i + 1] - p;
+                rr = (sd[i] - g) * s + 2.0 * c * b;
+                p = s * rr;
+                double dn = g + p;
+                __syncthreads();
+                sd[i + 1] = dn;
+                __syncthreads();
+                g = c * rr - b;
+                if (r < n) {
+                    double zi = sz[r + i * n], fz = sz[r + (i + 1) * n];
+                    sz[r + (i + 1) * n] = s * zi + c * fz;
+                    sz[r + i * n] = c * zi - s * fz;
+                }
+            }
+            if (underflow) continue;
+            double dl = sd[l] - p;
+            __syncthreads();
+            sd[l] = dl;
+            se[l] = g;
+            se[m] = 0.0;
+            __syncthreads();
+        }
+    }
+    // selection sort, ascending
+    for (int ii = 1; ii < n; ii++) {
+        int i = ii - 1, k = i;
+        double p = sd[i];
+        for (int j = ii; j < n; j++)
+            if (sd[j] < p) { k = j; p = sd[j]; }
+        if (k != i) {
+            double di = sd[i];
+            __syncthreads();
+            sd[k] = di;
+            sd[i] = p;
+            __syncthreads();
+            if (r < n) {
+                double t = sz[r + i * n];
+                sz[r + i * n] = sz[r + k * n];
+                sz[r + k * n] = t;
+            }
+        }
+    }
+    __syncthreads();
+    if (r < n) {
+        long base = (long)(L.s - P.off);
+        for (int c = 0; c < n; c++) P.q[base + r + (base + c) * P.ld] = sz[r + c * n];
+        W.Dv[cur][L.s + r] = sd[r];
+        W.vm[cur][L.s + r] = L.s + r;
+        W.indxq[L.s + r] = r + 1;
+        W.supp_lo[L.s + r] = L.s;
+        W.supp_hi[L.s + r] = L.s + n;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Merge set-up (mdlaed1.f90:178-180, mdlaed2.f90:222-245): z from the last row of
+// Q1 and the first row of Q2, sign flip for rho < 0, scaling by 1/sqrt(2),
+// INDXQ of the second half shifted by n1, DLAMDA = D(INDXQ).
+// Flattened over all merges: chunk_start[m] = first 256-element chunk of merge m.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_prep(const Merge *mg, const int *chunk_start, int nm,
+                                             const Prob *probs, Work W, int cur)
+{
+    int m = find_seg(chunk_start, nm, blockIdx.x);
+    Merge M = mg[m];
+    int i = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    if (i >= M.n) return;
+    int g = M.s + i;
+    double zz;
+    if (probs) {
+        Prob P = probs[M.prob];
+        int slot = W.vm[cur][g];
+        int row = (i < M.n1) ? (M.s + M.n1 - 1) : (M.s + M.n1);
+        zz = qload(P, W.supp_lo, W.supp_hi, row, slot);
+    } else {
+        zz = W.z[g];  // single-merge entry: z supplied by the caller
+    }
+    if (M.rho < 0.0 && i >= M.n1) zz = -1.0 * zz;
+    const double t = 1.0 / sqrt(2.0);
+    zz = t * zz;
+    W.z[g] = zz;
+    int q2 = W.indxq[g] + (i >= M.n1 ? M.n1 : 0);
+    W.indxq2[g] = q2;
+    W.dlamda[g] = W.Dv[cur][M.s + q2 - 1];
+}
+
+// Stable merge of two sorted runs (LAPACK DLAMRG semantics, ties to the first
+// run) by the merge path, one block per merge; falls back to the sequential
+// merge when a run is out of order so the permutation stays DLAMRG's.
+// a1 = base of run 1 read with stride s1 (s1 = -1 reads a1[n1-1 .. 0]).
+__device__ void block_merge(const double *a, int n1, int n2, int s1, int s2, int *out, int *sflag)
+{
+    int n = n1 + n2, tid = threadIdx.x;
+    auto A1 = [&](int p) { return s1 > 0 ? a[p] : a[n1 - 1 - p]; };
+    auto A2 = [&](int q) { return s2 > 0 ? a[n1 + q] : a[n - 1 - q]; };
+    auto I1 = [&](int p) { return s1 > 0 ? p + 1 : n1 - p; };
+    auto I2 = [&](int q) { return s2 > 0 ? n1 + q + 1 : n - q; };
+    if (tid == 0) *sflag = 0;
+    __syncthreads();
+    int bad = 0;
+    for (int p = tid; p + 1 < n1; p += blockDim.x) bad |= (A1(p) > A1(p + 1));
+    for (int q = tid; q + 1 < n2; q += blockDim.x) bad |= (A2(q) > A2(q + 1));
+    if (bad) atomicOr(sflag, 1);
+    __syncthreads();
+    if (*sflag) {
+        if (tid == 0) {
+            int i1 = 0, i2 = 0, o = 0;
+            while (i1 < n1 && i2 < n2) {
+                if (A1(i1) <= A2(i2)) out[o++] = I1(i1++);
+                else out[o++] = I2(i2++);
+            }
+            while (i1 < n1) out[o++] = I1(i1++);
+            while (i2 < n2) out[o++] = I2(i2++);
+        }
+    } else {
+        for (int p = tid; p < n1; p += blockDim.x) {
+            double v = A1(p);
+            int lo = 0, hi = n2;  // count of run-2 values strictly below v
+            while (lo < hi) { int md = (lo + hi) >> 1; if (A2(md) < v) lo = md + 1; else hi = md; }
+            out[p + lo] = I1(p);
+        }
+        for (int q = tid; q < n2; q += blockDim.x) {
+            double v = A2(q);
+            int lo = 0, hi = n1;  // count of run-1 values <= v
+            while (lo < hi) { int md = (lo + hi) >> 1; if (A1(md) <= v) lo = md + 1; else hi = md; }
+            out[q + lo] = I2(q);
+        }
+    }
+    __syncthreads();
+}
+
+// DLAMRG(N1,N2,DLAMDA,1,1,INDXC) and INDX = INDXQ(INDXC) (mdlaed2.f90:252-255)
+__global__ __launch_bounds__(WG) void k_sortmerge(const Merge *mg, Work W)
+{
+    __shared__ int sflag;
+    Merge M = mg[blockIdx.x];
+    block_merge(W.dlamda + M.s, M.n1, M.n - M.n1, 1, 1, W.indxc + M.s, &sflag);
+    for (int i = threadIdx.x; i < M.n; i += WG) W.indx[M.s + i] = W.indxq2[M.s + W.indxc[M.s + i] - 1];
+}
+
+// ---------------------------------------------------------------------------
+// Deflation (mdlaed2.f90:264-465), one block per merge.  The tolerance and the
+// small-z screen are parallel; the PJ/NJ scan that decides Givens deflations is
+// the sequential recurrence of the reference, run by one lane over sorted data
+// staged through LDS; the grouping of columns into types 1-4 is a parallel
+// stable partition.
+// ---------------------------------------------------------------------------
+#define DCH 1024
+__global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur, MergeOut *mo,
+                                                int hss_min_k)
+{
+    __shared__ double red_z[WG], red_d[WG];
+    __shared__ double cz[DCH], cd[DCH];
+    __shared__ int ci[DCH];
+    __shared__ int sk, sk2, scnt[4], tcnt[4][WG];
+    Merge M = mg[blockIdx.x];
+    const int n = M.n, n1 = M.n1, s = M.s, tid = threadIdx.x;
+    const double *D = W.Dv[cur] + s;
+    double *z = W.z + s;
+    int *indx = W.indx + s, *indxp = W.indxp + s, *coltyp = W.coltyp + s;
+    double mz = 0.0, md = 0.0;
+    for (int i = tid; i < n; i += WG) { mz = fmax(mz, fabs(z[i])); md = fmax(md, fabs(D[i])); }
+    red_z[tid] = mz; red_d[tid] = md;
+    __syncthreads();
+    for (int w = WG / 2; w > 0; w >>= 1) {
+        if (tid < w) { red_z[tid] = fmax(red_z[tid], red_z[tid + w]); red_d[tid] = fmax(red_d[tid], red_d[tid + w]); }
+        __syncthreads();
+    }
+    const double rho = fabs(2.0 * M.rho);
+    const double tol = 8.0 * HSDC_EPS * (red_d[0] > red_z[0] ? red_d[0] : red_z[0]);
+    MergeOut *o = mo + blockIdx.x;
+    if (rho * red_z[0] <= tol) {
+        if (tid == 0) {
+            o->k = 0; o->ctot[0] = o->ctot[1] = o->ctot[2] = o->ctot[3] = 0;
+            o->hss = 0; o->nrot = 0; o->flags = 1; o->rho = rho; o->tol = tol;
+        }
+        return;
+    }
+    for (int i = tid; i < n; i += WG) coltyp[i] = (i < n1) ? 1 : 3;
+    __syncthreads();
+    // sequential scan over sorted order
+    int K = 0, K2 = n + 1, PJ = 0, have = 0, nrot = 0;
+    double zP = 0.0, DP = 0.0;
+    double *ddef = W.ddef + s;  // ddef[pos-1] = D of the deflated column at INDXP(pos)
+    for (int c0 = 0; c0 < n; c0 += DCH) {
+        int cl = min(DCH, n - c0);
+        __syncthreads();
+        for (int j = tid; j < cl; j += WG) {
+            int nj = indx[c0 + j];
+            ci[j] = nj; cz[j] = z[nj - 1]; cd[j] = D[nj - 1];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int j = 0; j < cl; j++) {
+                int NJ = ci[j];
+                double zN = cz[j], dN = cd[j];
+                if (rho * fabs(zN) <= tol) {
+                    K2--; coltyp[NJ - 1] = 4; indxp[K2 - 1] = NJ; ddef[K2 - 1] = dN;
+                    continue;
+                }
+                if (!have) { PJ = NJ; zP = zN; DP = dN; have = 1; continue; }
+                double ss = zP, cc = zN;
+                double tau = lapy2_d(cc, ss);
+                double t = dN - DP;
+                cc = cc / tau;
+                ss = -ss / tau;
+                if (fabs(t * cc * ss) <= tol) {
+                    if (coltyp[NJ - 1] != coltyp[PJ - 1]) coltyp[NJ - 1] = 2;
+                    coltyp[PJ - 1] = 4;
+                    W.rot_pj[s + nrot] = PJ; W.rot_nj[s + nrot] = NJ;
+                    W.rot_c[s + nrot] = cc; W.rot_s[s + nrot] = ss;
+                    nrot++;
+                    double tP = DP * (cc * cc) + dN * (ss * ss);
+                    double dNn = DP * (ss * ss) + dN * (cc * cc);
+                    K2--;
+                    int i = 1;
+                    for (;;) {
+                        if (K2 + i <= n) {
+                            if (tP < ddef[K2 + i - 1]) {
+                                indxp[K2 + i - 2] = indxp[K2 + i - 1]; ddef[K2 + i - 2] = ddef[K2 + i - 1];
+                                indxp[K2 + i - 1] = PJ; ddef[K2 + i - 1] = tP;
+                                i++;
+                                continue;
+                            }
+                            indxp[K2 + i - 2] = PJ; ddef[K2 + i - 2] = tP;
+                        } else {
+                            indxp[K2 + i - 2] = PJ; ddef[K2 + i - 2] = tP;
+                        }
+                        break;
+                    }
+                    PJ = NJ; zP = tau; DP = dNn;
+                } else {
+                    K++;
+                    W.dlamda[s + K - 1] = DP; W.w[s + K - 1] = zP; indxp[K - 1] = PJ;
+                    PJ = NJ; zP = zN; DP = dN;
+                }
+            }
+        }
+    }
+    if (tid == 0) {
+        K++;
+        W.dlamda[s + K - 1] = DP; W.w[s + K - 1] = zP; indxp[K - 1] = PJ;
+        sk = K; sk2 = nrot;
+    }
+    __syncthreads();
+    // CTOT and the type grouping (mdlaed2.f90:393-431) as a stable partition
+    const int per = (n + WG - 1) / WG;
+    const int b0 = tid * per, b1 = min(n, b0 + per);
+    int cnt[4] = {0, 0, 0, 0};
+    for (int j = b0; j < b1; j++) cnt[coltyp[indxp[j] - 1] - 1]++;
+    for (int t = 0; t < 4; t++) tcnt[t][tid] = cnt[t];
+    __syncthreads();
+    if (tid < 4) {
+        int acc = 0;
+        for (int q = 0; q < WG; q++) { int v = tcnt[tid][q]; tcnt[tid][q] = acc; acc += v; }
+        scnt[tid] = acc;
+    }
+    __syncthreads();
+    K = n - scnt[3];
+    const int hss = (K >= hss_min_k);
+    if (!hss) {
+        int psm[4];
+        psm[0] = 0; psm[1] = scnt[0]; psm[2] = psm[1] + scnt[1]; psm[3] = psm[2] + scnt[2];
+        int pos[4];
+        for (int t = 0; t < 4; t++) pos[t] = psm[t] + tcnt[t][tid];
+        for (int j = b0; j < b1; j++) {
+            int js = indxp[j];
+            int ct = coltyp[js - 1] - 1;
+            indx[pos[ct]] = js;
+            W.indxc[s + pos[ct]] = j + 1;
+            pos[ct]++;
+        }
+    } else {
+        for (int j = b0; j < b1; j++) indx[j] = indxp[j];
+    }
+    if (tid == 0) {
+        o->k = K;
+        for (int t = 0; t < 4; t++) o->ctot[t] = scnt[t];
+        o->hss = hss; o->nrot = sk2; o->flags = 0; o->rho = rho; o->tol = tol;
+    }
+}
+
+// Zero the rows of the merge block outside the support of every column that a
+// Givens rotation touches, then widen its support to the block.
+__global__ __launch_bounds__(WG) void k_rot_prep(const Merge *mg, const Prob *probs, Work W, int cur,
+                                                 const MergeOut *mo)
+{
+    Merge M = mg[blockIdx.x];
+    int nrot = mo[blockIdx.x].nrot;
+    if (nrot == 0 || mo[blockIdx.x].flags) return;
+    Prob P = probs[M.prob];
+    for (int r = 0; r < 2 * nrot; r++) {
+        int j = (r & 1) ? W.rot_nj[M.s + (r >> 1)] : W.rot_pj[M.s + (r >> 1)];
+        int slot = W.vm[cur][M.s + j - 1];
+        int lo = W.supp_lo[slot], hi = W.supp_hi[slot];
+        __syncthreads();
+        if (lo == M.s && hi == M.s + M.n) continue;
+        double *col = P.q + (long)(slot - P.off) * P.ld - P.off;
+        for (int row = M.s + threadIdx.x; row < M.s + M.n; row += WG)
+            if (row < lo || row >= hi) col[row] = 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0) { W.supp_lo[slot] = M.s; W.supp_hi[slot] = M.s + M.n; }
+        __syncthreads();
+    }
+}
+
+// DROT of every recorded pair in order (mdlaed2.f90:352); rows are independent.
+__global__ __launch_bounds__(WG) void k_rotate(const Merge *mg, const int *chunk_start, int nm,
+                                               const Prob *probs, Work W, int cur, const MergeOut *mo)
+{
+    int m = find_seg(chunk_start, nm, blockIdx.x);
+    Merge M = mg[m];
+    int nrot = mo[m].nrot;
+    if (nrot == 0 || mo[m].flags) return;
+    int i = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    if (i >= M.n) return;
+    Prob P = probs[M.prob];
+    long row = M.s + i - P.off;
+    for (int r = 0; r < nrot; r++) {
+        int ps = W.vm[cur][M.s + W.rot_pj[M.s + r] - 1], ns = W.vm[cur][M.s + W.rot_nj[M.s + r] - 1];
+        double c = W.rot_c[M.s + r], sn = W.rot_s[M.s + r];
+        double *xp = P.q + row + (long)(ps - P.off) * P.ld;
+        double *yp = P.q + row + (long)(ns - P.off) * P.ld;
+        double x = *xp, y = *yp;
+        double tmp = c * x + sn * y;
+        *yp = c * y - sn * x;
+        *xp = tmp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Secular equation (replaces DLAED4, mdlaed3.f90:208 / DLAED3): the rational
+// "middle way" iteration with a bracketing safeguard, statement for statement as
+// the oracle's orc_laed4.  One lane per root; every lane of a block belongs to
+// the same merge (block_start pads each merge to whole blocks).
+// ---------------------------------------------------------------------------
+struct SecOut { double lam, tau, dl0, dl1; int org, nit, info; };
+
+__device__ SecOut sec_solve(int n, int j, const double *d, const double *z, double rho, int maxit)
+{
+    SecOut o;
+    o.dl0 = o.dl1 = 0.0; o.info = 0; o.nit = 0;
+    if (n == 1) {
+        o.lam = d[0] + rho * z[0] * z[0]; o.org = 1; o.tau = rho * z[0] * z[0];
+        return o;
+    }
+    if (n == 2) {
+        double del = d[1] - d[0], b, c, w, t, dl0, dl1;
+        if (j == 1) {
+            w = 1.0 + 2.0 * rho * (z[1] * z[1] - z[0] * z[0]) / del;
+            if (w > 0.0) {
+                b = del + rho * (z[0] * z[0] + z[1] * z[1]);
+                c = rho * z[0] * z[0] * del;
+                t = 2.0 * c / (b + sqrt(fabs(b * b - 4.0 * c)));
+                o.lam = d[0] + t; o.org = 1; o.tau = t;
+                dl0 = -z[0] / t; dl1 = z[1] / (del - t);
+            } else {
+                b = -del + rho * (z[0] * z[0] + z[1] * z[1]);
+                c = rho * z[1] * z[1] * del;
+                if (b > 0.0) t = -2.0 * c / (b + sqrt(b * b + 4.0 * c));
+                else t = (b - sqrt(b * b + 4.0 * c)) / 2.0;
+                o.lam = d[1] + t; o.org = 2; o.tau = t;
+                dl0 = -z[0] / (del + t); dl1 = -z[1] / t;
+            }
+        } else {
+            b = -del + rho * (z[0] * z[0] + z[1] * z[1]);
+            c = rho * z[1] * z[1] * del;
+            if (b > 0.0) t = (b + sqrt(b * b + 4.0 * c)) / 2.0;
+            else t = 2.0 * c / (-b + sqrt(b * b + 4.0 * c));
+            o.lam = d[1] + t; o.org = 2; o.tau = t;
+            dl0 = -z[0] / (del + t); dl1 = -z[1] / t;
+        }
+        double sn = sqrt(dl0 * dl0 + dl1 * dl1);
+        o.dl0 = dl0 / sn; o.dl1 = dl1 / sn;
+        return o;
+    }
+    const double rhoinv = 1.0 / rho;
+    int ii;
+    double t, lo, hi;
+    if (j == n) {
+        double zz = 0.0;
+        for (int i = 0; i < n; i++) zz += z[i] * z[i];
+        double hh = rho * zz, mid = 0.5 * hh, dn = d[n - 1];
+        double c = rhoinv;
+        for (int i = 0; i < n - 2; i++) c += z[i] * z[i] / ((d[i] - dn) - mid);
+        double dl = dn - d[n - 2];
+        double z1 = z[n - 2] * z[n - 2], z2 = z[n - 1] * z[n - 1];
+        if (c <= 0.0) {
+            t = hh;
+        } else {
+            double b = c * dl - z1 - z2;
+            double sq = sqrt(fabs(b * b + 4.0 * c * z2 * dl));
+            if (b <= 0.0) t = (sq - b) / (2.0 * c);
+            else t = 2.0 * z2 * dl / (b + sq);
+        }
+        if (!(t > 0.0 && t <= hh)) t = mid;
+        ii = n; lo = 0.0; hi = hh;
+    } else {
+        double del = d[j] - d[j - 1], mid = 0.5 * del;
+        double c = rhoinv;
+        for (int i = 0; i < n; i++) {
+            if (i == j - 1 || i == j) continue;
+            c += z[i] * z[i] / ((d[i] - d[j - 1]) - mid);
+        }
+        double zl = z[j - 1] * z[j - 1], zr = z[j] * z[j];
+        double fmid = c + (zr - zl) / mid;
+        if (fmid >= 0.0) {
+            double b = c * del + zl + zr, cc = zl * del;
+            double sq = sqrt(fabs(b * b - 4.0 * c * cc));
+            if (b > 0.0) t = 2.0 * cc / (b + sq);
+            else t = (b - sq) / (2.0 * c);
+            if (!(t > 0.0 && t <= mid)) t = 0.5 * mid;
+            ii = j; lo = 0.0; hi = mid;
+        } else {
+            double b = c * del - zl - zr, cc = zr * del;
+            double sq = sqrt(fabs(b * b + 4.0 * c * cc));
+            if (b <= 0.0) t = -2.0 * cc / (sq - b);
+            else t = -(b + sq) / (2.0 * c);
+            if (!(t >= -mid && t < 0.0)) t = -0.5 * mid;
+            ii = j + 1; lo = -mid; hi = 0.0;
+        }
+    }
+    const double dorg = d[ii - 1];
+    const int split = (j == n) ? n - 1 : j;
+    int it, info = 1;
+    for (it = 0; it <= maxit; it++) {
+        double psi = 0.0, dpsi = 0.0, epsi = 0.0, phi = 0.0, dphi = 0.0, ephi = 0.0;
+        for (int i = 0; i < split; i++) {
+            if (i == ii - 1) continue;
+            double dl = (d[i] - dorg) - t;
+            double q = z[i] / dl;
+            psi += z[i] * q;
+            dpsi += q * q;
+            epsi += psi;
+        }
+        for (int i = n - 1; i >= split; i--) {
+            if (i == ii - 1) continue;
+            double dl = (d[i] - dorg) - t;
+            double q = z[i] / dl;
+            phi += z[i] * q;
+            dphi += q * q;
+            ephi += phi;
+        }
+        double qo = z[ii - 1] / (-t);
+        double torg = z[ii - 1] * qo;
+        double w = ((rhoinv + phi) + psi) + torg;
+        double erretm = 8.0 * (phi - psi) + (ephi - epsi) + 2.0 * rhoinv + 3.0 * fabs(torg) +
+                        fabs(t) * ((dpsi + dphi) + qo * qo);
+        if (ii - 1 < split) dpsi += qo * qo; else dphi += qo * qo;
+        if (fabs(w) <= HSDC_EPS * erretm || it == maxit) {
+            if (fabs(w) <= HSDC_EPS * erretm) info = 0;
+            break;
+        }
+        if (w > 0.0) hi = t; else lo = t;
+        double dl = (d[split - 1] - dorg) - t;
+        double dr = (d[split] - dorg) - t;
+        double a = (dl + dr) * w - dl * dr * (dpsi + dphi);
+        double b = dl * dr * w;
+        double c = w - dl * dpsi - dr * dphi;
+        double eta;
+        if (c == 0.0) {
+            eta = (a != 0.0) ? b / a : -w / (dpsi + dphi);
+        } else {
+            double sq = sqrt(fabs(a * a - 4.0 * b * c));
+            if (a <= 0.0) eta = (a - sq) / (2.0 * c);
+            else eta = 2.0 * b / (a + sq);
+        }
+        if (w * eta >= 0.0) eta = -w / (dpsi + dphi);
+        double tn = t + eta;
+        if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+        if (tn == t) { info = 0; break; }
+        t = tn;
+    }
+    o.nit = it + 1;
+    o.lam = dorg + t; o.org = ii; o.tau = t; o.info = info;
+    return o;
+}
+
+__global__ __launch_bounds__(64) void k_secular(const Merge *mg, const int *block_start, int nm,
+                                                const MergeOut *mo, Work W, int maxit, int *info)
+{
+    int m = find_seg(block_start, nm, blockIdx.x);
+    const MergeOut o = mo[m];
+    int j = (blockIdx.x - block_start[m]) * 64 + threadIdx.x + 1;
+    if (j > o.k || o.flags) return;
+    int s = mg[m].s;
+    SecOut r = sec_solve(o.k, j, W.dlamda + s, W.w + s, o.rho, maxit);
+    W.lam[s + j - 1] = r.lam;
+    W.tau[s + j - 1] = r.tau;
+    W.org[s + j - 1] = r.org;
+    W.nit[s + j - 1] = r.nit;
+    if (o.k == 2) { W.difl[s + j - 1] = r.dl0; W.difr[s + j - 1] = r.dl1; }
+    if (r.info) atomicOr(info, 1);
+}
+
+// delta_i(j) = d_i - lambda_j, formed as the secular solver forms it
+__device__ __forceinline__ double delta_ij(const double *d, int i, const int *org, const double *tau, int j)
+{
+    return (d[i] - d[org[j] - 1]) - tau[j];
+}
+
+// Loewner z of the dense branch, LAPACK DLAED3 order: W(I) = DELTA_I(I) times
+// DELTA_I(J)/(DLAMDA(I)-DLAMDA(J)) for J ascending, z = sign(sqrt(-W), w).
+// Also the HSS branch (MDLAED3, mdlaed3.f90:207-230): product from 1 with the
+// J = I factor taken in place, z = sign(sqrt|prod|, w).
+__global__ __launch_bounds__(WG) void k_loewner(const Merge *mg, const int *chunk_start, int nm,
+                                                const MergeOut *mo, Work W)
+{
+    int m = find_seg(chunk_start, nm, blockIdx.x);
+    const MergeOut o = mo[m];
+    int i = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    if (o.flags || o.k < 3 || i >= o.k) return;
+    int s = mg[m].s, K = o.k;
+    const double *d = W.dlamda + s, *tau = W.tau + s;
+    const int *org = W.org + s;
+    double di = d[i], wv;
+    if (!o.hss) {
+        wv = delta_ij(d, i, org, tau, i);
+        for (int jj = 0; jj < K; jj++) {
+            if (jj == i) continue;
+            wv = wv * (delta_ij(d, i, org, tau, jj) / (di - d[jj]));
+        }
+        W.wnew[s + i] = copysign(sqrt(-wv), W.w[s + i]);
+    } else {
+        wv = 1.0;
+        for (int jj = 0; jj < K; jj++) {
+            if (jj == i) wv = wv * delta_ij(d, i, org, tau, i);
+            else wv = wv * (delta_ij(d, i, org, tau, jj) / (di - d[jj]));
+        }
+        W.wnew[s + i] = copysign(sqrt(fabs(wv)), W.w[s + i]);
+    }
+}
+
+// Dense branch eigenvector matrix of the secular problem (DLAED3, after label
+// 70): S(t,j) = (W(I)/DELTA_I(J)) / ||W./DELTA(:,J)||, I = INDXC(t); written
+// row-major (S[t*K + j]) so one lane per column stores coalesced.
+__global__ __launch_bounds__(WG) void k_sbuild(const Merge *mg, const int *chunk_start, int nm,
+                                               const MergeOut *mo, const long *soff, Work W, double *S)
+{
+    int m = find_seg(chunk_start, nm, blockIdx.x);
+    const MergeOut o = mo[m];
+    int j = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    if (o.flags || o.hss || j >= o.k) return;
+    int s = mg[m].s, K = o.k;
+    double *Sm = S + soff[m];
+    const int *indxc = W.indxc + s;
+    if (K == 1) { Sm[0] = 1.0; return; }
+    if (K == 2) {
+        double dv[2] = {W.difl[s + j], W.difr[s + j]};
+        for (int t = 0; t < 2; t++) Sm[t * K + j] = dv[indxc[t] - 1];
+        return;
+    }
+    const double *d = W.dlamda + s, *tau = W.tau + s, *wz = W.wnew + s;
+    const int *org = W.org + s;
+    double acc = 0.0;
+    for (int i = 0; i < K; i++) {
+        double v = wz[i] / delta_ij(d, i, org, tau, j);
+        acc += v * v;
+    }
+    double temp = sqrt(acc);
+    for (int t = 0; t < K; t++) {
+        int i = indxc[t] - 1;
+        Sm[(long)t * K + j] = (wz[i] / delta_ij(d, i, org, tau, j)) / temp;
+    }
+}
+
+// HSS branch scalars (mdlaed3.f90:215-249): DIFL, DIFR, alpha_j = 1/||col j||.
+__global__ __launch_bounds__(WG) void k_alpha(const Merge *mg, const int *chunk_start, int nm,
+                                              const MergeOut *mo, Work W)
+{
+    int m = find_seg(chunk_start, nm, blockIdx.x);
+    const MergeOut o = mo[m];
+    int j = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    if (o.flags || !o.hss || j >= o.k) return;
+    int s = mg[m].s, K = o.k;
+    const double *d = W.dlamda + s, *tau = W.tau + s, *wz = W.wnew + s;
+    const int *org = W.org + s;
+    double difl = -delta_ij(d, j, org, tau, j);
+    double difr = (j + 1 < K) ? -delta_ij(d, j + 1, org, tau, j) : 0.0;
+    W.difl[s + j] = difl;
+    W.difr[s + j] = difr;
+    double dlamj = -d[j], difrj = 0.0, dlamjp = 0.0;
+    if (j + 1 < K) { difrj = -difr; dlamjp = -d[j + 1]; }
+    double acc = 0.0;
+    for (int i = 0; i < K; i++) {
+        double v;
+        if (i == j) v = wz[j] / difl;
+        else if (i < j) v = wz[i] / ((d[i] + dlamj) - difl);
+        else v = wz[i] / ((d[i] + dlamjp) + difrj);
+        acc += v * v;
+    }
+    W.alpha[s + j] = 1.0 / sqrt(acc);
+}
+
+// ---------------------------------------------------------------------------
+// Dense branch eigenvector update on FP64 MFMA (DLAED3's two DGEMMs):
+//   rows of Q1: Ct(0:n1, :)  = Q2(0:n1, types 1,2) * S(types 1,2, :)
+//   rows of Q2: Ct(n1:n, :)  = Q2(n1:n, types 2,3) * S(types 2,3, :)
+// The A operand is gathered straight from the physical eigenvector columns
+// (no packed Q2 copy), zero outside each column's support.
+// Work item = one 64x64 output tile; 4 waves, each 32x32 as 2x2 MFMA 16x16x4.
+// ---------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+struct GTile { int m, half, r0, c0; };
+
+__global__ __launch_bounds__(WG) void k_gemm(const Merge *mg, const GTile *tiles, const Prob *probs,
+                                             const MergeOut *mo, const long *soff, const long *coff,
+                                             Work W, int cur, const double *S, double *Ct)
+{
+    __shared__ double As[16][64 + 1];
+    __shared__ double Bs[16][64 + 1];
+    GTile T = tiles[blockIdx.x];
+    Merge M = mg[T.m];
+    const MergeOut o = mo[T.m];
+    Prob P = probs[M.prob];
+    const int K = o.k, n1 = M.n1, n2 = M.n - M.n1;
+    const int nrow = T.half ? n2 : n1;
+    const int rowg0 = M.s + (T.half ? n1 : 0);
+    const int kdim = T.half ? (o.ctot[1] + o.ctot[2]) : (o.ctot[0] + o.ctot[1]);
+    const int tbase = T.half ? o.ctot[0] : 0;  // first grouped column / S row used
+    const double *Sm = S + soff[T.m];
+    const int *indx = W.indx + M.s;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+    d4 acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < kdim; k0 += 16) {
+        for (int e = 0; e < 4; e++) {
+            int idx = tid + WG * e;
+            int r = idx & 63, kk = idx >> 6;
+            double v = 0.0;
+            int t = k0 + kk, row = T.r0 + r;
+            if (t < kdim && row < nrow) {
+                int slot = W.vm[cur][M.s + indx[tbase + t] - 1];
+                v = qload(P, W.supp_lo, W.supp_hi, rowg0 + row, slot);
+            }
+            As[kk][r] = v;
+            int c = idx & 63;
+            double b = 0.0;
+            int col = T.c0 + c;
+            if (t < kdim && col < K) b = Sm[(long)(tbase + t) * K + col];
+            Bs[kk][c] = b;
+        }
+        __syncthreads();
+        for (int kq = 0; kq < 16; kq += 4) {
+            int kl = kq + (lane >> 4);
+            double a0 = As[kl][wr + (lane & 15)];
+            double a1 = As[kl][wr + 16 + (lane & 15)];
+            double b0 = Bs[kl][wc + (lane & 15)];
+            double b1 = Bs[kl][wc + 16 + (lane & 15)];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    double *C = Ct + coff[T.m];
+    const int rowoff = T.half ? n1 : 0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int i = 0; i < 4; i++) {
+                int row = T.r0 + wr + a * 16 + (lane >> 4) + 4 * i;
+                int col = T.c0 + wc + b * 16 + (lane & 15);
+                if (row < nrow && col < K) C[(long)(rowoff + row) + (long)col * M.n] = acc[a][b][i];
+            }
+}
+
+// ---------------------------------------------------------------------------
+// New column bookkeeping of the merged block: virtual column v < K is root v
+// (its vector from Ct), v >= K the deflated column INDXP(v) kept in place.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_newcols(const Merge *mg, const int *chunk_start, int nm,
+                                                const MergeOut *mo, Work W, int cur)
+{
+    int m = find_seg(chunk_start, nm, blockIdx.x);
+    Merge M = mg[m];
+    const MergeOut o = mo[m];
+    int v = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    if (v >= M.n) return;
+    int g = M.s + v, nxt = cur ^ 1;
+    if (o.flags) {
+        int src = W.indx[g];
+        W.vm[nxt][g] = W.vm[cur][M.s + src - 1];
+        W.Dv[nxt][g] = W.Dv[cur][M.s + src - 1];
+        W.indxq[g] = v + 1;
+        return;
+    }
+    int src = W.indxp[g];
+    W.vm[nxt][g] = W.vm[cur][M.s + src - 1];
+    W.Dv[nxt][g] = (v < o.k) ? W.lam[g] : W.ddef[g];
+}
+
+// copy the K updated eigenvectors from Ct into their physical slots
+__global__ __launch_bounds__(WG) void k_scatter(const Merge *mg, const int *col_start, int nm,
+                                                const MergeOut *mo, const long *coff, const Prob *probs,
+                                                Work W, int nxt, const double *Ct)
+{
+    int m = find_seg(col_start, nm, blockIdx.x);
+    Merge M = mg[m];
+    int v = blockIdx.x - col_start[m];
+    if (v >= mo[m].k) return;
+    Prob P = probs[M.prob];
+    int slot = W.vm[nxt][M.s + v];
+    const double *src = Ct + coff[m] + (long)v * M.n;
+    double *dst = P.q + (long)(slot - P.off) * P.ld + (M.s - P.off);
+    for (int r = threadIdx.x; r < M.n; r += WG) dst[r] = src[r];
+    if (threadIdx.x == 0) { W.supp_lo[slot] = M.s; W.supp_hi[slot] = M.s + M.n; }
+}
+
+// INDXQ of the merged block: DLAMRG(K, N-K, D, 1, -1) (mdlaed1.f90:221)
+__global__ __launch_bounds__(WG) void k_lamrg(const Merge *mg, const MergeOut *mo, Work W, int nxt)
+{
+    __shared__ int sflag;
+    Merge M = mg[blockIdx.x];
+    const MergeOut o = mo[blockIdx.x];
+    if (o.flags) return;  // identity, set by k_newcols
+    block_merge(W.Dv[nxt] + M.s, o.k, M.n - o.k, 1, -1, W.indxq + M.s, &sflag);
+}
+
+// ---------------------------------------------------------------------------
+// Final reordering (mdlaed0.f90:355-362 and the scale-back of mdstedc.f90:358):
+// Z(:, i) = Q(:, slot of INDXQ(i)), zero outside the column's support.
+// One block per output column.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_prob, Work W, int cur,
+                                              const int *outcol, double *Z, const long *zoff,
+                                              const long *ldz, int zrows_global, double *dout,
+                                              const double *scale)
+{
+    int g = blockIdx.x;  // global column
+    int p = col_prob[g];
+    Prob P = probs[p];
+    int i = g - P.off;
+    int v = W.indxq[g];  // local virtual column (1-based)
+    int vg = P.off + v - 1;
+    int slot = W.vm[P.par][vg];
+    int oc = outcol ? outcol[g] : g;  // output column (global numbering)
+    if (threadIdx.x == 0) dout[oc] = W.Dv[P.par][vg] * scale[p];
+    const int lo = W.supp_lo[slot], hi = W.supp_hi[slot];
+    const double *src = P.q + (long)(slot - P.off) * P.ld - P.off;
+    double *dst;
+    int r0, r1;
+    if (zrows_global) {  // one n x n output holding every problem (block diagonal)
+        dst = Z + (long)oc * ldz[0];
+        r0 = 0; r1 = zrows_global;
+    } else {             // one output block per problem
+        dst = Z + zoff[p] + (long)(oc - P.off) * ldz[p] - P.off;
+        r0 = P.off; r1 = P.off + P.n;
+    }
+    for (int r = r0 + threadIdx.x; r < r1; r += WG) dst[r] = (r >= lo && r < hi) ? src[r] : 0.0;
+    (void)i;
+}
+
+// Plain C = A * B (column-major) on FP64 MFMA, for COMPZ = 'V' (Z := Zin * Q).
+__global__ __launch_bounds__(WG) void k_dgemm_nn(int m, int n, int k, const double *A, long lda,
+                                                 const double *B, long ldb, double *C, long ldc)
+{
+    __shared__ double As[16][64 + 1];
+    __shared__ double Bs[16][64 + 1];
+    const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+    d4 acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < k; k0 += 16) {
+        for (int e = 0; e < 4; e++) {
+            int idx = tid + WG * e;
+            int r = idx & 63, kk = idx >> 6;
+            int t = k0 + kk;
+            As[kk][r] = (t < k && r0 + r < m) ? A[(long)(r0 + r) + (long)t * lda] : 0.0;
+            Bs[kk][r] = (t < k && c0 + r < n) ? B[(long)t + (long)(c0 + r) * ldb] : 0.0;
+        }
+        __syncthreads();
+        for (int kq = 0; kq < 16; kq += 4) {
+            int kl = kq + (lane >> 4);
+            double a0 = As[kl][wr + (lane & 15)], a1 = As[kl][wr + 16 + (lane & 15)];
+            double b0 = Bs[kl][wc + (lane & 15)], b1 = Bs[kl][wc + 16 + (lane & 15)];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int i = 0; i < 4; i++) {
+                int row = r0 + wr + a * 16 + (lane >> 4) + 4 * i;
+                int col = c0 + wc + b * 16 + (lane & 15);
+                if (row < m && col < n) C[(long)row + (long)col * ldc] = acc[a][b][i];
+            }
+}

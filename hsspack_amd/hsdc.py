@@ -1,0 +1,189 @@
+"""Python host mirror of the hsdc C ABI (include/hsdc.h).
+
+Mirrors HSSPACK's LAPACK-style interface for the tridiagonal DC path:
+``stedc`` is MDSTEDC (SRC/mdstedc.f90:139) with the same argument meaning and
+INFO convention; ``deflate`` / ``secular`` expose the MDLAED2 deflation and the
+DLAED4 secular solve of one merge.  Device memory is owned by torch tensors or by
+the library's context.  The library must be built (hsspack_amd/build.py); there is
+no fallback: a missing libhsdc.so raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhsdc.so")
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_P = ctypes.POINTER
+_vp = ctypes.c_void_p
+
+HSDC_ERR_HIP = -1000
+HSDC_ERR_NOMEM = -1001
+HSDC_ERR_NODEVICE = -1002
+
+# every symbol include/hsdc.h declares
+EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_set_params",
+           "hsdc_get_stats", "hsdc_set_stream", "hsdc_stedc", "hsdc_stedc_dev",
+           "hsdc_stedc_batched_dev", "hsdc_deflate", "hsdc_secular", "hsdc_laed3", "hsdc_loewner",
+           "hsdc_hss_evc", "hsdc_hss_dense"]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("smlsiz", _i), ("hss_min_k", _i), ("hss_leaf", _i), ("hss_tol", _d),
+                ("maxit", _i), ("batched", _i), ("learned", _i)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("levels", _i), ("merges", _i), ("hss_merges", _i), ("roots", ctypes.c_longlong),
+                ("secular_iters", ctypes.c_longlong), ("deflated", ctypes.c_longlong),
+                ("k_top", _i), ("rank_max", _i), ("ms_total", _d)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libhsdc.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("hsspack_amd/libhsdc.so is not built (run hsspack_amd/build.py)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.hsdc_ctx_create.argtypes = [_P(_vp)]
+        L.hsdc_ctx_destroy.argtypes = [_vp]
+        L.hsdc_set_params.argtypes = [_vp, _P(Params)]
+        L.hsdc_get_stats.argtypes = [_vp, _P(Stats)]
+        L.hsdc_set_stream.argtypes = [_vp, _vp]
+        L.hsdc_stedc.argtypes = [_vp, ctypes.c_char, _i, _P(_d), _P(_d), _P(_d), _i, _P(_d), _i,
+                                 _P(_i), _i]
+        L.hsdc_stedc_dev.argtypes = [_vp, _i, _vp, _vp, _vp, _i]
+        L.hsdc_stedc_batched_dev.argtypes = [_vp, _i, _P(_i), _P(ctypes.c_long), _vp, _vp, _vp,
+                                             _P(ctypes.c_long)]
+        L.hsdc_deflate.argtypes = [_vp, _i, _i, _P(_d), _P(_d), _P(_i), _d, _i, _P(_i), _P(_d),
+                                   _P(_d), _P(_d), _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _P(_i),
+                                   _P(_i), _P(_d), _P(_d)]
+        L.hsdc_secular.argtypes = [_vp, _i, _P(_d), _P(_d), _d, _P(_d), _P(_i), _P(_d), _P(_i)]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P(_d))
+
+
+def _ip(a):
+    return a.ctypes.data_as(_P(_i))
+
+
+def default_params():
+    p = Params()
+    lib().hsdc_default_params(ctypes.byref(p))
+    return p
+
+
+class Context:
+    """One library context (device buffers + parameters); not thread safe."""
+
+    def __init__(self, **params):
+        self._h = _vp()
+        rc = lib().hsdc_ctx_create(ctypes.byref(self._h))
+        if rc != 0:
+            raise RuntimeError("hsdc_ctx_create failed: %d" % rc)
+        self.params = default_params()
+        self.set_params(**params)
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        rc = lib().hsdc_set_params(self._h, ctypes.byref(self.params))
+        if rc != 0:
+            raise ValueError("hsdc_set_params rejected %r (%d)" % (kw, rc))
+
+    def stats(self):
+        s = Stats()
+        lib().hsdc_get_stats(self._h, ctypes.byref(s))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().hsdc_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- MDSTEDC (host arrays) ----
+    def stedc(self, d, e, compz="I", z=None, lwork=None, liwork=None):
+        """Returns (d, z, info) like MDSTEDC; d and e are copied."""
+        n = len(d)
+        d = np.array(d, np.float64)
+        e = np.array(e, np.float64) if n > 1 else np.zeros(1)
+        ldz = max(1, n)
+        if z is None:
+            z = np.zeros((n, n), order="F")
+        else:
+            z = np.array(z, np.float64, order="F")
+        if lwork is None:
+            lwork = 1 + 4 * n + n * n if compz != "V" else 1 + 3 * n + 2 * n * 17 + 4 * n * n
+        if liwork is None:
+            liwork = 3 + 5 * n if compz != "V" else 6 + 6 * n + 5 * n * 17
+        work = np.zeros(max(1, min(lwork, 4)) if lwork > 0 else 1)
+        iwork = np.zeros(1, np.int32)
+        info = lib().hsdc_stedc(self._h, ctypes.c_char(compz.encode()), n, _dp(d), _dp(e),
+                                _dp(z) if n else None, ldz, _dp(work), lwork, _ip(iwork), liwork)
+        return d, z, info
+
+    # ---- device-resident MDSTEDC on torch tensors ----
+    def stedc_dev(self, d, e, z):
+        """d (n), e (n-1 or n), z (n x n, column-major i.e. z.T contiguous) on cuda."""
+        n = d.shape[0]
+        rc = lib().hsdc_stedc_dev(self._h, n, d.data_ptr(), e.data_ptr(), z.data_ptr(), n)
+        return rc
+
+    def stedc_batched_dev(self, ns, offs, d, e, z, zoffs):
+        ns = np.ascontiguousarray(ns, np.int32)
+        offs = np.ascontiguousarray(offs, np.int64)
+        zoffs = np.ascontiguousarray(zoffs, np.int64)
+        return lib().hsdc_stedc_batched_dev(self._h, len(ns), _ip(ns),
+                                            offs.ctypes.data_as(_P(ctypes.c_long)), d.data_ptr(),
+                                            e.data_ptr(), z.data_ptr(),
+                                            zoffs.ctypes.data_as(_P(ctypes.c_long)))
+
+    # ---- one merge ----
+    def deflate(self, d, z, indxq, rho, n1, hss_min_k=1 << 30):
+        n = len(d)
+        d = np.array(d, np.float64)
+        z = np.array(z, np.float64)
+        indxq = np.array(indxq, np.int32)
+        out = dict(dlamda=np.zeros(n), w=np.zeros(n), indxp=np.zeros(n, np.int32),
+                   indx=np.zeros(n, np.int32), indxc=np.zeros(n, np.int32),
+                   coltyp=np.zeros(max(n, 4), np.int32), rot_pj=np.zeros(n, np.int32),
+                   rot_nj=np.zeros(n, np.int32), rot_c=np.zeros(n), rot_s=np.zeros(n))
+        k, nrot = _i(0), _i(0)
+        rho_o = _d(0)
+        info = lib().hsdc_deflate(self._h, n, n1, _dp(d), _dp(z), _ip(indxq), rho, hss_min_k,
+                                  ctypes.byref(k), ctypes.byref(rho_o), _dp(out["dlamda"]),
+                                  _dp(out["w"]), _ip(out["indxp"]), _ip(out["indx"]),
+                                  _ip(out["indxc"]), _ip(out["coltyp"]), ctypes.byref(nrot),
+                                  _ip(out["rot_pj"]), _ip(out["rot_nj"]), _dp(out["rot_c"]),
+                                  _dp(out["rot_s"]))
+        r = nrot.value
+        out.update(info=info, k=k.value, rho=rho_o.value, d=d, indxq=indxq,
+                   rot=(out["rot_pj"][:r], out["rot_nj"][:r], out["rot_c"][:r], out["rot_s"][:r]))
+        return out
+
+    def secular(self, dlamda, w, rho):
+        k = len(dlamda)
+        dl = np.ascontiguousarray(dlamda, np.float64)
+        ww = np.ascontiguousarray(w, np.float64)
+        lam, tau = np.zeros(k), np.zeros(k)
+        org, nit = np.zeros(k, np.int32), np.zeros(k, np.int32)
+        info = lib().hsdc_secular(self._h, k, _dp(dl), _dp(ww), rho, _dp(lam), _ip(org), _dp(tau),
+                                  _ip(nit))
+        return dict(lam=lam, org=org, tau=tau, niter=nit, info=info)

@@ -1,0 +1,120 @@
+/*
+ * hsdc -- MI355X (gfx950) divide-and-conquer symmetric tridiagonal eigensolver:
+ * the merge path of HSSPACK's MDSTEDC (SRC/mdstedc.f90 -> mdlaed0 -> mdlaed1 ->
+ * mdlaed2 -> mdlaed3 -> DHSSEVC -> fastHssmmL) as HIP kernels behind a C ABI.
+ *
+ * Conventions
+ *   - Plain pointers and sizes; int32 sizes; FP64 values; column-major matrices.
+ *   - Index arrays carry 1-based values, as the Fortran routines they replace.
+ *   - Return value: 0 on success, -i when argument i is illegal (the LAPACK
+ *     convention of the routine replaced), > 0 for a numerical failure encoded as
+ *     the replaced routine encodes it, and HSDC_ERR_* (below) for runtime errors.
+ *   - Functions named *_dev take device pointers; all others take host pointers
+ *     and stage through device memory owned by the context.
+ *   - One context per host thread; calls on one context are serialised.
+ */
+#ifndef HSDC_H
+#define HSDC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* runtime failures, disjoint from every INFO the replaced routines return */
+#define HSDC_ERR_HIP       (-1000) /* a HIP call failed */
+#define HSDC_ERR_NOMEM     (-1001) /* device allocation failed */
+#define HSDC_ERR_NODEVICE  (-1002) /* no gfx950 device */
+#define HSDC_ERR_INTERNAL  (-1003)
+
+typedef struct hsdc_ctx hsdc_ctx;
+
+/* Tunables.  Defaults are HSSPACK's compiled-in constants. */
+typedef struct {
+    int smlsiz;      /* DC leaf bound, ILAENV(9,'DSTEDC') = 25 (mdstedc.f90:211) */
+    int hss_min_k;   /* K >= hss_min_k takes the HSS branch; BSMLZ = 1500 (mdlaed2.f90:175) */
+    int hss_leaf;    /* HSS leaf block, Ni = 200 (mdlaed3.f90:253) */
+    double hss_tol;  /* HSS compression tolerance, TOL = 1e-17 (CauchyHssEig.f90:2523) */
+    int maxit;       /* secular iteration cap */
+    int batched;     /* 1: one launch per DC level over all merges; 0: one merge at a time */
+    int learned;     /* 1: MLP proposes the secular starting iterate */
+} hsdc_params;
+
+/* Run statistics of the last call on a context. */
+typedef struct {
+    int levels, merges, hss_merges;
+    long long roots, secular_iters;
+    long long deflated;
+    int k_top;          /* K of the last (top) merge */
+    int rank_max;       /* largest HSS generator rank */
+    double ms_total;    /* device time of the solve (events), ms */
+} hsdc_stats;
+
+void hsdc_default_params(hsdc_params *p);
+int  hsdc_ctx_create(hsdc_ctx **ctx);
+int  hsdc_ctx_destroy(hsdc_ctx *ctx);
+int  hsdc_set_params(hsdc_ctx *ctx, const hsdc_params *p);
+int  hsdc_get_stats(hsdc_ctx *ctx, hsdc_stats *s);
+/* HIP stream used by the context (hipStream_t); 0 = the null stream. */
+int  hsdc_set_stream(hsdc_ctx *ctx, void *stream);
+
+/* MDSTEDC (mdstedc.f90:139).  compz 'N', 'I' or 'V'; d(n), e(n-1), z(ldz,n) host
+ * arrays; work/iwork follow the LAPACK workspace contract (lwork = -1 or
+ * liwork = -1 is a size query that returns work[0], iwork[0]). */
+int hsdc_stedc(hsdc_ctx *ctx, char compz, int n, double *d, double *e, double *z, int ldz,
+               double *work, int lwork, int *iwork, int liwork);
+
+/* Device-resident MDSTEDC with COMPZ = 'I': d_d(n), d_e(n-1), d_z(ldz,n) device
+ * arrays; eigenvalues overwrite d_d, e is destroyed. */
+int hsdc_stedc_dev(hsdc_ctx *ctx, int n, double *d_d, double *d_e, double *d_z, int ldz);
+
+/* Many independent tridiagonals in one solve (one launch per DC level over all
+ * merges of all problems).  Problem p has order ns[p]; its diagonal is
+ * d_d[offs[p] .. offs[p]+ns[p]), its off-diagonal d_e[offs[p] .. offs[p]+ns[p]-1),
+ * its eigenvectors go to d_z + zoffs[p] with leading dimension ns[p].  Device
+ * arrays except ns/offs/zoffs (host). */
+int hsdc_stedc_batched_dev(hsdc_ctx *ctx, int nprob, const int *ns, const long *offs,
+                           double *d_d, double *d_e, double *d_z, const long *zoffs);
+
+/* ---- single merge kernels (host pointers) ---- */
+
+/* MDLAED2's deflation (mdlaed2.f90:149-409) on the merge data alone.  Inputs d(n),
+ * z(n), indxq(n), rho, n1 as MDLAED2.  Outputs: *k, dlamda(k), w(k), indxp(n),
+ * indx(n), indxc(n), coltyp(n) (first 4 = CTOT), d updated, *rho_out, and the
+ * Givens rotations (*nrot, rot_pj, rot_nj (1-based), rot_c, rot_s) in the order
+ * MDLAED2 applies them with DROT (mdlaed2.f90:352).  hss_min_k picks the column
+ * grouping exactly as BSMLZ does. */
+int hsdc_deflate(hsdc_ctx *ctx, int n, int n1, double *d, double *z, int *indxq, double rho,
+                 int hss_min_k, int *k, double *rho_out, double *dlamda, double *w, int *indxp,
+                 int *indx, int *indxc, int *coltyp, int *nrot, int *rot_pj, int *rot_nj,
+                 double *rot_c, double *rot_s);
+
+/* The secular equation of one merge: all k roots of 1 + rho*sum w_i^2/(dlamda_i - x)
+ * (DLAED4 at mdlaed3.f90:208), as lam(k), origin pole org(k) (1-based), tau(k) with
+ * lam = dlamda(org) + tau, and iterations per root niter(k). */
+int hsdc_secular(hsdc_ctx *ctx, int k, const double *dlamda, const double *w, double rho,
+                 double *lam, int *org, double *tau, int *niter);
+
+/* LAPACK DLAED3 (dense branch, mdlaed1.f90:199) on device. */
+int hsdc_laed3(hsdc_ctx *ctx, int k, int n, int n1, double *d, double *q, int ldq, double rho,
+               double *dlamda, const double *q2, const int *indx, const int *ctot, double *w);
+
+/* MDLAED3's secular / Loewner / alpha half (mdlaed3.f90:201-249): roots d(k),
+ * updated z (in place), difl(k), difr(k), alpha(k). */
+int hsdc_loewner(hsdc_ctx *ctx, int k, double rho, double *dlamda, double *z, double *d,
+                 double *difl, double *difr, double *alpha);
+
+/* DHSSEVC (CauchyHssEig.f90:2444): build the HSS form of
+ * C(i,j) = u(i) v(j) / (f(i) - d(j)) (accurate differences from difl/difr) with
+ * leaf block ni and tolerance tol, then q(1:nrow, 1:k) := q * C (ldq).  On return
+ * ranks[0] holds the largest generator rank. */
+int hsdc_hss_evc(hsdc_ctx *ctx, int k, int ni, double tol, const double *f, const double *d,
+                 const double *u, const double *v, const double *difl, const double *difr,
+                 double *q, int ldq, int nrow, int *ranks);
+
+/* Dense expansion of the HSS form built by the last hsdc_hss_evc call (k x k). */
+int hsdc_hss_dense(hsdc_ctx *ctx, double *a, int lda);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

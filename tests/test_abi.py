@@ -1,0 +1,59 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol, and
+validates arguments exactly as MDSTEDC does (no device needed)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from hsspack_amd import hsdc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "hsdc.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void)\s+(hsdc_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = hsdc.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 10
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(hsdc.EXPORTS)
+
+
+def test_stedc_argument_errors_and_query():
+    ctx = hsdc.Context()
+    n = 50
+    d = np.zeros(n)
+    e = np.zeros(n)
+    L = hsdc.lib()
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    z = np.zeros((n, n), order="F")
+    work = np.zeros(2)
+    iwork = np.zeros(2, np.int32)
+    h = ctx._h
+    assert L.hsdc_stedc(h, b"X", n, dp(d), dp(e), dp(z), n, dp(work), 1, ip(iwork), 1) == -1
+    assert L.hsdc_stedc(h, b"I", -1, dp(d), dp(e), dp(z), n, dp(work), 1, ip(iwork), 1) == -2
+    assert L.hsdc_stedc(h, b"I", n, dp(d), dp(e), dp(z), n - 1, dp(work), 1, ip(iwork), 1) == -6
+    assert L.hsdc_stedc(h, b"I", n, dp(d), dp(e), dp(z), n, dp(work), 1, ip(iwork), 1) == -8
+    assert L.hsdc_stedc(h, b"I", n, dp(d), dp(e), dp(z), n, dp(work), 1 + 4 * n + n * n, ip(iwork), 1) == -10
+    assert L.hsdc_stedc(h, b"I", n, dp(d), dp(e), dp(z), n, dp(work), -1, ip(iwork), 1) == 0
+    assert work[0] == 1 + 4 * n + n * n and iwork[0] == 3 + 5 * n
+    assert L.hsdc_stedc(h, b"V", n, dp(d), dp(e), dp(z), n, dp(work), -1, ip(iwork), 1) == 0
+    lgn = int(np.ceil(np.log2(n)))
+    assert work[0] == 1 + 3 * n + 2 * n * lgn + 4 * n * n
+    ctx.close()
+
+
+def test_params_validation():
+    ctx = hsdc.Context()
+    with pytest.raises(ValueError):
+        ctx.set_params(smlsiz=100)
+    ctx.set_params(smlsiz=25, hss_min_k=1024, hss_leaf=128, hss_tol=1e-13)
+    ctx.close()

@@ -1,0 +1,195 @@
+"""GPU parity of the HIP divide-and-conquer path against the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import lapack_ref as L
+from hsspack_amd.workloads import tridiag_cfg
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_dev):
+    from hsspack_amd import hsdc
+    c = hsdc.Context()
+    yield c
+    c.close()
+
+
+def tri(d, e):
+    return np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+
+
+def solve_dev(torch, ctx, d, e, **params):
+    if params:
+        ctx.set_params(**params)
+    n = len(d)
+    dev = torch.device("cuda:0")
+    dd = torch.tensor(d, device=dev)
+    ee = torch.tensor(np.append(e, 0.0), device=dev)
+    zz = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    rc = ctx.stedc_dev(dd, ee, zz)
+    torch.cuda.synchronize()
+    return rc, dd.cpu().numpy(), zz.cpu().numpy().T.copy()
+
+
+def check_eig(d, e, lam, z, tol=1e-12):
+    T = tri(d, e)
+    nrm = np.abs(T).max()
+    n = len(d)
+    orth = np.abs(z.T @ z - np.eye(n)).max()
+    res = np.abs(T @ z - z * lam).max() / nrm
+    assert orth <= tol, orth
+    assert res <= tol, res
+
+
+def test_secular_bitexact_vs_oracle(ctx):
+    rng = np.random.default_rng(11)
+    for k in (1, 2, 3, 5, 17, 64, 65, 300):
+        d = np.sort(rng.uniform(-1, 1, k))
+        if k > 4:
+            d[2] = d[1] + 1e-12
+        z = rng.standard_normal(k)
+        z /= np.linalg.norm(z)
+        rho = 0.7
+        g = ctx.secular(d, z, rho)
+        assert g["info"] == 0
+        for j in range(1, k + 1):
+            r = O.laed4(d, z, rho, j)
+            assert g["lam"][j - 1] == r["lam"], (k, j)
+            assert g["tau"][j - 1] == r["tau"], (k, j)
+            assert g["org"][j - 1] == r["org"]
+            if k > 2:
+                assert g["niter"][j - 1] == r["niter"]
+
+
+def _merge_cases():
+    from golden.make_golden import merge_inputs
+    out = []
+    for name, (d, e) in [("rand", tridiag_cfg(2, 96, 11)), ("toep", tridiag_cfg(1, 80, 12)),
+                         ("exact", (np.full(64, 2.0), np.full(63, 1.0))),
+                         ("neg", (np.linspace(-1, 1, 50), -np.abs(tridiag_cfg(2, 49, 13)[0]) - 0.1)),
+                         ("big", tridiag_cfg(2, 3000, 14)), ("bigexact", (np.full(2500, 2.0), np.full(2499, 1.0)))]:
+        n = len(d)
+        out.append((name, merge_inputs(d, e[: n - 1], n // 2), n // 2))
+    return out
+
+
+@pytest.mark.parametrize("hss_min_k", [1 << 30, 16])
+def test_deflate_bitexact_vs_oracle(ctx, hss_min_k):
+    for name, (dd, q, indxq, rho, z), n1 in _merge_cases():
+        r = O.laed2(dd, q, indxq, rho, z, n1, bsmlz=hss_min_k)
+        g = ctx.deflate(dd, z, indxq, rho, n1, hss_min_k=hss_min_k)
+        k = r["k"]
+        assert g["info"] == 0 and g["k"] == k, name
+        assert g["rho"] == r["rho"]
+        assert np.array_equal(g["indxp"], r["indxp"]), name
+        assert np.array_equal(g["indx"], r["indx"]), name
+        if r["hflag"]:
+            assert np.array_equal(g["indxc"], r["indxc"]), name
+        assert np.array_equal(g["coltyp"][:4], r["coltyp"][:4]), name
+        assert np.array_equal(g["dlamda"][:k], r["dlamda"][:k]), name
+        assert np.array_equal(g["w"][:k], r["w"][:k]), name
+        for a, b in zip(g["rot"], r["rot"]):
+            assert np.array_equal(a, b), name
+        assert np.array_equal(g["d"][k:], r["d"][k:]), name
+
+
+@pytest.mark.parametrize("cfg,n", [(1, 26), (2, 100), (1, 257), (2, 513), (1, 1200), (2, 2000)])
+def test_stedc_dev_dense_vs_oracle(torch_dev, ctx, cfg, n):
+    d, e = tridiag_cfg(cfg, n, 100 + n)
+    rc, lam, Z = solve_dev(torch_dev, ctx, d, e, hss_min_k=1 << 30, batched=1)
+    assert rc == 0
+    ref = O.stedc(d, e, prm=O.params(bsmlz=1 << 30))
+    w, _, _ = L.dstedc(d, e)
+    nrm = np.abs(tri(d, e)).max()
+    assert np.abs(lam - ref["d"]).max() <= 4 * n * EPS * nrm
+    assert np.abs(lam - w).max() <= 4 * n * EPS * nrm
+    check_eig(d, e, lam, Z)
+
+
+def test_plain_schedule_bitexact_with_batched(torch_dev, ctx):
+    d, e = tridiag_cfg(2, 700, 5)
+    rc1, l1, z1 = solve_dev(torch_dev, ctx, d, e, hss_min_k=1 << 30, batched=1)
+    rc2, l2, z2 = solve_dev(torch_dev, ctx, d, e, hss_min_k=1 << 30, batched=0)
+    ctx.set_params(batched=1)
+    assert rc1 == 0 and rc2 == 0
+    assert np.array_equal(l1, l2)
+    assert np.array_equal(z1, z2)
+
+
+def test_first_level_k_matches_oracle(torch_dev, ctx):
+    """Leaves are bit-exact, so the first merge level sees identical inputs."""
+    d, e = tridiag_cfg(2, 800, 21)
+    ref = O.stedc(d, e, prm=O.params(bsmlz=1 << 30), trace_cap=1000)
+    solve_dev(torch_dev, ctx, d, e, hss_min_k=1 << 30)
+    st = ctx.stats()
+    assert st["merges"] == len(ref["trace"]["k"])
+    assert st["roots"] + st["deflated"] == int(np.sum(ref["trace"]["n"]))
+
+
+def test_split_matrix(torch_dev, ctx):
+    rng = np.random.default_rng(9)
+    n = 150
+    d = rng.uniform(-1, 1, n)
+    e = rng.uniform(-1, 1, n - 1)
+    e[[20, 57, 100]] = 0.0
+    rc, lam, Z = solve_dev(torch_dev, ctx, d, e, hss_min_k=1 << 30)
+    assert rc == 0
+    assert np.all(np.diff(lam) >= 0)
+    w, _, _ = L.dstedc(d, e)
+    assert np.abs(lam - w).max() <= 4 * n * EPS * np.abs(tri(d, e)).max()
+    check_eig(d, e, lam, Z)
+
+
+def test_batched_problems(torch_dev, ctx):
+    torch = torch_dev
+    sizes = [30, 257, 64, 1000, 5, 400]
+    ds, es = [], []
+    for i, n in enumerate(sizes):
+        d, e = tridiag_cfg(1 + (i % 2), n, 300 + i)
+        ds.append(d)
+        es.append(np.append(e, 0.0))
+    offs = np.cumsum([0] + sizes[:-1])
+    zoffs = np.cumsum([0] + [n * n for n in sizes[:-1]])
+    dev = torch.device("cuda:0")
+    D = torch.tensor(np.concatenate(ds), device=dev)
+    E = torch.tensor(np.concatenate(es), device=dev)
+    Z = torch.zeros(int(sum(n * n for n in sizes)), dtype=torch.float64, device=dev)
+    ctx.set_params(hss_min_k=1 << 30)
+    rc = ctx.stedc_batched_dev(sizes, offs, D, E, Z, zoffs)
+    torch.cuda.synchronize()
+    assert rc == 0
+    Dh, Zh = D.cpu().numpy(), Z.cpu().numpy()
+    for i, n in enumerate(sizes):
+        lam = Dh[offs[i]:offs[i] + n]
+        z = Zh[zoffs[i]:zoffs[i] + n * n].reshape(n, n).T
+        w, _, _ = L.dstedc(ds[i], es[i][: n - 1])
+        assert np.abs(lam - w).max() <= 4 * n * EPS * np.abs(tri(ds[i], es[i][:n - 1])).max()
+        check_eig(ds[i], es[i][:n - 1], lam, z)
+
+
+def test_host_api_and_info(ctx):
+    n = 300
+    d, e = tridiag_cfg(2, n, 3)
+    ctx.set_params(hss_min_k=1 << 30)
+    lam, z, info = ctx.stedc(d, e)
+    assert info == 0
+    w, _, _ = L.dstedc(d, e)
+    assert np.abs(lam - w).max() <= 4 * n * EPS * np.abs(tri(d, e)).max()
+    check_eig(d, e, lam, z)
+    # COMPZ = 'V' applies the input Z
+    q0, _ = np.linalg.qr(np.random.default_rng(2).standard_normal((n, n)))
+    lam2, z2, info = ctx.stedc(d, e, compz="V", z=q0)
+    assert info == 0
+    np.testing.assert_allclose(np.abs(z2), np.abs(q0 @ z), atol=1e-12)

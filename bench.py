@@ -1,0 +1,201 @@
+"""Benchmark: seconds to all eigenpairs of an n = 32768 FP64 symmetric tridiagonal
+(d, e ~ U(-1,1), seed 2; HSS leaf 128, tolerance 1e-13, HSS branch for K >= 1024)
+with the batched divide-and-conquer path on one MI355X, device resident.
+
+One step = one complete solve (eigenvalues and the n x n eigenvector matrix) with
+the input already in HBM.  Prints one JSON line.  Also reported in the line: the
+merge-at-a-time ("plain") schedule on the same input, a roofline entry for the
+final reordering kernel (event-timed inside the timed steps), orthogonality and
+residual of the result, and the CPU baseline (the oracle's restatement of the
+reference path, single thread, timed on this host on a bounded sample).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def check_result(torch, d, e, lam, zz):
+    """max|Z'Z - I| and max|TZ - Z diag(lam)| / max|T| on the device.
+    zz holds Z column-major, i.e. zz[j, :] is eigenvector j."""
+    n = zz.shape[0]
+    dev = zz.device
+    dd = torch.tensor(d, device=dev)
+    ee = torch.tensor(e, device=dev)
+    orth = 0.0
+    blk = 4096
+    for i0 in range(0, n, blk):
+        g = zz[i0:i0 + blk] @ zz.T
+        g[:, i0:i0 + blk].diagonal().sub_(1.0)
+        orth = max(orth, g.abs().max().item())
+        del g
+    lam_t = torch.tensor(lam, device=dev)
+    res = 0.0
+    for i0 in range(0, n, blk):
+        z = zz[i0:i0 + blk]
+        tz = z * dd
+        tz[:, 1:] += z[:, :-1] * ee
+        tz[:, :-1] += z[:, 1:] * ee
+        tz -= z * lam_t[i0:i0 + blk, None]
+        res = max(res, tz.abs().max().item())
+    nrm = max(abs(d).max(), abs(e).max())
+    return orth, res / nrm
+
+
+def cpu_baseline(cfg, seed, hss):
+    """The oracle (C restatement of the reference path, one thread) on two sizes of
+    the same generator; extrapolated to n = 32768 with the measured exponent."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    from hsspack_amd.workloads import tridiag_cfg
+    L = O.lib()
+    times = {}
+    for n in (8192, 16384):
+        d, e = tridiag_cfg(cfg, n, seed)
+        z = np.zeros((n, n), order="F")
+        lwork, liwork = 1 + 4 * n + n * n, 3 + 5 * n
+        work = np.zeros(lwork)
+        iwork = np.zeros(liwork, np.int32)
+        z[:] = 0.0
+        work[:] = 0.0
+        prm = O.params(bsmlz=hss[0], ni=hss[1], tol=hss[2])
+        t0 = time.perf_counter()
+        info = L.orc_stedc(ctypes.c_char(b"I"), n, O.dp(d), O.dp(e), O.dp(z), n, O.dp(work), lwork,
+                           O.ip(iwork), liwork, ctypes.byref(prm), None)
+        times[n] = time.perf_counter() - t0
+        assert info == 0
+        log("cpu oracle n=%d: %.2f s" % (n, times[n]))
+        del z, work
+    p = np.log2(times[16384] / times[8192])
+    est = times[16384] * 2.0 ** p
+    return dict(value=est, unit="s", cores=1, kind="port",
+                sample="oracle (C restatement of mdstedc/mdlaed0-3 + DHSSEVC, 1 thread) on the "
+                       "same generator at n=8192 (%.2f s) and n=16384 (%.2f s), extrapolated to "
+                       "n=32768 with the measured exponent %.2f" % (times[8192], times[16384], p))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--cfg", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from hsspack_amd import hsdc
+    from hsspack_amd.workloads import tridiag_cfg
+
+    n = args.n
+    hss = (1024, 128, 1e-13)
+    d, e = tridiag_cfg(args.cfg, n, args.seed)
+    dev = torch.device("cuda:0")
+    D0 = torch.tensor(d, device=dev)
+    E0 = torch.tensor(np.append(e, 0.0), device=dev)
+    D = torch.empty_like(D0)
+    E = torch.empty_like(E0)
+    Z = torch.empty((n, n), dtype=torch.float64, device=dev)
+    ctx = hsdc.Context(hss_min_k=hss[0], hss_leaf=hss[1], hss_tol=hss[2], batched=1, profile=1)
+
+    def step():
+        D.copy_(D0)
+        E.copy_(E0)
+        rc = ctx.stedc_dev(D, E, Z)
+        if rc != 0:
+            raise RuntimeError("hsdc_stedc_dev returned %d" % rc)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    fin_ms, fin_bytes = [], 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        st = ctx.stats()
+        fin_ms.append(st["ms_final"])
+        fin_bytes = st["final_bytes"]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1000.0 / args.steps
+    st = ctx.stats()
+    log("batched: %.2f ms/solve, stats %s" % (ms_step, st))
+
+    plain_s = None
+    if not args.no_plain:
+        ctx.set_params(batched=0)
+        step()
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - tp0
+        ctx.set_params(batched=1)
+        log("plain (merge at a time): %.2f ms/solve" % (plain_s * 1000))
+        step()
+        torch.cuda.synchronize()
+
+    orth = res = None
+    if not args.no_check:
+        orth, res = check_result(torch, d, e, D.cpu().numpy(), Z)
+        log("orthogonality %.3e residual %.3e" % (orth, res))
+
+    fin_avg = sum(fin_ms) / len(fin_ms)
+    achieved = fin_bytes / (fin_avg * 1e-3) / 1e9
+    out = {
+        "metric": "s to all eigenpairs, n=32768 FP64 tridiag (orth<=1e-12): learned+batched vs plain",
+        "value": ms_step / 1000.0,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "tridiagonal DC eigensolve, n=%d, d,e~U(-1,1) seed %d, HSS leaf 128, "
+                               "tol 1e-13, HSS for K>=1024, all eigenpairs, device resident"
+                               % (n, args.seed),
+                   "n": n, "schedule": "batched (one launch per DC level)",
+                   "plain_s": plain_s,
+                   "speedup_vs_plain": (plain_s / (ms_step / 1000.0)) if plain_s else None,
+                   "orthogonality": orth, "residual": res,
+                   "merges": st["merges"], "levels": st["levels"], "k_top": st["k_top"],
+                   "roots": st["roots"], "deflated": st["deflated"],
+                   "hss_merges": st["hss_merges"]},
+        "roofline": {"kernel": "k_final (eigenvector reordering, mdlaed0.f90:355-362)",
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": fin_bytes, "avg_ms": fin_avg},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cfg, args.seed, hss)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -33,7 +33,7 @@ __global__ void k_scatter(const Merge *, const int *, int, const MergeOut *, con
                           Work, int, const double *);
 __global__ void k_lamrg(const Merge *, const MergeOut *, Work, int);
 __global__ void k_final(const Prob *, const int *, Work, int, const int *, double *, const long *,
-                        const long *, int, double *, const double *);
+                        const long *, int, double *, const double *, unsigned long long *);
 __global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, long, double *, long);
 
 #define WGS 256
@@ -80,6 +80,9 @@ int ensure_pin(PinBuf &b, size_t bytes)
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+// every sub-table of a packed upload starts on a 64-byte boundary, so 8-byte
+// fields and 64-bit atomics in it are naturally aligned
+inline size_t al64(size_t x) { return (x + 63) & ~(size_t)63; }
 
 }  // namespace
 
@@ -251,9 +254,9 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
         c->st.k_top = k;
     }
     // upload derived tables in one copy
-    size_t o1 = 0, o2 = o1 + sizeof(int) * (nm + 1), o3 = o2 + sizeof(int) * (nm + 1),
-           o4 = o3 + sizeof(int) * (nm + 1), o5 = o4 + sizeof(long) * nm, o6 = o5 + sizeof(long) * nm,
-           o7 = o6 + sizeof(GTile) * tiles.size();
+    size_t o1 = 0, o2 = al64(o1 + sizeof(int) * (nm + 1)), o3 = al64(o2 + sizeof(int) * (nm + 1)),
+           o4 = al64(o3 + sizeof(int) * (nm + 1)), o5 = al64(o4 + sizeof(long) * nm),
+           o6 = al64(o5 + sizeof(long) * nm), o7 = al64(o6 + sizeof(GTile) * tiles.size());
     CK(ensure_pin(c->pin, o7));
     char *hp = (char *)c->pin.p;
     memcpy(hp + o1, chunkK.data(), sizeof(int) * (nm + 1));
@@ -325,10 +328,11 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     size_t nl = pl.leaves.size(), nmtot = 0;
     for (auto &L : pl.lv) nmtot += L.size();
     size_t np = pl.probs.size();
-    size_t a0 = 0, a1 = a0 + sizeof(Prob) * np, a2 = a1 + sizeof(Leaf) * nl, a3 = a2 + sizeof(Merge) * nmtot,
-           a4 = a3 + sizeof(double) * ntot, a5 = a4 + sizeof(double) * ntot, a6 = a5 + sizeof(double) * np,
-           a7 = a6 + sizeof(int) * ntot, a8 = a7 + sizeof(long) * np, a9 = a8 + sizeof(long) * np,
-           a10 = a9 + sizeof(int) * ntot, a11 = a10 + 64;
+    size_t a0 = 0, a1 = al64(a0 + sizeof(Prob) * np), a2 = al64(a1 + sizeof(Leaf) * nl),
+           a3 = al64(a2 + sizeof(Merge) * nmtot), a4 = al64(a3 + sizeof(double) * ntot),
+           a5 = al64(a4 + sizeof(double) * ntot), a6 = al64(a5 + sizeof(double) * np),
+           a7 = al64(a6 + sizeof(int) * ntot), a8 = al64(a7 + sizeof(long) * np),
+           a9 = al64(a8 + sizeof(long) * np), a10 = al64(a9 + sizeof(int) * ntot), a11 = a10 + 64;
     // chunk tables: per level (nm+1 entries) followed by per merge {0, chunks}
     std::vector<int> chunks;
     std::vector<size_t> lvchunk, mchunk;
@@ -339,7 +343,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         for (auto &M : L) { acc += cdiv(M.n, WGS); chunks.push_back(acc); }
         for (auto &M : L) { mchunk.push_back(chunks.size()); chunks.push_back(0); chunks.push_back(cdiv(M.n, WGS)); }
     }
-    size_t a12 = a11 + sizeof(int) * chunks.size();
+    size_t a12 = al64(a11 + sizeof(int) * chunks.size());
     CK(ensure(c->probs, a12));
     CK(ensure_pin(c->pin, a12));
     char *h = (char *)c->pin.p;
@@ -409,18 +413,38 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         c->st.levels++;
     }
     int zrows = zglobal ? ntot : 0;
+    unsigned long long *drows = (unsigned long long *)(dp + a10 + 16);
+    hipEvent_t f0 = nullptr, f1 = nullptr;
+    if (c->prm.profile) {
+        HK(hipEventCreate(&f0));
+        HK(hipEventCreate(&f1));
+        HK(hipEventRecord(f0, s));
+    }
     hipLaunchKernelGGL(k_final, dim3(ntot), dim3(WGS), 0, s, dprobs, dcolprob, W, cur, doutcol, dZ, dzoff,
-                       dldz, zrows, d_out_dev, dscale);
+                       dldz, zrows, d_out_dev, dscale, drows);
     HK(hipGetLastError());
+    if (c->prm.profile) HK(hipEventRecord(f1, s));
     HK(hipEventRecord(e1, s));
     int hinfo = 0;
+    unsigned long long rows = 0;
     HK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    HK(hipMemcpyAsync(&rows, drows, sizeof(rows), hipMemcpyDeviceToHost, s));
     HK(hipStreamSynchronize(s));
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
     c->st.ms_total = ms;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    long long zrows_written = 0;
+    for (auto &P : pl.probs) zrows_written += (long long)(zglobal ? ntot : P.n) * P.n;
+    c->st.final_bytes = 8LL * (zrows_written + (long long)rows) + 16LL * ntot;
+    if (c->prm.profile) {
+        float fm = 0;
+        (void)hipEventElapsedTime(&fm, f0, f1);
+        c->st.ms_final = fm;
+        (void)hipEventDestroy(f0);
+        (void)hipEventDestroy(f1);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return hinfo ? 1 : 0;
 }
 
@@ -468,6 +492,7 @@ void hsdc_default_params(hsdc_params *p)
     p->maxit = 100;
     p->batched = 1;
     p->learned = 0;
+    p->profile = 0;
 }
 
 // The context touches no device until the first solve, so argument checking and
@@ -632,15 +657,15 @@ int hsdc_stedc_batched_dev(hsdc_ctx *c, int nprob, const int *ns, const long *of
 static int single_merge_setup(hsdc_ctx *c, int n, int n1, double rho, Merge **dm, int **dchunk)
 {
     CK(alloc_work(c, std::max(n, 1)));
-    CK(ensure(c->probs, sizeof(Merge) + sizeof(int) * 2 + 64));
+    CK(ensure(c->probs, 128));
     Merge M;
     M.s = 0; M.n = n; M.n1 = n1; M.prob = 0; M.rho = rho;
     int ch[2] = {0, cdiv(n, WGS)};
     char *dp = (char *)c->probs.p;
     HK(hipMemcpy(dp, &M, sizeof(Merge), hipMemcpyHostToDevice));
-    HK(hipMemcpy(dp + sizeof(Merge), ch, sizeof(ch), hipMemcpyHostToDevice));
+    HK(hipMemcpy(dp + 64, ch, sizeof(ch), hipMemcpyHostToDevice));
     *dm = (Merge *)dp;
-    *dchunk = (int *)(dp + sizeof(Merge));
+    *dchunk = (int *)(dp + 64);
     return 0;
 }
 
@@ -710,12 +735,12 @@ int hsdc_secular(hsdc_ctx *c, int k, const double *dlamda, const double *w, doub
     int *dch;
     CK(single_merge_setup(c, k, k / 2, rho, &dm, &dch));
     Work W = c->W;
-    CK(ensure(c->mo, sizeof(MergeOut) + 64));
+    CK(ensure(c->mo, al64(sizeof(MergeOut)) + 64));
     MergeOut o;
     memset(&o, 0, sizeof(o));
     o.k = k; o.rho = rho; o.hss = 0; o.flags = 0;
     MergeOut *dmo = (MergeOut *)c->mo.p;
-    int *dinfo = (int *)((char *)c->mo.p + sizeof(MergeOut));
+    int *dinfo = (int *)((char *)c->mo.p + al64(sizeof(MergeOut)));
     int zero = 0;
     int bs[2] = {0, cdiv(k, 64)};
     HK(hipMemcpy(dmo, &o, sizeof(o), hipMemcpyHostToDevice));

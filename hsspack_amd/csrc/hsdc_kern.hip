@@ -821,7 +821,7 @@ __global__ __launch_bounds__(WG) void k_lamrg(const Merge *mg, const MergeOut *m
 __global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_prob, Work W, int cur,
                                               const int *outcol, double *Z, const long *zoff,
                                               const long *ldz, int zrows_global, double *dout,
-                                              const double *scale)
+                                              const double *scale, unsigned long long *rows_read)
 {
     int g = blockIdx.x;  // global column
     int p = col_prob[g];
@@ -844,6 +844,7 @@ __global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_
         r0 = P.off; r1 = P.off + P.n;
     }
     for (int r = r0 + threadIdx.x; r < r1; r += WG) dst[r] = (r >= lo && r < hi) ? src[r] : 0.0;
+    if (threadIdx.x == 0) atomicAdd(rows_read, (unsigned long long)(hi - lo));
     (void)i;
 }
 

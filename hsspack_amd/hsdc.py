@@ -33,13 +33,14 @@ EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_s
 
 class Params(ctypes.Structure):
     _fields_ = [("smlsiz", _i), ("hss_min_k", _i), ("hss_leaf", _i), ("hss_tol", _d),
-                ("maxit", _i), ("batched", _i), ("learned", _i)]
+                ("maxit", _i), ("batched", _i), ("learned", _i), ("profile", _i)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("levels", _i), ("merges", _i), ("hss_merges", _i), ("roots", ctypes.c_longlong),
                 ("secular_iters", ctypes.c_longlong), ("deflated", ctypes.c_longlong),
-                ("k_top", _i), ("rank_max", _i), ("ms_total", _d)]
+                ("k_top", _i), ("rank_max", _i), ("ms_total", _d), ("ms_final", _d),
+                ("final_bytes", ctypes.c_longlong)]
 
 
 _LIB = None

@@ -37,6 +37,7 @@ typedef struct {
     int maxit;       /* secular iteration cap */
     int batched;     /* 1: one launch per DC level over all merges; 0: one merge at a time */
     int learned;     /* 1: MLP proposes the secular starting iterate */
+    int profile;     /* 1: time the final reordering kernel with HIP events */
 } hsdc_params;
 
 /* Run statistics of the last call on a context. */
@@ -47,6 +48,8 @@ typedef struct {
     int k_top;          /* K of the last (top) merge */
     int rank_max;       /* largest HSS generator rank */
     double ms_total;    /* device time of the solve (events), ms */
+    double ms_final;    /* final reordering kernel (events, profile = 1), ms */
+    long long final_bytes;  /* its algorithmic HBM bytes: Z written + Q rows read */
 } hsdc_stats;
 
 void hsdc_default_params(hsdc_params *p);

@@ -180,7 +180,7 @@ static int alloc_work(hsdc_ctx *c, int ntot)
 {
     if (!have_device()) return HSDC_ERR_NODEVICE;
     if ((size_t)ntot <= c->wcap && c->wcap) return 0;
-    size_t nd = 17, ni = 16;
+    size_t nd = 18, ni = 18;
     CK(ensure(c->wsd, sizeof(double) * nd * (size_t)ntot));
     CK(ensure(c->wsi, sizeof(int) * ni * (size_t)ntot));
     double *D = (double *)c->wsd.p;
@@ -190,11 +190,11 @@ static int alloc_work(hsdc_ctx *c, int ntot)
     W.Dv[0] = D + 0 * n; W.Dv[1] = D + 1 * n; W.z = D + 2 * n; W.dlamda = D + 3 * n; W.w = D + 4 * n;
     W.ddef = D + 5 * n; W.zs = D + 6 * n; W.ds = D + 7 * n; W.rot_c = D + 8 * n; W.rot_s = D + 9 * n;
     W.lam = D + 10 * n; W.tau = D + 11 * n; W.wnew = D + 12 * n; W.alpha = D + 13 * n;
-    W.difl = D + 14 * n; W.difr = D + 15 * n;
+    W.difl = D + 14 * n; W.difr = D + 15 * n; W.evd = D + 16 * n;
     W.vm[0] = I + 0 * n; W.vm[1] = I + 1 * n; W.indxq = I + 2 * n; W.supp_lo = I + 3 * n;
     W.supp_hi = I + 4 * n; W.indxq2 = I + 5 * n; W.indx = I + 6 * n; W.indxc = I + 7 * n;
     W.indxp = I + 8 * n; W.coltyp = I + 9 * n; W.rot_pj = I + 10 * n; W.rot_nj = I + 11 * n;
-    W.org = I + 12 * n; W.nit = I + 13 * n;
+    W.org = I + 12 * n; W.nit = I + 13 * n; W.nsl = I + 14 * n; W.evq = I + 15 * n;
     c->wcap = ntot;
     return 0;
 }

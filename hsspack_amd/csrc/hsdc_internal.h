@@ -46,6 +46,8 @@ struct Work {
     double *z, *dlamda, *w, *ddef, *zs, *ds;
     int *indxq2, *indx, *indxc, *indxp, *coltyp;
     int *rot_pj, *rot_nj;
+    int *nsl, *evq;     // deflation: survivor list, rotation event positions
+    double *evd;        // deflation: value of each rotated-out column
     double *rot_c, *rot_s;
     double *lam, *tau, *wnew, *alpha, *difl, *difr;
     int *org, *nit;

@@ -233,25 +233,54 @@ __global__ __launch_bounds__(WG) void k_sortmerge(const Merge *mg, Work W)
 }
 
 // ---------------------------------------------------------------------------
-// Deflation (mdlaed2.f90:264-465), one block per merge.  The tolerance and the
-// small-z screen are parallel; the PJ/NJ scan that decides Givens deflations is
-// the sequential recurrence of the reference, run by one lane over sorted data
-// staged through LDS; the grouping of columns into types 1-4 is a parallel
-// stable partition.
+// Deflation (mdlaed2.f90:264-465), one block per merge, same decisions as the
+// reference's sequential PJ/NJ scan:
+//  1. parallel: tolerance, sorted gather, small-z screen, list of survivors;
+//  2. parallel: the close-pair test of every consecutive survivor pair on the
+//     unrotated values; before the first pair that deflates no value has been
+//     modified, so those survivors are non-deflated as computed;
+//  3. one lane continues the scan from that pair over survivors only;
+//  4. the deflated region INDXP(K+1:N), which the reference keeps with an
+//     insertion step (mdlaed2.f90:357-370), is replayed by one lane over runs of
+//     small-z columns (binary search inside a run) and written out in parallel.
+// A merge whose replay would exceed the LDS segment table runs the plain
+// sequential scan instead.
 // ---------------------------------------------------------------------------
 #define DCH 1024
+#define MAXSEG 1536
+__device__ int block_scan_excl(int v, int *tmp, int *total)
+{
+    int tid = threadIdx.x;
+    tmp[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < WG; off <<= 1) {
+        int t = (tid >= off) ? tmp[tid - off] : 0;
+        __syncthreads();
+        tmp[tid] += t;
+        __syncthreads();
+    }
+    int incl = tmp[tid];
+    if (total) *total = tmp[WG - 1];
+    __syncthreads();
+    return incl - v;
+}
+
 __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur, MergeOut *mo,
                                                 int hss_min_k)
 {
     __shared__ double red_z[WG], red_d[WG];
+    __shared__ int sk, sk2, scnt[4], tcnt[4][WG], itmp[WG], sL, sfirst, sfallback, snseg, stot;
+    __shared__ int seg_a[MAXSEG], seg_b[MAXSEG], seg_nx[MAXSEG], seg_pos[MAXSEG], seg_ord[MAXSEG];
+    __shared__ double seg_d[MAXSEG];
     __shared__ double cz[DCH], cd[DCH];
     __shared__ int ci[DCH];
-    __shared__ int sk, sk2, scnt[4], tcnt[4][WG];
     Merge M = mg[blockIdx.x];
     const int n = M.n, n1 = M.n1, s = M.s, tid = threadIdx.x;
     const double *D = W.Dv[cur] + s;
     double *z = W.z + s;
     int *indx = W.indx + s, *indxp = W.indxp + s, *coltyp = W.coltyp + s;
+    double *zs = W.zs + s, *ds = W.ds + s, *ddef = W.ddef + s;
+    int *nsl = W.nsl + s;
     double mz = 0.0, md = 0.0;
     for (int i = tid; i < n; i += WG) { mz = fmax(mz, fabs(z[i])); md = fmax(md, fabs(D[i])); }
     red_z[tid] = mz; red_d[tid] = md;
@@ -270,79 +299,226 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
         }
         return;
     }
-    for (int i = tid; i < n; i += WG) coltyp[i] = (i < n1) ? 1 : 3;
+    const int per = (n + WG - 1) / WG;
+    const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
+    // ---- 1. sorted gather, small-z screen, survivor list
+    int cnt = 0;
+    for (int j = b0; j < b1; j++) {
+        int nj = indx[j];
+        double zz = z[nj - 1], dd = D[nj - 1];
+        zs[j] = zz; ds[j] = dd;
+        coltyp[nj - 1] = (rho * fabs(zz) <= tol) ? 4 : ((nj <= n1) ? 1 : 3);
+        cnt += (rho * fabs(zz) > tol);
+    }
+    int L;
+    int base = block_scan_excl(cnt, itmp, &L);
+    for (int j = b0; j < b1; j++)
+        if (rho * fabs(zs[j]) > tol) nsl[base++] = j;
+    if (tid == 0) { sL = L; sfirst = L - 1; sfallback = 0; }
     __syncthreads();
-    // sequential scan over sorted order
-    int K = 0, K2 = n + 1, PJ = 0, have = 0, nrot = 0;
-    double zP = 0.0, DP = 0.0;
-    double *ddef = W.ddef + s;  // ddef[pos-1] = D of the deflated column at INDXP(pos)
-    for (int c0 = 0; c0 < n; c0 += DCH) {
-        int cl = min(DCH, n - c0);
-        __syncthreads();
-        for (int j = tid; j < cl; j += WG) {
-            int nj = indx[c0 + j];
-            ci[j] = nj; cz[j] = z[nj - 1]; cd[j] = D[nj - 1];
+    // the run replay below relies on DLAMDA being sorted in scan order
+    for (int j = b0; j < b1; j++)
+        if (j + 1 < n && ds[j] > ds[j + 1]) sfallback = 1;
+    __syncthreads();
+    // ---- 2. speculative close-pair test on unrotated values
+    int myfirst = L - 1;
+    for (int q = tid; q + 1 < L; q += WG) {
+        int p = nsl[q], r = nsl[q + 1];
+        double ss = zs[p], cc = zs[r];
+        double tau = lapy2_d(cc, ss);
+        double t = ds[r] - ds[p];
+        cc = cc / tau;
+        ss = -ss / tau;
+        if (fabs(t * cc * ss) <= tol) { myfirst = q; break; }
+    }
+    atomicMin(&sfirst, myfirst);
+    __syncthreads();
+    const int first = sfirst;
+    for (int q = tid; q < first; q += WG) {
+        int p = nsl[q];
+        W.dlamda[s + q] = ds[p]; W.w[s + q] = zs[p]; indxp[q] = indx[p];
+    }
+    // ---- 3. sequential continuation over survivors
+    if (tid == 0 && !sfallback) {
+        int K = first, nrot = 0;
+        int p0 = nsl[first];
+        int PJ = indx[p0];
+        double zP = zs[p0], DP = ds[p0];
+        for (int q = first + 1; q < L; q++) {
+            int r = nsl[q];
+            int NJ = indx[r];
+            double zN = zs[r], dN = ds[r];
+            double ss = zP, cc = zN;
+            double tau = lapy2_d(cc, ss);
+            double t = dN - DP;
+            cc = cc / tau;
+            ss = -ss / tau;
+            if (fabs(t * cc * ss) <= tol) {
+                if (coltyp[NJ - 1] != coltyp[PJ - 1]) coltyp[NJ - 1] = 2;
+                coltyp[PJ - 1] = 4;
+                double tP = DP * (cc * cc) + dN * (ss * ss);
+                double dNn = DP * (ss * ss) + dN * (cc * cc);
+                W.rot_pj[s + nrot] = PJ; W.rot_nj[s + nrot] = NJ;
+                W.rot_c[s + nrot] = cc; W.rot_s[s + nrot] = ss;
+                W.evq[s + nrot] = q; W.evd[s + nrot] = tP;
+                nrot++;
+                PJ = NJ; zP = tau; DP = dNn;
+            } else {
+                W.dlamda[s + K] = DP; W.w[s + K] = zP; indxp[K] = PJ;
+                K++;
+                PJ = NJ; zP = zN; DP = dN;
+            }
         }
-        __syncthreads();
-        if (tid == 0) {
-            for (int j = 0; j < cl; j++) {
-                int NJ = ci[j];
-                double zN = cz[j], dN = cd[j];
-                if (rho * fabs(zN) <= tol) {
-                    K2--; coltyp[NJ - 1] = 4; indxp[K2 - 1] = NJ; ddef[K2 - 1] = dN;
-                    continue;
-                }
-                if (!have) { PJ = NJ; zP = zN; DP = dN; have = 1; continue; }
-                double ss = zP, cc = zN;
-                double tau = lapy2_d(cc, ss);
-                double t = dN - DP;
-                cc = cc / tau;
-                ss = -ss / tau;
-                if (fabs(t * cc * ss) <= tol) {
-                    if (coltyp[NJ - 1] != coltyp[PJ - 1]) coltyp[NJ - 1] = 2;
-                    coltyp[PJ - 1] = 4;
-                    W.rot_pj[s + nrot] = PJ; W.rot_nj[s + nrot] = NJ;
-                    W.rot_c[s + nrot] = cc; W.rot_s[s + nrot] = ss;
-                    nrot++;
-                    double tP = DP * (cc * cc) + dN * (ss * ss);
-                    double dNn = DP * (ss * ss) + dN * (cc * cc);
-                    K2--;
-                    int i = 1;
-                    for (;;) {
-                        if (K2 + i <= n) {
-                            if (tP < ddef[K2 + i - 1]) {
-                                indxp[K2 + i - 2] = indxp[K2 + i - 1]; ddef[K2 + i - 2] = ddef[K2 + i - 1];
-                                indxp[K2 + i - 1] = PJ; ddef[K2 + i - 1] = tP;
-                                i++;
-                                continue;
-                            }
-                            indxp[K2 + i - 2] = PJ; ddef[K2 + i - 2] = tP;
-                        } else {
-                            indxp[K2 + i - 2] = PJ; ddef[K2 + i - 2] = tP;
-                        }
+        W.dlamda[s + K] = DP; W.w[s + K] = zP; indxp[K] = PJ;
+        K++;
+        sk = K; sk2 = nrot;
+        // ---- 4. replay the deflated region as a list of segments, front first.
+        // Segment: small-z run of sorted positions [a, b) (front holds b-1) or,
+        // with a = -1, one rotated column b with value d.
+        int nseg = 0, head = -1;
+        auto push_run = [&](int a, int b) {
+            if (a >= b) return;
+            if (nseg >= MAXSEG) { nseg = MAXSEG + 1; return; }
+            seg_a[nseg] = a; seg_b[nseg] = b; seg_nx[nseg] = head; head = nseg++;
+        };
+        int ev = 0;
+        push_run(0, nsl[0]);
+        for (int q = 1; q < L && nseg <= MAXSEG; q++) {
+            push_run(nsl[q - 1] + 1, nsl[q]);
+            while (ev < nrot && W.evq[s + ev] == q && nseg <= MAXSEG) {
+                double dx = W.evd[s + ev];
+                int x = W.rot_pj[s + ev];
+                // walk from the front past every element with D > dx
+                int prev = -1, cseg = head, split_t = -1;
+                while (cseg >= 0) {
+                    if (seg_a[cseg] < 0) {
+                        if (dx < seg_d[cseg]) { prev = cseg; cseg = seg_nx[cseg]; continue; }
                         break;
                     }
-                    PJ = NJ; zP = tau; DP = dNn;
+                    int a = seg_a[cseg], b = seg_b[cseg];
+                    // front order ds[b-1] >= ds[b-2] >= ...; t = first offset with ds <= dx
+                    int lo = 0, hi = b - a;
+                    while (lo < hi) { int md2 = (lo + hi) >> 1; if (ds[b - 1 - md2] <= dx) hi = md2; else lo = md2 + 1; }
+                    if (lo == b - a) { prev = cseg; cseg = seg_nx[cseg]; continue; }
+                    split_t = lo;
+                    break;
+                }
+                if (nseg + 2 > MAXSEG) { nseg = MAXSEG + 1; break; }
+                int xs = nseg++;
+                seg_a[xs] = -1; seg_b[xs] = x; seg_d[xs] = dx;
+                if (split_t > 0) {
+                    // cseg [a,b) -> front part [b-t, b), x, rest [a, b-t)
+                    int rest = nseg++;
+                    int a = seg_a[cseg], b = seg_b[cseg];
+                    seg_a[rest] = a; seg_b[rest] = b - split_t; seg_nx[rest] = seg_nx[cseg];
+                    seg_a[cseg] = b - split_t;
+                    seg_nx[cseg] = xs;
+                    seg_nx[xs] = rest;
                 } else {
-                    K++;
-                    W.dlamda[s + K - 1] = DP; W.w[s + K - 1] = zP; indxp[K - 1] = PJ;
-                    PJ = NJ; zP = zN; DP = dN;
+                    seg_nx[xs] = cseg;
+                    if (prev < 0) head = xs; else seg_nx[prev] = xs;
+                }
+                ev++;
+            }
+        }
+        if (nseg <= MAXSEG) push_run(nsl[L - 1] + 1, n);
+        if (nseg > MAXSEG) {
+            sfallback = 1;
+        } else {
+            // front-to-back order and region positions (INDXP(K+1..N) from the front)
+            int pos = K, cnt2 = 0;
+            for (int g = head; g >= 0; g = seg_nx[g]) {
+                seg_ord[cnt2] = g;
+                seg_pos[cnt2] = pos;
+                pos += (seg_a[g] < 0) ? 1 : (seg_b[g] - seg_a[g]);
+                cnt2++;
+            }
+            snseg = cnt2;
+        }
+    }
+    __syncthreads();
+    if (!sfallback) {
+        for (int k = 0; k < snseg; k++) {
+            int g = seg_ord[k], P = seg_pos[k];
+            int a = seg_a[g], b = seg_b[g];
+            if (a < 0) {
+                if (tid == 0) { indxp[P] = b; ddef[P] = seg_d[g]; }
+            } else {
+                for (int u = tid; u < b - a; u += WG) {
+                    int j = b - 1 - u;
+                    indxp[P + u] = indx[j];
+                    ddef[P + u] = ds[j];
                 }
             }
         }
-    }
-    if (tid == 0) {
-        K++;
-        W.dlamda[s + K - 1] = DP; W.w[s + K - 1] = zP; indxp[K - 1] = PJ;
-        sk = K; sk2 = nrot;
+    } else {
+        // ---- fallback: the reference's scan over every column in sorted order
+        for (int i = tid; i < n; i += WG) coltyp[i] = (i < n1) ? 1 : 3;
+        int K2 = n + 1, PJ = 0, have = 0, nrot = 0, KK = 0;
+        double zP = 0.0, DP = 0.0;
+        for (int c0 = 0; c0 < n; c0 += DCH) {
+            int cl = min(DCH, n - c0);
+            __syncthreads();
+            for (int j = tid; j < cl; j += WG) { ci[j] = indx[c0 + j]; cz[j] = zs[c0 + j]; cd[j] = ds[c0 + j]; }
+            __syncthreads();
+            if (tid == 0) {
+                for (int j = 0; j < cl; j++) {
+                    int NJ = ci[j];
+                    double zN = cz[j], dN = cd[j];
+                    if (rho * fabs(zN) <= tol) {
+                        K2--; coltyp[NJ - 1] = 4; indxp[K2 - 1] = NJ; ddef[K2 - 1] = dN;
+                        continue;
+                    }
+                    if (!have) { PJ = NJ; zP = zN; DP = dN; have = 1; continue; }
+                    double ss = zP, cc = zN;
+                    double tau = lapy2_d(cc, ss);
+                    double t = dN - DP;
+                    cc = cc / tau;
+                    ss = -ss / tau;
+                    if (fabs(t * cc * ss) <= tol) {
+                        if (coltyp[NJ - 1] != coltyp[PJ - 1]) coltyp[NJ - 1] = 2;
+                        coltyp[PJ - 1] = 4;
+                        W.rot_pj[s + nrot] = PJ; W.rot_nj[s + nrot] = NJ;
+                        W.rot_c[s + nrot] = cc; W.rot_s[s + nrot] = ss;
+                        nrot++;
+                        double tP = DP * (cc * cc) + dN * (ss * ss);
+                        double dNn = DP * (ss * ss) + dN * (cc * cc);
+                        K2--;
+                        int i = 1;
+                        for (;;) {
+                            if (K2 + i <= n) {
+                                if (tP < ddef[K2 + i - 1]) {
+                                    indxp[K2 + i - 2] = indxp[K2 + i - 1]; ddef[K2 + i - 2] = ddef[K2 + i - 1];
+                                    indxp[K2 + i - 1] = PJ; ddef[K2 + i - 1] = tP;
+                                    i++;
+                                    continue;
+                                }
+                                indxp[K2 + i - 2] = PJ; ddef[K2 + i - 2] = tP;
+                            } else {
+                                indxp[K2 + i - 2] = PJ; ddef[K2 + i - 2] = tP;
+                            }
+                            break;
+                        }
+                        PJ = NJ; zP = tau; DP = dNn;
+                    } else {
+                        KK++;
+                        W.dlamda[s + KK - 1] = DP; W.w[s + KK - 1] = zP; indxp[KK - 1] = PJ;
+                        PJ = NJ; zP = zN; DP = dN;
+                    }
+                }
+            }
+        }
+        if (tid == 0) {
+            KK++;
+            W.dlamda[s + KK - 1] = DP; W.w[s + KK - 1] = zP; indxp[KK - 1] = PJ;
+            sk = KK; sk2 = nrot;
+        }
     }
     __syncthreads();
     // CTOT and the type grouping (mdlaed2.f90:393-431) as a stable partition
-    const int per = (n + WG - 1) / WG;
-    const int b0 = tid * per, b1 = min(n, b0 + per);
-    int cnt[4] = {0, 0, 0, 0};
-    for (int j = b0; j < b1; j++) cnt[coltyp[indxp[j] - 1] - 1]++;
-    for (int t = 0; t < 4; t++) tcnt[t][tid] = cnt[t];
+    int c4[4] = {0, 0, 0, 0};
+    for (int j = b0; j < b1; j++) c4[coltyp[indxp[j] - 1] - 1]++;
+    for (int t = 0; t < 4; t++) tcnt[t][tid] = c4[t];
     __syncthreads();
     if (tid < 4) {
         int acc = 0;
@@ -350,8 +526,8 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
         scnt[tid] = acc;
     }
     __syncthreads();
-    K = n - scnt[3];
-    const int hss = (K >= hss_min_k);
+    const int KF = n - scnt[3];
+    const int hss = (KF >= hss_min_k);
     if (!hss) {
         int psm[4];
         psm[0] = 0; psm[1] = scnt[0]; psm[2] = psm[1] + scnt[1]; psm[3] = psm[2] + scnt[2];
@@ -368,7 +544,7 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
         for (int j = b0; j < b1; j++) indx[j] = indxp[j];
     }
     if (tid == 0) {
-        o->k = K;
+        o->k = KF;
         for (int t = 0; t < 4; t++) o->ctot[t] = scnt[t];
         o->hss = hss; o->nrot = sk2; o->flags = 0; o->rho = rho; o->tol = tol;
     }

@@ -12,6 +12,8 @@
 #include <vector>
 
 #include "hsdc_internal.h"
+#include "hsdc_hss.h"
+#include "hsdc_hss_host.h"
 #include "../../include/hsdc.h"
 
 // kernels (hsdc_kern.hip)
@@ -35,6 +37,17 @@ __global__ void k_lamrg(const Merge *, const MergeOut *, Work, int);
 __global__ void k_final(const Prob *, const int *, Work, int, const int *, double *, const long *,
                         const long *, int, double *, const double *, unsigned long long *);
 __global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, long, double *, long);
+__global__ void k_hss_init(const HssMerge *);
+__global__ void k_hss_reset(const HssMerge *);
+__global__ void k_hss_dgen(const HssMerge *, const HssNode *, int);
+__global__ void k_hss_rowcomp(const HssMerge *, const HssNode *, int, double);
+__global__ void k_hss_colcomp(const HssMerge *, const HssNode *, int, double);
+__global__ void k_hss_compact(const HssMerge *, const HssCompact *, const int *, int);
+__global__ void k_hss_bgen(const HssMerge *, const HssNode *, int);
+__global__ void k_bgemm(const BG *);
+__global__ void k_gatherx(const Merge *, const int *, const int *, int, const Prob *, Work, int, double *,
+                          const long *);
+__global__ void k_cbuild(const HssMerge *, int, double *);
 
 #define WGS 256
 
@@ -92,6 +105,11 @@ struct hsdc_ctx {
     hipStream_t stream = 0;
     // device storage
     DevBuf q, ct, sb, din, ein, wsd, wsi, tabs, lvl, mo, probs, misc, zout, dout;
+    DevBuf hss, htab, xb, gf, cb, yb;
+    // generators of the last hsdc_hss_evc call (host copies), for hsdc_hss_dense
+    std::vector<double> evc_gen;
+    int evc_k = 0, evc_ni = 0;
+    double evc_tol = 0.0;
     PinBuf pin, pin2;
     Work W;
     size_t wcap = 0;
@@ -199,6 +217,431 @@ static int alloc_work(hsdc_ctx *c, int ntot)
     return 0;
 }
 
+// ----------------------------------------------------------------------------
+// HSS branch of one DC level (MDLAED3's DHSSEVC, CauchyHssEig.f90:2444-2598):
+// build the HSS form of every HSS merge's eigenvector matrix, then
+// Ct(:, 1:K) = Q2 * H.  `sel` lists the slice-local merges that take this branch.
+// ----------------------------------------------------------------------------
+struct HssSpec {
+    int K;
+    const double *FF, *D0, *U0, *V0, *DIFL, *DIFR;  // device generators (K)
+    const double *X;   // nrow x K, leading dimension ldx
+    double *Y;         // nrow x K result X * H, leading dimension ldy
+    long ldx, ldy;
+    int nrow;
+};
+
+struct HssPlanM {
+    int li;              // slice-local merge index
+    HssTree T;
+    bool ok;
+    HssMerge H;          // device pointers
+    std::vector<int> usz, vsz, bsz, dsz;
+    std::vector<long> pu, pv, pb, pd;
+    int err = 0;
+    long gcols = 0, fcols = 0;
+    std::vector<int> lg, lf;
+};
+
+static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double tol,
+                     std::vector<HssPlanM> *keep)
+{
+    hipStream_t s = c->stream;
+    std::vector<HssPlanM> P(sp.size());
+    // ---- device memory for every merge
+    size_t tot = 0;
+    std::vector<size_t> base(sp.size()), metasz(sp.size());
+    for (size_t h = 0; h < sp.size(); h++) {
+        P[h].li = (int)h;
+        int K = sp[h].K;
+        P[h].ok = hss_make_tree(P[h].T, K, ni);
+        int ltr = std::max(P[h].T.ltr, 1);
+        long dds = 0;
+        for (int x : P[h].T.m) dds += (long)x * x;
+        size_t o = al64(6 * sizeof(double) * K) + al64(2 * sizeof(int) * K) + al64(sizeof(int) * (11 * ltr + 8)) +
+                   al64(sizeof(long) * (4 * ltr)) + al64(sizeof(unsigned long long) * 2);
+        metasz[h] = o;
+        o += al64(sizeof(double) * std::max(dds, 1L));
+        o += al64(sizeof(double) * (size_t)12 * ni * K);           // arena
+        o += al64(sizeof(double) * (size_t)3 * K * std::max(P[h].T.lm, 1));  // scratch
+        base[h] = tot;
+        tot += o;
+    }
+    CK(ensure(c->hss, tot));
+    char *hb = (char *)c->hss.p;
+    for (size_t h = 0; h < sp.size(); h++) {
+        HssPlanM &Q = P[h];
+        int K = sp[h].K, ltr = std::max(Q.T.ltr, 1);
+        char *p = hb + base[h];
+        HssMerge &H = Q.H;
+        H.FF = sp[h].FF; H.D0 = sp[h].D0; H.U0 = sp[h].U0; H.V0 = sp[h].V0;
+        H.DIFL = sp[h].DIFL; H.DIFR = sp[h].DIFR;
+        double *dp = (double *)p;
+        H.D = dp; H.U = dp + K; H.V = dp + 2 * K; H.T = dp + 3 * K;
+        p += al64(6 * sizeof(double) * K);
+        int *ip = (int *)p;
+        H.PL = ip; H.PU = ip + K;
+        p += al64(2 * sizeof(int) * K);
+        ip = (int *)p;
+        char *meta0 = p;
+        H.lr = ip; H.lc = ip + 2 * ltr; H.usz = ip + 4 * ltr; H.vsz = ip + 6 * ltr; H.bsz = ip + 8 * ltr;
+        H.dsz = ip + 10 * ltr; H.lda = ip + 11 * ltr; H.lca = ip + 11 * ltr + 1; H.err = ip + 11 * ltr + 2;
+        p += al64(sizeof(int) * (11 * ltr + 8));
+        long *lp = (long *)p;
+        H.pu = lp; H.pv = lp + ltr; H.pb = lp + 2 * ltr; H.pd = lp + 3 * ltr;
+        p += al64(sizeof(long) * (4 * ltr));
+        H.arena_top = (unsigned long long *)p; H.scratch_top = H.arena_top + 1;
+        p += al64(sizeof(unsigned long long) * 2);
+        HK(hipMemsetAsync(meta0, 0, p - meta0, s));
+        H.dd = (double *)p;
+        long dds = 0;
+        for (int x : Q.T.m) dds += (long)x * x;
+        p += al64(sizeof(double) * std::max(dds, 1L));
+        H.arena = (double *)p; H.arena_cap = (long)12 * ni * K;
+        p += al64(sizeof(double) * (size_t)12 * ni * K);
+        H.scratch = (double *)p; H.scratch_cap = (long)3 * K * std::max(Q.T.lm, 1);
+        H.K = K; H.ltr = Q.T.ltr; H.lvl = Q.T.lvl;
+    }
+    // ---- host tables: merges, leaves (with leaf lr/lc and D offsets), level steps
+    std::vector<HssMerge> hh;
+    std::vector<int> hmap;  // table index -> plan index
+    for (size_t h = 0; h < P.size(); h++) if (P[h].ok) { hmap.push_back((int)h); hh.push_back(P[h].H); }
+    const int nh = (int)hh.size();
+    int maxlvl = 0;
+    for (int t = 0; t < nh; t++) maxlvl = std::max(maxlvl, P[hmap[t]].T.lvl);
+    std::vector<HssNode> nodes;   // leaves first, then per step
+    std::vector<int> init_lrlc;   // per merge: 2*ltr lr then 2*ltr lc (leaves set, others 0)
+    std::vector<long> init_pd;
+    size_t leaf0 = 0, nleaf = 0;
+    std::vector<size_t> step_off(maxlvl + 1), step_cnt(maxlvl + 1);
+    std::vector<HssCompact> comps;
+    std::vector<int> order;
+    std::vector<size_t> comp_off(maxlvl + 1), comp_cnt(maxlvl + 1);
+    for (int t = 0; t < nh; t++) {
+        const HssTree &T = P[hmap[t]].T;
+        long pdt = 0;
+        int it = 0;
+        for (int i = 1; i <= T.ltr; i++)
+            if (T.c1[i] == 0) {
+                HssNode nd;
+                nd.mrg = t; nd.node = i; nd.c1 = 0; nd.c2 = 0; nd.sib = 0;
+                nd.leaf_r0 = T.leaf_r0[i]; nd.leaf_m = T.m[it++];
+                nodes.push_back(nd);
+            }
+        (void)pdt;
+    }
+    nleaf = nodes.size();
+    for (int st = 0; st < maxlvl; st++) {
+        step_off[st] = nodes.size();
+        comp_off[st] = comps.size();
+        for (int t = 0; t < nh; t++) {
+            const HssTree &T = P[hmap[t]].T;
+            int LL = T.lvl - st;
+            if (LL < 1) continue;
+            for (int nd_id : T.lev[LL]) {
+                HssNode nd;
+                nd.mrg = t; nd.node = nd_id; nd.c1 = T.c1[nd_id]; nd.c2 = T.c2[nd_id];
+                int par = T.tr[nd_id - 1];
+                nd.sib = (T.c1[par] == nd_id) ? T.c2[par] : T.c1[par];
+                nd.leaf_r0 = T.leaf_r0[nd_id]; nd.leaf_m = 0;
+                nodes.push_back(nd);
+            }
+            HssCompact cp;
+            cp.mrg = t; cp.start = (int)order.size(); cp.nlevel = (int)T.lev[LL].size();
+            for (int nd_id : T.lev[LL]) order.push_back(nd_id);
+            for (int LLt = LL - 1; LLt >= 1; LLt--)
+                for (int nd_id : T.lev[LLt]) if (T.c1[nd_id] == 0) order.push_back(nd_id);
+            cp.count = (int)order.size() - cp.start;
+            comps.push_back(cp);
+        }
+        step_cnt[st] = nodes.size() - step_off[st];
+        comp_cnt[st] = comps.size() - comp_off[st];
+    }
+    // initial leaf ranges and D offsets go straight into each merge's arrays
+    for (int t = 0; t < nh; t++) {
+        const HssTree &T = P[hmap[t]].T;
+        std::vector<int> lrlc(4 * T.ltr, 0);
+        std::vector<long> pd(T.ltr, 0);
+        long pdt = 0;
+        int it = 0;
+        for (int i = 1; i <= T.ltr; i++)
+            if (T.c1[i] == 0) {
+                lrlc[2 * (i - 1)] = T.leaf_r0[i] + 1;
+                lrlc[2 * (i - 1) + 1] = T.leaf_r0[i] + T.m[it];
+                pd[i - 1] = pdt;
+                pdt += (long)T.m[it] * T.m[it];
+                it++;
+            }
+        for (int i = 0; i < 2 * T.ltr; i++) lrlc[2 * T.ltr + i] = lrlc[i];
+        HK(hipMemcpyAsync(hh[t].lr, lrlc.data(), sizeof(int) * 4 * T.ltr, hipMemcpyHostToDevice, s));
+        HK(hipMemcpyAsync(hh[t].pd, pd.data(), sizeof(long) * T.ltr, hipMemcpyHostToDevice, s));
+        HK(hipStreamSynchronize(s));
+    }
+    size_t t0 = 0, t1 = al64(t0 + sizeof(HssMerge) * std::max(nh, 1)),
+           t2 = al64(t1 + sizeof(HssNode) * std::max(nodes.size(), (size_t)1)),
+           t3 = al64(t2 + sizeof(HssCompact) * std::max(comps.size(), (size_t)1)),
+           t4 = al64(t3 + sizeof(int) * std::max(order.size(), (size_t)1));
+    CK(ensure(c->htab, t4));
+    {
+        std::vector<char> hbuf(t4);
+        memcpy(hbuf.data() + t0, hh.data(), sizeof(HssMerge) * nh);
+        memcpy(hbuf.data() + t1, nodes.data(), sizeof(HssNode) * nodes.size());
+        if (!comps.empty()) memcpy(hbuf.data() + t2, comps.data(), sizeof(HssCompact) * comps.size());
+        if (!order.empty()) memcpy(hbuf.data() + t3, order.data(), sizeof(int) * order.size());
+        HK(hipMemcpyAsync(c->htab.p, hbuf.data(), t4, hipMemcpyHostToDevice, s));
+        HK(hipStreamSynchronize(s));
+    }
+    char *tb = (char *)c->htab.p;
+    const HssMerge *dH = (const HssMerge *)(tb + t0);
+    const HssNode *dN = (const HssNode *)(tb + t1);
+    const HssCompact *dC = (const HssCompact *)(tb + t2);
+    const int *dO = (const int *)(tb + t3);
+    if (nh > 0) {
+        int kmax = 0;
+        for (auto &H : hh) kmax = std::max(kmax, H.K);
+        hipLaunchKernelGGL(k_hss_init, dim3(std::min(cdiv(kmax, WGS), 256), nh), dim3(WGS), 0, s, dH);
+        hipLaunchKernelGGL(k_hss_dgen, dim3(nleaf), dim3(WGS), 0, s, dH, dN + leaf0, (int)nleaf);
+        for (int st = 0; st < maxlvl; st++) {
+            int nn = (int)step_cnt[st], nc = (int)comp_cnt[st];
+            const HssNode *sn = dN + step_off[st];
+            const HssCompact *sc = dC + comp_off[st];
+            hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
+            hipLaunchKernelGGL(k_hss_rowcomp, dim3(nn), dim3(WGS), 0, s, dH, sn, nn, tol);
+            hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 1);
+            hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
+            hipLaunchKernelGGL(k_hss_colcomp, dim3(nn), dim3(WGS), 0, s, dH, sn, nn, tol);
+            hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 0);
+            hipLaunchKernelGGL(k_hss_bgen, dim3(nn), dim3(WGS), 0, s, dH, sn, nn);
+        }
+        HK(hipGetLastError());
+    }
+    // ---- read back the generator directory
+    for (int t = 0; t < nh; t++) {
+        HssPlanM &Q = P[hmap[t]];
+        int ltr = Q.T.ltr;
+        std::vector<int> meta(11 * ltr + 8);
+        std::vector<long> offs(4 * ltr);
+        HK(hipMemcpyAsync(meta.data(), hh[t].lr, sizeof(int) * meta.size(), hipMemcpyDeviceToHost, s));
+        HK(hipMemcpyAsync(offs.data(), hh[t].pu, sizeof(long) * offs.size(), hipMemcpyDeviceToHost, s));
+        HK(hipStreamSynchronize(s));
+        Q.usz.assign(meta.begin() + 4 * ltr, meta.begin() + 6 * ltr);
+        Q.vsz.assign(meta.begin() + 6 * ltr, meta.begin() + 8 * ltr);
+        Q.bsz.assign(meta.begin() + 8 * ltr, meta.begin() + 10 * ltr);
+        Q.dsz.assign(meta.begin() + 10 * ltr, meta.begin() + 11 * ltr);
+        Q.err = meta[11 * ltr + 2];
+        Q.pu.assign(offs.begin(), offs.begin() + ltr);
+        Q.pv.assign(offs.begin() + ltr, offs.begin() + 2 * ltr);
+        Q.pb.assign(offs.begin() + 2 * ltr, offs.begin() + 3 * ltr);
+        Q.pd.assign(offs.begin() + 3 * ltr, offs.begin() + 4 * ltr);
+        if (Q.err) Q.ok = false;
+        for (int i = 0; i < ltr - 1; i++) {
+            c->st.rank_max = std::max(c->st.rank_max, Q.usz[2 * i + 1]);
+            c->st.rank_max = std::max(c->st.rank_max, Q.vsz[2 * i]);
+        }
+    }
+    // G / F column layout (fastHssmmL_omp lg / lf, ConstructHssd.f90:1472-1510)
+    long gft = 0;
+    std::vector<long> gbase(P.size()), fbase(P.size());
+    long cbt = 0;
+    std::vector<long> cbase(P.size(), -1);
+    for (size_t h = 0; h < P.size(); h++) {
+        HssPlanM &Q = P[h];
+        if (!Q.ok) {
+            cbase[h] = cbt;
+            cbt += (long)sp[h].K * sp[h].K;
+            continue;
+        }
+        const HssTree &T = Q.T;
+        Q.lg.assign(T.ltr + 1, 0);
+        Q.lf.assign(T.ltr + 1, 0);
+        long it = 0;
+        for (int LL = T.lvl; LL >= 1; LL--)
+            for (int nd : T.lev[LL]) { Q.lg[nd] = (int)it; it += Q.usz[2 * (nd - 1) + 1]; }
+        Q.gcols = it;
+        it = 0;
+        for (int LL = 1; LL <= T.lvl; LL++)
+            for (int nd : T.lev[LL]) {
+                int par = T.tr[nd - 1];
+                int nj = (T.c1[par] == nd) ? T.c2[par] : T.c1[par];
+                Q.lf[nd] = (int)it;
+                it += Q.bsz[2 * (nj - 1) + 1];
+            }
+        Q.fcols = it;
+        gbase[h] = gft;
+        gft += (long)sp[h].nrow * Q.gcols;
+        fbase[h] = gft;
+        gft += (long)sp[h].nrow * Q.fcols;
+    }
+    CK(ensure(c->gf, sizeof(double) * std::max(gft, 1L)));
+    CK(ensure(c->cb, sizeof(double) * std::max(cbt, 1L)));
+    double *GF = (double *)c->gf.p, *CB = (double *)c->cb.p;
+    // ---- GEMM steps
+    std::vector<std::vector<BG>> steps;
+    auto bg = [](const double *A, long lda, const double *B, long ldb, double *C, long ldc, int nrow, int n, int k,
+                 int beta) {
+        BG g; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc; g.nrow = nrow; g.n = n; g.k = k;
+        g.beta = beta; g.btrans = 0; return g;
+    };
+    // dense fallback merges: C = Cauchy matrix, Y = X * C
+    std::vector<BG> dense_items;
+    for (size_t h = 0; h < P.size(); h++) {
+        if (P[h].ok) continue;
+        int K = sp[h].K;
+        // the HssMerge of this merge may not be in the device table; upload it alone
+        HssMerge H1 = P[h].H;
+        CK(ensure(c->tabs, sizeof(HssMerge)));
+        HK(hipMemcpyAsync(c->tabs.p, &H1, sizeof(HssMerge), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_cbuild, dim3(std::min(cdiv((long)K * K, WGS), 4096)), dim3(WGS), 0, s,
+                           (const HssMerge *)c->tabs.p, 0, CB + cbase[h]);
+        HK(hipStreamSynchronize(s));
+        dense_items.push_back(bg(sp[h].X, sp[h].ldx, CB + cbase[h], K, sp[h].Y, sp[h].ldy, sp[h].nrow, K, K, 0));
+    }
+    if (!dense_items.empty()) steps.push_back(dense_items);
+    int mlvl = 0;
+    for (auto &Q : P) if (Q.ok) mlvl = std::max(mlvl, Q.T.lvl);
+    // upsweep
+    for (int st = 0; st < mlvl; st++) {
+        std::vector<BG> v;
+        for (size_t h = 0; h < P.size(); h++) {
+            HssPlanM &Q = P[h];
+            if (!Q.ok) continue;
+            int LL = Q.T.lvl - st;
+            if (LL < 1) continue;
+            const long nr = sp[h].nrow;
+            double *G = GF + gbase[h];
+            const double *ar = Q.H.arena;
+            for (int nd : Q.T.lev[LL]) {
+                int ur = Q.usz[2 * (nd - 1)], uc = Q.usz[2 * (nd - 1) + 1];
+                bool leaf = Q.T.c1[nd] == 0;
+                const double *A = leaf ? sp[h].X + (long)Q.T.leaf_r0[nd] * sp[h].ldx : G + (long)Q.lg[Q.T.c1[nd]] * nr;
+                long lda = leaf ? sp[h].ldx : nr;
+                if (uc > 0) v.push_back(bg(A, lda, ar + Q.pu[nd - 1], ur, G + (long)Q.lg[nd] * nr, nr, (int)nr, uc, ur, 0));
+            }
+        }
+        if (!v.empty()) steps.push_back(v);
+    }
+    // siblings and downsweep, level by level from the root
+    for (int LL = 1; LL <= mlvl; LL++) {
+        std::vector<BG> vs, vd;
+        for (size_t h = 0; h < P.size(); h++) {
+            HssPlanM &Q = P[h];
+            if (!Q.ok || LL > Q.T.lvl) continue;
+            const long nr = sp[h].nrow;
+            double *G = GF + gbase[h], *F = GF + fbase[h];
+            const double *ar = Q.H.arena;
+            for (int nd : Q.T.lev[LL]) {
+                int par = Q.T.tr[nd - 1];
+                int nj = (Q.T.c1[par] == nd) ? Q.T.c2[par] : Q.T.c1[par];
+                int br = Q.bsz[2 * (nj - 1)], bc = Q.bsz[2 * (nj - 1) + 1];
+                if (bc > 0)
+                    vs.push_back(bg(G + (long)Q.lg[nj] * nr, nr, ar + Q.pb[nj - 1], std::max(br, 1),
+                                    F + (long)Q.lf[nd] * nr, nr, (int)nr, bc, br, LL >= 2 ? 1 : 0));
+                if (Q.T.c1[nd] != 0) {
+                    int vr = Q.vsz[2 * (nd - 1)], vc = Q.vsz[2 * (nd - 1) + 1];
+                    if (vc > 0)
+                        vd.push_back(bg(F + (long)Q.lf[nd] * nr, nr, ar + Q.pv[nd - 1], std::max(vr, 1),
+                                        F + (long)Q.lf[Q.T.c1[nd]] * nr, nr, (int)nr, vc, vr, 0));
+                }
+            }
+        }
+        if (!vs.empty()) steps.push_back(vs);
+        if (!vd.empty()) steps.push_back(vd);
+    }
+    // leaves: Y_i = F_i V_i + X_i D_i
+    {
+        std::vector<BG> v1, v2;
+        for (size_t h = 0; h < P.size(); h++) {
+            HssPlanM &Q = P[h];
+            if (!Q.ok) continue;
+            const long nr = sp[h].nrow;
+            double *F = GF + fbase[h];
+            double *Y = sp[h].Y;
+            const double *ar = Q.H.arena;
+            for (int nd = 1; nd <= Q.T.ltr - 1; nd++) {
+                if (Q.T.c1[nd] != 0) continue;
+                int vr = Q.vsz[2 * (nd - 1)], vc = Q.vsz[2 * (nd - 1) + 1], md = Q.dsz[nd - 1];
+                long lx = Q.T.leaf_r0[nd];
+                v1.push_back(bg(F + (long)Q.lf[nd] * nr, nr, ar + Q.pv[nd - 1], std::max(vr, 1),
+                                Y + lx * sp[h].ldy, sp[h].ldy, (int)nr, vc, vr, 0));
+                v2.push_back(bg(sp[h].X + lx * sp[h].ldx, sp[h].ldx, Q.H.dd + Q.pd[nd - 1], md, Y + lx * sp[h].ldy,
+                                sp[h].ldy, (int)nr, md, md, 1));
+            }
+        }
+        if (!v1.empty()) steps.push_back(v1);
+        if (!v2.empty()) steps.push_back(v2);
+    }
+    size_t nitems = 0;
+    for (auto &v : steps) nitems += v.size();
+    CK(ensure(c->tabs, sizeof(BG) * std::max(nitems, (size_t)1)));
+    {
+        std::vector<BG> all;
+        for (auto &v : steps) all.insert(all.end(), v.begin(), v.end());
+        HK(hipMemcpyAsync(c->tabs.p, all.data(), sizeof(BG) * all.size(), hipMemcpyHostToDevice, s));
+        HK(hipStreamSynchronize(s));
+    }
+    size_t so = 0;
+    for (auto &v : steps) {
+        int rmax = 0, nmax = 0;
+        for (auto &g : v) { rmax = std::max(rmax, g.nrow); nmax = std::max(nmax, g.n); }
+        hipLaunchKernelGGL(k_bgemm, dim3(cdiv(rmax, 64), (unsigned)v.size(), cdiv(nmax, 64)), dim3(WGS), 0, s,
+                           (const BG *)c->tabs.p + so);
+        so += v.size();
+    }
+    HK(hipGetLastError());
+    // the tables above are host vectors copied synchronously; wait before they go
+    HK(hipStreamSynchronize(s));
+    if (keep) *keep = P;
+    return 0;
+}
+
+// HSS branch of the merges `sel` of one DC level: X = Q2 gathered from the
+// eigenvector slots, Ct(:, 1:K) = X * H.
+static int run_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, int m0,
+                   const std::vector<MergeOut> &ho, const std::vector<int> &sel, const std::vector<long> &coff,
+                   Prob *dprobs, int cur)
+{
+    hipStream_t s = c->stream;
+    Work W = c->W;
+    std::vector<int> hidx(sel.size()), xcol(sel.size() + 1, 0);
+    std::vector<long> xoff(sel.size());
+    long xt = 0;
+    for (size_t h = 0; h < sel.size(); h++) {
+        const Merge &M = hm[m0 + sel[h]];
+        hidx[h] = sel[h];
+        xoff[h] = xt;
+        xt += (long)M.n * ho[sel[h]].k;
+        xcol[h + 1] = xcol[h] + ho[sel[h]].k;
+    }
+    CK(ensure(c->xb, sizeof(double) * std::max(xt, 1L)));
+    double *X = (double *)c->xb.p;
+    {
+        size_t u0 = 0, u1 = al64(sizeof(int) * sel.size()), u2 = al64(u1 + sizeof(int) * (sel.size() + 1)),
+               u3 = al64(u2 + sizeof(long) * sel.size());
+        std::vector<char> buf(u3);
+        memcpy(buf.data() + u0, hidx.data(), sizeof(int) * sel.size());
+        memcpy(buf.data() + u1, xcol.data(), sizeof(int) * (sel.size() + 1));
+        memcpy(buf.data() + u2, xoff.data(), sizeof(long) * sel.size());
+        CK(ensure(c->misc, u3));
+        HK(hipMemcpyAsync(c->misc.p, buf.data(), u3, hipMemcpyHostToDevice, s));
+        HK(hipStreamSynchronize(s));
+        char *mp = (char *)c->misc.p;
+        hipLaunchKernelGGL(k_gatherx, dim3(xcol[sel.size()]), dim3(WGS), 0, s, dmg, (const int *)(mp + u0),
+                           (const int *)(mp + u1), (int)sel.size(), dprobs, W, cur, X, (const long *)(mp + u2));
+        HK(hipGetLastError());
+    }
+    std::vector<HssSpec> sp(sel.size());
+    double *Ct = (double *)c->ct.p;
+    for (size_t h = 0; h < sel.size(); h++) {
+        const Merge &M = hm[m0 + sel[h]];
+        HssSpec &q = sp[h];
+        q.K = ho[sel[h]].k;
+        q.FF = W.dlamda + M.s; q.D0 = W.lam + M.s; q.U0 = W.wnew + M.s; q.V0 = W.alpha + M.s;
+        q.DIFL = W.difl + M.s; q.DIFR = W.difr + M.s;
+        q.X = X + xoff[h]; q.ldx = M.n; q.Y = Ct + coff[sel[h]]; q.ldy = M.n; q.nrow = M.n;
+    }
+    return hss_apply(c, sp, c->prm.hss_leaf, c->prm.hss_tol, nullptr);
+}
+
 // one level (or one merge of it): merges [m0, m1) of table dm (device) / hm (host);
 // dchunkN is the device chunk table of exactly this slice (first entry 0)
 static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm, int m0, int m1,
@@ -286,6 +729,13 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
         if (!tiles.empty())
             hipLaunchKernelGGL(k_gemm, dim3(tiles.size()), dim3(WGS), 0, s, dmg, dtiles, dprobs, dmo, dsoff,
                                dcoff, W, cur, (const double *)c->sb.p, (double *)c->ct.p);
+        std::vector<int> sel;
+        for (int i = 0; i < nm; i++)
+            if (!ho[i].flags && ho[i].hss && ho[i].k > 0) sel.push_back(i);
+        if (!sel.empty()) {
+            hipLaunchKernelGGL(k_alpha, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, W);
+            CK(run_hss(c, dmg, hm, m0, ho, sel, coff, dprobs, cur));
+        }
     }
     hipLaunchKernelGGL(k_newcols, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dmo, W, cur);
     if (anyk)
@@ -758,6 +1208,170 @@ int hsdc_secular(hsdc_ctx *c, int k, const double *dlamda, const double *w, doub
     HK(hipMemcpy(org, W.org, sizeof(int) * k, hipMemcpyDeviceToHost));
     HK(hipMemcpy(niter, W.nit, sizeof(int) * k, hipMemcpyDeviceToHost));
     return info ? 1 : 0;
+}
+
+int hsdc_laed3(hsdc_ctx *c, int k, int n, int n1, double *d, double *q, int ldq, double rho, double *dlamda,
+               const double *q2, const int *indx, const int *ctot, double *w)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (k < 0) return -1;
+    if (n < k) return -2;
+    if (ldq < (n > 1 ? n : 1)) return -6;
+    if (k == 0) return 0;
+    Merge *dm;
+    int *dch;
+    CK(single_merge_setup(c, n, n1, rho, &dm, &dch));
+    Work W = c->W;
+    hipStream_t s = c->stream;
+    const int n2 = n - n1, n12 = ctot[0] + ctot[1], n23 = ctot[1] + ctot[2];
+    const long q2len = (long)n1 * n12 + (long)n2 * n23;
+    CK(ensure(c->mo, al64(sizeof(MergeOut)) + 128));
+    MergeOut o;
+    memset(&o, 0, sizeof(o));
+    o.k = k; o.rho = rho; o.hss = 0;
+    for (int t = 0; t < 4; t++) o.ctot[t] = ctot[t];
+    MergeOut *dmo = (MergeOut *)c->mo.p;
+    int *dinfo = (int *)((char *)c->mo.p + al64(sizeof(MergeOut)));
+    int *dtab = dinfo + 16;  // block table {0, blocks} and chunk table {0, chunks}
+    int zero = 0, tabs[4] = {0, cdiv(k, 64), 0, cdiv(k, WGS)};
+    HK(hipMemcpy(dmo, &o, sizeof(o), hipMemcpyHostToDevice));
+    HK(hipMemcpy(dinfo, &zero, sizeof(int), hipMemcpyHostToDevice));
+    HK(hipMemcpy(dtab, tabs, sizeof(tabs), hipMemcpyHostToDevice));
+    HK(hipMemcpy(W.dlamda, dlamda, sizeof(double) * k, hipMemcpyHostToDevice));
+    HK(hipMemcpy(W.w, w, sizeof(double) * k, hipMemcpyHostToDevice));
+    HK(hipMemcpy(W.indxc, indx, sizeof(int) * k, hipMemcpyHostToDevice));
+    CK(ensure(c->sb, sizeof(double) * (size_t)k * k + 64));
+    CK(ensure(c->xb, sizeof(double) * std::max(q2len, 1L)));
+    CK(ensure(c->yb, sizeof(double) * (size_t)n * k));
+    CK(ensure(c->lvl, 64));
+    long zoff = 0;
+    HK(hipMemcpy(c->lvl.p, &zoff, sizeof(long), hipMemcpyHostToDevice));
+    if (q2len) HK(hipMemcpy(c->xb.p, q2, sizeof(double) * q2len, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_secular, dim3(tabs[1]), dim3(64), 0, s, (const Merge *)dm, (const int *)dtab, 1,
+                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo);
+    hipLaunchKernelGGL(k_loewner, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
+                       (const MergeOut *)dmo, W);
+    hipLaunchKernelGGL(k_sbuild, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
+                       (const MergeOut *)dmo, (const long *)c->lvl.p, W, (double *)c->sb.p);
+    // the two GEMMs of DLAED3 with the packed Q2 (types 1,2 rows on top, 2,3 below)
+    const double *S = (const double *)c->sb.p, *Q2 = (const double *)c->xb.p;
+    double *Y = (double *)c->yb.p;
+    BG g[2];
+    g[0].A = Q2; g[0].lda = std::max(n1, 1); g[0].B = S; g[0].ldb = k; g[0].C = Y; g[0].ldc = n;
+    g[0].nrow = n1; g[0].n = k; g[0].k = n12; g[0].beta = 0; g[0].btrans = 1;
+    g[1].A = Q2 + (long)n1 * n12; g[1].lda = std::max(n2, 1); g[1].B = S + (long)ctot[0] * k; g[1].ldb = k;
+    g[1].C = Y + n1; g[1].ldc = n; g[1].nrow = n2; g[1].n = k; g[1].k = n23; g[1].beta = 0; g[1].btrans = 1;
+    CK(ensure(c->tabs, sizeof(g)));
+    HK(hipMemcpy(c->tabs.p, g, sizeof(g), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_bgemm, dim3(cdiv(std::max(n1, n2), 64), 2, cdiv(k, 64)), dim3(WGS), 0, s,
+                       (const BG *)c->tabs.p);
+    HK(hipGetLastError());
+    int info = 0;
+    HK(hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost));
+    HK(hipMemcpy(d, W.lam, sizeof(double) * k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy2D(q, sizeof(double) * ldq, Y, sizeof(double) * n, sizeof(double) * n, k, hipMemcpyDeviceToHost));
+    if (k >= 3) HK(hipMemcpy(w, W.wnew, sizeof(double) * k, hipMemcpyDeviceToHost));
+    return info ? 1 : 0;
+}
+
+int hsdc_loewner(hsdc_ctx *c, int k, double rho, double *dlamda, double *z, double *d, double *difl,
+                 double *difr, double *alpha)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (k < 3) return -2;
+    Merge *dm;
+    int *dch;
+    CK(single_merge_setup(c, k, k / 2, rho, &dm, &dch));
+    Work W = c->W;
+    hipStream_t s = c->stream;
+    CK(ensure(c->mo, al64(sizeof(MergeOut)) + 128));
+    MergeOut o;
+    memset(&o, 0, sizeof(o));
+    o.k = k; o.rho = rho; o.hss = 1;
+    MergeOut *dmo = (MergeOut *)c->mo.p;
+    int *dinfo = (int *)((char *)c->mo.p + al64(sizeof(MergeOut)));
+    int *dtab = dinfo + 16;
+    int zero = 0, tabs[4] = {0, cdiv(k, 64), 0, cdiv(k, WGS)};
+    HK(hipMemcpy(dmo, &o, sizeof(o), hipMemcpyHostToDevice));
+    HK(hipMemcpy(dinfo, &zero, sizeof(int), hipMemcpyHostToDevice));
+    HK(hipMemcpy(dtab, tabs, sizeof(tabs), hipMemcpyHostToDevice));
+    HK(hipMemcpy(W.dlamda, dlamda, sizeof(double) * k, hipMemcpyHostToDevice));
+    HK(hipMemcpy(W.w, z, sizeof(double) * k, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_secular, dim3(tabs[1]), dim3(64), 0, s, (const Merge *)dm, (const int *)dtab, 1,
+                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo);
+    hipLaunchKernelGGL(k_loewner, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
+                       (const MergeOut *)dmo, W);
+    hipLaunchKernelGGL(k_alpha, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
+                       (const MergeOut *)dmo, W);
+    HK(hipGetLastError());
+    int info = 0;
+    HK(hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost));
+    HK(hipMemcpy(d, W.lam, sizeof(double) * k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(z, W.wnew, sizeof(double) * k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(difl, W.difl, sizeof(double) * k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(difr, W.difr, sizeof(double) * k, hipMemcpyDeviceToHost));
+    HK(hipMemcpy(alpha, W.alpha, sizeof(double) * k, hipMemcpyDeviceToHost));
+    return info ? 1 : 0;
+}
+
+int hsdc_hss_evc(hsdc_ctx *c, int k, int ni, double tol, const double *f, const double *d, const double *u,
+                 const double *v, const double *difl, const double *difr, double *q, int ldq, int nrow, int *ranks)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (k < 1) return -2;
+    if (ni < 8) return -3;
+    if (ldq < std::max(nrow, 1)) return -12;
+    CK(alloc_work(c, k));
+    Work W = c->W;
+    hipStream_t s = c->stream;
+    const double *src[6] = {f, d, u, v, difl, difr};
+    double *dst[6] = {W.dlamda, W.lam, W.wnew, W.alpha, W.difl, W.difr};
+    for (int i = 0; i < 6; i++) HK(hipMemcpy(dst[i], src[i], sizeof(double) * k, hipMemcpyHostToDevice));
+    c->evc_gen.assign(6 * (size_t)k, 0.0);
+    for (int i = 0; i < 6; i++) memcpy(c->evc_gen.data() + (size_t)i * k, src[i], sizeof(double) * k);
+    c->evc_k = k; c->evc_ni = ni; c->evc_tol = tol;
+    CK(ensure(c->xb, sizeof(double) * (size_t)std::max(nrow, 1) * k));
+    CK(ensure(c->yb, sizeof(double) * (size_t)std::max(nrow, 1) * k));
+    HK(hipMemcpy2D(c->xb.p, sizeof(double) * nrow, q, sizeof(double) * ldq, sizeof(double) * nrow, k,
+                   hipMemcpyHostToDevice));
+    HssSpec sp;
+    sp.K = k;
+    sp.FF = W.dlamda; sp.D0 = W.lam; sp.U0 = W.wnew; sp.V0 = W.alpha; sp.DIFL = W.difl; sp.DIFR = W.difr;
+    sp.X = (const double *)c->xb.p; sp.ldx = nrow; sp.Y = (double *)c->yb.p; sp.ldy = nrow; sp.nrow = nrow;
+    std::vector<HssSpec> v1(1, sp);
+    std::vector<HssPlanM> keep;
+    CK(hss_apply(c, v1, ni, tol, &keep));
+    HK(hipMemcpy2D(q, sizeof(double) * ldq, c->yb.p, sizeof(double) * nrow, sizeof(double) * nrow, k,
+                   hipMemcpyDeviceToHost));
+    if (ranks) {
+        const HssPlanM &Q = keep[0];
+        if (!Q.ok) {
+            ranks[0] = -1;
+        } else {
+            for (int i = 0; i < Q.T.ltr; i++) {
+                ranks[2 * i] = Q.usz[2 * i + 1];
+                ranks[2 * i + 1] = Q.vsz[2 * i];
+            }
+        }
+    }
+    return 0;
+}
+
+int hsdc_hss_dense(hsdc_ctx *c, double *a, int lda)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    int k = c->evc_k;
+    if (k < 1) return -1;
+    if (lda < k) return -3;
+    std::vector<double> g = c->evc_gen;
+    std::vector<double> eye((size_t)k * k, 0.0);
+    for (int i = 0; i < k; i++) eye[(size_t)i * k + i] = 1.0;
+    int rc = hsdc_hss_evc(c, k, c->evc_ni, c->evc_tol, g.data(), g.data() + k, g.data() + 2 * (size_t)k,
+                          g.data() + 3 * (size_t)k, g.data() + 4 * (size_t)k, g.data() + 5 * (size_t)k, eye.data(), k,
+                          k, nullptr);
+    if (rc) return rc;
+    for (int j = 0; j < k; j++) memcpy(a + (size_t)j * lda, eye.data() + (size_t)j * k, sizeof(double) * k);
+    return 0;
 }
 
 int hsdc_stedc(hsdc_ctx *c, char compz, int n, double *d, double *e, double *z, int ldz, double *work,

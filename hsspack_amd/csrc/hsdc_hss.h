@@ -1,0 +1,56 @@
+// Device descriptors of the HSS construction and multiply.
+#pragma once
+#include <stdint.h>
+
+#define HSS_MAXM 512  // largest row/column block a node may compress (LDS tile)
+
+// One HSS merge (the K x K Cauchy-like eigenvector matrix of one rank-one merge).
+// Arrays are 1-based through A1() in the kernels, like the Fortran they restate.
+struct HssMerge {
+    const double *FF, *D0, *U0, *V0, *DIFL, *DIFR;  // poles, roots, z, alpha, differences (K)
+    double *D, *U, *V;      // compacted column / row generators (K)
+    int *PL, *PU;           // their original indices (K)
+    double *T;              // compaction scratch (K doubles + K ints)
+    int *lr, *lc;           // node row / column ranges in the compacted arrays (2*ltr)
+    int *lda, *lca;         // current compacted row / column counts
+    int *usz, *vsz, *bsz, *dsz;   // generator shapes (2 per node; dsz 1 per node)
+    long *pu, *pv, *pb, *pd;      // generator offsets (arena / dd)
+    double *arena;          // U, V, B generators
+    unsigned long long *arena_top;
+    long arena_cap;
+    double *dd;             // leaf D blocks
+    double *scratch;        // private copies of the compressed block rows / columns
+    unsigned long long *scratch_top;
+    long scratch_cap;
+    int *err;               // bit0 block too large, bit1 arena full, bit2 scratch full
+    int K, ltr, lvl;
+};
+
+// One node of one merge's post-ordered HSS tree (aux_hss.f90:737-800).
+struct HssNode {
+    int mrg;        // index into the HssMerge table
+    int node;       // 1-based post-order id
+    int c1, c2;     // children (0 for a leaf)
+    int sib;        // sibling
+    int leaf_r0;    // leaf: first original index - 1
+    int leaf_m;     // leaf: size
+};
+
+// compaction order of one merge at one level: the nodes of the level, then the
+// leaves of shallower levels (CauchyHssEig.f90:2140-2168, 2195-2227)
+struct HssCompact {
+    int mrg;
+    int start, count;  // slice of the order table
+    int nlevel;        // the first nlevel entries are nodes of the level
+};
+
+// One GEMM of the batched HSS multiply: C(nrow x n) = A(nrow x k) B(k x n) + beta C
+struct BG {
+    const double *A;
+    const double *B;
+    double *C;
+    long lda, ldb, ldc;
+    int nrow, n, k;
+    int beta;
+    int btrans;  // 1: B(t, j) = B[t * ldb + j]
+};

@@ -1,0 +1,896 @@
+// HSS form of the eigenvector matrix of a rank-one merge and its product with the
+// old eigenvectors, for gfx950.
+//
+//   construction  Cauchy2hssEig_omp / ComprcauchyEig / rrluCauchyEigrow,col
+//                 (CauchyHssEig.f90:7-467, 1945-2440): one workgroup per HSS node
+//                 runs the structured rank-revealing LU on the Cauchy generators;
+//                 every element of the Schur complement it touches is formed from
+//                 the generators with the accurate differences of
+//                 computfd / computdd (CauchyHssEig.f90:470-568), so nothing of the
+//                 K x K matrix is ever stored.  Statement order follows the
+//                 reference, which keeps pivots, ranks and generators identical to
+//                 the CPU oracle's restatement.
+//   multiply      fastHssmmL_omp (ConstructHssd.f90:1394-1630): X := X * H as
+//                 batched FP64 MFMA GEMMs, one launch per tree level, over all
+//                 nodes of all HSS merges of the DC level and all row tiles of X.
+#include <float.h>
+#include "hsdc_internal.h"
+#include "hsdc_hss.h"
+
+#define WG 256
+#define A1(a, i) (a)[(i) - 1]
+
+// ---------------------------------------------------------------------------
+// accurate differences: f_{PL(j)} - lambda_{PU(k)} and lambda_{PU(j)} - lambda_{PU(k)}
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double fd(const HssMerge &H, int pd, int pw)
+{
+    if (pd <= pw) return (A1(H.FF, pd) - A1(H.FF, pw)) - A1(H.DIFL, pw);
+    return (A1(H.FF, pd) - A1(H.FF, pw + 1)) - A1(H.DIFR, pw);
+}
+__device__ __forceinline__ double dd(const HssMerge &H, int pj, int pk)
+{
+    if (pj < pk) return ((A1(H.FF, pj + 1) - A1(H.FF, pk)) + A1(H.DIFR, pj)) - A1(H.DIFL, pk);
+    return ((A1(H.FF, pj) - A1(H.FF, pk + 1)) + A1(H.DIFL, pj)) - A1(H.DIFR, pk);
+}
+
+// block-wide (max |value|, first index); idx <= 0 means "no entry"
+struct VI { double v; int i; };
+__device__ __forceinline__ VI vi_better(VI a, VI b)
+{
+    if (b.i <= 0) return a;
+    if (a.i <= 0) return b;
+    if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+    return a;
+}
+__device__ VI block_argmax(VI x, VI *sh)
+{
+    int tid = threadIdx.x;
+    for (int o = 32; o > 0; o >>= 1) {
+        VI y;
+        y.v = __shfl_down(x.v, o, 64);
+        y.i = __shfl_down(x.i, o, 64);
+        x = vi_better(x, y);
+    }
+    if ((tid & 63) == 0) sh[tid >> 6] = x;
+    __syncthreads();
+    VI r = sh[0];
+    for (int w = 1; w < WG / 64; w++) r = vi_better(r, sh[w]);
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// leaf D blocks (CauchylikeEig on the leaf rows/cols, CauchyHssEig.f90:2089-2100)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_hss_dgen(const HssMerge *hm, const HssNode *nodes, int nn)
+{
+    int b = blockIdx.x;
+    if (b >= nn) return;
+    HssNode N = nodes[b];
+    const HssMerge H = hm[N.mrg];
+    int m = N.leaf_m, r0 = N.leaf_r0;  // leaf rows/cols r0+1 .. r0+m (1-based)
+    double *out = H.dd + H.pd[N.node - 1];
+    for (int e = threadIdx.x; e < m * m; e += WG) {
+        int i = e % m, j = e / m;
+        int pf = r0 + 1 + i, pw = r0 + 1 + j;
+        double den = (pf <= pw) ? (A1(H.FF, pf) - A1(H.FF, pw)) - A1(H.DIFL, pw)
+                                : (A1(H.FF, pf) - A1(H.FF, pw + 1)) - A1(H.DIFR, pw);
+        out[e] = A1(H.U0, pf) * A1(H.V0, pw) / den;
+    }
+    if (threadIdx.x == 0) H.dsz[N.node - 1] = m;
+}
+
+__device__ long arena_take(const HssMerge &H, long n)
+{
+    unsigned long long o = atomicAdd(H.arena_top, (unsigned long long)n);
+    if ((long)(o + n) > H.arena_cap) { atomicOr(H.err, 2); return -1; }
+    return (long)o;
+}
+__device__ long scratch_take(const HssMerge &H, long n)
+{
+    unsigned long long o = atomicAdd(H.scratch_top, (unsigned long long)n);
+    if ((long)(o + n) > H.scratch_cap) { atomicOr(H.err, 4); return -1; }
+    return (long)o;
+}
+
+// ---------------------------------------------------------------------------
+// Row compression of one node: rrluCauchyEigrow + CauchylikeEigQ (ComprcauchyEig 'r')
+// rows: the node's M row generators (LDS); columns: the N column generators
+// outside the node (private global copies).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const HssNode *nodes, int nn,
+                                                    double tol)
+{
+    __shared__ double F[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
+    __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
+    __shared__ VI red[WG / 64];
+    __shared__ long s_off;
+    __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
+    __shared__ double s_nmax;
+    const int tid = threadIdx.x;
+    if ((int)blockIdx.x >= nn) return;
+    HssNode Nd = nodes[blockIdx.x];
+    const HssMerge H = hm[Nd.mrg];
+    int *lr = H.lr, *lc = H.lc;
+    const int node = Nd.node;
+    if (tid == 0) {
+        if (Nd.c1 > 0) {
+            lr[2 * (node - 1)] = lr[2 * (Nd.c1 - 1)]; lr[2 * (node - 1) + 1] = lr[2 * (Nd.c2 - 1) + 1];
+            lc[2 * (node - 1)] = lc[2 * (Nd.c1 - 1)]; lc[2 * (node - 1) + 1] = lc[2 * (Nd.c2 - 1) + 1];
+        }
+        int M = lr[2 * (node - 1) + 1] - lr[2 * (node - 1)] + 1;
+        int nic = lc[2 * (node - 1) + 1] - lc[2 * (node - 1)] + 1;
+        s_M = M;
+        s_N = *H.lca - nic;
+        // a merge whose construction already failed is finished densely; skip it
+        s_flag = (*H.err != 0 || M > HSS_MAXM || M < 1 || nic < 1 || s_N < 0) ? 1 : 0;
+        s_off = s_flag ? -1 : scratch_take(H, 3L * (s_N > 0 ? s_N : 1));
+        if (s_flag) atomicOr(H.err, 1);
+    }
+    __syncthreads();
+    if (s_flag || s_off < 0) return;
+    const int M = s_M, N = s_N, l0 = lr[2 * (node - 1)], c0 = lc[2 * (node - 1)], c1e = lc[2 * (node - 1) + 1];
+    double *D = H.scratch + s_off, *V = D + (N > 0 ? N : 1);
+    int *PU = (int *)(V + (N > 0 ? N : 1));
+    for (int j = tid; j < N; j += WG) {
+        int src = (j < c0 - 1) ? j + 1 : j + 1 + (c1e - c0 + 1);
+        D[j] = A1(H.D, src); V[j] = A1(H.V, src); PU[j] = A1(H.PU, src);
+    }
+    for (int i = tid; i < M; i += WG) {
+        PL[i] = A1(H.PL, l0 + i);
+        A[i] = A1(H.U, l0 + i);
+        F[i] = A1(H.FF, PL[i]);
+        Z[i] = A[i];
+        PQ[i] = i + 1;
+        W[i] = 0.0;
+    }
+    __syncthreads();
+    const int mn = M < N ? M : N;
+    double amax = 0.0;
+    int Rk = mn;
+    const double rho = 1.5;
+    for (int k = 1; k <= mn; k++) {
+        // ---- CauchyPivtEig: rook search on the k-th Schur complement
+        {
+            VI x = {0.0, 0};
+            int pwk = A1(PU, k);
+            for (int i = k + tid; i <= M; i += WG) {
+                int pli = A1(PL, i);
+                double v = fabs(A1(Z, i) / fd(H, pli, pwk));
+                x = vi_better(x, VI{v, i - k + 1});
+            }
+            VI cl = block_argmax(x, red);
+            VI y = {0.0, 0};
+            int fk = A1(PL, k);
+            for (int j = k + tid; j <= N; j += WG) {
+                double v = fabs(A1(V, j) / fd(H, fk, A1(PU, j)));
+                y = vi_better(y, VI{v, j - k + 1});
+            }
+            VI rw = block_argmax(y, red);
+            double junkL = cl.v * fabs(A1(V, k)), junkU = rw.v * fabs(A1(Z, k));
+            double piv = fabs(A1(Z, k) * A1(V, k) / fd(H, A1(PL, k), A1(PU, k)));
+            if (!(junkL <= piv && junkU <= piv)) {
+                int flg = (junkL > junkU) ? 1 : 0;
+                int jjL = cl.i, jjU = rw.i;
+                for (;;) {
+                    if (flg == 1) {
+                        int jj = jjL + k - 1;
+                        __syncthreads();
+                        if (tid == 0) {
+                            double t;
+                            t = A1(F, k); A1(F, k) = A1(F, jj); A1(F, jj) = t;
+                            t = A1(Z, k); A1(Z, k) = A1(Z, jj); A1(Z, jj) = t;
+                            t = A1(A, k); A1(A, k) = A1(A, jj); A1(A, jj) = t;
+                            int u = A1(PL, k); A1(PL, k) = A1(PL, jj); A1(PL, jj) = u;
+                            u = A1(PQ, k); A1(PQ, k) = A1(PQ, jj); A1(PQ, jj) = u;
+                        }
+                        __syncthreads();
+                        VI y2 = {0.0, 0};
+                        int fk2 = A1(PL, k);
+                        for (int j = k + tid; j <= N; j += WG) {
+                            double v = fabs(A1(V, j) / fd(H, fk2, A1(PU, j)));
+                            y2 = vi_better(y2, VI{v, j - k + 1});
+                        }
+                        jjU = block_argmax(y2, red).i;
+                        if (jjU == 1) break;
+                        flg = 0;
+                    }
+                    int jj = jjU + k - 1;
+                    __syncthreads();
+                    if (tid == 0) {
+                        double t;
+                        t = A1(D, k); A1(D, k) = A1(D, jj); A1(D, jj) = t;
+                        t = A1(V, k); A1(V, k) = A1(V, jj); A1(V, jj) = t;
+                        int u = A1(PU, k); A1(PU, k) = A1(PU, jj); A1(PU, jj) = u;
+                    }
+                    __syncthreads();
+                    VI x2 = {0.0, 0};
+                    int pwk2 = A1(PU, k);
+                    for (int i = k + tid; i <= M; i += WG) {
+                        double v = fabs(A1(Z, i) / fd(H, A1(PL, i), pwk2));
+                        x2 = vi_better(x2, VI{v, i - k + 1});
+                    }
+                    jjL = block_argmax(x2, red).i;
+                    if (jjL == 1) break;
+                    flg = 1;
+                }
+            }
+        }
+        __syncthreads();
+        double ukk = A1(Z, k) * A1(V, k) / fd(H, A1(PL, k), A1(PU, k));
+        amax = fmax(amax, fabs(ukk));
+        if (fabs(ukk) < tol * amax) {
+            // ---- CauchyPivtEig_CP: complete pivoting over the Schur complement
+            VI best = {0.0, 0};
+            VI bestrow = {0.0, 0};
+            for (int j = k + tid; j <= N; j += WG) {
+                int pw = A1(PU, j);
+                double mv = -DBL_MAX;
+                int mi = 0;
+                for (int i = k; i <= M; i++) {
+                    double v = fabs(A1(Z, i) / fd(H, A1(PL, i), pw));
+                    if (mi == 0 || v > mv) { mv = v; mi = i - k + 1; }
+                }
+                double jt = fabs(mv * A1(V, j));
+                VI c = {jt, j - k + 1};
+                VI nb = vi_better(best, c);
+                if (nb.i == c.i) bestrow.i = mi;
+                best = nb;
+            }
+            VI win = block_argmax(best, red);
+            __syncthreads();
+            if (win.i > 0 && best.i == win.i) { s_ii = bestrow.i; s_jj = win.i; }
+            __syncthreads();
+            if (tid == 0) {
+                int ii = s_ii, jj = s_jj;
+                int a = ii + k - 1, b = jj + k - 1;
+                double t;
+                t = A1(F, k); A1(F, k) = A1(F, a); A1(F, a) = t;
+                t = A1(Z, k); A1(Z, k) = A1(Z, a); A1(Z, a) = t;
+                t = A1(A, k); A1(A, k) = A1(A, a); A1(A, a) = t;
+                int u = A1(PL, k); A1(PL, k) = A1(PL, a); A1(PL, a) = u;
+                u = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = u;
+                t = A1(D, k); A1(D, k) = A1(D, b); A1(D, b) = t;
+                t = A1(V, k); A1(V, k) = A1(V, b); A1(V, b) = t;
+                u = A1(PU, k); A1(PU, k) = A1(PU, b); A1(PU, b) = u;
+            }
+            __syncthreads();
+            ukk = A1(Z, k) * A1(V, k) / fd(H, A1(PL, k), A1(PU, k));
+            amax = fmax(amax, fabs(ukk));
+            if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
+        }
+        // ---- generator updates of the (k+1)-th Schur complement
+        const double Fk = A1(F, k);
+        const int plk = A1(PL, k), puk = A1(PU, k);
+        for (int j = k + 1 + tid; j <= M; j += WG) {
+            double temp = fd(H, A1(PL, j), puk);
+            temp = (A1(F, j) - Fk) / temp;
+            A1(Z, j) = A1(Z, j) * temp;
+        }
+        for (int j = k + 1 + tid; j <= N; j += WG) {
+            double temp = dd(H, A1(PU, j), puk);
+            double temp2 = -1.0 * fd(H, plk, A1(PU, j));
+            temp = temp / temp2;
+            A1(V, j) = A1(V, j) * temp;
+        }
+        for (int j = 1 + tid; j <= k - 1; j += WG) {
+            double temp = -1.0 * fd(H, A1(PL, j), puk);
+            A1(W, j) = A1(W, j) * (temp / (Fk - A1(F, j)));
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double wk = fd(H, plk, puk) / A1(A, k);
+            for (int j = 1; j <= k - 1; j++) {
+                double temp = -1.0 * fd(H, plk, A1(PU, j));
+                wk = wk * (temp / (A1(F, j) - Fk));
+            }
+            A1(W, k) = wk;
+        }
+        __syncthreads();
+        // ---- every 10 steps: swap while max |Mat_N| > rho (searchMax2Eig)
+        if (k % 10 == 0 && M - k > 0) {
+            for (;;) {
+                VI x = {0.0, 0};
+                for (int jc = 1 + tid; jc <= k; jc += WG) {
+                    double fj = A1(F, jc), mv = 0.0;
+                    int mi = 0;
+                    for (int i = 1; i <= M - k; i++) {
+                        double v = fabs(A1(Z, k + i) / (A1(F, k + i) - fj));
+                        if (mi == 0 || v > mv) { mv = v; mi = i; }
+                    }
+                    double junk = mv * fabs(A1(W, jc));
+                    VI c = {junk, jc};
+                    VI nb = vi_better(x, c);
+                    if (nb.i == jc) IP[jc - 1] = mi;
+                    x = nb;
+                }
+                VI win = block_argmax(x, red);
+                if (!(win.v > rho)) break;
+                const int ii = win.i, jj = IP[ii - 1] + k;
+                __syncthreads();
+                const double Fii = A1(F, ii), Fjj = A1(F, jj);
+                const int plii = A1(PL, ii), pljj = A1(PL, jj);
+                for (int j = k + 1 + tid; j <= N; j += WG) {
+                    double temp = -1.0 * fd(H, plii, A1(PU, j));
+                    double temp2 = -1.0 * fd(H, pljj, A1(PU, j));
+                    temp = temp / temp2;
+                    A1(V, j) = A1(V, j) * temp;
+                }
+                for (int j = k + 1 + tid; j <= M; j += WG) {
+                    if (j == jj) continue;
+                    A1(Z, j) = A1(Z, j) * ((A1(F, j) - Fjj) / (A1(F, j) - Fii));
+                }
+                for (int j = 1 + tid; j <= k; j += WG) {
+                    if (j == ii) continue;
+                    A1(W, j) = A1(W, j) * ((A1(F, j) - Fii) / (A1(F, j) - Fjj));
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    double uj = A1(A, ii) * ((Fii - Fjj) / fd(H, plii, A1(PU, ii)));
+                    double wi = fd(H, pljj, A1(PU, ii)) / A1(A, jj);
+                    for (int j = 1; j <= k; j++) {
+                        if (j == ii) continue;
+                        double temp = fd(H, plii, A1(PU, j));
+                        uj = uj * ((Fii - A1(F, j)) / temp);
+                        temp = -1.0 * fd(H, pljj, A1(PU, j));
+                        wi = wi * (temp / (A1(F, j) - Fjj));
+                    }
+                    A1(Z, jj) = uj;
+                    A1(W, ii) = wi;
+                    int u = A1(PL, ii); A1(PL, ii) = A1(PL, jj); A1(PL, jj) = u;
+                    u = A1(PQ, ii); A1(PQ, ii) = A1(PQ, jj); A1(PQ, jj) = u;
+                    double t = A1(F, ii); A1(F, ii) = A1(F, jj); A1(F, jj) = t;
+                    t = A1(A, ii); A1(A, ii) = A1(A, jj); A1(A, jj) = t;
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    // ---- U generator (CauchylikeEigQ, then rows back to the node order)
+    if (tid == 0) {
+        for (int i = 1; i <= M; i++) IP[A1(PQ, i) - 1] = i;  // invp
+        int r = Rk;
+        if (r < mn) {
+            s_Rk = r;
+            s_off = arena_take(H, (long)M * (r > 0 ? r : 1));
+        } else {
+            s_Rk = M;
+            s_off = arena_take(H, (long)M * M);
+        }
+        H.usz[2 * (node - 1)] = M;
+        H.usz[2 * (node - 1) + 1] = s_Rk;
+        H.pu[node - 1] = s_off;
+    }
+    __syncthreads();
+    const long off = s_off;
+    if (off >= 0) {
+        double *Ug = H.arena + off;
+        const int r = s_Rk;
+        if (Rk < mn) {
+            for (int e = tid; e < M * r; e += WG) {
+                int i = e % M + 1, j = e / M + 1;
+                int src = IP[i - 1];  // row src of Q
+                double q;
+                if (src <= r) q = (src == j) ? 1.0 : 0.0;
+                else q = A1(Z, src) * A1(W, j) / (A1(F, src) - A1(F, j));
+                Ug[e] = q;
+            }
+        } else {
+            for (int e = tid; e < M * M; e += WG) {
+                int i = e % M + 1, j = e / M;
+                Ug[e] = (IP[i - 1] - 1 == j) ? 1.0 : 0.0;
+            }
+        }
+    }
+    // permuted row generators and their indices back to the compacted arrays
+    for (int i = tid; i < M; i += WG) {
+        A1(H.U, l0 + i) = A[i];
+        A1(H.PL, l0 + i) = PL[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column compression of one node: rrluCauchyEigcol + CauchylikeEigQ2
+// (ComprcauchyEig 'c').  RRLU rows: the node's M columns (roots, alpha, LDS);
+// RRLU columns: the N row generators outside the node (private global copies).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const HssNode *nodes, int nn,
+                                                    double tol)
+{
+    __shared__ double Dr[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
+    __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
+    __shared__ VI red[WG / 64];
+    __shared__ long s_off;
+    __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
+    const int tid = threadIdx.x;
+    if ((int)blockIdx.x >= nn) return;
+    HssNode Nd = nodes[blockIdx.x];
+    const HssMerge H = hm[Nd.mrg];
+    int *lr = H.lr, *lc = H.lc;
+    const int node = Nd.node;
+    if (tid == 0) {
+        int M = lc[2 * (node - 1) + 1] - lc[2 * (node - 1)] + 1;
+        int nir = lr[2 * (node - 1) + 1] - lr[2 * (node - 1)] + 1;
+        s_M = M;
+        s_N = *H.lda - nir;
+        s_flag = (*H.err != 0 || M > HSS_MAXM || M < 1 || nir < 1 || s_N < 0) ? 1 : 0;
+        s_off = s_flag ? -1 : scratch_take(H, 3L * (s_N > 0 ? s_N : 1));
+        if (s_flag) atomicOr(H.err, 1);
+    }
+    __syncthreads();
+    if (s_flag || s_off < 0) return;
+    const int M = s_M, N = s_N, c0 = lc[2 * (node - 1)], r0 = lr[2 * (node - 1)], r1e = lr[2 * (node - 1) + 1];
+    double *F = H.scratch + s_off, *V = F + (N > 0 ? N : 1);
+    int *PU = (int *)(V + (N > 0 ? N : 1));
+    for (int j = tid; j < N; j += WG) {
+        int src = (j < r0 - 1) ? j + 1 : j + 1 + (r1e - r0 + 1);
+        V[j] = A1(H.U, src);
+        PU[j] = A1(H.PL, src);
+        F[j] = A1(H.FF, PU[j]);
+    }
+    for (int i = tid; i < M; i += WG) {
+        Dr[i] = A1(H.D, c0 + i);
+        A[i] = A1(H.V, c0 + i);
+        PL[i] = A1(H.PU, c0 + i);
+        Z[i] = A[i];
+        PQ[i] = i + 1;
+        W[i] = 0.0;
+    }
+    __syncthreads();
+    const int mn = M < N ? M : N;
+    double amax = 0.0;
+    int Rk = mn;
+    const double rho = 1.5;
+    for (int k = 1; k <= mn; k++) {
+        // ---- CauchyPivtEigm
+        {
+            VI x = {0.0, 0};
+            int puk = A1(PU, k);
+            for (int i = k + tid; i <= M; i += WG) {
+                double v = fabs(A1(Z, i) / fd(H, puk, A1(PL, i)));
+                x = vi_better(x, VI{v, i - k + 1});
+            }
+            VI cl = block_argmax(x, red);
+            VI y = {0.0, 0};
+            int plk = A1(PL, k);
+            for (int j = k + tid; j <= N; j += WG) {
+                double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk));
+                y = vi_better(y, VI{v, j - k + 1});
+            }
+            VI rw = block_argmax(y, red);
+            double junkL = cl.v * fabs(A1(V, k)), junkU = rw.v * fabs(A1(Z, k));
+            double den;
+            if (A1(PL, k) < A1(PU, k))
+                den = (A1(H.FF, A1(PL, k) + 1) - A1(H.FF, A1(PU, k))) + A1(H.DIFR, A1(PL, k));
+            else
+                den = (A1(H.FF, A1(PL, k)) - A1(H.FF, A1(PU, k))) + A1(H.DIFL, A1(PL, k));
+            double piv = fabs(A1(Z, k) * A1(V, k) / den);
+            if (!(junkL <= piv && junkU <= piv)) {
+                int flg = (junkL > junkU) ? 1 : 0;
+                int jjL = cl.i, jjU = rw.i;
+                for (;;) {
+                    if (flg == 1) {
+                        int jj = jjL + k - 1;
+                        __syncthreads();
+                        if (tid == 0) {
+                            double t;
+                            t = A1(Dr, k); A1(Dr, k) = A1(Dr, jj); A1(Dr, jj) = t;
+                            t = A1(Z, k); A1(Z, k) = A1(Z, jj); A1(Z, jj) = t;
+                            t = A1(A, k); A1(A, k) = A1(A, jj); A1(A, jj) = t;
+                            int u = A1(PL, k); A1(PL, k) = A1(PL, jj); A1(PL, jj) = u;
+                            u = A1(PQ, k); A1(PQ, k) = A1(PQ, jj); A1(PQ, jj) = u;
+                        }
+                        __syncthreads();
+                        VI y2 = {0.0, 0};
+                        int plk2 = A1(PL, k);
+                        for (int j = k + tid; j <= N; j += WG) {
+                            double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk2));
+                            y2 = vi_better(y2, VI{v, j - k + 1});
+                        }
+                        jjU = block_argmax(y2, red).i;
+                        if (jjU == 1) break;
+                        flg = 0;
+                    }
+                    int jj = jjU + k - 1;
+                    __syncthreads();
+                    if (tid == 0) {
+                        double t;
+                        t = A1(F, k); A1(F, k) = A1(F, jj); A1(F, jj) = t;
+                        t = A1(V, k); A1(V, k) = A1(V, jj); A1(V, jj) = t;
+                        int u = A1(PU, k); A1(PU, k) = A1(PU, jj); A1(PU, jj) = u;
+                    }
+                    __syncthreads();
+                    VI x2 = {0.0, 0};
+                    int puk2 = A1(PU, k);
+                    for (int i = k + tid; i <= M; i += WG) {
+                        double v = fabs(A1(Z, i) / fd(H, puk2, A1(PL, i)));
+                        x2 = vi_better(x2, VI{v, i - k + 1});
+                    }
+                    jjL = block_argmax(x2, red).i;
+                    if (jjL == 1) break;
+                    flg = 1;
+                }
+            }
+        }
+        __syncthreads();
+        double ukk = A1(Z, k) * A1(V, k) / (-1.0 * fd(H, A1(PU, k), A1(PL, k)));
+        amax = fmax(amax, fabs(ukk));
+        if (fabs(ukk) < tol * amax) {
+            // ---- CauchyPivtEigm_CP
+            VI best = {0.0, 0};
+            int bestrow = 0;
+            for (int j = k + tid; j <= N; j += WG) {
+                int pu = A1(PU, j);
+                double mv = 0.0;
+                int mi = 0;
+                for (int i = k; i <= M; i++) {
+                    double v = fabs(A1(Z, i) / fd(H, pu, A1(PL, i)));
+                    if (mi == 0 || v > mv) { mv = v; mi = i - k + 1; }
+                }
+                VI c = {fabs(mv * A1(V, j)), j - k + 1};
+                VI nb = vi_better(best, c);
+                if (nb.i == c.i) bestrow = mi;
+                best = nb;
+            }
+            VI win = block_argmax(best, red);
+            if (win.i > 0 && best.i == win.i) { s_ii = bestrow; s_jj = win.i; }
+            __syncthreads();
+            if (tid == 0) {
+                int a = s_ii + k - 1, b = s_jj + k - 1;
+                double t;
+                t = A1(Dr, k); A1(Dr, k) = A1(Dr, a); A1(Dr, a) = t;
+                t = A1(Z, k); A1(Z, k) = A1(Z, a); A1(Z, a) = t;
+                t = A1(A, k); A1(A, k) = A1(A, a); A1(A, a) = t;
+                int u = A1(PL, k); A1(PL, k) = A1(PL, a); A1(PL, a) = u;
+                u = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = u;
+                t = A1(F, k); A1(F, k) = A1(F, b); A1(F, b) = t;
+                t = A1(V, k); A1(V, k) = A1(V, b); A1(V, b) = t;
+                u = A1(PU, k); A1(PU, k) = A1(PU, b); A1(PU, b) = u;
+            }
+            __syncthreads();
+            ukk = A1(Z, k) * A1(V, k) / (-1.0 * fd(H, A1(PU, k), A1(PL, k)));
+            amax = fmax(amax, fabs(ukk));
+            if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
+        }
+        const double Fk = A1(F, k);
+        const int plk = A1(PL, k), puk = A1(PU, k);
+        for (int j = k + 1 + tid; j <= M; j += WG) {
+            double temp = dd(H, A1(PL, j), plk);
+            double temp2 = -1.0 * fd(H, puk, A1(PL, j));
+            temp = temp / temp2;
+            A1(Z, j) = A1(Z, j) * temp;
+        }
+        for (int j = k + 1 + tid; j <= N; j += WG) {
+            double temp = fd(H, A1(PU, j), plk);
+            A1(V, j) = A1(V, j) * ((A1(F, j) - Fk) / temp);
+        }
+        for (int j = 1 + tid; j <= k - 1; j += WG) {
+            double temp = fd(H, puk, A1(PL, j));
+            double temp2 = dd(H, plk, A1(PL, j));
+            temp = temp / temp2;
+            A1(W, j) = A1(W, j) * temp;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double wk = (-1.0 * fd(H, puk, plk)) / A1(A, k);
+            for (int j = 1; j <= k - 1; j++) {
+                double temp = fd(H, A1(PU, j), plk);
+                double temp2 = dd(H, A1(PL, j), plk);
+                temp = temp / temp2;
+                wk = wk * temp;
+            }
+            A1(W, k) = wk;
+        }
+        __syncthreads();
+        if (k % 10 == 0 && M - k > 0) {
+            for (;;) {
+                VI x = {0.0, 0};
+                for (int jc = 1 + tid; jc <= k; jc += WG) {
+                    int pljc = A1(PL, jc);
+                    double mv = 0.0;
+                    int mi = 0;
+                    for (int i = 1; i <= M - k; i++) {
+                        double tt = dd(H, A1(PL, i + k), pljc);
+                        double v = fabs(A1(Z, k + i) / tt);
+                        if (mi == 0 || v > mv) { mv = v; mi = i; }
+                    }
+                    VI c = {mv * fabs(A1(W, jc)), jc};
+                    VI nb = vi_better(x, c);
+                    if (nb.i == jc) IP[jc - 1] = mi;
+                    x = nb;
+                }
+                VI win = block_argmax(x, red);
+                if (!(win.v > rho)) break;
+                const int ii = win.i, jj = IP[ii - 1] + k;
+                __syncthreads();
+                const int plii = A1(PL, ii), pljj = A1(PL, jj);
+                for (int j = k + 1 + tid; j <= N; j += WG) {
+                    double temp = fd(H, A1(PU, j), plii);
+                    double temp2 = fd(H, A1(PU, j), pljj);
+                    temp = temp / temp2;
+                    A1(V, j) = A1(V, j) * temp;
+                }
+                for (int j = k + 1 + tid; j <= M; j += WG) {
+                    if (j == jj) continue;
+                    double temp = dd(H, A1(PL, j), pljj);
+                    double temp2 = dd(H, A1(PL, j), plii);
+                    temp = temp / temp2;
+                    A1(Z, j) = A1(Z, j) * temp;
+                }
+                for (int j = 1 + tid; j <= k; j += WG) {
+                    if (j == ii) continue;
+                    double temp = dd(H, A1(PL, j), plii);
+                    double temp2 = dd(H, A1(PL, j), pljj);
+                    temp = temp / temp2;
+                    A1(W, j) = A1(W, j) * temp;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    double temp = dd(H, plii, pljj);
+                    double temp2 = -1.0 * fd(H, A1(PU, ii), plii);
+                    double uj = A1(A, ii) * (temp / temp2);
+                    double wi = (-1.0 * fd(H, A1(PU, ii), pljj)) / A1(A, jj);
+                    for (int j = 1; j <= k; j++) {
+                        if (j == ii) continue;
+                        temp = dd(H, plii, A1(PL, j));
+                        temp2 = -1.0 * fd(H, A1(PU, j), plii);
+                        uj = uj * (temp / temp2);
+                        temp = fd(H, A1(PU, j), pljj);
+                        temp2 = dd(H, A1(PL, j), pljj);
+                        wi = wi * (temp / temp2);
+                    }
+                    A1(Z, jj) = uj;
+                    A1(W, ii) = wi;
+                    int u = A1(PL, ii); A1(PL, ii) = A1(PL, jj); A1(PL, jj) = u;
+                    u = A1(PQ, ii); A1(PQ, ii) = A1(PQ, jj); A1(PQ, jj) = u;
+                    double t = A1(Dr, ii); A1(Dr, ii) = A1(Dr, jj); A1(Dr, jj) = t;
+                    t = A1(A, ii); A1(A, ii) = A1(A, jj); A1(A, jj) = t;
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    // ---- V generator (CauchylikeEigQ2, columns back to the node order)
+    if (tid == 0) {
+        for (int i = 1; i <= M; i++) IP[A1(PQ, i) - 1] = i;
+        if (Rk < mn) {
+            s_Rk = Rk;
+            s_off = arena_take(H, (long)(Rk > 0 ? Rk : 1) * M);
+        } else {
+            s_Rk = M;
+            s_off = arena_take(H, (long)M * M);
+        }
+        H.vsz[2 * (node - 1)] = s_Rk;
+        H.vsz[2 * (node - 1) + 1] = M;
+        H.pv[node - 1] = s_off;
+    }
+    __syncthreads();
+    const long off = s_off;
+    if (off >= 0) {
+        double *Vg = H.arena + off;
+        const int r = s_Rk;
+        if (Rk < mn) {
+            for (int e = tid; e < r * M; e += WG) {
+                int i = e % r + 1, j = e / r + 1;
+                int src = IP[j - 1];  // column src of Q
+                double q;
+                if (src <= r) {
+                    q = (src == i) ? 1.0 : 0.0;
+                } else {
+                    int pdjj = A1(PL, src), pdii = A1(PL, i);
+                    double den;
+                    if (pdii < pdjj) den = (A1(H.FF, pdii + 1) - A1(H.FF, pdjj)) + (A1(H.DIFR, pdii) - A1(H.DIFL, pdjj));
+                    else den = (A1(H.FF, pdii) - A1(H.FF, pdjj + 1)) + (A1(H.DIFL, pdii) - A1(H.DIFR, pdjj));
+                    q = -1.0 * A1(W, i) * A1(Z, src) / den;
+                }
+                Vg[e] = q;
+            }
+        } else {
+            for (int e = tid; e < M * M; e += WG) {
+                int i = e % M, j = e / M + 1;
+                Vg[e] = (IP[j - 1] - 1 == i) ? 1.0 : 0.0;
+            }
+        }
+    }
+    for (int i = tid; i < M; i += WG) {
+        A1(H.D, c0 + i) = Dr[i];
+        A1(H.V, c0 + i) = A[i];
+        A1(H.PU, c0 + i) = PL[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compaction after a level (one block per merge): keep the first rank entries of
+// every node of the level, then the full ranges of shallower leaves, in order.
+// rows: U/PL with usz(2,:); cols: V/D/PU with vsz(1,:).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_hss_compact(const HssMerge *hm, const HssCompact *cp, const int *order,
+                                                    int rows)
+{
+    __shared__ int s_start[1024 + 1];
+    HssCompact C = cp[blockIdx.x];
+    const HssMerge H = hm[C.mrg];
+    const int tid = threadIdx.x;
+    int *rng = rows ? H.lr : H.lc;
+    if (*H.err) return;
+    if (tid == 0) {
+        int lt = 0;
+        for (int q = 0; q < C.count; q++) {
+            int nd = order[C.start + q];
+            int cnt;
+            if (q < C.nlevel) cnt = rows ? H.usz[2 * (nd - 1) + 1] : H.vsz[2 * (nd - 1)];
+            else cnt = rng[2 * (nd - 1) + 1] - rng[2 * (nd - 1)] + 1;
+            int have = rng[2 * (nd - 1) + 1] - rng[2 * (nd - 1)] + 1;
+            if (cnt < 0 || cnt > have) cnt = 0;
+            s_start[q] = lt;
+            lt += cnt;
+        }
+        s_start[C.count] = lt;
+    }
+    __syncthreads();
+    double *Td = H.T, *Td2 = H.T + H.K;
+    int *Ti = (int *)(H.T + 2 * H.K);
+    for (int q = 0; q < C.count; q++) {
+        int nd = order[C.start + q];
+        int src0 = rng[2 * (nd - 1)];  // 1-based
+        int cnt = s_start[q + 1] - s_start[q];
+        for (int t = tid; t < cnt; t += WG) {
+            int s = src0 + t, d = s_start[q] + t;
+            if (rows) { Td[d] = A1(H.U, s); Ti[d] = A1(H.PL, s); }
+            else { Td[d] = A1(H.V, s); Td2[d] = A1(H.D, s); Ti[d] = A1(H.PU, s); }
+        }
+    }
+    __syncthreads();
+    const int tot = s_start[C.count];
+    for (int d = tid; d < tot; d += WG) {
+        if (rows) { H.U[d] = Td[d]; H.PL[d] = Ti[d]; }
+        else { H.V[d] = Td[d]; H.D[d] = Td2[d]; H.PU[d] = Ti[d]; }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int q = 0; q < C.count; q++) {
+            int nd = order[C.start + q];
+            rng[2 * (nd - 1)] = s_start[q] + 1;
+            rng[2 * (nd - 1) + 1] = s_start[q + 1];
+        }
+        if (rows) *H.lda = tot; else *H.lca = tot;
+    }
+}
+
+// B generators of the nodes of one level: B_i = C(row skeleton of i, column
+// skeleton of its sibling) (CauchyHssEig.f90:2229-2244).
+__global__ __launch_bounds__(WG) void k_hss_bgen(const HssMerge *hm, const HssNode *nodes, int nn)
+{
+    __shared__ long s_off;
+    if ((int)blockIdx.x >= nn) return;
+    HssNode Nd = nodes[blockIdx.x];
+    const HssMerge H = hm[Nd.mrg];
+    const int node = Nd.node, sib = Nd.sib;
+    const int r0 = H.lr[2 * (node - 1)], mi = H.lr[2 * (node - 1) + 1] - r0 + 1;
+    const int c0 = H.lc[2 * (sib - 1)], nj = H.lc[2 * (sib - 1) + 1] - c0 + 1;
+    if (*H.err || mi < 0 || nj < 0) return;
+    if (threadIdx.x == 0) {
+        s_off = arena_take(H, (long)mi * nj);
+        H.bsz[2 * (node - 1)] = mi;
+        H.bsz[2 * (node - 1) + 1] = nj;
+        H.pb[node - 1] = s_off;
+    }
+    __syncthreads();
+    if (s_off < 0) return;
+    double *B = H.arena + s_off;
+    for (int e = threadIdx.x; e < mi * nj; e += WG) {
+        int i = e % mi, j = e / mi;
+        int pf = A1(H.PL, r0 + i), pw = A1(H.PU, c0 + j);
+        double fj = A1(H.FF, pw), vj = A1(H.V, c0 + j);
+        double den = (pf <= pw) ? (A1(H.FF, pf) - fj) - A1(H.DIFL, pw)
+                                : (A1(H.FF, pf) - A1(H.FF, pw + 1)) - A1(H.DIFR, pw);
+        B[e] = A1(H.U, r0 + i) * vj / den;
+    }
+}
+
+// initial compacted arrays: D = roots, U = z, V = alpha, PL = PU = identity
+__global__ __launch_bounds__(WG) void k_hss_init(const HssMerge *hm)
+{
+    const HssMerge H = hm[blockIdx.y];
+    for (int i = blockIdx.x * WG + threadIdx.x; i < H.K; i += gridDim.x * WG) {
+        H.D[i] = H.D0[i]; H.U[i] = H.U0[i]; H.V[i] = H.V0[i];
+        H.PL[i] = i + 1; H.PU[i] = i + 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *H.lda = H.K; *H.lca = H.K; }
+}
+
+// ---------------------------------------------------------------------------
+// Batched GEMM of the HSS multiply: work item (blockIdx.y) = one BG, block
+// (blockIdx.x, blockIdx.z) = one 64 x 64 tile of its output.  FP64 MFMA
+// 16x16x4, 4 waves of 32 x 32.
+// ---------------------------------------------------------------------------
+typedef double d4h __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
+{
+    __shared__ double As[16][64 + 1];
+    __shared__ double Bs[16][64 + 1];
+    const BG g = items[blockIdx.y];
+    const int r0 = blockIdx.x * 64, c0 = blockIdx.z * 64;
+    if (r0 >= g.nrow || c0 >= g.n) return;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+    d4h acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = (d4h){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < g.k; k0 += 16) {
+        for (int e = 0; e < 4; e++) {
+            int idx = tid + WG * e;
+            int r = idx & 63, kk = idx >> 6;
+            int t = k0 + kk;
+            As[kk][r] = (t < g.k && r0 + r < g.nrow) ? g.A[(long)(r0 + r) + (long)t * g.lda] : 0.0;
+            Bs[kk][r] = (t < g.k && c0 + r < g.n)
+                            ? (g.btrans ? g.B[(long)t * g.ldb + (c0 + r)] : g.B[(long)t + (long)(c0 + r) * g.ldb])
+                            : 0.0;
+        }
+        __syncthreads();
+        for (int kq = 0; kq < 16; kq += 4) {
+            int kl = kq + (lane >> 4);
+            double a0 = As[kl][wr + (lane & 15)], a1 = As[kl][wr + 16 + (lane & 15)];
+            double b0 = Bs[kl][wc + (lane & 15)], b1 = Bs[kl][wc + 16 + (lane & 15)];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int i = 0; i < 4; i++) {
+                int row = r0 + wr + a * 16 + (lane >> 4) + 4 * i;
+                int col = c0 + wc + b * 16 + (lane & 15);
+                if (row < g.nrow && col < g.n) {
+                    double *cp = g.C + (long)row + (long)col * g.ldc;
+                    *cp = g.beta ? (*cp + acc[a][b][i]) : acc[a][b][i];
+                }
+            }
+}
+
+// X(:, t) = Q2(:, t): the K surviving old eigenvectors of each HSS merge packed
+// densely (mdlaed2.f90:467-499), zero outside each column's support.
+__global__ __launch_bounds__(WG) void k_gatherx(const Merge *mg, const int *hidx, const int *col_start, int nh,
+                                                const Prob *probs, Work W, int cur, double *X, const long *xoff)
+{
+    int h = 0;
+    {
+        int lo = 0, hi = nh - 1, b = blockIdx.x;
+        while (lo < hi) { int md = (lo + hi + 1) >> 1; if (col_start[md] <= b) lo = md; else hi = md - 1; }
+        h = lo;
+    }
+    const Merge M = mg[hidx[h]];
+    const int t = blockIdx.x - col_start[h];
+    const Prob P = probs[M.prob];
+    const int slot = W.vm[cur][M.s + W.indx[M.s + t] - 1];
+    const int lo = W.supp_lo[slot], hi = W.supp_hi[slot];
+    const double *src = P.q + (long)(slot - P.off) * P.ld - P.off;
+    double *dst = X + xoff[h] + (long)t * M.n - M.s;
+    for (int r = M.s + threadIdx.x; r < M.s + M.n; r += WG) dst[r] = (r >= lo && r < hi) ? src[r] : 0.0;
+}
+
+// dense C(i,j) = u_i v_j / (f_i - lambda_j) with accurate differences
+// (CauchylikeEig with identity permutations), used when a merge's HSS form cannot
+// be built within the LDS / arena limits
+__global__ __launch_bounds__(WG) void k_cbuild(const HssMerge *hm, int h, double *C)
+{
+    const HssMerge H = hm[h];
+    long KK = (long)H.K * H.K;
+    for (long e = (long)blockIdx.x * WG + threadIdx.x; e < KK; e += (long)gridDim.x * WG) {
+        int i = (int)(e % H.K) + 1, j = (int)(e / H.K) + 1;
+        double den = (i <= j) ? (A1(H.FF, i) - A1(H.FF, j)) - A1(H.DIFL, j)
+                              : (A1(H.FF, i) - A1(H.FF, j + 1)) - A1(H.DIFR, j);
+        C[e] = A1(H.U0, i) * A1(H.V0, j) / den;
+    }
+}
+
+__global__ void k_hss_reset(const HssMerge *hm)
+{
+    if (threadIdx.x == 0) *hm[blockIdx.x].scratch_top = 0ULL;
+}

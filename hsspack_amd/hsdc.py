@@ -67,6 +67,12 @@ def lib():
                                    _P(_d), _P(_d), _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _P(_i),
                                    _P(_i), _P(_d), _P(_d)]
         L.hsdc_secular.argtypes = [_vp, _i, _P(_d), _P(_d), _d, _P(_d), _P(_i), _P(_d), _P(_i)]
+        L.hsdc_laed3.argtypes = [_vp, _i, _i, _i, _P(_d), _P(_d), _i, _d, _P(_d), _P(_d), _P(_i),
+                                 _P(_i), _P(_d)]
+        L.hsdc_loewner.argtypes = [_vp, _i, _d, _P(_d), _P(_d), _P(_d), _P(_d), _P(_d), _P(_d)]
+        L.hsdc_hss_evc.argtypes = [_vp, _i, _i, _d, _P(_d), _P(_d), _P(_d), _P(_d), _P(_d), _P(_d),
+                                   _P(_d), _i, _i, _P(_i)]
+        L.hsdc_hss_dense.argtypes = [_vp, _P(_d), _i]
         _LIB = L
     return _LIB
 
@@ -188,3 +194,45 @@ class Context:
         info = lib().hsdc_secular(self._h, k, _dp(dl), _dp(ww), rho, _dp(lam), _ip(org), _dp(tau),
                                   _ip(nit))
         return dict(lam=lam, org=org, tau=tau, niter=nit, info=info)
+
+    def laed3(self, k, n, n1, rho, dlamda, q2, indx, ctot, w):
+        """LAPACK DLAED3 semantics on the device: returns (d, q, w_updated, info)."""
+        d = np.zeros(n)
+        q = np.zeros((n, max(k, 1)), order="F")
+        dl = np.array(dlamda, np.float64)
+        q2 = np.ascontiguousarray(q2, np.float64)
+        ix = np.ascontiguousarray(indx, np.int32)
+        ct = np.ascontiguousarray(ctot[:4], np.int32)
+        ww = np.array(w, np.float64)
+        info = lib().hsdc_laed3(self._h, k, n, n1, _dp(d), _dp(q), n, rho, _dp(dl), _dp(q2), _ip(ix),
+                                _ip(ct), _dp(ww))
+        return d, q, ww, info
+
+    def loewner(self, dlamda, z, rho):
+        k = len(dlamda)
+        dl = np.array(dlamda, np.float64)
+        zz = np.array(z, np.float64)
+        d, difl, difr, alpha = np.zeros(k), np.zeros(k), np.zeros(k), np.zeros(k)
+        info = lib().hsdc_loewner(self._h, k, rho, _dp(dl), _dp(zz), _dp(d), _dp(difl), _dp(difr),
+                                  _dp(alpha))
+        return dict(d=d, z=zz, difl=difl, difr=difr, alpha=alpha, info=info)
+
+    def hss_evc(self, f, d, u, v, difl, difr, q, ni, tol):
+        """q (nrow x k) := q * H; returns (q, ranks per node [U cols, V rows])."""
+        k = len(f)
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (f, d, u, v, difl, difr)]
+        q = np.array(q, np.float64, order="F")
+        nrow = q.shape[0]
+        ranks = np.zeros(2 * (2 * (k // max(ni, 1)) + 4), np.int32)
+        info = lib().hsdc_hss_evc(self._h, k, ni, tol, *[_dp(a) for a in arrs], _dp(q), max(nrow, 1),
+                                  nrow, _ip(ranks))
+        if info != 0:
+            raise RuntimeError("hsdc_hss_evc: %d" % info)
+        return q, ranks
+
+    def hss_dense(self, k):
+        a = np.zeros((k, k), order="F")
+        info = lib().hsdc_hss_dense(self._h, _dp(a), k)
+        if info != 0:
+            raise RuntimeError("hsdc_hss_dense: %d" % info)
+        return a

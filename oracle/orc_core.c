@@ -673,6 +673,18 @@ int orc_mdlaed3(int k, int n, double *d, double *q, int ldq, double rho, double 
         }
         orc_hss_mmL(&H, q2, n, n);
         orc_hss_free(&H);
+    } else {
+        /* a tree of one leaf has no off-diagonal blocks: fastHssmmL_omp would
+         * return X unchanged (ConstructHssd.f90:1455-1458); apply the dense
+         * Cauchy-like matrix instead */
+        double *cmat = (double *)malloc(sizeof(double) * (size_t)k * k);
+        double *x = (double *)malloc(sizeof(double) * (size_t)n * k);
+        orc_cauchy_dense(k, dlamda, z, alpha, difl, difr, cmat, k);
+        memcpy(x, q2, sizeof(double) * (size_t)n * k);
+        gemm_nn(n, k, k, x, n, cmat, k, q2, n);
+        free(cmat);
+        free(x);
+        info = 0;
     }
     for (j = 0; j < k; j++) memcpy(q + (long)j * ldq, q2 + (long)j * n, sizeof(double) * n);
     return info;

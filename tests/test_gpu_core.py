@@ -193,3 +193,21 @@ def test_host_api_and_info(ctx):
     lam2, z2, info = ctx.stedc(d, e, compz="V", z=q0)
     assert info == 0
     np.testing.assert_allclose(np.abs(z2), np.abs(q0 @ z), atol=1e-12)
+
+
+@pytest.mark.parametrize("cfg,n,ni", [(1, 700, 32), (2, 700, 32), (1, 1100, 64), (1, 2100, 128)])
+def test_stedc_hss_branch_vs_oracle(torch_dev, ctx, cfg, n, ni):
+    d, e = tridiag_cfg(cfg, n, 40 + n)
+    rc, lam, Z = solve_dev(torch_dev, ctx, d, e, hss_min_k=128, hss_leaf=ni, hss_tol=1e-13)
+    st = ctx.stats()
+    ctx.set_params(hss_min_k=1 << 30, hss_leaf=200)
+    assert rc == 0
+    ref = O.stedc(d, e, prm=O.params(bsmlz=128, ni=ni, tol=1e-13), trace_cap=4096)
+    w, _, _ = L.dstedc(d, e)
+    nrm = np.abs(tri(d, e)).max()
+    assert np.abs(lam - w).max() <= 4 * n * EPS * nrm
+    assert np.abs(lam - ref["d"]).max() <= 4 * n * EPS * nrm
+    check_eig(d, e, lam, Z)
+    if cfg == 1:
+        assert st["hss_merges"] > 0
+        assert st["rank_max"] > 0

@@ -1,0 +1,84 @@
+"""GPU parity of the HSS branch (secular/Loewner/alpha of MDLAED3, the structured
+RRLU construction of DHSSEVC, the HSS multiply) and of the dense DLAED3 branch."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from hsspack_amd import hsdc
+    c = hsdc.Context()
+    yield c
+    c.close()
+
+
+def _secular_case(K, seed):
+    rng = np.random.default_rng(seed)
+    d = np.sort(rng.uniform(-1, 1, K))
+    z = rng.standard_normal(K)
+    z /= np.linalg.norm(z)
+    return d, z, 1.3
+
+
+@pytest.mark.parametrize("K", [3, 40, 700])
+def test_loewner_bitexact(ctx, K):
+    d, z, rho = _secular_case(K, K)
+    g = ctx.loewner(d, z, rho)
+    r = O.mdlaed3_vectors(d, z, rho)
+    assert g["info"] == 0
+    for key in ("d", "z", "difl", "alpha"):
+        assert np.array_equal(g[key], r[key]), key
+    assert np.array_equal(g["difr"][:-1], r["difr"][:-1])
+
+
+@pytest.mark.parametrize("K,ni,tol", [(600, 64, 1e-13), (1000, 128, 1e-13), (1300, 200, 1e-17),
+                                      (2048, 128, 1e-13), (900, 64, 1e-8)])
+def test_hss_ranks_and_product_vs_oracle(ctx, K, ni, tol):
+    d, z, rho = _secular_case(K, K + 1)
+    v = O.mdlaed3_vectors(d, z, rho)
+    H = O.hss_build(v["dlamda"], v["d"], v["z"], v["alpha"], v["difl"], v["difr"], ni, tol)
+    try:
+        usz, vsz = O.hss_ranks(H)
+        Hd_ref = O.hss_dense(H)
+    finally:
+        O.hss_free(H)
+    x = np.random.default_rng(2).standard_normal((131, K))
+    y, ranks = ctx.hss_evc(v["dlamda"], v["d"], v["z"], v["alpha"], v["difl"], v["difr"], x, ni, tol)
+    ltr = usz.shape[0]
+    # identical pivoting: every node's U and V rank equals the oracle's
+    assert ranks[0] != -1
+    assert np.array_equal(ranks[: 2 * ltr:2][: ltr - 1], usz[: ltr - 1, 1])
+    assert np.array_equal(ranks[1: 2 * ltr:2][: ltr - 1], vsz[: ltr - 1, 0])
+    C = O.cauchy_dense(v["dlamda"], v["z"], v["alpha"], v["difl"], v["difr"])
+    Hd = ctx.hss_dense(K)
+    scale = np.abs(C).max()
+    assert np.abs(Hd - Hd_ref).max() <= 64 * K ** 0.5 * EPS * scale
+    assert np.abs(Hd - C).max() <= max(20 * tol, 64 * EPS) * K ** 0.5 * scale
+    assert np.abs(y - x @ Hd_ref).max() <= 1e-12 * np.abs(x).max() * K ** 0.5
+
+
+def test_laed3_dense_branch_vs_oracle(ctx, golden):
+    for i in range(int(golden["laed2_count"][0])):
+        if "laed3_%d_d" % i not in golden.files:
+            continue
+        ref = {kk[len("laed2_%d_out_" % i):]: golden[kk] for kk in golden.files
+               if kk.startswith("laed2_%d_out_" % i)}
+        n = len(ref["d"])
+        n1 = int(golden["laed2_%d_in_n1" % i][0])
+        k = int(ref["k"][0])
+        rho = float(ref["rho"][0])
+        d_o, q_o, info_o = O.laed3(k, n, n1, ref["q"], rho, ref["dlamda"], ref["q2"], ref["indxc"],
+                                   ref["coltyp"][:4], ref["w"])
+        d_g, q_g, w_g, info = ctx.laed3(k, n, n1, rho, ref["dlamda"], ref["q2"], ref["indxc"],
+                                        ref["coltyp"][:4], ref["w"])
+        assert info == 0 and info_o == 0
+        assert np.array_equal(d_g[:k], d_o[:k])   # secular roots bit-exact
+        np.testing.assert_allclose(q_g[:, :k], q_o[:, :k], rtol=0, atol=1e-14)

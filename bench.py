@@ -1,15 +1,19 @@
 """Benchmark: seconds to all eigenpairs of an n = 32768 FP64 symmetric tridiagonal
 (d, e ~ U(-1,1), seed 2; HSS leaf 128, tolerance 1e-13, HSS branch for K >= 1024)
-with the batched divide-and-conquer path on one MI355X, device resident.
+with the batched divide-and-conquer path and the learned secular start on one
+MI355X, device resident.
 
 One step = one complete solve (eigenvalues and the n x n eigenvector matrix) with
 the input already in HBM.  Prints one JSON line.  Also reported in the line: the
-merge-at-a-time ("plain") schedule on the same input, a roofline entry for the
-final reordering kernel (event-timed inside the timed steps), orthogonality and
-residual of the result, and the CPU baseline (the oracle's restatement of the
-reference path, single thread, timed on this host on a bounded sample).
+merge-at-a-time ("plain") schedule with the classical start on the same input, the
+batched schedule with the classical start, secular iteration counts of each, a
+roofline entry for the final reordering kernel (event-timed inside the timed
+steps), orthogonality and residual of the result, and the CPU baseline (the
+oracle's restatement of the reference path, single thread, timed on this host on a
+bounded sample).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cfg 1 --seed 4]
+                       [--no-cpu] [--no-plain] [--no-check]
 """
 import argparse
 import ctypes
@@ -22,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+GEN = {1: "d=2+U(-1e-3,1e-3), e=1+U(-1e-3,1e-3)", 2: "d,e~U(-1,1)"}
 
 
 def log(*a):
@@ -101,8 +106,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the solver runs on a single MI355X and has no "
+              "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -118,7 +123,13 @@ def main():
     D = torch.empty_like(D0)
     E = torch.empty_like(E0)
     Z = torch.empty((n, n), dtype=torch.float64, device=dev)
-    ctx = hsdc.Context(hss_min_k=hss[0], hss_leaf=hss[1], hss_tol=hss[2], batched=1, profile=1)
+    ctx = hsdc.Context(hss_min_k=hss[0], hss_leaf=hss[1], hss_tol=hss[2], batched=1, profile=1, learned=1)
+    ctx.mlp_load(os.path.join(ROOT, "hsspack_amd", "data", "secular_mlp.bin"))
+
+    def secstats(st):
+        return {"iters_per_root": st["secular_iters"] / max(st["roots"], 1),
+                "wave_iters": st["secular_wave_iters"], "critical_path_iters": st["secular_crit_iters"],
+                "learned_roots": st["learned_roots"]}
 
     def step():
         D.copy_(D0)
@@ -143,6 +154,19 @@ def main():
     st = ctx.stats()
     log("batched: %.2f ms/solve, stats %s" % (ms_step, st))
 
+    sec_learned = secstats(st)
+    # ablation: batched schedule, classical start
+    ctx.set_params(learned=0)
+    step()
+    torch.cuda.synchronize()
+    tc0 = time.perf_counter()
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    classical_s = (time.perf_counter() - tc0) / 3
+    sec_classical = secstats(ctx.stats())
+    log("batched, classical start: %.2f ms/solve, %s" % (classical_s * 1000, sec_classical))
+
     plain_s = None
     if not args.no_plain:
         ctx.set_params(batched=0)
@@ -152,10 +176,10 @@ def main():
         step()
         torch.cuda.synchronize()
         plain_s = time.perf_counter() - tp0
-        ctx.set_params(batched=1)
-        log("plain (merge at a time): %.2f ms/solve" % (plain_s * 1000))
-        step()
-        torch.cuda.synchronize()
+        log("plain (merge at a time, classical start): %.2f ms/solve" % (plain_s * 1000))
+    ctx.set_params(batched=1, learned=1)
+    step()
+    torch.cuda.synchronize()
 
     orth = res = None
     if not args.no_check:
@@ -177,12 +201,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "tridiagonal DC eigensolve, n=%d, d,e~U(-1,1) seed %d, HSS leaf 128, "
+        "config": {"workload": "tridiagonal DC eigensolve, n=%d, %s seed %d, HSS leaf 128, "
                                "tol 1e-13, HSS for K>=1024, all eigenpairs, device resident"
-                               % (n, args.seed),
-                   "n": n, "schedule": "batched (one launch per DC level)",
+                               % (n, GEN[args.cfg], args.seed),
+                   "n": n, "schedule": "batched (one launch per DC level) + learned secular start",
                    "plain_s": plain_s,
                    "speedup_vs_plain": (plain_s / (ms_step / 1000.0)) if plain_s else None,
+                   "batched_classical_s": classical_s,
+                   "secular_learned": sec_learned, "secular_classical": sec_classical,
                    "orthogonality": orth, "residual": res,
                    "merges": st["merges"], "levels": st["levels"], "k_top": st["k_top"],
                    "roots": st["roots"], "deflated": st["deflated"],

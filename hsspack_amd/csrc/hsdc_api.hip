@@ -6,6 +6,8 @@
 // the per-merge schedule (params.batched = 0) launches them one at a time.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
+#include <cmath>
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
@@ -14,6 +16,7 @@
 #include "hsdc_internal.h"
 #include "hsdc_hss.h"
 #include "hsdc_hss_host.h"
+#include "hsdc_mlp.h"
 #include "../../include/hsdc.h"
 
 // kernels (hsdc_kern.hip)
@@ -24,7 +27,8 @@ __global__ void k_sortmerge(const Merge *, Work);
 __global__ void k_deflate(const Merge *, Work, int, MergeOut *, int);
 __global__ void k_rot_prep(const Merge *, const Prob *, Work, int, const MergeOut *);
 __global__ void k_rotate(const Merge *, const int *, int, const Prob *, Work, int, const MergeOut *);
-__global__ void k_secular(const Merge *, const int *, int, const MergeOut *, Work, int, int *);
+__global__ void k_secular(const Merge *, const int *, int, const MergeOut *, Work, int, int *, const SecCtl *,
+                          unsigned int *);
 __global__ void k_loewner(const Merge *, const int *, int, const MergeOut *, Work);
 __global__ void k_sbuild(const Merge *, const int *, int, const MergeOut *, const long *, Work, double *);
 __global__ void k_alpha(const Merge *, const int *, int, const MergeOut *, Work);
@@ -48,6 +52,10 @@ __global__ void k_bgemm(const BG *);
 __global__ void k_gatherx(const Merge *, const int *, const int *, int, const Prob *, Work, int, double *,
                           const long *);
 __global__ void k_cbuild(const HssMerge *, int, double *);
+// learned-start training (hsdc_mlp.hip)
+__global__ void k_mlp_grad(const float *, const float *, const float *, const int *, int, float, float *, float *);
+__global__ void k_mlp_sum(const float *, int, float *, const float *, float *);
+__global__ void k_mlp_adam(float *, const float *, float *, float *, float, float, float);
 
 #define WGS 256
 
@@ -113,7 +121,17 @@ struct hsdc_ctx {
     PinBuf pin, pin2;
     Work W;
     size_t wcap = 0;
+    // learned secular start: weights (device + host copy), harvest buffer
+    // (count | features | targets), secular control block of the running solve
+    DevBuf mlpw, hv, sec, mlptr;
+    std::vector<float> hw;
+    long long hv_cap = 0;
+    const SecCtl *dsec = nullptr;
+    unsigned int *dlvlmax = nullptr;  // largest iteration count of each level's secular launch
+    int cur_level = 0;
 };
+
+#define SEC_LEVELS 128
 
 // ----------------------------------------------------------------------------
 // host plan of one solve
@@ -214,6 +232,57 @@ static int alloc_work(hsdc_ctx *c, int ntot)
     W.indxp = I + 8 * n; W.coltyp = I + 9 * n; W.rot_pj = I + 10 * n; W.rot_nj = I + 11 * n;
     W.org = I + 12 * n; W.nit = I + 13 * n; W.nsl = I + 14 * n; W.evq = I + 15 * n;
     c->wcap = ntot;
+    return 0;
+}
+
+// Secular control block of the next solve: learned-start weights when enabled and
+// loaded, the harvest buffer when harvesting, and zeroed iteration counters.
+static int sec_setup(hsdc_ctx *c)
+{
+    size_t hdr = al64(sizeof(SecCtl));
+    CK(ensure(c->sec, hdr + 64 + sizeof(unsigned int) * SEC_LEVELS));
+    char *base = (char *)c->sec.p;
+    struct { SecCtl C; } h;
+    memset(&h, 0, sizeof(h));
+    h.C.w = (c->prm.learned && !c->hw.empty()) ? (const float *)c->mlpw.p : nullptr;
+    h.C.iters = (unsigned long long *)(base + hdr);
+    h.C.used = (unsigned long long *)(base + hdr + 8);
+    h.C.wave_iters = (unsigned long long *)(base + hdr + 16);
+    if (c->hv_cap > 0) {
+        char *hv = (char *)c->hv.p;
+        h.C.count = (unsigned long long *)hv;
+        h.C.hf = (float *)(hv + 64);
+        h.C.ht = (float *)(hv + 64 + al64(sizeof(float) * MLP_IN * (size_t)c->hv_cap));
+        h.C.cap = (unsigned long long)c->hv_cap;
+    }
+    h.C.kmin = c->prm.learned_min_k;
+    h.C.gate = (float)c->prm.learned_gate;
+    unsigned char zero[64 + sizeof(unsigned int) * SEC_LEVELS];
+    memset(zero, 0, sizeof(zero));
+    HK(hipMemcpyAsync(base, &h.C, sizeof(SecCtl), hipMemcpyHostToDevice, c->stream));
+    HK(hipMemcpyAsync(base + hdr, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+    HK(hipStreamSynchronize(c->stream));
+    c->dsec = (const SecCtl *)base;
+    c->dlvlmax = (unsigned int *)(base + hdr + 64);
+    c->cur_level = 0;
+    return 0;
+}
+
+// iteration counters of the last solve into the statistics (stream synchronised)
+static int sec_collect(hsdc_ctx *c)
+{
+    unsigned long long v[3] = {0, 0, 0};
+    unsigned int lm[SEC_LEVELS];
+    HK(hipMemcpyAsync(v, (char *)c->sec.p + al64(sizeof(SecCtl)), sizeof(v), hipMemcpyDeviceToHost,
+                      c->stream));
+    HK(hipMemcpyAsync(lm, c->dlvlmax, sizeof(lm), hipMemcpyDeviceToHost, c->stream));
+    HK(hipStreamSynchronize(c->stream));
+    long long crit = 0;
+    for (int i = 0; i < SEC_LEVELS; i++) crit += lm[i];
+    c->st.secular_crit_iters = crit;
+    c->st.secular_iters = (long long)v[0];
+    c->st.learned_roots = (long long)v[1];
+    c->st.secular_wave_iters = (long long)v[2];
     return 0;
 }
 
@@ -722,7 +791,7 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
     }
     if (anyk) {
         hipLaunchKernelGGL(k_secular, dim3(blockK[nm]), dim3(64), 0, s, dmg, dblockK, nm, dmo, W,
-                           c->prm.maxit, dinfo);
+                           c->prm.maxit, dinfo, c->dsec, c->dlvlmax + std::min(c->cur_level, SEC_LEVELS - 1));
         hipLaunchKernelGGL(k_loewner, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, W);
         hipLaunchKernelGGL(k_sbuild, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, dsoff, W,
                            (double *)c->sb.p);
@@ -837,6 +906,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     int *dinfo = (int *)(dp + a10);
     Work W = c->W;
     int cur = 0;
+    CK(sec_setup(c));
     hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(64), 0, s, dleaves, (int)nl, dprobs, dd, de, W, cur);
     HK(hipGetLastError());
     // the host tables above live in c->pin, which run_merges reuses: wait for the upload
@@ -846,6 +916,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         const std::vector<Merge> &hm = pl.lv[L];
         const Merge *dm = dmerges + lvoff[L];
         const int nm = (int)hm.size();
+        c->cur_level = (int)L;
         if (c->prm.batched) {
             int totN = chunks[lvchunk[L] + nm];
             CK(run_merges(c, dm, hm, 0, nm, dchunks + lvchunk[L], totN, dprobs, cur, dinfo));
@@ -895,6 +966,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    CK(sec_collect(c));
     return hinfo ? 1 : 0;
 }
 
@@ -943,6 +1015,8 @@ void hsdc_default_params(hsdc_params *p)
     p->batched = 1;
     p->learned = 0;
     p->profile = 0;
+    p->learned_min_k = 256;
+    p->learned_gate = 0.1;
 }
 
 // The context touches no device until the first solve, so argument checking and
@@ -960,8 +1034,9 @@ int hsdc_ctx_create(hsdc_ctx **ctx)
 int hsdc_ctx_destroy(hsdc_ctx *c)
 {
     if (!c) return 0;
-    DevBuf *bufs[] = {&c->q, &c->ct, &c->sb, &c->din, &c->ein, &c->wsd, &c->wsi, &c->tabs, &c->lvl,
-                      &c->mo, &c->probs, &c->misc, &c->zout, &c->dout};
+    DevBuf *bufs[] = {&c->q,  &c->ct,  &c->sb,   &c->din, &c->ein,  &c->wsd,  &c->wsi, &c->tabs,
+                      &c->lvl, &c->mo, &c->probs, &c->misc, &c->zout, &c->dout, &c->hss, &c->htab,
+                      &c->xb, &c->gf,  &c->cb,   &c->yb,  &c->mlpw, &c->hv,   &c->sec, &c->mlptr};
     for (DevBuf *b : bufs) if (b->p) hipFree(b->p);
     if (c->pin.p) hipHostFree(c->pin.p);
     if (c->pin2.p) hipHostFree(c->pin2.p);
@@ -974,6 +1049,7 @@ int hsdc_set_params(hsdc_ctx *c, const hsdc_params *p)
     if (!c || !p) return -1;
     if (p->smlsiz < 1 || p->smlsiz > 64) return -2;
     if (p->hss_leaf < 8 || p->hss_tol < 0 || p->maxit < 0) return -2;
+    if (!(p->learned_gate >= 0.0)) return -2;
     c->prm = *p;
     if (c->prm.hss_min_k < 4) c->prm.hss_min_k = 4;
     return 0;
@@ -1198,8 +1274,9 @@ int hsdc_secular(hsdc_ctx *c, int k, const double *dlamda, const double *w, doub
     HK(hipMemcpy(dch, bs, sizeof(bs), hipMemcpyHostToDevice));
     HK(hipMemcpy(W.dlamda, dlamda, sizeof(double) * k, hipMemcpyHostToDevice));
     HK(hipMemcpy(W.w, w, sizeof(double) * k, hipMemcpyHostToDevice));
+    CK(sec_setup(c));
     hipLaunchKernelGGL(k_secular, dim3(bs[1]), dim3(64), 0, c->stream, (const Merge *)dm, (const int *)dch, 1,
-                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo);
+                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo, c->dsec, c->dlvlmax);
     HK(hipGetLastError());
     int info = 0;
     HK(hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost));
@@ -1207,6 +1284,7 @@ int hsdc_secular(hsdc_ctx *c, int k, const double *dlamda, const double *w, doub
     HK(hipMemcpy(tau, W.tau, sizeof(double) * k, hipMemcpyDeviceToHost));
     HK(hipMemcpy(org, W.org, sizeof(int) * k, hipMemcpyDeviceToHost));
     HK(hipMemcpy(niter, W.nit, sizeof(int) * k, hipMemcpyDeviceToHost));
+    CK(sec_collect(c));
     return info ? 1 : 0;
 }
 
@@ -1247,8 +1325,9 @@ int hsdc_laed3(hsdc_ctx *c, int k, int n, int n1, double *d, double *q, int ldq,
     long zoff = 0;
     HK(hipMemcpy(c->lvl.p, &zoff, sizeof(long), hipMemcpyHostToDevice));
     if (q2len) HK(hipMemcpy(c->xb.p, q2, sizeof(double) * q2len, hipMemcpyHostToDevice));
+    CK(sec_setup(c));
     hipLaunchKernelGGL(k_secular, dim3(tabs[1]), dim3(64), 0, s, (const Merge *)dm, (const int *)dtab, 1,
-                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo);
+                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo, c->dsec, c->dlvlmax);
     hipLaunchKernelGGL(k_loewner, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
                        (const MergeOut *)dmo, W);
     hipLaunchKernelGGL(k_sbuild, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
@@ -1297,8 +1376,9 @@ int hsdc_loewner(hsdc_ctx *c, int k, double rho, double *dlamda, double *z, doub
     HK(hipMemcpy(dtab, tabs, sizeof(tabs), hipMemcpyHostToDevice));
     HK(hipMemcpy(W.dlamda, dlamda, sizeof(double) * k, hipMemcpyHostToDevice));
     HK(hipMemcpy(W.w, z, sizeof(double) * k, hipMemcpyHostToDevice));
+    CK(sec_setup(c));
     hipLaunchKernelGGL(k_secular, dim3(tabs[1]), dim3(64), 0, s, (const Merge *)dm, (const int *)dtab, 1,
-                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo);
+                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo, c->dsec, c->dlvlmax);
     hipLaunchKernelGGL(k_loewner, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
                        (const MergeOut *)dmo, W);
     hipLaunchKernelGGL(k_alpha, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
@@ -1431,6 +1511,224 @@ int hsdc_stedc(hsdc_ctx *c, char compz, int n, double *d, double *e, double *z, 
         HK(hipMemcpy2D(z, sizeof(double) * ldz, zc, sizeof(double) * n, sizeof(double) * n, n,
                        hipMemcpyDeviceToHost));
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Learned starting iterate of the secular solver
+// ---------------------------------------------------------------------------
+int hsdc_mlp_set_weights(hsdc_ctx *c, const float *w, int nparam)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (!w) { c->hw.clear(); return 0; }
+    if (nparam != MLP_NPARAM) return -3;
+    for (int i = 0; i < nparam; i++)
+        if (!std::isfinite(w[i])) return -2;
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    CK(ensure(c->mlpw, sizeof(float) * MLP_NPARAM));
+    HK(hipMemcpy(c->mlpw.p, w, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+    c->hw.assign(w, w + MLP_NPARAM);
+    return 0;
+}
+
+int hsdc_mlp_get_weights(hsdc_ctx *c, float *w, int nparam)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (nparam != MLP_NPARAM) return -3;
+    if (c->hw.empty()) return 1;
+    memcpy(w, c->hw.data(), sizeof(float) * MLP_NPARAM);
+    return 0;
+}
+
+// FNV-1a over the weight bytes (the checksum of the weights file)
+static unsigned int mlp_fnv(const float *w, int n)
+{
+    const unsigned char *b = (const unsigned char *)w;
+    unsigned int h = 2166136261u;
+    for (size_t i = 0; i < sizeof(float) * (size_t)n; i++) { h ^= b[i]; h *= 16777619u; }
+    return h;
+}
+
+// Weights file: "HSDCMLP1", int32 inputs, int32 hidden, int32 nparam, uint32 FNV-1a of
+// the weights, float32 weights; little-endian.
+int hsdc_mlp_load(hsdc_ctx *c, const char *path)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (!path) return -2;
+    FILE *f = fopen(path, "rb");
+    if (!f) return -2;
+    char magic[8];
+    int hdr[3];
+    unsigned int sum = 0;
+    std::vector<float> w(MLP_NPARAM);
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "HSDCMLP1", 8) == 0 &&
+              fread(hdr, sizeof(int), 3, f) == 3 && hdr[0] == MLP_IN && hdr[1] == MLP_H &&
+              hdr[2] == MLP_NPARAM && fread(&sum, sizeof(sum), 1, f) == 1 &&
+              fread(w.data(), sizeof(float), MLP_NPARAM, f) == (size_t)MLP_NPARAM;
+    fclose(f);
+    if (!ok || mlp_fnv(w.data(), MLP_NPARAM) != sum) return 1;
+    return hsdc_mlp_set_weights(c, w.data(), MLP_NPARAM);
+}
+
+int hsdc_mlp_harvest(hsdc_ctx *c, long long cap)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (cap < 0) return -2;
+    c->hv_cap = 0;
+    if (cap == 0) return 0;
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    size_t bytes = 64 + al64(sizeof(float) * MLP_IN * (size_t)cap) + sizeof(float) * (size_t)cap;
+    CK(ensure(c->hv, bytes));
+    unsigned long long zero = 0;
+    HK(hipMemcpy(c->hv.p, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    c->hv_cap = cap;
+    return 0;
+}
+
+int hsdc_mlp_samples(hsdc_ctx *c, float *x, float *t, long long maxn, long long *n)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (!n) return -5;
+    *n = 0;
+    if (c->hv_cap == 0) return 1;
+    unsigned long long cnt = 0;
+    HK(hipMemcpy(&cnt, c->hv.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    long long m = std::min<long long>((long long)cnt, std::min(maxn, c->hv_cap));
+    if (!x || !t) {
+        *n = std::min<long long>((long long)cnt, c->hv_cap);
+        return 0;
+    }
+    const char *hv = (const char *)c->hv.p;
+    if (m > 0) {
+        HK(hipMemcpy(x, hv + 64, sizeof(float) * MLP_IN * (size_t)m, hipMemcpyDeviceToHost));
+        HK(hipMemcpy(t, hv + 64 + al64(sizeof(float) * MLP_IN * (size_t)c->hv_cap), sizeof(float) * (size_t)m,
+                     hipMemcpyDeviceToHost));
+    }
+    *n = m;
+    return 0;
+}
+
+// training workspace: X | T | idx | partials | block losses | grad | m | v | batch losses
+struct MlpTrainBuf {
+    float *X, *T, *part, *lossp, *g, *m, *v, *lb;
+    int *idx;
+    int nblk;
+};
+
+static int mlp_trainbuf(hsdc_ctx *c, long long ns, int nblk, int nbatch, MlpTrainBuf &b)
+{
+    size_t o1 = 0, o2 = al64(o1 + sizeof(float) * MLP_IN * (size_t)ns), o3 = al64(o2 + sizeof(float) * ns),
+           o4 = al64(o3 + sizeof(int) * ns), o5 = al64(o4 + sizeof(float) * (size_t)nblk * MLP_NPARAM),
+           o6 = al64(o5 + sizeof(float) * nblk), o7 = al64(o6 + sizeof(float) * MLP_NPARAM),
+           o8 = al64(o7 + sizeof(float) * MLP_NPARAM), o9 = al64(o8 + sizeof(float) * MLP_NPARAM),
+           o10 = al64(o9 + sizeof(float) * std::max(nbatch, 1));
+    CK(ensure(c->mlptr, o10));
+    char *p = (char *)c->mlptr.p;
+    b.X = (float *)(p + o1); b.T = (float *)(p + o2); b.idx = (int *)(p + o3); b.part = (float *)(p + o4);
+    b.lossp = (float *)(p + o5); b.g = (float *)(p + o6); b.m = (float *)(p + o7); b.v = (float *)(p + o8);
+    b.lb = (float *)(p + o9);
+    b.nblk = nblk;
+    return 0;
+}
+
+static int mlp_grad_blocks(long long nb) { return (int)std::min<long long>(256, std::max<long long>(1, (nb + 31) / 32)); }
+
+int hsdc_mlp_grad(hsdc_ctx *c, const float *x, const float *t, int ns, float *grad, float *loss)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (ns < 1) return -4;
+    if (c->hw.empty()) return 1;
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    int nblk = mlp_grad_blocks(ns);
+    MlpTrainBuf b;
+    CK(mlp_trainbuf(c, ns, nblk, 1, b));
+    hipStream_t s = c->stream;
+    HK(hipMemcpyAsync(b.X, x, sizeof(float) * MLP_IN * (size_t)ns, hipMemcpyHostToDevice, s));
+    HK(hipMemcpyAsync(b.T, t, sizeof(float) * ns, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nblk), dim3(256), 0, s, (const float *)c->mlpw.p, (const float *)b.X,
+                       (const float *)b.T, (const int *)nullptr, ns, 1.0f / (float)ns, b.part, b.lossp);
+    hipLaunchKernelGGL(k_mlp_sum, dim3(cdiv(MLP_NPARAM, 256)), dim3(256), 0, s, (const float *)b.part, nblk, b.g,
+                       (const float *)b.lossp, b.lb);
+    HK(hipGetLastError());
+    HK(hipMemcpyAsync(grad, b.g, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, s));
+    HK(hipMemcpyAsync(loss, b.lb, sizeof(float), hipMemcpyDeviceToHost, s));
+    HK(hipStreamSynchronize(s));
+    return 0;
+}
+
+static unsigned long long mlp_sm64(unsigned long long &st)
+{
+    unsigned long long z = (st += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Adam over shuffled mini-batches, learning rate lr * (1 + cos(pi e / epochs)) / 2 in
+// epoch e.  Starts from the loaded weights, or from a seeded uniform initialisation
+// (bound sqrt(6 / fan_in), last layer scaled by 0.1, zero biases) when none are loaded.
+int hsdc_mlp_train(hsdc_ctx *c, const float *x, const float *t, long long ns, int epochs, int batch, float lr,
+                   unsigned long long seed, float *loss_hist)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (ns < 1 || ns > 0x7fffffffLL) return -4;
+    if (epochs < 1) return -5;
+    if (batch < 1) return -6;
+    if (!(lr > 0.0f)) return -7;
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    unsigned long long st = seed * 0x100000001ULL + 77;
+    if (c->hw.empty()) {
+        std::vector<float> w(MLP_NPARAM, 0.0f);
+        auto uni = [&](float a) { return (float)(((mlp_sm64(st) >> 11) * 0x1.0p-53 * 2.0 - 1.0) * a); };
+        for (int i = 0; i < MLP_H * MLP_IN; i++) w[MLP_W1 + i] = uni(sqrtf(6.0f / MLP_IN));
+        for (int i = 0; i < MLP_H * MLP_H; i++) w[MLP_W2 + i] = uni(sqrtf(6.0f / MLP_H));
+        for (int i = 0; i < MLP_H; i++) w[MLP_W3 + i] = uni(sqrtf(6.0f / MLP_H) * 0.1f);
+        CK(hsdc_mlp_set_weights(c, w.data(), MLP_NPARAM));
+    }
+    const int bs = (int)std::min<long long>(batch, ns);
+    const int nbatch = (int)((ns + bs - 1) / bs);
+    const int nblk = mlp_grad_blocks(bs);
+    MlpTrainBuf b;
+    CK(mlp_trainbuf(c, ns, nblk, nbatch, b));
+    hipStream_t s = c->stream;
+    float *W = (float *)c->mlpw.p;
+    HK(hipMemcpyAsync(b.X, x, sizeof(float) * MLP_IN * (size_t)ns, hipMemcpyHostToDevice, s));
+    HK(hipMemcpyAsync(b.T, t, sizeof(float) * ns, hipMemcpyHostToDevice, s));
+    std::vector<float> zeros(MLP_NPARAM, 0.0f);
+    HK(hipMemcpyAsync(b.m, zeros.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, s));
+    HK(hipMemcpyAsync(b.v, zeros.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, s));
+    HK(hipStreamSynchronize(s));
+    std::vector<int> perm(ns);
+    std::vector<float> lb(nbatch);
+    for (long long i = 0; i < ns; i++) perm[i] = (int)i;
+    long long step = 0;
+    for (int e = 0; e < epochs; e++) {
+        for (long long i = ns - 1; i > 0; i--) std::swap(perm[i], perm[mlp_sm64(st) % (unsigned long long)(i + 1)]);
+        HK(hipMemcpyAsync(b.idx, perm.data(), sizeof(int) * ns, hipMemcpyHostToDevice, s));
+        const float lre = lr * 0.5f * (1.0f + cosf(3.14159265f * (float)e / (float)epochs));
+        for (int bi = 0; bi < nbatch; bi++) {
+            const int b0 = bi * bs, nb = (int)std::min<long long>(bs, ns - b0);
+            step++;
+            const float b1c = 1.0f - powf(0.9f, (float)step), b2c = 1.0f - powf(0.999f, (float)step);
+            hipLaunchKernelGGL(k_mlp_grad, dim3(nblk), dim3(256), 0, s, (const float *)W, (const float *)b.X,
+                               (const float *)b.T, (const int *)(b.idx + b0), nb, 1.0f / (float)nb, b.part, b.lossp);
+            hipLaunchKernelGGL(k_mlp_sum, dim3(cdiv(MLP_NPARAM, 256)), dim3(256), 0, s, (const float *)b.part, nblk,
+                               b.g, (const float *)b.lossp, b.lb + bi);
+            hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(MLP_NPARAM, 256)), dim3(256), 0, s, W, (const float *)b.g, b.m,
+                               b.v, lre, b1c, b2c);
+        }
+        HK(hipGetLastError());
+        HK(hipMemcpyAsync(lb.data(), b.lb, sizeof(float) * nbatch, hipMemcpyDeviceToHost, s));
+        HK(hipStreamSynchronize(s));
+        if (loss_hist) {
+            double a = 0.0;
+            for (int bi = 0; bi < nbatch; bi++) a += lb[bi];
+            loss_hist[e] = (float)(a / nbatch);
+        }
+    }
+    HK(hipMemcpy(c->hw.data(), W, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+    for (float v : c->hw)
+        if (!std::isfinite(v)) return 2;
     return 0;
 }
 

@@ -52,3 +52,17 @@ struct Work {
     double *lam, *tau, *wnew, *alpha, *difl, *difr;
     int *org, *nit;
 };
+
+// control block of the secular kernel: learned-start weights, statistics and
+// optional harvesting of training samples
+struct SecCtl {
+    const float *w;              // MLP weights or nullptr
+    unsigned long long *iters;   // total iterations
+    unsigned long long *used;    // roots started from the learned iterate
+    unsigned long long *wave_iters;  // sum over wavefronts of their largest iteration count
+    float *hf, *ht;              // harvest: features (cap x 34) and targets (cap)
+    unsigned long long *count;
+    unsigned long long cap;
+    int kmin;                    // merges with k < kmin use (and harvest) the classical start only
+    float gate;                  // the proposal is taken when |log-correction| > gate
+};

@@ -9,6 +9,7 @@
 // oracle and compiled with -ffp-contract=off, so a merge given identical inputs
 // returns bit-identical roots, index sets and rotations.
 #include "hsdc_internal.h"
+#include "hsdc_mlp.h"
 
 #define WG 256
 
@@ -372,6 +373,8 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
         W.dlamda[s + K] = DP; W.w[s + K] = zP; indxp[K] = PJ;
         K++;
         sk = K; sk2 = nrot;
+        if (nrot == 0) sfallback = 2;  // no rotation: the region is the small-z columns reversed
+        if (nrot != 0) {
         // ---- 4. replay the deflated region as a list of segments, front first.
         // Segment: small-z run of sorted positions [a, b) (front holds b-1) or,
         // with a = -1, one rotated column b with value d.
@@ -435,9 +438,20 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
             }
             snseg = cnt2;
         }
+        }
     }
     __syncthreads();
-    if (!sfallback) {
+    if (sfallback == 2) {
+        // small-z column at sorted position j is the r-th pushed, r = j - #survivors before j
+        for (int j = tid; j < n; j += WG) {
+            if (rho * fabs(zs[j]) > tol) continue;
+            int lo = 0, hi = sL;
+            while (lo < hi) { int md2 = (lo + hi) >> 1; if (nsl[md2] < j) lo = md2 + 1; else hi = md2; }
+            int r = j - lo;
+            indxp[n - 1 - r] = indx[j];
+            ddef[n - 1 - r] = ds[j];
+        }
+    } else if (!sfallback) {
         for (int k = 0; k < snseg; k++) {
             int g = seg_ord[k], P = seg_pos[k];
             int a = seg_a[g], b = seg_b[g];
@@ -604,12 +618,16 @@ __global__ __launch_bounds__(WG) void k_rotate(const Merge *mg, const int *chunk
 // the oracle's orc_laed4.  One lane per root; every lane of a block belongs to
 // the same merge (block_start pads each merge to whole blocks).
 // ---------------------------------------------------------------------------
-struct SecOut { double lam, tau, dl0, dl1; int org, nit, info; };
+struct SecOut { double lam, tau, dl0, dl1, tc; int org, nit, info, used; };
 
-__device__ SecOut sec_solve(int n, int j, const double *d, const double *z, double rho, int maxit)
+// wmlp: weights of the learned starting iterate (nullptr: classical start only),
+// taken when the proposed log-correction exceeds gate in magnitude; feat: receives
+// the 34 features when non-null
+__device__ SecOut sec_solve(int n, int j, const double *d, const double *z, double rho, int maxit,
+                            const float *wmlp, float gate, float *feat)
 {
     SecOut o;
-    o.dl0 = o.dl1 = 0.0; o.info = 0; o.nit = 0;
+    o.dl0 = o.dl1 = 0.0; o.info = 0; o.nit = 0; o.used = 0; o.tc = 0.0;
     if (n == 1) {
         o.lam = d[0] + rho * z[0] * z[0]; o.org = 1; o.tau = rho * z[0] * z[0];
         return o;
@@ -646,7 +664,7 @@ __device__ SecOut sec_solve(int n, int j, const double *d, const double *z, doub
     }
     const double rhoinv = 1.0 / rho;
     int ii;
-    double t, lo, hi;
+    double t, lo, hi, wb;
     if (j == n) {
         double zz = 0.0;
         for (int i = 0; i < n; i++) zz += z[i] * z[i];
@@ -664,9 +682,10 @@ __device__ SecOut sec_solve(int n, int j, const double *d, const double *z, doub
             else t = 2.0 * z2 * dl / (b + sq);
         }
         if (!(t > 0.0 && t <= hh)) t = mid;
-        ii = n; lo = 0.0; hi = hh;
+        ii = n; lo = 0.0; hi = hh; wb = hh;
     } else {
         double del = d[j] - d[j - 1], mid = 0.5 * del;
+        wb = del;
         double c = rhoinv;
         for (int i = 0; i < n; i++) {
             if (i == j - 1 || i == j) continue;
@@ -690,26 +709,50 @@ __device__ SecOut sec_solve(int n, int j, const double *d, const double *z, doub
             ii = j + 1; lo = -mid; hi = 0.0;
         }
     }
+    // learned starting iterate: kept when it lies in the bracket of the classical
+    // start, which otherwise stays the fallback
+    o.tc = t;
+    if (wmlp || feat) {
+        double xc = (ii == j) ? t / wb : 1.0 + t / wb;
+        float fl[MLP_IN];
+        sec_features(n, j - 1, d, z, rho, wb, xc, fl);
+        if (feat)
+            for (int q = 0; q < MLP_IN; q++) feat[q] = fl[q];
+        if (wmlp) {
+            float y = mlp_forward(wmlp, fl);
+            y = y < -16.0f ? -16.0f : (y > 16.0f ? 16.0f : y);
+            double tl = t * sec_exp((double)y);
+            if (fabsf(y) > gate && tl > lo && tl < hi) { t = tl; o.used = 1; }
+        }
+    }
     const double dorg = d[ii - 1];
     const int split = (j == n) ? n - 1 : j;
     int it, info = 1;
     for (it = 0; it <= maxit; it++) {
+        // psi over poles left of the root (ascending), phi over poles right of it
+        // (descending), the origin pole excluded; the two sums advance in the same
+        // loop for instruction-level parallelism, each in its own fixed order
         double psi = 0.0, dpsi = 0.0, epsi = 0.0, phi = 0.0, dphi = 0.0, ephi = 0.0;
-        for (int i = 0; i < split; i++) {
-            if (i == ii - 1) continue;
-            double dl = (d[i] - dorg) - t;
-            double q = z[i] / dl;
-            psi += z[i] * q;
-            dpsi += q * q;
-            epsi += psi;
-        }
-        for (int i = n - 1; i >= split; i--) {
-            if (i == ii - 1) continue;
-            double dl = (d[i] - dorg) - t;
-            double q = z[i] / dl;
-            phi += z[i] * q;
-            dphi += q * q;
-            ephi += phi;
+        const int na = split - ((ii - 1 < split) ? 1 : 0);          // psi: i = 0 .. na-1
+        const int nb = (n - split) - ((ii - 1 >= split) ? 1 : 0);   // phi: i = n-1 .. n-nb
+        const int nab = na > nb ? na : nb;
+#pragma unroll 4
+        for (int t2 = 0; t2 < nab; t2++) {
+            if (t2 < na) {
+                double dl = (d[t2] - dorg) - t;
+                double q = z[t2] / dl;
+                psi += z[t2] * q;
+                dpsi += q * q;
+                epsi += psi;
+            }
+            if (t2 < nb) {
+                int i = n - 1 - t2;
+                double dl = (d[i] - dorg) - t;
+                double q = z[i] / dl;
+                phi += z[i] * q;
+                dphi += q * q;
+                ephi += phi;
+            }
         }
         double qo = z[ii - 1] / (-t);
         double torg = z[ii - 1] * qo;
@@ -746,21 +789,57 @@ __device__ SecOut sec_solve(int n, int j, const double *d, const double *z, doub
     return o;
 }
 
+// Root lanes.  sec->w: learned-start weights or null; sec->hf/ht: when non-null,
+// (features, log(converged tau / classical starting tau)) of every root of a merge
+// with k >= max(3, sec->kmin) are appended for training, up to sec->cap samples.
 __global__ __launch_bounds__(64) void k_secular(const Merge *mg, const int *block_start, int nm,
-                                                const MergeOut *mo, Work W, int maxit, int *info)
+                                                const MergeOut *mo, Work W, int maxit, int *info,
+                                                const SecCtl *sec, unsigned int *itmax)
 {
     int m = find_seg(block_start, nm, blockIdx.x);
     const MergeOut o = mo[m];
     int j = (blockIdx.x - block_start[m]) * 64 + threadIdx.x + 1;
-    if (j > o.k || o.flags) return;
-    int s = mg[m].s;
-    SecOut r = sec_solve(o.k, j, W.dlamda + s, W.w + s, o.rho, maxit);
-    W.lam[s + j - 1] = r.lam;
-    W.tau[s + j - 1] = r.tau;
-    W.org[s + j - 1] = r.org;
-    W.nit[s + j - 1] = r.nit;
-    if (o.k == 2) { W.difl[s + j - 1] = r.dl0; W.difr[s + j - 1] = r.dl1; }
-    if (r.info) atomicOr(info, 1);
+    const SecCtl C = *sec;
+    int nit = 0, used = 0;
+    if (j <= o.k && !o.flags) {
+        int s = mg[m].s;
+        float fe[MLP_IN];
+        const bool big = o.k >= 3 && o.k >= C.kmin;
+        const bool want = (C.hf != nullptr) && big;
+        SecOut r = sec_solve(o.k, j, W.dlamda + s, W.w + s, o.rho, maxit, big ? C.w : nullptr, C.gate,
+                             want ? fe : nullptr);
+        nit = r.nit;
+        used = r.used;
+        const double ratio = r.tau / r.tc;
+        if (want && r.info == 0 && ratio > 0.0 && ratio < 1e300) {
+            unsigned long long slot = atomicAdd(C.count, 1ULL);
+            if (slot < C.cap) {
+                for (int q = 0; q < MLP_IN; q++) C.hf[slot * MLP_IN + q] = fe[q];
+                C.ht[slot] = (float)log(ratio);
+            }
+        }
+        W.lam[s + j - 1] = r.lam;
+        W.tau[s + j - 1] = r.tau;
+        W.org[s + j - 1] = r.org;
+        W.nit[s + j - 1] = r.nit;
+        if (o.k == 2) { W.difl[s + j - 1] = r.dl0; W.difr[s + j - 1] = r.dl1; }
+        if (r.info) atomicOr(info, 1);
+    }
+    // statistics, one atomic per wavefront: total iterations, learned starts, and the
+    // wavefront's largest count (a wavefront runs as long as its slowest root, a
+    // launch as long as its slowest wavefront)
+    int mx = nit, sum = nit, nu = used;
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = max(mx, __shfl_xor(mx, off));
+        sum += __shfl_xor(sum, off);
+        nu += __shfl_xor(nu, off);
+    }
+    if (threadIdx.x == 0 && mx > 0) {
+        atomicAdd(C.iters, (unsigned long long)sum);
+        if (nu) atomicAdd(C.used, (unsigned long long)nu);
+        atomicAdd(C.wave_iters, (unsigned long long)mx);
+        atomicMax(itmax, (unsigned int)mx);
+    }
 }
 
 // delta_i(j) = d_i - lambda_j, formed as the secular solver forms it

@@ -28,19 +28,26 @@ HSDC_ERR_NODEVICE = -1002
 EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_set_params",
            "hsdc_get_stats", "hsdc_set_stream", "hsdc_stedc", "hsdc_stedc_dev",
            "hsdc_stedc_batched_dev", "hsdc_deflate", "hsdc_secular", "hsdc_laed3", "hsdc_loewner",
-           "hsdc_hss_evc", "hsdc_hss_dense"]
+           "hsdc_hss_evc", "hsdc_hss_dense", "hsdc_mlp_set_weights", "hsdc_mlp_get_weights",
+           "hsdc_mlp_load", "hsdc_mlp_harvest", "hsdc_mlp_samples", "hsdc_mlp_grad", "hsdc_mlp_train"]
+
+# learned secular start: 34 inputs -> 64 -> 64 -> 1
+MLP_IN, MLP_H = 34, 64
+MLP_NPARAM = MLP_H * MLP_IN + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1
 
 
 class Params(ctypes.Structure):
     _fields_ = [("smlsiz", _i), ("hss_min_k", _i), ("hss_leaf", _i), ("hss_tol", _d),
-                ("maxit", _i), ("batched", _i), ("learned", _i), ("profile", _i)]
+                ("maxit", _i), ("batched", _i), ("learned", _i), ("profile", _i),
+                ("learned_min_k", _i), ("learned_gate", _d)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("levels", _i), ("merges", _i), ("hss_merges", _i), ("roots", ctypes.c_longlong),
                 ("secular_iters", ctypes.c_longlong), ("deflated", ctypes.c_longlong),
                 ("k_top", _i), ("rank_max", _i), ("ms_total", _d), ("ms_final", _d),
-                ("final_bytes", ctypes.c_longlong)]
+                ("final_bytes", ctypes.c_longlong), ("learned_roots", ctypes.c_longlong),
+                ("secular_wave_iters", ctypes.c_longlong), ("secular_crit_iters", ctypes.c_longlong)]
 
 
 _LIB = None
@@ -73,12 +80,59 @@ def lib():
         L.hsdc_hss_evc.argtypes = [_vp, _i, _i, _d, _P(_d), _P(_d), _P(_d), _P(_d), _P(_d), _P(_d),
                                    _P(_d), _i, _i, _P(_i)]
         L.hsdc_hss_dense.argtypes = [_vp, _P(_d), _i]
+        _f = _P(ctypes.c_float)
+        _ll = ctypes.c_longlong
+        L.hsdc_mlp_set_weights.argtypes = [_vp, _f, _i]
+        L.hsdc_mlp_get_weights.argtypes = [_vp, _f, _i]
+        L.hsdc_mlp_load.argtypes = [_vp, ctypes.c_char_p]
+        L.hsdc_mlp_harvest.argtypes = [_vp, _ll]
+        L.hsdc_mlp_samples.argtypes = [_vp, _f, _f, _ll, _P(_ll)]
+        L.hsdc_mlp_grad.argtypes = [_vp, _f, _f, _i, _f, _f]
+        L.hsdc_mlp_train.argtypes = [_vp, _f, _f, _ll, _i, _i, ctypes.c_float, ctypes.c_ulonglong, _f]
         _LIB = L
     return _LIB
 
 
 def _dp(a):
     return a.ctypes.data_as(_P(_d))
+
+
+def _fp(a):
+    return a.ctypes.data_as(_P(ctypes.c_float))
+
+
+def _fnv1a(b):
+    h = 2166136261
+    for x in b:
+        h = ((h ^ x) * 16777619) & 0xFFFFFFFF
+    return h
+
+
+def save_mlp_weights(path, w):
+    """Writes the weights file read by hsdc_mlp_load: b"HSDCMLP1", int32 inputs,
+    hidden, nparam, uint32 FNV-1a of the weight bytes, float32 weights (little-endian)."""
+    w = np.ascontiguousarray(w, "<f4")
+    if w.size != MLP_NPARAM:
+        raise ValueError("expected %d weights" % MLP_NPARAM)
+    b = w.tobytes()
+    hdr = np.array([MLP_IN, MLP_H, MLP_NPARAM], "<i4").tobytes()
+    with open(path, "wb") as f:
+        f.write(b"HSDCMLP1" + hdr + np.array([_fnv1a(b)], "<u4").tobytes() + b)
+
+
+def load_mlp_weights(path):
+    """Reads and verifies a weights file; returns float32 weights."""
+    raw = open(path, "rb").read()
+    if raw[:8] != b"HSDCMLP1":
+        raise ValueError("not a weights file: %s" % path)
+    hdr = np.frombuffer(raw[8:20], "<i4")
+    if tuple(hdr) != (MLP_IN, MLP_H, MLP_NPARAM):
+        raise ValueError("shape mismatch %s" % (tuple(hdr),))
+    sum_ = int(np.frombuffer(raw[20:24], "<u4")[0])
+    b = raw[24:24 + 4 * MLP_NPARAM]
+    if len(b) != 4 * MLP_NPARAM or _fnv1a(b) != sum_:
+        raise ValueError("checksum mismatch: %s" % path)
+    return np.frombuffer(b, "<f4").copy()
 
 
 def _ip(a):
@@ -236,3 +290,63 @@ class Context:
         if info != 0:
             raise RuntimeError("hsdc_hss_dense: %d" % info)
         return a
+
+    # ---- learned secular start ----
+    def mlp_set_weights(self, w):
+        if w is None:
+            return lib().hsdc_mlp_set_weights(self._h, None, 0)
+        w = np.ascontiguousarray(w, np.float32)
+        rc = lib().hsdc_mlp_set_weights(self._h, _fp(w), w.size)
+        if rc != 0:
+            raise ValueError("hsdc_mlp_set_weights: %d" % rc)
+        return rc
+
+    def mlp_get_weights(self):
+        w = np.zeros(MLP_NPARAM, np.float32)
+        rc = lib().hsdc_mlp_get_weights(self._h, _fp(w), MLP_NPARAM)
+        return w if rc == 0 else None
+
+    def mlp_load(self, path):
+        rc = lib().hsdc_mlp_load(self._h, os.fsencode(path))
+        if rc != 0:
+            raise ValueError("hsdc_mlp_load(%s): %d" % (path, rc))
+
+    def mlp_harvest(self, cap):
+        rc = lib().hsdc_mlp_harvest(self._h, int(cap))
+        if rc != 0:
+            raise RuntimeError("hsdc_mlp_harvest: %d" % rc)
+
+    def mlp_samples(self, maxn):
+        x = np.zeros((int(maxn), MLP_IN), np.float32)
+        t = np.zeros(int(maxn), np.float32)
+        n = ctypes.c_longlong(0)
+        rc = lib().hsdc_mlp_samples(self._h, _fp(x), _fp(t), int(maxn), ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError("hsdc_mlp_samples: %d" % rc)
+        return x[:n.value], t[:n.value]
+
+    def mlp_count(self):
+        n = ctypes.c_longlong(0)
+        rc = lib().hsdc_mlp_samples(self._h, None, None, 0, ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError("hsdc_mlp_samples: %d" % rc)
+        return n.value
+
+    def mlp_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        rc = lib().hsdc_mlp_grad(self._h, _fp(x), _fp(t), len(t), _fp(g), _fp(loss))
+        if rc != 0:
+            raise RuntimeError("hsdc_mlp_grad: %d" % rc)
+        return g, float(loss[0])
+
+    def mlp_train(self, x, t, epochs, batch, lr, seed=0):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        hist = np.zeros(epochs, np.float32)
+        rc = lib().hsdc_mlp_train(self._h, _fp(x), _fp(t), len(t), epochs, batch, lr, seed, _fp(hist))
+        if rc != 0:
+            raise RuntimeError("hsdc_mlp_train: %d" % rc)
+        return hist

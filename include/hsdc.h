@@ -38,6 +38,8 @@ typedef struct {
     int batched;     /* 1: one launch per DC level over all merges; 0: one merge at a time */
     int learned;     /* 1: MLP proposes the secular starting iterate */
     int profile;     /* 1: time the final reordering kernel with HIP events */
+    int learned_min_k; /* merges with K below this keep the classical start (default 256) */
+    double learned_gate; /* the proposal is taken when its |log-correction| exceeds this */
 } hsdc_params;
 
 /* Run statistics of the last call on a context. */
@@ -50,6 +52,9 @@ typedef struct {
     double ms_total;    /* device time of the solve (events), ms */
     double ms_final;    /* final reordering kernel (events, profile = 1), ms */
     long long final_bytes;  /* its algorithmic HBM bytes: Z written + Q rows read */
+    long long learned_roots; /* roots started from the learned proposal */
+    long long secular_wave_iters; /* sum over 64-root wavefronts of their largest count */
+    long long secular_crit_iters; /* sum over DC levels of the largest count of the level */
 } hsdc_stats;
 
 void hsdc_default_params(hsdc_params *p);
@@ -116,6 +121,41 @@ int hsdc_hss_evc(hsdc_ctx *ctx, int k, int ni, double tol, const double *f, cons
 
 /* Dense expansion of the HSS form built by the last hsdc_hss_evc call (k x k). */
 int hsdc_hss_dense(hsdc_ctx *ctx, double *a, int lda);
+
+/* ---- learned starting iterate of the secular solver ----
+ * These have no counterpart in HSSPACK, whose DLAED4 always starts from the
+ * classical two-pole guess.  With params.learned = 1 and weights loaded, every root
+ * of a merge with K >= params.learned_min_k starts from an FP32 network's proposal
+ * (inputs: 16 nearest pole gaps, 16 normalised z^2, rho, the classical guess; two
+ * hidden layers of 64, ReLU) when its log-correction y of the classical offset
+ * exceeds params.learned_gate in magnitude and the proposal lies inside the
+ * classical bracket, from the classical guess otherwise.  The FP64 iteration and its stopping test are
+ * unchanged, so the result is a converged DLAED4 root either way. */
+#define HSDC_MLP_IN 34
+#define HSDC_MLP_NPARAM 6465  /* W1(64x34) b1(64) W2(64x64) b2(64) w3(64) b3 */
+
+/* Weights (host float32, HSDC_MLP_NPARAM); w = NULL unloads them. */
+int hsdc_mlp_set_weights(hsdc_ctx *ctx, const float *w, int nparam);
+/* Returns 1 when no weights are loaded. */
+int hsdc_mlp_get_weights(hsdc_ctx *ctx, float *w, int nparam);
+/* Weights file: "HSDCMLP1", int32 34, int32 64, int32 nparam, uint32 FNV-1a of the
+ * weight bytes, float32 weights (little-endian).  1 = bad file or checksum. */
+int hsdc_mlp_load(hsdc_ctx *ctx, const char *path);
+/* Training data from the solver's own merges: every later solve on this context
+ * appends (features, log(converged tau / classical starting tau)) of each root of
+ * each merge with K >= learned_min_k, up to cap samples; cap = 0 stops harvesting.
+ * The network's output y proposes tau = tau_classical * exp(clamp(y, -16, 16)). */
+int hsdc_mlp_harvest(hsdc_ctx *ctx, long long cap);
+/* Copies up to maxn harvested samples: x(34, n) row-major per sample, t(n).  With
+ * x or t NULL, *n receives the number of samples held and nothing is copied. */
+int hsdc_mlp_samples(hsdc_ctx *ctx, float *x, float *t, long long maxn, long long *n);
+/* Gradient of the loaded weights on a batch (device kernels):
+ * L = (1/ns) sum_s (y_s - t_s)^2; grad(HSDC_MLP_NPARAM). */
+int hsdc_mlp_grad(hsdc_ctx *ctx, const float *x, const float *t, int ns, float *grad, float *loss);
+/* Adam training of the loaded weights (seeded initialisation when none are loaded)
+ * on the device; loss_hist(epochs) receives the mean batch loss of each epoch. */
+int hsdc_mlp_train(hsdc_ctx *ctx, const float *x, const float *t, long long ns, int epochs, int batch,
+                   float lr, unsigned long long seed, float *loss_hist);
 
 #ifdef __cplusplus
 }

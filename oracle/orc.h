@@ -61,6 +61,28 @@ void orc_dlamrg(int n1, int n2, const double *a, int s1, int s2, int *index);
 int orc_laed4(int n, int j, const double *d, const double *z, double *delta, double rho,
               double *dlam, int *org, double *tau, int *niter, int maxit);
 
+/* Learned starting iterate (FP32 network 34 -> 64 -> 64 -> 1, ReLU): feature map of
+ * root r (0-based) with bracket width W and classical start xc (bracket fraction),
+ * forward pass y, and the secular solve started from tau_classical * exp(clamp(y, -16,
+ * 16)) when |y| > gate and that lies inside the bracket (*used = 1), from the
+ * classical start otherwise.  wmlp == NULL is orc_laed4.  orc_sec_exp is the exp both sides use. */
+#define ORC_MLP_IN 34
+#define ORC_MLP_H 64
+#define ORC_MLP_W1 0
+#define ORC_MLP_B1 (ORC_MLP_W1 + ORC_MLP_H * ORC_MLP_IN)
+#define ORC_MLP_W2 (ORC_MLP_B1 + ORC_MLP_H)
+#define ORC_MLP_B2 (ORC_MLP_W2 + ORC_MLP_H * ORC_MLP_H)
+#define ORC_MLP_W3 (ORC_MLP_B2 + ORC_MLP_H)
+#define ORC_MLP_B3 (ORC_MLP_W3 + ORC_MLP_H)
+#define ORC_MLP_NPARAM (ORC_MLP_B3 + 1)
+void orc_mlp_features(int n, int r, const double *d, const double *z, double rho, double W, double xc,
+                      float *f);
+float orc_mlp_forward(const float *w, const float *x);
+double orc_sec_exp(double x);
+int orc_laed4_mlp(int n, int j, const double *d, const double *z, double *delta, double rho,
+                  double *dlam, int *org, double *tau, int *niter, int maxit, const float *wmlp,
+                  float gate, int *used);
+
 /* Implicit-shift QL on a symmetric tridiagonal of order n, eigenvalues ascending and
  * eigenvectors in z (ldz), z is overwritten with the eigenvector matrix. */
 int orc_steqr(int n, double *d, double *e, double *z, int ldz);

@@ -149,10 +149,96 @@ static void sec_init(int n, int j, const double *d, const double *z, double rho,
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Learned starting iterate of the secular iteration: feature map of one root and
+ * the FP32 network 34 -> 64 -> 64 -> 1 (ReLU).  Weight layout: W1 (64x34, row u =
+ * hidden unit), b1 (64), W2 (64x64), b2 (64), w3 (64), b3.  Features of root r
+ * (0-based) with bracket width W:
+ *   0..15  u/(1+|u|), u = W/(d[r+t]-d[r]) for t = -8..-1, 2..9 (0 when absent)
+ *   16..31 z[r+t]^2 for t = -7..8 normalised to sum 1 (0 when absent)
+ *   32     q/(1+q), q = W/(rho * that sum)
+ *   33     the classical starting iterate as a fraction of the bracket. */
+void orc_mlp_features(int n, int r, const double *d, const double *z, double rho, double W, double xc,
+                      float *f)
+{
+    static const int T[16] = {-8, -7, -6, -5, -4, -3, -2, -1, 2, 3, 4, 5, 6, 7, 8, 9};
+    for (int q = 0; q < 16; q++) {
+        int i = r + T[q];
+        float v = 0.0f;
+        if (i >= 0 && i < n) {
+            double u = W / (d[i] - d[r]);
+            v = (float)(u / (1.0 + fabs(u)));
+        }
+        f[q] = v;
+    }
+    double zw[16], S = 0.0;
+    for (int q = 0; q < 16; q++) {
+        int i = r - 7 + q;
+        zw[q] = (i >= 0 && i < n) ? z[i] * z[i] : 0.0;
+        S = S + zw[q];
+    }
+    for (int q = 0; q < 16; q++) f[16 + q] = (float)(zw[q] / S);
+    double qv = W / (rho * S);
+    f[32] = (float)(qv / (1.0 + qv));
+    f[33] = (float)xc;
+}
+
+/* e^x for |x| <= 16: 2^k e^r, k the nearest integer to x / ln 2, e^r by its Taylor
+ * series to r^13 in Horner form (the device computes the same sequence) */
+double orc_sec_exp(double x)
+{
+    /* ln 2 split in two (Cody and Waite): kf * LN2_HI is exact for |kf| < 2^11 */
+    const double LN2 = 0.69314718055994530942, LN2_HI = 6.93147180369123816490e-01,
+                 LN2_LO = 1.90821492927058770002e-10;
+    double kf = floor(x / LN2 + 0.5);
+    double r = (x - kf * LN2_HI) - kf * LN2_LO;
+    double p = 1.0 / 6227020800.0;
+    p = p * r + 1.0 / 479001600.0;
+    p = p * r + 1.0 / 39916800.0;
+    p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0;
+    p = p * r + 1.0 / 40320.0;
+    p = p * r + 1.0 / 5040.0;
+    p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0;
+    p = p * r + 1.0 / 24.0;
+    p = p * r + 1.0 / 6.0;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp(p, (int)kf);
+}
+
+float orc_mlp_forward(const float *w, const float *x)
+{
+    float h1[ORC_MLP_H];
+    for (int u = 0; u < ORC_MLP_H; u++) {
+        float a = w[ORC_MLP_B1 + u];
+        for (int k = 0; k < ORC_MLP_IN; k++) a = a + w[ORC_MLP_W1 + u * ORC_MLP_IN + k] * x[k];
+        h1[u] = a > 0.0f ? a : 0.0f;
+    }
+    float y = w[ORC_MLP_B3];
+    for (int u = 0; u < ORC_MLP_H; u++) {
+        float a = w[ORC_MLP_B2 + u];
+        for (int v = 0; v < ORC_MLP_H; v++) a = a + w[ORC_MLP_W2 + u * ORC_MLP_H + v] * h1[v];
+        a = a > 0.0f ? a : 0.0f;
+        y = y + w[ORC_MLP_W3 + u] * a;
+    }
+    return y;
+}
+
 int orc_laed4(int n, int j, const double *d, const double *z, double *delta, double rho,
               double *dlam, int *org, double *tau, int *niter, int maxit)
 {
+    return orc_laed4_mlp(n, j, d, z, delta, rho, dlam, org, tau, niter, maxit, NULL, 0.0f, NULL);
+}
+
+int orc_laed4_mlp(int n, int j, const double *d, const double *z, double *delta, double rho,
+                  double *dlam, int *org, double *tau, int *niter, int maxit, const float *wmlp,
+                  float gate, int *used)
+{
     if (niter) *niter = 0;
+    if (used) *used = 0;
     if (n == 1) {
         *dlam = d[0] + rho * z[0] * z[0];
         delta[0] = 1.0;
@@ -170,6 +256,20 @@ int orc_laed4(int n, int j, const double *d, const double *z, double *delta, dou
     int ii;
     double t, lo, hi;
     sec_init(n, j, d, z, rho, &ii, &t, &lo, &hi);
+    if (wmlp) {
+        /* the proposal replaces the classical start only inside its bracket */
+        double wb = (j == n) ? hi : d[j] - d[j - 1];
+        double xc = (ii == j) ? t / wb : 1.0 + t / wb;
+        float fl[ORC_MLP_IN];
+        orc_mlp_features(n, j - 1, d, z, rho, wb, xc, fl);
+        float y = orc_mlp_forward(wmlp, fl);
+        y = y < -16.0f ? -16.0f : (y > 16.0f ? 16.0f : y);
+        double tl = t * orc_sec_exp((double)y);
+        if (fabsf(y) > gate && tl > lo && tl < hi) {
+            t = tl;
+            if (used) *used = 1;
+        }
+    }
     const double rhoinv = 1.0 / rho;
     const double dorg = d[ii - 1];
     const int split = (j == n) ? n - 1 : j; /* psi over 1..split, phi over split+1..n */

@@ -73,6 +73,47 @@ def laed4(d, z, rho, j, maxit=100):
                 info=info)
 
 
+def laed4_mlp(d, z, rho, j, w, maxit=100, gate=0.0):
+    """orc_laed4 started from the learned proposal of weights w (float32)."""
+    n = len(d)
+    d = np.ascontiguousarray(d, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    w = np.ascontiguousarray(w, np.float32)
+    delta = np.zeros(n)
+    lam, tau = _d(0), _d(0)
+    org, nit, used = _i(0), _i(0), _i(0)
+    info = lib().orc_laed4_mlp(n, j, dp(d), dp(z), dp(delta), _d(rho), ctypes.byref(lam),
+                               ctypes.byref(org), ctypes.byref(tau), ctypes.byref(nit), maxit,
+                               w.ctypes.data_as(_P(ctypes.c_float)), ctypes.c_float(gate),
+                               ctypes.byref(used))
+    return dict(lam=lam.value, org=org.value, tau=tau.value, niter=nit.value, info=info,
+                used=used.value)
+
+
+def mlp_forward(w, x):
+    """C restatement of the FP32 forward pass, one sample."""
+    w = np.ascontiguousarray(w, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    L = lib()
+    L.orc_mlp_forward.restype = ctypes.c_float
+    return L.orc_mlp_forward(w.ctypes.data_as(_P(ctypes.c_float)), x.ctypes.data_as(_P(ctypes.c_float)))
+
+
+def sec_exp(x):
+    L = lib()
+    L.orc_sec_exp.restype = _d
+    return L.orc_sec_exp(_d(x))
+
+
+def mlp_features(d, z, rho, r, W, xc):
+    d = np.ascontiguousarray(d, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    f = np.zeros(34, np.float32)
+    lib().orc_mlp_features(len(d), int(r), dp(d), dp(z), _d(rho), _d(W), _d(xc),
+                           f.ctypes.data_as(_P(ctypes.c_float)))
+    return f
+
+
 def dlamrg(a, n1, n2, s1, s2):
     a = np.ascontiguousarray(a, np.float64)
     idx = np.zeros(n1 + n2, np.int32)
@@ -220,6 +261,7 @@ def cauchy_dense(f, u, v, difl, difr):
 def _setup():
     L = lib()
     L.orc_laed4.restype = _i
+    L.orc_laed4_mlp.restype = _i
     L.orc_stedc.restype = _i
     L.orc_laed2.restype = _i
     L.orc_laed3.restype = _i

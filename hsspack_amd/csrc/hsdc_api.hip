@@ -24,6 +24,8 @@ struct GTile { int m, half, r0, c0; };
 __global__ void k_leaf(const Leaf *, int, const Prob *, const double *, const double *, Work, int);
 __global__ void k_prep(const Merge *, const int *, int, const Prob *, Work, int);
 __global__ void k_sortmerge(const Merge *, Work);
+__global__ void k_mergepath(const Merge *, const int *, int, Work);
+__global__ void k_fill_u64(unsigned long long *, long, unsigned long long);
 __global__ void k_deflate(const Merge *, Work, int, MergeOut *, int);
 __global__ void k_rot_prep(const Merge *, const Prob *, Work, int, const MergeOut *);
 __global__ void k_rotate(const Merge *, const int *, int, const Prob *, Work, int, const MergeOut *);
@@ -123,7 +125,7 @@ struct hsdc_ctx {
     size_t wcap = 0;
     // learned secular start: weights (device + host copy), harvest buffer
     // (count | features | targets), secular control block of the running solve
-    DevBuf mlpw, hv, sec, mlptr;
+    DevBuf mlpw, hv, sec, mlptr, maux;
     std::vector<float> hw;
     long long hv_cap = 0;
     const SecCtl *dsec = nullptr;
@@ -231,6 +233,11 @@ static int alloc_work(hsdc_ctx *c, int ntot)
     W.supp_hi = I + 4 * n; W.indxq2 = I + 5 * n; W.indx = I + 6 * n; W.indxc = I + 7 * n;
     W.indxp = I + 8 * n; W.coltyp = I + 9 * n; W.rot_pj = I + 10 * n; W.rot_nj = I + 11 * n;
     W.org = I + 12 * n; W.nit = I + 13 * n; W.nsl = I + 14 * n; W.evq = I + 15 * n;
+    // per-merge records of one launch (a level has at most ntot / 2 merges), kept
+    // zero between launches by their consumer
+    CK(ensure(c->maux, sizeof(unsigned long long) * 4 * ((size_t)ntot / 2 + 2)));
+    HK(hipMemset(c->maux.p, 0, sizeof(unsigned long long) * 4 * ((size_t)ntot / 2 + 2)));
+    W.maux = (unsigned long long *)c->maux.p;
     c->wcap = ntot;
     return 0;
 }
@@ -724,6 +731,7 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
     MergeOut *dmo = (MergeOut *)c->mo.p;
     Work W = c->W;
     hipLaunchKernelGGL(k_prep, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur);
+    hipLaunchKernelGGL(k_mergepath, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, W);
     hipLaunchKernelGGL(k_sortmerge, dim3(nm), dim3(WGS), 0, s, dmg, W);
     hipLaunchKernelGGL(k_deflate, dim3(nm), dim3(WGS), 0, s, dmg, W, cur, dmo, c->prm.hss_min_k);
     // K of every merge back to the host
@@ -907,6 +915,10 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     Work W = c->W;
     int cur = 0;
     CK(sec_setup(c));
+    {
+        long nmx = 4 * ((long)c->wcap / 2 + 2);
+        hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(nmx, WGS)), dim3(WGS), 0, s, W.maux, nmx, 0ULL);
+    }
     hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(64), 0, s, dleaves, (int)nl, dprobs, dd, de, W, cur);
     HK(hipGetLastError());
     // the host tables above live in c->pin, which run_merges reuses: wait for the upload
@@ -1036,7 +1048,8 @@ int hsdc_ctx_destroy(hsdc_ctx *c)
     if (!c) return 0;
     DevBuf *bufs[] = {&c->q,  &c->ct,  &c->sb,   &c->din, &c->ein,  &c->wsd,  &c->wsi, &c->tabs,
                       &c->lvl, &c->mo, &c->probs, &c->misc, &c->zout, &c->dout, &c->hss, &c->htab,
-                      &c->xb, &c->gf,  &c->cb,   &c->yb,  &c->mlpw, &c->hv,   &c->sec, &c->mlptr};
+                      &c->xb, &c->gf,  &c->cb,   &c->yb,  &c->mlpw, &c->hv,   &c->sec, &c->mlptr,
+                      &c->maux};
     for (DevBuf *b : bufs) if (b->p) hipFree(b->p);
     if (c->pin.p) hipHostFree(c->pin.p);
     if (c->pin2.p) hipHostFree(c->pin2.p);
@@ -1190,6 +1203,8 @@ static int single_merge_setup(hsdc_ctx *c, int n, int n1, double rho, Merge **dm
     char *dp = (char *)c->probs.p;
     HK(hipMemcpy(dp, &M, sizeof(Merge), hipMemcpyHostToDevice));
     HK(hipMemcpy(dp + 64, ch, sizeof(ch), hipMemcpyHostToDevice));
+    unsigned long long zero[4] = {0, 0, 0, 0};
+    HK(hipMemcpy(c->W.maux, zero, sizeof(zero), hipMemcpyHostToDevice));
     *dm = (Merge *)dp;
     *dchunk = (int *)(dp + 64);
     return 0;
@@ -1214,6 +1229,8 @@ int hsdc_deflate(hsdc_ctx *c, int n, int n1, double *d, double *z, int *indxq, d
     HK(hipMemcpy(W.indxq, indxq, sizeof(int) * n, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_prep, dim3(cdiv(n, WGS)), dim3(WGS), 0, c->stream, (const Merge *)dm, (const int *)dch, 1,
                        (const Prob *)nullptr, W, 0);
+    hipLaunchKernelGGL(k_mergepath, dim3(cdiv(n, WGS)), dim3(WGS), 0, c->stream, (const Merge *)dm,
+                       (const int *)dch, 1, W);
     hipLaunchKernelGGL(k_sortmerge, dim3(1), dim3(WGS), 0, c->stream, (const Merge *)dm, W);
     hipLaunchKernelGGL(k_deflate, dim3(1), dim3(WGS), 0, c->stream, (const Merge *)dm, W, 0, dmo, hss_min_k);
     HK(hipGetLastError());

@@ -51,6 +51,7 @@ struct Work {
     double *rot_c, *rot_s;
     double *lam, *tau, *wnew, *alpha, *difl, *difr;
     int *org, *nit;
+    unsigned long long *maux;  // per merge of a launch: max|z|, max|D| (bit patterns), unsorted flag, pad
 };
 
 // control block of the secular kernel: learned-start weights, statistics and

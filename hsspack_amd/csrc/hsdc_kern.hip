@@ -159,24 +159,89 @@ __global__ __launch_bounds__(WG) void k_prep(const Merge *mg, const int *chunk_s
     int m = find_seg(chunk_start, nm, blockIdx.x);
     Merge M = mg[m];
     int i = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
-    if (i >= M.n) return;
-    int g = M.s + i;
-    double zz;
-    if (probs) {
-        Prob P = probs[M.prob];
-        int slot = W.vm[cur][g];
-        int row = (i < M.n1) ? (M.s + M.n1 - 1) : (M.s + M.n1);
-        zz = qload(P, W.supp_lo, W.supp_hi, row, slot);
-    } else {
-        zz = W.z[g];  // single-merge entry: z supplied by the caller
+    double az = 0.0, ad = 0.0;
+    if (i < M.n) {
+        int g = M.s + i;
+        double zz;
+        if (probs) {
+            Prob P = probs[M.prob];
+            int slot = W.vm[cur][g];
+            int row = (i < M.n1) ? (M.s + M.n1 - 1) : (M.s + M.n1);
+            zz = qload(P, W.supp_lo, W.supp_hi, row, slot);
+        } else {
+            zz = W.z[g];  // single-merge entry: z supplied by the caller
+        }
+        if (M.rho < 0.0 && i >= M.n1) zz = -1.0 * zz;
+        const double t = 1.0 / sqrt(2.0);
+        zz = t * zz;
+        W.z[g] = zz;
+        int q2 = W.indxq[g] + (i >= M.n1 ? M.n1 : 0);
+        W.indxq2[g] = q2;
+        const double v = W.Dv[cur][M.s + q2 - 1];
+        W.dlamda[g] = v;
+        // DLAMRG's runs must be ascending for the merge path; flag the merge otherwise
+        if (i + 1 < M.n && i + 1 != M.n1) {
+            int q2n = W.indxq[g + 1] + (i + 1 >= M.n1 ? M.n1 : 0);
+            if (v > W.Dv[cur][M.s + q2n - 1]) W.maux[4 * m + 2] = 1ULL;
+        }
+        az = fabs(zz);
+        ad = fabs(v);
     }
-    if (M.rho < 0.0 && i >= M.n1) zz = -1.0 * zz;
-    const double t = 1.0 / sqrt(2.0);
-    zz = t * zz;
-    W.z[g] = zz;
-    int q2 = W.indxq[g] + (i >= M.n1 ? M.n1 : 0);
-    W.indxq2[g] = q2;
-    W.dlamda[g] = W.Dv[cur][M.s + q2 - 1];
+    // max |z| and max |D| of the merge (the deflation tolerance, mdlaed2.f90:264-266);
+    // non-negative doubles order as their bit patterns
+    for (int off = 32; off > 0; off >>= 1) {
+        az = fmax(az, __shfl_xor(az, off));
+        ad = fmax(ad, __shfl_xor(ad, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&W.maux[4 * m + 0], (unsigned long long)__double_as_longlong(az));
+        atomicMax(&W.maux[4 * m + 1], (unsigned long long)__double_as_longlong(ad));
+    }
+}
+
+__device__ __forceinline__ double maux_tol(const Work &W, int m)
+{
+    double zmax = __longlong_as_double((long long)W.maux[4 * m + 0]);
+    double dmax = __longlong_as_double((long long)W.maux[4 * m + 1]);
+    return 8.0 * HSDC_EPS * (dmax > zmax ? dmax : zmax);
+}
+
+// The merged order (DLAMRG(N1, N2, DLAMDA, 1, 1, INDXC), mdlaed2.f90:252-255) of every
+// merge with ascending runs, one thread per element over all merges: each element's
+// rank in the other run by binary search (ties to the first run).  Also the sorted
+// gathers of the deflation scan: INDX = INDXQ(INDXC), z and D in that order, and the
+// small-z column type (mdlaed2.f90:264-300).
+__global__ __launch_bounds__(WG) void k_mergepath(const Merge *mg, const int *chunk_start, int nm, Work W)
+{
+    int m = find_seg(chunk_start, nm, blockIdx.x);
+    if (W.maux[4 * m + 2]) return;  // k_sortmerge merges it sequentially
+    Merge M = mg[m];
+    int i = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    if (i >= M.n) return;
+    const double *a = W.dlamda + M.s;
+    const int n1 = M.n1, n2 = M.n - M.n1;
+    const double v = a[i];
+    int out, ic;
+    if (i < n1) {
+        int lo = 0, hi = n2;  // run-2 values strictly below v
+        while (lo < hi) { int md = (lo + hi) >> 1; if (a[n1 + md] < v) lo = md + 1; else hi = md; }
+        out = i + lo;
+        ic = i + 1;
+    } else {
+        int q = i - n1, lo = 0, hi = n1;  // run-1 values <= v
+        while (lo < hi) { int md = (lo + hi) >> 1; if (a[md] <= v) lo = md + 1; else hi = md; }
+        out = q + lo;
+        ic = n1 + q + 1;
+    }
+    const int s = M.s;
+    W.indxc[s + out] = ic;
+    const int nj = W.indxq2[s + ic - 1];
+    W.indx[s + out] = nj;
+    const double zz = W.z[s + nj - 1];
+    W.zs[s + out] = zz;
+    W.ds[s + out] = v;  // D(INDX(out)) = DLAMDA(INDXC(out))
+    const double rho = fabs(2.0 * M.rho), tol = maux_tol(W, m);
+    W.coltyp[s + nj - 1] = (rho * fabs(zz) <= tol) ? 4 : ((nj <= n1) ? 1 : 3);
 }
 
 // Stable merge of two sorted runs (LAPACK DLAMRG semantics, ties to the first
@@ -224,13 +289,23 @@ __device__ void block_merge(const double *a, int n1, int n2, int s1, int s2, int
     __syncthreads();
 }
 
-// DLAMRG(N1,N2,DLAMDA,1,1,INDXC) and INDX = INDXQ(INDXC) (mdlaed2.f90:252-255)
+// The merges k_mergepath leaves (a run out of order): DLAMRG(N1,N2,DLAMDA,1,1,INDXC)
+// by one block, then the same gathers.
 __global__ __launch_bounds__(WG) void k_sortmerge(const Merge *mg, Work W)
 {
     __shared__ int sflag;
+    if (!W.maux[4 * blockIdx.x + 2]) return;
     Merge M = mg[blockIdx.x];
     block_merge(W.dlamda + M.s, M.n1, M.n - M.n1, 1, 1, W.indxc + M.s, &sflag);
-    for (int i = threadIdx.x; i < M.n; i += WG) W.indx[M.s + i] = W.indxq2[M.s + W.indxc[M.s + i] - 1];
+    const double rho = fabs(2.0 * M.rho), tol = maux_tol(W, blockIdx.x);
+    for (int i = threadIdx.x; i < M.n; i += WG) {
+        int nj = W.indxq2[M.s + W.indxc[M.s + i] - 1];
+        W.indx[M.s + i] = nj;
+        double zz = W.z[M.s + nj - 1];
+        W.zs[M.s + i] = zz;
+        W.ds[M.s + i] = W.dlamda[M.s + W.indxc[M.s + i] - 1];
+        W.coltyp[M.s + nj - 1] = (rho * fabs(zz) <= tol) ? 4 : ((nj <= M.n1) ? 1 : 3);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -249,28 +324,10 @@ __global__ __launch_bounds__(WG) void k_sortmerge(const Merge *mg, Work W)
 // ---------------------------------------------------------------------------
 #define DCH 1024
 #define MAXSEG 1536
-__device__ int block_scan_excl(int v, int *tmp, int *total)
-{
-    int tid = threadIdx.x;
-    tmp[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < WG; off <<= 1) {
-        int t = (tid >= off) ? tmp[tid - off] : 0;
-        __syncthreads();
-        tmp[tid] += t;
-        __syncthreads();
-    }
-    int incl = tmp[tid];
-    if (total) *total = tmp[WG - 1];
-    __syncthreads();
-    return incl - v;
-}
-
 __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur, MergeOut *mo,
                                                 int hss_min_k)
 {
-    __shared__ double red_z[WG], red_d[WG];
-    __shared__ int sk, sk2, scnt[4], tcnt[4][WG], itmp[WG], sL, sfirst, sfallback, snseg, stot;
+    __shared__ int sk, sk2, scnt[4], wcnt[4][4], sL, sfirst, sfallback, snseg;
     __shared__ int seg_a[MAXSEG], seg_b[MAXSEG], seg_nx[MAXSEG], seg_pos[MAXSEG], seg_ord[MAXSEG];
     __shared__ double seg_d[MAXSEG];
     __shared__ double cz[DCH], cd[DCH];
@@ -282,44 +339,59 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
     int *indx = W.indx + s, *indxp = W.indxp + s, *coltyp = W.coltyp + s;
     double *zs = W.zs + s, *ds = W.ds + s, *ddef = W.ddef + s;
     int *nsl = W.nsl + s;
-    double mz = 0.0, md = 0.0;
-    for (int i = tid; i < n; i += WG) { mz = fmax(mz, fabs(z[i])); md = fmax(md, fabs(D[i])); }
-    red_z[tid] = mz; red_d[tid] = md;
-    __syncthreads();
-    for (int w = WG / 2; w > 0; w >>= 1) {
-        if (tid < w) { red_z[tid] = fmax(red_z[tid], red_z[tid + w]); red_d[tid] = fmax(red_d[tid], red_d[tid + w]); }
-        __syncthreads();
-    }
+    (void)z;
+    (void)D;
     const double rho = fabs(2.0 * M.rho);
-    const double tol = 8.0 * HSDC_EPS * (red_d[0] > red_z[0] ? red_d[0] : red_z[0]);
+    const double zmax = __longlong_as_double((long long)W.maux[4 * blockIdx.x + 0]);
+    const double tol = maux_tol(W, blockIdx.x);
     MergeOut *o = mo + blockIdx.x;
-    if (rho * red_z[0] <= tol) {
+    __syncthreads();
+    if (tid == 0) {
+        // the per-merge record is consumed: clear it for the next launch
+        W.maux[4 * blockIdx.x + 0] = W.maux[4 * blockIdx.x + 1] = W.maux[4 * blockIdx.x + 2] = 0ULL;
+    }
+    if (rho * zmax <= tol) {
         if (tid == 0) {
             o->k = 0; o->ctot[0] = o->ctot[1] = o->ctot[2] = o->ctot[3] = 0;
             o->hss = 0; o->nrot = 0; o->flags = 1; o->rho = rho; o->tol = tol;
         }
         return;
     }
-    const int per = (n + WG - 1) / WG;
-    const int b0 = min(n, tid * per), b1 = min(n, b0 + per);
-    // ---- 1. sorted gather, small-z screen, survivor list
-    int cnt = 0;
-    for (int j = b0; j < b1; j++) {
-        int nj = indx[j];
-        double zz = z[nj - 1], dd = D[nj - 1];
-        zs[j] = zz; ds[j] = dd;
-        coltyp[nj - 1] = (rho * fabs(zz) <= tol) ? 4 : ((nj <= n1) ? 1 : 3);
-        cnt += (rho * fabs(zz) > tol);
-    }
-    int L;
-    int base = block_scan_excl(cnt, itmp, &L);
-    for (int j = b0; j < b1; j++)
-        if (rho * fabs(zs[j]) > tol) nsl[base++] = j;
-    if (tid == 0) { sL = L; sfirst = L - 1; sfallback = 0; }
+    // Wavefront w owns sorted positions [w0, w1); the passes below walk them 64 at a
+    // time and compact with ballots, which keeps every list in ascending order.
+    const int lane = tid & 63, wv = tid >> 6;
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
+    const int wper = (n + 3) / 4;
+    const int w0 = min(n, wv * wper), w1 = min(n, w0 + wper);
+    // ---- 1. survivors of the small-z screen (zs, ds, INDX and the small-z column
+    // types come from the merge kernels); DLAMDA must be ascending in scan order for
+    // the run replay below
+    if (tid == 0) sfallback = 0;
     __syncthreads();
-    // the run replay below relies on DLAMDA being sorted in scan order
-    for (int j = b0; j < b1; j++)
-        if (j + 1 < n && ds[j] > ds[j + 1]) sfallback = 1;
+    int cnt = 0, bad = 0;
+    for (int j0 = w0; j0 < w1; j0 += 64) {
+        int j = j0 + lane;
+        bool sv = false;
+        if (j < w1) {
+            sv = rho * fabs(zs[j]) > tol;
+            if (j + 1 < n && ds[j] > ds[j + 1]) bad = 1;
+        }
+        cnt += __popcll(__ballot(sv));
+    }
+    if (bad) sfallback = 1;
+    if (lane == 0) wcnt[0][wv] = cnt;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wv; w++) base += wcnt[0][w];
+    const int L = wcnt[0][0] + wcnt[0][1] + wcnt[0][2] + wcnt[0][3];
+    for (int j0 = w0; j0 < w1; j0 += 64) {
+        int j = j0 + lane;
+        bool sv = (j < w1) && rho * fabs(zs[j]) > tol;
+        unsigned long long msk = __ballot(sv);
+        if (sv) nsl[base + __popcll(msk & lt)] = j;
+        base += __popcll(msk);
+    }
+    if (tid == 0) { sL = L; sfirst = L - 1; }
     __syncthreads();
     // ---- 2. speculative close-pair test on unrotated values
     int myfirst = L - 1;
@@ -339,37 +411,57 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
         int p = nsl[q];
         W.dlamda[s + q] = ds[p]; W.w[s + q] = zs[p]; indxp[q] = indx[p];
     }
-    // ---- 3. sequential continuation over survivors
+    // ---- 3. sequential continuation over survivors, staged through LDS in chunks
+    int K = first, nrot = 0, PJ = 0;
+    double zP = 0.0, DP = 0.0;
     if (tid == 0 && !sfallback) {
-        int K = first, nrot = 0;
         int p0 = nsl[first];
-        int PJ = indx[p0];
-        double zP = zs[p0], DP = ds[p0];
-        for (int q = first + 1; q < L; q++) {
-            int r = nsl[q];
-            int NJ = indx[r];
-            double zN = zs[r], dN = ds[r];
-            double ss = zP, cc = zN;
-            double tau = lapy2_d(cc, ss);
-            double t = dN - DP;
-            cc = cc / tau;
-            ss = -ss / tau;
-            if (fabs(t * cc * ss) <= tol) {
-                if (coltyp[NJ - 1] != coltyp[PJ - 1]) coltyp[NJ - 1] = 2;
-                coltyp[PJ - 1] = 4;
-                double tP = DP * (cc * cc) + dN * (ss * ss);
-                double dNn = DP * (ss * ss) + dN * (cc * cc);
-                W.rot_pj[s + nrot] = PJ; W.rot_nj[s + nrot] = NJ;
-                W.rot_c[s + nrot] = cc; W.rot_s[s + nrot] = ss;
-                W.evq[s + nrot] = q; W.evd[s + nrot] = tP;
-                nrot++;
-                PJ = NJ; zP = tau; DP = dNn;
-            } else {
-                W.dlamda[s + K] = DP; W.w[s + K] = zP; indxp[K] = PJ;
-                K++;
-                PJ = NJ; zP = zN; DP = dN;
+        PJ = indx[p0];
+        zP = zs[p0];
+        DP = ds[p0];
+    }
+    if (!sfallback) {
+        for (int c0 = first + 1; c0 < L; c0 += DCH) {
+            const int cl = min(DCH, L - c0);
+            __syncthreads();
+            for (int u = tid; u < cl; u += WG) {
+                int r = nsl[c0 + u];
+                ci[u] = indx[r];
+                cz[u] = zs[r];
+                cd[u] = ds[r];
+            }
+            __syncthreads();
+            if (tid == 0) {
+                for (int u = 0; u < cl; u++) {
+                    const int q = c0 + u;
+                    int NJ = ci[u];
+                    double zN = cz[u], dN = cd[u];
+                    double ss = zP, cc = zN;
+                    double tau = lapy2_d(cc, ss);
+                    double t = dN - DP;
+                    cc = cc / tau;
+                    ss = -ss / tau;
+                    if (fabs(t * cc * ss) <= tol) {
+                        if (coltyp[NJ - 1] != coltyp[PJ - 1]) coltyp[NJ - 1] = 2;
+                        coltyp[PJ - 1] = 4;
+                        double tP = DP * (cc * cc) + dN * (ss * ss);
+                        double dNn = DP * (ss * ss) + dN * (cc * cc);
+                        W.rot_pj[s + nrot] = PJ; W.rot_nj[s + nrot] = NJ;
+                        W.rot_c[s + nrot] = cc; W.rot_s[s + nrot] = ss;
+                        W.evq[s + nrot] = q; W.evd[s + nrot] = tP;
+                        nrot++;
+                        PJ = NJ; zP = tau; DP = dNn;
+                    } else {
+                        W.dlamda[s + K] = DP; W.w[s + K] = zP; indxp[K] = PJ;
+                        K++;
+                        PJ = NJ; zP = zN; DP = dN;
+                    }
+                }
             }
         }
+        __syncthreads();
+    }
+    if (tid == 0 && !sfallback) {
         W.dlamda[s + K] = DP; W.w[s + K] = zP; indxp[K] = PJ;
         K++;
         sk = K; sk2 = nrot;
@@ -531,31 +623,38 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
     __syncthreads();
     // CTOT and the type grouping (mdlaed2.f90:393-431) as a stable partition
     int c4[4] = {0, 0, 0, 0};
-    for (int j = b0; j < b1; j++) c4[coltyp[indxp[j] - 1] - 1]++;
-    for (int t = 0; t < 4; t++) tcnt[t][tid] = c4[t];
-    __syncthreads();
-    if (tid < 4) {
-        int acc = 0;
-        for (int q = 0; q < WG; q++) { int v = tcnt[tid][q]; tcnt[tid][q] = acc; acc += v; }
-        scnt[tid] = acc;
+    for (int j0 = w0; j0 < w1; j0 += 64) {
+        int j = j0 + lane;
+        int ct = (j < w1) ? coltyp[indxp[j] - 1] - 1 : -1;
+        for (int t = 0; t < 4; t++) c4[t] += __popcll(__ballot(ct == t));
     }
+    if (lane == 0)
+        for (int t = 0; t < 4; t++) wcnt[t][wv] = c4[t];
+    __syncthreads();
+    if (tid < 4) scnt[tid] = wcnt[tid][0] + wcnt[tid][1] + wcnt[tid][2] + wcnt[tid][3];
     __syncthreads();
     const int KF = n - scnt[3];
     const int hss = (KF >= hss_min_k);
     if (!hss) {
-        int psm[4];
-        psm[0] = 0; psm[1] = scnt[0]; psm[2] = psm[1] + scnt[1]; psm[3] = psm[2] + scnt[2];
         int pos[4];
-        for (int t = 0; t < 4; t++) pos[t] = psm[t] + tcnt[t][tid];
-        for (int j = b0; j < b1; j++) {
-            int js = indxp[j];
-            int ct = coltyp[js - 1] - 1;
-            indx[pos[ct]] = js;
-            W.indxc[s + pos[ct]] = j + 1;
-            pos[ct]++;
+        pos[0] = 0; pos[1] = scnt[0]; pos[2] = pos[1] + scnt[1]; pos[3] = pos[2] + scnt[2];
+        for (int t = 0; t < 4; t++)
+            for (int w = 0; w < wv; w++) pos[t] += wcnt[t][w];
+        for (int j0 = w0; j0 < w1; j0 += 64) {
+            int j = j0 + lane;
+            int js = (j < w1) ? indxp[j] : 0;
+            int ct = (j < w1) ? coltyp[js - 1] - 1 : -1;
+            unsigned long long mk[4];
+            for (int t = 0; t < 4; t++) mk[t] = __ballot(ct == t);
+            if (ct >= 0) {
+                int p = pos[ct] + __popcll(mk[ct] & lt);
+                indx[p] = js;
+                W.indxc[s + p] = j + 1;
+            }
+            for (int t = 0; t < 4; t++) pos[t] += __popcll(mk[t]);
         }
     } else {
-        for (int j = b0; j < b1; j++) indx[j] = indxp[j];
+        for (int j = tid; j < n; j += WG) indx[j] = indxp[j];
     }
     if (tid == 0) {
         o->k = KF;
@@ -1142,4 +1241,10 @@ __global__ __launch_bounds__(WG) void k_dgemm_nn(int m, int n, int k, const doub
                 int col = c0 + wc + b * 16 + (lane & 15);
                 if (row < m && col < n) C[(long)row + (long)col * ldc] = acc[a][b][i];
             }
+}
+
+__global__ __launch_bounds__(WG) void k_fill_u64(unsigned long long *p, long n, unsigned long long v)
+{
+    long i = (long)blockIdx.x * WG + threadIdx.x;
+    if (i < n) p[i] = v;
 }

@@ -82,6 +82,15 @@ def _merge_cases():
                          ("big", tridiag_cfg(2, 3000, 14)), ("bigexact", (np.full(2500, 2.0), np.full(2499, 1.0)))]:
         n = len(d)
         out.append((name, merge_inputs(d, e[: n - 1], n // 2), n // 2))
+    # runs out of order (INDXQ not sorting the halves): the sequential DLAMRG path and
+    # the plain deflation scan
+    for name, (d, e) in [("unsorted", tridiag_cfg(2, 96, 15)), ("unsorted_toep", tridiag_cfg(1, 300, 16))]:
+        n = len(d)
+        dd, q, indxq, rho, z = merge_inputs(d, e[: n - 1], n // 2)
+        indxq = indxq.copy()
+        indxq[[3, 10]] = indxq[[10, 3]]
+        indxq[[n // 2 + 2, n // 2 + 7]] = indxq[[n // 2 + 7, n // 2 + 2]]
+        out.append((name, (dd, q, indxq, rho, z), n // 2))
     return out
 
 

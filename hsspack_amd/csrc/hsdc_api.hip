@@ -1471,6 +1471,48 @@ int hsdc_hss_dense(hsdc_ctx *c, double *a, int lda)
     return 0;
 }
 
+// MDLAED0 (mdlaed0.f90:121-380) on host arrays: icompq 0 eigenvalues only, 1 Q(qsiz x n) :=
+// Q * V, 2 Q := V.  LDQS and the WORK/IWORK arrays of the Fortran routine are not needed.
+int hsdc_laed0(hsdc_ctx *c, int icompq, int qsiz, int n, double *d, double *e, double *q, int ldq)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (icompq < 0 || icompq > 2) return -1;
+    if (icompq == 1 && qsiz < std::max(0, n)) return -2;
+    if (n < 0) return -3;
+    if (ldq < std::max(1, n)) return -7;
+    if (icompq == 1 && ldq < qsiz) return -7;
+    if (n == 0) return 0;
+    if (n == 1) {
+        if (icompq == 2) q[0] = 1.0;
+        return 0;
+    }
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    CK(ensure(c->din, sizeof(double) * 2 * n));
+    double *dd = (double *)c->din.p, *de = dd + n;
+    CK(ensure(c->zout, sizeof(double) * (size_t)n * n));
+    double *dz = (double *)c->zout.p;
+    HK(hipMemcpy(dd, d, sizeof(double) * n, hipMemcpyHostToDevice));
+    HK(hipMemcpy(de, e, sizeof(double) * (n - 1), hipMemcpyHostToDevice));
+    int info = hsdc_stedc_dev(c, n, dd, de, dz, n);
+    if (info) return info;
+    HK(hipMemcpy(d, dd, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (icompq == 2) {
+        HK(hipMemcpy2D(q, sizeof(double) * ldq, dz, sizeof(double) * n, sizeof(double) * n, n,
+                       hipMemcpyDeviceToHost));
+    } else if (icompq == 1) {
+        CK(ensure(c->misc, sizeof(double) * 2 * (size_t)qsiz * n));
+        double *qa = (double *)c->misc.p, *qc = qa + (size_t)qsiz * n;
+        HK(hipMemcpy2D(qa, sizeof(double) * qsiz, q, sizeof(double) * ldq, sizeof(double) * qsiz, n,
+                       hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_dgemm_nn, dim3(cdiv(qsiz, 64), cdiv(n, 64)), dim3(WGS), 0, c->stream, qsiz, n, n,
+                           (const double *)qa, (long)qsiz, (const double *)dz, (long)n, qc, (long)qsiz);
+        HK(hipGetLastError());
+        HK(hipMemcpy2D(q, sizeof(double) * ldq, qc, sizeof(double) * qsiz, sizeof(double) * qsiz, n,
+                       hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
 int hsdc_stedc(hsdc_ctx *c, char compz, int n, double *d, double *e, double *z, int ldz, double *work,
                int lwork, int *iwork, int liwork)
 {

@@ -27,7 +27,7 @@ HSDC_ERR_NODEVICE = -1002
 # every symbol include/hsdc.h declares
 EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_set_params",
            "hsdc_get_stats", "hsdc_set_stream", "hsdc_stedc", "hsdc_stedc_dev",
-           "hsdc_stedc_batched_dev", "hsdc_deflate", "hsdc_secular", "hsdc_laed3", "hsdc_loewner",
+           "hsdc_stedc_batched_dev", "hsdc_laed0", "hsdc_deflate", "hsdc_secular", "hsdc_laed3", "hsdc_loewner",
            "hsdc_hss_evc", "hsdc_hss_dense", "hsdc_mlp_set_weights", "hsdc_mlp_get_weights",
            "hsdc_mlp_load", "hsdc_mlp_harvest", "hsdc_mlp_samples", "hsdc_mlp_grad", "hsdc_mlp_train"]
 
@@ -68,6 +68,7 @@ def lib():
         L.hsdc_stedc.argtypes = [_vp, ctypes.c_char, _i, _P(_d), _P(_d), _P(_d), _i, _P(_d), _i,
                                  _P(_i), _i]
         L.hsdc_stedc_dev.argtypes = [_vp, _i, _vp, _vp, _vp, _i]
+        L.hsdc_laed0.argtypes = [_vp, _i, _i, _i, _P(_d), _P(_d), _P(_d), _i]
         L.hsdc_stedc_batched_dev.argtypes = [_vp, _i, _P(_i), _P(ctypes.c_long), _vp, _vp, _vp,
                                              _P(ctypes.c_long)]
         L.hsdc_deflate.argtypes = [_vp, _i, _i, _P(_d), _P(_d), _P(_i), _d, _i, _P(_i), _P(_d),

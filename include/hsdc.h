@@ -71,6 +71,11 @@ int  hsdc_set_stream(hsdc_ctx *ctx, void *stream);
 int hsdc_stedc(hsdc_ctx *ctx, char compz, int n, double *d, double *e, double *z, int ldz,
                double *work, int lwork, int *iwork, int liwork);
 
+/* MDLAED0 (mdlaed0.f90:121): DC of one unreduced block.  icompq 0: eigenvalues only;
+ * 1: q(qsiz, n) := q * V; 2: q := V (the eigenvectors).  d(n), e(n-1), q(ldq, n) host
+ * arrays; the routine's QSTORE/WORK/IWORK are not needed.  INFO as DLAED0 (-1, -2, -3, -7). */
+int hsdc_laed0(hsdc_ctx *ctx, int icompq, int qsiz, int n, double *d, double *e, double *q, int ldq);
+
 /* Device-resident MDSTEDC with COMPZ = 'I': d_d(n), d_e(n-1), d_z(ldz,n) device
  * arrays; eigenvalues overwrite d_d, e is destroyed. */
 int hsdc_stedc_dev(hsdc_ctx *ctx, int n, double *d_d, double *d_e, double *d_z, int ldz);

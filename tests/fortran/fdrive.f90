@@ -1,0 +1,100 @@
+! Test driver of the Fortran drop-ins (hsspack_amd/fortran/hsdc_mdstedc.f90), the way a
+! program using HSSPACK calls them.  usage: fdrive MODE IN OUT
+!   stedc  IN: int32 n, char compz (int32), float64 d(n), e(n-1)  OUT: int32 info,
+!          int32 lwmin, int32 liwmin, d(n), z(n,n)
+!   laed0  IN: int32 icompq, qsiz, n, d(n), e(n-1), q(qsiz,n)    OUT: info, d(n), q(qsiz,n)
+!   laed1  IN: int32 n, cutpnt, float64 rho, d(n), q(n,n), int32 indxq(n)
+!          OUT: info, wflag, d(n), q(n,n), indxq(n)
+!   errors OUT: for each illegal call, info and the XERBLA (name, argument) it made
+module xerbla_log
+   implicit none
+   character(len=6) :: last_name = '      '
+   integer :: last_arg = 0
+end module
+
+subroutine xerbla(srname, info)
+   use xerbla_log
+   character(len=*) :: srname
+   integer :: info
+   last_name = srname
+   last_arg = info
+end subroutine
+
+program fdrive
+   use xerbla_log
+   implicit none
+   character(len=16) :: mode
+   character(len=512) :: fin, fout
+   integer :: n, icompq, qsiz, cutpnt, info, wflag, lwork, liwork, cz, ldq
+   double precision :: rho
+   double precision, allocatable :: d(:), e(:), q(:, :), work(:), qs(:, :)
+   integer, allocatable :: iwork(:), indxq(:)
+   character :: compz
+   integer :: u, v
+   call get_command_argument(1, mode)
+   call get_command_argument(2, fin)
+   call get_command_argument(3, fout)
+   open(newunit=v, file=trim(fout), access='stream', form='unformatted', status='replace')
+   select case (trim(mode))
+   case ('stedc')
+      open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+      read(u) n, cz
+      compz = achar(cz)
+      allocate(d(n), e(max(n - 1, 1)), q(n, n))
+      read(u) d
+      if (n > 1) read(u) e(1:n - 1)
+      if (compz == 'V' .or. compz == 'v') read(u) q
+      close(u)
+      allocate(work(1), iwork(1))
+      call mdstedc(compz, n, d, e, q, n, work, -1, iwork, -1, info)
+      lwork = int(work(1))
+      liwork = iwork(1)
+      deallocate(work, iwork)
+      allocate(work(lwork), iwork(liwork))
+      call mdstedc(compz, n, d, e, q, n, work, lwork, iwork, liwork, info)
+      write(v) info, lwork, liwork, d, q
+   case ('laed0')
+      open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+      read(u) icompq, qsiz, n
+      ldq = max(qsiz, n)
+      allocate(d(n), e(max(n - 1, 1)), q(ldq, n), qs(n, n), work(1), iwork(1))
+      read(u) d
+      if (n > 1) read(u) e(1:n - 1)
+      if (icompq == 1) read(u) q(1:qsiz, 1:n)
+      close(u)
+      call mdlaed0(icompq, qsiz, n, d, e, q, ldq, qs, n, work, iwork, info, wflag)
+      if (icompq == 1) then
+         write(v) info, d, q(1:qsiz, 1:n)
+      else
+         write(v) info, d, q(1:n, 1:n)
+      end if
+   case ('laed1')
+      open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+      read(u) n, cutpnt, rho
+      allocate(d(n), q(n, n), indxq(n), work(4 * n + n * n + 3 * n + 16), iwork(4 * n))
+      read(u) d, q, indxq
+      close(u)
+      wflag = 0
+      call mdlaed1(n, d, q, n, indxq, rho, cutpnt, work, iwork, info, wflag)
+      write(v) info, wflag, d, q, indxq
+   case ('errors')
+      allocate(d(8), e(8), q(8, 8), work(200), iwork(200), indxq(8), qs(8, 8))
+      d = 1
+      e = 0
+      call mdstedc('X', 8, d, e, q, 8, work, 200, iwork, 200, info)
+      write(v) info, last_arg, last_name
+      call mdstedc('I', 8, d, e, q, 4, work, 200, iwork, 200, info)
+      write(v) info, last_arg, last_name
+      call mdstedc('I', 8, d, e, q, 8, work, 10, iwork, 200, info)
+      write(v) info, last_arg, last_name
+      call mdlaed0(3, 8, 8, d, e, q, 8, qs, 8, work, iwork, info, wflag)
+      write(v) info, last_arg, last_name
+      call mdlaed0(2, 8, 8, d, e, q, 8, qs, 4, work, iwork, info, wflag)
+      write(v) info, last_arg, last_name
+      call mdlaed1(8, d, q, 8, indxq, rho, 5, work, iwork, info, wflag)
+      write(v) info, last_arg, last_name
+      call mdlaed3(9, 8, d, q, 8, rho, d, q, d, d, d, d, work, info)
+      write(v) info, last_arg, last_name
+   end select
+   close(v)
+end program

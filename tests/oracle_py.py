@@ -114,6 +114,23 @@ def mlp_features(d, z, rho, r, W, xc):
     return f
 
 
+def laed1(d, q, indxq, rho, cutpnt, prm=None):
+    """MDLAED1 on copies: returns (info, d, q, indxq, k, hss)."""
+    n = len(d)
+    d = np.array(d, np.float64)
+    q = np.array(q, np.float64, order="F")
+    indxq = np.array(indxq, np.int32)
+    work = np.zeros(4 * n + n * n + 16)
+    iwork = np.zeros(4 * n + 16, np.int32)
+    prm = prm or params()
+    k, hss, rmax = _i(0), _i(0), _i(0)
+    it = ctypes.c_longlong(0)
+    info = lib().orc_laed1(n, dp(d), dp(q), n, ip(indxq), _d(rho), int(cutpnt), dp(work), ip(iwork),
+                           ctypes.byref(prm), ctypes.byref(k), ctypes.byref(hss), ctypes.byref(rmax),
+                           ctypes.byref(it))
+    return info, d, q, indxq, k.value, hss.value
+
+
 def dlamrg(a, n1, n2, s1, s2):
     a = np.ascontiguousarray(a, np.float64)
     idx = np.zeros(n1 + n2, np.int32)

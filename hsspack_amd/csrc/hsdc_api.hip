@@ -31,6 +31,7 @@ __global__ void k_rot_prep(const Merge *, const Prob *, Work, int, const MergeOu
 __global__ void k_rotate(const Merge *, const int *, int, const Prob *, Work, int, const MergeOut *);
 __global__ void k_secular(const Merge *, const int *, int, const MergeOut *, Work, int, int *, const SecCtl *,
                           unsigned int *);
+__global__ void k_secstart(const Merge *, const int *, int, const MergeOut *, Work, const SecCtl *);
 __global__ void k_loewner(const Merge *, const int *, int, const MergeOut *, Work);
 __global__ void k_sbuild(const Merge *, const int *, int, const MergeOut *, const long *, Work, double *);
 __global__ void k_alpha(const Merge *, const int *, int, const MergeOut *, Work);
@@ -131,6 +132,8 @@ struct hsdc_ctx {
     const SecCtl *dsec = nullptr;
     unsigned int *dlvlmax = nullptr;  // largest iteration count of each level's secular launch
     int cur_level = 0;
+    bool sec_pre = false;              // k_secstart runs before k_secular
+    DevBuf featb;
 };
 
 #define SEC_LEVELS 128
@@ -218,7 +221,7 @@ static int alloc_work(hsdc_ctx *c, int ntot)
 {
     if (!have_device()) return HSDC_ERR_NODEVICE;
     if ((size_t)ntot <= c->wcap && c->wcap) return 0;
-    size_t nd = 18, ni = 18;
+    size_t nd = 22, ni = 19;
     CK(ensure(c->wsd, sizeof(double) * nd * (size_t)ntot));
     CK(ensure(c->wsi, sizeof(int) * ni * (size_t)ntot));
     double *D = (double *)c->wsd.p;
@@ -233,6 +236,8 @@ static int alloc_work(hsdc_ctx *c, int ntot)
     W.supp_hi = I + 4 * n; W.indxq2 = I + 5 * n; W.indx = I + 6 * n; W.indxc = I + 7 * n;
     W.indxp = I + 8 * n; W.coltyp = I + 9 * n; W.rot_pj = I + 10 * n; W.rot_nj = I + 11 * n;
     W.org = I + 12 * n; W.nit = I + 13 * n; W.nsl = I + 14 * n; W.evq = I + 15 * n;
+    W.st_t = D + 18 * n; W.st_tc = D + 19 * n; W.st_lo = D + 20 * n; W.st_hi = D + 21 * n;
+    W.st_ii = I + 18 * n;
     // per-merge records of one launch (a level has at most ntot / 2 merges), kept
     // zero between launches by their consumer
     CK(ensure(c->maux, sizeof(unsigned long long) * 4 * ((size_t)ntot / 2 + 2)));
@@ -261,6 +266,8 @@ static int sec_setup(hsdc_ctx *c)
         h.C.hf = (float *)(hv + 64);
         h.C.ht = (float *)(hv + 64 + al64(sizeof(float) * MLP_IN * (size_t)c->hv_cap));
         h.C.cap = (unsigned long long)c->hv_cap;
+        CK(ensure(c->featb, sizeof(float) * MLP_IN * std::max(c->wcap, (size_t)1)));
+        h.C.feat = (float *)c->featb.p;
     }
     h.C.kmin = c->prm.learned_min_k;
     h.C.gate = (float)c->prm.learned_gate;
@@ -272,6 +279,7 @@ static int sec_setup(hsdc_ctx *c)
     c->dsec = (const SecCtl *)base;
     c->dlvlmax = (unsigned int *)(base + hdr + 64);
     c->cur_level = 0;
+    c->sec_pre = (h.C.w != nullptr) || (h.C.hf != nullptr);
     return 0;
 }
 
@@ -752,7 +760,7 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
         const Merge &M = hm[m0 + i];
         int k = o.flags ? 0 : o.k;
         chunkK[i + 1] = chunkK[i] + cdiv(k, WGS);
-        blockK[i + 1] = blockK[i] + cdiv(k, 64);
+        blockK[i + 1] = blockK[i] + (k > 0 ? cdiv(k, SEC_WG / sec_team(k)) : 0);
         colK[i + 1] = colK[i] + k;
         anyrot |= (o.nrot > 0 && !o.flags);
         anyk |= (k > 0);
@@ -798,7 +806,10 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
         hipLaunchKernelGGL(k_rotate, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur, dmo);
     }
     if (anyk) {
-        hipLaunchKernelGGL(k_secular, dim3(blockK[nm]), dim3(64), 0, s, dmg, dblockK, nm, dmo, W,
+        if (c->sec_pre)
+            hipLaunchKernelGGL(k_secstart, dim3(blockK[nm]), dim3(SEC_WG), 0, s, dmg, dblockK, nm,
+                               (const MergeOut *)dmo, W, c->dsec);
+        hipLaunchKernelGGL(k_secular, dim3(blockK[nm]), dim3(SEC_WG), 0, s, dmg, dblockK, nm, dmo, W,
                            c->prm.maxit, dinfo, c->dsec, c->dlvlmax + std::min(c->cur_level, SEC_LEVELS - 1));
         hipLaunchKernelGGL(k_loewner, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, W);
         hipLaunchKernelGGL(k_sbuild, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, dsoff, W,
@@ -1049,7 +1060,7 @@ int hsdc_ctx_destroy(hsdc_ctx *c)
     DevBuf *bufs[] = {&c->q,  &c->ct,  &c->sb,   &c->din, &c->ein,  &c->wsd,  &c->wsi, &c->tabs,
                       &c->lvl, &c->mo, &c->probs, &c->misc, &c->zout, &c->dout, &c->hss, &c->htab,
                       &c->xb, &c->gf,  &c->cb,   &c->yb,  &c->mlpw, &c->hv,   &c->sec, &c->mlptr,
-                      &c->maux};
+                      &c->maux, &c->featb};
     for (DevBuf *b : bufs) if (b->p) hipFree(b->p);
     if (c->pin.p) hipHostFree(c->pin.p);
     if (c->pin2.p) hipHostFree(c->pin2.p);
@@ -1285,14 +1296,17 @@ int hsdc_secular(hsdc_ctx *c, int k, const double *dlamda, const double *w, doub
     MergeOut *dmo = (MergeOut *)c->mo.p;
     int *dinfo = (int *)((char *)c->mo.p + al64(sizeof(MergeOut)));
     int zero = 0;
-    int bs[2] = {0, cdiv(k, 64)};
+    int bs[2] = {0, cdiv(k, SEC_WG / sec_team(k))};
     HK(hipMemcpy(dmo, &o, sizeof(o), hipMemcpyHostToDevice));
     HK(hipMemcpy(dinfo, &zero, sizeof(int), hipMemcpyHostToDevice));
     HK(hipMemcpy(dch, bs, sizeof(bs), hipMemcpyHostToDevice));
     HK(hipMemcpy(W.dlamda, dlamda, sizeof(double) * k, hipMemcpyHostToDevice));
     HK(hipMemcpy(W.w, w, sizeof(double) * k, hipMemcpyHostToDevice));
     CK(sec_setup(c));
-    hipLaunchKernelGGL(k_secular, dim3(bs[1]), dim3(64), 0, c->stream, (const Merge *)dm, (const int *)dch, 1,
+    if (c->sec_pre)
+        hipLaunchKernelGGL(k_secstart, dim3(bs[1]), dim3(SEC_WG), 0, c->stream, (const Merge *)dm,
+                           (const int *)dch, 1, (const MergeOut *)dmo, W, c->dsec);
+    hipLaunchKernelGGL(k_secular, dim3(bs[1]), dim3(SEC_WG), 0, c->stream, (const Merge *)dm, (const int *)dch, 1,
                        (const MergeOut *)dmo, W, c->prm.maxit, dinfo, c->dsec, c->dlvlmax);
     HK(hipGetLastError());
     int info = 0;
@@ -1328,7 +1342,7 @@ int hsdc_laed3(hsdc_ctx *c, int k, int n, int n1, double *d, double *q, int ldq,
     MergeOut *dmo = (MergeOut *)c->mo.p;
     int *dinfo = (int *)((char *)c->mo.p + al64(sizeof(MergeOut)));
     int *dtab = dinfo + 16;  // block table {0, blocks} and chunk table {0, chunks}
-    int zero = 0, tabs[4] = {0, cdiv(k, 64), 0, cdiv(k, WGS)};
+    int zero = 0, tabs[4] = {0, cdiv(k, SEC_WG / sec_team(k)), 0, cdiv(k, WGS)};
     HK(hipMemcpy(dmo, &o, sizeof(o), hipMemcpyHostToDevice));
     HK(hipMemcpy(dinfo, &zero, sizeof(int), hipMemcpyHostToDevice));
     HK(hipMemcpy(dtab, tabs, sizeof(tabs), hipMemcpyHostToDevice));
@@ -1343,7 +1357,10 @@ int hsdc_laed3(hsdc_ctx *c, int k, int n, int n1, double *d, double *q, int ldq,
     HK(hipMemcpy(c->lvl.p, &zoff, sizeof(long), hipMemcpyHostToDevice));
     if (q2len) HK(hipMemcpy(c->xb.p, q2, sizeof(double) * q2len, hipMemcpyHostToDevice));
     CK(sec_setup(c));
-    hipLaunchKernelGGL(k_secular, dim3(tabs[1]), dim3(64), 0, s, (const Merge *)dm, (const int *)dtab, 1,
+    if (c->sec_pre)
+        hipLaunchKernelGGL(k_secstart, dim3(tabs[1]), dim3(SEC_WG), 0, s, (const Merge *)dm, (const int *)dtab,
+                           1, (const MergeOut *)dmo, W, c->dsec);
+    hipLaunchKernelGGL(k_secular, dim3(tabs[1]), dim3(SEC_WG), 0, s, (const Merge *)dm, (const int *)dtab, 1,
                        (const MergeOut *)dmo, W, c->prm.maxit, dinfo, c->dsec, c->dlvlmax);
     hipLaunchKernelGGL(k_loewner, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
                        (const MergeOut *)dmo, W);
@@ -1387,14 +1404,17 @@ int hsdc_loewner(hsdc_ctx *c, int k, double rho, double *dlamda, double *z, doub
     MergeOut *dmo = (MergeOut *)c->mo.p;
     int *dinfo = (int *)((char *)c->mo.p + al64(sizeof(MergeOut)));
     int *dtab = dinfo + 16;
-    int zero = 0, tabs[4] = {0, cdiv(k, 64), 0, cdiv(k, WGS)};
+    int zero = 0, tabs[4] = {0, cdiv(k, SEC_WG / sec_team(k)), 0, cdiv(k, WGS)};
     HK(hipMemcpy(dmo, &o, sizeof(o), hipMemcpyHostToDevice));
     HK(hipMemcpy(dinfo, &zero, sizeof(int), hipMemcpyHostToDevice));
     HK(hipMemcpy(dtab, tabs, sizeof(tabs), hipMemcpyHostToDevice));
     HK(hipMemcpy(W.dlamda, dlamda, sizeof(double) * k, hipMemcpyHostToDevice));
     HK(hipMemcpy(W.w, z, sizeof(double) * k, hipMemcpyHostToDevice));
     CK(sec_setup(c));
-    hipLaunchKernelGGL(k_secular, dim3(tabs[1]), dim3(64), 0, s, (const Merge *)dm, (const int *)dtab, 1,
+    if (c->sec_pre)
+        hipLaunchKernelGGL(k_secstart, dim3(tabs[1]), dim3(SEC_WG), 0, s, (const Merge *)dm, (const int *)dtab,
+                           1, (const MergeOut *)dmo, W, c->dsec);
+    hipLaunchKernelGGL(k_secular, dim3(tabs[1]), dim3(SEC_WG), 0, s, (const Merge *)dm, (const int *)dtab, 1,
                        (const MergeOut *)dmo, W, c->prm.maxit, dinfo, c->dsec, c->dlvlmax);
     hipLaunchKernelGGL(k_loewner, dim3(tabs[3]), dim3(WGS), 0, s, (const Merge *)dm, (const int *)(dtab + 2), 1,
                        (const MergeOut *)dmo, W);

@@ -5,6 +5,21 @@
 
 #define HSDC_EPS 0x1p-53 /* DLAMCH('Epsilon') */
 
+// Lanes that share one secular root of a K-root merge: the largest power of two
+// <= 64 that leaves every lane at least 4 poles.  Summation order inside the
+// solver depends on it, so host tables and the kernel use this one rule.
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline int sec_team(int k)
+{
+    int t = 1;
+    while (t < 64 && 8 * t <= k) t *= 2;
+    return t;
+}
+#define SEC_WG 256   // threads per block of the secular kernel
+#define SEC_LDS 2048 // merges with K <= SEC_LDS stage poles and z in LDS
+
 // One eigenproblem of the batch; its eigenvector block Q is n x n, column-major.
 struct Prob {
     double *q;  // device base of Q
@@ -52,6 +67,8 @@ struct Work {
     double *lam, *tau, *wnew, *alpha, *difl, *difr;
     int *org, *nit;
     unsigned long long *maux;  // per merge of a launch: max|z|, max|D| (bit patterns), unsorted flag, pad
+    double *st_t, *st_tc, *st_lo, *st_hi;  // per root: starting iterate, classical start, bracket
+    int *st_ii;                // per root: origin pole | learned-start flag << 30
 };
 
 // control block of the secular kernel: learned-start weights, statistics and
@@ -62,6 +79,7 @@ struct SecCtl {
     unsigned long long *used;    // roots started from the learned iterate
     unsigned long long *wave_iters;  // sum over wavefronts of their largest iteration count
     float *hf, *ht;              // harvest: features (cap x 34) and targets (cap)
+    float *feat;                 // harvest: features of every root (34 per global index)
     unsigned long long *count;
     unsigned long long cap;
     int kmin;                    // merges with k < kmin use (and harvest) the classical start only

@@ -714,16 +714,17 @@ __global__ __launch_bounds__(WG) void k_rotate(const Merge *mg, const int *chunk
 // ---------------------------------------------------------------------------
 // Secular equation (replaces DLAED4, mdlaed3.f90:208 / DLAED3): the rational
 // "middle way" iteration with a bracketing safeguard, statement for statement as
-// the oracle's orc_laed4.  One lane per root; every lane of a block belongs to
-// the same merge (block_start pads each merge to whole blocks).
+// the oracle's orc_laed4.  A team of T = sec_team(K) lanes shares one root: lane l
+// sums the poles i = l, l+T, ... (psi ascending, phi descending) and the team
+// combines its partial sums by a butterfly, so every lane of the team holds the
+// same iterate and takes the same branches.  A block holds SEC_WG / T roots of one
+// merge; merges with K <= SEC_LDS read poles and z from LDS.
 // ---------------------------------------------------------------------------
 struct SecOut { double lam, tau, dl0, dl1, tc; int org, nit, info, used; };
+struct SecInit { double t, lo, hi, wb; int ii; };
 
-// wmlp: weights of the learned starting iterate (nullptr: classical start only),
-// taken when the proposed log-correction exceeds gate in magnitude; feat: receives
-// the 34 features when non-null
-__device__ SecOut sec_solve(int n, int j, const double *d, const double *z, double rho, int maxit,
-                            const float *wmlp, float gate, float *feat)
+// the closed forms of K = 1 and K = 2 (LAPACK DLAED5 for K = 2)
+__device__ SecOut sec_small(int n, int j, const double *d, const double *z, double rho)
 {
     SecOut o;
     o.dl0 = o.dl1 = 0.0; o.info = 0; o.nit = 0; o.used = 0; o.tc = 0.0;
@@ -731,45 +732,51 @@ __device__ SecOut sec_solve(int n, int j, const double *d, const double *z, doub
         o.lam = d[0] + rho * z[0] * z[0]; o.org = 1; o.tau = rho * z[0] * z[0];
         return o;
     }
-    if (n == 2) {
-        double del = d[1] - d[0], b, c, w, t, dl0, dl1;
-        if (j == 1) {
-            w = 1.0 + 2.0 * rho * (z[1] * z[1] - z[0] * z[0]) / del;
-            if (w > 0.0) {
-                b = del + rho * (z[0] * z[0] + z[1] * z[1]);
-                c = rho * z[0] * z[0] * del;
-                t = 2.0 * c / (b + sqrt(fabs(b * b - 4.0 * c)));
-                o.lam = d[0] + t; o.org = 1; o.tau = t;
-                dl0 = -z[0] / t; dl1 = z[1] / (del - t);
-            } else {
-                b = -del + rho * (z[0] * z[0] + z[1] * z[1]);
-                c = rho * z[1] * z[1] * del;
-                if (b > 0.0) t = -2.0 * c / (b + sqrt(b * b + 4.0 * c));
-                else t = (b - sqrt(b * b + 4.0 * c)) / 2.0;
-                o.lam = d[1] + t; o.org = 2; o.tau = t;
-                dl0 = -z[0] / (del + t); dl1 = -z[1] / t;
-            }
+    double del = d[1] - d[0], b, c, w, t, dl0, dl1;
+    if (j == 1) {
+        w = 1.0 + 2.0 * rho * (z[1] * z[1] - z[0] * z[0]) / del;
+        if (w > 0.0) {
+            b = del + rho * (z[0] * z[0] + z[1] * z[1]);
+            c = rho * z[0] * z[0] * del;
+            t = 2.0 * c / (b + sqrt(fabs(b * b - 4.0 * c)));
+            o.lam = d[0] + t; o.org = 1; o.tau = t;
+            dl0 = -z[0] / t; dl1 = z[1] / (del - t);
         } else {
             b = -del + rho * (z[0] * z[0] + z[1] * z[1]);
             c = rho * z[1] * z[1] * del;
-            if (b > 0.0) t = (b + sqrt(b * b + 4.0 * c)) / 2.0;
-            else t = 2.0 * c / (-b + sqrt(b * b + 4.0 * c));
+            if (b > 0.0) t = -2.0 * c / (b + sqrt(b * b + 4.0 * c));
+            else t = (b - sqrt(b * b + 4.0 * c)) / 2.0;
             o.lam = d[1] + t; o.org = 2; o.tau = t;
             dl0 = -z[0] / (del + t); dl1 = -z[1] / t;
         }
-        double sn = sqrt(dl0 * dl0 + dl1 * dl1);
-        o.dl0 = dl0 / sn; o.dl1 = dl1 / sn;
-        return o;
+    } else {
+        b = -del + rho * (z[0] * z[0] + z[1] * z[1]);
+        c = rho * z[1] * z[1] * del;
+        if (b > 0.0) t = (b + sqrt(b * b + 4.0 * c)) / 2.0;
+        else t = 2.0 * c / (-b + sqrt(b * b + 4.0 * c));
+        o.lam = d[1] + t; o.org = 2; o.tau = t;
+        dl0 = -z[0] / (del + t); dl1 = -z[1] / t;
     }
+    double sn = sqrt(dl0 * dl0 + dl1 * dl1);
+    o.dl0 = dl0 / sn; o.dl1 = dl1 / sn;
+    return o;
+}
+
+// classical starting iterate (n >= 3): the two nearest poles exactly, the others
+// through their value at the bracket midpoint; origin pole and bracket
+__device__ SecInit sec_init(int n, int j, const double *d, const double *z, double rho, int T, int l)
+{
+    SecInit q;
     const double rhoinv = 1.0 / rho;
-    int ii;
-    double t, lo, hi, wb;
+    double t;
     if (j == n) {
         double zz = 0.0;
-        for (int i = 0; i < n; i++) zz += z[i] * z[i];
+        for (int i = l; i < n; i += T) zz += z[i] * z[i];
+        zz = team_sum(zz, T);
         double hh = rho * zz, mid = 0.5 * hh, dn = d[n - 1];
-        double c = rhoinv;
-        for (int i = 0; i < n - 2; i++) c += z[i] * z[i] / ((d[i] - dn) - mid);
+        double cs = 0.0;
+        for (int i = l; i < n - 2; i += T) cs += z[i] * z[i] / ((d[i] - dn) - mid);
+        double c = rhoinv + team_sum(cs, T);
         double dl = dn - d[n - 2];
         double z1 = z[n - 2] * z[n - 2], z2 = z[n - 1] * z[n - 1];
         if (c <= 0.0) {
@@ -781,15 +788,16 @@ __device__ SecOut sec_solve(int n, int j, const double *d, const double *z, doub
             else t = 2.0 * z2 * dl / (b + sq);
         }
         if (!(t > 0.0 && t <= hh)) t = mid;
-        ii = n; lo = 0.0; hi = hh; wb = hh;
+        q.ii = n; q.lo = 0.0; q.hi = hh; q.wb = hh;
     } else {
         double del = d[j] - d[j - 1], mid = 0.5 * del;
-        wb = del;
-        double c = rhoinv;
-        for (int i = 0; i < n; i++) {
+        q.wb = del;
+        double cs = 0.0;
+        for (int i = l; i < n; i += T) {
             if (i == j - 1 || i == j) continue;
-            c += z[i] * z[i] / ((d[i] - d[j - 1]) - mid);
+            cs += z[i] * z[i] / ((d[i] - d[j - 1]) - mid);
         }
+        double c = rhoinv + team_sum(cs, T);
         double zl = z[j - 1] * z[j - 1], zr = z[j] * z[j];
         double fmid = c + (zr - zl) / mid;
         if (fmid >= 0.0) {
@@ -798,65 +806,68 @@ __device__ SecOut sec_solve(int n, int j, const double *d, const double *z, doub
             if (b > 0.0) t = 2.0 * cc / (b + sq);
             else t = (b - sq) / (2.0 * c);
             if (!(t > 0.0 && t <= mid)) t = 0.5 * mid;
-            ii = j; lo = 0.0; hi = mid;
+            q.ii = j; q.lo = 0.0; q.hi = mid;
         } else {
             double b = c * del - zl - zr, cc = zr * del;
             double sq = sqrt(fabs(b * b + 4.0 * c * cc));
             if (b <= 0.0) t = -2.0 * cc / (sq - b);
             else t = -(b + sq) / (2.0 * c);
             if (!(t >= -mid && t < 0.0)) t = -0.5 * mid;
-            ii = j + 1; lo = -mid; hi = 0.0;
+            q.ii = j + 1; q.lo = -mid; q.hi = 0.0;
         }
     }
-    // learned starting iterate: kept when it lies in the bracket of the classical
-    // start, which otherwise stays the fallback
-    o.tc = t;
-    if (wmlp || feat) {
-        double xc = (ii == j) ? t / wb : 1.0 + t / wb;
-        float fl[MLP_IN];
-        sec_features(n, j - 1, d, z, rho, wb, xc, fl);
-        if (feat)
-            for (int q = 0; q < MLP_IN; q++) feat[q] = fl[q];
-        if (wmlp) {
-            float y = mlp_forward(wmlp, fl);
-            y = y < -16.0f ? -16.0f : (y > 16.0f ? 16.0f : y);
-            double tl = t * sec_exp((double)y);
-            if (fabsf(y) > gate && tl > lo && tl < hi) { t = tl; o.used = 1; }
-        }
-    }
+    q.t = t;
+    return q;
+}
+
+// the iteration from a starting iterate inside the bracket of q
+__device__ SecOut sec_iterate(int n, int j, const double *d, const double *z, double rho, int maxit, SecInit q,
+                              int T, int l)
+{
+    SecOut o;
+    o.dl0 = o.dl1 = 0.0; o.used = 0; o.tc = 0.0;
+    const double rhoinv = 1.0 / rho;
+    const int ii = q.ii;
+    double t = q.t, lo = q.lo, hi = q.hi;
+    // rounding of the team's butterfly: log2(T) more additions per sum
+    int lgT = 0;
+    while ((1 << lgT) < T) lgT++;
     const double dorg = d[ii - 1];
     const int split = (j == n) ? n - 1 : j;
     int it, info = 1;
     for (it = 0; it <= maxit; it++) {
         // psi over poles left of the root (ascending), phi over poles right of it
-        // (descending), the origin pole excluded; the two sums advance in the same
-        // loop for instruction-level parallelism, each in its own fixed order
+        // (descending), the origin pole excluded (it is the last psi or the last phi
+        // pole); the two sums advance in the same loop for instruction-level
+        // parallelism, each in its own fixed order
         double psi = 0.0, dpsi = 0.0, epsi = 0.0, phi = 0.0, dphi = 0.0, ephi = 0.0;
         const int na = split - ((ii - 1 < split) ? 1 : 0);          // psi: i = 0 .. na-1
         const int nb = (n - split) - ((ii - 1 >= split) ? 1 : 0);   // phi: i = n-1 .. n-nb
         const int nab = na > nb ? na : nb;
 #pragma unroll 4
-        for (int t2 = 0; t2 < nab; t2++) {
+        for (int t2 = l; t2 < nab; t2 += T) {
             if (t2 < na) {
                 double dl = (d[t2] - dorg) - t;
-                double q = z[t2] / dl;
-                psi += z[t2] * q;
-                dpsi += q * q;
+                double qq = z[t2] / dl;
+                psi += z[t2] * qq;
+                dpsi += qq * qq;
                 epsi += psi;
             }
             if (t2 < nb) {
                 int i = n - 1 - t2;
                 double dl = (d[i] - dorg) - t;
-                double q = z[i] / dl;
-                phi += z[i] * q;
-                dphi += q * q;
+                double qq = z[i] / dl;
+                phi += z[i] * qq;
+                dphi += qq * qq;
                 ephi += phi;
             }
         }
+        psi = team_sum(psi, T); dpsi = team_sum(dpsi, T); epsi = team_sum(epsi, T);
+        phi = team_sum(phi, T); dphi = team_sum(dphi, T); ephi = team_sum(ephi, T);
         double qo = z[ii - 1] / (-t);
         double torg = z[ii - 1] * qo;
         double w = ((rhoinv + phi) + psi) + torg;
-        double erretm = 8.0 * (phi - psi) + (ephi - epsi) + 2.0 * rhoinv + 3.0 * fabs(torg) +
+        double erretm = (8.0 + lgT) * (phi - psi) + (ephi - epsi) + 2.0 * rhoinv + 3.0 * fabs(torg) +
                         fabs(t) * ((dpsi + dphi) + qo * qo);
         if (ii - 1 < split) dpsi += qo * qo; else dphi += qo * qo;
         if (fabs(w) <= HSDC_EPS * erretm || it == maxit) {
@@ -888,41 +899,112 @@ __device__ SecOut sec_solve(int n, int j, const double *d, const double *z, doub
     return o;
 }
 
-// Root lanes.  sec->w: learned-start weights or null; sec->hf/ht: when non-null,
-// (features, log(converged tau / classical starting tau)) of every root of a merge
-// with k >= max(3, sec->kmin) are appended for training, up to sec->cap samples.
-__global__ __launch_bounds__(64) void k_secular(const Merge *mg, const int *block_start, int nm,
-                                                const MergeOut *mo, Work W, int maxit, int *info,
-                                                const SecCtl *sec, unsigned int *itmax)
+// block -> merge, root and team geometry of both secular kernels
+struct SecGeom { int m, k, s, T, l, j; };
+__device__ __forceinline__ SecGeom sec_geom(const Merge *mg, const int *block_start, int nm, const MergeOut *mo)
 {
-    int m = find_seg(block_start, nm, blockIdx.x);
-    const MergeOut o = mo[m];
-    int j = (blockIdx.x - block_start[m]) * 64 + threadIdx.x + 1;
+    SecGeom g;
+    g.m = find_seg(block_start, nm, blockIdx.x);
+    g.k = mo[g.m].flags ? 0 : mo[g.m].k;
+    g.s = mg[g.m].s;
+    g.T = sec_team(g.k > 0 ? g.k : 1);
+    g.l = threadIdx.x & (g.T - 1);
+    g.j = (blockIdx.x - block_start[g.m]) * (SEC_WG / g.T) + threadIdx.x / g.T + 1;
+    return g;
+}
+
+// Learned starting iterates (merges with K >= max(3, sec->kmin), when weights are
+// loaded or samples are harvested): the classical start, the 34 features, the
+// network's proposal behind the gate; written per root for k_secular.
+__global__ __launch_bounds__(SEC_WG) void k_secstart(const Merge *mg, const int *block_start, int nm,
+                                                     const MergeOut *mo, Work W, const SecCtl *sec)
+{
+    const SecGeom g = sec_geom(mg, block_start, nm, mo);
     const SecCtl C = *sec;
+    if (g.k < 3 || g.k < C.kmin || g.j > g.k) return;
+    const double *d = W.dlamda + g.s, *z = W.w + g.s;
+    const double rho = mo[g.m].rho;
+    SecInit q = sec_init(g.k, g.j, d, z, rho, g.T, g.l);
+    const double tc = q.t;
+    double xc = (q.ii == g.j) ? q.t / q.wb : 1.0 + q.t / q.wb;
+    float fl[MLP_IN];
+    sec_features(g.k, g.j - 1, d, z, rho, q.wb, xc, fl);
+    int used = 0;
+    if (C.w && g.T >= 8) {
+        float y = mlp_forward_team(C.w, fl, g.T, g.l);
+        y = y < -16.0f ? -16.0f : (y > 16.0f ? 16.0f : y);
+        double tl = q.t * sec_exp((double)y);
+        if (fabsf(y) > C.gate && tl > q.lo && tl < q.hi) { q.t = tl; used = 1; }
+    }
+    if (g.l == 0) {
+        const int r = g.s + g.j - 1;
+        W.st_t[r] = q.t; W.st_tc[r] = tc; W.st_lo[r] = q.lo; W.st_hi[r] = q.hi;
+        W.st_ii[r] = q.ii | (used << 30);
+        if (C.hf)
+            for (int f = 0; f < MLP_IN; f++) C.feat[(long)r * MLP_IN + f] = fl[f];
+    }
+}
+
+// Root teams.  Merges that k_secstart covered start from its iterate; the others
+// from the classical one.  sec->hf/ht: when non-null, (features, log(converged tau /
+// classical starting tau)) of every root k_secstart covered are appended for
+// training, up to sec->cap samples.  block_start[m]: first block of merge m
+// (SEC_WG / sec_team(K) roots per block).
+__global__ __launch_bounds__(SEC_WG) void k_secular(const Merge *mg, const int *block_start, int nm,
+                                                    const MergeOut *mo, Work W, int maxit, int *info,
+                                                    const SecCtl *sec, unsigned int *itmax)
+{
+    __shared__ double sd[SEC_LDS], sz[SEC_LDS];
+    const SecGeom g = sec_geom(mg, block_start, nm, mo);
+    if (g.k == 0) return;  // uniform over the block
+    const int k = g.k, s = g.s, T = g.T, l = g.l, j = g.j;
+    const double *d = W.dlamda + s, *z = W.w + s;
+    if (k <= SEC_LDS) {
+        for (int i = threadIdx.x; i < k; i += SEC_WG) { sd[i] = d[i]; sz[i] = z[i]; }
+        __syncthreads();
+        d = sd;
+        z = sz;
+    }
+    const SecCtl C = *sec;
+    const double rho = mo[g.m].rho;
     int nit = 0, used = 0;
-    if (j <= o.k && !o.flags) {
-        int s = mg[m].s;
-        float fe[MLP_IN];
-        const bool big = o.k >= 3 && o.k >= C.kmin;
-        const bool want = (C.hf != nullptr) && big;
-        SecOut r = sec_solve(o.k, j, W.dlamda + s, W.w + s, o.rho, maxit, big ? C.w : nullptr, C.gate,
-                             want ? fe : nullptr);
-        nit = r.nit;
-        used = r.used;
-        const double ratio = r.tau / r.tc;
-        if (want && r.info == 0 && ratio > 0.0 && ratio < 1e300) {
-            unsigned long long slot = atomicAdd(C.count, 1ULL);
-            if (slot < C.cap) {
-                for (int q = 0; q < MLP_IN; q++) C.hf[slot * MLP_IN + q] = fe[q];
-                C.ht[slot] = (float)log(ratio);
-            }
+    if (j <= k) {
+        SecOut r;
+        const bool pre = k >= 3 && k >= C.kmin && (C.w || C.hf);
+        if (k < 3) {
+            r = sec_small(k, j, d, z, rho);
+        } else if (pre) {
+            const int rr = s + j - 1;
+            SecInit q;
+            q.t = W.st_t[rr]; q.lo = W.st_lo[rr]; q.hi = W.st_hi[rr]; q.wb = 0.0;
+            q.ii = W.st_ii[rr] & ((1 << 30) - 1);
+            r = sec_iterate(k, j, d, z, rho, maxit, q, T, l);
+            r.used = W.st_ii[rr] >> 30;
+            r.tc = W.st_tc[rr];
+        } else {
+            SecInit q = sec_init(k, j, d, z, rho, T, l);
+            r = sec_iterate(k, j, d, z, rho, maxit, q, T, l);
+            r.tc = q.t;
         }
-        W.lam[s + j - 1] = r.lam;
-        W.tau[s + j - 1] = r.tau;
-        W.org[s + j - 1] = r.org;
-        W.nit[s + j - 1] = r.nit;
-        if (o.k == 2) { W.difl[s + j - 1] = r.dl0; W.difr[s + j - 1] = r.dl1; }
-        if (r.info) atomicOr(info, 1);
+        if (l == 0) {
+            nit = r.nit;
+            used = r.used;
+            const double ratio = r.tau / r.tc;
+            if (pre && C.hf && r.info == 0 && ratio > 0.0 && ratio < 1e300) {
+                unsigned long long slot = atomicAdd(C.count, 1ULL);
+                if (slot < C.cap) {
+                    for (int q = 0; q < MLP_IN; q++)
+                        C.hf[slot * MLP_IN + q] = C.feat[(long)(s + j - 1) * MLP_IN + q];
+                    C.ht[slot] = (float)log(ratio);
+                }
+            }
+            W.lam[s + j - 1] = r.lam;
+            W.tau[s + j - 1] = r.tau;
+            W.org[s + j - 1] = r.org;
+            W.nit[s + j - 1] = r.nit;
+            if (k == 2) { W.difl[s + j - 1] = r.dl0; W.difr[s + j - 1] = r.dl1; }
+            if (r.info) atomicOr(info, 1);
+        }
     }
     // statistics, one atomic per wavefront: total iterations, learned starts, and the
     // wavefront's largest count (a wavefront runs as long as its slowest root, a
@@ -933,7 +1015,7 @@ __global__ __launch_bounds__(64) void k_secular(const Merge *mg, const int *bloc
         sum += __shfl_xor(sum, off);
         nu += __shfl_xor(nu, off);
     }
-    if (threadIdx.x == 0 && mx > 0) {
+    if ((threadIdx.x & 63) == 0 && mx > 0) {
         atomicAdd(C.iters, (unsigned long long)sum);
         if (nu) atomicAdd(C.used, (unsigned long long)nu);
         atomicAdd(C.wave_iters, (unsigned long long)mx);

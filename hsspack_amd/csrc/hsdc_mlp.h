@@ -50,24 +50,64 @@ __device__ inline void sec_features(int n, int r, const double *d, const double 
     f[33] = (float)xc;
 }
 
-__device__ inline float mlp_forward(const float *w, const float *x)
+// sum over the T lanes of a team (aligned group of T lanes): butterfly, every lane
+// ends with the same value
+__device__ inline double team_sum(double v, int T)
 {
-    float h1[MLP_H];
+    for (int off = T >> 1; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline float team_sumf(float v, int T)
+{
+    for (int off = T >> 1; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Forward pass shared by the T lanes of a team: lane l computes the hidden units
+// u = l, l+T, ...; the first layer's outputs reach every lane by shuffles; y = b3 +
+// the team sum of the lanes' partial w3.h2.  Each unit sums its inputs in the same
+// order as the one-lane form.
+template <int T>
+__device__ inline float mlp_forward_team_t(const float *w, const float *x, int l)
+{
+    constexpr int NU = MLP_H / T;  // units per lane
+    float h1[NU];
 #pragma unroll
-    for (int u = 0; u < MLP_H; u++) {
+    for (int m = 0; m < NU; m++) {
+        int u = l + m * T;
         float a = w[MLP_B1 + u];
         for (int k = 0; k < MLP_IN; k++) a = a + w[MLP_W1 + u * MLP_IN + k] * x[k];
-        h1[u] = a > 0.0f ? a : 0.0f;
+        h1[m] = a > 0.0f ? a : 0.0f;
     }
-    float y = w[MLP_B3];
-    for (int u = 0; u < MLP_H; u++) {
+    const int base = (threadIdx.x & 63) & ~(T - 1);
+    float yp = 0.0f;
+#pragma unroll 1
+    for (int m = 0; m < NU; m++) {
+        int u = l + m * T;
         float a = w[MLP_B2 + u];
+        // unit v = q * T + src lives on lane src of the team, slot q: v ascending
 #pragma unroll
-        for (int v = 0; v < MLP_H; v++) a = a + w[MLP_W2 + u * MLP_H + v] * h1[v];
+        for (int q = 0; q < NU; q++) {
+#pragma unroll 2
+            for (int src = 0; src < T; src++) {
+                float hv = __shfl(h1[q], base + src, 64);
+                a = a + w[MLP_W2 + u * MLP_H + q * T + src] * hv;
+            }
+        }
         a = a > 0.0f ? a : 0.0f;
-        y = y + w[MLP_W3 + u] * a;
+        yp = yp + w[MLP_W3 + u] * a;
     }
-    return y;
+    return w[MLP_B3] + team_sumf(yp, T);
+}
+
+__device__ inline float mlp_forward_team(const float *w, const float *x, int T, int l)
+{
+    switch (T) {
+    case 8: return mlp_forward_team_t<8>(w, x, l);
+    case 16: return mlp_forward_team_t<16>(w, x, l);
+    case 32: return mlp_forward_team_t<32>(w, x, l);
+    default: return mlp_forward_team_t<64>(w, x, l);
+    }
 }
 
 // e^x for |x| <= 16: 2^k e^r with k the nearest integer to x/ln 2, |r| <= ln 2 / 2, and

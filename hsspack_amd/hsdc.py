@@ -59,6 +59,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("hsspack_amd/libhsdc.so is not built (run hsspack_amd/build.py)")
+        # A process must hold a single HIP runtime.  torch ships its own
+        # libamdhip64.so.7; loading torch first makes this library bind to that copy
+        # (same soname) instead of bringing up a second runtime beside it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         L.hsdc_ctx_create.argtypes = [_P(_vp)]
         L.hsdc_ctx_destroy.argtypes = [_vp]

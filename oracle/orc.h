@@ -78,6 +78,9 @@ int orc_laed4(int n, int j, const double *d, const double *z, double *delta, dou
 void orc_mlp_features(int n, int r, const double *d, const double *z, double rho, double W, double xc,
                       float *f);
 float orc_mlp_forward(const float *w, const float *x);
+float orc_mlp_forward_team(const float *w, const float *x, int T);
+/* lanes per secular root on the device for an n-root equation (summation order) */
+int orc_sec_team(int n);
 double orc_sec_exp(double x);
 int orc_laed4_mlp(int n, int j, const double *d, const double *z, double *delta, double rho,
                   double *dlam, int *org, double *tau, int *niter, int maxit, const float *wmlp,

@@ -94,18 +94,58 @@ static void laed2x2(int j, const double *d, const double *z, double *delta, doub
     delta[1] /= s;
 }
 
+/* Lanes sharing one root of an n-root secular equation on the device: the largest
+ * power of two <= 64 that leaves each lane at least 4 poles.  The device sums the
+ * poles of a root in T interleaved partial sums (lane l takes i = l, l+T, ...)
+ * that a butterfly combines; the oracle sums in the same order, so both produce
+ * the same bits.  T = 1 is the plain left-to-right order. */
+int orc_sec_team(int n)
+{
+    int t = 1;
+    while (t < 64 && 8 * t <= n) t *= 2;
+    return t;
+}
+
+/* butterfly over T partial sums: step off adds lane l ^ off to lane l; every lane
+ * ends with the same value (addition commutes) */
+static double tsum(double *v, int T)
+{
+    double nv[64];
+    for (int off = T >> 1; off > 0; off >>= 1) {
+        for (int l = 0; l < T; l++) nv[l] = v[l] + v[l ^ off];
+        for (int l = 0; l < T; l++) v[l] = nv[l];
+    }
+    return v[0];
+}
+
+static float tsumf(float *v, int T)
+{
+    float nv[64];
+    for (int off = T >> 1; off > 0; off >>= 1) {
+        for (int l = 0; l < T; l++) nv[l] = v[l] + v[l ^ off];
+        for (int l = 0; l < T; l++) v[l] = nv[l];
+    }
+    return v[0];
+}
+
 /* initial iterate for root j (1-based, n>=3) -> origin, tau, bracket */
-static void sec_init(int n, int j, const double *d, const double *z, double rho,
+static void sec_init(int n, int j, const double *d, const double *z, double rho, int T,
                      int *orgp, double *taup, double *lop, double *hip)
 {
-    double rhoinv = 1.0 / rho;
+    double rhoinv = 1.0 / rho, ps[64];
     if (j == n) {
         /* root in (d_n, d_n + rho*|z|^2] */
-        double zz = 0.0;
-        for (int i = 0; i < n; i++) zz += z[i] * z[i];
+        for (int l = 0; l < T; l++) {
+            ps[l] = 0.0;
+            for (int i = l; i < n; i += T) ps[l] += z[i] * z[i];
+        }
+        double zz = tsum(ps, T);
         double hi = rho * zz, mid = 0.5 * hi, dn = d[n - 1];
-        double c = rhoinv;
-        for (int i = 0; i < n - 2; i++) c += z[i] * z[i] / ((d[i] - dn) - mid);
+        for (int l = 0; l < T; l++) {
+            ps[l] = 0.0;
+            for (int i = l; i < n - 2; i += T) ps[l] += z[i] * z[i] / ((d[i] - dn) - mid);
+        }
+        double c = rhoinv + tsum(ps, T);
         double dl = dn - d[n - 2];
         double z1 = z[n - 2] * z[n - 2], z2 = z[n - 1] * z[n - 1];
         double t;
@@ -122,11 +162,14 @@ static void sec_init(int n, int j, const double *d, const double *z, double rho,
         return;
     }
     double del = d[j] - d[j - 1], mid = 0.5 * del;
-    double c = rhoinv;
-    for (int i = 0; i < n; i++) {
-        if (i == j - 1 || i == j) continue;
-        c += z[i] * z[i] / ((d[i] - d[j - 1]) - mid);
+    for (int l = 0; l < T; l++) {
+        ps[l] = 0.0;
+        for (int i = l; i < n; i += T) {
+            if (i == j - 1 || i == j) continue;
+            ps[l] += z[i] * z[i] / ((d[i] - d[j - 1]) - mid);
+        }
     }
+    double c = rhoinv + tsum(ps, T);
     double zl = z[j - 1] * z[j - 1], zr = z[j] * z[j];
     double fmid = c + (zr - zl) / mid;
     double t;
@@ -209,22 +252,31 @@ double orc_sec_exp(double x)
     return ldexp(p, (int)kf);
 }
 
-float orc_mlp_forward(const float *w, const float *x)
+/* forward pass in the device's order for a team of T lanes: units as one sum each;
+ * y = b3 + butterfly over the lanes' partial sums of w3.h2 (lane l: units l, l+T, ...) */
+float orc_mlp_forward_team(const float *w, const float *x, int T)
 {
-    float h1[ORC_MLP_H];
+    float h1[ORC_MLP_H], yp[64];
     for (int u = 0; u < ORC_MLP_H; u++) {
         float a = w[ORC_MLP_B1 + u];
         for (int k = 0; k < ORC_MLP_IN; k++) a = a + w[ORC_MLP_W1 + u * ORC_MLP_IN + k] * x[k];
         h1[u] = a > 0.0f ? a : 0.0f;
     }
-    float y = w[ORC_MLP_B3];
-    for (int u = 0; u < ORC_MLP_H; u++) {
-        float a = w[ORC_MLP_B2 + u];
-        for (int v = 0; v < ORC_MLP_H; v++) a = a + w[ORC_MLP_W2 + u * ORC_MLP_H + v] * h1[v];
-        a = a > 0.0f ? a : 0.0f;
-        y = y + w[ORC_MLP_W3 + u] * a;
+    for (int l = 0; l < T; l++) {
+        yp[l] = 0.0f;
+        for (int u = l; u < ORC_MLP_H; u += T) {
+            float a = w[ORC_MLP_B2 + u];
+            for (int v = 0; v < ORC_MLP_H; v++) a = a + w[ORC_MLP_W2 + u * ORC_MLP_H + v] * h1[v];
+            a = a > 0.0f ? a : 0.0f;
+            yp[l] = yp[l] + w[ORC_MLP_W3 + u] * a;
+        }
     }
-    return y;
+    return w[ORC_MLP_B3] + tsumf(yp, T);
+}
+
+float orc_mlp_forward(const float *w, const float *x)
+{
+    return orc_mlp_forward_team(w, x, 64);
 }
 
 int orc_laed4(int n, int j, const double *d, const double *z, double *delta, double rho,
@@ -255,14 +307,15 @@ int orc_laed4_mlp(int n, int j, const double *d, const double *z, double *delta,
     }
     int ii;
     double t, lo, hi;
-    sec_init(n, j, d, z, rho, &ii, &t, &lo, &hi);
-    if (wmlp) {
+    const int T = orc_sec_team(n);
+    sec_init(n, j, d, z, rho, T, &ii, &t, &lo, &hi);
+    if (wmlp && T >= 8) {
         /* the proposal replaces the classical start only inside its bracket */
         double wb = (j == n) ? hi : d[j] - d[j - 1];
         double xc = (ii == j) ? t / wb : 1.0 + t / wb;
         float fl[ORC_MLP_IN];
         orc_mlp_features(n, j - 1, d, z, rho, wb, xc, fl);
-        float y = orc_mlp_forward(wmlp, fl);
+        float y = orc_mlp_forward_team(wmlp, fl, T);
         y = y < -16.0f ? -16.0f : (y > 16.0f ? 16.0f : y);
         double tl = t * orc_sec_exp((double)y);
         if (fabsf(y) > gate && tl > lo && tl < hi) {
@@ -273,32 +326,43 @@ int orc_laed4_mlp(int n, int j, const double *d, const double *z, double *delta,
     const double rhoinv = 1.0 / rho;
     const double dorg = d[ii - 1];
     const int split = (j == n) ? n - 1 : j; /* psi over 1..split, phi over split+1..n */
+    /* the origin pole is the last psi pole (ii = j) or the last phi pole (ii = j+1) */
+    const int na = split - ((ii - 1 < split) ? 1 : 0), nb = (n - split) - ((ii - 1 >= split) ? 1 : 0);
+    int lgT = 0;
+    while ((1 << lgT) < T) lgT++;
     int it, info = 1;
     for (it = 0; it <= maxit; it++) {
-        /* sums over the poles left (psi) and right (phi) of the root, the origin
-         * pole's term kept apart and added last */
-        double psi = 0.0, dpsi = 0.0, epsi = 0.0, phi = 0.0, dphi = 0.0, ephi = 0.0;
-        for (int i = 0; i < split; i++) {
-            if (i == ii - 1) continue;
-            double dl = (d[i] - dorg) - t;
-            double q = z[i] / dl;
-            psi += z[i] * q;
-            dpsi += q * q;
-            epsi += psi;
+        /* sums over the poles left (psi, ascending) and right (phi, descending) of
+         * the root in T interleaved partial sums, the origin pole's term kept apart
+         * and added last */
+        double P[6][64];
+        for (int l = 0; l < T; l++) {
+            double psi = 0.0, dpsi = 0.0, epsi = 0.0, phi = 0.0, dphi = 0.0, ephi = 0.0;
+            for (int t2 = l; t2 < na; t2 += T) {
+                double dl = (d[t2] - dorg) - t;
+                double q = z[t2] / dl;
+                psi += z[t2] * q;
+                dpsi += q * q;
+                epsi += psi;
+            }
+            for (int t2 = l; t2 < nb; t2 += T) {
+                int i = n - 1 - t2;
+                double dl = (d[i] - dorg) - t;
+                double q = z[i] / dl;
+                phi += z[i] * q;
+                dphi += q * q;
+                ephi += phi;
+            }
+            P[0][l] = psi; P[1][l] = dpsi; P[2][l] = epsi; P[3][l] = phi; P[4][l] = dphi; P[5][l] = ephi;
         }
-        for (int i = n - 1; i >= split; i--) {
-            if (i == ii - 1) continue;
-            double dl = (d[i] - dorg) - t;
-            double q = z[i] / dl;
-            phi += z[i] * q;
-            dphi += q * q;
-            ephi += phi;
-        }
+        double psi = tsum(P[0], T), dpsi = tsum(P[1], T), epsi = tsum(P[2], T);
+        double phi = tsum(P[3], T), dphi = tsum(P[4], T), ephi = tsum(P[5], T);
         double qo = z[ii - 1] / (-t);
         double torg = z[ii - 1] * qo;
         double w = ((rhoinv + phi) + psi) + torg;
-        /* running-error bound of the evaluation of w, in the form of Li's analysis */
-        double erretm = 8.0 * (phi - psi) + (ephi - epsi) + 2.0 * rhoinv + 3.0 * fabs(torg) +
+        /* running-error bound of the evaluation of w, in the form of Li's analysis,
+         * with log2(T) more additions per sum for the butterfly */
+        double erretm = (8.0 + lgT) * (phi - psi) + (ephi - epsi) + 2.0 * rhoinv + 3.0 * fabs(torg) +
                         fabs(t) * ((dpsi + dphi) + qo * qo);
         if (ii - 1 < split) dpsi += qo * qo; else dphi += qo * qo;
         if (fabs(w) <= EPS * erretm || it == maxit) {

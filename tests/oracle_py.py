@@ -99,6 +99,10 @@ def mlp_forward(w, x):
     return L.orc_mlp_forward(w.ctypes.data_as(_P(ctypes.c_float)), x.ctypes.data_as(_P(ctypes.c_float)))
 
 
+def sec_team(n):
+    return int(lib().orc_sec_team(int(n)))
+
+
 def sec_exp(x):
     L = lib()
     L.orc_sec_exp.restype = _d

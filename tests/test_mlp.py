@@ -104,7 +104,7 @@ def test_learned_start_keeps_the_root():
     mp = pytest.importorskip("mpmath")
     mp.mp.dps = 40
     errs = []
-    for seed, k in ((7, 30), (8, 24)):
+    for seed, k in ((7, 70), (8, 64)):
         d, z, rho = secular_case(k, seed)
         dm = [mp.mpf(x) for x in d]
         zm = [mp.mpf(x) for x in z]
@@ -243,7 +243,8 @@ def test_device_learned_secular_bitexact_vs_oracle(ctx, which, gate):
             assert st["learned_roots"] == used
             assert st["secular_iters"] == int(np.sum(g["niter"]))
             nit = np.asarray(g["niter"])
-            assert st["secular_wave_iters"] == sum(int(nit[b:b + 64].max()) for b in range(0, k, 64))
+            per_wave = 64 // O.sec_team(k)  # roots per wavefront
+            assert st["secular_wave_iters"] == sum(int(nit[b:b + per_wave].max()) for b in range(0, k, per_wave))
     finally:
         ctx.set_params(learned=0, learned_min_k=256, learned_gate=hsdc.default_params().learned_gate)
 

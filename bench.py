@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-13, help="HSS compression tolerance")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device three times (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the solver runs on a single MI355X and has no "
@@ -111,11 +114,30 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        for _ in range(3):
+            dst.copy_(src)
+        e0.record()
+        for _ in range(5):
+            dst.copy_(src)
+        e1.record()
+        for _ in range(5):
+            dst.fill_(0.0)
+        e2.record()
+        torch.cuda.synchronize()
+        nb = src.numel() * 8
+        print(json.dumps({"copy_probe_bytes": nb,
+                          "copy_GBps": 2 * nb * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9,
+                          "fill_GBps": nb * 5 / (e1.elapsed_time(e2) * 1e-3) / 1e9}))
+        return
     from hsspack_amd import hsdc
     from hsspack_amd.workloads import tridiag_cfg
 
     n = args.n
-    hss = (1024, 128, 1e-13)
+    hss = (1024, 128, args.tol)
     d, e = tridiag_cfg(args.cfg, n, args.seed)
     dev = torch.device("cuda:0")
     D0 = torch.tensor(d, device=dev)
@@ -202,8 +224,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "tridiagonal DC eigensolve, n=%d, %s seed %d, HSS leaf 128, "
-                               "tol 1e-13, HSS for K>=1024, all eigenpairs, device resident"
-                               % (n, GEN[args.cfg], args.seed),
+                               "tol %g, HSS for K>=1024, all eigenpairs, device resident"
+                               % (n, GEN[args.cfg], args.seed, args.tol),
                    "n": n, "schedule": "batched (one launch per DC level) + learned secular start",
                    "plain_s": plain_s,
                    "speedup_vs_plain": (plain_s / (ms_step / 1000.0)) if plain_s else None,

@@ -26,7 +26,8 @@ __global__ void k_prep(const Merge *, const int *, int, const Prob *, Work, int)
 __global__ void k_sortmerge(const Merge *, Work);
 __global__ void k_mergepath(const Merge *, const int *, int, Work);
 __global__ void k_fill_u64(unsigned long long *, long, unsigned long long);
-__global__ void k_deflate(const Merge *, Work, int, MergeOut *, int);
+template <int NT>
+__global__ void k_deflate(const Merge *, Work, int, MergeOut *, int, int);
 __global__ void k_rot_prep(const Merge *, const Prob *, Work, int, const MergeOut *);
 __global__ void k_rotate(const Merge *, const int *, int, const Prob *, Work, int, const MergeOut *);
 __global__ void k_secular(const Merge *, const int *, int, const MergeOut *, Work, int, int *, const SecCtl *,
@@ -741,7 +742,17 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
     hipLaunchKernelGGL(k_prep, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur);
     hipLaunchKernelGGL(k_mergepath, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, W);
     hipLaunchKernelGGL(k_sortmerge, dim3(nm), dim3(WGS), 0, s, dmg, W);
-    hipLaunchKernelGGL(k_deflate, dim3(nm), dim3(WGS), 0, s, dmg, W, cur, dmo, c->prm.hss_min_k);
+    // sorted D of merges up to 4096 in LDS for the deflated-region replay
+    int nmax = 0;
+    for (int i = m0; i < m1; i++) nmax = std::max(nmax, hm[i].n);
+    const int ds_cap = nmax <= 4096 ? nmax : 0;
+    // large merges get 16 wavefronts: their O(N) passes are latency bound
+    if (nmax >= 2048)
+        hipLaunchKernelGGL(k_deflate<1024>, dim3(nm), dim3(1024), sizeof(double) * ds_cap, s, dmg, W, cur, dmo,
+                           c->prm.hss_min_k, ds_cap);
+    else
+        hipLaunchKernelGGL(k_deflate<256>, dim3(nm), dim3(WGS), sizeof(double) * ds_cap, s, dmg, W, cur, dmo,
+                           c->prm.hss_min_k, ds_cap);
     // K of every merge back to the host
     CK(ensure_pin(c->pin2, need_mo));
     HK(hipMemcpyAsync(c->pin2.p, dmo, need_mo, hipMemcpyDeviceToHost, s));
@@ -870,7 +881,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
            a3 = al64(a2 + sizeof(Merge) * nmtot), a4 = al64(a3 + sizeof(double) * ntot),
            a5 = al64(a4 + sizeof(double) * ntot), a6 = al64(a5 + sizeof(double) * np),
            a7 = al64(a6 + sizeof(int) * ntot), a8 = al64(a7 + sizeof(long) * np),
-           a9 = al64(a8 + sizeof(long) * np), a10 = al64(a9 + sizeof(int) * ntot), a11 = a10 + 64;
+           a9 = al64(a8 + sizeof(long) * np), a10 = al64(a9 + sizeof(int) * ntot), a11 = a10 + 64 + 512;
     // chunk tables: per level (nm+1 entries) followed by per merge {0, chunks}
     std::vector<int> chunks;
     std::vector<size_t> lvchunk, mchunk;
@@ -910,7 +921,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     memcpy(h + a7, zo.data(), sizeof(long) * np);
     memcpy(h + a8, lz.data(), sizeof(long) * np);
     if (outcol) memcpy(h + a9, outcol->data(), sizeof(int) * ntot);
-    memset(h + a10, 0, 64);
+    memset(h + a10, 0, 64 + 512);
     if (!chunks.empty()) memcpy(h + a11, chunks.data(), sizeof(int) * chunks.size());
     char *dp = (char *)c->probs.p;
     HK(hipMemcpyAsync(dp, h, a12, hipMemcpyHostToDevice, s));
@@ -957,7 +968,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         c->st.levels++;
     }
     int zrows = zglobal ? ntot : 0;
-    unsigned long long *drows = (unsigned long long *)(dp + a10 + 16);
+    unsigned long long *drows = (unsigned long long *)(dp + a10 + 64);  // 64 counters
     hipEvent_t f0 = nullptr, f1 = nullptr;
     if (c->prm.profile) {
         HK(hipEventCreate(&f0));
@@ -970,10 +981,11 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     if (c->prm.profile) HK(hipEventRecord(f1, s));
     HK(hipEventRecord(e1, s));
     int hinfo = 0;
-    unsigned long long rows = 0;
+    unsigned long long rowc[64], rows = 0;
     HK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
-    HK(hipMemcpyAsync(&rows, drows, sizeof(rows), hipMemcpyDeviceToHost, s));
+    HK(hipMemcpyAsync(rowc, drows, sizeof(rowc), hipMemcpyDeviceToHost, s));
     HK(hipStreamSynchronize(s));
+    for (int q = 0; q < 64; q++) rows += rowc[q];
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     c->st.ms_total = ms;
@@ -1243,7 +1255,13 @@ int hsdc_deflate(hsdc_ctx *c, int n, int n1, double *d, double *z, int *indxq, d
     hipLaunchKernelGGL(k_mergepath, dim3(cdiv(n, WGS)), dim3(WGS), 0, c->stream, (const Merge *)dm,
                        (const int *)dch, 1, W);
     hipLaunchKernelGGL(k_sortmerge, dim3(1), dim3(WGS), 0, c->stream, (const Merge *)dm, W);
-    hipLaunchKernelGGL(k_deflate, dim3(1), dim3(WGS), 0, c->stream, (const Merge *)dm, W, 0, dmo, hss_min_k);
+    const int ds_cap = n <= 4096 ? n : 0;
+    if (n >= 2048)
+        hipLaunchKernelGGL(k_deflate<1024>, dim3(1), dim3(1024), sizeof(double) * ds_cap, c->stream,
+                           (const Merge *)dm, W, 0, dmo, hss_min_k, ds_cap);
+    else
+        hipLaunchKernelGGL(k_deflate<256>, dim3(1), dim3(WGS), sizeof(double) * ds_cap, c->stream,
+                           (const Merge *)dm, W, 0, dmo, hss_min_k, ds_cap);
     HK(hipGetLastError());
     MergeOut o;
     HK(hipMemcpy(&o, dmo, sizeof(MergeOut), hipMemcpyDeviceToHost));

@@ -324,14 +324,19 @@ __global__ __launch_bounds__(WG) void k_sortmerge(const Merge *mg, Work W)
 // ---------------------------------------------------------------------------
 #define DCH 1024
 #define MAXSEG 1536
-__global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur, MergeOut *mo,
-                                                int hss_min_k)
+#define DU 8  // sub-tiles of 64 per step of the ballot passes
+template <int NT>
+__global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur, MergeOut *mo,
+                                                int hss_min_k, int ds_cap)
 {
-    __shared__ int sk, sk2, scnt[4], wcnt[4][4], sL, sfirst, sfallback, snseg;
+    constexpr int NW = NT / 64;  // wavefronts
+    extern __shared__ double dsl[];  // ds_cap doubles: sorted D for the replay searches
+    __shared__ int sk, sk2, scnt[4], wcnt[4][NW], sL, sfirst, sfallback, snseg;
     __shared__ int seg_a[MAXSEG], seg_b[MAXSEG], seg_nx[MAXSEG], seg_pos[MAXSEG], seg_ord[MAXSEG];
     __shared__ double seg_d[MAXSEG];
     __shared__ double cz[DCH], cd[DCH];
-    __shared__ int ci[DCH];
+    __shared__ int ci[DCH], cty[DCH], lsn[DCH];
+    __shared__ int snr;
     Merge M = mg[blockIdx.x];
     const int n = M.n, n1 = M.n1, s = M.s, tid = threadIdx.x;
     const double *D = W.Dv[cur] + s;
@@ -361,41 +366,65 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
     // time and compact with ballots, which keeps every list in ascending order.
     const int lane = tid & 63, wv = tid >> 6;
     const unsigned long long lt = (1ULL << lane) - 1ULL;
-    const int wper = (n + 3) / 4;
+    const int wper = (n + NW - 1) / NW;
     const int w0 = min(n, wv * wper), w1 = min(n, w0 + wper);
     // ---- 1. survivors of the small-z screen (zs, ds, INDX and the small-z column
     // types come from the merge kernels); DLAMDA must be ascending in scan order for
     // the run replay below
     if (tid == 0) sfallback = 0;
     __syncthreads();
+    // (DU sub-tiles of 64 per step: their loads are independent and issue together)
     int cnt = 0, bad = 0;
-    for (int j0 = w0; j0 < w1; j0 += 64) {
-        int j = j0 + lane;
-        bool sv = false;
-        if (j < w1) {
-            sv = rho * fabs(zs[j]) > tol;
-            if (j + 1 < n && ds[j] > ds[j + 1]) bad = 1;
+    for (int j0 = w0; j0 < w1; j0 += 64 * DU) {
+        bool sv[DU];
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            const int j = j0 + 64 * u + lane;
+            sv[u] = false;
+            if (j < w1) {
+                const double a = ds[j], b = (j + 1 < n) ? ds[j + 1] : a;
+                sv[u] = rho * fabs(zs[j]) > tol;
+                if (a > b) bad = 1;
+            }
         }
-        cnt += __popcll(__ballot(sv));
+#pragma unroll
+        for (int u = 0; u < DU; u++) cnt += __popcll(__ballot(sv[u]));
     }
     if (bad) sfallback = 1;
     if (lane == 0) wcnt[0][wv] = cnt;
     __syncthreads();
     int base = 0;
     for (int w = 0; w < wv; w++) base += wcnt[0][w];
-    const int L = wcnt[0][0] + wcnt[0][1] + wcnt[0][2] + wcnt[0][3];
-    for (int j0 = w0; j0 < w1; j0 += 64) {
-        int j = j0 + lane;
-        bool sv = (j < w1) && rho * fabs(zs[j]) > tol;
-        unsigned long long msk = __ballot(sv);
-        if (sv) nsl[base + __popcll(msk & lt)] = j;
-        base += __popcll(msk);
+    int L = 0;
+    for (int w = 0; w < NW; w++) L += wcnt[0][w];
+    // survivors from the front of NSL in order; small-z positions from the back
+    // (rank r at NSL(n-1-r)), which is their order in the deflated region when no
+    // rotation interleaves (mdlaed2.f90:304-325)
+    int nbase = w0 - base;
+    for (int j0 = w0; j0 < w1; j0 += 64 * DU) {
+        bool sv[DU], in[DU];
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            const int j = j0 + 64 * u + lane;
+            in[u] = j < w1;
+            sv[u] = in[u] && rho * fabs(zs[j]) > tol;
+        }
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            const int j = j0 + 64 * u + lane;
+            unsigned long long msk = __ballot(sv[u]);
+            unsigned long long nmk = __ballot(in[u] && !sv[u]);
+            if (sv[u]) nsl[base + __popcll(msk & lt)] = j;
+            else if (in[u]) nsl[n - 1 - (nbase + __popcll(nmk & lt))] = j;
+            base += __popcll(msk);
+            nbase += __popcll(nmk);
+        }
     }
     if (tid == 0) { sL = L; sfirst = L - 1; }
     __syncthreads();
     // ---- 2. speculative close-pair test on unrotated values
     int myfirst = L - 1;
-    for (int q = tid; q + 1 < L; q += WG) {
+    for (int q = tid; q + 1 < L; q += NT) {
         int p = nsl[q], r = nsl[q + 1];
         double ss = zs[p], cc = zs[r];
         double tau = lapy2_d(cc, ss);
@@ -407,34 +436,45 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
     atomicMin(&sfirst, myfirst);
     __syncthreads();
     const int first = sfirst;
-    for (int q = tid; q < first; q += WG) {
+    const double *dsr = ds;
+    if (n <= ds_cap) {
+        for (int i = tid; i < n; i += NT) dsl[i] = ds[i];
+        __syncthreads();
+        dsr = dsl;
+    }
+    for (int q = tid; q < first; q += NT) {
         int p = nsl[q];
         W.dlamda[s + q] = ds[p]; W.w[s + q] = zs[p]; indxp[q] = indx[p];
     }
     // ---- 3. sequential continuation over survivors, staged through LDS in chunks
-    int K = first, nrot = 0, PJ = 0;
+    int K = first, nrot = 0, PJ = 0, ctP = 0;
     double zP = 0.0, DP = 0.0;
     if (tid == 0 && !sfallback) {
         int p0 = nsl[first];
         PJ = indx[p0];
         zP = zs[p0];
         DP = ds[p0];
+        ctP = coltyp[PJ - 1];
     }
     if (!sfallback) {
         for (int c0 = first + 1; c0 < L; c0 += DCH) {
             const int cl = min(DCH, L - c0);
             __syncthreads();
-            for (int u = tid; u < cl; u += WG) {
+            // a survivor's type is still its initial one when its chunk is staged: only
+            // earlier survivors have been changed
+            for (int u = tid; u < cl; u += NT) {
                 int r = nsl[c0 + u];
-                ci[u] = indx[r];
+                int nj = indx[r];
+                ci[u] = nj;
                 cz[u] = zs[r];
                 cd[u] = ds[r];
+                cty[u] = coltyp[nj - 1];
             }
             __syncthreads();
             if (tid == 0) {
                 for (int u = 0; u < cl; u++) {
                     const int q = c0 + u;
-                    int NJ = ci[u];
+                    int NJ = ci[u], ctN = cty[u];
                     double zN = cz[u], dN = cd[u];
                     double ss = zP, cc = zN;
                     double tau = lapy2_d(cc, ss);
@@ -442,7 +482,7 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
                     cc = cc / tau;
                     ss = -ss / tau;
                     if (fabs(t * cc * ss) <= tol) {
-                        if (coltyp[NJ - 1] != coltyp[PJ - 1]) coltyp[NJ - 1] = 2;
+                        if (ctN != ctP) { coltyp[NJ - 1] = 2; ctN = 2; }
                         coltyp[PJ - 1] = 4;
                         double tP = DP * (cc * cc) + dN * (ss * ss);
                         double dNn = DP * (ss * ss) + dN * (cc * cc);
@@ -450,15 +490,22 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
                         W.rot_c[s + nrot] = cc; W.rot_s[s + nrot] = ss;
                         W.evq[s + nrot] = q; W.evd[s + nrot] = tP;
                         nrot++;
-                        PJ = NJ; zP = tau; DP = dNn;
+                        PJ = NJ; zP = tau; DP = dNn; ctP = ctN;
                     } else {
                         W.dlamda[s + K] = DP; W.w[s + K] = zP; indxp[K] = PJ;
                         K++;
-                        PJ = NJ; zP = zN; DP = dN;
+                        PJ = NJ; zP = zN; DP = dN; ctP = ctN;
                     }
                 }
             }
         }
+        if (tid == 0) snr = nrot;
+        __syncthreads();
+        // survivors and rotation events into LDS for the replay below
+        if (L <= DCH)
+            for (int u = tid; u < L; u += NT) lsn[u] = nsl[u];
+        if (snr <= DCH)
+            for (int u = tid; u < snr; u += NT) { cty[u] = W.evq[s + u]; cz[u] = W.evd[s + u]; ci[u] = W.rot_pj[s + u]; }
         __syncthreads();
     }
     if (tid == 0 && !sfallback) {
@@ -477,12 +524,14 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
             seg_a[nseg] = a; seg_b[nseg] = b; seg_nx[nseg] = head; head = nseg++;
         };
         int ev = 0;
-        push_run(0, nsl[0]);
+        const int *nsr = (L <= DCH) ? lsn : nsl;
+        const bool evl = nrot <= DCH;
+        push_run(0, nsr[0]);
         for (int q = 1; q < L && nseg <= MAXSEG; q++) {
-            push_run(nsl[q - 1] + 1, nsl[q]);
-            while (ev < nrot && W.evq[s + ev] == q && nseg <= MAXSEG) {
-                double dx = W.evd[s + ev];
-                int x = W.rot_pj[s + ev];
+            push_run(nsr[q - 1] + 1, nsr[q]);
+            while (ev < nrot && (evl ? cty[ev] : W.evq[s + ev]) == q && nseg <= MAXSEG) {
+                double dx = evl ? cz[ev] : W.evd[s + ev];
+                int x = evl ? ci[ev] : W.rot_pj[s + ev];
                 // walk from the front past every element with D > dx
                 int prev = -1, cseg = head, split_t = -1;
                 while (cseg >= 0) {
@@ -493,7 +542,7 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
                     int a = seg_a[cseg], b = seg_b[cseg];
                     // front order ds[b-1] >= ds[b-2] >= ...; t = first offset with ds <= dx
                     int lo = 0, hi = b - a;
-                    while (lo < hi) { int md2 = (lo + hi) >> 1; if (ds[b - 1 - md2] <= dx) hi = md2; else lo = md2 + 1; }
+                    while (lo < hi) { int md2 = (lo + hi) >> 1; if (dsr[b - 1 - md2] <= dx) hi = md2; else lo = md2 + 1; }
                     if (lo == b - a) { prev = cseg; cseg = seg_nx[cseg]; continue; }
                     split_t = lo;
                     break;
@@ -516,7 +565,7 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
                 ev++;
             }
         }
-        if (nseg <= MAXSEG) push_run(nsl[L - 1] + 1, n);
+        if (nseg <= MAXSEG) push_run(nsr[L - 1] + 1, n);
         if (nseg > MAXSEG) {
             sfallback = 1;
         } else {
@@ -534,38 +583,41 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
     }
     __syncthreads();
     if (sfallback == 2) {
-        // small-z column at sorted position j is the r-th pushed, r = j - #survivors before j
-        for (int j = tid; j < n; j += WG) {
-            if (rho * fabs(zs[j]) > tol) continue;
-            int lo = 0, hi = sL;
-            while (lo < hi) { int md2 = (lo + hi) >> 1; if (nsl[md2] < j) lo = md2 + 1; else hi = md2; }
-            int r = j - lo;
-            indxp[n - 1 - r] = indx[j];
-            ddef[n - 1 - r] = ds[j];
+        // no rotation: the r-th small-z position goes to INDXP(n-1-r)
+#pragma unroll 8
+        for (int q = sL + tid; q < n; q += NT) {
+            const int j = nsl[q];
+            indxp[q] = indx[j];
+            ddef[q] = ds[j];
         }
     } else if (!sfallback) {
-        for (int k = 0; k < snseg; k++) {
-            int g = seg_ord[k], P = seg_pos[k];
-            int a = seg_a[g], b = seg_b[g];
+        // every position of the region at once: its segment by binary search over
+        // the segment start positions (LDS), then its source
+        const int K0 = sk, ns = snseg;
+#pragma unroll 4
+        for (int p = K0 + tid; p < n; p += NT) {
+            int lo = 0, hi = ns - 1;
+            while (lo < hi) { int md2 = (lo + hi + 1) >> 1; if (seg_pos[md2] <= p) lo = md2; else hi = md2 - 1; }
+            const int g = seg_ord[lo];
+            const int a = seg_a[g], b = seg_b[g];
             if (a < 0) {
-                if (tid == 0) { indxp[P] = b; ddef[P] = seg_d[g]; }
+                indxp[p] = b;
+                ddef[p] = seg_d[g];
             } else {
-                for (int u = tid; u < b - a; u += WG) {
-                    int j = b - 1 - u;
-                    indxp[P + u] = indx[j];
-                    ddef[P + u] = ds[j];
-                }
+                const int j = b - 1 - (p - seg_pos[lo]);
+                indxp[p] = indx[j];
+                ddef[p] = ds[j];
             }
         }
     } else {
         // ---- fallback: the reference's scan over every column in sorted order
-        for (int i = tid; i < n; i += WG) coltyp[i] = (i < n1) ? 1 : 3;
+        for (int i = tid; i < n; i += NT) coltyp[i] = (i < n1) ? 1 : 3;
         int K2 = n + 1, PJ = 0, have = 0, nrot = 0, KK = 0;
         double zP = 0.0, DP = 0.0;
         for (int c0 = 0; c0 < n; c0 += DCH) {
             int cl = min(DCH, n - c0);
             __syncthreads();
-            for (int j = tid; j < cl; j += WG) { ci[j] = indx[c0 + j]; cz[j] = zs[c0 + j]; cd[j] = ds[c0 + j]; }
+            for (int j = tid; j < cl; j += NT) { ci[j] = indx[c0 + j]; cz[j] = zs[c0 + j]; cd[j] = ds[c0 + j]; }
             __syncthreads();
             if (tid == 0) {
                 for (int j = 0; j < cl; j++) {
@@ -623,15 +675,27 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
     __syncthreads();
     // CTOT and the type grouping (mdlaed2.f90:393-431) as a stable partition
     int c4[4] = {0, 0, 0, 0};
-    for (int j0 = w0; j0 < w1; j0 += 64) {
-        int j = j0 + lane;
-        int ct = (j < w1) ? coltyp[indxp[j] - 1] - 1 : -1;
-        for (int t = 0; t < 4; t++) c4[t] += __popcll(__ballot(ct == t));
+    for (int j0 = w0; j0 < w1; j0 += 64 * DU) {
+        int js[DU], ct[DU];
+#pragma unroll
+        for (int u = 0; u < DU; u++) {
+            const int j = j0 + 64 * u + lane;
+            js[u] = (j < w1) ? indxp[j] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < DU; u++) ct[u] = js[u] ? coltyp[js[u] - 1] - 1 : -1;
+#pragma unroll
+        for (int u = 0; u < DU; u++)
+            for (int t = 0; t < 4; t++) c4[t] += __popcll(__ballot(ct[u] == t));
     }
     if (lane == 0)
         for (int t = 0; t < 4; t++) wcnt[t][wv] = c4[t];
     __syncthreads();
-    if (tid < 4) scnt[tid] = wcnt[tid][0] + wcnt[tid][1] + wcnt[tid][2] + wcnt[tid][3];
+    if (tid < 4) {
+        int a = 0;
+        for (int w = 0; w < NW; w++) a += wcnt[tid][w];
+        scnt[tid] = a;
+    }
     __syncthreads();
     const int KF = n - scnt[3];
     const int hss = (KF >= hss_min_k);
@@ -640,21 +704,30 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
         pos[0] = 0; pos[1] = scnt[0]; pos[2] = pos[1] + scnt[1]; pos[3] = pos[2] + scnt[2];
         for (int t = 0; t < 4; t++)
             for (int w = 0; w < wv; w++) pos[t] += wcnt[t][w];
-        for (int j0 = w0; j0 < w1; j0 += 64) {
-            int j = j0 + lane;
-            int js = (j < w1) ? indxp[j] : 0;
-            int ct = (j < w1) ? coltyp[js - 1] - 1 : -1;
-            unsigned long long mk[4];
-            for (int t = 0; t < 4; t++) mk[t] = __ballot(ct == t);
-            if (ct >= 0) {
-                int p = pos[ct] + __popcll(mk[ct] & lt);
-                indx[p] = js;
-                W.indxc[s + p] = j + 1;
+        for (int j0 = w0; j0 < w1; j0 += 64 * DU) {
+            int js[DU], ct[DU];
+#pragma unroll
+            for (int u = 0; u < DU; u++) {
+                const int j = j0 + 64 * u + lane;
+                js[u] = (j < w1) ? indxp[j] : 0;
             }
-            for (int t = 0; t < 4; t++) pos[t] += __popcll(mk[t]);
+#pragma unroll
+            for (int u = 0; u < DU; u++) ct[u] = js[u] ? coltyp[js[u] - 1] - 1 : -1;
+#pragma unroll
+            for (int u = 0; u < DU; u++) {
+                unsigned long long mk[4];
+                for (int t = 0; t < 4; t++) mk[t] = __ballot(ct[u] == t);
+                if (ct[u] >= 0) {
+                    int p = pos[ct[u]] + __popcll(mk[ct[u]] & lt);
+                    indx[p] = js[u];
+                    W.indxc[s + p] = j0 + 64 * u + lane + 1;
+                }
+                for (int t = 0; t < 4; t++) pos[t] += __popcll(mk[t]);
+            }
         }
     } else {
-        for (int j = tid; j < n; j += WG) indx[j] = indxp[j];
+#pragma unroll 8
+        for (int j = tid; j < n; j += NT) indx[j] = indxp[j];
     }
     if (tid == 0) {
         o->k = KF;
@@ -662,6 +735,9 @@ __global__ __launch_bounds__(WG) void k_deflate(const Merge *mg, Work W, int cur
         o->hss = hss; o->nrot = sk2; o->flags = 0; o->rho = rho; o->tol = tol;
     }
 }
+
+template __global__ void k_deflate<256>(const Merge *, Work, int, MergeOut *, int, int);
+template __global__ void k_deflate<1024>(const Merge *, Work, int, MergeOut *, int, int);
 
 // Zero the rows of the merge block outside the support of every column that a
 // Givens rotation touches, then widen its support to the block.
@@ -1279,8 +1355,29 @@ __global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_
         dst = Z + zoff[p] + (long)(oc - P.off) * ldz[p] - P.off;
         r0 = P.off; r1 = P.off + P.n;
     }
-    for (int r = r0 + threadIdx.x; r < r1; r += WG) dst[r] = (r >= lo && r < hi) ? src[r] : 0.0;
-    if (threadIdx.x == 0) atomicAdd(rows_read, (unsigned long long)(hi - lo));
+    // streaming stores, 16 bytes per lane when the column start is 16-byte aligned
+    double *d0 = dst + r0;
+    const int nr = r1 - r0;
+    typedef double dbl2 __attribute__((ext_vector_type(2)));
+    if ((((unsigned long)d0) & 15) == 0) {
+        dbl2 *d2 = (dbl2 *)d0;
+        for (int q = threadIdx.x; q < (nr >> 1); q += WG) {
+            const int r = r0 + 2 * q;
+            dbl2 v;
+            v.x = (r >= lo && r < hi) ? src[r] : 0.0;
+            v.y = (r + 1 >= lo && r + 1 < hi) ? src[r + 1] : 0.0;
+            __builtin_nontemporal_store(v, d2 + q);
+        }
+        if ((nr & 1) && threadIdx.x == 0) {
+            const int r = r1 - 1;
+            __builtin_nontemporal_store((r >= lo && r < hi) ? src[r] : 0.0, dst + r);
+        }
+    } else {
+        for (int r = r0 + threadIdx.x; r < r1; r += WG)
+            __builtin_nontemporal_store((r >= lo && r < hi) ? src[r] : 0.0, dst + r);
+    }
+    // rows read, spread over 64 counters
+    if (threadIdx.x == 0) atomicAdd(rows_read + (g & 63), (unsigned long long)(hi - lo));
     (void)i;
 }
 

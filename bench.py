@@ -26,6 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+# HBM bytes per k_final launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated on a
+# 1 GiB device copy (profiles/k_final_n32768_uniform_traffic.md); keyed by (generator, n, seed)
+K_FINAL_TRAFFIC = {(2, 32768, 2): (220992 + 8400995) * 1024}
 GEN = {1: "d=2+U(-1e-3,1e-3), e=1+U(-1e-3,1e-3)", 2: "d,e~U(-1,1)"}
 
 
@@ -93,6 +97,77 @@ def cpu_baseline(cfg, seed, hss):
                        "n=32768 with the measured exponent %.2f" % (times[8192], times[16384], p))
 
 
+def bench_batch(args, np, torch, hsdc):
+    """512 problems in one batched solve (one launch per DC level over all merges of all problems)
+    against a loop of single-problem solves of the same problems."""
+    from hsspack_amd.workloads import batch_problems
+    probs = batch_problems(512, 256, 8192, 3, cfg=1)
+    ns = [len(d) for d, _ in probs]
+    offs = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
+    sq = [n * n for n in ns]
+    zoffs = np.concatenate([[0], np.cumsum(sq)[:-1]]).astype(np.int64)
+    dev = torch.device("cuda:0")
+    D0 = torch.tensor(np.concatenate([d for d, _ in probs]), device=dev)
+    E0 = torch.tensor(np.concatenate([np.append(e, 0.0) for _, e in probs]), device=dev)
+    D, E = torch.empty_like(D0), torch.empty_like(E0)
+    Z = torch.empty(int(sum(sq)), dtype=torch.float64, device=dev)
+    hss = (1024, 128, args.tol if args.tol != 1e-13 else 1e-15)
+    ctx = hsdc.Context(hss_min_k=hss[0], hss_leaf=hss[1], hss_tol=hss[2], batched=1, learned=1)
+    ctx.mlp_load(os.path.join(ROOT, "hsspack_amd", "data", "secular_mlp.bin"))
+
+    def batched():
+        D.copy_(D0)
+        E.copy_(E0)
+        rc = ctx.stedc_batched_dev(ns, offs, D, E, Z, zoffs)
+        if rc != 0:
+            raise RuntimeError("hsdc_stedc_batched_dev returned %d" % rc)
+
+    for _ in range(args.warmup):
+        batched()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batched()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    st = ctx.stats()
+    log("batched: %.1f ms, stats %s" % (t_batch * 1e3, st))
+    # loop of single-problem solves (each with its own per-level launches)
+    D.copy_(D0)
+    E.copy_(E0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, n in enumerate(ns):
+        o, zo = int(offs[k]), int(zoffs[k])
+        rc = ctx.stedc_dev(D[o:o + n], E[o:o + n], Z[zo:zo + n * n].view(n, n))
+        if rc != 0:
+            raise RuntimeError("hsdc_stedc_dev returned %d" % rc)
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    log("loop of single solves: %.1f ms" % (t_loop * 1e3))
+    batched()
+    torch.cuda.synchronize()
+    # orthogonality and residual of every 32nd problem
+    worst = [0.0, 0.0]
+    Dh = D.cpu().numpy()
+    for k in range(0, len(ns), 32):
+        n, o, zo = ns[k], int(offs[k]), int(zoffs[k])
+        d, e = probs[k]
+        orth, res = check_result(torch, d, e, Dh[o:o + n], Z[zo:zo + n * n].view(n, n))
+        worst = [max(worst[0], orth), max(worst[1], res)]
+    log("checked problems: orthogonality %.3e residual %.3e" % tuple(worst))
+    print(json.dumps({
+        "metric": "s to all eigenpairs of 512 tridiagonals (n log-uniform in [256, 8192]): batched vs loop",
+        "value": t_batch, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_batch * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "512 tridiagonals, n log-uniform in [256, 8192] seed 3, d=2+U(-1e-3,1e-3), "
+                               "e=1+U(-1e-3,1e-3), HSS for K>=1024, leaf 128, tol %g, device resident" % hss[2],
+                   "total_n": int(sum(ns)), "loop_s": t_loop, "speedup_vs_loop": t_loop / t_batch,
+                   "levels": st["levels"], "merges": st["merges"], "hss_merges": st["hss_merges"],
+                   "orthogonality_worst_of_16": worst[0], "residual_worst_of_16": worst[1]}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +180,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-13, help="HSS compression tolerance")
+    ap.add_argument("--batch", action="store_true",
+                    help="512 independent tridiagonals, n log-uniform in [256, 8192] (near-Toeplitz entries, "
+                         "seed 3): one batched solve against a loop of single-problem solves")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device three times (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -135,6 +213,8 @@ def main():
         return
     from hsspack_amd import hsdc
     from hsspack_amd.workloads import tridiag_cfg
+    if args.batch:
+        return bench_batch(args, np, torch, hsdc)
 
     n = args.n
     hss = (1024, 128, args.tol)
@@ -234,10 +314,16 @@ def main():
                    "orthogonality": orth, "residual": res,
                    "merges": st["merges"], "levels": st["levels"], "k_top": st["k_top"],
                    "roots": st["roots"], "deflated": st["deflated"],
-                   "hss_merges": st["hss_merges"]},
+                   "hss_merges": st["hss_merges"], "hss_rank_max": st["rank_max"],
+                   "hss_multiply": ({"flops": st["hss_mm_flops"], "ms": st["ms_hss_mm"],
+                                     "TFLOPs": st["hss_mm_flops"] / (st["ms_hss_mm"] * 1e-3) / 1e12,
+                                     "frac_of_fp64_peak": st["hss_mm_flops"] / (st["ms_hss_mm"] * 1e-3) / 1e12
+                                     / FP64_PEAK_TFS}
+                                    if st["hss_mm_flops"] and st["ms_hss_mm"] else None)},
         "roofline": {"kernel": "k_final (eigenvector reordering, mdlaed0.f90:355-362)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": K_FINAL_TRAFFIC.get((args.cfg, n, args.seed)),
                      "bytes_per_launch": fin_bytes, "avg_ms": fin_avg},
     }
     if not args.no_cpu:

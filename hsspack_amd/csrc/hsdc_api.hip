@@ -664,6 +664,15 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         HK(hipMemcpyAsync(c->tabs.p, all.data(), sizeof(BG) * all.size(), hipMemcpyHostToDevice, s));
         HK(hipStreamSynchronize(s));
     }
+    // flops of the multiply (2 nrow n k per GEMM: fastHssmmL's DGEMM calls exactly)
+    for (auto &v : steps)
+        for (auto &g : v) c->st.hss_mm_flops += 2LL * g.nrow * g.n * g.k;
+    hipEvent_t m0 = nullptr, m1 = nullptr;
+    if (c->prm.profile) {
+        HK(hipEventCreate(&m0));
+        HK(hipEventCreate(&m1));
+        HK(hipEventRecord(m0, s));
+    }
     size_t so = 0;
     for (auto &v : steps) {
         int rmax = 0, nmax = 0;
@@ -673,8 +682,16 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         so += v.size();
     }
     HK(hipGetLastError());
+    if (c->prm.profile) HK(hipEventRecord(m1, s));
     // the tables above are host vectors copied synchronously; wait before they go
     HK(hipStreamSynchronize(s));
+    if (c->prm.profile) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, m0, m1);
+        c->st.ms_hss_mm += ms;
+        (void)hipEventDestroy(m0);
+        (void)hipEventDestroy(m1);
+    }
     if (keep) *keep = P;
     return 0;
 }

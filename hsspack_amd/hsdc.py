@@ -47,7 +47,8 @@ class Stats(ctypes.Structure):
                 ("secular_iters", ctypes.c_longlong), ("deflated", ctypes.c_longlong),
                 ("k_top", _i), ("rank_max", _i), ("ms_total", _d), ("ms_final", _d),
                 ("final_bytes", ctypes.c_longlong), ("learned_roots", ctypes.c_longlong),
-                ("secular_wave_iters", ctypes.c_longlong), ("secular_crit_iters", ctypes.c_longlong)]
+                ("secular_wave_iters", ctypes.c_longlong), ("secular_crit_iters", ctypes.c_longlong),
+                ("hss_mm_flops", ctypes.c_longlong), ("ms_hss_mm", _d)]
 
 
 _LIB = None

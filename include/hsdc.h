@@ -55,6 +55,8 @@ typedef struct {
     long long learned_roots; /* roots started from the learned proposal */
     long long secular_wave_iters; /* sum over 64-root wavefronts of their largest count */
     long long secular_crit_iters; /* sum over DC levels of the largest count of the level */
+    long long hss_mm_flops;  /* flops of the HSS multiplies (2 L (|H| + |DD|) per merge) */
+    double ms_hss_mm;        /* their device time (events, profile = 1), ms */
 } hsdc_stats;
 
 void hsdc_default_params(hsdc_params *p);

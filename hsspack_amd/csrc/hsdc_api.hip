@@ -677,7 +677,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
     for (auto &v : steps) {
         int rmax = 0, nmax = 0;
         for (auto &g : v) { rmax = std::max(rmax, g.nrow); nmax = std::max(nmax, g.n); }
-        hipLaunchKernelGGL(k_bgemm, dim3(cdiv(rmax, 64), (unsigned)v.size(), cdiv(nmax, 64)), dim3(WGS), 0, s,
+        hipLaunchKernelGGL(k_bgemm, dim3(cdiv(rmax, 128), (unsigned)v.size(), cdiv(nmax, 64)), dim3(WGS), 0, s,
                            (const BG *)c->tabs.p + so);
         so += v.size();
     }
@@ -1411,7 +1411,7 @@ int hsdc_laed3(hsdc_ctx *c, int k, int n, int n1, double *d, double *q, int ldq,
     g[1].C = Y + n1; g[1].ldc = n; g[1].nrow = n2; g[1].n = k; g[1].k = n23; g[1].beta = 0; g[1].btrans = 1;
     CK(ensure(c->tabs, sizeof(g)));
     HK(hipMemcpy(c->tabs.p, g, sizeof(g), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_bgemm, dim3(cdiv(std::max(n1, n2), 64), 2, cdiv(k, 64)), dim3(WGS), 0, s,
+    hipLaunchKernelGGL(k_bgemm, dim3(cdiv(std::max(n1, n2), 128), 2, cdiv(k, 64)), dim3(WGS), 0, s,
                        (const BG *)c->tabs.p);
     HK(hipGetLastError());
     int info = 0;

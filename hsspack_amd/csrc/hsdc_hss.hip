@@ -804,54 +804,81 @@ __global__ __launch_bounds__(WG) void k_hss_init(const HssMerge *hm)
 
 // ---------------------------------------------------------------------------
 // Batched GEMM of the HSS multiply: work item (blockIdx.y) = one BG, block
-// (blockIdx.x, blockIdx.z) = one 64 x 64 tile of its output.  FP64 MFMA
-// 16x16x4, 4 waves of 32 x 32.
+// (blockIdx.x, blockIdx.z) = one 128 x 64 tile of its output.  FP64 MFMA 16x16x4.
 // ---------------------------------------------------------------------------
 typedef double d4h __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
 {
-    __shared__ double As[16][64 + 1];
-    __shared__ double Bs[16][64 + 1];
+    // block = 4 wavefronts stacked along the rows (128 x 64 outputs); each wavefront
+    // a 32 x 64 tile, 8 accumulators of 16 x 16.  Operands go straight from global
+    // memory (L2) into the MFMA source registers, one k-step of 4 ahead.
     const BG g = items[blockIdx.y];
-    const int r0 = blockIdx.x * 64, c0 = blockIdx.z * 64;
-    if (r0 >= g.nrow || c0 >= g.n) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
-    d4h acc[2][2];
+    const int r0 = blockIdx.x * 128 + wv * 32, c0 = blockIdx.z * 64;
+    if (r0 >= g.nrow || c0 >= g.n) return;  // no barriers below: wavefronts are independent
+    const int lr = lane & 15, lk = lane >> 4;
+    d4h acc[2][4];
     for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = (d4h){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < g.k; k0 += 16) {
-        for (int e = 0; e < 4; e++) {
-            int idx = tid + WG * e;
-            int r = idx & 63, kk = idx >> 6;
-            int t = k0 + kk;
-            As[kk][r] = (t < g.k && r0 + r < g.nrow) ? g.A[(long)(r0 + r) + (long)t * g.lda] : 0.0;
-            Bs[kk][r] = (t < g.k && c0 + r < g.n)
-                            ? (g.btrans ? g.B[(long)t * g.ldb + (c0 + r)] : g.B[(long)t + (long)(c0 + r) * g.ldb])
-                            : 0.0;
+        for (int b = 0; b < 4; b++) acc[a][b] = (d4h){0.0, 0.0, 0.0, 0.0};
+    // per-lane operand streams: A(row, t) at pa[t * lda], B(t, col) at pb[t * sb]
+    const bool ok0 = r0 + lr < g.nrow, ok1 = r0 + 16 + lr < g.nrow;
+    const double *pa0 = g.A + (ok0 ? r0 + lr : 0), *pa1 = g.A + (ok1 ? r0 + 16 + lr : 0);
+    const long sb = g.btrans ? g.ldb : 1;
+    const double *pb[4];
+    bool okb[4];
+    for (int q = 0; q < 4; q++) {
+        const int cc = c0 + 16 * q + lr;
+        okb[q] = cc < g.n;
+        pb[q] = g.B + (okb[q] ? (g.btrans ? (long)cc : (long)cc * g.ldb) : 0);
+    }
+    const long lda = g.lda;
+    const int K = g.k;
+    auto ldA = [&](const double *p, bool ok, int t) -> double { return (ok && t < K) ? p[t * lda] : 0.0; };
+    auto ldB = [&](int q, int t) -> double { return (okb[q] && t < K) ? pb[q][t * sb] : 0.0; };
+    // two k-steps (8 k) in registers, the next two in flight
+    double a[2][2], b[2][4];
+    for (int h = 0; h < 2; h++) {
+        a[h][0] = ldA(pa0, ok0, 4 * h + lk);
+        a[h][1] = ldA(pa1, ok1, 4 * h + lk);
+        for (int q = 0; q < 4; q++) b[h][q] = ldB(q, 4 * h + lk);
+    }
+    for (int k0 = 0; k0 < K; k0 += 8) {
+        double na[2][2], nb[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int tn = k0 + 8 + 4 * h + lk;
+            na[h][0] = ldA(pa0, ok0, tn);
+            na[h][1] = ldA(pa1, ok1, tn);
+#pragma unroll
+            for (int q = 0; q < 4; q++) nb[h][q] = ldB(q, tn);
         }
-        __syncthreads();
-        for (int kq = 0; kq < 16; kq += 4) {
-            int kl = kq + (lane >> 4);
-            double a0 = As[kl][wr + (lane & 15)], a1 = As[kl][wr + 16 + (lane & 15)];
-            double b0 = Bs[kl][wc + (lane & 15)], b1 = Bs[kl][wc + 16 + (lane & 15)];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    acc[x][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[h][x], b[h][q], acc[x][q], 0, 0, 0);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            a[h][0] = na[h][0];
+            a[h][1] = na[h][1];
+#pragma unroll
+            for (int q = 0; q < 4; q++) b[h][q] = nb[h][q];
         }
-        __syncthreads();
     }
     for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++)
+        for (int b = 0; b < 4; b++) {
+            const int col = c0 + b * 16 + lr;
+            if (col >= g.n) continue;
             for (int i = 0; i < 4; i++) {
-                int row = r0 + wr + a * 16 + (lane >> 4) + 4 * i;
-                int col = c0 + wc + b * 16 + (lane & 15);
-                if (row < g.nrow && col < g.n) {
+                int row = r0 + a * 16 + lk + 4 * i;
+                if (row < g.nrow) {
                     double *cp = g.C + (long)row + (long)col * g.ldc;
                     *cp = g.beta ? (*cp + acc[a][b][i]) : acc[a][b][i];
                 }
             }
+        }
 }
 
 // X(:, t) = Q2(:, t): the K surviving old eigenvectors of each HSS merge packed

@@ -565,7 +565,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
     auto bg = [](const double *A, long lda, const double *B, long ldb, double *C, long ldc, int nrow, int n, int k,
                  int beta) {
         BG g; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc; g.nrow = nrow; g.n = n; g.k = k;
-        g.beta = beta; g.btrans = 0; return g;
+        g.beta = beta; g.btrans = 0; g.A2 = nullptr; g.B2 = nullptr; g.lda2 = 0; g.ldb2 = 0; g.k2 = 0; return g;
     };
     // dense fallback merges: C = Cauchy matrix, Y = X * C
     std::vector<BG> dense_items;
@@ -632,9 +632,9 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         if (!vs.empty()) steps.push_back(vs);
         if (!vd.empty()) steps.push_back(vd);
     }
-    // leaves: Y_i = F_i V_i + X_i D_i
+    // leaves: Y_i = F_i V_i + X_i D_i, both products in one GEMM item
     {
-        std::vector<BG> v1, v2;
+        std::vector<BG> v1;
         for (size_t h = 0; h < P.size(); h++) {
             HssPlanM &Q = P[h];
             if (!Q.ok) continue;
@@ -646,14 +646,15 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
                 if (Q.T.c1[nd] != 0) continue;
                 int vr = Q.vsz[2 * (nd - 1)], vc = Q.vsz[2 * (nd - 1) + 1], md = Q.dsz[nd - 1];
                 long lx = Q.T.leaf_r0[nd];
-                v1.push_back(bg(F + (long)Q.lf[nd] * nr, nr, ar + Q.pv[nd - 1], std::max(vr, 1),
-                                Y + lx * sp[h].ldy, sp[h].ldy, (int)nr, vc, vr, 0));
-                v2.push_back(bg(sp[h].X + lx * sp[h].ldx, sp[h].ldx, Q.H.dd + Q.pd[nd - 1], md, Y + lx * sp[h].ldy,
-                                sp[h].ldy, (int)nr, md, md, 1));
+                // V_i is vr x vc with vc = md (the leaf's columns)
+                BG g = bg(F + (long)Q.lf[nd] * nr, nr, ar + Q.pv[nd - 1], std::max(vr, 1), Y + lx * sp[h].ldy,
+                          sp[h].ldy, (int)nr, md, (vc > 0) ? vr : 0, 0);
+                g.A2 = sp[h].X + lx * sp[h].ldx; g.lda2 = sp[h].ldx;
+                g.B2 = Q.H.dd + Q.pd[nd - 1]; g.ldb2 = md; g.k2 = md;
+                v1.push_back(g);
             }
         }
         if (!v1.empty()) steps.push_back(v1);
-        if (!v2.empty()) steps.push_back(v2);
     }
     size_t nitems = 0;
     for (auto &v : steps) nitems += v.size();
@@ -666,7 +667,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
     }
     // flops of the multiply (2 nrow n k per GEMM: fastHssmmL's DGEMM calls exactly)
     for (auto &v : steps)
-        for (auto &g : v) c->st.hss_mm_flops += 2LL * g.nrow * g.n * g.k;
+        for (auto &g : v) c->st.hss_mm_flops += 2LL * g.nrow * g.n * (g.k + g.k2);
     hipEvent_t m0 = nullptr, m1 = nullptr;
     if (c->prm.profile) {
         HK(hipEventCreate(&m0));
@@ -1409,6 +1410,7 @@ int hsdc_laed3(hsdc_ctx *c, int k, int n, int n1, double *d, double *q, int ldq,
     g[0].nrow = n1; g[0].n = k; g[0].k = n12; g[0].beta = 0; g[0].btrans = 1;
     g[1].A = Q2 + (long)n1 * n12; g[1].lda = std::max(n2, 1); g[1].B = S + (long)ctot[0] * k; g[1].ldb = k;
     g[1].C = Y + n1; g[1].ldc = n; g[1].nrow = n2; g[1].n = k; g[1].k = n23; g[1].beta = 0; g[1].btrans = 1;
+    for (int t = 0; t < 2; t++) { g[t].A2 = nullptr; g[t].B2 = nullptr; g[t].lda2 = g[t].ldb2 = 0; g[t].k2 = 0; }
     CK(ensure(c->tabs, sizeof(g)));
     HK(hipMemcpy(c->tabs.p, g, sizeof(g), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_bgemm, dim3(cdiv(std::max(n1, n2), 128), 2, cdiv(k, 64)), dim3(WGS), 0, s,

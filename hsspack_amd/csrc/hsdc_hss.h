@@ -44,7 +44,8 @@ struct HssCompact {
     int nlevel;        // the first nlevel entries are nodes of the level
 };
 
-// One GEMM of the batched HSS multiply: C(nrow x n) = A(nrow x k) B(k x n) + beta C
+// One GEMM of the batched HSS multiply: C(nrow x n) = A(nrow x k) B(k x n)
+// [+ A2(nrow x k2) B2(k2 x n)] + beta C
 struct BG {
     const double *A;
     const double *B;
@@ -53,4 +54,7 @@ struct BG {
     int nrow, n, k;
     int beta;
     int btrans;  // 1: B(t, j) = B[t * ldb + j]
+    const double *A2, *B2;  // optional second product (k2 > 0), B2 column-major
+    long lda2, ldb2;
+    int k2;
 };

@@ -820,26 +820,36 @@ __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
     d4h acc[2][4];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 4; b++) acc[a][b] = (d4h){0.0, 0.0, 0.0, 0.0};
-    // per-lane operand streams: A(row, t) at pa[t * lda], B(t, col) at pb[t * sb]
+    // per-lane operand streams: A(row, t) at pa[t * lda], B(t, col) at pb[t * sb]; the
+    // k range [K1, K1 + k2) reads the second product's operands
     const bool ok0 = r0 + lr < g.nrow, ok1 = r0 + 16 + lr < g.nrow;
-    const double *pa0 = g.A + (ok0 ? r0 + lr : 0), *pa1 = g.A + (ok1 ? r0 + 16 + lr : 0);
+    const long ro0 = ok0 ? r0 + lr : 0, ro1 = ok1 ? r0 + 16 + lr : 0;
+    const double *pa0 = g.A + ro0, *pa1 = g.A + ro1;
+    const double *qa0 = g.k2 ? g.A2 + ro0 : nullptr, *qa1 = g.k2 ? g.A2 + ro1 : nullptr;
     const long sb = g.btrans ? g.ldb : 1;
-    const double *pb[4];
+    const double *pb[4], *qb[4];
     bool okb[4];
     for (int q = 0; q < 4; q++) {
         const int cc = c0 + 16 * q + lr;
         okb[q] = cc < g.n;
         pb[q] = g.B + (okb[q] ? (g.btrans ? (long)cc : (long)cc * g.ldb) : 0);
+        qb[q] = g.k2 ? g.B2 + (okb[q] ? (long)cc * g.ldb2 : 0) : nullptr;
     }
-    const long lda = g.lda;
-    const int K = g.k;
-    auto ldA = [&](const double *p, bool ok, int t) -> double { return (ok && t < K) ? p[t * lda] : 0.0; };
-    auto ldB = [&](int q, int t) -> double { return (okb[q] && t < K) ? pb[q][t * sb] : 0.0; };
+    const long lda = g.lda, lda2 = g.lda2;
+    const int K1 = g.k, K = g.k + g.k2;
+    auto ldA = [&](const double *p, const double *p2, bool ok, int t) -> double {
+        if (!ok || t >= K) return 0.0;
+        return t < K1 ? p[t * lda] : p2[(t - K1) * lda2];
+    };
+    auto ldB = [&](int q, int t) -> double {
+        if (!okb[q] || t >= K) return 0.0;
+        return t < K1 ? pb[q][t * sb] : qb[q][t - K1];
+    };
     // two k-steps (8 k) in registers, the next two in flight
     double a[2][2], b[2][4];
     for (int h = 0; h < 2; h++) {
-        a[h][0] = ldA(pa0, ok0, 4 * h + lk);
-        a[h][1] = ldA(pa1, ok1, 4 * h + lk);
+        a[h][0] = ldA(pa0, qa0, ok0, 4 * h + lk);
+        a[h][1] = ldA(pa1, qa1, ok1, 4 * h + lk);
         for (int q = 0; q < 4; q++) b[h][q] = ldB(q, 4 * h + lk);
     }
     for (int k0 = 0; k0 < K; k0 += 8) {
@@ -847,8 +857,8 @@ __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int tn = k0 + 8 + 4 * h + lk;
-            na[h][0] = ldA(pa0, ok0, tn);
-            na[h][1] = ldA(pa1, ok1, tn);
+            na[h][0] = ldA(pa0, qa0, ok0, tn);
+            na[h][1] = ldA(pa1, qa1, ok1, tn);
 #pragma unroll
             for (int q = 0; q < 4; q++) nb[h][q] = ldB(q, tn);
         }

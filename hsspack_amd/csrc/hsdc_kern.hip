@@ -325,6 +325,8 @@ __global__ __launch_bounds__(WG) void k_sortmerge(const Merge *mg, Work W)
 #define DCH 1024
 #define MAXSEG 1536
 #define DU 8  // sub-tiles of 64 per step of the ballot passes
+#define SPW 1024  // words of the speculative pair flags (merges with < 32768 survivors)
+#define RUNS 256  // runs of kept survivors the scan may record
 template <int NT>
 __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur, MergeOut *mo,
                                                 int hss_min_k, int ds_cap)
@@ -336,7 +338,9 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
     __shared__ double seg_d[MAXSEG];
     __shared__ double cz[DCH], cd[DCH];
     __shared__ int ci[DCH], cty[DCH], lsn[DCH];
-    __shared__ int snr;
+    __shared__ int snr, snrun;
+    __shared__ unsigned int spb[SPW];
+    __shared__ int run_k[RUNS], run_q[RUNS], run_n[RUNS], run_o[RUNS + 1];
     Merge M = mg[blockIdx.x];
     const int n = M.n, n1 = M.n1, s = M.s, tid = threadIdx.x;
     const double *D = W.Dv[cur] + s;
@@ -422,7 +426,13 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
     }
     if (tid == 0) { sL = L; sfirst = L - 1; }
     __syncthreads();
-    // ---- 2. speculative close-pair test on unrotated values
+    // ---- 2. speculative close-pair test of every consecutive survivor pair on the
+    // unrotated values: bit q of spb = pair (q, q+1) would deflate.  Where the scan
+    // reaches a pair with its left survivor unmodified, this is the scan's own test.
+    const bool have_bits = L < 32 * SPW;
+    if (have_bits)
+        for (int w = tid; w < SPW; w += NT) spb[w] = 0u;
+    __syncthreads();
     int myfirst = L - 1;
     for (int q = tid; q + 1 < L; q += NT) {
         int p = nsl[q], r = nsl[q + 1];
@@ -431,7 +441,11 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
         double t = ds[r] - ds[p];
         cc = cc / tau;
         ss = -ss / tau;
-        if (fabs(t * cc * ss) <= tol) { myfirst = q; break; }
+        if (fabs(t * cc * ss) <= tol) {
+            myfirst = min(myfirst, q);
+            if (have_bits) atomicOr(&spb[q >> 5], 1u << (q & 31));
+            else break;
+        }
     }
     atomicMin(&sfirst, myfirst);
     __syncthreads();
@@ -447,7 +461,8 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
         W.dlamda[s + q] = ds[p]; W.w[s + q] = zs[p]; indxp[q] = indx[p];
     }
     // ---- 3. sequential continuation over survivors, staged through LDS in chunks
-    int K = first, nrot = 0, PJ = 0, ctP = 0;
+    int K = first, nrot = 0, PJ = 0, ctP = 0, nrun = 0;
+    bool mod = false;  // PJ's values changed by a rotation
     double zP = 0.0, DP = 0.0;
     if (tid == 0 && !sfallback) {
         int p0 = nsl[first];
@@ -474,6 +489,41 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
             if (tid == 0) {
                 for (int u = 0; u < cl; u++) {
                     const int q = c0 + u;
+                    if (!mod && have_bits && nrun < RUNS) {
+                        // PJ = survivor q-1 unmodified: pairs up to the next speculative
+                        // deflation keep their left survivor as it is; record them as a run
+                        // b: first deflating pair (b, b+1) with b >= q-1, or lim + 1
+                        int b = q - 1;
+                        const int lim = c0 + cl - 1;
+                        for (;;) {
+                            unsigned int wd = spb[b >> 5] & (~0u << (b & 31));
+                            if (wd) { b = (b & ~31) + __ffs(wd) - 1; break; }
+                            b = (b | 31) + 1;
+                            if (b > lim) break;
+                        }
+                        if (b > lim) b = lim + 1;
+                        const int len = b - (q - 1);  // kept: survivors q-1 .. b-1
+                        if (len > 0) {
+                            run_k[nrun] = K; run_q[nrun] = q - 1; run_n[nrun] = len;
+                            nrun++;
+                            K += len;
+                            // PJ becomes survivor b, unmodified (in this chunk or the next)
+                            if (b <= lim) {
+                                const int ub = b - c0;
+                                PJ = ci[ub]; zP = cz[ub]; DP = cd[ub]; ctP = cty[ub];
+                                u = ub;  // the loop's u++ makes the next pair (b, b+1)
+                            } else {
+                                const int ub = lim - c0;
+                                PJ = ci[ub]; zP = cz[ub]; DP = cd[ub]; ctP = cty[ub];
+                                u = cl;  // the rest of this chunk is in the run
+                                // b = lim + 1 is the first survivor of the next chunk; the
+                                // run ends at lim: survivor lim is PJ now, not kept yet
+                                run_n[nrun - 1] = len - 1;
+                                K -= 1;
+                            }
+                            continue;
+                        }
+                    }
                     int NJ = ci[u], ctN = cty[u];
                     double zN = cz[u], dN = cd[u];
                     double ss = zP, cc = zN;
@@ -491,16 +541,36 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
                         W.evq[s + nrot] = q; W.evd[s + nrot] = tP;
                         nrot++;
                         PJ = NJ; zP = tau; DP = dNn; ctP = ctN;
+                        mod = true;
                     } else {
                         W.dlamda[s + K] = DP; W.w[s + K] = zP; indxp[K] = PJ;
                         K++;
                         PJ = NJ; zP = zN; DP = dN; ctP = ctN;
+                        mod = false;
                     }
                 }
             }
         }
-        if (tid == 0) snr = nrot;
+        if (tid == 0) { snr = nrot; snrun = nrun; }
         __syncthreads();
+        // kept survivors of the runs, in parallel: run r keeps survivors run_q + o at
+        // positions run_k + o, o < run_n (run_o: prefix of run_n)
+        if (tid == 0) {
+            int a = 0;
+            for (int r = 0; r < snrun; r++) { run_o[r] = a; a += run_n[r]; }
+            run_o[snrun] = a;
+        }
+        __syncthreads();
+        {
+            const int nr = snrun, tot = run_o[snrun];
+#pragma unroll 4
+            for (int x = tid; x < tot; x += NT) {
+                int lo = 0, hi = nr - 1;
+                while (lo < hi) { int md2 = (lo + hi + 1) >> 1; if (run_o[md2] <= x) lo = md2; else hi = md2 - 1; }
+                const int o = x - run_o[lo], kk = run_k[lo] + o, j = nsl[run_q[lo] + o];
+                W.dlamda[s + kk] = ds[j]; W.w[s + kk] = zs[j]; indxp[kk] = indx[j];
+            }
+        }
         // survivors and rotation events into LDS for the replay below
         if (L <= DCH)
             for (int u = tid; u < L; u += NT) lsn[u] = nsl[u];

@@ -348,7 +348,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         metasz[h] = o;
         o += al64(sizeof(double) * std::max(dds, 1L));
         o += al64(sizeof(double) * (size_t)12 * ni * K);           // arena
-        o += al64(sizeof(double) * (size_t)3 * K * std::max(P[h].T.lm, 1));  // scratch
+        o += al64(sizeof(double) * (size_t)7 * K * std::max(P[h].T.lm, 1));  // scratch
         base[h] = tot;
         tot += o;
     }
@@ -384,7 +384,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         p += al64(sizeof(double) * std::max(dds, 1L));
         H.arena = (double *)p; H.arena_cap = (long)12 * ni * K;
         p += al64(sizeof(double) * (size_t)12 * ni * K);
-        H.scratch = (double *)p; H.scratch_cap = (long)3 * K * std::max(Q.T.lm, 1);
+        H.scratch = (double *)p; H.scratch_cap = (long)7 * K * std::max(Q.T.lm, 1);
         H.K = K; H.ltr = Q.T.ltr; H.lvl = Q.T.lvl;
     }
     // ---- host tables: merges, leaves (with leaf lr/lc and D offsets), level steps
@@ -959,7 +959,10 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         long nmx = 4 * ((long)c->wcap / 2 + 2);
         hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(nmx, WGS)), dim3(WGS), 0, s, W.maux, nmx, 0ULL);
     }
-    hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(64), 0, s, dleaves, (int)nl, dprobs, dd, de, W, cur);
+    int lmax = 1;
+    for (auto &Lf : pl.leaves) lmax = std::max(lmax, Lf.m);
+    hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(64), sizeof(double) * lmax * lmax, s, dleaves, (int)nl, dprobs, dd, de,
+                       W, cur);
     HK(hipGetLastError());
     // the host tables above live in c->pin, which run_merges reuses: wait for the upload
     HK(hipStreamSynchronize(s));

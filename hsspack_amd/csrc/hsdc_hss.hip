@@ -125,18 +125,44 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
         s_N = *H.lca - nic;
         // a merge whose construction already failed is finished densely; skip it
         s_flag = (*H.err != 0 || M > HSS_MAXM || M < 1 || nic < 1 || s_N < 0) ? 1 : 0;
-        s_off = s_flag ? -1 : scratch_take(H, 3L * (s_N > 0 ? s_N : 1));
+        s_off = s_flag ? -1 : scratch_take(H, 7L * (s_N > 0 ? s_N : 1));
         if (s_flag) atomicOr(H.err, 1);
     }
     __syncthreads();
     if (s_flag || s_off < 0) return;
     const int M = s_M, N = s_N, l0 = lr[2 * (node - 1)], c0 = lc[2 * (node - 1)], c1e = lc[2 * (node - 1) + 1];
-    double *D = H.scratch + s_off, *V = D + (N > 0 ? N : 1);
-    int *PU = (int *)(V + (N > 0 ? N : 1));
+    const int NS = N > 0 ? N : 1;
+    double *D = H.scratch + s_off, *V = D + NS;
+    int *PU = (int *)(V + NS);
+    // per column the pole/difference values its accurate differences read
+    // (FF(pw), FF(pw+1), DIFL(pw), DIFR(pw) with pw = PU(j)), gathered once
+    double *CF0 = V + 2 * NS, *CF1 = CF0 + NS, *CL = CF1 + NS, *CR = CL + NS;
     for (int j = tid; j < N; j += WG) {
         int src = (j < c0 - 1) ? j + 1 : j + 1 + (c1e - c0 + 1);
-        D[j] = A1(H.D, src); V[j] = A1(H.V, src); PU[j] = A1(H.PU, src);
+        D[j] = A1(H.D, src); V[j] = A1(H.V, src);
+        const int pw = A1(H.PU, src);
+        PU[j] = pw;
+        CF0[j] = A1(H.FF, pw); CF1[j] = (pw < H.K) ? A1(H.FF, pw + 1) : 0.0;
+        CL[j] = A1(H.DIFL, pw); CR[j] = A1(H.DIFR, pw);
     }
+    // fd(H, pd, PU(j)) and dd(H, PU(j), pk) from the gathered values: same operations
+    auto fdj = [&](int pd, int j) -> double {
+        return (pd <= A1(PU, j)) ? (A1(H.FF, pd) - A1(CF0, j)) - A1(CL, j) : (A1(H.FF, pd) - A1(CF1, j)) - A1(CR, j);
+    };
+    auto ddj = [&](int j, int pk) -> double {
+        return (A1(PU, j) < pk) ? ((A1(CF1, j) - A1(H.FF, pk)) + A1(CR, j)) - A1(H.DIFL, pk)
+                                : ((A1(CF0, j) - A1(H.FF, pk + 1)) + A1(CL, j)) - A1(H.DIFR, pk);
+    };
+    auto swapc = [&](int a, int c) {
+        double t;
+        t = A1(D, a); A1(D, a) = A1(D, c); A1(D, c) = t;
+        t = A1(V, a); A1(V, a) = A1(V, c); A1(V, c) = t;
+        int u = A1(PU, a); A1(PU, a) = A1(PU, c); A1(PU, c) = u;
+        t = A1(CF0, a); A1(CF0, a) = A1(CF0, c); A1(CF0, c) = t;
+        t = A1(CF1, a); A1(CF1, a) = A1(CF1, c); A1(CF1, c) = t;
+        t = A1(CL, a); A1(CL, a) = A1(CL, c); A1(CL, c) = t;
+        t = A1(CR, a); A1(CR, a) = A1(CR, c); A1(CR, c) = t;
+    };
     for (int i = tid; i < M; i += WG) {
         PL[i] = A1(H.PL, l0 + i);
         A[i] = A1(H.U, l0 + i);
@@ -163,8 +189,9 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
             VI cl = block_argmax(x, red);
             VI y = {0.0, 0};
             int fk = A1(PL, k);
+            #pragma unroll 8
             for (int j = k + tid; j <= N; j += WG) {
-                double v = fabs(A1(V, j) / fd(H, fk, A1(PU, j)));
+                double v = fabs(A1(V, j) / fdj(fk, j));
                 y = vi_better(y, VI{v, j - k + 1});
             }
             VI rw = block_argmax(y, red);
@@ -188,8 +215,9 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
                         __syncthreads();
                         VI y2 = {0.0, 0};
                         int fk2 = A1(PL, k);
+                        #pragma unroll 8
                         for (int j = k + tid; j <= N; j += WG) {
-                            double v = fabs(A1(V, j) / fd(H, fk2, A1(PU, j)));
+                            double v = fabs(A1(V, j) / fdj(fk2, j));
                             y2 = vi_better(y2, VI{v, j - k + 1});
                         }
                         jjU = block_argmax(y2, red).i;
@@ -198,12 +226,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
                     }
                     int jj = jjU + k - 1;
                     __syncthreads();
-                    if (tid == 0) {
-                        double t;
-                        t = A1(D, k); A1(D, k) = A1(D, jj); A1(D, jj) = t;
-                        t = A1(V, k); A1(V, k) = A1(V, jj); A1(V, jj) = t;
-                        int u = A1(PU, k); A1(PU, k) = A1(PU, jj); A1(PU, jj) = u;
-                    }
+                    if (tid == 0) swapc(k, jj);
                     __syncthreads();
                     VI x2 = {0.0, 0};
                     int pwk2 = A1(PU, k);
@@ -225,11 +248,10 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
             VI best = {0.0, 0};
             VI bestrow = {0.0, 0};
             for (int j = k + tid; j <= N; j += WG) {
-                int pw = A1(PU, j);
                 double mv = -DBL_MAX;
                 int mi = 0;
                 for (int i = k; i <= M; i++) {
-                    double v = fabs(A1(Z, i) / fd(H, A1(PL, i), pw));
+                    double v = fabs(A1(Z, i) / fdj(A1(PL, i), j));
                     if (mi == 0 || v > mv) { mv = v; mi = i - k + 1; }
                 }
                 double jt = fabs(mv * A1(V, j));
@@ -251,9 +273,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
                 t = A1(A, k); A1(A, k) = A1(A, a); A1(A, a) = t;
                 int u = A1(PL, k); A1(PL, k) = A1(PL, a); A1(PL, a) = u;
                 u = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = u;
-                t = A1(D, k); A1(D, k) = A1(D, b); A1(D, b) = t;
-                t = A1(V, k); A1(V, k) = A1(V, b); A1(V, b) = t;
-                u = A1(PU, k); A1(PU, k) = A1(PU, b); A1(PU, b) = u;
+                swapc(k, b);
             }
             __syncthreads();
             ukk = A1(Z, k) * A1(V, k) / fd(H, A1(PL, k), A1(PU, k));
@@ -268,9 +288,10 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
             temp = (A1(F, j) - Fk) / temp;
             A1(Z, j) = A1(Z, j) * temp;
         }
+        #pragma unroll 8
         for (int j = k + 1 + tid; j <= N; j += WG) {
-            double temp = dd(H, A1(PU, j), puk);
-            double temp2 = -1.0 * fd(H, plk, A1(PU, j));
+            double temp = ddj(j, puk);
+            double temp2 = -1.0 * fdj(plk, j);
             temp = temp / temp2;
             A1(V, j) = A1(V, j) * temp;
         }
@@ -311,9 +332,10 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
                 __syncthreads();
                 const double Fii = A1(F, ii), Fjj = A1(F, jj);
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
+                #pragma unroll 8
                 for (int j = k + 1 + tid; j <= N; j += WG) {
-                    double temp = -1.0 * fd(H, plii, A1(PU, j));
-                    double temp2 = -1.0 * fd(H, pljj, A1(PU, j));
+                    double temp = -1.0 * fdj(plii, j);
+                    double temp2 = -1.0 * fdj(pljj, j);
                     temp = temp / temp2;
                     A1(V, j) = A1(V, j) * temp;
                 }
@@ -455,6 +477,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
             VI cl = block_argmax(x, red);
             VI y = {0.0, 0};
             int plk = A1(PL, k);
+            #pragma unroll 8
             for (int j = k + tid; j <= N; j += WG) {
                 double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk));
                 y = vi_better(y, VI{v, j - k + 1});
@@ -485,6 +508,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
                         __syncthreads();
                         VI y2 = {0.0, 0};
                         int plk2 = A1(PL, k);
+                        #pragma unroll 8
                         for (int j = k + tid; j <= N; j += WG) {
                             double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk2));
                             y2 = vi_better(y2, VI{v, j - k + 1});
@@ -521,6 +545,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
             // ---- CauchyPivtEigm_CP
             VI best = {0.0, 0};
             int bestrow = 0;
+            #pragma unroll 8
             for (int j = k + tid; j <= N; j += WG) {
                 int pu = A1(PU, j);
                 double mv = 0.0;
@@ -562,6 +587,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
             temp = temp / temp2;
             A1(Z, j) = A1(Z, j) * temp;
         }
+        #pragma unroll 8
         for (int j = k + 1 + tid; j <= N; j += WG) {
             double temp = fd(H, A1(PU, j), plk);
             A1(V, j) = A1(V, j) * ((A1(F, j) - Fk) / temp);
@@ -606,6 +632,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
                 const int ii = win.i, jj = IP[ii - 1] + k;
                 __syncthreads();
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
+                #pragma unroll 8
                 for (int j = k + 1 + tid; j <= N; j += WG) {
                     double temp = fd(H, A1(PU, j), plii);
                     double temp2 = fd(H, A1(PU, j), pljj);

@@ -48,12 +48,18 @@ __device__ __forceinline__ int find_seg(const int *start, int n, int b)
 __global__ __launch_bounds__(64) void k_leaf(const Leaf *leaves, int nleaf, const Prob *probs,
                                              const double *din, const double *ein, Work W, int cur)
 {
-    __shared__ double sd[64], se[64], sz[64 * 64];
+    // one wavefront per leaf: lane r owns row r of the eigenvector block; every lane
+    // runs the same scalar QL iteration on its own copy of d and e (identical values),
+    // so after the initial fill no lane reads another lane's data and the iteration
+    // needs no barriers
+    extern __shared__ double lds_leaf[];
+    __shared__ double sd[64], se[64];
     int li = blockIdx.x;
     if (li >= nleaf) return;
     Leaf L = leaves[li];
     Prob P = probs[L.prob];
     const int n = L.m, r = threadIdx.x;
+    double *sz = lds_leaf;  // n x n, column-major
     for (int i = r; i < n * n; i += 64) sz[i] = ((i % n) == (i / n)) ? 1.0 : 0.0;
     if (r < n) {
         sd[r] = din[L.s + r];
@@ -79,15 +85,11 @@ __global__ __launch_bounds__(64) void k_leaf(const Leaf *leaves, int nleaf, cons
             for (int i = m - 1; i >= l; i--) {
                 double f = s * se[i], b = c * se[i];
                 rr = lapy2_d(f, g);
-                __syncthreads();
                 se[i + 1] = rr;
-                __syncthreads();
                 if (rr == 0.0) {
                     double t = sd[i + 1] - p;
-                    __syncthreads();
                     sd[i + 1] = t;
                     se[m] = 0.0;
-                    __syncthreads();
                     underflow = 1;
                     break;
                 }
@@ -97,9 +99,7 @@ __global__ __launch_bounds__(64) void k_leaf(const Leaf *leaves, int nleaf, cons
                 rr = (sd[i] - g) * s + 2.0 * c * b;
                 p = s * rr;
                 double dn = g + p;
-                __syncthreads();
                 sd[i + 1] = dn;
-                __syncthreads();
                 g = c * rr - b;
                 if (r < n) {
                     double zi = sz[r + i * n], fz = sz[r + (i + 1) * n];
@@ -109,11 +109,9 @@ __global__ __launch_bounds__(64) void k_leaf(const Leaf *leaves, int nleaf, cons
             }
             if (underflow) continue;
             double dl = sd[l] - p;
-            __syncthreads();
             sd[l] = dl;
             se[l] = g;
             se[m] = 0.0;
-            __syncthreads();
         }
     }
     // selection sort, ascending
@@ -124,10 +122,8 @@ __global__ __launch_bounds__(64) void k_leaf(const Leaf *leaves, int nleaf, cons
             if (sd[j] < p) { k = j; p = sd[j]; }
         if (k != i) {
             double di = sd[i];
-            __syncthreads();
             sd[k] = di;
             sd[i] = p;
-            __syncthreads();
             if (r < n) {
                 double t = sz[r + i * n];
                 sz[r + i * n] = sz[r + k * n];

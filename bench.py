@@ -114,6 +114,10 @@ def bench_batch(args, np, torch, hsdc):
     hss = (1024, 128, args.tol if args.tol != 1e-13 else 1e-15)
     ctx = hsdc.Context(hss_min_k=hss[0], hss_leaf=hss[1], hss_tol=hss[2], batched=1, learned=1)
     ctx.mlp_load(os.path.join(ROOT, "hsspack_amd", "data", "secular_mlp.bin"))
+    D.copy_(D0)
+    E.copy_(E0)
+    leaf, smls, ttimes = ctx.autotune(ns, offs, D, E, Z, zoffs)
+    log("autotune: HSS leaf %d, smlsiz %d, candidate ms %s" % (leaf, smls, list(np.round(ttimes, 1))))
 
     def batched():
         D.copy_(D0)
@@ -162,9 +166,14 @@ def bench_batch(args, np, torch, hsdc):
         "ms_per_step": t_batch * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "512 tridiagonals, n log-uniform in [256, 8192] seed 3, d=2+U(-1e-3,1e-3), "
-                               "e=1+U(-1e-3,1e-3), HSS for K>=1024, leaf 128, tol %g, device resident" % hss[2],
+                               "e=1+U(-1e-3,1e-3), HSS for K>=1024, autotuned leaves, tol %g, device resident"
+                               % hss[2],
                    "total_n": int(sum(ns)), "loop_s": t_loop, "speedup_vs_loop": t_loop / t_batch,
                    "levels": st["levels"], "merges": st["merges"], "hss_merges": st["hss_merges"],
+                   "autotuned": {"hss_leaf": leaf, "smlsiz": smls,
+                                 "candidates_ms": {"smlsiz %d leaf %d" % (a, b): float(ttimes[3 * i + j])
+                                                   for i, a in enumerate((25, 40))
+                                                   for j, b in enumerate((64, 128, 256))}},
                    "orthogonality_worst_of_16": worst[0], "residual_worst_of_16": worst[1]}}), flush=True)
 
 

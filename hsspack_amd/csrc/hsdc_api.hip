@@ -1236,6 +1236,65 @@ int hsdc_stedc_batched_dev(hsdc_ctx *c, int nprob, const int *ns, const long *of
     return info;
 }
 
+// Autotuner: one timed batched solve per candidate (HSS leaf size x DC leaf bound) on
+// copies of the caller's problems; the fastest candidate goes into the context's
+// parameters.  Inputs are restored; d_z holds the last trial's vectors.
+int hsdc_autotune(hsdc_ctx *c, int nprob, const int *ns, const long *offs, double *d_d, double *d_e,
+                  double *d_z, const long *zoffs, int *best_leaf, int *best_smlsiz, double *times)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (nprob < 1) return -2;
+    long tot = 0;
+    for (int p = 0; p < nprob; p++) {
+        if (ns[p] < 1) return -3;
+        tot = std::max(tot, offs[p] + ns[p]);
+    }
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    DevBuf keep;
+    if (ensure(keep, sizeof(double) * 2 * (size_t)tot)) return HSDC_ERR_NOMEM;
+    double *kd = (double *)keep.p, *ke = kd + tot;
+    int rc = 0;
+    const int leaves[3] = {64, 128, 256}, smls[2] = {25, 40};
+    hsdc_params p0 = c->prm, best = c->prm;
+    double tbest = 1e300;
+    int t = 0;
+    if (hipMemcpy(kd, d_d, sizeof(double) * tot, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(ke, d_e, sizeof(double) * tot, hipMemcpyDeviceToDevice) != hipSuccess)
+        rc = HSDC_ERR_HIP;
+    for (int a = 0; a < 2 && rc == 0; a++)
+        for (int b = 0; b < 3 && rc == 0; b++, t++) {
+            hsdc_params q = p0;
+            q.smlsiz = smls[a];
+            q.hss_leaf = leaves[b];
+            q.batched = 1;
+            q.profile = 0;
+            c->prm = q;
+            // one untimed solve brings the buffers to size, the second is timed
+            for (int rep = 0; rep < 2 && rc == 0; rep++) {
+                if (hipMemcpy(d_d, kd, sizeof(double) * tot, hipMemcpyDeviceToDevice) != hipSuccess ||
+                    hipMemcpy(d_e, ke, sizeof(double) * tot, hipMemcpyDeviceToDevice) != hipSuccess) {
+                    rc = HSDC_ERR_HIP;
+                    break;
+                }
+                rc = hsdc_stedc_batched_dev(c, nprob, ns, offs, d_d, d_e, d_z, zoffs);
+            }
+            if (rc) break;
+            const double ms = c->st.ms_total;
+            if (times) times[t] = ms;
+            if (ms < tbest) { tbest = ms; best = q; }
+        }
+    (void)hipMemcpy(d_d, kd, sizeof(double) * tot, hipMemcpyDeviceToDevice);
+    (void)hipMemcpy(d_e, ke, sizeof(double) * tot, hipMemcpyDeviceToDevice);
+    hipFree(keep.p);
+    c->prm = p0;
+    if (rc) return rc;
+    c->prm.smlsiz = best.smlsiz;
+    c->prm.hss_leaf = best.hss_leaf;
+    if (best_leaf) *best_leaf = best.hss_leaf;
+    if (best_smlsiz) *best_smlsiz = best.smlsiz;
+    return 0;
+}
+
 // one merge of order n held at global index 0 of the work arrays
 static int single_merge_setup(hsdc_ctx *c, int n, int n1, double rho, Merge **dm, int **dchunk)
 {

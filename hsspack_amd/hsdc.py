@@ -27,7 +27,7 @@ HSDC_ERR_NODEVICE = -1002
 # every symbol include/hsdc.h declares
 EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_set_params",
            "hsdc_get_stats", "hsdc_set_stream", "hsdc_stedc", "hsdc_stedc_dev",
-           "hsdc_stedc_batched_dev", "hsdc_laed0", "hsdc_deflate", "hsdc_secular", "hsdc_laed3", "hsdc_loewner",
+           "hsdc_stedc_batched_dev", "hsdc_autotune", "hsdc_laed0", "hsdc_deflate", "hsdc_secular", "hsdc_laed3", "hsdc_loewner",
            "hsdc_hss_evc", "hsdc_hss_dense", "hsdc_mlp_set_weights", "hsdc_mlp_get_weights",
            "hsdc_mlp_load", "hsdc_mlp_harvest", "hsdc_mlp_samples", "hsdc_mlp_grad", "hsdc_mlp_train"]
 
@@ -77,6 +77,8 @@ def lib():
                                  _P(_i), _i]
         L.hsdc_stedc_dev.argtypes = [_vp, _i, _vp, _vp, _vp, _i]
         L.hsdc_laed0.argtypes = [_vp, _i, _i, _i, _P(_d), _P(_d), _P(_d), _i]
+        L.hsdc_autotune.argtypes = [_vp, _i, _P(_i), _P(ctypes.c_long), _vp, _vp, _vp, _P(ctypes.c_long),
+                                    _P(_i), _P(_i), _P(_d)]
         L.hsdc_stedc_batched_dev.argtypes = [_vp, _i, _P(_i), _P(ctypes.c_long), _vp, _vp, _vp,
                                              _P(ctypes.c_long)]
         L.hsdc_deflate.argtypes = [_vp, _i, _i, _P(_d), _P(_d), _P(_i), _d, _i, _P(_i), _P(_d),
@@ -224,6 +226,23 @@ class Context:
                                             offs.ctypes.data_as(_P(ctypes.c_long)), d.data_ptr(),
                                             e.data_ptr(), z.data_ptr(),
                                             zoffs.ctypes.data_as(_P(ctypes.c_long)))
+
+    def autotune(self, ns, offs, d, e, z, zoffs):
+        """Picks the HSS leaf size and DC leaf bound by timing batched solves; returns
+        (hss_leaf, smlsiz, times of the 6 candidates)."""
+        ns = np.ascontiguousarray(ns, np.int32)
+        offs = np.ascontiguousarray(offs, np.int64)
+        zoffs = np.ascontiguousarray(zoffs, np.int64)
+        bl, bs = _i(0), _i(0)
+        times = np.zeros(6)
+        rc = lib().hsdc_autotune(self._h, len(ns), _ip(ns), offs.ctypes.data_as(_P(ctypes.c_long)),
+                                 d.data_ptr(), e.data_ptr(), z.data_ptr(),
+                                 zoffs.ctypes.data_as(_P(ctypes.c_long)), ctypes.byref(bl), ctypes.byref(bs),
+                                 _dp(times))
+        if rc != 0:
+            raise RuntimeError("hsdc_autotune: %d" % rc)
+        self.params.hss_leaf, self.params.smlsiz = bl.value, bs.value
+        return bl.value, bs.value, times
 
     # ---- one merge ----
     def deflate(self, d, z, indxq, rho, n1, hss_min_k=1 << 30):

@@ -90,6 +90,15 @@ int hsdc_stedc_dev(hsdc_ctx *ctx, int n, double *d_d, double *d_e, double *d_z, 
 int hsdc_stedc_batched_dev(hsdc_ctx *ctx, int nprob, const int *ns, const long *offs,
                            double *d_d, double *d_e, double *d_z, const long *zoffs);
 
+/* Autotuner (no counterpart in HSSPACK, whose leaf sizes are compile-time constants:
+ * ILAENV(9) = 25, Ni = 200 at mdlaed3.f90:253).  Times one batched solve of the given
+ * problems (arguments as hsdc_stedc_batched_dev) for each HSS leaf size in {64, 128, 256}
+ * and DC leaf bound smlsiz in {25, 40}, and keeps the fastest pair in the context's
+ * parameters.  d_d/d_e are restored; d_z is overwritten.  times[6] (may be NULL)
+ * receives each candidate's device time in ms, smlsiz-major. */
+int hsdc_autotune(hsdc_ctx *ctx, int nprob, const int *ns, const long *offs, double *d_d, double *d_e,
+                  double *d_z, const long *zoffs, int *best_leaf, int *best_smlsiz, double *times);
+
 /* ---- single merge kernels (host pointers) ---- */
 
 /* MDLAED2's deflation (mdlaed2.f90:149-409) on the merge data alone.  Inputs d(n),

@@ -220,3 +220,39 @@ def test_stedc_hss_branch_vs_oracle(torch_dev, ctx, cfg, n, ni):
     if cfg == 1:
         assert st["hss_merges"] > 0
         assert st["rank_max"] > 0
+
+
+def test_autotune_picks_a_candidate_and_restores_inputs(torch_dev, ctx):
+    torch = torch_dev
+    sizes = [1500, 700, 1200]
+    ds, es = [], []
+    for i, n in enumerate(sizes):
+        d, e = tridiag_cfg(1, n, 500 + i)
+        ds.append(d)
+        es.append(np.append(e, 0.0))
+    offs = np.cumsum([0] + sizes[:-1])
+    zoffs = np.cumsum([0] + [n * n for n in sizes[:-1]])
+    dev = torch.device("cuda:0")
+    D = torch.tensor(np.concatenate(ds), device=dev)
+    E = torch.tensor(np.concatenate(es), device=dev)
+    D0, E0 = D.clone(), E.clone()
+    Z = torch.zeros(int(sum(n * n for n in sizes)), dtype=torch.float64, device=dev)
+    ctx.set_params(hss_min_k=1024, hss_tol=1e-15)
+    try:
+        leaf, smls, times = ctx.autotune(sizes, offs, D, E, Z, zoffs)
+        assert leaf in (64, 128, 256) and smls in (25, 40)
+        assert np.all(times > 0) and times[[25, 40].index(smls) * 3 + [64, 128, 256].index(leaf)] == times.min()
+        assert torch.equal(D, D0) and torch.equal(E, E0)
+        st = ctx.stats()
+        ctx.set_params()  # the tuned parameters are the context's now
+        assert ctx.params.hss_leaf == leaf and ctx.params.smlsiz == smls
+        assert ctx.stedc_batched_dev(sizes, offs, D, E, Z, zoffs) == 0
+        torch.cuda.synchronize()
+        Dh, Zh = D.cpu().numpy(), Z.cpu().numpy()
+        for i, n in enumerate(sizes):
+            lam = Dh[offs[i]:offs[i] + n]
+            z = Zh[zoffs[i]:zoffs[i] + n * n].reshape(n, n).T
+            check_eig(ds[i], es[i][:n - 1], lam, z)
+        del st
+    finally:
+        ctx.set_params(hss_min_k=1500, hss_tol=float(np.float32(1e-17)), hss_leaf=200, smlsiz=25)

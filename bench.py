@@ -277,6 +277,17 @@ def main():
     classical_s = (time.perf_counter() - tc0) / 3
     sec_classical = secstats(ctx.stats())
     log("batched, classical start: %.2f ms/solve, %s" % (classical_s * 1000, sec_classical))
+    # ablation: learned start, the host reading every level's K before its launches
+    ctx.set_params(learned=1, level_sync=1)
+    step()
+    torch.cuda.synchronize()
+    ts0 = time.perf_counter()
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    sync_s = (time.perf_counter() - ts0) / 3
+    ctx.set_params(level_sync=0)
+    log("batched, host round trip per level: %.2f ms/solve" % (sync_s * 1000))
 
     plain_s = None
     if not args.no_plain:
@@ -315,10 +326,11 @@ def main():
         "config": {"workload": "tridiagonal DC eigensolve, n=%d, %s seed %d, HSS leaf 128, "
                                "tol %g, HSS for K>=1024, all eigenpairs, device resident"
                                % (n, GEN[args.cfg], args.seed, args.tol),
-                   "n": n, "schedule": "batched (one launch per DC level) + learned secular start",
+                   "n": n, "schedule": "batched (one launch per DC level, whole solve queued, launch sizes "
+                                               "and tables derived on the device) + learned secular start",
                    "plain_s": plain_s,
                    "speedup_vs_plain": (plain_s / (ms_step / 1000.0)) if plain_s else None,
-                   "batched_classical_s": classical_s,
+                   "batched_classical_s": classical_s, "batched_level_sync_s": sync_s,
                    "secular_learned": sec_learned, "secular_classical": sec_classical,
                    "orthogonality": orth, "residual": res,
                    "merges": st["merges"], "levels": st["levels"], "k_top": st["k_top"],

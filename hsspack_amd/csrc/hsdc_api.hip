@@ -36,12 +36,13 @@ __global__ void k_secstart(const Merge *, const int *, int, const MergeOut *, Wo
 __global__ void k_loewner(const Merge *, const int *, int, const MergeOut *, Work);
 __global__ void k_sbuild(const Merge *, const int *, int, const MergeOut *, const long *, Work, double *);
 __global__ void k_alpha(const Merge *, const int *, int, const MergeOut *, Work);
-__global__ void k_gemm(const Merge *, const GTile *, const Prob *, const MergeOut *, const long *,
-                       const long *, Work, int, const double *, double *);
+__global__ void k_gemm(const Merge *, const GTile *, int, const int *, int, const Prob *, const MergeOut *,
+                       const long *, const long *, Work, int, const double *, double *);
 __global__ void k_newcols(const Merge *, const int *, int, const MergeOut *, Work, int);
 __global__ void k_scatter(const Merge *, const int *, int, const MergeOut *, const long *, const Prob *,
-                          Work, int, const double *);
-__global__ void k_lamrg(const Merge *, const MergeOut *, Work, int);
+                          Work, int, const double *, int);
+__global__ void k_lamrg(const Merge *, const int *, int, const MergeOut *, Work, int);
+__global__ void k_lvltab(const Merge *, int, const MergeOut *, Work, LvlTab, long, long, int *);
 __global__ void k_final(const Prob *, const int *, Work, int, const int *, double *, const long *,
                         const long *, int, double *, const double *, unsigned long long *);
 __global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, long, double *, long);
@@ -122,7 +123,7 @@ struct hsdc_ctx {
     std::vector<double> evc_gen;
     int evc_k = 0, evc_ni = 0;
     double evc_tol = 0.0;
-    PinBuf pin, pin2;
+    PinBuf pin, pin2, pin3;
     Work W;
     size_t wcap = 0;
     // learned secular start: weights (device + host copy), harvest buffer
@@ -135,6 +136,10 @@ struct hsdc_ctx {
     int cur_level = 0;
     bool sec_pre = false;              // k_secstart runs before k_secular
     DevBuf featb;
+    DevBuf ltab;  // tables of the device-scheduled level (k_lvltab)
+    hipGraphExec_t gexec = nullptr;  // the last captured solve and its shape
+    std::vector<long> gkey;
+    hipStream_t gstream = nullptr;   // stream of graph solves called on the null stream
 };
 
 #define SEC_LEVELS 128
@@ -199,7 +204,7 @@ static void plan_problem(Plan &pl, int pi, double *d, const double *e, int smlsi
             int matsiz = bend[i + 1] - s;
             int msd2 = (i == 0) ? bend[0] : matsiz / 2;
             Merge M;
-            M.s = off + s; M.n = matsiz; M.n1 = msd2; M.prob = pi;
+            M.s = off + s; M.n = matsiz; M.n1 = msd2; M.prob = pi; M.lvl = L;
             M.rho = e[off + s + msd2 - 1];
             pl.lv[L].push_back(M);
             nb.push_back(bend[i + 1]);
@@ -240,65 +245,91 @@ static int alloc_work(hsdc_ctx *c, int ntot)
     W.st_t = D + 18 * n; W.st_tc = D + 19 * n; W.st_lo = D + 20 * n; W.st_hi = D + 21 * n;
     W.st_ii = I + 18 * n;
     // per-merge records of one launch (a level has at most ntot / 2 merges), kept
-    // zero between launches by their consumer
-    CK(ensure(c->maux, sizeof(unsigned long long) * 4 * ((size_t)ntot / 2 + 2)));
-    HK(hipMemset(c->maux.p, 0, sizeof(unsigned long long) * 4 * ((size_t)ntot / 2 + 2)));
+    // zero between launches by their consumer; then the level gate and statistics
+    const size_t nmaux = 4 * ((size_t)ntot / 2 + 2);
+    CK(ensure(c->maux, sizeof(unsigned long long) * (nmaux + 8)));
+    HK(hipMemset(c->maux.p, 0, sizeof(unsigned long long) * (nmaux + 8)));
     W.maux = (unsigned long long *)c->maux.p;
+    W.gate = (int *)(W.maux + nmaux);
+    W.lstat = W.maux + nmaux + 1;
     c->wcap = ntot;
     return 0;
+}
+
+// Open the level gate and zero the level statistics (a kernel, not a memset, so
+// that the stream may be under graph capture); gate_open leaves the statistics.
+static void gate_reset(hsdc_ctx *c)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(WGS), 0, c->stream, (unsigned long long *)c->W.gate, 8L, 0ULL);
+}
+
+static void gate_open(hsdc_ctx *c)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(WGS), 0, c->stream, (unsigned long long *)c->W.gate, 1L, 0ULL);
 }
 
 // Secular control block of the next solve: learned-start weights when enabled and
 // loaded, the harvest buffer when harvesting, and zeroed iteration counters.
 static int sec_setup(hsdc_ctx *c)
 {
-    size_t hdr = al64(sizeof(SecCtl));
-    CK(ensure(c->sec, hdr + 64 + sizeof(unsigned int) * SEC_LEVELS));
+    // device layout: SecCtl | iters, used, wave_iters [SEC_SLOTS each] | level maxima [SEC_LEVELS][SEC_SLOTS]
+    const size_t hdr = al64(sizeof(SecCtl));
+    const long ncnt = 3 * SEC_SLOTS + SEC_LEVELS * SEC_SLOTS / 2;  // in 8-byte words
+    CK(ensure(c->sec, hdr + 8 * (size_t)ncnt));
+    CK(ensure_pin(c->pin3, sizeof(SecCtl)));
     char *base = (char *)c->sec.p;
-    struct { SecCtl C; } h;
+    SecCtl &h = *(SecCtl *)c->pin3.p;
     memset(&h, 0, sizeof(h));
-    h.C.w = (c->prm.learned && !c->hw.empty()) ? (const float *)c->mlpw.p : nullptr;
-    h.C.iters = (unsigned long long *)(base + hdr);
-    h.C.used = (unsigned long long *)(base + hdr + 8);
-    h.C.wave_iters = (unsigned long long *)(base + hdr + 16);
+    h.w = (c->prm.learned && !c->hw.empty()) ? (const float *)c->mlpw.p : nullptr;
+    h.iters = (unsigned long long *)(base + hdr);
+    h.used = h.iters + SEC_SLOTS;
+    h.wave_iters = h.iters + 2 * SEC_SLOTS;
     if (c->hv_cap > 0) {
         char *hv = (char *)c->hv.p;
-        h.C.count = (unsigned long long *)hv;
-        h.C.hf = (float *)(hv + 64);
-        h.C.ht = (float *)(hv + 64 + al64(sizeof(float) * MLP_IN * (size_t)c->hv_cap));
-        h.C.cap = (unsigned long long)c->hv_cap;
+        h.count = (unsigned long long *)hv;
+        h.hf = (float *)(hv + 64);
+        h.ht = (float *)(hv + 64 + al64(sizeof(float) * MLP_IN * (size_t)c->hv_cap));
+        h.cap = (unsigned long long)c->hv_cap;
         CK(ensure(c->featb, sizeof(float) * MLP_IN * std::max(c->wcap, (size_t)1)));
-        h.C.feat = (float *)c->featb.p;
+        h.feat = (float *)c->featb.p;
     }
-    h.C.kmin = c->prm.learned_min_k;
-    h.C.gate = (float)c->prm.learned_gate;
-    unsigned char zero[64 + sizeof(unsigned int) * SEC_LEVELS];
-    memset(zero, 0, sizeof(zero));
-    HK(hipMemcpyAsync(base, &h.C, sizeof(SecCtl), hipMemcpyHostToDevice, c->stream));
-    HK(hipMemcpyAsync(base + hdr, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
-    HK(hipStreamSynchronize(c->stream));
+    h.kmin = c->prm.learned_min_k;
+    h.gate = (float)c->prm.learned_gate;
+    // from pinned memory, and the counters cleared by a kernel: nothing waits here
+    HK(hipMemcpyAsync(base, &h, sizeof(SecCtl), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ncnt, WGS)), dim3(WGS), 0, c->stream, h.iters, ncnt, 0ULL);
+    HK(hipGetLastError());
     c->dsec = (const SecCtl *)base;
-    c->dlvlmax = (unsigned int *)(base + hdr + 64);
+    c->dlvlmax = (unsigned int *)(h.iters + 3 * SEC_SLOTS);
     c->cur_level = 0;
-    c->sec_pre = (h.C.w != nullptr) || (h.C.hf != nullptr);
+    c->sec_pre = (h.w != nullptr) || (h.hf != nullptr);
     return 0;
 }
 
 // iteration counters of the last solve into the statistics (stream synchronised)
 static int sec_collect(hsdc_ctx *c)
 {
-    unsigned long long v[3] = {0, 0, 0};
-    unsigned int lm[SEC_LEVELS];
-    HK(hipMemcpyAsync(v, (char *)c->sec.p + al64(sizeof(SecCtl)), sizeof(v), hipMemcpyDeviceToHost,
-                      c->stream));
-    HK(hipMemcpyAsync(lm, c->dlvlmax, sizeof(lm), hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned long long> v(3 * SEC_SLOTS);
+    std::vector<unsigned int> lm((size_t)SEC_LEVELS * SEC_SLOTS);
+    HK(hipMemcpyAsync(v.data(), (char *)c->sec.p + al64(sizeof(SecCtl)), sizeof(unsigned long long) * v.size(),
+                      hipMemcpyDeviceToHost, c->stream));
+    HK(hipMemcpyAsync(lm.data(), c->dlvlmax, sizeof(unsigned int) * lm.size(), hipMemcpyDeviceToHost, c->stream));
     HK(hipStreamSynchronize(c->stream));
-    long long crit = 0;
-    for (int i = 0; i < SEC_LEVELS; i++) crit += lm[i];
+    long long crit = 0, it = 0, us = 0, wi = 0;
+    for (int l = 0; l < SEC_LEVELS; l++) {
+        unsigned int m = 0;
+        for (int q = 0; q < SEC_SLOTS; q++) m = std::max(m, lm[(size_t)l * SEC_SLOTS + q]);
+        crit += m;
+    }
+    for (int q = 0; q < SEC_SLOTS; q++) {
+        it += (long long)v[q];
+        us += (long long)v[SEC_SLOTS + q];
+        wi += (long long)v[2 * SEC_SLOTS + q];
+    }
     c->st.secular_crit_iters = crit;
-    c->st.secular_iters = (long long)v[0];
-    c->st.learned_roots = (long long)v[1];
-    c->st.secular_wave_iters = (long long)v[2];
+    c->st.secular_iters = it;
+    c->st.learned_roots = us;
+    c->st.secular_wave_iters = wi;
     return 0;
 }
 
@@ -839,13 +870,15 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
             hipLaunchKernelGGL(k_secstart, dim3(blockK[nm]), dim3(SEC_WG), 0, s, dmg, dblockK, nm,
                                (const MergeOut *)dmo, W, c->dsec);
         hipLaunchKernelGGL(k_secular, dim3(blockK[nm]), dim3(SEC_WG), 0, s, dmg, dblockK, nm, dmo, W,
-                           c->prm.maxit, dinfo, c->dsec, c->dlvlmax + std::min(c->cur_level, SEC_LEVELS - 1));
+                           c->prm.maxit, dinfo, c->dsec,
+                           c->dlvlmax + SEC_SLOTS * std::min(c->cur_level, SEC_LEVELS - 1));
         hipLaunchKernelGGL(k_loewner, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, W);
         hipLaunchKernelGGL(k_sbuild, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, dsoff, W,
                            (double *)c->sb.p);
         if (!tiles.empty())
-            hipLaunchKernelGGL(k_gemm, dim3(tiles.size()), dim3(WGS), 0, s, dmg, dtiles, dprobs, dmo, dsoff,
-                               dcoff, W, cur, (const double *)c->sb.p, (double *)c->ct.p);
+            hipLaunchKernelGGL(k_gemm, dim3(tiles.size()), dim3(WGS), 0, s, dmg, dtiles, (int)tiles.size(),
+                               (const int *)nullptr, nm, dprobs, dmo, dsoff, dcoff, W, cur, (const double *)c->sb.p,
+                               (double *)c->ct.p);
         std::vector<int> sel;
         for (int i = 0; i < nm; i++)
             if (!ho[i].flags && ho[i].hss && ho[i].k > 0) sel.push_back(i);
@@ -857,10 +890,143 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
     hipLaunchKernelGGL(k_newcols, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dmo, W, cur);
     if (anyk)
         hipLaunchKernelGGL(k_scatter, dim3(colK[nm]), dim3(WGS), 0, s, dmg, dcolK, nm, dmo, dcoff, dprobs, W,
-                           cur ^ 1, (const double *)c->ct.p);
-    hipLaunchKernelGGL(k_lamrg, dim3(nm), dim3(WGS), 0, s, dmg, dmo, W, cur ^ 1);
+                           cur ^ 1, (const double *)c->ct.p, 0);
+    hipLaunchKernelGGL(k_lamrg, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur ^ 1);
     HK(hipGetLastError());
     cur ^= 1;
+    return 0;
+}
+
+// ----------------------------------------------------------------------------
+// Device schedule of a level: launch sizes come from the merge orders alone, the
+// level's kernels read the tables k_lvltab derives from the deflation results,
+// and nothing returns to the host until the end of the solve.  The K-sized
+// launches stride over their work items with at most DEV_GRID blocks.  A level
+// whose dense S and Ct blocks could exceed DEV_BUDGET bytes keeps the
+// synchronised schedule.
+// ----------------------------------------------------------------------------
+#define DEV_GRID 2048
+#define DEV_BUDGET (16LL << 30)
+
+struct LvlBound {
+    long blocks, tiles, cols;  // secular blocks, GEMM tiles, columns with K = N
+    long sbcap, ctcap;         // S and Ct doubles of the dense merges (K < hss_min_k)
+};
+
+static LvlBound level_bound(const std::vector<Merge> &hm, int hss_min_k)
+{
+    LvlBound b = {0, 0, 0, 0, 0};
+    const long kd = hss_min_k >= 1 ? hss_min_k - 1 : 0;  // largest dense K
+    for (const Merge &M : hm) {
+        const long kmax = std::min((long)M.n, kd);
+        b.blocks += cdiv(M.n, SEC_WG / sec_team(M.n));  // nondecreasing in K
+        b.tiles += (long)(cdiv(M.n1, 64) + cdiv(M.n - M.n1, 64)) * cdiv(kmax, 64);
+        b.cols += M.n;
+        b.sbcap += kmax * kmax;
+        b.ctcap += (long)M.n * kmax;
+    }
+    return b;
+}
+
+static inline int dev_grid(long n) { return (int)std::max(1L, std::min(n, (long)DEV_GRID)); }
+
+static LvlTab level_tables(hsdc_ctx *c, int nm)
+{
+    LvlTab t;
+    char *p = (char *)c->ltab.p;
+    size_t ni = al64(sizeof(int) * (nm + 1)), nl = al64(sizeof(long) * nm);
+    t.chunkK = (int *)p; t.blockK = (int *)(p + ni); t.colK = (int *)(p + 2 * ni); t.tileK = (int *)(p + 3 * ni);
+    t.soff = (long *)(p + 4 * ni); t.coff = (long *)(p + 4 * ni + nl);
+    return t;
+}
+
+static size_t level_tables_bytes(int nm) { return 4 * al64(sizeof(int) * (nm + 1)) + 2 * al64(sizeof(long) * nm); }
+
+// Queue one level (merges of table dmg) on the device schedule.
+static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, int L, const int *dchunkN,
+                     int totN, const LvlBound &B, Prob *dprobs, int cur, int *dinfo)
+{
+    const int nm = (int)hm.size();
+    hipStream_t s = c->stream;
+    Work W = c->W;
+    MergeOut *dmo = (MergeOut *)c->mo.p;
+    const LvlTab t = level_tables(c, nm);
+    int nmax = 0;
+    for (const Merge &M : hm) nmax = std::max(nmax, M.n);
+    const int ds_cap = nmax <= 4096 ? nmax : 0;
+    hipLaunchKernelGGL(k_prep, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur);
+    hipLaunchKernelGGL(k_mergepath, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, W);
+    hipLaunchKernelGGL(k_sortmerge, dim3(nm), dim3(WGS), 0, s, dmg, W);
+    if (nmax >= 2048)
+        hipLaunchKernelGGL(k_deflate<1024>, dim3(nm), dim3(1024), sizeof(double) * ds_cap, s, dmg, W, cur, dmo,
+                           c->prm.hss_min_k, ds_cap);
+    else
+        hipLaunchKernelGGL(k_deflate<256>, dim3(nm), dim3(WGS), sizeof(double) * ds_cap, s, dmg, W, cur, dmo,
+                           c->prm.hss_min_k, ds_cap);
+    hipLaunchKernelGGL(k_lvltab, dim3(1), dim3(256), 0, s, dmg, nm, (const MergeOut *)dmo, W, t, B.sbcap, B.ctcap,
+                       dinfo);
+    hipLaunchKernelGGL(k_rot_prep, dim3(nm), dim3(WGS), 0, s, dmg, dprobs, W, cur, (const MergeOut *)dmo);
+    hipLaunchKernelGGL(k_rotate, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur,
+                       (const MergeOut *)dmo);
+    if (c->sec_pre)
+        hipLaunchKernelGGL(k_secstart, dim3(dev_grid(B.blocks)), dim3(SEC_WG), 0, s, dmg, (const int *)t.blockK, nm,
+                           (const MergeOut *)dmo, W, c->dsec);
+    hipLaunchKernelGGL(k_secular, dim3(dev_grid(B.blocks)), dim3(SEC_WG), 0, s, dmg, (const int *)t.blockK, nm,
+                       (const MergeOut *)dmo, W, c->prm.maxit, dinfo, c->dsec,
+                       c->dlvlmax + SEC_SLOTS * std::min(L, SEC_LEVELS - 1));
+    hipLaunchKernelGGL(k_loewner, dim3(totN), dim3(WGS), 0, s, dmg, (const int *)t.chunkK, nm,
+                       (const MergeOut *)dmo, W);
+    hipLaunchKernelGGL(k_sbuild, dim3(totN), dim3(WGS), 0, s, dmg, (const int *)t.chunkK, nm, (const MergeOut *)dmo,
+                       (const long *)t.soff, W, (double *)c->sb.p);
+    if (B.tiles > 0)
+        hipLaunchKernelGGL(k_gemm, dim3(dev_grid(B.tiles)), dim3(WGS), 0, s, dmg, (const GTile *)nullptr, 0,
+                           (const int *)t.tileK, nm, dprobs, (const MergeOut *)dmo, (const long *)t.soff,
+                           (const long *)t.coff, W, cur, (const double *)c->sb.p, (double *)c->ct.p);
+    hipLaunchKernelGGL(k_newcols, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur);
+    hipLaunchKernelGGL(k_scatter, dim3(dev_grid(B.cols)), dim3(WGS), 0, s, dmg, (const int *)t.colK, nm,
+                       (const MergeOut *)dmo, (const long *)t.coff, dprobs, W, cur ^ 1, (const double *)c->ct.p, 1);
+    hipLaunchKernelGGL(k_lamrg, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur ^ 1);
+    HK(hipGetLastError());
+    return 0;
+}
+
+// The HSS merges of device-scheduled level L, which stopped at its gate: the host
+// reads the level's results, builds and applies their HSS matrices and copies the
+// vectors into place, then opens the gate.
+static int level_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, int totN, const LvlBound &B,
+                     Prob *dprobs, int cur)
+{
+    const int nm = (int)hm.size();
+    hipStream_t s = c->stream;
+    const LvlTab t = level_tables(c, nm);
+    MergeOut *dmo = (MergeOut *)c->mo.p;
+    const size_t bmo = sizeof(MergeOut) * nm, bco = sizeof(long) * nm;
+    CK(ensure_pin(c->pin2, al64(bmo) + bco));
+    char *hp = (char *)c->pin2.p;
+    HK(hipMemcpyAsync(hp, dmo, bmo, hipMemcpyDeviceToHost, s));
+    HK(hipMemcpyAsync(hp + al64(bmo), t.coff, bco, hipMemcpyDeviceToHost, s));
+    HK(hipStreamSynchronize(s));
+    std::vector<MergeOut> ho(nm);
+    std::vector<long> coff(nm);
+    memcpy(ho.data(), hp, bmo);
+    memcpy(coff.data(), hp + al64(bmo), bco);
+    std::vector<int> sel;
+    long ctneed = 1;
+    for (int i = 0; i < nm; i++)
+        if (!ho[i].flags && ho[i].hss && ho[i].k > 0) {
+            sel.push_back(i);
+            ctneed = std::max(ctneed, coff[i] + (long)hm[i].n * ho[i].k);
+        }
+    gate_open(c);
+    if (sel.empty()) return 0;
+    CK(ensure(c->ct, sizeof(double) * (size_t)ctneed));  // the dense vectors are in place already
+    Work W = c->W;
+    hipLaunchKernelGGL(k_alpha, dim3(totN), dim3(WGS), 0, s, dmg, (const int *)t.chunkK, nm, (const MergeOut *)dmo,
+                       W);
+    CK(run_hss(c, dmg, hm, 0, ho, sel, coff, dprobs, cur));
+    hipLaunchKernelGGL(k_scatter, dim3(dev_grid(B.cols)), dim3(WGS), 0, s, dmg, (const int *)t.colK, nm,
+                       (const MergeOut *)dmo, (const long *)t.coff, dprobs, W, cur ^ 1, (const double *)c->ct.p, 2);
+    HK(hipGetLastError());
     return 0;
 }
 
@@ -870,6 +1036,19 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
                  const std::vector<long> &zoff, const std::vector<long> &ldz, int zglobal,
                  double *d_out_dev, const std::vector<int> *outcol)
 {
+    // graph capture needs a stream other than the null stream: a solve on the null
+    // stream runs on the context's own stream once the null stream's work is done
+    struct StreamRestore {
+        hsdc_ctx *c;
+        hipStream_t s0;
+        ~StreamRestore() { c->stream = s0; }
+    } restore{c, c->stream};
+    const bool use_graph = c->prm.graph && c->prm.batched && !c->prm.level_sync && !c->prm.profile;
+    if (use_graph && c->stream == nullptr) {
+        HK(hipStreamSynchronize(nullptr));
+        if (!c->gstream) HK(hipStreamCreate(&c->gstream));
+        c->stream = c->gstream;
+    }
     hipStream_t s = c->stream;
     const int ntot = pl.ntot;
     hipEvent_t e0, e1;
@@ -952,61 +1131,174 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     long *dzoff = (long *)(dp + a7), *dldz = (long *)(dp + a8);
     int *doutcol = outcol ? (int *)(dp + a9) : nullptr;
     int *dinfo = (int *)(dp + a10);
-    Work W = c->W;
-    int cur = 0;
     CK(sec_setup(c));
-    {
-        long nmx = 4 * ((long)c->wcap / 2 + 2);
-        hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(nmx, WGS)), dim3(WGS), 0, s, W.maux, nmx, 0ULL);
+    // schedule of every level, and the buffers of the device-scheduled ones (sized
+    // before the first launch: a buffer that grows moves)
+    const int nlv = (int)pl.lv.size();
+    std::vector<LvlBound> lb(nlv);
+    std::vector<char> dev(nlv, 0);
+    size_t nm_max = 1;
+    long sbmax = 1, ctmax = 1;
+    bool all_dev = true;
+    for (int L = 0; L < nlv; L++) {
+        nm_max = std::max(nm_max, pl.lv[L].size());
+        lb[L] = level_bound(pl.lv[L], c->prm.hss_min_k);
+        dev[L] = c->prm.batched && !c->prm.level_sync && 8 * (lb[L].sbcap + lb[L].ctcap) <= DEV_BUDGET;
+        if (dev[L]) { sbmax = std::max(sbmax, lb[L].sbcap); ctmax = std::max(ctmax, lb[L].ctcap); }
+        all_dev &= dev[L] != 0;
     }
+    CK(ensure(c->mo, sizeof(MergeOut) * nm_max));
+    CK(ensure(c->ltab, level_tables_bytes((int)nm_max)));
+    CK(ensure(c->sb, sizeof(double) * (size_t)sbmax));
+    CK(ensure(c->ct, sizeof(double) * (size_t)ctmax));
     int lmax = 1;
     for (auto &Lf : pl.leaves) lmax = std::max(lmax, Lf.m);
-    hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(64), sizeof(double) * lmax * lmax, s, dleaves, (int)nl, dprobs, dd, de,
-                       W, cur);
-    HK(hipGetLastError());
-    // the host tables above live in c->pin, which run_merges reuses: wait for the upload
-    HK(hipStreamSynchronize(s));
-    size_t mi = 0;
-    for (size_t L = 0; L < pl.lv.size(); L++) {
-        const std::vector<Merge> &hm = pl.lv[L];
-        const Merge *dm = dmerges + lvoff[L];
-        const int nm = (int)hm.size();
-        c->cur_level = (int)L;
-        if (c->prm.batched) {
-            int totN = chunks[lvchunk[L] + nm];
-            CK(run_merges(c, dm, hm, 0, nm, dchunks + lvchunk[L], totN, dprobs, cur, dinfo));
-            mi += nm;
-        } else {
-            // one merge at a time, every merge keeping the same ping-pong parity
-            int base = cur;
-            for (int m = 0; m < nm; m++, mi++) {
-                int cc = base;
-                CK(run_merges(c, dm, hm, m, m + 1, dchunks + mchunk[mi], chunks[mchunk[mi] + 1], dprobs, cc,
-                              dinfo));
-            }
-            cur = base ^ 1;
-        }
-        c->st.levels++;
-    }
-    int zrows = zglobal ? ntot : 0;
+    const int zrows = zglobal ? ntot : 0;
     unsigned long long *drows = (unsigned long long *)(dp + a10 + 64);  // 64 counters
     hipEvent_t f0 = nullptr, f1 = nullptr;
     if (c->prm.profile) {
         HK(hipEventCreate(&f0));
         HK(hipEventCreate(&f1));
-        HK(hipEventRecord(f0, s));
     }
-    hipLaunchKernelGGL(k_final, dim3(ntot), dim3(WGS), 0, s, dprobs, dcolprob, W, cur, doutcol, dZ, dzoff,
-                       dldz, zrows, d_out_dev, dscale, drows);
-    HK(hipGetLastError());
-    if (c->prm.profile) HK(hipEventRecord(f1, s));
+    // first launches: records and gate cleared, leaves
+    auto launch_head = [&]() -> int {
+        long nmx = 4 * ((long)c->wcap / 2 + 2) + 8;  // per-merge records, gate, level statistics
+        hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(nmx, WGS)), dim3(WGS), 0, s, c->W.maux, nmx, 0ULL);
+        hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(64), sizeof(double) * lmax * lmax, s, dleaves, (int)nl, dprobs, dd,
+                           de, c->W, 0);
+        HK(hipGetLastError());
+        return 0;
+    };
+    auto launch_final = [&]() -> int {
+        if (c->prm.profile) HK(hipEventRecord(f0, s));
+        hipLaunchKernelGGL(k_final, dim3(ntot), dim3(WGS), 0, s, dprobs, dcolprob, c->W, nlv & 1, doutcol, dZ, dzoff,
+                           dldz, zrows, d_out_dev, dscale, drows);
+        HK(hipGetLastError());
+        if (c->prm.profile) HK(hipEventRecord(f1, s));
+        return 0;
+    };
+    auto launch_dev = [&](int L) -> int {
+        c->cur_level = L;
+        const int nm = (int)pl.lv[L].size();
+        return level_dev(c, dmerges + lvoff[L], pl.lv[L], L, dchunks + lvchunk[L], chunks[lvchunk[L] + nm], lb[L],
+                         dprobs, L & 1, dinfo);
+    };
+    // With every level on the device schedule the whole solve is one HIP graph,
+    // captured once per shape (plan, buffers, parameters) and replayed.
+    bool queued = false;  // levels and final kernel already in the stream
+    if (all_dev && use_graph && s != nullptr) {
+        std::vector<long> key = {(long)nlv, (long)np, (long)nl, (long)lmax, (long)ntot, (long)c->wcap, zrows,
+                                 (long)(outcol != nullptr), c->prm.maxit, c->prm.hss_min_k, (long)c->sec_pre,
+                                 (long)(uintptr_t)c->stream};
+        for (const void *ptr : {(const void *)c->wsd.p, (const void *)c->wsi.p, (const void *)c->maux.p,
+                                (const void *)c->probs.p, (const void *)c->mo.p, (const void *)c->ltab.p,
+                                (const void *)c->sb.p, (const void *)c->ct.p, (const void *)c->dsec,
+                                (const void *)c->dlvlmax, (const void *)dZ, (const void *)d_out_dev})
+            key.push_back((long)(uintptr_t)ptr);
+        for (int L = 0; L < nlv; L++) {
+            int nmax = 0;
+            for (const Merge &M : pl.lv[L]) nmax = std::max(nmax, M.n);
+            key.insert(key.end(), {(long)pl.lv[L].size(), (long)chunks[lvchunk[L] + pl.lv[L].size()], (long)nmax,
+                                   lb[L].blocks, lb[L].tiles, lb[L].cols, lb[L].sbcap, lb[L].ctcap});
+        }
+        if (!c->gexec || key != c->gkey) {
+            if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+            int rc = HSDC_ERR_HIP;
+            hipGraph_t g = nullptr;
+            hipError_t ec = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (ec == hipSuccess) {
+                rc = launch_head();
+                for (int L = 0; L < nlv && !rc; L++) rc = launch_dev(L);
+                if (!rc) rc = launch_final();
+                ec = hipStreamEndCapture(s, &g);
+            }
+            if (!rc && ec == hipSuccess && hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+                c->gexec = nullptr;
+                rc = HSDC_ERR_HIP;
+            }
+            if (g) (void)hipGraphDestroy(g);
+            if (ec != hipSuccess) rc = HSDC_ERR_HIP;
+            if (rc) {  // nothing ran: queue the same launches directly
+                (void)hipGetLastError();
+                if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+                c->gexec = nullptr;
+                c->gkey.clear();
+            } else {
+                c->gkey = key;
+            }
+        }
+        if (c->gexec) {
+            HK(hipGraphLaunch(c->gexec, s));
+            queued = true;
+        }
+    }
+    if (!queued) {
+        CK(launch_head());
+        // the host tables above live in c->pin, which run_merges reuses: wait for the upload
+        if (!all_dev) HK(hipStreamSynchronize(s));
+    }
+    // Runs of device-scheduled levels are queued back to back; the host looks at the
+    // gate once per run (normally once per solve, together with the final kernel).
+    // Level L works on ping-pong parity L & 1.
+    int L = 0, ndev = 0;
+    for (;;) {
+        int L1 = L;
+        if (queued) {
+            L1 = nlv;
+            queued = false;
+        } else {
+            for (; L1 < nlv && dev[L1]; L1++) CK(launch_dev(L1));
+            if (L1 == nlv) CK(launch_final());
+        }
+        const bool fin = L1 == nlv;
+        if (L1 > L || fin) {
+            int g = 0;
+            HK(hipMemcpyAsync(&g, c->W.gate, sizeof(int), hipMemcpyDeviceToHost, s));
+            HK(hipStreamSynchronize(s));
+            if (g) {  // level g - 1 holds HSS merges; the levels above it did nothing
+                const int Lh = g - 1, nm = (int)pl.lv[Lh].size();
+                for (int l = L; l <= Lh; l++) ndev += (int)pl.lv[l].size();
+                CK(level_hss(c, dmerges + lvoff[Lh], pl.lv[Lh], chunks[lvchunk[Lh] + nm], lb[Lh], dprobs, Lh & 1));
+                L = Lh + 1;
+                continue;
+            }
+            for (int l = L; l < L1; l++) ndev += (int)pl.lv[l].size();
+        }
+        if (fin) break;
+        // a level on the synchronised schedule
+        const std::vector<Merge> &hm = pl.lv[L1];
+        const Merge *dm = dmerges + lvoff[L1];
+        const int nm = (int)hm.size();
+        c->cur_level = L1;
+        int cur = L1 & 1;
+        if (c->prm.batched) {
+            CK(run_merges(c, dm, hm, 0, nm, dchunks + lvchunk[L1], chunks[lvchunk[L1] + nm], dprobs, cur, dinfo));
+        } else {
+            // one merge at a time, every merge keeping the same ping-pong parity
+            for (int m = 0; m < nm; m++) {
+                int cc = L1 & 1;
+                const size_t mi = lvoff[L1] + m;
+                CK(run_merges(c, dm, hm, m, m + 1, dchunks + mchunk[mi], chunks[mchunk[mi] + 1], dprobs, cc,
+                              dinfo));
+            }
+        }
+        L = L1 + 1;
+    }
+    c->st.levels += nlv;
+    c->st.merges += ndev;
     HK(hipEventRecord(e1, s));
     int hinfo = 0;
     unsigned long long rowc[64], rows = 0;
     HK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
     HK(hipMemcpyAsync(rowc, drows, sizeof(rowc), hipMemcpyDeviceToHost, s));
+    unsigned long long lst[5];
+    HK(hipMemcpyAsync(lst, c->W.lstat, sizeof(lst), hipMemcpyDeviceToHost, s));
     HK(hipStreamSynchronize(s));
     for (int q = 0; q < 64; q++) rows += rowc[q];
+    c->st.roots += (long long)lst[0];
+    c->st.deflated += (long long)lst[1];
+    c->st.hss_merges += (int)lst[2];
+    if (nlv > 0 && dev[nlv - 1]) c->st.k_top = (int)lst[3];  // K of the top merge
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     c->st.ms_total = ms;
@@ -1023,6 +1315,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     CK(sec_collect(c));
+    if (hinfo & 2) return HSDC_ERR_INTERNAL;  // device-scheduled level over its bounds
     return hinfo ? 1 : 0;
 }
 
@@ -1073,6 +1366,8 @@ void hsdc_default_params(hsdc_params *p)
     p->profile = 0;
     p->learned_min_k = 256;
     p->learned_gate = 0.1;
+    p->level_sync = 0;
+    p->graph = 1;
 }
 
 // The context touches no device until the first solve, so argument checking and
@@ -1093,10 +1388,13 @@ int hsdc_ctx_destroy(hsdc_ctx *c)
     DevBuf *bufs[] = {&c->q,  &c->ct,  &c->sb,   &c->din, &c->ein,  &c->wsd,  &c->wsi, &c->tabs,
                       &c->lvl, &c->mo, &c->probs, &c->misc, &c->zout, &c->dout, &c->hss, &c->htab,
                       &c->xb, &c->gf,  &c->cb,   &c->yb,  &c->mlpw, &c->hv,   &c->sec, &c->mlptr,
-                      &c->maux, &c->featb};
+                      &c->maux, &c->featb, &c->ltab};
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->gstream) (void)hipStreamDestroy(c->gstream);
     for (DevBuf *b : bufs) if (b->p) hipFree(b->p);
     if (c->pin.p) hipHostFree(c->pin.p);
     if (c->pin2.p) hipHostFree(c->pin2.p);
+    if (c->pin3.p) hipHostFree(c->pin3.p);
     delete c;
     return 0;
 }
@@ -1299,9 +1597,10 @@ int hsdc_autotune(hsdc_ctx *c, int nprob, const int *ns, const long *offs, doubl
 static int single_merge_setup(hsdc_ctx *c, int n, int n1, double rho, Merge **dm, int **dchunk)
 {
     CK(alloc_work(c, std::max(n, 1)));
+    gate_reset(c);
     CK(ensure(c->probs, 128));
     Merge M;
-    M.s = 0; M.n = n; M.n1 = n1; M.prob = 0; M.rho = rho;
+    M.s = 0; M.n = n; M.n1 = n1; M.prob = 0; M.lvl = 0; M.rho = rho;
     int ch[2] = {0, cdiv(n, WGS)};
     char *dp = (char *)c->probs.p;
     HK(hipMemcpy(dp, &M, sizeof(Merge), hipMemcpyHostToDevice));

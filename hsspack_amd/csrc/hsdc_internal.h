@@ -33,6 +33,7 @@ struct Prob {
 // cut n1 and coupling rho = E(s+n1-1) (signed, before MDLAED2 rescales it).
 struct Merge {
     int s, n, n1, prob;
+    int lvl;  // DC level of the merge (0 = first merge level)
     double rho;
 };
 
@@ -69,15 +70,35 @@ struct Work {
     unsigned long long *maux;  // per merge of a launch: max|z|, max|D| (bit patterns), unsorted flag, pad
     double *st_t, *st_tc, *st_lo, *st_hi;  // per root: starting iterate, classical start, bracket
     int *st_ii;                // per root: origin pole | learned-start flag << 30
+    // device-scheduled levels: nonzero gate = 1 + the level whose HSS merges wait for
+    // the host; the kernels of the levels above it return at once while it is set
+    int *gate;
+    unsigned long long *lstat;  // roots, deflated, HSS merges, K of the last merge
+};
+
+// a kernel of the merges mg stands still: a level below waits for the host
+__device__ __forceinline__ bool gate_closed(const Work &W, const Merge *mg)
+{
+    const int g = *W.gate;
+    return g != 0 && mg[0].lvl >= g;
+}
+
+// Tables of one level derived from its deflation results (k_lvltab): first chunk
+// (256 roots), secular block, column and GEMM tile of every merge, offsets of its
+// S and Ct blocks.  Ct holds the dense merges first, then the HSS merges.
+struct LvlTab {
+    int *chunkK, *blockK, *colK, *tileK;  // nm + 1 entries each
+    long *soff, *coff;                    // nm entries each
 };
 
 // control block of the secular kernel: learned-start weights, statistics and
 // optional harvesting of training samples
+#define SEC_SLOTS 64  // copies of each statistics counter (summed by the host)
 struct SecCtl {
     const float *w;              // MLP weights or nullptr
-    unsigned long long *iters;   // total iterations
-    unsigned long long *used;    // roots started from the learned iterate
-    unsigned long long *wave_iters;  // sum over wavefronts of their largest iteration count
+    unsigned long long *iters;   // total iterations [SEC_SLOTS]
+    unsigned long long *used;    // roots started from the learned iterate [SEC_SLOTS]
+    unsigned long long *wave_iters;  // sum over wavefronts of their largest iteration count [SEC_SLOTS]
     float *hf, *ht;              // harvest: features (cap x 34) and targets (cap)
     float *feat;                 // harvest: features of every root (34 per global index)
     unsigned long long *count;

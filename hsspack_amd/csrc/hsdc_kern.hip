@@ -30,11 +30,19 @@ __device__ __forceinline__ double qload(const Prob &P, const int *lo, const int 
 
 __device__ __forceinline__ int find_seg(const int *start, int n, int b)
 {
-    // largest m with start[m] <= b (start has n+1 entries, start[n] = total)
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= b) lo = mid; else hi = mid - 1;
+    // Largest m < n with start[m] <= b (start nondecreasing, start[0] <= b).  All 64
+    // lanes of a wavefront call it with the same arguments: each round they probe 64
+    // evenly spaced entries and the ballot narrows the range 64-fold, so a table of
+    // 4096 merges costs two dependent loads instead of twelve.
+    const int lane = threadIdx.x & 63;
+    int lo = 0, len = n;  // the answer lies in [lo, lo + len)
+    while (len > 1) {
+        const int step = (len + 63) >> 6;
+        const int idx = lo + lane * step;
+        const bool ok = idx < lo + len && start[idx] <= b;
+        const int cnt = __popcll(__ballot(ok));  // a prefix of the lanes, lane 0 included
+        lo += (cnt - 1) * step;
+        len = min(step, len - (cnt - 1) * step);
     }
     return lo;
 }
@@ -152,6 +160,7 @@ __global__ __launch_bounds__(64) void k_leaf(const Leaf *leaves, int nleaf, cons
 __global__ __launch_bounds__(WG) void k_prep(const Merge *mg, const int *chunk_start, int nm,
                                              const Prob *probs, Work W, int cur)
 {
+    if (gate_closed(W, mg)) return;
     int m = find_seg(chunk_start, nm, blockIdx.x);
     Merge M = mg[m];
     int i = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
@@ -209,6 +218,7 @@ __device__ __forceinline__ double maux_tol(const Work &W, int m)
 // small-z column type (mdlaed2.f90:264-300).
 __global__ __launch_bounds__(WG) void k_mergepath(const Merge *mg, const int *chunk_start, int nm, Work W)
 {
+    if (gate_closed(W, mg)) return;
     int m = find_seg(chunk_start, nm, blockIdx.x);
     if (W.maux[4 * m + 2]) return;  // k_sortmerge merges it sequentially
     Merge M = mg[m];
@@ -289,6 +299,7 @@ __device__ void block_merge(const double *a, int n1, int n2, int s1, int s2, int
 // by one block, then the same gathers.
 __global__ __launch_bounds__(WG) void k_sortmerge(const Merge *mg, Work W)
 {
+    if (gate_closed(W, mg)) return;
     __shared__ int sflag;
     if (!W.maux[4 * blockIdx.x + 2]) return;
     Merge M = mg[blockIdx.x];
@@ -327,6 +338,7 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur, MergeOut *mo,
                                                 int hss_min_k, int ds_cap)
 {
+    if (gate_closed(W, mg)) return;
     constexpr int NW = NT / 64;  // wavefronts
     extern __shared__ double dsl[];  // ds_cap doubles: sorted D for the replay searches
     __shared__ int sk, sk2, scnt[4], wcnt[4][NW], sL, sfirst, sfallback, snseg;
@@ -354,6 +366,7 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
     if (tid == 0) {
         // the per-merge record is consumed: clear it for the next launch
         W.maux[4 * blockIdx.x + 0] = W.maux[4 * blockIdx.x + 1] = W.maux[4 * blockIdx.x + 2] = 0ULL;
+        W.maux[4 * blockIdx.x + 3] = 0ULL;  // k_newcols' order flag of this level
     }
     if (rho * zmax <= tol) {
         if (tid == 0) {
@@ -810,6 +823,7 @@ template __global__ void k_deflate<1024>(const Merge *, Work, int, MergeOut *, i
 __global__ __launch_bounds__(WG) void k_rot_prep(const Merge *mg, const Prob *probs, Work W, int cur,
                                                  const MergeOut *mo)
 {
+    if (gate_closed(W, mg)) return;
     Merge M = mg[blockIdx.x];
     int nrot = mo[blockIdx.x].nrot;
     if (nrot == 0 || mo[blockIdx.x].flags) return;
@@ -833,6 +847,7 @@ __global__ __launch_bounds__(WG) void k_rot_prep(const Merge *mg, const Prob *pr
 __global__ __launch_bounds__(WG) void k_rotate(const Merge *mg, const int *chunk_start, int nm,
                                                const Prob *probs, Work W, int cur, const MergeOut *mo)
 {
+    if (gate_closed(W, mg)) return;
     int m = find_seg(chunk_start, nm, blockIdx.x);
     Merge M = mg[m];
     int nrot = mo[m].nrot;
@@ -1043,25 +1058,26 @@ __device__ SecOut sec_iterate(int n, int j, const double *d, const double *z, do
 
 // block -> merge, root and team geometry of both secular kernels
 struct SecGeom { int m, k, s, T, l, j; };
-__device__ __forceinline__ SecGeom sec_geom(const Merge *mg, const int *block_start, int nm, const MergeOut *mo)
+__device__ __forceinline__ SecGeom sec_geom(const Merge *mg, const int *block_start, int nm, const MergeOut *mo,
+                                            int vb)
 {
     SecGeom g;
-    g.m = find_seg(block_start, nm, blockIdx.x);
+    g.m = find_seg(block_start, nm, vb);
     g.k = mo[g.m].flags ? 0 : mo[g.m].k;
     g.s = mg[g.m].s;
     g.T = sec_team(g.k > 0 ? g.k : 1);
     g.l = threadIdx.x & (g.T - 1);
-    g.j = (blockIdx.x - block_start[g.m]) * (SEC_WG / g.T) + threadIdx.x / g.T + 1;
+    g.j = (vb - block_start[g.m]) * (SEC_WG / g.T) + threadIdx.x / g.T + 1;
     return g;
 }
 
 // Learned starting iterates (merges with K >= max(3, sec->kmin), when weights are
 // loaded or samples are harvested): the classical start, the 34 features, the
 // network's proposal behind the gate; written per root for k_secular.
-__global__ __launch_bounds__(SEC_WG) void k_secstart(const Merge *mg, const int *block_start, int nm,
-                                                     const MergeOut *mo, Work W, const SecCtl *sec)
+__device__ void secstart_block(const Merge *mg, const int *block_start, int nm, const MergeOut *mo, const Work &W,
+                               const SecCtl *sec, int vb)
 {
-    const SecGeom g = sec_geom(mg, block_start, nm, mo);
+    const SecGeom g = sec_geom(mg, block_start, nm, mo, vb);
     const SecCtl C = *sec;
     if (g.k < 3 || g.k < C.kmin || g.j > g.k) return;
     const double *d = W.dlamda + g.s, *z = W.w + g.s;
@@ -1087,17 +1103,30 @@ __global__ __launch_bounds__(SEC_WG) void k_secstart(const Merge *mg, const int 
     }
 }
 
+// virtual block vb of the level's secular launch (block_start[nm] of them)
+__global__ __launch_bounds__(SEC_WG) void k_secstart(const Merge *mg, const int *block_start, int nm,
+                                                     const MergeOut *mo, Work W, const SecCtl *sec)
+{
+    if (gate_closed(W, mg)) return;
+    const int nvb = block_start[nm];
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x)
+        secstart_block(mg, block_start, nm, mo, W, sec, vb);
+}
+
 // Root teams.  Merges that k_secstart covered start from its iterate; the others
 // from the classical one.  sec->hf/ht: when non-null, (features, log(converged tau /
 // classical starting tau)) of every root k_secstart covered are appended for
 // training, up to sec->cap samples.  block_start[m]: first block of merge m
 // (SEC_WG / sec_team(K) roots per block).
-__global__ __launch_bounds__(SEC_WG) void k_secular(const Merge *mg, const int *block_start, int nm,
-                                                    const MergeOut *mo, Work W, int maxit, int *info,
-                                                    const SecCtl *sec, unsigned int *itmax)
+struct SecAcc {  // statistics of one wavefront (held by its lane 0)
+    unsigned long long iters, used, wave_iters;
+    unsigned int itmax;
+};
+
+__device__ void secular_block(const Merge *mg, const int *block_start, int nm, const MergeOut *mo, const Work &W,
+                              int maxit, int *info, const SecCtl *sec, SecAcc &acc, int vb, double *sd, double *sz)
 {
-    __shared__ double sd[SEC_LDS], sz[SEC_LDS];
-    const SecGeom g = sec_geom(mg, block_start, nm, mo);
+    const SecGeom g = sec_geom(mg, block_start, nm, mo, vb);
     if (g.k == 0) return;  // uniform over the block
     const int k = g.k, s = g.s, T = g.T, l = g.l, j = g.j;
     const double *d = W.dlamda + s, *z = W.w + s;
@@ -1148,20 +1177,56 @@ __global__ __launch_bounds__(SEC_WG) void k_secular(const Merge *mg, const int *
             if (r.info) atomicOr(info, 1);
         }
     }
-    // statistics, one atomic per wavefront: total iterations, learned starts, and the
-    // wavefront's largest count (a wavefront runs as long as its slowest root, a
-    // launch as long as its slowest wavefront)
+    // statistics of the wavefront: total iterations, learned starts, and its largest
+    // count (a wavefront runs as long as its slowest root, a launch as long as its
+    // slowest wavefront)
     int mx = nit, sum = nit, nu = used;
     for (int off = 32; off > 0; off >>= 1) {
         mx = max(mx, __shfl_xor(mx, off));
         sum += __shfl_xor(sum, off);
         nu += __shfl_xor(nu, off);
     }
-    if ((threadIdx.x & 63) == 0 && mx > 0) {
-        atomicAdd(C.iters, (unsigned long long)sum);
-        if (nu) atomicAdd(C.used, (unsigned long long)nu);
-        atomicAdd(C.wave_iters, (unsigned long long)mx);
-        atomicMax(itmax, (unsigned int)mx);
+    acc.iters += (unsigned long long)sum;
+    acc.used += (unsigned long long)nu;
+    acc.wave_iters += (unsigned long long)mx;
+    acc.itmax = max(acc.itmax, (unsigned int)mx);
+}
+
+// The statistics leave a block as one atomic per counter, into one of SEC_SLOTS
+// copies picked by the block (same-address atomics from every wavefront of a wide
+// level serialise in L2); the host sums the copies (maximum for itmax).
+__global__ __launch_bounds__(SEC_WG) void k_secular(const Merge *mg, const int *block_start, int nm,
+                                                    const MergeOut *mo, Work W, int maxit, int *info,
+                                                    const SecCtl *sec, unsigned int *itmax)
+{
+    if (gate_closed(W, mg)) return;
+    __shared__ double sd[SEC_LDS], sz[SEC_LDS];
+    __shared__ unsigned long long red[SEC_WG / 64][3];
+    __shared__ unsigned int redm[SEC_WG / 64];
+    SecAcc acc = {0, 0, 0, 0};
+    const int nvb = block_start[nm];
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        secular_block(mg, block_start, nm, mo, W, maxit, info, sec, acc, vb, sd, sz);
+        __syncthreads();  // the next virtual block reloads the pole arrays
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wv][0] = acc.iters; red[wv][1] = acc.used; red[wv][2] = acc.wave_iters;
+        redm[wv] = acc.itmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        SecAcc t = {0, 0, 0, 0};
+        for (int w = 0; w < SEC_WG / 64; w++) {
+            t.iters += red[w][0]; t.used += red[w][1]; t.wave_iters += red[w][2];
+            t.itmax = max(t.itmax, redm[w]);
+        }
+        const SecCtl C = *sec;
+        const int slot = blockIdx.x & (SEC_SLOTS - 1);
+        if (t.iters) atomicAdd(C.iters + slot, t.iters);
+        if (t.used) atomicAdd(C.used + slot, t.used);
+        if (t.wave_iters) atomicAdd(C.wave_iters + slot, t.wave_iters);
+        if (t.itmax) atomicMax(itmax + slot, t.itmax);
     }
 }
 
@@ -1174,75 +1239,112 @@ __device__ __forceinline__ double delta_ij(const double *d, int i, const int *or
 // Loewner z of the dense branch, LAPACK DLAED3 order: W(I) = DELTA_I(I) times
 // DELTA_I(J)/(DLAMDA(I)-DLAMDA(J)) for J ascending, z = sign(sqrt(-W), w).
 // Also the HSS branch (MDLAED3, mdlaed3.f90:207-230): product from 1 with the
-// J = I factor taken in place, z = sign(sqrt|prod|, w).
+// J = I factor taken in place, z = sign(sqrt|prod|, w).  One thread per I; the
+// poles, tau and origins of a merge with K <= LW_LDS are staged in LDS first
+// (every thread walks all K of them).
+#define LW_LDS 2048
 __global__ __launch_bounds__(WG) void k_loewner(const Merge *mg, const int *chunk_start, int nm,
                                                 const MergeOut *mo, Work W)
 {
-    int m = find_seg(chunk_start, nm, blockIdx.x);
-    const MergeOut o = mo[m];
-    int i = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
-    if (o.flags || o.k < 3 || i >= o.k) return;
-    int s = mg[m].s, K = o.k;
-    const double *d = W.dlamda + s, *tau = W.tau + s;
-    const int *org = W.org + s;
-    double di = d[i], wv;
-    if (!o.hss) {
-        wv = delta_ij(d, i, org, tau, i);
-        for (int jj = 0; jj < K; jj++) {
-            if (jj == i) continue;
-            wv = wv * (delta_ij(d, i, org, tau, jj) / (di - d[jj]));
+    if (gate_closed(W, mg)) return;
+    __shared__ double sd[LW_LDS], st[LW_LDS];
+    __shared__ int so[LW_LDS];
+    const int nvb = chunk_start[nm];
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        const int m = find_seg(chunk_start, nm, vb);
+        const MergeOut o = mo[m];
+        if (o.flags || o.k < 3) continue;  // uniform over the block
+        const int s = mg[m].s, K = o.k, i = (vb - chunk_start[m]) * WG + threadIdx.x;
+        const double *d = W.dlamda + s, *tau = W.tau + s;
+        const int *org = W.org + s;
+        if (K <= LW_LDS) {
+            for (int t = threadIdx.x; t < K; t += WG) { sd[t] = d[t]; st[t] = tau[t]; so[t] = org[t]; }
+            __syncthreads();
+            d = sd; tau = st; org = so;
         }
-        W.wnew[s + i] = copysign(sqrt(-wv), W.w[s + i]);
-    } else {
-        wv = 1.0;
-        for (int jj = 0; jj < K; jj++) {
-            if (jj == i) wv = wv * delta_ij(d, i, org, tau, i);
-            else wv = wv * (delta_ij(d, i, org, tau, jj) / (di - d[jj]));
+        if (i < K) {
+            const double di = d[i];
+            double wv;
+            if (!o.hss) {
+                wv = delta_ij(d, i, org, tau, i);
+                for (int jj = 0; jj < K; jj++) {
+                    if (jj == i) continue;
+                    wv = wv * (delta_ij(d, i, org, tau, jj) / (di - d[jj]));
+                }
+                W.wnew[s + i] = copysign(sqrt(-wv), W.w[s + i]);
+            } else {
+                wv = 1.0;
+                for (int jj = 0; jj < K; jj++) {
+                    if (jj == i) wv = wv * delta_ij(d, i, org, tau, i);
+                    else wv = wv * (delta_ij(d, i, org, tau, jj) / (di - d[jj]));
+                }
+                W.wnew[s + i] = copysign(sqrt(fabs(wv)), W.w[s + i]);
+            }
         }
-        W.wnew[s + i] = copysign(sqrt(fabs(wv)), W.w[s + i]);
+        __syncthreads();  // the next virtual block restages
     }
 }
 
 // Dense branch eigenvector matrix of the secular problem (DLAED3, after label
 // 70): S(t,j) = (W(I)/DELTA_I(J)) / ||W./DELTA(:,J)||, I = INDXC(t); written
-// row-major (S[t*K + j]) so one lane per column stores coalesced.
+// row-major (S[t*K + j]) so one lane per column stores coalesced.  Poles, Loewner
+// z and INDXC of a merge with K <= LW_LDS are staged in LDS.
 __global__ __launch_bounds__(WG) void k_sbuild(const Merge *mg, const int *chunk_start, int nm,
                                                const MergeOut *mo, const long *soff, Work W, double *S)
 {
-    int m = find_seg(chunk_start, nm, blockIdx.x);
-    const MergeOut o = mo[m];
-    int j = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
-    if (o.flags || o.hss || j >= o.k) return;
-    int s = mg[m].s, K = o.k;
-    double *Sm = S + soff[m];
-    const int *indxc = W.indxc + s;
-    if (K == 1) { Sm[0] = 1.0; return; }
-    if (K == 2) {
-        double dv[2] = {W.difl[s + j], W.difr[s + j]};
-        for (int t = 0; t < 2; t++) Sm[t * K + j] = dv[indxc[t] - 1];
-        return;
-    }
-    const double *d = W.dlamda + s, *tau = W.tau + s, *wz = W.wnew + s;
-    const int *org = W.org + s;
-    double acc = 0.0;
-    for (int i = 0; i < K; i++) {
-        double v = wz[i] / delta_ij(d, i, org, tau, j);
-        acc += v * v;
-    }
-    double temp = sqrt(acc);
-    for (int t = 0; t < K; t++) {
-        int i = indxc[t] - 1;
-        Sm[(long)t * K + j] = (wz[i] / delta_ij(d, i, org, tau, j)) / temp;
+    if (gate_closed(W, mg)) return;
+    __shared__ double sd[LW_LDS], sw[LW_LDS];
+    __shared__ int sc[LW_LDS];
+    const int nvb = chunk_start[nm];
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        const int m = find_seg(chunk_start, nm, vb);
+        const MergeOut o = mo[m];
+        if (o.flags || o.hss || o.k == 0) continue;  // uniform over the block
+        const int s = mg[m].s, K = o.k, j = (vb - chunk_start[m]) * WG + threadIdx.x;
+        double *Sm = S + soff[m];
+        const int *indxc = W.indxc + s;
+        if (K <= 2) {
+            if (j < K) {
+                if (K == 1) {
+                    Sm[0] = 1.0;
+                } else {
+                    double dv[2] = {W.difl[s + j], W.difr[s + j]};
+                    for (int t = 0; t < 2; t++) Sm[t * K + j] = dv[indxc[t] - 1];
+                }
+            }
+            continue;
+        }
+        const double *d = W.dlamda + s, *wz = W.wnew + s;
+        if (K <= LW_LDS) {
+            for (int t = threadIdx.x; t < K; t += WG) { sd[t] = d[t]; sw[t] = wz[t]; sc[t] = indxc[t]; }
+            __syncthreads();
+            d = sd; wz = sw; indxc = sc;
+        }
+        if (j < K) {
+            // delta_i(j) = (d_i - d_org(j)) - tau_j, as delta_ij forms it
+            const double dorg = d[W.org[s + j] - 1], tj = W.tau[s + j];
+            double acc = 0.0;
+            for (int i = 0; i < K; i++) {
+                double v = wz[i] / ((d[i] - dorg) - tj);
+                acc += v * v;
+            }
+            double temp = sqrt(acc);
+            for (int t = 0; t < K; t++) {
+                int i = indxc[t] - 1;
+                Sm[(long)t * K + j] = (wz[i] / ((d[i] - dorg) - tj)) / temp;
+            }
+        }
+        __syncthreads();  // the next virtual block restages
     }
 }
 
 // HSS branch scalars (mdlaed3.f90:215-249): DIFL, DIFR, alpha_j = 1/||col j||.
-__global__ __launch_bounds__(WG) void k_alpha(const Merge *mg, const int *chunk_start, int nm,
-                                              const MergeOut *mo, Work W)
+__device__ __forceinline__ void k_alpha_at(const Merge *mg, const int *chunk_start, int nm, const MergeOut *mo,
+                                           const Work &W, int vb)
 {
-    int m = find_seg(chunk_start, nm, blockIdx.x);
+    int m = find_seg(chunk_start, nm, vb);
     const MergeOut o = mo[m];
-    int j = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    int j = (vb - chunk_start[m]) * WG + threadIdx.x;
     if (o.flags || !o.hss || j >= o.k) return;
     int s = mg[m].s, K = o.k;
     const double *d = W.dlamda + s, *tau = W.tau + s, *wz = W.wnew + s;
@@ -1264,6 +1366,15 @@ __global__ __launch_bounds__(WG) void k_alpha(const Merge *mg, const int *chunk_
     W.alpha[s + j] = 1.0 / sqrt(acc);
 }
 
+__global__ __launch_bounds__(WG) void k_alpha(const Merge *mg, const int *chunk_start, int nm,
+                                              const MergeOut *mo, Work W)
+{
+    if (gate_closed(W, mg)) return;
+    const int nvb = chunk_start[nm];
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x)
+        k_alpha_at(mg, chunk_start, nm, mo, W, vb);
+}
+
 // ---------------------------------------------------------------------------
 // Dense branch eigenvector update on FP64 MFMA (DLAED3's two DGEMMs):
 //   rows of Q1: Ct(0:n1, :)  = Q2(0:n1, types 1,2) * S(types 1,2, :)
@@ -1275,68 +1386,90 @@ __global__ __launch_bounds__(WG) void k_alpha(const Merge *mg, const int *chunk_
 typedef double d4 __attribute__((ext_vector_type(4)));
 struct GTile { int m, half, r0, c0; };
 
-__global__ __launch_bounds__(WG) void k_gemm(const Merge *mg, const GTile *tiles, const Prob *probs,
-                                             const MergeOut *mo, const long *soff, const long *coff,
-                                             Work W, int cur, const double *S, double *Ct)
+// Work item vt: tiles[vt] when the host built the list, else decoded from the
+// level's tile table (per merge, the row tiles of the Q1 half then of the Q2 half,
+// each across the ceil(K/64) column tiles).
+__device__ __forceinline__ GTile gtile_at(const GTile *tiles, const int *tile_start, int nm, const Merge *mg,
+                                          const MergeOut *mo, int vt)
 {
+    if (tiles) return tiles[vt];
+    GTile T;
+    T.m = find_seg(tile_start, nm, vt);
+    const int loc = vt - tile_start[T.m];
+    const int nc = (mo[T.m].k + 63) >> 6, rt1 = (mg[T.m].n1 + 63) >> 6;
+    const int rt = loc / nc;
+    T.c0 = (loc - rt * nc) * 64;
+    if (rt < rt1) { T.half = 0; T.r0 = rt * 64; }
+    else { T.half = 1; T.r0 = (rt - rt1) * 64; }
+    return T;
+}
+
+__global__ __launch_bounds__(WG) void k_gemm(const Merge *mg, const GTile *tiles, int ntiles, const int *tile_start,
+                                             int nm, const Prob *probs, const MergeOut *mo, const long *soff,
+                                             const long *coff, Work W, int cur, const double *S, double *Ct)
+{
+    if (gate_closed(W, mg)) return;
     __shared__ double As[16][64 + 1];
     __shared__ double Bs[16][64 + 1];
-    GTile T = tiles[blockIdx.x];
-    Merge M = mg[T.m];
-    const MergeOut o = mo[T.m];
-    Prob P = probs[M.prob];
-    const int K = o.k, n1 = M.n1, n2 = M.n - M.n1;
-    const int nrow = T.half ? n2 : n1;
-    const int rowg0 = M.s + (T.half ? n1 : 0);
-    const int kdim = T.half ? (o.ctot[1] + o.ctot[2]) : (o.ctot[0] + o.ctot[1]);
-    const int tbase = T.half ? o.ctot[0] : 0;  // first grouped column / S row used
-    const double *Sm = S + soff[T.m];
-    const int *indx = W.indx + M.s;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
-    d4 acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < kdim; k0 += 16) {
-        for (int e = 0; e < 4; e++) {
-            int idx = tid + WG * e;
-            int r = idx & 63, kk = idx >> 6;
-            double v = 0.0;
-            int t = k0 + kk, row = T.r0 + r;
-            if (t < kdim && row < nrow) {
-                int slot = W.vm[cur][M.s + indx[tbase + t] - 1];
-                v = qload(P, W.supp_lo, W.supp_hi, rowg0 + row, slot);
+    const int total = tiles ? ntiles : tile_start[nm];
+    for (int vt = blockIdx.x; vt < total; vt += gridDim.x) {
+        GTile T = gtile_at(tiles, tile_start, nm, mg, mo, vt);
+        Merge M = mg[T.m];
+        const MergeOut o = mo[T.m];
+        Prob P = probs[M.prob];
+        const int K = o.k, n1 = M.n1, n2 = M.n - M.n1;
+        const int nrow = T.half ? n2 : n1;
+        const int rowg0 = M.s + (T.half ? n1 : 0);
+        const int kdim = T.half ? (o.ctot[1] + o.ctot[2]) : (o.ctot[0] + o.ctot[1]);
+        const int tbase = T.half ? o.ctot[0] : 0;  // first grouped column / S row used
+        const double *Sm = S + soff[T.m];
+        const int *indx = W.indx + M.s;
+        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+        const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+        d4 acc[2][2];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < kdim; k0 += 16) {
+            for (int e = 0; e < 4; e++) {
+                int idx = tid + WG * e;
+                int r = idx & 63, kk = idx >> 6;
+                double v = 0.0;
+                int t = k0 + kk, row = T.r0 + r;
+                if (t < kdim && row < nrow) {
+                    int slot = W.vm[cur][M.s + indx[tbase + t] - 1];
+                    v = qload(P, W.supp_lo, W.supp_hi, rowg0 + row, slot);
+                }
+                As[kk][r] = v;
+                int c = idx & 63;
+                double b = 0.0;
+                int col = T.c0 + c;
+                if (t < kdim && col < K) b = Sm[(long)(tbase + t) * K + col];
+                Bs[kk][c] = b;
             }
-            As[kk][r] = v;
-            int c = idx & 63;
-            double b = 0.0;
-            int col = T.c0 + c;
-            if (t < kdim && col < K) b = Sm[(long)(tbase + t) * K + col];
-            Bs[kk][c] = b;
+            __syncthreads();
+            for (int kq = 0; kq < 16; kq += 4) {
+                int kl = kq + (lane >> 4);
+                double a0 = As[kl][wr + (lane & 15)];
+                double a1 = As[kl][wr + 16 + (lane & 15)];
+                double b0 = Bs[kl][wc + (lane & 15)];
+                double b1 = Bs[kl][wc + 16 + (lane & 15)];
+                acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (int kq = 0; kq < 16; kq += 4) {
-            int kl = kq + (lane >> 4);
-            double a0 = As[kl][wr + (lane & 15)];
-            double a1 = As[kl][wr + 16 + (lane & 15)];
-            double b0 = Bs[kl][wc + (lane & 15)];
-            double b1 = Bs[kl][wc + 16 + (lane & 15)];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-        }
-        __syncthreads();
+        double *C = Ct + coff[T.m];
+        const int rowoff = T.half ? n1 : 0;
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                for (int i = 0; i < 4; i++) {
+                    int row = T.r0 + wr + a * 16 + (lane >> 4) + 4 * i;
+                    int col = T.c0 + wc + b * 16 + (lane & 15);
+                    if (row < nrow && col < K) C[(long)(rowoff + row) + (long)col * M.n] = acc[a][b][i];
+                }
     }
-    double *C = Ct + coff[T.m];
-    const int rowoff = T.half ? n1 : 0;
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++)
-            for (int i = 0; i < 4; i++) {
-                int row = T.r0 + wr + a * 16 + (lane >> 4) + 4 * i;
-                int col = T.c0 + wc + b * 16 + (lane & 15);
-                if (row < nrow && col < K) C[(long)(rowoff + row) + (long)col * M.n] = acc[a][b][i];
-            }
 }
 
 // ---------------------------------------------------------------------------
@@ -1346,6 +1479,7 @@ __global__ __launch_bounds__(WG) void k_gemm(const Merge *mg, const GTile *tiles
 __global__ __launch_bounds__(WG) void k_newcols(const Merge *mg, const int *chunk_start, int nm,
                                                 const MergeOut *mo, Work W, int cur)
 {
+    if (gate_closed(W, mg)) return;
     int m = find_seg(chunk_start, nm, blockIdx.x);
     Merge M = mg[m];
     const MergeOut o = mo[m];
@@ -1361,18 +1495,26 @@ __global__ __launch_bounds__(WG) void k_newcols(const Merge *mg, const int *chun
     }
     int src = W.indxp[g];
     W.vm[nxt][g] = W.vm[cur][M.s + src - 1];
-    W.Dv[nxt][g] = (v < o.k) ? W.lam[g] : W.ddef[g];
+    const double dv = (v < o.k) ? W.lam[g] : W.ddef[g];
+    W.Dv[nxt][g] = dv;
+    // k_lamrg merges the roots (ascending) with the deflated values (read backwards);
+    // a pair out of that order sends the merge to DLAMRG's sequential loop
+    if (v + 1 < M.n && v + 1 != o.k) {
+        const double nx = (v + 1 < o.k) ? W.lam[g + 1] : W.ddef[g + 1];
+        if ((v < o.k) ? (dv > nx) : (dv < nx)) atomicOr(&W.maux[4 * m + 3], 1ULL);
+    }
 }
 
-// copy the K updated eigenvectors from Ct into their physical slots
-__global__ __launch_bounds__(WG) void k_scatter(const Merge *mg, const int *col_start, int nm,
-                                                const MergeOut *mo, const long *coff, const Prob *probs,
-                                                Work W, int nxt, const double *Ct)
+// copy the K updated eigenvectors from Ct into their physical slots; only = 1: the
+// dense merges, 2: the HSS merges, 0: both
+__device__ __forceinline__ void k_scatter_at(const Merge *mg, const int *col_start, int nm, const MergeOut *mo,
+                                             const long *coff, const Prob *probs, const Work &W, int nxt,
+                                             const double *Ct, int only, int vb)
 {
-    int m = find_seg(col_start, nm, blockIdx.x);
+    int m = find_seg(col_start, nm, vb);
     Merge M = mg[m];
-    int v = blockIdx.x - col_start[m];
-    if (v >= mo[m].k) return;
+    int v = vb - col_start[m];
+    if (v >= mo[m].k || (only == 1 && mo[m].hss) || (only == 2 && !mo[m].hss)) return;
     Prob P = probs[M.prob];
     int slot = W.vm[nxt][M.s + v];
     const double *src = Ct + coff[m] + (long)v * M.n;
@@ -1381,14 +1523,167 @@ __global__ __launch_bounds__(WG) void k_scatter(const Merge *mg, const int *col_
     if (threadIdx.x == 0) { W.supp_lo[slot] = M.s; W.supp_hi[slot] = M.s + M.n; }
 }
 
-// INDXQ of the merged block: DLAMRG(K, N-K, D, 1, -1) (mdlaed1.f90:221)
-__global__ __launch_bounds__(WG) void k_lamrg(const Merge *mg, const MergeOut *mo, Work W, int nxt)
+__global__ __launch_bounds__(WG) void k_scatter(const Merge *mg, const int *col_start, int nm,
+                                                const MergeOut *mo, const long *coff, const Prob *probs,
+                                                Work W, int nxt, const double *Ct, int only)
 {
-    __shared__ int sflag;
-    Merge M = mg[blockIdx.x];
-    const MergeOut o = mo[blockIdx.x];
+    if (gate_closed(W, mg)) return;
+    const int nvb = col_start[nm];
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x)
+        k_scatter_at(mg, col_start, nm, mo, coff, probs, W, nxt, Ct, only, vb);
+}
+
+// INDXQ of the merged block: DLAMRG(K, N-K, D, 1, -1) (mdlaed1.f90:221), by the
+// merge path over the whole grid (one thread per element, chunk_start as in
+// k_newcols).  A merge whose runs k_newcols found out of order takes DLAMRG's
+// sequential loop in the first thread of its first chunk, so the permutation is
+// DLAMRG's in every case.
+__global__ __launch_bounds__(WG) void k_lamrg(const Merge *mg, const int *chunk_start, int nm, const MergeOut *mo,
+                                              Work W, int nxt)
+{
+    if (gate_closed(W, mg)) return;
+    const int m = find_seg(chunk_start, nm, blockIdx.x);
+    const MergeOut o = mo[m];
     if (o.flags) return;  // identity, set by k_newcols
-    block_merge(W.Dv[nxt] + M.s, o.k, M.n - o.k, 1, -1, W.indxq + M.s, &sflag);
+    const Merge M = mg[m];
+    const int n = M.n, n1 = o.k, n2 = M.n - o.k, c = blockIdx.x - chunk_start[m];
+    const double *a = W.Dv[nxt] + M.s;
+    int *out = W.indxq + M.s;
+    if (W.maux[4 * m + 3]) {
+        if (c == 0 && threadIdx.x == 0) {
+            int i1 = 0, i2 = 0, q = 0;
+            while (i1 < n1 && i2 < n2) {
+                if (a[i1] <= a[n - 1 - i2]) { out[q++] = i1 + 1; i1++; }
+                else { out[q++] = n - i2; i2++; }
+            }
+            while (i1 < n1) { out[q++] = i1 + 1; i1++; }
+            while (i2 < n2) { out[q++] = n - i2; i2++; }
+        }
+        return;
+    }
+    const int i = c * WG + threadIdx.x;
+    if (i >= n) return;
+    if (i < n1) {
+        const double v = a[i];
+        int lo = 0, hi = n2;  // run-2 values strictly below v
+        while (lo < hi) { int md = (lo + hi) >> 1; if (a[n - 1 - md] < v) lo = md + 1; else hi = md; }
+        out[i + lo] = i + 1;
+    } else {
+        const int q = i - n1;
+        const double v = a[n - 1 - q];
+        int lo = 0, hi = n1;  // run-1 values <= v
+        while (lo < hi) { int md = (lo + hi) >> 1; if (a[md] <= v) lo = md + 1; else hi = md; }
+        out[q + lo] = n - q;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-scheduled levels.  k_lvltab turns the deflation results of a level into
+// the tables its remaining kernels read (what the host builds after a round trip
+// in the synchronised schedule), so a whole solve is queued without waiting.
+// Per merge: chunks of 256 roots, secular blocks, columns, 64x64 GEMM tiles, S
+// size and Ct size; Ct holds the dense merges first, then the HSS merges, whose
+// vectors the host computes once it has seen the level.  Dense merges have
+// K < hss_min_k, so sbcap and ctcap (doubles, sized by the host from the merge
+// orders) bound the dense totals; a level beyond them does no work and sets bit
+// 1 of *info.  One block; thread t owns a contiguous run of merges.
+// ---------------------------------------------------------------------------
+#define LT_NT 256
+#define LT_Q 7
+__device__ __forceinline__ void lt_counts(const Merge &M, const MergeOut &o, long *q)
+{
+    const int k = o.flags ? 0 : o.k;
+    const bool hss = !o.flags && o.hss;
+    const int rpb = SEC_WG / sec_team(k > 0 ? k : 1);
+    q[0] = (k + WG - 1) / WG;
+    q[1] = (k + rpb - 1) / rpb;
+    q[2] = k;
+    q[3] = (k > 0 && !hss) ? (long)(((M.n1 + 63) >> 6) + ((M.n - M.n1 + 63) >> 6)) * ((k + 63) >> 6) : 0;
+    q[4] = (k > 0 && !hss) ? (long)k * k : 0;
+    q[5] = hss ? 0 : (long)M.n * k;
+    q[6] = hss ? (long)M.n * k : 0;
+}
+
+__global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const MergeOut *mo, Work W, LvlTab t,
+                                                 long sbcap, long ctcap, int *info)
+{
+    if (gate_closed(W, mg)) return;
+    __shared__ long wsum[LT_Q][LT_NT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int per = (nm + LT_NT - 1) / LT_NT;
+    const int a = min(nm, tid * per), b = min(nm, a + per);
+    long own[LT_Q], q[LT_Q], base[LT_Q], tot[LT_Q];
+    unsigned long long roots = 0, defl = 0, nh = 0;
+#pragma unroll
+    for (int i = 0; i < LT_Q; i++) own[i] = 0;
+    for (int m = a; m < b; m++) {
+        const Merge M = mg[m];
+        const MergeOut o = mo[m];
+        lt_counts(M, o, q);
+#pragma unroll
+        for (int i = 0; i < LT_Q; i++) own[i] += q[i];
+        roots += q[2];
+        defl += M.n - q[2];
+        nh += (!o.flags && o.hss) ? 1 : 0;
+        if (m == nm - 1) W.lstat[3] = (unsigned long long)q[2];
+    }
+    // exclusive prefix of every quantity: wavefront scan, then over wavefront totals
+#pragma unroll
+    for (int i = 0; i < LT_Q; i++) {
+        long x = own[i];
+        for (int off = 1; off < 64; off <<= 1) {
+            long y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[i][wv] = x;
+        base[i] = x - own[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < LT_Q; i++) {
+        long pre = 0, all = 0;
+#pragma unroll
+        for (int w = 0; w < LT_NT / 64; w++) {
+            pre += (w < wv) ? wsum[i][w] : 0;
+            all += wsum[i][w];
+        }
+        base[i] += pre;
+        tot[i] = all;
+    }
+    const bool over = tot[4] > sbcap || tot[5] > ctcap;
+    for (int m = a; m < b; m++) {
+        const MergeOut o = mo[m];
+        lt_counts(mg[m], o, q);
+        const bool hss = !o.flags && o.hss;
+        t.chunkK[m] = over ? 0 : (int)base[0];
+        t.blockK[m] = over ? 0 : (int)base[1];
+        t.colK[m] = over ? 0 : (int)base[2];
+        t.tileK[m] = over ? 0 : (int)base[3];
+        t.soff[m] = base[4];
+        t.coff[m] = hss ? tot[5] + base[6] : base[5];
+#pragma unroll
+        for (int i = 0; i < LT_Q; i++) base[i] += q[i];
+    }
+    if (tid == 0) {
+        t.chunkK[nm] = over ? 0 : (int)tot[0];
+        t.blockK[nm] = over ? 0 : (int)tot[1];
+        t.colK[nm] = over ? 0 : (int)tot[2];
+        t.tileK[nm] = over ? 0 : (int)tot[3];
+        if (over) atomicOr(info, 2);
+    }
+    // statistics, one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) {
+        roots += __shfl_xor(roots, off);
+        defl += __shfl_xor(defl, off);
+        nh += __shfl_xor(nh, off);
+    }
+    if (lane == 0 && roots) atomicAdd(&W.lstat[0], roots);
+    if (lane == 0 && defl) atomicAdd(&W.lstat[1], defl);
+    if (lane == 0 && nh) atomicAdd(&W.lstat[2], nh);
+    // HSS merges: the levels above wait for the host (this level's own kernels do
+    // not: they skip its HSS merges)
+    const int anyh = __syncthreads_or(nh > 0);
+    if (tid == 0 && anyh && !over) *W.gate = mg[0].lvl + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1401,6 +1696,7 @@ __global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_
                                               const long *ldz, int zrows_global, double *dout,
                                               const double *scale, unsigned long long *rows_read)
 {
+    if (*W.gate) return;  // a level waits for the host
     int g = blockIdx.x;  // global column
     int p = col_prob[g];
     Prob P = probs[p];

@@ -39,7 +39,8 @@ MLP_NPARAM = MLP_H * MLP_IN + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1
 class Params(ctypes.Structure):
     _fields_ = [("smlsiz", _i), ("hss_min_k", _i), ("hss_leaf", _i), ("hss_tol", _d),
                 ("maxit", _i), ("batched", _i), ("learned", _i), ("profile", _i),
-                ("learned_min_k", _i), ("learned_gate", _d)]
+                ("learned_min_k", _i), ("learned_gate", _d),
+                ("level_sync", _i), ("graph", _i)]
 
 
 class Stats(ctypes.Structure):
@@ -173,6 +174,12 @@ class Context:
         rc = lib().hsdc_set_params(self._h, ctypes.byref(self.params))
         if rc != 0:
             raise ValueError("hsdc_set_params rejected %r (%d)" % (kw, rc))
+
+    def set_stream(self, stream):
+        """HIP stream of the context's launches: a torch stream, a raw handle, or None
+        for the null stream."""
+        h = getattr(stream, "cuda_stream", stream)
+        return lib().hsdc_set_stream(self._h, ctypes.c_void_p(h or 0))
 
     def stats(self):
         s = Stats()

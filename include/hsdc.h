@@ -40,6 +40,11 @@ typedef struct {
     int profile;     /* 1: time the final reordering kernel with HIP events */
     int learned_min_k; /* merges with K below this keep the classical start (default 256) */
     double learned_gate; /* the proposal is taken when its |log-correction| exceeds this */
+    int level_sync;  /* batched only.  0: the whole solve is queued at once, each level sizing its
+                        launches on the device (one host round trip per solve, plus one per level
+                        holding HSS merges); 1: the host reads every level's K before its launches */
+    int graph;       /* 1 (default): a solve whose levels are all queued at once is captured as one
+                        HIP graph per shape (plan, buffers, parameters) and replayed; 0: direct launches */
 } hsdc_params;
 
 /* Run statistics of the last call on a context. */

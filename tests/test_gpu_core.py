@@ -137,6 +137,66 @@ def test_plain_schedule_bitexact_with_batched(torch_dev, ctx):
     assert np.array_equal(z1, z2)
 
 
+STAT_KEYS = ("levels", "merges", "hss_merges", "roots", "deflated", "k_top", "secular_iters", "rank_max")
+
+
+@pytest.mark.parametrize("cfg,n,hss_min_k,ni", [(2, 3000, 1 << 30, 200), (1, 2100, 128, 64),
+                                               (1, 700, 64, 32), (2, 20, 1 << 30, 200)])
+def test_device_schedule_bitexact_with_level_sync(torch_dev, ctx, cfg, n, hss_min_k, ni):
+    """The queued schedule (launch sizes and tables derived on the device, the host
+    called back only for levels holding HSS merges) and the per-level host round
+    trip compute the same bits: same kernels, same per-root arithmetic."""
+    d, e = tridiag_cfg(cfg, n, 60 + n)
+    kw = dict(hss_min_k=hss_min_k, hss_leaf=ni, hss_tol=1e-13, batched=1)
+    rc1, l1, z1 = solve_dev(torch_dev, ctx, d, e, level_sync=1, **kw)
+    s1 = ctx.stats()
+    rc0, l0, z0 = solve_dev(torch_dev, ctx, d, e, level_sync=0, **kw)
+    s0 = ctx.stats()
+    ctx.set_params(hss_min_k=1 << 30, hss_leaf=200, level_sync=0)
+    assert rc0 == 0 and rc1 == 0
+    assert np.array_equal(l0, l1)
+    assert np.array_equal(z0, z1)
+    for key in STAT_KEYS:
+        assert s0[key] == s1[key], key
+    if hss_min_k < n and cfg == 1:
+        assert s0["hss_merges"] > 1  # the host was called back at several levels
+    check_eig(d, e, l0, z0)
+
+
+def test_device_schedule_batched_problems_with_hss(torch_dev, ctx):
+    torch = torch_dev
+    sizes = [700, 30, 1300, 257, 5, 900]
+    ds, es = [], []
+    for i, n in enumerate(sizes):
+        d, e = tridiag_cfg(1 + (i % 2), n, 700 + i)
+        ds.append(d)
+        es.append(np.append(e, 0.0))
+    offs = np.cumsum([0] + sizes[:-1])
+    zoffs = np.cumsum([0] + [n * n for n in sizes[:-1]])
+    dev = torch.device("cuda:0")
+    out = []
+    for sync in (1, 0):
+        D = torch.tensor(np.concatenate(ds), device=dev)
+        E = torch.tensor(np.concatenate(es), device=dev)
+        Z = torch.zeros(int(sum(n * n for n in sizes)), dtype=torch.float64, device=dev)
+        ctx.set_params(hss_min_k=200, hss_leaf=32, hss_tol=1e-13, level_sync=sync)
+        rc = ctx.stedc_batched_dev(sizes, offs, D, E, Z, zoffs)
+        torch.cuda.synchronize()
+        assert rc == 0
+        out.append((D.cpu().numpy(), Z.cpu().numpy(), ctx.stats()))
+    ctx.set_params(hss_min_k=1 << 30, hss_leaf=200, level_sync=0)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    for key in STAT_KEYS:
+        assert out[0][2][key] == out[1][2][key], key
+    assert out[1][2]["hss_merges"] > 0
+    Dh, Zh = out[1][0], out[1][1]
+    for i, n in enumerate(sizes):
+        lam = Dh[offs[i]:offs[i] + n]
+        z = Zh[zoffs[i]:zoffs[i] + n * n].reshape(n, n).T
+        check_eig(ds[i], es[i][:n - 1], lam, z)
+
+
 def test_first_level_k_matches_oracle(torch_dev, ctx):
     """Leaves are bit-exact, so the first merge level sees identical inputs."""
     d, e = tridiag_cfg(2, 800, 21)
@@ -256,3 +316,42 @@ def test_autotune_picks_a_candidate_and_restores_inputs(torch_dev, ctx):
         del st
     finally:
         ctx.set_params(hss_min_k=1500, hss_tol=float(np.float32(1e-17)), hss_leaf=200, smlsiz=25)
+
+
+def _solve_pair(torch, ctx, d, e, **kw):
+    rc, lam, z = solve_dev(torch, ctx, d, e, **kw)
+    assert rc == 0
+    return lam, z, ctx.stats()
+
+
+@pytest.mark.parametrize("cfg,n,hss_min_k,ni", [(2, 3000, 1 << 30, 200), (1, 2100, 128, 64)])
+def test_graph_replays_match_direct_launches(torch_dev, ctx, cfg, n, hss_min_k, ni):
+    """A whole-solve HIP graph is captured on the first call of a shape and replayed:
+    every replay, with other work and synchronisations between them and with new
+    inputs of the same shape, equals the directly launched solve bit for bit.  With
+    HSS merges the replay stops at the first such level and the host continues."""
+    torch = torch_dev
+    kw = dict(hss_min_k=hss_min_k, hss_leaf=ni, hss_tol=1e-13, batched=1, level_sync=0)
+    inputs = [tridiag_cfg(cfg, n, 80 + n), tridiag_cfg(cfg, n, 81 + n)]
+    eager = [_solve_pair(torch, ctx, d, e, graph=0, **kw) for d, e in inputs]
+    junk = torch.randn(1 << 20, device="cuda:0", dtype=torch.float64)
+    for rep, i in enumerate([0, 0, 1, 0, 1]):
+        lam, z, st = _solve_pair(torch, ctx, *inputs[i], graph=1, **kw)
+        assert np.array_equal(lam, eager[i][0]), (rep, i)
+        assert np.array_equal(z, eager[i][1]), (rep, i)
+        for key in STAT_KEYS:
+            assert st[key] == eager[i][2][key], (rep, key)
+        junk.mul_(1.0001).add_(0.5)  # other work on the device between replays
+        torch.cuda.synchronize()
+    # the same on a caller stream
+    strm = torch.cuda.Stream()
+    ctx.set_stream(strm)
+    try:
+        with torch.cuda.stream(strm):
+            lam, z, st = _solve_pair(torch, ctx, *inputs[1], graph=1, **kw)
+            lam2, z2, _ = _solve_pair(torch, ctx, *inputs[0], graph=1, **kw)
+    finally:
+        ctx.set_stream(None)
+    assert np.array_equal(lam, eager[1][0]) and np.array_equal(z, eager[1][1])
+    assert np.array_equal(lam2, eager[0][0]) and np.array_equal(z2, eager[0][1])
+    ctx.set_params(hss_min_k=1 << 30, hss_leaf=200, graph=1)

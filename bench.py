@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 # HBM bytes per k_final launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated on a
 # 1 GiB device copy (profiles/k_final_n32768_uniform_traffic.md); keyed by (generator, n, seed)
-K_FINAL_TRAFFIC = {(2, 32768, 2): (220992 + 8400995) * 1024}
+K_FINAL_TRAFFIC = {(2, 32768, 2): (219530 + 8395081) * 1024}
 GEN = {1: "d=2+U(-1e-3,1e-3), e=1+U(-1e-3,1e-3)", 2: "d,e~U(-1,1)"}
 
 
@@ -181,7 +181,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--cfg", type=int, default=2)
     ap.add_argument("--seed", type=int, default=2)

@@ -23,7 +23,6 @@
 struct GTile { int m, half, r0, c0; };
 __global__ void k_leaf(const Leaf *, int, const Prob *, const double *, const double *, Work, int);
 __global__ void k_prep(const Merge *, const int *, int, const Prob *, Work, int);
-__global__ void k_sortmerge(const Merge *, Work);
 __global__ void k_mergepath(const Merge *, const int *, int, Work);
 __global__ void k_fill_u64(unsigned long long *, long, unsigned long long);
 template <int NT>
@@ -42,9 +41,11 @@ __global__ void k_newcols(const Merge *, const int *, int, const MergeOut *, Wor
 __global__ void k_scatter(const Merge *, const int *, int, const MergeOut *, const long *, const Prob *,
                           Work, int, const double *, int);
 __global__ void k_lamrg(const Merge *, const int *, int, const MergeOut *, Work, int);
+__global__ void k_scatter_dense(const Merge *, const int *, int, const MergeOut *, const long *, const Prob *, Work,
+                                int, const double *);
 __global__ void k_lvltab(const Merge *, int, const MergeOut *, Work, LvlTab, long, long, int *);
-__global__ void k_final(const Prob *, const int *, Work, int, const int *, double *, const long *,
-                        const long *, int, double *, const double *, unsigned long long *);
+__global__ void k_final(const Prob *, const int *, Work, const int *, double *, const long *, const long *, int,
+                        double *, const double *, unsigned long long *, const int *, int, int);
 __global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, long, double *, long);
 __global__ void k_hss_init(const HssMerge *);
 __global__ void k_hss_reset(const HssMerge *);
@@ -790,7 +791,6 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
     Work W = c->W;
     hipLaunchKernelGGL(k_prep, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur);
     hipLaunchKernelGGL(k_mergepath, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, W);
-    hipLaunchKernelGGL(k_sortmerge, dim3(nm), dim3(WGS), 0, s, dmg, W);
     // sorted D of merges up to 4096 in LDS for the deflated-region replay
     int nmax = 0;
     for (int i = m0; i < m1; i++) nmax = std::max(nmax, hm[i].n);
@@ -911,11 +911,12 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
 struct LvlBound {
     long blocks, tiles, cols;  // secular blocks, GEMM tiles, columns with K = N
     long sbcap, ctcap;         // S and Ct doubles of the dense merges (K < hss_min_k)
+    long scat;                 // copy chunks of the dense merges
 };
 
 static LvlBound level_bound(const std::vector<Merge> &hm, int hss_min_k)
 {
-    LvlBound b = {0, 0, 0, 0, 0};
+    LvlBound b = {0, 0, 0, 0, 0, 0};
     const long kd = hss_min_k >= 1 ? hss_min_k - 1 : 0;  // largest dense K
     for (const Merge &M : hm) {
         const long kmax = std::min((long)M.n, kd);
@@ -924,6 +925,7 @@ static LvlBound level_bound(const std::vector<Merge> &hm, int hss_min_k)
         b.cols += M.n;
         b.sbcap += kmax * kmax;
         b.ctcap += (long)M.n * kmax;
+        b.scat += ((long)M.n * kmax + SC_CH - 1) / SC_CH;
     }
     return b;
 }
@@ -936,11 +938,12 @@ static LvlTab level_tables(hsdc_ctx *c, int nm)
     char *p = (char *)c->ltab.p;
     size_t ni = al64(sizeof(int) * (nm + 1)), nl = al64(sizeof(long) * nm);
     t.chunkK = (int *)p; t.blockK = (int *)(p + ni); t.colK = (int *)(p + 2 * ni); t.tileK = (int *)(p + 3 * ni);
-    t.soff = (long *)(p + 4 * ni); t.coff = (long *)(p + 4 * ni + nl);
+    t.scK = (int *)(p + 4 * ni);
+    t.soff = (long *)(p + 5 * ni); t.coff = (long *)(p + 5 * ni + nl);
     return t;
 }
 
-static size_t level_tables_bytes(int nm) { return 4 * al64(sizeof(int) * (nm + 1)) + 2 * al64(sizeof(long) * nm); }
+static size_t level_tables_bytes(int nm) { return 5 * al64(sizeof(int) * (nm + 1)) + 2 * al64(sizeof(long) * nm); }
 
 // Queue one level (merges of table dmg) on the device schedule.
 static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, int L, const int *dchunkN,
@@ -956,7 +959,6 @@ static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
     const int ds_cap = nmax <= 4096 ? nmax : 0;
     hipLaunchKernelGGL(k_prep, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur);
     hipLaunchKernelGGL(k_mergepath, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, W);
-    hipLaunchKernelGGL(k_sortmerge, dim3(nm), dim3(WGS), 0, s, dmg, W);
     if (nmax >= 2048)
         hipLaunchKernelGGL(k_deflate<1024>, dim3(nm), dim3(1024), sizeof(double) * ds_cap, s, dmg, W, cur, dmo,
                            c->prm.hss_min_k, ds_cap);
@@ -968,7 +970,7 @@ static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
     hipLaunchKernelGGL(k_rot_prep, dim3(nm), dim3(WGS), 0, s, dmg, dprobs, W, cur, (const MergeOut *)dmo);
     hipLaunchKernelGGL(k_rotate, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur,
                        (const MergeOut *)dmo);
-    if (c->sec_pre)
+    if (c->sec_pre && nmax >= std::max(3, c->prm.learned_min_k))  // else no merge can use it
         hipLaunchKernelGGL(k_secstart, dim3(dev_grid(B.blocks)), dim3(SEC_WG), 0, s, dmg, (const int *)t.blockK, nm,
                            (const MergeOut *)dmo, W, c->dsec);
     hipLaunchKernelGGL(k_secular, dim3(dev_grid(B.blocks)), dim3(SEC_WG), 0, s, dmg, (const int *)t.blockK, nm,
@@ -983,8 +985,9 @@ static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
                            (const int *)t.tileK, nm, dprobs, (const MergeOut *)dmo, (const long *)t.soff,
                            (const long *)t.coff, W, cur, (const double *)c->sb.p, (double *)c->ct.p);
     hipLaunchKernelGGL(k_newcols, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur);
-    hipLaunchKernelGGL(k_scatter, dim3(dev_grid(B.cols)), dim3(WGS), 0, s, dmg, (const int *)t.colK, nm,
-                       (const MergeOut *)dmo, (const long *)t.coff, dprobs, W, cur ^ 1, (const double *)c->ct.p, 1);
+    if (B.scat > 0)
+        hipLaunchKernelGGL(k_scatter_dense, dim3(dev_grid(B.scat)), dim3(WGS), 0, s, dmg, (const int *)t.scK, nm,
+                           (const MergeOut *)dmo, (const long *)t.coff, dprobs, W, cur ^ 1, (const double *)c->ct.p);
     hipLaunchKernelGGL(k_lamrg, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur ^ 1);
     HK(hipGetLastError());
     return 0;
@@ -1089,9 +1092,15 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         for (auto &M : L) { acc += cdiv(M.n, WGS); chunks.push_back(acc); }
         for (auto &M : L) { mchunk.push_back(chunks.size()); chunks.push_back(0); chunks.push_back(cdiv(M.n, WGS)); }
     }
-    size_t a12 = al64(a11 + sizeof(int) * chunks.size());
-    CK(ensure(c->probs, a12));
-    CK(ensure_pin(c->pin, a12));
+    // final copy: row chunks of FIN_CH per output column; per problem, its first
+    // block (one output block per problem) -- zglobal: one n x n output, ntot rows
+    std::vector<int> fbs(np + 1, 0);
+    for (size_t i = 0; i < np; i++) fbs[i + 1] = fbs[i] + pl.probs[i].n * cdiv(pl.probs[i].n, FIN_CH);
+    const int fin_cz = cdiv(ntot, FIN_CH);
+    const long fin_blocks = zglobal ? (long)ntot * fin_cz : (long)fbs[np];
+    size_t a12 = al64(a11 + sizeof(int) * chunks.size()), a13 = al64(a12 + sizeof(int) * (np + 1));
+    CK(ensure(c->probs, a13));
+    CK(ensure_pin(c->pin, a13));
     char *h = (char *)c->pin.p;
     memcpy(h + a0, hp.data(), sizeof(Prob) * np);
     memcpy(h + a1, pl.leaves.data(), sizeof(Leaf) * nl);
@@ -1120,8 +1129,10 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     if (outcol) memcpy(h + a9, outcol->data(), sizeof(int) * ntot);
     memset(h + a10, 0, 64 + 512);
     if (!chunks.empty()) memcpy(h + a11, chunks.data(), sizeof(int) * chunks.size());
+    memcpy(h + a12, fbs.data(), sizeof(int) * (np + 1));
     char *dp = (char *)c->probs.p;
-    HK(hipMemcpyAsync(dp, h, a12, hipMemcpyHostToDevice, s));
+    HK(hipMemcpyAsync(dp, h, a13, hipMemcpyHostToDevice, s));
+    const int *dfbs = (const int *)(dp + a12);
     const int *dchunks = (const int *)(dp + a11);
     Prob *dprobs = (Prob *)(dp + a0);
     Leaf *dleaves = (Leaf *)(dp + a1);
@@ -1171,8 +1182,8 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     };
     auto launch_final = [&]() -> int {
         if (c->prm.profile) HK(hipEventRecord(f0, s));
-        hipLaunchKernelGGL(k_final, dim3(ntot), dim3(WGS), 0, s, dprobs, dcolprob, c->W, nlv & 1, doutcol, dZ, dzoff,
-                           dldz, zrows, d_out_dev, dscale, drows);
+        hipLaunchKernelGGL(k_final, dim3(fin_blocks), dim3(WGS), 0, s, dprobs, dcolprob, c->W, doutcol, dZ, dzoff,
+                           dldz, zrows, d_out_dev, dscale, drows, dfbs, (int)np, fin_cz);
         HK(hipGetLastError());
         if (c->prm.profile) HK(hipEventRecord(f1, s));
         return 0;
@@ -1188,7 +1199,8 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     bool queued = false;  // levels and final kernel already in the stream
     if (all_dev && use_graph && s != nullptr) {
         std::vector<long> key = {(long)nlv, (long)np, (long)nl, (long)lmax, (long)ntot, (long)c->wcap, zrows,
-                                 (long)(outcol != nullptr), c->prm.maxit, c->prm.hss_min_k, (long)c->sec_pre,
+                                 fin_blocks, (long)(outcol != nullptr), c->prm.maxit, c->prm.hss_min_k,
+                                 (long)c->sec_pre, (long)c->prm.learned_min_k,
                                  (long)(uintptr_t)c->stream};
         for (const void *ptr : {(const void *)c->wsd.p, (const void *)c->wsi.p, (const void *)c->maux.p,
                                 (const void *)c->probs.p, (const void *)c->mo.p, (const void *)c->ltab.p,
@@ -1199,7 +1211,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
             int nmax = 0;
             for (const Merge &M : pl.lv[L]) nmax = std::max(nmax, M.n);
             key.insert(key.end(), {(long)pl.lv[L].size(), (long)chunks[lvchunk[L] + pl.lv[L].size()], (long)nmax,
-                                   lb[L].blocks, lb[L].tiles, lb[L].cols, lb[L].sbcap, lb[L].ctcap});
+                                   lb[L].blocks, lb[L].tiles, lb[L].cols, lb[L].sbcap, lb[L].ctcap, lb[L].scat});
         }
         if (!c->gexec || key != c->gkey) {
             if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -1633,7 +1645,6 @@ int hsdc_deflate(hsdc_ctx *c, int n, int n1, double *d, double *z, int *indxq, d
                        (const Prob *)nullptr, W, 0);
     hipLaunchKernelGGL(k_mergepath, dim3(cdiv(n, WGS)), dim3(WGS), 0, c->stream, (const Merge *)dm,
                        (const int *)dch, 1, W);
-    hipLaunchKernelGGL(k_sortmerge, dim3(1), dim3(WGS), 0, c->stream, (const Merge *)dm, W);
     const int ds_cap = n <= 4096 ? n : 0;
     if (n >= 2048)
         hipLaunchKernelGGL(k_deflate<1024>, dim3(1), dim3(1024), sizeof(double) * ds_cap, c->stream,

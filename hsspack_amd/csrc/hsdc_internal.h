@@ -84,11 +84,14 @@ __device__ __forceinline__ bool gate_closed(const Work &W, const Merge *mg)
 }
 
 // Tables of one level derived from its deflation results (k_lvltab): first chunk
-// (256 roots), secular block, column and GEMM tile of every merge, offsets of its
-// S and Ct blocks.  Ct holds the dense merges first, then the HSS merges.
+// (256 roots), secular block, column, GEMM tile and copy chunk of every merge,
+// offsets of its S and Ct blocks.  Ct holds the dense merges first, then the HSS
+// merges.
+#define SC_CH 2048    // elements per work item of the dense eigenvector copy
+#define FIN_CH 1024   // rows per block of the final reordering copy
 struct LvlTab {
-    int *chunkK, *blockK, *colK, *tileK;  // nm + 1 entries each
-    long *soff, *coff;                    // nm entries each
+    int *chunkK, *blockK, *colK, *tileK, *scK;  // nm + 1 entries each
+    long *soff, *coff;                          // nm entries each
 };
 
 // control block of the secular kernel: learned-start weights, statistics and

@@ -211,45 +211,6 @@ __device__ __forceinline__ double maux_tol(const Work &W, int m)
     return 8.0 * HSDC_EPS * (dmax > zmax ? dmax : zmax);
 }
 
-// The merged order (DLAMRG(N1, N2, DLAMDA, 1, 1, INDXC), mdlaed2.f90:252-255) of every
-// merge with ascending runs, one thread per element over all merges: each element's
-// rank in the other run by binary search (ties to the first run).  Also the sorted
-// gathers of the deflation scan: INDX = INDXQ(INDXC), z and D in that order, and the
-// small-z column type (mdlaed2.f90:264-300).
-__global__ __launch_bounds__(WG) void k_mergepath(const Merge *mg, const int *chunk_start, int nm, Work W)
-{
-    if (gate_closed(W, mg)) return;
-    int m = find_seg(chunk_start, nm, blockIdx.x);
-    if (W.maux[4 * m + 2]) return;  // k_sortmerge merges it sequentially
-    Merge M = mg[m];
-    int i = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
-    if (i >= M.n) return;
-    const double *a = W.dlamda + M.s;
-    const int n1 = M.n1, n2 = M.n - M.n1;
-    const double v = a[i];
-    int out, ic;
-    if (i < n1) {
-        int lo = 0, hi = n2;  // run-2 values strictly below v
-        while (lo < hi) { int md = (lo + hi) >> 1; if (a[n1 + md] < v) lo = md + 1; else hi = md; }
-        out = i + lo;
-        ic = i + 1;
-    } else {
-        int q = i - n1, lo = 0, hi = n1;  // run-1 values <= v
-        while (lo < hi) { int md = (lo + hi) >> 1; if (a[md] <= v) lo = md + 1; else hi = md; }
-        out = q + lo;
-        ic = n1 + q + 1;
-    }
-    const int s = M.s;
-    W.indxc[s + out] = ic;
-    const int nj = W.indxq2[s + ic - 1];
-    W.indx[s + out] = nj;
-    const double zz = W.z[s + nj - 1];
-    W.zs[s + out] = zz;
-    W.ds[s + out] = v;  // D(INDX(out)) = DLAMDA(INDXC(out))
-    const double rho = fabs(2.0 * M.rho), tol = maux_tol(W, m);
-    W.coltyp[s + nj - 1] = (rho * fabs(zz) <= tol) ? 4 : ((nj <= n1) ? 1 : 3);
-}
-
 // Stable merge of two sorted runs (LAPACK DLAMRG semantics, ties to the first
 // run) by the merge path, one block per merge; falls back to the sequential
 // merge when a run is out of order so the permutation stays DLAMRG's.
@@ -295,16 +256,13 @@ __device__ void block_merge(const double *a, int n1, int n2, int s1, int s2, int
     __syncthreads();
 }
 
-// The merges k_mergepath leaves (a run out of order): DLAMRG(N1,N2,DLAMDA,1,1,INDXC)
-// by one block, then the same gathers.
-__global__ __launch_bounds__(WG) void k_sortmerge(const Merge *mg, Work W)
+// A merge with a run out of order: DLAMRG(N1,N2,DLAMDA,1,1,INDXC) by one block,
+// then the gathers of k_mergepath.
+__device__ void sort_merge(const Merge &M, int m, const Work &W)
 {
-    if (gate_closed(W, mg)) return;
     __shared__ int sflag;
-    if (!W.maux[4 * blockIdx.x + 2]) return;
-    Merge M = mg[blockIdx.x];
     block_merge(W.dlamda + M.s, M.n1, M.n - M.n1, 1, 1, W.indxc + M.s, &sflag);
-    const double rho = fabs(2.0 * M.rho), tol = maux_tol(W, blockIdx.x);
+    const double rho = fabs(2.0 * M.rho), tol = maux_tol(W, m);
     for (int i = threadIdx.x; i < M.n; i += WG) {
         int nj = W.indxq2[M.s + W.indxc[M.s + i] - 1];
         W.indx[M.s + i] = nj;
@@ -313,6 +271,48 @@ __global__ __launch_bounds__(WG) void k_sortmerge(const Merge *mg, Work W)
         W.ds[M.s + i] = W.dlamda[M.s + W.indxc[M.s + i] - 1];
         W.coltyp[M.s + nj - 1] = (rho * fabs(zz) <= tol) ? 4 : ((nj <= M.n1) ? 1 : 3);
     }
+}
+
+// The merged order (DLAMRG(N1, N2, DLAMDA, 1, 1, INDXC), mdlaed2.f90:252-255) of every
+// merge with ascending runs, one thread per element over all merges: each element's
+// rank in the other run by binary search (ties to the first run).  Also the sorted
+// gathers of the deflation scan: INDX = INDXQ(INDXC), z and D in that order, and the
+// small-z column type (mdlaed2.f90:264-300).
+__global__ __launch_bounds__(WG) void k_mergepath(const Merge *mg, const int *chunk_start, int nm, Work W)
+{
+    if (gate_closed(W, mg)) return;
+    int m = find_seg(chunk_start, nm, blockIdx.x);
+    if (W.maux[4 * m + 2]) {  // a run out of order (k_prep): the first block merges it alone
+        if (blockIdx.x == chunk_start[m]) sort_merge(mg[m], m, W);
+        return;
+    }
+    Merge M = mg[m];
+    int i = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    if (i >= M.n) return;
+    const double *a = W.dlamda + M.s;
+    const int n1 = M.n1, n2 = M.n - M.n1;
+    const double v = a[i];
+    int out, ic;
+    if (i < n1) {
+        int lo = 0, hi = n2;  // run-2 values strictly below v
+        while (lo < hi) { int md = (lo + hi) >> 1; if (a[n1 + md] < v) lo = md + 1; else hi = md; }
+        out = i + lo;
+        ic = i + 1;
+    } else {
+        int q = i - n1, lo = 0, hi = n1;  // run-1 values <= v
+        while (lo < hi) { int md = (lo + hi) >> 1; if (a[md] <= v) lo = md + 1; else hi = md; }
+        out = q + lo;
+        ic = n1 + q + 1;
+    }
+    const int s = M.s;
+    W.indxc[s + out] = ic;
+    const int nj = W.indxq2[s + ic - 1];
+    W.indx[s + out] = nj;
+    const double zz = W.z[s + nj - 1];
+    W.zs[s + out] = zz;
+    W.ds[s + out] = v;  // D(INDX(out)) = DLAMDA(INDXC(out))
+    const double rho = fabs(2.0 * M.rho), tol = maux_tol(W, m);
+    W.coltyp[s + nj - 1] = (rho * fabs(zz) <= tol) ? 4 : ((nj <= n1) ? 1 : 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -1295,6 +1295,7 @@ __global__ __launch_bounds__(WG) void k_sbuild(const Merge *mg, const int *chunk
     if (gate_closed(W, mg)) return;
     __shared__ double sd[LW_LDS], sw[LW_LDS];
     __shared__ int sc[LW_LDS];
+    __shared__ double sdo[WG], stj[WG], stm[WG];  // per column of the block: d_org, tau, norm
     const int nvb = chunk_start[nm];
     for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
         const int m = find_seg(chunk_start, nm, vb);
@@ -1320,18 +1321,28 @@ __global__ __launch_bounds__(WG) void k_sbuild(const Merge *mg, const int *chunk
             __syncthreads();
             d = sd; wz = sw; indxc = sc;
         }
+        // column norms, one thread per column j (the sum runs over i in order);
+        // delta_i(j) = (d_i - d_org(j)) - tau_j, as delta_ij forms it
         if (j < K) {
-            // delta_i(j) = (d_i - d_org(j)) - tau_j, as delta_ij forms it
             const double dorg = d[W.org[s + j] - 1], tj = W.tau[s + j];
             double acc = 0.0;
             for (int i = 0; i < K; i++) {
                 double v = wz[i] / ((d[i] - dorg) - tj);
                 acc += v * v;
             }
-            double temp = sqrt(acc);
-            for (int t = 0; t < K; t++) {
+            sdo[threadIdx.x] = dorg;
+            stj[threadIdx.x] = tj;
+            stm[threadIdx.x] = sqrt(acc);
+        }
+        __syncthreads();
+        // the K x ncols entries, spread over all threads: WG / ncols rows per pass
+        const int c0 = j - threadIdx.x, ncols = min(WG, K - c0), rpp = WG / ncols;
+        const int jj = threadIdx.x % ncols, t0 = threadIdx.x / ncols;
+        if (t0 < rpp) {
+            const double dorg = sdo[jj], tj = stj[jj], temp = stm[jj];
+            for (int t = t0; t < K; t += rpp) {
                 int i = indxc[t] - 1;
-                Sm[(long)t * K + j] = (wz[i] / ((d[i] - dorg) - tj)) / temp;
+                Sm[(long)t * K + c0 + jj] = (wz[i] / ((d[i] - dorg) - tj)) / temp;
             }
         }
         __syncthreads();  // the next virtual block restages
@@ -1533,6 +1544,33 @@ __global__ __launch_bounds__(WG) void k_scatter(const Merge *mg, const int *col_
         k_scatter_at(mg, col_start, nm, mo, coff, probs, W, nxt, Ct, only, vb);
 }
 
+// The same copy for the dense merges of a device-scheduled level, over the
+// elements of each merge's N x K block of Ct in chunks of SC_CH (sc_start: first
+// chunk of every merge), so that the many short columns of the low levels do not
+// each hold a block.
+__global__ __launch_bounds__(WG) void k_scatter_dense(const Merge *mg, const int *sc_start, int nm,
+                                                      const MergeOut *mo, const long *coff, const Prob *probs,
+                                                      Work W, int nxt, const double *Ct)
+{
+    if (gate_closed(W, mg)) return;
+    const int nvb = sc_start[nm];
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        const int m = find_seg(sc_start, nm, vb);
+        const Merge M = mg[m];
+        const Prob P = probs[M.prob];
+        const long ne = (long)M.n * mo[m].k, e0 = (long)(vb - sc_start[m]) * SC_CH;
+        const double *src = Ct + coff[m];
+        double *q = P.q + (M.s - P.off);
+        for (long e = e0 + threadIdx.x; e < e0 + SC_CH && e < ne; e += WG) {
+            // 32-bit division (N K < 2^31 for every dense merge a level can hold)
+            const int v = (int)((unsigned)e / (unsigned)M.n), r = (int)e - v * M.n;
+            const int slot = W.vm[nxt][M.s + v];
+            q[(long)(slot - P.off) * P.ld + r] = src[e];
+            if (r == 0) { W.supp_lo[slot] = M.s; W.supp_hi[slot] = M.s + M.n; }
+        }
+    }
+}
+
 // INDXQ of the merged block: DLAMRG(K, N-K, D, 1, -1) (mdlaed1.f90:221), by the
 // merge path over the whole grid (one thread per element, chunk_start as in
 // k_newcols).  A merge whose runs k_newcols found out of order takes DLAMRG's
@@ -1589,7 +1627,7 @@ __global__ __launch_bounds__(WG) void k_lamrg(const Merge *mg, const int *chunk_
 // 1 of *info.  One block; thread t owns a contiguous run of merges.
 // ---------------------------------------------------------------------------
 #define LT_NT 256
-#define LT_Q 7
+#define LT_Q 8
 __device__ __forceinline__ void lt_counts(const Merge &M, const MergeOut &o, long *q)
 {
     const int k = o.flags ? 0 : o.k;
@@ -1602,6 +1640,7 @@ __device__ __forceinline__ void lt_counts(const Merge &M, const MergeOut &o, lon
     q[4] = (k > 0 && !hss) ? (long)k * k : 0;
     q[5] = hss ? 0 : (long)M.n * k;
     q[6] = hss ? (long)M.n * k : 0;
+    q[7] = hss ? 0 : ((long)M.n * k + SC_CH - 1) / SC_CH;
 }
 
 __global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const MergeOut *mo, Work W, LvlTab t,
@@ -1659,6 +1698,7 @@ __global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const
         t.blockK[m] = over ? 0 : (int)base[1];
         t.colK[m] = over ? 0 : (int)base[2];
         t.tileK[m] = over ? 0 : (int)base[3];
+        t.scK[m] = over ? 0 : (int)base[7];
         t.soff[m] = base[4];
         t.coff[m] = hss ? tot[5] + base[6] : base[5];
 #pragma unroll
@@ -1669,6 +1709,7 @@ __global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const
         t.blockK[nm] = over ? 0 : (int)tot[1];
         t.colK[nm] = over ? 0 : (int)tot[2];
         t.tileK[nm] = over ? 0 : (int)tot[3];
+        t.scK[nm] = over ? 0 : (int)tot[7];
         if (over) atomicOr(info, 2);
     }
     // statistics, one atomic per wavefront
@@ -1688,25 +1729,41 @@ __global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const
 
 // ---------------------------------------------------------------------------
 // Final reordering (mdlaed0.f90:355-362 and the scale-back of mdstedc.f90:358):
-// Z(:, i) = Q(:, slot of INDXQ(i)), zero outside the column's support.
-// One block per output column.
+// Z(:, i) = Q(:, slot of INDXQ(i)), zero outside the column's support.  One block
+// per FIN_CH rows of an output column, consecutive blocks on consecutive rows of
+// one column, so the blocks in flight write one contiguous front of Z (blocks
+// owning whole columns write thousands of separate streams and reach about 5.1
+// TB/s against 5.9 TB/s measured for this order).  zrows_global: one n x n output,
+// cz chunks per column; else problem p owns blocks [fbs[p], fbs[p+1]) of its own
+// output block.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_prob, Work W, int cur,
-                                              const int *outcol, double *Z, const long *zoff,
-                                              const long *ldz, int zrows_global, double *dout,
-                                              const double *scale, unsigned long long *rows_read)
+__global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_prob, Work W, const int *outcol,
+                                              double *Z, const long *zoff, const long *ldz, int zrows_global,
+                                              double *dout, const double *scale, unsigned long long *rows_read,
+                                              const int *fbs, int np, int cz)
 {
     if (*W.gate) return;  // a level waits for the host
-    int g = blockIdx.x;  // global column
-    int p = col_prob[g];
-    Prob P = probs[p];
-    int i = g - P.off;
-    int v = W.indxq[g];  // local virtual column (1-based)
-    int vg = P.off + v - 1;
-    int slot = W.vm[P.par][vg];
-    int oc = outcol ? outcol[g] : g;  // output column (global numbering)
-    if (threadIdx.x == 0) dout[oc] = W.Dv[P.par][vg] * scale[p];
+    int g, c, p;
+    if (zrows_global) {
+        g = blockIdx.x / cz;
+        c = blockIdx.x - g * cz;
+        p = col_prob[g];
+    } else {
+        p = find_seg(fbs, np, blockIdx.x);
+        const int n = probs[p].n, cp = (n + FIN_CH - 1) / FIN_CH, loc = blockIdx.x - fbs[p];
+        g = probs[p].off + loc / cp;
+        c = loc - (loc / cp) * cp;
+    }
+    const Prob P = probs[p];
+    const int v = W.indxq[g];  // local virtual column (1-based)
+    const int vg = P.off + v - 1;
+    const int slot = W.vm[P.par][vg];
+    const int oc = outcol ? outcol[g] : g;  // output column (global numbering)
     const int lo = W.supp_lo[slot], hi = W.supp_hi[slot];
+    if (c == 0 && threadIdx.x == 0) {
+        dout[oc] = W.Dv[P.par][vg] * scale[p];
+        atomicAdd(rows_read + (g & 63), (unsigned long long)(hi - lo));  // rows read, 64 counters
+    }
     const double *src = P.q + (long)(slot - P.off) * P.ld - P.off;
     double *dst;
     int r0, r1;
@@ -1717,30 +1774,28 @@ __global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_
         dst = Z + zoff[p] + (long)(oc - P.off) * ldz[p] - P.off;
         r0 = P.off; r1 = P.off + P.n;
     }
+    const int c0 = r0 + c * FIN_CH, c1 = min(r1, c0 + FIN_CH);  // this block's rows
     // streaming stores, 16 bytes per lane when the column start is 16-byte aligned
-    double *d0 = dst + r0;
-    const int nr = r1 - r0;
+    // (FIN_CH is even, so every chunk start keeps that alignment)
     typedef double dbl2 __attribute__((ext_vector_type(2)));
-    if ((((unsigned long)d0) & 15) == 0) {
-        dbl2 *d2 = (dbl2 *)d0;
+    if ((((unsigned long)(dst + r0)) & 15) == 0) {
+        dbl2 *d2 = (dbl2 *)(dst + c0);
+        const int nr = c1 - c0;
         for (int q = threadIdx.x; q < (nr >> 1); q += WG) {
-            const int r = r0 + 2 * q;
-            dbl2 v;
-            v.x = (r >= lo && r < hi) ? src[r] : 0.0;
-            v.y = (r + 1 >= lo && r + 1 < hi) ? src[r + 1] : 0.0;
-            __builtin_nontemporal_store(v, d2 + q);
+            const int r = c0 + 2 * q;
+            dbl2 val;
+            val.x = (r >= lo && r < hi) ? src[r] : 0.0;
+            val.y = (r + 1 >= lo && r + 1 < hi) ? src[r + 1] : 0.0;
+            __builtin_nontemporal_store(val, d2 + q);
         }
         if ((nr & 1) && threadIdx.x == 0) {
-            const int r = r1 - 1;
+            const int r = c1 - 1;
             __builtin_nontemporal_store((r >= lo && r < hi) ? src[r] : 0.0, dst + r);
         }
     } else {
-        for (int r = r0 + threadIdx.x; r < r1; r += WG)
+        for (int r = c0 + threadIdx.x; r < c1; r += WG)
             __builtin_nontemporal_store((r >= lo && r < hi) ? src[r] : 0.0, dst + r);
     }
-    // rows read, spread over 64 counters
-    if (threadIdx.x == 0) atomicAdd(rows_read + (g & 63), (unsigned long long)(hi - lo));
-    (void)i;
 }
 
 // Plain C = A * B (column-major) on FP64 MFMA, for COMPZ = 'V' (Z := Zin * Q).

@@ -41,6 +41,11 @@ __global__ void k_newcols(const Merge *, const int *, int, const MergeOut *, Wor
 __global__ void k_scatter(const Merge *, const int *, int, const MergeOut *, const long *, const Prob *,
                           Work, int, const double *, int);
 __global__ void k_lamrg(const Merge *, const int *, int, const MergeOut *, Work, int);
+__global__ void k_readback(Work, const int *, const unsigned long long *, const SecCtl *, const unsigned int *,
+                           SolveRec *);
+__global__ void k_inp_scan(const double *, const double *, int, Work);
+__global__ void k_inp_scale(const double *, const double *, int, const int *, Merge *, int, double *, double *,
+                            double *, Work);
 __global__ void k_scatter_dense(const Merge *, const int *, int, const MergeOut *, const long *, const Prob *, Work,
                                 int, const double *);
 __global__ void k_lvltab(const Merge *, int, const MergeOut *, Work, LvlTab, long, long, int *);
@@ -113,6 +118,20 @@ inline size_t al64(size_t x) { return (x + 63) & ~(size_t)63; }
 
 }  // namespace
 
+// host plan of one solve: its problems, the leaves and the merges of every level
+struct HostProb {
+    int off, n;        // global offset and order
+    double scale;      // eigenvalues are multiplied back by this (mdstedc.f90:358)
+    int levels;        // DC merge levels
+};
+
+struct Plan {
+    int ntot = 0;
+    std::vector<HostProb> probs;
+    std::vector<Leaf> leaves;
+    std::vector<std::vector<Merge>> lv;  // merges per level (level 0 = first merge level)
+};
+
 struct hsdc_ctx {
     hsdc_params prm;
     hsdc_stats st;
@@ -138,28 +157,17 @@ struct hsdc_ctx {
     bool sec_pre = false;              // k_secstart runs before k_secular
     DevBuf featb;
     DevBuf ltab;  // tables of the device-scheduled level (k_lvltab)
+    std::vector<long> tabkey;  // shape of the static tables kept on the device (device input)
+    Plan fplan;                // plan of a device-input solve, by order and leaf bound
+    int fplan_n = -1, fplan_sml = -1;
     hipGraphExec_t gexec = nullptr;  // the last captured solve and its shape
     std::vector<long> gkey;
     hipStream_t gstream = nullptr;   // stream of graph solves called on the null stream
+    hipEvent_t pf0 = nullptr, pf1 = nullptr;  // around the final kernel (profile = 1)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around a whole solve
+    SolveRec *rec = nullptr, *drec = nullptr;  // end-of-solve record: mapped host memory, its device address
 };
 
-#define SEC_LEVELS 128
-
-// ----------------------------------------------------------------------------
-// host plan of one solve
-// ----------------------------------------------------------------------------
-struct HostProb {
-    int off, n;        // global offset and order
-    double scale;      // eigenvalues are multiplied back by this (mdstedc.f90:358)
-    int levels;        // DC merge levels
-};
-
-struct Plan {
-    int ntot = 0;
-    std::vector<HostProb> probs;
-    std::vector<Leaf> leaves;
-    std::vector<std::vector<Merge>> lv;  // merges per level (level 0 = first merge level)
-};
 
 // DC tree of one problem (mdlaed0.f90:192-219): halve every block until the last
 // one is <= smlsiz, tear with |E| at each cut.
@@ -1033,12 +1041,19 @@ static int level_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
     return 0;
 }
 
-// Solve every problem of the plan; d_in/e_in are the torn, scaled host arrays.
+// Solve every problem of the plan; hd/he are the torn, scaled host arrays.
 // Output: either one global Z (zrows_global = ntot, ldz) or per-problem blocks.
+// With din_d/din_e (device, one problem, plan built from its order alone) the
+// device scans, scales and tears the input itself, the static tables stay on the
+// device between calls of the same shape, and SOLVE_ON_HOST comes back when the
+// input needs the host's split (or is zero or not finite).
+#define SOLVE_ON_HOST 0x7fff0001
 static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, double *dZ,
                  const std::vector<long> &zoff, const std::vector<long> &ldz, int zglobal,
-                 double *d_out_dev, const std::vector<int> *outcol)
+                 double *d_out_dev, const std::vector<int> *outcol, const double *din_d = nullptr,
+                 const double *din_e = nullptr)
 {
+    const bool dev_in = din_d != nullptr;
     // graph capture needs a stream other than the null stream: a solve on the null
     // stream runs on the context's own stream once the null stream's work is done
     struct StreamRestore {
@@ -1046,6 +1061,8 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         hipStream_t s0;
         ~StreamRestore() { c->stream = s0; }
     } restore{c, c->stream};
+    // (an event recorded under stream capture does not time the replays: profile = 1
+    // launches directly)
     const bool use_graph = c->prm.graph && c->prm.batched && !c->prm.level_sync && !c->prm.profile;
     if (use_graph && c->stream == nullptr) {
         HK(hipStreamSynchronize(nullptr));
@@ -1054,9 +1071,15 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     }
     hipStream_t s = c->stream;
     const int ntot = pl.ntot;
-    hipEvent_t e0, e1;
-    HK(hipEventCreate(&e0));
-    HK(hipEventCreate(&e1));
+    if (!c->ev0) {
+        HK(hipEventCreate(&c->ev0));
+        HK(hipEventCreate(&c->ev1));
+    }
+    if (!c->rec) {
+        HK(hipHostMalloc((void **)&c->rec, sizeof(SolveRec), hipHostMallocMapped | hipHostMallocCoherent));
+        HK(hipHostGetDevicePointer((void **)&c->drec, c->rec, 0));
+    }
+    hipEvent_t e0 = c->ev0, e1 = c->ev1;
     HK(hipEventRecord(e0, s));
     CK(alloc_work(c, ntot));
     // Q storage: one n x n block per problem
@@ -1099,40 +1122,59 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     const int fin_cz = cdiv(ntot, FIN_CH);
     const long fin_blocks = zglobal ? (long)ntot * fin_cz : (long)fbs[np];
     size_t a12 = al64(a11 + sizeof(int) * chunks.size()), a13 = al64(a12 + sizeof(int) * (np + 1));
-    CK(ensure(c->probs, a13));
-    CK(ensure_pin(c->pin, a13));
+    size_t a14 = dev_in ? al64(a13 + sizeof(int) * ntot) : a13;  // tear flags of the device input
+    CK(ensure(c->probs, a14));
+    std::vector<long> tkey;
+    if (dev_in)
+        tkey = {(long)ntot, (long)nl, (long)nmtot, (long)np, (long)ldz[0], (long)zglobal, c->prm.smlsiz,
+                (long)(uintptr_t)c->q.p, (long)(uintptr_t)c->probs.p};
+    const bool cached = dev_in && tkey == c->tabkey;
+    CK(ensure_pin(c->pin, a14));
     char *h = (char *)c->pin.p;
-    memcpy(h + a0, hp.data(), sizeof(Prob) * np);
-    memcpy(h + a1, pl.leaves.data(), sizeof(Leaf) * nl);
     size_t mo = 0;
     std::vector<size_t> lvoff;
     for (auto &L : pl.lv) {
         lvoff.push_back(mo);
-        memcpy(h + a2 + sizeof(Merge) * mo, L.data(), sizeof(Merge) * L.size());
+        if (!cached) memcpy(h + a2 + sizeof(Merge) * mo, L.data(), sizeof(Merge) * L.size());
         mo += L.size();
     }
-    memcpy(h + a3, hd, sizeof(double) * ntot);
-    memcpy(h + a4, he, sizeof(double) * ntot);
-    std::vector<double> scl(np);
-    std::vector<int> colprob(ntot);
-    for (size_t i = 0; i < np; i++) {
-        scl[i] = pl.probs[i].scale;
-        for (int j = 0; j < pl.probs[i].n; j++) colprob[pl.probs[i].off + j] = (int)i;
+    if (!cached) {
+        memcpy(h + a0, hp.data(), sizeof(Prob) * np);
+        memcpy(h + a1, pl.leaves.data(), sizeof(Leaf) * nl);
+        if (!dev_in) {
+            memcpy(h + a3, hd, sizeof(double) * ntot);
+            memcpy(h + a4, he, sizeof(double) * ntot);
+        } else {
+            std::vector<int> tf(ntot, 0);
+            for (const Leaf &Lf : pl.leaves) {
+                if (Lf.s > 0) tf[Lf.s] |= 1;
+                if (Lf.s + Lf.m < ntot) tf[Lf.s + Lf.m - 1] |= 2;
+            }
+            memcpy(h + a13, tf.data(), sizeof(int) * ntot);
+        }
+        std::vector<double> scl(np);
+        std::vector<int> colprob(ntot);
+        for (size_t i = 0; i < np; i++) {
+            scl[i] = pl.probs[i].scale;
+            for (int j = 0; j < pl.probs[i].n; j++) colprob[pl.probs[i].off + j] = (int)i;
+        }
+        memcpy(h + a5, scl.data(), sizeof(double) * np);
+        memcpy(h + a6, colprob.data(), sizeof(int) * ntot);
+        std::vector<long> zo(np, 0), lz(np, 1);
+        for (size_t i = 0; i < np && i < zoff.size(); i++) zo[i] = zoff[i];
+        for (size_t i = 0; i < np && i < ldz.size(); i++) lz[i] = ldz[i];
+        memcpy(h + a7, zo.data(), sizeof(long) * np);
+        memcpy(h + a8, lz.data(), sizeof(long) * np);
+        if (outcol) memcpy(h + a9, outcol->data(), sizeof(int) * ntot);
+        memset(h + a10, 0, 64 + 512);
+        if (!chunks.empty()) memcpy(h + a11, chunks.data(), sizeof(int) * chunks.size());
+        memcpy(h + a12, fbs.data(), sizeof(int) * (np + 1));
     }
-    memcpy(h + a5, scl.data(), sizeof(double) * np);
-    memcpy(h + a6, colprob.data(), sizeof(int) * ntot);
-    std::vector<long> zo(np, 0), lz(np, 1);
-    for (size_t i = 0; i < np && i < zoff.size(); i++) zo[i] = zoff[i];
-    for (size_t i = 0; i < np && i < ldz.size(); i++) lz[i] = ldz[i];
-    memcpy(h + a7, zo.data(), sizeof(long) * np);
-    memcpy(h + a8, lz.data(), sizeof(long) * np);
-    if (outcol) memcpy(h + a9, outcol->data(), sizeof(int) * ntot);
-    memset(h + a10, 0, 64 + 512);
-    if (!chunks.empty()) memcpy(h + a11, chunks.data(), sizeof(int) * chunks.size());
-    memcpy(h + a12, fbs.data(), sizeof(int) * (np + 1));
     char *dp = (char *)c->probs.p;
-    HK(hipMemcpyAsync(dp, h, a13, hipMemcpyHostToDevice, s));
+    if (!cached) HK(hipMemcpyAsync(dp, h, a14, hipMemcpyHostToDevice, s));
+    c->tabkey = tkey;  // empty (nothing kept) for host input
     const int *dfbs = (const int *)(dp + a12);
+    const int *dtflag = (const int *)(dp + a13);
     const int *dchunks = (const int *)(dp + a11);
     Prob *dprobs = (Prob *)(dp + a0);
     Leaf *dleaves = (Leaf *)(dp + a1);
@@ -1166,15 +1208,22 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     for (auto &Lf : pl.leaves) lmax = std::max(lmax, Lf.m);
     const int zrows = zglobal ? ntot : 0;
     unsigned long long *drows = (unsigned long long *)(dp + a10 + 64);  // 64 counters
-    hipEvent_t f0 = nullptr, f1 = nullptr;
-    if (c->prm.profile) {
-        HK(hipEventCreate(&f0));
-        HK(hipEventCreate(&f1));
+    // events around the final kernel (profile = 1), kept by the context
+    if (c->prm.profile && !c->pf0) {
+        HK(hipEventCreate(&c->pf0));
+        HK(hipEventCreate(&c->pf1));
     }
+    hipEvent_t f0 = c->pf0, f1 = c->pf1;
     // first launches: records and gate cleared, leaves
     auto launch_head = [&]() -> int {
         long nmx = 4 * ((long)c->wcap / 2 + 2) + 8;  // per-merge records, gate, level statistics
         hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(nmx, WGS)), dim3(WGS), 0, s, c->W.maux, nmx, 0ULL);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(WGS), 0, s, (unsigned long long *)(dp + a10), 72L, 0ULL);
+        if (dev_in) {
+            hipLaunchKernelGGL(k_inp_scan, dim3(cdiv(ntot, WGS)), dim3(WGS), 0, s, din_d, din_e, ntot, c->W);
+            hipLaunchKernelGGL(k_inp_scale, dim3(cdiv(ntot, WGS)), dim3(WGS), 0, s, din_d, din_e, ntot, dtflag,
+                               dmerges, (int)nmtot, dd, de, dscale, c->W);
+        }
         hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(64), sizeof(double) * lmax * lmax, s, dleaves, (int)nl, dprobs, dd,
                            de, c->W, 0);
         HK(hipGetLastError());
@@ -1186,6 +1235,9 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
                            dldz, zrows, d_out_dev, dscale, drows, dfbs, (int)np, fin_cz);
         HK(hipGetLastError());
         if (c->prm.profile) HK(hipEventRecord(f1, s));
+        hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, s, c->W, (const int *)dinfo,
+                           (const unsigned long long *)drows, c->dsec, (const unsigned int *)c->dlvlmax, c->drec);
+        HK(hipGetLastError());
         return 0;
     };
     auto launch_dev = [&](int L) -> int {
@@ -1200,12 +1252,13 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     if (all_dev && use_graph && s != nullptr) {
         std::vector<long> key = {(long)nlv, (long)np, (long)nl, (long)lmax, (long)ntot, (long)c->wcap, zrows,
                                  fin_blocks, (long)(outcol != nullptr), c->prm.maxit, c->prm.hss_min_k,
-                                 (long)c->sec_pre, (long)c->prm.learned_min_k,
+                                 (long)c->sec_pre, (long)c->prm.learned_min_k, (long)c->prm.profile,
                                  (long)(uintptr_t)c->stream};
         for (const void *ptr : {(const void *)c->wsd.p, (const void *)c->wsi.p, (const void *)c->maux.p,
                                 (const void *)c->probs.p, (const void *)c->mo.p, (const void *)c->ltab.p,
                                 (const void *)c->sb.p, (const void *)c->ct.p, (const void *)c->dsec,
-                                (const void *)c->dlvlmax, (const void *)dZ, (const void *)d_out_dev})
+                                (const void *)c->dlvlmax, (const void *)dZ, (const void *)d_out_dev,
+                                (const void *)din_d, (const void *)din_e})
             key.push_back((long)(uintptr_t)ptr);
         for (int L = 0; L < nlv; L++) {
             int nmax = 0;
@@ -1246,8 +1299,14 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     }
     if (!queued) {
         CK(launch_head());
-        // the host tables above live in c->pin, which run_merges reuses: wait for the upload
-        if (!all_dev) HK(hipStreamSynchronize(s));
+        // the host tables above live in c->pin, which run_merges reuses: wait for the
+        // upload (and, for device input, see whether the host has to take over)
+        if (!all_dev) {
+            int g = 0;
+            HK(hipMemcpyAsync(&g, c->W.gate, sizeof(int), hipMemcpyDeviceToHost, s));
+            HK(hipStreamSynchronize(s));
+            if (g < 0) return SOLVE_ON_HOST;
+        }
     }
     // Runs of device-scheduled levels are queued back to back; the host looks at the
     // gate once per run (normally once per solve, together with the final kernel).
@@ -1265,8 +1324,15 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         const bool fin = L1 == nlv;
         if (L1 > L || fin) {
             int g = 0;
-            HK(hipMemcpyAsync(&g, c->W.gate, sizeof(int), hipMemcpyDeviceToHost, s));
-            HK(hipStreamSynchronize(s));
+            if (fin) {  // the end-of-solve record carries the gate
+                HK(hipEventRecord(e1, s));
+                HK(hipStreamSynchronize(s));
+                g = c->rec->gate;
+            } else {
+                HK(hipMemcpyAsync(&g, c->W.gate, sizeof(int), hipMemcpyDeviceToHost, s));
+                HK(hipStreamSynchronize(s));
+            }
+            if (g < 0) return SOLVE_ON_HOST;  // the device input needs the host's split
             if (g) {  // level g - 1 holds HSS merges; the levels above it did nothing
                 const int Lh = g - 1, nm = (int)pl.lv[Lh].size();
                 for (int l = L; l <= Lh; l++) ndev += (int)pl.lv[l].size();
@@ -1298,19 +1364,17 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     }
     c->st.levels += nlv;
     c->st.merges += ndev;
-    HK(hipEventRecord(e1, s));
-    int hinfo = 0;
-    unsigned long long rowc[64], rows = 0;
-    HK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
-    HK(hipMemcpyAsync(rowc, drows, sizeof(rowc), hipMemcpyDeviceToHost, s));
-    unsigned long long lst[5];
-    HK(hipMemcpyAsync(lst, c->W.lstat, sizeof(lst), hipMemcpyDeviceToHost, s));
-    HK(hipStreamSynchronize(s));
-    for (int q = 0; q < 64; q++) rows += rowc[q];
-    c->st.roots += (long long)lst[0];
-    c->st.deflated += (long long)lst[1];
-    c->st.hss_merges += (int)lst[2];
-    if (nlv > 0 && dev[nlv - 1]) c->st.k_top = (int)lst[3];  // K of the top merge
+    const SolveRec R = *c->rec;
+    const int hinfo = R.info;
+    const unsigned long long rows = R.rows;
+    c->st.roots += (long long)R.roots;
+    c->st.deflated += (long long)R.deflated;
+    c->st.hss_merges += (int)R.hss;
+    if (nlv > 0 && dev[nlv - 1]) c->st.k_top = (int)R.ktop;  // K of the top merge
+    c->st.secular_crit_iters = (long long)R.crit;
+    c->st.secular_iters = (long long)R.iters;
+    c->st.learned_roots = (long long)R.used;
+    c->st.secular_wave_iters = (long long)R.wave_iters;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     c->st.ms_total = ms;
@@ -1319,14 +1383,9 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     c->st.final_bytes = 8LL * (zrows_written + (long long)rows) + 16LL * ntot;
     if (c->prm.profile) {
         float fm = 0;
-        (void)hipEventElapsedTime(&fm, f0, f1);
+        if (hipEventElapsedTime(&fm, f0, f1) != hipSuccess) { (void)hipGetLastError(); fm = 0; }
         c->st.ms_final = fm;
-        (void)hipEventDestroy(f0);
-        (void)hipEventDestroy(f1);
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    CK(sec_collect(c));
     if (hinfo & 2) return HSDC_ERR_INTERNAL;  // device-scheduled level over its bounds
     return hinfo ? 1 : 0;
 }
@@ -1403,6 +1462,11 @@ int hsdc_ctx_destroy(hsdc_ctx *c)
                       &c->maux, &c->featb, &c->ltab};
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
+    if (c->pf0) (void)hipEventDestroy(c->pf0);
+    if (c->pf1) (void)hipEventDestroy(c->pf1);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->rec) (void)hipHostFree(c->rec);
     for (DevBuf *b : bufs) if (b->p) hipFree(b->p);
     if (c->pin.p) hipHostFree(c->pin.p);
     if (c->pin2.p) hipHostFree(c->pin2.p);
@@ -1444,6 +1508,26 @@ int hsdc_stedc_dev(hsdc_ctx *c, int n, double *d_d, double *d_e, double *d_z, in
     if (ldz < (n > 1 ? n : 1)) return -6;
     memset(&c->st, 0, sizeof(c->st));
     if (n == 0) return 0;
+    if (n > c->prm.smlsiz && c->prm.batched && !c->prm.level_sync) {
+        // The input stays on the device: the plan comes from the order alone (one
+        // problem), the device scales and tears the input and screens it for splits;
+        // an input with a split, or zero, or not finite, is solved below instead.
+        if (c->fplan_n != n || c->fplan_sml != c->prm.smlsiz) {
+            c->fplan = Plan();
+            HostProb P;
+            P.off = 0; P.n = n; P.scale = 1.0; P.levels = 0;
+            c->fplan.probs.push_back(P);
+            c->fplan.ntot = n;
+            std::vector<double> zd(n, 0.0), ze(n, 0.0);
+            plan_problem(c->fplan, 0, zd.data(), ze.data(), c->prm.smlsiz);  // values filled on the device
+            c->fplan_n = n;
+            c->fplan_sml = c->prm.smlsiz;
+        }
+        std::vector<long> zoff(1, 0), lz(1, ldz);
+        int info = solve(c, c->fplan, nullptr, nullptr, d_z, zoff, lz, 1, d_d, nullptr, d_d, d_e);
+        if (info != SOLVE_ON_HOST) return info;
+        memset(&c->st, 0, sizeof(c->st));
+    }
     std::vector<double> hd(n), he(n, 0.0);
     HK(hipMemcpyAsync(hd.data(), d_d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     if (n > 1) HK(hipMemcpyAsync(he.data(), d_e, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, c->stream));
@@ -1611,6 +1695,7 @@ static int single_merge_setup(hsdc_ctx *c, int n, int n1, double rho, Merge **dm
     CK(alloc_work(c, std::max(n, 1)));
     gate_reset(c);
     CK(ensure(c->probs, 128));
+    c->tabkey.clear();  // the kept tables of a device-input solve are overwritten
     Merge M;
     M.s = 0; M.n = n; M.n1 = n1; M.prob = 0; M.lvl = 0; M.rho = rho;
     int ch[2] = {0, cdiv(n, WGS)};

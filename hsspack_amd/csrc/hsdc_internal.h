@@ -94,9 +94,17 @@ struct LvlTab {
     long *soff, *coff;                          // nm entries each
 };
 
+// End-of-solve figures, written by k_readback into mapped host memory: the gate,
+// INFO, rows read by the final copy, level statistics and secular counters.
+struct SolveRec {
+    int gate, info;
+    unsigned long long rows, roots, deflated, hss, ktop, iters, used, wave_iters, crit;
+};
+
 // control block of the secular kernel: learned-start weights, statistics and
 // optional harvesting of training samples
-#define SEC_SLOTS 64  // copies of each statistics counter (summed by the host)
+#define SEC_SLOTS 64    // copies of each statistics counter (summed at the end)
+#define SEC_LEVELS 128  // DC levels with their own largest-iteration counters
 struct SecCtl {
     const float *w;              // MLP weights or nullptr
     unsigned long long *iters;   // total iterations [SEC_SLOTS]

@@ -8,6 +8,8 @@
 // next level (roots, Loewner z) is written operation for operation like the CPU
 // oracle and compiled with -ffp-contract=off, so a merge given identical inputs
 // returns bit-identical roots, index sets and rotations.
+#include <climits>
+#include <cmath>
 #include "hsdc_internal.h"
 #include "hsdc_mlp.h"
 
@@ -48,6 +50,67 @@ __device__ __forceinline__ int find_seg(const int *start, int n, int b)
 }
 
 // ---------------------------------------------------------------------------
+// Device-resident input of MDSTEDC (mdstedc.f90:305-358) for a plan built without
+// looking at the values (one problem, no split).  k_inp_scan: ORGNRM = max|d|,|e|
+// (atomicMax on the bit patterns of non-negative doubles into maux tail word 6)
+// and a screen for splits: |e_i| <= 2 eps sqrt|d_i| sqrt|d_i+1| or a NaN marks the
+// input (tail word 7), twice MDSTEDC's own bound so that no rounding of the square
+// roots lets a split through.  k_inp_scale: the scaling by 1/ORGNRM, the rank-one
+// tears at the leaf boundaries (tflag bit 0: first row of a leaf, bit 1: last) in
+// the order the host applies them, RHO of every merge and the scale-back factor;
+// a marked input, ORGNRM = 0 or a non-finite ORGNRM sets the gate to INT_MIN
+// instead, so every kernel of the solve returns and the host solves it itself.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_inp_scan(const double *d, const double *e, int n, Work W)
+{
+    const int i = blockIdx.x * WG + threadIdx.x;
+    double m = 0.0;
+    int sus = 0;
+    if (i < n) {
+        m = fabs(d[i]);
+        if (i < n - 1) {
+            const double ae = fabs(e[i]);
+            m = fmax(m, ae);
+            const double tiny = HSDC_EPS * sqrt(fabs(d[i])) * sqrt(fabs(d[i + 1]));
+            sus = !(ae > 2.0 * tiny);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmax(m, __shfl_xor(m, off));
+        sus |= __shfl_xor(sus, off);
+    }
+    unsigned long long *tail = (unsigned long long *)W.gate;
+    if ((threadIdx.x & 63) == 0) {
+        if (m > 0.0) atomicMax(&tail[6], (unsigned long long)__double_as_longlong(m));
+        if (sus) atomicOr(&tail[7], 1ULL);
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_inp_scale(const double *d, const double *e, int n, const int *tflag,
+                                                  Merge *mg, int nmtot, double *dd, double *de, double *scale,
+                                                  Work W)
+{
+    const unsigned long long *tail = (const unsigned long long *)W.gate;
+    const double onrm = __longlong_as_double((long long)tail[6]);
+    if (tail[7] || !(onrm > 0.0) || !(onrm < INFINITY)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *W.gate = INT_MIN;
+        return;
+    }
+    const double mul = 1.0 / onrm;
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i < n) {
+        double v = d[i] * mul;
+        const int f = tflag[i];
+        if (f & 1) v -= fabs(e[i - 1] * mul);
+        if (f & 2) v -= fabs(e[i] * mul);
+        dd[i] = v;
+        de[i] = (i < n - 1) ? e[i] * mul : 0.0;
+    }
+    if (i < nmtot) mg[i].rho = e[mg[i].s + mg[i].n1 - 1] * mul;
+    if (i == 0) scale[0] = onrm;
+}
+
+// ---------------------------------------------------------------------------
 // Leaves: implicit-shift QL (the tql2 iteration) on each leaf of the DC tree
 // with eigenvectors, then selection sort (DSTEQR at mdlaed0.f90:263).  One wave
 // per leaf; the scalar recurrence is evaluated redundantly by every lane, lane r
@@ -56,6 +119,7 @@ __device__ __forceinline__ int find_seg(const int *start, int n, int b)
 __global__ __launch_bounds__(64) void k_leaf(const Leaf *leaves, int nleaf, const Prob *probs,
                                              const double *din, const double *ein, Work W, int cur)
 {
+    if (*W.gate) return;  // the device-resident input was sent back to the host
     // one wavefront per leaf: lane r owns row r of the eigenvector block; every lane
     // runs the same scalar QL iteration on its own copy of d and e (identical values),
     // so after the initial fill no lane reads another lane's data and the iteration
@@ -1795,6 +1859,36 @@ __global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_
     } else {
         for (int r = c0 + threadIdx.x; r < c1; r += WG)
             __builtin_nontemporal_store((r >= lo && r < hi) ? src[r] : 0.0, dst + r);
+    }
+}
+
+// The figures the host reads at the end of a solve, gathered by one wavefront into
+// one record (mapped host memory): counters kept in SEC_SLOTS = 64 copies are summed
+// (level maxima maximised over their copies, then summed over the levels).
+__global__ __launch_bounds__(64) void k_readback(Work W, const int *info, const unsigned long long *rows,
+                                                 const SecCtl *sec, const unsigned int *lvlmax, SolveRec *out)
+{
+    const int lane = threadIdx.x;
+    const SecCtl C = *sec;
+    unsigned long long r = rows[lane], it = C.iters[lane], us = C.used[lane], wi = C.wave_iters[lane];
+    unsigned long long crit = 0;
+    for (int l = 0; l < SEC_LEVELS; l++) {
+        unsigned int m = lvlmax[l * SEC_SLOTS + lane];
+        for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+        crit += m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        r += __shfl_xor(r, off);
+        it += __shfl_xor(it, off);
+        us += __shfl_xor(us, off);
+        wi += __shfl_xor(wi, off);
+    }
+    if (lane == 0) {
+        out->gate = *W.gate;
+        out->info = *info;
+        out->rows = r;
+        out->roots = W.lstat[0]; out->deflated = W.lstat[1]; out->hss = W.lstat[2]; out->ktop = W.lstat[3];
+        out->iters = it; out->used = us; out->wave_iters = wi; out->crit = crit;
     }
 }
 

@@ -44,7 +44,8 @@ typedef struct {
                         launches on the device (one host round trip per solve, plus one per level
                         holding HSS merges); 1: the host reads every level's K before its launches */
     int graph;       /* 1 (default): a solve whose levels are all queued at once is captured as one
-                        HIP graph per shape (plan, buffers, parameters) and replayed; 0: direct launches */
+                        HIP graph per shape (plan, buffers, parameters) and replayed; 0, or profile = 1
+                        (events around the final kernel): direct launches */
 } hsdc_params;
 
 /* Run statistics of the last call on a context. */

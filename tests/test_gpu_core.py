@@ -355,3 +355,46 @@ def test_graph_replays_match_direct_launches(torch_dev, ctx, cfg, n, hss_min_k, 
     assert np.array_equal(lam, eager[1][0]) and np.array_equal(z, eager[1][1])
     assert np.array_equal(lam2, eager[0][0]) and np.array_equal(z2, eager[0][1])
     ctx.set_params(hss_min_k=1 << 30, hss_leaf=200, graph=1)
+
+
+
+def test_device_input_path_matches_host_input_path(torch_dev, ctx):
+    """hsdc_stedc_dev scans, scales and tears a device-resident input on the device
+    (plan from the order alone, tables kept between calls); with the per-level host
+    round trip (level_sync = 1) the host prepares the input instead.  Both give the
+    same bits, also after other entry points have reused the context's tables."""
+    for cfg, n, seed in [(2, 3000, 31), (1, 2500, 32), (2, 3000, 33)]:
+        d, e = tridiag_cfg(cfg, n, seed)
+        rc1, lam_h, z_h = solve_dev(torch_dev, ctx, d, e, hss_min_k=1 << 30, batched=1, level_sync=1)
+        rc0, lam_d, z_d = solve_dev(torch_dev, ctx, d, e, level_sync=0)
+        assert rc0 == 0 and rc1 == 0
+        assert np.array_equal(lam_h, lam_d)
+        assert np.array_equal(z_h, z_d)
+        ctx.secular(np.array([0.1, 0.5, 0.9]), np.array([0.6, 0.6, 0.52915]), 0.3)  # reuses the tables
+
+
+@pytest.mark.parametrize("case", ["zero", "split", "near_split", "scaled_zero_e"])
+def test_device_input_handed_to_host(torch_dev, ctx, case):
+    """Inputs the device screen sends back to the host: zero, an exact split, an
+    off-diagonal within twice MDSTEDC's split bound, zero off-diagonals."""
+    n = 400
+    d, e = tridiag_cfg(2, n, 44)
+    d, e = d.copy(), e.copy()
+    if case == "zero":
+        d[:] = 0.0
+        e[:] = 0.0
+    elif case == "split":
+        e[123] = 0.0
+    elif case == "near_split":
+        e[200] = 1.5 * EPS * np.sqrt(abs(d[200])) * np.sqrt(abs(d[201]))
+    else:
+        e[::7] = 0.0
+    rc1, lam_h, z_h = solve_dev(torch_dev, ctx, d, e, hss_min_k=1 << 30, batched=1, level_sync=1)
+    rc0, lam, z = solve_dev(torch_dev, ctx, d, e, level_sync=0)
+    assert rc0 == 0 and rc1 == 0
+    assert np.array_equal(lam, lam_h)
+    assert np.array_equal(z, z_h)
+    if case == "zero":
+        assert np.array_equal(z, np.eye(n)) and not lam.any()
+    else:
+        check_eig(d, e, lam, z)

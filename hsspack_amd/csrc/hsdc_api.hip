@@ -21,7 +21,7 @@
 
 // kernels (hsdc_kern.hip)
 struct GTile { int m, half, r0, c0; };
-__global__ void k_leaf(const Leaf *, int, const Prob *, const double *, const double *, Work, int);
+__global__ void k_leaf(const Leaf *, int, const Prob *, const double *, const double *, Work, int, int, int);
 __global__ void k_prep(const Merge *, const int *, int, const Prob *, Work, int);
 __global__ void k_mergepath(const Merge *, const int *, int, Work);
 __global__ void k_fill_u64(unsigned long long *, long, unsigned long long);
@@ -1224,8 +1224,10 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
             hipLaunchKernelGGL(k_inp_scale, dim3(cdiv(ntot, WGS)), dim3(WGS), 0, s, din_d, din_e, ntot, dtflag,
                                dmerges, (int)nmtot, dd, de, dscale, c->W);
         }
-        hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(64), sizeof(double) * lmax * lmax, s, dleaves, (int)nl, dprobs, dd,
-                           de, c->W, 0);
+        // 64 / G leaves per wavefront, G the smallest of 16, 32, 64 that holds the largest leaf
+        const int G = lmax <= 16 ? 16 : (lmax <= 32 ? 32 : 64);
+        hipLaunchKernelGGL(k_leaf, dim3(cdiv((long)nl, 64 / G)), dim3(64), sizeof(double) * (64 / G) * lmax * lmax,
+                           s, dleaves, (int)nl, dprobs, dd, de, c->W, 0, G, lmax);
         HK(hipGetLastError());
         return 0;
     };

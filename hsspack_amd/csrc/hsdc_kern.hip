@@ -117,22 +117,29 @@ __global__ __launch_bounds__(WG) void k_inp_scale(const double *d, const double 
 // owns row r of the leaf's eigenvector block in LDS.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_leaf(const Leaf *leaves, int nleaf, const Prob *probs,
-                                             const double *din, const double *ein, Work W, int cur)
+                                             const double *din, const double *ein, Work W, int cur, int G,
+                                             int lcap)
 {
     if (*W.gate) return;  // the device-resident input was sent back to the host
-    // one wavefront per leaf: lane r owns row r of the eigenvector block; every lane
-    // runs the same scalar QL iteration on its own copy of d and e (identical values),
-    // so after the initial fill no lane reads another lane's data and the iteration
-    // needs no barriers
+    // 64 / G leaves per wavefront, G lanes (G >= leaf order) per leaf: lane r of a
+    // group owns row r of its leaf's eigenvector block; every lane of the group runs
+    // the same scalar QL iteration on the group's d and e (identical values written
+    // by every lane), so after the initial fill no lane reads another lane's row and
+    // the iteration needs no barriers.  Groups iterate independently (masked).
     extern __shared__ double lds_leaf[];
-    __shared__ double sd[64], se[64];
-    int li = blockIdx.x;
-    if (li >= nleaf) return;
-    Leaf L = leaves[li];
-    Prob P = probs[L.prob];
-    const int n = L.m, r = threadIdx.x;
-    double *sz = lds_leaf;  // n x n, column-major
-    for (int i = r; i < n * n; i += 64) sz[i] = ((i % n) == (i / n)) ? 1.0 : 0.0;
+    __shared__ double sdl[64], sel[64];
+    const int g = threadIdx.x / G, r = threadIdx.x - g * G, li = blockIdx.x * (64 / G) + g;
+    Leaf L = {0, 0, 0};
+    Prob P = {nullptr, 0, 0, 0, 0};
+    int n = 0;
+    if (li < nleaf) {
+        L = leaves[li];
+        P = probs[L.prob];
+        n = L.m;
+    }
+    double *sz = lds_leaf + (long)g * lcap * lcap;  // n x n, column-major
+    double *sd = sdl + g * G, *se = sel + g * G;
+    for (int i = r; i < n * n; i += G) sz[i] = ((i % n) == (i / n)) ? 1.0 : 0.0;
     if (r < n) {
         sd[r] = din[L.s + r];
         se[r] = (r < n - 1) ? ein[L.s + r] : 0.0;

@@ -55,7 +55,9 @@ __global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, 
 __global__ void k_hss_init(const HssMerge *);
 __global__ void k_hss_reset(const HssMerge *);
 __global__ void k_hss_dgen(const HssMerge *, const HssNode *, int);
+template <int NT>
 __global__ void k_hss_rowcomp(const HssMerge *, const HssNode *, int, double);
+template <int NT>
 __global__ void k_hss_colcomp(const HssMerge *, const HssNode *, int, double);
 __global__ void k_hss_compact(const HssMerge *, const HssCompact *, const int *, int);
 __global__ void k_hss_bgen(const HssMerge *, const HssNode *, int);
@@ -531,10 +533,16 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
             const HssNode *sn = dN + step_off[st];
             const HssCompact *sc = dC + comp_off[st];
             hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
-            hipLaunchKernelGGL(k_hss_rowcomp, dim3(nn), dim3(WGS), 0, s, dH, sn, nn, tol);
+            if (nn < 512)
+                hipLaunchKernelGGL(k_hss_rowcomp<512>, dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+            else
+                hipLaunchKernelGGL(k_hss_rowcomp<256>, dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
             hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 1);
             hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
-            hipLaunchKernelGGL(k_hss_colcomp, dim3(nn), dim3(WGS), 0, s, dH, sn, nn, tol);
+            if (nn < 512)
+                hipLaunchKernelGGL(k_hss_colcomp<512>, dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+            else
+                hipLaunchKernelGGL(k_hss_colcomp<256>, dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
             hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 0);
             hipLaunchKernelGGL(k_hss_bgen, dim3(nn), dim3(WGS), 0, s, dH, sn, nn);
         }

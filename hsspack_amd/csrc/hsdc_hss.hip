@@ -55,7 +55,7 @@ __device__ VI block_argmax(VI x, VI *sh)
     if ((tid & 63) == 0) sh[tid >> 6] = x;
     __syncthreads();
     VI r = sh[0];
-    for (int w = 1; w < WG / 64; w++) r = vi_better(r, sh[w]);
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = vi_better(r, sh[w]);
     __syncthreads();
     return r;
 }
@@ -99,12 +99,13 @@ __device__ long scratch_take(const HssMerge &H, long n)
 // rows: the node's M row generators (LDS); columns: the N column generators
 // outside the node (private global copies).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const HssNode *nodes, int nn,
+template <int HSS_CWG>
+__global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, const HssNode *nodes, int nn,
                                                     double tol)
 {
     __shared__ double F[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
     __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
-    __shared__ VI red[WG / 64];
+    __shared__ VI red[HSS_CWG / 64];
     __shared__ long s_off;
     __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
     __shared__ double s_nmax;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
     // per column the pole/difference values its accurate differences read
     // (FF(pw), FF(pw+1), DIFL(pw), DIFR(pw) with pw = PU(j)), gathered once
     double *CF0 = V + 2 * NS, *CF1 = CF0 + NS, *CL = CF1 + NS, *CR = CL + NS;
-    for (int j = tid; j < N; j += WG) {
+    for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < c0 - 1) ? j + 1 : j + 1 + (c1e - c0 + 1);
         D[j] = A1(H.D, src); V[j] = A1(H.V, src);
         const int pw = A1(H.PU, src);
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
         t = A1(CL, a); A1(CL, a) = A1(CL, c); A1(CL, c) = t;
         t = A1(CR, a); A1(CR, a) = A1(CR, c); A1(CR, c) = t;
     };
-    for (int i = tid; i < M; i += WG) {
+    for (int i = tid; i < M; i += HSS_CWG) {
         PL[i] = A1(H.PL, l0 + i);
         A[i] = A1(H.U, l0 + i);
         F[i] = A1(H.FF, PL[i]);
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
         {
             VI x = {0.0, 0};
             int pwk = A1(PU, k);
-            for (int i = k + tid; i <= M; i += WG) {
+            for (int i = k + tid; i <= M; i += HSS_CWG) {
                 int pli = A1(PL, i);
                 double v = fabs(A1(Z, i) / fd(H, pli, pwk));
                 x = vi_better(x, VI{v, i - k + 1});
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
             VI y = {0.0, 0};
             int fk = A1(PL, k);
             #pragma unroll 8
-            for (int j = k + tid; j <= N; j += WG) {
+            for (int j = k + tid; j <= N; j += HSS_CWG) {
                 double v = fabs(A1(V, j) / fdj(fk, j));
                 y = vi_better(y, VI{v, j - k + 1});
             }
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
                         VI y2 = {0.0, 0};
                         int fk2 = A1(PL, k);
                         #pragma unroll 8
-                        for (int j = k + tid; j <= N; j += WG) {
+                        for (int j = k + tid; j <= N; j += HSS_CWG) {
                             double v = fabs(A1(V, j) / fdj(fk2, j));
                             y2 = vi_better(y2, VI{v, j - k + 1});
                         }
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
                     __syncthreads();
                     VI x2 = {0.0, 0};
                     int pwk2 = A1(PU, k);
-                    for (int i = k + tid; i <= M; i += WG) {
+                    for (int i = k + tid; i <= M; i += HSS_CWG) {
                         double v = fabs(A1(Z, i) / fd(H, A1(PL, i), pwk2));
                         x2 = vi_better(x2, VI{v, i - k + 1});
                     }
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
             // ---- CauchyPivtEig_CP: complete pivoting over the Schur complement
             VI best = {0.0, 0};
             VI bestrow = {0.0, 0};
-            for (int j = k + tid; j <= N; j += WG) {
+            for (int j = k + tid; j <= N; j += HSS_CWG) {
                 double mv = -DBL_MAX;
                 int mi = 0;
                 for (int i = k; i <= M; i++) {
@@ -283,19 +284,19 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
         // ---- generator updates of the (k+1)-th Schur complement
         const double Fk = A1(F, k);
         const int plk = A1(PL, k), puk = A1(PU, k);
-        for (int j = k + 1 + tid; j <= M; j += WG) {
+        for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
             double temp = fd(H, A1(PL, j), puk);
             temp = (A1(F, j) - Fk) / temp;
             A1(Z, j) = A1(Z, j) * temp;
         }
         #pragma unroll 8
-        for (int j = k + 1 + tid; j <= N; j += WG) {
+        for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
             double temp = ddj(j, puk);
             double temp2 = -1.0 * fdj(plk, j);
             temp = temp / temp2;
             A1(V, j) = A1(V, j) * temp;
         }
-        for (int j = 1 + tid; j <= k - 1; j += WG) {
+        for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
             double temp = -1.0 * fd(H, A1(PL, j), puk);
             A1(W, j) = A1(W, j) * (temp / (Fk - A1(F, j)));
         }
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
                 VI x = {0.0, 0};
-                for (int jc = 1 + tid; jc <= k; jc += WG) {
+                for (int jc = 1 + tid; jc <= k; jc += HSS_CWG) {
                     double fj = A1(F, jc), mv = 0.0;
                     int mi = 0;
                     for (int i = 1; i <= M - k; i++) {
@@ -333,17 +334,17 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
                 const double Fii = A1(F, ii), Fjj = A1(F, jj);
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
                 #pragma unroll 8
-                for (int j = k + 1 + tid; j <= N; j += WG) {
+                for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
                     double temp = -1.0 * fdj(plii, j);
                     double temp2 = -1.0 * fdj(pljj, j);
                     temp = temp / temp2;
                     A1(V, j) = A1(V, j) * temp;
                 }
-                for (int j = k + 1 + tid; j <= M; j += WG) {
+                for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
                     if (j == jj) continue;
                     A1(Z, j) = A1(Z, j) * ((A1(F, j) - Fjj) / (A1(F, j) - Fii));
                 }
-                for (int j = 1 + tid; j <= k; j += WG) {
+                for (int j = 1 + tid; j <= k; j += HSS_CWG) {
                     if (j == ii) continue;
                     A1(W, j) = A1(W, j) * ((A1(F, j) - Fii) / (A1(F, j) - Fjj));
                 }
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
         double *Ug = H.arena + off;
         const int r = s_Rk;
         if (Rk < mn) {
-            for (int e = tid; e < M * r; e += WG) {
+            for (int e = tid; e < M * r; e += HSS_CWG) {
                 int i = e % M + 1, j = e / M + 1;
                 int src = IP[i - 1];  // row src of Q
                 double q;
@@ -400,14 +401,14 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
                 Ug[e] = q;
             }
         } else {
-            for (int e = tid; e < M * M; e += WG) {
+            for (int e = tid; e < M * M; e += HSS_CWG) {
                 int i = e % M + 1, j = e / M;
                 Ug[e] = (IP[i - 1] - 1 == j) ? 1.0 : 0.0;
             }
         }
     }
     // permuted row generators and their indices back to the compacted arrays
-    for (int i = tid; i < M; i += WG) {
+    for (int i = tid; i < M; i += HSS_CWG) {
         A1(H.U, l0 + i) = A[i];
         A1(H.PL, l0 + i) = PL[i];
     }
@@ -418,12 +419,13 @@ __global__ __launch_bounds__(WG) void k_hss_rowcomp(const HssMerge *hm, const Hs
 // (ComprcauchyEig 'c').  RRLU rows: the node's M columns (roots, alpha, LDS);
 // RRLU columns: the N row generators outside the node (private global copies).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const HssNode *nodes, int nn,
+template <int HSS_CWG>
+__global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, const HssNode *nodes, int nn,
                                                     double tol)
 {
     __shared__ double Dr[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
     __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
-    __shared__ VI red[WG / 64];
+    __shared__ VI red[HSS_CWG / 64];
     __shared__ long s_off;
     __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
     const int tid = threadIdx.x;
@@ -446,13 +448,13 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
     const int M = s_M, N = s_N, c0 = lc[2 * (node - 1)], r0 = lr[2 * (node - 1)], r1e = lr[2 * (node - 1) + 1];
     double *F = H.scratch + s_off, *V = F + (N > 0 ? N : 1);
     int *PU = (int *)(V + (N > 0 ? N : 1));
-    for (int j = tid; j < N; j += WG) {
+    for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < r0 - 1) ? j + 1 : j + 1 + (r1e - r0 + 1);
         V[j] = A1(H.U, src);
         PU[j] = A1(H.PL, src);
         F[j] = A1(H.FF, PU[j]);
     }
-    for (int i = tid; i < M; i += WG) {
+    for (int i = tid; i < M; i += HSS_CWG) {
         Dr[i] = A1(H.D, c0 + i);
         A[i] = A1(H.V, c0 + i);
         PL[i] = A1(H.PU, c0 + i);
@@ -470,7 +472,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
         {
             VI x = {0.0, 0};
             int puk = A1(PU, k);
-            for (int i = k + tid; i <= M; i += WG) {
+            for (int i = k + tid; i <= M; i += HSS_CWG) {
                 double v = fabs(A1(Z, i) / fd(H, puk, A1(PL, i)));
                 x = vi_better(x, VI{v, i - k + 1});
             }
@@ -478,7 +480,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
             VI y = {0.0, 0};
             int plk = A1(PL, k);
             #pragma unroll 8
-            for (int j = k + tid; j <= N; j += WG) {
+            for (int j = k + tid; j <= N; j += HSS_CWG) {
                 double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk));
                 y = vi_better(y, VI{v, j - k + 1});
             }
@@ -509,7 +511,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
                         VI y2 = {0.0, 0};
                         int plk2 = A1(PL, k);
                         #pragma unroll 8
-                        for (int j = k + tid; j <= N; j += WG) {
+                        for (int j = k + tid; j <= N; j += HSS_CWG) {
                             double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk2));
                             y2 = vi_better(y2, VI{v, j - k + 1});
                         }
@@ -528,7 +530,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
                     __syncthreads();
                     VI x2 = {0.0, 0};
                     int puk2 = A1(PU, k);
-                    for (int i = k + tid; i <= M; i += WG) {
+                    for (int i = k + tid; i <= M; i += HSS_CWG) {
                         double v = fabs(A1(Z, i) / fd(H, puk2, A1(PL, i)));
                         x2 = vi_better(x2, VI{v, i - k + 1});
                     }
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
             VI best = {0.0, 0};
             int bestrow = 0;
             #pragma unroll 8
-            for (int j = k + tid; j <= N; j += WG) {
+            for (int j = k + tid; j <= N; j += HSS_CWG) {
                 int pu = A1(PU, j);
                 double mv = 0.0;
                 int mi = 0;
@@ -581,18 +583,18 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
         }
         const double Fk = A1(F, k);
         const int plk = A1(PL, k), puk = A1(PU, k);
-        for (int j = k + 1 + tid; j <= M; j += WG) {
+        for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
             double temp = dd(H, A1(PL, j), plk);
             double temp2 = -1.0 * fd(H, puk, A1(PL, j));
             temp = temp / temp2;
             A1(Z, j) = A1(Z, j) * temp;
         }
         #pragma unroll 8
-        for (int j = k + 1 + tid; j <= N; j += WG) {
+        for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
             double temp = fd(H, A1(PU, j), plk);
             A1(V, j) = A1(V, j) * ((A1(F, j) - Fk) / temp);
         }
-        for (int j = 1 + tid; j <= k - 1; j += WG) {
+        for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
             double temp = fd(H, puk, A1(PL, j));
             double temp2 = dd(H, plk, A1(PL, j));
             temp = temp / temp2;
@@ -613,7 +615,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
                 VI x = {0.0, 0};
-                for (int jc = 1 + tid; jc <= k; jc += WG) {
+                for (int jc = 1 + tid; jc <= k; jc += HSS_CWG) {
                     int pljc = A1(PL, jc);
                     double mv = 0.0;
                     int mi = 0;
@@ -633,20 +635,20 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
                 __syncthreads();
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
                 #pragma unroll 8
-                for (int j = k + 1 + tid; j <= N; j += WG) {
+                for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
                     double temp = fd(H, A1(PU, j), plii);
                     double temp2 = fd(H, A1(PU, j), pljj);
                     temp = temp / temp2;
                     A1(V, j) = A1(V, j) * temp;
                 }
-                for (int j = k + 1 + tid; j <= M; j += WG) {
+                for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
                     if (j == jj) continue;
                     double temp = dd(H, A1(PL, j), pljj);
                     double temp2 = dd(H, A1(PL, j), plii);
                     temp = temp / temp2;
                     A1(Z, j) = A1(Z, j) * temp;
                 }
-                for (int j = 1 + tid; j <= k; j += WG) {
+                for (int j = 1 + tid; j <= k; j += HSS_CWG) {
                     if (j == ii) continue;
                     double temp = dd(H, A1(PL, j), plii);
                     double temp2 = dd(H, A1(PL, j), pljj);
@@ -700,7 +702,7 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
         double *Vg = H.arena + off;
         const int r = s_Rk;
         if (Rk < mn) {
-            for (int e = tid; e < r * M; e += WG) {
+            for (int e = tid; e < r * M; e += HSS_CWG) {
                 int i = e % r + 1, j = e / r + 1;
                 int src = IP[j - 1];  // column src of Q
                 double q;
@@ -716,13 +718,13 @@ __global__ __launch_bounds__(WG) void k_hss_colcomp(const HssMerge *hm, const Hs
                 Vg[e] = q;
             }
         } else {
-            for (int e = tid; e < M * M; e += WG) {
+            for (int e = tid; e < M * M; e += HSS_CWG) {
                 int i = e % M, j = e / M + 1;
                 Vg[e] = (IP[j - 1] - 1 == i) ? 1.0 : 0.0;
             }
         }
     }
-    for (int i = tid; i < M; i += WG) {
+    for (int i = tid; i < M; i += HSS_CWG) {
         A1(H.D, c0 + i) = Dr[i];
         A1(H.V, c0 + i) = A[i];
         A1(H.PU, c0 + i) = PL[i];
@@ -958,3 +960,11 @@ __global__ void k_hss_reset(const HssMerge *hm)
 {
     if (threadIdx.x == 0) *hm[blockIdx.x].scratch_top = 0ULL;
 }
+
+// 256 threads per node when a launch has nodes for every CU twice over, 512 when
+// it has fewer (the upper tree levels): the search over the node's columns is the
+// sequential part of every pivot step
+template __global__ void k_hss_rowcomp<256>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_rowcomp<512>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_colcomp<256>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_colcomp<512>(const HssMerge *, const HssNode *, int, double);

@@ -834,76 +834,94 @@ __global__ __launch_bounds__(WG) void k_hss_init(const HssMerge *hm)
 // ---------------------------------------------------------------------------
 // Batched GEMM of the HSS multiply: work item (blockIdx.y) = one BG, block
 // (blockIdx.x, blockIdx.z) = one 128 x 64 tile of its output.  FP64 MFMA 16x16x4.
+// The k range is walked in chunks of 32: the block stages a 128 x 32 slab of A and
+// a 32 x 64 slab of B in LDS (coalesced loads by all 256 threads), and while the
+// four wavefronts multiply out of LDS the next chunk is already loading into
+// registers.  Each wavefront owns 32 x 64 outputs in 8 accumulators; the k-steps
+// of 4 run in order, as the single-product reference order of the tests expects.
 // ---------------------------------------------------------------------------
 typedef double d4h __attribute__((ext_vector_type(4)));
+#define BG_BK 32
+#define BG_AP 132  // LDS pitch of the A slab (128 rows + 4: no bank conflicts between k rows)
+#define BG_BP 68   // LDS pitch of the B slab (64 columns + 4)
 __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
 {
-    // block = 4 wavefronts stacked along the rows (128 x 64 outputs); each wavefront
-    // a 32 x 64 tile, 8 accumulators of 16 x 16.  Operands go straight from global
-    // memory (L2) into the MFMA source registers, one k-step of 4 ahead.
+    __shared__ double As[BG_BK * BG_AP];
+    __shared__ double Bs[BG_BK * BG_BP];
     const BG g = items[blockIdx.y];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int r0 = blockIdx.x * 128 + wv * 32, c0 = blockIdx.z * 64;
-    if (r0 >= g.nrow || c0 >= g.n) return;  // no barriers below: wavefronts are independent
-    const int lr = lane & 15, lk = lane >> 4;
+    const int rb = blockIdx.x * 128, c0 = blockIdx.z * 64;
+    if (rb >= g.nrow || c0 >= g.n) return;  // the whole block
+    const int K1 = g.k, K = g.k + g.k2;
+    const long lda = g.lda, lda2 = g.lda2, ldb = g.ldb, ldb2 = g.ldb2;
+    // staging of one chunk: A element (row rb + (i & 127), k kc + (i >> 7)), i = tid + 256 q;
+    // B element i = tid + 256 q: (k kc + (i & 31), column c0 + (i >> 5)) column-major,
+    // (k kc + (i >> 6), column c0 + (i & 63)) for btrans
+    double ra[16], rbv[8];
+    auto stage = [&](int kc) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int i = tid + 256 * q, row = rb + (i & 127), t = kc + (i >> 7);
+            double v = 0.0;
+            if (row < g.nrow && t < K) v = t < K1 ? g.A[row + t * lda] : g.A2[row + (t - K1) * lda2];
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int i = tid + 256 * q;
+            int t, col;
+            if (g.btrans) { t = kc + (i >> 6); col = c0 + (i & 63); }
+            else { t = kc + (i & 31); col = c0 + (i >> 5); }
+            double v = 0.0;
+            if (col < g.n && t < K) {
+                if (t >= K1) v = g.B2[(long)col * ldb2 + (t - K1)];
+                else v = g.btrans ? g.B[(long)t * ldb + col] : g.B[(long)col * ldb + t];
+            }
+            rbv[q] = v;
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int i = tid + 256 * q;
+            As[(i >> 7) * BG_AP + (i & 127)] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int i = tid + 256 * q;
+            if (g.btrans) Bs[(i >> 6) * BG_BP + (i & 63)] = rbv[q];
+            else Bs[(i & 31) * BG_BP + (i >> 5)] = rbv[q];
+        }
+    };
+    const int lr = lane & 15, lk = lane >> 4, wr = wv * 32;
     d4h acc[2][4];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 4; b++) acc[a][b] = (d4h){0.0, 0.0, 0.0, 0.0};
-    // per-lane operand streams: A(row, t) at pa[t * lda], B(t, col) at pb[t * sb]; the
-    // k range [K1, K1 + k2) reads the second product's operands
-    const bool ok0 = r0 + lr < g.nrow, ok1 = r0 + 16 + lr < g.nrow;
-    const long ro0 = ok0 ? r0 + lr : 0, ro1 = ok1 ? r0 + 16 + lr : 0;
-    const double *pa0 = g.A + ro0, *pa1 = g.A + ro1;
-    const double *qa0 = g.k2 ? g.A2 + ro0 : nullptr, *qa1 = g.k2 ? g.A2 + ro1 : nullptr;
-    const long sb = g.btrans ? g.ldb : 1;
-    const double *pb[4], *qb[4];
-    bool okb[4];
-    for (int q = 0; q < 4; q++) {
-        const int cc = c0 + 16 * q + lr;
-        okb[q] = cc < g.n;
-        pb[q] = g.B + (okb[q] ? (g.btrans ? (long)cc : (long)cc * g.ldb) : 0);
-        qb[q] = g.k2 ? g.B2 + (okb[q] ? (long)cc * g.ldb2 : 0) : nullptr;
-    }
-    const long lda = g.lda, lda2 = g.lda2;
-    const int K1 = g.k, K = g.k + g.k2;
-    auto ldA = [&](const double *p, const double *p2, bool ok, int t) -> double {
-        if (!ok || t >= K) return 0.0;
-        return t < K1 ? p[t * lda] : p2[(t - K1) * lda2];
-    };
-    auto ldB = [&](int q, int t) -> double {
-        if (!okb[q] || t >= K) return 0.0;
-        return t < K1 ? pb[q][t * sb] : qb[q][t - K1];
-    };
-    // two k-steps (8 k) in registers, the next two in flight
-    double a[2][2], b[2][4];
-    for (int h = 0; h < 2; h++) {
-        a[h][0] = ldA(pa0, qa0, ok0, 4 * h + lk);
-        a[h][1] = ldA(pa1, qa1, ok1, 4 * h + lk);
-        for (int q = 0; q < 4; q++) b[h][q] = ldB(q, 4 * h + lk);
-    }
-    for (int k0 = 0; k0 < K; k0 += 8) {
-        double na[2][2], nb[2][4];
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int tn = k0 + 8 + 4 * h + lk;
-            na[h][0] = ldA(pa0, qa0, ok0, tn);
-            na[h][1] = ldA(pa1, qa1, ok1, tn);
-#pragma unroll
-            for (int q = 0; q < 4; q++) nb[h][q] = ldB(q, tn);
+    stage(0);
+    commit();
+    __syncthreads();
+    for (int kc = 0; kc < K; kc += BG_BK) {
+        const bool more = kc + BG_BK < K;
+        if (more) stage(kc + BG_BK);  // in flight while this chunk multiplies
+        const int ksteps = min(BG_BK, ((K - kc) + 3) & ~3);
+        for (int kq = 0; kq < ksteps; kq += 4) {
+            const double *ap = As + (kq + lk) * BG_AP + wr + lr;
+            const double *bp = Bs + (kq + lk) * BG_BP + lr;
+            const double a0 = ap[0], a1 = ap[16];
+            const double b0 = bp[0], b1 = bp[16], b2 = bp[32], b3 = bp[48];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[0][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b2, acc[0][2], 0, 0, 0);
+            acc[0][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b3, acc[0][3], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            acc[1][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b2, acc[1][2], 0, 0, 0);
+            acc[1][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b3, acc[1][3], 0, 0, 0);
         }
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-#pragma unroll
-            for (int x = 0; x < 2; x++)
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    acc[x][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[h][x], b[h][q], acc[x][q], 0, 0, 0);
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            a[h][0] = na[h][0];
-            a[h][1] = na[h][1];
-#pragma unroll
-            for (int q = 0; q < 4; q++) b[h][q] = nb[h][q];
+        __syncthreads();  // every wavefront is done with this chunk
+        if (more) {
+            commit();
+            __syncthreads();
         }
     }
     for (int a = 0; a < 2; a++)
@@ -911,7 +929,7 @@ __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
             const int col = c0 + b * 16 + lr;
             if (col >= g.n) continue;
             for (int i = 0; i < 4; i++) {
-                int row = r0 + a * 16 + lk + 4 * i;
+                int row = rb + wr + a * 16 + lk + 4 * i;
                 if (row < g.nrow) {
                     double *cp = g.C + (long)row + (long)col * g.ldc;
                     *cp = g.beta ? (*cp + acc[a][b][i]) : acc[a][b][i];

@@ -1318,8 +1318,9 @@ __global__ __launch_bounds__(WG) void k_loewner(const Merge *mg, const int *chun
                                                 const MergeOut *mo, Work W)
 {
     if (gate_closed(W, mg)) return;
-    __shared__ double sd[LW_LDS], st[LW_LDS];
-    __shared__ int so[LW_LDS];
+    // chunks of LW_LDS poles: d_j, d_org(j) and tau_j staged by the block, every
+    // thread's product walks them in order
+    __shared__ double sd[LW_LDS], so[LW_LDS], st[LW_LDS];
     const int nvb = chunk_start[nm];
     for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
         const int m = find_seg(chunk_start, nm, vb);
@@ -1328,31 +1329,34 @@ __global__ __launch_bounds__(WG) void k_loewner(const Merge *mg, const int *chun
         const int s = mg[m].s, K = o.k, i = (vb - chunk_start[m]) * WG + threadIdx.x;
         const double *d = W.dlamda + s, *tau = W.tau + s;
         const int *org = W.org + s;
-        if (K <= LW_LDS) {
-            for (int t = threadIdx.x; t < K; t += WG) { sd[t] = d[t]; st[t] = tau[t]; so[t] = org[t]; }
-            __syncthreads();
-            d = sd; tau = st; org = so;
-        }
-        if (i < K) {
-            const double di = d[i];
-            double wv;
-            if (!o.hss) {
-                wv = delta_ij(d, i, org, tau, i);
-                for (int jj = 0; jj < K; jj++) {
-                    if (jj == i) continue;
-                    wv = wv * (delta_ij(d, i, org, tau, jj) / (di - d[jj]));
-                }
-                W.wnew[s + i] = copysign(sqrt(-wv), W.w[s + i]);
-            } else {
-                wv = 1.0;
-                for (int jj = 0; jj < K; jj++) {
-                    if (jj == i) wv = wv * delta_ij(d, i, org, tau, i);
-                    else wv = wv * (delta_ij(d, i, org, tau, jj) / (di - d[jj]));
-                }
-                W.wnew[s + i] = copysign(sqrt(fabs(wv)), W.w[s + i]);
+        const bool own = i < K;
+        const double di = own ? d[i] : 0.0;
+        // delta_i(j) = (d_i - d_org(j)) - tau_j, as delta_ij forms it
+        double wv = (own && !o.hss) ? delta_ij(d, i, org, tau, i) : 1.0;
+        for (int c0 = 0; c0 < K; c0 += LW_LDS) {
+            const int cn = min(LW_LDS, K - c0);
+            for (int t = threadIdx.x; t < cn; t += WG) {
+                sd[t] = d[c0 + t];
+                so[t] = d[org[c0 + t] - 1];
+                st[t] = tau[c0 + t];
             }
+            __syncthreads();
+            if (own) {
+                if (!o.hss) {
+                    for (int jj = 0; jj < cn; jj++) {
+                        if (c0 + jj == i) continue;
+                        wv = wv * (((di - so[jj]) - st[jj]) / (di - sd[jj]));
+                    }
+                } else {
+                    for (int jj = 0; jj < cn; jj++) {
+                        if (c0 + jj == i) wv = wv * ((di - so[jj]) - st[jj]);
+                        else wv = wv * (((di - so[jj]) - st[jj]) / (di - sd[jj]));
+                    }
+                }
+            }
+            __syncthreads();  // the next chunk or virtual block restages
         }
-        __syncthreads();  // the next virtual block restages
+        if (own) W.wnew[s + i] = o.hss ? copysign(sqrt(fabs(wv)), W.w[s + i]) : copysign(sqrt(-wv), W.w[s + i]);
     }
 }
 
@@ -1421,40 +1425,49 @@ __global__ __launch_bounds__(WG) void k_sbuild(const Merge *mg, const int *chunk
 }
 
 // HSS branch scalars (mdlaed3.f90:215-249): DIFL, DIFR, alpha_j = 1/||col j||.
-__device__ __forceinline__ void k_alpha_at(const Merge *mg, const int *chunk_start, int nm, const MergeOut *mo,
-                                           const Work &W, int vb)
-{
-    int m = find_seg(chunk_start, nm, vb);
-    const MergeOut o = mo[m];
-    int j = (vb - chunk_start[m]) * WG + threadIdx.x;
-    if (o.flags || !o.hss || j >= o.k) return;
-    int s = mg[m].s, K = o.k;
-    const double *d = W.dlamda + s, *tau = W.tau + s, *wz = W.wnew + s;
-    const int *org = W.org + s;
-    double difl = -delta_ij(d, j, org, tau, j);
-    double difr = (j + 1 < K) ? -delta_ij(d, j + 1, org, tau, j) : 0.0;
-    W.difl[s + j] = difl;
-    W.difr[s + j] = difr;
-    double dlamj = -d[j], difrj = 0.0, dlamjp = 0.0;
-    if (j + 1 < K) { difrj = -difr; dlamjp = -d[j + 1]; }
-    double acc = 0.0;
-    for (int i = 0; i < K; i++) {
-        double v;
-        if (i == j) v = wz[j] / difl;
-        else if (i < j) v = wz[i] / ((d[i] + dlamj) - difl);
-        else v = wz[i] / ((d[i] + dlamjp) + difrj);
-        acc += v * v;
-    }
-    W.alpha[s + j] = 1.0 / sqrt(acc);
-}
-
+// One thread per column j; the poles and Loewner z are staged in chunks of LW_LDS.
 __global__ __launch_bounds__(WG) void k_alpha(const Merge *mg, const int *chunk_start, int nm,
                                               const MergeOut *mo, Work W)
 {
     if (gate_closed(W, mg)) return;
+    __shared__ double sd[LW_LDS], sw[LW_LDS];
     const int nvb = chunk_start[nm];
-    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x)
-        k_alpha_at(mg, chunk_start, nm, mo, W, vb);
+    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        const int m = find_seg(chunk_start, nm, vb);
+        const MergeOut o = mo[m];
+        if (o.flags || !o.hss) continue;  // uniform over the block
+        const int s = mg[m].s, K = o.k, j = (vb - chunk_start[m]) * WG + threadIdx.x;
+        const double *d = W.dlamda + s, *tau = W.tau + s, *wz = W.wnew + s;
+        const int *org = W.org + s;
+        const bool own = j < K;
+        double difl = 0.0, difr = 0.0, dlamj = 0.0, difrj = 0.0, dlamjp = 0.0, wzj = 0.0;
+        if (own) {
+            difl = -delta_ij(d, j, org, tau, j);
+            difr = (j + 1 < K) ? -delta_ij(d, j + 1, org, tau, j) : 0.0;
+            W.difl[s + j] = difl;
+            W.difr[s + j] = difr;
+            dlamj = -d[j];
+            if (j + 1 < K) { difrj = -difr; dlamjp = -d[j + 1]; }
+            wzj = wz[j];
+        }
+        double acc = 0.0;
+        for (int c0 = 0; c0 < K; c0 += LW_LDS) {
+            const int cn = min(LW_LDS, K - c0);
+            for (int t = threadIdx.x; t < cn; t += WG) { sd[t] = d[c0 + t]; sw[t] = wz[c0 + t]; }
+            __syncthreads();
+            if (own)
+                for (int ii = 0; ii < cn; ii++) {
+                    const int i = c0 + ii;
+                    double v;
+                    if (i == j) v = wzj / difl;
+                    else if (i < j) v = sw[ii] / ((sd[ii] + dlamj) - difl);
+                    else v = sw[ii] / ((sd[ii] + dlamjp) + difrj);
+                    acc += v * v;
+                }
+            __syncthreads();  // the next chunk or virtual block restages
+        }
+        if (own) W.alpha[s + j] = 1.0 / sqrt(acc);
+    }
 }
 
 // ---------------------------------------------------------------------------

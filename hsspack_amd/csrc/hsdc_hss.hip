@@ -177,6 +177,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     double amax = 0.0;
     int Rk = mn;
     const double rho = 1.5;
+    bool pre = false;     // pre_rw holds this step's first column search
+    VI pre_rw = {0.0, 0};
     for (int k = 1; k <= mn; k++) {
         // ---- CauchyPivtEig: rook search on the k-th Schur complement
         {
@@ -188,14 +190,17 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 x = vi_better(x, VI{v, i - k + 1});
             }
             VI cl = block_argmax(x, red);
-            VI y = {0.0, 0};
-            int fk = A1(PL, k);
-            #pragma unroll 8
-            for (int j = k + tid; j <= N; j += HSS_CWG) {
-                double v = fabs(A1(V, j) / fdj(fk, j));
-                y = vi_better(y, VI{v, j - k + 1});
+            VI rw = pre_rw;  // computed by the previous step's update pass when pre
+            if (!pre) {
+                VI y = {0.0, 0};
+                int fk = A1(PL, k);
+                #pragma unroll 8
+                for (int j = k + tid; j <= N; j += HSS_CWG) {
+                    double v = fabs(A1(V, j) / fdj(fk, j));
+                    y = vi_better(y, VI{v, j - k + 1});
+                }
+                rw = block_argmax(y, red);
             }
-            VI rw = block_argmax(y, red);
             double junkL = cl.v * fabs(A1(V, k)), junkU = rw.v * fabs(A1(Z, k));
             double piv = fabs(A1(Z, k) * A1(V, k) / fd(H, A1(PL, k), A1(PU, k)));
             if (!(junkL <= piv && junkU <= piv)) {
@@ -281,7 +286,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
         }
-        // ---- generator updates of the (k+1)-th Schur complement
+        // ---- generator updates of the (k+1)-th Schur complement; unless a swap check
+        // follows, the pass over the columns also runs step k+1's first column search
         const double Fk = A1(F, k);
         const int plk = A1(PL, k), puk = A1(PU, k);
         for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
@@ -289,12 +295,17 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             temp = (A1(F, j) - Fk) / temp;
             A1(Z, j) = A1(Z, j) * temp;
         }
+        const bool fuse = k + 1 <= mn && !(k % 10 == 0 && M - k > 0);
+        const int fk1 = fuse ? A1(PL, k + 1) : 0;
+        VI y1 = {0.0, 0};
         #pragma unroll 8
         for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
             double temp = ddj(j, puk);
             double temp2 = -1.0 * fdj(plk, j);
             temp = temp / temp2;
-            A1(V, j) = A1(V, j) * temp;
+            const double vj = A1(V, j) * temp;
+            A1(V, j) = vj;
+            if (fuse) y1 = vi_better(y1, VI{fabs(vj / fdj(fk1, j)), j - k});
         }
         for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
             double temp = -1.0 * fd(H, A1(PL, j), puk);
@@ -310,6 +321,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             A1(W, k) = wk;
         }
         __syncthreads();
+        pre = fuse;
+        if (fuse) pre_rw = block_argmax(y1, red);
         // ---- every 10 steps: swap while max |Mat_N| > rho (searchMax2Eig)
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
@@ -467,6 +480,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     double amax = 0.0;
     int Rk = mn;
     const double rho = 1.5;
+    bool pre = false;     // pre_rw holds this step's first column search
+    VI pre_rw = {0.0, 0};
     for (int k = 1; k <= mn; k++) {
         // ---- CauchyPivtEigm
         {
@@ -477,14 +492,17 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 x = vi_better(x, VI{v, i - k + 1});
             }
             VI cl = block_argmax(x, red);
-            VI y = {0.0, 0};
-            int plk = A1(PL, k);
-            #pragma unroll 8
-            for (int j = k + tid; j <= N; j += HSS_CWG) {
-                double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk));
-                y = vi_better(y, VI{v, j - k + 1});
+            VI rw = pre_rw;  // computed by the previous step's update pass when pre
+            if (!pre) {
+                VI y = {0.0, 0};
+                int plk = A1(PL, k);
+                #pragma unroll 8
+                for (int j = k + tid; j <= N; j += HSS_CWG) {
+                    double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk));
+                    y = vi_better(y, VI{v, j - k + 1});
+                }
+                rw = block_argmax(y, red);
             }
-            VI rw = block_argmax(y, red);
             double junkL = cl.v * fabs(A1(V, k)), junkU = rw.v * fabs(A1(Z, k));
             double den;
             if (A1(PL, k) < A1(PU, k))
@@ -589,10 +607,15 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             temp = temp / temp2;
             A1(Z, j) = A1(Z, j) * temp;
         }
+        const bool fuse = k + 1 <= mn && !(k % 10 == 0 && M - k > 0);
+        const int plk1 = fuse ? A1(PL, k + 1) : 0;
+        VI y1 = {0.0, 0};
         #pragma unroll 8
         for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
             double temp = fd(H, A1(PU, j), plk);
-            A1(V, j) = A1(V, j) * ((A1(F, j) - Fk) / temp);
+            const double vj = A1(V, j) * ((A1(F, j) - Fk) / temp);
+            A1(V, j) = vj;
+            if (fuse) y1 = vi_better(y1, VI{fabs(vj / fd(H, A1(PU, j), plk1)), j - k});
         }
         for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
             double temp = fd(H, puk, A1(PL, j));
@@ -612,6 +635,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             A1(W, k) = wk;
         }
         __syncthreads();
+        pre = fuse;
+        if (fuse) pre_rw = block_argmax(y1, red);
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
                 VI x = {0.0, 0};

@@ -160,6 +160,7 @@ struct hsdc_ctx {
     DevBuf featb;
     DevBuf ltab;  // tables of the device-scheduled level (k_lvltab)
     std::vector<long> tabkey;  // shape of the static tables kept on the device (device input)
+    std::vector<Prob> hprobs;  // the problems of the running solve (host copy)
     Plan fplan;                // plan of a device-input solve, by order and leaf bound
     int fplan_n = -1, fplan_sml = -1;
     hipGraphExec_t gexec = nullptr;  // the last captured solve and its shape
@@ -356,6 +357,10 @@ struct HssSpec {
     double *Y;         // nrow x K result X * H, leading dimension ldy
     long ldx, ldy;
     int nrow;
+    // optional: column j of the result goes to Y + (ymap[j] - ymoff) * ldy (the
+    // eigenvector slots of the merge, written in place of a copy through Ct)
+    const int *ymap = nullptr;
+    long ymoff = 0;
 };
 
 struct HssPlanM {
@@ -627,7 +632,10 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         hipLaunchKernelGGL(k_cbuild, dim3(std::min(cdiv((long)K * K, WGS), 4096)), dim3(WGS), 0, s,
                            (const HssMerge *)c->tabs.p, 0, CB + cbase[h]);
         HK(hipStreamSynchronize(s));
-        dense_items.push_back(bg(sp[h].X, sp[h].ldx, CB + cbase[h], K, sp[h].Y, sp[h].ldy, sp[h].nrow, K, K, 0));
+        BG g = bg(sp[h].X, sp[h].ldx, CB + cbase[h], K, sp[h].Y, sp[h].ldy, sp[h].nrow, K, K, 0);
+        g.cmap = sp[h].ymap;
+        g.cmoff = sp[h].ymoff;
+        dense_items.push_back(g);
     }
     if (!dense_items.empty()) steps.push_back(dense_items);
     int mlvl = 0;
@@ -695,8 +703,12 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
                 int vr = Q.vsz[2 * (nd - 1)], vc = Q.vsz[2 * (nd - 1) + 1], md = Q.dsz[nd - 1];
                 long lx = Q.T.leaf_r0[nd];
                 // V_i is vr x vc with vc = md (the leaf's columns)
-                BG g = bg(F + (long)Q.lf[nd] * nr, nr, ar + Q.pv[nd - 1], std::max(vr, 1), Y + lx * sp[h].ldy,
-                          sp[h].ldy, (int)nr, md, (vc > 0) ? vr : 0, 0);
+                BG g = bg(F + (long)Q.lf[nd] * nr, nr, ar + Q.pv[nd - 1], std::max(vr, 1),
+                          sp[h].ymap ? Y : Y + lx * sp[h].ldy, sp[h].ldy, (int)nr, md, (vc > 0) ? vr : 0, 0);
+                if (sp[h].ymap) {
+                    g.cmap = sp[h].ymap + lx;
+                    g.cmoff = sp[h].ymoff;
+                }
                 g.A2 = sp[h].X + lx * sp[h].ldx; g.lda2 = sp[h].ldx;
                 g.B2 = Q.H.dd + Q.pd[nd - 1]; g.ldb2 = md; g.k2 = md;
                 v1.push_back(g);
@@ -746,10 +758,10 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
 }
 
 // HSS branch of the merges `sel` of one DC level: X = Q2 gathered from the
-// eigenvector slots, Ct(:, 1:K) = X * H.
+// eigenvector slots, then X * H written straight into the slots of the merge's new
+// vectors (vm[cur ^ 1], which k_newcols has set).
 static int run_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, int m0,
-                   const std::vector<MergeOut> &ho, const std::vector<int> &sel, const std::vector<long> &coff,
-                   Prob *dprobs, int cur)
+                   const std::vector<MergeOut> &ho, const std::vector<int> &sel, Prob *dprobs, int cur)
 {
     hipStream_t s = c->stream;
     Work W = c->W;
@@ -781,14 +793,17 @@ static int run_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, 
         HK(hipGetLastError());
     }
     std::vector<HssSpec> sp(sel.size());
-    double *Ct = (double *)c->ct.p;
     for (size_t h = 0; h < sel.size(); h++) {
         const Merge &M = hm[m0 + sel[h]];
         HssSpec &q = sp[h];
         q.K = ho[sel[h]].k;
         q.FF = W.dlamda + M.s; q.D0 = W.lam + M.s; q.U0 = W.wnew + M.s; q.V0 = W.alpha + M.s;
         q.DIFL = W.difl + M.s; q.DIFR = W.difr + M.s;
-        q.X = X + xoff[h]; q.ldx = M.n; q.Y = Ct + coff[sel[h]]; q.ldy = M.n; q.nrow = M.n;
+        q.X = X + xoff[h]; q.ldx = M.n; q.nrow = M.n;
+        // new vector v of the merge into its slot vm[cur ^ 1][s + v] (set by k_newcols)
+        const Prob &PP = c->hprobs[M.prob];
+        q.Y = PP.q + (M.s - PP.off); q.ldy = PP.ld;
+        q.ymap = W.vm[cur ^ 1] + M.s; q.ymoff = PP.off;
     }
     return hss_apply(c, sp, c->prm.hss_leaf, c->prm.hss_tol, nullptr);
 }
@@ -843,7 +858,7 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
         soff[i] = stot;
         coff[i] = ctot;
         if (k > 0 && !o.hss) stot += (long)k * k;
-        ctot += (long)M.n * k;
+        if (!o.hss) ctot += (long)M.n * k;  // HSS merges write their vectors into the slots
         c->st.roots += k;
         c->st.deflated += M.n - k;
         if (o.hss) c->st.hss_merges++;
@@ -895,15 +910,17 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
             hipLaunchKernelGGL(k_gemm, dim3(tiles.size()), dim3(WGS), 0, s, dmg, dtiles, (int)tiles.size(),
                                (const int *)nullptr, nm, dprobs, dmo, dsoff, dcoff, W, cur, (const double *)c->sb.p,
                                (double *)c->ct.p);
+    }
+    hipLaunchKernelGGL(k_newcols, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dmo, W, cur);
+    if (anyk) {
         std::vector<int> sel;
         for (int i = 0; i < nm; i++)
             if (!ho[i].flags && ho[i].hss && ho[i].k > 0) sel.push_back(i);
-        if (!sel.empty()) {
+        if (!sel.empty()) {  // after k_newcols: the products go into the slots it assigned
             hipLaunchKernelGGL(k_alpha, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, W);
-            CK(run_hss(c, dmg, hm, m0, ho, sel, coff, dprobs, cur));
+            CK(run_hss(c, dmg, hm, m0, ho, sel, dprobs, cur));
         }
     }
-    hipLaunchKernelGGL(k_newcols, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dmo, W, cur);
     if (anyk)
         hipLaunchKernelGGL(k_scatter, dim3(colK[nm]), dim3(WGS), 0, s, dmg, dcolK, nm, dmo, dcoff, dprobs, W,
                            cur ^ 1, (const double *)c->ct.p, 0);
@@ -1019,30 +1036,21 @@ static int level_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
     hipStream_t s = c->stream;
     const LvlTab t = level_tables(c, nm);
     MergeOut *dmo = (MergeOut *)c->mo.p;
-    const size_t bmo = sizeof(MergeOut) * nm, bco = sizeof(long) * nm;
-    CK(ensure_pin(c->pin2, al64(bmo) + bco));
-    char *hp = (char *)c->pin2.p;
-    HK(hipMemcpyAsync(hp, dmo, bmo, hipMemcpyDeviceToHost, s));
-    HK(hipMemcpyAsync(hp + al64(bmo), t.coff, bco, hipMemcpyDeviceToHost, s));
+    const size_t bmo = sizeof(MergeOut) * nm;
+    CK(ensure_pin(c->pin2, bmo));
+    HK(hipMemcpyAsync(c->pin2.p, dmo, bmo, hipMemcpyDeviceToHost, s));
     HK(hipStreamSynchronize(s));
     std::vector<MergeOut> ho(nm);
-    std::vector<long> coff(nm);
-    memcpy(ho.data(), hp, bmo);
-    memcpy(coff.data(), hp + al64(bmo), bco);
+    memcpy(ho.data(), c->pin2.p, bmo);
     std::vector<int> sel;
-    long ctneed = 1;
     for (int i = 0; i < nm; i++)
-        if (!ho[i].flags && ho[i].hss && ho[i].k > 0) {
-            sel.push_back(i);
-            ctneed = std::max(ctneed, coff[i] + (long)hm[i].n * ho[i].k);
-        }
+        if (!ho[i].flags && ho[i].hss && ho[i].k > 0) sel.push_back(i);
     gate_open(c);
     if (sel.empty()) return 0;
-    CK(ensure(c->ct, sizeof(double) * (size_t)ctneed));  // the dense vectors are in place already
     Work W = c->W;
     hipLaunchKernelGGL(k_alpha, dim3(totN), dim3(WGS), 0, s, dmg, (const int *)t.chunkK, nm, (const MergeOut *)dmo,
                        W);
-    CK(run_hss(c, dmg, hm, 0, ho, sel, coff, dprobs, cur));
+    CK(run_hss(c, dmg, hm, 0, ho, sel, dprobs, cur));  // writes the new vectors into their slots
     hipLaunchKernelGGL(k_scatter, dim3(dev_grid(B.cols)), dim3(WGS), 0, s, dmg, (const int *)t.colK, nm,
                        (const MergeOut *)dmo, (const long *)t.coff, dprobs, W, cur ^ 1, (const double *)c->ct.p, 2);
     HK(hipGetLastError());
@@ -1104,6 +1112,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         hp[i].par = pl.probs[i].levels & 1;
         qo += (size_t)pl.probs[i].n * pl.probs[i].n;
     }
+    c->hprobs = hp;
     // static tables: probs | leaves | merges of all levels | d | e | scale | colprob | zoff | ldz | outcol | info
     size_t nl = pl.leaves.size(), nmtot = 0;
     for (auto &L : pl.lv) nmtot += L.size();

@@ -57,4 +57,7 @@ struct BG {
     const double *A2, *B2;  // optional second product (k2 > 0), B2 column-major
     long lda2, ldb2;
     int k2;
+    // optional column map of C: output column j goes to C + (cmap[j] - cmoff) * ldc
+    const int *cmap = nullptr;
+    long cmoff = 0;
 };

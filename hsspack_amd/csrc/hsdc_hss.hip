@@ -953,10 +953,11 @@ __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
         for (int b = 0; b < 4; b++) {
             const int col = c0 + b * 16 + lr;
             if (col >= g.n) continue;
+            const long pc = g.cmap ? (long)g.cmap[col] - g.cmoff : (long)col;
             for (int i = 0; i < 4; i++) {
                 int row = rb + wr + a * 16 + lk + 4 * i;
                 if (row < g.nrow) {
-                    double *cp = g.C + (long)row + (long)col * g.ldc;
+                    double *cp = g.C + (long)row + pc * g.ldc;
                     *cp = g.beta ? (*cp + acc[a][b][i]) : acc[a][b][i];
                 }
             }

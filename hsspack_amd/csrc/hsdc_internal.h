@@ -85,8 +85,7 @@ __device__ __forceinline__ bool gate_closed(const Work &W, const Merge *mg)
 
 // Tables of one level derived from its deflation results (k_lvltab): first chunk
 // (256 roots), secular block, column, GEMM tile and copy chunk of every merge,
-// offsets of its S and Ct blocks.  Ct holds the dense merges first, then the HSS
-// merges.
+// offsets of its S and Ct blocks (dense merges only).
 #define SC_CH 2048    // elements per work item of the dense eigenvector copy
 #define FIN_CH 1024   // rows per block of the final reordering copy
 struct LvlTab {

@@ -1600,8 +1600,9 @@ __global__ __launch_bounds__(WG) void k_newcols(const Merge *mg, const int *chun
     }
 }
 
-// copy the K updated eigenvectors from Ct into their physical slots; only = 1: the
-// dense merges, 2: the HSS merges, 0: both
+// copy the K updated eigenvectors of a dense merge from Ct into their physical
+// slots, and give every new vector (dense or HSS) the merge's row support;
+// only = 1: the dense merges, 2: the HSS merges, 0: both
 __device__ __forceinline__ void k_scatter_at(const Merge *mg, const int *col_start, int nm, const MergeOut *mo,
                                              const long *coff, const Prob *probs, const Work &W, int nxt,
                                              const double *Ct, int only, int vb)
@@ -1609,12 +1610,15 @@ __device__ __forceinline__ void k_scatter_at(const Merge *mg, const int *col_sta
     int m = find_seg(col_start, nm, vb);
     Merge M = mg[m];
     int v = vb - col_start[m];
-    if (v >= mo[m].k || (only == 1 && mo[m].hss) || (only == 2 && !mo[m].hss)) return;
+    const bool hss = mo[m].hss;
+    if (v >= mo[m].k || (only == 1 && hss) || (only == 2 && !hss)) return;
     Prob P = probs[M.prob];
     int slot = W.vm[nxt][M.s + v];
-    const double *src = Ct + coff[m] + (long)v * M.n;
-    double *dst = P.q + (long)(slot - P.off) * P.ld + (M.s - P.off);
-    for (int r = threadIdx.x; r < M.n; r += WG) dst[r] = src[r];
+    if (!hss) {  // the HSS multiply writes its vectors into the slots itself
+        const double *src = Ct + coff[m] + (long)v * M.n;
+        double *dst = P.q + (long)(slot - P.off) * P.ld + (M.s - P.off);
+        for (int r = threadIdx.x; r < M.n; r += WG) dst[r] = src[r];
+    }
     if (threadIdx.x == 0) { W.supp_lo[slot] = M.s; W.supp_hi[slot] = M.s + M.n; }
 }
 
@@ -1704,14 +1708,14 @@ __global__ __launch_bounds__(WG) void k_lamrg(const Merge *mg, const int *chunk_
 // the tables its remaining kernels read (what the host builds after a round trip
 // in the synchronised schedule), so a whole solve is queued without waiting.
 // Per merge: chunks of 256 roots, secular blocks, columns, 64x64 GEMM tiles, S
-// size and Ct size; Ct holds the dense merges first, then the HSS merges, whose
-// vectors the host computes once it has seen the level.  Dense merges have
+// size, Ct size and copy chunks, for the dense merges (the HSS merges' vectors
+// go straight into their slots once the host has seen the level).  Dense merges have
 // K < hss_min_k, so sbcap and ctcap (doubles, sized by the host from the merge
 // orders) bound the dense totals; a level beyond them does no work and sets bit
 // 1 of *info.  One block; thread t owns a contiguous run of merges.
 // ---------------------------------------------------------------------------
 #define LT_NT 256
-#define LT_Q 8
+#define LT_Q 7
 __device__ __forceinline__ void lt_counts(const Merge &M, const MergeOut &o, long *q)
 {
     const int k = o.flags ? 0 : o.k;
@@ -1723,8 +1727,7 @@ __device__ __forceinline__ void lt_counts(const Merge &M, const MergeOut &o, lon
     q[3] = (k > 0 && !hss) ? (long)(((M.n1 + 63) >> 6) + ((M.n - M.n1 + 63) >> 6)) * ((k + 63) >> 6) : 0;
     q[4] = (k > 0 && !hss) ? (long)k * k : 0;
     q[5] = hss ? 0 : (long)M.n * k;
-    q[6] = hss ? (long)M.n * k : 0;
-    q[7] = hss ? 0 : ((long)M.n * k + SC_CH - 1) / SC_CH;
+    q[6] = hss ? 0 : ((long)M.n * k + SC_CH - 1) / SC_CH;
 }
 
 __global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const MergeOut *mo, Work W, LvlTab t,
@@ -1777,14 +1780,13 @@ __global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const
     for (int m = a; m < b; m++) {
         const MergeOut o = mo[m];
         lt_counts(mg[m], o, q);
-        const bool hss = !o.flags && o.hss;
         t.chunkK[m] = over ? 0 : (int)base[0];
         t.blockK[m] = over ? 0 : (int)base[1];
         t.colK[m] = over ? 0 : (int)base[2];
         t.tileK[m] = over ? 0 : (int)base[3];
-        t.scK[m] = over ? 0 : (int)base[7];
+        t.scK[m] = over ? 0 : (int)base[6];
         t.soff[m] = base[4];
-        t.coff[m] = hss ? tot[5] + base[6] : base[5];
+        t.coff[m] = base[5];
 #pragma unroll
         for (int i = 0; i < LT_Q; i++) base[i] += q[i];
     }
@@ -1793,7 +1795,7 @@ __global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const
         t.blockK[nm] = over ? 0 : (int)tot[1];
         t.colK[nm] = over ? 0 : (int)tot[2];
         t.tileK[nm] = over ? 0 : (int)tot[3];
-        t.scK[nm] = over ? 0 : (int)tot[7];
+        t.scK[nm] = over ? 0 : (int)tot[6];
         if (over) atomicOr(info, 2);
     }
     // statistics, one atomic per wavefront

@@ -64,37 +64,44 @@ def check_result(torch, d, e, lam, zz):
     return orth, res / nrm
 
 
-def cpu_baseline(cfg, seed, hss):
-    """The oracle (C restatement of the reference path, one thread) on two sizes of
-    the same generator; extrapolated to n = 32768 with the measured exponent."""
+def cpu_baseline(cfg, seed, hss, n_full):
+    """The oracle (C restatement of the reference path, one thread) on the same
+    generator.  The headline workload (cfg 2: nearly everything deflates) runs at its
+    full order in a few seconds and is timed as it is; the HSS-heavy generators are
+    timed at 8192 and 16384 and extrapolated with the measured exponent."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as O
     from hsspack_amd.workloads import tridiag_cfg
     L = O.lib()
-    times = {}
-    for n in (8192, 16384):
+
+    def run(n):
         d, e = tridiag_cfg(cfg, n, seed)
         z = np.zeros((n, n), order="F")
         lwork, liwork = 1 + 4 * n + n * n, 3 + 5 * n
         work = np.zeros(lwork)
         iwork = np.zeros(liwork, np.int32)
-        z[:] = 0.0
-        work[:] = 0.0
         prm = O.params(bsmlz=hss[0], ni=hss[1], tol=hss[2])
         t0 = time.perf_counter()
         info = L.orc_stedc(ctypes.c_char(b"I"), n, O.dp(d), O.dp(e), O.dp(z), n, O.dp(work), lwork,
                            O.ip(iwork), liwork, ctypes.byref(prm), None)
-        times[n] = time.perf_counter() - t0
+        t = time.perf_counter() - t0
         assert info == 0
-        log("cpu oracle n=%d: %.2f s" % (n, times[n]))
-        del z, work
+        log("cpu oracle n=%d: %.2f s" % (n, t))
+        return t
+
+    if cfg == 2:
+        t = run(n_full)
+        return dict(value=t, unit="s", cores=1, kind="port",
+                    sample="oracle (C restatement of mdstedc/mdlaed0-3 + DHSSEVC, 1 thread), the full "
+                           "workload: n=%d, same generator and seed, all eigenpairs" % n_full)
+    times = {n: run(n) for n in (8192, 16384)}
     p = np.log2(times[16384] / times[8192])
-    est = times[16384] * 2.0 ** p
+    est = times[16384] * (n_full / 16384.0) ** p
     return dict(value=est, unit="s", cores=1, kind="port",
                 sample="oracle (C restatement of mdstedc/mdlaed0-3 + DHSSEVC, 1 thread) on the "
                        "same generator at n=8192 (%.2f s) and n=16384 (%.2f s), extrapolated to "
-                       "n=32768 with the measured exponent %.2f" % (times[8192], times[16384], p))
+                       "n=%d with the measured exponent %.2f" % (times[8192], times[16384], n_full, p))
 
 
 def bench_batch(args, np, torch, hsdc):
@@ -348,7 +355,7 @@ def main():
                      "bytes_per_launch": fin_bytes, "avg_ms": fin_avg},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cfg, args.seed, hss)
+        out["cpu_baseline"] = cpu_baseline(args.cfg, args.seed, hss, n)
     print(json.dumps(out), flush=True)
 
 

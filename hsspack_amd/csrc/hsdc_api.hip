@@ -46,8 +46,8 @@ __global__ void k_readback(Work, const int *, const unsigned long long *, const 
 __global__ void k_inp_scan(const double *, const double *, int, Work);
 __global__ void k_inp_scale(const double *, const double *, int, const int *, Merge *, int, double *, double *,
                             double *, Work);
-__global__ void k_scatter_dense(const Merge *, const int *, int, const MergeOut *, const long *, const Prob *, Work,
-                                int, const double *);
+__global__ void k_newcols_copy(const Merge *, const int *, int, int, const int *, const MergeOut *, const long *,
+                               const Prob *, Work, int, const double *);
 __global__ void k_lvltab(const Merge *, int, const MergeOut *, Work, LvlTab, long, long, int *);
 __global__ void k_final(const Prob *, const int *, Work, const int *, double *, const long *, const long *, int,
                         double *, const double *, unsigned long long *, const int *, int, int);
@@ -1017,10 +1017,9 @@ static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
         hipLaunchKernelGGL(k_gemm, dim3(dev_grid(B.tiles)), dim3(WGS), 0, s, dmg, (const GTile *)nullptr, 0,
                            (const int *)t.tileK, nm, dprobs, (const MergeOut *)dmo, (const long *)t.soff,
                            (const long *)t.coff, W, cur, (const double *)c->sb.p, (double *)c->ct.p);
-    hipLaunchKernelGGL(k_newcols, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur);
-    if (B.scat > 0)
-        hipLaunchKernelGGL(k_scatter_dense, dim3(dev_grid(B.scat)), dim3(WGS), 0, s, dmg, (const int *)t.scK, nm,
-                           (const MergeOut *)dmo, (const long *)t.coff, dprobs, W, cur ^ 1, (const double *)c->ct.p);
+    hipLaunchKernelGGL(k_newcols_copy, dim3(totN + (B.scat > 0 ? dev_grid(B.scat) : 0)), dim3(WGS), 0, s, dmg,
+                       dchunkN, nm, totN, (const int *)t.scK, (const MergeOut *)dmo, (const long *)t.coff, dprobs, W,
+                       cur, (const double *)c->ct.p);
     hipLaunchKernelGGL(k_lamrg, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur ^ 1);
     HK(hipGetLastError());
     return 0;

@@ -1571,14 +1571,13 @@ __global__ __launch_bounds__(WG) void k_gemm(const Merge *mg, const GTile *tiles
 // New column bookkeeping of the merged block: virtual column v < K is root v
 // (its vector from Ct), v >= K the deflated column INDXP(v) kept in place.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(WG) void k_newcols(const Merge *mg, const int *chunk_start, int nm,
-                                                const MergeOut *mo, Work W, int cur)
+__device__ __forceinline__ void newcols_at(const Merge *mg, const int *chunk_start, int nm, const MergeOut *mo,
+                                           const Work &W, int cur, int b)
 {
-    if (gate_closed(W, mg)) return;
-    int m = find_seg(chunk_start, nm, blockIdx.x);
+    int m = find_seg(chunk_start, nm, b);
     Merge M = mg[m];
     const MergeOut o = mo[m];
-    int v = (blockIdx.x - chunk_start[m]) * WG + threadIdx.x;
+    int v = (b - chunk_start[m]) * WG + threadIdx.x;
     if (v >= M.n) return;
     int g = M.s + v, nxt = cur ^ 1;
     if (o.flags) {
@@ -1598,6 +1597,13 @@ __global__ __launch_bounds__(WG) void k_newcols(const Merge *mg, const int *chun
         const double nx = (v + 1 < o.k) ? W.lam[g + 1] : W.ddef[g + 1];
         if ((v < o.k) ? (dv > nx) : (dv < nx)) atomicOr(&W.maux[4 * m + 3], 1ULL);
     }
+}
+
+__global__ __launch_bounds__(WG) void k_newcols(const Merge *mg, const int *chunk_start, int nm,
+                                                const MergeOut *mo, Work W, int cur)
+{
+    if (gate_closed(W, mg)) return;
+    newcols_at(mg, chunk_start, nm, mo, W, cur, blockIdx.x);
 }
 
 // copy the K updated eigenvectors of a dense merge from Ct into their physical
@@ -1632,17 +1638,23 @@ __global__ __launch_bounds__(WG) void k_scatter(const Merge *mg, const int *col_
         k_scatter_at(mg, col_start, nm, mo, coff, probs, W, nxt, Ct, only, vb);
 }
 
-// The same copy for the dense merges of a device-scheduled level, over the
-// elements of each merge's N x K block of Ct in chunks of SC_CH (sc_start: first
-// chunk of every merge), so that the many short columns of the low levels do not
-// each hold a block.
-__global__ __launch_bounds__(WG) void k_scatter_dense(const Merge *mg, const int *sc_start, int nm,
-                                                      const MergeOut *mo, const long *coff, const Prob *probs,
-                                                      Work W, int nxt, const double *Ct)
+// The new column bookkeeping and, for the dense merges of a device-scheduled level,
+// the copy of Ct into the slots, in one launch: blocks [0, nchunk) are k_newcols'
+// chunks; the rest walk the elements of each merge's N x K block of Ct in chunks of
+// SC_CH (sc_start: first chunk of every merge), so that the many short columns of
+// the low levels do not each hold a block.  The copy finds the slot of new vector
+// v as k_newcols does, vm[cur][INDXP(v)], without waiting for it.
+__global__ __launch_bounds__(WG) void k_newcols_copy(const Merge *mg, const int *chunk_start, int nm, int nchunk,
+                                                     const int *sc_start, const MergeOut *mo, const long *coff,
+                                                     const Prob *probs, Work W, int cur, const double *Ct)
 {
     if (gate_closed(W, mg)) return;
-    const int nvb = sc_start[nm];
-    for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+    if ((int)blockIdx.x < nchunk) {
+        newcols_at(mg, chunk_start, nm, mo, W, cur, blockIdx.x);
+        return;
+    }
+    const int nvb = sc_start[nm], g0 = blockIdx.x - nchunk, gs = gridDim.x - nchunk;
+    for (int vb = g0; vb < nvb; vb += gs) {
         const int m = find_seg(sc_start, nm, vb);
         const Merge M = mg[m];
         const Prob P = probs[M.prob];
@@ -1652,7 +1664,7 @@ __global__ __launch_bounds__(WG) void k_scatter_dense(const Merge *mg, const int
         for (long e = e0 + threadIdx.x; e < e0 + SC_CH && e < ne; e += WG) {
             // 32-bit division (N K < 2^31 for every dense merge a level can hold)
             const int v = (int)((unsigned)e / (unsigned)M.n), r = (int)e - v * M.n;
-            const int slot = W.vm[nxt][M.s + v];
+            const int slot = W.vm[cur][M.s + W.indxp[M.s + v] - 1];
             q[(long)(slot - P.off) * P.ld + r] = src[e];
             if (r == 0) { W.supp_lo[slot] = M.s; W.supp_hi[slot] = M.s + M.n; }
         }

@@ -50,7 +50,8 @@ __global__ void k_newcols_copy(const Merge *, const int *, int, int, const int *
                                const Prob *, Work, int, const double *);
 __global__ void k_lvltab(const Merge *, int, const MergeOut *, Work, LvlTab, long, long, int *);
 __global__ void k_final(const Prob *, const int *, Work, const int *, double *, const long *, const long *, int,
-                        double *, const double *, unsigned long long *, const int *, int, int);
+                        double *, const double *, unsigned long long *, const int *, int, int, int);
+__global__ void k_zfill(double *, long, long, long);
 __global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, long, double *, long);
 __global__ void k_hss_init(const HssMerge *);
 __global__ void k_hss_reset(const HssMerge *);
@@ -168,6 +169,9 @@ struct hsdc_ctx {
     hipStream_t gstream = nullptr;   // stream of graph solves called on the null stream
     hipEvent_t pf0 = nullptr, pf1 = nullptr;  // around the final kernel (profile = 1)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around a whole solve
+    hipStream_t zstream = nullptr;             // zero fill of Z beside the DC levels
+    hipEvent_t zev0 = nullptr, zev1 = nullptr; // its fork and join
+    hipEvent_t pz0 = nullptr, pz1 = nullptr;   // around it (profile = 1)
     SolveRec *rec = nullptr, *drec = nullptr;  // end-of-solve record: mapped host memory, its device address
 };
 
@@ -1228,10 +1232,29 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     if (c->prm.profile && !c->pf0) {
         HK(hipEventCreate(&c->pf0));
         HK(hipEventCreate(&c->pf1));
+        HK(hipEventCreate(&c->pz0));
+        HK(hipEventCreate(&c->pz1));
     }
     hipEvent_t f0 = c->pf0, f1 = c->pf1;
+    // one output holding the whole solve: zero it on a second stream while the levels
+    // run, the final reordering then writes the supported rows only
+    const bool zfill = zglobal != 0;
+    if (zfill && !c->zstream) {
+        HK(hipStreamCreateWithFlags(&c->zstream, hipStreamNonBlocking));
+        HK(hipEventCreateWithFlags(&c->zev0, hipEventDisableTiming));
+        HK(hipEventCreateWithFlags(&c->zev1, hipEventDisableTiming));
+    }
     // first launches: records and gate cleared, leaves
     auto launch_head = [&]() -> int {
+        if (zfill) {  // fork: the fill starts after the stream's earlier work on Z
+            HK(hipEventRecord(c->zev0, s));
+            HK(hipStreamWaitEvent(c->zstream, c->zev0, 0));
+            if (c->prm.profile) HK(hipEventRecord(c->pz0, c->zstream));
+            hipLaunchKernelGGL(k_zfill, dim3(32768), dim3(WGS), 0, c->zstream, dZ, (long)ntot, (long)ntot, ldz[0]);
+            HK(hipGetLastError());
+            if (c->prm.profile) HK(hipEventRecord(c->pz1, c->zstream));
+            HK(hipEventRecord(c->zev1, c->zstream));
+        }
         long nmx = 4 * ((long)c->wcap / 2 + 2) + 8;  // per-merge records, gate, level statistics
         hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(nmx, WGS)), dim3(WGS), 0, s, c->W.maux, nmx, 0ULL);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(WGS), 0, s, (unsigned long long *)(dp + a10), 72L, 0ULL);
@@ -1248,9 +1271,11 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         return 0;
     };
     auto launch_final = [&]() -> int {
+        if (zfill) HK(hipStreamWaitEvent(s, c->zev1, 0));  // join
         if (c->prm.profile) HK(hipEventRecord(f0, s));
-        hipLaunchKernelGGL(k_final, dim3(fin_blocks), dim3(WGS), 0, s, dprobs, dcolprob, c->W, doutcol, dZ, dzoff,
-                           dldz, zrows, d_out_dev, dscale, drows, dfbs, (int)np, fin_cz);
+        hipLaunchKernelGGL(k_final, dim3(zfill ? (long)ntot : fin_blocks), dim3(WGS), 0, s, dprobs, dcolprob, c->W,
+                           doutcol, dZ, dzoff, dldz, zrows, d_out_dev, dscale, drows, dfbs, (int)np, fin_cz,
+                           zfill ? 1 : 0);
         HK(hipGetLastError());
         if (c->prm.profile) HK(hipEventRecord(f1, s));
         hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, s, c->W, (const int *)dinfo,
@@ -1270,7 +1295,8 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     if (all_dev && use_graph && s != nullptr) {
         std::vector<long> key = {(long)nlv, (long)np, (long)nl, (long)lmax, (long)ntot, (long)c->wcap, zrows,
                                  fin_blocks, (long)(outcol != nullptr), c->prm.maxit, c->prm.hss_min_k,
-                                 (long)c->sec_pre, (long)c->prm.learned_min_k, (long)c->prm.profile,
+                                 (long)c->sec_pre, (long)c->prm.learned_min_k, (long)c->prm.profile, (long)zfill,
+                                 (long)ldz[0], (long)(uintptr_t)c->zstream,
                                  (long)(uintptr_t)c->stream};
         for (const void *ptr : {(const void *)c->wsd.p, (const void *)c->wsi.p, (const void *)c->maux.p,
                                 (const void *)c->probs.p, (const void *)c->mo.p, (const void *)c->ltab.p,
@@ -1398,11 +1424,18 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     c->st.ms_total = ms;
     long long zrows_written = 0;
     for (auto &P : pl.probs) zrows_written += (long long)(zglobal ? ntot : P.n) * P.n;
-    c->st.final_bytes = 8LL * (zrows_written + (long long)rows) + 16LL * ntot;
+    if (zfill) {
+        c->st.zfill_bytes = 8LL * zrows_written;
+        c->st.final_bytes = 16LL * (long long)rows + 16LL * ntot;  // supported rows read and written
+    } else {
+        c->st.final_bytes = 8LL * (zrows_written + (long long)rows) + 16LL * ntot;
+    }
     if (c->prm.profile) {
-        float fm = 0;
+        float fm = 0, zm = 0;
         if (hipEventElapsedTime(&fm, f0, f1) != hipSuccess) { (void)hipGetLastError(); fm = 0; }
         c->st.ms_final = fm;
+        if (zfill && hipEventElapsedTime(&zm, c->pz0, c->pz1) != hipSuccess) { (void)hipGetLastError(); zm = 0; }
+        c->st.ms_zfill = zm;
     }
     if (hinfo & 2) return HSDC_ERR_INTERNAL;  // device-scheduled level over its bounds
     return hinfo ? 1 : 0;
@@ -1484,6 +1517,8 @@ int hsdc_ctx_destroy(hsdc_ctx *c)
     if (c->pf1) (void)hipEventDestroy(c->pf1);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : {c->zev0, c->zev1, c->pz0, c->pz1}) if (e) (void)hipEventDestroy(e);
+    if (c->zstream) (void)hipStreamDestroy(c->zstream);
     if (c->rec) (void)hipHostFree(c->rec);
     for (DevBuf *b : bufs) if (b->p) hipFree(b->p);
     if (c->pin.p) hipHostFree(c->pin.p);

@@ -1826,6 +1826,29 @@ __global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const
 }
 
 // ---------------------------------------------------------------------------
+// Zero fill of the rows x ncols output Z (leading dimension ldz), launched on a
+// second stream when the solve starts: the DC levels are latency bound and leave
+// HBM idle, so the 8 n^2 bytes of zeros are written under them, and the final
+// reordering writes only the supported rows.  Consecutive blocks write
+// consecutive 4 KB pieces (one contiguous front, as a plain device fill).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_zfill(double *Z, long rows, long ncols, long ldz)
+{
+    typedef double dbl2 __attribute__((ext_vector_type(2)));
+    const dbl2 z2 = {0.0, 0.0};
+    if (ldz == rows && (((unsigned long)Z) & 15) == 0) {
+        const long n2 = rows * ncols;
+        dbl2 *d2 = (dbl2 *)Z;
+        for (long q = blockIdx.x * (long)WG + threadIdx.x; q < (n2 >> 1); q += (long)gridDim.x * WG)
+            __builtin_nontemporal_store(z2, d2 + q);
+        if ((n2 & 1) && blockIdx.x == 0 && threadIdx.x == 0) Z[n2 - 1] = 0.0;
+    } else {
+        for (long col = blockIdx.x; col < ncols; col += gridDim.x)
+            for (long r = threadIdx.x; r < rows; r += WG) __builtin_nontemporal_store(0.0, Z + col * ldz + r);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Final reordering (mdlaed0.f90:355-362 and the scale-back of mdstedc.f90:358):
 // Z(:, i) = Q(:, slot of INDXQ(i)), zero outside the column's support.  One block
 // per FIN_CH rows of an output column, consecutive blocks on consecutive rows of
@@ -1833,16 +1856,21 @@ __global__ __launch_bounds__(LT_NT) void k_lvltab(const Merge *mg, int nm, const
 // owning whole columns write thousands of separate streams and reach about 5.1
 // TB/s against 5.9 TB/s measured for this order).  zrows_global: one n x n output,
 // cz chunks per column; else problem p owns blocks [fbs[p], fbs[p+1]) of its own
-// output block.
+// output block.  zfilled: k_zfill has zeroed Z (on a second stream, during the DC
+// levels), and one block per column writes only the rows of its slot's support.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_prob, Work W, const int *outcol,
                                               double *Z, const long *zoff, const long *ldz, int zrows_global,
                                               double *dout, const double *scale, unsigned long long *rows_read,
-                                              const int *fbs, int np, int cz)
+                                              const int *fbs, int np, int cz, int zfilled)
 {
     if (*W.gate) return;  // a level waits for the host
     int g, c, p;
-    if (zrows_global) {
+    if (zfilled) {  // Z is zero already: one block per column writes the supported rows
+        g = blockIdx.x;
+        c = 0;
+        p = col_prob[g];
+    } else if (zrows_global) {
         g = blockIdx.x / cz;
         c = blockIdx.x - g * cz;
         p = col_prob[g];
@@ -1871,6 +1899,11 @@ __global__ __launch_bounds__(WG) void k_final(const Prob *probs, const int *col_
     } else {             // one output block per problem
         dst = Z + zoff[p] + (long)(oc - P.off) * ldz[p] - P.off;
         r0 = P.off; r1 = P.off + P.n;
+    }
+    if (zfilled) {
+        const int a = max(lo, r0), b = min(hi, r1);
+        for (int r = a + threadIdx.x; r < b; r += WG) __builtin_nontemporal_store(src[r], dst + r);
+        return;
     }
     const int c0 = r0 + c * FIN_CH, c1 = min(r1, c0 + FIN_CH);  // this block's rows
     // streaming stores, 16 bytes per lane when the column start is 16-byte aligned

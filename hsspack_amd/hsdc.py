@@ -49,7 +49,8 @@ class Stats(ctypes.Structure):
                 ("k_top", _i), ("rank_max", _i), ("ms_total", _d), ("ms_final", _d),
                 ("final_bytes", ctypes.c_longlong), ("learned_roots", ctypes.c_longlong),
                 ("secular_wave_iters", ctypes.c_longlong), ("secular_crit_iters", ctypes.c_longlong),
-                ("hss_mm_flops", ctypes.c_longlong), ("ms_hss_mm", _d)]
+                ("hss_mm_flops", ctypes.c_longlong), ("ms_hss_mm", _d), ("ms_zfill", _d),
+                ("zfill_bytes", ctypes.c_longlong)]
 
 
 _LIB = None

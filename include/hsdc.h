@@ -57,12 +57,15 @@ typedef struct {
     int rank_max;       /* largest HSS generator rank */
     double ms_total;    /* device time of the solve (events), ms */
     double ms_final;    /* final reordering kernel (events, profile = 1), ms */
-    long long final_bytes;  /* its algorithmic HBM bytes: Z written + Q rows read */
+    long long final_bytes;  /* its algorithmic HBM bytes: Z written (the supported rows only when
+                               Z was zero-filled beside the levels) + Q rows read */
     long long learned_roots; /* roots started from the learned proposal */
     long long secular_wave_iters; /* sum over 64-root wavefronts of their largest count */
     long long secular_crit_iters; /* sum over DC levels of the largest count of the level */
     long long hss_mm_flops;  /* flops of the HSS multiplies (2 L (|H| + |DD|) per merge) */
     double ms_hss_mm;        /* their device time (events, profile = 1), ms */
+    double ms_zfill;         /* zero fill of Z, run beside the DC levels (events, profile = 1), ms */
+    long long zfill_bytes;   /* its bytes: 8 n^2 (device-resident single-output solves; else 0) */
 } hsdc_stats;
 
 void hsdc_default_params(hsdc_params *p);

@@ -10,6 +10,8 @@ tridiagonal divide-and-conquer path, TEST/Testedc.f90:18-37 for the near-Toeplit
 family):
   cfg 1  d = 2 + U(-1e-3, 1e-3),  e = 1 + U(-1e-3, 1e-3)      (low deflation)
   cfg 2  d, e ~ U(-1, 1)                                       (heavy deflation)
+Upper bidiagonal (the SVD path):
+  cfg 5  d ~ U(0.5, 1.5),  e ~ U(0, 1)
 """
 import numpy as np
 
@@ -43,6 +45,12 @@ def tridiag_cfg(cfg, n, seed, sub=0):
     else:
         raise ValueError("unknown workload family %r" % cfg)
     return d, e
+
+
+def bidiag_cfg(n, seed=5, sub=0):
+    """(d, e) of the order-n upper bidiagonal of workload family 5."""
+    u = splitmix_uniform(seed, 2 * n - 1, sub)
+    return 0.5 + u[:n], u[n:].copy()
 
 
 def batch_sizes(count=512, lo=256, hi=8192, seed=3):

@@ -154,6 +154,41 @@ void orc_hss_free(orc_hss *H);
 void orc_cauchy_dense(int k, const double *f, const double *u, const double *v,
                       const double *difl, const double *difr, double *a, int lda);
 
+/* ---- bidiagonal SVD (mdbdsdc.f90 / mdlasd0-2.f90 + LAPACK DLASD3/DLASD4/DLASDQ) ---- */
+/* LAPACK DLASCL 'G' on a vector, DLARTG, DLASDT (node arrays 0-based, values 1-based). */
+void orc_lascl(double cfrom, double cto, int n, double *x);
+void orc_lartg(double f, double g, double *c, double *s, double *r);
+void orc_lasdt(int n, int *lvl, int *nd, int *inode, int *ndiml, int *ndimr, int msub);
+/* Leaf SVD of the m x (m+sqre) upper bidiagonal (replaces DLASDQ): singular values
+ * ascending in d, U m x m, VT (m+sqre)^2 with the null vector last when sqre = 1.
+ * One-sided Jacobi; returns the sweeps, < 0 on failure.  m + sqre <= 64. */
+int orc_bd_leaf(int m, int sqre, double *d, const double *e, double *u, int ldu, double *vt,
+                int ldvt);
+/* Root j (1-based) of 1/rho + sum z_i^2/((d_i-s)(d_i+s)) (replaces DLASD4): s = d(org)+tau. */
+int orc_lasd4(int k, int j, const double *d, const double *z, double rho, double *sigma, int *org,
+              double *tau, int *niter, int maxit);
+/* LAPACK DLASD2 == MDLASD2 (K < BSMLZ); rotation log optional (NULL). */
+int orc_lasd2(int nl, int nr, int sqre, int *k, double *d, double *z, double alpha, double beta,
+              double *u, int ldu, double *vt, int ldvt, double *dsigma, double *u2, int ldu2,
+              double *vt2, int ldvt2, int *idxp, int *idx, int *idxc, int *idxq, int *coltyp,
+              int *nrot, int *rot_p, int *rot_q, double *rot_c, double *rot_s);
+/* LAPACK DLASD3 (dense branch); q: 2k^2 + 3k doubles. */
+int orc_lasd3(int nl, int nr, int sqre, int k, double *d, double *q, const double *dsigma,
+              double *u, int ldu, const double *u2, int ldu2, double *vt, int ldvt,
+              const double *vt2, int ldvt2, const int *idxc, const int *ctot, double *z, int maxit,
+              long long *iters);
+/* MDLASD1 with the dense branch for every K. */
+int orc_lasd1(int nl, int nr, int sqre, double *d, double *alpha, double *beta, double *u, int ldu,
+              double *vt, int ldvt, int *idxq, const orc_params *p, int *kout, long long *iters,
+              int *nrot);
+/* MDLASD0; trace (may be NULL): level, n, k per merge and the deflation rotations in
+ * the hss slot. */
+int orc_lasd0(int n, int sqre, double *d, double *e, double *u, int ldu, double *vt, int ldvt,
+              const orc_params *p, orc_trace *tr);
+/* MDBDSDC, COMPQ 'N' or 'I' ('N' still forms the vectors in u, vt). */
+int orc_bdsdc(char uplo, char compq, int n, double *d, double *e, double *u, int ldu, double *vt,
+              int ldvt, const orc_params *p, orc_trace *tr);
+
 #ifdef __cplusplus
 }
 #endif

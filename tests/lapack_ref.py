@@ -129,3 +129,100 @@ def dlaed3(k, n, n1, q, rho, dlamda, q2, indx, ctot, w):
                         _dp(dlamda), _dp(q2), _ipa(indx), _ipa(ctot), _dp(w), _dp(s),
                         ctypes.byref(info))
     return d, q, info.value
+
+
+# ---------------------------------------------------------------- bidiagonal SVD
+# DLASDT, DLASD1-4, DLASDQ, DBDSDC: the LAPACK routines behind MDBDSDC / MDLASD0-2
+# (mdlasd0.f90:166-183, mdlasd1.f90:239-244) and the driver MDBDSDC is built from.
+def dlasdt(n, msub):
+    inode = np.zeros(n, np.int32)
+    ndiml = np.zeros(n, np.int32)
+    ndimr = np.zeros(n, np.int32)
+    lvl, nd = _i(0), _i(0)
+    lib().scipy_dlasdt_(_ip(n), ctypes.byref(lvl), ctypes.byref(nd), _ipa(inode), _ipa(ndiml),
+                        _ipa(ndimr), _ip(msub))
+    return lvl.value, nd.value, inode[:nd.value], ndiml[:nd.value], ndimr[:nd.value]
+
+
+def dlasd4(d, z, rho, j):
+    n = len(d)
+    d = np.ascontiguousarray(d, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    delta = np.zeros(n)
+    work = np.zeros(n)
+    sigma = _d(0)
+    info = _i(0)
+    lib().scipy_dlasd4_(_ip(n), _ip(j), _dp(d), _dp(z), _dp(delta), ctypes.byref(_d(rho)),
+                        ctypes.byref(sigma), _dp(work), ctypes.byref(info))
+    return sigma.value, delta, work, info.value
+
+
+def dlasdq(d, e, sqre):
+    """SVD of the n x (n+sqre) upper bidiagonal: (s ascending, U, VT, info)."""
+    n = len(d)
+    m = n + sqre
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64) if len(e) else np.zeros(1)
+    vt = np.asfortranarray(np.eye(m))
+    u = np.asfortranarray(np.eye(n))
+    c = np.zeros((1, 1), order="F")
+    work = np.zeros(4 * m + 8)
+    info = _i(0)
+    lib().scipy_dlasdq_(ctypes.c_char_p(b"U"), _ip(sqre), _ip(n), _ip(m), _ip(n), _ip(0), _dp(d),
+                        _dp(e), _dp(vt), _ip(m), _dp(u), _ip(n), _dp(c), _ip(1), _dp(work),
+                        ctypes.byref(info), ctypes.c_size_t(1))
+    return d, u, vt, info.value
+
+
+def dlasd2(nl, nr, sqre, d, alpha, beta, u, vt, idxq):
+    n = nl + nr + 1
+    m = n + sqre
+    d = np.array(d, np.float64)
+    u = np.array(u, np.float64, order="F")
+    vt = np.array(vt, np.float64, order="F")
+    idxq = np.array(idxq, np.int32)
+    z = np.zeros(m)
+    dsigma = np.zeros(n)
+    u2 = np.zeros((n, n), order="F")
+    vt2 = np.zeros((m, m), order="F")
+    idxp, idx, idxc, coltyp = (np.zeros(n, np.int32) for _ in range(4))
+    k, info = _i(0), _i(0)
+    lib().scipy_dlasd2_(_ip(nl), _ip(nr), _ip(sqre), ctypes.byref(k), _dp(d), _dp(z),
+                        ctypes.byref(_d(alpha)), ctypes.byref(_d(beta)), _dp(u), _ip(n), _dp(vt),
+                        _ip(m), _dp(dsigma), _dp(u2), _ip(n), _dp(vt2), _ip(m), _ipa(idxp),
+                        _ipa(idx), _ipa(idxc), _ipa(idxq), _ipa(coltyp), ctypes.byref(info))
+    return dict(k=k.value, d=d, z=z, u=u, vt=vt, dsigma=dsigma, u2=u2, vt2=vt2, idxp=idxp,
+                idx=idx, idxc=idxc, idxq=idxq, coltyp=coltyp, info=info.value)
+
+
+def dlasd1(nl, nr, sqre, d, alpha, beta, u, vt, idxq):
+    n = nl + nr + 1
+    m = n + sqre
+    d = np.array(d, np.float64)
+    u = np.array(u, np.float64, order="F")
+    vt = np.array(vt, np.float64, order="F")
+    idxq = np.array(idxq, np.int32)
+    iwork = np.zeros(4 * n, np.int32)
+    work = np.zeros(3 * m * m + 2 * m)
+    a, b, info = _d(alpha), _d(beta), _i(0)
+    lib().scipy_dlasd1_(_ip(nl), _ip(nr), _ip(sqre), _dp(d), ctypes.byref(a), ctypes.byref(b),
+                        _dp(u), _ip(n), _dp(vt), _ip(m), _ipa(idxq), _ipa(iwork), _dp(work),
+                        ctypes.byref(info))
+    return d, u, vt, idxq, info.value
+
+
+def dbdsdc(d, e, uplo=b"U"):
+    n = len(d)
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64) if n > 1 else np.zeros(1)
+    u = np.zeros((n, n), order="F")
+    vt = np.zeros((n, n), order="F")
+    q = np.zeros(1)
+    iq = np.zeros(1, np.int32)
+    work = np.zeros(3 * n * n + 4 * n + 8)
+    iwork = np.zeros(8 * n, np.int32)
+    info = _i(0)
+    lib().scipy_dbdsdc_(ctypes.c_char_p(uplo), ctypes.c_char_p(b"I"), _ip(n), _dp(d), _dp(e),
+                        _dp(u), _ip(n), _dp(vt), _ip(n), _dp(q), _ipa(iq), _dp(work), _ipa(iwork),
+                        ctypes.byref(info), ctypes.c_size_t(1), ctypes.c_size_t(1))
+    return d, u, vt, info.value

@@ -281,6 +281,9 @@ def cauchy_dense(f, u, v, difl, difr):
 
 def _setup():
     L = lib()
+    for f in ("orc_bd_leaf", "orc_lasd4", "orc_lasd2", "orc_lasd3", "orc_lasd1", "orc_lasd0",
+              "orc_bdsdc"):
+        getattr(L, f).restype = _i
     L.orc_laed4.restype = _i
     L.orc_laed4_mlp.restype = _i
     L.orc_stedc.restype = _i
@@ -305,3 +308,104 @@ def mdlaed3_vectors(dlamda, z, rho, maxit=100):
     if info != 0:
         raise RuntimeError("mdlaed3_vectors info=%d" % info)
     return dict(d=d, dlamda=dl, z=zz, difl=difl, difr=difr, alpha=alpha, iters=it.value)
+
+
+# ---------------------------------------------------------------- bidiagonal SVD
+def lasdt(n, msub):
+    inode = np.zeros(max(n, 1), np.int32)
+    ndiml = np.zeros(max(n, 1), np.int32)
+    ndimr = np.zeros(max(n, 1), np.int32)
+    lvl, nd = _i(0), _i(0)
+    lib().orc_lasdt(n, ctypes.byref(lvl), ctypes.byref(nd), ip(inode), ip(ndiml), ip(ndimr), msub)
+    return lvl.value, nd.value, inode[:nd.value], ndiml[:nd.value], ndimr[:nd.value]
+
+
+def bd_leaf(d, e, sqre):
+    """(s ascending, U, VT, sweeps) of the m x (m+sqre) upper bidiagonal leaf."""
+    m = len(d)
+    p = m + sqre
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64) if len(e) else np.zeros(1)
+    u = np.zeros((m, m), order="F")
+    vt = np.zeros((p, p), order="F")
+    sw = lib().orc_bd_leaf(m, sqre, dp(d), dp(e), dp(u), m, dp(vt), p)
+    return d, u, vt, sw
+
+
+def lasd4(d, z, rho, j, maxit=100):
+    k = len(d)
+    d = np.ascontiguousarray(d, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    sg, tau = _d(0), _d(0)
+    org, nit = _i(0), _i(0)
+    info = lib().orc_lasd4(k, j, dp(d), dp(z), _d(rho), ctypes.byref(sg), ctypes.byref(org),
+                           ctypes.byref(tau), ctypes.byref(nit), maxit)
+    return dict(sigma=sg.value, org=org.value, tau=tau.value, niter=nit.value, info=info)
+
+
+def lasd2(nl, nr, sqre, d, alpha, beta, u, vt, idxq):
+    n = nl + nr + 1
+    m = n + sqre
+    d = np.array(d, np.float64)
+    u = np.array(u, np.float64, order="F")
+    vt = np.array(vt, np.float64, order="F")
+    idxq = np.array(idxq, np.int32)
+    z = np.zeros(m)
+    dsigma = np.zeros(n)
+    u2 = np.zeros((n, n), order="F")
+    vt2 = np.zeros((m, m), order="F")
+    idxp, idx, idxc, coltyp = (np.zeros(n, np.int32) for _ in range(4))
+    k, nrot = _i(0), _i(0)
+    rp, rq = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    rc, rs = np.zeros(n), np.zeros(n)
+    lib().orc_lasd2(nl, nr, sqre, ctypes.byref(k), dp(d), dp(z), _d(alpha), _d(beta), dp(u), n,
+                    dp(vt), m, dp(dsigma), dp(u2), n, dp(vt2), m, ip(idxp), ip(idx), ip(idxc),
+                    ip(idxq), ip(coltyp), ctypes.byref(nrot), ip(rp), ip(rq), dp(rc), dp(rs))
+    r = nrot.value
+    return dict(k=k.value, d=d, z=z, u=u, vt=vt, dsigma=dsigma, u2=u2, vt2=vt2, idxp=idxp, idx=idx,
+                idxc=idxc, idxq=idxq, coltyp=coltyp, rot=(rp[:r], rq[:r], rc[:r], rs[:r]))
+
+
+def lasd1(nl, nr, sqre, d, alpha, beta, u, vt, idxq, prm=None):
+    n = nl + nr + 1
+    m = n + sqre
+    d = np.array(d, np.float64)
+    u = np.array(u, np.float64, order="F")
+    vt = np.array(vt, np.float64, order="F")
+    idxq = np.array(idxq, np.int32)
+    a, b = _d(alpha), _d(beta)
+    k, nrot = _i(0), _i(0)
+    it = ctypes.c_longlong(0)
+    prm = prm or params()
+    info = lib().orc_lasd1(nl, nr, sqre, dp(d), ctypes.byref(a), ctypes.byref(b), dp(u), n, dp(vt),
+                           m, ip(idxq), ctypes.byref(prm), ctypes.byref(k), ctypes.byref(it),
+                           ctypes.byref(nrot))
+    return dict(d=d, u=u, vt=vt, idxq=idxq, k=k.value, iters=it.value, nrot=nrot.value, info=info)
+
+
+def bdsdc(d, e, uplo=b"U", compq=b"I", prm=None, trace_cap=0):
+    n = len(d)
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64) if n > 1 else np.zeros(1)
+    u = np.zeros((n, n), order="F")
+    vt = np.zeros((n, n), order="F")
+    prm = prm or params()
+    tr = None
+    bufs = None
+    if trace_cap:
+        tr = Trace()
+        bufs = [np.zeros(trace_cap, np.int32) for _ in range(5)]
+        tr.cap = trace_cap
+        tr.nmerge = 0
+        tr.level, tr.n, tr.k, tr.hss, tr.rank_max = [ip(b) for b in bufs]
+        tr.roots = 0
+        tr.iters = 0
+    info = lib().orc_bdsdc(ctypes.c_char(uplo), ctypes.c_char(compq), n, dp(d), dp(e), dp(u),
+                           max(1, n), dp(vt), max(1, n), ctypes.byref(prm),
+                           ctypes.byref(tr) if tr is not None else None)
+    out = dict(d=d, u=u, vt=vt, info=info)
+    if tr is not None:
+        m = min(tr.nmerge, trace_cap)
+        out["trace"] = dict(level=bufs[0][:m].copy(), n=bufs[1][:m].copy(), k=bufs[2][:m].copy(),
+                            nrot=bufs[3][:m].copy(), roots=tr.roots, iters=tr.iters)
+    return out

@@ -1,0 +1,1672 @@
+// Bidiagonal divide-and-conquer SVD on gfx950: HSSPACK's MDBDSDC (SRC/mdbdsdc.f90)
+// -> MDLASD0 (mdlasd0.f90) -> MDLASD1 (mdlasd1.f90) -> MDLASD2 (mdlasd2.f90) and the
+// dense merge DLASD3/DLASD4 it calls below BSMLZ, as HIP kernels batched over every
+// merge of one tree level.
+//
+// Layout.  U (n x n) and V = VT^T (n x n) are column-major in two ping-pong buffers
+// each; a merge reads its children's columns from one and writes its own to the
+// other.  A column is only ever read inside its row support: the rows of the tree
+// node that last wrote it (or the whole merge block once a deflation rotation has
+// touched it), so neither buffer is cleared.  Working vectors live at the global
+// index of the merge block (s + i), disjoint between the merges of one level.
+//
+// Per level: k_bd_prep (scaling, z, sort: mdlasd1.f90:171-184, mdlasd2.f90:216-268),
+// k_bd_deflate (mdlasd2.f90:270-470, in parallel), a read of K, k_bd_rotate (the
+// deflation's Givens rotations), k_bd_znorm, k_bd_secular (DLASD4), k_bd_loewner,
+// k_bd_vec (DLASD3's vector formulas), k_bd_gemm (DLASD3's products, gathering
+// columns through the deflation's permutation), k_bd_copy (deflated columns) and
+// k_bd_post (unscaling and DLAMRG, mdlasd1.f90:282-286).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/hsdc.h"
+#include "hsdc_internal.h"
+
+#define BD_EPS 0x1p-53
+#define BD_SAFMIN 0x1p-1022
+#define JAC_TOL 0x1p-52
+#define JAC_SWEEPS 60
+
+// one merge of a level: block [s, s+n), n = nl+nr+1, m = n+sqre; U / V columns are read
+// from buffer rd and written to buffer wr
+struct BdMerge {
+    int s, nl, nr, sqre, rd, wr;
+};
+
+// per-merge results of a level
+struct BdOut {
+    double orgnrm, alpha, beta, tol, z1, zr1, c, sn, rho;
+    int k, nrot, ctot[4];
+};
+
+struct BdLeaf {
+    int s, m, sqre, wr;
+};
+
+struct BdWork {
+    double *D, *E;                          // n
+    int *idxq;                              // n (1-based values, local to each block)
+    double *td, *dsg, *zt, *z, *zs, *zl;    // n
+    double *sig, *tau;                      // n
+    int *org, *ctyp, *idxc, *idx, *idxp;    // n
+    int *surv, *srank, *clf, *gmap, *gsup, *rotf, *ti;  // n
+    int *rpq;                               // 2 n: rotation columns (p | fresh << 30, q)
+    double *rcs;                            // 2 n: rotation c, s
+    unsigned long long *iters;              // secular iteration counter
+};
+
+// ------------------------------------------------------------------------------
+// device helpers
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ double wsum64(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ double tsum_team(double v, int T)
+{
+    for (int off = T >> 1; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ double bd_lapy2(double x, double y)
+{
+    double xa = fabs(x), ya = fabs(y);
+    double w = xa > ya ? xa : ya, z = xa < ya ? xa : ya;
+    if (z == 0.0) return w;
+    double r = z / w;
+    return w * sqrt(1.0 + r * r);
+}
+
+// the multipliers DLASCL applies in turn to scale by cto/cfrom (at most a few)
+__host__ __device__ inline int lascl_muls(double cfrom, double cto, double *mul)
+{
+    const double smlnum = BD_SAFMIN, bignum = 1.0 / smlnum;
+    double cfromc = cfrom, ctoc = cto;
+    int nm = 0;
+    for (;;) {
+        double cfrom1 = cfromc * smlnum, m;
+        bool done = false;
+        if (cfrom1 == cfromc) {
+            m = ctoc / cfromc;
+            done = true;
+        } else {
+            double cto1 = ctoc / bignum;
+            if (cto1 == ctoc) {
+                m = ctoc;
+                done = true;
+                cfromc = 1.0;
+            } else if (fabs(cfrom1) > fabs(ctoc) && ctoc != 0.0) {
+                m = smlnum;
+                cfromc = cfrom1;
+            } else if (fabs(cto1) > fabs(cfromc)) {
+                m = bignum;
+                ctoc = cto1;
+            } else {
+                m = ctoc / cfromc;
+                done = true;
+                if (m == 1.0) return nm;
+            }
+        }
+        if (nm < 8) mul[nm++] = m;
+        if (done) return nm;
+    }
+}
+
+__device__ __forceinline__ double apply_muls(double x, const double *mul, int nm)
+{
+    for (int i = 0; i < nm; i++) x *= mul[i];
+    return x;
+}
+
+// exclusive block scan of one int per thread; returns the total
+template <int NT>
+__device__ int block_exscan(int v, int *sh, int &total)
+{
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int w = 0; w < NT / 64; w++) {
+            int c = sh[w];
+            sh[w] = acc;
+            acc += c;
+        }
+        sh[NT / 64] = acc;
+    }
+    __syncthreads();
+    int r = sh[wv] + x - v;
+    total = sh[NT / 64];
+    __syncthreads();
+    return r;
+}
+
+__device__ double block_max(double v, double *sh)
+{
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    if (t == 0) {
+        double m = sh[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, sh[w]);
+        sh[0] = m;
+    }
+    __syncthreads();
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------------------
+// leaves: one-sided Jacobi, one wavefront per leaf, lane r holds row r of every
+// column (replaces DLASDQ, mdlasd0.f90:152-183)
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdWork W, double *U0,
+                                                double *U1, double *V0, double *V1, long ld, int pmax,
+                                                int *fail)
+{
+    const int li = blockIdx.x;
+    if (li >= nleaf) return;
+    const BdLeaf L = lv[li];
+    const int s = L.s, m = L.m, pc = m + L.sqre, r = threadIdx.x;
+    double *U = L.wr ? U1 : U0, *V = L.wr ? V1 : V0;
+    extern __shared__ double lds[];
+    double *Wc = lds, *Vc = lds + (size_t)pmax * 64, *nn = Vc + (size_t)pmax * 64;
+    int *ord = (int *)(nn + 64);
+    __shared__ int jn_s;
+    const double *d = W.D + s, *e = W.E + s;
+    for (int j = 0; j < pc; j++) {
+        double w = 0.0;
+        if (r == j && j < m) w = d[j];
+        if (j >= 1 && r == j - 1) w = e[j - 1];
+        Wc[j * 64 + r] = w;
+        Vc[j * 64 + r] = (r == j) ? 1.0 : 0.0;
+    }
+    double fro = 0.0;
+    for (int j = 0; j < m; j++) fro = fro + d[j] * d[j];
+    for (int j = 0; j < pc - 1; j++) fro = fro + e[j] * e[j];
+    const double thr = (JAC_TOL * pc) * (JAC_TOL * pc) * fro;
+    __syncthreads();
+    int sweep, rot = 1;
+    for (sweep = 0; sweep < JAC_SWEEPS && rot; sweep++) {
+        rot = 0;
+        for (int p = 0; p < pc - 1; p++) {
+            for (int q = p + 1; q < pc; q++) {
+                double wp = Wc[p * 64 + r], wq = Wc[q * 64 + r];
+                double al = wsum64(wp * wp);
+                double be = wsum64(wq * wq);
+                double ga = wsum64(wp * wq);
+                if (al <= thr || be <= thr || !(fabs(ga) > (JAC_TOL * pc) * (sqrt(al) * sqrt(be)))) continue;
+                double zeta = (be - al) / (2.0 * ga), t;
+                if (fabs(zeta) > 1e150) t = 0.5 / zeta;
+                else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+                Wc[p * 64 + r] = c * wp - sn * wq;
+                Wc[q * 64 + r] = sn * wp + c * wq;
+                double vp = Vc[p * 64 + r], vq = Vc[q * 64 + r];
+                Vc[p * 64 + r] = c * vp - sn * vq;
+                Vc[q * 64 + r] = sn * vp + c * vq;
+                rot = 1;
+            }
+        }
+    }
+    if (rot && r == 0) atomicAdd(fail, 1);
+    for (int j = 0; j < pc; j++) {
+        double w = Wc[j * 64 + r];
+        double v = wsum64(w * w);
+        if (r == 0) nn[j] = v;
+    }
+    __syncthreads();
+    if (r == 0) {
+        int used[64];
+        for (int j = 0; j < pc; j++) used[j] = 0;
+        int jn = -1;
+        if (L.sqre) {
+            jn = 0;
+            for (int j = 1; j < pc; j++)
+                if (sqrt(nn[j]) < sqrt(nn[jn])) jn = j;
+            used[jn] = 1;
+        }
+        for (int k = 0; k < m; k++) {
+            int b = -1;
+            for (int j = 0; j < pc; j++)
+                if (!used[j] && (b < 0 || sqrt(nn[j]) < sqrt(nn[b]))) b = j;
+            used[b] = 1;
+            ord[k] = b;
+        }
+        jn_s = jn;
+    }
+    __syncthreads();
+    for (int k = 0; k < m; k++) {
+        int j = ord[k];
+        double sg = sqrt(nn[j]);
+        if (r == 0) W.D[s + k] = sg;
+        if (r < m) U[(long)(s + k) * ld + s + r] = nn[j] > thr ? Wc[j * 64 + r] / sg : 0.0;
+        if (r < pc) V[(long)(s + k) * ld + s + r] = Vc[j * 64 + r];
+        if (r == 0) W.idxq[s + k] = k + 1;
+    }
+    if (L.sqre && r < pc) V[(long)(s + m) * ld + s + r] = Vc[jn_s * 64 + r];
+    __syncthreads();
+    // left vectors of negligible columns: the first e_r keeping norm > 1/2 after two
+    // projections off the other columns
+    for (int k = 0; k < m; k++) {
+        if (nn[ord[k]] > thr) continue;
+        for (int r0 = 0; r0 < m; r0++) {
+            double x = (r == r0) ? 1.0 : 0.0;
+            for (int pass = 0; pass < 2; pass++)
+                for (int k2 = 0; k2 < m; k2++) {
+                    if (k2 == k) continue;
+                    double uk = r < m ? U[(long)(s + k2) * ld + s + r] : 0.0;
+                    double g = wsum64(r < m ? uk * x : 0.0);
+                    if (r < m) x = x - g * uk;
+                }
+            double nx = sqrt(wsum64(x * x));
+            if (nx > 0.5) {
+                if (r < m) U[(long)(s + k) * ld + s + r] = x / nx;
+                break;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------
+// merge step 1 (block per merge): ORGNRM scaling (mdlasd1.f90:171-184), z from the
+// children's right vectors, the shift, DSIGMA by IDXQ, DLAMRG and the sort
+// (mdlasd2.f90:216-262), TOL (mdlasd2.f90:266-268)
+// ------------------------------------------------------------------------------
+#define PREP_NT 256
+__global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W, BdOut *mo, const double *V0,
+                                                     const double *V1, long ld)
+{
+    __shared__ double shd[PREP_NT / 64 + 1];
+    const BdMerge M = mg[blockIdx.x];
+    const int s = M.s, nl = M.nl, nr = M.nr, n = nl + nr + 1, m = n + M.sqre, t = threadIdx.x;
+    const double *V = M.rd ? V1 : V0;
+    const double alpha0 = W.D[s + nl], beta0 = W.E[s + nl];
+    double mx = 0.0;
+    for (int i = t; i < n; i += PREP_NT)
+        if (i != nl) mx = fmax(mx, fabs(W.D[s + i]));
+    double orgnrm = fmax(fmax(fabs(alpha0), fabs(beta0)), block_max(mx, shd));
+    double mul[8];
+    const int nmul = lascl_muls(orgnrm, 1.0, mul);
+    const double alpha = alpha0 / orgnrm, beta = beta0 / orgnrm;
+    for (int i = t; i < n; i += PREP_NT) W.td[s + i] = apply_muls(i == nl ? 0.0 : W.D[s + i], mul, nmul);
+    __syncthreads();
+    // shift the left part one place, IDXQ adjusted, z, column types
+    for (int i = t; i < n; i += PREP_NT) {
+        double dv;
+        int q;
+        if (i >= 1 && i <= nl) {
+            dv = W.td[s + i - 1];
+            q = W.idxq[s + i - 1] + 1;
+        } else {
+            dv = W.td[s + i];
+            q = (i >= nl + 1) ? W.idxq[s + i] + nl + 1 : W.idxq[s + i];
+        }
+        W.dsg[s + i] = dv;  // staging
+        W.ti[s + i] = q;
+        W.ctyp[s + i] = (i >= 1 && i <= nl) ? 1 : 2;
+    }
+    for (int i = t; i < m; i += PREP_NT) {
+        double zv;
+        if (i == 0) zv = alpha * V[(long)(s + nl) * ld + s + nl];
+        else if (i <= nl) zv = alpha * V[(long)(s + i - 1) * ld + s + nl];
+        else zv = beta * V[(long)(s + i) * ld + s + nl + 1];
+        W.z[s + i] = zv;
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += PREP_NT) {
+        W.D[s + i] = W.dsg[s + i];
+        W.idxq[s + i] = W.ti[s + i];
+    }
+    __syncthreads();
+    // DSIGMA(I) = D(IDXQ(I)), U2(I,1) = Z(IDXQ(I)), IDXC(I) = COLTYP(IDXQ(I))
+    for (int i = t + 1; i < n; i += PREP_NT) {
+        int q = W.idxq[s + i] - 1;
+        W.dsg[s + i] = W.D[s + q];
+        W.zt[s + i] = W.z[s + q];
+        W.idxc[s + i] = W.ctyp[s + q];
+    }
+    __syncthreads();
+    // DLAMRG(NL, NR, DSIGMA(2), 1, 1, IDX(2)): ties take the left run first
+    const double *a = W.dsg + s + 1, *b = W.dsg + s + 1 + nl;
+    for (int i = t; i < nl + nr; i += PREP_NT) {
+        int pos, val;
+        if (i < nl) {
+            double x = a[i];
+            int lo = 0, hi = nr;
+            while (lo < hi) {
+                int md = (lo + hi) >> 1;
+                if (b[md] < x) lo = md + 1; else hi = md;
+            }
+            pos = i + lo;
+            val = i + 1;
+        } else {
+            int j = i - nl;
+            double x = b[j];
+            int lo = 0, hi = nl;
+            while (lo < hi) {
+                int md = (lo + hi) >> 1;
+                if (a[md] <= x) lo = md + 1; else hi = md;
+            }
+            pos = j + lo;
+            val = nl + j + 1;
+        }
+        W.idx[s + 1 + pos] = val;
+    }
+    __syncthreads();
+    for (int i = t + 1; i < n; i += PREP_NT) {
+        int ii = W.idx[s + i];  // 1-based into DSIGMA(2..) -> DSIGMA(1 + ii)
+        W.D[s + i] = W.dsg[s + ii];
+        W.z[s + i] = W.zt[s + ii];
+        W.ctyp[s + i] = W.idxc[s + ii];
+    }
+    __syncthreads();
+    if (t == 0) {
+        double tol = fmax(fabs(alpha), fabs(beta));
+        tol = 8.0 * BD_EPS * fmax(fabs(W.D[s + n - 1]), tol);
+        BdOut &o = mo[blockIdx.x];
+        o.orgnrm = orgnrm;
+        o.alpha = alpha;
+        o.beta = beta;
+        o.tol = tol;
+        o.z1 = W.z[s];
+    }
+}
+
+// column of U (0-based, local) behind sorted position p (mdlasd2.f90:340-345)
+__device__ __forceinline__ int bd_colidx(const BdWork &W, int s, int nl, int p)
+{
+    int v = W.idxq[s + W.idx[s + p]];
+    if (v <= nl + 1) v--;
+    return v - 1;
+}
+
+// ------------------------------------------------------------------------------
+// merge step 2 (block per merge): deflation (mdlasd2.f90:270-470).  The sequential
+// scan is restated in parallel: a position either has small z (deflated at itself)
+// or survives; a survivor whose singular value lies within TOL of the previous
+// survivor's rotates that one out (deflated at the survivor's position); the
+// rotations form chains along runs of such survivors, each chain walked by one
+// thread.  Deflation events fill IDXP from the back in position order.
+// ------------------------------------------------------------------------------
+#define DEF_NT 1024
+__global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork W, BdOut *mo)
+{
+    __shared__ int sh[DEF_NT / 64 + 2];
+    __shared__ int cnt[4];
+    const BdMerge M = mg[blockIdx.x];
+    const int s = M.s, nl = M.nl, nr = M.nr, n = nl + nr + 1, m = n + M.sqre, t = threadIdx.x;
+    BdOut &o = mo[blockIdx.x];
+    const double tol = o.tol;
+    const int np = n - 1;  // positions 1..n-1
+    const int per = (np + DEF_NT - 1) / DEF_NT;
+    const int p0 = 1 + t * per, p1 = min(1 + (t + 1) * per, n);
+    // survivors
+    int c = 0;
+    for (int p = p0; p < p1; p++) c += !(fabs(W.z[s + p]) <= tol);
+    int ns, base = block_exscan<DEF_NT>(c, sh, ns);
+    for (int p = p0; p < p1; p++) {
+        if (!(fabs(W.z[s + p]) <= tol)) {
+            W.surv[s + base] = p;
+            W.srank[s + p] = base;
+            base++;
+        } else {
+            W.srank[s + p] = -1;
+        }
+    }
+    for (int i = t; i < n; i += DEF_NT) W.rotf[s + i] = 0;
+    __syncthreads();
+    const int pt = (ns + DEF_NT - 1) / DEF_NT;
+    const int t0 = t * pt, t1 = min((t + 1) * pt, ns);
+    for (int q = t0; q < t1; q++)
+        W.clf[s + q] = (q >= 1) && (fabs(W.D[s + W.surv[s + q]] - W.D[s + W.surv[s + q - 1]]) <= tol);
+    __syncthreads();
+    // events in position order
+    c = 0;
+    for (int p = p0; p < p1; p++) {
+        int r = W.srank[s + p];
+        c += (r < 0) || W.clf[s + r];
+    }
+    int nev, eb = block_exscan<DEF_NT>(c, sh, nev);
+    for (int p = p0; p < p1; p++) {
+        int r = W.srank[s + p];
+        if (r < 0) {
+            W.idxp[s + n - 1 - eb] = p + 1;
+            W.ctyp[s + p] = 4;
+            eb++;
+        } else if (W.clf[s + r]) {
+            W.idxp[s + n - 1 - eb] = W.surv[s + r - 1] + 1;
+            eb++;
+        }
+    }
+    // rotation ranks and the non-deflated survivors, in survivor order
+    int cr = 0, ck = 0;
+    for (int q = t0; q < t1; q++) {
+        cr += W.clf[s + q];
+        ck += (q == ns - 1) || !W.clf[s + q + 1];
+    }
+    int nrot, rb = block_exscan<DEF_NT>(cr, sh, nrot);
+    int nk, kb = block_exscan<DEF_NT>(ck, sh, nk);
+    for (int q = t0; q < t1; q++) {
+        if (W.clf[s + q]) W.ti[s + q] = rb++;
+        if ((q == ns - 1) || !W.clf[s + q + 1]) {
+            W.idxp[s + 1 + kb] = W.surv[s + q] + 1;
+            W.srank[s + W.surv[s + q]] = -2 - kb;  // marks a non-deflated survivor
+            kb++;
+        }
+    }
+    __syncthreads();
+    // chains of rotations
+    for (int q = t0; q < t1; q++) {
+        if (W.clf[s + q] || q + 1 >= ns || !W.clf[s + q + 1]) continue;
+        double zc = W.z[s + W.surv[s + q]];
+        int ctc = W.ctyp[s + W.surv[s + q]];
+        int q2 = q + 1;
+        for (; q2 < ns && W.clf[s + q2]; q2++) {
+            int pj = W.surv[s + q2 - 1], qj = W.surv[s + q2];
+            double sv = zc, cv = W.z[s + qj];
+            double tau = bd_lapy2(cv, sv);
+            double cc = cv / tau, ss = -sv / tau;
+            W.z[s + pj] = 0.0;
+            zc = tau;
+            int ctq = W.ctyp[s + qj];
+            if (ctq != ctc) ctq = 3;
+            W.ctyp[s + pj] = 4;
+            ctc = ctq;
+            int rk = W.ti[s + q2];
+            int cp = bd_colidx(W, s, nl, pj), cq = bd_colidx(W, s, nl, qj);
+            W.rpq[2 * (s + rk)] = cp | ((q2 == q + 1) << 30);
+            W.rpq[2 * (s + rk) + 1] = cq;
+            W.rcs[2 * (s + rk)] = cc;
+            W.rcs[2 * (s + rk) + 1] = ss;
+            W.rotf[s + cp] = 1;
+            W.rotf[s + cq] = 1;
+        }
+        W.z[s + W.surv[s + q2 - 1]] = zc;
+        W.ctyp[s + W.surv[s + q2 - 1]] = ctc;
+    }
+    __syncthreads();
+    // U2(K,1) = Z(JPREV) of the non-deflated survivors; column-type totals
+    if (t < 4) cnt[t] = 0;
+    __syncthreads();
+    int lc[4] = {0, 0, 0, 0};
+    for (int p = p0; p < p1; p++) {
+        int r = W.srank[s + p];
+        if (r <= -2) W.zt[s + 1 + (-2 - r)] = W.z[s + p];
+        lc[W.ctyp[s + p] - 1]++;
+    }
+    for (int j = 0; j < 4; j++)
+        if (lc[j]) atomicAdd(&cnt[j], lc[j]);
+    __syncthreads();
+    const int k = 1 + nk;
+    const int ct0 = cnt[0], ct1 = cnt[1], ct2 = cnt[2], ct3 = cnt[3];
+    // IDXC: positions J = 2..N grouped by the type of column IDXP(J), stable
+    int psm[4] = {1, 1 + ct0, 1 + ct0 + ct1, 1 + ct0 + ct1 + ct2};  // 0-based starts
+    for (int ty = 0; ty < 4; ty++) {
+        int cc2 = 0;
+        for (int p = p0; p < p1; p++) cc2 += (W.ctyp[s + W.idxp[s + p] - 1] == ty + 1);
+        int tot, b2 = block_exscan<DEF_NT>(cc2, sh, tot);
+        for (int p = p0; p < p1; p++)
+            if (W.ctyp[s + W.idxp[s + p] - 1] == ty + 1) W.idxc[s + psm[ty] + b2++] = p + 1;
+    }
+    __syncthreads();
+    // DSIGMA(J) = D(IDXP(J)); gather map of U2 / VT2 columns and their supports
+    for (int p = p0; p < p1; p++) {
+        W.dsg[s + p] = W.D[s + W.idxp[s + p] - 1];
+        int src = bd_colidx(W, s, nl, W.idxp[s + W.idxc[s + p] - 1] - 1);
+        W.gmap[s + p] = src;
+        W.gsup[s + p] = W.rotf[s + src];
+    }
+    __syncthreads();
+    if (t == 0) {
+        W.dsg[s] = 0.0;
+        const double hlftol = tol / 2.0;
+        if (n > 1 && fabs(W.dsg[s + 1]) <= hlftol) W.dsg[s + 1] = hlftol;
+        double z1 = o.z1, zr1, cq = 1.0, sq = 0.0;
+        if (m > n) {
+            double zm = W.z[s + m - 1];
+            zr1 = bd_lapy2(z1, zm);
+            if (zr1 <= tol) {
+                zr1 = tol;
+            } else {
+                cq = z1 / zr1;
+                sq = zm / zr1;
+            }
+        } else {
+            zr1 = (fabs(z1) <= tol) ? tol : z1;
+        }
+        o.zr1 = zr1;
+        o.c = cq;
+        o.sn = sq;
+        o.k = k;
+        o.nrot = nrot;
+        o.ctot[0] = ct0; o.ctot[1] = ct1; o.ctot[2] = ct2; o.ctot[3] = ct3;
+        W.gmap[s] = -1;
+        W.gsup[s] = 0;
+    }
+    __syncthreads();
+    // Z(1..K) for the secular equation (mdlasd2.f90:497-500)
+    for (int i = t; i < k; i += DEF_NT) W.z[s + i] = (i == 0) ? o.zr1 : W.zt[s + i];
+}
+
+// ------------------------------------------------------------------------------
+// the deflation's Givens rotations (DROT at mdlasd2.f90:346-349) on U and V columns
+// of the read buffer; thread per row, rotations of a merge in order.  A column met
+// for the first time is read within its child's rows (zero elsewhere).
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bd_rotate(const BdMerge *mg, const int *bmerge, const int *brow,
+                                                   int nblk, BdWork W, const BdOut *mo, double *U0, double *U1,
+                                                   double *V0, double *V1, long ld)
+{
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
+    const int mi = bmerge[b];
+    const BdMerge M = mg[mi];
+    const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre;
+    const int r = brow[b] + threadIdx.x;
+    if (r >= m) return;
+    double *U = M.rd ? U1 : U0, *V = M.rd ? V1 : V0;
+    const int nrot = mo[mi].nrot;
+    for (int i = 0; i < nrot; i++) {
+        int pw = W.rpq[2 * (s + i)], q = W.rpq[2 * (s + i) + 1];
+        int p = pw & 0x3fffffff, fresh = pw >> 30;
+        double c = W.rcs[2 * (s + i)], sn = W.rcs[2 * (s + i) + 1];
+        if (r < n) {
+            bool inp = !fresh || (p < nl ? r < nl : r > nl);
+            bool inq = (q < nl ? r < nl : r > nl);
+            double x = inp ? U[(long)(s + p) * ld + s + r] : 0.0;
+            double y = inq ? U[(long)(s + q) * ld + s + r] : 0.0;
+            U[(long)(s + p) * ld + s + r] = c * x + sn * y;
+            U[(long)(s + q) * ld + s + r] = c * y - sn * x;
+        }
+        bool inp = !fresh || (p <= nl ? r <= nl : r > nl);
+        bool inq = (q <= nl ? r <= nl : r > nl);
+        double x = inp ? V[(long)(s + p) * ld + s + r] : 0.0;
+        double y = inq ? V[(long)(s + q) * ld + s + r] : 0.0;
+        V[(long)(s + p) * ld + s + r] = c * x + sn * y;
+        V[(long)(s + q) * ld + s + r] = c * y - sn * x;
+    }
+}
+
+// ------------------------------------------------------------------------------
+// |z| and z / |z| (DLASD3: RHO = DNRM2, DLASCL), one block per merge with K >= 2; the
+// norm in T-lane interleaved sums
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_bd_znorm(const BdMerge *mg, const int *mlist, int nl_, BdWork W, BdOut *mo)
+{
+    const int mi = mlist[blockIdx.x];
+    const int s = mg[mi].s, k = mo[mi].k, T = sec_team(k), l = threadIdx.x;
+    double ps = 0.0;
+    if (l < T)
+        for (int i = l; i < k; i += T) ps += W.z[s + i] * W.z[s + i];
+    double rho = __shfl(sqrt(tsum_team(l < T ? ps : 0.0, T)), 0);
+    double mul[8];
+    const int nm = lascl_muls(rho, 1.0, mul);
+    for (int i = l; i < k; i += 64) {
+        double zv = W.z[s + i];
+        W.zs[s + i] = zv;
+        W.z[s + i] = apply_muls(zv, mul, nm);
+    }
+    if (l == 0) mo[mi].rho = rho * rho;
+}
+
+// ------------------------------------------------------------------------------
+// secular roots (replaces DLASD4): one team of T lanes per root; the poles of the
+// equation in T interleaved partial sums combined by a butterfly
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ double sv_den(double di, double dorg, double tau)
+{
+    return ((di - dorg) - tau) * ((di + dorg) + tau);
+}
+
+__global__ __launch_bounds__(256) void k_bd_secular(const BdMerge *mg, const int *bmerge, const int *broot,
+                                                    int nblk, BdWork W, const BdOut *mo, int maxit, int *fail)
+{
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
+    const int mi = bmerge[b];
+    const int s = mg[mi].s, k = mo[mi].k, T = sec_team(k);
+    const int l = threadIdx.x & (T - 1);
+    const int j = broot[b] + threadIdx.x / T + 1;  // 1-based root
+    if (j > k) return;
+    const double *d = W.dsg + s, *z = W.z + s;
+    const double rho = mo[mi].rho, rhoinv = 1.0 / rho;
+    // starting iterate
+    int ii;
+    double t, lo, hi;
+    if (j == k) {
+        const double dk = d[k - 1];
+        hi = rho / (dk + sqrt(dk * dk + rho));
+        const double mid = (0.5 * rho) / (dk + sqrt(dk * dk + 0.5 * rho));
+        double ps = 0.0;
+        for (int i = l; i < k - 2; i += T) ps += z[i] * z[i] / (((d[i] - dk) - mid) * ((d[i] + dk) + mid));
+        const double c = rhoinv + tsum_team(ps, T);
+        const double dl = (dk - d[k - 2]) * (dk + d[k - 2]);
+        const double z1 = z[k - 2] * z[k - 2], z2 = z[k - 1] * z[k - 1];
+        double t2;
+        if (c <= 0.0) {
+            t2 = rho;
+        } else {
+            double bb = c * dl - z1 - z2;
+            double sq = sqrt(fabs(bb * bb + 4.0 * c * z2 * dl));
+            if (bb <= 0.0) t2 = (sq - bb) / (2.0 * c);
+            else t2 = 2.0 * z2 * dl / (bb + sq);
+        }
+        t = t2 / (dk + sqrt(dk * dk + t2));
+        if (!(t > 0.0 && t <= hi)) t = mid;
+        ii = k;
+        lo = 0.0;
+    } else {
+        const double dj = d[j - 1], dj1 = d[j];
+        const double delsq = (dj1 - dj) * (dj1 + dj);
+        const double sq2 = sqrt((dj * dj + dj1 * dj1) / 2.0);
+        const double tm = (delsq / 2.0) / (dj + sq2);
+        double ps = 0.0;
+        for (int i = l; i < k; i += T) {
+            if (i == j - 1 || i == j) continue;
+            ps += z[i] * z[i] / (((d[i] - dj) - tm) * ((d[i] + dj) + tm));
+        }
+        const double c = rhoinv + tsum_team(ps, T);
+        const double zl = z[j - 1] * z[j - 1], zr = z[j] * z[j];
+        const double fmid = c + (zr - zl) / (delsq / 2.0);
+        if (fmid >= 0.0) {
+            double bb = c * delsq + zl + zr, cc = zl * delsq;
+            double sq = sqrt(fabs(bb * bb - 4.0 * c * cc));
+            double t2 = (bb > 0.0) ? 2.0 * cc / (bb + sq) : (bb - sq) / (2.0 * c);
+            t = t2 / (dj + sqrt(dj * dj + t2));
+            if (!(t > 0.0 && t <= tm)) t = 0.5 * tm;
+            ii = j;
+            lo = 0.0;
+            hi = tm;
+        } else {
+            double bb = c * delsq - zl - zr, cc = zr * delsq;
+            double sq = sqrt(fabs(bb * bb + 4.0 * c * cc));
+            double t2 = (bb <= 0.0) ? -2.0 * cc / (sq - bb) : -(bb + sq) / (2.0 * c);
+            lo = -(delsq / 2.0) / (dj1 + sq2);
+            t = t2 / (dj1 + sqrt(fabs(dj1 * dj1 + t2)));
+            if (!(t >= lo && t < 0.0)) t = 0.5 * lo;
+            ii = j + 1;
+            hi = 0.0;
+        }
+    }
+    const double dorg = d[ii - 1], dorg2 = dorg + dorg;
+    const int split = (j == k) ? k - 1 : j;
+    const int na = split - ((ii - 1 < split) ? 1 : 0), nb = (k - split) - ((ii - 1 >= split) ? 1 : 0);
+    int lgT = 0;
+    while ((1 << lgT) < T) lgT++;
+    int it, info = 1;
+    for (it = 0; it <= maxit; it++) {
+        double psi = 0.0, dpsi = 0.0, epsi = 0.0, phi = 0.0, dphi = 0.0, ephi = 0.0;
+        for (int t2 = l; t2 < na; t2 += T) {
+            double dn = ((d[t2] - dorg) - t) * ((d[t2] + dorg) + t);
+            double q = z[t2] / dn;
+            psi += z[t2] * q;
+            dpsi += q * q;
+            epsi += psi;
+        }
+        for (int t2 = l; t2 < nb; t2 += T) {
+            int i = k - 1 - t2;
+            double dn = ((d[i] - dorg) - t) * ((d[i] + dorg) + t);
+            double q = z[i] / dn;
+            phi += z[i] * q;
+            dphi += q * q;
+            ephi += phi;
+        }
+        psi = tsum_team(psi, T);
+        dpsi = tsum_team(dpsi, T);
+        epsi = tsum_team(epsi, T);
+        phi = tsum_team(phi, T);
+        dphi = tsum_team(dphi, T);
+        ephi = tsum_team(ephi, T);
+        const double mu = t * (dorg2 + t);
+        const double qo = z[ii - 1] / (-mu);
+        const double torg = z[ii - 1] * qo;
+        const double w = ((rhoinv + phi) + psi) + torg;
+        const double erretm = (8.0 + lgT) * (phi - psi) + (ephi - epsi) + 2.0 * rhoinv + 3.0 * fabs(torg) +
+                              fabs(mu) * ((dpsi + dphi) + qo * qo);
+        if (ii - 1 < split) dpsi += qo * qo; else dphi += qo * qo;
+        if (fabs(w) <= BD_EPS * erretm || it == maxit) {
+            if (fabs(w) <= BD_EPS * erretm) info = 0;
+            break;
+        }
+        if (w > 0.0) hi = t; else lo = t;
+        const double dl = ((d[split - 1] - dorg) - t) * ((d[split - 1] + dorg) + t);
+        const double dr = ((d[split] - dorg) - t) * ((d[split] + dorg) + t);
+        const double a = (dl + dr) * w - dl * dr * (dpsi + dphi);
+        const double bq = dl * dr * w;
+        const double c = w - dl * dpsi - dr * dphi;
+        double eta;
+        if (c == 0.0) {
+            eta = (a != 0.0) ? bq / a : -w / (dpsi + dphi);
+        } else {
+            double sq = sqrt(fabs(a * a - 4.0 * bq * c));
+            if (a <= 0.0) eta = (a - sq) / (2.0 * c);
+            else eta = 2.0 * bq / (a + sq);
+        }
+        if (w * eta >= 0.0) eta = -w / (dpsi + dphi);
+        const double sg = dorg + t;
+        double tn = t + eta / (sg + sqrt(fabs(sg * sg + eta)));
+        if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+        if (tn == t) { info = 0; break; }
+        t = tn;
+    }
+    if (l == 0) {
+        W.sig[s + j - 1] = dorg + t;
+        W.org[s + j - 1] = ii;
+        W.tau[s + j - 1] = t;
+        atomicAdd(W.iters, (unsigned long long)(it + 1));
+        if (info) atomicAdd(fail, 1);
+    }
+}
+
+// ------------------------------------------------------------------------------
+// updated z (DLASD3, Loewner form): thread per index, products in DLASD3's order
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bd_loewner(const BdMerge *mg, const int *bmerge, const int *bfirst,
+                                                    int nblk, BdWork W, const BdOut *mo)
+{
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
+    const int mi = bmerge[b];
+    const int s = mg[mi].s, k = mo[mi].k;
+    const int i = bfirst[b] + threadIdx.x + 1;  // 1-based
+    if (i > k) return;
+    const double *ds = W.dsg + s;
+    const double di = ds[i - 1];
+    double zi = sv_den(di, ds[W.org[s + k - 1] - 1], W.tau[s + k - 1]);
+    for (int j = 1; j <= i - 1; j++)
+        zi = zi * (sv_den(di, ds[W.org[s + j - 1] - 1], W.tau[s + j - 1]) / (di - ds[j - 1]) / (di + ds[j - 1]));
+    for (int j = i; j <= k - 1; j++)
+        zi = zi * (sv_den(di, ds[W.org[s + j - 1] - 1], W.tau[s + j - 1]) / (di - ds[j]) / (di + ds[j]));
+    double a = sqrt(fabs(zi));
+    W.zl[s + i - 1] = W.zs[s + i - 1] >= 0.0 ? a : -a;
+}
+
+// ------------------------------------------------------------------------------
+// singular vectors of the deflated problem (DLASD3): team of T lanes per root i;
+// column i of Bu = left vector (rows in IDXC order), column i of Bv = row i of the
+// right-vector matrix
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bd_vec(const BdMerge *mg, const int *bmerge, const int *broot, int nblk,
+                                                BdWork W, const BdOut *mo, const long *qoff, double *Q)
+{
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
+    const int mi = bmerge[b];
+    const int s = mg[mi].s, k = mo[mi].k, T = sec_team(k);
+    const int l = threadIdx.x & (T - 1);
+    const int i = broot[b] + threadIdx.x / T + 1;
+    if (i > k) return;
+    const double *ds = W.dsg + s, *zl = W.zl + s;
+    const int *idxc = W.idxc + s;
+    const double dorg = ds[W.org[s + i - 1] - 1], ti = W.tau[s + i - 1];
+    double *bu = Q + qoff[mi] + (long)(i - 1) * k;
+    double *bv = Q + qoff[mi] + (long)k * k + (long)(i - 1) * k;
+    double pu = 0.0, pv = 0.0;
+    for (int j = l; j < k; j += T) {
+        double dj = ds[j];
+        double wv = zl[j] / ((dj - dorg) - ti) / ((dj + dorg) + ti);
+        double x = j == 0 ? -1.0 : dj * wv;
+        pu += x * x;
+        pv += wv * wv;
+    }
+    const double tu = sqrt(tsum_team(pu, T)), tv = sqrt(tsum_team(pv, T));
+    for (int j = l; j < k; j += T) {
+        if (j == 0) {
+            bu[0] = -1.0 / tu;
+            double d0 = ds[0];
+            double w0 = zl[0] / ((d0 - dorg) - ti) / ((d0 + dorg) + ti);
+            bv[0] = w0 / tv;
+        } else {
+            int jc = idxc[j] - 1;
+            double dj = ds[jc];
+            double wv = zl[jc] / ((dj - dorg) - ti) / ((dj + dorg) + ti);
+            bu[j] = dj * wv / tu;
+            bv[j] = wv / tv;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------
+// U2 / VT2 column J (0-based) of a merge at local row r: the gathered child column
+// inside its support, the special first columns (mdlasd2.f90:502-518)
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ double bd_acol(const BdWork &W, const BdMerge &M, const BdOut &o, bool isV,
+                                          const double *X, long ld, int J, int r)
+{
+    const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre;
+    if (J == 0) {
+        if (!isV) return r == nl ? 1.0 : 0.0;
+        if (m > n) return r <= nl ? o.c * X[(long)(s + nl) * ld + s + r] : o.sn * X[(long)(s + m - 1) * ld + s + r];
+        return r <= nl ? X[(long)(s + nl) * ld + s + r] : 0.0;
+    }
+    const int src = W.gmap[s + J];
+    bool in = W.gsup[s + J] || (isV ? (src <= nl ? r <= nl : r > nl) : (src < nl ? r < nl : r > nl));
+    return in ? X[(long)(s + src) * ld + s + r] : 0.0;
+}
+
+// ------------------------------------------------------------------------------
+// DLASD3's products U(:,1:K) = U2 Bu, V(:,1:K) = VT2^T Bv: 64 x 64 output tiles, 16
+// inner indices staged per step, each output summed over the inner index in
+// ascending order
+// ------------------------------------------------------------------------------
+struct BdTile {
+    int merge, isv, row0, col0;
+};
+
+__global__ __launch_bounds__(256) void k_bd_gemm(const BdMerge *mg, const BdTile *tiles, int ntile, BdWork W,
+                                                 const BdOut *mo, const long *qoff, const double *Q,
+                                                 double *U0, double *U1, double *V0, double *V1, long ld)
+{
+    __shared__ double As[16][65];
+    __shared__ double Bs[16][65];
+    const int bi = blockIdx.x;
+    if (bi >= ntile) return;
+    const BdTile tl = tiles[bi];
+    const BdMerge M = mg[tl.merge];
+    const BdOut &o = mo[tl.merge];
+    const bool isV = tl.isv;
+    const int s = M.s, n = M.nl + M.nr + 1, rows = isV ? n + M.sqre : n, k = o.k;
+    const double *X = isV ? (M.rd ? V1 : V0) : (M.rd ? U1 : U0);
+    double *Y = isV ? (M.wr ? V1 : V0) : (M.wr ? U1 : U0);
+    const double *B = Q + qoff[tl.merge] + (isV ? (long)k * k : 0);
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int c = 0; c < 4; c++) acc[a][c] = 0.0;
+    for (int j0 = 0; j0 < k; j0 += 16) {
+        for (int e = threadIdx.x; e < 1024; e += 256) {
+            int rr = e & 63, jj = e >> 6;
+            int r = tl.row0 + rr, J = j0 + jj;
+            As[jj][rr] = (r < rows && J < k) ? bd_acol(W, M, o, isV, X, ld, J, r) : 0.0;
+            int cc = e & 63, j2 = e >> 6;
+            int col = tl.col0 + cc, J2 = j0 + j2;
+            Bs[j2][cc] = (col < k && J2 < k) ? B[(long)col * k + J2] : 0.0;
+        }
+        __syncthreads();
+        const int jn = min(16, k - j0);
+        for (int jj = 0; jj < jn; jj++) {
+            double av[4], bv[4];
+            for (int a = 0; a < 4; a++) av[a] = As[jj][tx + 16 * a];
+            for (int c = 0; c < 4; c++) bv[c] = Bs[jj][ty + 16 * c];
+            for (int a = 0; a < 4; a++)
+                for (int c = 0; c < 4; c++) acc[a][c] = acc[a][c] + av[a] * bv[c];
+        }
+        __syncthreads();
+    }
+    for (int a = 0; a < 4; a++) {
+        int r = tl.row0 + tx + 16 * a;
+        if (r >= rows) continue;
+        for (int c = 0; c < 4; c++) {
+            int col = tl.col0 + ty + 16 * c;
+            if (col < k) Y[(long)(s + col) * ld + s + r] = acc[a][c];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------
+// columns that are copied, not multiplied: the deflated ones (mdlasd2.f90:520-528),
+// the last row of VT when M > N (mdlasd2.f90:504-512), and U(:,1) / VT(1,:) when
+// K = 1 (DLASD3's quick return)
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bd_copy(const BdMerge *mg, const int *cmerge, const int *ccol, int ncol,
+                                                 BdWork W, const BdOut *mo, double *U0, double *U1, double *V0,
+                                                 double *V1, long ld)
+{
+    const int b = blockIdx.x;
+    if (b >= ncol) return;
+    const int mi = cmerge[b], J = ccol[b];
+    const BdMerge M = mg[mi];
+    const BdOut &o = mo[mi];
+    const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre;
+    const double *XU = M.rd ? U1 : U0, *XV = M.rd ? V1 : V0;
+    double *YU = M.wr ? U1 : U0, *YV = M.wr ? V1 : V0;
+    for (int r = threadIdx.x; r < m; r += 256) {
+        if (J == m - 1 && m > n) {
+            // VT(M,:) after the rotation that folds it into VT2(1,:)
+            double v = r <= nl ? -o.sn * XV[(long)(s + nl) * ld + s + r] : o.c * XV[(long)(s + m - 1) * ld + s + r];
+            YV[(long)(s + m - 1) * ld + s + r] = v;
+            continue;
+        }
+        if (J == 0) {
+            // K = 1
+            if (r < n) {
+                double u = bd_acol(W, M, o, false, XU, ld, 0, r);
+                YU[(long)s * ld + s + r] = o.zr1 > 0.0 ? u : -u;
+            }
+            YV[(long)s * ld + s + r] = bd_acol(W, M, o, true, XV, ld, 0, r);
+            continue;
+        }
+        if (r < n) YU[(long)(s + J) * ld + s + r] = bd_acol(W, M, o, false, XU, ld, J, r);
+        YV[(long)(s + J) * ld + s + r] = bd_acol(W, M, o, true, XV, ld, J, r);
+    }
+}
+
+// ------------------------------------------------------------------------------
+// merge step 4 (block per merge): D(1:K) = roots, D(K+1:N) = deflated values, both
+// unscaled, IDXQ = DLAMRG(K, N-K, D, 1, -1) (mdlasd1.f90:282-286)
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bd_post(const BdMerge *mg, BdWork W, const BdOut *mo)
+{
+    const BdMerge M = mg[blockIdx.x];
+    const BdOut &o = mo[blockIdx.x];
+    const int s = M.s, n = M.nl + M.nr + 1, k = o.k, t = threadIdx.x;
+    double mul[8];
+    const int nm = lascl_muls(1.0, o.orgnrm, mul);
+    for (int i = t; i < n; i += 256) {
+        double v;
+        if (i < k) v = (k == 1) ? fabs(o.zr1) : W.sig[s + i];
+        else v = W.dsg[s + i];
+        W.D[s + i] = apply_muls(v, mul, nm);
+    }
+    __syncthreads();
+    const double *a = W.D + s;
+    const int n2 = n - k;
+    for (int i = t; i < n; i += 256) {
+        int pos, val;
+        if (i < k) {
+            double x = a[i];
+            int lo = 0, hi = n2;  // run 2 ascending: b_j = a[n-1-j]
+            while (lo < hi) {
+                int md = (lo + hi) >> 1;
+                if (a[n - 1 - md] < x) lo = md + 1; else hi = md;
+            }
+            pos = i + lo;
+            val = i + 1;
+        } else {
+            int j = i - k;
+            double x = a[n - 1 - j];
+            int lo = 0, hi = k;
+            while (lo < hi) {
+                int md = (lo + hi) >> 1;
+                if (a[md] <= x) lo = md + 1; else hi = md;
+            }
+            pos = j + lo;
+            val = n - j;
+        }
+        W.ti[s + pos] = val;
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += 256) W.idxq[s + i] = W.ti[s + i];
+}
+
+// ------------------------------------------------------------------------------
+// output: U(:,i) = Ufin(:,perm(i)), VT(i,:) = Vfin(:,perm(i))^T, rows outside the
+// column's subproblem zero (64 x 64 tiles, the VT write transposed through LDS)
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bd_out(const double *Uf, const double *Vf, long ld, const int *perm,
+                                                const int *plo, const int *phi, int n, double *U, long ldu,
+                                                double *VT, long ldvt)
+{
+    __shared__ double tile[64][65];
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int cc = ty; cc < 64; cc += 4) {
+        int i = c0 + cc, r = r0 + tx;
+        if (i >= n || r >= n) continue;
+        int src = perm[i];
+        bool in = r >= plo[src] && r < phi[src];
+        if (U) U[(long)i * ldu + r] = in ? Uf[(long)src * ld + r] : 0.0;
+        tile[cc][tx] = in ? Vf[(long)src * ld + r] : 0.0;
+    }
+    __syncthreads();
+    if (!VT) return;
+    // VT(i, r) for i = c0 + tx (contiguous in i), r = r0 + rr
+    for (int rr = ty; rr < 64; rr += 4) {
+        int i = c0 + tx, r = r0 + rr;
+        if (i >= n || r >= n) continue;
+        VT[(long)r * ldvt + i] = tile[tx][rr];
+    }
+}
+
+// DLASR('L','V','B') on the columns of U: the rotations that made a lower bidiagonal
+// upper (mdbdsdc.f90:413-414); thread per column
+__global__ __launch_bounds__(256) void k_bd_lasr(double *U, long ldu, int n, const double *cs, const double *sn)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    double *u = U + (long)c * ldu;
+    for (int j = n - 2; j >= 0; j--) {
+        double ct = cs[j], st = sn[j];
+        double t = u[j + 1];
+        u[j + 1] = ct * t - st * u[j];
+        u[j] = st * t + ct * u[j];
+    }
+}
+
+// ------------------------------------------------------------------------------
+// host driver
+// ------------------------------------------------------------------------------
+namespace {
+
+struct Buf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+int bensure(Buf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return 0;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return HSDC_ERR_NOMEM;
+    }
+    b.cap = bytes;
+    return 0;
+}
+
+inline int bcdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// LAPACK DLASDT: the tree of one subproblem (node arrays 0-based, values 1-based)
+void lasdt(int n, int &lvl, int &nd, std::vector<int> &inode, std::vector<int> &ndiml, std::vector<int> &ndimr,
+           int msub)
+{
+    int maxn = n > 1 ? n : 1;
+    double temp = std::log((double)maxn / (double)(msub + 1)) / std::log(2.0);
+    lvl = (int)temp + 1;
+    inode.assign(n, 0);
+    ndiml.assign(n, 0);
+    ndimr.assign(n, 0);
+    int i = n / 2;
+    inode[0] = i + 1;
+    ndiml[0] = i;
+    ndimr[0] = n - i - 1;
+    int il = -1, ir = 0, llst = 1;
+    for (int l = 1; l <= lvl - 1; l++) {
+        for (i = 0; i < llst; i++) {
+            il += 2;
+            ir += 2;
+            int nc = llst + i - 1;
+            ndiml[il] = ndiml[nc] / 2;
+            ndimr[il] = ndiml[nc] - ndiml[il] - 1;
+            inode[il] = inode[nc] - ndimr[il] - 1;
+            ndiml[ir] = ndimr[nc] / 2;
+            ndimr[ir] = ndimr[nc] - ndiml[ir] - 1;
+            inode[ir] = inode[nc] + ndiml[ir] + 1;
+        }
+        llst *= 2;
+    }
+    nd = llst * 2 - 1;
+}
+
+// LAPACK DLARTG (3.10 form)
+void lartg(double f, double g, double &c, double &s, double &r)
+{
+    const double rtmin = std::sqrt(BD_SAFMIN), rtmax = std::sqrt(1.0 / BD_SAFMIN / 2.0);
+    double f1 = std::fabs(f), g1 = std::fabs(g);
+    if (g == 0.0) {
+        c = 1.0; s = 0.0; r = f;
+    } else if (f == 0.0) {
+        c = 0.0; s = g > 0.0 ? 1.0 : -1.0; r = g1;
+    } else if (f1 > rtmin && f1 < rtmax && g1 > rtmin && g1 < rtmax) {
+        double d = std::sqrt(f * f + g * g);
+        c = f1 / d;
+        r = f > 0.0 ? d : -d;
+        s = g / r;
+    } else {
+        double u = std::fmin(1.0 / BD_SAFMIN, std::fmax(BD_SAFMIN, std::fmax(f1, g1)));
+        double fs = f / u, gs = g / u;
+        double d = std::sqrt(fs * fs + gs * gs);
+        c = std::fabs(fs) / d;
+        r = fs > 0.0 ? d : -d;
+        s = gs / r;
+        r *= u;
+    }
+}
+
+void host_lascl(double cfrom, double cto, int n, double *x)
+{
+    double mul[8];
+    int nm = lascl_muls(cfrom, cto, mul);
+    for (int s = 0; s < nm; s++)
+        for (int i = 0; i < n; i++) x[i] *= mul[s];
+}
+
+}  // namespace
+
+struct hsdc_bd_ctx {
+    hsdc_bd_params prm;
+    hsdc_bd_stats st;
+    hipStream_t stream = 0;
+    Buf ub[2], vb[2], wd, wi, mgb, mob, tab, qb, misc, din, ein, uo, vto, rot;
+    BdWork W;
+    long wcap = 0;
+    BdOut *hmo = nullptr;  // pinned
+    size_t hmo_cap = 0;
+    std::vector<int> tr;   // per merge of the last solve: block start, order, K, rotations
+};
+
+#define BCK(x) do { int _r = (x); if (_r) return _r; } while (0)
+#define BHK(x) do { if ((x) != hipSuccess) { (void)hipGetLastError(); return HSDC_ERR_HIP; } } while (0)
+
+extern "C" {
+
+void hsdc_bd_default_params(hsdc_bd_params *p)
+{
+    p->smlsiz = 25;
+    p->maxit = 100;
+}
+
+int hsdc_bd_ctx_create(hsdc_bd_ctx **ctx)
+{
+    if (!ctx) return HSDC_ERR_INTERNAL;
+    hsdc_bd_ctx *c = new hsdc_bd_ctx();
+    hsdc_bd_default_params(&c->prm);
+    std::memset(&c->st, 0, sizeof c->st);
+    *ctx = c;
+    return 0;
+}
+
+int hsdc_bd_ctx_destroy(hsdc_bd_ctx *c)
+{
+    if (!c) return 0;
+    Buf *all[] = {&c->ub[0], &c->ub[1], &c->vb[0], &c->vb[1], &c->wd, &c->wi, &c->mgb, &c->mob,
+                  &c->tab, &c->qb, &c->misc, &c->din, &c->ein, &c->uo, &c->vto, &c->rot};
+    for (Buf *b : all)
+        if (b->p) (void)hipFree(b->p);
+    if (c->hmo) (void)hipHostFree(c->hmo);
+    delete c;
+    return 0;
+}
+
+int hsdc_bd_set_params(hsdc_bd_ctx *c, const hsdc_bd_params *p)
+{
+    if (!c || !p) return HSDC_ERR_INTERNAL;
+    if (p->smlsiz < 3 || p->smlsiz > 63) return -2;
+    if (p->maxit < 1) return -3;
+    c->prm = *p;
+    return 0;
+}
+
+int hsdc_bd_get_stats(hsdc_bd_ctx *c, hsdc_bd_stats *s)
+{
+    if (!c || !s) return HSDC_ERR_INTERNAL;
+    *s = c->st;
+    return 0;
+}
+
+int hsdc_bd_set_stream(hsdc_bd_ctx *c, void *stream)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    c->stream = (hipStream_t)stream;
+    return 0;
+}
+
+int hsdc_bd_trace(hsdc_bd_ctx *c, int *s, int *n, int *k, int *nrot, int cap)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    const int cnt = (int)c->tr.size() / 4;
+    for (int i = 0; i < cnt && i < cap; i++) {
+        if (s) s[i] = c->tr[4 * i];
+        if (n) n[i] = c->tr[4 * i + 1];
+        if (k) k[i] = c->tr[4 * i + 2];
+        if (nrot) nrot[i] = c->tr[4 * i + 3];
+    }
+    return cnt;
+}
+
+}  // extern "C"
+
+static int bd_alloc(hsdc_bd_ctx *c, int n)
+{
+    if (n <= c->wcap) return 0;
+    const size_t nd = 11, ni = 18;
+    BCK(bensure(c->wd, sizeof(double) * (nd * (size_t)n + 2 * (size_t)n)));
+    BCK(bensure(c->wi, sizeof(int) * (ni * (size_t)n + 2 * (size_t)n) + 64));
+    double *D = (double *)c->wd.p;
+    int *I = (int *)c->wi.p;
+    size_t N = n;
+    BdWork &W = c->W;
+    W.D = D; W.E = D + N; W.td = D + 2 * N; W.dsg = D + 3 * N; W.zt = D + 4 * N; W.z = D + 5 * N;
+    W.zs = D + 6 * N; W.zl = D + 7 * N; W.sig = D + 8 * N; W.tau = D + 9 * N; W.rcs = D + 11 * N;
+    W.idxq = I; W.org = I + N; W.ctyp = I + 2 * N; W.idxc = I + 3 * N; W.idx = I + 4 * N;
+    W.idxp = I + 5 * N; W.surv = I + 6 * N; W.srank = I + 7 * N; W.clf = I + 8 * N; W.gmap = I + 9 * N;
+    W.gsup = I + 10 * N; W.rotf = I + 11 * N; W.ti = I + 12 * N; W.rpq = I + 13 * N;
+    W.iters = (unsigned long long *)(I + 18 * N + 2 * N);
+    c->wcap = n;
+    return 0;
+}
+
+// the whole SVD of the upper bidiagonal already on the device (W.D, W.E, scaled when
+// n > smlsiz); outputs U, VT (device, may be null for values only)
+static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const std::vector<double> &he,
+                    double orgnrm, double *dU, long ldu, double *dVT, long ldvt, double *dD,
+                    const double *rc, const double *rs)
+{
+    hipStream_t st = c->stream;
+    const int smlsiz = c->prm.smlsiz;
+    const long ld = n;
+    std::memset(&c->st, 0, sizeof c->st);
+    c->tr.clear();
+    hipEvent_t e0, e1;
+    BHK(hipEventCreate(&e0));
+    BHK(hipEventCreate(&e1));
+    BHK(hipEventRecord(e0, st));
+    for (int b = 0; b < 2; b++) {
+        BCK(bensure(c->ub[b], sizeof(double) * (size_t)n * n));
+        BCK(bensure(c->vb[b], sizeof(double) * (size_t)n * n));
+    }
+    double *U0 = (double *)c->ub[0].p, *U1 = (double *)c->ub[1].p;
+    double *V0 = (double *)c->vb[0].p, *V1 = (double *)c->vb[1].p;
+    BdWork &W = c->W;
+    // subproblems (mdbdsdc.f90:333-372) and their trees (mdlasd0.f90:166-221)
+    std::vector<BdLeaf> leaves;
+    std::vector<std::vector<BdMerge>> lv;
+    std::vector<int> plo(n), phi(n);
+    auto add_problem = [&](int start, int size) {
+        for (int i = start; i < start + size; i++) { plo[i] = start; phi[i] = start + size; }
+        if (size <= smlsiz) {
+            leaves.push_back({start, size, 0, 0});
+            return;
+        }
+        int nlvl, nd;
+        std::vector<int> inode, ndiml, ndimr;
+        lasdt(size, nlvl, nd, inode, ndiml, ndimr, smlsiz);
+        const int wr0 = nlvl & 1;  // leaves land where the last merge level ends in buffer 0
+        const int ndb1 = (nd + 1) / 2;
+        for (int i = ndb1; i <= nd; i++) {
+            int ic = inode[i - 1], nl = ndiml[i - 1], nr = ndimr[i - 1];
+            int nlf = ic - nl, nrf = ic + 1;
+            leaves.push_back({start + nlf - 1, nl, 1, wr0});
+            leaves.push_back({start + nrf - 1, nr, (i == nd) ? 0 : 1, wr0});
+        }
+        if ((int)lv.size() < nlvl) lv.resize(nlvl);
+        for (int lvl = nlvl; lvl >= 1; lvl--) {
+            int lf = (lvl == 1) ? 1 : 1 << (lvl - 1), ll = (lvl == 1) ? 1 : 2 * lf - 1;
+            int t = nlvl - lvl;  // merge step of this problem
+            for (int i = lf; i <= ll; i++) {
+                int ic = inode[i - 1], nl = ndiml[i - 1], nr = ndimr[i - 1];
+                int nlf = ic - nl;
+                int sq = (i == ll) ? 0 : 1;
+                int rd = (wr0 + t) & 1, wr = (wr0 + t + 1) & 1;
+                lv[t].push_back({start + nlf - 1, nl, nr, sq, rd, wr});
+            }
+        }
+    };
+    if (n <= smlsiz) {
+        add_problem(0, n);
+    } else {
+        const double eps = 0.9 * BD_EPS;
+        int start = 1;
+        const int nm1 = n - 1;
+        for (int i = 1; i <= nm1; i++) {
+            if (std::fabs(he[i - 1]) < eps || i == nm1) {
+                int nsize;
+                if (i < nm1) nsize = i - start + 1;
+                else if (std::fabs(he[i - 1]) >= eps) nsize = n - start + 1;
+                else {
+                    nsize = i - start + 1;
+                    add_problem(n - 1, 1);
+                }
+                add_problem(start - 1, nsize);
+                start = i + 1;
+            }
+        }
+    }
+    // device tables: leaves, merges of every step
+    std::vector<int> moff(lv.size() + 1, 0);
+    for (size_t t = 0; t < lv.size(); t++) moff[t + 1] = moff[t] + (int)lv[t].size();
+    const int nmg = moff.back();
+    BCK(bensure(c->mgb, sizeof(BdMerge) * (nmg + 1) + sizeof(BdLeaf) * (leaves.size() + 1) + 64));
+    BdMerge *dmg = (BdMerge *)c->mgb.p;
+    BdLeaf *dlv = (BdLeaf *)(dmg + nmg + 1);
+    {
+        std::vector<BdMerge> all;
+        for (auto &v : lv) all.insert(all.end(), v.begin(), v.end());
+        if (nmg) BHK(hipMemcpyAsync(dmg, all.data(), sizeof(BdMerge) * nmg, hipMemcpyHostToDevice, st));
+        BHK(hipMemcpyAsync(dlv, leaves.data(), sizeof(BdLeaf) * leaves.size(), hipMemcpyHostToDevice, st));
+    }
+    int maxm = 1;
+    for (auto &v : lv) for (auto &M : v) maxm = std::max(maxm, M.nl + M.nr + 1);
+    BCK(bensure(c->mob, sizeof(BdOut) * (nmg + 1)));
+    BdOut *dmo = (BdOut *)c->mob.p;
+    if (c->hmo_cap < (size_t)(nmg + 1)) {
+        if (c->hmo) (void)hipHostFree(c->hmo);
+        c->hmo = nullptr;
+        if (hipHostMalloc((void **)&c->hmo, sizeof(BdOut) * (nmg + 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            c->hmo_cap = 0;
+            return HSDC_ERR_NOMEM;
+        }
+        c->hmo_cap = nmg + 1;
+    }
+    int *dfail = (int *)(W.iters + 1);
+    BHK(hipMemsetAsync(W.iters, 0, 16, st));
+    // leaves
+    int pmax = 1;
+    for (auto &L : leaves) pmax = std::max(pmax, L.m + L.sqre);
+    size_t lds = sizeof(double) * (2 * (size_t)pmax * 64 + 64) + sizeof(int) * 64;
+    hipLaunchKernelGGL(k_bd_leaf, dim3((unsigned)leaves.size()), dim3(64), lds, st, dlv, (int)leaves.size(), W, U0,
+                       U1, V0, V1, ld, pmax, dfail);
+    BHK(hipGetLastError());
+    long long roots = 0, deflated = 0, rots = 0;
+    int ktop = 0;
+    for (size_t t = 0; t < lv.size(); t++) {
+        const int nm = (int)lv[t].size();
+        if (!nm) continue;
+        BdMerge *mg = dmg + moff[t];
+        BdOut *mo = dmo + moff[t];
+        hipLaunchKernelGGL(k_bd_prep, dim3(nm), dim3(PREP_NT), 0, st, mg, W, mo, V0, V1, ld);
+        hipLaunchKernelGGL(k_bd_deflate, dim3(nm), dim3(DEF_NT), 0, st, mg, W, mo);
+        BHK(hipGetLastError());
+        BHK(hipMemcpyAsync(c->hmo, mo, sizeof(BdOut) * nm, hipMemcpyDeviceToHost, st));
+        BHK(hipStreamSynchronize(st));
+        // launch tables of the level
+        std::vector<int> rb_m, rb_r, sb_m, sb_r, lb_m, lb_r, zl_m, cp_m, cp_c;
+        std::vector<long> qo(nm, 0);
+        std::vector<BdTile> tiles;
+        long qtot = 0;
+        for (int i = 0; i < nm; i++) {
+            const BdMerge &M = lv[t][i];
+            const BdOut &o = c->hmo[i];
+            const int n_ = M.nl + M.nr + 1, m_ = n_ + M.sqre, k = o.k;
+            if (k < 1 || k > n_) return HSDC_ERR_INTERNAL;
+            c->tr.insert(c->tr.end(), {M.s, n_, k, o.nrot});
+            roots += k;
+            deflated += n_ - k;
+            rots += o.nrot;
+            ktop = k;
+            if (o.nrot)
+                for (int r0 = 0; r0 < m_; r0 += 256) { rb_m.push_back(i); rb_r.push_back(r0); }
+            if (k >= 2) {
+                zl_m.push_back(i);
+                int T = sec_team(k), per = 256 / T;
+                for (int j0 = 0; j0 < k; j0 += per) { sb_m.push_back(i); sb_r.push_back(j0); }
+                for (int j0 = 0; j0 < k; j0 += 256) { lb_m.push_back(i); lb_r.push_back(j0); }
+                qo[i] = qtot;
+                qtot += 2 * (long)k * k;
+                for (int c0 = 0; c0 < k; c0 += 64) {
+                    for (int r0 = 0; r0 < n_; r0 += 64) tiles.push_back({i, 0, r0, c0});
+                    for (int r0 = 0; r0 < m_; r0 += 64) tiles.push_back({i, 1, r0, c0});
+                }
+            }
+            for (int J = (k == 1 ? 0 : k); J < n_; J++) {
+                if (J == 0 || J >= k) { cp_m.push_back(i); cp_c.push_back(J); }
+            }
+            if (m_ > n_) { cp_m.push_back(i); cp_c.push_back(m_ - 1); }
+        }
+        // pack the tables into one upload
+        std::vector<char> pk;
+        auto put = [&](const void *p, size_t bytes) {
+            size_t off = (pk.size() + 63) & ~(size_t)63;
+            pk.resize(off + bytes + 8);
+            if (bytes) std::memcpy(pk.data() + off, p, bytes);
+            return off;
+        };
+        size_t o_rbm = put(rb_m.data(), 4 * rb_m.size()), o_rbr = put(rb_r.data(), 4 * rb_r.size());
+        size_t o_sbm = put(sb_m.data(), 4 * sb_m.size()), o_sbr = put(sb_r.data(), 4 * sb_r.size());
+        size_t o_lbm = put(lb_m.data(), 4 * lb_m.size()), o_lbr = put(lb_r.data(), 4 * lb_r.size());
+        size_t o_zl = put(zl_m.data(), 4 * zl_m.size());
+        size_t o_cpm = put(cp_m.data(), 4 * cp_m.size()), o_cpc = put(cp_c.data(), 4 * cp_c.size());
+        size_t o_qo = put(qo.data(), 8 * qo.size()), o_tl = put(tiles.data(), sizeof(BdTile) * tiles.size());
+        BCK(bensure(c->tab, pk.size() + 64));
+        char *dt = (char *)c->tab.p;
+        BHK(hipMemcpyAsync(dt, pk.data(), pk.size(), hipMemcpyHostToDevice, st));
+        BCK(bensure(c->qb, sizeof(double) * (size_t)(qtot + 1)));
+        double *Q = (double *)c->qb.p;
+        if (!rb_m.empty())
+            hipLaunchKernelGGL(k_bd_rotate, dim3((unsigned)rb_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_rbm),
+                               (const int *)(dt + o_rbr), (int)rb_m.size(), W, (const BdOut *)mo, U0, U1, V0, V1, ld);
+        if (!zl_m.empty()) {
+            hipLaunchKernelGGL(k_bd_znorm, dim3((unsigned)zl_m.size()), dim3(64), 0, st, mg, (const int *)(dt + o_zl),
+                               (int)zl_m.size(), W, mo);
+            hipLaunchKernelGGL(k_bd_secular, dim3((unsigned)sb_m.size()), dim3(256), 0, st, mg,
+                               (const int *)(dt + o_sbm), (const int *)(dt + o_sbr), (int)sb_m.size(), W,
+                               (const BdOut *)mo, c->prm.maxit, dfail);
+            hipLaunchKernelGGL(k_bd_loewner, dim3((unsigned)lb_m.size()), dim3(256), 0, st, mg,
+                               (const int *)(dt + o_lbm), (const int *)(dt + o_lbr), (int)lb_m.size(), W,
+                               (const BdOut *)mo);
+            hipLaunchKernelGGL(k_bd_vec, dim3((unsigned)sb_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_sbm),
+                               (const int *)(dt + o_sbr), (int)sb_m.size(), W, (const BdOut *)mo,
+                               (const long *)(dt + o_qo), Q);
+            hipLaunchKernelGGL(k_bd_gemm, dim3((unsigned)tiles.size()), dim3(256), 0, st, mg,
+                               (const BdTile *)(dt + o_tl), (int)tiles.size(), W, (const BdOut *)mo,
+                               (const long *)(dt + o_qo), (const double *)Q, U0, U1, V0, V1, ld);
+        }
+        if (!cp_m.empty())
+            hipLaunchKernelGGL(k_bd_copy, dim3((unsigned)cp_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_cpm),
+                               (const int *)(dt + o_cpc), (int)cp_m.size(), W, (const BdOut *)mo, U0, U1, V0, V1, ld);
+        hipLaunchKernelGGL(k_bd_post, dim3(nm), dim3(256), 0, st, mg, W, (const BdOut *)mo);
+        BHK(hipGetLastError());
+        c->st.levels++;
+        c->st.merges += nm;
+    }
+    // singular values back to the host: unscale, MDBDSDC's selection sort (mdbdsdc.f90:383-404)
+    std::vector<double> D(n);
+    unsigned long long cnt[2];
+    BHK(hipMemcpyAsync(D.data(), W.D, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    BHK(hipMemcpyAsync(cnt, W.iters, 16, hipMemcpyDeviceToHost, st));
+    BHK(hipStreamSynchronize(st));
+    if ((int)(cnt[1] & 0xffffffffu)) return 1;
+    if (n > smlsiz) host_lascl(1.0, orgnrm, n, D.data());
+    // selection sort: position i takes the first largest of D(i..n); a segment tree over
+    // (value, position) finds it, the swap puts D(i) where the maximum was
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; i++) perm[i] = i;
+    {
+        int sz = 1;
+        while (sz < n) sz <<= 1;
+        std::vector<int> tr(2 * sz, -1);
+        auto better = [&](int a, int b) {
+            if (a < 0) return b;
+            if (b < 0) return a;
+            if (D[b] > D[a]) return b;
+            return a < b ? a : b;
+        };
+        for (int i = 0; i < n; i++) tr[sz + i] = i;
+        for (int i = sz - 1; i >= 1; i--) tr[i] = better(tr[2 * i], tr[2 * i + 1]);
+        auto update = [&](int p) {
+            p += sz;
+            for (p >>= 1; p >= 1; p >>= 1) tr[p] = better(tr[2 * p], tr[2 * p + 1]);
+        };
+        auto query = [&](int lo) {  // first maximum over [lo, n)
+            int best = -1, l = lo + sz, r = n - 1 + sz + 1;
+            std::vector<int> right;
+            while (l < r) {
+                if (l & 1) best = better(best, tr[l++]);
+                if (r & 1) right.push_back(tr[--r]);
+                l >>= 1;
+                r >>= 1;
+            }
+            for (int k = (int)right.size() - 1; k >= 0; k--) best = better(best, right[k]);
+            return best;
+        };
+        for (int i = 0; i < n - 1; i++) {
+            int kk = query(i);
+            // DLASR's rule: strictly larger than D(i) moves; ties keep the first
+            if (kk != i && D[kk] > D[i]) {
+                std::swap(D[kk], D[i]);
+                std::swap(perm[kk], perm[i]);
+                tr[sz + kk] = kk;
+                tr[sz + i] = i;
+                update(kk);
+                update(i);
+            }
+        }
+    }
+    // outputs
+    BCK(bensure(c->misc, sizeof(int) * 3 * (size_t)n + 64));
+    int *dperm = (int *)c->misc.p, *dplo = dperm + n, *dphi = dplo + n;
+    BHK(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    BHK(hipMemcpyAsync(dplo, plo.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    BHK(hipMemcpyAsync(dphi, phi.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    if (dD) BHK(hipMemcpyAsync(dD, D.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    if (dU || dVT) {
+        hipLaunchKernelGGL(k_bd_out, dim3(bcdiv(n, 64), bcdiv(n, 64)), dim3(256), 0, st, (const double *)U0,
+                           (const double *)V0, ld, (const int *)dperm, (const int *)dplo, (const int *)dphi, n, dU,
+                           ldu, dVT, ldvt);
+        BHK(hipGetLastError());
+        if (rc && dU) {
+            BCK(bensure(c->rot, sizeof(double) * 2 * (size_t)n));
+            double *dc = (double *)c->rot.p, *ds = dc + n;
+            BHK(hipMemcpyAsync(dc, rc, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
+            BHK(hipMemcpyAsync(ds, rs, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_bd_lasr, dim3(bcdiv(n, 256)), dim3(256), 0, st, dU, ldu, n, (const double *)dc,
+                               (const double *)ds);
+            BHK(hipGetLastError());
+        }
+    }
+    BHK(hipEventRecord(e1, st));
+    BHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->st.ms_total = ms;
+    c->st.roots = roots;
+    c->st.deflated = deflated;
+    c->st.rotations = rots;
+    c->st.k_top = ktop;
+    c->st.secular_iters = (long long)cnt[0];
+    c->st.leaves = (int)leaves.size();
+    return 0;
+}
+
+static __global__ void k_bd_eye(double *A, long lda, int n)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)n * n) return;
+    const int r = (int)(i % n), c = (int)(i / n);
+    A[(long)c * lda + r] = r == c ? 1.0 : 0.0;
+}
+
+extern "C" {
+
+// MDBDSDC on device arrays (mdbdsdc.f90:1-422, COMPQ = 'I'); d_u / d_vt may be null
+// for singular values only
+int hsdc_bdsdc_dev(hsdc_bd_ctx *c, char uplo, int n, double *d_d, double *d_e, double *d_u, int ldu,
+                   double *d_vt, int ldvt)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    int iuplo = (uplo == 'U' || uplo == 'u') ? 1 : (uplo == 'L' || uplo == 'l') ? 2 : 0;
+    if (!iuplo) return -1;
+    if (n < 0) return -3;
+    if (d_u && ldu < (n > 1 ? n : 1)) return -7;
+    if (d_vt && ldvt < (n > 1 ? n : 1)) return -9;
+    if (n == 0) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { (void)hipGetLastError(); return HSDC_ERR_NODEVICE; }
+    hipStream_t st = c->stream;
+    std::vector<double> hd(n), he(n > 1 ? n - 1 : 1, 0.0);
+    BHK(hipMemcpyAsync(hd.data(), d_d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (n > 1) BHK(hipMemcpyAsync(he.data(), d_e, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, st));
+    BHK(hipStreamSynchronize(st));
+    std::memset(&c->st, 0, sizeof c->st);
+    c->tr.clear();
+    if (n == 1) {
+        double u = hd[0] >= 0.0 ? 1.0 : -1.0, v = 1.0, dv = std::fabs(hd[0]);
+        if (d_u) BHK(hipMemcpyAsync(d_u, &u, sizeof(double), hipMemcpyHostToDevice, st));
+        if (d_vt) BHK(hipMemcpyAsync(d_vt, &v, sizeof(double), hipMemcpyHostToDevice, st));
+        BHK(hipMemcpyAsync(d_d, &dv, sizeof(double), hipMemcpyHostToDevice, st));
+        BHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    // lower bidiagonal: rotate to upper from the left (mdbdsdc.f90:216-233)
+    std::vector<double> rc, rs;
+    if (iuplo == 2) {
+        rc.resize(n - 1);
+        rs.resize(n - 1);
+        for (int i = 0; i < n - 1; i++) {
+            double cs, sn, r;
+            lartg(hd[i], he[i], cs, sn, r);
+            hd[i] = r;
+            he[i] = sn * hd[i + 1];
+            hd[i + 1] = cs * hd[i + 1];
+            rc[i] = cs;
+            rs[i] = -sn;
+        }
+    }
+    double orgnrm = 1.0;
+    if (n > c->prm.smlsiz) {
+        orgnrm = 0.0;
+        for (int i = 0; i < n; i++) orgnrm = std::fmax(orgnrm, std::fabs(hd[i]));
+        for (int i = 0; i < n - 1; i++) orgnrm = std::fmax(orgnrm, std::fabs(he[i]));
+        if (orgnrm == 0.0) {
+            // mdbdsdc.f90:292-294: U = VT = I, D unchanged
+            if (d_u) hipLaunchKernelGGL(k_bd_eye, dim3(bcdiv((long)n * n, 256)), dim3(256), 0, st, d_u, (long)ldu, n);
+            if (d_vt) hipLaunchKernelGGL(k_bd_eye, dim3(bcdiv((long)n * n, 256)), dim3(256), 0, st, d_vt, (long)ldvt, n);
+            BHK(hipGetLastError());
+            BHK(hipStreamSynchronize(st));
+            return 0;
+        }
+        host_lascl(orgnrm, 1.0, n, hd.data());
+        host_lascl(orgnrm, 1.0, n - 1, he.data());
+        const double eps = 0.9 * BD_EPS;
+        for (int i = 0; i < n; i++)
+            if (std::fabs(hd[i]) < eps) hd[i] = hd[i] >= 0.0 ? eps : -eps;
+    }
+    BCK(bd_alloc(c, n));
+    BHK(hipMemcpyAsync(c->W.D, hd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    BHK(hipMemcpyAsync(c->W.E, he.data(), sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
+    int info = bd_solve(c, n, hd, he, orgnrm, d_u, ldu, d_vt, ldvt, d_d, iuplo == 2 ? rc.data() : nullptr,
+                        iuplo == 2 ? rs.data() : nullptr);
+    if (info) return info;
+    BHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// MDBDSDC on host arrays (mdbdsdc.f90:1-2): COMPQ 'N' or 'I'.  q / iq (COMPQ = 'P') and
+// work / iwork are not used: the context owns the device workspace.
+int hsdc_bdsdc(hsdc_bd_ctx *c, char uplo, char compq, int n, double *d, double *e, double *u, int ldu,
+               double *vt, int ldvt, double *q, int *iq, double *work, int *iwork)
+{
+    (void)q; (void)iq; (void)work; (void)iwork;
+    if (!c) return HSDC_ERR_INTERNAL;
+    int iuplo = (uplo == 'U' || uplo == 'u') ? 1 : (uplo == 'L' || uplo == 'l') ? 2 : 0;
+    int icompq = (compq == 'N' || compq == 'n') ? 0 : (compq == 'P' || compq == 'p') ? 1
+                 : (compq == 'I' || compq == 'i') ? 2 : -1;
+    if (!iuplo) return -1;
+    if (icompq < 0) return -2;
+    if (icompq == 1) return -2;  // the compact form (DLASDA) is not provided
+    if (n < 0) return -3;
+    if (ldu < 1 || (icompq == 2 && ldu < n)) return -7;
+    if (ldvt < 1 || (icompq == 2 && ldvt < n)) return -9;
+    if (n == 0) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { (void)hipGetLastError(); return HSDC_ERR_NODEVICE; }
+    hipStream_t st = c->stream;
+    BCK(bensure(c->din, sizeof(double) * 2 * (size_t)n));
+    double *dd = (double *)c->din.p, *de = dd + n;
+    double *du = nullptr, *dvt = nullptr;
+    if (icompq == 2) {
+        BCK(bensure(c->uo, sizeof(double) * (size_t)n * n));
+        BCK(bensure(c->vto, sizeof(double) * (size_t)n * n));
+        du = (double *)c->uo.p;
+        dvt = (double *)c->vto.p;
+    }
+    BHK(hipMemcpyAsync(dd, d, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    if (n > 1) BHK(hipMemcpyAsync(de, e, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
+    int info = hsdc_bdsdc_dev(c, uplo, n, dd, de, du, n, dvt, n);
+    if (info) return info;
+    BHK(hipMemcpyAsync(d, dd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (icompq == 2) {
+        BHK(hipMemcpy2DAsync(u, sizeof(double) * ldu, du, sizeof(double) * n, sizeof(double) * n, n,
+                             hipMemcpyDeviceToHost, st));
+        BHK(hipMemcpy2DAsync(vt, sizeof(double) * ldvt, dvt, sizeof(double) * n, sizeof(double) * n, n,
+                             hipMemcpyDeviceToHost, st));
+    }
+    BHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // extern "C"

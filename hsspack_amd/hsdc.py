@@ -29,7 +29,9 @@ EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_s
            "hsdc_get_stats", "hsdc_set_stream", "hsdc_stedc", "hsdc_stedc_dev",
            "hsdc_stedc_batched_dev", "hsdc_autotune", "hsdc_laed0", "hsdc_deflate", "hsdc_secular", "hsdc_laed3", "hsdc_loewner",
            "hsdc_hss_evc", "hsdc_hss_dense", "hsdc_mlp_set_weights", "hsdc_mlp_get_weights",
-           "hsdc_mlp_load", "hsdc_mlp_harvest", "hsdc_mlp_samples", "hsdc_mlp_grad", "hsdc_mlp_train"]
+           "hsdc_mlp_load", "hsdc_mlp_harvest", "hsdc_mlp_samples", "hsdc_mlp_grad", "hsdc_mlp_train",
+           "hsdc_bd_default_params", "hsdc_bd_ctx_create", "hsdc_bd_ctx_destroy", "hsdc_bd_set_params",
+           "hsdc_bd_get_stats", "hsdc_bd_set_stream", "hsdc_bdsdc", "hsdc_bdsdc_dev", "hsdc_bd_trace"]
 
 # learned secular start: 34 inputs -> 64 -> 64 -> 1
 MLP_IN, MLP_H = 34, 64
@@ -51,6 +53,17 @@ class Stats(ctypes.Structure):
                 ("secular_wave_iters", ctypes.c_longlong), ("secular_crit_iters", ctypes.c_longlong),
                 ("hss_mm_flops", ctypes.c_longlong), ("ms_hss_mm", _d), ("ms_zfill", _d),
                 ("zfill_bytes", ctypes.c_longlong)]
+
+
+class BdParams(ctypes.Structure):
+    _fields_ = [("smlsiz", _i), ("maxit", _i)]
+
+
+class BdStats(ctypes.Structure):
+    _fields_ = [("levels", _i), ("merges", _i), ("leaves", _i), ("k_top", _i),
+                ("roots", ctypes.c_longlong), ("deflated", ctypes.c_longlong),
+                ("rotations", ctypes.c_longlong), ("secular_iters", ctypes.c_longlong),
+                ("ms_total", _d)]
 
 
 _LIB = None
@@ -102,6 +115,15 @@ def lib():
         L.hsdc_mlp_samples.argtypes = [_vp, _f, _f, _ll, _P(_ll)]
         L.hsdc_mlp_grad.argtypes = [_vp, _f, _f, _i, _f, _f]
         L.hsdc_mlp_train.argtypes = [_vp, _f, _f, _ll, _i, _i, ctypes.c_float, ctypes.c_ulonglong, _f]
+        L.hsdc_bd_ctx_create.argtypes = [_P(_vp)]
+        L.hsdc_bd_ctx_destroy.argtypes = [_vp]
+        L.hsdc_bd_set_params.argtypes = [_vp, _P(BdParams)]
+        L.hsdc_bd_get_stats.argtypes = [_vp, _P(BdStats)]
+        L.hsdc_bd_set_stream.argtypes = [_vp, _vp]
+        L.hsdc_bdsdc.argtypes = [_vp, ctypes.c_char, ctypes.c_char, _i, _P(_d), _P(_d), _P(_d), _i,
+                                 _P(_d), _i, _P(_d), _P(_i), _P(_d), _P(_i)]
+        L.hsdc_bdsdc_dev.argtypes = [_vp, ctypes.c_char, _i, _vp, _vp, _vp, _i, _vp, _i]
+        L.hsdc_bd_trace.argtypes = [_vp, _P(_i), _P(_i), _P(_i), _P(_i), _i]
         _LIB = L
     return _LIB
 
@@ -386,3 +408,70 @@ class Context:
         if rc != 0:
             raise RuntimeError("hsdc_mlp_train: %d" % rc)
         return hist
+
+
+class BdContext:
+    """Context of the bidiagonal DC SVD (MDBDSDC, SRC/mdbdsdc.f90); not thread safe."""
+
+    def __init__(self, **params):
+        self._h = _vp()
+        rc = lib().hsdc_bd_ctx_create(ctypes.byref(self._h))
+        if rc != 0:
+            raise RuntimeError("hsdc_bd_ctx_create failed: %d" % rc)
+        self.params = BdParams()
+        lib().hsdc_bd_default_params(ctypes.byref(self.params))
+        self.set_params(**params)
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        rc = lib().hsdc_bd_set_params(self._h, ctypes.byref(self.params))
+        if rc != 0:
+            raise ValueError("hsdc_bd_set_params rejected %r (%d)" % (kw, rc))
+
+    def set_stream(self, stream):
+        h = getattr(stream, "cuda_stream", stream)
+        return lib().hsdc_bd_set_stream(self._h, ctypes.c_void_p(h or 0))
+
+    def stats(self):
+        s = BdStats()
+        lib().hsdc_bd_get_stats(self._h, ctypes.byref(s))
+        return {f: getattr(s, f) for f, _ in BdStats._fields_}
+
+    def trace(self):
+        cnt = lib().hsdc_bd_trace(self._h, None, None, None, None, 0)
+        a = [np.zeros(max(cnt, 1), np.int32) for _ in range(4)]
+        lib().hsdc_bd_trace(self._h, *[_ip(x) for x in a], cnt)
+        return dict(s=a[0][:cnt], n=a[1][:cnt], k=a[2][:cnt], nrot=a[3][:cnt])
+
+    def close(self):
+        if self._h:
+            lib().hsdc_bd_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bdsdc(self, d, e, uplo="U", compq="I"):
+        """MDBDSDC on host arrays: returns (s descending, U, VT, info); d, e are copied."""
+        n = len(d)
+        d = np.array(d, np.float64)
+        e = np.array(e, np.float64) if n > 1 else np.zeros(1)
+        u = np.zeros((n, n), order="F")
+        vt = np.zeros((n, n), order="F")
+        ld = max(1, n)
+        info = lib().hsdc_bdsdc(self._h, ctypes.c_char(uplo.encode()), ctypes.c_char(compq.encode()), n,
+                                _dp(d), _dp(e), _dp(u) if n else None, ld, _dp(vt) if n else None, ld,
+                                None, None, None, None)
+        return d, u, vt, info
+
+    def bdsdc_dev(self, d, e, u, vt, uplo="U"):
+        """Device-resident MDBDSDC on torch tensors: d (n, overwritten by the singular
+        values), e (n-1), u and vt (n x n column-major, i.e. .T contiguous) or None."""
+        n = d.shape[0]
+        return lib().hsdc_bdsdc_dev(self._h, ctypes.c_char(uplo.encode()), n, d.data_ptr(), e.data_ptr(),
+                                    u.data_ptr() if u is not None else None, n,
+                                    vt.data_ptr() if vt is not None else None, n)

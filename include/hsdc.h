@@ -182,6 +182,47 @@ int hsdc_mlp_grad(hsdc_ctx *ctx, const float *x, const float *t, int ns, float *
 int hsdc_mlp_train(hsdc_ctx *ctx, const float *x, const float *t, long long ns, int epochs, int batch,
                    float lr, unsigned long long seed, float *loss_hist);
 
+/* ---------------------------------------------------------------------------
+ * Bidiagonal divide-and-conquer SVD: HSSPACK's MDBDSDC (SRC/mdbdsdc.f90) ->
+ * MDLASD0 -> MDLASD1 -> MDLASD2 and the dense merge (DLASD3 / DLASD4) it takes for
+ * K < BSMLZ (mdlasd2.f90:176).  Every merge takes the dense branch; the HSS branch
+ * (MDLASD8 / DHSSVCS, K >= BSMLZ) is not provided.
+ * ------------------------------------------------------------------------- */
+typedef struct hsdc_bd_ctx hsdc_bd_ctx;
+
+typedef struct {
+    int smlsiz;  /* leaf bound, ILAENV(9,'DBDSDC') = 25 (mdbdsdc.f90:193); 3..63 */
+    int maxit;   /* secular iteration cap */
+} hsdc_bd_params;
+
+typedef struct {
+    int levels, merges, leaves, k_top;
+    long long roots, deflated, rotations, secular_iters;
+    double ms_total;  /* device time of the last solve (events), ms */
+} hsdc_bd_stats;
+
+void hsdc_bd_default_params(hsdc_bd_params *p);
+int  hsdc_bd_ctx_create(hsdc_bd_ctx **ctx);
+int  hsdc_bd_ctx_destroy(hsdc_bd_ctx *ctx);
+int  hsdc_bd_set_params(hsdc_bd_ctx *ctx, const hsdc_bd_params *p);
+int  hsdc_bd_get_stats(hsdc_bd_ctx *ctx, hsdc_bd_stats *s);
+int  hsdc_bd_set_stream(hsdc_bd_ctx *ctx, void *stream);
+
+/* MDBDSDC (mdbdsdc.f90:1-2).  uplo 'U' or 'L'; compq 'N' (values) or 'I' (values and
+ * vectors; 'P', the compact form, returns -2).  d(n): diagonal in, singular values
+ * (descending) out; e(n-1) destroyed; u(ldu,n), vt(ldvt,n) host arrays.  q, iq,
+ * work, iwork are accepted for the reference's argument list and not used.  INFO as
+ * MDBDSDC: -i for an illegal argument i, 1 when the secular iteration or a leaf
+ * did not converge. */
+int hsdc_bdsdc(hsdc_bd_ctx *ctx, char uplo, char compq, int n, double *d, double *e, double *u, int ldu,
+               double *vt, int ldvt, double *q, int *iq, double *work, int *iwork);
+/* The same on device arrays (COMPQ = 'I'; d_u and d_vt may be NULL for values only). */
+int hsdc_bdsdc_dev(hsdc_bd_ctx *ctx, char uplo, int n, double *d_d, double *d_e, double *d_u, int ldu,
+                   double *d_vt, int ldvt);
+/* Merges of the last solve, in launch order: block start (0-based), order N, K and
+ * deflation rotations.  Returns the merge count (entries beyond cap are not written). */
+int  hsdc_bd_trace(hsdc_bd_ctx *ctx, int *s, int *n, int *k, int *nrot, int cap);
+
 #ifdef __cplusplus
 }
 #endif

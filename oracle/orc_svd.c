@@ -143,13 +143,15 @@ static double bfly64(double *v)
 }
 
 #define JAC_TOL 0x1p-52
-#define JAC_SWEEPS 40
+#define JAC_SWEEPS 60
 
 /* SVD of the m x (m+sqre) upper bidiagonal B (diagonal d, superdiagonal e, e has
  * m-1+sqre entries): B = U [diag(s) 0] VT, s ascending in d, U m x m (ldu), VT
  * (m+sqre) x (m+sqre) (ldvt) whose last row is the null vector when sqre = 1.
- * One-sided Jacobi on the columns of B in row-cyclic order until a sweep rotates
- * nothing; returns the number of sweeps, or -1 if JAC_SWEEPS did not suffice. */
+ * One-sided Jacobi on the columns of B in row-cyclic order; a pair is rotated while
+ * |w_p.w_q| > (m+sqre) 2^-52 |w_p||w_q| (the rounding level of the sums), until a
+ * sweep rotates nothing; returns the number of sweeps, or -1 if JAC_SWEEPS did not
+ * suffice. */
 int orc_bd_leaf(int m, int sqre, double *d, const double *e, double *u, int ldu, double *vt,
                 int ldvt)
 {
@@ -180,7 +182,7 @@ int orc_bd_leaf(int m, int sqre, double *d, const double *e, double *u, int ldu,
                 double be = bfly64(a);
                 for (int r = 0; r < 64; r++) a[r] = W[p][r] * W[q][r];
                 double ga = bfly64(a);
-                if (al <= thr || be <= thr || !(fabs(ga) > JAC_TOL * (sqrt(al) * sqrt(be)))) continue;
+                if (al <= thr || be <= thr || !(fabs(ga) > (JAC_TOL * pc) * (sqrt(al) * sqrt(be)))) continue;
                 double zeta = (be - al) / (2.0 * ga), t;
                 if (fabs(zeta) > 1e150) t = 0.5 / zeta;
                 else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));

@@ -1,0 +1,136 @@
+"""GPU parity of the bidiagonal divide-and-conquer SVD (MDBDSDC / MDLASD0-2) against
+the CPU oracle: bit-exact singular values, vectors and per-merge deflation (K and
+rotation counts), LAPACK DBDSDC to its error bound, and orthogonality / residual at
+the full workload size."""
+import numpy as np
+import pytest
+
+import lapack_ref as L
+import oracle_py as O
+from hsspack_amd import hsdc
+from hsspack_amd.workloads import bidiag_cfg
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = hsdc.BdContext()
+    yield c
+    c.close()
+
+
+def bidiag(d, e):
+    n = len(d)
+    return np.diag(d) + np.diag(e[: n - 1], 1)
+
+
+def oracle_trace(tr):
+    return tr["n"], tr["k"], tr["nrot"]
+
+
+def cases():
+    out = []
+    for n in (1, 2, 20, 25, 26, 60, 301, 1000, 2500):
+        d, e = bidiag_cfg(n, seed=5)
+        out.append(("cfg5-%d" % n, d, e, "U"))
+    d, e = bidiag_cfg(700, seed=8)
+    out.append(("lower-700", d, e, "L"))
+    n = 512
+    out.append(("const-512", np.ones(n), np.ones(n - 1), "U"))  # equal child spectra: rotations
+    d, e = bidiag_cfg(400, seed=9)
+    e[[50, 51, 200, 398]] = 1e-20  # splits, a trailing 1 x 1
+    d[123] = 1e-30                 # raised to eps
+    out.append(("split-400", d, e, "U"))
+    d, e = bidiag_cfg(300, seed=10)
+    e[:] *= 1e-9                   # heavy small-z deflation
+    out.append(("tiny-e-300", d, e, "U"))
+    return out
+
+
+@pytest.mark.parametrize("case", range(13))
+def test_bdsdc_bitexact_vs_oracle(ctx, case):
+    name, d, e, uplo = cases()[case]
+    print("case", name, flush=True)
+    s, u, vt, info = ctx.bdsdc(d, e, uplo=uplo)
+    assert info == 0
+    r = O.bdsdc(d, e, uplo=uplo.encode(), trace_cap=4096)
+    assert r["info"] == 0
+    assert np.array_equal(s, r["d"])
+    assert np.array_equal(u, r["u"])
+    assert np.array_equal(vt, r["vt"])
+    tr = ctx.trace()
+    ot = r["trace"]
+    # merge order: the device runs every problem's level t together, the oracle one
+    # problem after another; compare as multisets of (order, K, rotations)
+    mine = sorted(zip(tr["n"].tolist(), tr["k"].tolist(), tr["nrot"].tolist()))
+    ref = sorted(zip(ot["n"].tolist(), ot["k"].tolist(), ot["nrot"].tolist()))
+    assert mine == ref
+    st = ctx.stats()
+    assert st["roots"] == ot["roots"]
+    assert st["secular_iters"] == ot["iters"]
+
+
+def test_rotation_and_deflation_paths_exercised(ctx):
+    for name, d, e, uplo in cases():
+        if name in ("const-512", "tiny-e-300"):
+            ctx.bdsdc(d, e, uplo=uplo)
+            tr = ctx.trace()
+            if name == "const-512":
+                assert tr["nrot"].sum() > 0
+            else:
+                assert np.any(tr["k"] < tr["n"])
+
+
+@pytest.mark.parametrize("n", [1000, 4096])
+def test_bdsdc_vs_lapack(ctx, n):
+    d, e = bidiag_cfg(n, seed=5)
+    s, u, vt, info = ctx.bdsdc(d, e)
+    assert info == 0
+    sl = L.dbdsdc(d, e)[0]
+    B = bidiag(d, e)
+    nrm = np.abs(B).max()
+    assert np.abs(s - sl).max() <= 4 * n * EPS * nrm
+    assert np.abs(u.T @ u - np.eye(n)).max() <= 1e-12
+    assert np.abs(vt @ vt.T - np.eye(n)).max() <= 1e-12
+    assert np.abs((u * s) @ vt - B).max() / nrm <= 1e-12
+
+
+def test_values_only_matches_vectors_run(ctx):
+    d, e = bidiag_cfg(777, seed=5)
+    s1 = ctx.bdsdc(d, e, compq="I")[0]
+    s2, _, _, info = ctx.bdsdc(d, e, compq="N")
+    assert info == 0
+    assert np.array_equal(s1, s2)
+
+
+def test_bdsdc_device_resident_full_size(ctx):
+    """The headline bidiagonal workload (n = 16384) through the device entry: sorted
+    singular values, orthogonality and residual of the product through torch."""
+    torch = pytest.importorskip("torch")
+    n = 16384
+    d, e = bidiag_cfg(n, seed=5)
+    dd = torch.tensor(d, device="cuda")
+    de = torch.tensor(e, device="cuda")
+    U = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    VT = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    rc = ctx.bdsdc_dev(dd, de, U, VT)
+    torch.cuda.synchronize()
+    assert rc == 0
+    s = dd.cpu().numpy()
+    assert np.all(np.diff(s) <= 0)
+    Uc = U.t()  # column-major storage: U.t() is the matrix
+    Vc = VT.t()
+    I = torch.eye(n, dtype=torch.float64, device="cuda")
+    orth_u = (Uc.t() @ Uc - I).abs().max().item()
+    orth_v = (Vc @ Vc.t() - I).abs().max().item()
+    B = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    idx = torch.arange(n, device="cuda")
+    B[idx, idx] = torch.tensor(d, device="cuda")
+    B[idx[:-1], idx[1:]] = torch.tensor(e, device="cuda")
+    res = ((Uc * dd) @ Vc - B).abs().max().item() / B.abs().max().item()
+    print("orth_u %.2e orth_v %.2e res %.2e" % (orth_u, orth_v, res), flush=True)
+    assert orth_u <= 1e-12 and orth_v <= 1e-12 and res <= 1e-12
+    st = ctx.stats()
+    assert st["levels"] >= 9 and st["merges"] > 0

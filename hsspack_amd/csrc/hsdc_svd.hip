@@ -3,19 +3,26 @@
 // dense merge DLASD3/DLASD4 it calls below BSMLZ, as HIP kernels batched over every
 // merge of one tree level.
 //
-// Layout.  U (n x n) and V = VT^T (n x n) are column-major in two ping-pong buffers
-// each; a merge reads its children's columns from one and writes its own to the
-// other.  A column is only ever read inside its row support: the rows of the tree
-// node that last wrote it (or the whole merge block once a deflation rotation has
-// touched it), so neither buffer is cleared.  Working vectors live at the global
-// index of the merge block (s + i), disjoint between the merges of one level.
+// Layout.  U (n x n) and V = VT^T (n x n) are column-major, one buffer each.  A
+// singular vector ("virtual column" j of its merge block) lives in a physical slot
+// found through a map (vmU / vmV, ping-pong by level); every slot has a row support
+// [lo, hi), the rows of the tree node that last wrote it (the whole merge block once
+// a deflation rotation has touched it), and is read as zero outside it, so no buffer
+// is ever cleared.  A deflated column keeps its slot: only the map changes.  The K
+// new columns of a merge are formed in a staging buffer and scattered into the
+// slots that merge frees (its non-deflated inputs and its centre column).  The
+// slots of a block are the block's own columns, so merges of one level never share
+// one.  Working vectors live at the global index of the merge block (s + i).
 //
 // Per level: k_bd_prep (scaling, z, sort: mdlasd1.f90:171-184, mdlasd2.f90:216-268),
 // k_bd_deflate (mdlasd2.f90:270-470, in parallel), a read of K, k_bd_rotate (the
 // deflation's Givens rotations), k_bd_znorm, k_bd_secular (DLASD4), k_bd_loewner,
 // k_bd_vec (DLASD3's vector formulas), k_bd_gemm (DLASD3's products, gathering
-// columns through the deflation's permutation), k_bd_copy (deflated columns) and
-// k_bd_post (unscaling and DLAMRG, mdlasd1.f90:282-286).
+// columns through the deflation's permutation into staging), k_bd_newcol (the
+// columns formed without a product), k_bd_scatter (staging -> freed slots) and
+// k_bd_post (unscaling, DLAMRG (mdlasd1.f90:282-286), the new maps and supports).
+// U and VT are cleared on a second stream while the levels run; k_bd_fin then
+// writes the supported rows of every column in MDBDSDC's final order.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <climits>
@@ -54,8 +61,12 @@ struct BdWork {
     double *td, *dsg, *zt, *z, *zs, *zl;    // n
     double *sig, *tau;                      // n
     int *org, *ctyp, *idxc, *idx, *idxp;    // n
-    int *surv, *srank, *clf, *gmap, *gsup, *rotf, *ti;  // n
-    int *rpq;                               // 2 n: rotation columns (p | fresh << 30, q)
+    int *surv, *srank, *clf, *rotf, *ti;   // n
+    int *vmU[2], *vmV[2];                   // n: virtual column -> slot (ping-pong by level)
+    int *sulo, *suhi, *svlo, *svhi;         // n: row support [lo, hi) of every U / V slot
+    int *gU, *gV, *gUlo, *gUhi, *gVlo, *gVhi;  // n: slot and support of U2(:,J) / VT2(J,:)
+    int *fU, *fV;                           // n: slots receiving a merge's new columns
+    int *rp4;                               // 4 n: rotation slots (pU | fresh << 30, qU, pV, qV)
     double *rcs;                            // 2 n: rotation c, s
     unsigned long long *iters;              // secular iteration counter
 };
@@ -174,15 +185,13 @@ __device__ double block_max(double v, double *sh)
 // leaves: one-sided Jacobi, one wavefront per leaf, lane r holds row r of every
 // column (replaces DLASDQ, mdlasd0.f90:152-183)
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdWork W, double *U0,
-                                                double *U1, double *V0, double *V1, long ld, int pmax,
-                                                int *fail)
+__global__ __launch_bounds__(64) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdWork W, double *U, double *V,
+                                                long ld, int pmax, int *fail)
 {
     const int li = blockIdx.x;
     if (li >= nleaf) return;
     const BdLeaf L = lv[li];
     const int s = L.s, m = L.m, pc = m + L.sqre, r = threadIdx.x;
-    double *U = L.wr ? U1 : U0, *V = L.wr ? V1 : V0;
     extern __shared__ double lds[];
     double *Wc = lds, *Vc = lds + (size_t)pmax * 64, *nn = Vc + (size_t)pmax * 64;
     int *ord = (int *)(nn + 64);
@@ -259,6 +268,17 @@ __global__ __launch_bounds__(64) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdW
         if (r == 0) W.idxq[s + k] = k + 1;
     }
     if (L.sqre && r < pc) V[(long)(s + m) * ld + s + r] = Vc[jn_s * 64 + r];
+    // slot = column, supported on the leaf's rows
+    if (r < m) {
+        W.vmU[L.wr][s + r] = s + r;
+        W.sulo[s + r] = s;
+        W.suhi[s + r] = s + m;
+    }
+    if (r < pc) {
+        W.vmV[L.wr][s + r] = s + r;
+        W.svlo[s + r] = s;
+        W.svhi[s + r] = s + pc;
+    }
     __syncthreads();
     // left vectors of negligible columns: the first e_r keeping norm > 1/2 after two
     // projections off the other columns
@@ -289,13 +309,13 @@ __global__ __launch_bounds__(64) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdW
 // (mdlasd2.f90:216-262), TOL (mdlasd2.f90:266-268)
 // ------------------------------------------------------------------------------
 #define PREP_NT 256
-__global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W, BdOut *mo, const double *V0,
-                                                     const double *V1, long ld)
+__global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W, BdOut *mo, const double *V,
+                                                     long ld)
 {
     __shared__ double shd[PREP_NT / 64 + 1];
     const BdMerge M = mg[blockIdx.x];
     const int s = M.s, nl = M.nl, nr = M.nr, n = nl + nr + 1, m = n + M.sqre, t = threadIdx.x;
-    const double *V = M.rd ? V1 : V0;
+    const int *vm = W.vmV[M.rd];
     const double alpha0 = W.D[s + nl], beta0 = W.E[s + nl];
     double mx = 0.0;
     for (int i = t; i < n; i += PREP_NT)
@@ -321,12 +341,13 @@ __global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W
         W.ti[s + i] = q;
         W.ctyp[s + i] = (i >= 1 && i <= nl) ? 1 : 2;
     }
+    // row NLP1 (NLP2) of the children's right vectors; a slot reads as zero outside
+    // its support (a column deflated below this level ends above those rows)
     for (int i = t; i < m; i += PREP_NT) {
-        double zv;
-        if (i == 0) zv = alpha * V[(long)(s + nl) * ld + s + nl];
-        else if (i <= nl) zv = alpha * V[(long)(s + i - 1) * ld + s + nl];
-        else zv = beta * V[(long)(s + i) * ld + s + nl + 1];
-        W.z[s + i] = zv;
+        const int slot = vm[s + (i == 0 ? nl : (i <= nl ? i - 1 : i))];
+        const int g = (i <= nl) ? s + nl : s + nl + 1;
+        const double v = (g >= W.svlo[slot] && g < W.svhi[slot]) ? V[(long)slot * ld + g] : 0.0;
+        W.z[s + i] = (i <= nl ? alpha : beta) * v;
     }
     __syncthreads();
     for (int i = t; i < n; i += PREP_NT) {
@@ -490,8 +511,10 @@ __global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork
             ctc = ctq;
             int rk = W.ti[s + q2];
             int cp = bd_colidx(W, s, nl, pj), cq = bd_colidx(W, s, nl, qj);
-            W.rpq[2 * (s + rk)] = cp | ((q2 == q + 1) << 30);
-            W.rpq[2 * (s + rk) + 1] = cq;
+            W.rp4[4 * (s + rk)] = W.vmU[M.rd][s + cp] | ((q2 == q + 1) << 30);
+            W.rp4[4 * (s + rk) + 1] = W.vmU[M.rd][s + cq];
+            W.rp4[4 * (s + rk) + 2] = W.vmV[M.rd][s + cp];
+            W.rp4[4 * (s + rk) + 3] = W.vmV[M.rd][s + cq];
             W.rcs[2 * (s + rk)] = cc;
             W.rcs[2 * (s + rk) + 1] = ss;
             W.rotf[s + cp] = 1;
@@ -525,12 +548,21 @@ __global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork
             if (W.ctyp[s + W.idxp[s + p] - 1] == ty + 1) W.idxc[s + psm[ty] + b2++] = p + 1;
     }
     __syncthreads();
-    // DSIGMA(J) = D(IDXP(J)); gather map of U2 / VT2 columns and their supports
+    // DSIGMA(J) = D(IDXP(J)); slots of U2(:,J) and VT2(J,:) and their supports (the
+    // merge block for a column a rotation touched)
     for (int p = p0; p < p1; p++) {
         W.dsg[s + p] = W.D[s + W.idxp[s + p] - 1];
         int src = bd_colidx(W, s, nl, W.idxp[s + W.idxc[s + p] - 1] - 1);
-        W.gmap[s + p] = src;
-        W.gsup[s + p] = W.rotf[s + src];
+        int su = W.vmU[M.rd][s + src], sv = W.vmV[M.rd][s + src];
+        W.gU[s + p] = su;
+        W.gV[s + p] = sv;
+        if (W.rotf[s + src]) {
+            W.gUlo[s + p] = s; W.gUhi[s + p] = s + n;
+            W.gVlo[s + p] = s; W.gVhi[s + p] = s + m;
+        } else {
+            W.gUlo[s + p] = W.sulo[su]; W.gUhi[s + p] = W.suhi[su];
+            W.gVlo[s + p] = W.svlo[sv]; W.gVhi[s + p] = W.svhi[sv];
+        }
     }
     __syncthreads();
     if (t == 0) {
@@ -556,12 +588,23 @@ __global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork
         o.k = k;
         o.nrot = nrot;
         o.ctot[0] = ct0; o.ctot[1] = ct1; o.ctot[2] = ct2; o.ctot[3] = ct3;
-        W.gmap[s] = -1;
-        W.gsup[s] = 0;
+        W.gU[s] = -1;
+        W.gV[s] = -1;
+        // slots freed for the new columns: the centre column of U, the left null
+        // column of V and (M > N) the right one, then the non-deflated inputs
+        W.fU[s] = W.vmU[M.rd][s + nl];
+        W.fV[s] = W.vmV[M.rd][s + nl];
+        if (m > n) W.fV[s + k] = W.vmV[M.rd][s + m - 1];
     }
     __syncthreads();
     // Z(1..K) for the secular equation (mdlasd2.f90:497-500)
-    for (int i = t; i < k; i += DEF_NT) W.z[s + i] = (i == 0) ? o.zr1 : W.zt[s + i];
+    for (int i = t; i < k; i += DEF_NT) {
+        W.z[s + i] = (i == 0) ? o.zr1 : W.zt[s + i];
+        if (i > 0) {
+            W.fU[s + i] = W.gU[s + i];
+            W.fV[s + i] = W.gV[s + i];
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------
@@ -570,8 +613,8 @@ __global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork
 // for the first time is read within its child's rows (zero elsewhere).
 // ------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bd_rotate(const BdMerge *mg, const int *bmerge, const int *brow,
-                                                   int nblk, BdWork W, const BdOut *mo, double *U0, double *U1,
-                                                   double *V0, double *V1, long ld)
+                                                   int nblk, BdWork W, const BdOut *mo, double *U, double *V,
+                                                   long ld)
 {
     const int b = blockIdx.x;
     if (b >= nblk) return;
@@ -580,26 +623,26 @@ __global__ __launch_bounds__(256) void k_bd_rotate(const BdMerge *mg, const int 
     const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre;
     const int r = brow[b] + threadIdx.x;
     if (r >= m) return;
-    double *U = M.rd ? U1 : U0, *V = M.rd ? V1 : V0;
+    const int g = s + r;  // global row
     const int nrot = mo[mi].nrot;
     for (int i = 0; i < nrot; i++) {
-        int pw = W.rpq[2 * (s + i)], q = W.rpq[2 * (s + i) + 1];
-        int p = pw & 0x3fffffff, fresh = pw >> 30;
+        const int *rr = W.rp4 + 4 * (s + i);
+        int pu = rr[0] & 0x3fffffff, fresh = rr[0] >> 30, qu = rr[1], pv = rr[2], qv = rr[3];
         double c = W.rcs[2 * (s + i)], sn = W.rcs[2 * (s + i) + 1];
         if (r < n) {
-            bool inp = !fresh || (p < nl ? r < nl : r > nl);
-            bool inq = (q < nl ? r < nl : r > nl);
-            double x = inp ? U[(long)(s + p) * ld + s + r] : 0.0;
-            double y = inq ? U[(long)(s + q) * ld + s + r] : 0.0;
-            U[(long)(s + p) * ld + s + r] = c * x + sn * y;
-            U[(long)(s + q) * ld + s + r] = c * y - sn * x;
+            bool inp = !fresh || (g >= W.sulo[pu] && g < W.suhi[pu]);
+            bool inq = g >= W.sulo[qu] && g < W.suhi[qu];
+            double x = inp ? U[(long)pu * ld + g] : 0.0;
+            double y = inq ? U[(long)qu * ld + g] : 0.0;
+            U[(long)pu * ld + g] = c * x + sn * y;
+            U[(long)qu * ld + g] = c * y - sn * x;
         }
-        bool inp = !fresh || (p <= nl ? r <= nl : r > nl);
-        bool inq = (q <= nl ? r <= nl : r > nl);
-        double x = inp ? V[(long)(s + p) * ld + s + r] : 0.0;
-        double y = inq ? V[(long)(s + q) * ld + s + r] : 0.0;
-        V[(long)(s + p) * ld + s + r] = c * x + sn * y;
-        V[(long)(s + q) * ld + s + r] = c * y - sn * x;
+        bool inp = !fresh || (g >= W.svlo[pv] && g < W.svhi[pv]);
+        bool inq = g >= W.svlo[qv] && g < W.svhi[qv];
+        double x = inp ? V[(long)pv * ld + g] : 0.0;
+        double y = inq ? V[(long)qv * ld + g] : 0.0;
+        V[(long)pv * ld + g] = c * x + sn * y;
+        V[(long)qv * ld + g] = c * y - sn * x;
     }
 }
 
@@ -849,15 +892,15 @@ __global__ __launch_bounds__(256) void k_bd_vec(const BdMerge *mg, const int *bm
 __device__ __forceinline__ double bd_acol(const BdWork &W, const BdMerge &M, const BdOut &o, bool isV,
                                           const double *X, long ld, int J, int r)
 {
-    const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre;
+    const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre, g = s + r;
     if (J == 0) {
         if (!isV) return r == nl ? 1.0 : 0.0;
-        if (m > n) return r <= nl ? o.c * X[(long)(s + nl) * ld + s + r] : o.sn * X[(long)(s + m - 1) * ld + s + r];
-        return r <= nl ? X[(long)(s + nl) * ld + s + r] : 0.0;
+        const int *vm = W.vmV[M.rd];
+        if (m > n) return r <= nl ? o.c * X[(long)vm[s + nl] * ld + g] : o.sn * X[(long)vm[s + m - 1] * ld + g];
+        return r <= nl ? X[(long)vm[s + nl] * ld + g] : 0.0;
     }
-    const int src = W.gmap[s + J];
-    bool in = W.gsup[s + J] || (isV ? (src <= nl ? r <= nl : r > nl) : (src < nl ? r < nl : r > nl));
-    return in ? X[(long)(s + src) * ld + s + r] : 0.0;
+    if (isV) return (g >= W.gVlo[s + J] && g < W.gVhi[s + J]) ? X[(long)W.gV[s + J] * ld + g] : 0.0;
+    return (g >= W.gUlo[s + J] && g < W.gUhi[s + J]) ? X[(long)W.gU[s + J] * ld + g] : 0.0;
 }
 
 // ------------------------------------------------------------------------------
@@ -871,7 +914,8 @@ struct BdTile {
 
 __global__ __launch_bounds__(256) void k_bd_gemm(const BdMerge *mg, const BdTile *tiles, int ntile, BdWork W,
                                                  const BdOut *mo, const long *qoff, const double *Q,
-                                                 double *U0, double *U1, double *V0, double *V1, long ld)
+                                                 const double *U, const double *V, long ld, const long *soff,
+                                                 double *stg)
 {
     __shared__ double As[16][65];
     __shared__ double Bs[16][65];
@@ -881,9 +925,9 @@ __global__ __launch_bounds__(256) void k_bd_gemm(const BdMerge *mg, const BdTile
     const BdMerge M = mg[tl.merge];
     const BdOut &o = mo[tl.merge];
     const bool isV = tl.isv;
-    const int s = M.s, n = M.nl + M.nr + 1, rows = isV ? n + M.sqre : n, k = o.k;
-    const double *X = isV ? (M.rd ? V1 : V0) : (M.rd ? U1 : U0);
-    double *Y = isV ? (M.wr ? V1 : V0) : (M.wr ? U1 : U0);
+    const int n = M.nl + M.nr + 1, rows = isV ? n + M.sqre : n, k = o.k;
+    const double *X = isV ? V : U;
+    double *Y = stg + soff[2 * tl.merge + (isV ? 1 : 0)];
     const double *B = Q + qoff[tl.merge] + (isV ? (long)k * k : 0);
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     double acc[4][4];
@@ -914,47 +958,55 @@ __global__ __launch_bounds__(256) void k_bd_gemm(const BdMerge *mg, const BdTile
         if (r >= rows) continue;
         for (int c = 0; c < 4; c++) {
             int col = tl.col0 + ty + 16 * c;
-            if (col < k) Y[(long)(s + col) * ld + s + r] = acc[a][c];
+            if (col < k) Y[(long)col * rows + r] = acc[a][c];
         }
     }
 }
 
 // ------------------------------------------------------------------------------
-// columns that are copied, not multiplied: the deflated ones (mdlasd2.f90:520-528),
-// the last row of VT when M > N (mdlasd2.f90:504-512), and U(:,1) / VT(1,:) when
-// K = 1 (DLASD3's quick return)
+// new columns formed without a product, into staging: U(:,1) and VT(1,:) when K = 1
+// (DLASD3's quick return) and the last row of VT when M > N (mdlasd2.f90:504-512)
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bd_copy(const BdMerge *mg, const int *cmerge, const int *ccol, int ncol,
-                                                 BdWork W, const BdOut *mo, double *U0, double *U1, double *V0,
-                                                 double *V1, long ld)
+__global__ __launch_bounds__(256) void k_bd_newcol(const BdMerge *mg, const int *mlist, int nlist, BdWork W,
+                                                   const BdOut *mo, const double *U, const double *V, long ld,
+                                                   const long *soff, double *stg)
+{
+    const int b = blockIdx.x;
+    if (b >= nlist) return;
+    const int mi = mlist[b];
+    const BdMerge M = mg[mi];
+    const BdOut &o = mo[mi];
+    const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre, k = o.k;
+    double *SU = stg + soff[2 * mi], *SV = stg + soff[2 * mi + 1];
+    const int *vm = W.vmV[M.rd];
+    for (int r = threadIdx.x; r < m; r += 256) {
+        if (k == 1) {
+            if (r < n) {
+                double u = bd_acol(W, M, o, false, U, ld, 0, r);
+                SU[r] = o.zr1 > 0.0 ? u : -u;
+            }
+            SV[r] = bd_acol(W, M, o, true, V, ld, 0, r);
+        }
+        if (m > n) {
+            const int g = s + r;
+            SV[(long)k * m + r] = r <= nl ? -o.sn * V[(long)vm[s + nl] * ld + g] : o.c * V[(long)vm[s + m - 1] * ld + g];
+        }
+    }
+}
+
+// staging -> the slots freed by the merge (rows of the merge block)
+__global__ __launch_bounds__(256) void k_bd_scatter(const BdMerge *mg, const int *smerge, const int *scol, int ncol,
+                                                    BdWork W, const long *soff, const double *stg, double *U,
+                                                    double *V, long ld)
 {
     const int b = blockIdx.x;
     if (b >= ncol) return;
-    const int mi = cmerge[b], J = ccol[b];
+    const int mi = smerge[b], cw = scol[b], isV = cw >> 30, i = cw & 0x3fffffff;
     const BdMerge M = mg[mi];
-    const BdOut &o = mo[mi];
-    const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre;
-    const double *XU = M.rd ? U1 : U0, *XV = M.rd ? V1 : V0;
-    double *YU = M.wr ? U1 : U0, *YV = M.wr ? V1 : V0;
-    for (int r = threadIdx.x; r < m; r += 256) {
-        if (J == m - 1 && m > n) {
-            // VT(M,:) after the rotation that folds it into VT2(1,:)
-            double v = r <= nl ? -o.sn * XV[(long)(s + nl) * ld + s + r] : o.c * XV[(long)(s + m - 1) * ld + s + r];
-            YV[(long)(s + m - 1) * ld + s + r] = v;
-            continue;
-        }
-        if (J == 0) {
-            // K = 1
-            if (r < n) {
-                double u = bd_acol(W, M, o, false, XU, ld, 0, r);
-                YU[(long)s * ld + s + r] = o.zr1 > 0.0 ? u : -u;
-            }
-            YV[(long)s * ld + s + r] = bd_acol(W, M, o, true, XV, ld, 0, r);
-            continue;
-        }
-        if (r < n) YU[(long)(s + J) * ld + s + r] = bd_acol(W, M, o, false, XU, ld, J, r);
-        YV[(long)(s + J) * ld + s + r] = bd_acol(W, M, o, true, XV, ld, J, r);
-    }
+    const int s = M.s, n = M.nl + M.nr + 1, rows = isV ? n + M.sqre : n;
+    const double *S = stg + soff[2 * mi + isV] + (long)i * rows;
+    double *Y = isV ? V + (long)W.fV[s + i] * ld : U + (long)W.fU[s + i] * ld;
+    for (int r = threadIdx.x; r < rows; r += 256) Y[s + r] = S[r];
 }
 
 // ------------------------------------------------------------------------------
@@ -1003,34 +1055,60 @@ __global__ __launch_bounds__(256) void k_bd_post(const BdMerge *mg, BdWork W, co
     }
     __syncthreads();
     for (int i = t; i < n; i += 256) W.idxq[s + i] = W.ti[s + i];
+    // the block's new maps: new columns in freed slots (supported on the whole block),
+    // deflated ones where they were (the block once a rotation touched them)
+    const int m = n + M.sqre;
+    int *vu = W.vmU[M.wr], *vv = W.vmV[M.wr];
+    for (int i = t; i < n; i += 256) {
+        int su, sv;
+        if (i < k) {
+            su = W.fU[s + i];
+            sv = W.fV[s + i];
+            W.sulo[su] = s; W.suhi[su] = s + n;
+            W.svlo[sv] = s; W.svhi[sv] = s + m;
+        } else {
+            su = W.gU[s + i];
+            sv = W.gV[s + i];
+            W.sulo[su] = W.gUlo[s + i]; W.suhi[su] = W.gUhi[s + i];
+            W.svlo[sv] = W.gVlo[s + i]; W.svhi[sv] = W.gVhi[s + i];
+        }
+        vu[s + i] = su;
+        vv[s + i] = sv;
+    }
+    if (t == 0 && m > n) {
+        int sv = W.fV[s + k];
+        vv[s + m - 1] = sv;
+        W.svlo[sv] = s;
+        W.svhi[sv] = s + m;
+    }
 }
 
 // ------------------------------------------------------------------------------
-// output: U(:,i) = Ufin(:,perm(i)), VT(i,:) = Vfin(:,perm(i))^T, rows outside the
-// column's subproblem zero (64 x 64 tiles, the VT write transposed through LDS)
+// output: zeros everywhere (second stream, beside the levels), then column i of U
+// and row i of VT from the slot of virtual column perm(i), supported rows only
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bd_out(const double *Uf, const double *Vf, long ld, const int *perm,
-                                                const int *plo, const int *phi, int n, double *U, long ldu,
-                                                double *VT, long ldvt)
+__global__ __launch_bounds__(256) void k_bd_zero(double *A, long lda, int n)
 {
-    __shared__ double tile[64][65];
-    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int cc = ty; cc < 64; cc += 4) {
-        int i = c0 + cc, r = r0 + tx;
-        if (i >= n || r >= n) continue;
-        int src = perm[i];
-        bool in = r >= plo[src] && r < phi[src];
-        if (U) U[(long)i * ldu + r] = in ? Uf[(long)src * ld + r] : 0.0;
-        tile[cc][tx] = in ? Vf[(long)src * ld + r] : 0.0;
+    const long tot = (long)n * n;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+        const long c = e / n, r = e - c * n;
+        A[c * lda + r] = 0.0;
     }
-    __syncthreads();
-    if (!VT) return;
-    // VT(i, r) for i = c0 + tx (contiguous in i), r = r0 + rr
-    for (int rr = ty; rr < 64; rr += 4) {
-        int i = c0 + tx, r = r0 + rr;
-        if (i >= n || r >= n) continue;
-        VT[(long)r * ldvt + i] = tile[tx][rr];
+}
+
+__global__ __launch_bounds__(256) void k_bd_fin(const BdWork W, const double *Us, const double *Vs, long ld,
+                                                const int *perm, int n, double *U, long ldu, double *VT, long ldvt)
+{
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int src = perm[i];
+    if (U) {
+        const int su = W.vmU[0][src], lo = W.sulo[su], hi = W.suhi[su];
+        for (int r = lo + threadIdx.x; r < hi; r += 256) U[(long)i * ldu + r] = Us[(long)su * ld + r];
+    }
+    if (VT) {
+        const int sv = W.vmV[0][src], lo = W.svlo[sv], hi = W.svhi[sv];
+        for (int r = lo + threadIdx.x; r < hi; r += 256) VT[(long)r * ldvt + i] = Vs[(long)sv * ld + r];
     }
 }
 
@@ -1148,7 +1226,10 @@ struct hsdc_bd_ctx {
     hsdc_bd_params prm;
     hsdc_bd_stats st;
     hipStream_t stream = 0;
-    Buf ub[2], vb[2], wd, wi, mgb, mob, tab, qb, misc, din, ein, uo, vto, rot;
+    Buf ub[1], vb[1], wd, wi, mgb, mob, tab, qb, stg, misc, din, ein, uo, vto, rot;
+    hipStream_t zstream = nullptr;     // zero fill of the outputs beside the levels
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // solve start / end, fill fork / join
+    std::vector<int> iota;
     BdWork W;
     long wcap = 0;
     BdOut *hmo = nullptr;  // pinned
@@ -1180,10 +1261,13 @@ int hsdc_bd_ctx_create(hsdc_bd_ctx **ctx)
 int hsdc_bd_ctx_destroy(hsdc_bd_ctx *c)
 {
     if (!c) return 0;
-    Buf *all[] = {&c->ub[0], &c->ub[1], &c->vb[0], &c->vb[1], &c->wd, &c->wi, &c->mgb, &c->mob,
-                  &c->tab, &c->qb, &c->misc, &c->din, &c->ein, &c->uo, &c->vto, &c->rot};
+    Buf *all[] = {&c->ub[0], &c->vb[0], &c->wd, &c->wi, &c->mgb, &c->mob, &c->tab, &c->qb, &c->stg,
+                  &c->misc, &c->din, &c->ein, &c->uo, &c->vto, &c->rot};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
+    for (int i = 0; i < 4; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->zstream) (void)hipStreamDestroy(c->zstream);
     if (c->hmo) (void)hipHostFree(c->hmo);
     delete c;
     return 0;
@@ -1230,9 +1314,9 @@ int hsdc_bd_trace(hsdc_bd_ctx *c, int *s, int *n, int *k, int *nrot, int cap)
 static int bd_alloc(hsdc_bd_ctx *c, int n)
 {
     if (n <= c->wcap) return 0;
-    const size_t nd = 11, ni = 18;
-    BCK(bensure(c->wd, sizeof(double) * (nd * (size_t)n + 2 * (size_t)n)));
-    BCK(bensure(c->wi, sizeof(int) * (ni * (size_t)n + 2 * (size_t)n) + 64));
+    const size_t nd = 13, ni = 36;
+    BCK(bensure(c->wd, sizeof(double) * nd * (size_t)n));
+    BCK(bensure(c->wi, sizeof(int) * ni * (size_t)n + 64));
     double *D = (double *)c->wd.p;
     int *I = (int *)c->wi.p;
     size_t N = n;
@@ -1240,41 +1324,48 @@ static int bd_alloc(hsdc_bd_ctx *c, int n)
     W.D = D; W.E = D + N; W.td = D + 2 * N; W.dsg = D + 3 * N; W.zt = D + 4 * N; W.z = D + 5 * N;
     W.zs = D + 6 * N; W.zl = D + 7 * N; W.sig = D + 8 * N; W.tau = D + 9 * N; W.rcs = D + 11 * N;
     W.idxq = I; W.org = I + N; W.ctyp = I + 2 * N; W.idxc = I + 3 * N; W.idx = I + 4 * N;
-    W.idxp = I + 5 * N; W.surv = I + 6 * N; W.srank = I + 7 * N; W.clf = I + 8 * N; W.gmap = I + 9 * N;
-    W.gsup = I + 10 * N; W.rotf = I + 11 * N; W.ti = I + 12 * N; W.rpq = I + 13 * N;
-    W.iters = (unsigned long long *)(I + 18 * N + 2 * N);
+    W.idxp = I + 5 * N; W.surv = I + 6 * N; W.srank = I + 7 * N; W.clf = I + 8 * N;
+    W.rotf = I + 9 * N; W.ti = I + 10 * N;
+    W.vmU[0] = I + 11 * N; W.vmU[1] = I + 12 * N; W.vmV[0] = I + 13 * N; W.vmV[1] = I + 14 * N;
+    W.sulo = I + 15 * N; W.suhi = I + 16 * N; W.svlo = I + 17 * N; W.svhi = I + 18 * N;
+    W.gU = I + 19 * N; W.gV = I + 20 * N; W.gUlo = I + 21 * N; W.gUhi = I + 22 * N; W.gVlo = I + 23 * N;
+    W.gVhi = I + 24 * N; W.fU = I + 25 * N; W.fV = I + 26 * N; W.rp4 = I + 27 * N;  // 4 N
+    W.iters = (unsigned long long *)(I + 36 * N);
     c->wcap = n;
     return 0;
 }
 
 // the whole SVD of the upper bidiagonal already on the device (W.D, W.E, scaled when
 // n > smlsiz); outputs U, VT (device, may be null for values only)
-static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const std::vector<double> &he,
-                    double orgnrm, double *dU, long ldu, double *dVT, long ldvt, double *dD,
-                    const double *rc, const double *rs)
+static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double orgnrm, double *dU, long ldu,
+                    double *dVT, long ldvt, double *dD, const double *rc, const double *rs)
 {
     hipStream_t st = c->stream;
     const int smlsiz = c->prm.smlsiz;
     const long ld = n;
     std::memset(&c->st, 0, sizeof c->st);
     c->tr.clear();
-    hipEvent_t e0, e1;
-    BHK(hipEventCreate(&e0));
-    BHK(hipEventCreate(&e1));
-    BHK(hipEventRecord(e0, st));
-    for (int b = 0; b < 2; b++) {
-        BCK(bensure(c->ub[b], sizeof(double) * (size_t)n * n));
-        BCK(bensure(c->vb[b], sizeof(double) * (size_t)n * n));
+    if (!c->ev[0])
+        for (int i = 0; i < 4; i++) BHK(hipEventCreateWithFlags(&c->ev[i], i < 2 ? hipEventDefault : hipEventDisableTiming));
+    if (!c->zstream) BHK(hipStreamCreateWithFlags(&c->zstream, hipStreamNonBlocking));
+    BHK(hipEventRecord(c->ev[0], st));
+    // U and VT cleared beside the levels
+    if (dU || dVT) {
+        BHK(hipEventRecord(c->ev[2], st));
+        BHK(hipStreamWaitEvent(c->zstream, c->ev[2], 0));
+        if (dU) hipLaunchKernelGGL(k_bd_zero, dim3(8192), dim3(256), 0, c->zstream, dU, ldu, n);
+        if (dVT) hipLaunchKernelGGL(k_bd_zero, dim3(8192), dim3(256), 0, c->zstream, dVT, ldvt, n);
+        BHK(hipGetLastError());
+        BHK(hipEventRecord(c->ev[3], c->zstream));
     }
-    double *U0 = (double *)c->ub[0].p, *U1 = (double *)c->ub[1].p;
-    double *V0 = (double *)c->vb[0].p, *V1 = (double *)c->vb[1].p;
+    BCK(bensure(c->ub[0], sizeof(double) * (size_t)n * n));
+    BCK(bensure(c->vb[0], sizeof(double) * (size_t)n * n));
+    double *U = (double *)c->ub[0].p, *V = (double *)c->vb[0].p;
     BdWork &W = c->W;
     // subproblems (mdbdsdc.f90:333-372) and their trees (mdlasd0.f90:166-221)
     std::vector<BdLeaf> leaves;
     std::vector<std::vector<BdMerge>> lv;
-    std::vector<int> plo(n), phi(n);
     auto add_problem = [&](int start, int size) {
-        for (int i = start; i < start + size; i++) { plo[i] = start; phi[i] = start + size; }
         if (size <= smlsiz) {
             leaves.push_back({start, size, 0, 0});
             return;
@@ -1282,7 +1373,7 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
         int nlvl, nd;
         std::vector<int> inode, ndiml, ndimr;
         lasdt(size, nlvl, nd, inode, ndiml, ndimr, smlsiz);
-        const int wr0 = nlvl & 1;  // leaves land where the last merge level ends in buffer 0
+        const int wr0 = nlvl & 1;  // leaves use the map the last merge level ends in as map 0
         const int ndb1 = (nd + 1) / 2;
         for (int i = ndb1; i <= nd; i++) {
             int ic = inode[i - 1], nl = ndiml[i - 1], nr = ndimr[i - 1];
@@ -1323,7 +1414,7 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
             }
         }
     }
-    // device tables: leaves, merges of every step
+    // device tables: merges of every step, leaves; U's centre columns map to themselves
     std::vector<int> moff(lv.size() + 1, 0);
     for (size_t t = 0; t < lv.size(); t++) moff[t + 1] = moff[t] + (int)lv[t].size();
     const int nmg = moff.back();
@@ -1335,9 +1426,13 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
         for (auto &v : lv) all.insert(all.end(), v.begin(), v.end());
         if (nmg) BHK(hipMemcpyAsync(dmg, all.data(), sizeof(BdMerge) * nmg, hipMemcpyHostToDevice, st));
         BHK(hipMemcpyAsync(dlv, leaves.data(), sizeof(BdLeaf) * leaves.size(), hipMemcpyHostToDevice, st));
+        if ((int)c->iota.size() != n) {
+            c->iota.resize(n);
+            for (int i = 0; i < n; i++) c->iota[i] = i;
+        }
+        BHK(hipMemcpyAsync(W.vmU[0], c->iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        BHK(hipMemcpyAsync(W.vmU[1], c->iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     }
-    int maxm = 1;
-    for (auto &v : lv) for (auto &M : v) maxm = std::max(maxm, M.nl + M.nr + 1);
     BCK(bensure(c->mob, sizeof(BdOut) * (nmg + 1)));
     BdOut *dmo = (BdOut *)c->mob.p;
     if (c->hmo_cap < (size_t)(nmg + 1)) {
@@ -1356,8 +1451,8 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
     int pmax = 1;
     for (auto &L : leaves) pmax = std::max(pmax, L.m + L.sqre);
     size_t lds = sizeof(double) * (2 * (size_t)pmax * 64 + 64) + sizeof(int) * 64;
-    hipLaunchKernelGGL(k_bd_leaf, dim3((unsigned)leaves.size()), dim3(64), lds, st, dlv, (int)leaves.size(), W, U0,
-                       U1, V0, V1, ld, pmax, dfail);
+    hipLaunchKernelGGL(k_bd_leaf, dim3((unsigned)leaves.size()), dim3(64), lds, st, dlv, (int)leaves.size(), W, U, V,
+                       ld, pmax, dfail);
     BHK(hipGetLastError());
     long long roots = 0, deflated = 0, rots = 0;
     int ktop = 0;
@@ -1366,16 +1461,16 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
         if (!nm) continue;
         BdMerge *mg = dmg + moff[t];
         BdOut *mo = dmo + moff[t];
-        hipLaunchKernelGGL(k_bd_prep, dim3(nm), dim3(PREP_NT), 0, st, mg, W, mo, V0, V1, ld);
+        hipLaunchKernelGGL(k_bd_prep, dim3(nm), dim3(PREP_NT), 0, st, mg, W, mo, (const double *)V, ld);
         hipLaunchKernelGGL(k_bd_deflate, dim3(nm), dim3(DEF_NT), 0, st, mg, W, mo);
         BHK(hipGetLastError());
         BHK(hipMemcpyAsync(c->hmo, mo, sizeof(BdOut) * nm, hipMemcpyDeviceToHost, st));
         BHK(hipStreamSynchronize(st));
         // launch tables of the level
-        std::vector<int> rb_m, rb_r, sb_m, sb_r, lb_m, lb_r, zl_m, cp_m, cp_c;
-        std::vector<long> qo(nm, 0);
+        std::vector<int> rb_m, rb_r, sb_m, sb_r, lb_m, lb_r, zl_m, nc_m, sc_m, sc_c;
+        std::vector<long> qo(nm, 0), so(2 * (size_t)nm, 0);
         std::vector<BdTile> tiles;
-        long qtot = 0;
+        long qtot = 0, stot = 0;
         for (int i = 0; i < nm; i++) {
             const BdMerge &M = lv[t][i];
             const BdOut &o = c->hmo[i];
@@ -1388,6 +1483,10 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
             ktop = k;
             if (o.nrot)
                 for (int r0 = 0; r0 < m_; r0 += 256) { rb_m.push_back(i); rb_r.push_back(r0); }
+            so[2 * i] = stot;
+            stot += (long)n_ * k;
+            so[2 * i + 1] = stot;
+            stot += (long)m_ * (k + M.sqre);
             if (k >= 2) {
                 zl_m.push_back(i);
                 int T = sec_team(k), per = 256 / T;
@@ -1400,10 +1499,9 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
                     for (int r0 = 0; r0 < m_; r0 += 64) tiles.push_back({i, 1, r0, c0});
                 }
             }
-            for (int J = (k == 1 ? 0 : k); J < n_; J++) {
-                if (J == 0 || J >= k) { cp_m.push_back(i); cp_c.push_back(J); }
-            }
-            if (m_ > n_) { cp_m.push_back(i); cp_c.push_back(m_ - 1); }
+            if (k == 1 || m_ > n_) nc_m.push_back(i);
+            for (int j = 0; j < k; j++) { sc_m.push_back(i); sc_c.push_back(j); }
+            for (int j = 0; j < k + M.sqre; j++) { sc_m.push_back(i); sc_c.push_back(j | (1 << 30)); }
         }
         // pack the tables into one upload
         std::vector<char> pk;
@@ -1416,17 +1514,20 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
         size_t o_rbm = put(rb_m.data(), 4 * rb_m.size()), o_rbr = put(rb_r.data(), 4 * rb_r.size());
         size_t o_sbm = put(sb_m.data(), 4 * sb_m.size()), o_sbr = put(sb_r.data(), 4 * sb_r.size());
         size_t o_lbm = put(lb_m.data(), 4 * lb_m.size()), o_lbr = put(lb_r.data(), 4 * lb_r.size());
-        size_t o_zl = put(zl_m.data(), 4 * zl_m.size());
-        size_t o_cpm = put(cp_m.data(), 4 * cp_m.size()), o_cpc = put(cp_c.data(), 4 * cp_c.size());
-        size_t o_qo = put(qo.data(), 8 * qo.size()), o_tl = put(tiles.data(), sizeof(BdTile) * tiles.size());
+        size_t o_zl = put(zl_m.data(), 4 * zl_m.size()), o_nc = put(nc_m.data(), 4 * nc_m.size());
+        size_t o_scm = put(sc_m.data(), 4 * sc_m.size()), o_scc = put(sc_c.data(), 4 * sc_c.size());
+        size_t o_qo = put(qo.data(), 8 * qo.size()), o_so = put(so.data(), 8 * so.size());
+        size_t o_tl = put(tiles.data(), sizeof(BdTile) * tiles.size());
         BCK(bensure(c->tab, pk.size() + 64));
         char *dt = (char *)c->tab.p;
         BHK(hipMemcpyAsync(dt, pk.data(), pk.size(), hipMemcpyHostToDevice, st));
         BCK(bensure(c->qb, sizeof(double) * (size_t)(qtot + 1)));
-        double *Q = (double *)c->qb.p;
+        BCK(bensure(c->stg, sizeof(double) * (size_t)(stot + 1)));
+        double *Q = (double *)c->qb.p, *S = (double *)c->stg.p;
+        const long *dso = (const long *)(dt + o_so);
         if (!rb_m.empty())
             hipLaunchKernelGGL(k_bd_rotate, dim3((unsigned)rb_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_rbm),
-                               (const int *)(dt + o_rbr), (int)rb_m.size(), W, (const BdOut *)mo, U0, U1, V0, V1, ld);
+                               (const int *)(dt + o_rbr), (int)rb_m.size(), W, (const BdOut *)mo, U, V, ld);
         if (!zl_m.empty()) {
             hipLaunchKernelGGL(k_bd_znorm, dim3((unsigned)zl_m.size()), dim3(64), 0, st, mg, (const int *)(dt + o_zl),
                                (int)zl_m.size(), W, mo);
@@ -1441,11 +1542,14 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
                                (const long *)(dt + o_qo), Q);
             hipLaunchKernelGGL(k_bd_gemm, dim3((unsigned)tiles.size()), dim3(256), 0, st, mg,
                                (const BdTile *)(dt + o_tl), (int)tiles.size(), W, (const BdOut *)mo,
-                               (const long *)(dt + o_qo), (const double *)Q, U0, U1, V0, V1, ld);
+                               (const long *)(dt + o_qo), (const double *)Q, (const double *)U, (const double *)V, ld,
+                               dso, S);
         }
-        if (!cp_m.empty())
-            hipLaunchKernelGGL(k_bd_copy, dim3((unsigned)cp_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_cpm),
-                               (const int *)(dt + o_cpc), (int)cp_m.size(), W, (const BdOut *)mo, U0, U1, V0, V1, ld);
+        if (!nc_m.empty())
+            hipLaunchKernelGGL(k_bd_newcol, dim3((unsigned)nc_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_nc),
+                               (int)nc_m.size(), W, (const BdOut *)mo, (const double *)U, (const double *)V, ld, dso, S);
+        hipLaunchKernelGGL(k_bd_scatter, dim3((unsigned)sc_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_scm),
+                           (const int *)(dt + o_scc), (int)sc_m.size(), W, dso, (const double *)S, U, V, ld);
         hipLaunchKernelGGL(k_bd_post, dim3(nm), dim3(256), 0, st, mg, W, (const BdOut *)mo);
         BHK(hipGetLastError());
         c->st.levels++;
@@ -1457,7 +1561,10 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
     BHK(hipMemcpyAsync(D.data(), W.D, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     BHK(hipMemcpyAsync(cnt, W.iters, 16, hipMemcpyDeviceToHost, st));
     BHK(hipStreamSynchronize(st));
-    if ((int)(cnt[1] & 0xffffffffu)) return 1;
+    if ((int)(cnt[1] & 0xffffffffu)) {
+        if (dU || dVT) BHK(hipStreamWaitEvent(st, c->ev[3], 0));
+        return 1;
+    }
     if (n > smlsiz) host_lascl(1.0, orgnrm, n, D.data());
     // selection sort: position i takes the first largest of D(i..n); a segment tree over
     // (value, position) finds it, the swap puts D(i) where the maximum was
@@ -1471,50 +1578,44 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
             if (a < 0) return b;
             if (b < 0) return a;
             if (D[b] > D[a]) return b;
+            if (D[a] > D[b]) return a;
             return a < b ? a : b;
         };
         for (int i = 0; i < n; i++) tr[sz + i] = i;
         for (int i = sz - 1; i >= 1; i--) tr[i] = better(tr[2 * i], tr[2 * i + 1]);
-        auto update = [&](int p) {
-            p += sz;
-            for (p >>= 1; p >= 1; p >>= 1) tr[p] = better(tr[2 * p], tr[2 * p + 1]);
+        auto update = [&](int q) {
+            q += sz;
+            for (q >>= 1; q >= 1; q >>= 1) tr[q] = better(tr[2 * q], tr[2 * q + 1]);
         };
         auto query = [&](int lo) {  // first maximum over [lo, n)
-            int best = -1, l = lo + sz, r = n - 1 + sz + 1;
-            std::vector<int> right;
+            int best = -1, l = lo + sz, r = n + sz;
             while (l < r) {
                 if (l & 1) best = better(best, tr[l++]);
-                if (r & 1) right.push_back(tr[--r]);
+                if (r & 1) best = better(best, tr[--r]);
                 l >>= 1;
                 r >>= 1;
             }
-            for (int k = (int)right.size() - 1; k >= 0; k--) best = better(best, right[k]);
             return best;
         };
         for (int i = 0; i < n - 1; i++) {
             int kk = query(i);
-            // DLASR's rule: strictly larger than D(i) moves; ties keep the first
             if (kk != i && D[kk] > D[i]) {
                 std::swap(D[kk], D[i]);
                 std::swap(perm[kk], perm[i]);
-                tr[sz + kk] = kk;
-                tr[sz + i] = i;
                 update(kk);
                 update(i);
             }
         }
     }
     // outputs
-    BCK(bensure(c->misc, sizeof(int) * 3 * (size_t)n + 64));
-    int *dperm = (int *)c->misc.p, *dplo = dperm + n, *dphi = dplo + n;
+    BCK(bensure(c->misc, sizeof(int) * (size_t)n + 64));
+    int *dperm = (int *)c->misc.p;
     BHK(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    BHK(hipMemcpyAsync(dplo, plo.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    BHK(hipMemcpyAsync(dphi, phi.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     if (dD) BHK(hipMemcpyAsync(dD, D.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
     if (dU || dVT) {
-        hipLaunchKernelGGL(k_bd_out, dim3(bcdiv(n, 64), bcdiv(n, 64)), dim3(256), 0, st, (const double *)U0,
-                           (const double *)V0, ld, (const int *)dperm, (const int *)dplo, (const int *)dphi, n, dU,
-                           ldu, dVT, ldvt);
+        BHK(hipStreamWaitEvent(st, c->ev[3], 0));
+        hipLaunchKernelGGL(k_bd_fin, dim3(n), dim3(256), 0, st, W, (const double *)U, (const double *)V, ld,
+                           (const int *)dperm, n, dU, ldu, dVT, ldvt);
         BHK(hipGetLastError());
         if (rc && dU) {
             BCK(bensure(c->rot, sizeof(double) * 2 * (size_t)n));
@@ -1526,12 +1627,10 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &hd, const 
             BHK(hipGetLastError());
         }
     }
-    BHK(hipEventRecord(e1, st));
-    BHK(hipEventSynchronize(e1));
+    BHK(hipEventRecord(c->ev[1], st));
+    BHK(hipEventSynchronize(c->ev[1]));
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
     c->st.ms_total = ms;
     c->st.roots = roots;
     c->st.deflated = deflated;
@@ -1618,9 +1717,13 @@ int hsdc_bdsdc_dev(hsdc_bd_ctx *c, char uplo, int n, double *d_d, double *d_e, d
     BCK(bd_alloc(c, n));
     BHK(hipMemcpyAsync(c->W.D, hd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
     BHK(hipMemcpyAsync(c->W.E, he.data(), sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
-    int info = bd_solve(c, n, hd, he, orgnrm, d_u, ldu, d_vt, ldvt, d_d, iuplo == 2 ? rc.data() : nullptr,
+    int info = bd_solve(c, n, he, orgnrm, d_u, ldu, d_vt, ldvt, d_d, iuplo == 2 ? rc.data() : nullptr,
                         iuplo == 2 ? rs.data() : nullptr);
-    if (info) return info;
+    if (info) {
+        // the fill of the outputs may still run on the second stream
+        if (c->zstream) (void)hipStreamSynchronize(c->zstream);
+        return info;
+    }
     BHK(hipStreamSynchronize(st));
     return 0;
 }

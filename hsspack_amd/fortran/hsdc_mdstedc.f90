@@ -1,5 +1,6 @@
 ! Fortran drop-ins for HSSPACK's tridiagonal divide-and-conquer entry points
-! (SRC/mdstedc.f90, mdlaed0.f90, mdlaed1.f90, mdlaed2.f90, mdlaed3.f90): the same
+! (SRC/mdstedc.f90, mdlaed0.f90, mdlaed1.f90, mdlaed2.f90, mdlaed3.f90) and its
+! bidiagonal SVD driver (SRC/mdbdsdc.f90): the same
 ! names, argument lists, INFO codes and XERBLA calls, with the work done by the
 ! MI355X library libhsdc.so through its C interface (include/hsdc.h).
 !
@@ -78,6 +79,23 @@ module hsdc_c
          real(c_double), value :: tol
          real(c_double) :: f(*), d(*), u(*), v(*), difl(*), difr(*), q(*)
          type(c_ptr), value :: ranks
+      end function
+      integer(c_int) function hsdc_bd_ctx_create(ctx) bind(C, name='hsdc_bd_ctx_create')
+         import :: c_int, c_ptr
+         type(c_ptr) :: ctx
+      end function
+      integer(c_int) function hsdc_bd_ctx_destroy(ctx) bind(C, name='hsdc_bd_ctx_destroy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function hsdc_bdsdc(ctx, uplo, compq, n, d, e, u, ldu, vt, ldvt, q, iq, work, iwork) &
+            bind(C, name='hsdc_bdsdc')
+         import :: c_int, c_ptr, c_char, c_double
+         type(c_ptr), value :: ctx
+         character(kind=c_char), value :: uplo, compq
+         integer(c_int), value :: n, ldu, ldvt
+         real(c_double) :: d(*), e(*), u(*), vt(*), q(*), work(*)
+         integer(c_int) :: iq(*), iwork(*)
       end function
    end interface
 
@@ -451,4 +469,29 @@ subroutine mdlaed1(n, d, q, ldq, indxq, rho, cutpnt, work, iwork, info, wflag)
          indxq(i) = i
       end do
    end if
+end subroutine
+
+
+! MDBDSDC (mdbdsdc.f90:1-2).  COMPQ 'N' or 'I'; 'P' (the compact form) returns
+! INFO = -2.  Q/IQ/WORK/IWORK are accepted and unused.
+subroutine mdbdsdc(uplo, compq, n, d, e, u, ldu, vt, ldvt, q, iq, work, iwork, info)
+   use hsdc_c
+   implicit none
+   character :: uplo, compq
+   integer :: n, ldu, ldvt, info
+   double precision :: d(*), e(*), u(ldu, *), vt(ldvt, *), q(*), work(*)
+   integer :: iq(*), iwork(*)
+   type(c_ptr) :: ctx
+   integer(c_int) :: rc
+   info = 0
+   rc = hsdc_bd_ctx_create(ctx)
+   if (rc /= 0) then
+      info = 1003
+      return
+   end if
+   rc = hsdc_bdsdc(ctx, uplo, compq, n, d, e, u, ldu, vt, ldvt, q, iq, work, iwork)
+   rc = int(hsdc_info(rc), c_int)
+   if (hsdc_bd_ctx_destroy(ctx) /= 0) rc = 1003
+   info = rc
+   if (info < 0) call xerbla('DBDSDC', -info)
 end subroutine

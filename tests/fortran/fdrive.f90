@@ -5,6 +5,7 @@
 !   laed0  IN: int32 icompq, qsiz, n, d(n), e(n-1), q(qsiz,n)    OUT: info, d(n), q(qsiz,n)
 !   laed1  IN: int32 n, cutpnt, float64 rho, d(n), q(n,n), int32 indxq(n)
 !          OUT: info, wflag, d(n), q(n,n), indxq(n)
+!   bdsdc  IN: int32 n, uplo (int32), float64 d(n), e(n-1)  OUT: int32 info, d(n), u(n,n), vt(n,n)
 !   errors OUT: for each illegal call, info and the XERBLA (name, argument) it made
 module xerbla_log
    implicit none
@@ -30,7 +31,9 @@ program fdrive
    double precision, allocatable :: d(:), e(:), q(:, :), work(:), qs(:, :)
    integer, allocatable :: iwork(:), indxq(:)
    character :: compz
-   integer :: u, v
+   integer :: u, v, cu
+   character :: uplo
+   double precision, allocatable :: vt(:, :)
    call get_command_argument(1, mode)
    call get_command_argument(2, fin)
    call get_command_argument(3, fout)
@@ -77,6 +80,16 @@ program fdrive
       wflag = 0
       call mdlaed1(n, d, q, n, indxq, rho, cutpnt, work, iwork, info, wflag)
       write(v) info, wflag, d, q, indxq
+   case ('bdsdc')
+      open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+      read(u) n, cu
+      uplo = achar(cu)
+      allocate(d(n), e(max(n - 1, 1)), q(n, n), vt(n, n), work(1), iwork(1), indxq(1))
+      read(u) d
+      if (n > 1) read(u) e(1:n - 1)
+      close(u)
+      call mdbdsdc(uplo, 'I', n, d, e, q, n, vt, n, work, indxq, work, iwork, info)
+      write(v) info, d, q, vt
    case ('errors')
       allocate(d(8), e(8), q(8, 8), work(200), iwork(200), indxq(8), qs(8, 8))
       d = 1
@@ -94,6 +107,16 @@ program fdrive
       call mdlaed1(8, d, q, 8, indxq, rho, 5, work, iwork, info, wflag)
       write(v) info, last_arg, last_name
       call mdlaed3(9, 8, d, q, 8, rho, d, q, d, d, d, d, work, info)
+      write(v) info, last_arg, last_name
+      call mdbdsdc('X', 'I', 8, d, e, q, 8, qs, 8, work, iwork, work, iwork, info)
+      write(v) info, last_arg, last_name
+      call mdbdsdc('U', 'X', 8, d, e, q, 8, qs, 8, work, iwork, work, iwork, info)
+      write(v) info, last_arg, last_name
+      call mdbdsdc('U', 'I', -1, d, e, q, 8, qs, 8, work, iwork, work, iwork, info)
+      write(v) info, last_arg, last_name
+      call mdbdsdc('U', 'I', 8, d, e, q, 4, qs, 8, work, iwork, work, iwork, info)
+      write(v) info, last_arg, last_name
+      call mdbdsdc('L', 'I', 8, d, e, q, 8, qs, 2, work, iwork, work, iwork, info)
       write(v) info, last_arg, last_name
    end select
    close(v)

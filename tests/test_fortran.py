@@ -34,16 +34,17 @@ def f64(a):
 
 def test_library_exports_the_fortran_names():
     out = subprocess.run(["nm", "-D", "--defined-only", FLIB], capture_output=True, text=True, check=True).stdout
-    for name in ("mdstedc_", "mdlaed0_", "mdlaed1_", "mdlaed2_", "mdlaed3_"):
+    for name in ("mdstedc_", "mdlaed0_", "mdlaed1_", "mdlaed2_", "mdlaed3_", "mdbdsdc_"):
         assert (" T " + name) in out, name
 
 
 def test_argument_errors_call_xerbla_like_the_reference(tmp_path):
     """Illegal arguments return INFO = -i and call XERBLA with the LAPACK routine name
-    (mdstedc.f90:241, mdlaed0.f90:179, mdlaed1.f90:153, mdlaed3.f90:176); no device needed."""
+    (mdstedc.f90:241, mdlaed0.f90:179, mdlaed1.f90:153, mdlaed3.f90:176, mdbdsdc.f90:174-187); no
+    device needed."""
     raw = run("errors", tmp_path)
     want = [(-1, "DSTEDC"), (-6, "DSTEDC"), (-8, "DSTEDC"), (-1, "DLAED0"), (-9, "DLAED0"), (-7, "DLAED1"),
-            (-2, "DLAED3")]
+            (-2, "DLAED3"), (-1, "DBDSDC"), (-2, "DBDSDC"), (-3, "DBDSDC"), (-7, "DBDSDC"), (-9, "DBDSDC")]
     rec = 14
     assert len(raw) == rec * len(want)
     for i, (info, name) in enumerate(want):
@@ -142,3 +143,21 @@ def test_mdlaed1_vs_oracle(tmp_path, cfg, n):
     assert np.array_equal(lam[k:], r_d[k:])
     assert np.array_equal(iq, r_iq)
     assert np.abs(qq - r_q).max() <= 1e-11
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,uplo", [(300, "U"), (777, "L"), (20, "U")])
+def test_mdbdsdc_vs_oracle(tmp_path, n, uplo):
+    """MDBDSDC from Fortran: singular values, U and VT bit-exact against the oracle."""
+    from hsspack_amd.workloads import bidiag_cfg
+    d, e = bidiag_cfg(n, seed=11)
+    raw = run("bdsdc", tmp_path, i32(n, ord(uplo)) + f64(d) + f64(e))
+    info = np.frombuffer(raw[:4], "<i4")[0]
+    s = np.frombuffer(raw[4:4 + 8 * n], "<f8")
+    u = np.frombuffer(raw[4 + 8 * n:4 + 8 * n + 8 * n * n], "<f8").reshape(n, n).T
+    vt = np.frombuffer(raw[4 + 8 * n + 8 * n * n:], "<f8").reshape(n, n).T
+    assert info == 0
+    r = O.bdsdc(d, e, uplo=uplo.encode())
+    assert np.array_equal(s, r["d"])
+    assert np.array_equal(u, r["u"])
+    assert np.array_equal(vt, r["vt"])

@@ -164,6 +164,62 @@ __device__ int block_exscan(int v, int *sh, int &total)
     return r;
 }
 
+// Ranks the flagged positions of [lo, hi) in position order: each wavefront walks one
+// contiguous segment 64 positions at a time (ballot + popcount), a block scan over the
+// wavefront counts gives each segment its base.  emit(p, rank) for every flagged p;
+// returns the total.  flag must be free of side effects (it is evaluated twice).
+template <typename F, typename G>
+__device__ int wave_rank(int lo, int hi, F flag, G emit, int *sh)
+{
+    const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int len = hi > lo ? hi - lo : 0;
+    const int seg = (((len + nw - 1) / nw) + 63) & ~63;
+    const int a = lo + w * seg, b = min(a + seg, hi);
+    int cnt = 0;
+    for (int p0 = a; p0 < b; p0 += 64) {
+        const int p = p0 + lane;
+        cnt += __popcll(__ballot(p < b && flag(p)));
+    }
+    if (lane == 0) sh[w] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int i = 0; i < nw; i++) {
+            int c = sh[i];
+            sh[i] = acc;
+            acc += c;
+        }
+        sh[nw] = acc;
+    }
+    __syncthreads();
+    int base = sh[w];
+    const int total = sh[nw];
+    __syncthreads();
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int p0 = a; p0 < b; p0 += 64) {
+        const int p = p0 + lane;
+        const bool f = p < b && flag(p);
+        const unsigned long long msk = __ballot(f);
+        if (f) emit(p, base + __popcll(msk & below));
+        base += __popcll(msk);
+    }
+    return total;
+}
+
+// merge path: how many of the first diag outputs of the stable merge of ascending
+// a (na) and b (nb) come from a (ties take a first); bget(j) reads b[j]
+template <typename B>
+__device__ int corank(int diag, const double *a, int na, B bget, int nb)
+{
+    int lo = diag - nb > 0 ? diag - nb : 0, hi = diag < na ? diag : na;
+    while (lo < hi) {
+        const int i = (lo + hi) >> 1, j = diag - i;
+        if (j > 0 && a[i] <= bget(j - 1)) lo = i + 1;
+        else hi = i;
+    }
+    return lo;
+}
+
 __device__ double block_max(double v, double *sh)
 {
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -308,7 +364,7 @@ __global__ __launch_bounds__(64) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdW
 // children's right vectors, the shift, DSIGMA by IDXQ, DLAMRG and the sort
 // (mdlasd2.f90:216-262), TOL (mdlasd2.f90:266-268)
 // ------------------------------------------------------------------------------
-#define PREP_NT 256
+#define PREP_NT 1024
 __global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W, BdOut *mo, const double *V,
                                                      long ld)
 {
@@ -363,31 +419,18 @@ __global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W
         W.idxc[s + i] = W.ctyp[s + q];
     }
     __syncthreads();
-    // DLAMRG(NL, NR, DSIGMA(2), 1, 1, IDX(2)): ties take the left run first
-    const double *a = W.dsg + s + 1, *b = W.dsg + s + 1 + nl;
-    for (int i = t; i < nl + nr; i += PREP_NT) {
-        int pos, val;
-        if (i < nl) {
-            double x = a[i];
-            int lo = 0, hi = nr;
-            while (lo < hi) {
-                int md = (lo + hi) >> 1;
-                if (b[md] < x) lo = md + 1; else hi = md;
-            }
-            pos = i + lo;
-            val = i + 1;
-        } else {
-            int j = i - nl;
-            double x = b[j];
-            int lo = 0, hi = nl;
-            while (lo < hi) {
-                int md = (lo + hi) >> 1;
-                if (a[md] <= x) lo = md + 1; else hi = md;
-            }
-            pos = j + lo;
-            val = nl + j + 1;
+    // DLAMRG(NL, NR, DSIGMA(2), 1, 1, IDX(2)): ties take the left run first; merge path,
+    // each thread places a contiguous run of outputs
+    {
+        const double *a = W.dsg + s + 1, *b = W.dsg + s + 1 + nl;
+        const int tot = nl + nr, per = (tot + PREP_NT - 1) / PREP_NT;
+        const int o0 = min(t * per, tot), o1 = min(o0 + per, tot);
+        auto bget = [b](int j) { return b[j]; };
+        int i = corank(o0, a, nl, bget, nr), j = o0 - i;
+        for (int o = o0; o < o1; o++) {
+            if (i < nl && (j >= nr || a[i] <= b[j])) W.idx[s + 1 + o] = ++i;
+            else W.idx[s + 1 + o] = nl + (++j);
         }
-        W.idx[s + 1 + pos] = val;
     }
     __syncthreads();
     for (int i = t + 1; i < n; i += PREP_NT) {
@@ -434,80 +477,56 @@ __global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork
     const int s = M.s, nl = M.nl, nr = M.nr, n = nl + nr + 1, m = n + M.sqre, t = threadIdx.x;
     BdOut &o = mo[blockIdx.x];
     const double tol = o.tol;
-    const int np = n - 1;  // positions 1..n-1
-    const int per = (np + DEF_NT - 1) / DEF_NT;
-    const int p0 = 1 + t * per, p1 = min(1 + (t + 1) * per, n);
-    // survivors
-    int c = 0;
-    for (int p = p0; p < p1; p++) c += !(fabs(W.z[s + p]) <= tol);
-    int ns, base = block_exscan<DEF_NT>(c, sh, ns);
-    for (int p = p0; p < p1; p++) {
-        if (!(fabs(W.z[s + p]) <= tol)) {
-            W.surv[s + base] = p;
-            W.srank[s + p] = base;
-            base++;
-        } else {
-            W.srank[s + p] = -1;
-        }
+    double *z = W.z + s;
+    const double *D = W.D + s;
+    int *surv = W.surv + s, *srank = W.srank + s, *clf = W.clf + s, *idxp = W.idxp + s, *ctyp = W.ctyp + s;
+    for (int i = t; i < n; i += DEF_NT) {
+        srank[i] = -1;
+        W.rotf[s + i] = 0;
     }
-    for (int i = t; i < n; i += DEF_NT) W.rotf[s + i] = 0;
     __syncthreads();
-    const int pt = (ns + DEF_NT - 1) / DEF_NT;
-    const int t0 = t * pt, t1 = min((t + 1) * pt, ns);
-    for (int q = t0; q < t1; q++)
-        W.clf[s + q] = (q >= 1) && (fabs(W.D[s + W.surv[s + q]] - W.D[s + W.surv[s + q - 1]]) <= tol);
+    // survivors of the small-z test, in position order
+    const int ns = wave_rank(1, n, [&](int p) { return !(fabs(z[p]) <= tol); },
+                             [&](int p, int r) { surv[r] = p; srank[p] = r; }, sh);
     __syncthreads();
-    // events in position order
-    c = 0;
-    for (int p = p0; p < p1; p++) {
-        int r = W.srank[s + p];
-        c += (r < 0) || W.clf[s + r];
-    }
-    int nev, eb = block_exscan<DEF_NT>(c, sh, nev);
-    for (int p = p0; p < p1; p++) {
-        int r = W.srank[s + p];
-        if (r < 0) {
-            W.idxp[s + n - 1 - eb] = p + 1;
-            W.ctyp[s + p] = 4;
-            eb++;
-        } else if (W.clf[s + r]) {
-            W.idxp[s + n - 1 - eb] = W.surv[s + r - 1] + 1;
-            eb++;
-        }
-    }
+    // a survivor within TOL of the previous survivor rotates that one out
+    for (int q = t; q < ns; q += DEF_NT) clf[q] = (q >= 1) && (fabs(D[surv[q]] - D[surv[q - 1]]) <= tol);
+    __syncthreads();
+    // deflation events in position order fill IDXP from the back
+    wave_rank(1, n, [&](int p) { int r = srank[p]; return r < 0 || clf[r]; },
+              [&](int p, int e) {
+                  int r = srank[p];
+                  if (r < 0) {
+                      idxp[n - 1 - e] = p + 1;
+                      ctyp[p] = 4;
+                  } else {
+                      idxp[n - 1 - e] = surv[r - 1] + 1;
+                  }
+              }, sh);
     // rotation ranks and the non-deflated survivors, in survivor order
-    int cr = 0, ck = 0;
-    for (int q = t0; q < t1; q++) {
-        cr += W.clf[s + q];
-        ck += (q == ns - 1) || !W.clf[s + q + 1];
-    }
-    int nrot, rb = block_exscan<DEF_NT>(cr, sh, nrot);
-    int nk, kb = block_exscan<DEF_NT>(ck, sh, nk);
-    for (int q = t0; q < t1; q++) {
-        if (W.clf[s + q]) W.ti[s + q] = rb++;
-        if ((q == ns - 1) || !W.clf[s + q + 1]) {
-            W.idxp[s + 1 + kb] = W.surv[s + q] + 1;
-            W.srank[s + W.surv[s + q]] = -2 - kb;  // marks a non-deflated survivor
-            kb++;
-        }
-    }
+    const int nrot = wave_rank(0, ns, [&](int q) { return clf[q] != 0; }, [&](int q, int r) { W.ti[s + q] = r; }, sh);
+    const int nk = wave_rank(0, ns, [&](int q) { return q == ns - 1 || !clf[q + 1]; },
+                             [&](int q, int r) {
+                                 idxp[1 + r] = surv[q] + 1;
+                                 srank[surv[q]] = -2 - r;  // marks a non-deflated survivor
+                             }, sh);
     __syncthreads();
     // chains of rotations
-    for (int q = t0; q < t1; q++) {
-        if (W.clf[s + q] || q + 1 >= ns || !W.clf[s + q + 1]) continue;
-        double zc = W.z[s + W.surv[s + q]];
-        int ctc = W.ctyp[s + W.surv[s + q]];
+    for (int q = t; q < ns; q += DEF_NT) {
+        if (clf[q] || q + 1 >= ns || !clf[q + 1]) continue;
+        double zc = z[surv[q]];
+        int ctc = ctyp[surv[q]];
         int q2 = q + 1;
-        for (; q2 < ns && W.clf[s + q2]; q2++) {
-            int pj = W.surv[s + q2 - 1], qj = W.surv[s + q2];
-            double sv = zc, cv = W.z[s + qj];
+        for (; q2 < ns && clf[q2]; q2++) {
+            int pj = surv[q2 - 1], qj = surv[q2];
+            double sv = zc, cv = z[qj];
             double tau = bd_lapy2(cv, sv);
             double cc = cv / tau, ss = -sv / tau;
-            W.z[s + pj] = 0.0;
+            z[pj] = 0.0;
             zc = tau;
-            int ctq = W.ctyp[s + qj];
+            int ctq = ctyp[qj];
             if (ctq != ctc) ctq = 3;
-            W.ctyp[s + pj] = 4;
+            ctyp[pj] = 4;
             ctc = ctq;
             int rk = W.ti[s + q2];
             int cp = bd_colidx(W, s, nl, pj), cq = bd_colidx(W, s, nl, qj);
@@ -520,18 +539,18 @@ __global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork
             W.rotf[s + cp] = 1;
             W.rotf[s + cq] = 1;
         }
-        W.z[s + W.surv[s + q2 - 1]] = zc;
-        W.ctyp[s + W.surv[s + q2 - 1]] = ctc;
+        z[surv[q2 - 1]] = zc;
+        ctyp[surv[q2 - 1]] = ctc;
     }
     __syncthreads();
     // U2(K,1) = Z(JPREV) of the non-deflated survivors; column-type totals
     if (t < 4) cnt[t] = 0;
     __syncthreads();
     int lc[4] = {0, 0, 0, 0};
-    for (int p = p0; p < p1; p++) {
-        int r = W.srank[s + p];
-        if (r <= -2) W.zt[s + 1 + (-2 - r)] = W.z[s + p];
-        lc[W.ctyp[s + p] - 1]++;
+    for (int p = 1 + t; p < n; p += DEF_NT) {
+        int r = srank[p];
+        if (r <= -2) W.zt[s + 1 + (-2 - r)] = z[p];
+        lc[ctyp[p] - 1]++;
     }
     for (int j = 0; j < 4; j++)
         if (lc[j]) atomicAdd(&cnt[j], lc[j]);
@@ -539,18 +558,16 @@ __global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork
     const int k = 1 + nk;
     const int ct0 = cnt[0], ct1 = cnt[1], ct2 = cnt[2], ct3 = cnt[3];
     // IDXC: positions J = 2..N grouped by the type of column IDXP(J), stable
-    int psm[4] = {1, 1 + ct0, 1 + ct0 + ct1, 1 + ct0 + ct1 + ct2};  // 0-based starts
+    const int psm[4] = {1, 1 + ct0, 1 + ct0 + ct1, 1 + ct0 + ct1 + ct2};  // 0-based starts
     for (int ty = 0; ty < 4; ty++) {
-        int cc2 = 0;
-        for (int p = p0; p < p1; p++) cc2 += (W.ctyp[s + W.idxp[s + p] - 1] == ty + 1);
-        int tot, b2 = block_exscan<DEF_NT>(cc2, sh, tot);
-        for (int p = p0; p < p1; p++)
-            if (W.ctyp[s + W.idxp[s + p] - 1] == ty + 1) W.idxc[s + psm[ty] + b2++] = p + 1;
+        const int base = psm[ty];
+        wave_rank(1, n, [&](int p) { return ctyp[idxp[p] - 1] == ty + 1; },
+                  [&](int p, int r) { W.idxc[s + base + r] = p + 1; }, sh);
     }
     __syncthreads();
     // DSIGMA(J) = D(IDXP(J)); slots of U2(:,J) and VT2(J,:) and their supports (the
     // merge block for a column a rotation touched)
-    for (int p = p0; p < p1; p++) {
+    for (int p = 1 + t; p < n; p += DEF_NT) {
         W.dsg[s + p] = W.D[s + W.idxp[s + p] - 1];
         int src = bd_colidx(W, s, nl, W.idxp[s + W.idxc[s + p] - 1] - 1);
         int su = W.vmU[M.rd][s + src], sv = W.vmV[M.rd][s + src];
@@ -1013,53 +1030,38 @@ __global__ __launch_bounds__(256) void k_bd_scatter(const BdMerge *mg, const int
 // merge step 4 (block per merge): D(1:K) = roots, D(K+1:N) = deflated values, both
 // unscaled, IDXQ = DLAMRG(K, N-K, D, 1, -1) (mdlasd1.f90:282-286)
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bd_post(const BdMerge *mg, BdWork W, const BdOut *mo)
+#define POST_NT 1024
+__global__ __launch_bounds__(POST_NT) void k_bd_post(const BdMerge *mg, BdWork W, const BdOut *mo)
 {
     const BdMerge M = mg[blockIdx.x];
     const BdOut &o = mo[blockIdx.x];
     const int s = M.s, n = M.nl + M.nr + 1, k = o.k, t = threadIdx.x;
     double mul[8];
     const int nm = lascl_muls(1.0, o.orgnrm, mul);
-    for (int i = t; i < n; i += 256) {
+    for (int i = t; i < n; i += POST_NT) {
         double v;
         if (i < k) v = (k == 1) ? fabs(o.zr1) : W.sig[s + i];
         else v = W.dsg[s + i];
         W.D[s + i] = apply_muls(v, mul, nm);
     }
     __syncthreads();
-    const double *a = W.D + s;
-    const int n2 = n - k;
-    for (int i = t; i < n; i += 256) {
-        int pos, val;
-        if (i < k) {
-            double x = a[i];
-            int lo = 0, hi = n2;  // run 2 ascending: b_j = a[n-1-j]
-            while (lo < hi) {
-                int md = (lo + hi) >> 1;
-                if (a[n - 1 - md] < x) lo = md + 1; else hi = md;
-            }
-            pos = i + lo;
-            val = i + 1;
-        } else {
-            int j = i - k;
-            double x = a[n - 1 - j];
-            int lo = 0, hi = k;
-            while (lo < hi) {
-                int md = (lo + hi) >> 1;
-                if (a[md] <= x) lo = md + 1; else hi = md;
-            }
-            pos = j + lo;
-            val = n - j;
+    // DLAMRG(K, N-K, D, 1, -1, IDXQ): run 2 read backwards; merge path
+    {
+        const double *a = W.D + s;
+        const int n2 = n - k, per = (n + POST_NT - 1) / POST_NT;
+        const int o0 = min(t * per, n), o1 = min(o0 + per, n);
+        auto bget = [a, n](int j) { return a[n - 1 - j]; };
+        int i = corank(o0, a, k, bget, n2), j = o0 - i;
+        for (int o = o0; o < o1; o++) {
+            if (i < k && (j >= n2 || a[i] <= a[n - 1 - j])) W.idxq[s + o] = ++i;
+            else W.idxq[s + o] = n - (j++);
         }
-        W.ti[s + pos] = val;
     }
-    __syncthreads();
-    for (int i = t; i < n; i += 256) W.idxq[s + i] = W.ti[s + i];
     // the block's new maps: new columns in freed slots (supported on the whole block),
     // deflated ones where they were (the block once a rotation touched them)
     const int m = n + M.sqre;
     int *vu = W.vmU[M.wr], *vv = W.vmV[M.wr];
-    for (int i = t; i < n; i += 256) {
+    for (int i = t; i < n; i += POST_NT) {
         int su, sv;
         if (i < k) {
             su = W.fU[s + i];
@@ -1550,7 +1552,7 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
                                (int)nc_m.size(), W, (const BdOut *)mo, (const double *)U, (const double *)V, ld, dso, S);
         hipLaunchKernelGGL(k_bd_scatter, dim3((unsigned)sc_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_scm),
                            (const int *)(dt + o_scc), (int)sc_m.size(), W, dso, (const double *)S, U, V, ld);
-        hipLaunchKernelGGL(k_bd_post, dim3(nm), dim3(256), 0, st, mg, W, (const BdOut *)mo);
+        hipLaunchKernelGGL(k_bd_post, dim3(nm), dim3(POST_NT), 0, st, mg, W, (const BdOut *)mo);
         BHK(hipGetLastError());
         c->st.levels++;
         c->st.merges += nm;

@@ -265,35 +265,41 @@ __global__ __launch_bounds__(64) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdW
     for (int j = 0; j < pc - 1; j++) fro = fro + e[j] * e[j];
     const double thr = (JAC_TOL * pc) * (JAC_TOL * pc) * fro;
     __syncthreads();
+    // squared column norms, each the butterfly sum of the column as it now stands (the
+    // same value a fresh sum would give), recomputed only when a rotation changes it;
+    // every lane holds and writes the same values (one wavefront)
+    for (int j = 0; j < pc; j++) {
+        double w = Wc[j * 64 + r];
+        nn[j] = wsum64(w * w);
+    }
     int sweep, rot = 1;
     for (sweep = 0; sweep < JAC_SWEEPS && rot; sweep++) {
         rot = 0;
         for (int p = 0; p < pc - 1; p++) {
             for (int q = p + 1; q < pc; q++) {
                 double wp = Wc[p * 64 + r], wq = Wc[q * 64 + r];
-                double al = wsum64(wp * wp);
-                double be = wsum64(wq * wq);
+                double al = nn[p];
+                double be = nn[q];
                 double ga = wsum64(wp * wq);
                 if (al <= thr || be <= thr || !(fabs(ga) > (JAC_TOL * pc) * (sqrt(al) * sqrt(be)))) continue;
                 double zeta = (be - al) / (2.0 * ga), t;
                 if (fabs(zeta) > 1e150) t = 0.5 / zeta;
                 else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
-                Wc[p * 64 + r] = c * wp - sn * wq;
-                Wc[q * 64 + r] = sn * wp + c * wq;
+                const double np_ = c * wp - sn * wq, nq_ = sn * wp + c * wq;
+                Wc[p * 64 + r] = np_;
+                Wc[q * 64 + r] = nq_;
                 double vp = Vc[p * 64 + r], vq = Vc[q * 64 + r];
                 Vc[p * 64 + r] = c * vp - sn * vq;
                 Vc[q * 64 + r] = sn * vp + c * vq;
+                const double ap = wsum64(np_ * np_), aq = wsum64(nq_ * nq_);
+                nn[p] = ap;
+                nn[q] = aq;
                 rot = 1;
             }
         }
     }
     if (rot && r == 0) atomicAdd(fail, 1);
-    for (int j = 0; j < pc; j++) {
-        double w = Wc[j * 64 + r];
-        double v = wsum64(w * w);
-        if (r == 0) nn[j] = v;
-    }
     __syncthreads();
     if (r == 0) {
         int used[64];
@@ -1114,6 +1120,25 @@ __global__ __launch_bounds__(256) void k_bd_fin(const BdWork W, const double *Us
     }
 }
 
+// MDBDSDC's final order for one subproblem (mdbdsdc.f90:383-404): with distinct values
+// the selection sort puts the i-th largest singular value (and its vectors) at
+// position i, i.e. the top merge's ascending IDXQ read backwards.  Writes the order,
+// the unscaled values in it, and flags any two equal values (the selection sort's
+// order of equal values is then reproduced on the host).
+__global__ __launch_bounds__(1024) void k_bd_perm(const BdWork W, int n, double orgnrm, int scaled, int *perm,
+                                                  double *Dout, int *tie)
+{
+    double mul[8];
+    const int nm = scaled ? lascl_muls(1.0, orgnrm, mul) : 0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const int src = W.idxq[n - 1 - i] - 1;
+        const double v = apply_muls(W.D[src], mul, nm);
+        perm[i] = src;
+        Dout[i] = v;
+        if (i + 1 < n && v == apply_muls(W.D[W.idxq[n - 2 - i] - 1], mul, nm)) atomicOr(tie, 1);
+    }
+}
+
 // DLASR('L','V','B') on the columns of U: the rotations that made a lower bidiagonal
 // upper (mdbdsdc.f90:413-414); thread per column
 __global__ __launch_bounds__(256) void k_bd_lasr(double *U, long ldu, int n, const double *cs, const double *sn)
@@ -1228,7 +1253,7 @@ struct hsdc_bd_ctx {
     hsdc_bd_params prm;
     hsdc_bd_stats st;
     hipStream_t stream = 0;
-    Buf ub[1], vb[1], wd, wi, mgb, mob, tab, qb, stg, misc, din, ein, uo, vto, rot;
+    Buf ub[1], vb[1], wd, wi, mgb, mob, tab, qb, stg, misc, din, din2, ein, uo, vto, rot;
     hipStream_t zstream = nullptr;     // zero fill of the outputs beside the levels
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // solve start / end, fill fork / join
     std::vector<int> iota;
@@ -1264,7 +1289,7 @@ int hsdc_bd_ctx_destroy(hsdc_bd_ctx *c)
 {
     if (!c) return 0;
     Buf *all[] = {&c->ub[0], &c->vb[0], &c->wd, &c->wi, &c->mgb, &c->mob, &c->tab, &c->qb, &c->stg,
-                  &c->misc, &c->din, &c->ein, &c->uo, &c->vto, &c->rot};
+                  &c->misc, &c->din, &c->din2, &c->ein, &c->uo, &c->vto, &c->rot};
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 4; i++)
@@ -1367,7 +1392,9 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
     // subproblems (mdbdsdc.f90:333-372) and their trees (mdlasd0.f90:166-221)
     std::vector<BdLeaf> leaves;
     std::vector<std::vector<BdMerge>> lv;
+    int nprob = 0;
     auto add_problem = [&](int start, int size) {
+        nprob++;
         if (size <= smlsiz) {
             leaves.push_back({start, size, 0, 0});
             return;
@@ -1557,22 +1584,57 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
         c->st.levels++;
         c->st.merges += nm;
     }
-    // singular values back to the host: unscale, MDBDSDC's selection sort (mdbdsdc.f90:383-404)
-    std::vector<double> D(n);
+    // outputs in MDBDSDC's final order (mdbdsdc.f90:383-414)
+    BCK(bensure(c->misc, sizeof(int) * (size_t)n + 64));
+    int *dperm = (int *)c->misc.p, *dtie = dfail + 1;
+    BCK(bensure(c->din2, sizeof(double) * (size_t)n));
+    double *dDo = dD ? dD : (double *)c->din2.p;
+    auto write_vectors = [&](bool clear) -> int {
+        if (!(dU || dVT)) return 0;
+        if (clear) {
+            if (dU) hipLaunchKernelGGL(k_bd_zero, dim3(8192), dim3(256), 0, st, dU, ldu, n);
+            if (dVT) hipLaunchKernelGGL(k_bd_zero, dim3(8192), dim3(256), 0, st, dVT, ldvt, n);
+        } else {
+            BHK(hipStreamWaitEvent(st, c->ev[3], 0));
+        }
+        hipLaunchKernelGGL(k_bd_fin, dim3(n), dim3(256), 0, st, W, (const double *)U, (const double *)V, ld,
+                           (const int *)dperm, n, dU, ldu, dVT, ldvt);
+        BHK(hipGetLastError());
+        if (rc && dU) {
+            BCK(bensure(c->rot, sizeof(double) * 2 * (size_t)n));
+            double *dc = (double *)c->rot.p, *ds = dc + n;
+            BHK(hipMemcpyAsync(dc, rc, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
+            BHK(hipMemcpyAsync(ds, rs, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_bd_lasr, dim3(bcdiv(n, 256)), dim3(256), 0, st, dU, ldu, n, (const double *)dc,
+                               (const double *)ds);
+            BHK(hipGetLastError());
+        }
+        return 0;
+    };
     unsigned long long cnt[2];
-    BHK(hipMemcpyAsync(D.data(), W.D, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    bool host_order = nprob != 1;
+    if (!host_order) {
+        hipLaunchKernelGGL(k_bd_perm, dim3(1), dim3(1024), 0, st, W, n, orgnrm, n > smlsiz ? 1 : 0, dperm, dDo, dtie);
+        BHK(hipGetLastError());
+        BCK(write_vectors(false));
+    }
     BHK(hipMemcpyAsync(cnt, W.iters, 16, hipMemcpyDeviceToHost, st));
     BHK(hipStreamSynchronize(st));
     if ((int)(cnt[1] & 0xffffffffu)) {
         if (dU || dVT) BHK(hipStreamWaitEvent(st, c->ev[3], 0));
         return 1;
     }
-    if (n > smlsiz) host_lascl(1.0, orgnrm, n, D.data());
-    // selection sort: position i takes the first largest of D(i..n); a segment tree over
-    // (value, position) finds it, the swap puts D(i) where the maximum was
-    std::vector<int> perm(n);
-    for (int i = 0; i < n; i++) perm[i] = i;
-    {
+    if ((int)(cnt[1] >> 32)) host_order = true;  // equal singular values
+    if (host_order) {
+        // selection sort on the host: position i takes the first largest of D(i..n); a
+        // segment tree over (value, position) finds it, the swap puts D(i) where the
+        // maximum was
+        std::vector<double> D(n);
+        BHK(hipMemcpyAsync(D.data(), W.D, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        BHK(hipStreamSynchronize(st));
+        if (n > smlsiz) host_lascl(1.0, orgnrm, n, D.data());
+        std::vector<int> perm(n);
+        for (int i = 0; i < n; i++) perm[i] = i;
         int sz = 1;
         while (sz < n) sz <<= 1;
         std::vector<int> tr(2 * sz, -1);
@@ -1608,26 +1670,11 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
                 update(i);
             }
         }
-    }
-    // outputs
-    BCK(bensure(c->misc, sizeof(int) * (size_t)n + 64));
-    int *dperm = (int *)c->misc.p;
-    BHK(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    if (dD) BHK(hipMemcpyAsync(dD, D.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-    if (dU || dVT) {
-        BHK(hipStreamWaitEvent(st, c->ev[3], 0));
-        hipLaunchKernelGGL(k_bd_fin, dim3(n), dim3(256), 0, st, W, (const double *)U, (const double *)V, ld,
-                           (const int *)dperm, n, dU, ldu, dVT, ldvt);
-        BHK(hipGetLastError());
-        if (rc && dU) {
-            BCK(bensure(c->rot, sizeof(double) * 2 * (size_t)n));
-            double *dc = (double *)c->rot.p, *ds = dc + n;
-            BHK(hipMemcpyAsync(dc, rc, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
-            BHK(hipMemcpyAsync(ds, rs, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_bd_lasr, dim3(bcdiv(n, 256)), dim3(256), 0, st, dU, ldu, n, (const double *)dc,
-                               (const double *)ds);
-            BHK(hipGetLastError());
-        }
+        BHK(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        BHK(hipMemcpyAsync(dDo, D.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        // vectors written already in IDXQ order are rewritten over cleared outputs
+        BCK(write_vectors(nprob == 1));
+        BHK(hipStreamSynchronize(st));
     }
     BHK(hipEventRecord(c->ev[1], st));
     BHK(hipEventSynchronize(c->ev[1]));

@@ -57,3 +57,33 @@ def test_params_validation():
         ctx.set_params(smlsiz=100)
     ctx.set_params(smlsiz=25, hss_min_k=1024, hss_leaf=128, hss_tol=1e-13)
     ctx.close()
+
+
+def test_bdsdc_argument_errors_without_device():
+    """MDBDSDC's checks (mdbdsdc.f90:174-187) answer before any device work."""
+    L = hsdc.lib()
+    ctx = hsdc.BdContext()
+    n = 8
+    d = np.ones(n)
+    e = np.ones(n)
+    u = np.zeros((n, n), order="F")
+    vt = np.zeros((n, n), order="F")
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    c = ctypes.c_char
+    calls = [((b"X", b"I", n, n, n), -1), ((b"U", b"X", n, n, n), -2), ((b"U", b"P", n, n, n), -2),
+             ((b"U", b"I", -1, n, n), -3), ((b"U", b"I", n, n - 1, n), -7), ((b"L", b"I", n, n, 0), -9),
+             ((b"U", b"N", n, 1, 1), None)]
+    for (uplo, compq, nn, ldu, ldvt), want in calls:
+        if want is None:
+            continue
+        rc = L.hsdc_bdsdc(ctx._h, c(uplo), c(compq), nn, dp(d), dp(e), dp(u), ldu, dp(vt), ldvt,
+                          None, None, None, None)
+        assert rc == want, (uplo, compq, nn, ldu, ldvt, rc)
+    assert L.hsdc_bdsdc(ctx._h, c(b"U"), c(b"I"), 0, dp(d), dp(e), dp(u), 1, dp(vt), 1,
+                        None, None, None, None) == 0
+    p = hsdc.BdParams()
+    L.hsdc_bd_default_params(ctypes.byref(p))
+    assert (p.smlsiz, p.maxit) == (25, 100)
+    p.smlsiz = 2
+    assert L.hsdc_bd_set_params(ctx._h, ctypes.byref(p)) == -2
+    ctx.close()

@@ -134,3 +134,35 @@ def test_bdsdc_device_resident_full_size(ctx):
     assert orth_u <= 1e-12 and orth_v <= 1e-12 and res <= 1e-12
     st = ctx.stats()
     assert st["levels"] >= 9 and st["merges"] > 0
+
+
+@pytest.mark.parametrize("n,uplo", [(1, "U"), (1, "L"), (2, "L"), (3, "U"), (25, "L")])
+def test_small_orders_bitexact(ctx, n, uplo):
+    d, e = bidiag_cfg(n, seed=12)
+    d[0] = -d[0]  # a negative diagonal entry: U carries the sign
+    s, u, vt, info = ctx.bdsdc(d, e, uplo=uplo)
+    assert info == 0
+    r = O.bdsdc(d, e, uplo=uplo.encode())
+    assert np.array_equal(s, r["d"]) and np.array_equal(u, r["u"]) and np.array_equal(vt, r["vt"])
+    B = bidiag(d, e) if uplo == "U" else bidiag(d, e).T
+    assert np.abs((u * s) @ vt - B).max() <= 4 * n * EPS * max(1.0, np.abs(B).max()) + 1e-14
+
+
+def test_device_entry_values_only_and_smlsiz(ctx):
+    """hsdc_bdsdc_dev with U and VT omitted gives the same singular values; a larger leaf
+    bound (40) stays bit-exact with the oracle run at the same bound."""
+    torch = pytest.importorskip("torch")
+    n = 900
+    d, e = bidiag_cfg(n, seed=13)
+    s_ref = ctx.bdsdc(d, e)[0]
+    dd = torch.tensor(d, device="cuda")
+    de = torch.tensor(e, device="cuda")
+    assert ctx.bdsdc_dev(dd, de, None, None) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(dd.cpu().numpy(), s_ref)
+    c40 = hsdc.BdContext(smlsiz=40)
+    s, u, vt, info = c40.bdsdc(d, e)
+    assert info == 0
+    r = O.bdsdc(d, e, prm=O.params(smlsiz=40))
+    assert np.array_equal(s, r["d"]) and np.array_equal(u, r["u"]) and np.array_equal(vt, r["vt"])
+    c40.close()

@@ -1262,6 +1262,8 @@ struct hsdc_bd_ctx {
     BdOut *hmo = nullptr;  // pinned
     size_t hmo_cap = 0;
     std::vector<int> tr;   // per merge of the last solve: block start, order, K, rotations
+    char *hpk = nullptr;   // pinned staging of a level's launch tables
+    size_t hpk_cap = 0;
 };
 
 #define BCK(x) do { int _r = (x); if (_r) return _r; } while (0)
@@ -1296,6 +1298,7 @@ int hsdc_bd_ctx_destroy(hsdc_bd_ctx *c)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->zstream) (void)hipStreamDestroy(c->zstream);
     if (c->hmo) (void)hipHostFree(c->hmo);
+    if (c->hpk) (void)hipHostFree(c->hpk);
     delete c;
     return 0;
 }
@@ -1549,7 +1552,20 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
         size_t o_tl = put(tiles.data(), sizeof(BdTile) * tiles.size());
         BCK(bensure(c->tab, pk.size() + 64));
         char *dt = (char *)c->tab.p;
-        BHK(hipMemcpyAsync(dt, pk.data(), pk.size(), hipMemcpyHostToDevice, st));
+        // through pinned memory: the copy is queued without the host waiting for it (the
+        // next level refills the buffer only after its own deflation has been read)
+        if (c->hpk_cap < pk.size()) {
+            if (c->hpk) (void)hipHostFree(c->hpk);
+            c->hpk = nullptr;
+            c->hpk_cap = 0;
+            if (hipHostMalloc((void **)&c->hpk, pk.size() * 2) != hipSuccess) {
+                (void)hipGetLastError();
+                return HSDC_ERR_NOMEM;
+            }
+            c->hpk_cap = pk.size() * 2;
+        }
+        std::memcpy(c->hpk, pk.data(), pk.size());
+        BHK(hipMemcpyAsync(dt, c->hpk, pk.size(), hipMemcpyHostToDevice, st));
         BCK(bensure(c->qb, sizeof(double) * (size_t)(qtot + 1)));
         BCK(bensure(c->stg, sizeof(double) * (size_t)(stot + 1)));
         double *Q = (double *)c->qb.p, *S = (double *)c->stg.p;

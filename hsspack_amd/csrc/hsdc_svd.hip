@@ -55,6 +55,25 @@ struct BdLeaf {
     int s, m, sqre, wr;
 };
 
+// launch tables of one level, built on the device from its deflation results
+// (k_bd_lvltab): per merge, the first work item of every grid-stride kernel (nm + 1
+// prefix counts each) and the offsets of its vector and staging blocks
+struct BdTab {
+    int *rot, *sec, *low, *tile, *sc;
+    long *qoff, *soff;
+};
+
+// merge owning work item b of a prefix table (pre[0] = 0 <= b < pre[nm])
+__device__ __forceinline__ int bd_find(const int *pre, int nm, int b)
+{
+    int lo = 0, hi = nm;
+    while (hi - lo > 1) {
+        const int md = (lo + hi) >> 1;
+        if (pre[md] <= b) lo = md; else hi = md;
+    }
+    return lo;
+}
+
 struct BdWork {
     double *D, *E;                          // n
     int *idxq;                              // n (1-based values, local to each block)
@@ -635,17 +654,16 @@ __global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork
 // of the read buffer; thread per row, rotations of a merge in order.  A column met
 // for the first time is read within its child's rows (zero elsewhere).
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bd_rotate(const BdMerge *mg, const int *bmerge, const int *brow,
-                                                   int nblk, BdWork W, const BdOut *mo, double *U, double *V,
-                                                   long ld)
+__global__ __launch_bounds__(256) void k_bd_rotate(const BdMerge *mg, int nmg, BdTab tab, BdWork W, const BdOut *mo,
+                                                   double *U, double *V, long ld)
 {
-    const int b = blockIdx.x;
-    if (b >= nblk) return;
-    const int mi = bmerge[b];
+  const int tot = tab.rot[nmg];
+  for (int b = blockIdx.x; b < tot; b += gridDim.x) {
+    const int mi = bd_find(tab.rot, nmg, b);
     const BdMerge M = mg[mi];
     const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre;
-    const int r = brow[b] + threadIdx.x;
-    if (r >= m) return;
+    const int r = (b - tab.rot[mi]) * 256 + threadIdx.x;
+    if (r >= m) continue;
     const int g = s + r;  // global row
     const int nrot = mo[mi].nrot;
     for (int i = 0; i < nrot; i++) {
@@ -667,16 +685,18 @@ __global__ __launch_bounds__(256) void k_bd_rotate(const BdMerge *mg, const int 
         V[(long)pv * ld + g] = c * x + sn * y;
         V[(long)qv * ld + g] = c * y - sn * x;
     }
+  }
 }
 
 // ------------------------------------------------------------------------------
 // |z| and z / |z| (DLASD3: RHO = DNRM2, DLASCL), one block per merge with K >= 2; the
 // norm in T-lane interleaved sums
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_bd_znorm(const BdMerge *mg, const int *mlist, int nl_, BdWork W, BdOut *mo)
+__global__ __launch_bounds__(64) void k_bd_znorm(const BdMerge *mg, BdWork W, BdOut *mo)
 {
-    const int mi = mlist[blockIdx.x];
+    const int mi = blockIdx.x;
     const int s = mg[mi].s, k = mo[mi].k, T = sec_team(k), l = threadIdx.x;
+    if (k < 2) return;
     double ps = 0.0;
     if (l < T)
         for (int i = l; i < k; i += T) ps += W.z[s + i] * W.z[s + i];
@@ -700,16 +720,16 @@ __device__ __forceinline__ double sv_den(double di, double dorg, double tau)
     return ((di - dorg) - tau) * ((di + dorg) + tau);
 }
 
-__global__ __launch_bounds__(256) void k_bd_secular(const BdMerge *mg, const int *bmerge, const int *broot,
-                                                    int nblk, BdWork W, const BdOut *mo, int maxit, int *fail)
+__global__ __launch_bounds__(256) void k_bd_secular(const BdMerge *mg, int nmg, BdTab tab, BdWork W, const BdOut *mo,
+                                                    int maxit, int *fail)
 {
-    const int b = blockIdx.x;
-    if (b >= nblk) return;
-    const int mi = bmerge[b];
+  const int tot = tab.sec[nmg];
+  for (int b = blockIdx.x; b < tot; b += gridDim.x) {
+    const int mi = bd_find(tab.sec, nmg, b);
     const int s = mg[mi].s, k = mo[mi].k, T = sec_team(k);
     const int l = threadIdx.x & (T - 1);
-    const int j = broot[b] + threadIdx.x / T + 1;  // 1-based root
-    if (j > k) return;
+    const int j = (b - tab.sec[mi]) * (256 / T) + threadIdx.x / T + 1;  // 1-based root
+    if (j > k) continue;
     const double *d = W.dsg + s, *z = W.z + s;
     const double rho = mo[mi].rho, rhoinv = 1.0 / rho;
     // starting iterate
@@ -838,20 +858,20 @@ __global__ __launch_bounds__(256) void k_bd_secular(const BdMerge *mg, const int
         atomicAdd(W.iters, (unsigned long long)(it + 1));
         if (info) atomicAdd(fail, 1);
     }
+  }
 }
 
 // ------------------------------------------------------------------------------
 // updated z (DLASD3, Loewner form): thread per index, products in DLASD3's order
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bd_loewner(const BdMerge *mg, const int *bmerge, const int *bfirst,
-                                                    int nblk, BdWork W, const BdOut *mo)
+__global__ __launch_bounds__(256) void k_bd_loewner(const BdMerge *mg, int nmg, BdTab tab, BdWork W, const BdOut *mo)
 {
-    const int b = blockIdx.x;
-    if (b >= nblk) return;
-    const int mi = bmerge[b];
+  const int tot = tab.low[nmg];
+  for (int b = blockIdx.x; b < tot; b += gridDim.x) {
+    const int mi = bd_find(tab.low, nmg, b);
     const int s = mg[mi].s, k = mo[mi].k;
-    const int i = bfirst[b] + threadIdx.x + 1;  // 1-based
-    if (i > k) return;
+    const int i = (b - tab.low[mi]) * 256 + threadIdx.x + 1;  // 1-based
+    if (i > k) continue;
     const double *ds = W.dsg + s;
     const double di = ds[i - 1];
     double zi = sv_den(di, ds[W.org[s + k - 1] - 1], W.tau[s + k - 1]);
@@ -861,6 +881,7 @@ __global__ __launch_bounds__(256) void k_bd_loewner(const BdMerge *mg, const int
         zi = zi * (sv_den(di, ds[W.org[s + j - 1] - 1], W.tau[s + j - 1]) / (di - ds[j]) / (di + ds[j]));
     double a = sqrt(fabs(zi));
     W.zl[s + i - 1] = W.zs[s + i - 1] >= 0.0 ? a : -a;
+  }
 }
 
 // ------------------------------------------------------------------------------
@@ -868,16 +889,17 @@ __global__ __launch_bounds__(256) void k_bd_loewner(const BdMerge *mg, const int
 // column i of Bu = left vector (rows in IDXC order), column i of Bv = row i of the
 // right-vector matrix
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bd_vec(const BdMerge *mg, const int *bmerge, const int *broot, int nblk,
-                                                BdWork W, const BdOut *mo, const long *qoff, double *Q)
+__global__ __launch_bounds__(256) void k_bd_vec(const BdMerge *mg, int nmg, BdTab tab, BdWork W, const BdOut *mo,
+                                                double *Q)
 {
-    const int b = blockIdx.x;
-    if (b >= nblk) return;
-    const int mi = bmerge[b];
+  const long *qoff = tab.qoff;
+  const int tot = tab.sec[nmg];
+  for (int b = blockIdx.x; b < tot; b += gridDim.x) {
+    const int mi = bd_find(tab.sec, nmg, b);
     const int s = mg[mi].s, k = mo[mi].k, T = sec_team(k);
     const int l = threadIdx.x & (T - 1);
-    const int i = broot[b] + threadIdx.x / T + 1;
-    if (i > k) return;
+    const int i = (b - tab.sec[mi]) * (256 / T) + threadIdx.x / T + 1;
+    if (i > k) continue;
     const double *ds = W.dsg + s, *zl = W.zl + s;
     const int *idxc = W.idxc + s;
     const double dorg = ds[W.org[s + i - 1] - 1], ti = W.tau[s + i - 1];
@@ -906,6 +928,7 @@ __global__ __launch_bounds__(256) void k_bd_vec(const BdMerge *mg, const int *bm
             bv[j] = wv / tv;
         }
     }
+  }
 }
 
 // ------------------------------------------------------------------------------
@@ -931,27 +954,27 @@ __device__ __forceinline__ double bd_acol(const BdWork &W, const BdMerge &M, con
 // inner indices staged per step, each output summed over the inner index in
 // ascending order
 // ------------------------------------------------------------------------------
-struct BdTile {
-    int merge, isv, row0, col0;
-};
-
-__global__ __launch_bounds__(256) void k_bd_gemm(const BdMerge *mg, const BdTile *tiles, int ntile, BdWork W,
-                                                 const BdOut *mo, const long *qoff, const double *Q,
-                                                 const double *U, const double *V, long ld, const long *soff,
+__global__ __launch_bounds__(256) void k_bd_gemm(const BdMerge *mg, int nmg, BdTab tab, BdWork W, const BdOut *mo,
+                                                 const double *Q, const double *U, const double *V, long ld,
                                                  double *stg)
 {
-    __shared__ double As[16][65];
-    __shared__ double Bs[16][65];
-    const int bi = blockIdx.x;
-    if (bi >= ntile) return;
-    const BdTile tl = tiles[bi];
-    const BdMerge M = mg[tl.merge];
-    const BdOut &o = mo[tl.merge];
-    const bool isV = tl.isv;
-    const int n = M.nl + M.nr + 1, rows = isV ? n + M.sqre : n, k = o.k;
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  const int ttot = tab.tile[nmg];
+  for (int bi = blockIdx.x; bi < ttot; bi += gridDim.x) {
+    // tiles of a merge: for every 64 output columns, the U row tiles then the V row tiles
+    const int mi = bd_find(tab.tile, nmg, bi);
+    const BdMerge M = mg[mi];
+    const BdOut &o = mo[mi];
+    const int n = M.nl + M.nr + 1, m = n + M.sqre, k = o.k;
+    const int ntu = (n + 63) / 64, ntv = (m + 63) / 64, lt = bi - tab.tile[mi];
+    const int rem = lt % (ntu + ntv);
+    const bool isV = rem >= ntu;
+    const int row0 = (isV ? rem - ntu : rem) * 64, col0 = (lt / (ntu + ntv)) * 64;
+    const int rows = isV ? m : n;
     const double *X = isV ? V : U;
-    double *Y = stg + soff[2 * tl.merge + (isV ? 1 : 0)];
-    const double *B = Q + qoff[tl.merge] + (isV ? (long)k * k : 0);
+    double *Y = stg + tab.soff[2 * mi + (isV ? 1 : 0)];
+    const double *B = Q + tab.qoff[mi] + (isV ? (long)k * k : 0);
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     double acc[4][4];
     for (int a = 0; a < 4; a++)
@@ -959,10 +982,10 @@ __global__ __launch_bounds__(256) void k_bd_gemm(const BdMerge *mg, const BdTile
     for (int j0 = 0; j0 < k; j0 += 16) {
         for (int e = threadIdx.x; e < 1024; e += 256) {
             int rr = e & 63, jj = e >> 6;
-            int r = tl.row0 + rr, J = j0 + jj;
+            int r = row0 + rr, J = j0 + jj;
             As[jj][rr] = (r < rows && J < k) ? bd_acol(W, M, o, isV, X, ld, J, r) : 0.0;
             int cc = e & 63, j2 = e >> 6;
-            int col = tl.col0 + cc, J2 = j0 + j2;
+            int col = col0 + cc, J2 = j0 + j2;
             Bs[j2][cc] = (col < k && J2 < k) ? B[(long)col * k + J2] : 0.0;
         }
         __syncthreads();
@@ -977,30 +1000,29 @@ __global__ __launch_bounds__(256) void k_bd_gemm(const BdMerge *mg, const BdTile
         __syncthreads();
     }
     for (int a = 0; a < 4; a++) {
-        int r = tl.row0 + tx + 16 * a;
+        int r = row0 + tx + 16 * a;
         if (r >= rows) continue;
         for (int c = 0; c < 4; c++) {
-            int col = tl.col0 + ty + 16 * c;
+            int col = col0 + ty + 16 * c;
             if (col < k) Y[(long)col * rows + r] = acc[a][c];
         }
     }
+  }
 }
 
 // ------------------------------------------------------------------------------
 // new columns formed without a product, into staging: U(:,1) and VT(1,:) when K = 1
 // (DLASD3's quick return) and the last row of VT when M > N (mdlasd2.f90:504-512)
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bd_newcol(const BdMerge *mg, const int *mlist, int nlist, BdWork W,
-                                                   const BdOut *mo, const double *U, const double *V, long ld,
-                                                   const long *soff, double *stg)
+__global__ __launch_bounds__(256) void k_bd_newcol(const BdMerge *mg, BdTab tab, BdWork W, const BdOut *mo,
+                                                   const double *U, const double *V, long ld, double *stg)
 {
-    const int b = blockIdx.x;
-    if (b >= nlist) return;
-    const int mi = mlist[b];
+    const int mi = blockIdx.x;
     const BdMerge M = mg[mi];
     const BdOut &o = mo[mi];
     const int s = M.s, nl = M.nl, n = nl + M.nr + 1, m = n + M.sqre, k = o.k;
-    double *SU = stg + soff[2 * mi], *SV = stg + soff[2 * mi + 1];
+    if (!(k == 1 || m > n)) return;
+    double *SU = stg + tab.soff[2 * mi], *SV = stg + tab.soff[2 * mi + 1];
     const int *vm = W.vmV[M.rd];
     for (int r = threadIdx.x; r < m; r += 256) {
         if (k == 1) {
@@ -1018,18 +1040,89 @@ __global__ __launch_bounds__(256) void k_bd_newcol(const BdMerge *mg, const int 
 }
 
 // staging -> the slots freed by the merge (rows of the merge block)
-__global__ __launch_bounds__(256) void k_bd_scatter(const BdMerge *mg, const int *smerge, const int *scol, int ncol,
-                                                    BdWork W, const long *soff, const double *stg, double *U,
-                                                    double *V, long ld)
+__global__ __launch_bounds__(256) void k_bd_scatter(const BdMerge *mg, int nmg, BdTab tab, BdWork W,
+                                                    const BdOut *mo, const double *stg, double *U, double *V, long ld)
 {
-    const int b = blockIdx.x;
-    if (b >= ncol) return;
-    const int mi = smerge[b], cw = scol[b], isV = cw >> 30, i = cw & 0x3fffffff;
+  const int tot = tab.sc[nmg];
+  for (int b = blockIdx.x; b < tot; b += gridDim.x) {
+    // a merge's new columns: the K of U, then the K + sqre of V
+    const int mi = bd_find(tab.sc, nmg, b);
     const BdMerge M = mg[mi];
+    const int k = mo[mi].k, j = b - tab.sc[mi], isV = j >= k, i = isV ? j - k : j;
     const int s = M.s, n = M.nl + M.nr + 1, rows = isV ? n + M.sqre : n;
-    const double *S = stg + soff[2 * mi + isV] + (long)i * rows;
+    const double *S = stg + tab.soff[2 * mi + isV] + (long)i * rows;
     double *Y = isV ? V + (long)W.fV[s + i] * ld : U + (long)W.fU[s + i] * ld;
     for (int r = threadIdx.x; r < rows; r += 256) Y[s + r] = S[r];
+  }
+}
+
+// exclusive block scan of one 64-bit value per thread (1024 threads); returns the total
+__device__ long block_exscan_l(long v, long *sh, long &total)
+{
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, nw = blockDim.x >> 6;
+    long x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        long y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    if (t == 0) {
+        long acc = 0;
+        for (int w = 0; w < nw; w++) {
+            long c = sh[w];
+            sh[w] = acc;
+            acc += c;
+        }
+        sh[nw] = acc;
+    }
+    __syncthreads();
+    long r = sh[wv] + x - v;
+    total = sh[nw];
+    __syncthreads();
+    return r;
+}
+
+// the level's launch tables from its deflation results: per-merge work counts and
+// their exclusive prefix sums, 1024 merges per pass
+__global__ __launch_bounds__(1024) void k_bd_lvltab(const BdMerge *mg, int nmg, const BdOut *mo, BdTab tab)
+{
+    __shared__ long sh[1024 / 64 + 1];
+    long base[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i0 = 0; i0 < nmg; i0 += 1024) {
+        const int i = i0 + threadIdx.x;
+        long v[7] = {0, 0, 0, 0, 0, 0, 0};  // rot, sec, low, tile, sc, q, staging
+        if (i < nmg) {
+            const BdMerge M = mg[i];
+            const int n = M.nl + M.nr + 1, m = n + M.sqre, k = mo[i].k;
+            if (mo[i].nrot) v[0] = (m + 255) / 256;
+            if (k >= 2) {
+                const int T = sec_team(k);
+                v[1] = (k + 256 / T - 1) / (256 / T);
+                v[2] = (k + 255) / 256;
+                v[3] = (long)((k + 63) / 64) * ((n + 63) / 64 + (m + 63) / 64);
+                v[5] = 2 * (long)k * k;
+            }
+            v[4] = 2 * k + M.sqre;
+            v[6] = (long)n * k + (long)m * (k + M.sqre);
+        }
+        long ex[7], tot[7];
+        for (int q = 0; q < 7; q++) ex[q] = base[q] + block_exscan_l(v[q], sh, tot[q]);
+        if (i < nmg) {
+            const BdMerge M = mg[i];
+            const int n = M.nl + M.nr + 1, k = mo[i].k;
+            tab.rot[i] = (int)ex[0]; tab.sec[i] = (int)ex[1]; tab.low[i] = (int)ex[2]; tab.tile[i] = (int)ex[3];
+            tab.sc[i] = (int)ex[4];
+            tab.qoff[i] = ex[5];
+            tab.soff[2 * i] = ex[6];
+            tab.soff[2 * i + 1] = ex[6] + (long)n * k;
+        }
+        for (int q = 0; q < 7; q++) base[q] += tot[q];
+    }
+    if (threadIdx.x == 0) {
+        tab.rot[nmg] = (int)base[0]; tab.sec[nmg] = (int)base[1]; tab.low[nmg] = (int)base[2];
+        tab.tile[nmg] = (int)base[3]; tab.sc[nmg] = (int)base[4];
+    }
 }
 
 // ------------------------------------------------------------------------------
@@ -1139,6 +1232,13 @@ __global__ __launch_bounds__(1024) void k_bd_perm(const BdWork W, int n, double 
     }
 }
 
+// identity maps of U's slots (the centre columns map to themselves)
+static __global__ void k_bd_iota(int *a, int *b, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = b[i] = i;
+}
+
 // DLASR('L','V','B') on the columns of U: the rotations that made a lower bidiagonal
 // upper (mdbdsdc.f90:413-414); thread per column
 __global__ __launch_bounds__(256) void k_bd_lasr(double *U, long ldu, int n, const double *cs, const double *sn)
@@ -1181,6 +1281,7 @@ int bensure(Buf &b, size_t bytes)
 }
 
 inline int bcdiv(long a, long b) { return (int)((a + b - 1) / b); }
+inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
 
 // LAPACK DLASDT: the tree of one subproblem (node arrays 0-based, values 1-based)
 void lasdt(int n, int &lvl, int &nd, std::vector<int> &inode, std::vector<int> &ndiml, std::vector<int> &ndimr,
@@ -1256,14 +1357,15 @@ struct hsdc_bd_ctx {
     Buf ub[1], vb[1], wd, wi, mgb, mob, tab, qb, stg, misc, din, din2, ein, uo, vto, rot;
     hipStream_t zstream = nullptr;     // zero fill of the outputs beside the levels
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // solve start / end, fill fork / join
-    std::vector<int> iota;
+    std::vector<int> plan_key;   // shape of the tree tables on the device
+    void *plan_buf = nullptr;
+    double *hde = nullptr;       // pinned staging of d and e
+    size_t hde_cap = 0;
     BdWork W;
     long wcap = 0;
     BdOut *hmo = nullptr;  // pinned
     size_t hmo_cap = 0;
     std::vector<int> tr;   // per merge of the last solve: block start, order, K, rotations
-    char *hpk = nullptr;   // pinned staging of a level's launch tables
-    size_t hpk_cap = 0;
 };
 
 #define BCK(x) do { int _r = (x); if (_r) return _r; } while (0)
@@ -1298,7 +1400,7 @@ int hsdc_bd_ctx_destroy(hsdc_bd_ctx *c)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->zstream) (void)hipStreamDestroy(c->zstream);
     if (c->hmo) (void)hipHostFree(c->hmo);
-    if (c->hpk) (void)hipHostFree(c->hpk);
+    if (c->hde) (void)hipHostFree(c->hde);
     delete c;
     return 0;
 }
@@ -1454,16 +1556,21 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
     BdMerge *dmg = (BdMerge *)c->mgb.p;
     BdLeaf *dlv = (BdLeaf *)(dmg + nmg + 1);
     {
-        std::vector<BdMerge> all;
-        for (auto &v : lv) all.insert(all.end(), v.begin(), v.end());
-        if (nmg) BHK(hipMemcpyAsync(dmg, all.data(), sizeof(BdMerge) * nmg, hipMemcpyHostToDevice, st));
-        BHK(hipMemcpyAsync(dlv, leaves.data(), sizeof(BdLeaf) * leaves.size(), hipMemcpyHostToDevice, st));
-        if ((int)c->iota.size() != n) {
-            c->iota.resize(n);
-            for (int i = 0; i < n; i++) c->iota[i] = i;
+        // the tree tables stay on the device between solves of the same shape
+        std::vector<int> key = {n, smlsiz, (int)leaves.size(), nmg};
+        for (auto &L : leaves) key.insert(key.end(), {L.s, L.m, L.sqre, L.wr});
+        for (auto &v : lv)
+            for (auto &M : v) key.insert(key.end(), {M.s, M.nl, M.nr, M.sqre, M.rd, M.wr});
+        if (key != c->plan_key || c->plan_buf != c->mgb.p) {
+            std::vector<BdMerge> all;
+            for (auto &v : lv) all.insert(all.end(), v.begin(), v.end());
+            if (nmg) BHK(hipMemcpyAsync(dmg, all.data(), sizeof(BdMerge) * nmg, hipMemcpyHostToDevice, st));
+            BHK(hipMemcpyAsync(dlv, leaves.data(), sizeof(BdLeaf) * leaves.size(), hipMemcpyHostToDevice, st));
+            BHK(hipStreamSynchronize(st));  // the host vectors go out of scope
+            c->plan_key.swap(key);
+            c->plan_buf = c->mgb.p;
         }
-        BHK(hipMemcpyAsync(W.vmU[0], c->iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        BHK(hipMemcpyAsync(W.vmU[1], c->iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_bd_iota, dim3(bcdiv(n, 256)), dim3(256), 0, st, W.vmU[0], W.vmU[1], n);
     }
     BCK(bensure(c->mob, sizeof(BdOut) * (nmg + 1)));
     BdOut *dmo = (BdOut *)c->mob.p;
@@ -1486,115 +1593,68 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
     hipLaunchKernelGGL(k_bd_leaf, dim3((unsigned)leaves.size()), dim3(64), lds, st, dlv, (int)leaves.size(), W, U, V,
                        ld, pmax, dfail);
     BHK(hipGetLastError());
-    long long roots = 0, deflated = 0, rots = 0;
-    int ktop = 0;
-    for (size_t t = 0; t < lv.size(); t++) {
+    // Every level is queued without a host round trip: k_bd_lvltab sizes the level's
+    // work from its deflation results on the device, the kernels after it walk those
+    // tables in grid-stride loops whose grids (and the vector and staging buffers) are
+    // the host's bounds for K = N.
+    const int nlv = (int)lv.size();
+    std::vector<size_t> toff(nlv + 1, 0);
+    long qcap = 1, scap = 1;
+    std::vector<int> g_rot(nlv), g_sec(nlv), g_low(nlv), g_tile(nlv), g_sc(nlv);
+    for (int t = 0; t < nlv; t++) {
+        const int nm = (int)lv[t].size();
+        toff[t + 1] = toff[t] + al8(sizeof(int) * 5 * (size_t)(nm + 1)) + sizeof(long) * 3 * (size_t)nm + 64;
+        long q = 0, st = 0, br = 0, bs = 0, bl = 0, bt = 0, bc = 0;
+        for (const BdMerge &M : lv[t]) {
+            const long n_ = M.nl + M.nr + 1, m_ = n_ + M.sqre;
+            const int T = sec_team((int)n_);
+            q += 2 * n_ * n_;
+            st += n_ * n_ + m_ * (n_ + M.sqre);
+            br += (m_ + 255) / 256;
+            bs += (n_ + 256 / T - 1) / (256 / T);
+            bl += (n_ + 255) / 256;
+            bt += ((n_ + 63) / 64) * ((n_ + 63) / 64 + (m_ + 63) / 64);
+            bc += 2 * n_ + M.sqre;
+        }
+        qcap = std::max(qcap, q);
+        scap = std::max(scap, st);
+        g_rot[t] = (int)std::min(br, 4096L);
+        g_sec[t] = (int)std::min(bs, 8192L);
+        g_low[t] = (int)std::min(bl, 4096L);
+        g_tile[t] = (int)std::min(bt, 4096L);
+        g_sc[t] = (int)std::min(bc, 16384L);
+    }
+    BCK(bensure(c->tab, toff[nlv] + 64));
+    BCK(bensure(c->qb, sizeof(double) * (size_t)qcap));
+    BCK(bensure(c->stg, sizeof(double) * (size_t)scap));
+    double *Q = (double *)c->qb.p, *S = (double *)c->stg.p;
+    for (int t = 0; t < nlv; t++) {
         const int nm = (int)lv[t].size();
         if (!nm) continue;
         BdMerge *mg = dmg + moff[t];
         BdOut *mo = dmo + moff[t];
+        char *tb = (char *)c->tab.p + toff[t];
+        BdTab tab;
+        int *ti = (int *)tb;
+        tab.rot = ti; tab.sec = ti + (nm + 1); tab.low = ti + 2 * (nm + 1); tab.tile = ti + 3 * (nm + 1);
+        tab.sc = ti + 4 * (nm + 1);
+        tab.qoff = (long *)(tb + al8(sizeof(int) * 5 * (size_t)(nm + 1)));
+        tab.soff = tab.qoff + nm;
         hipLaunchKernelGGL(k_bd_prep, dim3(nm), dim3(PREP_NT), 0, st, mg, W, mo, (const double *)V, ld);
         hipLaunchKernelGGL(k_bd_deflate, dim3(nm), dim3(DEF_NT), 0, st, mg, W, mo);
-        BHK(hipGetLastError());
-        BHK(hipMemcpyAsync(c->hmo, mo, sizeof(BdOut) * nm, hipMemcpyDeviceToHost, st));
-        BHK(hipStreamSynchronize(st));
-        // launch tables of the level
-        std::vector<int> rb_m, rb_r, sb_m, sb_r, lb_m, lb_r, zl_m, nc_m, sc_m, sc_c;
-        std::vector<long> qo(nm, 0), so(2 * (size_t)nm, 0);
-        std::vector<BdTile> tiles;
-        long qtot = 0, stot = 0;
-        for (int i = 0; i < nm; i++) {
-            const BdMerge &M = lv[t][i];
-            const BdOut &o = c->hmo[i];
-            const int n_ = M.nl + M.nr + 1, m_ = n_ + M.sqre, k = o.k;
-            if (k < 1 || k > n_) return HSDC_ERR_INTERNAL;
-            c->tr.insert(c->tr.end(), {M.s, n_, k, o.nrot});
-            roots += k;
-            deflated += n_ - k;
-            rots += o.nrot;
-            ktop = k;
-            if (o.nrot)
-                for (int r0 = 0; r0 < m_; r0 += 256) { rb_m.push_back(i); rb_r.push_back(r0); }
-            so[2 * i] = stot;
-            stot += (long)n_ * k;
-            so[2 * i + 1] = stot;
-            stot += (long)m_ * (k + M.sqre);
-            if (k >= 2) {
-                zl_m.push_back(i);
-                int T = sec_team(k), per = 256 / T;
-                for (int j0 = 0; j0 < k; j0 += per) { sb_m.push_back(i); sb_r.push_back(j0); }
-                for (int j0 = 0; j0 < k; j0 += 256) { lb_m.push_back(i); lb_r.push_back(j0); }
-                qo[i] = qtot;
-                qtot += 2 * (long)k * k;
-                for (int c0 = 0; c0 < k; c0 += 64) {
-                    for (int r0 = 0; r0 < n_; r0 += 64) tiles.push_back({i, 0, r0, c0});
-                    for (int r0 = 0; r0 < m_; r0 += 64) tiles.push_back({i, 1, r0, c0});
-                }
-            }
-            if (k == 1 || m_ > n_) nc_m.push_back(i);
-            for (int j = 0; j < k; j++) { sc_m.push_back(i); sc_c.push_back(j); }
-            for (int j = 0; j < k + M.sqre; j++) { sc_m.push_back(i); sc_c.push_back(j | (1 << 30)); }
-        }
-        // pack the tables into one upload
-        std::vector<char> pk;
-        auto put = [&](const void *p, size_t bytes) {
-            size_t off = (pk.size() + 63) & ~(size_t)63;
-            pk.resize(off + bytes + 8);
-            if (bytes) std::memcpy(pk.data() + off, p, bytes);
-            return off;
-        };
-        size_t o_rbm = put(rb_m.data(), 4 * rb_m.size()), o_rbr = put(rb_r.data(), 4 * rb_r.size());
-        size_t o_sbm = put(sb_m.data(), 4 * sb_m.size()), o_sbr = put(sb_r.data(), 4 * sb_r.size());
-        size_t o_lbm = put(lb_m.data(), 4 * lb_m.size()), o_lbr = put(lb_r.data(), 4 * lb_r.size());
-        size_t o_zl = put(zl_m.data(), 4 * zl_m.size()), o_nc = put(nc_m.data(), 4 * nc_m.size());
-        size_t o_scm = put(sc_m.data(), 4 * sc_m.size()), o_scc = put(sc_c.data(), 4 * sc_c.size());
-        size_t o_qo = put(qo.data(), 8 * qo.size()), o_so = put(so.data(), 8 * so.size());
-        size_t o_tl = put(tiles.data(), sizeof(BdTile) * tiles.size());
-        BCK(bensure(c->tab, pk.size() + 64));
-        char *dt = (char *)c->tab.p;
-        // through pinned memory: the copy is queued without the host waiting for it (the
-        // next level refills the buffer only after its own deflation has been read)
-        if (c->hpk_cap < pk.size()) {
-            if (c->hpk) (void)hipHostFree(c->hpk);
-            c->hpk = nullptr;
-            c->hpk_cap = 0;
-            if (hipHostMalloc((void **)&c->hpk, pk.size() * 2) != hipSuccess) {
-                (void)hipGetLastError();
-                return HSDC_ERR_NOMEM;
-            }
-            c->hpk_cap = pk.size() * 2;
-        }
-        std::memcpy(c->hpk, pk.data(), pk.size());
-        BHK(hipMemcpyAsync(dt, c->hpk, pk.size(), hipMemcpyHostToDevice, st));
-        BCK(bensure(c->qb, sizeof(double) * (size_t)(qtot + 1)));
-        BCK(bensure(c->stg, sizeof(double) * (size_t)(stot + 1)));
-        double *Q = (double *)c->qb.p, *S = (double *)c->stg.p;
-        const long *dso = (const long *)(dt + o_so);
-        if (!rb_m.empty())
-            hipLaunchKernelGGL(k_bd_rotate, dim3((unsigned)rb_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_rbm),
-                               (const int *)(dt + o_rbr), (int)rb_m.size(), W, (const BdOut *)mo, U, V, ld);
-        if (!zl_m.empty()) {
-            hipLaunchKernelGGL(k_bd_znorm, dim3((unsigned)zl_m.size()), dim3(64), 0, st, mg, (const int *)(dt + o_zl),
-                               (int)zl_m.size(), W, mo);
-            hipLaunchKernelGGL(k_bd_secular, dim3((unsigned)sb_m.size()), dim3(256), 0, st, mg,
-                               (const int *)(dt + o_sbm), (const int *)(dt + o_sbr), (int)sb_m.size(), W,
-                               (const BdOut *)mo, c->prm.maxit, dfail);
-            hipLaunchKernelGGL(k_bd_loewner, dim3((unsigned)lb_m.size()), dim3(256), 0, st, mg,
-                               (const int *)(dt + o_lbm), (const int *)(dt + o_lbr), (int)lb_m.size(), W,
-                               (const BdOut *)mo);
-            hipLaunchKernelGGL(k_bd_vec, dim3((unsigned)sb_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_sbm),
-                               (const int *)(dt + o_sbr), (int)sb_m.size(), W, (const BdOut *)mo,
-                               (const long *)(dt + o_qo), Q);
-            hipLaunchKernelGGL(k_bd_gemm, dim3((unsigned)tiles.size()), dim3(256), 0, st, mg,
-                               (const BdTile *)(dt + o_tl), (int)tiles.size(), W, (const BdOut *)mo,
-                               (const long *)(dt + o_qo), (const double *)Q, (const double *)U, (const double *)V, ld,
-                               dso, S);
-        }
-        if (!nc_m.empty())
-            hipLaunchKernelGGL(k_bd_newcol, dim3((unsigned)nc_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_nc),
-                               (int)nc_m.size(), W, (const BdOut *)mo, (const double *)U, (const double *)V, ld, dso, S);
-        hipLaunchKernelGGL(k_bd_scatter, dim3((unsigned)sc_m.size()), dim3(256), 0, st, mg, (const int *)(dt + o_scm),
-                           (const int *)(dt + o_scc), (int)sc_m.size(), W, dso, (const double *)S, U, V, ld);
+        hipLaunchKernelGGL(k_bd_lvltab, dim3(1), dim3(1024), 0, st, mg, nm, (const BdOut *)mo, tab);
+        hipLaunchKernelGGL(k_bd_rotate, dim3(g_rot[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo, U, V, ld);
+        hipLaunchKernelGGL(k_bd_znorm, dim3(nm), dim3(64), 0, st, mg, W, mo);
+        hipLaunchKernelGGL(k_bd_secular, dim3(g_sec[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo,
+                           c->prm.maxit, dfail);
+        hipLaunchKernelGGL(k_bd_loewner, dim3(g_low[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo);
+        hipLaunchKernelGGL(k_bd_vec, dim3(g_sec[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo, Q);
+        hipLaunchKernelGGL(k_bd_gemm, dim3(g_tile[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo,
+                           (const double *)Q, (const double *)U, (const double *)V, ld, S);
+        hipLaunchKernelGGL(k_bd_newcol, dim3(nm), dim3(256), 0, st, mg, tab, W, (const BdOut *)mo, (const double *)U,
+                           (const double *)V, ld, S);
+        hipLaunchKernelGGL(k_bd_scatter, dim3(g_sc[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo,
+                           (const double *)S, U, V, ld);
         hipLaunchKernelGGL(k_bd_post, dim3(nm), dim3(POST_NT), 0, st, mg, W, (const BdOut *)mo);
         BHK(hipGetLastError());
         c->st.levels++;
@@ -1641,6 +1701,23 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
         return 1;
     }
     if ((int)(cnt[1] >> 32)) host_order = true;  // equal singular values
+    // per-merge results of every level (trace and statistics)
+    long long roots = 0, deflated = 0, rots = 0;
+    int ktop = 0;
+    if (nmg) {
+        BHK(hipMemcpy(c->hmo, dmo, sizeof(BdOut) * nmg, hipMemcpyDeviceToHost));
+        for (int t = 0; t < nlv; t++)
+            for (size_t i = 0; i < lv[t].size(); i++) {
+                const BdMerge &M = lv[t][i];
+                const BdOut &o = c->hmo[moff[t] + i];
+                const int n_ = M.nl + M.nr + 1;
+                c->tr.insert(c->tr.end(), {M.s, n_, o.k, o.nrot});
+                roots += o.k;
+                deflated += n_ - o.k;
+                rots += o.nrot;
+                ktop = o.k;
+            }
+    }
     if (host_order) {
         // selection sort on the host: position i takes the first largest of D(i..n); a
         // segment tree over (value, position) finds it, the swap puts D(i) where the
@@ -1731,10 +1808,21 @@ int hsdc_bdsdc_dev(hsdc_bd_ctx *c, char uplo, int n, double *d_d, double *d_e, d
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { (void)hipGetLastError(); return HSDC_ERR_NODEVICE; }
     hipStream_t st = c->stream;
-    std::vector<double> hd(n), he(n > 1 ? n - 1 : 1, 0.0);
-    BHK(hipMemcpyAsync(hd.data(), d_d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    if (n > 1) BHK(hipMemcpyAsync(he.data(), d_e, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, st));
+    if (c->hde_cap < 2 * (size_t)n) {
+        if (c->hde) (void)hipHostFree(c->hde);
+        c->hde = nullptr;
+        c->hde_cap = 0;
+        if (hipHostMalloc((void **)&c->hde, sizeof(double) * 2 * (size_t)n) != hipSuccess) {
+            (void)hipGetLastError();
+            return HSDC_ERR_NOMEM;
+        }
+        c->hde_cap = 2 * (size_t)n;
+    }
+    BHK(hipMemcpyAsync(c->hde, d_d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (n > 1) BHK(hipMemcpyAsync(c->hde + n, d_e, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, st));
     BHK(hipStreamSynchronize(st));
+    std::vector<double> hd(c->hde, c->hde + n), he(n > 1 ? n - 1 : 1, 0.0);
+    if (n > 1) std::copy(c->hde + n, c->hde + 2 * n - 1, he.begin());
     std::memset(&c->st, 0, sizeof c->st);
     c->tr.clear();
     if (n == 1) {
@@ -1780,8 +1868,10 @@ int hsdc_bdsdc_dev(hsdc_bd_ctx *c, char uplo, int n, double *d_d, double *d_e, d
             if (std::fabs(hd[i]) < eps) hd[i] = hd[i] >= 0.0 ? eps : -eps;
     }
     BCK(bd_alloc(c, n));
-    BHK(hipMemcpyAsync(c->W.D, hd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-    BHK(hipMemcpyAsync(c->W.E, he.data(), sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
+    std::copy(hd.begin(), hd.end(), c->hde);
+    std::copy(he.begin(), he.begin() + (n - 1), c->hde + n);
+    BHK(hipMemcpyAsync(c->W.D, c->hde, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    BHK(hipMemcpyAsync(c->W.E, c->hde + n, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
     int info = bd_solve(c, n, he, orgnrm, d_u, ldu, d_vt, ldvt, d_d, iuplo == 2 ? rc.data() : nullptr,
                         iuplo == 2 ? rs.data() : nullptr);
     if (info) {

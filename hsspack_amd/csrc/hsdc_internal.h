@@ -17,6 +17,44 @@ inline int sec_team(int k)
     while (t < 64 && 8 * t <= k) t *= 2;
     return t;
 }
+// the value lane l ^ OFF holds (OFF a power of two below 64): cross-lane moves
+// without the LDS round trip where the pattern allows -- DPP quad permutations for 1
+// and 2, a 16-lane row rotation by 8 for 8, ds_swizzle's xor mode (32-lane groups) for
+// 4 and 16, ds_bpermute for 32.  Same values as __shfl_xor, so sums built from it
+// keep their order and bits.
+template <int OFF>
+__device__ __forceinline__ int xlane32(int x)
+{
+    if constexpr (OFF == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    else if constexpr (OFF == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    else if constexpr (OFF == 8) return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
+    else if constexpr (OFF == 4) return __builtin_amdgcn_ds_swizzle(x, 0x101F);   // and 0x1f, xor 4
+    else if constexpr (OFF == 16) return __builtin_amdgcn_ds_swizzle(x, 0x401F);  // and 0x1f, xor 16
+    else return __builtin_amdgcn_ds_bpermute((int)((threadIdx.x ^ OFF) << 2), x);
+}
+
+template <int OFF>
+__device__ __forceinline__ double xlane(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = xlane32<OFF>((int)(unsigned)(u & 0xffffffffull));
+    const int hi = xlane32<OFF>((int)(unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// butterfly over the T (power of two <= 64) lanes of a team, lanes aligned to T:
+// lane l adds lane l ^ off for off = T/2 .. 1, every lane ends with the same value
+__device__ __forceinline__ double team_bfly(double v, int T)
+{
+    if (T > 32) v = v + xlane<32>(v);
+    if (T > 16) v = v + xlane<16>(v);
+    if (T > 8) v = v + xlane<8>(v);
+    if (T > 4) v = v + xlane<4>(v);
+    if (T > 2) v = v + xlane<2>(v);
+    if (T > 1) v = v + xlane<1>(v);
+    return v;
+}
+
 #define SEC_WG 256   // threads per block of the secular kernel
 #define SEC_LDS 2048 // merges with K <= SEC_LDS stage poles and z in LDS
 

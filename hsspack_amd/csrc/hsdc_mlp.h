@@ -14,6 +14,7 @@
 // All arithmetic is plain IEEE double/float with no contraction and exp is the
 // series below, so the CPU restatement gives bit-identical proposals.
 #pragma once
+#include "hsdc_internal.h"
 
 #define MLP_IN 34
 #define MLP_H 64
@@ -52,11 +53,7 @@ __device__ inline void sec_features(int n, int r, const double *d, const double 
 
 // sum over the T lanes of a team (aligned group of T lanes): butterfly, every lane
 // ends with the same value
-__device__ inline double team_sum(double v, int T)
-{
-    for (int off = T >> 1; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
-    return v;
-}
+__device__ inline double team_sum(double v, int T) { return team_bfly(v, T); }
 __device__ inline float team_sumf(float v, int T)
 {
     for (int off = T >> 1; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);

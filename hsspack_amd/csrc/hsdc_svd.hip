@@ -95,15 +95,16 @@ struct BdWork {
 // ------------------------------------------------------------------------------
 __device__ __forceinline__ double wsum64(double v)
 {
-    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+    v = v + xlane<32>(v);
+    v = v + xlane<16>(v);
+    v = v + xlane<8>(v);
+    v = v + xlane<4>(v);
+    v = v + xlane<2>(v);
+    v = v + xlane<1>(v);
     return v;
 }
 
-__device__ __forceinline__ double tsum_team(double v, int T)
-{
-    for (int off = T >> 1; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
-    return v;
-}
+__device__ __forceinline__ double tsum_team(double v, int T) { return team_bfly(v, T); }
 
 __device__ __forceinline__ double bd_lapy2(double x, double y)
 {

@@ -43,15 +43,25 @@ __device__ __forceinline__ VI vi_better(VI a, VI b)
     if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
     return a;
 }
+template <int O>
+__device__ __forceinline__ VI vi_step(VI x)
+{
+    VI y;
+    y.v = xlane<O>(x.v);
+    y.i = xlane32<O>(x.i);
+    return vi_better(x, y);
+}
+// (value, index) maximum with ties to the smaller index is a total order, so the
+// butterfly's pairing does not change the result
 __device__ VI block_argmax(VI x, VI *sh)
 {
     int tid = threadIdx.x;
-    for (int o = 32; o > 0; o >>= 1) {
-        VI y;
-        y.v = __shfl_down(x.v, o, 64);
-        y.i = __shfl_down(x.i, o, 64);
-        x = vi_better(x, y);
-    }
+    x = vi_step<32>(x);
+    x = vi_step<16>(x);
+    x = vi_step<8>(x);
+    x = vi_step<4>(x);
+    x = vi_step<2>(x);
+    x = vi_step<1>(x);
     if ((tid & 63) == 0) sh[tid >> 6] = x;
     __syncthreads();
     VI r = sh[0];

@@ -56,9 +56,9 @@ __global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, 
 __global__ void k_hss_init(const HssMerge *);
 __global__ void k_hss_reset(const HssMerge *);
 __global__ void k_hss_dgen(const HssMerge *, const HssNode *, int);
-template <int NT>
+template <int NT, bool SQ>
 __global__ void k_hss_rowcomp(const HssMerge *, const HssNode *, int, double);
-template <int NT>
+template <int NT, bool SQ>
 __global__ void k_hss_colcomp(const HssMerge *, const HssNode *, int, double);
 __global__ void k_hss_compact(const HssMerge *, const HssCompact *, const int *, int);
 __global__ void k_hss_bgen(const HssMerge *, const HssNode *, int);
@@ -144,7 +144,7 @@ struct hsdc_ctx {
     DevBuf hss, htab, xb, gf, cb, yb;
     // generators of the last hsdc_hss_evc call (host copies), for hsdc_hss_dense
     std::vector<double> evc_gen;
-    int evc_k = 0, evc_ni = 0;
+    int evc_k = 0, evc_ni = 0, evc_sq = 0;
     double evc_tol = 0.0;
     PinBuf pin, pin2, pin3;
     Work W;
@@ -354,19 +354,6 @@ static int sec_collect(hsdc_ctx *c)
 // build the HSS form of every HSS merge's eigenvector matrix, then
 // Ct(:, 1:K) = Q2 * H.  `sel` lists the slice-local merges that take this branch.
 // ----------------------------------------------------------------------------
-struct HssSpec {
-    int K;
-    const double *FF, *D0, *U0, *V0, *DIFL, *DIFR;  // device generators (K)
-    const double *X;   // nrow x K, leading dimension ldx
-    double *Y;         // nrow x K result X * H, leading dimension ldy
-    long ldx, ldy;
-    int nrow;
-    // optional: column j of the result goes to Y + (ymap[j] - ymoff) * ldy (the
-    // eigenvector slots of the merge, written in place of a copy through Ct)
-    const int *ymap = nullptr;
-    long ymoff = 0;
-};
-
 struct HssPlanM {
     int li;              // slice-local merge index
     HssTree T;
@@ -384,6 +371,10 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
 {
     hipStream_t s = c->stream;
     std::vector<HssPlanM> P(sp.size());
+    // one batch is all eigenvector matrices or all SVD vector matrices
+    const int sq = sp.empty() ? 0 : sp[0].sq;
+    for (const HssSpec &q : sp)
+        if (q.sq != sq) return HSDC_ERR_INTERNAL;
     // ---- device memory for every merge
     size_t tot = 0;
     std::vector<size_t> base(sp.size()), metasz(sp.size());
@@ -399,7 +390,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         metasz[h] = o;
         o += al64(sizeof(double) * std::max(dds, 1L));
         o += al64(sizeof(double) * (size_t)12 * ni * K);           // arena
-        o += al64(sizeof(double) * (size_t)7 * K * std::max(P[h].T.lm, 1));  // scratch
+        o += al64(sizeof(double) * (size_t)8 * K * std::max(P[h].T.lm, 1));  // scratch
         base[h] = tot;
         tot += o;
     }
@@ -435,8 +426,8 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         p += al64(sizeof(double) * std::max(dds, 1L));
         H.arena = (double *)p; H.arena_cap = (long)12 * ni * K;
         p += al64(sizeof(double) * (size_t)12 * ni * K);
-        H.scratch = (double *)p; H.scratch_cap = (long)7 * K * std::max(Q.T.lm, 1);
-        H.K = K; H.ltr = Q.T.ltr; H.lvl = Q.T.lvl;
+        H.scratch = (double *)p; H.scratch_cap = (long)8 * K * std::max(Q.T.lm, 1);
+        H.K = K; H.ltr = Q.T.ltr; H.lvl = Q.T.lvl; H.sq = sp[h].sq;
     }
     // ---- host tables: merges, leaves (with leaf lr/lc and D offsets), level steps
     std::vector<HssMerge> hh;
@@ -542,16 +533,24 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
             const HssNode *sn = dN + step_off[st];
             const HssCompact *sc = dC + comp_off[st];
             hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
-            if (nn < 512)
-                hipLaunchKernelGGL(k_hss_rowcomp<512>, dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+            if (sq && nn < 512)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, true>), dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+            else if (sq)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
+            else if (nn < 512)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, false>), dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
             else
-                hipLaunchKernelGGL(k_hss_rowcomp<256>, dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
             hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 1);
             hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
-            if (nn < 512)
-                hipLaunchKernelGGL(k_hss_colcomp<512>, dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+            if (sq && nn < 512)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, true>), dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+            else if (sq)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
+            else if (nn < 512)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, false>), dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
             else
-                hipLaunchKernelGGL(k_hss_colcomp<256>, dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
             hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 0);
             hipLaunchKernelGGL(k_hss_bgen, dim3(nn), dim3(WGS), 0, s, dH, sn, nn);
         }
@@ -1978,8 +1977,8 @@ int hsdc_loewner(hsdc_ctx *c, int k, double rho, double *dlamda, double *z, doub
     return info ? 1 : 0;
 }
 
-int hsdc_hss_evc(hsdc_ctx *c, int k, int ni, double tol, const double *f, const double *d, const double *u,
-                 const double *v, const double *difl, const double *difr, double *q, int ldq, int nrow, int *ranks)
+static int hss_one(hsdc_ctx *c, int sq, int k, int ni, double tol, const double *f, const double *d, const double *u,
+                   const double *v, const double *difl, const double *difr, double *q, int ldq, int nrow, int *ranks)
 {
     if (!c) return HSDC_ERR_INTERNAL;
     if (k < 1) return -2;
@@ -1993,7 +1992,7 @@ int hsdc_hss_evc(hsdc_ctx *c, int k, int ni, double tol, const double *f, const 
     for (int i = 0; i < 6; i++) HK(hipMemcpy(dst[i], src[i], sizeof(double) * k, hipMemcpyHostToDevice));
     c->evc_gen.assign(6 * (size_t)k, 0.0);
     for (int i = 0; i < 6; i++) memcpy(c->evc_gen.data() + (size_t)i * k, src[i], sizeof(double) * k);
-    c->evc_k = k; c->evc_ni = ni; c->evc_tol = tol;
+    c->evc_k = k; c->evc_ni = ni; c->evc_tol = tol; c->evc_sq = sq;
     CK(ensure(c->xb, sizeof(double) * (size_t)std::max(nrow, 1) * k));
     CK(ensure(c->yb, sizeof(double) * (size_t)std::max(nrow, 1) * k));
     HK(hipMemcpy2D(c->xb.p, sizeof(double) * nrow, q, sizeof(double) * ldq, sizeof(double) * nrow, k,
@@ -2002,6 +2001,7 @@ int hsdc_hss_evc(hsdc_ctx *c, int k, int ni, double tol, const double *f, const 
     sp.K = k;
     sp.FF = W.dlamda; sp.D0 = W.lam; sp.U0 = W.wnew; sp.V0 = W.alpha; sp.DIFL = W.difl; sp.DIFR = W.difr;
     sp.X = (const double *)c->xb.p; sp.ldx = nrow; sp.Y = (double *)c->yb.p; sp.ldy = nrow; sp.nrow = nrow;
+    sp.sq = sq;
     std::vector<HssSpec> v1(1, sp);
     std::vector<HssPlanM> keep;
     CK(hss_apply(c, v1, ni, tol, &keep));
@@ -2021,6 +2021,42 @@ int hsdc_hss_evc(hsdc_ctx *c, int k, int ni, double tol, const double *f, const 
     return 0;
 }
 
+int hsdc_hss_evc(hsdc_ctx *c, int k, int ni, double tol, const double *f, const double *d, const double *u,
+                 const double *v, const double *difl, const double *difr, double *q, int ldq, int nrow, int *ranks)
+{
+    return hss_one(c, 0, k, ni, tol, f, d, u, v, difl, difr, q, ldq, nrow, ranks);
+}
+
+int hsdc_hss_svc(hsdc_ctx *c, int k, int ni, double tol, const double *f, const double *d, const double *u,
+                 const double *v, const double *difl, const double *difr, double *q, int ldq, int nrow, int *ranks)
+{
+    return hss_one(c, 1, k, ni, tol, f, d, u, v, difl, difr, q, ldq, nrow, ranks);
+}
+
+// HSS batch of another solver (the bidiagonal SVD's merges) on its stream: build
+// every matrix of `sp`, Y := X * H.  *rank_max (may be null) gets the largest
+// generator rank; *dense the number of matrices applied densely (tree too small or
+// a node over the LDS / arena limits).
+int hsdc_hss_batch(hsdc_ctx *c, hipStream_t s, const std::vector<HssSpec> &sp, int ni, double tol, int *rank_max,
+                   int *dense)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    hipStream_t s0 = c->stream;
+    c->stream = s;
+    std::vector<HssPlanM> keep;
+    const int rc = hss_apply(c, sp, ni, tol, &keep);
+    c->stream = s0;
+    if (rc) return rc;
+    int rm = 0, nd = 0;
+    for (const HssPlanM &Q : keep) {
+        if (!Q.ok) { nd++; continue; }
+        for (int i = 0; i < Q.T.ltr - 1; i++) rm = std::max(rm, std::max(Q.usz[2 * i + 1], Q.vsz[2 * i]));
+    }
+    if (rank_max) *rank_max = rm;
+    if (dense) *dense = nd;
+    return 0;
+}
+
 int hsdc_hss_dense(hsdc_ctx *c, double *a, int lda)
 {
     if (!c) return HSDC_ERR_INTERNAL;
@@ -2030,9 +2066,9 @@ int hsdc_hss_dense(hsdc_ctx *c, double *a, int lda)
     std::vector<double> g = c->evc_gen;
     std::vector<double> eye((size_t)k * k, 0.0);
     for (int i = 0; i < k; i++) eye[(size_t)i * k + i] = 1.0;
-    int rc = hsdc_hss_evc(c, k, c->evc_ni, c->evc_tol, g.data(), g.data() + k, g.data() + 2 * (size_t)k,
-                          g.data() + 3 * (size_t)k, g.data() + 4 * (size_t)k, g.data() + 5 * (size_t)k, eye.data(), k,
-                          k, nullptr);
+    int rc = hss_one(c, c->evc_sq, k, c->evc_ni, c->evc_tol, g.data(), g.data() + k, g.data() + 2 * (size_t)k,
+                     g.data() + 3 * (size_t)k, g.data() + 4 * (size_t)k, g.data() + 5 * (size_t)k, eye.data(), k,
+                     k, nullptr);
     if (rc) return rc;
     for (int j = 0; j < k; j++) memcpy(a + (size_t)j * lda, eye.data() + (size_t)j * k, sizeof(double) * k);
     return 0;

@@ -24,6 +24,7 @@ struct HssMerge {
     long scratch_cap;
     int *err;               // bit0 block too large, bit1 arena full, bit2 scratch full
     int K, ltr, lvl;
+    int sq;                 // 1: u_i v_j / (f_i^2 - d_j^2), the SVD's vector matrices
 };
 
 // One node of one merge's post-ordered HSS tree (aux_hss.f90:737-800).
@@ -60,4 +61,19 @@ struct BG {
     // optional column map of C: output column j goes to C + (cmap[j] - cmoff) * ldc
     const int *cmap = nullptr;
     long cmoff = 0;
+};
+
+// One matrix of an HSS batch: generators on the device (K each), the product
+// Y = X * H with X nrow x K.  Column j of Y goes to Y + (ymap[j] - ymoff) * ldy when
+// ymap is set.
+struct HssSpec {
+    int K;
+    const double *FF, *D0, *U0, *V0, *DIFL, *DIFR;  // poles, roots, u, v, differences
+    const double *X;
+    double *Y;
+    long ldx, ldy;
+    int nrow;
+    const int *ymap = nullptr;
+    long ymoff = 0;
+    int sq = 0;        // squared variables (HssMerge::sq)
 };

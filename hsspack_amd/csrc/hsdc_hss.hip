@@ -33,6 +33,26 @@ __device__ __forceinline__ double dd(const HssMerge &H, int pj, int pk)
     if (pj < pk) return ((A1(H.FF, pj + 1) - A1(H.FF, pk)) + A1(H.DIFR, pj)) - A1(H.DIFL, pk);
     return ((A1(H.FF, pj) - A1(H.FF, pk + 1)) + A1(H.DIFL, pj)) - A1(H.DIFR, pk);
 }
+// The SVD's vector matrices u_i v_j / (f_i^2 - sigma_j^2) are Cauchy-like in the
+// squared variables: with SQ every displacement is the accurate difference times the
+// sum (D0 holds the roots sigma): pole-root, root-root and pole-pole.
+template <bool SQ>
+__device__ __forceinline__ double fdq(const HssMerge &H, int pd, int pw)
+{
+    const double r = fd(H, pd, pw);
+    return SQ ? r * (A1(H.FF, pd) + A1(H.D0, pw)) : r;
+}
+template <bool SQ>
+__device__ __forceinline__ double ddq(const HssMerge &H, int pj, int pk)
+{
+    const double r = dd(H, pj, pk);
+    return SQ ? r * (A1(H.D0, pj) + A1(H.D0, pk)) : r;
+}
+template <bool SQ>
+__device__ __forceinline__ double ppq(double a, double b)
+{
+    return SQ ? (a - b) * (a + b) : a - b;
+}
 
 // block-wide (max |value|, first index); idx <= 0 means "no entry"
 struct VI { double v; int i; };
@@ -86,6 +106,7 @@ __global__ __launch_bounds__(WG) void k_hss_dgen(const HssMerge *hm, const HssNo
         int pf = r0 + 1 + i, pw = r0 + 1 + j;
         double den = (pf <= pw) ? (A1(H.FF, pf) - A1(H.FF, pw)) - A1(H.DIFL, pw)
                                 : (A1(H.FF, pf) - A1(H.FF, pw + 1)) - A1(H.DIFR, pw);
+        if (H.sq) den = den * (A1(H.FF, pf) + A1(H.D0, pw));
         out[e] = A1(H.U0, pf) * A1(H.V0, pw) / den;
     }
     if (threadIdx.x == 0) H.dsz[N.node - 1] = m;
@@ -109,7 +130,7 @@ __device__ long scratch_take(const HssMerge &H, long n)
 // rows: the node's M row generators (LDS); columns: the N column generators
 // outside the node (private global copies).
 // ---------------------------------------------------------------------------
-template <int HSS_CWG>
+template <int HSS_CWG, bool SQ>
 __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, const HssNode *nodes, int nn,
                                                     double tol)
 {
@@ -136,7 +157,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         s_N = *H.lca - nic;
         // a merge whose construction already failed is finished densely; skip it
         s_flag = (*H.err != 0 || M > HSS_MAXM || M < 1 || nic < 1 || s_N < 0) ? 1 : 0;
-        s_off = s_flag ? -1 : scratch_take(H, 7L * (s_N > 0 ? s_N : 1));
+        s_off = s_flag ? -1 : scratch_take(H, 8L * (s_N > 0 ? s_N : 1));
         if (s_flag) atomicOr(H.err, 1);
     }
     __syncthreads();
@@ -147,7 +168,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     int *PU = (int *)(V + NS);
     // per column the pole/difference values its accurate differences read
     // (FF(pw), FF(pw+1), DIFL(pw), DIFR(pw) with pw = PU(j)), gathered once
-    double *CF0 = V + 2 * NS, *CF1 = CF0 + NS, *CL = CF1 + NS, *CR = CL + NS;
+    double *CF0 = V + 2 * NS, *CF1 = CF0 + NS, *CL = CF1 + NS, *CR = CL + NS, *CS = CR + NS;
     for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < c0 - 1) ? j + 1 : j + 1 + (c1e - c0 + 1);
         D[j] = A1(H.D, src); V[j] = A1(H.V, src);
@@ -155,14 +176,18 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         PU[j] = pw;
         CF0[j] = A1(H.FF, pw); CF1[j] = (pw < H.K) ? A1(H.FF, pw + 1) : 0.0;
         CL[j] = A1(H.DIFL, pw); CR[j] = A1(H.DIFR, pw);
+        CS[j] = SQ ? A1(H.D0, pw) : 0.0;
     }
-    // fd(H, pd, PU(j)) and dd(H, PU(j), pk) from the gathered values: same operations
+    // fdq<SQ>(H, pd, PU(j)) and ddq<SQ>(H, PU(j), pk) from the gathered values: same operations
     auto fdj = [&](int pd, int j) -> double {
-        return (pd <= A1(PU, j)) ? (A1(H.FF, pd) - A1(CF0, j)) - A1(CL, j) : (A1(H.FF, pd) - A1(CF1, j)) - A1(CR, j);
+        const double r = (pd <= A1(PU, j)) ? (A1(H.FF, pd) - A1(CF0, j)) - A1(CL, j)
+                                           : (A1(H.FF, pd) - A1(CF1, j)) - A1(CR, j);
+        return SQ ? r * (A1(H.FF, pd) + A1(CS, j)) : r;
     };
     auto ddj = [&](int j, int pk) -> double {
-        return (A1(PU, j) < pk) ? ((A1(CF1, j) - A1(H.FF, pk)) + A1(CR, j)) - A1(H.DIFL, pk)
-                                : ((A1(CF0, j) - A1(H.FF, pk + 1)) + A1(CL, j)) - A1(H.DIFR, pk);
+        const double r = (A1(PU, j) < pk) ? ((A1(CF1, j) - A1(H.FF, pk)) + A1(CR, j)) - A1(H.DIFL, pk)
+                                          : ((A1(CF0, j) - A1(H.FF, pk + 1)) + A1(CL, j)) - A1(H.DIFR, pk);
+        return SQ ? r * (A1(CS, j) + A1(H.D0, pk)) : r;
     };
     auto swapc = [&](int a, int c) {
         double t;
@@ -173,6 +198,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         t = A1(CF1, a); A1(CF1, a) = A1(CF1, c); A1(CF1, c) = t;
         t = A1(CL, a); A1(CL, a) = A1(CL, c); A1(CL, c) = t;
         t = A1(CR, a); A1(CR, a) = A1(CR, c); A1(CR, c) = t;
+        t = A1(CS, a); A1(CS, a) = A1(CS, c); A1(CS, c) = t;
     };
     for (int i = tid; i < M; i += HSS_CWG) {
         PL[i] = A1(H.PL, l0 + i);
@@ -196,7 +222,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             int pwk = A1(PU, k);
             for (int i = k + tid; i <= M; i += HSS_CWG) {
                 int pli = A1(PL, i);
-                double v = fabs(A1(Z, i) / fd(H, pli, pwk));
+                double v = fabs(A1(Z, i) / fdq<SQ>(H, pli, pwk));
                 x = vi_better(x, VI{v, i - k + 1});
             }
             VI cl = block_argmax(x, red);
@@ -212,7 +238,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 rw = block_argmax(y, red);
             }
             double junkL = cl.v * fabs(A1(V, k)), junkU = rw.v * fabs(A1(Z, k));
-            double piv = fabs(A1(Z, k) * A1(V, k) / fd(H, A1(PL, k), A1(PU, k)));
+            double piv = fabs(A1(Z, k) * A1(V, k) / fdq<SQ>(H, A1(PL, k), A1(PU, k)));
             if (!(junkL <= piv && junkU <= piv)) {
                 int flg = (junkL > junkU) ? 1 : 0;
                 int jjL = cl.i, jjU = rw.i;
@@ -247,7 +273,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     VI x2 = {0.0, 0};
                     int pwk2 = A1(PU, k);
                     for (int i = k + tid; i <= M; i += HSS_CWG) {
-                        double v = fabs(A1(Z, i) / fd(H, A1(PL, i), pwk2));
+                        double v = fabs(A1(Z, i) / fdq<SQ>(H, A1(PL, i), pwk2));
                         x2 = vi_better(x2, VI{v, i - k + 1});
                     }
                     jjL = block_argmax(x2, red).i;
@@ -257,7 +283,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             }
         }
         __syncthreads();
-        double ukk = A1(Z, k) * A1(V, k) / fd(H, A1(PL, k), A1(PU, k));
+        double ukk = A1(Z, k) * A1(V, k) / fdq<SQ>(H, A1(PL, k), A1(PU, k));
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
             // ---- CauchyPivtEig_CP: complete pivoting over the Schur complement
@@ -292,7 +318,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 swapc(k, b);
             }
             __syncthreads();
-            ukk = A1(Z, k) * A1(V, k) / fd(H, A1(PL, k), A1(PU, k));
+            ukk = A1(Z, k) * A1(V, k) / fdq<SQ>(H, A1(PL, k), A1(PU, k));
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
         }
@@ -301,8 +327,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         const double Fk = A1(F, k);
         const int plk = A1(PL, k), puk = A1(PU, k);
         for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
-            double temp = fd(H, A1(PL, j), puk);
-            temp = (A1(F, j) - Fk) / temp;
+            double temp = fdq<SQ>(H, A1(PL, j), puk);
+            temp = ppq<SQ>(A1(F, j), Fk) / temp;
             A1(Z, j) = A1(Z, j) * temp;
         }
         const bool fuse = k + 1 <= mn && !(k % 10 == 0 && M - k > 0);
@@ -318,15 +344,15 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             if (fuse) y1 = vi_better(y1, VI{fabs(vj / fdj(fk1, j)), j - k});
         }
         for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
-            double temp = -1.0 * fd(H, A1(PL, j), puk);
-            A1(W, j) = A1(W, j) * (temp / (Fk - A1(F, j)));
+            double temp = -1.0 * fdq<SQ>(H, A1(PL, j), puk);
+            A1(W, j) = A1(W, j) * (temp / ppq<SQ>(Fk, A1(F, j)));
         }
         __syncthreads();
         if (tid == 0) {
-            double wk = fd(H, plk, puk) / A1(A, k);
+            double wk = fdq<SQ>(H, plk, puk) / A1(A, k);
             for (int j = 1; j <= k - 1; j++) {
-                double temp = -1.0 * fd(H, plk, A1(PU, j));
-                wk = wk * (temp / (A1(F, j) - Fk));
+                double temp = -1.0 * fdq<SQ>(H, plk, A1(PU, j));
+                wk = wk * (temp / ppq<SQ>(A1(F, j), Fk));
             }
             A1(W, k) = wk;
         }
@@ -341,7 +367,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     double fj = A1(F, jc), mv = 0.0;
                     int mi = 0;
                     for (int i = 1; i <= M - k; i++) {
-                        double v = fabs(A1(Z, k + i) / (A1(F, k + i) - fj));
+                        double v = fabs(A1(Z, k + i) / ppq<SQ>(A1(F, k + i), fj));
                         if (mi == 0 || v > mv) { mv = v; mi = i; }
                     }
                     double junk = mv * fabs(A1(W, jc));
@@ -365,22 +391,22 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 }
                 for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
                     if (j == jj) continue;
-                    A1(Z, j) = A1(Z, j) * ((A1(F, j) - Fjj) / (A1(F, j) - Fii));
+                    A1(Z, j) = A1(Z, j) * (ppq<SQ>(A1(F, j), Fjj) / ppq<SQ>(A1(F, j), Fii));
                 }
                 for (int j = 1 + tid; j <= k; j += HSS_CWG) {
                     if (j == ii) continue;
-                    A1(W, j) = A1(W, j) * ((A1(F, j) - Fii) / (A1(F, j) - Fjj));
+                    A1(W, j) = A1(W, j) * (ppq<SQ>(A1(F, j), Fii) / ppq<SQ>(A1(F, j), Fjj));
                 }
                 __syncthreads();
                 if (tid == 0) {
-                    double uj = A1(A, ii) * ((Fii - Fjj) / fd(H, plii, A1(PU, ii)));
-                    double wi = fd(H, pljj, A1(PU, ii)) / A1(A, jj);
+                    double uj = A1(A, ii) * (ppq<SQ>(Fii, Fjj) / fdq<SQ>(H, plii, A1(PU, ii)));
+                    double wi = fdq<SQ>(H, pljj, A1(PU, ii)) / A1(A, jj);
                     for (int j = 1; j <= k; j++) {
                         if (j == ii) continue;
-                        double temp = fd(H, plii, A1(PU, j));
-                        uj = uj * ((Fii - A1(F, j)) / temp);
-                        temp = -1.0 * fd(H, pljj, A1(PU, j));
-                        wi = wi * (temp / (A1(F, j) - Fjj));
+                        double temp = fdq<SQ>(H, plii, A1(PU, j));
+                        uj = uj * (ppq<SQ>(Fii, A1(F, j)) / temp);
+                        temp = -1.0 * fdq<SQ>(H, pljj, A1(PU, j));
+                        wi = wi * (temp / ppq<SQ>(A1(F, j), Fjj));
                     }
                     A1(Z, jj) = uj;
                     A1(W, ii) = wi;
@@ -420,7 +446,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 int src = IP[i - 1];  // row src of Q
                 double q;
                 if (src <= r) q = (src == j) ? 1.0 : 0.0;
-                else q = A1(Z, src) * A1(W, j) / (A1(F, src) - A1(F, j));
+                else q = A1(Z, src) * A1(W, j) / ppq<SQ>(A1(F, src), A1(F, j));
                 Ug[e] = q;
             }
         } else {
@@ -442,7 +468,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
 // (ComprcauchyEig 'c').  RRLU rows: the node's M columns (roots, alpha, LDS);
 // RRLU columns: the N row generators outside the node (private global copies).
 // ---------------------------------------------------------------------------
-template <int HSS_CWG>
+template <int HSS_CWG, bool SQ>
 __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, const HssNode *nodes, int nn,
                                                     double tol)
 {
@@ -498,7 +524,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             VI x = {0.0, 0};
             int puk = A1(PU, k);
             for (int i = k + tid; i <= M; i += HSS_CWG) {
-                double v = fabs(A1(Z, i) / fd(H, puk, A1(PL, i)));
+                double v = fabs(A1(Z, i) / fdq<SQ>(H, puk, A1(PL, i)));
                 x = vi_better(x, VI{v, i - k + 1});
             }
             VI cl = block_argmax(x, red);
@@ -508,7 +534,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 int plk = A1(PL, k);
                 #pragma unroll 8
                 for (int j = k + tid; j <= N; j += HSS_CWG) {
-                    double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk));
+                    double v = fabs(A1(V, j) / fdq<SQ>(H, A1(PU, j), plk));
                     y = vi_better(y, VI{v, j - k + 1});
                 }
                 rw = block_argmax(y, red);
@@ -519,6 +545,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 den = (A1(H.FF, A1(PL, k) + 1) - A1(H.FF, A1(PU, k))) + A1(H.DIFR, A1(PL, k));
             else
                 den = (A1(H.FF, A1(PL, k)) - A1(H.FF, A1(PU, k))) + A1(H.DIFL, A1(PL, k));
+            if (SQ) den = den * (A1(H.D0, A1(PL, k)) + A1(H.FF, A1(PU, k)));
             double piv = fabs(A1(Z, k) * A1(V, k) / den);
             if (!(junkL <= piv && junkU <= piv)) {
                 int flg = (junkL > junkU) ? 1 : 0;
@@ -540,7 +567,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                         int plk2 = A1(PL, k);
                         #pragma unroll 8
                         for (int j = k + tid; j <= N; j += HSS_CWG) {
-                            double v = fabs(A1(V, j) / fd(H, A1(PU, j), plk2));
+                            double v = fabs(A1(V, j) / fdq<SQ>(H, A1(PU, j), plk2));
                             y2 = vi_better(y2, VI{v, j - k + 1});
                         }
                         jjU = block_argmax(y2, red).i;
@@ -559,7 +586,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     VI x2 = {0.0, 0};
                     int puk2 = A1(PU, k);
                     for (int i = k + tid; i <= M; i += HSS_CWG) {
-                        double v = fabs(A1(Z, i) / fd(H, puk2, A1(PL, i)));
+                        double v = fabs(A1(Z, i) / fdq<SQ>(H, puk2, A1(PL, i)));
                         x2 = vi_better(x2, VI{v, i - k + 1});
                     }
                     jjL = block_argmax(x2, red).i;
@@ -569,7 +596,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             }
         }
         __syncthreads();
-        double ukk = A1(Z, k) * A1(V, k) / (-1.0 * fd(H, A1(PU, k), A1(PL, k)));
+        double ukk = A1(Z, k) * A1(V, k) / (-1.0 * fdq<SQ>(H, A1(PU, k), A1(PL, k)));
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
             // ---- CauchyPivtEigm_CP
@@ -581,7 +608,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 double mv = 0.0;
                 int mi = 0;
                 for (int i = k; i <= M; i++) {
-                    double v = fabs(A1(Z, i) / fd(H, pu, A1(PL, i)));
+                    double v = fabs(A1(Z, i) / fdq<SQ>(H, pu, A1(PL, i)));
                     if (mi == 0 || v > mv) { mv = v; mi = i - k + 1; }
                 }
                 VI c = {fabs(mv * A1(V, j)), j - k + 1};
@@ -605,15 +632,15 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 u = A1(PU, k); A1(PU, k) = A1(PU, b); A1(PU, b) = u;
             }
             __syncthreads();
-            ukk = A1(Z, k) * A1(V, k) / (-1.0 * fd(H, A1(PU, k), A1(PL, k)));
+            ukk = A1(Z, k) * A1(V, k) / (-1.0 * fdq<SQ>(H, A1(PU, k), A1(PL, k)));
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
         }
         const double Fk = A1(F, k);
         const int plk = A1(PL, k), puk = A1(PU, k);
         for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
-            double temp = dd(H, A1(PL, j), plk);
-            double temp2 = -1.0 * fd(H, puk, A1(PL, j));
+            double temp = ddq<SQ>(H, A1(PL, j), plk);
+            double temp2 = -1.0 * fdq<SQ>(H, puk, A1(PL, j));
             temp = temp / temp2;
             A1(Z, j) = A1(Z, j) * temp;
         }
@@ -622,23 +649,23 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         VI y1 = {0.0, 0};
         #pragma unroll 8
         for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
-            double temp = fd(H, A1(PU, j), plk);
-            const double vj = A1(V, j) * ((A1(F, j) - Fk) / temp);
+            double temp = fdq<SQ>(H, A1(PU, j), plk);
+            const double vj = A1(V, j) * (ppq<SQ>(A1(F, j), Fk) / temp);
             A1(V, j) = vj;
-            if (fuse) y1 = vi_better(y1, VI{fabs(vj / fd(H, A1(PU, j), plk1)), j - k});
+            if (fuse) y1 = vi_better(y1, VI{fabs(vj / fdq<SQ>(H, A1(PU, j), plk1)), j - k});
         }
         for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
-            double temp = fd(H, puk, A1(PL, j));
-            double temp2 = dd(H, plk, A1(PL, j));
+            double temp = fdq<SQ>(H, puk, A1(PL, j));
+            double temp2 = ddq<SQ>(H, plk, A1(PL, j));
             temp = temp / temp2;
             A1(W, j) = A1(W, j) * temp;
         }
         __syncthreads();
         if (tid == 0) {
-            double wk = (-1.0 * fd(H, puk, plk)) / A1(A, k);
+            double wk = (-1.0 * fdq<SQ>(H, puk, plk)) / A1(A, k);
             for (int j = 1; j <= k - 1; j++) {
-                double temp = fd(H, A1(PU, j), plk);
-                double temp2 = dd(H, A1(PL, j), plk);
+                double temp = fdq<SQ>(H, A1(PU, j), plk);
+                double temp2 = ddq<SQ>(H, A1(PL, j), plk);
                 temp = temp / temp2;
                 wk = wk * temp;
             }
@@ -655,7 +682,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     double mv = 0.0;
                     int mi = 0;
                     for (int i = 1; i <= M - k; i++) {
-                        double tt = dd(H, A1(PL, i + k), pljc);
+                        double tt = ddq<SQ>(H, A1(PL, i + k), pljc);
                         double v = fabs(A1(Z, k + i) / tt);
                         if (mi == 0 || v > mv) { mv = v; mi = i; }
                     }
@@ -671,38 +698,38 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
                 #pragma unroll 8
                 for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
-                    double temp = fd(H, A1(PU, j), plii);
-                    double temp2 = fd(H, A1(PU, j), pljj);
+                    double temp = fdq<SQ>(H, A1(PU, j), plii);
+                    double temp2 = fdq<SQ>(H, A1(PU, j), pljj);
                     temp = temp / temp2;
                     A1(V, j) = A1(V, j) * temp;
                 }
                 for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
                     if (j == jj) continue;
-                    double temp = dd(H, A1(PL, j), pljj);
-                    double temp2 = dd(H, A1(PL, j), plii);
+                    double temp = ddq<SQ>(H, A1(PL, j), pljj);
+                    double temp2 = ddq<SQ>(H, A1(PL, j), plii);
                     temp = temp / temp2;
                     A1(Z, j) = A1(Z, j) * temp;
                 }
                 for (int j = 1 + tid; j <= k; j += HSS_CWG) {
                     if (j == ii) continue;
-                    double temp = dd(H, A1(PL, j), plii);
-                    double temp2 = dd(H, A1(PL, j), pljj);
+                    double temp = ddq<SQ>(H, A1(PL, j), plii);
+                    double temp2 = ddq<SQ>(H, A1(PL, j), pljj);
                     temp = temp / temp2;
                     A1(W, j) = A1(W, j) * temp;
                 }
                 __syncthreads();
                 if (tid == 0) {
-                    double temp = dd(H, plii, pljj);
-                    double temp2 = -1.0 * fd(H, A1(PU, ii), plii);
+                    double temp = ddq<SQ>(H, plii, pljj);
+                    double temp2 = -1.0 * fdq<SQ>(H, A1(PU, ii), plii);
                     double uj = A1(A, ii) * (temp / temp2);
-                    double wi = (-1.0 * fd(H, A1(PU, ii), pljj)) / A1(A, jj);
+                    double wi = (-1.0 * fdq<SQ>(H, A1(PU, ii), pljj)) / A1(A, jj);
                     for (int j = 1; j <= k; j++) {
                         if (j == ii) continue;
-                        temp = dd(H, plii, A1(PL, j));
-                        temp2 = -1.0 * fd(H, A1(PU, j), plii);
+                        temp = ddq<SQ>(H, plii, A1(PL, j));
+                        temp2 = -1.0 * fdq<SQ>(H, A1(PU, j), plii);
                         uj = uj * (temp / temp2);
-                        temp = fd(H, A1(PU, j), pljj);
-                        temp2 = dd(H, A1(PL, j), pljj);
+                        temp = fdq<SQ>(H, A1(PU, j), pljj);
+                        temp2 = ddq<SQ>(H, A1(PL, j), pljj);
                         wi = wi * (temp / temp2);
                     }
                     A1(Z, jj) = uj;
@@ -748,6 +775,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     double den;
                     if (pdii < pdjj) den = (A1(H.FF, pdii + 1) - A1(H.FF, pdjj)) + (A1(H.DIFR, pdii) - A1(H.DIFL, pdjj));
                     else den = (A1(H.FF, pdii) - A1(H.FF, pdjj + 1)) + (A1(H.DIFL, pdii) - A1(H.DIFR, pdjj));
+                    if (SQ) den = den * (A1(H.D0, pdii) + A1(H.D0, pdjj));
                     q = -1.0 * A1(W, i) * A1(Z, src) / den;
                 }
                 Vg[e] = q;
@@ -851,6 +879,7 @@ __global__ __launch_bounds__(WG) void k_hss_bgen(const HssMerge *hm, const HssNo
         double fj = A1(H.FF, pw), vj = A1(H.V, c0 + j);
         double den = (pf <= pw) ? (A1(H.FF, pf) - fj) - A1(H.DIFL, pw)
                                 : (A1(H.FF, pf) - A1(H.FF, pw + 1)) - A1(H.DIFR, pw);
+        if (H.sq) den = den * (A1(H.FF, pf) + A1(H.D0, pw));
         B[e] = A1(H.U, r0 + i) * vj / den;
     }
 }
@@ -1006,6 +1035,7 @@ __global__ __launch_bounds__(WG) void k_cbuild(const HssMerge *hm, int h, double
         int i = (int)(e % H.K) + 1, j = (int)(e / H.K) + 1;
         double den = (i <= j) ? (A1(H.FF, i) - A1(H.FF, j)) - A1(H.DIFL, j)
                               : (A1(H.FF, i) - A1(H.FF, j + 1)) - A1(H.DIFR, j);
+        if (H.sq) den = den * (A1(H.FF, i) + A1(H.D0, j));
         C[e] = A1(H.U0, i) * A1(H.V0, j) / den;
     }
 }
@@ -1018,7 +1048,11 @@ __global__ void k_hss_reset(const HssMerge *hm)
 // 256 threads per node when a launch has nodes for every CU twice over, 512 when
 // it has fewer (the upper tree levels): the search over the node's columns is the
 // sequential part of every pivot step
-template __global__ void k_hss_rowcomp<256>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_rowcomp<512>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_colcomp<256>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_colcomp<512>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_rowcomp<256, false>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_rowcomp<512, false>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_colcomp<256, false>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_colcomp<512, false>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_rowcomp<256, true>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_rowcomp<512, true>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_colcomp<256, true>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_colcomp<512, true>(const HssMerge *, const HssNode *, int, double);

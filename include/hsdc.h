@@ -144,7 +144,15 @@ int hsdc_hss_evc(hsdc_ctx *ctx, int k, int ni, double tol, const double *f, cons
                  const double *u, const double *v, const double *difl, const double *difr,
                  double *q, int ldq, int nrow, int *ranks);
 
-/* Dense expansion of the HSS form built by the last hsdc_hss_evc call (k x k). */
+/* DHSSVCS (CauchyHssvd_VPS.f90:3587-3804, the SVD merge's vector matrices): as
+ * hsdc_hss_evc for C(i,j) = u(i) v(j) / (f(i)^2 - d(j)^2), every displacement
+ * formed as the accurate difference times the sum (d(j) are the roots sigma_j). */
+int hsdc_hss_svc(hsdc_ctx *ctx, int k, int ni, double tol, const double *f, const double *d,
+                 const double *u, const double *v, const double *difl, const double *difr,
+                 double *q, int ldq, int nrow, int *ranks);
+
+/* Dense expansion of the HSS form built by the last hsdc_hss_evc / hsdc_hss_svc call
+ * (k x k). */
 int hsdc_hss_dense(hsdc_ctx *ctx, double *a, int lda);
 
 /* ---- learned starting iterate of the secular solver ----

@@ -41,6 +41,8 @@ typedef struct {
     int ni;       /* HSS leaf block size (mdlaed3.f90:253) */
     double tol;   /* HSS compression tolerance (CauchyHssEig.f90:2523) */
     int maxit;    /* secular iteration cap */
+    int bd_bsmlz; /* an SVD merge with K >= bd_bsmlz takes the HSS branch (mdlasd2.f90:172) */
+    int bd_ni;    /* its HSS leaf block size (mdlasd8.f90:273) */
 } orc_params;
 
 /* per-merge trace filled by orc_stedc when non-NULL */
@@ -153,6 +155,14 @@ void orc_hss_free(orc_hss *H);
 /* Cauchy fill with accurate differences (CauchylikeEig, CauchyHssEig.f90:1688). */
 void orc_cauchy_dense(int k, const double *f, const double *u, const double *v,
                       const double *difl, const double *difr, double *a, int lda);
+/* The SVD's vector matrices A(i,j) = u(i)v(j)/(f(i)^2 - d0(j)^2) (DHSSVCS,
+ * CauchyHssVD_VPS.f90:3587-3804): the same construction in the squared variables,
+ * every displacement formed as accurate difference x sum. */
+int orc_hss_build_sq(orc_hss *H, int k, int ni, const double *f, const double *d0,
+                     const double *u, const double *v, const double *difl, const double *difr,
+                     double tol);
+void orc_cauchy_dense_sq(int k, const double *f, const double *d0, const double *u, const double *v,
+                         const double *difl, const double *difr, double *a, int lda);
 
 /* ---- bidiagonal SVD (mdbdsdc.f90 / mdlasd0-2.f90 + LAPACK DLASD3/DLASD4/DLASDQ) ---- */
 /* LAPACK DLASCL 'G' on a vector, DLARTG, DLASDT (node arrays 0-based, values 1-based). */
@@ -180,9 +190,14 @@ int orc_lasd3(int nl, int nr, int sqre, int k, double *d, double *q, const doubl
 /* MDLASD1 with the dense branch for every K. */
 int orc_lasd1(int nl, int nr, int sqre, double *d, double *alpha, double *beta, double *u, int ldu,
               double *vt, int ldvt, int *idxq, const orc_params *p, int *kout, long long *iters,
-              int *nrot);
-/* MDLASD0; trace (may be NULL): level, n, k per merge and the deflation rotations in
- * the hss slot. */
+              int *nrot, int *hss_rank);
+/* MDLASD8 (mdlasd8.f90:201-278): roots d, updated z, DIFL, DIFR, APHAU, APHAV. */
+int orc_lasd8(int k, double *d, const double *dsigma, double *z, double *difl, double *difr,
+              double *aphau, double *aphav, int maxit, long long *iters);
+/* 1 when an HSS tree of k columns in leaves of ni has 2 to 512 leaves. */
+int orc_hss_tree_ok(int k, int ni);
+/* MDLASD0; trace (may be NULL): level, n, k per merge, the deflation rotations in the
+ * hss slot and the largest HSS generator rank in rank_max (0: dense branch). */
 int orc_lasd0(int n, int sqre, double *d, double *e, double *u, int ldu, double *vt, int ldvt,
               const orc_params *p, orc_trace *tr);
 /* MDBDSDC, COMPQ 'N' or 'I' ('N' still forms the vectors in u, vt). */

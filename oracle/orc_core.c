@@ -18,6 +18,8 @@ void orc_default_params(orc_params *p)
     p->ni = 200;             /* mdlaed3.f90:253 */
     p->tol = (double)1.0e-17f; /* CauchyHssEig.f90:2523, a REAL literal */
     p->maxit = 100;
+    p->bd_bsmlz = 2000;      /* mdlasd2.f90:172 */
+    p->bd_ni = 40;           /* mdlasd8.f90:273 */
 }
 
 /* ------------------------------------------------------------------ */

@@ -80,20 +80,23 @@ static void npart(int n, int ni, int *tr, int *ltr, int *m, int *lm)
 /* ---------------- accurate differences ---------------- */
 /* F(j)-D(k) = f_{PL(j)} - lambda_{PU(k)}  (computfd, CauchyHssEig.f90:519) */
 static inline double computfd(const double *FF, int j, int k, const int *PL, const int *PU,
-                              const double *DIFL, const double *DIFR)
+                              const double *DIFL, const double *DIFR, const double *S)
 {
     int pd = A1(PL, j), pw = A1(PU, k);
-    if (pd <= pw) return (A1(FF, pd) - A1(FF, pw)) - A1(DIFL, pw);
-    return (A1(FF, pd) - A1(FF, pw + 1)) - A1(DIFR, pw);
+    double r = (pd <= pw) ? (A1(FF, pd) - A1(FF, pw)) - A1(DIFL, pw) : (A1(FF, pd) - A1(FF, pw + 1)) - A1(DIFR, pw);
+    return S ? r * (A1(FF, pd) + A1(S, pw)) : r;
 }
 /* D(j)-D(k) = lambda_{PU(j)} - lambda_{PU(k)}  (computdd, CauchyHssEig.f90:470) */
 static inline double computdd(const double *FF, int j, int k, const int *PU,
-                              const double *DIFL, const double *DIFR)
+                              const double *DIFL, const double *DIFR, const double *S)
 {
     int pj = A1(PU, j), pk = A1(PU, k);
-    if (pj < pk) return ((A1(FF, pj + 1) - A1(FF, pk)) + A1(DIFR, pj)) - A1(DIFL, pk);
-    return ((A1(FF, pj) - A1(FF, pk + 1)) + A1(DIFL, pj)) - A1(DIFR, pk);
+    double r = (pj < pk) ? ((A1(FF, pj + 1) - A1(FF, pk)) + A1(DIFR, pj)) - A1(DIFL, pk)
+                         : ((A1(FF, pj) - A1(FF, pk + 1)) + A1(DIFL, pj)) - A1(DIFR, pk);
+    return S ? r * (A1(S, pj) + A1(S, pk)) : r;
 }
+/* difference of two poles: f_a - f_b, or f_a^2 - f_b^2 as (f_a - f_b)(f_a + f_b) */
+static inline double pp(double a, double b, const double *S) { return S ? (a - b) * (a + b) : a - b; }
 
 static inline void dswp(double *a, int i, int j) { double t = A1(a, i); A1(a, i) = A1(a, j); A1(a, j) = t; }
 static inline void iswp(int *a, int i, int j) { int t = A1(a, i); A1(a, i) = A1(a, j); A1(a, j) = t; }
@@ -107,51 +110,51 @@ static inline void iswp(int *a, int i, int j) { int t = A1(a, i); A1(a, i) = A1(
 
 /* CauchyMaxEigmc: column entries U(i)/(f_{PL(i)} - lambda_{PU(1)}) */
 static void maxEigmc(const double *FF, const int *PL, const int *PU, const double *DIFL,
-                     const double *DIFR, const double *U, int n, double *junk, int *jj)
+                     const double *DIFR, const double *S, const double *U, int n, double *junk, int *jj)
 {
     double best = -DBL_MAX; int loc = 0, first = 1;
     int dk = A1(PU, 1);
     for (int i = 1; i <= n; i++) {
         int pli = A1(PL, i);
-        double v;
-        if (pli <= dk) v = A1(U, i) / ((A1(FF, pli) - A1(FF, dk)) - A1(DIFL, dk));
-        else v = A1(U, i) / ((A1(FF, pli) - A1(FF, dk + 1)) - A1(DIFR, dk));
+        double den = (pli <= dk) ? (A1(FF, pli) - A1(FF, dk)) - A1(DIFL, dk) : (A1(FF, pli) - A1(FF, dk + 1)) - A1(DIFR, dk);
+        if (S) den = den * (A1(FF, pli) + A1(S, dk));
+        double v = A1(U, i) / den;
         MAXLOC_UPDATE(v, i);
     }
     *junk = best; *jj = loc;
 }
 /* CauchyMaxEigm: row entries V(i)/(f_{Fk} - lambda_{PU(i)}) */
 static void maxEigm(const double *FF, int fk, const int *PU, const double *DIFL,
-                    const double *DIFR, const double *V, int n, double *junk, int *jj)
+                    const double *DIFR, const double *S, const double *V, int n, double *junk, int *jj)
 {
     double best = -DBL_MAX; int loc = 0, first = 1;
     double f = A1(FF, fk);
     for (int i = 1; i <= n; i++) {
         int pui = A1(PU, i);
-        double v;
-        if (fk <= pui) v = A1(V, i) / ((f - A1(FF, pui)) - A1(DIFL, pui));
-        else v = A1(V, i) / ((f - A1(FF, pui + 1)) - A1(DIFR, pui));
+        double den = (fk <= pui) ? (f - A1(FF, pui)) - A1(DIFL, pui) : (f - A1(FF, pui + 1)) - A1(DIFR, pui);
+        if (S) den = den * (f + A1(S, pui));
+        double v = A1(V, i) / den;
         MAXLOC_UPDATE(v, i);
     }
     *junk = best; *jj = loc;
 }
 /* CauchyMaxEig: U(i)/(D(i)-F) */
-static void maxEig(const double *D, double f, const double *U, int n, double *junk, int *jj)
+static void maxEig(const double *D, double f, const double *S, const double *U, int n, double *junk, int *jj)
 {
     double best = -DBL_MAX; int loc = 0, first = 1;
     for (int i = 1; i <= n; i++) {
-        double v = A1(U, i) / (A1(D, i) - f);
+        double v = A1(U, i) / pp(A1(D, i), f, S);
         MAXLOC_UPDATE(v, i);
     }
     *junk = best; *jj = loc;
 }
 /* CauchyMaxEigmm: U(i)/(D(i+N)-D(k)) via computdd */
 static void maxEigmm(const double *FF, const int *PL, int k, const double *DIFL,
-                     const double *DIFR, const double *U, int m, int n, double *junk, int *jj)
+                     const double *DIFR, const double *S, const double *U, int m, int n, double *junk, int *jj)
 {
     double best = -DBL_MAX; int loc = 0, first = 1;
     for (int i = 1; i <= m; i++) {
-        double tt = computdd(FF, i + n, k, PL, DIFL, DIFR);
+        double tt = computdd(FF, i + n, k, PL, DIFL, DIFR, S);
         double v = A1(U, i) / tt;
         MAXLOC_UPDATE(v, i);
     }
@@ -161,18 +164,20 @@ static void maxEigmm(const double *FF, const int *PL, int k, const double *DIFL,
 /* CauchyPivtEig (CauchyHssEig.f90:571): rook-style pivot for the row RRLU */
 static void pivtEig(double *F, double *D, double *U, double *V, int *PL, int *PU, int k,
                     double *A, int M, int N, int *PQ, const double *FF, const double *DIFL,
-                    const double *DIFR)
+                    const double *DIFR, const double *S)
 {
     double junkL, junkU, piv;
     int jjL, jjU;
-    maxEigmc(FF, PL + k - 1, PU + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &junkL, &jjL);
-    maxEigm(FF, A1(PL, k), PU + k - 1, DIFL, DIFR, V + k - 1, N - k + 1, &junkU, &jjU);
+    maxEigmc(FF, PL + k - 1, PU + k - 1, DIFL, DIFR, S, U + k - 1, M - k + 1, &junkL, &jjL);
+    maxEigm(FF, A1(PL, k), PU + k - 1, DIFL, DIFR, S, V + k - 1, N - k + 1, &junkU, &jjU);
     junkL = junkL * fabs(A1(V, k));
     junkU = junkU * fabs(A1(U, k));
-    if (A1(PL, k) <= A1(PU, k))
-        piv = fabs(A1(U, k) * A1(V, k) / ((A1(FF, A1(PL, k)) - A1(FF, A1(PU, k))) - A1(DIFL, A1(PU, k))));
-    else
-        piv = fabs(A1(U, k) * A1(V, k) / ((A1(FF, A1(PL, k)) - A1(FF, A1(PU, k) + 1)) - A1(DIFR, A1(PU, k))));
+    {
+        int a = A1(PL, k), b = A1(PU, k);
+        double den = (a <= b) ? (A1(FF, a) - A1(FF, b)) - A1(DIFL, b) : (A1(FF, a) - A1(FF, b + 1)) - A1(DIFR, b);
+        if (S) den = den * (A1(FF, a) + A1(S, b));
+        piv = fabs(A1(U, k) * A1(V, k) / den);
+    }
     if (junkL <= piv && junkU <= piv) return;
     int flg = (junkL > junkU) ? 1 : 0;
     for (;;) {
@@ -180,13 +185,13 @@ static void pivtEig(double *F, double *D, double *U, double *V, int *PL, int *PU
             int jj = jjL;
             dswp(F, k, jj + k - 1); dswp(U, k, jj + k - 1); dswp(A, k, jj + k - 1);
             iswp(PL, k, jj + k - 1); iswp(PQ, k, jj + k - 1);
-            maxEigm(FF, A1(PL, k), PU + k - 1, DIFL, DIFR, V + k - 1, N - k + 1, &junkU, &jjU);
+            maxEigm(FF, A1(PL, k), PU + k - 1, DIFL, DIFR, S, V + k - 1, N - k + 1, &junkU, &jjU);
             if (jjU == 1) return;
             flg = 0;
         }
         int jj = jjU;
         dswp(D, k, jj + k - 1); dswp(V, k, jj + k - 1); iswp(PU, k, jj + k - 1);
-        maxEigmc(FF, PL + k - 1, PU + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &junkL, &jjL);
+        maxEigmc(FF, PL + k - 1, PU + k - 1, DIFL, DIFR, S, U + k - 1, M - k + 1, &junkL, &jjL);
         if (jjL == 1) return;
         flg = 1;
     }
@@ -195,15 +200,15 @@ static void pivtEig(double *F, double *D, double *U, double *V, int *PL, int *PU
 /* CauchyPivtEig_CP (CauchyHssEig.f90:1049): complete pivoting on the Schur complement */
 static void pivtEigCP(double *F, double *D, double *U, double *V, int *PL, int *PU, int k,
                       double *A, int M, int N, int *PQ, const double *FF, const double *DIFL,
-                      const double *DIFR)
+                      const double *DIFR, const double *S)
 {
     double ju, junk;
     int jju, ii, jj = 1;
-    maxEigmc(FF, PL + k - 1, PU + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &ju, &jju);
+    maxEigmc(FF, PL + k - 1, PU + k - 1, DIFL, DIFR, S, U + k - 1, M - k + 1, &ju, &jju);
     junk = fabs(ju * A1(V, k));
     ii = jju;
     for (int j = 2; j <= N - k + 1; j++) {
-        maxEigmc(FF, PL + k - 1, PU + k + j - 2, DIFL, DIFR, U + k - 1, M - k + 1, &ju, &jju);
+        maxEigmc(FF, PL + k - 1, PU + k + j - 2, DIFL, DIFR, S, U + k - 1, M - k + 1, &ju, &jju);
         double jt = fabs(ju * A1(V, k + j - 1));
         if (jt > junk) { junk = jt; ii = jju; jj = j; }
     }
@@ -215,18 +220,20 @@ static void pivtEigCP(double *F, double *D, double *U, double *V, int *PL, int *
 /* CauchyPivtEigm (CauchyHssEig.f90:737): rook-style pivot for the column RRLU */
 static void pivtEigm(double *D, double *F, double *U, double *V, int *PL, int *PU, int k,
                      double *A, int M, int N, int *PQ, const double *FF, const double *DIFL,
-                     const double *DIFR)
+                     const double *DIFR, const double *S)
 {
     double junkL, junkU, piv;
     int jjL, jjU;
-    maxEigm(FF, A1(PU, k), PL + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &junkL, &jjL);
-    maxEigmc(FF, PU + k - 1, PL + k - 1, DIFL, DIFR, V + k - 1, N - k + 1, &junkU, &jjU);
+    maxEigm(FF, A1(PU, k), PL + k - 1, DIFL, DIFR, S, U + k - 1, M - k + 1, &junkL, &jjL);
+    maxEigmc(FF, PU + k - 1, PL + k - 1, DIFL, DIFR, S, V + k - 1, N - k + 1, &junkU, &jjU);
     junkL = junkL * fabs(A1(V, k));
     junkU = junkU * fabs(A1(U, k));
-    if (A1(PL, k) < A1(PU, k))
-        piv = fabs(A1(U, k) * A1(V, k) / ((A1(FF, A1(PL, k) + 1) - A1(FF, A1(PU, k))) + A1(DIFR, A1(PL, k))));
-    else
-        piv = fabs(A1(U, k) * A1(V, k) / ((A1(FF, A1(PL, k)) - A1(FF, A1(PU, k))) + A1(DIFL, A1(PL, k))));
+    {
+        int a = A1(PL, k), b = A1(PU, k);  /* root a, pole b */
+        double den = (a < b) ? (A1(FF, a + 1) - A1(FF, b)) + A1(DIFR, a) : (A1(FF, a) - A1(FF, b)) + A1(DIFL, a);
+        if (S) den = den * (A1(S, a) + A1(FF, b));
+        piv = fabs(A1(U, k) * A1(V, k) / den);
+    }
     if (junkL <= piv && junkU <= piv) return;
     int flg = (junkL > junkU) ? 1 : 0;
     for (;;) {
@@ -234,13 +241,13 @@ static void pivtEigm(double *D, double *F, double *U, double *V, int *PL, int *P
             int jj = jjL;
             dswp(D, k, jj + k - 1); dswp(U, k, jj + k - 1); dswp(A, k, jj + k - 1);
             iswp(PL, k, jj + k - 1); iswp(PQ, k, jj + k - 1);
-            maxEigmc(FF, PU + k - 1, PL + k - 1, DIFL, DIFR, V + k - 1, N - k + 1, &junkU, &jjU);
+            maxEigmc(FF, PU + k - 1, PL + k - 1, DIFL, DIFR, S, V + k - 1, N - k + 1, &junkU, &jjU);
             if (jjU == 1) return;
             flg = 0;
         }
         int jj = jjU;
         dswp(F, k, jj + k - 1); dswp(V, k, jj + k - 1); iswp(PU, k, jj + k - 1);
-        maxEigm(FF, A1(PU, k), PL + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &junkL, &jjL);
+        maxEigm(FF, A1(PU, k), PL + k - 1, DIFL, DIFR, S, U + k - 1, M - k + 1, &junkL, &jjL);
         if (jjL == 1) return;
         flg = 1;
     }
@@ -249,15 +256,15 @@ static void pivtEigm(double *D, double *F, double *U, double *V, int *PL, int *P
 /* CauchyPivtEigm_CP (CauchyHssEig.f90:861) */
 static void pivtEigmCP(double *D, double *F, double *U, double *V, int *PL, int *PU, int k,
                        double *A, int M, int N, int *PQ, const double *FF, const double *DIFL,
-                       const double *DIFR)
+                       const double *DIFR, const double *S)
 {
     double ju, junk;
     int jju, ii, jj = 1;
-    maxEigm(FF, A1(PU, k), PL + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &ju, &jju);
+    maxEigm(FF, A1(PU, k), PL + k - 1, DIFL, DIFR, S, U + k - 1, M - k + 1, &ju, &jju);
     junk = fabs(ju * A1(V, k));
     ii = jju;
     for (int j = 2; j <= N - k + 1; j++) {
-        maxEigm(FF, A1(PU, k + j - 1), PL + k - 1, DIFL, DIFR, U + k - 1, M - k + 1, &ju, &jju);
+        maxEigm(FF, A1(PU, k + j - 1), PL + k - 1, DIFL, DIFR, S, U + k - 1, M - k + 1, &ju, &jju);
         double jt = fabs(ju * A1(V, k + j - 1));
         if (jt > junk) { junk = jt; ii = jju; jj = j; }
     }
@@ -268,7 +275,7 @@ static void pivtEigmCP(double *D, double *F, double *U, double *V, int *PL, int 
 
 /* rrluCauchyEigrow (CauchyHssEig.f90:7-212) */
 static void rrluRow(double *F, double *D, double *A, double *V, const double *DIFL,
-                    const double *DIFR, double tol, double *U, double *W, const double *FF,
+                    const double *DIFR, const double *S, double tol, double *U, double *W, const double *FF,
                     int *PL, int *PU, int M, int N, int *Rk, int *PQ, long long *nflops,
                     long long *nswap)
 {
@@ -278,37 +285,37 @@ static void rrluRow(double *F, double *D, double *A, double *V, const double *DI
     for (j = 1; j <= M; j++) { A1(U, j) = A1(A, j); A1(PQ, j) = j; }
     *Rk = mn;
     for (k = 1; k <= mn; k++) {
-        pivtEig(F, D, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR);
-        temp = computfd(FF, k, k, PL, PU, DIFL, DIFR);
+        pivtEig(F, D, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR, S);
+        temp = computfd(FF, k, k, PL, PU, DIFL, DIFR, S);
         ukk = A1(U, k) * A1(V, k) / temp;
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
-            pivtEigCP(F, D, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR);
-            temp = computfd(FF, k, k, PL, PU, DIFL, DIFR);
+            pivtEigCP(F, D, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR, S);
+            temp = computfd(FF, k, k, PL, PU, DIFL, DIFR, S);
             ukk = A1(U, k) * A1(V, k) / temp;
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { *Rk = k - 1; break; }
         }
         for (j = k + 1; j <= M; j++) {
-            temp = computfd(FF, j, k, PL, PU, DIFL, DIFR);
-            temp = (A1(F, j) - A1(F, k)) / temp;
+            temp = computfd(FF, j, k, PL, PU, DIFL, DIFR, S);
+            temp = pp(A1(F, j), A1(F, k), S) / temp;
             A1(U, j) = A1(U, j) * temp;
         }
         for (j = k + 1; j <= N; j++) {
-            temp = computdd(FF, j, k, PU, DIFL, DIFR);
-            temp2 = -1.0 * computfd(FF, k, j, PL, PU, DIFL, DIFR);
+            temp = computdd(FF, j, k, PU, DIFL, DIFR, S);
+            temp2 = -1.0 * computfd(FF, k, j, PL, PU, DIFL, DIFR, S);
             temp = temp / temp2;
             A1(V, j) = A1(V, j) * temp;
         }
         for (j = 1; j <= k - 1; j++) {
-            temp = -1.0 * computfd(FF, j, k, PL, PU, DIFL, DIFR);
-            A1(W, j) = A1(W, j) * (temp / (A1(F, k) - A1(F, j)));
+            temp = -1.0 * computfd(FF, j, k, PL, PU, DIFL, DIFR, S);
+            A1(W, j) = A1(W, j) * (temp / pp(A1(F, k), A1(F, j), S));
         }
-        temp = computfd(FF, k, k, PL, PU, DIFL, DIFR);
+        temp = computfd(FF, k, k, PL, PU, DIFL, DIFR, S);
         A1(W, k) = temp / A1(A, k);
         for (j = 1; j <= k - 1; j++) {
-            temp = -1.0 * computfd(FF, k, j, PL, PU, DIFL, DIFR);
-            A1(W, k) = A1(W, k) * (temp / (A1(F, j) - A1(F, k)));
+            temp = -1.0 * computfd(FF, k, j, PL, PU, DIFL, DIFR, S);
+            A1(W, k) = A1(W, k) * (temp / pp(A1(F, j), A1(F, k), S));
         }
         *nflops += 5 * (M - k) + 7 * (N - k) + 2 * 6 * (k - 1);
         int flgl = k % prd;
@@ -316,11 +323,11 @@ static void rrluRow(double *F, double *D, double *A, double *V, const double *DI
             flgl = 1;
             /* searchMax2Eig(U(k+1),W,F(k+1),F,M-k,k,...) */
             double junk; int jjL;
-            maxEig(F + k, A1(F, 1), U + k, M - k, &junk, &jjL);
+            maxEig(F + k, A1(F, 1), S, U + k, M - k, &junk, &jjL);
             junk = junk * fabs(A1(W, 1));
             nmax = junk; ii = 1; jj = jjL;
             for (j = 2; j <= k; j++) {
-                maxEig(F + k, A1(F, j), U + k, M - k, &junk, &jjL);
+                maxEig(F + k, A1(F, j), S, U + k, M - k, &junk, &jjL);
                 junk = junk * fabs(A1(W, j));
                 if (junk > nmax) { nmax = junk; ii = j; jj = jjL; }
             }
@@ -329,29 +336,29 @@ static void rrluRow(double *F, double *D, double *A, double *V, const double *DI
                 flgl = 0;
                 jj = jj + k;
                 for (j = k + 1; j <= N; j++) {
-                    temp = -1.0 * computfd(FF, ii, j, PL, PU, DIFL, DIFR);
-                    temp2 = -1.0 * computfd(FF, jj, j, PL, PU, DIFL, DIFR);
+                    temp = -1.0 * computfd(FF, ii, j, PL, PU, DIFL, DIFR, S);
+                    temp2 = -1.0 * computfd(FF, jj, j, PL, PU, DIFL, DIFR, S);
                     temp = temp / temp2;
                     A1(V, j) = A1(V, j) * temp;
                 }
                 for (j = k + 1; j <= jj - 1; j++)
-                    A1(U, j) = A1(U, j) * ((A1(F, j) - A1(F, jj)) / (A1(F, j) - A1(F, ii)));
+                    A1(U, j) = A1(U, j) * (pp(A1(F, j), A1(F, jj), S) / pp(A1(F, j), A1(F, ii), S));
                 for (j = jj + 1; j <= M; j++)
-                    A1(U, j) = A1(U, j) * ((A1(F, j) - A1(F, jj)) / (A1(F, j) - A1(F, ii)));
+                    A1(U, j) = A1(U, j) * (pp(A1(F, j), A1(F, jj), S) / pp(A1(F, j), A1(F, ii), S));
                 for (j = 1; j <= ii - 1; j++)
-                    A1(W, j) = A1(W, j) * ((A1(F, j) - A1(F, ii)) / (A1(F, j) - A1(F, jj)));
+                    A1(W, j) = A1(W, j) * (pp(A1(F, j), A1(F, ii), S) / pp(A1(F, j), A1(F, jj), S));
                 for (j = ii + 1; j <= k; j++)
-                    A1(W, j) = A1(W, j) * ((A1(F, j) - A1(F, ii)) / (A1(F, j) - A1(F, jj)));
-                temp = computfd(FF, ii, ii, PL, PU, DIFL, DIFR);
-                A1(U, jj) = A1(A, ii) * ((A1(F, ii) - A1(F, jj)) / temp);
-                temp = computfd(FF, jj, ii, PL, PU, DIFL, DIFR);
+                    A1(W, j) = A1(W, j) * (pp(A1(F, j), A1(F, ii), S) / pp(A1(F, j), A1(F, jj), S));
+                temp = computfd(FF, ii, ii, PL, PU, DIFL, DIFR, S);
+                A1(U, jj) = A1(A, ii) * (pp(A1(F, ii), A1(F, jj), S) / temp);
+                temp = computfd(FF, jj, ii, PL, PU, DIFL, DIFR, S);
                 A1(W, ii) = temp / A1(A, jj);
                 for (j = 1; j <= k; j++) {
                     if (j == ii) continue;
-                    temp = computfd(FF, ii, j, PL, PU, DIFL, DIFR);
-                    A1(U, jj) = A1(U, jj) * ((A1(F, ii) - A1(F, j)) / temp);
-                    temp = -1.0 * computfd(FF, jj, j, PL, PU, DIFL, DIFR);
-                    A1(W, ii) = A1(W, ii) * (temp / (A1(F, j) - A1(F, jj)));
+                    temp = computfd(FF, ii, j, PL, PU, DIFL, DIFR, S);
+                    A1(U, jj) = A1(U, jj) * (pp(A1(F, ii), A1(F, j), S) / temp);
+                    temp = -1.0 * computfd(FF, jj, j, PL, PU, DIFL, DIFR, S);
+                    A1(W, ii) = A1(W, ii) * (temp / pp(A1(F, j), A1(F, jj), S));
                 }
                 iswp(PL, ii, jj); iswp(PQ, ii, jj);
                 dswp(F, ii, jj); dswp(A, ii, jj);
@@ -363,7 +370,7 @@ static void rrluRow(double *F, double *D, double *A, double *V, const double *DI
 
 /* rrluCauchyEigcol (CauchyHssEig.f90:215-467) */
 static void rrluCol(double *D, double *F, double *A, double *V, const double *DIFL,
-                    const double *DIFR, double tol, double *U, double *W, const double *FF,
+                    const double *DIFR, const double *S, double tol, double *U, double *W, const double *FF,
                     int *PL, int *PU, int M, int N, int *Rk, int *PQ, long long *nflops,
                     long long *nswap)
 {
@@ -373,38 +380,38 @@ static void rrluCol(double *D, double *F, double *A, double *V, const double *DI
     for (j = 1; j <= M; j++) { A1(U, j) = A1(A, j); A1(PQ, j) = j; }
     *Rk = mn;
     for (k = 1; k <= mn; k++) {
-        pivtEigm(D, F, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR);
-        temp = -1.0 * computfd(FF, k, k, PU, PL, DIFL, DIFR);
+        pivtEigm(D, F, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR, S);
+        temp = -1.0 * computfd(FF, k, k, PU, PL, DIFL, DIFR, S);
         ukk = A1(U, k) * A1(V, k) / temp;
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
-            pivtEigmCP(D, F, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR);
-            temp = -1.0 * computfd(FF, k, k, PU, PL, DIFL, DIFR);
+            pivtEigmCP(D, F, U, V, PL, PU, k, A, M, N, PQ, FF, DIFL, DIFR, S);
+            temp = -1.0 * computfd(FF, k, k, PU, PL, DIFL, DIFR, S);
             ukk = A1(U, k) * A1(V, k) / temp;
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { *Rk = k - 1; break; }
         }
         for (j = k + 1; j <= M; j++) {
-            temp = computdd(FF, j, k, PL, DIFL, DIFR);
-            temp2 = -1.0 * computfd(FF, k, j, PU, PL, DIFL, DIFR);
+            temp = computdd(FF, j, k, PL, DIFL, DIFR, S);
+            temp2 = -1.0 * computfd(FF, k, j, PU, PL, DIFL, DIFR, S);
             temp = temp / temp2;
             A1(U, j) = A1(U, j) * temp;
         }
         for (j = k + 1; j <= N; j++) {
-            temp = computfd(FF, j, k, PU, PL, DIFL, DIFR);
-            A1(V, j) = A1(V, j) * ((A1(F, j) - A1(F, k)) / temp);
+            temp = computfd(FF, j, k, PU, PL, DIFL, DIFR, S);
+            A1(V, j) = A1(V, j) * (pp(A1(F, j), A1(F, k), S) / temp);
         }
         for (j = 1; j <= k - 1; j++) {
-            temp = computfd(FF, k, j, PU, PL, DIFL, DIFR);
-            temp2 = computdd(FF, k, j, PL, DIFL, DIFR);
+            temp = computfd(FF, k, j, PU, PL, DIFL, DIFR, S);
+            temp2 = computdd(FF, k, j, PL, DIFL, DIFR, S);
             temp = temp / temp2;
             A1(W, j) = A1(W, j) * temp;
         }
-        temp = -1.0 * computfd(FF, k, k, PU, PL, DIFL, DIFR);
+        temp = -1.0 * computfd(FF, k, k, PU, PL, DIFL, DIFR, S);
         A1(W, k) = temp / A1(A, k);
         for (j = 1; j <= k - 1; j++) {
-            temp = computfd(FF, j, k, PU, PL, DIFL, DIFR);
-            temp2 = computdd(FF, j, k, PL, DIFL, DIFR);
+            temp = computfd(FF, j, k, PU, PL, DIFL, DIFR, S);
+            temp2 = computdd(FF, j, k, PL, DIFL, DIFR, S);
             temp = temp / temp2;
             A1(W, k) = A1(W, k) * temp;
         }
@@ -413,11 +420,11 @@ static void rrluCol(double *D, double *F, double *A, double *V, const double *DI
         while (flgl < 1) {
             flgl = 1;
             double junk; int jjL;
-            maxEigmm(FF, PL, 1, DIFL, DIFR, U + k, M - k, k, &junk, &jjL);
+            maxEigmm(FF, PL, 1, DIFL, DIFR, S, U + k, M - k, k, &junk, &jjL);
             junk = junk * fabs(A1(W, 1));
             nmax = junk; ii = 1; jj = jjL;
             for (j = 2; j <= k; j++) {
-                maxEigmm(FF, PL, j, DIFL, DIFR, U + k, M - k, k, &junk, &jjL);
+                maxEigmm(FF, PL, j, DIFL, DIFR, S, U + k, M - k, k, &junk, &jjL);
                 junk = junk * fabs(A1(W, j));
                 if (junk > nmax) { nmax = junk; ii = j; jj = jjL; }
             }
@@ -426,39 +433,39 @@ static void rrluCol(double *D, double *F, double *A, double *V, const double *DI
                 flgl = 0;
                 jj = jj + k;
                 for (j = k + 1; j <= N; j++) {
-                    temp = computfd(FF, j, ii, PU, PL, DIFL, DIFR);
-                    temp2 = computfd(FF, j, jj, PU, PL, DIFL, DIFR);
+                    temp = computfd(FF, j, ii, PU, PL, DIFL, DIFR, S);
+                    temp2 = computfd(FF, j, jj, PU, PL, DIFL, DIFR, S);
                     temp = temp / temp2;
                     A1(V, j) = A1(V, j) * temp;
                 }
                 for (j = k + 1; j <= M; j++) {
                     if (j == jj) continue;
-                    temp = computdd(FF, j, jj, PL, DIFL, DIFR);
-                    temp2 = computdd(FF, j, ii, PL, DIFL, DIFR);
+                    temp = computdd(FF, j, jj, PL, DIFL, DIFR, S);
+                    temp2 = computdd(FF, j, ii, PL, DIFL, DIFR, S);
                     temp = temp / temp2;
                     A1(U, j) = A1(U, j) * temp;
                 }
                 for (j = 1; j <= k; j++) {
                     if (j == ii) continue;
-                    temp = computdd(FF, j, ii, PL, DIFL, DIFR);
-                    temp2 = computdd(FF, j, jj, PL, DIFL, DIFR);
+                    temp = computdd(FF, j, ii, PL, DIFL, DIFR, S);
+                    temp2 = computdd(FF, j, jj, PL, DIFL, DIFR, S);
                     temp = temp / temp2;
                     A1(W, j) = A1(W, j) * temp;
                 }
-                temp = computdd(FF, ii, jj, PL, DIFL, DIFR);
-                temp2 = -1.0 * computfd(FF, ii, ii, PU, PL, DIFL, DIFR);
+                temp = computdd(FF, ii, jj, PL, DIFL, DIFR, S);
+                temp2 = -1.0 * computfd(FF, ii, ii, PU, PL, DIFL, DIFR, S);
                 temp = temp / temp2;
                 A1(U, jj) = A1(A, ii) * temp;
-                temp = -1.0 * computfd(FF, ii, jj, PU, PL, DIFL, DIFR);
+                temp = -1.0 * computfd(FF, ii, jj, PU, PL, DIFL, DIFR, S);
                 A1(W, ii) = temp / A1(A, jj);
                 for (j = 1; j <= k; j++) {
                     if (j == ii) continue;
-                    temp = computdd(FF, ii, j, PL, DIFL, DIFR);
-                    temp2 = -1.0 * computfd(FF, j, ii, PU, PL, DIFL, DIFR);
+                    temp = computdd(FF, ii, j, PL, DIFL, DIFR, S);
+                    temp2 = -1.0 * computfd(FF, j, ii, PU, PL, DIFL, DIFR, S);
                     temp = temp / temp2;
                     A1(U, jj) = A1(U, jj) * temp;
-                    temp = computfd(FF, j, jj, PU, PL, DIFL, DIFR);
-                    temp2 = computdd(FF, j, jj, PL, DIFL, DIFR);
+                    temp = computfd(FF, j, jj, PU, PL, DIFL, DIFR, S);
+                    temp2 = computdd(FF, j, jj, PL, DIFL, DIFR, S);
                     temp = temp / temp2;
                     A1(W, ii) = A1(W, ii) * temp;
                 }
@@ -480,7 +487,7 @@ static void invp(int *P, int n)
 
 /* CauchylikeEig (CauchyHssEig.f90:1688): A(i,j) = U(i)V(j)/(f_{PL(i)} - lambda_{PU(j)}) */
 static void cauchylikeEig(double *a, int lda, const double *F, const double *U, const double *V,
-                          const double *DIFL, const double *DIFR, const int *PL, const int *PU,
+                          const double *DIFL, const double *DIFR, const double *S, const int *PL, const int *PU,
                           int M, int N)
 {
     for (int j = 1; j <= N; j++) {
@@ -491,6 +498,7 @@ static void cauchylikeEig(double *a, int lda, const double *F, const double *U, 
             double den;
             if (pfii <= pdjj) den = (A1(F, pfii) - fj) - A1(DIFL, pdjj);
             else den = (A1(F, pfii) - A1(F, pdjj + 1)) - A1(DIFR, pdjj);
+            if (S) den = den * (A1(F, pfii) + A1(S, pdjj));
             a[(i - 1) + (long)(j - 1) * lda] = A1(U, i) * vj / den;
         }
     }
@@ -510,14 +518,14 @@ static long push(orc_hss *H, long *cap, long n)
 
 /* ComprcauchyEig (CauchyHssEig.f90:2253-2440) */
 static void comprEig(orc_hss *H, long *cap, char rc, double *F, double *D, double *U, double *V,
-                     const double *DIFL, const double *DIFR, const double *FF, int *PL, int *PU,
+                     const double *DIFL, const double *DIFR, const double *S, const double *FF, int *PL, int *PU,
                      double tol, int M, int N, int nodi, int *Rk)
 {
     int mn = M < N ? M : N, i, j;
     if (rc == 'r') {
         double *Z = (double *)calloc(M + 1, sizeof(double)), *W = (double *)calloc(mn + 1, sizeof(double));
         int *PQ = (int *)malloc(sizeof(int) * (M + 1));
-        rrluRow(F, D, U, V, DIFL, DIFR, tol, Z, W, FF, PL, PU, M, N, Rk, PQ, &H->nflops, &H->nswap);
+        rrluRow(F, D, U, V, DIFL, DIFR, S, tol, Z, W, FF, PL, PU, M, N, Rk, PQ, &H->nflops, &H->nswap);
         invp(PQ, M);
         int r = *Rk;
         if (r < mn) {
@@ -526,7 +534,7 @@ static void comprEig(orc_hss *H, long *cap, char rc, double *F, double *D, doubl
             for (j = 1; j <= r; j++) {
                 Q[(j - 1) + (long)(j - 1) * M] = 1.0;
                 for (i = r + 1; i <= M; i++)
-                    Q[(i - 1) + (long)(j - 1) * M] = A1(Z, i) * A1(W, j) / (A1(F, i) - A1(F, j));
+                    Q[(i - 1) + (long)(j - 1) * M] = A1(Z, i) * A1(W, j) / pp(A1(F, i), A1(F, j), S);
             }
             long off = push(H, cap, (long)M * r);
             for (j = 0; j < r; j++)
@@ -545,7 +553,7 @@ static void comprEig(orc_hss *H, long *cap, char rc, double *F, double *D, doubl
         /* column compression: the rows of the RRLU are this node's N columns */
         double *Z = (double *)calloc(N + 1, sizeof(double)), *W = (double *)calloc(mn + 1, sizeof(double));
         int *PQ = (int *)malloc(sizeof(int) * (N + 1));
-        rrluCol(D, F, V, U, DIFL, DIFR, tol, Z, W, FF, PU, PL, N, M, Rk, PQ, &H->nflops, &H->nswap);
+        rrluCol(D, F, V, U, DIFL, DIFR, S, tol, Z, W, FF, PU, PL, N, M, Rk, PQ, &H->nflops, &H->nswap);
         invp(PQ, N);
         int r = *Rk;
         if (r < mn) {
@@ -559,6 +567,7 @@ static void comprEig(orc_hss *H, long *cap, char rc, double *F, double *D, doubl
                     double den;
                     if (pdii < pdjj) den = (A1(FF, pdii + 1) - A1(FF, pdjj)) + (A1(DIFR, pdii) - A1(DIFL, pdjj));
                     else den = (A1(FF, pdii) - A1(FF, pdjj + 1)) + (A1(DIFL, pdii) - A1(DIFR, pdjj));
+                    if (S) den = den * (A1(S, pdii) + A1(S, pdjj));
                     Q[(i - 1) + (long)(j - 1) * r] = -1.0 * A1(W, i) * A1(Z, j) / den;
                 }
             }
@@ -589,9 +598,13 @@ void orc_hss_free(orc_hss *H)
 
 #define TTR(H, LL, i) (H)->ttr[((LL) - 1) * (H)->ltr + (i) - 1]
 
-/* DHSSEVC construction + Cauchy2hssEig_omp, sequential */
-int orc_hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, const double *u0,
-                  const double *v0, const double *DIFL, const double *DIFR, double tol)
+/* DHSSEVC construction + Cauchy2hssEig_omp, sequential.  S == NULL: the
+ * eigenvector matrix u_i v_j / (f_i - d_j); S == d0: the SVD's vector matrices
+ * u_i v_j / (f_i^2 - d_j^2), every displacement formed as the accurate difference
+ * times the sum (the same RRLU in the variables f^2 and d^2). */
+static int hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, const double *u0,
+                     const double *v0, const double *DIFL, const double *DIFR, const double *S,
+                     double tol)
 {
     memset(H, 0, sizeof(*H));
     H->k = K;
@@ -666,7 +679,7 @@ int orc_hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, 
     for (int i = 1; i <= ltr; i++) {
         if (CH1(ch, i) == 0 && CH2(ch, i) == 0) {
             int mi = LR(i, 2) - LR(i, 1) + 1;
-            cauchylikeEig(H->dd + H->pd[i - 1], mi, f, U + LR(i, 1) - 1, V + LR(i, 1) - 1, DIFL, DIFR,
+            cauchylikeEig(H->dd + H->pd[i - 1], mi, f, U + LR(i, 1) - 1, V + LR(i, 1) - 1, DIFL, DIFR, S,
                           PL + LR(i, 1) - 1, PU + LR(i, 1) - 1, mi, mi);
             H->dsz[i - 1] = mi;
         }
@@ -690,7 +703,7 @@ int orc_hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, 
                 PUi[j] = A1(PU, indx[j]); Vi[j] = A1(V, indx[j]); Di[j] = A1(D, indx[j]);
             }
             for (int j = 0; j < mi; j++) Fi[j] = A1(f, A1(PL, LR(nodi, 1) + j));
-            comprEig(H, &cap, 'r', Fi, Di, U + LR(nodi, 1) - 1, Vi, DIFL, DIFR, f,
+            comprEig(H, &cap, 'r', Fi, Di, U + LR(nodi, 1) - 1, Vi, DIFL, DIFR, S, f,
                      PL + LR(nodi, 1) - 1, PUi, tol, mi, nt, nodi, &Rk);
         }
         /* compact surviving row generators */
@@ -725,7 +738,7 @@ int orc_hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, 
             for (int j = 0; j < nt; j++) {
                 Ui[j] = A1(U, indx[j]); PLi[j] = A1(PL, indx[j]); Fi[j] = A1(f, A1(PL, indx[j]));
             }
-            comprEig(H, &cap, 'c', Fi, D + LC(nodi, 1) - 1, Ui, V + LC(nodi, 1) - 1, DIFL, DIFR, f,
+            comprEig(H, &cap, 'c', Fi, D + LC(nodi, 1) - 1, Ui, V + LC(nodi, 1) - 1, DIFL, DIFR, S, f,
                      PLi, PU + LC(nodi, 1) - 1, tol, nt, mi, nodi, &Rk);
         }
         /* compact surviving column generators */
@@ -764,7 +777,7 @@ int orc_hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, 
             int mi = LR(nodi, 2) - LR(nodi, 1) + 1;
             int nj = LC(nodj, 2) - LC(nodj, 1) + 1;
             long off = push(H, &cap, (long)mi * nj);
-            cauchylikeEig(H->h + off, mi, f, U + LR(nodi, 1) - 1, V + LC(nodj, 1) - 1, DIFL, DIFR,
+            cauchylikeEig(H->h + off, mi, f, U + LR(nodi, 1) - 1, V + LC(nodj, 1) - 1, DIFL, DIFR, S,
                           PL + LR(nodi, 1) - 1, PU + LC(nodj, 1) - 1, mi, nj);
             H->bsz[2 * (nodi - 1)] = mi; H->bsz[2 * (nodi - 1) + 1] = nj; H->pb[nodi - 1] = off;
             H->nflops += 6L * mi * nj;
@@ -775,6 +788,18 @@ int orc_hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, 
     free(D); free(U); free(V); free(PL); free(PU); free(lr); free(lc); free(indx);
     free(Di); free(Fi); free(Ui); free(Vi); free(PLi); free(PUi);
     return 0;
+}
+
+int orc_hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, const double *u0,
+                  const double *v0, const double *DIFL, const double *DIFR, double tol)
+{
+    return hss_build(H, K, ni, f, d0, u0, v0, DIFL, DIFR, NULL, tol);
+}
+
+int orc_hss_build_sq(orc_hss *H, int K, int ni, const double *f, const double *d0, const double *u0,
+                     const double *v0, const double *DIFL, const double *DIFR, double tol)
+{
+    return hss_build(H, K, ni, f, d0, u0, v0, DIFL, DIFR, d0, tol);
 }
 
 static void gemm_acc(int m, int n, int k, const double *a, int lda, const double *b, int ldb,
@@ -867,6 +892,15 @@ void orc_cauchy_dense(int k, const double *f, const double *u, const double *v,
 {
     int *p = (int *)malloc(sizeof(int) * k);
     for (int i = 0; i < k; i++) p[i] = i + 1;
-    cauchylikeEig(a, lda, f, u, v, difl, difr, p, p, k, k);
+    cauchylikeEig(a, lda, f, u, v, difl, difr, NULL, p, p, k, k);
+    free(p);
+}
+
+void orc_cauchy_dense_sq(int k, const double *f, const double *d0, const double *u, const double *v,
+                         const double *difl, const double *difr, double *a, int lda)
+{
+    int *p = (int *)malloc(sizeof(int) * k);
+    for (int i = 0; i < k; i++) p[i] = i + 1;
+    cauchylikeEig(a, lda, f, u, v, difl, difr, d0, p, p, k, k);
     free(p);
 }

@@ -604,6 +604,76 @@ done:
 }
 
 /* ------------------------------------------------------------------ */
+/* The secular half of DLASD3 / MDLASD8: z scaled to unit norm (T-lane sums), the
+ * roots d(1:k) with their origins and offsets, and z replaced by the updated z of the
+ * Loewner products (DLASD3's order); zs keeps the signs of the input z.  Returns 0 or
+ * the 1-based index of a root that did not converge. */
+static int sv_solve(int k, double *d, const double *dsigma, double *z, double *zs, int *org,
+                    double *tau, int maxit, long long *iters)
+{
+    int i, j;
+    memcpy(zs, z, sizeof(double) * k);
+    const int T = orc_sec_team(k);
+    double ps[64];
+    for (int l = 0; l < T; l++) {
+        ps[l] = 0.0;
+        for (i = l; i < k; i += T) ps[l] += z[i] * z[i];
+    }
+    double rho = sqrt(tsumT(ps, T));
+    orc_lascl(rho, 1.0, k, z);
+    rho = rho * rho;
+    for (j = 1; j <= k; j++) {
+        int nit = 0;
+        double sg;
+        int r = orc_lasd4(k, j, dsigma, z, rho, &sg, &org[j - 1], &tau[j - 1], &nit, maxit);
+        d[j - 1] = sg;
+        if (iters) *iters += nit;
+        if (r) return j;
+    }
+    for (i = 1; i <= k; i++) {
+        double di = dsigma[i - 1];
+        double zi = sv_den(di, dsigma[org[k - 1] - 1], tau[k - 1]);
+        for (j = 1; j <= i - 1; j++)
+            zi = zi * (sv_den(di, dsigma[org[j - 1] - 1], tau[j - 1]) / (di - dsigma[j - 1]) /
+                       (di + dsigma[j - 1]));
+        for (j = i; j <= k - 1; j++)
+            zi = zi * (sv_den(di, dsigma[org[j - 1] - 1], tau[j - 1]) / (di - dsigma[j]) /
+                       (di + dsigma[j]));
+        double a = sqrt(fabs(zi));
+        z[i - 1] = zs[i - 1] >= 0.0 ? a : -a;
+    }
+    return 0;
+}
+
+/* 1 / 2-norms of the left vector (-1, d_2 w_2, ..., d_k w_k) and the right vector
+ * (w_1, ..., w_k) of root i, w_j = z_j / ((d_j - sigma_i)(d_j + sigma_i)), in T-lane
+ * interleaved sums; wv receives w. */
+static void sv_norms(int k, const double *dsigma, const double *z, double dorg, double ti, double *wv,
+                     double *tu, double *tv)
+{
+    const int T = orc_sec_team(k);
+    double ps[64];
+    int j;
+    for (j = 1; j <= k; j++) {
+        double dj = dsigma[j - 1];
+        wv[j - 1] = z[j - 1] / ((dj - dorg) - ti) / ((dj + dorg) + ti);
+    }
+    for (int l = 0; l < T; l++) {
+        ps[l] = 0.0;
+        for (j = l; j < k; j += T) {
+            double x = j == 0 ? -1.0 : dsigma[j] * wv[j];
+            ps[l] += x * x;
+        }
+    }
+    *tu = sqrt(tsumT(ps, T));
+    for (int l = 0; l < T; l++) {
+        ps[l] = 0.0;
+        for (j = l; j < k; j += T) ps[l] += wv[j] * wv[j];
+    }
+    *tv = sqrt(tsumT(ps, T));
+}
+
+/* ------------------------------------------------------------------ */
 /* DLASD3 (the dense branch, mdlasd1.f90:239-244): roots, updated z, singular vectors
  * of the deflated problem and their products with U2 / VT2.  q is workspace of
  * 2 k^2 + 3 k doubles. */
@@ -622,67 +692,20 @@ int orc_lasd3(int nl, int nr, int sqre, int k, double *d, double *q, const doubl
     }
     double *qu = q, *qv = q + (long)k * k, *zs = qv + (long)k * k, *tau = zs + k, *wv = tau + k;
     int *org = (int *)malloc(sizeof(int) * k);
-    memcpy(zs, z, sizeof(double) * k);
     const int T = orc_sec_team(k);
-    /* |z| in T-lane interleaved partial sums */
     double ps[64];
-    for (int l = 0; l < T; l++) {
-        ps[l] = 0.0;
-        for (i = l; i < k; i += T) ps[l] += z[i] * z[i];
-    }
-    double rho = sqrt(tsumT(ps, T));
-    orc_lascl(rho, 1.0, k, z);
-    rho = rho * rho;
-    int info = 0;
-    for (j = 1; j <= k; j++) {
-        int nit = 0;
-        double sg;
-        int r = orc_lasd4(k, j, dsigma, z, rho, &sg, &org[j - 1], &tau[j - 1], &nit, maxit);
-        D_(j) = sg;
-        if (iters) *iters += nit;
-        if (r) { info = j; break; }
-    }
+    int info = sv_solve(k, d, dsigma, z, zs, org, tau, maxit, iters);
     if (info) { free(org); return info; }
-    /* updated z (Loewner): products in DLASD3's order */
-    for (i = 1; i <= k; i++) {
-        double di = dsigma[i - 1];
-        double zi = sv_den(di, dsigma[org[k - 1] - 1], tau[k - 1]);
-        for (j = 1; j <= i - 1; j++)
-            zi = zi * (sv_den(di, dsigma[org[j - 1] - 1], tau[j - 1]) / (di - dsigma[j - 1]) /
-                       (di + dsigma[j - 1]));
-        for (j = i; j <= k - 1; j++)
-            zi = zi * (sv_den(di, dsigma[org[j - 1] - 1], tau[j - 1]) / (di - dsigma[j]) /
-                       (di + dsigma[j]));
-        double a = sqrt(fabs(zi));
-        Z_(i) = zs[i - 1] >= 0.0 ? a : -a;
-    }
     /* singular vectors of the deflated problem: column i of qu (left, rows in IDXC
      * order), row i of qv (right) */
     for (i = 1; i <= k; i++) {
-        double dorg = dsigma[org[i - 1] - 1], ti = tau[i - 1];
-        for (j = 1; j <= k; j++) {
-            double dj = dsigma[j - 1];
-            wv[j - 1] = Z_(j) / ((dj - dorg) - ti) / ((dj + dorg) + ti);
-        }
-        /* left: (-1, d_j v_j) */
-        for (int l = 0; l < T; l++) {
-            ps[l] = 0.0;
-            for (j = l; j < k; j += T) {
-                double x = j == 0 ? -1.0 : dsigma[j] * wv[j];
-                ps[l] += x * x;
-            }
-        }
-        double tu = sqrt(tsumT(ps, T));
+        double tu, tv;
+        sv_norms(k, dsigma, z, dsigma[org[i - 1] - 1], tau[i - 1], wv, &tu, &tv);
         qu[(long)(i - 1) * k] = -1.0 / tu;
         for (j = 2; j <= k; j++) {
             int jc = idxc[j - 1];
             qu[(j - 1) + (long)(i - 1) * k] = dsigma[jc - 1] * wv[jc - 1] / tu;
         }
-        for (int l = 0; l < T; l++) {
-            ps[l] = 0.0;
-            for (j = l; j < k; j += T) ps[l] += wv[j] * wv[j];
-        }
-        double tv = sqrt(tsumT(ps, T));
         qv[(i - 1) + 0] = wv[0] / tv;
         for (j = 2; j <= k; j++) qv[(i - 1) + (long)(j - 1) * k] = wv[idxc[j - 1] - 1] / tv;
     }
@@ -701,6 +724,8 @@ int orc_lasd3(int nl, int nr, int sqre, int k, double *d, double *q, const doubl
             VT_(i, c) = acc;
         }
     (void)ctot;
+    (void)T;
+    (void)ps;
     free(org);
     return 0;
 }
@@ -712,10 +737,105 @@ int orc_lasd3(int nl, int nr, int sqre, int k, double *d, double *q, const doubl
 #undef VT2_
 
 /* ------------------------------------------------------------------ */
-/* MDLASD1 (mdlasd1.f90:1-291), dense branch for every K. */
+/* MDLASD8 (mdlasd8.f90:201-278): roots d(1:k), the updated z, DIFL(j) = sigma_j - d_j
+ * and DIFR(j) = sigma_j - d_{j+1} (from the root's origin and offset; DIFR(k) = 0) and
+ * the normalisers APHAU / APHAV of the left / right vectors (1 / 2-norm, the dense
+ * branch's sums).  Returns 0 or the index of a root that did not converge. */
+int orc_lasd8(int k, double *d, const double *dsigma, double *z, double *difl, double *difr,
+              double *aphau, double *aphav, int maxit, long long *iters)
+{
+    double *zs = malloc(sizeof(double) * 3 * (size_t)k), *tau = zs + k, *wv = tau + k;
+    int *org = malloc(sizeof(int) * k);
+    int info = sv_solve(k, d, dsigma, z, zs, org, tau, maxit, iters);
+    if (!info) {
+        for (int j = 1; j <= k; j++) {
+            const double dorg = dsigma[org[j - 1] - 1], tj = tau[j - 1];
+            double tu, tv;
+            difl[j - 1] = -((dsigma[j - 1] - dorg) - tj);
+            difr[j - 1] = j < k ? -((dsigma[j] - dorg) - tj) : 0.0;
+            sv_norms(k, dsigma, z, dorg, tj, wv, &tu, &tv);
+            aphau[j - 1] = 1.0 / tu;
+            aphav[j - 1] = 1.0 / tv;
+        }
+    }
+    free(zs);
+    free(org);
+    return info;
+}
+
+/* An HSS tree of K columns in leaves of about ni has at least two leaves and at most
+ * 512 (npart, aux_hss.f90:737-760). */
+int orc_hss_tree_ok(int k, int ni)
+{
+    if (ni < 1) return 0;
+    int l = k / ni;
+    if (l < 1) return 0;
+    if (k % ni >= ni / 2) l++;
+    return l >= 2 && l <= 512;
+}
+
+/* The HSS branch of MDLASD1 (mdlasd1.f90:220-278, K >= BSMLZ): MDLASD8, then
+ * DHSSVCS's two HSS forms in the squared variables (CauchyHssvd_VPS.f90:3587-3804):
+ *   right  H_V(j,i) = z_j APHAV_i / (d_j^2 - sigma_i^2),   V(:,1:k) = V2 H_V
+ *   left   H_U(j,i) = d_j z_j APHAU_i / (d_j^2 - sigma_i^2), U(:,1:k) = U2 H_U - U2(:,1) APHAU^T
+ * (d_1 = 0, so row 1 of H_U is zero and the left vectors' first entry -APHAU_i enters as
+ * the rank-one term: DLASD3's U(1,i) = -1 before normalisation).  U2 / VT2 come from
+ * MDLASD2 with their columns / rows in IDXC order; they are taken back to the order of
+ * DSIGMA here.  *rank gets the largest generator rank of the two forms. */
+static int lasd3_hss(int nl, int nr, int sqre, int k, double *d, const double *dsigma, double *u,
+                     int ldu, const double *u2, int ldu2, double *vt, int ldvt, const double *vt2,
+                     int ldvt2, const int *idxc, double *z, const orc_params *p, long long *iters,
+                     int *rank)
+{
+    const int n = nl + nr + 1, m = n + sqre;
+    double *g = malloc(sizeof(double) * 5 * (size_t)k);
+    double *difl = g, *difr = g + k, *au = g + 2 * k, *av = g + 3 * k, *dz = g + 4 * k;
+    int info = orc_lasd8(k, d, dsigma, z, difl, difr, au, av, p->maxit, iters);
+    if (info) { free(g); return info; }
+    for (int j = 0; j < k; j++) dz[j] = dsigma[j] * z[j];
+    double *xu = malloc(sizeof(double) * (size_t)n * k), *xv = malloc(sizeof(double) * (size_t)m * k);
+    for (int j = 1; j <= k; j++) {
+        const int c = j == 1 ? 1 : idxc[j - 1];
+        for (int r = 0; r < n; r++) xu[r + (long)(c - 1) * n] = u2[r + (long)(j - 1) * ldu2];
+        for (int r = 0; r < m; r++) xv[r + (long)(c - 1) * m] = vt2[(j - 1) + (long)r * ldvt2];
+    }
+    int rk = 0;
+    orc_hss H;
+    info = orc_hss_build_sq(&H, k, p->bd_ni, dsigma, d, z, av, difl, difr, p->tol);
+    if (!info) {
+        for (int i = 0; i < H.ltr - 1; i++) {
+            if (H.usz[2 * i + 1] > rk) rk = H.usz[2 * i + 1];
+            if (H.vsz[2 * i] > rk) rk = H.vsz[2 * i];
+        }
+        orc_hss_mmL(&H, xv, m, m);
+        orc_hss_free(&H);
+        info = orc_hss_build_sq(&H, k, p->bd_ni, dsigma, d, dz, au, difl, difr, p->tol);
+    }
+    if (!info) {
+        for (int i = 0; i < H.ltr - 1; i++) {
+            if (H.usz[2 * i + 1] > rk) rk = H.usz[2 * i + 1];
+            if (H.vsz[2 * i] > rk) rk = H.vsz[2 * i];
+        }
+        double *u21 = malloc(sizeof(double) * n);
+        memcpy(u21, xu, sizeof(double) * n);
+        orc_hss_mmL(&H, xu, n, n);
+        orc_hss_free(&H);
+        for (int i = 0; i < k; i++)
+            for (int r = 0; r < n; r++) u[r + (long)i * ldu] = xu[r + (long)i * n] - u21[r] * au[i];
+        for (int i = 0; i < k; i++)
+            for (int c = 0; c < m; c++) vt[i + (long)c * ldvt] = xv[c + (long)i * m];
+        free(u21);
+    }
+    free(xu); free(xv); free(g);
+    if (rank) *rank = rk;
+    return info ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* MDLASD1 (mdlasd1.f90:1-291), the dense branch (DLASD3) for K < bd_bsmlz, the HSS branch above. */
 int orc_lasd1(int nl, int nr, int sqre, double *d, double *alpha, double *beta, double *u, int ldu,
               double *vt, int ldvt, int *idxq, const orc_params *p, int *kout, long long *iters,
-              int *nrot)
+              int *nrot, int *hss_rank)
 {
     const int n = nl + nr + 1, m = n + sqre;
     double orgnrm = fmax(fabs(*alpha), fabs(*beta));
@@ -732,10 +852,16 @@ int orc_lasd1(int nl, int nr, int sqre, double *d, double *alpha, double *beta, 
     int k;
     orc_lasd2(nl, nr, sqre, &k, d, z, *alpha, *beta, u, ldu, vt, ldvt, dsigma, u2, n, vt2, m, idxp,
               idx, idxc, idxq, coltyp, nrot, NULL, NULL, NULL, NULL);
-    double *q = malloc(sizeof(double) * (2 * (long)k * k + 3 * (long)k));
-    int info = orc_lasd3(nl, nr, sqre, k, d, q, dsigma, u, ldu, u2, n, vt, ldvt, vt2, m, idxc, coltyp,
+    int info, rk = 0;
+    if (k >= p->bd_bsmlz && orc_hss_tree_ok(k, p->bd_ni)) {
+        info = lasd3_hss(nl, nr, sqre, k, d, dsigma, u, ldu, u2, n, vt, ldvt, vt2, m, idxc, z, p, iters, &rk);
+    } else {
+        double *q = malloc(sizeof(double) * (2 * (long)k * k + 3 * (long)k));
+        info = orc_lasd3(nl, nr, sqre, k, d, q, dsigma, u, ldu, u2, n, vt, ldvt, vt2, m, idxc, coltyp,
                          z, p->maxit, iters);
-    free(q);
+        free(q);
+    }
+    if (hss_rank) *hss_rank = rk;
     free(z); free(dsigma); free(u2); free(vt2); free(iw);
     if (info) return info;
     orc_lascl(1.0, orgnrm, n, d);
@@ -776,18 +902,19 @@ int orc_lasd0(int n, int sqre, double *d, double *e, double *u, int ldu, double 
             int nlf = ic - nl;
             int sqrei = (sqre == 0 && i == ll) ? sqre : 1;
             double alpha = d[ic - 1], beta = e[ic - 1];
-            int k = 0, nrot = 0;
+            int k = 0, nrot = 0, rk = 0;
             long long it = 0;
             info = orc_lasd1(nl, nr, sqrei, d + nlf - 1, &alpha, &beta,
                              u + (nlf - 1) + (long)(nlf - 1) * ldu, ldu,
                              vt + (nlf - 1) + (long)(nlf - 1) * ldvt, ldvt, idxq + nlf - 1, p, &k, &it,
-                             &nrot);
+                             &nrot, &rk);
             if (tr) {
                 if (tr->nmerge < tr->cap) {
                     tr->level[tr->nmerge] = lvl;
                     tr->n[tr->nmerge] = nl + nr + 1;
                     tr->k[tr->nmerge] = k;
                     if (tr->hss) tr->hss[tr->nmerge] = nrot;
+                    if (tr->rank_max) tr->rank_max[tr->nmerge] = rk;
                 }
                 tr->nmerge++;
                 tr->roots += k;

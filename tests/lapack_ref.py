@@ -157,6 +157,26 @@ def dlasd4(d, z, rho, j):
     return sigma.value, delta, work, info.value
 
 
+def dlasd8(dsigma, z):
+    """DLASD8 (ICOMPQ = 1): roots, updated z, DIFL, DIFR(:,1) and DIFR(:,2) = the 2-norms
+    of the right vectors before normalisation (MDLASD8's APHAV holds their reciprocals),
+    for a deflated merge with poles dsigma (dsigma[0] = 0)."""
+    k = len(dsigma)
+    d = np.zeros(k)
+    zz = np.array(z, np.float64)
+    vf = np.zeros(k)
+    vl = np.zeros(k)
+    difl = np.zeros(k)
+    difr = np.zeros((k, 2), order="F")
+    ds = np.array(dsigma, np.float64)
+    work = np.zeros(3 * k)
+    info = _i(0)
+    lib().scipy_dlasd8_(_ip(1), _ip(k), _dp(d), _dp(zz), _dp(vf), _dp(vl), _dp(difl), _dp(difr), _ip(k), _dp(ds),
+                        _dp(work), ctypes.byref(info))
+    return dict(sigma=d, z=zz, difl=difl, difr=difr[:, 0].copy(), vnorm=difr[:, 1].copy(),
+                info=info.value)
+
+
 def dlasdq(d, e, sqre):
     """SVD of the n x (n+sqre) upper bidiagonal: (s ascending, U, VT, info)."""
     n = len(d)

@@ -15,7 +15,8 @@ _P = ctypes.POINTER
 
 
 class Params(ctypes.Structure):
-    _fields_ = [("smlsiz", _i), ("bsmlz", _i), ("ni", _i), ("tol", _d), ("maxit", _i)]
+    _fields_ = [("smlsiz", _i), ("bsmlz", _i), ("ni", _i), ("tol", _d), ("maxit", _i),
+                ("bd_bsmlz", _i), ("bd_ni", _i)]
 
 
 class Trace(ctypes.Structure):
@@ -51,10 +52,11 @@ def ip(a):
     return a.ctypes.data_as(_P(_i))
 
 
-def params(smlsiz=25, bsmlz=1500, ni=200, tol=None, maxit=100):
+def params(smlsiz=25, bsmlz=1500, ni=200, tol=None, maxit=100, bd_bsmlz=2000, bd_ni=40):
     p = Params()
     lib().orc_default_params(ctypes.byref(p))
     p.smlsiz, p.bsmlz, p.ni, p.maxit = smlsiz, bsmlz, ni, maxit
+    p.bd_bsmlz, p.bd_ni = bd_bsmlz, bd_ni
     if tol is not None:
         p.tol = tol
     return p
@@ -248,6 +250,38 @@ def hss_build(f, d0, u, v, difl, difr, ni, tol):
     return H
 
 
+def hss_build_sq(f, d0, u, v, difl, difr, ni, tol):
+    """HSS form of u_i v_j / (f_i^2 - d0_j^2) (the SVD's vector matrices)."""
+    k = len(f)
+    H = Hss()
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (f, d0, u, v, difl, difr)]
+    info = lib().orc_hss_build_sq(ctypes.byref(H), k, ni, *[dp(a) for a in arrs], _d(tol))
+    if info != 0:
+        raise RuntimeError("orc_hss_build_sq failed %d" % info)
+    return H
+
+
+def cauchy_dense_sq(f, d0, u, v, difl, difr):
+    k = len(f)
+    a = np.zeros((k, k), order="F")
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (f, d0, u, v, difl, difr)]
+    lib().orc_cauchy_dense_sq(k, *[dp(x) for x in arrs], dp(a), k)
+    return a
+
+
+def lasd8(dsigma, z, maxit=100):
+    """MDLASD8: roots, updated z, DIFL, DIFR, APHAU, APHAV of one deflated SVD merge."""
+    k = len(dsigma)
+    ds = np.ascontiguousarray(dsigma, np.float64)
+    zz = np.array(z, np.float64)
+    out = [np.zeros(k) for _ in range(5)]
+    it = ctypes.c_longlong(0)
+    info = lib().orc_lasd8(k, dp(out[0]), dp(ds), dp(zz), dp(out[1]), dp(out[2]), dp(out[3]), dp(out[4]),
+                           maxit, ctypes.byref(it))
+    return dict(sigma=out[0], z=zz, difl=out[1], difr=out[2], aphau=out[3], aphav=out[4], info=info,
+                iters=it.value)
+
+
 def hss_dense(H):
     k = H.k
     a = np.zeros((k, k), order="F")
@@ -291,6 +325,9 @@ def _setup():
     L.orc_laed3.restype = _i
     L.orc_steqr.restype = _i
     L.orc_hss_build.restype = _i
+    L.orc_hss_build_sq.restype = _i
+    L.orc_lasd8.restype = _i
+    L.orc_hss_tree_ok.restype = _i
 
 
 def mdlaed3_vectors(dlamda, z, rho, maxit=100):
@@ -374,13 +411,14 @@ def lasd1(nl, nr, sqre, d, alpha, beta, u, vt, idxq, prm=None):
     vt = np.array(vt, np.float64, order="F")
     idxq = np.array(idxq, np.int32)
     a, b = _d(alpha), _d(beta)
-    k, nrot = _i(0), _i(0)
+    k, nrot, rk = _i(0), _i(0), _i(0)
     it = ctypes.c_longlong(0)
     prm = prm or params()
     info = lib().orc_lasd1(nl, nr, sqre, dp(d), ctypes.byref(a), ctypes.byref(b), dp(u), n, dp(vt),
                            m, ip(idxq), ctypes.byref(prm), ctypes.byref(k), ctypes.byref(it),
-                           ctypes.byref(nrot))
-    return dict(d=d, u=u, vt=vt, idxq=idxq, k=k.value, iters=it.value, nrot=nrot.value, info=info)
+                           ctypes.byref(nrot), ctypes.byref(rk))
+    return dict(d=d, u=u, vt=vt, idxq=idxq, k=k.value, iters=it.value, nrot=nrot.value,
+                hss_rank=rk.value, info=info)
 
 
 def bdsdc(d, e, uplo=b"U", compq=b"I", prm=None, trace_cap=0):
@@ -407,5 +445,6 @@ def bdsdc(d, e, uplo=b"U", compq=b"I", prm=None, trace_cap=0):
     if tr is not None:
         m = min(tr.nmerge, trace_cap)
         out["trace"] = dict(level=bufs[0][:m].copy(), n=bufs[1][:m].copy(), k=bufs[2][:m].copy(),
-                            nrot=bufs[3][:m].copy(), roots=tr.roots, iters=tr.iters)
+                            nrot=bufs[3][:m].copy(), rank_max=bufs[4][:m].copy(), roots=tr.roots,
+                            iters=tr.iters)
     return out

@@ -312,3 +312,86 @@ def test_bdsdc_trace_k_matches_lapack_merges():
     assert tr["level"].size > 0
     assert np.all(tr["k"] >= 1) and np.all(tr["k"] <= tr["n"])
     assert tr["roots"] == int(tr["k"].sum())
+
+
+# ---------------------------------------------------------------- MDLASD8 and the HSS branch
+def lasd8_case(k, seed, toeplitz=False):
+    rng = np.random.default_rng(seed)
+    if toeplitz:
+        d = np.concatenate([[0.0], 1.0 + np.cumsum(rng.uniform(0.5, 1.5, k - 1)) / k])
+    else:
+        d = np.concatenate([[0.0], np.sort(rng.uniform(0.05, 1.0, k - 1))])
+    z = rng.uniform(0.1, 1.0, k) * rng.choice([-1, 1], k)
+    return d, z
+
+
+@pytest.mark.parametrize("k", [3, 17, 64, 300])
+def test_lasd8_matches_lapack(k):
+    d, z = lasd8_case(k, k)
+    o = O.lasd8(d, z)
+    r = L.dlasd8(d, z)
+    assert o["info"] == 0 and r["info"] == 0
+    np.testing.assert_allclose(o["sigma"], r["sigma"], rtol=4 * k * EPS, atol=0)
+    # the differences sigma_j - d_j and sigma_j - d_{j+1}, each to a few ulps of itself
+    np.testing.assert_allclose(o["difl"], r["difl"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(o["difr"][:-1], r["difr"][:-1], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(np.abs(o["z"]), np.abs(r["z"]), rtol=1e-12, atol=0)
+    np.testing.assert_allclose(o["aphav"], 1.0 / r["vnorm"], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("k,toeplitz", [(40, False), (257, False), (400, True)])
+def test_svd_vector_matrices_are_the_merge_singular_vectors(k, toeplitz):
+    """From MDLASD8's outputs: V(i,j) = zhat_i APHAV_j / (d_i^2 - sigma_j^2) and U with
+    rows d_i zhat_i APHAU_j / (d_i^2 - sigma_j^2), row 1 = -APHAU, are the singular
+    vectors of the arrowhead matrix e_1 zhat^T + diag(d): orthonormal, M V = U Sigma."""
+    d, z = lasd8_case(k, 3 * k, toeplitz)
+    o = O.lasd8(d, z)
+    s, zh = o["sigma"], o["z"]
+    V = O.cauchy_dense_sq(d, s, zh, o["aphav"], o["difl"], o["difr"])
+    U = O.cauchy_dense_sq(d, s, d * zh, o["aphau"], o["difl"], o["difr"])
+    U[0, :] = -o["aphau"]
+    M = np.diag(d)
+    M[0, :] = zh
+    assert np.abs(V.T @ V - np.eye(k)).max() <= 1e-13
+    assert np.abs(U.T @ U - np.eye(k)).max() <= 1e-13
+    assert np.abs(M @ V - U * s[None, :]).max() <= 1e-13 * np.abs(M).max()
+
+
+@pytest.mark.parametrize("k,ni,toeplitz", [(300, 40, False), (600, 40, True), (1000, 64, False)])
+def test_hss_squared_matches_dense(k, ni, toeplitz):
+    d, z = lasd8_case(k, k + ni, toeplitz)
+    o = O.lasd8(d, z)
+    tol = float(np.float32(1e-17))
+    for u, v in ((o["z"], o["aphav"]), (d * o["z"], o["aphau"])):
+        H = O.hss_build_sq(d, o["sigma"], u, v, o["difl"], o["difr"], ni, tol)
+        try:
+            Hd = O.hss_dense(H)
+            usz, vsz = O.hss_ranks(H)
+            x = np.random.default_rng(1).standard_normal((37, k))
+            y = O.hss_mmL(H, x)
+        finally:
+            O.hss_free(H)
+        C = O.cauchy_dense_sq(d, o["sigma"], u, v, o["difl"], o["difr"])
+        scale = np.abs(C).max()
+        assert np.abs(Hd - C).max() <= 64 * EPS * k ** 0.5 * scale
+        assert np.abs(y - x @ C).max() <= 1e-13 * np.abs(x).max() * k ** 0.5 * scale
+        assert usz[:-1, 1].max() < k // 2   # compressed, not dense
+
+
+@pytest.mark.parametrize("gen,n,bsmlz", [("toeplitz", 1000, 100), ("uniform", 2000, 64),
+                                         ("toeplitz", 2200, 2000)])
+def test_bdsdc_hss_branch_matches_lapack(gen, n, bsmlz):
+    rng = np.random.default_rng(n)
+    if gen == "toeplitz":
+        d, e = 1.0 + 1e-3 * rng.standard_normal(n), 1.0 + 1e-3 * rng.standard_normal(n - 1)
+    else:
+        d, e = 0.5 + rng.random(n), rng.random(n - 1)
+    o = O.bdsdc(d, e, prm=O.params(bd_bsmlz=bsmlz), trace_cap=4 * n)
+    assert o["info"] == 0
+    tr = o["trace"]
+    hss = tr["rank_max"] > 0
+    assert hss.any() and np.all(tr["k"][hss] >= bsmlz)
+    assert np.all(tr["k"][~hss] < bsmlz)
+    s_ref = L.dbdsdc(d, e)[0]
+    np.testing.assert_allclose(o["d"], s_ref, rtol=0, atol=4 * n * EPS * s_ref[0])
+    check_svd(bidiag(d, e), o["d"], o["u"], o["vt"], tol=1e-12)

@@ -389,7 +389,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
                    al64(sizeof(long) * (4 * ltr)) + al64(sizeof(unsigned long long) * 2);
         metasz[h] = o;
         o += al64(sizeof(double) * std::max(dds, 1L));
-        o += al64(sizeof(double) * (size_t)12 * ni * K);           // arena
+        o += al64(sizeof(double) * (size_t)12 * std::max(ni, 128) * K);  // arena
         o += al64(sizeof(double) * (size_t)8 * K * std::max(P[h].T.lm, 1));  // scratch
         base[h] = tot;
         tot += o;
@@ -424,8 +424,8 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         long dds = 0;
         for (int x : Q.T.m) dds += (long)x * x;
         p += al64(sizeof(double) * std::max(dds, 1L));
-        H.arena = (double *)p; H.arena_cap = (long)12 * ni * K;
-        p += al64(sizeof(double) * (size_t)12 * ni * K);
+        H.arena = (double *)p; H.arena_cap = (long)12 * std::max(ni, 128) * K;
+        p += al64(sizeof(double) * (size_t)12 * std::max(ni, 128) * K);
         H.scratch = (double *)p; H.scratch_cap = (long)8 * K * std::max(Q.T.lm, 1);
         H.K = K; H.ltr = Q.T.ltr; H.lvl = Q.T.lvl; H.sq = sp[h].sq;
     }
@@ -1473,6 +1473,33 @@ static void make_plan_split(Plan &pl, int n, double *d, double *e, int smlsiz)
 // ============================================================================
 // C ABI
 // ============================================================================
+// HSS batch of another solver (the bidiagonal SVD's merges) on its stream: build
+// every matrix of `sp`, Y := X * H.  rank (may be null) gets per matrix its largest
+// generator rank (0 when applied densely: tree too small or a node over the LDS /
+// arena limits); *dense (may be null) the number of such matrices.
+int hsdc_hss_batch(hsdc_ctx *c, hipStream_t s, const std::vector<HssSpec> &sp, int ni, double tol,
+                   std::vector<int> *rank, int *dense)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    hipStream_t s0 = c->stream;
+    c->stream = s;
+    std::vector<HssPlanM> keep;
+    const int rc = hss_apply(c, sp, ni, tol, &keep);
+    c->stream = s0;
+    if (rc) return rc;
+    int nd = 0;
+    if (rank) rank->assign(keep.size(), 0);
+    for (size_t h = 0; h < keep.size(); h++) {
+        const HssPlanM &Q = keep[h];
+        if (!Q.ok) { nd++; continue; }
+        int rm = 0;
+        for (int i = 0; i < Q.T.ltr - 1; i++) rm = std::max(rm, std::max(Q.usz[2 * i + 1], Q.vsz[2 * i]));
+        if (rank) (*rank)[h] = rm;
+    }
+    if (dense) *dense = nd;
+    return 0;
+}
+
 extern "C" {
 
 void hsdc_default_params(hsdc_params *p)
@@ -2033,29 +2060,6 @@ int hsdc_hss_svc(hsdc_ctx *c, int k, int ni, double tol, const double *f, const 
     return hss_one(c, 1, k, ni, tol, f, d, u, v, difl, difr, q, ldq, nrow, ranks);
 }
 
-// HSS batch of another solver (the bidiagonal SVD's merges) on its stream: build
-// every matrix of `sp`, Y := X * H.  *rank_max (may be null) gets the largest
-// generator rank; *dense the number of matrices applied densely (tree too small or
-// a node over the LDS / arena limits).
-int hsdc_hss_batch(hsdc_ctx *c, hipStream_t s, const std::vector<HssSpec> &sp, int ni, double tol, int *rank_max,
-                   int *dense)
-{
-    if (!c) return HSDC_ERR_INTERNAL;
-    hipStream_t s0 = c->stream;
-    c->stream = s;
-    std::vector<HssPlanM> keep;
-    const int rc = hss_apply(c, sp, ni, tol, &keep);
-    c->stream = s0;
-    if (rc) return rc;
-    int rm = 0, nd = 0;
-    for (const HssPlanM &Q : keep) {
-        if (!Q.ok) { nd++; continue; }
-        for (int i = 0; i < Q.T.ltr - 1; i++) rm = std::max(rm, std::max(Q.usz[2 * i + 1], Q.vsz[2 * i]));
-    }
-    if (rank_max) *rank_max = rm;
-    if (dense) *dense = nd;
-    return 0;
-}
 
 int hsdc_hss_dense(hsdc_ctx *c, double *a, int lda)
 {

@@ -33,6 +33,11 @@
 
 #include "../../include/hsdc.h"
 #include "hsdc_internal.h"
+#include "hsdc_hss.h"
+
+// HSS batch on a stream (defined with the tridiagonal solver's HSS driver)
+int hsdc_hss_batch(hsdc_ctx *c, hipStream_t s, const std::vector<HssSpec> &sp, int ni, double tol,
+                   std::vector<int> *rank, int *dense);
 
 #define BD_EPS 0x1p-53
 #define BD_SAFMIN 0x1p-1022
@@ -49,6 +54,14 @@ struct BdMerge {
 struct BdOut {
     double orgnrm, alpha, beta, tol, z1, zr1, c, sn, rho;
     int k, nrot, ctot[4];
+    int hss;  // 1: the merge takes the HSS branch (set by the host after deflation)
+};
+
+// one HSS merge of a level: merge index, offsets of its generators (5 K: d z, APHAU,
+// APHAV, DIFL, DIFR) and of its gathered U2 (n x K) and V2 (m x K) in DSIGMA order
+struct BdHss {
+    int mi;
+    long goff, xoff;
 };
 
 struct BdLeaf {
@@ -471,6 +484,7 @@ __global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W
         tol = 8.0 * BD_EPS * fmax(fabs(W.D[s + n - 1]), tol);
         BdOut &o = mo[blockIdx.x];
         o.orgnrm = orgnrm;
+        o.hss = 0;
         o.alpha = alpha;
         o.beta = beta;
         o.tol = tol;
@@ -897,6 +911,7 @@ __global__ __launch_bounds__(256) void k_bd_vec(const BdMerge *mg, int nmg, BdTa
   const int tot = tab.sec[nmg];
   for (int b = blockIdx.x; b < tot; b += gridDim.x) {
     const int mi = bd_find(tab.sec, nmg, b);
+    if (mo[mi].hss) continue;  // its vectors come from the HSS forms
     const int s = mg[mi].s, k = mo[mi].k, T = sec_team(k);
     const int l = threadIdx.x & (T - 1);
     const int i = (b - tab.sec[mi]) * (256 / T) + threadIdx.x / T + 1;
@@ -1057,6 +1072,74 @@ __global__ __launch_bounds__(256) void k_bd_scatter(const BdMerge *mg, int nmg, 
   }
 }
 
+// ------------------------------------------------------------------------------
+// HSS branch (mdlasd1.f90:220-278, MDLASD8 mdlasd8.f90:201-278): per root i of an HSS
+// merge the normalisers of its left and right vectors (the dense branch's sums),
+// DIFL(i) = sigma_i - d_i, DIFR(i) = sigma_i - d_{i+1}, and d_i z_i
+// ------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bd_hssgen(const BdMerge *mg, const BdHss *hs, BdWork W, const BdOut *mo,
+                                                   double *G)
+{
+    const BdHss h = hs[blockIdx.y];
+    const int s = mg[h.mi].s, k = mo[h.mi].k, T = sec_team(k);
+    const int l = threadIdx.x & (T - 1);
+    const int i = blockIdx.x * (256 / T) + threadIdx.x / T + 1;
+    if (i > k) return;
+    const double *ds = W.dsg + s, *zl = W.zl + s;
+    const double dorg = ds[W.org[s + i - 1] - 1], ti = W.tau[s + i - 1];
+    double pu = 0.0, pv = 0.0;
+    for (int j = l; j < k; j += T) {
+        double dj = ds[j];
+        double wv = zl[j] / ((dj - dorg) - ti) / ((dj + dorg) + ti);
+        double x = j == 0 ? -1.0 : dj * wv;
+        pu += x * x;
+        pv += wv * wv;
+    }
+    const double tu = sqrt(tsum_team(pu, T)), tv = sqrt(tsum_team(pv, T));
+    if (l == 0) {
+        double *g = G + h.goff;
+        g[i - 1] = ds[i - 1] * zl[i - 1];
+        g[k + i - 1] = 1.0 / tu;
+        g[2 * k + i - 1] = 1.0 / tv;
+        g[3 * k + i - 1] = -((ds[i - 1] - dorg) - ti);
+        g[4 * k + i - 1] = i < k ? -((ds[i] - dorg) - ti) : 0.0;
+    }
+}
+
+// U2 and V2 = VT2^T of an HSS merge gathered with their columns in DSIGMA order
+// (column idxc(J) takes U2(:,J)), zero outside the supports
+__global__ __launch_bounds__(256) void k_bd_xgather(const BdMerge *mg, const BdHss *hs, BdWork W, const BdOut *mo,
+                                                    const double *U, const double *V, long ld, double *X)
+{
+    const BdHss h = hs[blockIdx.y];
+    const BdMerge M = mg[h.mi];
+    const BdOut &o = mo[h.mi];
+    const int s = M.s, n = M.nl + M.nr + 1, m = n + M.sqre, k = o.k;
+    double *XU = X + h.xoff, *XV = XU + (long)n * k;
+    const long tot = (long)(n + m) * k;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+        const bool isV = e >= (long)n * k;
+        const long f = isV ? e - (long)n * k : e;
+        const int rows = isV ? m : n;
+        const int r = (int)(f % rows), J = (int)(f / rows);
+        const int c = J == 0 ? 0 : W.idxc[s + J] - 1;
+        (isV ? XV : XU)[(long)c * rows + r] = bd_acol(W, M, o, isV, isV ? V : U, ld, J, r);
+    }
+}
+
+// U(:,1:K) = U2 H_U - U2(:,1) APHAU^T: U2(:,1) is the unit vector at the centre row
+__global__ __launch_bounds__(256) void k_bd_hssfix(const BdMerge *mg, const BdHss *hs, BdTab tab, const BdOut *mo,
+                                                   const double *G, double *stg)
+{
+    const BdHss h = hs[blockIdx.y];
+    const BdMerge M = mg[h.mi];
+    const int n = M.nl + M.nr + 1, k = mo[h.mi].k;
+    double *SU = stg + tab.soff[2 * h.mi];
+    const double *au = G + h.goff + k;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < k; i += gridDim.x * 256)
+        SU[(long)i * n + M.nl] -= au[i];
+}
+
 // exclusive block scan of one 64-bit value per thread (1024 threads); returns the total
 __device__ long block_exscan_l(long v, long *sh, long &total)
 {
@@ -1101,8 +1184,10 @@ __global__ __launch_bounds__(1024) void k_bd_lvltab(const BdMerge *mg, int nmg, 
                 const int T = sec_team(k);
                 v[1] = (k + 256 / T - 1) / (256 / T);
                 v[2] = (k + 255) / 256;
-                v[3] = (long)((k + 63) / 64) * ((n + 63) / 64 + (m + 63) / 64);
-                v[5] = 2 * (long)k * k;
+                if (!mo[i].hss) {  // the HSS branch forms no K x K vectors and no tiles
+                    v[3] = (long)((k + 63) / 64) * ((n + 63) / 64 + (m + 63) / 64);
+                    v[5] = 2 * (long)k * k;
+                }
             }
             v[4] = 2 * k + M.sqre;
             v[6] = (long)n * k + (long)m * (k + M.sqre);
@@ -1366,7 +1451,10 @@ struct hsdc_bd_ctx {
     long wcap = 0;
     BdOut *hmo = nullptr;  // pinned
     size_t hmo_cap = 0;
-    std::vector<int> tr;   // per merge of the last solve: block start, order, K, rotations
+    std::vector<int> tr;   // per merge of the last solve: block start, order, K, rotations, HSS rank
+    std::vector<int> hrank;  // per merge: largest HSS generator rank (0: dense branch)
+    hsdc_ctx *hctx = nullptr;  // HSS construction and multiply (created on the first HSS merge)
+    Buf hgen, hx, htb;         // HSS generators, gathered U2 / V2, the level's BdHss table
 };
 
 #define BCK(x) do { int _r = (x); if (_r) return _r; } while (0)
@@ -1378,6 +1466,9 @@ void hsdc_bd_default_params(hsdc_bd_params *p)
 {
     p->smlsiz = 25;
     p->maxit = 100;
+    p->hss_min_k = 2000;              // BSMLZ, mdlasd2.f90:172
+    p->hss_leaf = 40;                 // Ni, mdlasd8.f90:277
+    p->hss_tol = (double)1.0e-17f;    // TOL, CauchyHssvd_VPS.f90:3694 (a REAL literal)
 }
 
 int hsdc_bd_ctx_create(hsdc_bd_ctx **ctx)
@@ -1394,7 +1485,8 @@ int hsdc_bd_ctx_destroy(hsdc_bd_ctx *c)
 {
     if (!c) return 0;
     Buf *all[] = {&c->ub[0], &c->vb[0], &c->wd, &c->wi, &c->mgb, &c->mob, &c->tab, &c->qb, &c->stg,
-                  &c->misc, &c->din, &c->din2, &c->ein, &c->uo, &c->vto, &c->rot};
+                  &c->misc, &c->din, &c->din2, &c->ein, &c->uo, &c->vto, &c->rot, &c->hgen, &c->hx, &c->htb};
+    if (c->hctx) hsdc_ctx_destroy(c->hctx);
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 4; i++)
@@ -1411,6 +1503,9 @@ int hsdc_bd_set_params(hsdc_bd_ctx *c, const hsdc_bd_params *p)
     if (!c || !p) return HSDC_ERR_INTERNAL;
     if (p->smlsiz < 3 || p->smlsiz > 63) return -2;
     if (p->maxit < 1) return -3;
+    if (p->hss_min_k < 1) return -4;
+    if (p->hss_leaf < 8) return -5;
+    if (!(p->hss_tol >= 0.0)) return -6;
     c->prm = *p;
     return 0;
 }
@@ -1429,15 +1524,16 @@ int hsdc_bd_set_stream(hsdc_bd_ctx *c, void *stream)
     return 0;
 }
 
-int hsdc_bd_trace(hsdc_bd_ctx *c, int *s, int *n, int *k, int *nrot, int cap)
+int hsdc_bd_trace(hsdc_bd_ctx *c, int *s, int *n, int *k, int *nrot, int *hss_rank, int cap)
 {
     if (!c) return HSDC_ERR_INTERNAL;
-    const int cnt = (int)c->tr.size() / 4;
+    const int cnt = (int)c->tr.size() / 5;
     for (int i = 0; i < cnt && i < cap; i++) {
-        if (s) s[i] = c->tr[4 * i];
-        if (n) n[i] = c->tr[4 * i + 1];
-        if (k) k[i] = c->tr[4 * i + 2];
-        if (nrot) nrot[i] = c->tr[4 * i + 3];
+        if (s) s[i] = c->tr[5 * i];
+        if (n) n[i] = c->tr[5 * i + 1];
+        if (k) k[i] = c->tr[5 * i + 2];
+        if (nrot) nrot[i] = c->tr[5 * i + 3];
+        if (hss_rank) hss_rank[i] = c->tr[5 * i + 4];
     }
     return cnt;
 }
@@ -1470,6 +1566,86 @@ static int bd_alloc(hsdc_bd_ctx *c, int n)
 
 // the whole SVD of the upper bidiagonal already on the device (W.D, W.E, scaled when
 // n > smlsiz); outputs U, VT (device, may be null for values only)
+// An HSS tree over K columns in leaves of about ni has 2 to 512 leaves (the
+// construction's limits; npart, aux_hss.f90:737-760).
+static bool bd_hss_tree_ok(int k, int ni)
+{
+    int l = k / ni;
+    if (l < 1) return false;
+    if (k % ni >= ni / 2) l++;
+    return l >= 2 && l <= 512;
+}
+
+// The HSS branch of the merges `hs` of one level (mdlasd1.f90:220-278, DHSSVCS
+// CauchyHssvd_VPS.f90:3587-3804), after their roots and updated z: generators, U2 and
+// V2 gathered in DSIGMA order, the two HSS forms per merge built and applied as
+// X := X * H into the merge's staging (the dense branch's layout), then the first
+// row's rank-one term of the left vectors.  rank[mi] receives each merge's largest
+// generator rank.
+static int bd_hss_level(hsdc_bd_ctx *c, const BdMerge *mg, const BdOut *mo, const std::vector<BdMerge> &L,
+                        const BdOut *ho, const std::vector<BdHss> &hs, BdTab tab, const double *U, const double *V,
+                        long ld, double *S, int *rank)
+{
+    hipStream_t st = c->stream;
+    const int nh = (int)hs.size();
+    long gtot = 0, xtot = 0, gx = 1, xmax = 1;
+    int kmax = 1;
+    for (const BdHss &h : hs) {
+        const BdMerge &M = L[h.mi];
+        const int k = ho[h.mi].k, n_ = M.nl + M.nr + 1, m_ = n_ + M.sqre;
+        gtot += 5L * k;
+        xtot += (long)(n_ + m_) * k;
+        kmax = std::max(kmax, k);
+        const int T = sec_team(k);
+        gx = std::max(gx, (long)((k + 256 / T - 1) / (256 / T)));
+        xmax = std::max(xmax, (long)(n_ + m_) * k);
+    }
+    BCK(bensure(c->hgen, sizeof(double) * (size_t)gtot));
+    BCK(bensure(c->hx, sizeof(double) * (size_t)xtot));
+    BCK(bensure(c->htb, sizeof(BdHss) * (size_t)nh));
+    if (!c->hctx) BCK(hsdc_ctx_create(&c->hctx));
+    BdHss *dh = (BdHss *)c->htb.p;
+    double *G = (double *)c->hgen.p, *X = (double *)c->hx.p;
+    BHK(hipMemcpyAsync(dh, hs.data(), sizeof(BdHss) * nh, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_bd_hssgen, dim3((unsigned)gx, nh), dim3(256), 0, st, mg, (const BdHss *)dh, c->W, mo, G);
+    hipLaunchKernelGGL(k_bd_xgather, dim3((unsigned)std::min(bcdiv(xmax, 256), 4096), nh), dim3(256), 0, st, mg,
+                       (const BdHss *)dh, c->W, mo, U, V, ld, X);
+    BHK(hipGetLastError());
+    // staging offsets of the level (k_bd_lvltab's prefix, from the K read back)
+    std::vector<long> soff(L.size() + 1, 0);
+    for (size_t i = 0; i < L.size(); i++) {
+        const int n_ = L[i].nl + L[i].nr + 1, m_ = n_ + L[i].sqre, k = ho[i].k;
+        soff[i + 1] = soff[i] + (long)n_ * k + (long)m_ * (k + L[i].sqre);
+    }
+    std::vector<HssSpec> sp;
+    for (const BdHss &h : hs) {
+        const BdMerge &M = L[h.mi];
+        const int k = ho[h.mi].k, n_ = M.nl + M.nr + 1, m_ = n_ + M.sqre, s = M.s;
+        const double *g = G + h.goff;
+        HssSpec a;
+        a.K = k;
+        a.FF = c->W.dsg + s; a.D0 = c->W.sig + s; a.DIFL = g + 3L * k; a.DIFR = g + 4L * k;
+        a.sq = 1;
+        HssSpec b = a;
+        a.U0 = g; a.V0 = g + k;                              // left: d z, APHAU
+        a.X = X + h.xoff; a.ldx = n_; a.nrow = n_;
+        a.Y = S + soff[h.mi]; a.ldy = n_;
+        b.U0 = c->W.zl + s; b.V0 = g + 2L * k;               // right: z, APHAV
+        b.X = X + h.xoff + (long)n_ * k; b.ldx = m_; b.nrow = m_;
+        b.Y = S + soff[h.mi] + (long)n_ * k; b.ldy = m_;
+        sp.push_back(a);
+        sp.push_back(b);
+    }
+    std::vector<int> rk;
+    int ndense = 0;
+    BCK(hsdc_hss_batch(c->hctx, st, sp, c->prm.hss_leaf, c->prm.hss_tol, &rk, &ndense));
+    hipLaunchKernelGGL(k_bd_hssfix, dim3((unsigned)bcdiv(kmax, 256), nh), dim3(256), 0, st, mg, (const BdHss *)dh,
+                       tab, mo, (const double *)G, S);
+    BHK(hipGetLastError());
+    for (int i = 0; i < nh; i++) rank[hs[i].mi] = std::max(rk[2 * i], rk[2 * i + 1]);
+    return 0;
+}
+
 static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double orgnrm, double *dU, long ldu,
                     double *dVT, long ldvt, double *dD, const double *rc, const double *rs)
 {
@@ -1587,6 +1763,7 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
     }
     int *dfail = (int *)(W.iters + 1);
     BHK(hipMemsetAsync(W.iters, 0, 16, st));
+    c->hrank.assign(nmg + 1, 0);
     // leaves
     int pmax = 1;
     for (auto &L : leaves) pmax = std::max(pmax, L.m + L.sqre);
@@ -1643,6 +1820,28 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
         tab.soff = tab.qoff + nm;
         hipLaunchKernelGGL(k_bd_prep, dim3(nm), dim3(PREP_NT), 0, st, mg, W, mo, (const double *)V, ld);
         hipLaunchKernelGGL(k_bd_deflate, dim3(nm), dim3(DEF_NT), 0, st, mg, W, mo);
+        // a level with a merge of order >= hss_min_k reads its K back: the HSS branch is
+        // planned on the host (tree shape from K); the others stay queued
+        std::vector<BdHss> hsel;
+        bool maybe_hss = false;
+        for (const BdMerge &M : lv[t]) maybe_hss |= M.nl + M.nr + 1 >= c->prm.hss_min_k;
+        if (maybe_hss) {
+            BdOut *ho = c->hmo + moff[t];
+            BHK(hipMemcpyAsync(ho, mo, sizeof(BdOut) * nm, hipMemcpyDeviceToHost, st));
+            BHK(hipStreamSynchronize(st));
+            long goff = 0, xoff = 0;
+            for (int i = 0; i < nm; i++) {
+                const BdMerge &M = lv[t][i];
+                const int k = ho[i].k, n_ = M.nl + M.nr + 1, m_ = n_ + M.sqre;
+                if (k >= c->prm.hss_min_k && bd_hss_tree_ok(k, c->prm.hss_leaf)) {
+                    ho[i].hss = 1;
+                    hsel.push_back({i, goff, xoff});
+                    goff += 5L * k;
+                    xoff += (long)(n_ + m_) * k;
+                }
+            }
+            if (!hsel.empty()) BHK(hipMemcpyAsync(mo, ho, sizeof(BdOut) * nm, hipMemcpyHostToDevice, st));
+        }
         hipLaunchKernelGGL(k_bd_lvltab, dim3(1), dim3(1024), 0, st, mg, nm, (const BdOut *)mo, tab);
         hipLaunchKernelGGL(k_bd_rotate, dim3(g_rot[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo, U, V, ld);
         hipLaunchKernelGGL(k_bd_znorm, dim3(nm), dim3(64), 0, st, mg, W, mo);
@@ -1652,6 +1851,8 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
         hipLaunchKernelGGL(k_bd_vec, dim3(g_sec[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo, Q);
         hipLaunchKernelGGL(k_bd_gemm, dim3(g_tile[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo,
                            (const double *)Q, (const double *)U, (const double *)V, ld, S);
+        if (!hsel.empty())
+            BCK(bd_hss_level(c, mg, mo, lv[t], c->hmo + moff[t], hsel, tab, U, V, ld, S, c->hrank.data() + moff[t]));
         hipLaunchKernelGGL(k_bd_newcol, dim3(nm), dim3(256), 0, st, mg, tab, W, (const BdOut *)mo, (const double *)U,
                            (const double *)V, ld, S);
         hipLaunchKernelGGL(k_bd_scatter, dim3(g_sc[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo,
@@ -1712,7 +1913,10 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
                 const BdMerge &M = lv[t][i];
                 const BdOut &o = c->hmo[moff[t] + i];
                 const int n_ = M.nl + M.nr + 1;
-                c->tr.insert(c->tr.end(), {M.s, n_, o.k, o.nrot});
+                const int rk = c->hrank[moff[t] + i];
+                c->tr.insert(c->tr.end(), {M.s, n_, o.k, o.nrot, rk});
+                if (o.hss) c->st.hss_merges++;
+                c->st.hss_rank_max = std::max(c->st.hss_rank_max, rk);
                 roots += o.k;
                 deflated += n_ - o.k;
                 rots += o.nrot;

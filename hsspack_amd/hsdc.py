@@ -56,14 +56,14 @@ class Stats(ctypes.Structure):
 
 
 class BdParams(ctypes.Structure):
-    _fields_ = [("smlsiz", _i), ("maxit", _i)]
+    _fields_ = [("smlsiz", _i), ("maxit", _i), ("hss_min_k", _i), ("hss_leaf", _i), ("hss_tol", _d)]
 
 
 class BdStats(ctypes.Structure):
     _fields_ = [("levels", _i), ("merges", _i), ("leaves", _i), ("k_top", _i),
                 ("roots", ctypes.c_longlong), ("deflated", ctypes.c_longlong),
                 ("rotations", ctypes.c_longlong), ("secular_iters", ctypes.c_longlong),
-                ("ms_total", _d)]
+                ("ms_total", _d), ("hss_merges", _i), ("hss_rank_max", _i)]
 
 
 _LIB = None
@@ -125,7 +125,7 @@ def lib():
         L.hsdc_bdsdc.argtypes = [_vp, ctypes.c_char, ctypes.c_char, _i, _P(_d), _P(_d), _P(_d), _i,
                                  _P(_d), _i, _P(_d), _P(_i), _P(_d), _P(_i)]
         L.hsdc_bdsdc_dev.argtypes = [_vp, ctypes.c_char, _i, _vp, _vp, _vp, _i, _vp, _i]
-        L.hsdc_bd_trace.argtypes = [_vp, _P(_i), _P(_i), _P(_i), _P(_i), _i]
+        L.hsdc_bd_trace.argtypes = [_vp, _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _i]
         _LIB = L
     return _LIB
 
@@ -447,10 +447,10 @@ class BdContext:
         return {f: getattr(s, f) for f, _ in BdStats._fields_}
 
     def trace(self):
-        cnt = lib().hsdc_bd_trace(self._h, None, None, None, None, 0)
-        a = [np.zeros(max(cnt, 1), np.int32) for _ in range(4)]
+        cnt = lib().hsdc_bd_trace(self._h, None, None, None, None, None, 0)
+        a = [np.zeros(max(cnt, 1), np.int32) for _ in range(5)]
         lib().hsdc_bd_trace(self._h, *[_ip(x) for x in a], cnt)
-        return dict(s=a[0][:cnt], n=a[1][:cnt], k=a[2][:cnt], nrot=a[3][:cnt])
+        return dict(s=a[0][:cnt], n=a[1][:cnt], k=a[2][:cnt], nrot=a[3][:cnt], hss_rank=a[4][:cnt])
 
     def close(self):
         if self._h:

@@ -192,21 +192,26 @@ int hsdc_mlp_train(hsdc_ctx *ctx, const float *x, const float *t, long long ns, 
 
 /* ---------------------------------------------------------------------------
  * Bidiagonal divide-and-conquer SVD: HSSPACK's MDBDSDC (SRC/mdbdsdc.f90) ->
- * MDLASD0 -> MDLASD1 -> MDLASD2 and the dense merge (DLASD3 / DLASD4) it takes for
- * K < BSMLZ (mdlasd2.f90:176).  Every merge takes the dense branch; the HSS branch
- * (MDLASD8 / DHSSVCS, K >= BSMLZ) is not provided.
+ * MDLASD0 -> MDLASD1 -> MDLASD2, then per merge the dense branch (DLASD3 / DLASD4,
+ * K < BSMLZ) or the HSS branch (MDLASD8 / DHSSVCS, K >= BSMLZ, mdlasd1.f90:220-278):
+ * the left and right vector matrices in HSS form, U2 * H_U and V2 * H_V.
  * ------------------------------------------------------------------------- */
 typedef struct hsdc_bd_ctx hsdc_bd_ctx;
 
 typedef struct {
-    int smlsiz;  /* leaf bound, ILAENV(9,'DBDSDC') = 25 (mdbdsdc.f90:193); 3..63 */
-    int maxit;   /* secular iteration cap */
+    int smlsiz;     /* leaf bound, ILAENV(9,'DBDSDC') = 25 (mdbdsdc.f90:193); 3..63 */
+    int maxit;      /* secular iteration cap */
+    int hss_min_k;  /* a merge with K >= hss_min_k takes the HSS branch: BSMLZ = 2000
+                       (mdlasd2.f90:172); >= 1 */
+    int hss_leaf;   /* HSS leaf block, Ni = 40 (mdlasd8.f90:277); >= 8 */
+    double hss_tol; /* HSS compression tolerance, 1.0E-17 (CauchyHssvd_VPS.f90:3694) */
 } hsdc_bd_params;
 
 typedef struct {
     int levels, merges, leaves, k_top;
     long long roots, deflated, rotations, secular_iters;
     double ms_total;  /* device time of the last solve (events), ms */
+    int hss_merges, hss_rank_max;  /* merges of the HSS branch, largest generator rank */
 } hsdc_bd_stats;
 
 void hsdc_bd_default_params(hsdc_bd_params *p);
@@ -227,9 +232,10 @@ int hsdc_bdsdc(hsdc_bd_ctx *ctx, char uplo, char compq, int n, double *d, double
 /* The same on device arrays (COMPQ = 'I'; d_u and d_vt may be NULL for values only). */
 int hsdc_bdsdc_dev(hsdc_bd_ctx *ctx, char uplo, int n, double *d_d, double *d_e, double *d_u, int ldu,
                    double *d_vt, int ldvt);
-/* Merges of the last solve, in launch order: block start (0-based), order N, K and
- * deflation rotations.  Returns the merge count (entries beyond cap are not written). */
-int  hsdc_bd_trace(hsdc_bd_ctx *ctx, int *s, int *n, int *k, int *nrot, int cap);
+/* Merges of the last solve, in launch order: block start (0-based), order N, K,
+ * deflation rotations and the largest HSS generator rank (0: dense branch); any
+ * pointer may be null.  Returns the merge count (entries beyond cap are not written). */
+int  hsdc_bd_trace(hsdc_bd_ctx *ctx, int *s, int *n, int *k, int *nrot, int *hss_rank, int cap);
 
 #ifdef __cplusplus
 }

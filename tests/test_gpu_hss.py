@@ -82,3 +82,38 @@ def test_laed3_dense_branch_vs_oracle(ctx, golden):
         assert info == 0 and info_o == 0
         assert np.array_equal(d_g[:k], d_o[:k])   # secular roots bit-exact
         np.testing.assert_allclose(q_g[:, :k], q_o[:, :k], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("K,ni,toeplitz", [(300, 40, False), (640, 40, True), (1500, 64, False)])
+def test_hss_squared_ranks_and_product_vs_oracle(ctx, K, ni, toeplitz):
+    """The SVD's vector matrices (DHSSVCS): identical pivoting and ranks to the
+    oracle's squared-variable construction, the expansion within rounding of it."""
+    rng = np.random.default_rng(K)
+    if toeplitz:
+        d = np.concatenate([[0.0], 1.0 + np.cumsum(rng.uniform(0.5, 1.5, K - 1)) / K])
+    else:
+        d = np.concatenate([[0.0], np.sort(rng.uniform(0.05, 1.0, K - 1))])
+    z = rng.uniform(0.1, 1.0, K) * rng.choice([-1, 1], K)
+    o = O.lasd8(d, z)
+    assert o["info"] == 0
+    tol = float(np.float32(1e-17))
+    for u, v in ((o["z"], o["aphav"]), (d * o["z"], o["aphau"])):
+        print("case K", K, "ni", ni, flush=True)
+        H = O.hss_build_sq(d, o["sigma"], u, v, o["difl"], o["difr"], ni, tol)
+        try:
+            usz, vsz = O.hss_ranks(H)
+            Hd_ref = O.hss_dense(H)
+        finally:
+            O.hss_free(H)
+        x = np.random.default_rng(2).standard_normal((131, K))
+        y, ranks = ctx.hss_svc(d, o["sigma"], u, v, o["difl"], o["difr"], x, ni, tol)
+        ltr = usz.shape[0]
+        assert ranks[0] != -1
+        assert np.array_equal(ranks[: 2 * ltr:2][: ltr - 1], usz[: ltr - 1, 1])
+        assert np.array_equal(ranks[1: 2 * ltr:2][: ltr - 1], vsz[: ltr - 1, 0])
+        C = O.cauchy_dense_sq(d, o["sigma"], u, v, o["difl"], o["difr"])
+        Hd = ctx.hss_dense(K)
+        scale = np.abs(C).max()
+        assert np.abs(Hd - Hd_ref).max() <= 64 * K ** 0.5 * EPS * scale
+        assert np.abs(Hd - C).max() <= 64 * K ** 0.5 * EPS * scale
+        assert np.abs(y - x @ Hd_ref).max() <= 1e-12 * np.abs(x).max() * K ** 0.5 * scale

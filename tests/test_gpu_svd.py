@@ -166,3 +166,82 @@ def test_device_entry_values_only_and_smlsiz(ctx):
     r = O.bdsdc(d, e, prm=O.params(smlsiz=40))
     assert np.array_equal(s, r["d"]) and np.array_equal(u, r["u"]) and np.array_equal(vt, r["vt"])
     c40.close()
+
+
+# ---------------------------------------------------------------- HSS branch (K >= hss_min_k)
+def near_toeplitz(n, seed):
+    rng = np.random.default_rng(seed)
+    return 1.0 + 1e-3 * rng.standard_normal(n), 1.0 + 1e-3 * rng.standard_normal(n - 1)
+
+
+@pytest.mark.parametrize("gen,n,min_k", [("toeplitz", 1000, 100), ("uniform", 2000, 64),
+                                         ("toeplitz", 2200, 2000)])
+def test_bdsdc_hss_branch_vs_oracle(gen, n, min_k):
+    """Merges with K >= hss_min_k take the HSS branch on both sides: the same merges,
+    K and generator ranks as the oracle, singular values within 4 n eps, vectors
+    orthonormal with residual <= 1e-12 and within 1e-11 of the oracle's."""
+    d, e = near_toeplitz(n, 7) if gen == "toeplitz" else bidiag_cfg(n, seed=7)
+    print("case", gen, n, min_k, flush=True)
+    c = hsdc.BdContext(hss_min_k=min_k)
+    try:
+        s, u, vt, info = c.bdsdc(d, e)
+        tr = c.trace()
+        st = c.stats()
+    finally:
+        c.close()
+    assert info == 0
+    r = O.bdsdc(d, e, prm=O.params(bd_bsmlz=min_k), trace_cap=4 * n)
+    assert r["info"] == 0
+    ot = r["trace"]
+    assert np.array_equal(tr["k"], ot["k"])
+    hs = tr["hss_rank"] > 0
+    assert hs.any() and np.array_equal(hs, ot["rank_max"] > 0)
+    assert st["hss_merges"] == hs.sum()
+    # ranks of the first HSS level match exactly (same inputs); later levels see
+    # vectors that differ in rounding, so their ranks may move by a little
+    assert np.all(np.abs(tr["hss_rank"] - ot["rank_max"]) <= 4)
+    tol_s = 4 * n * EPS * max(s[0], 1.0)
+    assert np.abs(s - r["d"]).max() <= tol_s
+    B = bidiag(d, e)
+    assert np.abs(u.T @ u - np.eye(n)).max() <= 1e-12
+    assert np.abs(vt @ vt.T - np.eye(n)).max() <= 1e-12
+    assert np.abs(u @ np.diag(s) @ vt - B).max() / np.abs(B).max() <= 1e-12
+    assert np.abs(u - r["u"]).max() <= 1e-11
+    assert np.abs(vt - r["vt"]).max() <= 1e-11
+
+
+def test_bdsdc_hss_branch_full_size():
+    """The reference threshold (BSMLZ 2000) on a bidiagonal that barely deflates:
+    n = 8192, the top merges take the HSS branch; device resident."""
+    torch = pytest.importorskip("torch")
+    n = 8192
+    d, e = near_toeplitz(n, 11)
+    print("case toeplitz", n, flush=True)
+    c = hsdc.BdContext()
+    try:
+        dd = torch.tensor(d, device="cuda")
+        de = torch.tensor(e, device="cuda")
+        U = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        VT = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        rc = c.bdsdc_dev(dd, de, U, VT)
+        torch.cuda.synchronize()
+        st = c.stats()
+    finally:
+        c.close()
+    assert rc == 0
+    assert st["hss_merges"] >= 3 and st["hss_rank_max"] > 0
+    s = dd.cpu().numpy()
+    assert np.all(np.diff(s) <= 0)
+    Uc = U.t()  # column-major storage: U.t() is the matrix
+    Vc = VT.t()
+    I = torch.eye(n, dtype=torch.float64, device="cuda")
+    orth_u = (Uc.t() @ Uc - I).abs().max().item()
+    orth_v = (Vc @ Vc.t() - I).abs().max().item()
+    B = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    idx = torch.arange(n, device="cuda")
+    B[idx, idx] = torch.tensor(d, device="cuda")
+    B[idx[:-1], idx[1:]] = torch.tensor(e, device="cuda")
+    res = ((Uc * dd) @ Vc - B).abs().max().item() / B.abs().max().item()
+    print("hss merges %d rank %d orth_u %.2e orth_v %.2e res %.2e" % (st["hss_merges"], st["hss_rank_max"],
+                                                                  orth_u, orth_v, res), flush=True)
+    assert orth_u <= 1e-12 and orth_v <= 1e-12 and res <= 1e-12

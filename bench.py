@@ -187,10 +187,14 @@ def bench_batch(args, np, torch, hsdc):
 
 def bench_svd(args, np, torch, hsdc):
     """Bidiagonal SVD workload: all singular triplets of the order-16384 upper bidiagonal
-    d ~ U(0.5, 1.5), e ~ U(0, 1) (seed 5), device resident, through hsdc_bdsdc_dev."""
-    from hsspack_amd.workloads import bidiag_cfg
+    d ~ U(0.5, 1.5), e ~ U(0, 1) (seed 5), device resident, through hsdc_bdsdc_dev.
+    With --svd-toeplitz the near-Toeplitz bidiagonal (1 + 1e-3 N(0,1), seed 11), whose
+    large merges take the HSS branch; the same matrix with the dense branch only is
+    timed beside it."""
+    from hsspack_amd.workloads import bidiag_cfg, bidiag_toeplitz
     n = args.n if args.n != 32768 else 16384
-    d, e = bidiag_cfg(n, seed=5)
+    toep = args.svd_toeplitz
+    d, e = bidiag_toeplitz(n, 11) if toep else bidiag_cfg(n, seed=5)
     dev = torch.device("cuda:0")
     D0 = torch.tensor(d, device=dev)
     E0 = torch.tensor(e, device=dev)
@@ -217,6 +221,23 @@ def bench_svd(args, np, torch, hsdc):
     st = ctx.stats()
     log("bdsdc n=%d: %.2f ms, stats %s" % (n, t * 1e3, st))
     out = {}
+    if toep:
+        out.update(hss_merges=st["hss_merges"], hss_rank_max=st["hss_rank_max"])
+        if not args.no_plain:
+            # the same matrix with every merge on the dense branch (DLASD3)
+            ctx.set_params(hss_min_k=n + 1)
+            step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                step()
+            torch.cuda.synchronize()
+            td = (time.perf_counter() - t0) / 2
+            ctx.set_params(hss_min_k=2000)
+            step()
+            torch.cuda.synchronize()
+            out.update(dense_only_s=td, speedup_vs_dense=td / t)
+            log("dense branch only: %.2f ms (HSS branch %.2fx faster)" % (td * 1e3, td / t))
     if not args.no_check:
         I = torch.eye(n, dtype=torch.float64, device=dev)
         Uc, Vc = U.t(), VT.t()
@@ -234,7 +255,7 @@ def bench_svd(args, np, torch, hsdc):
                    sorted_descending=bool(np.all(np.diff(s) <= 0)))
         log("orthogonality U %.2e V %.2e residual %.2e" % (orth_u, orth_v, res))
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not toep:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_py as O
         times = {}
@@ -256,8 +277,11 @@ def bench_svd(args, np, torch, hsdc):
         "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": dict({"workload": "upper bidiagonal n=%d, d=U(0.5,1.5), e=U(0,1) seed 5, leaf 25, all "
-                                    "singular triplets, device resident" % n,
+        "config": dict({"workload": ("upper bidiagonal n=%d, d,e = 1+1e-3 N(0,1) seed 11 (near-Toeplitz), "
+                                     "leaf 25, HSS branch for K>=2000 (leaf 40, tol 1e-17), all singular "
+                                     "triplets, device resident" % n) if toep else
+                                    ("upper bidiagonal n=%d, d=U(0.5,1.5), e=U(0,1) seed 5, leaf 25, all "
+                                     "singular triplets, device resident" % n),
                         "levels": st["levels"], "merges": st["merges"], "roots": st["roots"],
                         "deflated": st["deflated"], "rotations": st["rotations"], "k_top": st["k_top"],
                         "iters_per_root": st["secular_iters"] / max(st["roots"], 1),
@@ -283,6 +307,9 @@ def main():
     ap.add_argument("--svd", action="store_true",
                     help="bidiagonal SVD: all singular triplets of the order-16384 upper bidiagonal "
                          "d=U(0.5,1.5), e=U(0,1), seed 5 (MDBDSDC path)")
+    ap.add_argument("--svd-toeplitz", action="store_true",
+                    help="with --svd: the near-Toeplitz bidiagonal (HSS branch at the top merges), "
+                         "timed against the dense branch alone")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device three times (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()

@@ -53,6 +53,14 @@ def bidiag_cfg(n, seed=5, sub=0):
     return 0.5 + u[:n], u[n:].copy()
 
 
+def bidiag_toeplitz(n, seed=11):
+    """(d, e) of an order-n near-Toeplitz upper bidiagonal: 1 + 1e-3 N(0,1) entries.
+    Its singular vectors are delocalised, so the DC merges barely deflate (K ~ N) and
+    the large merges take the HSS branch."""
+    rng = np.random.default_rng(seed)
+    return 1.0 + 1e-3 * rng.standard_normal(n), 1.0 + 1e-3 * rng.standard_normal(n - 1)
+
+
 def batch_sizes(count=512, lo=256, hi=8192, seed=3):
     """Problem orders drawn log-uniform in [lo, hi] from stream (seed, 0)."""
     u = splitmix_uniform(seed, count, 0)

@@ -170,8 +170,8 @@ def test_device_entry_values_only_and_smlsiz(ctx):
 
 # ---------------------------------------------------------------- HSS branch (K >= hss_min_k)
 def near_toeplitz(n, seed):
-    rng = np.random.default_rng(seed)
-    return 1.0 + 1e-3 * rng.standard_normal(n), 1.0 + 1e-3 * rng.standard_normal(n - 1)
+    from hsspack_amd.workloads import bidiag_toeplitz
+    return bidiag_toeplitz(n, seed)
 
 
 @pytest.mark.parametrize("gen,n,min_k", [("toeplitz", 1000, 100), ("uniform", 2000, 64),

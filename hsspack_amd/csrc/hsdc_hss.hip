@@ -157,48 +157,28 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         s_N = *H.lca - nic;
         // a merge whose construction already failed is finished densely; skip it
         s_flag = (*H.err != 0 || M > HSS_MAXM || M < 1 || nic < 1 || s_N < 0) ? 1 : 0;
-        s_off = s_flag ? -1 : scratch_take(H, 8L * (s_N > 0 ? s_N : 1));
+        s_off = s_flag ? -1 : scratch_take(H, 2L * (s_N > 0 ? s_N : 1));
         if (s_flag) atomicOr(H.err, 1);
     }
     __syncthreads();
     if (s_flag || s_off < 0) return;
     const int M = s_M, N = s_N, l0 = lr[2 * (node - 1)], c0 = lc[2 * (node - 1)], c1e = lc[2 * (node - 1) + 1];
     const int NS = N > 0 ? N : 1;
-    double *D = H.scratch + s_off, *V = D + NS;
+    // private per column: the generator V (updated every step) and the root index; the
+    // poles and differences its accurate differences read are shared by every node of
+    // the merge and gathered by index
+    double *V = H.scratch + s_off;
     int *PU = (int *)(V + NS);
-    // per column the pole/difference values its accurate differences read
-    // (FF(pw), FF(pw+1), DIFL(pw), DIFR(pw) with pw = PU(j)), gathered once
-    double *CF0 = V + 2 * NS, *CF1 = CF0 + NS, *CL = CF1 + NS, *CR = CL + NS, *CS = CR + NS;
     for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < c0 - 1) ? j + 1 : j + 1 + (c1e - c0 + 1);
-        D[j] = A1(H.D, src); V[j] = A1(H.V, src);
-        const int pw = A1(H.PU, src);
-        PU[j] = pw;
-        CF0[j] = A1(H.FF, pw); CF1[j] = (pw < H.K) ? A1(H.FF, pw + 1) : 0.0;
-        CL[j] = A1(H.DIFL, pw); CR[j] = A1(H.DIFR, pw);
-        CS[j] = SQ ? A1(H.D0, pw) : 0.0;
+        V[j] = A1(H.V, src);
+        PU[j] = A1(H.PU, src);
     }
-    // fdq<SQ>(H, pd, PU(j)) and ddq<SQ>(H, PU(j), pk) from the gathered values: same operations
-    auto fdj = [&](int pd, int j) -> double {
-        const double r = (pd <= A1(PU, j)) ? (A1(H.FF, pd) - A1(CF0, j)) - A1(CL, j)
-                                           : (A1(H.FF, pd) - A1(CF1, j)) - A1(CR, j);
-        return SQ ? r * (A1(H.FF, pd) + A1(CS, j)) : r;
-    };
-    auto ddj = [&](int j, int pk) -> double {
-        const double r = (A1(PU, j) < pk) ? ((A1(CF1, j) - A1(H.FF, pk)) + A1(CR, j)) - A1(H.DIFL, pk)
-                                          : ((A1(CF0, j) - A1(H.FF, pk + 1)) + A1(CL, j)) - A1(H.DIFR, pk);
-        return SQ ? r * (A1(CS, j) + A1(H.D0, pk)) : r;
-    };
+    auto fdj = [&](int pd, int j) -> double { return fdq<SQ>(H, pd, A1(PU, j)); };
+    auto ddj = [&](int j, int pk) -> double { return ddq<SQ>(H, A1(PU, j), pk); };
     auto swapc = [&](int a, int c) {
-        double t;
-        t = A1(D, a); A1(D, a) = A1(D, c); A1(D, c) = t;
-        t = A1(V, a); A1(V, a) = A1(V, c); A1(V, c) = t;
+        double t = A1(V, a); A1(V, a) = A1(V, c); A1(V, c) = t;
         int u = A1(PU, a); A1(PU, a) = A1(PU, c); A1(PU, c) = u;
-        t = A1(CF0, a); A1(CF0, a) = A1(CF0, c); A1(CF0, c) = t;
-        t = A1(CF1, a); A1(CF1, a) = A1(CF1, c); A1(CF1, c) = t;
-        t = A1(CL, a); A1(CL, a) = A1(CL, c); A1(CL, c) = t;
-        t = A1(CR, a); A1(CR, a) = A1(CR, c); A1(CR, c) = t;
-        t = A1(CS, a); A1(CS, a) = A1(CS, c); A1(CS, c) = t;
     };
     for (int i = tid; i < M; i += HSS_CWG) {
         PL[i] = A1(H.PL, l0 + i);

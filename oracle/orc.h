@@ -42,7 +42,7 @@ typedef struct {
     double tol;   /* HSS compression tolerance (CauchyHssEig.f90:2523) */
     int maxit;    /* secular iteration cap */
     int bd_bsmlz; /* an SVD merge with K >= bd_bsmlz takes the HSS branch (mdlasd2.f90:172) */
-    int bd_ni;    /* its HSS leaf block size (mdlasd8.f90:273) */
+    int bd_ni;    /* its HSS leaf block size (mdlasd8.f90:277) */
 } orc_params;
 
 /* per-merge trace filled by orc_stedc when non-NULL */

@@ -19,7 +19,7 @@ void orc_default_params(orc_params *p)
     p->tol = (double)1.0e-17f; /* CauchyHssEig.f90:2523, a REAL literal */
     p->maxit = 100;
     p->bd_bsmlz = 2000;      /* mdlasd2.f90:172 */
-    p->bd_ni = 40;           /* mdlasd8.f90:273 */
+    p->bd_ni = 40;           /* mdlasd8.f90:277 */
 }
 
 /* ------------------------------------------------------------------ */

@@ -469,19 +469,20 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         s_M = M;
         s_N = *H.lda - nir;
         s_flag = (*H.err != 0 || M > HSS_MAXM || M < 1 || nir < 1 || s_N < 0) ? 1 : 0;
-        s_off = s_flag ? -1 : scratch_take(H, 3L * (s_N > 0 ? s_N : 1));
+        s_off = s_flag ? -1 : scratch_take(H, 2L * (s_N > 0 ? s_N : 1));
         if (s_flag) atomicOr(H.err, 1);
     }
     __syncthreads();
     if (s_flag || s_off < 0) return;
     const int M = s_M, N = s_N, c0 = lc[2 * (node - 1)], r0 = lr[2 * (node - 1)], r1e = lr[2 * (node - 1) + 1];
-    double *F = H.scratch + s_off, *V = F + (N > 0 ? N : 1);
+    // private per outside row: its generator (updated every step) and pole index; the
+    // pole value is gathered from the merge's shared array
+    double *V = H.scratch + s_off;
     int *PU = (int *)(V + (N > 0 ? N : 1));
     for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < r0 - 1) ? j + 1 : j + 1 + (r1e - r0 + 1);
         V[j] = A1(H.U, src);
         PU[j] = A1(H.PL, src);
-        F[j] = A1(H.FF, PU[j]);
     }
     for (int i = tid; i < M; i += HSS_CWG) {
         Dr[i] = A1(H.D, c0 + i);
@@ -558,7 +559,6 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     __syncthreads();
                     if (tid == 0) {
                         double t;
-                        t = A1(F, k); A1(F, k) = A1(F, jj); A1(F, jj) = t;
                         t = A1(V, k); A1(V, k) = A1(V, jj); A1(V, jj) = t;
                         int u = A1(PU, k); A1(PU, k) = A1(PU, jj); A1(PU, jj) = u;
                     }
@@ -607,7 +607,6 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 t = A1(A, k); A1(A, k) = A1(A, a); A1(A, a) = t;
                 int u = A1(PL, k); A1(PL, k) = A1(PL, a); A1(PL, a) = u;
                 u = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = u;
-                t = A1(F, k); A1(F, k) = A1(F, b); A1(F, b) = t;
                 t = A1(V, k); A1(V, k) = A1(V, b); A1(V, b) = t;
                 u = A1(PU, k); A1(PU, k) = A1(PU, b); A1(PU, b) = u;
             }
@@ -616,7 +615,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
         }
-        const double Fk = A1(F, k);
+        const double Fk = A1(H.FF, A1(PU, k));
         const int plk = A1(PL, k), puk = A1(PU, k);
         for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
             double temp = ddq<SQ>(H, A1(PL, j), plk);
@@ -630,7 +629,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         #pragma unroll 8
         for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
             double temp = fdq<SQ>(H, A1(PU, j), plk);
-            const double vj = A1(V, j) * (ppq<SQ>(A1(F, j), Fk) / temp);
+            const double vj = A1(V, j) * (ppq<SQ>(A1(H.FF, A1(PU, j)), Fk) / temp);
             A1(V, j) = vj;
             if (fuse) y1 = vi_better(y1, VI{fabs(vj / fdq<SQ>(H, A1(PU, j), plk1)), j - k});
         }

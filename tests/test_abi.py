@@ -84,6 +84,13 @@ def test_bdsdc_argument_errors_without_device():
     p = hsdc.BdParams()
     L.hsdc_bd_default_params(ctypes.byref(p))
     assert (p.smlsiz, p.maxit) == (25, 100)
+    # the HSS branch: BSMLZ 2000 (mdlasd2.f90:172), Ni 40 (mdlasd8.f90:277), TOL 1.0E-17
+    assert (p.hss_min_k, p.hss_leaf) == (2000, 40) and p.hss_tol == float(np.float32(1e-17))
+    for field, bad, want in (("hss_min_k", 0, -4), ("hss_leaf", 4, -5), ("hss_tol", -1.0, -6)):
+        q = hsdc.BdParams()
+        L.hsdc_bd_default_params(ctypes.byref(q))
+        setattr(q, field, bad)
+        assert L.hsdc_bd_set_params(ctx._h, ctypes.byref(q)) == want, field
     p.smlsiz = 2
     assert L.hsdc_bd_set_params(ctx._h, ctypes.byref(p)) == -2
     ctx.close()

@@ -161,3 +161,26 @@ def test_mdbdsdc_vs_oracle(tmp_path, n, uplo):
     assert np.array_equal(s, r["d"])
     assert np.array_equal(u, r["u"])
     assert np.array_equal(vt, r["vt"])
+
+
+@pytest.mark.gpu
+def test_mdbdsdc_hss_branch_from_fortran(tmp_path):
+    """MDBDSDC from Fortran on a bidiagonal whose top merge reaches K >= 2000 (the
+    reference's BSMLZ): the HSS branch, with singular values within 4 n eps of the
+    oracle's and orthogonality / residual <= 1e-12."""
+    from hsspack_amd.workloads import bidiag_toeplitz
+    n = 2200
+    d, e = bidiag_toeplitz(n, 7)
+    raw = run("bdsdc", tmp_path, i32(n, ord("U")) + f64(d) + f64(e))
+    info = np.frombuffer(raw[:4], "<i4")[0]
+    s = np.frombuffer(raw[4:4 + 8 * n], "<f8")
+    u = np.frombuffer(raw[4 + 8 * n:4 + 8 * n + 8 * n * n], "<f8").reshape(n, n).T
+    vt = np.frombuffer(raw[4 + 8 * n + 8 * n * n:], "<f8").reshape(n, n).T
+    assert info == 0
+    r = O.bdsdc(d, e, trace_cap=4 * n)
+    assert (r["trace"]["rank_max"] > 0).any()  # the oracle took the HSS branch too
+    assert np.abs(s - r["d"]).max() <= 4 * n * 2.0 ** -53 * s[0]
+    B = np.diag(d) + np.diag(e, 1)
+    assert np.abs(u.T @ u - np.eye(n)).max() <= 1e-12
+    assert np.abs(vt @ vt.T - np.eye(n)).max() <= 1e-12
+    assert np.abs((u * s) @ vt - B).max() / np.abs(B).max() <= 1e-12

@@ -106,6 +106,18 @@ struct BdWork {
 // ------------------------------------------------------------------------------
 // device helpers
 // ------------------------------------------------------------------------------
+// butterfly over the 32 lanes of a half wavefront; equals wsum64 of the same values
+// when the other half is zero (its step adds exact zeros)
+__device__ __forceinline__ double wsum32(double v)
+{
+    v = v + xlane<16>(v);
+    v = v + xlane<8>(v);
+    v = v + xlane<4>(v);
+    v = v + xlane<2>(v);
+    v = v + xlane<1>(v);
+    return v;
+}
+
 __device__ __forceinline__ double wsum64(double v)
 {
     v = v + xlane<32>(v);
@@ -271,28 +283,52 @@ __device__ double block_max(double v, double *sh)
 }
 
 // ------------------------------------------------------------------------------
-// leaves: one-sided Jacobi, one wavefront per leaf, lane r holds row r of every
-// column (replaces DLASDQ, mdlasd0.f90:152-183)
+// leaves: one-sided Jacobi in round-robin order (replaces DLASDQ, mdlasd0.f90:152-183),
+// one block per leaf: lane r of every wavefront addresses row r of the LDS columns,
+// and each wavefront rotates one pair of a round at a time
 // ------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdWork W, double *U, double *V,
-                                                long ld, int pmax, int *fail)
+// pair i of round t of the circle method over P (even) columns: column 0 fixed, the
+// others moved one place per round; p < q
+__device__ __forceinline__ void bd_rr_pair(int t, int i, int P, int &p, int &q)
+{
+    int a, b;
+    if (i == 0) {
+        a = 0;
+        b = (P - 2 + t) % (P - 1) + 1;
+    } else {
+        a = (i - 1 + t) % (P - 1) + 1;
+        b = (P - 2 - i + t) % (P - 1) + 1;
+    }
+    p = a < b ? a : b;
+    q = a < b ? b : a;
+}
+
+// HALF (leaves of at most 32 columns): each half of a wavefront rotates its own pair,
+// 32 lanes per column; otherwise a wavefront per pair
+template <bool HALF>
+__global__ __launch_bounds__(1024) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdWork W, double *U, double *V,
+                                                  long ld, int pmax, int *fail)
 {
     const int li = blockIdx.x;
     if (li >= nleaf) return;
     const BdLeaf L = lv[li];
-    const int s = L.s, m = L.m, pc = m + L.sqre, r = threadIdx.x;
+    const int s = L.s, m = L.m, pc = m + L.sqre, r = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nw = (int)(blockDim.x >> 6);
+    const int ppw = HALF ? 2 : 1;                  // pairs per wavefront at a time
+    const int hf = HALF ? (r >> 5) : 0, rr = HALF ? (r & 31) : r;  // half, row
     extern __shared__ double lds[];
-    double *Wc = lds, *Vc = lds + (size_t)pmax * 64, *nn = Vc + (size_t)pmax * 64;
-    int *ord = (int *)(nn + 64);
-    __shared__ int jn_s;
+    double *Wc = lds, *Vc = lds + (size_t)pmax * 64, *nn = Vc + (size_t)pmax * 64, *sq = nn + 64;
+    int *ord = (int *)(sq + 64);
+    __shared__ int jn_s, rot_s;
     const double *d = W.D + s, *e = W.E + s;
-    for (int j = 0; j < pc; j++) {
-        double w = 0.0;
-        if (r == j && j < m) w = d[j];
-        if (j >= 1 && r == j - 1) w = e[j - 1];
-        Wc[j * 64 + r] = w;
-        Vc[j * 64 + r] = (r == j) ? 1.0 : 0.0;
-    }
+    if (wv == 0)
+        for (int j = 0; j < pc; j++) {
+            double w = 0.0;
+            if (r == j && j < m) w = d[j];
+            if (j >= 1 && r == j - 1) w = e[j - 1];
+            Wc[j * 64 + r] = w;
+            Vc[j * 64 + r] = (r == j) ? 1.0 : 0.0;
+        }
     double fro = 0.0;
     for (int j = 0; j < m; j++) fro = fro + d[j] * d[j];
     for (int j = 0; j < pc - 1; j++) fro = fro + e[j] * e[j];
@@ -300,100 +336,120 @@ __global__ __launch_bounds__(64) void k_bd_leaf(const BdLeaf *lv, int nleaf, BdW
     __syncthreads();
     // squared column norms, each the butterfly sum of the column as it now stands (the
     // same value a fresh sum would give), recomputed only when a rotation changes it;
-    // every lane holds and writes the same values (one wavefront)
-    for (int j = 0; j < pc; j++) {
-        double w = Wc[j * 64 + r];
-        nn[j] = wsum64(w * w);
-    }
+    // every lane of the wavefront owning the column writes the same value
+    if (wv == 0)
+        for (int j = 0; j < pc; j++) {
+            double w = Wc[j * 64 + r];
+            const double a = wsum64(w * w);
+            nn[j] = a;
+            sq[j] = sqrt(a);   // the norm, kept beside its square
+        }
+    // round-robin sweeps: the P/2 pairs of a round touch disjoint columns, one pair per
+    // wavefront (or half wavefront) at a time, a barrier between rounds
+    const int P = pc + (pc & 1);
     int sweep, rot = 1;
     for (sweep = 0; sweep < JAC_SWEEPS && rot; sweep++) {
-        rot = 0;
-        for (int p = 0; p < pc - 1; p++) {
-            for (int q = p + 1; q < pc; q++) {
-                double wp = Wc[p * 64 + r], wq = Wc[q * 64 + r];
+        if (threadIdx.x == 0) rot_s = 0;
+        __syncthreads();
+        for (int rd = 0; rd < P - 1; rd++) {
+            for (int pb = wv * ppw; pb < P / 2; pb += nw * ppw) {
+                const int pi = pb + hf;
+                if (pi >= P / 2) continue;
+                int p, q;
+                bd_rr_pair(rd, pi, P, p, q);
+                if (q >= pc) continue;
+                double wp = Wc[p * 64 + rr], wq = Wc[q * 64 + rr];
                 double al = nn[p];
                 double be = nn[q];
-                double ga = wsum64(wp * wq);
-                if (al <= thr || be <= thr || !(fabs(ga) > (JAC_TOL * pc) * (sqrt(al) * sqrt(be)))) continue;
+                double ga = HALF ? wsum32(wp * wq) : wsum64(wp * wq);
+                if (al <= thr || be <= thr || !(fabs(ga) > (JAC_TOL * pc) * (sq[p] * sq[q]))) continue;
                 double zeta = (be - al) / (2.0 * ga), t;
                 if (fabs(zeta) > 1e150) t = 0.5 / zeta;
                 else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
                 const double np_ = c * wp - sn * wq, nq_ = sn * wp + c * wq;
-                Wc[p * 64 + r] = np_;
-                Wc[q * 64 + r] = nq_;
-                double vp = Vc[p * 64 + r], vq = Vc[q * 64 + r];
-                Vc[p * 64 + r] = c * vp - sn * vq;
-                Vc[q * 64 + r] = sn * vp + c * vq;
-                const double ap = wsum64(np_ * np_), aq = wsum64(nq_ * nq_);
+                Wc[p * 64 + rr] = np_;
+                Wc[q * 64 + rr] = nq_;
+                double vp = Vc[p * 64 + rr], vq = Vc[q * 64 + rr];
+                Vc[p * 64 + rr] = c * vp - sn * vq;
+                Vc[q * 64 + rr] = sn * vp + c * vq;
+                const double ap = HALF ? wsum32(np_ * np_) : wsum64(np_ * np_);
+                const double aq = HALF ? wsum32(nq_ * nq_) : wsum64(nq_ * nq_);
                 nn[p] = ap;
                 nn[q] = aq;
-                rot = 1;
+                sq[p] = sqrt(ap);
+                sq[q] = sqrt(aq);
+                if (rr == 0) rot_s = 1;
             }
+            __syncthreads();
         }
+        rot = rot_s;
+        __syncthreads();  // every thread has read rot_s before the next sweep clears it
     }
-    if (rot && r == 0) atomicAdd(fail, 1);
-    __syncthreads();
-    if (r == 0) {
+    if (rot && threadIdx.x == 0) atomicAdd(fail, 1);
+    if (threadIdx.x == 0) {
         int used[64];
         for (int j = 0; j < pc; j++) used[j] = 0;
         int jn = -1;
         if (L.sqre) {
             jn = 0;
             for (int j = 1; j < pc; j++)
-                if (sqrt(nn[j]) < sqrt(nn[jn])) jn = j;
+                if (sq[j] < sq[jn]) jn = j;
             used[jn] = 1;
         }
         for (int k = 0; k < m; k++) {
             int b = -1;
             for (int j = 0; j < pc; j++)
-                if (!used[j] && (b < 0 || sqrt(nn[j]) < sqrt(nn[b]))) b = j;
+                if (!used[j] && (b < 0 || sq[j] < sq[b])) b = j;
             used[b] = 1;
             ord[k] = b;
         }
         jn_s = jn;
     }
     __syncthreads();
-    for (int k = 0; k < m; k++) {
-        int j = ord[k];
-        double sg = sqrt(nn[j]);
-        if (r == 0) W.D[s + k] = sg;
-        if (r < m) U[(long)(s + k) * ld + s + r] = nn[j] > thr ? Wc[j * 64 + r] / sg : 0.0;
-        if (r < pc) V[(long)(s + k) * ld + s + r] = Vc[j * 64 + r];
-        if (r == 0) W.idxq[s + k] = k + 1;
-    }
-    if (L.sqre && r < pc) V[(long)(s + m) * ld + s + r] = Vc[jn_s * 64 + r];
-    // slot = column, supported on the leaf's rows
-    if (r < m) {
-        W.vmU[L.wr][s + r] = s + r;
-        W.sulo[s + r] = s;
-        W.suhi[s + r] = s + m;
-    }
-    if (r < pc) {
-        W.vmV[L.wr][s + r] = s + r;
-        W.svlo[s + r] = s;
-        W.svhi[s + r] = s + pc;
+    if (wv == 0) {
+        for (int k = 0; k < m; k++) {
+            int j = ord[k];
+            double sg = sq[j];
+            if (r == 0) W.D[s + k] = sg;
+            if (r < m) U[(long)(s + k) * ld + s + r] = nn[j] > thr ? Wc[j * 64 + r] / sg : 0.0;
+            if (r < pc) V[(long)(s + k) * ld + s + r] = Vc[j * 64 + r];
+            if (r == 0) W.idxq[s + k] = k + 1;
+        }
+        if (L.sqre && r < pc) V[(long)(s + m) * ld + s + r] = Vc[jn_s * 64 + r];
+        // slot = column, supported on the leaf's rows
+        if (r < m) {
+            W.vmU[L.wr][s + r] = s + r;
+            W.sulo[s + r] = s;
+            W.suhi[s + r] = s + m;
+        }
+        if (r < pc) {
+            W.vmV[L.wr][s + r] = s + r;
+            W.svlo[s + r] = s;
+            W.svhi[s + r] = s + pc;
+        }
     }
     __syncthreads();
     // left vectors of negligible columns: the first e_r keeping norm > 1/2 after two
-    // projections off the other columns
+    // projections off the other columns (wavefront 0; the loop bounds are uniform)
     for (int k = 0; k < m; k++) {
         if (nn[ord[k]] > thr) continue;
-        for (int r0 = 0; r0 < m; r0++) {
-            double x = (r == r0) ? 1.0 : 0.0;
-            for (int pass = 0; pass < 2; pass++)
-                for (int k2 = 0; k2 < m; k2++) {
-                    if (k2 == k) continue;
-                    double uk = r < m ? U[(long)(s + k2) * ld + s + r] : 0.0;
-                    double g = wsum64(r < m ? uk * x : 0.0);
-                    if (r < m) x = x - g * uk;
+        if (wv == 0)
+            for (int r0 = 0; r0 < m; r0++) {
+                double x = (r == r0) ? 1.0 : 0.0;
+                for (int pass = 0; pass < 2; pass++)
+                    for (int k2 = 0; k2 < m; k2++) {
+                        if (k2 == k) continue;
+                        double uk = r < m ? U[(long)(s + k2) * ld + s + r] : 0.0;
+                        double g = wsum64(r < m ? uk * x : 0.0);
+                        if (r < m) x = x - g * uk;
+                    }
+                double nx = sqrt(wsum64(x * x));
+                if (nx > 0.5) {
+                    if (r < m) U[(long)(s + k) * ld + s + r] = x / nx;
+                    break;
                 }
-            double nx = sqrt(wsum64(x * x));
-            if (nx > 0.5) {
-                if (r < m) U[(long)(s + k) * ld + s + r] = x / nx;
-                break;
             }
-        }
         __syncthreads();
     }
 }
@@ -1767,9 +1823,17 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
     // leaves
     int pmax = 1;
     for (auto &L : leaves) pmax = std::max(pmax, L.m + L.sqre);
-    size_t lds = sizeof(double) * (2 * (size_t)pmax * 64 + 64) + sizeof(int) * 64;
-    hipLaunchKernelGGL(k_bd_leaf, dim3((unsigned)leaves.size()), dim3(64), lds, st, dlv, (int)leaves.size(), W, U, V,
-                       ld, pmax, dfail);
+    size_t lds = sizeof(double) * (2 * (size_t)pmax * 64 + 128) + sizeof(int) * 64;
+    // a round-robin round has P/2 pairs: one per half wavefront for leaves of <= 32
+    // columns, else one per wavefront; at most 16 wavefronts
+    const int npair = (pmax + (pmax & 1)) / 2;
+    if (pmax <= 32)
+        hipLaunchKernelGGL(k_bd_leaf<true>, dim3((unsigned)leaves.size()),
+                           dim3(64 * std::max(std::min((npair + 1) / 2, 16), 1)), lds, st, dlv, (int)leaves.size(), W,
+                           U, V, ld, pmax, dfail);
+    else
+        hipLaunchKernelGGL(k_bd_leaf<false>, dim3((unsigned)leaves.size()), dim3(64 * std::max(std::min(npair, 16), 1)),
+                           lds, st, dlv, (int)leaves.size(), W, U, V, ld, pmax, dfail);
     BHK(hipGetLastError());
     // Every level is queued without a host round trip: k_bd_lvltab sizes the level's
     // work from its deflation results on the device, the kernels after it walk those

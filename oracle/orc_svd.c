@@ -148,10 +148,27 @@ static double bfly64(double *v)
 /* SVD of the m x (m+sqre) upper bidiagonal B (diagonal d, superdiagonal e, e has
  * m-1+sqre entries): B = U [diag(s) 0] VT, s ascending in d, U m x m (ldu), VT
  * (m+sqre) x (m+sqre) (ldvt) whose last row is the null vector when sqre = 1.
- * One-sided Jacobi on the columns of B in row-cyclic order; a pair is rotated while
- * |w_p.w_q| > (m+sqre) 2^-52 |w_p||w_q| (the rounding level of the sums), until a
- * sweep rotates nothing; returns the number of sweeps, or -1 if JAC_SWEEPS did not
+ * One-sided Jacobi on the columns of B in round-robin order (a sweep = P-1 rounds of
+ * P/2 disjoint pairs, P = m+sqre rounded up to even, bd_rr_pair); a pair is rotated
+ * while |w_p.w_q| > (m+sqre) 2^-52 |w_p||w_q| (the rounding level of the sums), until
+ * a sweep rotates nothing; returns the number of sweeps, or -1 if JAC_SWEEPS did not
  * suffice. */
+/* pair i of round t of the circle method over P (even) columns: column 0 fixed, the
+ * others moved one place per round; p < q */
+static void bd_rr_pair(int t, int i, int P, int *p, int *q)
+{
+    int a, b;
+    if (i == 0) {
+        a = 0;
+        b = (P - 2 + t) % (P - 1) + 1;
+    } else {
+        a = (i - 1 + t) % (P - 1) + 1;
+        b = (P - 2 - i + t) % (P - 1) + 1;
+    }
+    *p = a < b ? a : b;
+    *q = a < b ? b : a;
+}
+
 int orc_bd_leaf(int m, int sqre, double *d, const double *e, double *u, int ldu, double *vt,
                 int ldvt)
 {
@@ -172,10 +189,14 @@ int orc_bd_leaf(int m, int sqre, double *d, const double *e, double *u, int ldu,
     for (int j = 0; j < pc - 1; j++) fro = fro + e[j] * e[j];
     const double thr = (JAC_TOL * pc) * (JAC_TOL * pc) * fro;
     int sweep, rot = 1;
+    const int P = pc + (pc & 1);
     for (sweep = 0; sweep < JAC_SWEEPS && rot; sweep++) {
         rot = 0;
-        for (int p = 0; p < pc - 1; p++) {
-            for (int q = p + 1; q < pc; q++) {
+        for (int rd = 0; rd < P - 1; rd++) {
+            for (int pi = 0; pi < P / 2; pi++) {
+                int p, q;
+                bd_rr_pair(rd, pi, P, &p, &q);
+                if (q >= pc) continue;
                 for (int r = 0; r < 64; r++) a[r] = W[p][r] * W[p][r];
                 double al = bfly64(a);
                 for (int r = 0; r < 64; r++) a[r] = W[q][r] * W[q][r];

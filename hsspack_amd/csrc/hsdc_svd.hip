@@ -212,7 +212,9 @@ __device__ int block_exscan(int v, int *sh, int &total)
 // Ranks the flagged positions of [lo, hi) in position order: each wavefront walks one
 // contiguous segment 64 positions at a time (ballot + popcount), a block scan over the
 // wavefront counts gives each segment its base.  emit(p, rank) for every flagged p;
-// returns the total.  flag must be free of side effects (it is evaluated twice).
+// returns the total.  The flags of a segment's first 64 steps are evaluated up front
+// into one bit per step (independent loads, in flight together) and reused by the emit
+// pass; flag must be free of side effects and must not read what emit writes.
 template <typename F, typename G>
 __device__ int wave_rank(int lo, int hi, F flag, G emit, int *sh)
 {
@@ -220,11 +222,19 @@ __device__ int wave_rank(int lo, int hi, F flag, G emit, int *sh)
     const int len = hi > lo ? hi - lo : 0;
     const int seg = (((len + nw - 1) / nw) + 63) & ~63;
     const int a = lo + w * seg, b = min(a + seg, hi);
-    int cnt = 0;
-    for (int p0 = a; p0 < b; p0 += 64) {
-        const int p = p0 + lane;
-        cnt += __popcll(__ballot(p < b && flag(p)));
+    const int nit = b > a ? (b - a + 63) / 64 : 0;
+    unsigned long long fm = 0;
+#pragma unroll 8
+    for (int i = 0; i < min(nit, 64); i++) {
+        const int p = a + 64 * i + lane;
+        if (p < b && flag(p)) fm |= 1ull << i;
     }
+    auto fl = [&](int i) -> bool {
+        const int p = a + 64 * i + lane;
+        return i < 64 ? ((fm >> i) & 1ull) != 0 : (p < b && flag(p));
+    };
+    int cnt = 0;
+    for (int i = 0; i < nit; i++) cnt += __popcll(__ballot(fl(i)));
     if (lane == 0) sh[w] = cnt;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -241,14 +251,32 @@ __device__ int wave_rank(int lo, int hi, F flag, G emit, int *sh)
     const int total = sh[nw];
     __syncthreads();
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int p0 = a; p0 < b; p0 += 64) {
-        const int p = p0 + lane;
-        const bool f = p < b && flag(p);
+    for (int i = 0; i < nit; i++) {
+        const int p = a + 64 * i + lane;
+        const bool f = fl(i);
         const unsigned long long msk = __ballot(f);
         if (f) emit(p, base + __popcll(msk & below));
         base += __popcll(msk);
     }
     return total;
+}
+
+// for (i = i0 + threadIdx.x; i < n; i += NT): the loads of U consecutive iterations
+// (load(i) -> value) are issued before their stores (store(i, value)), so their
+// dependent gathers overlap instead of waiting on the previous iteration's stores.
+// The iterations must be independent: no load reads what a store of the loop writes.
+template <int U, int NT, typename L, typename S>
+__device__ __forceinline__ void bd_batched(int i0, int n, L load, S store)
+{
+    for (int b = i0 + (int)threadIdx.x; b < n; b += U * NT) {
+        decltype(load(0)) v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (b + u * NT < n) v[u] = load(b + u * NT);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (b + u * NT < n) store(b + u * NT, v[u]);
+    }
 }
 
 // merge path: how many of the first diag outputs of the stable merge of ascending
@@ -494,25 +522,22 @@ __global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W
     }
     // row NLP1 (NLP2) of the children's right vectors; a slot reads as zero outside
     // its support (a column deflated below this level ends above those rows)
-    for (int i = t; i < m; i += PREP_NT) {
+    bd_batched<8, PREP_NT>(0, m, [&](int i) {
         const int slot = vm[s + (i == 0 ? nl : (i <= nl ? i - 1 : i))];
         const int g = (i <= nl) ? s + nl : s + nl + 1;
-        const double v = (g >= W.svlo[slot] && g < W.svhi[slot]) ? V[(long)slot * ld + g] : 0.0;
-        W.z[s + i] = (i <= nl ? alpha : beta) * v;
-    }
+        return (g >= W.svlo[slot] && g < W.svhi[slot]) ? V[(long)slot * ld + g] : 0.0;
+    }, [&](int i, double v) { W.z[s + i] = (i <= nl ? alpha : beta) * v; });
     __syncthreads();
-    for (int i = t; i < n; i += PREP_NT) {
-        W.D[s + i] = W.dsg[s + i];
-        W.idxq[s + i] = W.ti[s + i];
-    }
+    struct DQ { double d; int q; };
+    bd_batched<8, PREP_NT>(0, n, [&](int i) { return DQ{W.dsg[s + i], W.ti[s + i]}; },
+                           [&](int i, const DQ &v) { W.D[s + i] = v.d; W.idxq[s + i] = v.q; });
     __syncthreads();
     // DSIGMA(I) = D(IDXQ(I)), U2(I,1) = Z(IDXQ(I)), IDXC(I) = COLTYP(IDXQ(I))
-    for (int i = t + 1; i < n; i += PREP_NT) {
-        int q = W.idxq[s + i] - 1;
-        W.dsg[s + i] = W.D[s + q];
-        W.zt[s + i] = W.z[s + q];
-        W.idxc[s + i] = W.ctyp[s + q];
-    }
+    struct DZC { double d, z; int c; };
+    bd_batched<8, PREP_NT>(1, n, [&](int i) {
+        const int q = W.idxq[s + i] - 1;
+        return DZC{W.D[s + q], W.z[s + q], W.ctyp[s + q]};
+    }, [&](int i, const DZC &v) { W.dsg[s + i] = v.d; W.zt[s + i] = v.z; W.idxc[s + i] = v.c; });
     __syncthreads();
     // DLAMRG(NL, NR, DSIGMA(2), 1, 1, IDX(2)): ties take the left run first; merge path,
     // each thread places a contiguous run of outputs
@@ -528,12 +553,10 @@ __global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W
         }
     }
     __syncthreads();
-    for (int i = t + 1; i < n; i += PREP_NT) {
-        int ii = W.idx[s + i];  // 1-based into DSIGMA(2..) -> DSIGMA(1 + ii)
-        W.D[s + i] = W.dsg[s + ii];
-        W.z[s + i] = W.zt[s + ii];
-        W.ctyp[s + i] = W.idxc[s + ii];
-    }
+    bd_batched<8, PREP_NT>(1, n, [&](int i) {
+        const int ii = W.idx[s + i];  // 1-based into DSIGMA(2..) -> DSIGMA(1 + ii)
+        return DZC{W.dsg[s + ii], W.zt[s + ii], W.idxc[s + ii]};
+    }, [&](int i, const DZC &v) { W.D[s + i] = v.d; W.z[s + i] = v.z; W.ctyp[s + i] = v.c; });
     __syncthreads();
     if (t == 0) {
         double tol = fmax(fabs(alpha), fabs(beta));
@@ -663,20 +686,26 @@ __global__ __launch_bounds__(DEF_NT) void k_bd_deflate(const BdMerge *mg, BdWork
     __syncthreads();
     // DSIGMA(J) = D(IDXP(J)); slots of U2(:,J) and VT2(J,:) and their supports (the
     // merge block for a column a rotation touched)
-    for (int p = 1 + t; p < n; p += DEF_NT) {
-        W.dsg[s + p] = W.D[s + W.idxp[s + p] - 1];
-        int src = bd_colidx(W, s, nl, W.idxp[s + W.idxc[s + p] - 1] - 1);
-        int su = W.vmU[M.rd][s + src], sv = W.vmV[M.rd][s + src];
-        W.gU[s + p] = su;
-        W.gV[s + p] = sv;
+    struct GS { double d; int su, sv, ulo, uhi, vlo, vhi; };
+    bd_batched<8, DEF_NT>(1, n, [&](int p) {
+        GS g;
+        g.d = W.D[s + W.idxp[s + p] - 1];
+        const int src = bd_colidx(W, s, nl, W.idxp[s + W.idxc[s + p] - 1] - 1);
+        g.su = W.vmU[M.rd][s + src];
+        g.sv = W.vmV[M.rd][s + src];
         if (W.rotf[s + src]) {
-            W.gUlo[s + p] = s; W.gUhi[s + p] = s + n;
-            W.gVlo[s + p] = s; W.gVhi[s + p] = s + m;
+            g.ulo = s; g.uhi = s + n; g.vlo = s; g.vhi = s + m;
         } else {
-            W.gUlo[s + p] = W.sulo[su]; W.gUhi[s + p] = W.suhi[su];
-            W.gVlo[s + p] = W.svlo[sv]; W.gVhi[s + p] = W.svhi[sv];
+            g.ulo = W.sulo[g.su]; g.uhi = W.suhi[g.su]; g.vlo = W.svlo[g.sv]; g.vhi = W.svhi[g.sv];
         }
-    }
+        return g;
+    }, [&](int p, const GS &g) {
+        W.dsg[s + p] = g.d;
+        W.gU[s + p] = g.su;
+        W.gV[s + p] = g.sv;
+        W.gUlo[s + p] = g.ulo; W.gUhi[s + p] = g.uhi;
+        W.gVlo[s + p] = g.vlo; W.gVhi[s + p] = g.vhi;
+    });
     __syncthreads();
     if (t == 0) {
         W.dsg[s] = 0.0;
@@ -1279,12 +1308,9 @@ __global__ __launch_bounds__(POST_NT) void k_bd_post(const BdMerge *mg, BdWork W
     const int s = M.s, n = M.nl + M.nr + 1, k = o.k, t = threadIdx.x;
     double mul[8];
     const int nm = lascl_muls(1.0, o.orgnrm, mul);
-    for (int i = t; i < n; i += POST_NT) {
-        double v;
-        if (i < k) v = (k == 1) ? fabs(o.zr1) : W.sig[s + i];
-        else v = W.dsg[s + i];
-        W.D[s + i] = apply_muls(v, mul, nm);
-    }
+    bd_batched<8, POST_NT>(0, n, [&](int i) {
+        return (i < k) ? ((k == 1) ? fabs(o.zr1) : W.sig[s + i]) : W.dsg[s + i];
+    }, [&](int i, double v) { W.D[s + i] = apply_muls(v, mul, nm); });
     __syncthreads();
     // DLAMRG(K, N-K, D, 1, -1, IDXQ): run 2 read backwards; merge path
     {
@@ -1302,22 +1328,23 @@ __global__ __launch_bounds__(POST_NT) void k_bd_post(const BdMerge *mg, BdWork W
     // deflated ones where they were (the block once a rotation touched them)
     const int m = n + M.sqre;
     int *vu = W.vmU[M.wr], *vv = W.vmV[M.wr];
-    for (int i = t; i < n; i += POST_NT) {
-        int su, sv;
+    struct MS { int su, sv, ulo, uhi, vlo, vhi; };
+    bd_batched<8, POST_NT>(0, n, [&](int i) {
+        MS g;
         if (i < k) {
-            su = W.fU[s + i];
-            sv = W.fV[s + i];
-            W.sulo[su] = s; W.suhi[su] = s + n;
-            W.svlo[sv] = s; W.svhi[sv] = s + m;
+            g.su = W.fU[s + i]; g.sv = W.fV[s + i];
+            g.ulo = s; g.uhi = s + n; g.vlo = s; g.vhi = s + m;
         } else {
-            su = W.gU[s + i];
-            sv = W.gV[s + i];
-            W.sulo[su] = W.gUlo[s + i]; W.suhi[su] = W.gUhi[s + i];
-            W.svlo[sv] = W.gVlo[s + i]; W.svhi[sv] = W.gVhi[s + i];
+            g.su = W.gU[s + i]; g.sv = W.gV[s + i];
+            g.ulo = W.gUlo[s + i]; g.uhi = W.gUhi[s + i]; g.vlo = W.gVlo[s + i]; g.vhi = W.gVhi[s + i];
         }
-        vu[s + i] = su;
-        vv[s + i] = sv;
-    }
+        return g;
+    }, [&](int i, const MS &g) {
+        W.sulo[g.su] = g.ulo; W.suhi[g.su] = g.uhi;
+        W.svlo[g.sv] = g.vlo; W.svhi[g.sv] = g.vhi;
+        vu[s + i] = g.su;
+        vv[s + i] = g.sv;
+    });
     if (t == 0 && m > n) {
         int sv = W.fV[s + k];
         vv[s + m - 1] = sv;

@@ -40,6 +40,7 @@ int hsdc_hss_batch(hsdc_ctx *c, hipStream_t s, const std::vector<HssSpec> &sp, i
                    std::vector<int> *rank, int *dense);
 
 #define BD_EPS 0x1p-53
+#define BD_LDS_RUNS 19968  // doubles of dynamic LDS for a merge's sorted runs (156 KB)
 #define BD_SAFMIN 0x1p-1022
 #define JAC_TOL 0x1p-52
 #define JAC_SWEEPS 60
@@ -488,10 +489,13 @@ __global__ __launch_bounds__(1024) void k_bd_leaf(const BdLeaf *lv, int nleaf, B
 // (mdlasd2.f90:216-262), TOL (mdlasd2.f90:266-268)
 // ------------------------------------------------------------------------------
 #define PREP_NT 1024
+// lds_n: capacity in doubles of the dynamic LDS that holds the merged runs (0: they
+// are read from global memory)
 __global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W, BdOut *mo, const double *V,
-                                                     long ld)
+                                                     long ld, int lds_n)
 {
     __shared__ double shd[PREP_NT / 64 + 1];
+    extern __shared__ double lruns[];
     const BdMerge M = mg[blockIdx.x];
     const int s = M.s, nl = M.nl, nr = M.nr, n = nl + nr + 1, m = n + M.sqre, t = threadIdx.x;
     const int *vm = W.vmV[M.rd];
@@ -540,9 +544,16 @@ __global__ __launch_bounds__(PREP_NT) void k_bd_prep(const BdMerge *mg, BdWork W
     }, [&](int i, const DZC &v) { W.dsg[s + i] = v.d; W.zt[s + i] = v.z; W.idxc[s + i] = v.c; });
     __syncthreads();
     // DLAMRG(NL, NR, DSIGMA(2), 1, 1, IDX(2)): ties take the left run first; merge path,
-    // each thread places a contiguous run of outputs
+    // each thread places a contiguous run of outputs.  The two runs are staged in LDS
+    // when they fit: the co-rank search and the run read them with data-dependent
+    // indices, one after another
+    const bool inl = n - 1 <= lds_n;
+    if (inl) {
+        for (int i = t; i < n - 1; i += PREP_NT) lruns[i] = W.dsg[s + 1 + i];
+        __syncthreads();
+    }
     {
-        const double *a = W.dsg + s + 1, *b = W.dsg + s + 1 + nl;
+        const double *a = inl ? lruns : W.dsg + s + 1, *b = a + nl;
         const int tot = nl + nr, per = (tot + PREP_NT - 1) / PREP_NT;
         const int o0 = min(t * per, tot), o1 = min(o0 + per, tot);
         auto bget = [b](int j) { return b[j]; };
@@ -1301,8 +1312,9 @@ __global__ __launch_bounds__(1024) void k_bd_lvltab(const BdMerge *mg, int nmg, 
 // unscaled, IDXQ = DLAMRG(K, N-K, D, 1, -1) (mdlasd1.f90:282-286)
 // ------------------------------------------------------------------------------
 #define POST_NT 1024
-__global__ __launch_bounds__(POST_NT) void k_bd_post(const BdMerge *mg, BdWork W, const BdOut *mo)
+__global__ __launch_bounds__(POST_NT) void k_bd_post(const BdMerge *mg, BdWork W, const BdOut *mo, int lds_n)
 {
+    extern __shared__ double lruns[];
     const BdMerge M = mg[blockIdx.x];
     const BdOut &o = mo[blockIdx.x];
     const int s = M.s, n = M.nl + M.nr + 1, k = o.k, t = threadIdx.x;
@@ -1312,9 +1324,15 @@ __global__ __launch_bounds__(POST_NT) void k_bd_post(const BdMerge *mg, BdWork W
         return (i < k) ? ((k == 1) ? fabs(o.zr1) : W.sig[s + i]) : W.dsg[s + i];
     }, [&](int i, double v) { W.D[s + i] = apply_muls(v, mul, nm); });
     __syncthreads();
-    // DLAMRG(K, N-K, D, 1, -1, IDXQ): run 2 read backwards; merge path
+    // DLAMRG(K, N-K, D, 1, -1, IDXQ): run 2 read backwards; merge path over D staged in
+    // LDS when it fits
+    const bool inl = n <= lds_n;
+    if (inl) {
+        for (int i = t; i < n; i += POST_NT) lruns[i] = W.D[s + i];
+        __syncthreads();
+    }
     {
-        const double *a = W.D + s;
+        const double *a = inl ? lruns : W.D + s;
         const int n2 = n - k, per = (n + POST_NT - 1) / POST_NT;
         const int o0 = min(t * per, n), o1 = min(o0 + per, n);
         auto bget = [a, n](int j) { return a[n - 1 - j]; };
@@ -1909,7 +1927,13 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
         tab.sc = ti + 4 * (nm + 1);
         tab.qoff = (long *)(tb + al8(sizeof(int) * 5 * (size_t)(nm + 1)));
         tab.soff = tab.qoff + nm;
-        hipLaunchKernelGGL(k_bd_prep, dim3(nm), dim3(PREP_NT), 0, st, mg, W, mo, (const double *)V, ld);
+        // the merged runs in LDS (gfx950: 160 KB per workgroup, a few KB kept for the
+        // static arrays)
+        int nmax = 0;
+        for (const BdMerge &M : lv[t]) nmax = std::max(nmax, M.nl + M.nr + 1);
+        const int lds_n = nmax <= BD_LDS_RUNS ? nmax : 0;
+        hipLaunchKernelGGL(k_bd_prep, dim3(nm), dim3(PREP_NT), sizeof(double) * (size_t)lds_n, st, mg, W, mo,
+                           (const double *)V, ld, lds_n);
         hipLaunchKernelGGL(k_bd_deflate, dim3(nm), dim3(DEF_NT), 0, st, mg, W, mo);
         // a level with a merge of order >= hss_min_k reads its K back: the HSS branch is
         // planned on the host (tree shape from K); the others stay queued
@@ -1948,7 +1972,8 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
                            (const double *)V, ld, S);
         hipLaunchKernelGGL(k_bd_scatter, dim3(g_sc[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo,
                            (const double *)S, U, V, ld);
-        hipLaunchKernelGGL(k_bd_post, dim3(nm), dim3(POST_NT), 0, st, mg, W, (const BdOut *)mo);
+        hipLaunchKernelGGL(k_bd_post, dim3(nm), dim3(POST_NT), sizeof(double) * (size_t)lds_n, st, mg, W,
+                           (const BdOut *)mo, lds_n);
         BHK(hipGetLastError());
         c->st.levels++;
         c->st.merges += nm;

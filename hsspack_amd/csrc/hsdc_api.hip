@@ -1473,10 +1473,7 @@ static void make_plan_split(Plan &pl, int n, double *d, double *e, int smlsiz)
 // ============================================================================
 // C ABI
 // ============================================================================
-// HSS batch of another solver (the bidiagonal SVD's merges) on its stream: build
-// every matrix of `sp`, Y := X * H.  rank (may be null) gets per matrix its largest
-// generator rank (0 when applied densely: tree too small or a node over the LDS /
-// arena limits); *dense (may be null) the number of such matrices.
+// HSS batch of another solver on its stream (declared in hsdc_hss.h)
 int hsdc_hss_batch(hsdc_ctx *c, hipStream_t s, const std::vector<HssSpec> &sp, int ni, double tol,
                    std::vector<int> *rank, int *dense)
 {

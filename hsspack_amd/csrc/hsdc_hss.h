@@ -1,6 +1,8 @@
 // Device descriptors of the HSS construction and multiply.
 #pragma once
 #include <stdint.h>
+#include <vector>
+#include <hip/hip_runtime.h>
 
 #define HSS_MAXM 512  // largest row/column block a node may compress (LDS tile)
 
@@ -77,3 +79,11 @@ struct HssSpec {
     long ymoff = 0;
     int sq = 0;        // squared variables (HssMerge::sq)
 };
+
+// The HSS driver of the tridiagonal solver run for another solver's batch on stream s
+// (defined in hsdc_api.hip): builds every matrix of sp and sets Y := X * H.  rank
+// (may be null) gets per matrix its largest generator rank (0 when applied densely);
+// *dense (may be null) the number of matrices applied densely.
+struct hsdc_ctx;
+int hsdc_hss_batch(hsdc_ctx *c, hipStream_t s, const std::vector<HssSpec> &sp, int ni, double tol,
+                   std::vector<int> *rank, int *dense);

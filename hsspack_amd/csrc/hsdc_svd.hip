@@ -35,10 +35,6 @@
 #include "hsdc_internal.h"
 #include "hsdc_hss.h"
 
-// HSS batch on a stream (defined with the tridiagonal solver's HSS driver)
-int hsdc_hss_batch(hsdc_ctx *c, hipStream_t s, const std::vector<HssSpec> &sp, int ni, double tol,
-                   std::vector<int> *rank, int *dense);
-
 #define BD_EPS 0x1p-53
 #define BD_LDS_RUNS 19968  // doubles of dynamic LDS for a merge's sorted runs (156 KB)
 #define BD_SAFMIN 0x1p-1022
@@ -1708,6 +1704,7 @@ static int bd_hss_level(hsdc_bd_ctx *c, const BdMerge *mg, const BdOut *mo, cons
     BdHss *dh = (BdHss *)c->htb.p;
     double *G = (double *)c->hgen.p, *X = (double *)c->hx.p;
     BHK(hipMemcpyAsync(dh, hs.data(), sizeof(BdHss) * nh, hipMemcpyHostToDevice, st));
+    BHK(hipStreamSynchronize(st));  // hs is pageable host memory owned by the caller
     hipLaunchKernelGGL(k_bd_hssgen, dim3((unsigned)gx, nh), dim3(256), 0, st, mg, (const BdHss *)dh, c->W, mo, G);
     hipLaunchKernelGGL(k_bd_xgather, dim3((unsigned)std::min(bcdiv(xmax, 256), 4096), nh), dim3(256), 0, st, mg,
                        (const BdHss *)dh, c->W, mo, U, V, ld, X);

@@ -174,23 +174,23 @@ def near_toeplitz(n, seed):
     return bidiag_toeplitz(n, seed)
 
 
-@pytest.mark.parametrize("gen,n,min_k", [("toeplitz", 1000, 100), ("uniform", 2000, 64),
-                                         ("toeplitz", 2200, 2000)])
-def test_bdsdc_hss_branch_vs_oracle(gen, n, min_k):
+@pytest.mark.parametrize("gen,n,min_k,uplo", [("toeplitz", 1000, 100, "U"), ("uniform", 2000, 64, "U"),
+                                              ("toeplitz", 2200, 2000, "U"), ("toeplitz", 900, 100, "L")])
+def test_bdsdc_hss_branch_vs_oracle(gen, n, min_k, uplo):
     """Merges with K >= hss_min_k take the HSS branch on both sides: the same merges,
     K and generator ranks as the oracle, singular values within 4 n eps, vectors
     orthonormal with residual <= 1e-12 and within 1e-11 of the oracle's."""
     d, e = near_toeplitz(n, 7) if gen == "toeplitz" else bidiag_cfg(n, seed=7)
-    print("case", gen, n, min_k, flush=True)
+    print("case", gen, n, min_k, uplo, flush=True)
     c = hsdc.BdContext(hss_min_k=min_k)
     try:
-        s, u, vt, info = c.bdsdc(d, e)
+        s, u, vt, info = c.bdsdc(d, e, uplo=uplo)
         tr = c.trace()
         st = c.stats()
     finally:
         c.close()
     assert info == 0
-    r = O.bdsdc(d, e, prm=O.params(bd_bsmlz=min_k), trace_cap=4 * n)
+    r = O.bdsdc(d, e, uplo=uplo.encode(), prm=O.params(bd_bsmlz=min_k), trace_cap=4 * n)
     assert r["info"] == 0
     ot = r["trace"]
     assert np.array_equal(tr["k"], ot["k"])
@@ -202,7 +202,7 @@ def test_bdsdc_hss_branch_vs_oracle(gen, n, min_k):
     assert np.all(np.abs(tr["hss_rank"] - ot["rank_max"]) <= 4)
     tol_s = 4 * n * EPS * max(s[0], 1.0)
     assert np.abs(s - r["d"]).max() <= tol_s
-    B = bidiag(d, e)
+    B = bidiag(d, e) if uplo == "U" else bidiag(d, e).T
     assert np.abs(u.T @ u - np.eye(n)).max() <= 1e-12
     assert np.abs(vt @ vt.T - np.eye(n)).max() <= 1e-12
     assert np.abs(u @ np.diag(s) @ vt - B).max() / np.abs(B).max() <= 1e-12

@@ -419,7 +419,12 @@ class Context:
 
 
 class BdContext:
-    """Context of the bidiagonal DC SVD (MDBDSDC, SRC/mdbdsdc.f90); not thread safe."""
+    """Context of the bidiagonal DC SVD (MDBDSDC, SRC/mdbdsdc.f90); not thread safe.
+
+    Parameters (hsdc_bd_params): smlsiz (leaf bound, 25), maxit (secular iteration cap),
+    hss_min_k (a merge with K >= hss_min_k takes the HSS branch, MDLASD8 + DHSSVCS;
+    2000 as the reference's BSMLZ), hss_leaf (HSS leaf block, 40 as the reference's Ni),
+    hss_tol (HSS compression tolerance, 1e-17)."""
 
     def __init__(self, **params):
         self._h = _vp()

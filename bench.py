@@ -205,7 +205,8 @@ def bench_svd(args, np, torch, hsdc):
     # reference's Ni = 40 (the ranks, 60-97, do not shrink with the leaf, so larger
     # leaves mean fewer tree levels of intermediate products); threshold 2000 as in the
     # reference (1024 and 512 measured slower)
-    ctx = hsdc.BdContext(hss_leaf=128) if toep else hsdc.BdContext()
+    hss_leaf = args.hss_leaf or (128 if toep else 40)
+    ctx = hsdc.BdContext(hss_leaf=hss_leaf, hss_min_k=args.hss_min_k)
 
     def step():
         D.copy_(D0)
@@ -237,7 +238,7 @@ def bench_svd(args, np, torch, hsdc):
                 step()
             torch.cuda.synchronize()
             td = (time.perf_counter() - t0) / 2
-            ctx.set_params(hss_min_k=2000)
+            ctx.set_params(hss_min_k=args.hss_min_k)
             step()
             torch.cuda.synchronize()
             out.update(dense_only_s=td, speedup_vs_dense=td / t)
@@ -282,8 +283,8 @@ def bench_svd(args, np, torch, hsdc):
         "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": dict({"workload": ("upper bidiagonal n=%d, d,e = 1+1e-3 N(0,1) seed 11 (near-Toeplitz), "
-                                     "leaf 25, HSS branch for K>=2000 (HSS leaf 128, tol 1e-17), all singular "
-                                     "triplets, device resident" % n) if toep else
+                                     "leaf 25, HSS branch for K>=%d (HSS leaf %d, tol 1e-17), all singular "
+                                     "triplets, device resident" % (n, args.hss_min_k, hss_leaf)) if toep else
                                     ("upper bidiagonal n=%d, d=U(0.5,1.5), e=U(0,1) seed 5, leaf 25, all "
                                      "singular triplets, device resident" % n),
                         "levels": st["levels"], "merges": st["merges"], "roots": st["roots"],
@@ -314,6 +315,10 @@ def main():
     ap.add_argument("--svd-toeplitz", action="store_true",
                     help="with --svd: the near-Toeplitz bidiagonal (HSS branch at the top merges), "
                          "timed against the dense branch alone")
+    ap.add_argument("--hss-leaf", type=int, default=0,
+                    help="with --svd: HSS leaf block (default 128 with --svd-toeplitz, else 40)")
+    ap.add_argument("--hss-min-k", type=int, default=2000,
+                    help="with --svd: a merge with K >= this takes the HSS branch (default 2000)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device three times (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()

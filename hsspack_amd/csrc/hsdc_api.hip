@@ -16,6 +16,8 @@
 #include "hsdc_internal.h"
 #include "hsdc_hss.h"
 #include "hsdc_hss_host.h"
+
+#define HSS_LDS_V 17408  // doubles of dynamic LDS for an RRLU node's generator column
 #include "hsdc_mlp.h"
 #include "../../include/hsdc.h"
 
@@ -57,9 +59,9 @@ __global__ void k_hss_init(const HssMerge *);
 __global__ void k_hss_reset(const HssMerge *);
 __global__ void k_hss_dgen(const HssMerge *, const HssNode *, int);
 template <int NT, bool SQ>
-__global__ void k_hss_rowcomp(const HssMerge *, const HssNode *, int, double);
+__global__ void k_hss_rowcomp(const HssMerge *, const HssNode *, int, double, int);
 template <int NT, bool SQ>
-__global__ void k_hss_colcomp(const HssMerge *, const HssNode *, int, double);
+__global__ void k_hss_colcomp(const HssMerge *, const HssNode *, int, double, int);
 __global__ void k_hss_compact(const HssMerge *, const HssCompact *, const int *, int);
 __global__ void k_hss_bgen(const HssMerge *, const HssNode *, int);
 __global__ void k_bgemm(const BG *);
@@ -528,29 +530,42 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         for (auto &H : hh) kmax = std::max(kmax, H.K);
         hipLaunchKernelGGL(k_hss_init, dim3(std::min(cdiv(kmax, WGS), 256), nh), dim3(WGS), 0, s, dH);
         hipLaunchKernelGGL(k_hss_dgen, dim3(nleaf), dim3(WGS), 0, s, dH, dN + leaf0, (int)nleaf);
+        // the 512-thread RRLU launches (fewer nodes than two per CU) keep each node's
+        // private generator column in LDS when it fits: 160 KB per workgroup on gfx950,
+        // 22.2 KB of it the kernels' static arrays
+        const int ldsn = std::min(kmax, HSS_LDS_V);
+        const size_t ldsb = sizeof(double) * (size_t)ldsn;
         for (int st = 0; st < maxlvl; st++) {
             int nn = (int)step_cnt[st], nc = (int)comp_cnt[st];
             const HssNode *sn = dN + step_off[st];
             const HssCompact *sc = dC + comp_off[st];
             hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
             if (sq && nn < 512)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, true>), dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, true>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
+                                   tol, ldsn);
             else if (sq)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
+                                   0);
             else if (nn < 512)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, false>), dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, false>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
+                                   tol, ldsn);
             else
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
+                                   0);
             hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 1);
             hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
             if (sq && nn < 512)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, true>), dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, true>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
+                                   tol, ldsn);
             else if (sq)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
+                                   0);
             else if (nn < 512)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, false>), dim3(nn), dim3(512), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, false>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
+                                   tol, ldsn);
             else
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
+                                   0);
             hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 0);
             hipLaunchKernelGGL(k_hss_bgen, dim3(nn), dim3(WGS), 0, s, dH, sn, nn);
         }

@@ -132,8 +132,9 @@ __device__ long scratch_take(const HssMerge &H, long n)
 // ---------------------------------------------------------------------------
 template <int HSS_CWG, bool SQ>
 __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, const HssNode *nodes, int nn,
-                                                    double tol)
+                                                    double tol, int lds_cap)
 {
+    extern __shared__ double hss_dyn[];  // lds_cap doubles: V when the node's columns fit
     __shared__ double F[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
     __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
     __shared__ VI red[HSS_CWG / 64];
@@ -164,11 +165,11 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     if (s_flag || s_off < 0) return;
     const int M = s_M, N = s_N, l0 = lr[2 * (node - 1)], c0 = lc[2 * (node - 1)], c1e = lc[2 * (node - 1) + 1];
     const int NS = N > 0 ? N : 1;
-    // private per column: the generator V (updated every step) and the root index; the
-    // poles and differences its accurate differences read are shared by every node of
-    // the merge and gathered by index
-    double *V = H.scratch + s_off;
-    int *PU = (int *)(V + NS);
+    // private per column: the generator V (updated every step; in LDS when it fits) and
+    // the root index; the poles and differences its accurate differences read are
+    // shared by every node of the merge and gathered by index
+    double *V = (N <= lds_cap) ? hss_dyn : H.scratch + s_off;
+    int *PU = (int *)(H.scratch + s_off + NS);
     for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < c0 - 1) ? j + 1 : j + 1 + (c1e - c0 + 1);
         V[j] = A1(H.V, src);
@@ -450,8 +451,9 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
 // ---------------------------------------------------------------------------
 template <int HSS_CWG, bool SQ>
 __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, const HssNode *nodes, int nn,
-                                                    double tol)
+                                                    double tol, int lds_cap)
 {
+    extern __shared__ double hss_dyn[];  // lds_cap doubles: V when the node's rows fit
     __shared__ double Dr[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
     __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
     __shared__ VI red[HSS_CWG / 64];
@@ -475,10 +477,10 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     __syncthreads();
     if (s_flag || s_off < 0) return;
     const int M = s_M, N = s_N, c0 = lc[2 * (node - 1)], r0 = lr[2 * (node - 1)], r1e = lr[2 * (node - 1) + 1];
-    // private per outside row: its generator (updated every step) and pole index; the
-    // pole value is gathered from the merge's shared array
-    double *V = H.scratch + s_off;
-    int *PU = (int *)(V + (N > 0 ? N : 1));
+    // private per outside row: its generator (updated every step; in LDS when it fits)
+    // and pole index; the pole value is gathered from the merge's shared array
+    double *V = (N <= lds_cap) ? hss_dyn : H.scratch + s_off;
+    int *PU = (int *)(H.scratch + s_off + (N > 0 ? N : 1));
     for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < r0 - 1) ? j + 1 : j + 1 + (r1e - r0 + 1);
         V[j] = A1(H.U, src);
@@ -1027,11 +1029,11 @@ __global__ void k_hss_reset(const HssMerge *hm)
 // 256 threads per node when a launch has nodes for every CU twice over, 512 when
 // it has fewer (the upper tree levels): the search over the node's columns is the
 // sequential part of every pivot step
-template __global__ void k_hss_rowcomp<256, false>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_rowcomp<512, false>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_colcomp<256, false>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_colcomp<512, false>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_rowcomp<256, true>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_rowcomp<512, true>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_colcomp<256, true>(const HssMerge *, const HssNode *, int, double);
-template __global__ void k_hss_colcomp<512, true>(const HssMerge *, const HssNode *, int, double);
+template __global__ void k_hss_rowcomp<256, false>(const HssMerge *, const HssNode *, int, double, int);
+template __global__ void k_hss_rowcomp<512, false>(const HssMerge *, const HssNode *, int, double, int);
+template __global__ void k_hss_colcomp<256, false>(const HssMerge *, const HssNode *, int, double, int);
+template __global__ void k_hss_colcomp<512, false>(const HssMerge *, const HssNode *, int, double, int);
+template __global__ void k_hss_rowcomp<256, true>(const HssMerge *, const HssNode *, int, double, int);
+template __global__ void k_hss_rowcomp<512, true>(const HssMerge *, const HssNode *, int, double, int);
+template __global__ void k_hss_colcomp<256, true>(const HssMerge *, const HssNode *, int, double, int);
+template __global__ void k_hss_colcomp<512, true>(const HssMerge *, const HssNode *, int, double, int);

@@ -1259,9 +1259,21 @@ __device__ long block_exscan_l(long v, long *sh, long &total)
     return r;
 }
 
+// An HSS tree over K columns in leaves of about ni has 2 to 512 leaves (the
+// construction's limits; npart, aux_hss.f90:737-760).
+__host__ __device__ inline bool bd_hss_tree_ok(int k, int ni)
+{
+    int l = k / ni;
+    if (l < 1) return false;
+    if (k % ni >= ni / 2) l++;
+    return l >= 2 && l <= 512;
+}
+
 // the level's launch tables from its deflation results: per-merge work counts and
-// their exclusive prefix sums, 1024 merges per pass
-__global__ __launch_bounds__(1024) void k_bd_lvltab(const BdMerge *mg, int nmg, const BdOut *mo, BdTab tab)
+// their exclusive prefix sums, 1024 merges per pass; a merge with K >= hss_min_k (and
+// a valid HSS tree) is flagged for the HSS branch here, on the device
+__global__ __launch_bounds__(1024) void k_bd_lvltab(const BdMerge *mg, int nmg, BdOut *mo, BdTab tab, int hss_min_k,
+                                                    int hss_leaf)
 {
     __shared__ long sh[1024 / 64 + 1];
     long base[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1271,6 +1283,7 @@ __global__ __launch_bounds__(1024) void k_bd_lvltab(const BdMerge *mg, int nmg, 
         if (i < nmg) {
             const BdMerge M = mg[i];
             const int n = M.nl + M.nr + 1, m = n + M.sqre, k = mo[i].k;
+            mo[i].hss = (k >= hss_min_k && bd_hss_tree_ok(k, hss_leaf)) ? 1 : 0;
             if (mo[i].nrot) v[0] = (m + 255) / 256;
             if (k >= 2) {
                 const int T = sec_team(k);
@@ -1551,6 +1564,7 @@ struct hsdc_bd_ctx {
     std::vector<int> tr;   // per merge of the last solve: block start, order, K, rotations, HSS rank
     std::vector<int> hrank;  // per merge: largest HSS generator rank (0: dense branch)
     hsdc_ctx *hctx = nullptr;  // HSS construction and multiply (created on the first HSS merge)
+    hipEvent_t evh = nullptr;  // a level's deflation results are on the host
     Buf hgen, hx, htb;         // HSS generators, gathered U2 / V2, the level's BdHss table
 };
 
@@ -1584,6 +1598,7 @@ int hsdc_bd_ctx_destroy(hsdc_bd_ctx *c)
     Buf *all[] = {&c->ub[0], &c->vb[0], &c->wd, &c->wi, &c->mgb, &c->mob, &c->tab, &c->qb, &c->stg,
                   &c->misc, &c->din, &c->din2, &c->ein, &c->uo, &c->vto, &c->rot, &c->hgen, &c->hx, &c->htb};
     if (c->hctx) hsdc_ctx_destroy(c->hctx);
+    if (c->evh) (void)hipEventDestroy(c->evh);
     for (Buf *b : all)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 4; i++)
@@ -1663,16 +1678,6 @@ static int bd_alloc(hsdc_bd_ctx *c, int n)
 
 // the whole SVD of the upper bidiagonal already on the device (W.D, W.E, scaled when
 // n > smlsiz); outputs U, VT (device, may be null for values only)
-// An HSS tree over K columns in leaves of about ni has 2 to 512 leaves (the
-// construction's limits; npart, aux_hss.f90:737-760).
-static bool bd_hss_tree_ok(int k, int ni)
-{
-    int l = k / ni;
-    if (l < 1) return false;
-    if (k % ni >= ni / 2) l++;
-    return l >= 2 && l <= 512;
-}
-
 // The HSS branch of the merges `hs` of one level (mdlasd1.f90:220-278, DHSSVCS
 // CauchyHssvd_VPS.f90:3587-3804), after their roots and updated z: generators, U2 and
 // V2 gathered in DSIGMA order, the two HSS forms per merge built and applied as
@@ -1932,29 +1937,19 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
         hipLaunchKernelGGL(k_bd_prep, dim3(nm), dim3(PREP_NT), sizeof(double) * (size_t)lds_n, st, mg, W, mo,
                            (const double *)V, ld, lds_n);
         hipLaunchKernelGGL(k_bd_deflate, dim3(nm), dim3(DEF_NT), 0, st, mg, W, mo);
-        // a level with a merge of order >= hss_min_k reads its K back: the HSS branch is
-        // planned on the host (tree shape from K); the others stay queued
-        std::vector<BdHss> hsel;
+        // a level with a merge of order >= hss_min_k reads its deflation results back
+        // (K and the HSS flags k_bd_lvltab sets): the HSS branch is planned on the host
+        // (tree shape from K) while the level's secular and dense kernels run
         bool maybe_hss = false;
         for (const BdMerge &M : lv[t]) maybe_hss |= M.nl + M.nr + 1 >= c->prm.hss_min_k;
+        hipLaunchKernelGGL(k_bd_lvltab, dim3(1), dim3(1024), 0, st, mg, nm, mo, tab, c->prm.hss_min_k,
+                           c->prm.hss_leaf);
+        BdOut *ho = c->hmo + moff[t];
         if (maybe_hss) {
-            BdOut *ho = c->hmo + moff[t];
+            if (!c->evh) BHK(hipEventCreateWithFlags(&c->evh, hipEventDisableTiming));
             BHK(hipMemcpyAsync(ho, mo, sizeof(BdOut) * nm, hipMemcpyDeviceToHost, st));
-            BHK(hipStreamSynchronize(st));
-            long goff = 0, xoff = 0;
-            for (int i = 0; i < nm; i++) {
-                const BdMerge &M = lv[t][i];
-                const int k = ho[i].k, n_ = M.nl + M.nr + 1, m_ = n_ + M.sqre;
-                if (k >= c->prm.hss_min_k && bd_hss_tree_ok(k, c->prm.hss_leaf)) {
-                    ho[i].hss = 1;
-                    hsel.push_back({i, goff, xoff});
-                    goff += 5L * k;
-                    xoff += (long)(n_ + m_) * k;
-                }
-            }
-            if (!hsel.empty()) BHK(hipMemcpyAsync(mo, ho, sizeof(BdOut) * nm, hipMemcpyHostToDevice, st));
+            BHK(hipEventRecord(c->evh, st));
         }
-        hipLaunchKernelGGL(k_bd_lvltab, dim3(1), dim3(1024), 0, st, mg, nm, (const BdOut *)mo, tab);
         hipLaunchKernelGGL(k_bd_rotate, dim3(g_rot[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo, U, V, ld);
         hipLaunchKernelGGL(k_bd_znorm, dim3(nm), dim3(64), 0, st, mg, W, mo);
         hipLaunchKernelGGL(k_bd_secular, dim3(g_sec[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo,
@@ -1963,8 +1958,21 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
         hipLaunchKernelGGL(k_bd_vec, dim3(g_sec[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo, Q);
         hipLaunchKernelGGL(k_bd_gemm, dim3(g_tile[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo,
                            (const double *)Q, (const double *)U, (const double *)V, ld, S);
-        if (!hsel.empty())
-            BCK(bd_hss_level(c, mg, mo, lv[t], c->hmo + moff[t], hsel, tab, U, V, ld, S, c->hrank.data() + moff[t]));
+        if (maybe_hss) {
+            BHK(hipEventSynchronize(c->evh));
+            std::vector<BdHss> hsel;
+            long goff = 0, xoff = 0;
+            for (int i = 0; i < nm; i++) {
+                if (!ho[i].hss) continue;
+                const BdMerge &M = lv[t][i];
+                const int k = ho[i].k, n_ = M.nl + M.nr + 1, m_ = n_ + M.sqre;
+                hsel.push_back({i, goff, xoff});
+                goff += 5L * k;
+                xoff += (long)(n_ + m_) * k;
+            }
+            if (!hsel.empty())
+                BCK(bd_hss_level(c, mg, mo, lv[t], ho, hsel, tab, U, V, ld, S, c->hrank.data() + moff[t]));
+        }
         hipLaunchKernelGGL(k_bd_newcol, dim3(nm), dim3(256), 0, st, mg, tab, W, (const BdOut *)mo, (const double *)U,
                            (const double *)V, ld, S);
         hipLaunchKernelGGL(k_bd_scatter, dim3(g_sc[t]), dim3(256), 0, st, mg, nm, tab, W, (const BdOut *)mo,

@@ -210,6 +210,34 @@ def test_bdsdc_hss_branch_vs_oracle(gen, n, min_k, uplo):
     assert np.abs(vt - r["vt"]).max() <= 1e-11
 
 
+def test_bdsdc_hss_branch_small_trees():
+    """HSS branch with small leaves: threshold 16 and leaf 8 put the merges of this
+    matrix with K from 47 to 106 on the HSS branch (trees of 6 to 13 leaves of 8); both
+    sides take the same merges, and the vectors meet the orthogonality and residual
+    bounds."""
+    n = 400
+    d, e = bidiag_cfg(n, seed=21)
+    print("case small trees", n, flush=True)
+    c = hsdc.BdContext(hss_min_k=16, hss_leaf=8)
+    try:
+        s, u, vt, info = c.bdsdc(d, e)
+        tr = c.trace()
+    finally:
+        c.close()
+    assert info == 0
+    r = O.bdsdc(d, e, prm=O.params(bd_bsmlz=16, bd_ni=8), trace_cap=4 * n)
+    assert r["info"] == 0
+    ot = r["trace"]
+    assert np.array_equal(tr["k"], ot["k"])
+    hs = tr["hss_rank"] > 0
+    assert hs.sum() >= 4 and np.array_equal(hs, ot["rank_max"] > 0)
+    assert np.abs(s - r["d"]).max() <= 4 * n * EPS * s[0]
+    B = bidiag(d, e)
+    assert np.abs(u.T @ u - np.eye(n)).max() <= 1e-12
+    assert np.abs(vt @ vt.T - np.eye(n)).max() <= 1e-12
+    assert np.abs(u @ np.diag(s) @ vt - B).max() / np.abs(B).max() <= 1e-12
+
+
 def test_bdsdc_hss_branch_full_size():
     """The reference threshold (BSMLZ 2000) on a bidiagonal that barely deflates:
     n = 8192, the top merges take the HSS branch; device resident."""

@@ -369,7 +369,7 @@ struct HssPlanM {
 };
 
 static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double tol,
-                     std::vector<HssPlanM> *keep)
+                     std::vector<HssPlanM> *keep, bool left = false)
 {
     hipStream_t s = c->stream;
     std::vector<HssPlanM> P(sp.size());
@@ -610,9 +610,14 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         const HssTree &T = Q.T;
         Q.lg.assign(T.ltr + 1, 0);
         Q.lf.assign(T.ltr + 1, 0);
+        // X * H: G_i has the width of U_i's rank, F_i of the sibling's B columns; for
+        // left (H * X as (X^T H^T)^T) G_i has V_i's rank and F_i its own B's rows
         long it = 0;
         for (int LL = T.lvl; LL >= 1; LL--)
-            for (int nd : T.lev[LL]) { Q.lg[nd] = (int)it; it += Q.usz[2 * (nd - 1) + 1]; }
+            for (int nd : T.lev[LL]) {
+                Q.lg[nd] = (int)it;
+                it += left ? Q.vsz[2 * (nd - 1)] : Q.usz[2 * (nd - 1) + 1];
+            }
         Q.gcols = it;
         it = 0;
         for (int LL = 1; LL <= T.lvl; LL++)
@@ -620,7 +625,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
                 int par = T.tr[nd - 1];
                 int nj = (T.c1[par] == nd) ? T.c2[par] : T.c1[par];
                 Q.lf[nd] = (int)it;
-                it += Q.bsz[2 * (nj - 1) + 1];
+                it += left ? Q.bsz[2 * (nd - 1)] : Q.bsz[2 * (nj - 1) + 1];
             }
         Q.fcols = it;
         gbase[h] = gft;
@@ -651,6 +656,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
                            (const HssMerge *)c->tabs.p, 0, CB + cbase[h]);
         HK(hipStreamSynchronize(s));
         BG g = bg(sp[h].X, sp[h].ldx, CB + cbase[h], K, sp[h].Y, sp[h].ldy, sp[h].nrow, K, K, 0);
+        g.btrans = left ? 1 : 0;  // X^T C^T
         g.cmap = sp[h].ymap;
         g.cmoff = sp[h].ymoff;
         dense_items.push_back(g);
@@ -658,8 +664,86 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
     if (!dense_items.empty()) steps.push_back(dense_items);
     int mlvl = 0;
     for (auto &Q : P) if (Q.ok) mlvl = std::max(mlvl, Q.T.lvl);
+    if (left) {
+        // H X = (X^T H^T)^T, with X^T in sp.X (nrow = the columns of X): the transposed
+        // generators U' = V^T, V' = U^T, node i's B' = B_i^T (H's block (sibling, i) is
+        // U_sib B_sib V_i, so H^T's block (i, sibling) is V_i^T B_sib^T U_sib^T), D' = D^T,
+        // each read through btrans (fastHssmm, ConstructHssd.f90:926-1155)
+        auto bgt = [&](const double *A, long lda, const double *B, long ldb, double *C, long ldc, int nrow, int n,
+                       int k, int beta) {
+            BG g = bg(A, lda, B, ldb, C, ldc, nrow, n, k, beta);
+            g.btrans = 1;
+            return g;
+        };
+        for (int st = 0; st < mlvl; st++) {
+            std::vector<BG> v;
+            for (size_t h = 0; h < P.size(); h++) {
+                HssPlanM &Q = P[h];
+                if (!Q.ok) continue;
+                int LL = Q.T.lvl - st;
+                if (LL < 1) continue;
+                const long nr = sp[h].nrow;
+                double *G = GF + gbase[h];
+                const double *ar = Q.H.arena;
+                for (int nd : Q.T.lev[LL]) {
+                    int vr = Q.vsz[2 * (nd - 1)], vc = Q.vsz[2 * (nd - 1) + 1];
+                    bool leaf = Q.T.c1[nd] == 0;
+                    const double *A = leaf ? sp[h].X + (long)Q.T.leaf_r0[nd] * sp[h].ldx : G + (long)Q.lg[Q.T.c1[nd]] * nr;
+                    long lda = leaf ? sp[h].ldx : nr;
+                    if (vr > 0) v.push_back(bgt(A, lda, ar + Q.pv[nd - 1], vr, G + (long)Q.lg[nd] * nr, nr, (int)nr, vr, vc, 0));
+                }
+            }
+            if (!v.empty()) steps.push_back(v);
+        }
+        for (int LL = 1; LL <= mlvl; LL++) {
+            std::vector<BG> vs, vd;
+            for (size_t h = 0; h < P.size(); h++) {
+                HssPlanM &Q = P[h];
+                if (!Q.ok || LL > Q.T.lvl) continue;
+                const long nr = sp[h].nrow;
+                double *G = GF + gbase[h], *F = GF + fbase[h];
+                const double *ar = Q.H.arena;
+                for (int nd : Q.T.lev[LL]) {
+                    int par = Q.T.tr[nd - 1];
+                    int nj = (Q.T.c1[par] == nd) ? Q.T.c2[par] : Q.T.c1[par];
+                    int br = Q.bsz[2 * (nd - 1)], bc = Q.bsz[2 * (nd - 1) + 1];
+                    if (br > 0)
+                        vs.push_back(bgt(G + (long)Q.lg[nj] * nr, nr, ar + Q.pb[nd - 1], br, F + (long)Q.lf[nd] * nr, nr,
+                                         (int)nr, br, bc, LL >= 2 ? 1 : 0));
+                    if (Q.T.c1[nd] != 0) {
+                        int ur = Q.usz[2 * (nd - 1)], uc = Q.usz[2 * (nd - 1) + 1];
+                        if (ur > 0)
+                            vd.push_back(bgt(F + (long)Q.lf[nd] * nr, nr, ar + Q.pu[nd - 1], ur,
+                                             F + (long)Q.lf[Q.T.c1[nd]] * nr, nr, (int)nr, ur, uc, 0));
+                    }
+                }
+            }
+            if (!vs.empty()) steps.push_back(vs);
+            if (!vd.empty()) steps.push_back(vd);
+        }
+        // leaves: Y_i = F_i U_i^T, then Y_i += X_i D_i^T
+        std::vector<BG> v1, v2;
+        for (size_t h = 0; h < P.size(); h++) {
+            HssPlanM &Q = P[h];
+            if (!Q.ok) continue;
+            const long nr = sp[h].nrow;
+            double *F = GF + fbase[h];
+            const double *ar = Q.H.arena;
+            for (int nd = 1; nd <= Q.T.ltr - 1; nd++) {
+                if (Q.T.c1[nd] != 0) continue;
+                int ur = Q.usz[2 * (nd - 1)], uc = Q.usz[2 * (nd - 1) + 1], md = Q.dsz[nd - 1];
+                long lx = Q.T.leaf_r0[nd];
+                v1.push_back(bgt(F + (long)Q.lf[nd] * nr, nr, ar + Q.pu[nd - 1], std::max(ur, 1),
+                                 sp[h].Y + lx * sp[h].ldy, sp[h].ldy, (int)nr, md, uc, 0));
+                v2.push_back(bgt(sp[h].X + lx * sp[h].ldx, sp[h].ldx, Q.H.dd + Q.pd[nd - 1], md,
+                                 sp[h].Y + lx * sp[h].ldy, sp[h].ldy, (int)nr, md, md, 1));
+            }
+        }
+        if (!v1.empty()) steps.push_back(v1);
+        if (!v2.empty()) steps.push_back(v2);
+    }
     // upsweep
-    for (int st = 0; st < mlvl; st++) {
+    for (int st = 0; st < mlvl && !left; st++) {
         std::vector<BG> v;
         for (size_t h = 0; h < P.size(); h++) {
             HssPlanM &Q = P[h];
@@ -680,7 +764,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         if (!v.empty()) steps.push_back(v);
     }
     // siblings and downsweep, level by level from the root
-    for (int LL = 1; LL <= mlvl; LL++) {
+    for (int LL = 1; LL <= mlvl && !left; LL++) {
         std::vector<BG> vs, vd;
         for (size_t h = 0; h < P.size(); h++) {
             HssPlanM &Q = P[h];
@@ -707,7 +791,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         if (!vd.empty()) steps.push_back(vd);
     }
     // leaves: Y_i = F_i V_i + X_i D_i, both products in one GEMM item
-    {
+    if (!left) {
         std::vector<BG> v1;
         for (size_t h = 0; h < P.size(); h++) {
             HssPlanM &Q = P[h];
@@ -2016,13 +2100,16 @@ int hsdc_loewner(hsdc_ctx *c, int k, double rho, double *dlamda, double *z, doub
     return info ? 1 : 0;
 }
 
+// q (nrow x k) := q * H, or for left q (k x nrow) := H * q, computed as the transposed
+// product on a host-transposed copy
 static int hss_one(hsdc_ctx *c, int sq, int k, int ni, double tol, const double *f, const double *d, const double *u,
-                   const double *v, const double *difl, const double *difr, double *q, int ldq, int nrow, int *ranks)
+                   const double *v, const double *difl, const double *difr, double *q, int ldq, int nrow, int *ranks,
+                   bool left = false)
 {
     if (!c) return HSDC_ERR_INTERNAL;
     if (k < 1) return -2;
     if (ni < 8) return -3;
-    if (ldq < std::max(nrow, 1)) return -12;
+    if (ldq < std::max(left ? k : nrow, 1)) return -12;
     CK(alloc_work(c, k));
     Work W = c->W;
     hipStream_t s = c->stream;
@@ -2034,8 +2121,16 @@ static int hss_one(hsdc_ctx *c, int sq, int k, int ni, double tol, const double 
     c->evc_k = k; c->evc_ni = ni; c->evc_tol = tol; c->evc_sq = sq;
     CK(ensure(c->xb, sizeof(double) * (size_t)std::max(nrow, 1) * k));
     CK(ensure(c->yb, sizeof(double) * (size_t)std::max(nrow, 1) * k));
-    HK(hipMemcpy2D(c->xb.p, sizeof(double) * nrow, q, sizeof(double) * ldq, sizeof(double) * nrow, k,
-                   hipMemcpyHostToDevice));
+    std::vector<double> qt;
+    if (left) {
+        qt.resize((size_t)std::max(nrow, 1) * k);
+        for (int cc = 0; cc < nrow; cc++)
+            for (int r = 0; r < k; r++) qt[(size_t)r * nrow + cc] = q[(size_t)cc * ldq + r];
+        HK(hipMemcpy(c->xb.p, qt.data(), sizeof(double) * (size_t)nrow * k, hipMemcpyHostToDevice));
+    } else {
+        HK(hipMemcpy2D(c->xb.p, sizeof(double) * nrow, q, sizeof(double) * ldq, sizeof(double) * nrow, k,
+                       hipMemcpyHostToDevice));
+    }
     HssSpec sp;
     sp.K = k;
     sp.FF = W.dlamda; sp.D0 = W.lam; sp.U0 = W.wnew; sp.V0 = W.alpha; sp.DIFL = W.difl; sp.DIFR = W.difr;
@@ -2043,9 +2138,15 @@ static int hss_one(hsdc_ctx *c, int sq, int k, int ni, double tol, const double 
     sp.sq = sq;
     std::vector<HssSpec> v1(1, sp);
     std::vector<HssPlanM> keep;
-    CK(hss_apply(c, v1, ni, tol, &keep));
-    HK(hipMemcpy2D(q, sizeof(double) * ldq, c->yb.p, sizeof(double) * nrow, sizeof(double) * nrow, k,
-                   hipMemcpyDeviceToHost));
+    CK(hss_apply(c, v1, ni, tol, &keep, left));
+    if (left) {
+        HK(hipMemcpy(qt.data(), c->yb.p, sizeof(double) * (size_t)nrow * k, hipMemcpyDeviceToHost));
+        for (int cc = 0; cc < nrow; cc++)
+            for (int r = 0; r < k; r++) q[(size_t)cc * ldq + r] = qt[(size_t)r * nrow + cc];
+    } else {
+        HK(hipMemcpy2D(q, sizeof(double) * ldq, c->yb.p, sizeof(double) * nrow, sizeof(double) * nrow, k,
+                       hipMemcpyDeviceToHost));
+    }
     if (ranks) {
         const HssPlanM &Q = keep[0];
         if (!Q.ok) {
@@ -2070,6 +2171,20 @@ int hsdc_hss_svc(hsdc_ctx *c, int k, int ni, double tol, const double *f, const 
                  const double *v, const double *difl, const double *difr, double *q, int ldq, int nrow, int *ranks)
 {
     return hss_one(c, 1, k, ni, tol, f, d, u, v, difl, difr, q, ldq, nrow, ranks);
+}
+
+int hsdc_hss_cmm(hsdc_ctx *c, char side, int squared, int k, int ni, double tol, const double *f, const double *d,
+                 const double *u, const double *v, const double *difl, const double *difr, double *x, int ldx,
+                 int ncol, int *ranks)
+{
+    const bool left = side == 'L' || side == 'l';
+    if (!left && side != 'R' && side != 'r') return -2;
+    if (squared != 0 && squared != 1) return -3;
+    if (ncol < 0) return -15;
+    if (ncol == 0) return 0;
+    int rc = hss_one(c, squared, k, ni, tol, f, d, u, v, difl, difr, x, ldx, ncol, ranks, left);
+    // hss_one numbers k, ni, ldx from 2 on; here two arguments come first
+    return rc < 0 && rc > HSDC_ERR_HIP ? rc - 2 : rc;
 }
 
 

@@ -28,7 +28,7 @@ HSDC_ERR_NODEVICE = -1002
 EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_set_params",
            "hsdc_get_stats", "hsdc_set_stream", "hsdc_stedc", "hsdc_stedc_dev",
            "hsdc_stedc_batched_dev", "hsdc_autotune", "hsdc_laed0", "hsdc_deflate", "hsdc_secular", "hsdc_laed3", "hsdc_loewner",
-           "hsdc_hss_evc", "hsdc_hss_svc", "hsdc_hss_dense", "hsdc_mlp_set_weights", "hsdc_mlp_get_weights",
+           "hsdc_hss_evc", "hsdc_hss_svc", "hsdc_hss_cmm", "hsdc_hss_dense", "hsdc_mlp_set_weights", "hsdc_mlp_get_weights",
            "hsdc_mlp_load", "hsdc_mlp_harvest", "hsdc_mlp_samples", "hsdc_mlp_grad", "hsdc_mlp_train",
            "hsdc_bd_default_params", "hsdc_bd_ctx_create", "hsdc_bd_ctx_destroy", "hsdc_bd_set_params",
            "hsdc_bd_get_stats", "hsdc_bd_set_stream", "hsdc_bdsdc", "hsdc_bdsdc_dev", "hsdc_bd_trace"]
@@ -107,6 +107,8 @@ def lib():
                                    _P(_d), _i, _i, _P(_i)]
         L.hsdc_hss_svc.argtypes = [_vp, _i, _i, _d, _P(_d), _P(_d), _P(_d), _P(_d), _P(_d), _P(_d),
                                    _P(_d), _i, _i, _P(_i)]
+        L.hsdc_hss_cmm.argtypes = [_vp, ctypes.c_char, _i, _i, _i, _d, _P(_d), _P(_d), _P(_d), _P(_d), _P(_d),
+                                   _P(_d), _P(_d), _i, _i, _P(_i)]
         L.hsdc_hss_dense.argtypes = [_vp, _P(_d), _i]
         _f = _P(ctypes.c_float)
         _ll = ctypes.c_longlong
@@ -338,6 +340,20 @@ class Context:
     def hss_svc(self, f, d, u, v, difl, difr, q, ni, tol):
         """As hss_evc for H(i,j) = u_i v_j / (f_i^2 - d_j^2) (the SVD's vector matrices)."""
         return self._hss(lib().hsdc_hss_svc, "hsdc_hss_svc", f, d, u, v, difl, difr, q, ni, tol)
+
+    def hss_cmm(self, side, squared, f, d, u, v, difl, difr, x, ni, tol):
+        """CauchyHssMM: side 'R' x (ncol x k) := x * H, side 'L' x (k x ncol) := H * x for the
+        HSS form of the Cauchy-like matrix (squared 1: the SVD's u_i v_j / (f_i^2 - d_j^2))."""
+        k = len(f)
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (f, d, u, v, difl, difr)]
+        x = np.array(x, np.float64, order="F")
+        ncol = x.shape[0] if side == "R" else x.shape[1]
+        ranks = np.zeros(2 * (2 * (k // max(ni, 1)) + 4), np.int32)
+        info = lib().hsdc_hss_cmm(self._h, ctypes.c_char(side.encode()), int(squared), k, ni, tol,
+                                  *[_dp(a) for a in arrs], _dp(x), max(x.shape[0], 1), ncol, _ip(ranks))
+        if info != 0:
+            raise RuntimeError("hsdc_hss_cmm: %d" % info)
+        return x, ranks
 
     def _hss(self, fn, name, f, d, u, v, difl, difr, q, ni, tol):
         k = len(f)

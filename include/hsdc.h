@@ -151,6 +151,15 @@ int hsdc_hss_svc(hsdc_ctx *ctx, int k, int ni, double tol, const double *f, cons
                  const double *u, const double *v, const double *difl, const double *difr,
                  double *q, int ldq, int nrow, int *ranks);
 
+/* The standalone construct-and-multiply of the Cauchy-like matrices (CauchyHssMM
+ * cauchyhssmm.f90:1-123, DHSSMM / DHSSMML with fastHssmm ConstructHssd.f90:926-1155 and
+ * fastHssmmL :1394-1630): builds the HSS form of C (squared = 0: as hsdc_hss_evc;
+ * 1: as hsdc_hss_svc), then side 'R': x (ncol x k, ldx) := x * C, or side 'L':
+ * x (k x ncol, ldx) := C * x.  ranks as for hsdc_hss_evc. */
+int hsdc_hss_cmm(hsdc_ctx *ctx, char side, int squared, int k, int ni, double tol, const double *f,
+                 const double *d, const double *u, const double *v, const double *difl,
+                 const double *difr, double *x, int ldx, int ncol, int *ranks);
+
 /* Dense expansion of the HSS form built by the last hsdc_hss_evc / hsdc_hss_svc call
  * (k x k). */
 int hsdc_hss_dense(hsdc_ctx *ctx, double *a, int lda);

@@ -94,3 +94,20 @@ def test_bdsdc_argument_errors_without_device():
     p.smlsiz = 2
     assert L.hsdc_bd_set_params(ctx._h, ctypes.byref(p)) == -2
     ctx.close()
+
+
+def test_hss_cmm_argument_errors_without_device():
+    """hsdc_hss_cmm rejects a bad side, mode or column count before any device work."""
+    L = hsdc.lib()
+    ctx = hsdc.Context()
+    k = 4
+    a = np.ones(k)
+    x = np.zeros((k, 2), order="F")
+    dp = lambda v: v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    args = lambda side, sq, ncol: (ctx._h, ctypes.c_char(side), sq, k, 8, 1e-17, dp(a), dp(a), dp(a), dp(a), dp(a),
+                                   dp(a), dp(x), k, ncol, None)
+    assert L.hsdc_hss_cmm(*args(b"X", 0, 2)) == -2
+    assert L.hsdc_hss_cmm(*args(b"L", 2, 2)) == -3
+    assert L.hsdc_hss_cmm(*args(b"L", 0, -1)) == -15
+    assert L.hsdc_hss_cmm(*args(b"R", 1, 0)) == 0
+    ctx.close()

@@ -117,3 +117,36 @@ def test_hss_squared_ranks_and_product_vs_oracle(ctx, K, ni, toeplitz):
         assert np.abs(Hd - Hd_ref).max() <= 64 * K ** 0.5 * EPS * scale
         assert np.abs(Hd - C).max() <= 64 * K ** 0.5 * EPS * scale
         assert np.abs(y - x @ Hd_ref).max() <= 1e-12 * np.abs(x).max() * K ** 0.5 * scale
+
+
+@pytest.mark.parametrize("squared,K,ni", [(0, 1000, 64), (0, 1300, 200), (1, 640, 40)])
+def test_hss_left_and_right_products_vs_oracle(ctx, squared, K, ni):
+    """CauchyHssMM / DHSSMM: C * x (fastHssmm, the transposed generators) and x * C
+    (fastHssmmL) against the oracle's dense expansion of the same HSS form."""
+    tol = float(np.float32(1e-17))
+    if squared:
+        rng = np.random.default_rng(K)
+        f = np.concatenate([[0.0], np.sort(rng.uniform(0.05, 1.0, K - 1))])
+        z = rng.uniform(0.1, 1.0, K) * rng.choice([-1, 1], K)
+        o = O.lasd8(f, z)
+        d, u, v, difl, difr = o["sigma"], o["z"], o["aphav"], o["difl"], o["difr"]
+        H = O.hss_build_sq(f, d, u, v, difl, difr, ni, tol)
+    else:
+        dd, z, rho = _secular_case(K, K + 3)
+        o = O.mdlaed3_vectors(dd, z, rho)
+        f, d, u, v, difl, difr = o["dlamda"], o["d"], o["z"], o["alpha"], o["difl"], o["difr"]
+        H = O.hss_build(f, d, u, v, difl, difr, ni, tol)
+    try:
+        Hd_ref = O.hss_dense(H)
+    finally:
+        O.hss_free(H)
+    print("case squared", squared, "K", K, "ni", ni, flush=True)
+    rng = np.random.default_rng(5)
+    xr = rng.standard_normal((131, K))
+    xl = rng.standard_normal((K, 77))
+    yr, ranks = ctx.hss_cmm("R", squared, f, d, u, v, difl, difr, xr, ni, tol)
+    yl, _ = ctx.hss_cmm("L", squared, f, d, u, v, difl, difr, xl, ni, tol)
+    assert ranks[0] != -1
+    scale = np.abs(Hd_ref).max()
+    assert np.abs(yr - xr @ Hd_ref).max() <= 1e-12 * K ** 0.5 * scale * np.abs(xr).max()
+    assert np.abs(yl - Hd_ref @ xl).max() <= 1e-12 * K ** 0.5 * scale * np.abs(xl).max()

@@ -42,7 +42,7 @@ __global__ void k_gemm(const Merge *, const GTile *, int, const int *, int, cons
 __global__ void k_newcols(const Merge *, const int *, int, const MergeOut *, Work, int);
 __global__ void k_scatter(const Merge *, const int *, int, const MergeOut *, const long *, const Prob *,
                           Work, int, const double *, int);
-__global__ void k_lamrg(const Merge *, const int *, int, const MergeOut *, Work, int);
+__global__ void k_lamrg(const Merge *, const int *, int, const MergeOut *, Work, int, int *);
 __global__ void k_readback(Work, const int *, const unsigned long long *, const SecCtl *, const unsigned int *,
                            SolveRec *);
 __global__ void k_inp_scan(const double *, const double *, int, Work);
@@ -175,6 +175,14 @@ struct hsdc_ctx {
     hipEvent_t zev0 = nullptr, zev1 = nullptr; // its fork and join
     hipEvent_t pz0 = nullptr, pz1 = nullptr;   // around it (profile = 1)
     SolveRec *rec = nullptr, *drec = nullptr;  // end-of-solve record: mapped host memory, its device address
+    // per-merge trace of the last solve: K, rotations and branch written by k_lamrg
+    // into dtrc (3 ints per merge, level by level), the merges and their HSS ranks
+    // kept by the host; valid until a single-merge call reuses the tables
+    int *dtrc = nullptr;
+    std::vector<Merge> trc_mg;
+    std::vector<int> trc_rank;
+    int trc_ntot = 0;
+    bool trc_valid = false;
 };
 
 
@@ -863,7 +871,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
 // eigenvector slots, then X * H written straight into the slots of the merge's new
 // vectors (vm[cur ^ 1], which k_newcols has set).
 static int run_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, int m0,
-                   const std::vector<MergeOut> &ho, const std::vector<int> &sel, Prob *dprobs, int cur)
+                   const std::vector<MergeOut> &ho, const std::vector<int> &sel, Prob *dprobs, int cur, int gbase)
 {
     hipStream_t s = c->stream;
     Work W = c->W;
@@ -907,13 +915,25 @@ static int run_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, 
         q.Y = PP.q + (M.s - PP.off); q.ldy = PP.ld;
         q.ymap = W.vm[cur ^ 1] + M.s; q.ymoff = PP.off;
     }
-    return hss_apply(c, sp, c->prm.hss_leaf, c->prm.hss_tol, nullptr);
+    std::vector<HssPlanM> kept;
+    CK(hss_apply(c, sp, c->prm.hss_leaf, c->prm.hss_tol, &kept));
+    // largest generator rank of each merge into the trace (0: dense Cauchy product)
+    for (size_t h = 0; h < sel.size(); h++) {
+        const size_t gm = (size_t)gbase + m0 + sel[h];
+        if (gm >= c->trc_rank.size()) continue;
+        int r = 0;
+        const HssPlanM &Q = kept[h];
+        if (Q.ok)
+            for (int i = 0; i < Q.T.ltr; i++) r = std::max(r, std::max(Q.usz[2 * i + 1], Q.vsz[2 * i]));
+        c->trc_rank[gm] = r;
+    }
+    return 0;
 }
 
 // one level (or one merge of it): merges [m0, m1) of table dm (device) / hm (host);
 // dchunkN is the device chunk table of exactly this slice (first entry 0)
 static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm, int m0, int m1,
-                      const int *dchunkN, int totN, Prob *dprobs, int &cur, int *dinfo)
+                      const int *dchunkN, int totN, Prob *dprobs, int &cur, int *dinfo, int gbase)
 {
     const int nm = m1 - m0;
     hipStream_t s = c->stream;
@@ -1020,13 +1040,14 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
             if (!ho[i].flags && ho[i].hss && ho[i].k > 0) sel.push_back(i);
         if (!sel.empty()) {  // after k_newcols: the products go into the slots it assigned
             hipLaunchKernelGGL(k_alpha, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, W);
-            CK(run_hss(c, dmg, hm, m0, ho, sel, dprobs, cur));
+            CK(run_hss(c, dmg, hm, m0, ho, sel, dprobs, cur, gbase));
         }
     }
     if (anyk)
         hipLaunchKernelGGL(k_scatter, dim3(colK[nm]), dim3(WGS), 0, s, dmg, dcolK, nm, dmo, dcoff, dprobs, W,
                            cur ^ 1, (const double *)c->ct.p, 0);
-    hipLaunchKernelGGL(k_lamrg, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur ^ 1);
+    hipLaunchKernelGGL(k_lamrg, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur ^ 1,
+                       c->dtrc ? c->dtrc + 3 * ((long)gbase + m0) : nullptr);
     HK(hipGetLastError());
     cur ^= 1;
     return 0;
@@ -1082,7 +1103,7 @@ static size_t level_tables_bytes(int nm) { return 5 * al64(sizeof(int) * (nm + 1
 
 // Queue one level (merges of table dmg) on the device schedule.
 static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, int L, const int *dchunkN,
-                     int totN, const LvlBound &B, Prob *dprobs, int cur, int *dinfo)
+                     int totN, const LvlBound &B, Prob *dprobs, int cur, int *dinfo, int gbase)
 {
     const int nm = (int)hm.size();
     hipStream_t s = c->stream;
@@ -1122,7 +1143,8 @@ static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
     hipLaunchKernelGGL(k_newcols_copy, dim3(totN + (B.scat > 0 ? dev_grid(B.scat) : 0)), dim3(WGS), 0, s, dmg,
                        dchunkN, nm, totN, (const int *)t.scK, (const MergeOut *)dmo, (const long *)t.coff, dprobs, W,
                        cur, (const double *)c->ct.p);
-    hipLaunchKernelGGL(k_lamrg, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur ^ 1);
+    hipLaunchKernelGGL(k_lamrg, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, (const MergeOut *)dmo, W, cur ^ 1,
+                       c->dtrc ? c->dtrc + 3 * (long)gbase : nullptr);
     HK(hipGetLastError());
     return 0;
 }
@@ -1131,7 +1153,7 @@ static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
 // reads the level's results, builds and applies their HSS matrices and copies the
 // vectors into place, then opens the gate.
 static int level_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm, int totN, const LvlBound &B,
-                     Prob *dprobs, int cur)
+                     Prob *dprobs, int cur, int gbase)
 {
     const int nm = (int)hm.size();
     hipStream_t s = c->stream;
@@ -1151,7 +1173,7 @@ static int level_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
     Work W = c->W;
     hipLaunchKernelGGL(k_alpha, dim3(totN), dim3(WGS), 0, s, dmg, (const int *)t.chunkK, nm, (const MergeOut *)dmo,
                        W);
-    CK(run_hss(c, dmg, hm, 0, ho, sel, dprobs, cur));  // writes the new vectors into their slots
+    CK(run_hss(c, dmg, hm, 0, ho, sel, dprobs, cur, gbase));  // writes the new vectors into their slots
     hipLaunchKernelGGL(k_scatter, dim3(dev_grid(B.cols)), dim3(WGS), 0, s, dmg, (const int *)t.colK, nm,
                        (const MergeOut *)dmo, (const long *)t.coff, dprobs, W, cur ^ 1, (const double *)c->ct.p, 2);
     HK(hipGetLastError());
@@ -1241,7 +1263,8 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     const long fin_blocks = zglobal ? (long)ntot * fin_cz : (long)fbs[np];
     size_t a12 = al64(a11 + sizeof(int) * chunks.size()), a13 = al64(a12 + sizeof(int) * (np + 1));
     size_t a14 = dev_in ? al64(a13 + sizeof(int) * ntot) : a13;  // tear flags of the device input
-    CK(ensure(c->probs, a14));
+    const size_t a15 = al64(a14 + sizeof(int) * 3 * std::max(nmtot, (size_t)1));  // per-merge trace
+    CK(ensure(c->probs, a15));
     std::vector<long> tkey;
     if (dev_in)
         tkey = {(long)ntot, (long)nl, (long)nmtot, (long)np, (long)ldz[0], (long)zglobal, c->prm.smlsiz,
@@ -1293,6 +1316,12 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     c->tabkey = tkey;  // empty (nothing kept) for host input
     const int *dfbs = (const int *)(dp + a12);
     const int *dtflag = (const int *)(dp + a13);
+    c->dtrc = (int *)(dp + a14);
+    c->trc_mg.clear();
+    for (auto &Lm : pl.lv) c->trc_mg.insert(c->trc_mg.end(), Lm.begin(), Lm.end());
+    c->trc_rank.assign(nmtot, 0);
+    c->trc_ntot = ntot;
+    c->trc_valid = true;
     const int *dchunks = (const int *)(dp + a11);
     Prob *dprobs = (Prob *)(dp + a0);
     Leaf *dleaves = (Leaf *)(dp + a1);
@@ -1385,7 +1414,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         c->cur_level = L;
         const int nm = (int)pl.lv[L].size();
         return level_dev(c, dmerges + lvoff[L], pl.lv[L], L, dchunks + lvchunk[L], chunks[lvchunk[L] + nm], lb[L],
-                         dprobs, L & 1, dinfo);
+                         dprobs, L & 1, dinfo, (int)lvoff[L]);
     };
     // With every level on the device schedule the whole solve is one HIP graph,
     // captured once per shape (plan, buffers, parameters) and replayed.
@@ -1478,7 +1507,8 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
             if (g) {  // level g - 1 holds HSS merges; the levels above it did nothing
                 const int Lh = g - 1, nm = (int)pl.lv[Lh].size();
                 for (int l = L; l <= Lh; l++) ndev += (int)pl.lv[l].size();
-                CK(level_hss(c, dmerges + lvoff[Lh], pl.lv[Lh], chunks[lvchunk[Lh] + nm], lb[Lh], dprobs, Lh & 1));
+                CK(level_hss(c, dmerges + lvoff[Lh], pl.lv[Lh], chunks[lvchunk[Lh] + nm], lb[Lh], dprobs, Lh & 1,
+                             (int)lvoff[Lh]));
                 L = Lh + 1;
                 continue;
             }
@@ -1492,14 +1522,15 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         c->cur_level = L1;
         int cur = L1 & 1;
         if (c->prm.batched) {
-            CK(run_merges(c, dm, hm, 0, nm, dchunks + lvchunk[L1], chunks[lvchunk[L1] + nm], dprobs, cur, dinfo));
+            CK(run_merges(c, dm, hm, 0, nm, dchunks + lvchunk[L1], chunks[lvchunk[L1] + nm], dprobs, cur, dinfo,
+                          (int)lvoff[L1]));
         } else {
             // one merge at a time, every merge keeping the same ping-pong parity
             for (int m = 0; m < nm; m++) {
                 int cc = L1 & 1;
                 const size_t mi = lvoff[L1] + m;
                 CK(run_merges(c, dm, hm, m, m + 1, dchunks + mchunk[mi], chunks[mchunk[mi] + 1], dprobs, cc,
-                              dinfo));
+                              dinfo, (int)lvoff[L1]));
             }
         }
         L = L1 + 1;
@@ -1665,6 +1696,38 @@ int hsdc_get_stats(hsdc_ctx *c, hsdc_stats *s)
 {
     if (!c || !s) return -1;
     *s = c->st;
+    return 0;
+}
+
+int hsdc_trace(hsdc_ctx *c, int *s, int *n, int *k, int *nrot, int *hss, int *hss_rank, int cap)
+{
+    if (!c) return -1;
+    if (cap < 0) return -8;
+    if (!c->trc_valid || !c->dtrc) return 0;
+    const int nm = (int)c->trc_mg.size();
+    std::vector<int> t(3 * (size_t)std::max(nm, 1));
+    if (nm > 0) {
+        HK(hipMemcpyAsync(t.data(), c->dtrc, sizeof(int) * 3 * nm, hipMemcpyDeviceToHost, c->stream));
+        HK(hipStreamSynchronize(c->stream));
+    }
+    for (int i = 0; i < nm && i < cap; i++) {
+        if (s) s[i] = c->trc_mg[i].s;
+        if (n) n[i] = c->trc_mg[i].n;
+        if (k) k[i] = t[3 * i];
+        if (nrot) nrot[i] = t[3 * i + 1];
+        if (hss) hss[i] = t[3 * i + 2];
+        if (hss_rank) hss_rank[i] = c->trc_rank[i];
+    }
+    return nm;
+}
+
+int hsdc_trace_indxq(hsdc_ctx *c, int *indxq, int n)
+{
+    if (!c) return -1;
+    if (!indxq) return -2;
+    if (!c->trc_valid || n != c->trc_ntot || n < 1) return -3;
+    HK(hipMemcpyAsync(indxq, c->W.indxq, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    HK(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -1871,6 +1934,8 @@ static int single_merge_setup(hsdc_ctx *c, int n, int n1, double rho, Merge **dm
     gate_reset(c);
     CK(ensure(c->probs, 128));
     c->tabkey.clear();  // the kept tables of a device-input solve are overwritten
+    c->trc_valid = false;  // and the trace of the last solve with them
+    c->dtrc = nullptr;
     Merge M;
     M.s = 0; M.n = n; M.n1 = n1; M.prob = 0; M.lvl = 0; M.rho = rho;
     int ch[2] = {0, cdiv(n, WGS)};

@@ -1677,14 +1677,20 @@ __global__ __launch_bounds__(WG) void k_newcols_copy(const Merge *mg, const int 
 // sequential loop in the first thread of its first chunk, so the permutation is
 // DLAMRG's in every case.
 __global__ __launch_bounds__(WG) void k_lamrg(const Merge *mg, const int *chunk_start, int nm, const MergeOut *mo,
-                                              Work W, int nxt)
+                                              Work W, int nxt, int *trc)
 {
     if (gate_closed(W, mg)) return;
     const int m = find_seg(chunk_start, nm, blockIdx.x);
     const MergeOut o = mo[m];
+    const int c = blockIdx.x - chunk_start[m];
+    if (trc && c == 0 && threadIdx.x == 0) {  // the merge's trace entry: K, rotations, branch
+        trc[3 * m] = o.flags ? 0 : o.k;
+        trc[3 * m + 1] = o.flags ? 0 : o.nrot;
+        trc[3 * m + 2] = (!o.flags && o.hss) ? 1 : 0;
+    }
     if (o.flags) return;  // identity, set by k_newcols
     const Merge M = mg[m];
-    const int n = M.n, n1 = o.k, n2 = M.n - o.k, c = blockIdx.x - chunk_start[m];
+    const int n = M.n, n1 = o.k, n2 = M.n - o.k;
     const double *a = W.Dv[nxt] + M.s;
     int *out = W.indxq + M.s;
     if (W.maux[4 * m + 3]) {

@@ -31,7 +31,8 @@ EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_s
            "hsdc_hss_evc", "hsdc_hss_svc", "hsdc_hss_cmm", "hsdc_hss_dense", "hsdc_mlp_set_weights", "hsdc_mlp_get_weights",
            "hsdc_mlp_load", "hsdc_mlp_harvest", "hsdc_mlp_samples", "hsdc_mlp_grad", "hsdc_mlp_train",
            "hsdc_bd_default_params", "hsdc_bd_ctx_create", "hsdc_bd_ctx_destroy", "hsdc_bd_set_params",
-           "hsdc_bd_get_stats", "hsdc_bd_set_stream", "hsdc_bdsdc", "hsdc_bdsdc_dev", "hsdc_bd_trace"]
+           "hsdc_bd_get_stats", "hsdc_bd_set_stream", "hsdc_bdsdc", "hsdc_bdsdc_dev", "hsdc_bd_trace",
+           "hsdc_trace", "hsdc_trace_indxq"]
 
 # learned secular start: 34 inputs -> 64 -> 64 -> 1
 MLP_IN, MLP_H = 34, 64
@@ -128,6 +129,8 @@ def lib():
                                  _P(_d), _i, _P(_d), _P(_i), _P(_d), _P(_i)]
         L.hsdc_bdsdc_dev.argtypes = [_vp, ctypes.c_char, _i, _vp, _vp, _vp, _i, _vp, _i]
         L.hsdc_bd_trace.argtypes = [_vp, _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _i]
+        L.hsdc_trace.argtypes = [_vp, _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _i]
+        L.hsdc_trace_indxq.argtypes = [_vp, _P(_i), _i]
         _LIB = L
     return _LIB
 
@@ -223,6 +226,24 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def trace(self):
+        """Merges of the last solve (level by level, MDLAED0's order): dict of arrays s, n,
+        k, nrot, hss (1: HSS branch), hss_rank (largest generator rank, 0 dense)."""
+        cnt = lib().hsdc_trace(self._h, None, None, None, None, None, None, 0)
+        if cnt < 0:
+            raise RuntimeError("hsdc_trace: %d" % cnt)
+        a = [np.zeros(max(cnt, 1), np.int32) for _ in range(6)]
+        lib().hsdc_trace(self._h, *[_ip(x) for x in a], cnt)
+        return dict(zip(("s", "n", "k", "nrot", "hss", "hss_rank"), (x[:cnt] for x in a)))
+
+    def trace_indxq(self, n):
+        """INDXQ (1-based) after the top merge of the last single-problem solve of order n."""
+        q = np.zeros(n, np.int32)
+        rc = lib().hsdc_trace_indxq(self._h, _ip(q), n)
+        if rc != 0:
+            raise RuntimeError("hsdc_trace_indxq: %d" % rc)
+        return q
 
     # ---- MDSTEDC (host arrays) ----
     def stedc(self, d, e, compz="I", z=None, lwork=None, liwork=None):
@@ -345,8 +366,14 @@ class Context:
         """CauchyHssMM: side 'R' x (ncol x k) := x * H, side 'L' x (k x ncol) := H * x for the
         HSS form of the Cauchy-like matrix (squared 1: the SVD's u_i v_j / (f_i^2 - d_j^2))."""
         k = len(f)
+        side = side.upper()
         arrs = [np.ascontiguousarray(a, np.float64) for a in (f, d, u, v, difl, difr)]
         x = np.array(x, np.float64, order="F")
+        if x.ndim != 2 or side not in ("R", "L"):
+            raise ValueError("x must be 2-D and side 'R' or 'L'")
+        if (side == "R" and x.shape[1] != k) or (side == "L" and x.shape[0] != k):
+            raise ValueError("x has shape %s; side %s needs %s k = %d" % (x.shape, side,
+                             "columns" if side == "R" else "rows", k))
         ncol = x.shape[0] if side == "R" else x.shape[1]
         ranks = np.zeros(2 * (2 * (k // max(ni, 1)) + 4), np.int32)
         info = lib().hsdc_hss_cmm(self._h, ctypes.c_char(side.encode()), int(squared), k, ni, tol,
@@ -359,6 +386,8 @@ class Context:
         k = len(f)
         arrs = [np.ascontiguousarray(a, np.float64) for a in (f, d, u, v, difl, difr)]
         q = np.array(q, np.float64, order="F")
+        if q.ndim != 2 or q.shape[1] != k:
+            raise ValueError("q has shape %s; it needs k = %d columns" % (q.shape, k))
         nrow = q.shape[0]
         ranks = np.zeros(2 * (2 * (k // max(ni, 1)) + 4), np.int32)
         info = fn(self._h, k, ni, tol, *[_dp(a) for a in arrs], _dp(q), max(nrow, 1), nrow, _ip(ranks))

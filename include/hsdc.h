@@ -76,6 +76,21 @@ int  hsdc_get_stats(hsdc_ctx *ctx, hsdc_stats *s);
 /* HIP stream used by the context (hipStream_t); 0 = the null stream. */
 int  hsdc_set_stream(hsdc_ctx *ctx, void *stream);
 
+/* Merges of the last DC solve on the context (hsdc_stedc, hsdc_stedc_dev, hsdc_laed0,
+ * hsdc_stedc_batched_dev), level by level and left to right within a level -- for one
+ * unsplit problem the order of MDLAED0's merge loop (mdlaed0.f90:294-341), one MDLAED1
+ * call (mdlaed1.f90:109) each: block start s (0-based global index), order n, K after
+ * deflation (MDLAED2, mdlaed2.f90:149-409), the deflation rotations applied (DROT,
+ * mdlaed2.f90:352), the branch (hss 1: MDLAED3 with DHSSEVC, mdlaed1.f90:210; 0: DLAED3,
+ * :199) and the largest HSS generator rank (0 on the dense branch).  Any pointer may
+ * be null; entries beyond cap are not written.  Returns the merge count (0 when a
+ * single-merge call has reused the solve's tables since). */
+int hsdc_trace(hsdc_ctx *ctx, int *s, int *n, int *k, int *nrot, int *hss, int *hss_rank, int cap);
+/* INDXQ after the top merge of the last solve of one unsplit problem of order n
+ * (1-based; the permutation MDLAED0 applies at mdlaed0.f90:355-362).  -3 when the
+ * last solve had another order or the tables were reused. */
+int hsdc_trace_indxq(hsdc_ctx *ctx, int *indxq, int n);
+
 /* MDSTEDC (mdstedc.f90:139).  compz 'N', 'I' or 'V'; d(n), e(n-1), z(ldz,n) host
  * arrays; work/iwork follow the LAPACK workspace contract (lwork = -1 or
  * liwork = -1 is a size query that returns work[0], iwork[0]). */

@@ -45,11 +45,15 @@ typedef struct {
     int bd_ni;    /* its HSS leaf block size (mdlasd8.f90:277) */
 } orc_params;
 
-/* per-merge trace filled by orc_stedc when non-NULL */
+/* per-merge trace filled by orc_stedc when non-NULL; nrot (deflation rotations per
+ * merge) and indxq (INDXQ of the top merge of an unsplit problem of order indxq_n,
+ * the permutation of mdlaed0.f90:355-362) are optional (NULL) */
 typedef struct {
     int cap, nmerge;
     int *level, *n, *k, *hss, *rank_max;
     long long roots, iters;
+    int *nrot, *indxq;
+    int indxq_n;
 } orc_trace;
 
 void orc_default_params(orc_params *p);

@@ -591,9 +591,11 @@ L80:
                 Q_(i, nj) = c * y - s * x;
                 Q_(i, pj) = tmp;
             }
-            if (nrot) {
-                rot_pj[*nrot] = pj; rot_nj[*nrot] = nj;
-                rot_c[*nrot] = c; rot_s[*nrot] = s;
+            if (nrot) {  /* with no log arrays only the count is kept */
+                if (rot_pj) {
+                    rot_pj[*nrot] = pj; rot_nj[*nrot] = nj;
+                    rot_c[*nrot] = c; rot_s[*nrot] = s;
+                }
                 (*nrot)++;
             }
             t = d[pj - 1] * (c * c) + d[nj - 1] * (s * s);
@@ -866,10 +868,11 @@ int orc_mdlaed3_vectors(int k, double *d, double rho, double *dlamda, double *z,
 }
 
 /* ------------------------------------------------------------------ */
-/* MDLAED1 (mdlaed1.f90:109-233) */
-int orc_laed1(int n, double *d, double *q, int ldq, int *indxq, double rho, int cutpnt,
-              double *work, int *iwork, const orc_params *p, int *kout, int *hssout,
-              int *rank_max, long long *iters)
+/* MDLAED1 (mdlaed1.f90:109-233); *nrotout (may be NULL) receives the deflation
+ * rotations MDLAED2 applied */
+static int laed1_rot(int n, double *d, double *q, int ldq, int *indxq, double rho, int cutpnt,
+                     double *work, int *iwork, const orc_params *p, int *kout, int *hssout,
+                     int *rank_max, long long *iters, int *nrotout)
 {
     int info, k = 0, hflag = 1, i;
     if (n < 0) return -1;
@@ -880,9 +883,11 @@ int orc_laed1(int n, double *d, double *q, int ldq, int *indxq, double rho, int 
     int *indx = iwork, *indxc = iwork + n, *coltyp = iwork + 2 * n, *indxp = iwork + 3 * n;
     for (i = 0; i < cutpnt; i++) zz[i] = q[(cutpnt - 1) + (long)i * ldq];
     for (i = cutpnt; i < n; i++) zz[i] = q[cutpnt + (long)i * ldq];
+    int nrot = 0;
     info = orc_laed2(&k, n, cutpnt, d, q, ldq, indxq, &rho, zz, dlamda, w, q2, indx, indxc,
-                     indxp, coltyp, p->bsmlz, &hflag, NULL, NULL, NULL, NULL, NULL);
+                     indxp, coltyp, p->bsmlz, &hflag, &nrot, NULL, NULL, NULL, NULL);
     if (kout) *kout = k;
+    if (nrotout) *nrotout = nrot;
     if (hssout) *hssout = (k != 0 && !hflag);
     if (info != 0) return info;
     if (k != 0) {
@@ -906,6 +911,13 @@ int orc_laed1(int n, double *d, double *q, int ldq, int *indxq, double rho, int 
         for (i = 1; i <= n; i++) indxq[i - 1] = i;
     }
     return 0;
+}
+
+int orc_laed1(int n, double *d, double *q, int ldq, int *indxq, double rho, int cutpnt,
+              double *work, int *iwork, const orc_params *p, int *kout, int *hssout,
+              int *rank_max, long long *iters)
+{
+    return laed1_rot(n, d, q, ldq, indxq, rho, cutpnt, work, iwork, p, kout, hssout, rank_max, iters, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -953,11 +965,11 @@ int orc_laed0(int n, double *d, double *e, double *q, int ldq, double *work, int
         for (i = 0; i <= spm2; i += 2) {
             if (i == 0) { submat = 1; matsiz = iwork[1]; msd2 = iwork[0]; }
             else { submat = iwork[i - 1] + 1; matsiz = iwork[i + 1] - iwork[i - 1]; msd2 = matsiz / 2; }
-            int kk = 0, hs = 0, rmax = 0;
+            int kk = 0, hs = 0, rmax = 0, nr = 0;
             long long it0 = tr ? tr->iters : 0, itl = 0;
-            info = orc_laed1(matsiz, &d[submat - 1], &q[(submat - 1) + (long)(submat - 1) * ldq], ldq,
+            info = laed1_rot(matsiz, &d[submat - 1], &q[(submat - 1) + (long)(submat - 1) * ldq], ldq,
                              &iwork[indxq + submat - 1], e[submat + msd2 - 2], msd2, work,
-                             &iwork[subpbs], p, &kk, &hs, &rmax, &itl);
+                             &iwork[subpbs], p, &kk, &hs, &rmax, &itl, &nr);
             if (tr) {
                 tr->iters = it0 + itl;
                 tr->roots += kk;
@@ -965,6 +977,7 @@ int orc_laed0(int n, double *d, double *e, double *q, int ldq, double *work, int
                     tr->level[tr->nmerge] = curlvl; tr->n[tr->nmerge] = matsiz;
                     tr->k[tr->nmerge] = kk; tr->hss[tr->nmerge] = hs;
                     tr->rank_max[tr->nmerge] = rmax;
+                    if (tr->nrot) tr->nrot[tr->nmerge] = nr;
                 }
                 tr->nmerge++;
             }
@@ -974,6 +987,7 @@ int orc_laed0(int n, double *d, double *e, double *q, int ldq, double *work, int
         subpbs /= 2;
         curlvl++;
     }
+    if (tr && tr->indxq && tr->indxq_n == n) memcpy(tr->indxq, &iwork[indxq], sizeof(int) * n);
     for (i = 1; i <= n; i++) {
         j = iwork[indxq + i - 1];
         work[i - 1] = d[j - 1];

@@ -91,6 +91,16 @@ def dstedc(d, e, compz=b"I"):
     return d, z, info.value
 
 
+def dsterf(d, e):
+    """LAPACK DSTERF: eigenvalues only (ascending)."""
+    n = len(d)
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64) if n > 1 else np.zeros(1)
+    info = _i(0)
+    lib().scipy_dsterf_(_ip(n), _dp(d), _dp(e), ctypes.byref(info))
+    return d, info.value
+
+
 def dlaed2(d, q, indxq, rho, z, n1):
     """LAPACK DLAED2 on copies; returns a dict of every output."""
     n = len(d)

@@ -22,7 +22,7 @@ class Params(ctypes.Structure):
 class Trace(ctypes.Structure):
     _fields_ = [("cap", _i), ("nmerge", _i), ("level", _P(_i)), ("n", _P(_i)), ("k", _P(_i)),
                 ("hss", _P(_i)), ("rank_max", _P(_i)), ("roots", ctypes.c_longlong),
-                ("iters", ctypes.c_longlong)]
+                ("iters", ctypes.c_longlong), ("nrot", _P(_i)), ("indxq", _P(_i)), ("indxq_n", _i)]
 
 
 class Hss(ctypes.Structure):
@@ -221,10 +221,13 @@ def stedc(d, e, compz=b"I", prm=None, trace_cap=0, z=None):
     bufs = None
     if trace_cap:
         tr = Trace()
-        bufs = [np.zeros(trace_cap, np.int32) for _ in range(5)]
+        bufs = [np.zeros(trace_cap, np.int32) for _ in range(6)]
+        qbuf = np.zeros(max(n, 1), np.int32)
         tr.cap = trace_cap
         tr.nmerge = 0
-        tr.level, tr.n, tr.k, tr.hss, tr.rank_max = [ip(b) for b in bufs]
+        tr.level, tr.n, tr.k, tr.hss, tr.rank_max, tr.nrot = [ip(b) for b in bufs]
+        tr.indxq = ip(qbuf)
+        tr.indxq_n = n
         tr.roots = 0
         tr.iters = 0
     c = compz if isinstance(compz, bytes) else compz.encode()
@@ -235,8 +238,8 @@ def stedc(d, e, compz=b"I", prm=None, trace_cap=0, z=None):
     if tr is not None:
         m = min(tr.nmerge, trace_cap)
         out["trace"] = dict(level=bufs[0][:m].copy(), n=bufs[1][:m].copy(), k=bufs[2][:m].copy(),
-                            hss=bufs[3][:m].copy(), rank_max=bufs[4][:m].copy(),
-                            roots=tr.roots, iters=tr.iters)
+                            hss=bufs[3][:m].copy(), rank_max=bufs[4][:m].copy(), nrot=bufs[5][:m].copy(),
+                            indxq=qbuf[:n].copy(), roots=tr.roots, iters=tr.iters)
     return out
 
 

@@ -197,16 +197,6 @@ def test_device_schedule_batched_problems_with_hss(torch_dev, ctx):
         check_eig(ds[i], es[i][:n - 1], lam, z)
 
 
-def test_first_level_k_matches_oracle(torch_dev, ctx):
-    """Leaves are bit-exact, so the first merge level sees identical inputs."""
-    d, e = tridiag_cfg(2, 800, 21)
-    ref = O.stedc(d, e, prm=O.params(bsmlz=1 << 30), trace_cap=1000)
-    solve_dev(torch_dev, ctx, d, e, hss_min_k=1 << 30)
-    st = ctx.stats()
-    assert st["merges"] == len(ref["trace"]["k"])
-    assert st["roots"] + st["deflated"] == int(np.sum(ref["trace"]["n"]))
-
-
 def test_split_matrix(torch_dev, ctx):
     rng = np.random.default_rng(9)
     n = 150

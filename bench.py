@@ -119,7 +119,7 @@ def bench_batch(args, np, torch, hsdc):
     E0 = torch.tensor(np.concatenate([np.append(e, 0.0) for _, e in probs]), device=dev)
     D, E = torch.empty_like(D0), torch.empty_like(E0)
     Z = torch.empty(int(sum(sq)), dtype=torch.float64, device=dev)
-    hss = (1024, 128, args.tol if args.tol != 1e-13 else 1e-15)
+    hss = (1024, 128, args.tol)
     ctx = hsdc.Context(hss_min_k=hss[0], hss_leaf=hss[1], hss_tol=hss[2], batched=1, learned=1)
     ctx.mlp_load(os.path.join(ROOT, "hsspack_amd", "data", "secular_mlp.bin"))
     D.copy_(D0)

@@ -4,17 +4,24 @@ with the batched divide-and-conquer path and the learned secular start on one
 MI355X, device resident.
 
 One step = one complete solve (eigenvalues and the n x n eigenvector matrix) with
-the input already in HBM.  Prints one JSON line.  Also reported in the line: the
-merge-at-a-time ("plain") schedule with the classical start on the same input, the
-batched schedule with the classical start, secular iteration counts of each, a
-roofline entry for the dominant HBM kernel -- the zero fill of the output, which
-runs on a second stream beside the DC levels, with the final reordering that then
-writes the supported rows (both event-timed inside the timed steps) --,
-orthogonality and residual of the result, and the CPU baseline (the oracle's
-restatement of the reference path, single thread, timed on this host).
+the input already in HBM; `value` times the solve as used (levels queued at once,
+replayed as a HIP graph).  Prints one JSON line.  Also in the line:
+  * a profiled pass of the same solve (direct launches, HIP events) giving the
+    roofline of the dominant kernel -- the zero fill of the output, on a second stream
+    beside the DC levels -- and the rates of the hot-path kernels (secular sweep and
+    Loewner / alpha sweeps in GB/s of their algorithmic bytes, the HSS multiply in
+    TFLOP/s, the HSS construction against HSSPACK's flop model);
+  * ablations: classical secular start, a host round trip per level, the
+    merge-at-a-time ("plain") schedule; orthogonality and residual; the host-to-host
+    time of the host-array entry (Z copied back);
+  * config.hss_workload: the same for the n = 32768 near-Toeplitz matrix (63 HSS merges,
+    learned start active) at HSS tolerance 1e-15 and 1e-13;
+  * CPU baselines on the host's threads: the oracle (restatement of the reference path)
+    and LAPACK dstedc.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cfg 1 --seed 4]
-                       [--no-cpu] [--no-plain] [--no-check]
+                       [--no-cpu] [--no-plain] [--no-check] [--no-hss]
+                       [--batch | --svd [--svd-toeplitz]]
 """
 import argparse
 import ctypes
@@ -62,14 +69,40 @@ def check_result(torch, d, e, lam, zz):
         tz -= z * lam_t[i0:i0 + blk, None]
         res = max(res, tz.abs().max().item())
     nrm = max(abs(d).max(), abs(e).max())
-    return orth, res / nrm
+    return float(orth), float(res / nrm)
+
+
+MLP_FILE = os.path.join(ROOT, "hsspack_amd", "data", "secular_mlp.bin")
+
+
+def host_threads():
+    """Threads the CPU baselines use: OMP_NUM_THREADS when set (16 on the GPU host), else every core."""
+    return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _extrapolate(np, times, n_full):
+    (n0, t0), (n1, t1) = sorted(times.items())
+    p = np.log(t1 / t0) / np.log(n1 / n0)
+    return t1 * (n_full / float(n1)) ** p, p
 
 
 def cpu_baseline(cfg, seed, hss, n_full):
-    """The oracle (C restatement of the reference path, one thread) on the same
-    generator.  The headline workload (cfg 2: nearly everything deflates) runs at its
-    full order in a few seconds and is timed as it is; the HSS-heavy generators are
-    timed at 8192 and 16384 and extrapolated with the measured exponent."""
+    """The oracle (C restatement of the reference path) on the same generator, with OpenMP
+    over the loops whose outputs are independent (GEMM columns, the identity fill, the
+    final permutation; the reference runs its GEMMs in threaded BLAS) on host_threads()
+    threads.  The headline generator (nearly everything deflates) runs at its full order;
+    the HSS-heavy one is timed at 4096 and 8192 and extrapolated with the measured
+    exponent (its full order would take about ten minutes)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as O
@@ -88,21 +121,241 @@ def cpu_baseline(cfg, seed, hss, n_full):
                            O.ip(iwork), liwork, ctypes.byref(prm), None)
         t = time.perf_counter() - t0
         assert info == 0
-        log("cpu oracle n=%d: %.2f s" % (n, t))
+        log("cpu oracle n=%d: %.2f s (%d threads)" % (n, t, host_threads()))
         return t
 
+    base = dict(unit="s", cores=host_threads(), cpu=cpu_model(), kind="port")
+    what = "oracle (C restatement of mdstedc/mdlaed0-3 + DHSSEVC, OpenMP over independent GEMM columns and copies)"
     if cfg == 2:
-        t = run(n_full)
-        return dict(value=t, unit="s", cores=1, kind="port",
-                    sample="oracle (C restatement of mdstedc/mdlaed0-3 + DHSSEVC, 1 thread), the full "
-                           "workload: n=%d, same generator and seed, all eigenpairs" % n_full)
-    times = {n: run(n) for n in (8192, 16384)}
-    p = np.log2(times[16384] / times[8192])
-    est = times[16384] * (n_full / 16384.0) ** p
-    return dict(value=est, unit="s", cores=1, kind="port",
-                sample="oracle (C restatement of mdstedc/mdlaed0-3 + DHSSEVC, 1 thread) on the "
-                       "same generator at n=8192 (%.2f s) and n=16384 (%.2f s), extrapolated to "
-                       "n=%d with the measured exponent %.2f" % (times[8192], times[16384], n_full, p))
+        return dict(base, value=run(n_full),
+                    sample="%s, the full workload: n=%d, same generator and seed, all eigenpairs" % (what, n_full))
+    times = {n: run(n) for n in (4096, 8192)}
+    est, p = _extrapolate(np, times, n_full)
+    return dict(base, value=est, sample="%s on the same generator at n=4096 (%.2f s) and n=8192 (%.2f s), "
+                                        "extrapolated to n=%d with the measured exponent %.2f"
+                                        % (what, times[4096], times[8192], n_full, p))
+
+
+def cpu_lapack(cfg, seed, n_full):
+    """LAPACK dstedc (scipy's OpenBLAS, threaded) on the same matrix, all eigenpairs; as
+    cpu_baseline, at the full order for the headline generator, extrapolated from 4096
+    and 8192 for the HSS-heavy one."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import lapack_ref as LR
+    from hsspack_amd.workloads import tridiag_cfg
+
+    def run(n):
+        d, e = tridiag_cfg(cfg, n, seed)
+        t0 = time.perf_counter()
+        _, z, info = LR.dstedc(d, e)
+        t = time.perf_counter() - t0
+        assert info == 0
+        del z
+        log("cpu LAPACK dstedc n=%d: %.2f s" % (n, t))
+        return t
+
+    base = dict(unit="s", cores=host_threads(), cpu=cpu_model(), kind="lapack")
+    if cfg == 2:
+        return dict(base, value=run(n_full), sample="LAPACK dstedc (OpenBLAS 0.3.28), full workload n=%d" % n_full)
+    times = {n: run(n) for n in (4096, 8192)}
+    est, p = _extrapolate(np, times, n_full)
+    return dict(base, value=est, sample="LAPACK dstedc (OpenBLAS 0.3.28) at n=4096 (%.2f s) and 8192 (%.2f s), "
+                                        "extrapolated to n=%d with exponent %.2f" % (times[4096], times[8192],
+                                                                                     n_full, p))
+
+
+def secstats(st):
+    return {"iters_per_root": st["secular_iters"] / max(st["roots"], 1),
+            "wave_iters": st["secular_wave_iters"], "critical_path_iters": st["secular_crit_iters"],
+            "learned_roots": st["learned_roots"]}
+
+
+def bench_tridiag(args, np, torch, hsdc, cfg, n, seed, tol, d, e, steps, warmup, plain=True, check=True,
+                  sync_ablation=False, h2h=True):
+    """One tridiagonal workload.  value_s: seconds per device-resident solve, the batched
+    schedule with the learned start, levels queued at once and replayed as a HIP graph.
+    Then a profiled pass of the same solve (direct launches, HIP events around the zero fill,
+    the final reordering, the secular, Loewner / alpha and HSS launches) gives the rates,
+    and ablations time the classical start, the per-level host round trip and the
+    merge-at-a-time schedule."""
+    dev = torch.device("cuda:0")
+    D0 = torch.tensor(d, device=dev)
+    E0 = torch.tensor(np.append(e, 0.0), device=dev)
+    D = torch.empty_like(D0)
+    E = torch.empty_like(E0)
+    Z = torch.empty((n, n), dtype=torch.float64, device=dev)
+    ctx = hsdc.Context(hss_min_k=1024, hss_leaf=128, hss_tol=tol, batched=1, learned=1, graph=1, profile=0)
+    ctx.mlp_load(MLP_FILE)
+
+    def step():
+        D.copy_(D0)
+        E.copy_(E0)
+        rc = ctx.stedc_dev(D, E, Z)
+        if rc != 0:
+            raise RuntimeError("hsdc_stedc_dev returned %d" % rc)
+
+    def timed(k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    for _ in range(warmup):
+        step()
+    value = timed(steps)
+    st = ctx.stats()
+    tr = ctx.trace()
+    sec_learned = secstats(st)
+    log("n=%d tol %g: %.2f ms/solve (graph), stats %s" % (n, tol, value * 1e3, st))
+    # profiled pass
+    ctx.set_params(profile=1)
+    step()
+    keys = ("ms_zfill", "ms_final", "ms_secular", "ms_loewner", "ms_hss_build", "ms_hss_mm", "ms_total")
+    acc = {k: 0.0 for k in keys}
+    nprof = max(2, min(steps, 5))
+    t_prof = 0.0
+    for _ in range(nprof):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        t_prof += time.perf_counter() - t0
+        sp = ctx.stats()
+        for k in keys:
+            acc[k] += sp[k] / nprof
+    ctx.set_params(profile=0)
+    t_prof /= nprof
+    log("profiled pass: %.2f ms/solve, %s" % (t_prof * 1e3, {k: round(v, 3) for k, v in acc.items()}))
+
+    def rate(nbytes, ms):
+        return nbytes / (ms * 1e-3) / 1e9 if ms > 0 else None
+
+    hot = {}
+    if sp["secular_evals"] and acc["ms_secular"] > 0:
+        b = 16 * sp["secular_evals"]
+        hot["k_secular"] = {"bytes": b, "ms": acc["ms_secular"], "GBps": rate(b, acc["ms_secular"]),
+                            "frac_of_hbm": rate(b, acc["ms_secular"]) / HBM_PEAK_GBS,
+                            "pole_evaluations": sp["secular_evals"],
+                            "unit_note": "16 B per pole evaluation (one pole, one z), K evaluations per iteration; "
+                                         "with the k_secstart learned-start launches"}
+    if sp["loewner_pairs"] and acc["ms_loewner"] > 0:
+        b = 16 * sp["loewner_pairs"]
+        hot["k_loewner+k_alpha"] = {"bytes": b, "ms": acc["ms_loewner"], "GBps": rate(b, acc["ms_loewner"]),
+                                    "frac_of_hbm": rate(b, acc["ms_loewner"]) / HBM_PEAK_GBS,
+                                    "pairs": sp["loewner_pairs"],
+                                    "unit_note": "16 B per (i, j) operand pair, K^2 per merge for the products, "
+                                                 "K^2 more per HSS merge for alpha"}
+    if sp["hss_mm_flops"] and acc["ms_hss_mm"] > 0:
+        tf = sp["hss_mm_flops"] / (acc["ms_hss_mm"] * 1e-3) / 1e12
+        hot["k_bgemm"] = {"flops": sp["hss_mm_flops"], "ms": acc["ms_hss_mm"], "TFLOPs": tf,
+                          "frac_of_fp64_peak": tf / FP64_PEAK_TFS,
+                          "unit_note": "2 L (|H| + |DD|) per merge: the DGEMMs of fastHssmmL"}
+    if sp["hss_nflops"] and acc["ms_hss_build"] > 0:
+        gf = sp["hss_nflops"] / (acc["ms_hss_build"] * 1e-3) / 1e9
+        hot["hss_construction"] = {"nflops": sp["hss_nflops"], "ms": acc["ms_hss_build"], "GFLOPs": gf,
+                                   "frac_of_fp64_peak": gf / 1e3 / FP64_PEAK_TFS,
+                                   "unit_note": "HSSPACK's nflops model of the structured RRLU "
+                                                "(rrluCauchyEigrow/col and pivot searches); latency bound"}
+    if sp["zfill_bytes"]:
+        b = sp["zfill_bytes"]
+        roof = {"kernel": "k_zfill (the zero part of the reordered eigenvector matrix, written on a second "
+                          "stream beside the DC levels)",
+                "bound": "hbm", "achieved": rate(b, acc["ms_zfill"]), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": rate(b, acc["ms_zfill"]) / HBM_PEAK_GBS,
+                "traffic": K_ZFILL_TRAFFIC.get((cfg, n, seed)), "bytes_per_launch": b, "avg_ms": acc["ms_zfill"],
+                "timed_in": "profiled pass of %d solves (direct launches, HIP events on the fill's stream)" % nprof,
+                "final": {"kernel": "k_final (eigenvector reordering, mdlaed0.f90:355-362: the supported rows)",
+                          "bytes_per_launch": sp["final_bytes"], "avg_ms": acc["ms_final"],
+                          "achieved": rate(sp["final_bytes"], acc["ms_final"])}}
+    else:
+        g = rate(sp["final_bytes"], acc["ms_final"])
+        roof = {"kernel": "k_final", "bound": "hbm", "achieved": g, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": g / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": sp["final_bytes"],
+                "avg_ms": acc["ms_final"]}
+    # ablations
+    ctx.set_params(learned=0)
+    step()
+    classical = timed(3)
+    sec_classical = secstats(ctx.stats())
+    ctx.set_params(learned=1)
+    log("classical start: %.2f ms/solve %s" % (classical * 1e3, sec_classical))
+    sync_s = None
+    if sync_ablation:
+        ctx.set_params(level_sync=1)
+        step()
+        sync_s = timed(3)
+        ctx.set_params(level_sync=0)
+        log("host round trip per level: %.2f ms/solve" % (sync_s * 1e3))
+    plain_s = None
+    if plain:
+        ctx.set_params(batched=0, learned=0)
+        step()
+        plain_s = timed(1)
+        ctx.set_params(batched=1, learned=1)
+        log("plain (merge at a time, classical start): %.2f ms/solve" % (plain_s * 1e3))
+    step()
+    torch.cuda.synchronize()
+    orth = res = None
+    if check:
+        orth, res = check_result(torch, d, e, D.cpu().numpy(), Z)
+        log("orthogonality %.3e residual %.3e" % (orth, res))
+    out = {"value_s": value,
+           "schedule": "batched (one launch per DC level, whole solve queued, launch sizes and tables derived "
+                       "on the device, HIP graph replay) + learned secular start",
+           "profiled_pass_s": t_prof, "batched_classical_s": classical, "batched_level_sync_s": sync_s,
+           "plain_s": plain_s, "speedup_vs_plain": plain_s / value if plain_s else None,
+           "secular_learned": sec_learned, "secular_classical": sec_classical,
+           "iterations_saved_per_root": sec_classical["iters_per_root"] - sec_learned["iters_per_root"],
+           "orthogonality": orth, "residual": res,
+           "meets_1e12": bool(orth <= 1e-12 and res <= 1e-12) if check else None,
+           "merges": int(len(tr["k"])), "levels": st["levels"], "k_top": int(tr["k"][-1]) if len(tr["k"]) else 0,
+           "roots": st["roots"], "deflated": st["deflated"], "rotations": int(tr["nrot"].sum()),
+           "hss_merges": int(tr["hss"].sum()), "hss_rank_max": int(tr["hss_rank"].max()) if len(tr["k"]) else 0,
+           "hot_path_kernels": hot, "roofline": roof}
+    del Z
+    torch.cuda.empty_cache()
+    if h2h:
+        # host arrays in and out (MDSTEDC's own interface): the D2H copy of Z is included
+        zh = np.zeros((n, n), order="F")
+        work, iwork = np.zeros(4), np.zeros(1, np.int32)
+        lwork, liwork = 1 + 4 * n + n * n, 3 + 5 * n
+        ts = []
+        for _ in range(2):
+            dh, eh = np.array(d), np.array(e)
+            t0 = time.perf_counter()
+            info = hsdc.lib().hsdc_stedc(ctx._h, ctypes.c_char(b"I"), n, hsdc._dp(dh), hsdc._dp(eh), hsdc._dp(zh),
+                                         n, hsdc._dp(work), lwork, hsdc._ip(iwork), liwork)
+            ts.append(time.perf_counter() - t0)
+            if info != 0:
+                raise RuntimeError("hsdc_stedc returned %d" % info)
+        out["host_to_host_s"] = ts[-1]
+        log("host to host (hsdc_stedc, Z copied back): %.3f s" % ts[-1])
+        del zh
+    ctx.close()
+    return out
+
+
+def bench_hss_workload(args, np, torch, hsdc):
+    """The near-Toeplitz generator at n = 32768 (d = 2 + U(-1e-3, 1e-3), e = 1 + U(-1e-3, 1e-3),
+    seed 4): K close to N at every level, 63 HSS merges.  At HSS tolerance 1e-15 (which meets
+    orthogonality and residual <= 1e-12) and 1e-13 (which does not on this matrix)."""
+    from hsspack_amd.workloads import tridiag_cfg
+    n = 32768
+    d, e = tridiag_cfg(1, n, 4)
+    out = {"workload": "tridiagonal DC eigensolve, n=%d, %s seed 4, HSS leaf 128, HSS for K>=1024, all "
+                       "eigenpairs, device resident" % (n, GEN[1])}
+    for tol in (1e-15, 1e-13):
+        r = bench_tridiag(args, np, torch, hsdc, 1, n, 4, tol, d, e, 3, 1, plain=(tol == 1e-15), check=True,
+                          sync_ablation=False, h2h=False)
+        r.pop("roofline")
+        out["tol_%g" % tol] = r
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(1, 4, (1024, 128, 1e-15), n)
+        out["cpu_lapack_dstedc"] = cpu_lapack(1, 4, n)
+    return out
 
 
 def bench_batch(args, np, torch, hsdc):
@@ -305,6 +558,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-hss", action="store_true", help="skip the n=32768 near-Toeplitz (HSS) sub-record")
     ap.add_argument("--tol", type=float, default=1e-13, help="HSS compression tolerance")
     ap.add_argument("--batch", action="store_true",
                     help="512 independent tridiagonals, n log-uniform in [256, 8192] (near-Toeplitz entries, "
@@ -355,142 +609,40 @@ def main():
         return bench_svd(args, np, torch, hsdc)
 
     n = args.n
-    hss = (1024, 128, args.tol)
     d, e = tridiag_cfg(args.cfg, n, args.seed)
-    dev = torch.device("cuda:0")
-    D0 = torch.tensor(d, device=dev)
-    E0 = torch.tensor(np.append(e, 0.0), device=dev)
-    D = torch.empty_like(D0)
-    E = torch.empty_like(E0)
-    Z = torch.empty((n, n), dtype=torch.float64, device=dev)
-    ctx = hsdc.Context(hss_min_k=hss[0], hss_leaf=hss[1], hss_tol=hss[2], batched=1, profile=1, learned=1)
-    ctx.mlp_load(os.path.join(ROOT, "hsspack_amd", "data", "secular_mlp.bin"))
-
-    def secstats(st):
-        return {"iters_per_root": st["secular_iters"] / max(st["roots"], 1),
-                "wave_iters": st["secular_wave_iters"], "critical_path_iters": st["secular_crit_iters"],
-                "learned_roots": st["learned_roots"]}
-
-    def step():
-        D.copy_(D0)
-        E.copy_(E0)
-        rc = ctx.stedc_dev(D, E, Z)
-        if rc != 0:
-            raise RuntimeError("hsdc_stedc_dev returned %d" % rc)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    fin_ms, fin_bytes, zf_ms, zf_bytes = [], 0, [], 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        st = ctx.stats()
-        fin_ms.append(st["ms_final"])
-        fin_bytes = st["final_bytes"]
-        zf_ms.append(st["ms_zfill"])
-        zf_bytes = st["zfill_bytes"]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1000.0 / args.steps
-    st = ctx.stats()
-    log("batched: %.2f ms/solve, stats %s" % (ms_step, st))
-
-    sec_learned = secstats(st)
-    # ablation: batched schedule, classical start
-    ctx.set_params(learned=0)
-    step()
-    torch.cuda.synchronize()
-    tc0 = time.perf_counter()
-    for _ in range(3):
-        step()
-    torch.cuda.synchronize()
-    classical_s = (time.perf_counter() - tc0) / 3
-    sec_classical = secstats(ctx.stats())
-    log("batched, classical start: %.2f ms/solve, %s" % (classical_s * 1000, sec_classical))
-    # ablation: learned start, the host reading every level's K before its launches
-    ctx.set_params(learned=1, level_sync=1)
-    step()
-    torch.cuda.synchronize()
-    ts0 = time.perf_counter()
-    for _ in range(3):
-        step()
-    torch.cuda.synchronize()
-    sync_s = (time.perf_counter() - ts0) / 3
-    ctx.set_params(level_sync=0)
-    log("batched, host round trip per level: %.2f ms/solve" % (sync_s * 1000))
-
-    plain_s = None
-    if not args.no_plain:
-        ctx.set_params(batched=0)
-        step()
-        torch.cuda.synchronize()
-        tp0 = time.perf_counter()
-        step()
-        torch.cuda.synchronize()
-        plain_s = time.perf_counter() - tp0
-        log("plain (merge at a time, classical start): %.2f ms/solve" % (plain_s * 1000))
-    ctx.set_params(batched=1, learned=1)
-    step()
-    torch.cuda.synchronize()
-
-    orth = res = None
-    if not args.no_check:
-        orth, res = check_result(torch, d, e, D.cpu().numpy(), Z)
-        log("orthogonality %.3e residual %.3e" % (orth, res))
-
-    fin_avg = sum(fin_ms) / len(fin_ms)
-    fin_gbs = fin_bytes / (fin_avg * 1e-3) / 1e9
-    zf_avg = sum(zf_ms) / len(zf_ms)
-    final_entry = {"kernel": "k_final (eigenvector reordering, mdlaed0.f90:355-362: the supported rows)",
-                   "bytes_per_launch": fin_bytes, "avg_ms": fin_avg, "achieved": fin_gbs}
-    if zf_bytes:
-        achieved = zf_bytes / (zf_avg * 1e-3) / 1e9
-        roof = {"kernel": "k_zfill (the zero part of the reordered eigenvector matrix, written on a second "
-                          "stream beside the DC levels)",
-                "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": K_ZFILL_TRAFFIC.get((args.cfg, n, args.seed)),
-                "bytes_per_launch": zf_bytes, "avg_ms": zf_avg, "final": final_entry}
-    else:
-        roof = dict(final_entry, bound="hbm", peak=HBM_PEAK_GBS, unit="GB/s", frac=fin_gbs / HBM_PEAK_GBS,
-                    traffic=None)
+    r = bench_tridiag(args, np, torch, hsdc, args.cfg, n, args.seed, args.tol, d, e, args.steps, args.warmup,
+                      plain=not args.no_plain, check=not args.no_check, sync_ablation=True)
+    roof = r.pop("roofline")
     out = {
         "metric": "s to all eigenpairs, n=32768 FP64 tridiag (orth<=1e-12): learned+batched vs plain",
-        "value": ms_step / 1000.0,
+        "value": r["value_s"],
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": r["value_s"] * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "tridiagonal DC eigensolve, n=%d, %s seed %d, HSS leaf 128, "
-                               "tol %g, HSS for K>=1024, all eigenpairs, device resident"
-                               % (n, GEN[args.cfg], args.seed, args.tol),
-                   "n": n, "schedule": "batched (one launch per DC level, whole solve queued, launch sizes "
-                                               "and tables derived on the device) + learned secular start",
-                   "plain_s": plain_s,
-                   "speedup_vs_plain": (plain_s / (ms_step / 1000.0)) if plain_s else None,
-                   "batched_classical_s": classical_s, "batched_level_sync_s": sync_s,
-                   "secular_learned": sec_learned, "secular_classical": sec_classical,
-                   "orthogonality": orth, "residual": res,
-                   "merges": st["merges"], "levels": st["levels"], "k_top": st["k_top"],
-                   "roots": st["roots"], "deflated": st["deflated"],
-                   "hss_merges": st["hss_merges"], "hss_rank_max": st["rank_max"],
-                   "hss_multiply": ({"flops": st["hss_mm_flops"], "ms": st["ms_hss_mm"],
-                                     "TFLOPs": st["hss_mm_flops"] / (st["ms_hss_mm"] * 1e-3) / 1e12,
-                                     "frac_of_fp64_peak": st["hss_mm_flops"] / (st["ms_hss_mm"] * 1e-3) / 1e12
-                                     / FP64_PEAK_TFS}
-                                    if st["hss_mm_flops"] and st["ms_hss_mm"] else None)},
+        "config": dict({"workload": "tridiagonal DC eigensolve, n=%d, %s seed %d, HSS leaf 128, tol %g, HSS for "
+                                    "K>=1024, all eigenpairs, device resident" % (n, GEN[args.cfg], args.seed,
+                                                                                   args.tol),
+                        "n": n}, **{k: v for k, v in r.items() if k != "value_s"}),
         "roofline": roof,
     }
+    if not args.no_hss:
+        out["config"]["hss_workload"] = bench_hss_workload(args, np, torch, hsdc)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cfg, args.seed, hss, n)
-    print(json.dumps(out), flush=True)
-
+        out["cpu_baseline"] = cpu_baseline(args.cfg, args.seed, (1024, 128, args.tol), n)
+        out["config"]["cpu_lapack_dstedc"] = cpu_lapack(args.cfg, args.seed, n)
+        gpu_h2h = out["config"].get("host_to_host_s")
+        if gpu_h2h:
+            out["config"]["gpu_host_to_host_vs_cpu"] = {
+                "vs_oracle": out["cpu_baseline"]["value"] / gpu_h2h,
+                "vs_lapack_dstedc": out["config"]["cpu_lapack_dstedc"]["value"] / gpu_h2h}
+    print(json.dumps(out, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
 
 if __name__ == "__main__":
     main()

@@ -183,6 +183,11 @@ struct hsdc_ctx {
     std::vector<int> trc_rank;
     int trc_ntot = 0;
     bool trc_valid = false;
+    // profile = 1: event pairs around the secular (kind 0) and Loewner / alpha (kind 1)
+    // launches of a solve, read at its end
+    std::vector<hipEvent_t> pev;
+    std::vector<int> pkind;
+    size_t pused = 0;
 };
 
 
@@ -294,13 +299,42 @@ static void gate_open(hsdc_ctx *c)
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(WGS), 0, c->stream, (unsigned long long *)c->W.gate, 1L, 0ULL);
 }
 
+// profile = 1: record the opening (kind >= 0) or closing (kind < 0) event of a timed
+// launch group on the context's stream
+static void prof_mark(hsdc_ctx *c, int kind)
+{
+    if (!c->prm.profile) return;
+    if (c->pused == c->pev.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) { (void)hipGetLastError(); return; }
+        c->pev.push_back(e);
+        c->pkind.push_back(0);
+    }
+    c->pkind[c->pused] = kind;
+    if (hipEventRecord(c->pev[c->pused], c->stream) != hipSuccess) { (void)hipGetLastError(); return; }
+    c->pused++;
+}
+
+// the timed groups of the finished solve into the statistics
+static void prof_collect(hsdc_ctx *c)
+{
+    for (size_t i = 0; i + 1 < c->pused; i++) {
+        if (c->pkind[i] < 0 || c->pkind[i + 1] >= 0) continue;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->pev[i], c->pev[i + 1]) != hipSuccess) { (void)hipGetLastError(); continue; }
+        if (c->pkind[i] == 0) c->st.ms_secular += ms;
+        else c->st.ms_loewner += ms;
+    }
+    c->pused = 0;
+}
+
 // Secular control block of the next solve: learned-start weights when enabled and
 // loaded, the harvest buffer when harvesting, and zeroed iteration counters.
 static int sec_setup(hsdc_ctx *c)
 {
-    // device layout: SecCtl | iters, used, wave_iters [SEC_SLOTS each] | level maxima [SEC_LEVELS][SEC_SLOTS]
+    // device layout: SecCtl | iters, used, wave_iters, evals [SEC_SLOTS each] | level maxima [SEC_LEVELS][SEC_SLOTS]
     const size_t hdr = al64(sizeof(SecCtl));
-    const long ncnt = 3 * SEC_SLOTS + SEC_LEVELS * SEC_SLOTS / 2;  // in 8-byte words
+    const long ncnt = 4 * SEC_SLOTS + SEC_LEVELS * SEC_SLOTS / 2;  // in 8-byte words
     CK(ensure(c->sec, hdr + 8 * (size_t)ncnt));
     CK(ensure_pin(c->pin3, sizeof(SecCtl)));
     char *base = (char *)c->sec.p;
@@ -310,6 +344,7 @@ static int sec_setup(hsdc_ctx *c)
     h.iters = (unsigned long long *)(base + hdr);
     h.used = h.iters + SEC_SLOTS;
     h.wave_iters = h.iters + 2 * SEC_SLOTS;
+    h.evals = h.iters + 3 * SEC_SLOTS;
     if (c->hv_cap > 0) {
         char *hv = (char *)c->hv.p;
         h.count = (unsigned long long *)hv;
@@ -326,7 +361,7 @@ static int sec_setup(hsdc_ctx *c)
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ncnt, WGS)), dim3(WGS), 0, c->stream, h.iters, ncnt, 0ULL);
     HK(hipGetLastError());
     c->dsec = (const SecCtl *)base;
-    c->dlvlmax = (unsigned int *)(h.iters + 3 * SEC_SLOTS);
+    c->dlvlmax = (unsigned int *)(h.iters + 4 * SEC_SLOTS);
     c->cur_level = 0;
     c->sec_pre = (h.w != nullptr) || (h.hf != nullptr);
     return 0;
@@ -335,13 +370,13 @@ static int sec_setup(hsdc_ctx *c)
 // iteration counters of the last solve into the statistics (stream synchronised)
 static int sec_collect(hsdc_ctx *c)
 {
-    std::vector<unsigned long long> v(3 * SEC_SLOTS);
+    std::vector<unsigned long long> v(4 * SEC_SLOTS);
     std::vector<unsigned int> lm((size_t)SEC_LEVELS * SEC_SLOTS);
     HK(hipMemcpyAsync(v.data(), (char *)c->sec.p + al64(sizeof(SecCtl)), sizeof(unsigned long long) * v.size(),
                       hipMemcpyDeviceToHost, c->stream));
     HK(hipMemcpyAsync(lm.data(), c->dlvlmax, sizeof(unsigned int) * lm.size(), hipMemcpyDeviceToHost, c->stream));
     HK(hipStreamSynchronize(c->stream));
-    long long crit = 0, it = 0, us = 0, wi = 0;
+    long long crit = 0, it = 0, us = 0, wi = 0, ev = 0;
     for (int l = 0; l < SEC_LEVELS; l++) {
         unsigned int m = 0;
         for (int q = 0; q < SEC_SLOTS; q++) m = std::max(m, lm[(size_t)l * SEC_SLOTS + q]);
@@ -351,11 +386,13 @@ static int sec_collect(hsdc_ctx *c)
         it += (long long)v[q];
         us += (long long)v[SEC_SLOTS + q];
         wi += (long long)v[2 * SEC_SLOTS + q];
+        ev += (long long)v[3 * SEC_SLOTS + q];
     }
     c->st.secular_crit_iters = crit;
     c->st.secular_iters = it;
     c->st.learned_roots = us;
     c->st.secular_wave_iters = wi;
+    c->st.secular_evals = ev;
     return 0;
 }
 
@@ -396,7 +433,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         long dds = 0;
         for (int x : P[h].T.m) dds += (long)x * x;
         size_t o = al64(6 * sizeof(double) * K) + al64(2 * sizeof(int) * K) + al64(sizeof(int) * (11 * ltr + 8)) +
-                   al64(sizeof(long) * (4 * ltr)) + al64(sizeof(unsigned long long) * 2);
+                   al64(sizeof(long) * (4 * ltr)) + al64(sizeof(unsigned long long) * 3);
         metasz[h] = o;
         o += al64(sizeof(double) * std::max(dds, 1L));
         o += al64(sizeof(double) * (size_t)12 * std::max(ni, 128) * K);  // arena
@@ -427,8 +464,8 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         long *lp = (long *)p;
         H.pu = lp; H.pv = lp + ltr; H.pb = lp + 2 * ltr; H.pd = lp + 3 * ltr;
         p += al64(sizeof(long) * (4 * ltr));
-        H.arena_top = (unsigned long long *)p; H.scratch_top = H.arena_top + 1;
-        p += al64(sizeof(unsigned long long) * 2);
+        H.arena_top = (unsigned long long *)p; H.scratch_top = H.arena_top + 1; H.flops = H.arena_top + 2;
+        p += al64(sizeof(unsigned long long) * 3);
         HK(hipMemsetAsync(meta0, 0, p - meta0, s));
         H.dd = (double *)p;
         long dds = 0;
@@ -533,6 +570,12 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
     const HssNode *dN = (const HssNode *)(tb + t1);
     const HssCompact *dC = (const HssCompact *)(tb + t2);
     const int *dO = (const int *)(tb + t3);
+    hipEvent_t b0 = nullptr, b1 = nullptr;
+    if (c->prm.profile && nh > 0) {
+        HK(hipEventCreate(&b0));
+        HK(hipEventCreate(&b1));
+        HK(hipEventRecord(b0, s));
+    }
     if (nh > 0) {
         int kmax = 0;
         for (auto &H : hh) kmax = std::max(kmax, H.K);
@@ -579,6 +622,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         }
         HK(hipGetLastError());
     }
+    if (b1) HK(hipEventRecord(b1, s));
     // ---- read back the generator directory
     for (int t = 0; t < nh; t++) {
         HssPlanM &Q = P[hmap[t]];
@@ -587,7 +631,10 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         std::vector<long> offs(4 * ltr);
         HK(hipMemcpyAsync(meta.data(), hh[t].lr, sizeof(int) * meta.size(), hipMemcpyDeviceToHost, s));
         HK(hipMemcpyAsync(offs.data(), hh[t].pu, sizeof(long) * offs.size(), hipMemcpyDeviceToHost, s));
+        unsigned long long nfl = 0;
+        HK(hipMemcpyAsync(&nfl, hh[t].flops, sizeof(nfl), hipMemcpyDeviceToHost, s));
         HK(hipStreamSynchronize(s));
+        c->st.hss_nflops += (long long)nfl;
         Q.usz.assign(meta.begin() + 4 * ltr, meta.begin() + 6 * ltr);
         Q.vsz.assign(meta.begin() + 6 * ltr, meta.begin() + 8 * ltr);
         Q.bsz.assign(meta.begin() + 8 * ltr, meta.begin() + 10 * ltr);
@@ -602,6 +649,14 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
             c->st.rank_max = std::max(c->st.rank_max, Q.usz[2 * i + 1]);
             c->st.rank_max = std::max(c->st.rank_max, Q.vsz[2 * i]);
         }
+    }
+    if (b0) {
+        float ms = 0;
+        HK(hipStreamSynchronize(s));
+        if (hipEventElapsedTime(&ms, b0, b1) == hipSuccess) c->st.ms_hss_build += ms;
+        (void)hipGetLastError();
+        (void)hipEventDestroy(b0);
+        (void)hipEventDestroy(b1);
     }
     // G / F column layout (fastHssmmL_omp lg / lf, ConstructHssd.f90:1472-1510)
     long gft = 0;
@@ -1019,13 +1074,17 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
         hipLaunchKernelGGL(k_rotate, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur, dmo);
     }
     if (anyk) {
+        prof_mark(c, 0);
         if (c->sec_pre)
             hipLaunchKernelGGL(k_secstart, dim3(blockK[nm]), dim3(SEC_WG), 0, s, dmg, dblockK, nm,
                                (const MergeOut *)dmo, W, c->dsec);
         hipLaunchKernelGGL(k_secular, dim3(blockK[nm]), dim3(SEC_WG), 0, s, dmg, dblockK, nm, dmo, W,
                            c->prm.maxit, dinfo, c->dsec,
                            c->dlvlmax + SEC_SLOTS * std::min(c->cur_level, SEC_LEVELS - 1));
+        prof_mark(c, -1);
+        prof_mark(c, 1);
         hipLaunchKernelGGL(k_loewner, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, W);
+        prof_mark(c, -1);
         hipLaunchKernelGGL(k_sbuild, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, dsoff, W,
                            (double *)c->sb.p);
         if (!tiles.empty())
@@ -1039,7 +1098,9 @@ static int run_merges(hsdc_ctx *c, const Merge *dm, const std::vector<Merge> &hm
         for (int i = 0; i < nm; i++)
             if (!ho[i].flags && ho[i].hss && ho[i].k > 0) sel.push_back(i);
         if (!sel.empty()) {  // after k_newcols: the products go into the slots it assigned
+            prof_mark(c, 1);
             hipLaunchKernelGGL(k_alpha, dim3(chunkK[nm]), dim3(WGS), 0, s, dmg, dchunkK, nm, dmo, W);
+            prof_mark(c, -1);
             CK(run_hss(c, dmg, hm, m0, ho, sel, dprobs, cur, gbase));
         }
     }
@@ -1126,14 +1187,18 @@ static int level_dev(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
     hipLaunchKernelGGL(k_rot_prep, dim3(nm), dim3(WGS), 0, s, dmg, dprobs, W, cur, (const MergeOut *)dmo);
     hipLaunchKernelGGL(k_rotate, dim3(totN), dim3(WGS), 0, s, dmg, dchunkN, nm, dprobs, W, cur,
                        (const MergeOut *)dmo);
+    prof_mark(c, 0);
     if (c->sec_pre && nmax >= std::max(3, c->prm.learned_min_k))  // else no merge can use it
         hipLaunchKernelGGL(k_secstart, dim3(dev_grid(B.blocks)), dim3(SEC_WG), 0, s, dmg, (const int *)t.blockK, nm,
                            (const MergeOut *)dmo, W, c->dsec);
     hipLaunchKernelGGL(k_secular, dim3(dev_grid(B.blocks)), dim3(SEC_WG), 0, s, dmg, (const int *)t.blockK, nm,
                        (const MergeOut *)dmo, W, c->prm.maxit, dinfo, c->dsec,
                        c->dlvlmax + SEC_SLOTS * std::min(L, SEC_LEVELS - 1));
+    prof_mark(c, -1);
+    prof_mark(c, 1);
     hipLaunchKernelGGL(k_loewner, dim3(totN), dim3(WGS), 0, s, dmg, (const int *)t.chunkK, nm,
                        (const MergeOut *)dmo, W);
+    prof_mark(c, -1);
     hipLaunchKernelGGL(k_sbuild, dim3(totN), dim3(WGS), 0, s, dmg, (const int *)t.chunkK, nm, (const MergeOut *)dmo,
                        (const long *)t.soff, W, (double *)c->sb.p);
     if (B.tiles > 0)
@@ -1171,8 +1236,10 @@ static int level_hss(hsdc_ctx *c, const Merge *dmg, const std::vector<Merge> &hm
     gate_open(c);
     if (sel.empty()) return 0;
     Work W = c->W;
+    prof_mark(c, 1);
     hipLaunchKernelGGL(k_alpha, dim3(totN), dim3(WGS), 0, s, dmg, (const int *)t.chunkK, nm, (const MergeOut *)dmo,
                        W);
+    prof_mark(c, -1);
     CK(run_hss(c, dmg, hm, 0, ho, sel, dprobs, cur, gbase));  // writes the new vectors into their slots
     hipLaunchKernelGGL(k_scatter, dim3(dev_grid(B.cols)), dim3(WGS), 0, s, dmg, (const int *)t.colK, nm,
                        (const MergeOut *)dmo, (const long *)t.coff, dprobs, W, cur ^ 1, (const double *)c->ct.p, 2);
@@ -1219,6 +1286,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         HK(hipHostGetDevicePointer((void **)&c->drec, c->rec, 0));
     }
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    c->pused = 0;
     HK(hipEventRecord(e0, s));
     CK(alloc_work(c, ntot));
     // Q storage: one n x n block per problem
@@ -1548,6 +1616,7 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     c->st.secular_iters = (long long)R.iters;
     c->st.learned_roots = (long long)R.used;
     c->st.secular_wave_iters = (long long)R.wave_iters;
+    c->st.secular_evals = (long long)R.evals;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     c->st.ms_total = ms;
@@ -1560,6 +1629,14 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
         c->st.final_bytes = 8LL * (zrows_written + (long long)rows) + 16LL * ntot;
     }
     if (c->prm.profile) {
+        prof_collect(c);
+        // (i, j) pairs of the Loewner and alpha sweeps, from the merges' K and branch
+        std::vector<int> tk(c->trc_mg.size()), th(c->trc_mg.size());
+        const int ntr = hsdc_trace(c, nullptr, nullptr, tk.data(), nullptr, th.data(), nullptr, (int)tk.size());
+        for (int i = 0; i < ntr; i++) {
+            const long long k = tk[i];
+            if (k >= 3) c->st.loewner_pairs += k * k * (th[i] ? 2 : 1);
+        }
         float fm = 0, zm = 0;
         if (hipEventElapsedTime(&fm, f0, f1) != hipSuccess) { (void)hipGetLastError(); fm = 0; }
         c->st.ms_final = fm;

@@ -24,6 +24,7 @@ struct HssMerge {
     double *scratch;        // private copies of the compressed block rows / columns
     unsigned long long *scratch_top;
     long scratch_cap;
+    unsigned long long *flops;  // the RRLU flop count of the reference's nflops model
     int *err;               // bit0 block too large, bit1 arena full, bit2 scratch full
     int K, ltr, lvl;
     int sq;                 // 1: u_i v_j / (f_i^2 - d_j^2), the SVD's vector matrices

@@ -196,8 +196,13 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     const double rho = 1.5;
     bool pre = false;     // pre_rw holds this step's first column search
     VI pre_rw = {0.0, 0};
+    // flop count of the reference's model (nflops of rrluCauchyEigrow and its pivot
+    // searches, CauchyHssEig.f90:105, 147, 205, 680-703); the loop is uniform over the
+    // block, thread 0 adds it to the merge's total
+    unsigned long long fl = 0;
     for (int k = 1; k <= mn; k++) {
         // ---- CauchyPivtEig: rook search on the k-th Schur complement
+        fl += 4ull * (M + N - 2 * k + 3);
         {
             VI x = {0.0, 0};
             int pwk = A1(PU, k);
@@ -244,6 +249,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                             y2 = vi_better(y2, VI{v, j - k + 1});
                         }
                         jjU = block_argmax(y2, red).i;
+                        fl += 4ull * (N - k + 1);
                         if (jjU == 1) break;
                         flg = 0;
                     }
@@ -258,6 +264,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                         x2 = vi_better(x2, VI{v, i - k + 1});
                     }
                     jjL = block_argmax(x2, red).i;
+                    fl += 4ull * (M - k + 1);
                     if (jjL == 1) break;
                     flg = 1;
                 }
@@ -268,6 +275,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
             // ---- CauchyPivtEig_CP: complete pivoting over the Schur complement
+            fl += 4ull * (M - k + 1) * (N - k + 1);
             VI best = {0.0, 0};
             VI bestrow = {0.0, 0};
             for (int j = k + tid; j <= N; j += HSS_CWG) {
@@ -338,6 +346,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             A1(W, k) = wk;
         }
         __syncthreads();
+        fl += 5ull * (M - k) + 7ull * (N - k) + 12ull * (k - 1);
         pre = fuse;
         if (fuse) pre_rw = block_argmax(y1, red);
         // ---- every 10 steps: swap while max |Mat_N| > rho (searchMax2Eig)
@@ -360,6 +369,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 VI win = block_argmax(x, red);
                 if (!(win.v > rho)) break;
                 const int ii = win.i, jj = IP[ii - 1] + k;
+                fl += 10ull * (N - k) + 8ull * M + 18ull * k;
                 __syncthreads();
                 const double Fii = A1(F, ii), Fjj = A1(F, jj);
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
@@ -403,6 +413,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     }
     // ---- U generator (CauchylikeEigQ, then rows back to the node order)
     if (tid == 0) {
+        atomicAdd(H.flops, fl);
         for (int i = 1; i <= M; i++) IP[A1(PQ, i) - 1] = i;  // invp
         int r = Rk;
         if (r < mn) {
@@ -501,8 +512,12 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     const double rho = 1.5;
     bool pre = false;     // pre_rw holds this step's first column search
     VI pre_rw = {0.0, 0};
+    // flop count of the reference's model (rrluCauchyEigcol, CauchyHssEig.f90:321, 371,
+    // 460, and its pivot searches), added to the merge's total by thread 0
+    unsigned long long fl = 0;
     for (int k = 1; k <= mn; k++) {
         // ---- CauchyPivtEigm
+        fl += 4ull * (M + N - 2 * k + 3);
         {
             VI x = {0.0, 0};
             int puk = A1(PU, k);
@@ -554,6 +569,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                             y2 = vi_better(y2, VI{v, j - k + 1});
                         }
                         jjU = block_argmax(y2, red).i;
+                        fl += 4ull * (N - k + 1);
                         if (jjU == 1) break;
                         flg = 0;
                     }
@@ -572,6 +588,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                         x2 = vi_better(x2, VI{v, i - k + 1});
                     }
                     jjL = block_argmax(x2, red).i;
+                    fl += 4ull * (M - k + 1);
                     if (jjL == 1) break;
                     flg = 1;
                 }
@@ -582,6 +599,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
             // ---- CauchyPivtEigm_CP
+            fl += 4ull * (M - k + 1) * (N - k + 1);
             VI best = {0.0, 0};
             int bestrow = 0;
             #pragma unroll 8
@@ -653,6 +671,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             A1(W, k) = wk;
         }
         __syncthreads();
+        fl += 6ull * (M - k) + 5ull * (N - k) + 14ull * (k - 1);
         pre = fuse;
         if (fuse) pre_rw = block_argmax(y1, red);
         if (k % 10 == 0 && M - k > 0) {
@@ -675,6 +694,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 VI win = block_argmax(x, red);
                 if (!(win.v > rho)) break;
                 const int ii = win.i, jj = IP[ii - 1] + k;
+                fl += 10ull * (N - k) + 12ull * M + 22ull * k;
                 __syncthreads();
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
                 #pragma unroll 8
@@ -727,6 +747,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     }
     // ---- V generator (CauchylikeEigQ2, columns back to the node order)
     if (tid == 0) {
+        atomicAdd(H.flops, fl);
         for (int i = 1; i <= M; i++) IP[A1(PQ, i) - 1] = i;
         if (Rk < mn) {
             s_Rk = Rk;

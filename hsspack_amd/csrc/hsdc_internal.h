@@ -135,7 +135,7 @@ struct LvlTab {
 // INFO, rows read by the final copy, level statistics and secular counters.
 struct SolveRec {
     int gate, info;
-    unsigned long long rows, roots, deflated, hss, ktop, iters, used, wave_iters, crit;
+    unsigned long long rows, roots, deflated, hss, ktop, iters, used, wave_iters, crit, evals;
 };
 
 // control block of the secular kernel: learned-start weights, statistics and
@@ -147,6 +147,7 @@ struct SecCtl {
     unsigned long long *iters;   // total iterations [SEC_SLOTS]
     unsigned long long *used;    // roots started from the learned iterate [SEC_SLOTS]
     unsigned long long *wave_iters;  // sum over wavefronts of their largest iteration count [SEC_SLOTS]
+    unsigned long long *evals;   // pole evaluations: sum over roots of K x iterations [SEC_SLOTS]
     float *hf, *ht;              // harvest: features (cap x 34) and targets (cap)
     float *feat;                 // harvest: features of every root (34 per global index)
     unsigned long long *count;

@@ -1190,7 +1190,7 @@ __global__ __launch_bounds__(SEC_WG) void k_secstart(const Merge *mg, const int 
 // training, up to sec->cap samples.  block_start[m]: first block of merge m
 // (SEC_WG / sec_team(K) roots per block).
 struct SecAcc {  // statistics of one wavefront (held by its lane 0)
-    unsigned long long iters, used, wave_iters;
+    unsigned long long iters, used, wave_iters, evals;
     unsigned int itmax;
 };
 
@@ -1260,6 +1260,7 @@ __device__ void secular_block(const Merge *mg, const int *block_start, int nm, c
     acc.iters += (unsigned long long)sum;
     acc.used += (unsigned long long)nu;
     acc.wave_iters += (unsigned long long)mx;
+    acc.evals += (unsigned long long)k * (unsigned long long)sum;  // pole evaluations: K per iteration
     acc.itmax = max(acc.itmax, (unsigned int)mx);
 }
 
@@ -1272,9 +1273,9 @@ __global__ __launch_bounds__(SEC_WG) void k_secular(const Merge *mg, const int *
 {
     if (gate_closed(W, mg)) return;
     __shared__ double sd[SEC_LDS], sz[SEC_LDS];
-    __shared__ unsigned long long red[SEC_WG / 64][3];
+    __shared__ unsigned long long red[SEC_WG / 64][4];
     __shared__ unsigned int redm[SEC_WG / 64];
-    SecAcc acc = {0, 0, 0, 0};
+    SecAcc acc = {0, 0, 0, 0, 0};
     const int nvb = block_start[nm];
     for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
         secular_block(mg, block_start, nm, mo, W, maxit, info, sec, acc, vb, sd, sz);
@@ -1282,14 +1283,14 @@ __global__ __launch_bounds__(SEC_WG) void k_secular(const Merge *mg, const int *
     }
     const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        red[wv][0] = acc.iters; red[wv][1] = acc.used; red[wv][2] = acc.wave_iters;
+        red[wv][0] = acc.iters; red[wv][1] = acc.used; red[wv][2] = acc.wave_iters; red[wv][3] = acc.evals;
         redm[wv] = acc.itmax;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        SecAcc t = {0, 0, 0, 0};
+        SecAcc t = {0, 0, 0, 0, 0};
         for (int w = 0; w < SEC_WG / 64; w++) {
-            t.iters += red[w][0]; t.used += red[w][1]; t.wave_iters += red[w][2];
+            t.iters += red[w][0]; t.used += red[w][1]; t.wave_iters += red[w][2]; t.evals += red[w][3];
             t.itmax = max(t.itmax, redm[w]);
         }
         const SecCtl C = *sec;
@@ -1297,6 +1298,7 @@ __global__ __launch_bounds__(SEC_WG) void k_secular(const Merge *mg, const int *
         if (t.iters) atomicAdd(C.iters + slot, t.iters);
         if (t.used) atomicAdd(C.used + slot, t.used);
         if (t.wave_iters) atomicAdd(C.wave_iters + slot, t.wave_iters);
+        if (t.evals) atomicAdd(C.evals + slot, t.evals);
         if (t.itmax) atomicMax(itmax + slot, t.itmax);
     }
 }
@@ -1943,7 +1945,8 @@ __global__ __launch_bounds__(64) void k_readback(Work W, const int *info, const 
 {
     const int lane = threadIdx.x;
     const SecCtl C = *sec;
-    unsigned long long r = rows[lane], it = C.iters[lane], us = C.used[lane], wi = C.wave_iters[lane];
+    unsigned long long r = rows[lane], it = C.iters[lane], us = C.used[lane], wi = C.wave_iters[lane],
+                       ev = C.evals[lane];
     unsigned long long crit = 0;
     for (int l = 0; l < SEC_LEVELS; l++) {
         unsigned int m = lvlmax[l * SEC_SLOTS + lane];
@@ -1955,13 +1958,14 @@ __global__ __launch_bounds__(64) void k_readback(Work W, const int *info, const 
         it += __shfl_xor(it, off);
         us += __shfl_xor(us, off);
         wi += __shfl_xor(wi, off);
+        ev += __shfl_xor(ev, off);
     }
     if (lane == 0) {
         out->gate = *W.gate;
         out->info = *info;
         out->rows = r;
         out->roots = W.lstat[0]; out->deflated = W.lstat[1]; out->hss = W.lstat[2]; out->ktop = W.lstat[3];
-        out->iters = it; out->used = us; out->wave_iters = wi; out->crit = crit;
+        out->iters = it; out->used = us; out->wave_iters = wi; out->crit = crit; out->evals = ev;
     }
 }
 

@@ -53,7 +53,9 @@ class Stats(ctypes.Structure):
                 ("final_bytes", ctypes.c_longlong), ("learned_roots", ctypes.c_longlong),
                 ("secular_wave_iters", ctypes.c_longlong), ("secular_crit_iters", ctypes.c_longlong),
                 ("hss_mm_flops", ctypes.c_longlong), ("ms_hss_mm", _d), ("ms_zfill", _d),
-                ("zfill_bytes", ctypes.c_longlong)]
+                ("zfill_bytes", ctypes.c_longlong), ("secular_evals", ctypes.c_longlong),
+                ("ms_secular", _d), ("ms_loewner", _d), ("loewner_pairs", ctypes.c_longlong),
+                ("hss_nflops", ctypes.c_longlong), ("ms_hss_build", _d)]
 
 
 class BdParams(ctypes.Structure):

@@ -66,6 +66,16 @@ typedef struct {
     double ms_hss_mm;        /* their device time (events, profile = 1), ms */
     double ms_zfill;         /* zero fill of Z, run beside the DC levels (events, profile = 1), ms */
     long long zfill_bytes;   /* its bytes: 8 n^2 (device-resident single-output solves; else 0) */
+    long long secular_evals; /* pole evaluations of the secular solves: sum over roots of K x iterations
+                                (16 bytes each: one pole, one z) */
+    double ms_secular;       /* device time of the secular launches (profile = 1), ms */
+    double ms_loewner;       /* of the Loewner z and alpha launches (profile = 1), ms */
+    long long loewner_pairs; /* (i, j) pairs those launches visit: K^2 per merge for the Loewner
+                                products, K^2 more per HSS merge for alpha (16 bytes each) */
+    long long hss_nflops;    /* the HSS construction's flops in HSSPACK's model: nflops of
+                                rrluCauchyEigrow / rrluCauchyEigcol and their pivot searches
+                                (CauchyHssEig.f90:105-205, 321-460, 680-703) */
+    double ms_hss_build;     /* device time of the HSS construction (profile = 1), ms */
 } hsdc_stats;
 
 void hsdc_default_params(hsdc_params *p);

@@ -474,6 +474,7 @@ static int ql_core(int n, double *d, double *e, double *z, int ldz, int want)
 int orc_steqr(int n, double *d, double *e, double *z, int ldz)
 {
     if (n <= 0) return 0;
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < n; i++) z[i + (long)j * ldz] = (i == j) ? 1.0 : 0.0;
     if (n == 1) return 0;
@@ -717,6 +718,9 @@ L100:
 static void gemm_nn(int m, int n, int k, const double *a, int lda, const double *b, int ldb,
                     double *c, int ldc)
 {
+    /* columns of C are independent: the threads split them, each column summed in
+     * the same order as one thread would (the results do not depend on the count) */
+#pragma omp parallel for schedule(static) if ((long)m * n * k > 4000000L)
     for (int j = 0; j < n; j++) {
         double *cj = c + (long)j * ldc;
         for (int i = 0; i < m; i++) cj[i] = 0.0;
@@ -988,13 +992,15 @@ int orc_laed0(int n, double *d, double *e, double *q, int ldq, double *work, int
         curlvl++;
     }
     if (tr && tr->indxq && tr->indxq_n == n) memcpy(tr->indxq, &iwork[indxq], sizeof(int) * n);
-    for (i = 1; i <= n; i++) {
-        j = iwork[indxq + i - 1];
-        work[i - 1] = d[j - 1];
-        memcpy(work + (long)n * i, &q[(long)(j - 1) * ldq], sizeof(double) * n);
+#pragma omp parallel for schedule(static)
+    for (int ii = 1; ii <= n; ii++) {
+        const int jj = iwork[indxq + ii - 1];
+        work[ii - 1] = d[jj - 1];
+        memcpy(work + (long)n * ii, &q[(long)(jj - 1) * ldq], sizeof(double) * n);
     }
     memcpy(d, work, sizeof(double) * n);
-    for (j = 0; j < n; j++) memcpy(&q[(long)j * ldq], work + (long)n * (j + 1), sizeof(double) * n);
+#pragma omp parallel for schedule(static)
+    for (int jj = 0; jj < n; jj++) memcpy(&q[(long)jj * ldq], work + (long)n * (jj + 1), sizeof(double) * n);
     return 0;
 L130:
     return submat * (n + 1) + submat + matsiz - 1;
@@ -1049,6 +1055,7 @@ int orc_stedc(char compz, int n, double *d, double *e, double *z, int ldz, doubl
         return info;
     }
     if (n <= smlsiz) return orc_steqr(n, d, e, z, ldz);
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < n; i++) z[i + (long)j * ldz] = (i == j) ? 1.0 : 0.0;
     double orgnrm = 0.0;

@@ -805,6 +805,8 @@ int orc_hss_build_sq(orc_hss *H, int K, int ni, const double *f, const double *d
 static void gemm_acc(int m, int n, int k, const double *a, int lda, const double *b, int ldb,
                      double beta, double *c, int ldc)
 {
+    /* independent columns split over the threads; each summed in one thread's order */
+#pragma omp parallel for schedule(static) if ((long)m * n * k > 4000000L)
     for (int j = 0; j < n; j++) {
         double *cj = c + (long)j * ldc;
         if (beta == 0.0) for (int i = 0; i < m; i++) cj[i] = 0.0;

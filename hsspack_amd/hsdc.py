@@ -32,7 +32,7 @@ EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_s
            "hsdc_mlp_load", "hsdc_mlp_harvest", "hsdc_mlp_samples", "hsdc_mlp_grad", "hsdc_mlp_train",
            "hsdc_bd_default_params", "hsdc_bd_ctx_create", "hsdc_bd_ctx_destroy", "hsdc_bd_set_params",
            "hsdc_bd_get_stats", "hsdc_bd_set_stream", "hsdc_bdsdc", "hsdc_bdsdc_dev", "hsdc_bd_trace",
-           "hsdc_trace", "hsdc_trace_indxq"]
+           "hsdc_trace", "hsdc_trace_indxq", "hsdc_loewner_noalloc"]
 
 # learned secular start: 34 inputs -> 64 -> 64 -> 1
 MLP_IN, MLP_H = 34, 64
@@ -133,6 +133,7 @@ def lib():
         L.hsdc_bd_trace.argtypes = [_vp, _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _i]
         L.hsdc_trace.argtypes = [_vp, _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _i]
         L.hsdc_trace_indxq.argtypes = [_vp, _P(_i), _i]
+        L.hsdc_loewner_noalloc.argtypes = [_i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         _LIB = L
     return _LIB
 
@@ -187,6 +188,19 @@ def default_params():
     p = Params()
     lib().hsdc_default_params(ctypes.byref(p))
     return p
+
+
+NOALLOC_KMAX = 65536
+
+
+def loewner_noalloc(k, rho, dlamda, z, d, znew, difl, difr, alpha, info, stream=None):
+    """hsdc_loewner_noalloc on torch device tensors (float64 of k, info int32 of 1):
+    allocates nothing, queues on `stream` (torch stream, raw handle or None) and
+    returns the status without waiting."""
+    h = getattr(stream, "cuda_stream", stream)
+    return lib().hsdc_loewner_noalloc(int(k), float(rho), dlamda.data_ptr(), z.data_ptr(), d.data_ptr(),
+                                      znew.data_ptr(), difl.data_ptr(), difr.data_ptr(), alpha.data_ptr(),
+                                      info.data_ptr(), ctypes.c_void_p(h or 0))
 
 
 class Context:

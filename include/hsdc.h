@@ -161,6 +161,23 @@ int hsdc_laed3(hsdc_ctx *ctx, int k, int n, int n1, double *d, double *q, int ld
 int hsdc_loewner(hsdc_ctx *ctx, int k, double rho, double *dlamda, double *z, double *d,
                  double *difl, double *difr, double *alpha);
 
+/* No counterpart in HSSPACK.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * hsdc_loewner on device arrays with no context, allocating nothing.  Inputs
+ * d_dlamda(k) (poles, ascending), d_z(k), rho; outputs d_d(k) roots, d_znew(k)
+ * Loewner z, d_difl(k), d_difr(k), d_alpha(k), *d_info (0, or nonzero when a root
+ * did not converge), all device arrays; the classical starting iterate, at most 100
+ * iterations per root.  Work is queued on `stream` (hipStream_t, 0 = null stream)
+ * and the call returns without waiting.  Its scratch (per root: tau, origin,
+ * iteration count; a control block) is one static device array per GPU of
+ * 16 KB + 16 bytes x HSDC_NOALLOC_KMAX (about 1 MB), held until the process ends:
+ * k > HSDC_NOALLOC_KMAX returns -1, as does k < 3.  Calls on one GPU share that
+ * array, so calls that overlap on one GPU (other streams, other threads) race. */
+#define HSDC_NOALLOC_KMAX 65536
+int hsdc_loewner_noalloc(int k, double rho, const double *d_dlamda, const double *d_z, double *d_d,
+                         double *d_znew, double *d_difl, double *d_difr, double *d_alpha, int *d_info,
+                         void *stream);
+
 /* DHSSEVC (CauchyHssEig.f90:2444): build the HSS form of
  * C(i,j) = u(i) v(j) / (f(i) - d(j)) (accurate differences from difl/difr) with
  * leaf block ni and tolerance tol, then q(1:nrow, 1:k) := q * C (ldq).  On return

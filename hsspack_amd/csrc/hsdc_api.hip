@@ -55,6 +55,9 @@ __global__ void k_final(const Prob *, const int *, Work, const int *, double *, 
                         double *, const double *, unsigned long long *, const int *, int, int, int);
 __global__ void k_zfill(double *, long, long, long);
 __global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, long, double *, long);
+__global__ void k_transpose(int, int, const double *, long, double *, long);
+__global__ void k_cauchy2hss(C2H);
+__global__ void k_cfill_general(int, const double *, const double *, const double *, const double *, double *);
 __global__ void k_hss_init(const HssMerge *);
 __global__ void k_hss_reset(const HssMerge *);
 __global__ void k_hss_dgen(const HssMerge *, const HssNode *, int);
@@ -143,7 +146,7 @@ struct hsdc_ctx {
     hipStream_t stream = 0;
     // device storage
     DevBuf q, ct, sb, din, ein, wsd, wsi, tabs, lvl, mo, probs, misc, zout, dout;
-    DevBuf hss, htab, xb, gf, cb, yb;
+    DevBuf hss, htab, xb, gf, cb, yb, tb;
     // generators of the last hsdc_hss_evc call (host copies), for hsdc_hss_dense
     std::vector<double> evc_gen;
     int evc_k = 0, evc_ni = 0, evc_sq = 0;
@@ -413,251 +416,14 @@ struct HssPlanM {
     std::vector<int> lg, lf;
 };
 
-static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double tol,
-                     std::vector<HssPlanM> *keep, bool left = false)
+// The product of a batch of HSS matrices whose generators are on the device (P: tree,
+// generator directory, arena and leaf blocks of each; a plan that is not ok takes the
+// dense Cauchy product of its HssMerge): Y := X * H, or for left H X as (X^T H^T)^T on
+// X^T.  fastHssmmL_omp (ConstructHssd.f90:1394-1630) and fastHssmm (:926-1155) as
+// batched FP64 MFMA GEMMs, one launch per tree step over every matrix.
+static int hss_mult(hsdc_ctx *c, const std::vector<HssSpec> &sp, std::vector<HssPlanM> &P, bool left)
 {
     hipStream_t s = c->stream;
-    std::vector<HssPlanM> P(sp.size());
-    // one batch is all eigenvector matrices or all SVD vector matrices
-    const int sq = sp.empty() ? 0 : sp[0].sq;
-    for (const HssSpec &q : sp)
-        if (q.sq != sq) return HSDC_ERR_INTERNAL;
-    // ---- device memory for every merge
-    size_t tot = 0;
-    std::vector<size_t> base(sp.size()), metasz(sp.size());
-    for (size_t h = 0; h < sp.size(); h++) {
-        P[h].li = (int)h;
-        int K = sp[h].K;
-        P[h].ok = hss_make_tree(P[h].T, K, ni);
-        int ltr = std::max(P[h].T.ltr, 1);
-        long dds = 0;
-        for (int x : P[h].T.m) dds += (long)x * x;
-        size_t o = al64(6 * sizeof(double) * K) + al64(2 * sizeof(int) * K) + al64(sizeof(int) * (11 * ltr + 8)) +
-                   al64(sizeof(long) * (4 * ltr)) + al64(sizeof(unsigned long long) * 3);
-        metasz[h] = o;
-        o += al64(sizeof(double) * std::max(dds, 1L));
-        o += al64(sizeof(double) * (size_t)12 * std::max(ni, 128) * K);  // arena
-        o += al64(sizeof(double) * (size_t)8 * K * std::max(P[h].T.lm, 1));  // scratch
-        base[h] = tot;
-        tot += o;
-    }
-    CK(ensure(c->hss, tot));
-    char *hb = (char *)c->hss.p;
-    for (size_t h = 0; h < sp.size(); h++) {
-        HssPlanM &Q = P[h];
-        int K = sp[h].K, ltr = std::max(Q.T.ltr, 1);
-        char *p = hb + base[h];
-        HssMerge &H = Q.H;
-        H.FF = sp[h].FF; H.D0 = sp[h].D0; H.U0 = sp[h].U0; H.V0 = sp[h].V0;
-        H.DIFL = sp[h].DIFL; H.DIFR = sp[h].DIFR;
-        double *dp = (double *)p;
-        H.D = dp; H.U = dp + K; H.V = dp + 2 * K; H.T = dp + 3 * K;
-        p += al64(6 * sizeof(double) * K);
-        int *ip = (int *)p;
-        H.PL = ip; H.PU = ip + K;
-        p += al64(2 * sizeof(int) * K);
-        ip = (int *)p;
-        char *meta0 = p;
-        H.lr = ip; H.lc = ip + 2 * ltr; H.usz = ip + 4 * ltr; H.vsz = ip + 6 * ltr; H.bsz = ip + 8 * ltr;
-        H.dsz = ip + 10 * ltr; H.lda = ip + 11 * ltr; H.lca = ip + 11 * ltr + 1; H.err = ip + 11 * ltr + 2;
-        p += al64(sizeof(int) * (11 * ltr + 8));
-        long *lp = (long *)p;
-        H.pu = lp; H.pv = lp + ltr; H.pb = lp + 2 * ltr; H.pd = lp + 3 * ltr;
-        p += al64(sizeof(long) * (4 * ltr));
-        H.arena_top = (unsigned long long *)p; H.scratch_top = H.arena_top + 1; H.flops = H.arena_top + 2;
-        p += al64(sizeof(unsigned long long) * 3);
-        HK(hipMemsetAsync(meta0, 0, p - meta0, s));
-        H.dd = (double *)p;
-        long dds = 0;
-        for (int x : Q.T.m) dds += (long)x * x;
-        p += al64(sizeof(double) * std::max(dds, 1L));
-        H.arena = (double *)p; H.arena_cap = (long)12 * std::max(ni, 128) * K;
-        p += al64(sizeof(double) * (size_t)12 * std::max(ni, 128) * K);
-        H.scratch = (double *)p; H.scratch_cap = (long)8 * K * std::max(Q.T.lm, 1);
-        H.K = K; H.ltr = Q.T.ltr; H.lvl = Q.T.lvl; H.sq = sp[h].sq;
-    }
-    // ---- host tables: merges, leaves (with leaf lr/lc and D offsets), level steps
-    std::vector<HssMerge> hh;
-    std::vector<int> hmap;  // table index -> plan index
-    for (size_t h = 0; h < P.size(); h++) if (P[h].ok) { hmap.push_back((int)h); hh.push_back(P[h].H); }
-    const int nh = (int)hh.size();
-    int maxlvl = 0;
-    for (int t = 0; t < nh; t++) maxlvl = std::max(maxlvl, P[hmap[t]].T.lvl);
-    std::vector<HssNode> nodes;   // leaves first, then per step
-    std::vector<int> init_lrlc;   // per merge: 2*ltr lr then 2*ltr lc (leaves set, others 0)
-    std::vector<long> init_pd;
-    size_t leaf0 = 0, nleaf = 0;
-    std::vector<size_t> step_off(maxlvl + 1), step_cnt(maxlvl + 1);
-    std::vector<HssCompact> comps;
-    std::vector<int> order;
-    std::vector<size_t> comp_off(maxlvl + 1), comp_cnt(maxlvl + 1);
-    for (int t = 0; t < nh; t++) {
-        const HssTree &T = P[hmap[t]].T;
-        long pdt = 0;
-        int it = 0;
-        for (int i = 1; i <= T.ltr; i++)
-            if (T.c1[i] == 0) {
-                HssNode nd;
-                nd.mrg = t; nd.node = i; nd.c1 = 0; nd.c2 = 0; nd.sib = 0;
-                nd.leaf_r0 = T.leaf_r0[i]; nd.leaf_m = T.m[it++];
-                nodes.push_back(nd);
-            }
-        (void)pdt;
-    }
-    nleaf = nodes.size();
-    for (int st = 0; st < maxlvl; st++) {
-        step_off[st] = nodes.size();
-        comp_off[st] = comps.size();
-        for (int t = 0; t < nh; t++) {
-            const HssTree &T = P[hmap[t]].T;
-            int LL = T.lvl - st;
-            if (LL < 1) continue;
-            for (int nd_id : T.lev[LL]) {
-                HssNode nd;
-                nd.mrg = t; nd.node = nd_id; nd.c1 = T.c1[nd_id]; nd.c2 = T.c2[nd_id];
-                int par = T.tr[nd_id - 1];
-                nd.sib = (T.c1[par] == nd_id) ? T.c2[par] : T.c1[par];
-                nd.leaf_r0 = T.leaf_r0[nd_id]; nd.leaf_m = 0;
-                nodes.push_back(nd);
-            }
-            HssCompact cp;
-            cp.mrg = t; cp.start = (int)order.size(); cp.nlevel = (int)T.lev[LL].size();
-            for (int nd_id : T.lev[LL]) order.push_back(nd_id);
-            for (int LLt = LL - 1; LLt >= 1; LLt--)
-                for (int nd_id : T.lev[LLt]) if (T.c1[nd_id] == 0) order.push_back(nd_id);
-            cp.count = (int)order.size() - cp.start;
-            comps.push_back(cp);
-        }
-        step_cnt[st] = nodes.size() - step_off[st];
-        comp_cnt[st] = comps.size() - comp_off[st];
-    }
-    // initial leaf ranges and D offsets go straight into each merge's arrays
-    for (int t = 0; t < nh; t++) {
-        const HssTree &T = P[hmap[t]].T;
-        std::vector<int> lrlc(4 * T.ltr, 0);
-        std::vector<long> pd(T.ltr, 0);
-        long pdt = 0;
-        int it = 0;
-        for (int i = 1; i <= T.ltr; i++)
-            if (T.c1[i] == 0) {
-                lrlc[2 * (i - 1)] = T.leaf_r0[i] + 1;
-                lrlc[2 * (i - 1) + 1] = T.leaf_r0[i] + T.m[it];
-                pd[i - 1] = pdt;
-                pdt += (long)T.m[it] * T.m[it];
-                it++;
-            }
-        for (int i = 0; i < 2 * T.ltr; i++) lrlc[2 * T.ltr + i] = lrlc[i];
-        HK(hipMemcpyAsync(hh[t].lr, lrlc.data(), sizeof(int) * 4 * T.ltr, hipMemcpyHostToDevice, s));
-        HK(hipMemcpyAsync(hh[t].pd, pd.data(), sizeof(long) * T.ltr, hipMemcpyHostToDevice, s));
-        HK(hipStreamSynchronize(s));
-    }
-    size_t t0 = 0, t1 = al64(t0 + sizeof(HssMerge) * std::max(nh, 1)),
-           t2 = al64(t1 + sizeof(HssNode) * std::max(nodes.size(), (size_t)1)),
-           t3 = al64(t2 + sizeof(HssCompact) * std::max(comps.size(), (size_t)1)),
-           t4 = al64(t3 + sizeof(int) * std::max(order.size(), (size_t)1));
-    CK(ensure(c->htab, t4));
-    {
-        std::vector<char> hbuf(t4);
-        memcpy(hbuf.data() + t0, hh.data(), sizeof(HssMerge) * nh);
-        memcpy(hbuf.data() + t1, nodes.data(), sizeof(HssNode) * nodes.size());
-        if (!comps.empty()) memcpy(hbuf.data() + t2, comps.data(), sizeof(HssCompact) * comps.size());
-        if (!order.empty()) memcpy(hbuf.data() + t3, order.data(), sizeof(int) * order.size());
-        HK(hipMemcpyAsync(c->htab.p, hbuf.data(), t4, hipMemcpyHostToDevice, s));
-        HK(hipStreamSynchronize(s));
-    }
-    char *tb = (char *)c->htab.p;
-    const HssMerge *dH = (const HssMerge *)(tb + t0);
-    const HssNode *dN = (const HssNode *)(tb + t1);
-    const HssCompact *dC = (const HssCompact *)(tb + t2);
-    const int *dO = (const int *)(tb + t3);
-    hipEvent_t b0 = nullptr, b1 = nullptr;
-    if (c->prm.profile && nh > 0) {
-        HK(hipEventCreate(&b0));
-        HK(hipEventCreate(&b1));
-        HK(hipEventRecord(b0, s));
-    }
-    if (nh > 0) {
-        int kmax = 0;
-        for (auto &H : hh) kmax = std::max(kmax, H.K);
-        hipLaunchKernelGGL(k_hss_init, dim3(std::min(cdiv(kmax, WGS), 256), nh), dim3(WGS), 0, s, dH);
-        hipLaunchKernelGGL(k_hss_dgen, dim3(nleaf), dim3(WGS), 0, s, dH, dN + leaf0, (int)nleaf);
-        // the 512-thread RRLU launches (fewer nodes than two per CU) keep each node's
-        // private generator column in LDS when it fits: 160 KB per workgroup on gfx950,
-        // 22.2 KB of it the kernels' static arrays
-        const int ldsn = std::min(kmax, HSS_LDS_V);
-        const size_t ldsb = sizeof(double) * (size_t)ldsn;
-        for (int st = 0; st < maxlvl; st++) {
-            int nn = (int)step_cnt[st], nc = (int)comp_cnt[st];
-            const HssNode *sn = dN + step_off[st];
-            const HssCompact *sc = dC + comp_off[st];
-            hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
-            if (sq && nn < 512)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, true>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
-                                   tol, ldsn);
-            else if (sq)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
-                                   0);
-            else if (nn < 512)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, false>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
-                                   tol, ldsn);
-            else
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
-                                   0);
-            hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 1);
-            hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
-            if (sq && nn < 512)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, true>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
-                                   tol, ldsn);
-            else if (sq)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
-                                   0);
-            else if (nn < 512)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, false>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
-                                   tol, ldsn);
-            else
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
-                                   0);
-            hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 0);
-            hipLaunchKernelGGL(k_hss_bgen, dim3(nn), dim3(WGS), 0, s, dH, sn, nn);
-        }
-        HK(hipGetLastError());
-    }
-    if (b1) HK(hipEventRecord(b1, s));
-    // ---- read back the generator directory
-    for (int t = 0; t < nh; t++) {
-        HssPlanM &Q = P[hmap[t]];
-        int ltr = Q.T.ltr;
-        std::vector<int> meta(11 * ltr + 8);
-        std::vector<long> offs(4 * ltr);
-        HK(hipMemcpyAsync(meta.data(), hh[t].lr, sizeof(int) * meta.size(), hipMemcpyDeviceToHost, s));
-        HK(hipMemcpyAsync(offs.data(), hh[t].pu, sizeof(long) * offs.size(), hipMemcpyDeviceToHost, s));
-        unsigned long long nfl = 0;
-        HK(hipMemcpyAsync(&nfl, hh[t].flops, sizeof(nfl), hipMemcpyDeviceToHost, s));
-        HK(hipStreamSynchronize(s));
-        c->st.hss_nflops += (long long)nfl;
-        Q.usz.assign(meta.begin() + 4 * ltr, meta.begin() + 6 * ltr);
-        Q.vsz.assign(meta.begin() + 6 * ltr, meta.begin() + 8 * ltr);
-        Q.bsz.assign(meta.begin() + 8 * ltr, meta.begin() + 10 * ltr);
-        Q.dsz.assign(meta.begin() + 10 * ltr, meta.begin() + 11 * ltr);
-        Q.err = meta[11 * ltr + 2];
-        Q.pu.assign(offs.begin(), offs.begin() + ltr);
-        Q.pv.assign(offs.begin() + ltr, offs.begin() + 2 * ltr);
-        Q.pb.assign(offs.begin() + 2 * ltr, offs.begin() + 3 * ltr);
-        Q.pd.assign(offs.begin() + 3 * ltr, offs.begin() + 4 * ltr);
-        if (Q.err) Q.ok = false;
-        for (int i = 0; i < ltr - 1; i++) {
-            c->st.rank_max = std::max(c->st.rank_max, Q.usz[2 * i + 1]);
-            c->st.rank_max = std::max(c->st.rank_max, Q.vsz[2 * i]);
-        }
-    }
-    if (b0) {
-        float ms = 0;
-        HK(hipStreamSynchronize(s));
-        if (hipEventElapsedTime(&ms, b0, b1) == hipSuccess) c->st.ms_hss_build += ms;
-        (void)hipGetLastError();
-        (void)hipEventDestroy(b0);
-        (void)hipEventDestroy(b1);
-    }
     // G / F column layout (fastHssmmL_omp lg / lf, ConstructHssd.f90:1472-1510)
     long gft = 0;
     std::vector<long> gbase(P.size()), fbase(P.size());
@@ -918,6 +684,255 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         (void)hipEventDestroy(m0);
         (void)hipEventDestroy(m1);
     }
+    return 0;
+}
+
+static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double tol,
+                     std::vector<HssPlanM> *keep, bool left = false)
+{
+    hipStream_t s = c->stream;
+    std::vector<HssPlanM> P(sp.size());
+    // one batch is all eigenvector matrices or all SVD vector matrices
+    const int sq = sp.empty() ? 0 : sp[0].sq;
+    for (const HssSpec &q : sp)
+        if (q.sq != sq) return HSDC_ERR_INTERNAL;
+    // ---- device memory for every merge
+    size_t tot = 0;
+    std::vector<size_t> base(sp.size()), metasz(sp.size());
+    for (size_t h = 0; h < sp.size(); h++) {
+        P[h].li = (int)h;
+        int K = sp[h].K;
+        P[h].ok = hss_make_tree(P[h].T, K, ni);
+        int ltr = std::max(P[h].T.ltr, 1);
+        long dds = 0;
+        for (int x : P[h].T.m) dds += (long)x * x;
+        size_t o = al64(6 * sizeof(double) * K) + al64(2 * sizeof(int) * K) + al64(sizeof(int) * (11 * ltr + 8)) +
+                   al64(sizeof(long) * (4 * ltr)) + al64(sizeof(unsigned long long) * 3);
+        metasz[h] = o;
+        o += al64(sizeof(double) * std::max(dds, 1L));
+        o += al64(sizeof(double) * (size_t)12 * std::max(ni, 128) * K);  // arena
+        o += al64(sizeof(double) * (size_t)8 * K * std::max(P[h].T.lm, 1));  // scratch
+        base[h] = tot;
+        tot += o;
+    }
+    CK(ensure(c->hss, tot));
+    char *hb = (char *)c->hss.p;
+    for (size_t h = 0; h < sp.size(); h++) {
+        HssPlanM &Q = P[h];
+        int K = sp[h].K, ltr = std::max(Q.T.ltr, 1);
+        char *p = hb + base[h];
+        HssMerge &H = Q.H;
+        H.FF = sp[h].FF; H.D0 = sp[h].D0; H.U0 = sp[h].U0; H.V0 = sp[h].V0;
+        H.DIFL = sp[h].DIFL; H.DIFR = sp[h].DIFR;
+        double *dp = (double *)p;
+        H.D = dp; H.U = dp + K; H.V = dp + 2 * K; H.T = dp + 3 * K;
+        p += al64(6 * sizeof(double) * K);
+        int *ip = (int *)p;
+        H.PL = ip; H.PU = ip + K;
+        p += al64(2 * sizeof(int) * K);
+        ip = (int *)p;
+        char *meta0 = p;
+        H.lr = ip; H.lc = ip + 2 * ltr; H.usz = ip + 4 * ltr; H.vsz = ip + 6 * ltr; H.bsz = ip + 8 * ltr;
+        H.dsz = ip + 10 * ltr; H.lda = ip + 11 * ltr; H.lca = ip + 11 * ltr + 1; H.err = ip + 11 * ltr + 2;
+        p += al64(sizeof(int) * (11 * ltr + 8));
+        long *lp = (long *)p;
+        H.pu = lp; H.pv = lp + ltr; H.pb = lp + 2 * ltr; H.pd = lp + 3 * ltr;
+        p += al64(sizeof(long) * (4 * ltr));
+        H.arena_top = (unsigned long long *)p; H.scratch_top = H.arena_top + 1; H.flops = H.arena_top + 2;
+        p += al64(sizeof(unsigned long long) * 3);
+        HK(hipMemsetAsync(meta0, 0, p - meta0, s));
+        H.dd = (double *)p;
+        long dds = 0;
+        for (int x : Q.T.m) dds += (long)x * x;
+        p += al64(sizeof(double) * std::max(dds, 1L));
+        H.arena = (double *)p; H.arena_cap = (long)12 * std::max(ni, 128) * K;
+        p += al64(sizeof(double) * (size_t)12 * std::max(ni, 128) * K);
+        H.scratch = (double *)p; H.scratch_cap = (long)8 * K * std::max(Q.T.lm, 1);
+        H.K = K; H.ltr = Q.T.ltr; H.lvl = Q.T.lvl; H.sq = sp[h].sq;
+    }
+    // ---- host tables: merges, leaves (with leaf lr/lc and D offsets), level steps
+    std::vector<HssMerge> hh;
+    std::vector<int> hmap;  // table index -> plan index
+    for (size_t h = 0; h < P.size(); h++) if (P[h].ok) { hmap.push_back((int)h); hh.push_back(P[h].H); }
+    const int nh = (int)hh.size();
+    int maxlvl = 0;
+    for (int t = 0; t < nh; t++) maxlvl = std::max(maxlvl, P[hmap[t]].T.lvl);
+    std::vector<HssNode> nodes;   // leaves first, then per step
+    std::vector<int> init_lrlc;   // per merge: 2*ltr lr then 2*ltr lc (leaves set, others 0)
+    std::vector<long> init_pd;
+    size_t leaf0 = 0, nleaf = 0;
+    std::vector<size_t> step_off(maxlvl + 1), step_cnt(maxlvl + 1);
+    std::vector<HssCompact> comps;
+    std::vector<int> order;
+    std::vector<size_t> comp_off(maxlvl + 1), comp_cnt(maxlvl + 1);
+    for (int t = 0; t < nh; t++) {
+        const HssTree &T = P[hmap[t]].T;
+        long pdt = 0;
+        int it = 0;
+        for (int i = 1; i <= T.ltr; i++)
+            if (T.c1[i] == 0) {
+                HssNode nd;
+                nd.mrg = t; nd.node = i; nd.c1 = 0; nd.c2 = 0; nd.sib = 0;
+                nd.leaf_r0 = T.leaf_r0[i]; nd.leaf_m = T.m[it++];
+                nodes.push_back(nd);
+            }
+        (void)pdt;
+    }
+    nleaf = nodes.size();
+    for (int st = 0; st < maxlvl; st++) {
+        step_off[st] = nodes.size();
+        comp_off[st] = comps.size();
+        for (int t = 0; t < nh; t++) {
+            const HssTree &T = P[hmap[t]].T;
+            int LL = T.lvl - st;
+            if (LL < 1) continue;
+            for (int nd_id : T.lev[LL]) {
+                HssNode nd;
+                nd.mrg = t; nd.node = nd_id; nd.c1 = T.c1[nd_id]; nd.c2 = T.c2[nd_id];
+                int par = T.tr[nd_id - 1];
+                nd.sib = (T.c1[par] == nd_id) ? T.c2[par] : T.c1[par];
+                nd.leaf_r0 = T.leaf_r0[nd_id]; nd.leaf_m = 0;
+                nodes.push_back(nd);
+            }
+            HssCompact cp;
+            cp.mrg = t; cp.start = (int)order.size(); cp.nlevel = (int)T.lev[LL].size();
+            for (int nd_id : T.lev[LL]) order.push_back(nd_id);
+            for (int LLt = LL - 1; LLt >= 1; LLt--)
+                for (int nd_id : T.lev[LLt]) if (T.c1[nd_id] == 0) order.push_back(nd_id);
+            cp.count = (int)order.size() - cp.start;
+            comps.push_back(cp);
+        }
+        step_cnt[st] = nodes.size() - step_off[st];
+        comp_cnt[st] = comps.size() - comp_off[st];
+    }
+    // initial leaf ranges and D offsets go straight into each merge's arrays
+    for (int t = 0; t < nh; t++) {
+        const HssTree &T = P[hmap[t]].T;
+        std::vector<int> lrlc(4 * T.ltr, 0);
+        std::vector<long> pd(T.ltr, 0);
+        long pdt = 0;
+        int it = 0;
+        for (int i = 1; i <= T.ltr; i++)
+            if (T.c1[i] == 0) {
+                lrlc[2 * (i - 1)] = T.leaf_r0[i] + 1;
+                lrlc[2 * (i - 1) + 1] = T.leaf_r0[i] + T.m[it];
+                pd[i - 1] = pdt;
+                pdt += (long)T.m[it] * T.m[it];
+                it++;
+            }
+        for (int i = 0; i < 2 * T.ltr; i++) lrlc[2 * T.ltr + i] = lrlc[i];
+        HK(hipMemcpyAsync(hh[t].lr, lrlc.data(), sizeof(int) * 4 * T.ltr, hipMemcpyHostToDevice, s));
+        HK(hipMemcpyAsync(hh[t].pd, pd.data(), sizeof(long) * T.ltr, hipMemcpyHostToDevice, s));
+        HK(hipStreamSynchronize(s));
+    }
+    size_t t0 = 0, t1 = al64(t0 + sizeof(HssMerge) * std::max(nh, 1)),
+           t2 = al64(t1 + sizeof(HssNode) * std::max(nodes.size(), (size_t)1)),
+           t3 = al64(t2 + sizeof(HssCompact) * std::max(comps.size(), (size_t)1)),
+           t4 = al64(t3 + sizeof(int) * std::max(order.size(), (size_t)1));
+    CK(ensure(c->htab, t4));
+    {
+        std::vector<char> hbuf(t4);
+        memcpy(hbuf.data() + t0, hh.data(), sizeof(HssMerge) * nh);
+        memcpy(hbuf.data() + t1, nodes.data(), sizeof(HssNode) * nodes.size());
+        if (!comps.empty()) memcpy(hbuf.data() + t2, comps.data(), sizeof(HssCompact) * comps.size());
+        if (!order.empty()) memcpy(hbuf.data() + t3, order.data(), sizeof(int) * order.size());
+        HK(hipMemcpyAsync(c->htab.p, hbuf.data(), t4, hipMemcpyHostToDevice, s));
+        HK(hipStreamSynchronize(s));
+    }
+    char *tb = (char *)c->htab.p;
+    const HssMerge *dH = (const HssMerge *)(tb + t0);
+    const HssNode *dN = (const HssNode *)(tb + t1);
+    const HssCompact *dC = (const HssCompact *)(tb + t2);
+    const int *dO = (const int *)(tb + t3);
+    hipEvent_t b0 = nullptr, b1 = nullptr;
+    if (c->prm.profile && nh > 0) {
+        HK(hipEventCreate(&b0));
+        HK(hipEventCreate(&b1));
+        HK(hipEventRecord(b0, s));
+    }
+    if (nh > 0) {
+        int kmax = 0;
+        for (auto &H : hh) kmax = std::max(kmax, H.K);
+        hipLaunchKernelGGL(k_hss_init, dim3(std::min(cdiv(kmax, WGS), 256), nh), dim3(WGS), 0, s, dH);
+        hipLaunchKernelGGL(k_hss_dgen, dim3(nleaf), dim3(WGS), 0, s, dH, dN + leaf0, (int)nleaf);
+        // the 512-thread RRLU launches (fewer nodes than two per CU) keep each node's
+        // private generator column in LDS when it fits: 160 KB per workgroup on gfx950,
+        // 22.2 KB of it the kernels' static arrays
+        const int ldsn = std::min(kmax, HSS_LDS_V);
+        const size_t ldsb = sizeof(double) * (size_t)ldsn;
+        for (int st = 0; st < maxlvl; st++) {
+            int nn = (int)step_cnt[st], nc = (int)comp_cnt[st];
+            const HssNode *sn = dN + step_off[st];
+            const HssCompact *sc = dC + comp_off[st];
+            hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
+            if (sq && nn < 512)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, true>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
+                                   tol, ldsn);
+            else if (sq)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
+                                   0);
+            else if (nn < 512)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<512, false>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
+                                   tol, ldsn);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_rowcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
+                                   0);
+            hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 1);
+            hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);
+            if (sq && nn < 512)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, true>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
+                                   tol, ldsn);
+            else if (sq)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, true>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
+                                   0);
+            else if (nn < 512)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<512, false>), dim3(nn), dim3(512), ldsb, s, dH, sn, nn,
+                                   tol, ldsn);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_hss_colcomp<256, false>), dim3(nn), dim3(256), 0, s, dH, sn, nn, tol,
+                                   0);
+            hipLaunchKernelGGL(k_hss_compact, dim3(nc), dim3(WGS), 0, s, dH, sc, dO, 0);
+            hipLaunchKernelGGL(k_hss_bgen, dim3(nn), dim3(WGS), 0, s, dH, sn, nn);
+        }
+        HK(hipGetLastError());
+    }
+    if (b1) HK(hipEventRecord(b1, s));
+    // ---- read back the generator directory
+    for (int t = 0; t < nh; t++) {
+        HssPlanM &Q = P[hmap[t]];
+        int ltr = Q.T.ltr;
+        std::vector<int> meta(11 * ltr + 8);
+        std::vector<long> offs(4 * ltr);
+        HK(hipMemcpyAsync(meta.data(), hh[t].lr, sizeof(int) * meta.size(), hipMemcpyDeviceToHost, s));
+        HK(hipMemcpyAsync(offs.data(), hh[t].pu, sizeof(long) * offs.size(), hipMemcpyDeviceToHost, s));
+        unsigned long long nfl = 0;
+        HK(hipMemcpyAsync(&nfl, hh[t].flops, sizeof(nfl), hipMemcpyDeviceToHost, s));
+        HK(hipStreamSynchronize(s));
+        c->st.hss_nflops += (long long)nfl;
+        Q.usz.assign(meta.begin() + 4 * ltr, meta.begin() + 6 * ltr);
+        Q.vsz.assign(meta.begin() + 6 * ltr, meta.begin() + 8 * ltr);
+        Q.bsz.assign(meta.begin() + 8 * ltr, meta.begin() + 10 * ltr);
+        Q.dsz.assign(meta.begin() + 10 * ltr, meta.begin() + 11 * ltr);
+        Q.err = meta[11 * ltr + 2];
+        Q.pu.assign(offs.begin(), offs.begin() + ltr);
+        Q.pv.assign(offs.begin() + ltr, offs.begin() + 2 * ltr);
+        Q.pb.assign(offs.begin() + 2 * ltr, offs.begin() + 3 * ltr);
+        Q.pd.assign(offs.begin() + 3 * ltr, offs.begin() + 4 * ltr);
+        if (Q.err) Q.ok = false;
+        for (int i = 0; i < ltr - 1; i++) {
+            c->st.rank_max = std::max(c->st.rank_max, Q.usz[2 * i + 1]);
+            c->st.rank_max = std::max(c->st.rank_max, Q.vsz[2 * i]);
+        }
+    }
+    if (b0) {
+        float ms = 0;
+        HK(hipStreamSynchronize(s));
+        if (hipEventElapsedTime(&ms, b0, b1) == hipSuccess) c->st.ms_hss_build += ms;
+        (void)hipGetLastError();
+        (void)hipEventDestroy(b0);
+        (void)hipEventDestroy(b1);
+    }
+    CK(hss_mult(c, sp, P, left));
     if (keep) *keep = P;
     return 0;
 }
@@ -1739,7 +1754,7 @@ int hsdc_ctx_destroy(hsdc_ctx *c)
     if (!c) return 0;
     DevBuf *bufs[] = {&c->q,  &c->ct,  &c->sb,   &c->din, &c->ein,  &c->wsd,  &c->wsi, &c->tabs,
                       &c->lvl, &c->mo, &c->probs, &c->misc, &c->zout, &c->dout, &c->hss, &c->htab,
-                      &c->xb, &c->gf,  &c->cb,   &c->yb,  &c->mlpw, &c->hv,   &c->sec, &c->mlptr,
+                      &c->xb, &c->gf,  &c->cb,   &c->yb, &c->tb,  &c->mlpw, &c->hv,   &c->sec, &c->mlptr,
                       &c->maux, &c->featb, &c->ltab};
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
@@ -1806,6 +1821,44 @@ int hsdc_trace_indxq(hsdc_ctx *c, int *indxq, int n)
     HK(hipMemcpyAsync(indxq, c->W.indxq, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
     HK(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+// Contexts owned by the calling host thread (created on first use, kept until
+// hsdc_thread_ctx_release or the end of the process).  Device buffers persist between
+// calls, so a repeated call of the same order allocates nothing.
+static thread_local hsdc_ctx *t_ctx = nullptr;
+static thread_local hsdc_bd_ctx *t_bd_ctx = nullptr;
+
+hsdc_ctx *hsdc_thread_ctx(void)
+{
+    if (!t_ctx && hsdc_ctx_create(&t_ctx) != 0) t_ctx = nullptr;
+    if (t_ctx) {
+        hsdc_params p;
+        hsdc_default_params(&p);
+        (void)hsdc_set_params(t_ctx, &p);
+        (void)hsdc_set_stream(t_ctx, nullptr);
+    }
+    return t_ctx;
+}
+
+hsdc_bd_ctx *hsdc_bd_thread_ctx(void)
+{
+    if (!t_bd_ctx && hsdc_bd_ctx_create(&t_bd_ctx) != 0) t_bd_ctx = nullptr;
+    if (t_bd_ctx) {
+        hsdc_bd_params p;
+        hsdc_bd_default_params(&p);
+        (void)hsdc_bd_set_params(t_bd_ctx, &p);
+        (void)hsdc_bd_set_stream(t_bd_ctx, nullptr);
+    }
+    return t_bd_ctx;
+}
+
+void hsdc_thread_ctx_release(void)
+{
+    if (t_ctx) (void)hsdc_ctx_destroy(t_ctx);
+    if (t_bd_ctx) (void)hsdc_bd_ctx_destroy(t_bd_ctx);
+    t_ctx = nullptr;
+    t_bd_ctx = nullptr;
 }
 
 int hsdc_set_stream(hsdc_ctx *c, void *stream)
@@ -2251,6 +2304,7 @@ static int hss_one(hsdc_ctx *c, int sq, int k, int ni, double tol, const double 
     if (!c) return HSDC_ERR_INTERNAL;
     if (k < 1) return -2;
     if (ni < 8) return -3;
+    if (nrow < 0) return -13;
     if (ldq < std::max(left ? k : nrow, 1)) return -12;
     CK(alloc_work(c, k));
     Work W = c->W;
@@ -2263,12 +2317,15 @@ static int hss_one(hsdc_ctx *c, int sq, int k, int ni, double tol, const double 
     c->evc_k = k; c->evc_ni = ni; c->evc_tol = tol; c->evc_sq = sq;
     CK(ensure(c->xb, sizeof(double) * (size_t)std::max(nrow, 1) * k));
     CK(ensure(c->yb, sizeof(double) * (size_t)std::max(nrow, 1) * k));
-    std::vector<double> qt;
-    if (left) {
-        qt.resize((size_t)std::max(nrow, 1) * k);
-        for (int cc = 0; cc < nrow; cc++)
-            for (int r = 0; r < k; r++) qt[(size_t)r * nrow + cc] = q[(size_t)cc * ldq + r];
-        HK(hipMemcpy(c->xb.p, qt.data(), sizeof(double) * (size_t)nrow * k, hipMemcpyHostToDevice));
+    if (left) {  // q is k x nrow: staged as is, transposed on the device into X^T (nrow x k)
+        CK(ensure(c->tb, sizeof(double) * (size_t)std::max(nrow, 1) * k));
+        if (nrow > 0) {
+            HK(hipMemcpy2D(c->tb.p, sizeof(double) * k, q, sizeof(double) * ldq, sizeof(double) * k, nrow,
+                           hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_transpose, dim3(cdiv(k, 64), cdiv(nrow, 64)), dim3(WGS), 0, s, k, nrow,
+                               (const double *)c->tb.p, (long)k, (double *)c->xb.p, (long)nrow);
+            HK(hipGetLastError());
+        }
     } else {
         HK(hipMemcpy2D(c->xb.p, sizeof(double) * nrow, q, sizeof(double) * ldq, sizeof(double) * nrow, k,
                        hipMemcpyHostToDevice));
@@ -2282,9 +2339,14 @@ static int hss_one(hsdc_ctx *c, int sq, int k, int ni, double tol, const double 
     std::vector<HssPlanM> keep;
     CK(hss_apply(c, v1, ni, tol, &keep, left));
     if (left) {
-        HK(hipMemcpy(qt.data(), c->yb.p, sizeof(double) * (size_t)nrow * k, hipMemcpyDeviceToHost));
-        for (int cc = 0; cc < nrow; cc++)
-            for (int r = 0; r < k; r++) q[(size_t)cc * ldq + r] = qt[(size_t)r * nrow + cc];
+        if (nrow > 0) {  // (X^T H^T)^T back to k x nrow on the device (the cb staging is free again)
+            CK(ensure(c->tb, sizeof(double) * (size_t)nrow * k));
+            hipLaunchKernelGGL(k_transpose, dim3(cdiv(nrow, 64), cdiv(k, 64)), dim3(WGS), 0, s, nrow, k,
+                               (const double *)c->yb.p, (long)nrow, (double *)c->tb.p, (long)k);
+            HK(hipGetLastError());
+            HK(hipMemcpy2D(q, sizeof(double) * ldq, c->tb.p, sizeof(double) * k, sizeof(double) * k, nrow,
+                           hipMemcpyDeviceToHost));
+        }
     } else {
         HK(hipMemcpy2D(q, sizeof(double) * ldq, c->yb.p, sizeof(double) * nrow, sizeof(double) * nrow, k,
                        hipMemcpyDeviceToHost));
@@ -2320,15 +2382,207 @@ int hsdc_hss_cmm(hsdc_ctx *c, char side, int squared, int k, int ni, double tol,
                  int ncol, int *ranks)
 {
     const bool left = side == 'L' || side == 'l';
+    if (!c) return HSDC_ERR_INTERNAL;
     if (!left && side != 'R' && side != 'r') return -2;
     if (squared != 0 && squared != 1) return -3;
+    if (k < 1) return -4;
+    if (ni < 8) return -5;
     if (ncol < 0) return -15;
-    if (ncol == 0) return 0;
+    if (ldx < std::max(left ? k : ncol, 1)) return -14;
+    if (ncol == 0) {  // nothing to multiply: no HSS form is built, and none is left to expand
+        c->evc_k = 0;
+        return 0;
+    }
     int rc = hss_one(c, squared, k, ni, tol, f, d, u, v, difl, difr, x, ldx, ncol, ranks, left);
     // hss_one numbers k, ni, ldx from 2 on; here two arguments come first
     return rc < 0 && rc > HSDC_ERR_HIP ? rc - 2 : rc;
 }
 
+
+// CauchyHssMM (cauchyhssmm.f90:1-123): B (n x n, ldb) := A * B for the general
+// Cauchy-like A(i,j) = u(i) v(j) / (d(i) - f(j)).  The HSS form is built by
+// cauchy2hss's traversal (k_cauchy2hss, one workgroup), then H * B runs as the
+// transposed product (B^T H^T)^T through the batched multiply, ncol-wide column
+// blocks at a time (at least 1024).  A tree of one leaf, more than 512 leaves or a
+// full generator arena take the dense product instead.
+int hsdc_cauchyhssmm(hsdc_ctx *c, const double *d, const double *f, const double *u, const double *v, int n, int ni,
+                     double tol, int ncol, double *b, int ldb, double *time, int *ranks)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (!d) return -1;
+    if (!f) return -2;
+    if (!u) return -3;
+    if (!v) return -4;
+    if (n < 0) return -5;
+    if (ni < 8) return -6;
+    if (!(tol >= 0.0)) return -7;
+    if (ncol < 1) return -8;
+    if (ldb < std::max(n, 1)) return -10;
+    if (time) { time[0] = 0.0; time[1] = 0.0; }
+    c->evc_k = 0;  // hsdc_hss_dense expands only the eigenvector / SVD forms
+    if (n == 0) return 0;
+    // A must be finite and defined: no d(i) may equal an f(j)
+    {
+        std::vector<double> ds(d, d + n), fs(f, f + n);
+        for (int i = 0; i < n; i++)
+            if (!std::isfinite(d[i]) || !std::isfinite(f[i]) || !std::isfinite(u[i]) || !std::isfinite(v[i]))
+                return -1;
+        std::sort(ds.begin(), ds.end());
+        std::sort(fs.begin(), fs.end());
+        for (size_t i = 0, j = 0; i < ds.size() && j < fs.size();) {
+            if (ds[i] == fs[j]) return -1;
+            if (ds[i] < fs[j]) i++; else j++;
+        }
+    }
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    hipStream_t s = c->stream;
+    if (!c->ev0) {
+        HK(hipEventCreate(&c->ev0));
+        HK(hipEventCreate(&c->ev1));
+    }
+    HssTree T;
+    const bool tree = hss_make_tree(T, n, ni);
+    const size_t nn = (size_t)n;
+    // device storage: inputs, working generators and scratch (16 n), the matrix B
+    CK(ensure(c->xb, sizeof(double) * 16 * nn));
+    double *gd = (double *)c->xb.p;
+    const double *hsrc[4] = {d, f, u, v};
+    for (int i = 0; i < 4; i++) HK(hipMemcpyAsync(gd + i * nn, hsrc[i], sizeof(double) * nn, hipMemcpyHostToDevice, s));
+    CK(ensure(c->q, sizeof(double) * nn * nn));
+    double *dB = (double *)c->q.p;
+    HK(hipMemcpy2DAsync(dB, sizeof(double) * nn, b, sizeof(double) * ldb, sizeof(double) * nn, nn,
+                        hipMemcpyHostToDevice, s));
+    std::vector<HssPlanM> P(1);
+    HssPlanM &Q = P[0];
+    Q.ok = false;
+    float ms_build = 0.0f;
+    if (tree) {
+        const int ltr = T.ltr;
+        std::vector<int> ch(2 * ltr), rng(2 * ltr);
+        std::vector<long> pd(ltr, 0);
+        long pdt = 0;
+        int it = 0;
+        for (int i = 1; i <= ltr; i++) {
+            ch[2 * (i - 1)] = T.c1[i];
+            ch[2 * (i - 1) + 1] = T.c2[i];
+            if (T.c1[i] == 0) {
+                rng[2 * (i - 1)] = T.leaf_r0[i] + 1;
+                rng[2 * (i - 1) + 1] = T.leaf_r0[i] + T.m[it];
+                pd[i - 1] = pdt;
+                pdt += (long)T.m[it] * T.m[it];
+                it++;
+            } else {
+                rng[2 * (i - 1)] = rng[2 * (T.c1[i] - 1)];
+                rng[2 * (i - 1) + 1] = rng[2 * (T.c2[i] - 1) + 1];
+            }
+        }
+        // tables: ch | rng | pd | usz vsz bsz (2 each) dsz | pu pv pb | lsr lsc | err | nswap
+        const long acap = (long)12 * std::max(ni, 128) * n;
+        size_t o0 = 0, o1 = al64(o0 + sizeof(int) * 2 * ltr), o2 = al64(o1 + sizeof(int) * 2 * ltr),
+               o3 = al64(o2 + sizeof(long) * ltr), o4 = al64(o3 + sizeof(int) * 7 * ltr),
+               o5 = al64(o4 + sizeof(long) * 3 * ltr), o6 = al64(o5 + sizeof(int) * 2 * (T.lvl + 8)),
+               o7 = al64(o6 + 64), o8 = al64(o7 + sizeof(double) * std::max(pdt, 1L)),
+               o9 = o8 + sizeof(double) * acap;
+        CK(ensure(c->hss, o9));
+        char *hb = (char *)c->hss.p;
+        std::vector<char> tab(o3, 0);
+        memcpy(tab.data() + o0, ch.data(), sizeof(int) * 2 * ltr);
+        memcpy(tab.data() + o1, rng.data(), sizeof(int) * 2 * ltr);
+        memcpy(tab.data() + o2, pd.data(), sizeof(long) * ltr);
+        HK(hipMemcpyAsync(hb, tab.data(), o3, hipMemcpyHostToDevice, s));
+        HK(hipMemsetAsync(hb + o3, 0, o7 - o3, s));
+        C2H C;
+        C.n = n; C.ltr = ltr; C.lvl = T.lvl; C.tol = tol;
+        C.ch = (const int *)(hb + o0); C.rng = (const int *)(hb + o1); C.pd = (const long *)(hb + o2);
+        int *sz = (int *)(hb + o3);
+        C.usz = sz; C.vsz = sz + 2 * ltr; C.bsz = sz + 4 * ltr; C.dsz = sz + 6 * ltr;
+        long *po = (long *)(hb + o4);
+        C.pu = po; C.pv = po + ltr; C.pb = po + 2 * ltr;
+        C.lsr = (int *)(hb + o5); C.lsc = C.lsr + T.lvl + 8;
+        C.err = (int *)(hb + o6); C.nswap = (unsigned long long *)(hb + o6 + 8);
+        C.dd = (double *)(hb + o7); C.arena = (double *)(hb + o8); C.arena_cap = acap;
+        C.D = gd + 4 * nn; C.F = gd + 5 * nn; C.U = gd + 6 * nn; C.V = gd + 7 * nn;
+        for (int i = 0; i < 4; i++)
+            HK(hipMemcpyAsync(gd + (4 + i) * nn, gd + i * nn, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+        C.Di = gd + 8 * nn; C.Fi = gd + 9 * nn; C.Ui = gd + 10 * nn; C.Vi = gd + 11 * nn; C.Z = gd + 12 * nn;
+        C.W = gd + 13 * nn;
+        C.PL = (int *)(gd + 14 * nn); C.PU = (int *)(gd + 15 * nn);
+        HK(hipEventRecord(c->ev0, s));
+        hipLaunchKernelGGL(k_cauchy2hss, dim3(1), dim3(1024), 0, s, C);
+        HK(hipGetLastError());
+        HK(hipEventRecord(c->ev1, s));
+        // generator directory back to the host
+        std::vector<int> szh(7 * ltr);
+        std::vector<long> poh(3 * ltr);
+        int err = 0;
+        HK(hipMemcpyAsync(szh.data(), sz, sizeof(int) * 7 * ltr, hipMemcpyDeviceToHost, s));
+        HK(hipMemcpyAsync(poh.data(), po, sizeof(long) * 3 * ltr, hipMemcpyDeviceToHost, s));
+        HK(hipMemcpyAsync(&err, C.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        HK(hipStreamSynchronize(s));
+        (void)hipEventElapsedTime(&ms_build, c->ev0, c->ev1);
+        if (!err) {
+            Q.T = T;
+            Q.ok = true;
+            Q.usz.assign(szh.begin(), szh.begin() + 2 * ltr);
+            Q.vsz.assign(szh.begin() + 2 * ltr, szh.begin() + 4 * ltr);
+            Q.bsz.assign(szh.begin() + 4 * ltr, szh.begin() + 6 * ltr);
+            Q.dsz.assign(szh.begin() + 6 * ltr, szh.begin() + 7 * ltr);
+            Q.pu.assign(poh.begin(), poh.begin() + ltr);
+            Q.pv.assign(poh.begin() + ltr, poh.begin() + 2 * ltr);
+            Q.pb.assign(poh.begin() + 2 * ltr, poh.begin() + 3 * ltr);
+            Q.pd = pd;
+            memset(&Q.H, 0, sizeof(Q.H));
+            Q.H.arena = C.arena;
+            Q.H.dd = C.dd;
+            Q.H.K = n;
+            if (ranks)
+                for (int i = 0; i < ltr; i++) {
+                    ranks[2 * i] = Q.usz[2 * i + 1];
+                    ranks[2 * i + 1] = Q.vsz[2 * i];
+                }
+        }
+    }
+    if (!Q.ok && ranks) ranks[0] = -1;
+    // B := A B, in column blocks of nb: B_blk^T (nb x n) on the device, times H^T
+    const int nb = std::min(n, std::max(ncol, 1024));
+    CK(ensure(c->yb, sizeof(double) * nn * nb));
+    CK(ensure(c->tb, sizeof(double) * nn * nb));
+    double *Xt = (double *)c->tb.p, *Yt = (double *)c->yb.p;
+    if (!Q.ok) {
+        CK(ensure(c->cb, sizeof(double) * nn * nn));
+        hipLaunchKernelGGL(k_cfill_general, dim3(std::min(cdiv((long)n * n, WGS), 8192)), dim3(WGS), 0, s, n,
+                           (const double *)gd, (const double *)(gd + nn), (const double *)(gd + 2 * nn),
+                           (const double *)(gd + 3 * nn), (double *)c->cb.p);
+    }
+    HK(hipEventRecord(c->ev0, s));
+    for (int c0 = 0; c0 < n; c0 += nb) {
+        const int w = std::min(nb, n - c0);
+        double *Bb = dB + (size_t)c0 * nn;
+        if (Q.ok) {
+            hipLaunchKernelGGL(k_transpose, dim3(cdiv(n, 64), cdiv(w, 64)), dim3(WGS), 0, s, n, w, (const double *)Bb,
+                               (long)n, Xt, (long)w);
+            HssSpec sp;
+            sp.K = n; sp.X = Xt; sp.ldx = w; sp.Y = Yt; sp.ldy = w; sp.nrow = w;
+            std::vector<HssSpec> v1(1, sp);
+            CK(hss_mult(c, v1, P, true));
+            hipLaunchKernelGGL(k_transpose, dim3(cdiv(w, 64), cdiv(n, 64)), dim3(WGS), 0, s, w, n, (const double *)Yt,
+                               (long)w, Bb, (long)n);
+        } else {
+            hipLaunchKernelGGL(k_dgemm_nn, dim3(cdiv(n, 64), cdiv(w, 64)), dim3(WGS), 0, s, n, w, n,
+                               (const double *)c->cb.p, (long)n, (const double *)Bb, (long)n, Yt, (long)n);
+            HK(hipMemcpyAsync(Bb, Yt, sizeof(double) * nn * w, hipMemcpyDeviceToDevice, s));
+        }
+        HK(hipGetLastError());
+    }
+    HK(hipEventRecord(c->ev1, s));
+    HK(hipMemcpy2DAsync(b, sizeof(double) * ldb, dB, sizeof(double) * nn, sizeof(double) * nn, nn,
+                        hipMemcpyDeviceToHost, s));
+    HK(hipStreamSynchronize(s));
+    float ms_mm = 0.0f;
+    (void)hipEventElapsedTime(&ms_mm, c->ev0, c->ev1);
+    if (time) { time[0] = ms_build * 1e-3; time[1] = ms_mm * 1e-3; }
+    return 0;
+}
 
 int hsdc_hss_dense(hsdc_ctx *c, double *a, int lda)
 {

@@ -30,6 +30,31 @@ struct HssMerge {
     int sq;                 // 1: u_i v_j / (f_i^2 - d_j^2), the SVD's vector matrices
 };
 
+// The general Cauchy-like matrix A(i,j) = u(i) v(j) / (d(i) - f(j)) of order n (direct
+// differences) and its HSS construction by cauchy2hss's post-order traversal with a
+// stack of compressed skeletons (CauchyHss_VP2.f90:387-613), one workgroup.  Arrays
+// of n: the working generators (copies of the inputs), the node scratch; per node
+// (1-based ids, 2 entries each where paired): children, original index range, and
+// the generator directory in the layout of HssMerge.
+struct C2H {
+    int n, ltr, lvl;
+    double tol;
+    const int *ch;          // 2 per node: children (0, 0 for a leaf)
+    const int *rng;         // 2 per node: first and last original index (1-based)
+    double *D, *F, *U, *V;  // working generators (n)
+    double *Di, *Fi, *Ui, *Vi, *Z, *W;  // node scratch (n)
+    int *PL, *PU;           // permutations (n)
+    int *lsr, *lsc;         // skeleton stacks (lvl + 3)
+    double *arena;          // U, V, B generators
+    long arena_cap;
+    double *dd;             // leaf D blocks, at pd
+    const long *pd;
+    int *usz, *vsz, *bsz, *dsz;
+    long *pu, *pv, *pb;
+    int *err;                    // bit1: arena full
+    unsigned long long *nswap;   // swaps of the RRLU
+};
+
 // One node of one merge's post-ordered HSS tree (aux_hss.f90:737-800).
 struct HssNode {
     int mrg;        // index into the HssMerge table

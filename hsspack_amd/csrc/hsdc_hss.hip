@@ -1058,3 +1058,339 @@ template __global__ void k_hss_rowcomp<256, true>(const HssMerge *, const HssNod
 template __global__ void k_hss_rowcomp<512, true>(const HssMerge *, const HssNode *, int, double, int);
 template __global__ void k_hss_colcomp<256, true>(const HssMerge *, const HssNode *, int, double, int);
 template __global__ void k_hss_colcomp<512, true>(const HssMerge *, const HssNode *, int, double, int);
+
+// ===========================================================================
+// General Cauchy-like matrices A(i,j) = u(i) v(j) / (d(i) - f(j)), direct
+// differences: cauchy2hss (CauchyHss_VP2.f90:387-613).  The reference walks the
+// tree in post order with a stack of compressed skeleton rows and columns, so a
+// node's compression sees the skeletons of the nodes before it; that order fixes
+// the ranks, and it is kept: one workgroup walks the tree, and every RRLU step
+// (rrluCauchy, :7-134, rho = 1.02, a swap check every 10 steps) runs across the
+// workgroup's threads, its scalar decisions taken identically by every thread from
+// the same global values.  Statement order is the Fortran's.
+// ===========================================================================
+#define C2H_WG 1024
+
+// CauchyMax (:166-188): max |u(i) / (D(i) - F)|, i = 1..n, first index
+__device__ VI g_cmax(const double *D, double F, const double *u, int n, VI *red)
+{
+    VI x = {0.0, 0};
+    for (int i = 1 + threadIdx.x; i <= n; i += C2H_WG) x = vi_better(x, VI{fabs(A1(u, i) / (A1(D, i) - F)), i});
+    return block_argmax(x, red);
+}
+
+__device__ __forceinline__ void g_dswp(double *a, int i, int j) { double t = A1(a, i); A1(a, i) = A1(a, j); A1(a, j) = t; }
+__device__ __forceinline__ void g_iswp(int *a, int i, int j) { int t = A1(a, i); A1(a, i) = A1(a, j); A1(a, j) = t; }
+
+// CauchyPivtG (:191-243): rook pivoting on the k-th Schur complement
+__device__ void g_pivt(double *D, double *F, double *u, double *v, int *PL, int *PU, int k, double *A, int M, int N,
+                       VI *red)
+{
+    const VI L = g_cmax(D + k - 1, A1(F, k), u + k - 1, M - k + 1, red);
+    const VI R = g_cmax(F + k - 1, A1(D, k), v + k - 1, N - k + 1, red);
+    const double junkL = L.v * fabs(A1(v, k)), junkU = R.v * fabs(A1(u, k));
+    const double piv = fabs(A1(u, k) * A1(v, k) / (A1(D, k) - A1(F, k)));
+    if (junkL <= piv && junkU <= piv) return;
+    int flg = (junkL > junkU) ? 1 : 0, jjL = L.i, jjU = R.i;
+    // every round moves a strictly larger entry into the pivot, so finite data ends
+    // within M + N rounds; the bound only guarantees an exit for overflowed generators
+    for (int round = 0; round < M + N + 2; round++) {
+        if (flg == 1) {
+            const int jj = jjL + k - 1;
+            __syncthreads();  // every wave has read the pivot entries it decided on
+            if (threadIdx.x == 0) { g_dswp(D, k, jj); g_dswp(u, k, jj); g_dswp(A, k, jj); g_iswp(PL, k, jj); }
+            __syncthreads();
+            jjU = g_cmax(F + k - 1, A1(D, k), v + k - 1, N - k + 1, red).i;
+            if (jjU == 1) return;
+            flg = 0;
+        }
+        const int jj = jjU + k - 1;
+        __syncthreads();
+        if (threadIdx.x == 0) { g_dswp(F, k, jj); g_dswp(v, k, jj); g_iswp(PU, k, jj); }
+        __syncthreads();
+        jjL = g_cmax(D + k - 1, A1(F, k), u + k - 1, M - k + 1, red).i;
+        if (jjL == 1) return;
+        flg = 1;
+    }
+}
+
+// CauchyPivtG_CP (:246-296): complete pivoting, one thread per column
+__device__ void g_pivt_cp(double *D, double *F, double *u, double *v, int *PL, int *PU, int k, double *A, int M,
+                          int N, VI *red, int *s_ii)
+{
+    VI best = {0.0, 0};
+    int bestrow = 1;
+    for (int j = 1 + threadIdx.x; j <= N - k + 1; j += C2H_WG) {
+        const double fj = A1(F, k + j - 1);
+        double mv = 0.0;
+        int mi = 1;
+        for (int i = 1; i <= M - k + 1; i++) {
+            const double x = fabs(A1(u, k + i - 1) / (A1(D, k + i - 1) - fj));
+            if (i == 1 || x > mv) { mv = x; mi = i; }
+        }
+        const VI c = {fabs(mv * A1(v, k + j - 1)), j};
+        const VI nb = vi_better(best, c);
+        if (nb.i == c.i) bestrow = mi;
+        best = nb;
+    }
+    const VI win = block_argmax(best, red);
+    if (best.i == win.i && win.i > 0) *s_ii = bestrow;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int ii = *s_ii + k - 1, jj = win.i + k - 1;
+        g_dswp(D, k, ii); g_dswp(u, k, ii); g_dswp(A, k, ii); g_iswp(PL, k, ii);
+        g_dswp(F, k, jj); g_dswp(v, k, jj); g_iswp(PU, k, jj);
+    }
+    __syncthreads();
+}
+
+// rrluCauchy (:7-134) of the M x N matrix A(i) V(j) / (D(i) - F(j)); returns Rk
+__device__ int g_rrlu(double *D, double *F, double *A, double *V, double tol, double *U, double *W, int *PL, int *PU,
+                      int M, int N, VI *red, int *s_ii, unsigned long long &nsw)
+{
+    const int tid = threadIdx.x, mn = M < N ? M : N;
+    const double rho = 1.02;
+    for (int j = 1 + tid; j <= M; j += C2H_WG) { A1(U, j) = A1(A, j); A1(PL, j) = j; }
+    for (int j = 1 + tid; j <= N; j += C2H_WG) A1(PU, j) = j;
+    __syncthreads();
+    int Rk = mn;
+    double amax = 0.0;
+    for (int k = 1; k <= mn; k++) {
+        g_pivt(D, F, U, V, PL, PU, k, A, M, N, red);
+        __syncthreads();
+        double ukk = A1(U, k) * A1(V, k) / (A1(D, k) - A1(F, k));
+        amax = fmax(amax, fabs(ukk));
+        if (fabs(ukk) < tol * amax) {
+            g_pivt_cp(D, F, U, V, PL, PU, k, A, M, N, red, s_ii);
+            ukk = A1(U, k) * A1(V, k) / (A1(D, k) - A1(F, k));
+            amax = fmax(amax, fabs(ukk));
+            if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
+        }
+        const double Dk = A1(D, k), Fk = A1(F, k);
+        for (int j = k + 1 + tid; j <= M; j += C2H_WG) A1(U, j) = A1(U, j) * (A1(D, j) - Dk) / (A1(D, j) - Fk);
+        for (int j = k + 1 + tid; j <= N; j += C2H_WG) A1(V, j) = A1(V, j) * (A1(F, j) - Fk) / (A1(F, j) - Dk);
+        for (int j = 1 + tid; j <= k - 1; j += C2H_WG) A1(W, j) = A1(W, j) * ((Fk - A1(D, j)) / (Dk - A1(D, j)));
+        __syncthreads();
+        if (tid == 0) {
+            double wk = (Dk - Fk) / A1(A, k);
+            for (int j = 1; j <= k - 1; j++) wk = wk * (A1(F, j) - Dk) / (A1(D, j) - Dk);
+            A1(W, k) = wk;
+        }
+        __syncthreads();
+        if (k % 10 != 0) continue;
+        // each swap raises |det| of the pivot block by more than rho; bounded all the same
+        for (int sw = 0; sw < 2 * M + 8; sw++) {
+            // searchMax2 (:137-163): the largest entry of Mat_N(i, j) = U(k+i) W(j) / (D(k+i) - D(j))
+            VI x = {0.0, 0};
+            int xrow = 1;
+            for (int jc = 1 + tid; jc <= k; jc += C2H_WG) {
+                const double dj = A1(D, jc);
+                double mv = 0.0;
+                int mi = 1;
+                for (int i = 1; i <= M - k; i++) {
+                    const double t = fabs(A1(U, k + i) / (A1(D, k + i) - dj));
+                    if (i == 1 || t > mv) { mv = t; mi = i; }
+                }
+                const VI c = {mv * fabs(A1(W, jc)), jc};
+                const VI nb = vi_better(x, c);
+                if (nb.i == c.i) xrow = mi;
+                x = nb;
+            }
+            const VI win = block_argmax(x, red);
+            if (x.i == win.i && win.i > 0) *s_ii = xrow;
+            __syncthreads();
+            if (!(win.v > rho)) break;
+            nsw++;
+            const int ii = win.i, jj = *s_ii + k;
+            const double Dii = A1(D, ii), Djj = A1(D, jj);
+            for (int j = k + 1 + tid; j <= N; j += C2H_WG) A1(V, j) = A1(V, j) * ((A1(F, j) - Dii) / (A1(F, j) - Djj));
+            for (int j = k + 1 + tid; j <= M; j += C2H_WG)
+                if (j != jj) A1(U, j) = A1(U, j) * ((A1(D, j) - Djj) / (A1(D, j) - Dii));
+            for (int j = 1 + tid; j <= k; j += C2H_WG)
+                if (j != ii) A1(W, j) = A1(W, j) * ((A1(D, j) - Dii) / (A1(D, j) - Djj));
+            __syncthreads();
+            if (tid == 0) {
+                double uj = A1(A, ii) * ((Dii - Djj) / (Dii - A1(F, ii)));
+                double wi = (Djj - A1(F, ii)) / A1(A, jj);
+                for (int j = 1; j <= k; j++) {
+                    if (j == ii) continue;
+                    uj = uj * ((Dii - A1(D, j)) / (Dii - A1(F, j)));
+                    wi = wi * ((A1(F, j) - Djj) / (A1(D, j) - Djj));
+                }
+                A1(U, jj) = uj;
+                A1(W, ii) = wi;
+                g_iswp(PL, ii, jj);
+                g_dswp(D, ii, jj);
+                g_dswp(A, ii, jj);
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    return Rk;
+}
+
+// comprcauchy (:616-725): rows ('r': M node rows against N outside columns) or
+// columns ('c': M node columns against N outside rows) of node `nodi`; the
+// interpolative generator (Cauchylike2, :743-778, rows or columns put back in the
+// node's order) goes to the arena
+__device__ int g_compr(const C2H &C, bool row, double *D, double *F, double *U, double *V, int M, int N, int nodi,
+                       VI *red, int *s_ii, long &top, unsigned long long &nsw)
+{
+    const int tid = threadIdx.x, mn = M < N ? M : N;
+    for (int j = 1 + tid; j <= M; j += C2H_WG) A1(C.Z, j) = 0.0;
+    for (int j = 1 + tid; j <= mn; j += C2H_WG) A1(C.W, j) = 0.0;
+    __syncthreads();
+    const int Rk = row ? g_rrlu(D, F, U, V, C.tol, C.Z, C.W, C.PL, C.PU, M, N, red, s_ii, nsw)
+                       : g_rrlu(F, D, V, U, C.tol, C.Z, C.W, C.PL, C.PU, M, N, red, s_ii, nsw);
+    const double *X = row ? D : F;
+    // invp: PU (free after the RRLU) takes the position of every original index
+    for (int i = 1 + tid; i <= M; i += C2H_WG) A1(C.PU, A1(C.PL, i)) = i;
+    const int r = Rk < M ? Rk : M;
+    const long off = top;
+    __syncthreads();
+    if (off + (long)M * r > C.arena_cap) {
+        if (tid == 0) atomicOr(C.err, 2);
+        return -1;
+    }
+    double *Q = C.arena + off;
+    if (row) {
+        for (long e = tid; e < (long)M * r; e += C2H_WG) {
+            const int i = (int)(e % M) + 1, j = (int)(e / M) + 1, s = A1(C.PU, i);
+            double q;
+            if (Rk >= M) q = (s == j) ? 1.0 : 0.0;
+            else if (s <= r) q = (s == j) ? 1.0 : 0.0;
+            else q = A1(C.Z, s) * A1(C.W, j) / (A1(X, s) - A1(X, j));
+            Q[e] = q;
+        }
+        if (tid == 0) { C.usz[2 * (nodi - 1)] = M; C.usz[2 * (nodi - 1) + 1] = r; C.pu[nodi - 1] = off; }
+    } else {
+        for (long e = tid; e < (long)M * r; e += C2H_WG) {
+            const int i = (int)(e % r) + 1, j = (int)(e / r) + 1, s = A1(C.PU, j);
+            double q;
+            if (Rk >= M) q = (s == i) ? 1.0 : 0.0;
+            else if (s <= r) q = (s == i) ? 1.0 : 0.0;
+            else q = A1(C.W, i) * A1(C.Z, s) / (A1(X, s) - A1(X, i));
+            Q[e] = q;
+        }
+        if (tid == 0) { C.vsz[2 * (nodi - 1)] = r; C.vsz[2 * (nodi - 1) + 1] = M; C.pv[nodi - 1] = off; }
+    }
+    top = off + (long)M * r;
+    __syncthreads();
+    return Rk;
+}
+
+// Cauchylike (Cauchylike.f90:1-13) into a (lda), M x N
+__device__ void g_cfill(double *a, int lda, const double *D, const double *F, const double *u, const double *v, int M,
+                        int N)
+{
+    for (long e = threadIdx.x; e < (long)M * N; e += C2H_WG) {
+        const int i = (int)(e % M) + 1, j = (int)(e / M) + 1;
+        a[(i - 1) + (long)(j - 1) * lda] = A1(u, i) * A1(v, j) / (A1(D, i) - A1(F, j));
+    }
+}
+
+__global__ __launch_bounds__(C2H_WG) void k_cauchy2hss(C2H C)
+{
+    __shared__ VI red[C2H_WG / 64];
+    __shared__ int s_ii;
+    const int tid = threadIdx.x, lda = C.n;
+    long top = 0;               // arena bump, the same in every thread
+    unsigned long long nsw = 0;
+    int ns = 1;
+    int *lsr = C.lsr, *lsc = C.lsc;
+    if (tid == 0) { A1(lsr, 1) = 1; A1(lsc, 1) = 1; }
+    __syncthreads();
+    // gather the node's row side (D, U at idx(j)) and the outside column side
+    // (the stacked column skeletons 1..cs-1, then the original columns after `last`)
+    auto gather = [&](bool row, int m0, int mi, int cs, int last, int nt) {
+        double *A = row ? C.D : C.F, *Au = row ? C.U : C.V, *B = row ? C.F : C.D, *Bv = row ? C.V : C.U;
+        double *Ai = row ? C.Di : C.Fi, *Aui = row ? C.Ui : C.Vi, *Bi = row ? C.Fi : C.Di, *Bvi = row ? C.Vi : C.Ui;
+        for (int j = tid; j < mi; j += C2H_WG) { Ai[j] = A1(A, m0 + j); Aui[j] = A1(Au, m0 + j); }
+        for (int j = tid; j < nt; j += C2H_WG) {
+            const int src = (j < cs - 1) ? j + 1 : last + 1 + (j - (cs - 1));
+            Bi[j] = A1(B, src);
+            Bvi[j] = A1(Bv, src);
+        }
+        __syncthreads();
+    };
+    for (int i = 1; i <= C.ltr; i++) {
+        const int c1 = C.ch[2 * (i - 1)], c2 = C.ch[2 * (i - 1) + 1];
+        const int l1 = C.rng[2 * (i - 1)], l2 = C.rng[2 * (i - 1) + 1];
+        if (c1 == 0) {  // leaf
+            const int mi = l2 - l1 + 1, nii = lda - l2;
+            g_cfill(C.dd + C.pd[i - 1], mi, C.D + l1 - 1, C.F + l1 - 1, C.U + l1 - 1, C.V + l1 - 1, mi, mi);
+            if (tid == 0) C.dsz[i - 1] = mi;
+            int cs = (ns == 1) ? 1 : A1(lsc, ns);
+            gather(true, l1, mi, cs, l2, nii + cs - 1);
+            int Rk = g_compr(C, true, C.Di, C.Fi, C.Ui, C.Vi, mi, nii + cs - 1, i, red, &s_ii, top, nsw);
+            if (Rk < 0) return;
+            const int r0 = A1(lsr, ns);
+            for (int j = tid; j < Rk; j += C2H_WG) { A1(C.D, r0 + j) = C.Di[j]; A1(C.U, r0 + j) = C.Ui[j]; }
+            __syncthreads();
+            if (tid == 0) A1(lsr, ns + 1) = r0 + Rk;
+            __syncthreads();
+            const int rs = (ns == 1) ? 1 : A1(lsr, ns);
+            gather(false, l1, mi, rs, l2, nii + rs - 1);
+            Rk = g_compr(C, false, C.Di, C.Fi, C.Ui, C.Vi, mi, nii + rs - 1, i, red, &s_ii, top, nsw);
+            if (Rk < 0) return;
+            const int q0 = A1(lsc, ns);
+            for (int j = tid; j < Rk; j += C2H_WG) { A1(C.F, q0 + j) = C.Fi[j]; A1(C.V, q0 + j) = C.Vi[j]; }
+            __syncthreads();
+            if (tid == 0) A1(lsc, ns + 1) = q0 + Rk;
+            __syncthreads();
+            ns++;
+        } else {  // parent: B of both children, then its own compressions
+            int mi = A1(lsr, ns - 1) - A1(lsr, ns - 2), nii = A1(lsc, ns) - A1(lsc, ns - 1);
+            if (top + (long)mi * nii > C.arena_cap) { if (tid == 0) atomicOr(C.err, 2); return; }
+            g_cfill(C.arena + top, mi, C.D + A1(lsr, ns - 2) - 1, C.F + A1(lsc, ns - 1) - 1, C.U + A1(lsr, ns - 2) - 1,
+                    C.V + A1(lsc, ns - 1) - 1, mi, nii);
+            if (tid == 0) { C.bsz[2 * (c1 - 1)] = mi; C.bsz[2 * (c1 - 1) + 1] = nii; C.pb[c1 - 1] = top; }
+            top += (long)mi * nii;
+            mi = A1(lsr, ns) - A1(lsr, ns - 1);
+            nii = A1(lsc, ns - 1) - A1(lsc, ns - 2);
+            if (top + (long)mi * nii > C.arena_cap) { if (tid == 0) atomicOr(C.err, 2); return; }
+            g_cfill(C.arena + top, mi, C.D + A1(lsr, ns - 1) - 1, C.F + A1(lsc, ns - 2) - 1, C.U + A1(lsr, ns - 1) - 1,
+                    C.V + A1(lsc, ns - 2) - 1, mi, nii);
+            if (tid == 0) { C.bsz[2 * (c2 - 1)] = mi; C.bsz[2 * (c2 - 1) + 1] = nii; C.pb[c2 - 1] = top; }
+            top += (long)mi * nii;
+            __syncthreads();
+            if (i == C.ltr) break;
+            // rows of the merged skeleton against the stacked columns before and the columns after
+            mi = A1(lsr, ns) - A1(lsr, ns - 2);
+            nii = lda - l2;
+            int cs = (ns == 3) ? 1 : A1(lsc, ns - 2);
+            gather(true, A1(lsr, ns - 2), mi, cs, l2, nii + cs - 1);
+            int Rk = g_compr(C, true, C.Di, C.Fi, C.Ui, C.Vi, mi, nii + cs - 1, i, red, &s_ii, top, nsw);
+            if (Rk < 0) return;
+            const int r0 = A1(lsr, ns - 2);
+            for (int j = tid; j < Rk; j += C2H_WG) { A1(C.D, r0 + j) = C.Di[j]; A1(C.U, r0 + j) = C.Ui[j]; }
+            __syncthreads();
+            if (tid == 0) A1(lsr, ns - 1) = r0 + Rk;
+            __syncthreads();
+            // columns of the merged skeleton
+            mi = A1(lsc, ns) - A1(lsc, ns - 2);
+            const int rs = (ns == 3) ? 1 : A1(lsr, ns - 2);
+            gather(false, A1(lsc, ns - 2), mi, rs, l2, nii + rs - 1);
+            Rk = g_compr(C, false, C.Di, C.Fi, C.Ui, C.Vi, mi, nii + rs - 1, i, red, &s_ii, top, nsw);
+            if (Rk < 0) return;
+            const int q0 = A1(lsc, ns - 2);
+            for (int j = tid; j < Rk; j += C2H_WG) { A1(C.F, q0 + j) = C.Fi[j]; A1(C.V, q0 + j) = C.Vi[j]; }
+            __syncthreads();
+            if (tid == 0) A1(lsc, ns - 1) = q0 + Rk;
+            __syncthreads();
+            ns--;
+        }
+    }
+    if (tid == 0) *C.nswap = nsw;
+}
+
+// the dense general Cauchy-like matrix (fallback for a tree of one leaf)
+__global__ __launch_bounds__(WG) void k_cfill_general(int n, const double *d, const double *f, const double *u,
+                                                      const double *v, double *a)
+{
+    for (long e = blockIdx.x * (long)WG + threadIdx.x; e < (long)n * n; e += (long)gridDim.x * WG) {
+        const int i = (int)(e % n), j = (int)(e / n);
+        a[e] = u[i] * v[j] / (d[i] - f[j]);
+    }
+}

@@ -2015,3 +2015,21 @@ __global__ __launch_bounds__(WG) void k_fill_u64(unsigned long long *p, long n, 
     long i = (long)blockIdx.x * WG + threadIdx.x;
     if (i < n) p[i] = v;
 }
+
+// B (cols x rows, ldb) := A^T for A (rows x cols, lda), column-major: 64 x 64 tiles
+// through LDS (coalesced reads and writes), one tile per block.
+__global__ __launch_bounds__(WG) void k_transpose(int rows, int cols, const double *A, long lda, double *B, long ldb)
+{
+    __shared__ double t[64][65];
+    const long r0 = (long)blockIdx.x * 64, c0 = (long)blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int j = ty; j < 64; j += WG / 64) {
+        const long r = r0 + tx, cc = c0 + j;
+        if (r < rows && cc < cols) t[j][tx] = A[r + cc * lda];
+    }
+    __syncthreads();
+    for (int j = ty; j < 64; j += WG / 64) {
+        const long cc = c0 + tx, r = r0 + j;
+        if (r < rows && cc < cols) B[cc + r * ldb] = t[tx][j];
+    }
+}

@@ -4,10 +4,12 @@
 ! names, argument lists, INFO codes and XERBLA calls, with the work done by the
 ! MI355X library libhsdc.so through its C interface (include/hsdc.h).
 !
-! Each call creates a library context (device buffers), runs, and destroys it, so
-! the routines keep no state between calls.  WORK/IWORK are accepted for
-! interface compatibility (MDSTEDC checks LWORK/LIWORK and answers queries as the
-! original); the device needs no host workspace.  XERBLA is the caller's (LAPACK's).
+! Each call runs on the calling thread's own library context (hsdc_thread_ctx, reset
+! to the reference's constants at every call), whose device buffers stay allocated
+! between calls, so a repeated call of the same order allocates no device memory;
+! hsdc_thread_ctx_release frees them.  WORK/IWORK are accepted for interface
+! compatibility (MDSTEDC checks LWORK/LIWORK and answers queries as the original);
+! the device needs no host workspace.  XERBLA is the caller's (LAPACK's).
 
 module hsdc_c
    use iso_c_binding
@@ -80,13 +82,24 @@ module hsdc_c
          real(c_double) :: f(*), d(*), u(*), v(*), difl(*), difr(*), q(*)
          type(c_ptr), value :: ranks
       end function
-      integer(c_int) function hsdc_bd_ctx_create(ctx) bind(C, name='hsdc_bd_ctx_create')
-         import :: c_int, c_ptr
-         type(c_ptr) :: ctx
+      type(c_ptr) function hsdc_thread_ctx() bind(C, name='hsdc_thread_ctx')
+         import :: c_ptr
       end function
-      integer(c_int) function hsdc_bd_ctx_destroy(ctx) bind(C, name='hsdc_bd_ctx_destroy')
+      type(c_ptr) function hsdc_bd_thread_ctx() bind(C, name='hsdc_bd_thread_ctx')
+         import :: c_ptr
+      end function
+      integer(c_int) function hsdc_set_params(ctx, p) bind(C, name='hsdc_set_params')
          import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, p
+      end function
+      integer(c_int) function hsdc_cauchyhssmm(ctx, d, f, u, v, n, ni, tol, ncol, b, ldb, time, ranks) &
+            bind(C, name='hsdc_cauchyhssmm')
+         import :: c_int, c_ptr, c_double
          type(c_ptr), value :: ctx
+         real(c_double) :: d(*), f(*), u(*), v(*), b(*), time(*)
+         integer(c_int), value :: n, ni, ncol, ldb
+         real(c_double), value :: tol
+         type(c_ptr), value :: ranks
       end function
       integer(c_int) function hsdc_bdsdc(ctx, uplo, compq, n, d, e, u, ldu, vt, ldvt, q, iq, work, iwork) &
             bind(C, name='hsdc_bdsdc')
@@ -175,15 +188,13 @@ subroutine mdstedc(compz, n, d, e, z, ldz, work, lwork, iwork, liwork, info)
    type(c_ptr) :: ctx
    integer(c_int) :: rc
    info = 0
-   rc = hsdc_ctx_create(ctx)
-   if (rc /= 0) then
+   ctx = hsdc_thread_ctx()
+   if (.not. c_associated(ctx)) then
       info = 1003
       return
    end if
    rc = hsdc_stedc(ctx, compz, n, d, e, z, ldz, work, lwork, iwork, liwork)
-   rc = int(hsdc_info(rc), c_int)
-   if (hsdc_ctx_destroy(ctx) /= 0) rc = 1003
-   info = rc
+   info = hsdc_info(rc)
    if (info < 0) call xerbla('DSTEDC', -info)
 end subroutine
 
@@ -215,10 +226,13 @@ subroutine mdlaed0(icompq, qsiz, n, d, e, q, ldq, qstore, ldqs, work, iwork, inf
    end if
    if (n == 0) return
    wflag = 1
-   rc = hsdc_ctx_create(ctx)
+   ctx = hsdc_thread_ctx()
+   if (.not. c_associated(ctx)) then
+      info = 1003
+      return
+   end if
    rc = hsdc_laed0(ctx, icompq, qsiz, n, d, e, q, ldq)
    info = hsdc_info(rc)
-   rc = hsdc_ctx_destroy(ctx)
 end subroutine
 
 ! MDLAED2 (mdlaed2.f90:149): deflation on the device (hsdc_deflate), the Givens
@@ -254,10 +268,11 @@ subroutine mdlaed2(k, n, n1, d, q, ldq, indxq, rho, z, dlamda, w, q2, indx, indx
    n2 = n - n1
    allocate(rpj(max(n, 1)), rnj(max(n, 1)), rc_(max(n, 1)), rs_(max(n, 1)), drot(max(n, 1)), dfl(max(n, 1)))
    dfl(1:n) = d(1:n)
-   rc = hsdc_ctx_create(ctx)
-   rc = hsdc_deflate(ctx, n, n1, dfl, z, indxq, rho, hsdc_bsmlz, kk, rho_o, dlamda, w, indxp, indx, indxc, &
-                     coltyp, nrot, rpj, rnj, rc_, rs_)
-   if (hsdc_ctx_destroy(ctx) /= 0 .and. rc == 0) rc = -1003
+   ctx = hsdc_thread_ctx()
+   rc = -1003
+   if (c_associated(ctx)) &
+      rc = hsdc_deflate(ctx, n, n1, dfl, z, indxq, rho, hsdc_bsmlz, kk, rho_o, dlamda, w, indxp, indx, indxc, &
+                        coltyp, nrot, rpj, rnj, rc_, rs_)
    if (rc /= 0) then
       info = hsdc_info(rc)
       return
@@ -366,7 +381,11 @@ subroutine mdlaed3(k, n, d, q, ldq, rho, dlamda, q2, z, difl, difr, alpha, work,
       return
    end if
    if (k == 0) return
-   rc = hsdc_ctx_create(ctx)
+   ctx = hsdc_thread_ctx()
+   if (.not. c_associated(ctx)) then
+      info = 1003
+      return
+   end if
    if (k >= 3) then
       rc = hsdc_loewner(ctx, k, rho, dlamda, z, d, difl, difr, alpha)
       if (rc == 0) rc = hsdc_hss_evc(ctx, k, hsdc_ni, hsdc_tol, dlamda, d, z, alpha, difl, difr, q2, n, n, &
@@ -397,7 +416,6 @@ subroutine mdlaed3(k, n, d, q, ldq, rho, dlamda, q2, z, difl, difr, alpha, work,
          end do
       end if
    end if
-   if (hsdc_ctx_destroy(ctx) /= 0 .and. rc == 0) rc = -1003
    info = hsdc_info(rc)
 end subroutine
 
@@ -448,10 +466,11 @@ subroutine mdlaed1(n, d, q, ldq, indxq, rho, cutpnt, work, iwork, info, wflag)
    if (info /= 0) return
    if (k /= 0) then
       if (hflag) then
-         rc = hsdc_ctx_create(ctx)
-         rc = hsdc_laed3(ctx, k, n, cutpnt, d, q, ldq, rho, work(idlmda), work(iq2), iwork(indxc), &
-                         iwork(coltyp), work(iw))
-         if (hsdc_ctx_destroy(ctx) /= 0 .and. rc == 0) rc = -1003
+         ctx = hsdc_thread_ctx()
+         rc = -1003
+         if (c_associated(ctx)) &
+            rc = hsdc_laed3(ctx, k, n, cutpnt, d, q, ldq, rho, work(idlmda), work(iq2), iwork(indxc), &
+                            iwork(coltyp), work(iw))
          info = hsdc_info(rc)
       else
          wflag = 1
@@ -484,14 +503,40 @@ subroutine mdbdsdc(uplo, compq, n, d, e, u, ldu, vt, ldvt, q, iq, work, iwork, i
    type(c_ptr) :: ctx
    integer(c_int) :: rc
    info = 0
-   rc = hsdc_bd_ctx_create(ctx)
-   if (rc /= 0) then
+   ctx = hsdc_bd_thread_ctx()
+   if (.not. c_associated(ctx)) then
       info = 1003
       return
    end if
    rc = hsdc_bdsdc(ctx, uplo, compq, n, d, e, u, ldu, vt, ldvt, q, iq, work, iwork)
-   rc = int(hsdc_info(rc), c_int)
-   if (hsdc_bd_ctx_destroy(ctx) /= 0) rc = 1003
-   info = rc
+   info = hsdc_info(rc)
    if (info < 0) call xerbla('DBDSDC', -info)
+end subroutine
+
+
+! CauchyHssMM (cauchyhssmm.f90:1-123): B(1:N, 1:N) := A * B for the Cauchy-like
+! A(i,j) = U(i) V(j) / (D(i) - F(j)), A put in HSS form by cauchy2hss with leaf size
+! NI and tolerance TOL, NCOL columns of B multiplied at a time (or more); TIME(1) and
+! TIME(2) receive the construction and multiplication times in seconds.  D, F, U, V
+! are left unchanged (the reference destroys them).  The reference has no INFO: an
+! illegal argument, or D and F sharing a value, stops with a message, as its failed
+! allocations do.
+subroutine cauchyhssmm(d, f, u, v, n, ni, tol, ncol, b, ldb, time)
+   use hsdc_c
+   implicit none
+   integer :: n, ni, ncol, ldb
+   double precision :: tol
+   double precision :: d(*), f(*), u(*), v(*), b(ldb, *), time(2)
+   type(c_ptr) :: ctx
+   integer(c_int) :: rc
+   ctx = hsdc_thread_ctx()
+   if (.not. c_associated(ctx)) then
+      write (*, *) 'cauchyhssmm: no device context'
+      stop
+   end if
+   rc = hsdc_cauchyhssmm(ctx, d, f, u, v, n, ni, tol, ncol, b, ldb, time, c_null_ptr)
+   if (rc /= 0) then
+      write (*, *) 'cauchyhssmm failed: ', rc
+      stop
+   end if
 end subroutine

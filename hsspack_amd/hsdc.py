@@ -32,7 +32,8 @@ EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_s
            "hsdc_mlp_load", "hsdc_mlp_harvest", "hsdc_mlp_samples", "hsdc_mlp_grad", "hsdc_mlp_train",
            "hsdc_bd_default_params", "hsdc_bd_ctx_create", "hsdc_bd_ctx_destroy", "hsdc_bd_set_params",
            "hsdc_bd_get_stats", "hsdc_bd_set_stream", "hsdc_bdsdc", "hsdc_bdsdc_dev", "hsdc_bd_trace",
-           "hsdc_trace", "hsdc_trace_indxq", "hsdc_loewner_noalloc"]
+           "hsdc_trace", "hsdc_trace_indxq", "hsdc_loewner_noalloc", "hsdc_cauchyhssmm", "hsdc_thread_ctx",
+           "hsdc_bd_thread_ctx", "hsdc_thread_ctx_release"]
 
 # learned secular start: 34 inputs -> 64 -> 64 -> 1
 MLP_IN, MLP_H = 34, 64
@@ -134,6 +135,10 @@ def lib():
         L.hsdc_trace.argtypes = [_vp, _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _i]
         L.hsdc_trace_indxq.argtypes = [_vp, _P(_i), _i]
         L.hsdc_loewner_noalloc.argtypes = [_i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.hsdc_cauchyhssmm.argtypes = [_vp, _P(_d), _P(_d), _P(_d), _P(_d), _i, _i, _d, _i, _P(_d), _i, _P(_d),
+                                       _P(_i)]
+        L.hsdc_thread_ctx.restype = _vp
+        L.hsdc_bd_thread_ctx.restype = _vp
         _LIB = L
     return _LIB
 
@@ -410,6 +415,24 @@ class Context:
         if info != 0:
             raise RuntimeError("%s: %d" % (name, info))
         return q, ranks
+
+    def cauchyhssmm(self, d, f, u, v, b, ni, tol, ncol=128):
+        """CauchyHssMM: b (n x n) := A b for A(i,j) = u_i v_j / (d_i - f_j); returns (b,
+        [construction s, multiply s], ranks per node (U, V) or None when applied densely)."""
+        n = len(d)
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (d, f, u, v)]
+        if any(a.shape != (n,) for a in arrs):
+            raise ValueError("d, f, u, v must have the same length")
+        b = np.array(b, np.float64, order="F")
+        if b.shape != (n, n):
+            raise ValueError("b must be %d x %d" % (n, n))
+        t = np.zeros(2)
+        ranks = np.zeros(2 * max(2 * (n // max(ni, 1)) + 4, 1), np.int32)
+        info = lib().hsdc_cauchyhssmm(self._h, *[_dp(a) for a in arrs], n, ni, tol, ncol, _dp(b), max(n, 1),
+                                      _dp(t), _ip(ranks))
+        if info != 0:
+            raise RuntimeError("hsdc_cauchyhssmm: %d" % info)
+        return b, t, (None if ranks[0] < 0 else ranks)
 
     def hss_dense(self, k):
         a = np.zeros((k, k), order="F")

@@ -193,14 +193,37 @@ int hsdc_hss_svc(hsdc_ctx *ctx, int k, int ni, double tol, const double *f, cons
                  const double *u, const double *v, const double *difl, const double *difr,
                  double *q, int ldq, int nrow, int *ranks);
 
-/* The standalone construct-and-multiply of the Cauchy-like matrices (CauchyHssMM
- * cauchyhssmm.f90:1-123, DHSSMM / DHSSMML with fastHssmm ConstructHssd.f90:926-1155 and
- * fastHssmmL :1394-1630): builds the HSS form of C (squared = 0: as hsdc_hss_evc;
- * 1: as hsdc_hss_svc), then side 'R': x (ncol x k, ldx) := x * C, or side 'L':
- * x (k x ncol, ldx) := C * x.  ranks as for hsdc_hss_evc. */
+/* Construct-and-multiply of the merge's vector matrices (the eigenvector form of
+ * DHSSEVC, or with squared = 1 the SVD form of DHSSVCS), on either side: the HSS form
+ * of C (squared = 0: as hsdc_hss_evc; 1: as hsdc_hss_svc), then side 'R': x (ncol x k,
+ * ldx) := x * C (fastHssmmL, ConstructHssd.f90:1394-1630), or side 'L': x (k x ncol,
+ * ldx) := C * x (fastHssmm, :926-1155).  These forms take their differences from
+ * difl/difr and need interlaced f and d; a general Cauchy-like matrix goes through
+ * hsdc_cauchyhssmm.  ranks as for hsdc_hss_evc. */
 int hsdc_hss_cmm(hsdc_ctx *ctx, char side, int squared, int k, int ni, double tol, const double *f,
                  const double *d, const double *u, const double *v, const double *difl,
                  const double *difr, double *x, int ldx, int ncol, int *ranks);
+
+/* CauchyHssMM (cauchyhssmm.f90:1-123): b (n x n, ldb) := A * b for the general
+ * Cauchy-like A(i,j) = u(i) v(j) / (d(i) - f(j)) with direct differences.  A is put in
+ * HSS form by cauchy2hss (CauchyHss_VP2.f90:387-613: the structured RRLU with rho =
+ * 1.02, the post-ordered tree of leaf size ni walked with a stack of compressed
+ * skeletons, compression tolerance tol), then multiplied, ncol columns of b at a time
+ * or more.  d, f, u, v (n) are not modified (the reference destroys them).  time[0]
+ * and time[1] (may be NULL) receive the construction and multiplication times in
+ * seconds, ranks (may be NULL; 2 per tree node) the rank of each node's U and V, or
+ * ranks[0] = -1 when A was applied densely (a tree of one leaf).  INFO: -i for an
+ * illegal argument i; -1 also when some d(i) equals some f(j) or an input is not finite. */
+int hsdc_cauchyhssmm(hsdc_ctx *ctx, const double *d, const double *f, const double *u, const double *v,
+                     int n, int ni, double tol, int ncol, double *b, int ldb, double *time, int *ranks);
+
+/* Contexts owned by the calling host thread, for callers that keep no handle (the
+ * Fortran drop-ins): created on first use with the default parameters (reset to them
+ * at every hsdc_thread_ctx / hsdc_bd_thread_ctx call) and kept, with their device
+ * buffers, until hsdc_thread_ctx_release or the end of the process.  NULL when no
+ * context could be created. */
+hsdc_ctx *hsdc_thread_ctx(void);
+void hsdc_thread_ctx_release(void);  /* also releases the thread's hsdc_bd_thread_ctx */
 
 /* Dense expansion of the HSS form built by the last hsdc_hss_evc / hsdc_hss_svc call
  * (k x k). */
@@ -248,6 +271,8 @@ int hsdc_mlp_train(hsdc_ctx *ctx, const float *x, const float *t, long long ns, 
  * the left and right vector matrices in HSS form, U2 * H_U and V2 * H_V.
  * ------------------------------------------------------------------------- */
 typedef struct hsdc_bd_ctx hsdc_bd_ctx;
+/* The calling thread's own SVD context (see hsdc_thread_ctx). */
+hsdc_bd_ctx *hsdc_bd_thread_ctx(void);
 
 typedef struct {
     int smlsiz;     /* leaf bound, ILAENV(9,'DBDSDC') = 25 (mdbdsdc.f90:193); 3..63 */

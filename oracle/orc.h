@@ -167,6 +167,14 @@ int orc_hss_build_sq(orc_hss *H, int k, int ni, const double *f, const double *d
                      double tol);
 void orc_cauchy_dense_sq(int k, const double *f, const double *d0, const double *u, const double *v,
                          const double *difl, const double *difr, double *a, int lda);
+/* The general Cauchy-like matrix A(i,j) = u(i) v(j) / (d(i) - f(j)) of order n, direct
+ * differences: its HSS form by cauchy2hss (CauchyHss_VP2.f90:387-613: structured RRLU
+ * with rho = 1.02, post-order traversal with the stack of compressed skeletons), and
+ * its dense fill (Cauchylike.f90:1-13).  Returns -1 for a tree of one leaf. */
+int orc_cauchy2hss(orc_hss *H, int n, int ni, const double *d, const double *f, const double *u,
+                   const double *v, double tol);
+void orc_cauchy_dense_general(int n, const double *d, const double *f, const double *u, const double *v,
+                              double *a, int lda);
 
 /* ---- bidiagonal SVD (mdbdsdc.f90 / mdlasd0-2.f90 + LAPACK DLASD3/DLASD4/DLASDQ) ---- */
 /* LAPACK DLASCL 'G' on a vector, DLARTG, DLASDT (node arrays 0-based, values 1-based). */

@@ -598,13 +598,10 @@ void orc_hss_free(orc_hss *H)
 
 #define TTR(H, LL, i) (H)->ttr[((LL) - 1) * (H)->ltr + (i) - 1]
 
-/* DHSSEVC construction + Cauchy2hssEig_omp, sequential.  S == NULL: the
- * eigenvector matrix u_i v_j / (f_i - d_j); S == d0: the SVD's vector matrices
- * u_i v_j / (f_i^2 - d_j^2), every displacement formed as the accurate difference
- * times the sum (the same RRLU in the variables f^2 and d^2). */
-static int hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, const double *u0,
-                     const double *v0, const double *DIFL, const double *DIFR, const double *S,
-                     double tol)
+/* The post-ordered HSS tree of K indices in leaves of about ni (npart, child,
+ * hsslevel, GetTreeInfo: aux_hss.f90:238-827) and the generator directory and
+ * storage of H.  Returns the number of levels (< 1: a single leaf). */
+static int hss_tree_setup(orc_hss *H, int K, int ni)
 {
     memset(H, 0, sizeof(*H));
     H->k = K;
@@ -646,7 +643,20 @@ static int hss_build(orc_hss *H, int K, int ni, const double *f, const double *d
     long cap = 10L * ni * K;
     H->h = (double *)malloc(sizeof(double) * cap);
     H->nh = 0;
+    return lvl;
+}
 
+/* DHSSEVC construction + Cauchy2hssEig_omp, sequential.  S == NULL: the
+ * eigenvector matrix u_i v_j / (f_i - d_j); S == d0: the SVD's vector matrices
+ * u_i v_j / (f_i^2 - d_j^2), every displacement formed as the accurate difference
+ * times the sum (the same RRLU in the variables f^2 and d^2). */
+static int hss_build(orc_hss *H, int K, int ni, const double *f, const double *d0, const double *u0,
+                     const double *v0, const double *DIFL, const double *DIFR, const double *S,
+                     double tol)
+{
+    if (hss_tree_setup(H, K, ni) < 1) return -1;
+    const int ltr = H->ltr, lvl = H->lvl;
+    long cap = 10L * ni * K;
     int *ch = H->ch;
     double *D = (double *)malloc(sizeof(double) * K), *U = (double *)malloc(sizeof(double) * K),
            *V = (double *)malloc(sizeof(double) * K);
@@ -905,4 +915,344 @@ void orc_cauchy_dense_sq(int k, const double *f, const double *d0, const double 
     for (int i = 0; i < k; i++) p[i] = i + 1;
     cauchylikeEig(a, lda, f, u, v, difl, difr, d0, p, p, k, k);
     free(p);
+}
+
+/* =====================================================================
+ * General Cauchy-like matrices A(i,j) = u(i) v(j) / (d(i) - f(j)) with direct
+ * differences (CauchyHss_VP2.f90): the structured RRLU rrluCauchy (:7-134, rho =
+ * 1.02, a swap check every 10 steps) with its pivot searches CauchyPivtG (:191-243),
+ * CauchyPivtG_CP (:246-296), searchMax2 (:137-163), CauchyMax (:166-188); the
+ * interpolative generators of comprcauchy (:616-725) through Cauchylike2
+ * (:743-778); and the stack-ordered post-order construction cauchy2hss
+ * (:387-613).  Statement order is the Fortran's; the result goes into the same
+ * orc_hss layout as the eigenvector construction (U_i: M x r, V_i: r x M, B_i of a
+ * node: its row skeleton against its sibling's column skeleton, leaf D_i dense).
+ * ===================================================================== */
+
+/* CauchyMax: the largest |u(i) / (D(i) - F)|, i = 1..n, and its first index */
+static void cmaxG(const double *D, double F, const double *u, int n, double *junk, int *jj)
+{
+    double best = 0.0;
+    int bi = 1;
+    for (int i = 1; i <= n; i++) {
+        double x = fabs(A1(u, i) / (A1(D, i) - F));
+        if (i == 1 || x > best) { best = x; bi = i; }
+    }
+    *junk = best;
+    *jj = bi;
+}
+
+/* CauchyPivtG: rook pivoting on the k-th Schur complement */
+static void pivtG(double *D, double *F, double *u, double *v, int *PL, int *PU, int k, double *A, int M, int N)
+{
+    double junkL, junkU;
+    int jjL, jjU;
+    cmaxG(D + k - 1, A1(F, k), u + k - 1, M - k + 1, &junkL, &jjL);
+    cmaxG(F + k - 1, A1(D, k), v + k - 1, N - k + 1, &junkU, &jjU);
+    junkL = junkL * fabs(A1(v, k));
+    junkU = junkU * fabs(A1(u, k));
+    double piv = fabs(A1(u, k) * A1(v, k) / (A1(D, k) - A1(F, k)));
+    if (junkL <= piv && junkU <= piv) return;
+    int flg = (junkL > junkU) ? 1 : 0;
+    for (;;) {
+        if (flg == 1) {
+            int jj = jjL;
+            dswp(D, k, jj + k - 1); dswp(u, k, jj + k - 1); dswp(A, k, jj + k - 1); iswp(PL, k, jj + k - 1);
+            cmaxG(F + k - 1, A1(D, k), v + k - 1, N - k + 1, &junkU, &jjU);
+            if (jjU == 1) return;
+            flg = 0;
+        }
+        int jj = jjU;
+        dswp(F, k, jj + k - 1); dswp(v, k, jj + k - 1); iswp(PU, k, jj + k - 1);
+        cmaxG(D + k - 1, A1(F, k), u + k - 1, M - k + 1, &junkL, &jjL);
+        if (jjL == 1) return;
+        flg = 1;
+    }
+}
+
+/* CauchyPivtG_CP: complete pivoting over the k-th Schur complement */
+static void pivtGCP(double *D, double *F, double *u, double *v, int *PL, int *PU, int k, double *A, int M, int N)
+{
+    double best = 0.0;
+    int ii = 1, jj = 1;
+    for (int j = 1; j <= N - k + 1; j++) {
+        double junk;
+        int iiu;
+        cmaxG(D + k - 1, A1(F, k + j - 1), u + k - 1, M - k + 1, &junk, &iiu);
+        junk = fabs(junk * A1(v, k + j - 1));
+        if (j == 1 || junk > best) { best = junk; ii = iiu; jj = j; }
+    }
+    dswp(D, k, ii + k - 1); dswp(u, k, ii + k - 1); dswp(A, k, ii + k - 1); iswp(PL, k, ii + k - 1);
+    dswp(F, k, jj + k - 1); dswp(v, k, jj + k - 1); iswp(PU, k, jj + k - 1);
+}
+
+/* searchMax2: the largest entry of Mat_N(i, j) = U(i) W(j) / (D2(i) - D1(j)), column by column */
+static void searchMax2G(const double *U, const double *W, const double *D2, const double *D1, int ldu, int ldw,
+                        int *ii, int *jj, double *nmax)
+{
+    double junk;
+    int jjl;
+    cmaxG(D2, A1(D1, 1), U, ldu, &junk, &jjl);
+    junk = junk * fabs(A1(W, 1));
+    *nmax = junk; *ii = 1; *jj = jjl;
+    for (int j = 2; j <= ldw; j++) {
+        cmaxG(D2, A1(D1, j), U, ldu, &junk, &jjl);
+        junk = junk * fabs(A1(W, j));
+        if (junk > *nmax) { *nmax = junk; *ii = j; *jj = jjl; }
+    }
+}
+
+/* rrluCauchy: structured RRLU of the M x N Cauchy-like matrix A(i)V(j)/(D(i)-F(j)) */
+static void rrluG(double *D, double *F, double *A, double *V, double tol, double *U, double *W, int *PL, int *PU,
+                  int M, int N, int *Rk, long long *nswap)
+{
+    int mn = M < N ? M : N, j, k, ii, jj;
+    double amax = 0.0, ukk, nmax;
+    const double rho = 1.02;
+    const int prd = 10;
+    for (j = 1; j <= M; j++) { A1(U, j) = A1(A, j); A1(PL, j) = j; }
+    for (j = 1; j <= N; j++) A1(PU, j) = j;
+    *Rk = mn;
+    for (k = 1; k <= mn; k++) {
+        pivtG(D, F, U, V, PL, PU, k, A, M, N);
+        ukk = A1(U, k) * A1(V, k) / (A1(D, k) - A1(F, k));
+        amax = fmax(amax, fabs(ukk));
+        if (fabs(ukk) < tol * amax) {
+            pivtGCP(D, F, U, V, PL, PU, k, A, M, N);
+            ukk = A1(U, k) * A1(V, k) / (A1(D, k) - A1(F, k));
+            amax = fmax(amax, fabs(ukk));
+            if (fabs(ukk) < tol * amax) { *Rk = k - 1; break; }
+        }
+        for (j = k + 1; j <= M; j++) A1(U, j) = A1(U, j) * (A1(D, j) - A1(D, k)) / (A1(D, j) - A1(F, k));
+        for (j = k + 1; j <= N; j++) A1(V, j) = A1(V, j) * (A1(F, j) - A1(F, k)) / (A1(F, j) - A1(D, k));
+        for (j = 1; j <= k - 1; j++) A1(W, j) = A1(W, j) * ((A1(F, k) - A1(D, j)) / (A1(D, k) - A1(D, j)));
+        A1(W, k) = (A1(D, k) - A1(F, k)) / A1(A, k);
+        for (j = 1; j <= k - 1; j++) A1(W, k) = A1(W, k) * (A1(F, j) - A1(D, k)) / (A1(D, j) - A1(D, k));
+        int flgl = k % prd;
+        while (flgl < 1) {
+            flgl = 1;
+            searchMax2G(U + k, W, D + k, D, M - k, k, &ii, &jj, &nmax);
+            if (nmax > rho) {
+                if (nswap) (*nswap)++;
+                flgl = 0;
+                jj = jj + k;
+                for (j = k + 1; j <= N; j++) A1(V, j) = A1(V, j) * ((A1(F, j) - A1(D, ii)) / (A1(F, j) - A1(D, jj)));
+                for (j = k + 1; j <= jj - 1; j++)
+                    A1(U, j) = A1(U, j) * ((A1(D, j) - A1(D, jj)) / (A1(D, j) - A1(D, ii)));
+                for (j = jj + 1; j <= M; j++) A1(U, j) = A1(U, j) * ((A1(D, j) - A1(D, jj)) / (A1(D, j) - A1(D, ii)));
+                for (j = 1; j <= ii - 1; j++) A1(W, j) = A1(W, j) * ((A1(D, j) - A1(D, ii)) / (A1(D, j) - A1(D, jj)));
+                for (j = ii + 1; j <= k; j++) A1(W, j) = A1(W, j) * ((A1(D, j) - A1(D, ii)) / (A1(D, j) - A1(D, jj)));
+                A1(U, jj) = A1(A, ii) * ((A1(D, ii) - A1(D, jj)) / (A1(D, ii) - A1(F, ii)));
+                A1(W, ii) = (A1(D, jj) - A1(F, ii)) / A1(A, jj);
+                for (j = 1; j <= ii - 1; j++) {
+                    A1(U, jj) = A1(U, jj) * ((A1(D, ii) - A1(D, j)) / (A1(D, ii) - A1(F, j)));
+                    A1(W, ii) = A1(W, ii) * ((A1(F, j) - A1(D, jj)) / (A1(D, j) - A1(D, jj)));
+                }
+                for (j = ii + 1; j <= k; j++) {
+                    A1(U, jj) = A1(U, jj) * ((A1(D, ii) - A1(D, j)) / (A1(D, ii) - A1(F, j)));
+                    A1(W, ii) = A1(W, ii) * ((A1(F, j) - A1(D, jj)) / (A1(D, j) - A1(D, jj)));
+                }
+                iswp(PL, ii, jj);
+                dswp(D, ii, jj);
+                dswp(A, ii, jj);
+            }
+        }
+    }
+}
+
+/* comprcauchy: rows ('r', M node rows against N outside columns) or columns ('c', M
+ * node columns against N outside rows) of one node; generators into H */
+static void comprG(orc_hss *H, long *cap, char rc, double *D, double *F, double *U, double *V, double tol, int M,
+                   int N, int nodi, int *Rk)
+{
+    int mn = M < N ? M : N;
+    double *Z = (double *)calloc(M + 1, sizeof(double)), *W = (double *)calloc(mn + 1, sizeof(double));
+    int *PL = (int *)malloc(sizeof(int) * (M + 1)), *PU = (int *)malloc(sizeof(int) * (N + 1));
+    if (rc == 'r') rrluG(D, F, U, V, tol, Z, W, PL, PU, M, N, Rk, &H->nswap);
+    else rrluG(F, D, V, U, tol, Z, W, PL, PU, M, N, Rk, &H->nswap);
+    invp(PL, M);
+    int r = *Rk;
+    const double *X = (rc == 'r') ? D : F;  /* the compressed side's points */
+    if (rc == 'r') {
+        long off = push(H, cap, (long)M * (r < M ? r : M));
+        if (r < M) {
+            /* Cauchylike2(Q, D(r+1), D, Z(r+1), W, M, r), then rows Q(PL, :) */
+            for (int j = 1; j <= r; j++)
+                for (int i = 1; i <= M; i++) {
+                    int s = A1(PL, i);
+                    double q;
+                    if (s <= r) q = (s == j) ? 1.0 : 0.0;
+                    else q = A1(Z, s) * A1(W, j) / (A1(X, s) - A1(X, j));
+                    H->h[off + (i - 1) + (long)(j - 1) * M] = q;
+                }
+            H->usz[2 * (nodi - 1)] = M; H->usz[2 * (nodi - 1) + 1] = r;
+        } else {
+            for (int j = 1; j <= M; j++)
+                for (int i = 1; i <= M; i++) H->h[off + (i - 1) + (long)(j - 1) * M] = (A1(PL, i) == j) ? 1.0 : 0.0;
+            H->usz[2 * (nodi - 1)] = M; H->usz[2 * (nodi - 1) + 1] = M;
+        }
+        H->pu[nodi - 1] = off;
+    } else {
+        long off = push(H, cap, (long)(r < M ? r : M) * M);
+        if (r < M) {
+            /* Cauchylike2(Q, F(r+1), F, Z(r+1), W, r, M), then columns Q(:, PL) */
+            for (int j = 1; j <= M; j++) {
+                int s = A1(PL, j);
+                for (int i = 1; i <= r; i++) {
+                    double q;
+                    if (s <= r) q = (s == i) ? 1.0 : 0.0;
+                    else q = A1(W, i) * A1(Z, s) / (A1(X, s) - A1(X, i));
+                    H->h[off + (i - 1) + (long)(j - 1) * r] = q;
+                }
+            }
+            H->vsz[2 * (nodi - 1)] = r; H->vsz[2 * (nodi - 1) + 1] = M;
+        } else {
+            for (int j = 1; j <= M; j++)
+                for (int i = 1; i <= M; i++) H->h[off + (i - 1) + (long)(j - 1) * M] = (A1(PL, j) == i) ? 1.0 : 0.0;
+            H->vsz[2 * (nodi - 1)] = M; H->vsz[2 * (nodi - 1) + 1] = M;
+        }
+        H->pv[nodi - 1] = off;
+    }
+    free(Z); free(W); free(PL); free(PU);
+}
+
+/* Cauchylike (Cauchylike.f90:1-13): A(i,j) = u(i) v(j) / (D(i) - F(j)) */
+static void cauchyG(double *a, int lda, const double *D, const double *F, const double *u, const double *v, int M,
+                    int N)
+{
+    for (int j = 1; j <= N; j++)
+        for (int i = 1; i <= M; i++) a[(i - 1) + (long)(j - 1) * lda] = A1(u, i) * A1(v, j) / (A1(D, i) - A1(F, j));
+}
+
+int orc_cauchy2hss(orc_hss *H, int n, int ni, const double *d0, const double *f0, const double *u0, const double *v0,
+                   double tol)
+{
+    if (hss_tree_setup(H, n, ni) < 1) return -1;
+    const int ltr = H->ltr, lvl = H->lvl, lda = n;
+    long cap = 10L * ni * n;
+    int *ch = H->ch;
+    double *D = (double *)malloc(sizeof(double) * n), *F = (double *)malloc(sizeof(double) * n),
+           *U = (double *)malloc(sizeof(double) * n), *V = (double *)malloc(sizeof(double) * n);
+    memcpy(D, d0, sizeof(double) * n); memcpy(F, f0, sizeof(double) * n);
+    memcpy(U, u0, sizeof(double) * n); memcpy(V, v0, sizeof(double) * n);
+    int *l = (int *)calloc(2 * ltr + 2, sizeof(int)), *lsr = (int *)calloc(lvl + 3, sizeof(int)),
+        *lsc = (int *)calloc(lvl + 3, sizeof(int)), *indx1 = (int *)malloc(sizeof(int) * n),
+        *indx2 = (int *)malloc(sizeof(int) * n);
+    double *Di = (double *)malloc(sizeof(double) * n), *Fi = (double *)malloc(sizeof(double) * n),
+           *Ui = (double *)malloc(sizeof(double) * n), *Vi = (double *)malloc(sizeof(double) * n);
+#define L(i, c) l[2 * ((i) - 1) + (c) - 1]
+    int lt = 1, it = 1;
+    long pnd = 0;
+    for (int i = 1; i <= ltr; i++) {
+        if (CH1(ch, i) == 0 && CH2(ch, i) == 0) {
+            L(i, 1) = lt; L(i, 2) = lt + A1(H->m, it) - 1;
+            lt = L(i, 2) + 1;
+            it++;
+        } else {
+            L(i, 1) = L(CH1(ch, i), 1); L(i, 2) = L(CH2(ch, i), 2);
+        }
+    }
+    int ns = 1, Rk, mi, nii, nt;
+    A1(lsr, ns) = 1;
+    A1(lsc, ns) = 1;
+    for (int i = 1; i <= ltr; i++) {
+        if (CH1(ch, i) == 0) {  /* leaf */
+            mi = L(i, 2) - L(i, 1) + 1;
+            nii = lda - L(i, 2);
+            for (int j = 0; j < mi; j++) indx1[j] = L(i, 1) + j;
+            cauchyG(H->dd + pnd, mi, D + L(i, 1) - 1, F + L(i, 1) - 1, U + L(i, 1) - 1, V + L(i, 1) - 1, mi, mi);
+            H->dsz[i - 1] = mi;
+            H->pd[i - 1] = pnd;
+            pnd += (long)mi * mi;
+            /* off-diagonal row compression: the stacked column skeletons, then the columns to the right */
+            if (ns == 1) {
+                for (int j = 0; j < nii; j++) indx2[j] = L(i, 2) + 1 + j;
+                nt = nii;
+            } else {
+                for (int j = 0; j < A1(lsc, ns) - 1; j++) indx2[j] = j + 1;
+                for (int j = 0; j < nii; j++) indx2[A1(lsc, ns) - 1 + j] = L(i, 2) + 1 + j;
+                nt = nii + A1(lsc, ns) - 1;
+            }
+            for (int j = 0; j < mi; j++) { Di[j] = A1(D, indx1[j]); Ui[j] = A1(U, indx1[j]); }
+            for (int j = 0; j < nt; j++) { Fi[j] = A1(F, indx2[j]); Vi[j] = A1(V, indx2[j]); }
+            comprG(H, &cap, 'r', Di, Fi, Ui, Vi, tol, mi, nt, i, &Rk);
+            A1(lsr, ns + 1) = A1(lsr, ns) + Rk;
+            for (int j = 0; j < Rk; j++) { A1(D, A1(lsr, ns) + j) = Di[j]; A1(U, A1(lsr, ns) + j) = Ui[j]; }
+            /* off-diagonal column compression */
+            if (ns == 1) {
+                for (int j = 0; j < nii; j++) indx2[j] = L(i, 2) + 1 + j;
+                nt = nii;
+            } else {
+                for (int j = 0; j < A1(lsr, ns) - 1; j++) indx2[j] = j + 1;
+                for (int j = 0; j < nii; j++) indx2[A1(lsr, ns) - 1 + j] = L(i, 2) + 1 + j;
+                nt = nii + A1(lsr, ns) - 1;
+            }
+            for (int j = 0; j < nt; j++) { Di[j] = A1(D, indx2[j]); Ui[j] = A1(U, indx2[j]); }
+            for (int j = 0; j < mi; j++) { Fi[j] = A1(F, indx1[j]); Vi[j] = A1(V, indx1[j]); }
+            comprG(H, &cap, 'c', Di, Fi, Ui, Vi, tol, mi, nt, i, &Rk);
+            A1(lsc, ns + 1) = A1(lsc, ns) + Rk;
+            for (int j = 0; j < Rk; j++) { A1(F, A1(lsc, ns) + j) = Fi[j]; A1(V, A1(lsc, ns) + j) = Vi[j]; }
+            ns++;
+        } else {  /* parent: the B generators of its children, then its own compressions */
+            int c1 = CH1(ch, i), c2 = CH2(ch, i);
+            mi = A1(lsr, ns - 1) - A1(lsr, ns - 2);
+            nii = A1(lsc, ns) - A1(lsc, ns - 1);
+            long off = push(H, &cap, (long)mi * nii);
+            cauchyG(H->h + off, mi, D + A1(lsr, ns - 2) - 1, F + A1(lsc, ns - 1) - 1, U + A1(lsr, ns - 2) - 1,
+                    V + A1(lsc, ns - 1) - 1, mi, nii);
+            H->bsz[2 * (c1 - 1)] = mi; H->bsz[2 * (c1 - 1) + 1] = nii; H->pb[c1 - 1] = off;
+            mi = A1(lsr, ns) - A1(lsr, ns - 1);
+            nii = A1(lsc, ns - 1) - A1(lsc, ns - 2);
+            off = push(H, &cap, (long)mi * nii);
+            cauchyG(H->h + off, mi, D + A1(lsr, ns - 1) - 1, F + A1(lsc, ns - 2) - 1, U + A1(lsr, ns - 1) - 1,
+                    V + A1(lsc, ns - 2) - 1, mi, nii);
+            H->bsz[2 * (c2 - 1)] = mi; H->bsz[2 * (c2 - 1) + 1] = nii; H->pb[c2 - 1] = off;
+            if (i == ltr) break;
+            /* off-diagonal row compression of the merged skeleton rows */
+            mi = A1(lsr, ns) - A1(lsr, ns - 2);
+            nii = lda - L(i, 2);
+            for (int j = 0; j < mi; j++) indx1[j] = A1(lsr, ns - 2) + j;
+            if (ns == 3) {
+                for (int j = 0; j < nii; j++) indx2[j] = L(i, 2) + 1 + j;
+                nt = nii;
+            } else {
+                for (int j = 0; j < A1(lsc, ns - 2) - 1; j++) indx2[j] = j + 1;
+                for (int j = 0; j < nii; j++) indx2[A1(lsc, ns - 2) - 1 + j] = L(i, 2) + 1 + j;
+                nt = nii + A1(lsc, ns - 2) - 1;
+            }
+            for (int j = 0; j < mi; j++) { Di[j] = A1(D, indx1[j]); Ui[j] = A1(U, indx1[j]); }
+            for (int j = 0; j < nt; j++) { Fi[j] = A1(F, indx2[j]); Vi[j] = A1(V, indx2[j]); }
+            comprG(H, &cap, 'r', Di, Fi, Ui, Vi, tol, mi, nt, i, &Rk);
+            A1(lsr, ns - 1) = A1(lsr, ns - 2) + Rk;
+            for (int j = 0; j < Rk; j++) { A1(D, A1(lsr, ns - 2) + j) = Di[j]; A1(U, A1(lsr, ns - 2) + j) = Ui[j]; }
+            /* off-diagonal column compression of the merged skeleton columns */
+            mi = A1(lsc, ns) - A1(lsc, ns - 2);
+            nii = lda - L(i, 2);
+            for (int j = 0; j < mi; j++) indx1[j] = A1(lsc, ns - 2) + j;
+            if (ns == 3) {
+                for (int j = 0; j < nii; j++) indx2[j] = L(i, 2) + 1 + j;
+                nt = nii;
+            } else {
+                for (int j = 0; j < A1(lsr, ns - 2) - 1; j++) indx2[j] = j + 1;
+                for (int j = 0; j < nii; j++) indx2[A1(lsr, ns - 2) - 1 + j] = L(i, 2) + 1 + j;
+                nt = nii + A1(lsr, ns - 2) - 1;
+            }
+            for (int j = 0; j < nt; j++) { Di[j] = A1(D, indx2[j]); Ui[j] = A1(U, indx2[j]); }
+            for (int j = 0; j < mi; j++) { Fi[j] = A1(F, indx1[j]); Vi[j] = A1(V, indx1[j]); }
+            comprG(H, &cap, 'c', Di, Fi, Ui, Vi, tol, mi, nt, i, &Rk);
+            A1(lsc, ns - 1) = A1(lsc, ns - 2) + Rk;
+            for (int j = 0; j < Rk; j++) { A1(F, A1(lsc, ns - 2) + j) = Fi[j]; A1(V, A1(lsc, ns - 2) + j) = Vi[j]; }
+            ns--;
+        }
+    }
+#undef L
+    free(D); free(F); free(U); free(V); free(l); free(lsr); free(lsc); free(indx1); free(indx2);
+    free(Di); free(Fi); free(Ui); free(Vi);
+    return 0;
+}
+
+void orc_cauchy_dense_general(int n, const double *d, const double *f, const double *u, const double *v, double *a,
+                              int lda)
+{
+    cauchyG(a, lda, d, f, u, v, n, n);
 }

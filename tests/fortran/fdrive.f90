@@ -6,6 +6,8 @@
 !   laed1  IN: int32 n, cutpnt, float64 rho, d(n), q(n,n), int32 indxq(n)
 !          OUT: info, wflag, d(n), q(n,n), indxq(n)
 !   bdsdc  IN: int32 n, uplo (int32), float64 d(n), e(n-1)  OUT: int32 info, d(n), u(n,n), vt(n,n)
+!   cauchy IN: int32 n, ni, ncol, float64 tol, d(n), f(n), u(n), v(n), b(n,n)
+!          OUT: b(n,n) after CAUCHYHSSMM, then b(n,n) after a second call on the input b
 !   errors OUT: for each illegal call, info and the XERBLA (name, argument) it made
 module xerbla_log
    implicit none
@@ -33,7 +35,9 @@ program fdrive
    character :: compz
    integer :: u, v, cu
    character :: uplo
-   double precision, allocatable :: vt(:, :)
+   double precision, allocatable :: vt(:, :), cf(:), cu_(:), cv(:), b0(:, :)
+   double precision :: ctol, ctime(2)
+   integer :: cni, cncol
    call get_command_argument(1, mode)
    call get_command_argument(2, fin)
    call get_command_argument(3, fout)
@@ -90,6 +94,18 @@ program fdrive
       close(u)
       call mdbdsdc(uplo, 'I', n, d, e, q, n, vt, n, work, indxq, work, iwork, info)
       write(v) info, d, q, vt
+   case ('cauchy')
+      open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+      read(u) n, cni, cncol, ctol
+      allocate(d(n), cf(n), cu_(n), cv(n), q(n, n), b0(n, n))
+      read(u) d, cf, cu_, cv, b0
+      close(u)
+      q = b0
+      call cauchyhssmm(d, cf, cu_, cv, n, cni, ctol, cncol, q, n, ctime)
+      write(v) q
+      q = b0
+      call cauchyhssmm(d, cf, cu_, cv, n, cni, ctol, cncol, q, n, ctime)
+      write(v) q
    case ('errors')
       allocate(d(8), e(8), q(8, 8), work(200), iwork(200), indxq(8), qs(8, 8))
       d = 1

@@ -308,6 +308,25 @@ def hss_free(H):
     lib().orc_hss_free(ctypes.byref(H))
 
 
+def cauchy2hss(d, f, u, v, ni, tol):
+    """HSS form of the general Cauchy-like u_i v_j / (d_i - f_j) (cauchy2hss, VP2)."""
+    n = len(d)
+    H = Hss()
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (d, f, u, v)]
+    info = lib().orc_cauchy2hss(ctypes.byref(H), n, ni, *[dp(a) for a in arrs], _d(tol))
+    if info != 0:
+        raise RuntimeError("orc_cauchy2hss failed %d" % info)
+    return H
+
+
+def cauchy_dense_general(d, f, u, v):
+    n = len(d)
+    a = np.zeros((n, n), order="F")
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (d, f, u, v)]
+    lib().orc_cauchy_dense_general(n, *[dp(x) for x in arrs], dp(a), n)
+    return a
+
+
 def cauchy_dense(f, u, v, difl, difr):
     k = len(f)
     a = np.zeros((k, k), order="F")
@@ -328,6 +347,7 @@ def _setup():
     L.orc_laed3.restype = _i
     L.orc_steqr.restype = _i
     L.orc_hss_build.restype = _i
+    L.orc_cauchy2hss.restype = _i
     L.orc_hss_build_sq.restype = _i
     L.orc_lasd8.restype = _i
     L.orc_hss_tree_ok.restype = _i

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     txt = open(os.path.join(ROOT, "include", "hsdc.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void)\s+(hsdc_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|hsdc_ctx\s*\*|hsdc_bd_ctx\s*\*)\s*(hsdc_\w+)\s*\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -110,4 +110,50 @@ def test_hss_cmm_argument_errors_without_device():
     assert L.hsdc_hss_cmm(*args(b"L", 2, 2)) == -3
     assert L.hsdc_hss_cmm(*args(b"L", 0, -1)) == -15
     assert L.hsdc_hss_cmm(*args(b"R", 1, 0)) == 0
+    ctx.close()
+
+
+def test_cauchyhssmm_argument_errors():
+    """CauchyHssMM's arguments are checked before any device work (LAPACK order:
+    every argument, then the quick return for n = 0)."""
+    ctx = hsdc.Context()
+    L = hsdc.lib()
+    n = 40
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    gap = 0.01
+    f = 0.1 + (2 * np.arange(1, n + 1) - 1) * gap
+    d = f + gap
+    u, v = np.ones(n), np.ones(n)
+    b = np.zeros((n, n), order="F")
+    t = np.zeros(2)
+    h = ctx._h
+    call = lambda nn, ni, tol, ncol, ldb, dd=d: L.hsdc_cauchyhssmm(h, dp(dd), dp(f), dp(u), dp(v), nn, ni, tol, ncol,
+                                                                   dp(b), ldb, dp(t), None)
+    assert call(-1, 16, 1e-12, 8, n) == -5
+    assert call(n, 4, 1e-12, 8, n) == -6
+    assert call(n, 16, -1.0, 8, n) == -7
+    assert call(n, 16, 1e-12, 0, n) == -8
+    assert call(n, 16, 1e-12, 8, n - 1) == -10
+    assert call(0, 16, 1e-12, 8, 1) == 0
+    dd = d.copy()
+    dd[7] = f[3]  # A(8, 4) would divide by zero
+    assert call(n, 16, 1e-12, 8, n, dd) == -1
+    dd[7] = np.nan
+    assert call(n, 16, 1e-12, 8, n, dd) == -1
+    ctx.close()
+
+
+def test_hss_cmm_checks_arguments_before_the_quick_return():
+    ctx = hsdc.Context()
+    L = hsdc.lib()
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    a = np.ones(8)
+    x = np.zeros((8, 8), order="F")
+    h = ctx._h
+    # ncol = 0 is a quick return only once k, ni and ldx are valid
+    assert L.hsdc_hss_cmm(h, b"R", 0, 0, 16, 1e-12, *[dp(a)] * 6, dp(x), 8, 0, None) == -4
+    assert L.hsdc_hss_cmm(h, b"R", 0, 8, 4, 1e-12, *[dp(a)] * 6, dp(x), 8, 0, None) == -5
+    assert L.hsdc_hss_cmm(h, b"L", 0, 8, 16, 1e-12, *[dp(a)] * 6, dp(x), 4, 0, None) == -14
+    assert L.hsdc_hss_cmm(h, b"R", 0, 8, 16, 1e-12, *[dp(a)] * 6, dp(x), 8, 0, None) == 0
+    assert L.hsdc_hss_cmm(h, b"R", 0, 8, 16, 1e-12, *[dp(a)] * 6, dp(x), 8, -1, None) == -15
     ctx.close()

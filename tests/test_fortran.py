@@ -34,7 +34,7 @@ def f64(a):
 
 def test_library_exports_the_fortran_names():
     out = subprocess.run(["nm", "-D", "--defined-only", FLIB], capture_output=True, text=True, check=True).stdout
-    for name in ("mdstedc_", "mdlaed0_", "mdlaed1_", "mdlaed2_", "mdlaed3_", "mdbdsdc_"):
+    for name in ("mdstedc_", "mdlaed0_", "mdlaed1_", "mdlaed2_", "mdlaed3_", "mdbdsdc_", "cauchyhssmm_"):
         assert (" T " + name) in out, name
 
 
@@ -184,3 +184,22 @@ def test_mdbdsdc_hss_branch_from_fortran(tmp_path):
     assert np.abs(u.T @ u - np.eye(n)).max() <= 1e-12
     assert np.abs(vt @ vt.T - np.eye(n)).max() <= 1e-12
     assert np.abs((u * s) @ vt - B).max() / np.abs(B).max() <= 1e-12
+
+
+@pytest.mark.gpu
+def test_cauchyhssmm_from_fortran(tmp_path):
+    """CAUCHYHSSMM(D, F, U, V, N, NI, TOL, NCOL, B, LDB, TIME) as testCauchyHss_serial.f90
+    calls it (interlaced points, NI = 120, NCOL = 128, TOL = 1e-17), twice in one program:
+    the second call runs on the thread's kept context and gives the same product."""
+    from test_oracle_cauchy import interlaced
+    n, ni, ncol, tol = 1000, 120, 128, 1e-17
+    d, f, u, v = interlaced(n, 21)
+    b = np.random.default_rng(4).uniform(0, 1, (n, n))
+    payload = i32(n, ni, ncol) + f64([tol]) + f64(d) + f64(f) + f64(u) + f64(v) + f64(b.T)
+    raw = run("cauchy", tmp_path, payload)
+    y1 = np.frombuffer(raw[:8 * n * n], "<f8").reshape(n, n).T
+    y2 = np.frombuffer(raw[8 * n * n:], "<f8").reshape(n, n).T
+    A = O.cauchy_dense_general(d, f, u, v)
+    ref = A @ b
+    assert np.abs(y1 - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(y1, y2)

@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "hsdc_internal.h"
@@ -152,6 +153,8 @@ struct hsdc_ctx {
     int evc_k = 0, evc_ni = 0, evc_sq = 0;
     double evc_tol = 0.0;
     PinBuf pin, pin2, pin3;
+    PinBuf stage;                       // staging of large device-to-host copies
+    hipEvent_t stev[3] = {nullptr, nullptr, nullptr};
     Work W;
     size_t wcap = 0;
     // learned secular start: weights (device + host copy), harvest buffer
@@ -1662,6 +1665,57 @@ static int solve(hsdc_ctx *c, Plan &pl, const double *hd, const double *he, doub
     return hinfo ? 1 : 0;
 }
 
+// Device-to-host copy of a rows x cols column-major matrix (device leading dimension
+// rows) into pageable host memory (leading dimension ldh).  A plain pageable copy runs
+// through the runtime's single staging loop; here blocks of columns go by DMA into
+// three pinned staging buffers while host threads copy the previous block out, so the
+// PCIe transfer and the host copy overlap.
+static int copy_d2h_matrix(hsdc_ctx *c, double *host, long ldh, const double *dev, long rows, long cols)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    hipStream_t s = c->stream;
+    const size_t colb = sizeof(double) * (size_t)rows;
+    const long per = std::max(1L, (long)((64L << 20) / (long)colb));  // columns per block (64 MB)
+    const int NB = 3;
+    CK(ensure_pin(c->stage, (size_t)NB * per * colb));
+    for (int i = 0; i < NB; i++)
+        if (!c->stev[i]) HK(hipEventCreateWithFlags(&c->stev[i], hipEventDisableTiming));
+    double *st = (double *)c->stage.p;
+    const long nblk = (cols + per - 1) / per;
+    auto issue = [&](long b) -> int {
+        const long w = std::min(per, cols - b * per);
+        HK(hipMemcpyAsync(st + (size_t)(b % NB) * per * rows, dev + (size_t)b * per * rows, colb * w,
+                          hipMemcpyDeviceToHost, s));
+        HK(hipEventRecord(c->stev[b % NB], s));
+        return 0;
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int T = (int)std::min(8u, std::max(1u, hw / 2));
+    for (long b = 0; b < std::min<long>(NB, nblk); b++) CK(issue(b));
+    for (long b = 0; b < nblk; b++) {
+        HK(hipEventSynchronize(c->stev[b % NB]));
+        const long w = std::min(per, cols - b * per);
+        const double *src = st + (size_t)(b % NB) * per * rows;
+        double *dst = host + (size_t)b * per * ldh;
+        auto part = [&](int t) {
+            const long c0 = w * t / T, c1 = w * (t + 1) / T;
+            if (ldh == rows) memcpy(dst + (size_t)c0 * ldh, src + (size_t)c0 * rows, colb * (c1 - c0));
+            else
+                for (long j = c0; j < c1; j++) memcpy(dst + (size_t)j * ldh, src + (size_t)j * rows, colb);
+        };
+        if (T == 1 || (long)colb * w < (4L << 20)) {
+            part(0);
+        } else {
+            std::vector<std::thread> th;
+            for (int t = 1; t < T; t++) th.emplace_back(part, t);
+            part(0);
+            for (auto &x : th) x.join();
+        }
+        if (b + NB < nblk) CK(issue(b + NB));
+    }
+    return 0;
+}
+
 // Splitting and scaling of MDSTEDC (mdstedc.f90:305-359) into a plan of problems.
 static void make_plan_split(Plan &pl, int n, double *d, double *e, int smlsiz)
 {
@@ -1769,6 +1823,9 @@ int hsdc_ctx_destroy(hsdc_ctx *c)
     if (c->pin.p) hipHostFree(c->pin.p);
     if (c->pin2.p) hipHostFree(c->pin2.p);
     if (c->pin3.p) hipHostFree(c->pin3.p);
+    if (c->stage.p) hipHostFree(c->stage.p);
+    for (hipEvent_t e : c->stev) if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->pev) if (e) (void)hipEventDestroy(e);
     delete c;
     return 0;
 }
@@ -2686,8 +2743,7 @@ int hsdc_stedc(hsdc_ctx *c, char compz, int n, double *d, double *e, double *z, 
     if (info) return info;
     HK(hipMemcpy(d, dd, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (icompz == 2) {
-        HK(hipMemcpy2D(z, sizeof(double) * ldz, dz, sizeof(double) * n, sizeof(double) * n, n,
-                       hipMemcpyDeviceToHost));
+        CK(copy_d2h_matrix(c, z, ldz, dz, n, n));
     } else if (icompz == 1) {
         // COMPZ = 'V': Z := Zin * Q (the back-transformation of DSTEDC) on the device
         CK(ensure(c->misc, sizeof(double) * 2 * (size_t)n * n));
@@ -2697,8 +2753,7 @@ int hsdc_stedc(hsdc_ctx *c, char compz, int n, double *d, double *e, double *z, 
         hipLaunchKernelGGL(k_dgemm_nn, dim3(cdiv(n, 64), cdiv(n, 64)), dim3(WGS), 0, c->stream, n, n, n,
                            (const double *)za, (long)n, (const double *)dz, (long)n, zc, (long)n);
         HK(hipGetLastError());
-        HK(hipMemcpy2D(z, sizeof(double) * ldz, zc, sizeof(double) * n, sizeof(double) * n, n,
-                       hipMemcpyDeviceToHost));
+        CK(copy_d2h_matrix(c, z, ldz, zc, n, n));
     }
     return 0;
 }

@@ -70,3 +70,31 @@ def batch_sizes(count=512, lo=256, hi=8192, seed=3):
 def batch_problems(count=512, lo=256, hi=8192, seed=3, cfg=1):
     sizes = batch_sizes(count, lo, hi, seed)
     return [tridiag_cfg(cfg, int(n), seed, sub=k + 1) for k, n in enumerate(sizes)]
+
+
+def band_cfg(n, kd, seed=6, sub=0):
+    """(kd + 1) x n upper band storage of the symmetric band matrix of workload family 6:
+    entries N(0,1) by Box-Muller on stream (seed, sub), AB filled column-major, the
+    entries above the matrix (AB(i, j) with i + j <= kd, 0-based) left zero."""
+    cnt = (kd + 1) * n
+    u = splitmix_uniform(seed, 2 * ((cnt + 1) // 2), sub)
+    u1, u2 = u[0::2], u[1::2]
+    r = np.sqrt(-2.0 * np.log1p(-u1))  # 1 - u1 lies in (0, 1]
+    g = np.empty(2 * len(u1))
+    g[0::2] = r * np.cos(2.0 * np.pi * u2)
+    g[1::2] = r * np.sin(2.0 * np.pi * u2)
+    ab = g[:cnt].reshape(n, kd + 1).T.copy(order="F")
+    for j in range(min(kd, n)):
+        ab[:kd - j, j] = 0.0
+    return ab
+
+
+def band_dense(ab):
+    """The dense symmetric matrix of an upper band storage."""
+    kd = ab.shape[0] - 1
+    n = ab.shape[1]
+    a = np.zeros((n, n))
+    for j in range(n):
+        for i in range(max(0, j - kd), j + 1):
+            a[i, j] = a[j, i] = ab[kd + i - j, j]
+    return a

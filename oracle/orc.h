@@ -216,6 +216,26 @@ int orc_lasd0(int n, int sqre, double *d, double *e, double *u, int ldu, double 
 int orc_bdsdc(char uplo, char compq, int n, double *d, double *e, double *u, int ldu, double *vt,
               int ldvt, const orc_params *p, orc_trace *tr);
 
+/* ---- banded divide and conquer (mdsbevd.f90, mdsbed1-5.f90) ---- */
+/* per rank-one update (KD per merge, merges level by level): order n, K, rotations */
+typedef struct {
+    int cap, nupd, nmerge;
+    int *k, *n, *nrot;
+    long long iters;
+    double *sv;   /* optional: the singular values of each cut's coupling block (kd per cut) */
+} orc_band_trace;
+/* Cyclic two-sided Jacobi eigensolver of a dense symmetric m x m matrix (replaces
+ * DSYEVD at the leaves): eigenvalues ascending, vectors in v.  Sweeps, < 0 on failure. */
+int orc_sym_jacobi(int m, double *a, int lda, double *w, double *v, int ldv);
+/* One-sided Jacobi SVD of an m x m matrix (replaces DGESVD on the coupling blocks):
+ * c = u diag(s) v^T with s descending. */
+int orc_jacobi_svd(int m, double *c, int ldc, double *s, double *u, int ldu, double *v, int ldv);
+/* MDSBEVD with COMPZ = 'I', UPLO = 'U': the band ab (ldab >= kd + 1, upper storage
+ * AB(kd+1+i-j, j) = A(i,j)) of order n; eigenvalues ascending in d, vectors in q.
+ * smlsiz >= 2 kd - 1 so that every leaf holds at least kd rows.  Returns INFO. */
+int orc_sbevd(int n, int kd, const double *ab, int ldab, double *d, double *q, int ldq, int smlsiz,
+              orc_band_trace *tr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -327,6 +327,52 @@ def cauchy_dense_general(d, f, u, v):
     return a
 
 
+class BandTrace(ctypes.Structure):
+    _fields_ = [("cap", _i), ("nupd", _i), ("nmerge", _i), ("k", _P(_i)), ("n", _P(_i)), ("nrot", _P(_i)),
+                ("iters", ctypes.c_longlong), ("sv", _P(_d))]
+
+
+def sym_jacobi(a):
+    m = a.shape[0]
+    a = np.array(a, np.float64, order="F")
+    w, v = np.zeros(m), np.zeros((m, m), order="F")
+    sw = lib().orc_sym_jacobi(m, dp(a), m, dp(w), dp(v), m)
+    return w, v, sw
+
+
+def jacobi_svd(c):
+    m = c.shape[0]
+    c = np.array(c, np.float64, order="F")
+    s, u, v = np.zeros(m), np.zeros((m, m), order="F"), np.zeros((m, m), order="F")
+    sw = lib().orc_jacobi_svd(m, dp(c), m, dp(s), dp(u), m, dp(v), m)
+    return s, u, v, sw
+
+
+def sbevd(ab, kd, smlsiz=None, trace_cap=0):
+    """MDSBEVD (COMPZ 'I', UPLO 'U') on the upper band storage ab (kd+1 x n)."""
+    ab = np.array(ab, np.float64, order="F")
+    n = ab.shape[1]
+    smlsiz = smlsiz or max(2 * kd - 1, 25)
+    d, q = np.zeros(n), np.zeros((n, n), order="F")
+    tr = None
+    if trace_cap:
+        tr = BandTrace()
+        bufs = [np.zeros(trace_cap, np.int32) for _ in range(3)]
+        ncut = n // max(1, kd) + 2
+        sv = np.zeros(max(1, ncut * kd))
+        tr.cap = trace_cap
+        tr.k, tr.n, tr.nrot = [ip(b) for b in bufs]
+        tr.sv = dp(sv)
+    info = lib().orc_sbevd(n, kd, dp(ab), ab.shape[0], dp(d), dp(q), max(n, 1), smlsiz,
+                           ctypes.byref(tr) if tr is not None else None)
+    out = dict(d=d, q=q, info=info)
+    if tr is not None:
+        m = min(tr.nupd, trace_cap)
+        out["trace"] = dict(k=bufs[0][:m].copy(), n=bufs[1][:m].copy(), nrot=bufs[2][:m].copy(),
+                            nmerge=tr.nmerge, iters=tr.iters, sv=sv)
+    return out
+
+
 def cauchy_dense(f, u, v, difl, difr):
     k = len(f)
     a = np.zeros((k, k), order="F")
@@ -348,6 +394,9 @@ def _setup():
     L.orc_steqr.restype = _i
     L.orc_hss_build.restype = _i
     L.orc_cauchy2hss.restype = _i
+    L.orc_sbevd.restype = _i
+    L.orc_sym_jacobi.restype = _i
+    L.orc_jacobi_svd.restype = _i
     L.orc_hss_build_sq.restype = _i
     L.orc_lasd8.restype = _i
     L.orc_hss_tree_ok.restype = _i

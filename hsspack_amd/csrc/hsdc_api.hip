@@ -1884,6 +1884,7 @@ int hsdc_trace_indxq(hsdc_ctx *c, int *indxq, int n)
 // hsdc_thread_ctx_release or the end of the process).  Device buffers persist between
 // calls, so a repeated call of the same order allocates nothing.
 static thread_local hsdc_ctx *t_ctx = nullptr;
+__attribute__((visibility("hidden"))) void hsdc_sb_thread_release(void);  // the thread's banded context (hsdc_band.hip)
 static thread_local hsdc_bd_ctx *t_bd_ctx = nullptr;
 
 hsdc_ctx *hsdc_thread_ctx(void)
@@ -1916,6 +1917,7 @@ void hsdc_thread_ctx_release(void)
     if (t_bd_ctx) (void)hsdc_bd_ctx_destroy(t_bd_ctx);
     t_ctx = nullptr;
     t_bd_ctx = nullptr;
+    hsdc_sb_thread_release();
 }
 
 int hsdc_set_stream(hsdc_ctx *c, void *stream)

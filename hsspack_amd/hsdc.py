@@ -33,7 +33,9 @@ EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_s
            "hsdc_bd_default_params", "hsdc_bd_ctx_create", "hsdc_bd_ctx_destroy", "hsdc_bd_set_params",
            "hsdc_bd_get_stats", "hsdc_bd_set_stream", "hsdc_bdsdc", "hsdc_bdsdc_dev", "hsdc_bd_trace",
            "hsdc_trace", "hsdc_trace_indxq", "hsdc_loewner_noalloc", "hsdc_cauchyhssmm", "hsdc_thread_ctx",
-           "hsdc_bd_thread_ctx", "hsdc_thread_ctx_release"]
+           "hsdc_bd_thread_ctx", "hsdc_thread_ctx_release", "hsdc_sb_default_params", "hsdc_sb_ctx_create",
+           "hsdc_sb_ctx_destroy", "hsdc_sb_set_params", "hsdc_sb_get_stats", "hsdc_sb_set_stream",
+           "hsdc_sb_thread_ctx", "hsdc_sbevd", "hsdc_sbevd_dev", "hsdc_sb_trace"]
 
 # learned secular start: 34 inputs -> 64 -> 64 -> 1
 MLP_IN, MLP_H = 34, 64
@@ -68,6 +70,17 @@ class BdStats(ctypes.Structure):
                 ("roots", ctypes.c_longlong), ("deflated", ctypes.c_longlong),
                 ("rotations", ctypes.c_longlong), ("secular_iters", ctypes.c_longlong),
                 ("ms_total", _d), ("hss_merges", _i), ("hss_rank_max", _i)]
+
+
+class SbParams(ctypes.Structure):
+    _fields_ = [("smlsiz", _i), ("maxit", _i), ("hss_min_k", _i), ("hss_leaf", _i), ("hss_tol", _d)]
+
+
+class SbStats(ctypes.Structure):
+    _fields_ = [("levels", _i), ("merges", _i), ("leaves", _i), ("updates", _i), ("k_top", _i),
+                ("roots", ctypes.c_longlong), ("deflated", ctypes.c_longlong),
+                ("rotations", ctypes.c_longlong), ("secular_iters", ctypes.c_longlong),
+                ("hss_updates", _i), ("hss_rank_max", _i), ("ms_total", _d)]
 
 
 _LIB = None
@@ -139,6 +152,16 @@ def lib():
                                        _P(_i)]
         L.hsdc_thread_ctx.restype = _vp
         L.hsdc_bd_thread_ctx.restype = _vp
+        L.hsdc_sb_ctx_create.argtypes = [_P(_vp)]
+        L.hsdc_sb_ctx_destroy.argtypes = [_vp]
+        L.hsdc_sb_set_params.argtypes = [_vp, _P(SbParams)]
+        L.hsdc_sb_get_stats.argtypes = [_vp, _P(SbStats)]
+        L.hsdc_sb_set_stream.argtypes = [_vp, _vp]
+        L.hsdc_sb_thread_ctx.restype = _vp
+        L.hsdc_sbevd.argtypes = [_vp, ctypes.c_char, ctypes.c_char, _i, _i, _P(_d), _i, _P(_d), _P(_d), _i,
+                                 _P(_d), _i, _P(_i), _i]
+        L.hsdc_sbevd_dev.argtypes = [_vp, ctypes.c_char, _i, _i, _vp, _i, _vp, _vp, _i]
+        L.hsdc_sb_trace.argtypes = [_vp, _P(_i), _P(_i), _P(_i), _P(_i), _P(_i), _i]
         _LIB = L
     return _LIB
 
@@ -572,3 +595,77 @@ class BdContext:
         return lib().hsdc_bdsdc_dev(self._h, ctypes.c_char(uplo.encode()), n, d.data_ptr(), e.data_ptr(),
                                     u.data_ptr() if u is not None else None, n,
                                     vt.data_ptr() if vt is not None else None, n)
+
+
+class SbContext:
+    """Context of the banded DC eigensolver (MDSBEVD, SRC/mdsbevd.f90); not thread safe.
+
+    Parameters (hsdc_sb_params): smlsiz (leaf bound, 2 KD - 1 <= smlsiz <= 128, default
+    128), maxit (secular iteration cap), hss_min_k (an update with K >= hss_min_k takes the
+    HSS branch, MDSBED5 + DHSSEVC; 1500 as the reference's BSMLZ; 0 = never), hss_leaf
+    (HSS leaf block, 40 as the reference's Ni), hss_tol (HSS compression tolerance)."""
+
+    def __init__(self, **params):
+        self._h = _vp()
+        rc = lib().hsdc_sb_ctx_create(ctypes.byref(self._h))
+        if rc != 0:
+            raise RuntimeError("hsdc_sb_ctx_create failed: %d" % rc)
+        self.params = SbParams()
+        lib().hsdc_sb_default_params(ctypes.byref(self.params))
+        self.set_params(**params)
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        rc = lib().hsdc_sb_set_params(self._h, ctypes.byref(self.params))
+        if rc != 0:
+            raise ValueError("hsdc_sb_set_params rejected %r (%d)" % (kw, rc))
+
+    def set_stream(self, stream):
+        h = getattr(stream, "cuda_stream", stream)
+        return lib().hsdc_sb_set_stream(self._h, ctypes.c_void_p(h or 0))
+
+    def stats(self):
+        s = SbStats()
+        lib().hsdc_sb_get_stats(self._h, ctypes.byref(s))
+        return {f: getattr(s, f) for f, _ in SbStats._fields_}
+
+    def trace(self):
+        """Per rank-one update (level by level, update by update, merge by merge): block
+        start, order N, K, rotations, largest HSS rank (0: dense)."""
+        cnt = lib().hsdc_sb_trace(self._h, None, None, None, None, None, 0)
+        a = [np.zeros(max(cnt, 1), np.int32) for _ in range(5)]
+        lib().hsdc_sb_trace(self._h, *[_ip(x) for x in a], cnt)
+        return dict(s=a[0][:cnt], n=a[1][:cnt], k=a[2][:cnt], nrot=a[3][:cnt], hss_rank=a[4][:cnt])
+
+    def close(self):
+        if self._h:
+            lib().hsdc_sb_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sbevd(self, ab, kd, uplo="U", compz="I"):
+        """MDSBEVD on host arrays: ab (kd+1 x n band storage, any layout; read
+        column-major).  Returns (w ascending, Q (n x n) or None, info)."""
+        ab = np.asfortranarray(np.asarray(ab, np.float64))
+        ldab, n = ab.shape
+        w = np.zeros(max(n, 1))
+        vec = compz.upper() == "I"
+        q = np.zeros((n, n), order="F") if vec else None
+        info = lib().hsdc_sbevd(self._h, ctypes.c_char(compz.encode()), ctypes.c_char(uplo.encode()), n, kd,
+                                _dp(ab) if n else None, ldab, _dp(w), _dp(q) if (vec and n) else None, max(1, n),
+                                None, 0, None, 0)
+        return w[:n], q, info
+
+    def sbevd_dev(self, ab, kd, w, q, uplo="U"):
+        """Device-resident MDSBEVD on torch tensors: ab (n, ldab) -- band storage
+        column-major, i.e. row j of the tensor is column j of AB --, w (n) receives the
+        eigenvalues, q (n x n column-major, i.e. .T contiguous) the vectors or None."""
+        n, ldab = ab.shape
+        return lib().hsdc_sbevd_dev(self._h, ctypes.c_char(uplo.encode()), n, kd, ab.data_ptr(), ldab,
+                                    w.data_ptr(), q.data_ptr() if q is not None else None, max(1, n))

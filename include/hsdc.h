@@ -313,6 +313,61 @@ int hsdc_bdsdc_dev(hsdc_bd_ctx *ctx, char uplo, int n, double *d_d, double *d_e,
  * pointer may be null.  Returns the merge count (entries beyond cap are not written). */
 int  hsdc_bd_trace(hsdc_bd_ctx *ctx, int *s, int *n, int *k, int *nrot, int *hss_rank, int cap);
 
+/* ---------------------------------------------------------------------------
+ * Banded divide and conquer: HSSPACK's MDSBEVD (SRC/mdsbevd.f90:183-184) with its merge
+ * MDSBED1 (SRC/mdsbed1.f90:2): each KD x KD coupling block split off by its SVD
+ * (DSymSplit, BasicMM.f90:1195), the leaves solved densely, and every merge taken as KD
+ * rank-one updates -- dense secular vectors below hss_min_k, the HSS form of the
+ * Cauchy-like vector matrix at or above it (MDSBED5 -> DHSSEVC, mdsbed5.f90:277).
+ * KD <= 64 and leaves of at most 128 rows (smlsiz); the leaves must hold KD rows, so
+ * smlsiz >= 2 KD - 1 whenever the matrix is split.
+ * ------------------------------------------------------------------------- */
+typedef struct hsdc_sb_ctx hsdc_sb_ctx;
+/* The calling thread's own banded context (see hsdc_thread_ctx). */
+hsdc_sb_ctx *hsdc_sb_thread_ctx(void);
+
+typedef struct {
+    int smlsiz;     /* leaf bound, 2 KD - 1 <= smlsiz <= 128 (default 128; the reference takes
+                       ILAENV(9) = 25 for every KD, mdsbevd.f90:256) */
+    int maxit;      /* secular iteration cap */
+    int hss_min_k;  /* an update with K >= hss_min_k takes the HSS branch: BSMLZ = 1500
+                       (mdsbed1.f90:134); 0 = never */
+    int hss_leaf;   /* HSS leaf block, Ni = 40 (mdsbed5.f90:276); >= 8 */
+    double hss_tol; /* HSS compression tolerance, 1.0E-17 */
+} hsdc_sb_params;
+
+typedef struct {
+    int levels, merges, leaves, updates, k_top;
+    long long roots, deflated, rotations, secular_iters;
+    int hss_updates, hss_rank_max;  /* updates of the HSS branch, largest generator rank */
+    double ms_total;                /* device time of the last solve (events), ms */
+} hsdc_sb_stats;
+
+void hsdc_sb_default_params(hsdc_sb_params *p);
+int  hsdc_sb_ctx_create(hsdc_sb_ctx **ctx);
+int  hsdc_sb_ctx_destroy(hsdc_sb_ctx *ctx);
+int  hsdc_sb_set_params(hsdc_sb_ctx *ctx, const hsdc_sb_params *p);
+int  hsdc_sb_get_stats(hsdc_sb_ctx *ctx, hsdc_sb_stats *s);
+int  hsdc_sb_set_stream(hsdc_sb_ctx *ctx, void *stream);
+
+/* MDSBEVD (mdsbevd.f90:183).  compz 'I' (values and vectors) or 'N' (values only; the
+ * reference prints a message and returns for 'N'); uplo 'U' or 'L' band storage
+ * ab(ldab, n), ldab >= kd + 1 (the reference reads 'U' whatever uplo says).  d(n):
+ * eigenvalues ascending; q(ldq, n): eigenvectors (compz 'I').  work, iwork are accepted
+ * for the reference's argument list and not used.  INFO: -i for an illegal argument i
+ * (-20: smlsiz below 2 KD - 1, or a single leaf above 128 rows), 1 when a Jacobi leaf or
+ * split, or a secular root, did not converge. */
+int hsdc_sbevd(hsdc_sb_ctx *ctx, char compz, char uplo, int n, int kd, const double *ab, int ldab, double *d,
+               double *q, int ldq, double *work, int lwork, int *iwork, int liwork);
+/* The same on device arrays (d_q may be NULL for values only). */
+int hsdc_sbevd_dev(hsdc_sb_ctx *ctx, char uplo, int n, int kd, const double *d_ab, int ldab, double *d_d,
+                   double *d_q, int ldq);
+/* Rank-one updates of the last solve, level by level, update by update, merge by merge:
+ * block start (0-based), order N, K, deflation rotations and the largest HSS generator
+ * rank (0: dense); any pointer may be null.  Returns the update count (entries beyond
+ * cap are not written). */
+int  hsdc_sb_trace(hsdc_sb_ctx *ctx, int *s, int *n, int *k, int *nrot, int *hss_rank, int cap);
+
 #ifdef __cplusplus
 }
 #endif

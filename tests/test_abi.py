@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     txt = open(os.path.join(ROOT, "include", "hsdc.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|hsdc_ctx\s*\*|hsdc_bd_ctx\s*\*)\s*(hsdc_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|hsdc_\w*ctx\s*\*)\s*(hsdc_\w+)\s*\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

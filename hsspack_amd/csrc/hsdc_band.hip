@@ -44,6 +44,8 @@ __global__ void k_loewner(const Merge *, const int *, int, const MergeOut *, Wor
 __global__ void k_alpha(const Merge *, const int *, int, const MergeOut *, Work);
 __global__ void k_sbuild(const Merge *, const int *, int, const MergeOut *, const long *, Work, double *);
 __global__ void k_bgemm(const BG *);
+template <int BM, int BN, int BK, int DB, int TA, int TB>
+__global__ void k_bgemm_w(const BG *);
 __global__ void k_fill_u64(unsigned long long *, long, unsigned long long);
 
 namespace {
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
     __shared__ double zc[DCH], dc[DCH];
     __shared__ int nc[DCH];
     __shared__ double red[2][BWG / 64];
-    __shared__ int flag, s_k, s_nrot;
+    __shared__ int flag, s_k, s_nrot, s_cnt, s_rot, pre[BWG];
     const SbMerge M = mg[blockIdx.x];
     const int s = M.s, n = M.n, n1 = M.n1, tid = threadIdx.x, up = A.up;
     const double rho0 = A.cuts[(long)M.cut * (2L * A.kd * A.kd + A.kd) + 2 * A.kd * A.kd + up];
@@ -463,7 +465,60 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
     if (rho * az <= tol) {  // all deflated (MDSBED2's early exit): INDXP = INDX, K = 0
         for (int r = tid; r < n; r += BWG) ip[r] = ix[r];
         if (tid == 0) { s_k = 0; s_nrot = 0; }
+        __syncthreads();
     } else {
+        // Parallel pre-pass: the survivors (rho |z| > tol) in sorted order, numbered by a
+        // block-wide prefix count, and each survivor's rotation test against the survivor
+        // before it on the staged values.  The scan's first rotation would be one of these
+        // tests, so when none passes there is no rotation at all and the scan's result is
+        // the survivors kept in order, the others deflated from the back in order.
+        int *pos = A.rpj + s, *L = A.rnj + s;  // scratch until a rotation is logged
+        const int seg = (n + BWG - 1) / BWG, r0 = min(n, tid * seg), r1 = min(n, r0 + seg);
+        int cnt = 0;
+        for (int r = r0; r < r1; r++) cnt += !(rho * fabs(A.srtz[s + r]) <= tol);
+        pre[tid] = cnt;
+        if (tid == 0) s_rot = 0;
+        __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int t = 0; t < BWG; t++) { const int c = pre[t]; pre[t] = acc; acc += c; }
+            s_cnt = acc;
+        }
+        __syncthreads();
+        int q = pre[tid];
+        for (int r = r0; r < r1; r++) {
+            const bool big = !(rho * fabs(A.srtz[s + r]) <= tol);
+            pos[r] = big ? q : -1 - (r - q);  // survivor number, or -1 - number among the deflated
+            if (big) L[q++] = r;
+        }
+        __syncthreads();
+        const int ns = s_cnt;
+        int any = 0;
+        for (int j = 1 + tid; j < ns; j += BWG) {
+            const int rp = L[j - 1], rn = L[j];
+            const double zp = A.srtz[s + rp], zn = A.srtz[s + rn];
+            const double tau = hypot(zn, zp), t0 = A.srtd[s + rn] - A.srtd[s + rp];
+            const double c = zn / tau, sn = -zp / tau;
+            any |= fabs(t0 * c * sn) <= tol;
+        }
+        if (any) atomicOr(&s_rot, 1);
+        __syncthreads();
+        if (!s_rot) {
+            for (int r = tid; r < n; r += BWG) {
+                const int qq = pos[r];
+                if (qq >= 0) {
+                    A.dlamda[s + qq] = A.srtd[s + r];
+                    A.w[s + qq] = A.srtz[s + r];
+                    ip[qq] = ix[r];
+                } else {
+                    ip[n + qq] = ix[r];  // n - 1 - (-1 - qq)
+                }
+            }
+            if (tid == 0) { s_k = ns; s_nrot = 0; }
+            __syncthreads();
+        }
+    }
+    if (!(rho * az <= tol) && s_rot) {  // a rotation happens: thread 0's scan in order
         int k = 0, k2 = n, pj = -1, nrot = 0;
         double zp = 0.0, dp = 0.0;
         for (int c0 = 0; c0 < n; c0 += DCH) {
@@ -1064,8 +1119,13 @@ int sb_solve(hsdc_sb_ctx *c, int lower, int n, int kd, const double *dab, long l
                                    nm, (const MergeOut *)dmo, W);
             hipLaunchKernelGGL(k_sbuild, dim3(gch), dim3(BWG), 0, st, (const Merge *)dm2, (const int *)T.chunk, nm,
                                (const MergeOut *)dmo, so, W, (double *)c->s.p);
-            hipLaunchKernelGGL(k_bgemm, dim3((unsigned)cdivl(L.nmax, 128), nm, (unsigned)cdivl(L.nmax, 64)),
-                               dim3(BWG), 0, st, (const BG *)T.bq);
+            if (L.nmax >= 4096)  // wide tiles where they fill the chip (same bits as k_bgemm)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<256, 128, 16, 1, 4, 4>),
+                                   dim3((unsigned)cdivl(L.nmax, 256), nm, (unsigned)cdivl(L.nmax, 128)), dim3(512), 0,
+                                   st, (const BG *)T.bq);
+            else
+                hipLaunchKernelGGL(k_bgemm, dim3((unsigned)cdivl(L.nmax, 128), nm, (unsigned)cdivl(L.nmax, 64)),
+                                   dim3(BWG), 0, st, (const BG *)T.bq);
             if (up + 1 < kd)
                 hipLaunchKernelGGL(k_bgemm, dim3((unsigned)cdivl(L.nmax, 128), nm, 1), dim3(BWG), 0, st,
                                    (const BG *)T.bz);

@@ -1005,6 +1005,140 @@ __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
         }
 }
 
+// ---------------------------------------------------------------------------
+// The same batched GEMM with wide tiles for large items: block = BM x BN outputs,
+// wavefronts of (16 TA) x (16 TB) each (TA x TB accumulators of 16 x 16), k in chunks of
+// BK staged through LDS (DB: two slabs, one barrier per chunk) with the next chunk in
+// registers meanwhile.  At 256 x 128 with 64 x 64 per wavefront (eight wavefronts, two
+// per SIMD) each wavefront reads 4 A and 4 B operands from LDS per 16 MFMAs (k_bgemm: 6
+// per 8), and a block loads BK (BM + BN) doubles per 2 BM BN BK flops -- 21 flops per
+// byte against 11 at 128 x 64, which keeps the slabs' L2 traffic well under its
+// bandwidth at the FP64 MFMA rate.  Measured on square products of order 4096-16384:
+// 45 TF/s against k_bgemm's 35; below about 3000 rows and columns it has too few blocks
+// to fill the 256 CUs and k_bgemm is faster.  The k-steps of 4 run in the same order as
+// in k_bgemm, so both kernels give the same bits.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int BK, int DB, int TA, int TB>
+__global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bgemm_w(const BG *items)
+{
+    constexpr int WM = 16 * TA, WN = 16 * TB;  // rows and columns of one wavefront's tile
+    constexpr int NT = (BM / WM) * (BN / WN) * 64;
+    constexpr int AP = BM + 4, BP = BN + 4;
+    constexpr int QA = BM * BK / NT, QB = BN * BK / NT;
+    static_assert(QA * NT == BM * BK && QB * NT == BN * BK, "staging split");
+    __shared__ double Asb[(DB ? 2 : 1) * BK * AP];
+    __shared__ double Bsb[(DB ? 2 : 1) * BK * BP];
+    double *As = Asb, *Bs = Bsb;
+    const BG g = items[blockIdx.y];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int rb = blockIdx.x * BM, c0 = blockIdx.z * BN;
+    if (rb >= g.nrow || c0 >= g.n) return;  // the whole block
+    const int K1 = g.k, K = g.k + g.k2;
+    const long lda = g.lda, lda2 = g.lda2, ldb = g.ldb, ldb2 = g.ldb2;
+    // staging: A element i = tid + NT q at (row rb + i % BM, k kc + i / BM); B element i at
+    // (k kc + i / BN, column c0 + i % BN) for btrans, (k kc + i % BK, column c0 + i / BK)
+    // column-major
+    double ra[QA], rbv[QB];
+    auto stage = [&](int kc) {
+#pragma unroll
+        for (int q = 0; q < QA; q++) {
+            const int i = tid + NT * q, row = rb + (i % BM), t = kc + i / BM;
+            double v = 0.0;
+            if (row < g.nrow && t < K) v = t < K1 ? g.A[row + t * lda] : g.A2[row + (t - K1) * lda2];
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+            const int i = tid + NT * q;
+            int t, col;
+            if (g.btrans) { t = kc + i / BN; col = c0 + i % BN; }
+            else { t = kc + i % BK; col = c0 + i / BK; }
+            double v = 0.0;
+            if (col < g.n && t < K) {
+                if (t >= K1) v = g.B2[(long)col * ldb2 + (t - K1)];
+                else v = g.btrans ? g.B[(long)t * ldb + col] : g.B[(long)col * ldb + t];
+            }
+            rbv[q] = v;
+        }
+    };
+    auto commit = [&](double *Ad, double *Bd) {
+#pragma unroll
+        for (int q = 0; q < QA; q++) {
+            const int i = tid + NT * q;
+            Ad[(i / BM) * AP + (i % BM)] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+            const int i = tid + NT * q;
+            if (g.btrans) Bd[(i / BN) * BP + (i % BN)] = rbv[q];
+            else Bd[(i % BK) * BP + (i / BK)] = rbv[q];
+        }
+    };
+    const int lr = lane & 15, lk = lane >> 4;
+    const int wr = (wv % (BM / WM)) * WM, wc = (wv / (BM / WM)) * WN;
+    d4h acc[TA][TB];
+#pragma unroll
+    for (int a = 0; a < TA; a++)
+#pragma unroll
+        for (int b = 0; b < TB; b++) acc[a][b] = (d4h){0.0, 0.0, 0.0, 0.0};
+    stage(0);
+    commit(As, Bs);
+    __syncthreads();
+    for (int kc = 0; kc < K; kc += BK) {
+        const bool more = kc + BK < K;
+        if (more) stage(kc + BK);  // in flight while this chunk multiplies
+        const int ksteps = min(BK, ((K - kc) + 3) & ~3);
+        for (int kq = 0; kq < ksteps; kq += 4) {
+            const double *ap = As + (kq + lk) * AP + wr + lr;
+            const double *bp = Bs + (kq + lk) * BP + wc + lr;
+            double av[TA], bv[TB];
+#pragma unroll
+            for (int a = 0; a < TA; a++) av[a] = ap[16 * a];
+#pragma unroll
+            for (int b = 0; b < TB; b++) bv[b] = bp[16 * b];
+#pragma unroll
+            for (int a = 0; a < TA; a++)
+#pragma unroll
+                for (int b = 0; b < TB; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        if (DB) {
+            // the other slab was last read before the previous barrier: fill it now, then
+            // one barrier makes it readable and frees this one
+            if (more) {
+                double *An = (As == Asb) ? Asb + BK * AP : Asb, *Bn = (Bs == Bsb) ? Bsb + BK * BP : Bsb;
+                commit(An, Bn);
+                As = An;
+                Bs = Bn;
+            }
+            __syncthreads();
+        } else {
+            __syncthreads();  // every wavefront is done with this chunk
+            if (more) {
+                commit(As, Bs);
+                __syncthreads();
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < TA; a++)
+#pragma unroll
+        for (int b = 0; b < TB; b++) {
+            const int col = c0 + wc + b * 16 + lr;
+            if (col >= g.n) continue;
+            const long pc = g.cmap ? (long)g.cmap[col] - g.cmoff : (long)col;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = rb + wr + a * 16 + lk + 4 * i;
+                if (row < g.nrow) {
+                    double *cp = g.C + (long)row + pc * g.ldc;
+                    *cp = g.beta ? (*cp + acc[a][b][i]) : acc[a][b][i];
+                }
+            }
+        }
+}
+template __global__ void k_bgemm_w<256, 128, 16, 1, 4, 4>(const BG *);
+
 // X(:, t) = Q2(:, t): the K surviving old eigenvectors of each HSS merge packed
 // densely (mdlaed2.f90:467-499), zero outside each column's support.
 __global__ __launch_bounds__(WG) void k_gatherx(const Merge *mg, const int *hidx, const int *col_start, int nh,

@@ -1,7 +1,8 @@
 """Banded divide and conquer on the device (MDSBEVD, mdsbevd.f90:183-498; MDSBED1,
 mdsbed1.f90:109-312) against the oracle's restatement and LAPACK.
 
-* Dense branch (hss_min_k = 0) at orders the oracle finishes in seconds: every rank-one
+* Dense branch (hss_min_k = 0) at orders the oracle finishes in seconds (4100: the
+  top merge's products take the wide-tile GEMM): every rank-one
   update's K and rotation count identical to the oracle's (MDSBED2 / MDSBED22's
   deflation), eigenvalues within 4 n eps |A|_2 of the oracle and of LAPACK dsbevd
   (scipy eig_banded), vectors within 1e-10 of the oracle's, orthogonality and residual
@@ -80,7 +81,7 @@ def lower_storage(ab):
 
 
 @pytest.mark.parametrize("n,kd", [(40, 0), (60, 1), (200, 3), (300, 8), (257, 16), (512, 32), (700, 64),
-                                  (1000, 5)])
+                                  (1000, 5), (4100, 2)])
 def test_dense_vs_oracle(torch_dev, n, kd):
     ab = band_cfg(n, kd, seed=6, sub=n)
     A = band_dense(ab)

@@ -18,7 +18,7 @@
 #include "hsdc_hss.h"
 #include "hsdc_hss_host.h"
 
-#define HSS_LDS_V 17408  // doubles of dynamic LDS for an RRLU node's generator column
+#define HSS_LDS_V 16384  // doubles of dynamic LDS for an RRLU node's generator column
 #include "hsdc_mlp.h"
 #include "../../include/hsdc.h"
 
@@ -860,7 +860,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         hipLaunchKernelGGL(k_hss_dgen, dim3(nleaf), dim3(WGS), 0, s, dH, dN + leaf0, (int)nleaf);
         // the 512-thread RRLU launches (fewer nodes than two per CU) keep each node's
         // private generator column in LDS when it fits: 160 KB per workgroup on gfx950,
-        // 22.2 KB of it the kernels' static arrays
+        // 30.2 KB of it the kernels' static arrays
         const int ldsn = std::min(kmax, HSS_LDS_V);
         const size_t ldsb = sizeof(double) * (size_t)ldsn;
         for (int st = 0; st < maxlvl; st++) {
@@ -1697,18 +1697,19 @@ static int copy_d2h_matrix(hsdc_ctx *c, double *host, long ldh, const double *de
         const long w = std::min(per, cols - b * per);
         const double *src = st + (size_t)(b % NB) * per * rows;
         double *dst = host + (size_t)b * per * ldh;
-        auto part = [&](int t) {
-            const long c0 = w * t / T, c1 = w * (t + 1) / T;
+        // columns [w t / nt, w (t + 1) / nt) of the block, t < nt
+        auto part = [&](int t, int nt) {
+            const long c0 = w * t / nt, c1 = w * (t + 1) / nt;
             if (ldh == rows) memcpy(dst + (size_t)c0 * ldh, src + (size_t)c0 * rows, colb * (c1 - c0));
             else
                 for (long j = c0; j < c1; j++) memcpy(dst + (size_t)j * ldh, src + (size_t)j * rows, colb);
         };
         if (T == 1 || (long)colb * w < (4L << 20)) {
-            part(0);
+            part(0, 1);
         } else {
             std::vector<std::thread> th;
-            for (int t = 1; t < T; t++) th.emplace_back(part, t);
-            part(0);
+            for (int t = 1; t < T; t++) th.emplace_back(part, t, T);
+            part(0, T);
             for (auto &x : th) x.join();
         }
         if (b + NB < nblk) CK(issue(b + NB));

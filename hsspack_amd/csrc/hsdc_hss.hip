@@ -90,6 +90,28 @@ __device__ VI block_argmax(VI x, VI *sh)
     return r;
 }
 
+// The same with one barrier: successive calls alternate between the two halves of sh
+// (2 x blockDim / 64 entries), so a call's writes never meet the previous call's reads
+// and the trailing barrier that protects a single buffer is not needed.  tog is
+// uniform over the block and flips on every call.
+__device__ __forceinline__ VI block_argmax_t(VI x, VI *sh, int &tog)
+{
+    const int tid = threadIdx.x, nw = (int)(blockDim.x >> 6);
+    VI *b = sh + tog * nw;
+    tog ^= 1;
+    x = vi_step<32>(x);
+    x = vi_step<16>(x);
+    x = vi_step<8>(x);
+    x = vi_step<4>(x);
+    x = vi_step<2>(x);
+    x = vi_step<1>(x);
+    if ((tid & 63) == 0) b[tid >> 6] = x;
+    __syncthreads();
+    VI r = b[0];
+    for (int w = 1; w < nw; w++) r = vi_better(r, b[w]);
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // leaf D blocks (CauchylikeEig on the leaf rows/cols, CauchyHssEig.f90:2089-2100)
 // ---------------------------------------------------------------------------
@@ -136,8 +158,10 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
 {
     extern __shared__ double hss_dyn[];  // lds_cap doubles: V when the node's columns fit
     __shared__ double F[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
+    __shared__ double T1[HSS_MAXM], T2[HSS_MAXM];  // factors of thread 0's ordered products
     __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
-    __shared__ VI red[HSS_CWG / 64];
+    __shared__ VI red[2 * (HSS_CWG / 64)];
+    int tog = 0;  // which half of red the next block_argmax_t uses
     __shared__ long s_off;
     __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
     __shared__ double s_nmax;
@@ -211,7 +235,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 double v = fabs(A1(Z, i) / fdq<SQ>(H, pli, pwk));
                 x = vi_better(x, VI{v, i - k + 1});
             }
-            VI cl = block_argmax(x, red);
+            VI cl = block_argmax_t(x, red, tog);
             VI rw = pre_rw;  // computed by the previous step's update pass when pre
             if (!pre) {
                 VI y = {0.0, 0};
@@ -221,7 +245,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     double v = fabs(A1(V, j) / fdj(fk, j));
                     y = vi_better(y, VI{v, j - k + 1});
                 }
-                rw = block_argmax(y, red);
+                rw = block_argmax_t(y, red, tog);
             }
             double junkL = cl.v * fabs(A1(V, k)), junkU = rw.v * fabs(A1(Z, k));
             double piv = fabs(A1(Z, k) * A1(V, k) / fdq<SQ>(H, A1(PL, k), A1(PU, k)));
@@ -248,7 +272,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                             double v = fabs(A1(V, j) / fdj(fk2, j));
                             y2 = vi_better(y2, VI{v, j - k + 1});
                         }
-                        jjU = block_argmax(y2, red).i;
+                        jjU = block_argmax_t(y2, red, tog).i;
                         fl += 4ull * (N - k + 1);
                         if (jjU == 1) break;
                         flg = 0;
@@ -263,7 +287,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                         double v = fabs(A1(Z, i) / fdq<SQ>(H, A1(PL, i), pwk2));
                         x2 = vi_better(x2, VI{v, i - k + 1});
                     }
-                    jjL = block_argmax(x2, red).i;
+                    jjL = block_argmax_t(x2, red, tog).i;
                     fl += 4ull * (M - k + 1);
                     if (jjL == 1) break;
                     flg = 1;
@@ -291,7 +315,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 if (nb.i == c.i) bestrow.i = mi;
                 best = nb;
             }
-            VI win = block_argmax(best, red);
+            VI win = block_argmax_t(best, red, tog);
             __syncthreads();
             if (win.i > 0 && best.i == win.i) { s_ii = bestrow.i; s_jj = win.i; }
             __syncthreads();
@@ -335,20 +359,20 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
             double temp = -1.0 * fdq<SQ>(H, A1(PL, j), puk);
             A1(W, j) = A1(W, j) * (temp / ppq<SQ>(Fk, A1(F, j)));
+            // the factors of W(k) (their gathers in parallel; the product stays in order)
+            const double t2 = -1.0 * fdq<SQ>(H, plk, A1(PU, j));
+            A1(T1, j) = t2 / ppq<SQ>(A1(F, j), Fk);
         }
         __syncthreads();
         if (tid == 0) {
             double wk = fdq<SQ>(H, plk, puk) / A1(A, k);
-            for (int j = 1; j <= k - 1; j++) {
-                double temp = -1.0 * fdq<SQ>(H, plk, A1(PU, j));
-                wk = wk * (temp / ppq<SQ>(A1(F, j), Fk));
-            }
+            for (int j = 1; j <= k - 1; j++) wk = wk * A1(T1, j);
             A1(W, k) = wk;
         }
         __syncthreads();
         fl += 5ull * (M - k) + 7ull * (N - k) + 12ull * (k - 1);
         pre = fuse;
-        if (fuse) pre_rw = block_argmax(y1, red);
+        if (fuse) pre_rw = block_argmax_t(y1, red, tog);
         // ---- every 10 steps: swap while max |Mat_N| > rho (searchMax2Eig)
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
@@ -366,7 +390,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     if (nb.i == jc) IP[jc - 1] = mi;
                     x = nb;
                 }
-                VI win = block_argmax(x, red);
+                VI win = block_argmax_t(x, red, tog);
                 if (!(win.v > rho)) break;
                 const int ii = win.i, jj = IP[ii - 1] + k;
                 fl += 10ull * (N - k) + 8ull * M + 18ull * k;
@@ -387,6 +411,10 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 for (int j = 1 + tid; j <= k; j += HSS_CWG) {
                     if (j == ii) continue;
                     A1(W, j) = A1(W, j) * (ppq<SQ>(A1(F, j), Fii) / ppq<SQ>(A1(F, j), Fjj));
+                    double temp = fdq<SQ>(H, plii, A1(PU, j));
+                    A1(T1, j) = ppq<SQ>(Fii, A1(F, j)) / temp;
+                    temp = -1.0 * fdq<SQ>(H, pljj, A1(PU, j));
+                    A1(T2, j) = temp / ppq<SQ>(A1(F, j), Fjj);
                 }
                 __syncthreads();
                 if (tid == 0) {
@@ -394,10 +422,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     double wi = fdq<SQ>(H, pljj, A1(PU, ii)) / A1(A, jj);
                     for (int j = 1; j <= k; j++) {
                         if (j == ii) continue;
-                        double temp = fdq<SQ>(H, plii, A1(PU, j));
-                        uj = uj * (ppq<SQ>(Fii, A1(F, j)) / temp);
-                        temp = -1.0 * fdq<SQ>(H, pljj, A1(PU, j));
-                        wi = wi * (temp / ppq<SQ>(A1(F, j), Fjj));
+                        uj = uj * A1(T1, j);
+                        wi = wi * A1(T2, j);
                     }
                     A1(Z, jj) = uj;
                     A1(W, ii) = wi;
@@ -466,8 +492,10 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
 {
     extern __shared__ double hss_dyn[];  // lds_cap doubles: V when the node's rows fit
     __shared__ double Dr[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
+    __shared__ double T1[HSS_MAXM], T2[HSS_MAXM];  // factors of thread 0's ordered products
     __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
-    __shared__ VI red[HSS_CWG / 64];
+    __shared__ VI red[2 * (HSS_CWG / 64)];
+    int tog = 0;  // which half of red the next block_argmax_t uses
     __shared__ long s_off;
     __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
     const int tid = threadIdx.x;
@@ -525,7 +553,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 double v = fabs(A1(Z, i) / fdq<SQ>(H, puk, A1(PL, i)));
                 x = vi_better(x, VI{v, i - k + 1});
             }
-            VI cl = block_argmax(x, red);
+            VI cl = block_argmax_t(x, red, tog);
             VI rw = pre_rw;  // computed by the previous step's update pass when pre
             if (!pre) {
                 VI y = {0.0, 0};
@@ -535,7 +563,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     double v = fabs(A1(V, j) / fdq<SQ>(H, A1(PU, j), plk));
                     y = vi_better(y, VI{v, j - k + 1});
                 }
-                rw = block_argmax(y, red);
+                rw = block_argmax_t(y, red, tog);
             }
             double junkL = cl.v * fabs(A1(V, k)), junkU = rw.v * fabs(A1(Z, k));
             double den;
@@ -568,7 +596,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                             double v = fabs(A1(V, j) / fdq<SQ>(H, A1(PU, j), plk2));
                             y2 = vi_better(y2, VI{v, j - k + 1});
                         }
-                        jjU = block_argmax(y2, red).i;
+                        jjU = block_argmax_t(y2, red, tog).i;
                         fl += 4ull * (N - k + 1);
                         if (jjU == 1) break;
                         flg = 0;
@@ -587,7 +615,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                         double v = fabs(A1(Z, i) / fdq<SQ>(H, puk2, A1(PL, i)));
                         x2 = vi_better(x2, VI{v, i - k + 1});
                     }
-                    jjL = block_argmax(x2, red).i;
+                    jjL = block_argmax_t(x2, red, tog).i;
                     fl += 4ull * (M - k + 1);
                     if (jjL == 1) break;
                     flg = 1;
@@ -616,7 +644,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 if (nb.i == c.i) bestrow = mi;
                 best = nb;
             }
-            VI win = block_argmax(best, red);
+            VI win = block_argmax_t(best, red, tog);
             if (win.i > 0 && best.i == win.i) { s_ii = bestrow; s_jj = win.i; }
             __syncthreads();
             if (tid == 0) {
@@ -658,22 +686,21 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             double temp2 = ddq<SQ>(H, plk, A1(PL, j));
             temp = temp / temp2;
             A1(W, j) = A1(W, j) * temp;
+            // the factors of W(k) (their gathers in parallel; the product stays in order)
+            temp = fdq<SQ>(H, A1(PU, j), plk);
+            temp2 = ddq<SQ>(H, A1(PL, j), plk);
+            A1(T1, j) = temp / temp2;
         }
         __syncthreads();
         if (tid == 0) {
             double wk = (-1.0 * fdq<SQ>(H, puk, plk)) / A1(A, k);
-            for (int j = 1; j <= k - 1; j++) {
-                double temp = fdq<SQ>(H, A1(PU, j), plk);
-                double temp2 = ddq<SQ>(H, A1(PL, j), plk);
-                temp = temp / temp2;
-                wk = wk * temp;
-            }
+            for (int j = 1; j <= k - 1; j++) wk = wk * A1(T1, j);
             A1(W, k) = wk;
         }
         __syncthreads();
         fl += 6ull * (M - k) + 5ull * (N - k) + 14ull * (k - 1);
         pre = fuse;
-        if (fuse) pre_rw = block_argmax(y1, red);
+        if (fuse) pre_rw = block_argmax_t(y1, red, tog);
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
                 VI x = {0.0, 0};
@@ -691,7 +718,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     if (nb.i == jc) IP[jc - 1] = mi;
                     x = nb;
                 }
-                VI win = block_argmax(x, red);
+                VI win = block_argmax_t(x, red, tog);
                 if (!(win.v > rho)) break;
                 const int ii = win.i, jj = IP[ii - 1] + k;
                 fl += 10ull * (N - k) + 12ull * M + 22ull * k;
@@ -717,6 +744,12 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     double temp2 = ddq<SQ>(H, A1(PL, j), pljj);
                     temp = temp / temp2;
                     A1(W, j) = A1(W, j) * temp;
+                    temp = ddq<SQ>(H, plii, A1(PL, j));
+                    temp2 = -1.0 * fdq<SQ>(H, A1(PU, j), plii);
+                    A1(T1, j) = temp / temp2;
+                    temp = fdq<SQ>(H, A1(PU, j), pljj);
+                    temp2 = ddq<SQ>(H, A1(PL, j), pljj);
+                    A1(T2, j) = temp / temp2;
                 }
                 __syncthreads();
                 if (tid == 0) {
@@ -726,12 +759,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     double wi = (-1.0 * fdq<SQ>(H, A1(PU, ii), pljj)) / A1(A, jj);
                     for (int j = 1; j <= k; j++) {
                         if (j == ii) continue;
-                        temp = ddq<SQ>(H, plii, A1(PL, j));
-                        temp2 = -1.0 * fdq<SQ>(H, A1(PU, j), plii);
-                        uj = uj * (temp / temp2);
-                        temp = fdq<SQ>(H, A1(PU, j), pljj);
-                        temp2 = ddq<SQ>(H, A1(PL, j), pljj);
-                        wi = wi * (temp / temp2);
+                        uj = uj * A1(T1, j);
+                        wi = wi * A1(T2, j);
                     }
                     A1(Z, jj) = uj;
                     A1(W, ii) = wi;

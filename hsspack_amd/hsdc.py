@@ -290,16 +290,16 @@ class Context:
         return q
 
     # ---- MDSTEDC (host arrays) ----
-    def stedc(self, d, e, compz="I", z=None, lwork=None, liwork=None):
-        """Returns (d, z, info) like MDSTEDC; d and e are copied."""
+    def stedc(self, d, e, compz="I", z=None, lwork=None, liwork=None, ldz=None):
+        """Returns (d, z, info) like MDSTEDC; d and e are copied.  ldz (>= n): leading
+        dimension of the Z array the library writes (z is returned as its n x n part)."""
         n = len(d)
         d = np.array(d, np.float64)
         e = np.array(e, np.float64) if n > 1 else np.zeros(1)
-        ldz = max(1, n)
-        if z is None:
-            z = np.zeros((n, n), order="F")
-        else:
-            z = np.array(z, np.float64, order="F")
+        ldz = max(1, n) if ldz is None else ldz
+        zb = np.zeros((ldz, n), order="F")
+        if z is not None:
+            zb[:n, :] = np.asarray(z, np.float64)
         if lwork is None:
             lwork = 1 + 4 * n + n * n if compz != "V" else 1 + 3 * n + 2 * n * 17 + 4 * n * n
         if liwork is None:
@@ -307,8 +307,8 @@ class Context:
         work = np.zeros(max(1, min(lwork, 4)) if lwork > 0 else 1)
         iwork = np.zeros(1, np.int32)
         info = lib().hsdc_stedc(self._h, ctypes.c_char(compz.encode()), n, _dp(d), _dp(e),
-                                _dp(z) if n else None, ldz, _dp(work), lwork, _ip(iwork), liwork)
-        return d, z, info
+                                _dp(zb) if n else None, ldz, _dp(work), lwork, _ip(iwork), liwork)
+        return d, np.asfortranarray(zb[:n, :]), info
 
     # ---- device-resident MDSTEDC on torch tensors ----
     def stedc_dev(self, d, e, z):

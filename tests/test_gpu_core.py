@@ -254,6 +254,17 @@ def test_host_api_and_info(ctx):
     np.testing.assert_allclose(np.abs(z2), np.abs(q0 @ z), atol=1e-12)
 
 
+@pytest.mark.parametrize("n,pad", [(1500, 0), (1500, 3), (4000, 1)])
+def test_host_api_staged_copy(ctx, n, pad):
+    """Host-array COMPZ='I' at orders whose Z goes back through the staged copy with
+    several host threads (4000: two staged blocks), with ldz = n and ldz > n."""
+    d, e = tridiag_cfg(2, n, 5 + pad)
+    ctx.set_params(hss_min_k=1 << 30)
+    lam, z, info = ctx.stedc(d, e, ldz=n + pad)
+    assert info == 0
+    check_eig(d, e, lam, z)
+
+
 @pytest.mark.parametrize("cfg,n,ni", [(1, 700, 32), (2, 700, 32), (1, 1100, 64), (1, 2100, 128)])
 def test_stedc_hss_branch_vs_oracle(torch_dev, ctx, cfg, n, ni):
     d, e = tridiag_cfg(cfg, n, 40 + n)

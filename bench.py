@@ -21,7 +21,7 @@ replayed as a HIP graph).  Prints one JSON line.  Also in the line:
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cfg 1 --seed 4]
                        [--no-cpu] [--no-plain] [--no-check] [--no-hss]
-                       [--batch | --svd [--svd-toeplitz]]
+                       [--batch | --svd [--svd-toeplitz] | --band [--kd 64]]
 """
 import argparse
 import ctypes
@@ -547,6 +547,141 @@ def bench_svd(args, np, torch, hsdc):
         "cpu_baseline": cpu}), flush=True)
 
 
+def bench_band(args, np, torch, hsdc):
+    """Banded workload: all eigenpairs of the order-16384 symmetric band matrix with
+    KD = 64 and N(0,1) entries (seed 6), device resident, through hsdc_sbevd_dev:
+    leaves of 128 rows, the HSS branch for updates with K >= 1500 (HSS leaf 128,
+    tolerance 1e-17).  Also: a profiled pass (HSS construction and multiply rates), the
+    dense-branch-only ablation, orthogonality and residual, the oracle at n = 1024 and
+    2048 extrapolated, and LAPACK dsbevd (scipy eig_banded) on the host's threads."""
+    from hsspack_amd.workloads import band_cfg
+    n = args.n if args.n != 32768 else 16384
+    kd = args.kd
+    ab = band_cfg(n, kd, seed=6, sub=0)
+    dev = torch.device("cuda:0")
+    abt = torch.tensor(np.ascontiguousarray(ab.T), device=dev)
+    w = torch.empty(n, dtype=torch.float64, device=dev)
+    q = torch.empty((n, n), dtype=torch.float64, device=dev)
+    hss_leaf = args.hss_leaf or 128
+    hmin = args.hss_min_k if args.hss_min_k != 2000 else 1500
+    ctx = hsdc.SbContext(hss_min_k=hmin, hss_leaf=hss_leaf, hss_tol=1e-17)
+
+    def step():
+        rc = ctx.sbevd_dev(abt, kd, w, q)
+        if rc != 0:
+            raise RuntimeError("hsdc_sbevd_dev returned %d" % rc)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    st = ctx.stats()
+    log("sbevd n=%d kd=%d: %.1f ms, stats %s" % (n, kd, t * 1e3, st))
+    # profiled pass: event-timed HSS phases
+    ctx.set_params(profile=1)
+    step()
+    torch.cuda.synchronize()
+    sp = ctx.stats()
+    ctx.set_params(profile=0)
+    out = dict(levels=st["levels"], leaves=st["leaves"], updates=st["updates"], roots=st["roots"],
+               deflated=st["deflated"], rotations=st["rotations"], hss_updates=st["hss_updates"],
+               hss_rank_max=st["hss_rank_max"], iters_per_root=st["secular_iters"] / max(st["roots"], 1),
+               ms_device=st["ms_total"], dense_tflop=st["dense_flops"] / 1e12)
+    build_rate = sp["hss_nflops"] / (sp["ms_hss_build"] * 1e-3) / 1e12 if sp["ms_hss_build"] > 0 else None
+    mm_rate = sp["hss_mm_flops"] / (sp["ms_hss_mm"] * 1e-3) / 1e12 if sp["ms_hss_mm"] > 0 else None
+    out["hss_construction"] = dict(ms=sp["ms_hss_build"], model_tflop=sp["hss_nflops"] / 1e12,
+                                   tflops_in_model=build_rate, frac_fp64_peak=(build_rate or 0) / FP64_PEAK_TFS,
+                                   share_of_solve=sp["ms_hss_build"] / sp["ms_total"])
+    out["hss_multiply"] = dict(ms=sp["ms_hss_mm"], tflop=sp["hss_mm_flops"] / 1e12, tflops=mm_rate,
+                               frac_fp64_peak=(mm_rate or 0) / FP64_PEAK_TFS)
+    roof = {"bound": "mfma", "kernel": "HSS construction (k_hss_rowcomp + k_hss_colcomp), the largest share",
+            "achieved": build_rate, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+            "frac": (build_rate or 0) / FP64_PEAK_TFS, "traffic": None,
+            "note": "flops in HSSPACK's RRLU model over the event-timed construction; the structured "
+                    "RRLU is a chain of dependent pivot steps per HSS node, so the rate is latency-bound"}
+    if not args.no_plain:
+        ctx.set_params(hss_min_k=0)
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        td = time.perf_counter() - t0
+        sd = ctx.stats()
+        ctx.set_params(hss_min_k=hmin)
+        out.update(dense_only_s=td, dense_only_tflop=sd["dense_flops"] / 1e12,
+                   dense_only_tflops=sd["dense_flops"] / td / 1e12, speedup_vs_dense=td / t)
+        log("dense branch only: %.1f ms (%.1f TFLOP, %.1f TF/s)" % (td * 1e3, sd["dense_flops"] / 1e12,
+                                                                  sd["dense_flops"] / td / 1e12))
+    if not args.no_check:
+        A = torch.zeros((n, n), dtype=torch.float64, device=dev)
+        abd = torch.tensor(ab, device=dev)
+        j = torch.arange(n, device=dev)
+        for r in range(kd + 1):
+            off = kd - r
+            jj = j[off:]
+            A[jj - off, jj] = abd[r, off:]
+            A[jj, jj - off] = abd[r, off:]
+        Q = q.t()
+        orth = res = 0.0
+        for i0 in range(0, n, 4096):
+            g = Q[:, i0:i0 + 4096].t() @ Q
+            g[:, i0:i0 + 4096].diagonal().sub_(1.0)
+            orth = max(orth, g.abs().max().item())
+            del g
+            rr = A @ Q[:, i0:i0 + 4096] - Q[:, i0:i0 + 4096] * w[i0:i0 + 4096]
+            res = max(res, rr.abs().max().item())
+            del rr
+        res /= A.abs().max().item()
+        del A
+        out.update(orthogonality=orth, residual=res)
+        log("orthogonality %.2e residual %.2e" % (orth, res))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_py as O
+        import scipy.linalg as sl
+        times = {}
+        for m in (1024, 2048):
+            abm = band_cfg(m, kd, seed=6, sub=0)
+            t0 = time.perf_counter()
+            r = O.sbevd(abm, kd, smlsiz=128)
+            times[m] = time.perf_counter() - t0
+            assert r["info"] == 0
+            log("cpu oracle sbevd n=%d: %.2f s (%d threads)" % (m, times[m], host_threads()))
+        est, p = _extrapolate(np, times, n)
+        cpu = dict(value=est, unit="s", cores=host_threads(), cpu=cpu_model(), kind="port",
+                   sample="oracle (C restatement of mdsbevd/mdsbed1 with dense secular vectors, OpenMP over "
+                          "independent GEMM columns) on the same generator at n=1024 (%.2f s) and n=2048 (%.2f s), "
+                          "extrapolated to n=%d with the measured exponent %.2f" % (times[1024], times[2048], n, p))
+        lt = {}
+        for m in (4096, 8192):
+            abm = band_cfg(m, kd, seed=6, sub=0)
+            t0 = time.perf_counter()
+            sl.eig_banded(abm, lower=False)
+            lt[m] = time.perf_counter() - t0
+            log("cpu LAPACK eig_banded n=%d: %.2f s" % (m, lt[m]))
+        le, lp = _extrapolate(np, lt, n)
+        out["cpu_lapack_dsbevd"] = dict(value=le, unit="s", cores=host_threads(), kind="lapack",
+                                        sample="scipy eig_banded (LAPACK band reduction + tridiagonal solve, "
+                                               "OpenBLAS) at n=4096 (%.2f s) and 8192 (%.2f s), extrapolated "
+                                               "with exponent %.2f" % (lt[4096], lt[8192], lp))
+    print(json.dumps({
+        "metric": "s to all eigenpairs, n=%d FP64 symmetric band KD=%d (MDSBEVD path)" % (n, kd),
+        "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": dict({"workload": "symmetric band n=%d KD=%d, N(0,1) entries seed 6, leaves 128, HSS branch "
+                                    "for K>=%d (HSS leaf %d, tol 1e-17), all eigenpairs, device resident"
+                                    % (n, kd, hmin, hss_leaf)}, **out),
+        "roofline": roof, "cpu_baseline": cpu}, default=lambda o: o.item() if hasattr(o, "item") else str(o)),
+        flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -569,10 +704,15 @@ def main():
     ap.add_argument("--svd-toeplitz", action="store_true",
                     help="with --svd: the near-Toeplitz bidiagonal (HSS branch at the top merges), "
                          "timed against the dense branch alone")
+    ap.add_argument("--band", action="store_true",
+                    help="banded DC: all eigenpairs of the order-16384 symmetric band matrix, KD=64, N(0,1) "
+                         "entries, seed 6 (MDSBEVD path)")
+    ap.add_argument("--kd", type=int, default=64, help="with --band: the bandwidth")
     ap.add_argument("--hss-leaf", type=int, default=0,
-                    help="with --svd: HSS leaf block (default 128 with --svd-toeplitz, else 40)")
+                    help="with --svd / --band: HSS leaf block (default 128 with --svd-toeplitz and --band, else 40)")
     ap.add_argument("--hss-min-k", type=int, default=2000,
-                    help="with --svd: a merge with K >= this takes the HSS branch (default 2000)")
+                    help="with --svd / --band: a merge (update) with K >= this takes the HSS branch (default "
+                         "2000; 1500 with --band)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device three times (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -607,6 +747,8 @@ def main():
         return bench_batch(args, np, torch, hsdc)
     if args.svd:
         return bench_svd(args, np, torch, hsdc)
+    if args.band:
+        return bench_band(args, np, torch, hsdc)
 
     n = args.n
     d, e = tridiag_cfg(args.cfg, n, args.seed)

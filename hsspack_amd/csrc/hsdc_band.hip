@@ -1057,6 +1057,15 @@ int sb_solve(hsdc_sb_ctx *c, int lower, int n, int kd, const double *dab, long l
                        (long)n, Dp[0], indxq, dinfo);
     SHK(hipGetLastError());
     // ---- merges
+    hsdc_stats h0;  // the HSS context's totals before this solve (its figures are the change)
+    memset(&h0, 0, sizeof h0);
+    if (c->hctx) {
+        hsdc_params hp;
+        hsdc_default_params(&hp);
+        hp.profile = c->prm.profile;
+        SCK(hsdc_set_params(c->hctx, &hp));
+        SCK(hsdc_get_stats(c->hctx, &h0));
+    }
     int cur = 0, zc = 0;
     long tbase = 0;
     const int hmin = c->prm.hss_min_k;
@@ -1168,7 +1177,14 @@ int sb_solve(hsdc_sb_ctx *c, int lower, int n, int kd, const double *dab, long l
                     q.Y = YS + yo[h]; q.ldy = nx;
                     q.sq = 0;
                 }
-                if (!c->hctx) SCK(hsdc_ctx_create(&c->hctx));
+                if (!c->hctx) {
+                    SCK(hsdc_ctx_create(&c->hctx));
+                    hsdc_params hp;
+                    hsdc_default_params(&hp);
+                    hp.profile = c->prm.profile;
+                    SCK(hsdc_set_params(c->hctx, &hp));
+                    SCK(hsdc_get_stats(c->hctx, &h0));
+                }
                 int nd = 0;
                 SCK(hsdc_hss_batch(c->hctx, st, sp, c->prm.hss_leaf, c->prm.hss_tol, &rk, &nd));
                 hipLaunchKernelGGL(k_sb_hss_scatter, dim3(hc[nh]), dim3(BWG), 0, st, dm, (const int *)dhm,
@@ -1208,6 +1224,14 @@ int sb_solve(hsdc_sb_ctx *c, int lower, int n, int kd, const double *dab, long l
     for (int i = 0; i < SEC_SLOTS; i++) c->st.secular_iters += (long long)cnt[i];
     std::vector<int> tr(2 * (size_t)std::max(ntr, 1L));
     SHK(hipMemcpy(tr.data(), dtrc, sizeof(int) * 2 * ntr, hipMemcpyDeviceToHost));
+    if (c->hctx) {
+        hsdc_stats h1;
+        SCK(hsdc_get_stats(c->hctx, &h1));
+        c->st.hss_nflops = h1.hss_nflops - h0.hss_nflops;
+        c->st.hss_mm_flops = h1.hss_mm_flops - h0.hss_mm_flops;
+        c->st.ms_hss_build = h1.ms_hss_build - h0.ms_hss_build;
+        c->st.ms_hss_mm = h1.ms_hss_mm - h0.ms_hss_mm;
+    }
     std::vector<int> hr(ntr, 0);
     for (size_t i = 0; i + 1 < c->trace.size(); i += 2) hr[c->trace[i]] = c->trace[i + 1];
     c->trace.clear();
@@ -1220,6 +1244,8 @@ int sb_solve(hsdc_sb_ctx *c, int lower, int n, int kd, const double *dab, long l
                 c->trace.insert(c->trace.end(), {L.m[m].s, L.m[m].n, k, nr, hr[t]});
                 c->st.roots += k;
                 c->st.deflated += L.m[m].n - k;
+                if (hr[t] == 0)  // dense branch: Q(:, 1:K) := X S and Z(1:K, later) := S^T Z
+                    c->st.dense_flops += 2LL * L.m[m].n * k * k + 2LL * k * k * (kd - up - 1);
                 c->st.rotations += nr;
                 if (l == nlv - 1 && up == kd - 1) c->st.k_top = k;
             }
@@ -1247,17 +1273,12 @@ void hsdc_sb_default_params(hsdc_sb_params *p)
     p->hss_min_k = 1500;            // BSMLZ, mdsbed1.f90:134
     p->hss_leaf = 40;               // Ni, mdsbed5.f90:276
     p->hss_tol = (double)1.0e-17f;  // TOL of DHSSEVC, a REAL literal (CauchyHssEig.f90:2523)
+    p->profile = 0;
 }
 
 int hsdc_sb_ctx_create(hsdc_sb_ctx **ctx)
 {
     if (!ctx) return HSDC_ERR_INTERNAL;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-        (void)hipGetLastError();
-        *ctx = nullptr;
-        return HSDC_ERR_NODEVICE;
-    }
     hsdc_sb_ctx *c = new hsdc_sb_ctx();
     hsdc_sb_default_params(&c->prm);
     memset(&c->st, 0, sizeof c->st);
@@ -1287,6 +1308,7 @@ int hsdc_sb_set_params(hsdc_sb_ctx *c, const hsdc_sb_params *p)
     if (p->hss_min_k < 0) return -3;
     if (p->hss_leaf < 8) return -4;
     if (!(p->hss_tol > 0.0)) return -5;
+    if (p->profile != 0 && p->profile != 1) return -6;
     c->prm = *p;
     return 0;
 }
@@ -1317,6 +1339,16 @@ hsdc_sb_ctx *hsdc_sb_thread_ctx(void)
     return t_sb_ctx;
 }
 
+static bool sb_have_device()
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
 // checks shared by both entry points; 0 when the call may go on
 static int sb_args(hsdc_sb_ctx *c, char uplo, int n, int kd, int ldab, int ldq, bool vec)
 {
@@ -1341,6 +1373,7 @@ int hsdc_sbevd_dev(hsdc_sb_ctx *c, char uplo, int n, int kd, const double *d_ab,
     if (n == 0) return 0;
     if (!d_ab) return -5;
     if (!d_d) return -7;
+    if (!sb_have_device()) return HSDC_ERR_NODEVICE;
     const int lower = (uplo == 'L' || uplo == 'l');
     return sb_solve(c, lower, n, kd, d_ab, ldab, d_d, d_q, ldq);
 }
@@ -1359,6 +1392,7 @@ int hsdc_sbevd(hsdc_sb_ctx *c, char compz, char uplo, int n, int kd, const doubl
     if (!ab) return -5;
     if (!d) return -7;
     if (vec && !q) return -8;
+    if (!sb_have_device()) return HSDC_ERR_NODEVICE;
     hipStream_t st = c->stream;
     SCK(sb_ensure(c->ab, sizeof(double) * (size_t)ldab * n));
     SCK(sb_ensure(c->dd, sizeof(double) * (size_t)n * (vec ? n + 1 : 1)));

@@ -85,6 +85,18 @@ module hsdc_c
       type(c_ptr) function hsdc_thread_ctx() bind(C, name='hsdc_thread_ctx')
          import :: c_ptr
       end function
+      type(c_ptr) function hsdc_sb_thread_ctx() bind(C, name='hsdc_sb_thread_ctx')
+         import :: c_ptr
+      end function
+      integer(c_int) function hsdc_sbevd(ctx, compz, uplo, n, kd, ab, ldab, d, q, ldq, work, lwork, iwork, &
+                                         liwork) bind(C, name='hsdc_sbevd')
+         import :: c_int, c_ptr, c_char, c_double
+         type(c_ptr), value :: ctx
+         character(kind=c_char), value :: compz, uplo
+         integer(c_int), value :: n, kd, ldab, ldq, lwork, liwork
+         real(c_double) :: ab(*), d(*), q(*), work(*)
+         integer(c_int) :: iwork(*)
+      end function
       type(c_ptr) function hsdc_bd_thread_ctx() bind(C, name='hsdc_bd_thread_ctx')
          import :: c_ptr
       end function
@@ -539,4 +551,30 @@ subroutine cauchyhssmm(d, f, u, v, n, ni, tol, ncol, b, ldb, time)
       write (*, *) 'cauchyhssmm failed: ', rc
       stop
    end if
+end subroutine
+
+
+! MDSBEVD (mdsbevd.f90:183-184): all eigenvalues (ascending, in D) and, for COMPZ 'I',
+! the eigenvectors (Q) of the symmetric band matrix in AB (LDAB >= KD + 1, UPLO 'U' or
+! 'L' storage).  WORK/IWORK are accepted and unused.  INFO as the C entry hsdc_sbevd:
+! -i for an illegal argument i (XERBLA is called with the reference's name), 1 when a
+! leaf, a split or a secular root did not converge.
+subroutine mdsbevd(compz, uplo, n, kd, ab, ldab, d, q, ldq, work, lwork, iwork, liwork, info)
+   use hsdc_c
+   implicit none
+   character :: compz, uplo
+   integer :: n, kd, ldab, ldq, lwork, liwork, info
+   double precision :: ab(ldab, *), d(*), q(ldq, *), work(*)
+   integer :: iwork(*)
+   type(c_ptr) :: ctx
+   integer(c_int) :: rc
+   info = 0
+   ctx = hsdc_sb_thread_ctx()
+   if (.not. c_associated(ctx)) then
+      info = 1003
+      return
+   end if
+   rc = hsdc_sbevd(ctx, compz, uplo, n, kd, ab, ldab, d, q, ldq, work, lwork, iwork, liwork)
+   info = hsdc_info(rc)
+   if (info < 0) call xerbla('MDSBEVD', -info)
 end subroutine

@@ -73,14 +73,17 @@ class BdStats(ctypes.Structure):
 
 
 class SbParams(ctypes.Structure):
-    _fields_ = [("smlsiz", _i), ("maxit", _i), ("hss_min_k", _i), ("hss_leaf", _i), ("hss_tol", _d)]
+    _fields_ = [("smlsiz", _i), ("maxit", _i), ("hss_min_k", _i), ("hss_leaf", _i), ("hss_tol", _d),
+                ("profile", _i)]
 
 
 class SbStats(ctypes.Structure):
     _fields_ = [("levels", _i), ("merges", _i), ("leaves", _i), ("updates", _i), ("k_top", _i),
                 ("roots", ctypes.c_longlong), ("deflated", ctypes.c_longlong),
                 ("rotations", ctypes.c_longlong), ("secular_iters", ctypes.c_longlong),
-                ("hss_updates", _i), ("hss_rank_max", _i), ("ms_total", _d)]
+                ("hss_updates", _i), ("hss_rank_max", _i), ("ms_total", _d),
+                ("hss_nflops", ctypes.c_longlong), ("hss_mm_flops", ctypes.c_longlong),
+                ("dense_flops", ctypes.c_longlong), ("ms_hss_build", _d), ("ms_hss_mm", _d)]
 
 
 _LIB = None
@@ -603,7 +606,8 @@ class SbContext:
     Parameters (hsdc_sb_params): smlsiz (leaf bound, 2 KD - 1 <= smlsiz <= 128, default
     128), maxit (secular iteration cap), hss_min_k (an update with K >= hss_min_k takes the
     HSS branch, MDSBED5 + DHSSEVC; 1500 as the reference's BSMLZ; 0 = never), hss_leaf
-    (HSS leaf block, 40 as the reference's Ni), hss_tol (HSS compression tolerance)."""
+    (HSS leaf block, 40 as the reference's Ni), hss_tol (HSS compression tolerance),
+    profile (1: event-timed HSS construction and multiply in the stats)."""
 
     def __init__(self, **params):
         self._h = _vp()

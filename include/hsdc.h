@@ -334,6 +334,7 @@ typedef struct {
                        (mdsbed1.f90:134); 0 = never */
     int hss_leaf;   /* HSS leaf block, Ni = 40 (mdsbed5.f90:276); >= 8 */
     double hss_tol; /* HSS compression tolerance, 1.0E-17 */
+    int profile;    /* 1: time the HSS construction and multiply with events (ms_hss_*) */
 } hsdc_sb_params;
 
 typedef struct {
@@ -341,6 +342,10 @@ typedef struct {
     long long roots, deflated, rotations, secular_iters;
     int hss_updates, hss_rank_max;  /* updates of the HSS branch, largest generator rank */
     double ms_total;                /* device time of the last solve (events), ms */
+    long long hss_nflops;           /* HSS construction work in HSSPACK's RRLU flop model */
+    long long hss_mm_flops;         /* HSS multiply: 2 rows n k over its GEMMs */
+    long long dense_flops;          /* dense-branch products: 2 N K^2 + 2 K^2 (KD - j - 1) */
+    double ms_hss_build, ms_hss_mm; /* profile = 1: device time of the two HSS phases */
 } hsdc_sb_stats;
 
 void hsdc_sb_default_params(hsdc_sb_params *p);

@@ -8,6 +8,7 @@
 !   bdsdc  IN: int32 n, uplo (int32), float64 d(n), e(n-1)  OUT: int32 info, d(n), u(n,n), vt(n,n)
 !   cauchy IN: int32 n, ni, ncol, float64 tol, d(n), f(n), u(n), v(n), b(n,n)
 !          OUT: b(n,n) after CAUCHYHSSMM, then b(n,n) after a second call on the input b
+!   sbevd  IN: int32 n, kd, uplo (int32), float64 ab(kd+1,n)  OUT: int32 info, d(n), q(n,n)
 !   errors OUT: for each illegal call, info and the XERBLA (name, argument) it made
 module xerbla_log
    implicit none
@@ -37,7 +38,8 @@ program fdrive
    character :: uplo
    double precision, allocatable :: vt(:, :), cf(:), cu_(:), cv(:), b0(:, :)
    double precision :: ctol, ctime(2)
-   integer :: cni, cncol
+   integer :: cni, cncol, kd
+   double precision, allocatable :: ab(:, :)
    call get_command_argument(1, mode)
    call get_command_argument(2, fin)
    call get_command_argument(3, fout)
@@ -106,6 +108,15 @@ program fdrive
       q = b0
       call cauchyhssmm(d, cf, cu_, cv, n, cni, ctol, cncol, q, n, ctime)
       write(v) q
+   case ('sbevd')
+      open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+      read(u) n, kd, cu
+      uplo = achar(cu)
+      allocate(ab(kd + 1, n), d(n), q(n, n), work(1), iwork(1))
+      read(u) ab
+      close(u)
+      call mdsbevd('I', uplo, n, kd, ab, kd + 1, d, q, n, work, 1, iwork, 1, info)
+      write(v) info, d, q
    case ('errors')
       allocate(d(8), e(8), q(8, 8), work(200), iwork(200), indxq(8), qs(8, 8))
       d = 1
@@ -133,6 +144,10 @@ program fdrive
       call mdbdsdc('U', 'I', 8, d, e, q, 4, qs, 8, work, iwork, work, iwork, info)
       write(v) info, last_arg, last_name
       call mdbdsdc('L', 'I', 8, d, e, q, 8, qs, 2, work, iwork, work, iwork, info)
+      write(v) info, last_arg, last_name
+      call mdsbevd('V', 'U', 8, 2, q, 3, d, qs, 8, work, 1, iwork, 1, info)
+      write(v) info, last_arg, last_name
+      call mdsbevd('I', 'U', 8, 2, q, 2, d, qs, 8, work, 1, iwork, 1, info)
       write(v) info, last_arg, last_name
    end select
    close(v)

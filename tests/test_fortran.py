@@ -34,17 +34,20 @@ def f64(a):
 
 def test_library_exports_the_fortran_names():
     out = subprocess.run(["nm", "-D", "--defined-only", FLIB], capture_output=True, text=True, check=True).stdout
-    for name in ("mdstedc_", "mdlaed0_", "mdlaed1_", "mdlaed2_", "mdlaed3_", "mdbdsdc_", "cauchyhssmm_"):
+    for name in ("mdstedc_", "mdlaed0_", "mdlaed1_", "mdlaed2_", "mdlaed3_", "mdbdsdc_", "cauchyhssmm_",
+                 "mdsbevd_"):
         assert (" T " + name) in out, name
 
 
 def test_argument_errors_call_xerbla_like_the_reference(tmp_path):
     """Illegal arguments return INFO = -i and call XERBLA with the LAPACK routine name
-    (mdstedc.f90:241, mdlaed0.f90:179, mdlaed1.f90:153, mdlaed3.f90:176, mdbdsdc.f90:174-187); no
+    (mdstedc.f90:241, mdlaed0.f90:179, mdlaed1.f90:153, mdlaed3.f90:176, mdbdsdc.f90:174-187,
+    mdsbevd.f90:290, whose name the 6-character log of the driver's XERBLA truncates); no
     device needed."""
     raw = run("errors", tmp_path)
     want = [(-1, "DSTEDC"), (-6, "DSTEDC"), (-8, "DSTEDC"), (-1, "DLAED0"), (-9, "DLAED0"), (-7, "DLAED1"),
-            (-2, "DLAED3"), (-1, "DBDSDC"), (-2, "DBDSDC"), (-3, "DBDSDC"), (-7, "DBDSDC"), (-9, "DBDSDC")]
+            (-2, "DLAED3"), (-1, "DBDSDC"), (-2, "DBDSDC"), (-3, "DBDSDC"), (-7, "DBDSDC"), (-9, "DBDSDC"),
+            (-1, "MDSBEV"), (-6, "MDSBEV")]
     rec = 14
     assert len(raw) == rec * len(want)
     for i, (info, name) in enumerate(want):
@@ -203,3 +206,29 @@ def test_cauchyhssmm_from_fortran(tmp_path):
     ref = A @ b
     assert np.abs(y1 - ref).max() <= 1e-12 * np.abs(ref).max()
     assert np.array_equal(y1, y2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_mdsbevd_from_fortran(tmp_path, uplo):
+    """MDSBEVD('I', UPLO, N, KD, AB, LDAB, D, Q, LDQ, ...) from Fortran: the same
+    eigenpairs as the C entry on the same storage, eigenvalues against LAPACK dsbevd."""
+    import scipy.linalg as sl
+    from hsspack_amd import hsdc
+    from hsspack_amd.workloads import band_cfg
+    from test_gpu_band import lower_storage
+    n, kd = 600, 8
+    ab = band_cfg(n, kd, seed=6, sub=3)
+    st = ab if uplo == "U" else lower_storage(ab)
+    raw = run("sbevd", tmp_path, i32(n, kd, ord(uplo)) + f64(st.T))
+    info = np.frombuffer(raw[:4], "<i4")[0]
+    lam = np.frombuffer(raw[4:4 + 8 * n], "<f8")
+    q = np.frombuffer(raw[4 + 8 * n:], "<f8").reshape(n, n).T
+    assert info == 0
+    ctx = hsdc.SbContext()
+    w2, q2, info2 = ctx.sbevd(st, kd, uplo=uplo)
+    ctx.close()
+    assert info2 == 0
+    assert np.array_equal(lam, w2) and np.array_equal(q, q2)
+    w = sl.eig_banded(ab, lower=False, eigvals_only=True)
+    assert np.abs(lam - w).max() <= 4 * n * 2.0 ** -53 * np.abs(w).max()

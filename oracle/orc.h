@@ -175,6 +175,11 @@ int orc_cauchy2hss(orc_hss *H, int n, int ni, const double *d, const double *f, 
                    const double *v, double tol);
 void orc_cauchy_dense_general(int n, const double *d, const double *f, const double *u, const double *v,
                               double *a, int lda);
+/* A dense square matrix of order n (lda) in HSS form by truncated SVDs (DHSSMM /
+ * DHSSMML's mat2hssvd, ConstructHssd.f90:371-696): npart's tree with leaf size ni, every
+ * basis the singular vectors above tol (absolute).  Returns -1 for a tree of one leaf,
+ * 1 when a Jacobi SVD did not converge. */
+int orc_mat2hss(orc_hss *H, int n, int ni, const double *a, int lda, double tol);
 
 /* ---- bidiagonal SVD (mdbdsdc.f90 / mdlasd0-2.f90 + LAPACK DLASD3/DLASD4/DLASDQ) ---- */
 /* LAPACK DLASCL 'G' on a vector, DLARTG, DLASDT (node arrays 0-based, values 1-based). */

@@ -1256,3 +1256,314 @@ void orc_cauchy_dense_general(int n, const double *d, const double *f, const dou
 {
     cauchyG(a, lda, d, f, u, v, n, n);
 }
+
+/* ------------------------------------------------------------------------------------
+ * Dense matrix -> HSS by truncated SVDs (the construction half of DHSSMM / DHSSMML,
+ * DHSSMM.f90:1-125, DHSSMML.f90:2-122, mat2hssvd ConstructHssd.f90:371-696), restated
+ * level by level: the tree is npart's (the same as cauchy2hss's); every node of one
+ * depth is compressed from the current compressed matrix, whose row and column blocks
+ * are the active nodes' bases (a raw leaf: the identity).  A node's row basis U spans
+ * the left singular vectors of its block row (its rows against every other active
+ * node's columns) whose singular values exceed tol, its column basis W the right
+ * singular vectors of its block column likewise (mat2hssvd's count(svs > tol)); the
+ * generator V is W^T.  A sibling pair's B is the compressed block between them after
+ * both are compressed, a leaf's D its diagonal block of A.  Each SVD is Householder QR
+ * of the tall side, then one-sided Jacobi on R (orc_jacobi_svd).
+ * ------------------------------------------------------------------------------------ */
+
+/* One-sided Jacobi on the p columns (length l >= p) of c (ldc): pairs in round-robin
+ * order, a pair rotates when |c_a . c_b| > l eps |c_a| |c_b|; afterwards the columns are
+ * orthogonal, their norms the singular values.  sv (p) descending, u (l x p, ldu) the
+ * normalised columns in that order (zero for a zero column).  Returns the sweeps, < 0
+ * when 60 did not converge. */
+static int jacobi_cols(int l, int p, double *c, long ldc, double *sv, double *u, long ldu)
+{
+    const int P = p + (p & 1);
+    int *pos = (int *)malloc(sizeof(int) * (P > 0 ? P : 1)), conv = 0, sweeps;
+    for (sweeps = 1; sweeps <= 60 && !conv; sweeps++) {
+        int rot = 0;
+        for (int i = 0; i < P; i++) pos[i] = i;
+        for (int r = 0; r < P - 1; r++) {
+            for (int i = 0; i < P / 2; i++) {
+                int a = pos[i], b = pos[P - 1 - i];
+                if (a > b) { int t = a; a = b; b = t; }
+                if (b >= p) continue;
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int k = 0; k < l; k++) {
+                    const double x = c[k + (long)a * ldc], y = c[k + (long)b * ldc];
+                    al += x * x; be += y * y; ga += x * y;
+                }
+                if (ga == 0.0 || !(fabs(ga) > l * 0x1p-53 * sqrt(al * be))) continue;
+                const double ze = (be - al) / (2.0 * ga);
+                const double t = (ze >= 0.0 ? 1.0 : -1.0) / (fabs(ze) + sqrt(1.0 + ze * ze));
+                const double cc = 1.0 / sqrt(1.0 + t * t), ss = cc * t;
+                for (int k = 0; k < l; k++) {
+                    const double x = c[k + (long)a * ldc], y = c[k + (long)b * ldc];
+                    c[k + (long)a * ldc] = cc * x - ss * y;
+                    c[k + (long)b * ldc] = ss * x + cc * y;
+                }
+                rot++;
+            }
+            const int last = pos[P - 1];
+            for (int i = P - 1; i > 1; i--) pos[i] = pos[i - 1];
+            if (P > 1) pos[1] = last;
+        }
+        if (rot == 0) conv = 1;
+    }
+    int *ord = (int *)malloc(sizeof(int) * (p > 0 ? p : 1));
+    double *nr = (double *)malloc(sizeof(double) * (p > 0 ? p : 1));
+    for (int j = 0; j < p; j++) {
+        double q = 0.0;
+        for (int k = 0; k < l; k++) q += c[k + (long)j * ldc] * c[k + (long)j * ldc];
+        nr[j] = sqrt(q);
+        ord[j] = j;
+    }
+    for (int i = 0; i + 1 < p; i++) {  /* descending, ties by index */
+        int k = i;
+        for (int j = i + 1; j < p; j++) if (nr[ord[j]] > nr[ord[k]]) k = j;
+        const int t = ord[i]; ord[i] = ord[k]; ord[k] = t;
+    }
+    for (int j = 0; j < p; j++) {
+        const int s0 = ord[j];
+        sv[j] = nr[s0];
+        for (int k = 0; k < l; k++) u[k + (long)j * ldu] = nr[s0] > 0.0 ? c[k + (long)s0 * ldc] / nr[s0] : 0.0;
+    }
+    free(pos); free(ord); free(nr);
+    return conv ? sweeps - 1 : -1;
+}
+
+/* right singular vectors of the w x m matrix T (ldt, overwritten): v (m x m, columns in
+ * descending singular value order, the first min(w, m) meaningful), sv (m); returns the
+ * count above tol.  w >= m: Householder QR of T, then one-sided Jacobi on R's columns
+ * (their rotations are the right vectors); w < m: T^T has fewer columns than rows and
+ * its own columns are orthogonalised (their normalised results are the vectors) -- R
+ * would hold m columns in a w-dimensional space, which Jacobi cannot make orthogonal. */
+static int right_basis(int w, int m, double *T, long ldt, double tol, double *v, double *sv)
+{
+    if (w < m) {
+        double *X = (double *)malloc(sizeof(double) * (size_t)m * (w > 0 ? w : 1));
+        for (int j = 0; j < w; j++)
+            for (int i = 0; i < m; i++) X[i + (long)j * m] = T[j + (long)i * ldt];
+        for (long e = 0; e < (long)m * m; e++) v[e] = 0.0;
+        for (int j = 0; j < m; j++) sv[j] = 0.0;
+        int cnt = 0;
+        if (w > 0 && jacobi_cols(m, w, X, m, sv, v, m) < 0) cnt = -1;
+        else
+            for (int j = 0; j < w; j++) cnt += sv[j] > tol;
+        free(X);
+        return cnt;
+    }
+    const int p = m;
+    for (int k = 0; k < p; k++) {
+        double *x = T + k + (long)k * ldt;
+        const int len = w - k;
+        double xn = 0.0;
+        for (int i = 1; i < len; i++) xn += x[i] * x[i];
+        xn = sqrt(xn);
+        const double alpha = x[0];
+        if (xn == 0.0) continue;  /* H = I */
+        const double beta = -copysign(hypot(alpha, xn), alpha);
+        const double tau = (beta - alpha) / beta, scal = 1.0 / (alpha - beta);
+        for (int i = 1; i < len; i++) x[i] *= scal;  /* v = (1, x(1:)) */
+        for (int j = k + 1; j < m; j++) {
+            double *c = T + k + (long)j * ldt;
+            double dot = c[0];
+            for (int i = 1; i < len; i++) dot += x[i] * c[i];
+            dot *= tau;
+            c[0] -= dot;
+            for (int i = 1; i < len; i++) c[i] -= dot * x[i];
+        }
+        x[0] = beta;
+    }
+    double *R = (double *)calloc((size_t)m * m, sizeof(double)), *u = (double *)malloc(sizeof(double) * m * m);
+    for (int j = 0; j < m; j++)
+        for (int i = 0; i <= j && i < p; i++) R[i + (long)j * m] = T[i + (long)j * ldt];
+    int cnt = 0;
+    if (orc_jacobi_svd(m, R, m, sv, u, m, v, m) < 0) cnt = -1;
+    else
+        for (int j = 0; j < m; j++) cnt += sv[j] > tol;
+    free(R); free(u);
+    return cnt;
+}
+
+int orc_mat2hss(orc_hss *H, int n, int ni, const double *A, int lda, double tol)
+{
+    if (hss_tree_setup(H, n, ni) < 1) return -1;
+    const int ltr = H->ltr, lvl = H->lvl;
+    long cap = 10L * ni * n;
+    const int *ch = H->ch;
+    /* leaf ranges and depths */
+    int *r0 = (int *)calloc(ltr + 1, sizeof(int)), *dep = (int *)calloc(ltr + 1, sizeof(int));
+    {
+        int lt = 0, it = 1;
+        for (int nd = 1; nd <= ltr; nd++)
+            if (CH1(ch, nd) == 0) { A1(r0, nd) = lt; lt += A1(H->m, it); it++; }
+        for (int i = ltr - 1; i >= 1; i--) A1(dep, i) = A1(dep, A1(H->tr, i)) + 1;
+    }
+    /* active nodes in left-to-right order with their row / column blocks in the
+     * compressed matrix C (nr x nc, column-major) */
+    int na = 0;
+    int *act = (int *)malloc(sizeof(int) * ltr), *ars = (int *)malloc(sizeof(int) * ltr),
+        *arn = (int *)malloc(sizeof(int) * ltr), *acs = (int *)malloc(sizeof(int) * ltr),
+        *acn = (int *)malloc(sizeof(int) * ltr);
+    {
+        int it = 1;
+        for (int nd = 1; nd <= ltr; nd++)
+            if (CH1(ch, nd) == 0) {
+                act[na] = nd; ars[na] = acs[na] = A1(r0, nd); arn[na] = acn[na] = A1(H->m, it); na++; it++;
+            }
+    }
+    int nr = n, nc = n;
+    double *C = (double *)malloc(sizeof(double) * (size_t)n * n);
+    for (int j = 0; j < n; j++) memcpy(C + (size_t)j * n, A + (size_t)j * lda, sizeof(double) * n);
+    long pnd = 0;
+    /* leaf D blocks of A */
+    {
+        int it = 1;
+        for (int nd = 1; nd <= ltr; nd++)
+            if (CH1(ch, nd) == 0) {
+                const int mi = A1(H->m, it), s0 = A1(r0, nd);
+                for (int j = 0; j < mi; j++)
+                    for (int i = 0; i < mi; i++) H->dd[pnd + i + (long)j * mi] = A[(s0 + i) + (long)(s0 + j) * lda];
+                H->dsz[nd - 1] = mi; H->pd[nd - 1] = pnd; pnd += (long)mi * mi; it++;
+            }
+    }
+    int info = 0;
+    for (int LL = lvl; LL >= 1 && !info; LL--) {
+        /* the nodes of this depth and, for each, its constituents in the active list */
+        int nn = A1(H->lentr, LL);
+        int *nds = (int *)malloc(sizeof(int) * nn), *a0 = (int *)malloc(sizeof(int) * nn),
+            *a1 = (int *)malloc(sizeof(int) * nn);
+        for (int q = 0; q < nn; q++) {
+            const int nd = TTR(H, LL, q + 1);
+            nds[q] = nd;
+            const int f = CH1(ch, nd) ? CH1(ch, nd) : nd, l = CH1(ch, nd) ? CH2(ch, nd) : nd;
+            a0[q] = a1[q] = -1;
+            for (int t = 0; t < na; t++) { if (act[t] == f) a0[q] = t; if (act[t] == l) a1[q] = t; }
+        }
+        double **Ub = (double **)calloc(nn, sizeof(double *)), **Wb = (double **)calloc(nn, sizeof(double *));
+        int *ru = (int *)calloc(nn, sizeof(int)), *rw = (int *)calloc(nn, sizeof(int)),
+            *mr = (int *)calloc(nn, sizeof(int)), *mc = (int *)calloc(nn, sizeof(int));
+        for (int q = 0; q < nn && !info; q++) {
+            const int rs = ars[a0[q]], rn = ars[a1[q]] + arn[a1[q]] - rs;
+            const int cs = acs[a0[q]], cn = acs[a1[q]] + acn[a1[q]] - cs;
+            mr[q] = rn; mc[q] = cn;
+            /* block row: T = (C(rows, other columns))^T, (nc - cn) x rn */
+            const int w1 = nc - cn, w2 = nr - rn;
+            double *T = (double *)malloc(sizeof(double) * (size_t)(w1 > 0 ? w1 : 1) * rn);
+            for (int t = 0; t < rn; t++) {
+                int o = 0;
+                for (int c = 0; c < nc; c++) {
+                    if (c >= cs && c < cs + cn) continue;
+                    T[o + (long)t * w1] = C[(rs + t) + (size_t)c * nr];
+                    o++;
+                }
+            }
+            double *v = (double *)malloc(sizeof(double) * rn * rn), *sv = (double *)malloc(sizeof(double) * rn);
+            ru[q] = right_basis(w1, rn, T, w1 > 0 ? w1 : 1, tol, v, sv);
+            if (ru[q] < 0) info = 1;
+            Ub[q] = v;
+            free(T); free(sv);
+            /* block column: T = C(other rows, columns), (nr - rn) x cn */
+            T = (double *)malloc(sizeof(double) * (size_t)(w2 > 0 ? w2 : 1) * cn);
+            for (int t = 0; t < cn; t++) {
+                int o = 0;
+                for (int r = 0; r < nr; r++) {
+                    if (r >= rs && r < rs + rn) continue;
+                    T[o + (long)t * w2] = C[r + (size_t)(cs + t) * nr];
+                    o++;
+                }
+            }
+            v = (double *)malloc(sizeof(double) * cn * cn);
+            sv = (double *)malloc(sizeof(double) * cn);
+            rw[q] = right_basis(w2, cn, T, w2 > 0 ? w2 : 1, tol, v, sv);
+            if (rw[q] < 0) info = 1;
+            Wb[q] = v;
+            free(T); free(sv);
+            /* generators: U (rn x ru), V = W^T (rw x cn) */
+            const int nd = nds[q];
+            long off = push(H, &cap, (long)rn * (ru[q] > 0 ? ru[q] : 1));
+            for (int j = 0; j < ru[q]; j++)
+                for (int i = 0; i < rn; i++) H->h[off + i + (long)j * rn] = Ub[q][i + (long)j * rn];
+            H->usz[2 * (nd - 1)] = rn; H->usz[2 * (nd - 1) + 1] = ru[q]; H->pu[nd - 1] = off;
+            off = push(H, &cap, (long)(rw[q] > 0 ? rw[q] : 1) * cn);
+            for (int j = 0; j < cn; j++)
+                for (int i = 0; i < rw[q]; i++) H->h[off + i + (long)j * rw[q]] = Wb[q][j + (long)i * cn];
+            H->vsz[2 * (nd - 1)] = rw[q]; H->vsz[2 * (nd - 1) + 1] = cn; H->pv[nd - 1] = off;
+        }
+        if (!info) {
+            /* the new active list and the compressed matrix C' = blockdiag(U)^T C blockdiag(W) */
+            int nna = 0, nnr = 0, nnc = 0;
+            int *nact = (int *)malloc(sizeof(int) * ltr), *nrs = (int *)malloc(sizeof(int) * ltr),
+                *nrn = (int *)malloc(sizeof(int) * ltr), *ncs = (int *)malloc(sizeof(int) * ltr),
+                *ncn = (int *)malloc(sizeof(int) * ltr), *src = (int *)malloc(sizeof(int) * ltr);
+            for (int t = 0; t < na;) {
+                int q = -1;
+                for (int z = 0; z < nn; z++) if (a0[z] == t) q = z;
+                if (q >= 0) {
+                    nact[nna] = nds[q]; nrn[nna] = ru[q]; ncn[nna] = rw[q]; src[nna] = q;
+                    t = a1[q] + 1;
+                } else {
+                    nact[nna] = act[t]; nrn[nna] = arn[t]; ncn[nna] = acn[t]; src[nna] = -1 - t;
+                    t++;
+                }
+                nrs[nna] = nnr; ncs[nna] = nnc; nnr += nrn[nna]; nnc += ncn[nna]; nna++;
+            }
+            double *Tr = (double *)calloc((size_t)(nnr > 0 ? nnr : 1) * (nc > 0 ? nc : 1), sizeof(double));
+            for (int a = 0; a < nna; a++) {  /* rows */
+                const int s = src[a];
+                for (int c = 0; c < nc; c++)
+                    for (int i = 0; i < nrn[a]; i++) {
+                        double acc;
+                        if (s >= 0) {
+                            const int rs = ars[a0[s]];
+                            acc = 0.0;
+                            for (int t = 0; t < mr[s]; t++) acc += Ub[s][t + (long)i * mr[s]] * C[(rs + t) + (size_t)c * nr];
+                        } else {
+                            acc = C[(ars[-1 - s] + i) + (size_t)c * nr];
+                        }
+                        Tr[(nrs[a] + i) + (size_t)c * nnr] = acc;
+                    }
+            }
+            double *Cn = (double *)calloc((size_t)(nnr > 0 ? nnr : 1) * (nnc > 0 ? nnc : 1), sizeof(double));
+            for (int b = 0; b < nna; b++) {  /* columns */
+                const int s = src[b];
+                for (int j = 0; j < ncn[b]; j++)
+                    for (int r = 0; r < nnr; r++) {
+                        double acc;
+                        if (s >= 0) {
+                            const int cs = acs[a0[s]];
+                            acc = 0.0;
+                            for (int t = 0; t < mc[s]; t++) acc += Tr[r + (size_t)(cs + t) * nnr] * Wb[s][t + (long)j * mc[s]];
+                        } else {
+                            acc = Tr[r + (size_t)(acs[-1 - s] + j) * nnr];
+                        }
+                        Cn[r + (size_t)(ncs[b] + j) * nnr] = acc;
+                    }
+            }
+            free(Tr); free(C);
+            C = Cn; nr = nnr; nc = nnc;
+            na = nna;
+            memcpy(act, nact, sizeof(int) * nna); memcpy(ars, nrs, sizeof(int) * nna);
+            memcpy(arn, nrn, sizeof(int) * nna); memcpy(acs, ncs, sizeof(int) * nna);
+            memcpy(acn, ncn, sizeof(int) * nna);
+            free(nact); free(nrs); free(nrn); free(ncs); free(ncn); free(src);
+            /* B of every sibling pair of this depth: the compressed block between them */
+            for (int q = 0; q < nn; q++) {
+                const int nd = nds[q], par = A1(H->tr, nd);
+                const int sib = (CH1(ch, par) == nd) ? CH2(ch, par) : CH1(ch, par);
+                int ia = -1, ib = -1;
+                for (int t = 0; t < na; t++) { if (act[t] == nd) ia = t; if (act[t] == sib) ib = t; }
+                const int br = arn[ia], bc = acn[ib];
+                long off = push(H, &cap, (long)(br > 0 ? br : 1) * (bc > 0 ? bc : 1));
+                for (int j = 0; j < bc; j++)
+                    for (int i = 0; i < br; i++) H->h[off + i + (long)j * br] = C[(ars[ia] + i) + (size_t)(acs[ib] + j) * nr];
+                H->bsz[2 * (nd - 1)] = br; H->bsz[2 * (nd - 1) + 1] = bc; H->pb[nd - 1] = off;
+            }
+        }
+        for (int q = 0; q < nn; q++) { free(Ub[q]); free(Wb[q]); }
+        free(Ub); free(Wb); free(ru); free(rw); free(mr); free(mc); free(nds); free(a0); free(a1);
+    }
+    free(C); free(act); free(ars); free(arn); free(acs); free(acn); free(r0); free(dep);
+    return info;
+}

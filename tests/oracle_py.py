@@ -319,6 +319,17 @@ def cauchy2hss(d, f, u, v, ni, tol):
     return H
 
 
+def mat2hss(a, ni, tol):
+    """HSS form of a dense square matrix by truncated SVDs (mat2hssvd of DHSSMM / DHSSMML)."""
+    a = np.array(a, np.float64, order="F")
+    n = a.shape[0]
+    H = Hss()
+    info = lib().orc_mat2hss(ctypes.byref(H), n, ni, dp(a), n, _d(tol))
+    if info != 0:
+        raise RuntimeError("orc_mat2hss failed %d" % info)
+    return H
+
+
 def cauchy_dense_general(d, f, u, v):
     n = len(d)
     a = np.zeros((n, n), order="F")
@@ -394,6 +405,7 @@ def _setup():
     L.orc_steqr.restype = _i
     L.orc_hss_build.restype = _i
     L.orc_cauchy2hss.restype = _i
+    L.orc_mat2hss.restype = _i
     L.orc_sbevd.restype = _i
     L.orc_sym_jacobi.restype = _i
     L.orc_jacobi_svd.restype = _i

@@ -1,0 +1,96 @@
+"""The oracle's dense -> HSS construction (mat2hssvd of DHSSMM / DHSSMML,
+ConstructHssd.f90:371-696; DHSSMM.f90:1-125, DHSSMML.f90:2-122) pinned by what the
+mathematics fixes: with tol = 0 every basis keeps every nonzero singular value and the
+HSS form reproduces A to rounding; a diagonal plus rank-one matrix has off-diagonal
+ranks of at most one; every truncation drops singular values <= tol, so
+||A - H||_F <= tol sqrt(2 n lvl) bounds the product error of the reference's own test
+case (testfmm.f90:15-64: Cauchy matrix on interlaced points, n = 2000, ni = 64,
+tol = 1e-11); ranks fall as tol grows; the multiply agrees with the dense expansion."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+
+def fmm_matrix(n, seed):
+    """testfmm.f90:16-36: x_i = 0.1 + (2i - 1) gap, y = x + gap, gap = 8.9 / (2n),
+    A(i, j) = z_i v_j / (x_i - y_j) with z, v uniform in (0, 1)."""
+    gap = (9.0 - 0.1) / (2 * n)
+    x = 0.1 + gap * (2 * np.arange(1, n + 1) - 1)
+    y = x + gap
+    rng = np.random.default_rng(seed)
+    z, v = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    return O.cauchy_dense_general(x, y, z, v)
+
+
+def node_ranks(H):
+    usz, vsz = O.hss_ranks(H)
+    return usz[:-1, 1], vsz[:-1, 0]  # every node but the root
+
+
+def test_tol_zero_reproduces_a():
+    n, ni = 300, 32
+    A = np.random.default_rng(1).standard_normal((n, n))
+    H = O.mat2hss(A, ni, 0.0)
+    Hd = O.hss_dense(H)
+    O.hss_free(H)
+    assert np.abs(Hd - A).max() <= 1e-12 * np.abs(A).max()
+
+
+def test_diagonal_plus_rank_one_has_rank_one_blocks():
+    n, ni = 400, 40
+    rng = np.random.default_rng(2)
+    u, v = rng.standard_normal(n), rng.standard_normal(n)
+    A = np.diag(rng.standard_normal(n)) + np.outer(u, v)
+    H = O.mat2hss(A, ni, 1e-10)
+    ru, rv = node_ranks(H)
+    Hd = O.hss_dense(H)
+    O.hss_free(H)
+    assert ru.max() <= 1 and rv.max() <= 1 and ru.min() == 1
+    assert np.abs(Hd - A).max() <= 1e-12 * np.abs(A).max()
+
+
+def test_reference_case_product_error_bound():
+    n, ni, tol = 2000, 64, 1e-11
+    A = fmm_matrix(n, 3)
+    B = np.random.default_rng(4).uniform(0, 1, (n, n))
+    H = O.mat2hss(A, ni, tol)
+    lvl = H.lvl
+    ru, rv = node_ranks(H)
+    Y = O.hss_mmL(H, B.T).T  # (B^T H)^T = H^T B ... taken as H B through the transpose below
+    Hd = O.hss_dense(H)
+    O.hss_free(H)
+    assert np.allclose(Y, (B.T @ Hd).T, rtol=0, atol=1e-12 * np.abs(Y).max())
+    err = np.abs(Hd @ B - A @ B).max()
+    bound = tol * np.sqrt(2 * n * lvl) * np.linalg.norm(B, axis=0).max()
+    assert err <= bound, (err, bound)
+    assert np.abs(Hd - A).max() <= tol * np.sqrt(2 * n * lvl)
+    assert 0 < ru.max() < ni and 0 < rv.max() < ni  # compressible, not full
+
+
+def test_ranks_fall_as_tol_grows():
+    A = fmm_matrix(800, 5)
+    prev = None
+    for tol in (1e-14, 1e-11, 1e-8, 1e-5):
+        H = O.mat2hss(A, 50, tol)
+        ru, rv = node_ranks(H)
+        O.hss_free(H)
+        cur = ru + rv
+        if prev is not None:
+            assert np.all(cur <= prev)
+        prev = cur
+
+
+def test_multiply_matches_dense_expansion():
+    A = fmm_matrix(500, 6)
+    H = O.mat2hss(A, 40, 1e-12)
+    X = np.random.default_rng(7).standard_normal((33, 500))
+    Y = O.hss_mmL(H, X)
+    Hd = O.hss_dense(H)
+    O.hss_free(H)
+    assert np.abs(Y - X @ Hd).max() <= 1e-12 * np.abs(Y).max()
+
+
+def test_one_leaf_tree_is_refused():
+    with pytest.raises(RuntimeError):
+        O.mat2hss(np.eye(20), 32, 1e-12)

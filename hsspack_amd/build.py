@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhsdc.so")
 SOURCES = ["hsdc_kern.hip", "hsdc_hss.hip", "hsdc_mlp.hip", "hsdc_api.hip", "hsdc_svd.hip", "hsdc_noalloc.hip",
-           "hsdc_band.hip"]
+           "hsdc_band.hip", "hsdc_mat2hss.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 

@@ -58,6 +58,29 @@ __global__ void k_zfill(double *, long, long, long);
 __global__ void k_dgemm_nn(int, int, int, const double *, long, const double *, long, double *, long);
 __global__ void k_transpose(int, int, const double *, long, double *, long);
 __global__ void k_cauchy2hss(C2H);
+struct M2hJob {
+    int rs, rn, cs, cn;
+    int w, m;
+    long toff;
+    long boff;
+};
+struct M2hBlk {
+    int start, count;
+    int src;
+    int ostart, ocount;
+};
+struct M2hCopy {
+    int kind, rows, cols, job;
+    int r0, c0;
+    long dst;
+};
+__global__ void k_m2h_gather(const M2hJob *, const double *, long, double *);
+__global__ void k_m2h_svd(const M2hJob *, double *, double *, double, int *, int *);
+__global__ void k_m2h_rows(const M2hBlk *, int, const M2hJob *, const double *, const double *, long, int, double *,
+                           long);
+__global__ void k_m2h_cols(const M2hBlk *, int, const M2hJob *, const double *, const double *, long, int, double *);
+__global__ void k_m2h_copy(const M2hCopy *, const M2hJob *, const double *, const double *, long, const double *, long,
+                           double *, double *);
 __global__ void k_cfill_general(int, const double *, const double *, const double *, const double *, double *);
 __global__ void k_hss_init(const HssMerge *);
 __global__ void k_hss_reset(const HssMerge *);
@@ -90,7 +113,7 @@ int ensure(DevBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 8;
     if (b.cap >= bytes) return 0;
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
     size_t want = bytes + bytes / 8;
@@ -112,7 +135,7 @@ struct PinBuf {
 int ensure_pin(PinBuf &b, size_t bytes)
 {
     if (b.cap >= bytes) return 0;
-    if (b.p) hipHostFree(b.p);
+    if (b.p) (void)hipHostFree(b.p);
     b.p = nullptr;
     size_t want = bytes * 2 + 4096;
     if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) { b.p = nullptr; b.cap = 0; return HSDC_ERR_NOMEM; }
@@ -1820,11 +1843,11 @@ int hsdc_ctx_destroy(hsdc_ctx *c)
     for (hipEvent_t e : {c->zev0, c->zev1, c->pz0, c->pz1}) if (e) (void)hipEventDestroy(e);
     if (c->zstream) (void)hipStreamDestroy(c->zstream);
     if (c->rec) (void)hipHostFree(c->rec);
-    for (DevBuf *b : bufs) if (b->p) hipFree(b->p);
-    if (c->pin.p) hipHostFree(c->pin.p);
-    if (c->pin2.p) hipHostFree(c->pin2.p);
-    if (c->pin3.p) hipHostFree(c->pin3.p);
-    if (c->stage.p) hipHostFree(c->stage.p);
+    for (DevBuf *b : bufs) if (b->p) (void)hipFree(b->p);
+    if (c->pin.p) (void)hipHostFree(c->pin.p);
+    if (c->pin2.p) (void)hipHostFree(c->pin2.p);
+    if (c->pin3.p) (void)hipHostFree(c->pin3.p);
+    if (c->stage.p) (void)hipHostFree(c->stage.p);
     for (hipEvent_t e : c->stev) if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->pev) if (e) (void)hipEventDestroy(e);
     delete c;
@@ -2107,7 +2130,7 @@ int hsdc_autotune(hsdc_ctx *c, int nprob, const int *ns, const long *offs, doubl
         }
     (void)hipMemcpy(d_d, kd, sizeof(double) * tot, hipMemcpyDeviceToDevice);
     (void)hipMemcpy(d_e, ke, sizeof(double) * tot, hipMemcpyDeviceToDevice);
-    hipFree(keep.p);
+    (void)hipFree(keep.p);
     c->prm = p0;
     if (rc) return rc;
     c->prm.smlsiz = best.smlsiz;
@@ -2642,6 +2665,333 @@ int hsdc_cauchyhssmm(hsdc_ctx *c, const double *d, const double *f, const double
     (void)hipEventElapsedTime(&ms_mm, c->ev0, c->ev1);
     if (time) { time[0] = ms_build * 1e-3; time[1] = ms_mm * 1e-3; }
     return 0;
+}
+
+// HSS form of the dense n x n matrix dA (device, ld n) by truncated SVDs (mat2hssvd of
+// DHSSMM / DHSSMML, ConstructHssd.f90:371-696), depth by depth (the kernels of
+// hsdc_mat2hss.hip): the generators into c->hss (arena and leaf blocks) and their
+// directory into Q.  Q.ok stays false for a tree of one leaf or more than 512 leaves, a
+// Jacobi sweep limit or a full arena; the caller then multiplies densely.  ranks (may be
+// null; 2 per node): the ranks of each node's U and V.
+static int mat2hss_build(hsdc_ctx *c, const double *dA, int n, int ni, double tol, HssPlanM &Q, int *ranks,
+                         float *ms_build)
+{
+    hipStream_t s = c->stream;
+    Q.ok = false;
+    HssTree T;
+    if (!hss_make_tree(T, n, ni)) return 0;
+    const int ltr = T.ltr;
+    std::vector<int> usz(2 * ltr, 0), vsz(2 * ltr, 0), bsz(2 * ltr, 0), dsz(ltr, 0);
+    std::vector<long> pu(ltr, 0), pv(ltr, 0), pb(ltr, 0), pd(ltr, 0);
+    // leaf blocks and the arena (12 max(ni, 128) n doubles, as for the Cauchy forms)
+    long pdt = 0;
+    {
+        int it = 0;
+        for (int i = 1; i <= ltr; i++)
+            if (T.c1[i] == 0) { dsz[i - 1] = T.m[it]; pd[i - 1] = pdt; pdt += (long)T.m[it] * T.m[it]; it++; }
+    }
+    const long acap = (long)12 * std::max(ni, 128) * n;
+    CK(ensure(c->hss, sizeof(double) * (size_t)(pdt + acap)));
+    double *dd = (double *)c->hss.p, *arena = dd + pdt;
+    long atop = 0;
+    struct Act { int node, rs, rn, cs, cn; };
+    std::vector<Act> act;
+    {
+        int it = 0;
+        for (int i = 1; i <= ltr; i++)
+            if (T.c1[i] == 0) { act.push_back({i, T.leaf_r0[i], T.m[it], T.leaf_r0[i], T.m[it]}); it++; }
+    }
+    int nr = n, nc = n;
+    // C ping-pong: cb (current), misc (row-compressed and new); C starts as A
+    CK(ensure(c->cb, sizeof(double) * (size_t)n * n));
+    CK(ensure(c->misc, sizeof(double) * 2 * (size_t)n * n));
+    double *Cc = (double *)c->cb.p, *Tr = (double *)c->misc.p, *Cn = Tr + (size_t)n * n;
+    HK(hipMemcpyAsync(Cc, dA, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, s));
+    HK(hipEventRecord(c->ev0, s));
+    // leaf D blocks straight from A
+    {
+        std::vector<M2hCopy> cp;
+        for (int i = 1; i <= ltr; i++)
+            if (T.c1[i] == 0) cp.push_back({3, dsz[i - 1], dsz[i - 1], 0, T.leaf_r0[i], T.leaf_r0[i], pd[i - 1]});
+        CK(ensure(c->tabs, sizeof(M2hCopy) * cp.size()));
+        HK(hipMemcpy(c->tabs.p, cp.data(), sizeof(M2hCopy) * cp.size(), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_m2h_copy, dim3(16, (unsigned)cp.size()), dim3(WGS), 0, s, (const M2hCopy *)c->tabs.p,
+                           (const M2hJob *)nullptr, (const double *)nullptr, (const double *)nullptr, 0L, dA, (long)n,
+                           arena, dd);
+        HK(hipGetLastError());
+    }
+    bool fail = false;
+    for (int LL = T.lvl; LL >= 1 && !fail; LL--) {
+        const std::vector<int> &nds = T.lev[LL];
+        const int nn = (int)nds.size();
+        std::vector<int> a0(nn), a1(nn);
+        std::vector<M2hJob> jobs(2 * nn);
+        long so = 0, bo = 0;
+        for (int q = 0; q < nn; q++) {
+            const int nd = nds[q];
+            const int f = T.c1[nd] ? T.c1[nd] : nd, l = T.c1[nd] ? T.c2[nd] : nd;
+            a0[q] = a1[q] = -1;
+            for (int t = 0; t < (int)act.size(); t++) {
+                if (act[t].node == f) a0[q] = t;
+                if (act[t].node == l) a1[q] = t;
+            }
+            if (a0[q] < 0 || a1[q] < 0) return HSDC_ERR_INTERNAL;
+            const int rs = act[a0[q]].rs, rn = act[a1[q]].rs + act[a1[q]].rn - rs;
+            const int cs = act[a0[q]].cs, cn = act[a1[q]].cs + act[a1[q]].cn - cs;
+            for (int sd = 0; sd < 2; sd++) {
+                M2hJob &J = jobs[2 * q + sd];
+                J.rs = rs; J.rn = rn; J.cs = cs; J.cn = cn;
+                J.m = sd == 0 ? rn : cn;
+                J.w = sd == 0 ? nc - cn : nr - rn;
+                J.toff = so;
+                so += (long)J.w * J.m + 2L * J.m * J.m;
+                J.boff = bo;
+                bo += (long)J.m * J.m + 2L * J.m;
+            }
+        }
+        CK(ensure(c->xb, sizeof(double) * (size_t)std::max(so, 1L)));
+        CK(ensure(c->yb, sizeof(double) * (size_t)std::max(bo, 1L)));
+        double *S = (double *)c->xb.p, *Bs = (double *)c->yb.p;
+        const size_t jb = al64(sizeof(M2hJob) * jobs.size());
+        CK(ensure(c->tabs, jb + 64 + sizeof(int) * jobs.size()));
+        char *tb = (char *)c->tabs.p;
+        M2hJob *dj = (M2hJob *)tb;
+        int *derr = (int *)(tb + jb), *drank = derr + 16;
+        HK(hipMemcpyAsync(dj, jobs.data(), sizeof(M2hJob) * jobs.size(), hipMemcpyHostToDevice, s));
+        HK(hipMemsetAsync(derr, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_m2h_gather, dim3(64, (unsigned)jobs.size()), dim3(WGS), 0, s, (const M2hJob *)dj,
+                           (const double *)Cc, (long)nr, S);
+        hipLaunchKernelGGL(k_m2h_svd, dim3((unsigned)jobs.size()), dim3(WGS), 0, s, (const M2hJob *)dj, S, Bs, tol,
+                           drank, derr);
+        HK(hipGetLastError());
+        std::vector<int> rk(jobs.size());
+        int err = 0;
+        HK(hipMemcpyAsync(rk.data(), drank, sizeof(int) * rk.size(), hipMemcpyDeviceToHost, s));
+        HK(hipMemcpyAsync(&err, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+        HK(hipStreamSynchronize(s));
+        if (err) { fail = true; break; }
+        // generators of the level and the new active list
+        std::vector<M2hCopy> cp;
+        for (int q = 0; q < nn; q++) {
+            const int nd = nds[q], ru = rk[2 * q], rw = rk[2 * q + 1];
+            const M2hJob &Jr = jobs[2 * q], &Jc = jobs[2 * q + 1];
+            usz[2 * (nd - 1)] = Jr.m; usz[2 * (nd - 1) + 1] = ru; pu[nd - 1] = atop;
+            if ((long)Jr.m * ru > 0) cp.push_back({0, Jr.m, ru, 2 * q, 0, 0, atop});
+            atop += (long)Jr.m * ru;
+            vsz[2 * (nd - 1)] = rw; vsz[2 * (nd - 1) + 1] = Jc.m; pv[nd - 1] = atop;
+            if ((long)rw * Jc.m > 0) cp.push_back({1, rw, Jc.m, 2 * q + 1, 0, 0, atop});
+            atop += (long)rw * Jc.m;
+        }
+        std::vector<Act> nact;
+        std::vector<M2hBlk> rb, cbk;
+        int nnr = 0, nnc = 0;
+        for (int t = 0; t < (int)act.size();) {
+            int q = -1;
+            for (int z = 0; z < nn; z++) if (a0[z] == t) q = z;
+            Act a;
+            M2hBlk br, bc;
+            if (q >= 0) {
+                a = {nds[q], nnr, rk[2 * q], nnc, rk[2 * q + 1]};
+                br = {nnr, a.rn, q, jobs[2 * q].rs, jobs[2 * q].rn};
+                bc = {nnc, a.cn, q, jobs[2 * q].cs, jobs[2 * q].cn};
+                t = a1[q] + 1;
+            } else {
+                a = {act[t].node, nnr, act[t].rn, nnc, act[t].cn};
+                br = {nnr, a.rn, -1 - t, act[t].rs, act[t].rn};
+                bc = {nnc, a.cn, -1 - t, act[t].cs, act[t].cn};
+                t++;
+            }
+            nact.push_back(a);
+            if (a.rn > 0) rb.push_back(br);
+            if (a.cn > 0) cbk.push_back(bc);
+            nnr += a.rn;
+            nnc += a.cn;
+        }
+        // B of every sibling pair of this depth: the compressed block between them
+        for (int q = 0; q < nn; q++) {
+            const int nd = nds[q], par = T.tr[nd - 1];
+            const int sib = (T.c1[par] == nd) ? T.c2[par] : T.c1[par];
+            int ia = -1, ib = -1;
+            for (int t = 0; t < (int)nact.size(); t++) {
+                if (nact[t].node == nd) ia = t;
+                if (nact[t].node == sib) ib = t;
+            }
+            if (ia < 0 || ib < 0) return HSDC_ERR_INTERNAL;
+            const int br = nact[ia].rn, bcn = nact[ib].cn;
+            bsz[2 * (nd - 1)] = br; bsz[2 * (nd - 1) + 1] = bcn; pb[nd - 1] = atop;
+            if ((long)br * bcn > 0) cp.push_back({2, br, bcn, 0, nact[ia].rs, nact[ib].cs, atop});
+            atop += (long)br * bcn;
+        }
+        if (atop > acap) { fail = true; break; }
+        // C' = blockdiag(U)^T C blockdiag(W)
+        const size_t o1 = al64(sizeof(M2hJob) * jobs.size()), o2 = o1 + al64(sizeof(M2hBlk) * (rb.size() + 1)),
+                     o3 = o2 + al64(sizeof(M2hBlk) * (cbk.size() + 1)), o4 = o3 + sizeof(M2hCopy) * (cp.size() + 1);
+        CK(ensure(c->htab, o4));
+        char *hb = (char *)c->htab.p;
+        std::vector<char> up(o4, 0);
+        memcpy(up.data(), jobs.data(), sizeof(M2hJob) * jobs.size());
+        if (!rb.empty()) memcpy(up.data() + o1, rb.data(), sizeof(M2hBlk) * rb.size());
+        if (!cbk.empty()) memcpy(up.data() + o2, cbk.data(), sizeof(M2hBlk) * cbk.size());
+        if (!cp.empty()) memcpy(up.data() + o3, cp.data(), sizeof(M2hCopy) * cp.size());
+        HK(hipMemcpyAsync(hb, up.data(), o4, hipMemcpyHostToDevice, s));
+        const M2hJob *hj = (const M2hJob *)hb;
+        if (nnr > 0 && nc > 0 && !rb.empty())
+            hipLaunchKernelGGL(k_m2h_rows, dim3((unsigned)std::min<long>(cdiv((long)nnr * nc, WGS), 8192)), dim3(WGS), 0,
+                               s, (const M2hBlk *)(hb + o1), (int)rb.size(), hj, (const double *)Bs,
+                               (const double *)Cc, (long)nr, nc, Tr, (long)nnr);
+        if (nnr > 0 && nnc > 0 && !cbk.empty())
+            hipLaunchKernelGGL(k_m2h_cols, dim3((unsigned)std::min<long>(cdiv((long)nnr * nnc, WGS), 8192)), dim3(WGS),
+                               0, s, (const M2hBlk *)(hb + o2), (int)cbk.size(), hj, (const double *)Bs,
+                               (const double *)Tr, (long)nnr, nnc, Cn);
+        if (!cp.empty())
+            hipLaunchKernelGGL(k_m2h_copy, dim3(16, (unsigned)cp.size()), dim3(WGS), 0, s, (const M2hCopy *)(hb + o3),
+                               hj, (const double *)Bs, (const double *)Cn, (long)std::max(nnr, 1), dA, (long)n, arena,
+                               dd);
+        HK(hipGetLastError());
+        HK(hipStreamSynchronize(s));  // the host tables above are locals
+        std::swap(Cc, Cn);
+        nr = nnr;
+        nc = nnc;
+        act.swap(nact);
+    }
+    HK(hipEventRecord(c->ev1, s));
+    HK(hipEventSynchronize(c->ev1));
+    if (ms_build) (void)hipEventElapsedTime(ms_build, c->ev0, c->ev1);
+    if (fail) return 0;
+    Q.T = T;
+    Q.ok = true;
+    Q.usz = usz; Q.vsz = vsz; Q.bsz = bsz; Q.dsz = dsz;
+    Q.pu = pu; Q.pv = pv; Q.pb = pb; Q.pd = pd;
+    memset(&Q.H, 0, sizeof(Q.H));
+    Q.H.arena = arena;
+    Q.H.dd = dd;
+    Q.H.K = n;
+    if (ranks)
+        for (int i = 0; i < ltr; i++) {
+            ranks[2 * i] = usz[2 * i + 1];
+            ranks[2 * i + 1] = vsz[2 * i];
+        }
+    return 0;
+}
+
+// Checks shared by DHSSMM and DHSSMML (DHSSMM.f90:1-125 takes LDA = M = N and a square B).
+static int dhss_args(hsdc_ctx *c, const double *a, int lda, int m, int n, const double *b, int ldb, int k, double tol,
+                     int ni, int ncol, bool left)
+{
+    if (!c) return HSDC_ERR_INTERNAL;
+    if (!a) return -1;
+    if (lda < std::max(1, n)) return -2;
+    if (m != n) return -3;
+    if (n < 0) return -4;
+    if (!b) return -5;
+    if (ldb < std::max(1, left ? n : k)) return -6;
+    if (k < 0) return -7;
+    if (!(tol >= 0.0)) return -8;
+    if (ni < 1) return -10;
+    if (ncol < 1) return -11;
+    return 0;
+}
+
+// DHSSMM (DHSSMM.f90:1-125): b(1:n, 1:k) := A b with A (n x n) put in HSS form by
+// truncated SVDs (leaf ni, tolerance tol), the product H b taken as (b^T H^T)^T in column
+// blocks of at least 1024.  DHSSMML (DHSSMML.f90:2-122): b(1:k, 1:n) := b A, rows of b in
+// blocks.  A is not modified (the reference uses it as workspace); prank is accepted and
+// unused, as in mat2hssvd.
+static int dhss_run(hsdc_ctx *c, const double *a, int lda, int n, double *b, int ldb, int k, double tol, int ni,
+                    int ncol, double *time, int *ranks, bool left)
+{
+    if (time) { time[0] = 0.0; time[1] = 0.0; }
+    c->evc_k = 0;
+    if (n == 0 || k == 0) {
+        if (ranks) ranks[0] = -1;
+        return 0;
+    }
+    if (!have_device()) return HSDC_ERR_NODEVICE;
+    hipStream_t s = c->stream;
+    if (!c->ev0) {
+        HK(hipEventCreate(&c->ev0));
+        HK(hipEventCreate(&c->ev1));
+    }
+    const size_t nn = (size_t)n, kk = (size_t)k;
+    CK(ensure(c->zout, sizeof(double) * nn * nn));
+    double *dA = (double *)c->zout.p;
+    HK(hipMemcpy2DAsync(dA, sizeof(double) * nn, a, sizeof(double) * lda, sizeof(double) * nn, nn,
+                        hipMemcpyHostToDevice, s));
+    // b on the device: n x k (left) or k x n, leading dimension its row count
+    const size_t br = left ? nn : kk, bc = left ? kk : nn;
+    CK(ensure(c->q, sizeof(double) * br * bc));
+    double *dB = (double *)c->q.p;
+    HK(hipMemcpy2DAsync(dB, sizeof(double) * br, b, sizeof(double) * ldb, sizeof(double) * br, bc,
+                        hipMemcpyHostToDevice, s));
+    std::vector<HssPlanM> P(1);
+    float ms_build = 0.0f;
+    CK(mat2hss_build(c, dA, n, ni, tol, P[0], ranks, &ms_build));
+    if (!P[0].ok && ranks) ranks[0] = -1;
+    const int nb = (int)std::min<size_t>(kk, (size_t)std::max(ncol, 1024));
+    CK(ensure(c->tb, sizeof(double) * nn * nb));
+    CK(ensure(c->dout, sizeof(double) * nn * nb));
+    double *Xt = (double *)c->tb.p, *Yt = (double *)c->dout.p;
+    HK(hipEventRecord(c->ev0, s));
+    for (int c0 = 0; c0 < k; c0 += nb) {
+        const int w = std::min(nb, k - c0);
+        if (left) {  // columns c0 .. c0 + w of b (n x w): b_blk := A b_blk
+            double *Bb = dB + (size_t)c0 * nn;
+            if (P[0].ok) {
+                hipLaunchKernelGGL(k_transpose, dim3(cdiv(n, 64), cdiv(w, 64)), dim3(WGS), 0, s, n, w,
+                                   (const double *)Bb, (long)n, Xt, (long)w);
+                HssSpec sp;
+                sp.K = n; sp.X = Xt; sp.ldx = w; sp.Y = Yt; sp.ldy = w; sp.nrow = w;
+                std::vector<HssSpec> v1(1, sp);
+                CK(hss_mult(c, v1, P, true));
+                hipLaunchKernelGGL(k_transpose, dim3(cdiv(w, 64), cdiv(n, 64)), dim3(WGS), 0, s, w, n,
+                                   (const double *)Yt, (long)w, Bb, (long)n);
+            } else {
+                hipLaunchKernelGGL(k_dgemm_nn, dim3(cdiv(n, 64), cdiv(w, 64)), dim3(WGS), 0, s, n, w, n,
+                                   (const double *)dA, (long)n, (const double *)Bb, (long)n, Yt, (long)n);
+                HK(hipMemcpyAsync(Bb, Yt, sizeof(double) * nn * w, hipMemcpyDeviceToDevice, s));
+            }
+        } else {  // rows c0 .. c0 + w of b (w x n, in b's k x n block): b_blk := b_blk A
+            double *Bb = dB + c0;
+            if (P[0].ok) {
+                HssSpec sp;
+                sp.K = n; sp.X = Bb; sp.ldx = k; sp.Y = Yt; sp.ldy = w; sp.nrow = w;
+                std::vector<HssSpec> v1(1, sp);
+                CK(hss_mult(c, v1, P, false));
+            } else {
+                hipLaunchKernelGGL(k_dgemm_nn, dim3(cdiv(w, 64), cdiv(n, 64)), dim3(WGS), 0, s, w, n, n,
+                                   (const double *)Bb, (long)k, (const double *)dA, (long)n, Yt, (long)w);
+            }
+            HK(hipMemcpy2DAsync(Bb, sizeof(double) * kk, Yt, sizeof(double) * w, sizeof(double) * w, nn,
+                                hipMemcpyDeviceToDevice, s));
+        }
+        HK(hipGetLastError());
+    }
+    HK(hipEventRecord(c->ev1, s));
+    HK(hipMemcpy2DAsync(b, sizeof(double) * ldb, dB, sizeof(double) * br, sizeof(double) * br, bc,
+                        hipMemcpyDeviceToHost, s));
+    HK(hipStreamSynchronize(s));
+    float ms_mm = 0.0f;
+    (void)hipEventElapsedTime(&ms_mm, c->ev0, c->ev1);
+    if (time) { time[0] = ms_build * 1e-3; time[1] = ms_mm * 1e-3; }
+    return 0;
+}
+
+int hsdc_dhssmm(hsdc_ctx *c, const double *a, int lda, int m, int n, double *b, int ldb, int k, double tol, int prank,
+                int ni, int ncol, double *time, int *ranks)
+{
+    (void)prank;
+    const int rc = dhss_args(c, a, lda, m, n, b, ldb, k, tol, ni, ncol, true);
+    if (rc) return rc;
+    return dhss_run(c, a, lda, n, b, ldb, k, tol, ni, ncol, time, ranks, true);
+}
+
+int hsdc_dhssmml(hsdc_ctx *c, const double *a, int lda, int m, int n, double *b, int ldb, int k, double tol,
+                 int prank, int ni, int ncol, double *time, int *ranks)
+{
+    (void)prank;
+    const int rc = dhss_args(c, a, lda, m, n, b, ldb, k, tol, ni, ncol, false);
+    if (rc) return rc;
+    return dhss_run(c, a, lda, n, b, ldb, k, tol, ni, ncol, time, ranks, false);
 }
 
 int hsdc_hss_dense(hsdc_ctx *c, double *a, int lda)

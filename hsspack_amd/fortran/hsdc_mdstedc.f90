@@ -97,6 +97,24 @@ module hsdc_c
          real(c_double) :: ab(*), d(*), q(*), work(*)
          integer(c_int) :: iwork(*)
       end function
+      integer(c_int) function hsdc_dhssmm(ctx, a, lda, m, n, b, ldb, k, tol, prank, ni, ncol, time, ranks) &
+            bind(C, name='hsdc_dhssmm')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: lda, m, n, ldb, k, prank, ni, ncol
+         real(c_double), value :: tol
+         real(c_double) :: a(*), b(*), time(*)
+         type(c_ptr), value :: ranks
+      end function
+      integer(c_int) function hsdc_dhssmml(ctx, a, lda, m, n, b, ldb, k, tol, prank, ni, ncol, time, ranks) &
+            bind(C, name='hsdc_dhssmml')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: lda, m, n, ldb, k, prank, ni, ncol
+         real(c_double), value :: tol
+         real(c_double) :: a(*), b(*), time(*)
+         type(c_ptr), value :: ranks
+      end function
       type(c_ptr) function hsdc_bd_thread_ctx() bind(C, name='hsdc_bd_thread_ctx')
          import :: c_ptr
       end function
@@ -577,4 +595,41 @@ subroutine mdsbevd(compz, uplo, n, kd, ab, ldab, d, q, ldq, work, lwork, iwork, 
    rc = hsdc_sbevd(ctx, compz, uplo, n, kd, ab, ldab, d, q, ldq, work, lwork, iwork, liwork)
    info = hsdc_info(rc)
    if (info < 0) call xerbla('MDSBEVD', -info)
+end subroutine
+
+
+! DHSSMM (DHSSMM.f90:1-125): B(1:N, 1:K) := A * B with the dense A (LDA x N, M = N) put
+! in HSS form by truncated SVDs (leaf size NI, tolerance TOL), NCOL columns of B at a time
+! or more; TIME(1:2) construction and multiplication seconds.  PRANK is unused (as in the
+! reference's mat2hssvd); A is left unchanged.  INFO: 0, -i for an illegal argument i, or
+! 1000 + code for a runtime failure (the reference never sets it).
+subroutine dhssmm(a, lda, m, n, b, ldb, k, tol, prank, ni, ncol, time, info)
+   use hsdc_c
+   implicit none
+   integer :: lda, m, n, ldb, k, prank, ni, ncol, info
+   double precision :: tol
+   double precision :: a(lda, *), b(ldb, *), time(2)
+   type(c_ptr) :: ctx
+   ctx = hsdc_thread_ctx()
+   if (.not. c_associated(ctx)) then
+      info = 1003
+      return
+   end if
+   info = hsdc_info(hsdc_dhssmm(ctx, a, lda, m, n, b, ldb, k, tol, prank, ni, ncol, time, c_null_ptr))
+end subroutine
+
+! DHSSMML (DHSSMML.f90:2-122): B(1:K, 1:N) := B * A, otherwise as DHSSMM.
+subroutine dhssmml(a, lda, m, n, b, ldb, k, tol, prank, ni, ncol, time, info)
+   use hsdc_c
+   implicit none
+   integer :: lda, m, n, ldb, k, prank, ni, ncol, info
+   double precision :: tol
+   double precision :: a(lda, *), b(ldb, *), time(2)
+   type(c_ptr) :: ctx
+   ctx = hsdc_thread_ctx()
+   if (.not. c_associated(ctx)) then
+      info = 1003
+      return
+   end if
+   info = hsdc_info(hsdc_dhssmml(ctx, a, lda, m, n, b, ldb, k, tol, prank, ni, ncol, time, c_null_ptr))
 end subroutine

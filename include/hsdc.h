@@ -217,6 +217,21 @@ int hsdc_hss_cmm(hsdc_ctx *ctx, char side, int squared, int k, int ni, double to
 int hsdc_cauchyhssmm(hsdc_ctx *ctx, const double *d, const double *f, const double *u, const double *v,
                      int n, int ni, double tol, int ncol, double *b, int ldb, double *time, int *ranks);
 
+/* DHSSMM (DHSSMM.f90:1-125): b(1:n, 1:k) := A b for the dense a (n x n, lda; m must
+ * equal n, as the reference assumes), A put in HSS form by truncated SVDs (mat2hssvd,
+ * ConstructHssd.f90:371-696: npart's tree of leaf size ni, every basis the singular
+ * vectors whose singular values exceed tol), then multiplied ncol columns of b at a
+ * time or more.  a is not modified (the reference uses it as workspace); prank is
+ * accepted and unused, as in mat2hssvd.  time[0..1] (may be NULL): construction and
+ * multiplication seconds; ranks (may be NULL; 2 per tree node): the ranks of each node's
+ * U and V, or ranks[0] = -1 when A was applied densely.  INFO: -i for an illegal
+ * argument i (-1 a, -2 lda, -3 m != n, -4 n, -5 b, -6 ldb, -7 k, -8 tol, -10 ni, -11 ncol). */
+int hsdc_dhssmm(hsdc_ctx *ctx, const double *a, int lda, int m, int n, double *b, int ldb, int k, double tol,
+                int prank, int ni, int ncol, double *time, int *ranks);
+/* DHSSMML (DHSSMML.f90:2-122): b(1:k, 1:n) := b A with the same HSS form of A; ldb >= k. */
+int hsdc_dhssmml(hsdc_ctx *ctx, const double *a, int lda, int m, int n, double *b, int ldb, int k, double tol,
+                 int prank, int ni, int ncol, double *time, int *ranks);
+
 /* Contexts owned by the calling host thread, for callers that keep no handle (the
  * Fortran drop-ins): created on first use with the default parameters (reset to them
  * at every hsdc_thread_ctx / hsdc_bd_thread_ctx call) and kept, with their device

@@ -9,6 +9,8 @@
 !   cauchy IN: int32 n, ni, ncol, float64 tol, d(n), f(n), u(n), v(n), b(n,n)
 !          OUT: b(n,n) after CAUCHYHSSMM, then b(n,n) after a second call on the input b
 !   sbevd  IN: int32 n, kd, uplo (int32), float64 ab(kd+1,n)  OUT: int32 info, d(n), q(n,n)
+!   dhssmm IN: int32 n, ni, ncol, float64 tol, a(n,n), b(n,n)
+!          OUT: int32 info, b(n,n) after DHSSMM; int32 info, b(n,n) after DHSSMML on the input b
 !   errors OUT: for each illegal call, info and the XERBLA (name, argument) it made
 module xerbla_log
    implicit none
@@ -117,6 +119,18 @@ program fdrive
       close(u)
       call mdsbevd('I', uplo, n, kd, ab, kd + 1, d, q, n, work, 1, iwork, 1, info)
       write(v) info, d, q
+   case ('dhssmm')
+      open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+      read(u) n, cni, cncol, ctol
+      allocate(q(n, n), b0(n, n), qs(n, n))
+      read(u) q, b0
+      close(u)
+      qs = b0
+      call dhssmm(q, n, n, n, qs, n, n, ctol, 45, cni, cncol, ctime, info)
+      write(v) info, qs
+      qs = b0
+      call dhssmml(q, n, n, n, qs, n, n, ctol, 45, cni, cncol, ctime, info)
+      write(v) info, qs
    case ('errors')
       allocate(d(8), e(8), q(8, 8), work(200), iwork(200), indxq(8), qs(8, 8))
       d = 1

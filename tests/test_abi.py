@@ -157,3 +157,25 @@ def test_hss_cmm_checks_arguments_before_the_quick_return():
     assert L.hsdc_hss_cmm(h, b"R", 0, 8, 16, 1e-12, *[dp(a)] * 6, dp(x), 8, 0, None) == 0
     assert L.hsdc_hss_cmm(h, b"R", 0, 8, 16, 1e-12, *[dp(a)] * 6, dp(x), 8, -1, None) == -15
     ctx.close()
+
+
+def test_dhssmm_argument_errors():
+    """DHSSMM / DHSSMML argument checks run before any device work."""
+    ctx = hsdc.Context()
+    L = hsdc.lib()
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    a = np.zeros((10, 10), order="F")
+    b = np.zeros((10, 10), order="F")
+    h = ctx._h
+    for fn in (L.hsdc_dhssmm, L.hsdc_dhssmml):
+        assert fn(h, None, 10, 10, 10, dp(b), 10, 10, 1e-11, 0, 4, 2, None, None) == -1
+        assert fn(h, dp(a), 9, 10, 10, dp(b), 10, 10, 1e-11, 0, 4, 2, None, None) == -2
+        assert fn(h, dp(a), 10, 9, 10, dp(b), 10, 10, 1e-11, 0, 4, 2, None, None) == -3
+        assert fn(h, dp(a), 10, -1, -1, dp(b), 10, 10, 1e-11, 0, 4, 2, None, None) == -4
+        assert fn(h, dp(a), 10, 10, 10, None, 10, 10, 1e-11, 0, 4, 2, None, None) == -5
+        assert fn(h, dp(a), 10, 10, 10, dp(b), 9, 10, 1e-11, 0, 4, 2, None, None) == -6
+        assert fn(h, dp(a), 10, 10, 10, dp(b), 10, -1, 1e-11, 0, 4, 2, None, None) == -7
+        assert fn(h, dp(a), 10, 10, 10, dp(b), 10, 10, -1.0, 0, 4, 2, None, None) == -8
+        assert fn(h, dp(a), 10, 10, 10, dp(b), 10, 10, 1e-11, 0, 0, 2, None, None) == -10
+        assert fn(h, dp(a), 10, 10, 10, dp(b), 10, 10, 1e-11, 0, 4, 0, None, None) == -11
+    ctx.close()

@@ -35,7 +35,7 @@ def f64(a):
 def test_library_exports_the_fortran_names():
     out = subprocess.run(["nm", "-D", "--defined-only", FLIB], capture_output=True, text=True, check=True).stdout
     for name in ("mdstedc_", "mdlaed0_", "mdlaed1_", "mdlaed2_", "mdlaed3_", "mdbdsdc_", "cauchyhssmm_",
-                 "mdsbevd_"):
+                 "mdsbevd_", "dhssmm_", "dhssmml_"):
         assert (" T " + name) in out, name
 
 
@@ -232,3 +232,28 @@ def test_mdsbevd_from_fortran(tmp_path, uplo):
     assert np.array_equal(lam, w2) and np.array_equal(q, q2)
     w = sl.eig_banded(ab, lower=False, eigvals_only=True)
     assert np.abs(lam - w).max() <= 4 * n * 2.0 ** -53 * np.abs(w).max()
+
+
+@pytest.mark.gpu
+def test_dhssmm_and_dhssmml_from_fortran(tmp_path):
+    """DHSSMM(A, LDA, M, N, B, LDB, K, TOL, PRANK, NI, NCOL, TIME, INFO) and DHSSMML as
+    testfmm.f90 / testfmml.f90 call them (Cauchy matrix on interlaced points, PRANK = 45,
+    NI = 64, NCOL = 100, TOL = 1e-11): the same products as the C entries."""
+    from hsspack_amd import hsdc
+    from test_oracle_dhssmm import fmm_matrix
+    n, ni, ncol, tol = 1200, 64, 100, 1e-11
+    A = fmm_matrix(n, 8)
+    b = np.random.default_rng(9).uniform(0, 1, (n, n))
+    raw = run("dhssmm", tmp_path, i32(n, ni, ncol) + f64([tol]) + f64(A.T) + f64(b.T))
+    rec = 4 + 8 * n * n
+    info1 = np.frombuffer(raw[:4], "<i4")[0]
+    y1 = np.frombuffer(raw[4:rec], "<f8").reshape(n, n).T
+    info2 = np.frombuffer(raw[rec:rec + 4], "<i4")[0]
+    y2 = np.frombuffer(raw[rec + 4:], "<f8").reshape(n, n).T
+    assert info1 == 0 and info2 == 0
+    ctx = hsdc.Context()
+    c1, _, _ = ctx.dhssmm(A, b, ni, tol, ncol=ncol)
+    c2, _, _ = ctx.dhssmm(A, b, ni, tol, ncol=ncol, right=True)
+    ctx.close()
+    assert np.array_equal(y1, c1) and np.array_equal(y2, c2)
+    assert np.abs(y1 - A @ b).max() <= 1e-8 and np.abs(y2 - b @ A).max() <= 1e-8

@@ -94,6 +94,7 @@ __device__ __forceinline__ int seg_of(const int *start, int nm, int b)
 // ascending each element's place comes from a binary search in the other list; when
 // either is not (rounding can leave a deflated tail out of order), thread 0 runs the
 // sequential merge, which is what DLAMRG does in every case.
+template <int NT>
 __device__ void blk_lamrg(int n1, int n2, const double *a, int s1, int s2, int *idx, int *flag)
 {
     const int tid = threadIdx.x;
@@ -102,12 +103,12 @@ __device__ void blk_lamrg(int n1, int n2, const double *a, int s1, int s2, int *
     if (tid == 0) *flag = 0;
     __syncthreads();
     int bad = 0;
-    for (int i = tid; i + 1 < n1; i += BWG) bad |= !(a[p1(i)] <= a[p1(i + 1)]);
-    for (int j = tid; j + 1 < n2; j += BWG) bad |= !(a[p2(j)] <= a[p2(j + 1)]);
+    for (int i = tid; i + 1 < n1; i += NT) bad |= !(a[p1(i)] <= a[p1(i + 1)]);
+    for (int j = tid; j + 1 < n2; j += NT) bad |= !(a[p2(j)] <= a[p2(j + 1)]);
     if (bad) atomicOr(flag, 1);
     __syncthreads();
     if (*flag == 0) {
-        for (int i = tid; i < n1; i += BWG) {
+        for (int i = tid; i < n1; i += NT) {
             const double v = a[p1(i)];
             int lo = 0, hi = n2;  // list-2 entries < v
             while (lo < hi) {
@@ -116,7 +117,7 @@ __device__ void blk_lamrg(int n1, int n2, const double *a, int s1, int s2, int *
             }
             idx[i + lo] = p1(i) + 1;
         }
-        for (int j = tid; j < n2; j += BWG) {
+        for (int j = tid; j < n2; j += NT) {
             const double v = a[p2(j)];
             int lo = 0, hi = n1;  // list-1 entries <= v
             while (lo < hi) {
@@ -416,12 +417,13 @@ struct SbDefl {
     Merge *mg2;  // the secular kernels' view of the merges (rho written here)
 };
 
-__global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sb_deflate(const SbMerge *mg, SbDefl A)
 {
     __shared__ double zc[DCH], dc[DCH];
     __shared__ int nc[DCH];
-    __shared__ double red[2][BWG / 64];
-    __shared__ int flag, s_k, s_nrot, s_cnt, s_rot, pre[BWG];
+    __shared__ double red[2][NT / 64];
+    __shared__ int flag, s_k, s_nrot, s_cnt, s_rot, pre[NT];
     const SbMerge M = mg[blockIdx.x];
     const int s = M.s, n = M.n, n1 = M.n1, tid = threadIdx.x, up = A.up;
     const double rho0 = A.cuts[(long)M.cut * (2L * A.kd * A.kd + A.kd) + 2 * A.kd * A.kd + up];
@@ -429,7 +431,7 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
     double *d = A.D + s;
     int *iq = A.indxq + s, *ix = A.indx + s, *ip = A.indxp + s;
     double az = 0.0, ad = 0.0;
-    for (int j = tid; j < n; j += BWG) {
+    for (int j = tid; j < n; j += NT) {
         az = fmax(az, fabs(zcol[j] / sqrt(2.0)));
         ad = fmax(ad, fabs(d[j]));
     }
@@ -439,31 +441,31 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
     }
     if ((tid & 63) == 0) { red[0][tid >> 6] = az; red[1][tid >> 6] = ad; }
     if (up == 0)
-        for (int j = n1 + tid; j < n; j += BWG) iq[j] += n1;
+        for (int j = n1 + tid; j < n; j += NT) iq[j] += n1;
     __syncthreads();
     az = 0.0; ad = 0.0;
-    for (int q = 0; q < BWG / 64; q++) { az = fmax(az, red[0][q]); ad = fmax(ad, red[1][q]); }
+    for (int q = 0; q < NT / 64; q++) { az = fmax(az, red[0][q]); ad = fmax(ad, red[1][q]); }
     const double rho = fabs(2.0 * rho0);
     const double tol = 8.0 * HSDC_EPS * fmax(ad, az);
     // the sorted order
     if (up == 0) {
         double *dl = A.srtd + s;
-        for (int r = tid; r < n; r += BWG) dl[r] = d[iq[r] - 1];
+        for (int r = tid; r < n; r += NT) dl[r] = d[iq[r] - 1];
         __syncthreads();
-        blk_lamrg(n1, n - n1, dl, 1, 1, ip, &flag);  // INDXC, held in INDXP for now
-        for (int r = tid; r < n; r += BWG) ix[r] = iq[ip[r] - 1];
+        blk_lamrg<NT>(n1, n - n1, dl, 1, 1, ip, &flag);  // INDXC, held in INDXP for now
+        for (int r = tid; r < n; r += NT) ix[r] = iq[ip[r] - 1];
     } else {
-        for (int r = tid; r < n; r += BWG) ix[r] = iq[r];
+        for (int r = tid; r < n; r += NT) ix[r] = iq[r];
     }
     __syncthreads();
-    for (int r = tid; r < n; r += BWG) {
+    for (int r = tid; r < n; r += NT) {
         const int p = ix[r] - 1;
         A.srtz[s + r] = zcol[p] / sqrt(2.0);
         A.srtd[s + r] = d[p];
     }
     __syncthreads();
     if (rho * az <= tol) {  // all deflated (MDSBED2's early exit): INDXP = INDX, K = 0
-        for (int r = tid; r < n; r += BWG) ip[r] = ix[r];
+        for (int r = tid; r < n; r += NT) ip[r] = ix[r];
         if (tid == 0) { s_k = 0; s_nrot = 0; }
         __syncthreads();
     } else {
@@ -473,7 +475,7 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
         // tests, so when none passes there is no rotation at all and the scan's result is
         // the survivors kept in order, the others deflated from the back in order.
         int *pos = A.rpj + s, *L = A.rnj + s;  // scratch until a rotation is logged
-        const int seg = (n + BWG - 1) / BWG, r0 = min(n, tid * seg), r1 = min(n, r0 + seg);
+        const int seg = (n + NT - 1) / NT, r0 = min(n, tid * seg), r1 = min(n, r0 + seg);
         int cnt = 0;
         for (int r = r0; r < r1; r++) cnt += !(rho * fabs(A.srtz[s + r]) <= tol);
         pre[tid] = cnt;
@@ -481,7 +483,7 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
         __syncthreads();
         if (tid == 0) {
             int acc = 0;
-            for (int t = 0; t < BWG; t++) { const int c = pre[t]; pre[t] = acc; acc += c; }
+            for (int t = 0; t < NT; t++) { const int c = pre[t]; pre[t] = acc; acc += c; }
             s_cnt = acc;
         }
         __syncthreads();
@@ -494,7 +496,7 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
         __syncthreads();
         const int ns = s_cnt;
         int any = 0;
-        for (int j = 1 + tid; j < ns; j += BWG) {
+        for (int j = 1 + tid; j < ns; j += NT) {
             const int rp = L[j - 1], rn = L[j];
             const double zp = A.srtz[s + rp], zn = A.srtz[s + rn];
             const double tau = hypot(zn, zp), t0 = A.srtd[s + rn] - A.srtd[s + rp];
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
         if (any) atomicOr(&s_rot, 1);
         __syncthreads();
         if (!s_rot) {
-            for (int r = tid; r < n; r += BWG) {
+            for (int r = tid; r < n; r += NT) {
                 const int qq = pos[r];
                 if (qq >= 0) {
                     A.dlamda[s + qq] = A.srtd[s + r];
@@ -523,7 +525,7 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
         double zp = 0.0, dp = 0.0;
         for (int c0 = 0; c0 < n; c0 += DCH) {
             const int cnt = min(DCH, n - c0);
-            for (int t = tid; t < cnt; t += BWG) {
+            for (int t = tid; t < cnt; t += NT) {
                 zc[t] = A.srtz[s + c0 + t];
                 dc[t] = A.srtd[s + c0 + t];
                 nc[t] = ix[c0 + t] - 1;
@@ -580,8 +582,10 @@ __global__ __launch_bounds__(BWG) void k_sb_deflate(const SbMerge *mg, SbDefl A)
         A.mo[blockIdx.x] = o;
         A.mg2[blockIdx.x].rho = rho;
     }
-    for (int j = k + tid; j < n; j += BWG) A.Dn[s + j] = d[ip[j] - 1];
+    for (int j = k + tid; j < n; j += NT) A.Dn[s + j] = d[ip[j] - 1];
 }
+
+template __global__ void k_sb_deflate<1024>(const SbMerge *, SbDefl);
 
 // Launch tables of the update from each merge's K (one thread): secular blocks, 256-wide
 // chunks, and the two GEMM items of every merge.  Merges with hss[m] set take MDLAED3's
@@ -731,7 +735,7 @@ __global__ __launch_bounds__(BWG) void k_sb_post(const SbMerge *mg, const SbUpd 
         for (int j = threadIdx.x; j < n; j += BWG) out[j] = j + 1;
         return;
     }
-    blk_lamrg(k, n - k, a, 1, -1, out, &flag);
+    blk_lamrg<BWG>(k, n - k, a, 1, -1, out, &flag);
 }
 
 // Eigenvalues in INDXQ order and the matching columns of Q into the output (one
@@ -1095,7 +1099,7 @@ int sb_solve(hsdc_sb_ctx *c, int lower, int n, int kd, const double *dab, long l
             A.dlamda = W.dlamda; A.w = W.w; A.srtd = srtd; A.srtz = srtz;
             A.rpj = rpj; A.rnj = rnj; A.rcs = rcs; A.rsn = rsn;
             A.cuts = cuts; A.kd = kd; A.up = up; A.upd = dupd; A.mo = dmo; A.mg2 = dm2;
-            hipLaunchKernelGGL(k_sb_deflate, dim3(nm), dim3(BWG), 0, st, dm, A);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_sb_deflate<1024>), dim3(nm), dim3(1024), 0, st, dm, A);
             std::vector<int> hsel;  // merges of this update that take the HSS branch
             if (hss_lvl) {
                 SCK(sb_pin(c, sizeof(SbUpd) * nm + sizeof(int) * (4 * nm + 2) + sizeof(long) * nm + 64));

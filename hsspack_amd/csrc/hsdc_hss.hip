@@ -294,7 +294,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 }
             }
         }
-        __syncthreads();
+        // every swap of the search is followed by a barrier and the search only reads, so
+        // the update pass below may start without another
         double ukk = A1(Z, k) * A1(V, k) / fdq<SQ>(H, A1(PL, k), A1(PU, k));
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
@@ -363,16 +364,19 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             const double t2 = -1.0 * fdq<SQ>(H, plk, A1(PU, j));
             A1(T1, j) = t2 / ppq<SQ>(A1(F, j), Fk);
         }
-        __syncthreads();
+        fl += 5ull * (M - k) + 7ull * (N - k) + 12ull * (k - 1);
+        pre = fuse;
+        // The argmax's barrier also publishes the update pass (T1 for thread 0's product),
+        // and the next step reads W(k) only after its own first barrier, so a fused step
+        // needs no other; before a swap check, and at the last step, W(k) is published here.
+        if (fuse) pre_rw = block_argmax_t(y1, red, tog);
+        else __syncthreads();
         if (tid == 0) {
             double wk = fdq<SQ>(H, plk, puk) / A1(A, k);
             for (int j = 1; j <= k - 1; j++) wk = wk * A1(T1, j);
             A1(W, k) = wk;
         }
-        __syncthreads();
-        fl += 5ull * (M - k) + 7ull * (N - k) + 12ull * (k - 1);
-        pre = fuse;
-        if (fuse) pre_rw = block_argmax_t(y1, red, tog);
+        if (!fuse) __syncthreads();
         // ---- every 10 steps: swap while max |Mat_N| > rho (searchMax2Eig)
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
@@ -435,7 +439,6 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 __syncthreads();
             }
         }
-        __syncthreads();
     }
     // ---- U generator (CauchylikeEigQ, then rows back to the node order)
     if (tid == 0) {
@@ -622,7 +625,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 }
             }
         }
-        __syncthreads();
+        // every swap of the search is followed by a barrier and the search only reads, so
+        // the update pass below may start without another
         double ukk = A1(Z, k) * A1(V, k) / (-1.0 * fdq<SQ>(H, A1(PU, k), A1(PL, k)));
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
@@ -691,16 +695,17 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             temp2 = ddq<SQ>(H, A1(PL, j), plk);
             A1(T1, j) = temp / temp2;
         }
-        __syncthreads();
+        fl += 6ull * (M - k) + 5ull * (N - k) + 14ull * (k - 1);
+        pre = fuse;
+        // as in the row compression: the argmax's barrier publishes the update pass
+        if (fuse) pre_rw = block_argmax_t(y1, red, tog);
+        else __syncthreads();
         if (tid == 0) {
             double wk = (-1.0 * fdq<SQ>(H, puk, plk)) / A1(A, k);
             for (int j = 1; j <= k - 1; j++) wk = wk * A1(T1, j);
             A1(W, k) = wk;
         }
-        __syncthreads();
-        fl += 6ull * (M - k) + 5ull * (N - k) + 14ull * (k - 1);
-        pre = fuse;
-        if (fuse) pre_rw = block_argmax_t(y1, red, tog);
+        if (!fuse) __syncthreads();
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
                 VI x = {0.0, 0};
@@ -772,7 +777,6 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 __syncthreads();
             }
         }
-        __syncthreads();
     }
     // ---- V generator (CauchylikeEigQ2, columns back to the node order)
     if (tid == 0) {

@@ -92,6 +92,8 @@ __global__ void k_hss_colcomp(const HssMerge *, const HssNode *, int, double, in
 __global__ void k_hss_compact(const HssMerge *, const HssCompact *, const int *, int);
 __global__ void k_hss_bgen(const HssMerge *, const HssNode *, int);
 __global__ void k_bgemm(const BG *);
+template <int BM, int BN, int BK, int DB, int TA, int TB>
+__global__ void k_bgemm_w(const BG *, int);
 __global__ void k_gatherx(const Merge *, const int *, const int *, int, const Prob *, Work, int, double *,
                           const long *);
 __global__ void k_cbuild(const HssMerge *, int, double *);
@@ -442,6 +444,43 @@ struct HssPlanM {
     std::vector<int> lg, lf;
 };
 
+// One step of batched GEMM items (every item nrow <= rmax rows, n <= nmax columns): the
+// tile is chosen by the widest item, since a column block that most items leave empty
+// costs a pass over A for nothing.  Measured at n = 32768 (256 leaves of 128 or 254 of
+// 129, 8.6 GB of X per step):
+//   * n <= 64 (leaf upsweeps and siblings, ranks <= 64): 128 x 64 with eight 16 x 64
+//     wavefronts, 4.3 / 2.9 ms against k_bgemm's 5.7 / 3.7;
+//   * n <= 128: 128 x 128 with eight 32 x 64 wavefronts, 6.9 ms (upsweep, ranks 60-91)
+//     and 9.1 ms (leaf products) against 9.3 / 13.9 for k_bgemm's two 64-column blocks;
+//   * n <= 144 (leaves of 129-144 columns): 256 x 144 with eight 32 x 144 wavefronts,
+//     11.2 ms against 18.0 for two 128-column blocks;
+//   * wider: 128 x 128 tiles, the column block fastest when the last one is at least
+//     half full (a row panel of A read once), else slowest -- with the column block
+//     fastest, blocks that exit at once would take every other workgroup id, and ids go
+//     round-robin to the 8 XCDs, so half of them would get no work.
+// Every tile runs the k-steps in the same order, so all give the same bits.
+static void bgemm_step(const BG *items, unsigned ni, int rmax, int nmax, hipStream_t s)
+{
+    if (nmax <= 64) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<128, 64, 16, 1, 1, 4>), dim3(cdiv(rmax, 128), ni, 1), dim3(512), 0,
+                           s, items, 1);
+    } else if (nmax <= 128) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<128, 128, 16, 1, 2, 4>), dim3(cdiv(rmax, 128), ni, 1), dim3(512),
+                           0, s, items, 1);
+    } else if (nmax <= 144) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<256, 144, 16, 1, 2, 9>), dim3(cdiv(rmax, 256), ni, 1), dim3(512),
+                           0, s, items, 1);
+    } else {
+        const int nz = cdiv(nmax, 128);
+        if (nmax - 128 * (nz - 1) >= 64)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<128, 128, 16, 1, 2, 4>),
+                               dim3((unsigned)(cdiv(rmax, 128) * nz), ni, 1), dim3(512), 0, s, items, nz);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<128, 128, 16, 1, 2, 4>), dim3(cdiv(rmax, 128), ni, nz),
+                               dim3(512), 0, s, items, 0);
+    }
+}
+
 // The product of a batch of HSS matrices whose generators are on the device (P: tree,
 // generator directory, arena and leaf blocks of each; a plan that is not ok takes the
 // dense Cauchy product of its HssMerge): Y := X * H, or for left H X as (X^T H^T)^T on
@@ -695,8 +734,7 @@ static int hss_mult(hsdc_ctx *c, const std::vector<HssSpec> &sp, std::vector<Hss
     for (auto &v : steps) {
         int rmax = 0, nmax = 0;
         for (auto &g : v) { rmax = std::max(rmax, g.nrow); nmax = std::max(nmax, g.n); }
-        hipLaunchKernelGGL(k_bgemm, dim3(cdiv(rmax, 128), (unsigned)v.size(), cdiv(nmax, 64)), dim3(WGS), 0, s,
-                           (const BG *)c->tabs.p + so);
+        bgemm_step((const BG *)c->tabs.p + so, (unsigned)v.size(), rmax, nmax, s);
         so += v.size();
     }
     HK(hipGetLastError());

@@ -45,7 +45,7 @@ __global__ void k_alpha(const Merge *, const int *, int, const MergeOut *, Work)
 __global__ void k_sbuild(const Merge *, const int *, int, const MergeOut *, const long *, Work, double *);
 __global__ void k_bgemm(const BG *);
 template <int BM, int BN, int BK, int DB, int TA, int TB>
-__global__ void k_bgemm_w(const BG *);
+__global__ void k_bgemm_w(const BG *, int);
 __global__ void k_fill_u64(unsigned long long *, long, unsigned long long);
 
 namespace {
@@ -1133,9 +1133,9 @@ int sb_solve(hsdc_sb_ctx *c, int lower, int n, int kd, const double *dab, long l
             hipLaunchKernelGGL(k_sbuild, dim3(gch), dim3(BWG), 0, st, (const Merge *)dm2, (const int *)T.chunk, nm,
                                (const MergeOut *)dmo, so, W, (double *)c->s.p);
             if (L.nmax >= 4096)  // wide tiles where they fill the chip (same bits as k_bgemm)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<256, 128, 16, 1, 4, 4>),
-                                   dim3((unsigned)cdivl(L.nmax, 256), nm, (unsigned)cdivl(L.nmax, 128)), dim3(512), 0,
-                                   st, (const BG *)T.bq);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<256, 128, 16, 1, 2, 4>),
+                                   dim3((unsigned)cdivl(L.nmax, 256), nm, (unsigned)cdivl(L.nmax, 128)), dim3(1024), 0,
+                                   st, (const BG *)T.bq, 0);
             else
                 hipLaunchKernelGGL(k_bgemm, dim3((unsigned)cdivl(L.nmax, 128), nm, (unsigned)cdivl(L.nmax, 64)),
                                    dim3(BWG), 0, st, (const BG *)T.bq);

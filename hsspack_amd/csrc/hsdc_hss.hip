@@ -1042,29 +1042,35 @@ __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
 // The same batched GEMM with wide tiles for large items: block = BM x BN outputs,
 // wavefronts of (16 TA) x (16 TB) each (TA x TB accumulators of 16 x 16), k in chunks of
 // BK staged through LDS (DB: two slabs, one barrier per chunk) with the next chunk in
-// registers meanwhile.  At 256 x 128 with 64 x 64 per wavefront (eight wavefronts, two
-// per SIMD) each wavefront reads 4 A and 4 B operands from LDS per 16 MFMAs (k_bgemm: 6
-// per 8), and a block loads BK (BM + BN) doubles per 2 BM BN BK flops -- 21 flops per
-// byte against 11 at 128 x 64, which keeps the slabs' L2 traffic well under its
-// bandwidth at the FP64 MFMA rate.  Measured on square products of order 4096-16384:
-// 45 TF/s against k_bgemm's 35; below about 3000 rows and columns it has too few blocks
-// to fill the 256 CUs and k_bgemm is faster.  The k-steps of 4 run in the same order as
-// in k_bgemm, so both kernels give the same bits.
+// registers meanwhile.  nz > 0 numbers the blocks with the column block fastest
+// (blockIdx.x = row block * nz + column block, blockIdx.z = 0), so the blocks of one row
+// panel of A run together and the panel comes from HBM once; nz = 0 takes the row block
+// from blockIdx.x and the column block from blockIdx.z.  Measured (FP64 MFMA peak 78.6
+// TF/s):
+//   * square products of order 8192: 256 x 128 with sixteen 32 x 64 wavefronts 52.7 TF/s,
+//     eight 64 x 64 wavefronts 45.7, k_bgemm 34.8; below about 3000 rows and columns
+//     there are too few blocks to fill the 256 CUs and k_bgemm is faster;
+//   * the HSS multiply's steps at n = 32768: the tile follows the step's widest item
+//     (128 x 64, 128 x 128 or 256 x 144), 24-38 % faster than k_bgemm.
+// The k-steps of 4 run in the same order as in k_bgemm, so all give the same bits.
 // ---------------------------------------------------------------------------
 template <int BM, int BN, int BK, int DB, int TA, int TB>
-__global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bgemm_w(const BG *items)
+__global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bgemm_w(const BG *items, int nz)
 {
     constexpr int WM = 16 * TA, WN = 16 * TB;  // rows and columns of one wavefront's tile
     constexpr int NT = (BM / WM) * (BN / WN) * 64;
     constexpr int AP = BM + 4, BP = BN + 4;
-    constexpr int QA = BM * BK / NT, QB = BN * BK / NT;
-    static_assert(QA * NT == BM * BK && QB * NT == BN * BK, "staging split");
+    // staged elements per thread; a slab that does not split evenly over the threads
+    // (BN = 144) leaves the last round partly idle
+    constexpr int QA = (BM * BK + NT - 1) / NT, QB = (BN * BK + NT - 1) / NT;
+    constexpr bool EA = QA * NT == BM * BK, EB = QB * NT == BN * BK;
     __shared__ double Asb[(DB ? 2 : 1) * BK * AP];
     __shared__ double Bsb[(DB ? 2 : 1) * BK * BP];
     double *As = Asb, *Bs = Bsb;
     const BG g = items[blockIdx.y];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int rb = blockIdx.x * BM, c0 = blockIdx.z * BN;
+    const int bx = nz ? (int)blockIdx.x / nz : (int)blockIdx.x, bz = nz ? (int)blockIdx.x % nz : (int)blockIdx.z;
+    const int rb = bx * BM, c0 = bz * BN;
     if (rb >= g.nrow || c0 >= g.n) return;  // the whole block
     const int K1 = g.k, K = g.k + g.k2;
     const long lda = g.lda, lda2 = g.lda2, ldb = g.ldb, ldb2 = g.ldb2;
@@ -1077,7 +1083,7 @@ __global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bg
         for (int q = 0; q < QA; q++) {
             const int i = tid + NT * q, row = rb + (i % BM), t = kc + i / BM;
             double v = 0.0;
-            if (row < g.nrow && t < K) v = t < K1 ? g.A[row + t * lda] : g.A2[row + (t - K1) * lda2];
+            if ((EA || i < BM * BK) && row < g.nrow && t < K) v = t < K1 ? g.A[row + t * lda] : g.A2[row + (t - K1) * lda2];
             ra[q] = v;
         }
 #pragma unroll
@@ -1087,7 +1093,7 @@ __global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bg
             if (g.btrans) { t = kc + i / BN; col = c0 + i % BN; }
             else { t = kc + i % BK; col = c0 + i / BK; }
             double v = 0.0;
-            if (col < g.n && t < K) {
+            if ((EB || i < BN * BK) && col < g.n && t < K) {
                 if (t >= K1) v = g.B2[(long)col * ldb2 + (t - K1)];
                 else v = g.btrans ? g.B[(long)t * ldb + col] : g.B[(long)col * ldb + t];
             }
@@ -1098,11 +1104,12 @@ __global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bg
 #pragma unroll
         for (int q = 0; q < QA; q++) {
             const int i = tid + NT * q;
-            Ad[(i / BM) * AP + (i % BM)] = ra[q];
+            if (EA || i < BM * BK) Ad[(i / BM) * AP + (i % BM)] = ra[q];
         }
 #pragma unroll
         for (int q = 0; q < QB; q++) {
             const int i = tid + NT * q;
+            if (!EB && i >= BN * BK) continue;
             if (g.btrans) Bd[(i / BN) * BP + (i % BN)] = rbv[q];
             else Bd[(i % BK) * BP + (i / BK)] = rbv[q];
         }
@@ -1170,7 +1177,10 @@ __global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bg
             }
         }
 }
-template __global__ void k_bgemm_w<256, 128, 16, 1, 4, 4>(const BG *);
+template __global__ void k_bgemm_w<256, 128, 16, 1, 2, 4>(const BG *, int);
+template __global__ void k_bgemm_w<128, 128, 16, 1, 2, 4>(const BG *, int);
+template __global__ void k_bgemm_w<128, 64, 16, 1, 1, 4>(const BG *, int);
+template __global__ void k_bgemm_w<256, 144, 16, 1, 2, 9>(const BG *, int);
 
 // X(:, t) = Q2(:, t): the K surviving old eigenvectors of each HSS merge packed
 // densely (mdlaed2.f90:467-499), zero outside each column's support.

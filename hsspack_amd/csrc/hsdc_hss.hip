@@ -940,6 +940,14 @@ __global__ __launch_bounds__(WG) void k_hss_init(const HssMerge *hm)
 // of 4 run in order, as the single-product reference order of the tests expects.
 // ---------------------------------------------------------------------------
 typedef double d4h __attribute__((ext_vector_type(4)));
+// The operand pointers of a BG are read from the item table, so the compiler cannot see
+// that they point to global memory and would use FLAT loads, which count in LGKM_CNT as
+// well as VM_CNT: every wait for the LDS operand reads would then also wait for the next
+// chunk's loads.  Global-address-space views keep the two counters apart.
+typedef const __attribute__((address_space(1))) double gdbl;
+typedef __attribute__((address_space(1))) double gdblw;
+// what a staged element outside the operands reads
+static __device__ double bg_zero = 0.0;
 #define BG_BK 32
 #define BG_AP 132  // LDS pitch of the A slab (128 rows + 4: no bank conflicts between k rows)
 #define BG_BP 68   // LDS pitch of the B slab (64 columns + 4)
@@ -957,13 +965,14 @@ __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
     // B element i = tid + 256 q: (k kc + (i & 31), column c0 + (i >> 5)) column-major,
     // (k kc + (i >> 6), column c0 + (i & 63)) for btrans
     double ra[16], rbv[8];
+    gdbl *const zero = (gdbl *)&bg_zero;  // unconditional global loads (k_bgemm_w)
     auto stage = [&](int kc) {
 #pragma unroll
         for (int q = 0; q < 16; q++) {
             const int i = tid + 256 * q, row = rb + (i & 127), t = kc + (i >> 7);
-            double v = 0.0;
-            if (row < g.nrow && t < K) v = t < K1 ? g.A[row + t * lda] : g.A2[row + (t - K1) * lda2];
-            ra[q] = v;
+            const bool ok = row < g.nrow && t < K;
+            gdbl *p = t < K1 ? (gdbl *)g.A + (row + (long)t * lda) : (gdbl *)g.A2 + (row + (long)(t - K1) * lda2);
+            ra[q] = *(ok ? p : zero);
         }
 #pragma unroll
         for (int q = 0; q < 8; q++) {
@@ -971,12 +980,10 @@ __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
             int t, col;
             if (g.btrans) { t = kc + (i >> 6); col = c0 + (i & 63); }
             else { t = kc + (i & 31); col = c0 + (i >> 5); }
-            double v = 0.0;
-            if (col < g.n && t < K) {
-                if (t >= K1) v = g.B2[(long)col * ldb2 + (t - K1)];
-                else v = g.btrans ? g.B[(long)t * ldb + col] : g.B[(long)col * ldb + t];
-            }
-            rbv[q] = v;
+            const bool ok = col < g.n && t < K;
+            gdbl *p = t >= K1 ? (gdbl *)g.B2 + ((long)col * ldb2 + (t - K1))
+                              : (gdbl *)g.B + (g.btrans ? (long)t * ldb + col : (long)col * ldb + t);
+            rbv[q] = *(ok ? p : zero);
         }
     };
     auto commit = [&]() {
@@ -1027,11 +1034,11 @@ __global__ __launch_bounds__(WG) void k_bgemm(const BG *items)
         for (int b = 0; b < 4; b++) {
             const int col = c0 + b * 16 + lr;
             if (col >= g.n) continue;
-            const long pc = g.cmap ? (long)g.cmap[col] - g.cmoff : (long)col;
+            const long pc = g.cmap ? (long)((const __attribute__((address_space(1))) int *)g.cmap)[col] - g.cmoff : (long)col;
             for (int i = 0; i < 4; i++) {
                 int row = rb + wr + a * 16 + lk + 4 * i;
                 if (row < g.nrow) {
-                    double *cp = g.C + (long)row + pc * g.ldc;
+                    gdblw *cp = (gdblw *)g.C + ((long)row + pc * g.ldc);
                     *cp = g.beta ? (*cp + acc[a][b][i]) : acc[a][b][i];
                 }
             }
@@ -1078,13 +1085,19 @@ __global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bg
     // (k kc + i / BN, column c0 + i % BN) for btrans, (k kc + i % BK, column c0 + i / BK)
     // column-major
     double ra[QA], rbv[QB];
+    // Every staged element is one unconditional global load, an element outside the
+    // operands reading a global zero, so no load sits in a branch and nothing uses the
+    // loaded value before the commit (a branch around loads, or a select on their values,
+    // makes the compiler wait for them at once, which serialises the prefetch with the
+    // MFMAs).
+    gdbl *const zero = (gdbl *)&bg_zero;
     auto stage = [&](int kc) {
 #pragma unroll
         for (int q = 0; q < QA; q++) {
             const int i = tid + NT * q, row = rb + (i % BM), t = kc + i / BM;
-            double v = 0.0;
-            if ((EA || i < BM * BK) && row < g.nrow && t < K) v = t < K1 ? g.A[row + t * lda] : g.A2[row + (t - K1) * lda2];
-            ra[q] = v;
+            const bool ok = (EA || i < BM * BK) && row < g.nrow && t < K;
+            gdbl *p = t < K1 ? (gdbl *)g.A + (row + (long)t * lda) : (gdbl *)g.A2 + (row + (long)(t - K1) * lda2);
+            ra[q] = *(ok ? p : zero);
         }
 #pragma unroll
         for (int q = 0; q < QB; q++) {
@@ -1092,12 +1105,10 @@ __global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bg
             int t, col;
             if (g.btrans) { t = kc + i / BN; col = c0 + i % BN; }
             else { t = kc + i % BK; col = c0 + i / BK; }
-            double v = 0.0;
-            if ((EB || i < BN * BK) && col < g.n && t < K) {
-                if (t >= K1) v = g.B2[(long)col * ldb2 + (t - K1)];
-                else v = g.btrans ? g.B[(long)t * ldb + col] : g.B[(long)col * ldb + t];
-            }
-            rbv[q] = v;
+            const bool ok = (EB || i < BN * BK) && col < g.n && t < K;
+            gdbl *p = t >= K1 ? (gdbl *)g.B2 + ((long)col * ldb2 + (t - K1))
+                              : (gdbl *)g.B + (g.btrans ? (long)t * ldb + col : (long)col * ldb + t);
+            rbv[q] = *(ok ? p : zero);
         }
     };
     auto commit = [&](double *Ad, double *Bd) {
@@ -1166,12 +1177,12 @@ __global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bg
         for (int b = 0; b < TB; b++) {
             const int col = c0 + wc + b * 16 + lr;
             if (col >= g.n) continue;
-            const long pc = g.cmap ? (long)g.cmap[col] - g.cmoff : (long)col;
+            const long pc = g.cmap ? (long)((const __attribute__((address_space(1))) int *)g.cmap)[col] - g.cmoff : (long)col;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int row = rb + wr + a * 16 + lk + 4 * i;
                 if (row < g.nrow) {
-                    double *cp = g.C + (long)row + pc * g.ldc;
+                    gdblw *cp = (gdblw *)g.C + ((long)row + pc * g.ldc);
                     *cp = g.beta ? (*cp + acc[a][b][i]) : acc[a][b][i];
                 }
             }

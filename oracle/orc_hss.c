@@ -368,6 +368,12 @@ static void rrluRow(double *F, double *D, double *A, double *V, const double *DI
     }
 }
 
+/* Test hook: the row (rowcol 0) or column RRLU above on given generators, S = NULL;
+ * the tests compare it step for step with the reference's own routines. */
+void orc_rrlu(int rowcol, double *F, double *D, double *A, double *V, const double *DIFL, const double *DIFR,
+              double tol, double *U, double *W, const double *FF, int *PL, int *PU, int M, int N, int *Rk,
+              int *PQ, long long *nflops, long long *nswap);
+
 /* rrluCauchyEigcol (CauchyHssEig.f90:215-467) */
 static void rrluCol(double *D, double *F, double *A, double *V, const double *DIFL,
                     const double *DIFR, const double *S, double tol, double *U, double *W, const double *FF,
@@ -1566,4 +1572,14 @@ int orc_mat2hss(orc_hss *H, int n, int ni, const double *A, int lda, double tol)
     }
     free(C); free(act); free(ars); free(arn); free(acs); free(acn); free(r0); free(dep);
     return info;
+}
+
+void orc_rrlu(int rowcol, double *F, double *D, double *A, double *V, const double *DIFL, const double *DIFR,
+              double tol, double *U, double *W, const double *FF, int *PL, int *PU, int M, int N, int *Rk,
+              int *PQ, long long *nflops, long long *nswap)
+{
+    if (rowcol == 0)
+        rrluRow(F, D, A, V, DIFL, DIFR, NULL, tol, U, W, FF, PL, PU, M, N, Rk, PQ, nflops, nswap);
+    else
+        rrluCol(D, F, A, V, DIFL, DIFR, NULL, tol, U, W, FF, PL, PU, M, N, Rk, PQ, nflops, nswap);
 }

@@ -21,9 +21,15 @@ REF = "/root/reference/SRC"
 OUT = os.path.join(HERE, "_ref")
 FC = "/opt/rocm/llvm/bin/flang"
 # module order matters: a file's USEd modules come first
-SOURCES = ["aux_hss", "BasicMM", "ConstructHssd", "CauchyHssEig", "mdlaed0", "mdlaed1", "mdlaed2", "mdlaed3",
-           "mdstedc"]
-HOOKS = {"mdlaed1": {"mdlaed2_": "refw_mdlaed2_"}, "mdlaed2": {"drot_": "refw_drot_"}}
+SOURCES = ["comp_Wtime", "aux_hss", "BasicMM", "ConstructHssd", "CauchyHssEig", "CauchyHssvd_VPS",
+           "CauchyHss_VP2", "mdlaed0", "mdlaed1", "mdlaed2", "mdlaed3", "mdstedc",
+           "mdbdsdc", "mdlasd0", "mdlasd1", "mdlasd2", "mdlasd3", "mdlasd8",
+           "mdsbevd", "mdsbed1", "mdsbed2", "mdsbed22", "mdsbed3", "mdsbed4", "mdsbed5",
+           "DHSSMM", "cauchyhssmm", "Cauchylike", "SFtransf"]
+HOOKS = {"mdlaed1": {"mdlaed2_": "refw_mdlaed2_"}, "mdlaed2": {"drot_": "refw_drot_"},
+         "mdlasd1": {"mdlasd2_": "refw_mdlasd2_"}, "mdlasd2": {"drot_": "refw_drot_all_"},
+         "mdsbed1": {"mdsbed2_": "refw_mdsbed2_", "mdsbed22_": "refw_mdsbed22_"},
+         "mdsbed2": {"drot_": "refw_drot_q_"}, "mdsbed22": {"drot_": "refw_drot_q_"}}
 # The row pivot search's early-exit test (CauchyPivtEig, CauchyHssEig.f90:676 and :678)
 # forms its pivot with F(PL(k)): F is the node's local row-generator array (length M) but
 # PL(k) is a global pole index, so the test reads whatever that local buffer holds at
@@ -70,10 +76,17 @@ def build(verbose=False):
     mod_dir = os.path.join(OUT, "mod")
     os.makedirs(obj_dir, exist_ok=True)
     os.makedirs(mod_dir, exist_ok=True)
-    flags = ["-O2", "-fPIC", "-module-dir", mod_dir, "-I", mod_dir]
+    # -O0: flang's optimised MDSBEVD enters its merge loop once more with one subproblem
+    # left (label 80's IF (SUBPBS.GT.1) taken at SUBPBS = 1, mdsbevd.f90:405-456) and
+    # calls MDSBED1 with CUTPNT = N, which returns INFO = -7; unoptimised code runs the
+    # loop as written.  No fast-math and no FMA contraction at any level, so the
+    # arithmetic is the same.
+    flags = ["-O0", "-fPIC", "-module-dir", mod_dir, "-I", mod_dir]
     objs = []
     for name in SOURCES + ["ref_drive"]:
         src = os.path.join(HERE, name + ".f90") if name == "ref_drive" else os.path.join(REF, name + ".f90")
+        if not os.path.exists(src):
+            src = os.path.join(REF, name + ".f")  # fixed-form sources
         obj = os.path.join(obj_dir, name + ".o")
         run([FC] + flags + ["-c", src, "-o", obj], verbose)
         ren = {}
@@ -112,7 +125,7 @@ def build(verbose=False):
         obj = os.path.join(obj_dir, name + "_fix.o")
         fmod = os.path.join(OUT, "mod_fix")
         os.makedirs(fmod, exist_ok=True)
-        run([FC, "-O2", "-fPIC", "-module-dir", fmod, "-I", mod_dir, "-c", src, "-o", obj], verbose)
+        run([FC, "-O0", "-fPIC", "-module-dir", fmod, "-I", mod_dir, "-c", src, "-o", obj], verbose)
         run(["objcopy", "--redefine-syms=" + os.path.join(obj_dir, name + ".syms"), obj], verbose)
         fixed[SOURCES.index(name)] = obj
     link(os.path.join(OUT, "libhssref_fix.so"), fixed)

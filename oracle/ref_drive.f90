@@ -180,3 +180,291 @@ subroutine refd_rrlu(rowcol, f, d, a, v, difl, difr, tol, u, w, ff, pl, pu, m, n
         call rrluCauchyEigcol(d, f, a, v, difl, difr, tol, u, w, ff, pl, pu, m, n, rk, pq, nflops, nswap)
     end if
 end subroutine refd_rrlu
+
+! ---- bidiagonal SVD (MDBDSDC, mdbdsdc.f90:1) and banded DC (MDSBEVD, mdsbevd.f90:183) ----
+! Hooks: mdlasd1's MDLASD2 call (mdlasd1.f90:208) and mdlasd2's DROT (two per deflation
+! rotation, mdlasd2.f90:364-365); mdsbed1's MDSBED2 / MDSBED22 calls (mdsbed1.f90:205,
+! 262), whose deflation rotates Q's columns (length N) and Z's rows (shorter): only the
+! former are counted.
+
+subroutine refw_mdlasd2(nl, nr, sqre, k, d, z, alpha, beta, u, ldu, vt, ldvt, dsigma, u2, ldu2, vt2, ldvt2, &
+                        idxp, idx, idxc, idxq, coltyp, hflag, info)
+    use refdrv_trace
+    implicit none
+    integer :: nl, nr, sqre, k, ldu, ldvt, ldu2, ldvt2, info
+    double precision :: alpha, beta
+    logical :: hflag
+    integer :: idxp(*), idx(*), idxc(*), idxq(*), coltyp(*)
+    double precision :: d(*), z(*), u(*), vt(*), dsigma(*), u2(*), vt2(*)
+    external :: mdlasd2
+    rot_count = 0
+    call mdlasd2(nl, nr, sqre, k, d, z, alpha, beta, u, ldu, vt, ldvt, dsigma, u2, ldu2, vt2, ldvt2, &
+                 idxp, idx, idxc, idxq, coltyp, hflag, info)
+    if (nmerge < tcap) then
+        nmerge = nmerge + 1
+        tr_n(nmerge) = nl + nr + 1
+        tr_k(nmerge) = k
+        tr_rot(nmerge) = rot_count / 2
+        if (hflag) then
+            tr_hss(nmerge) = 0
+        else
+            tr_hss(nmerge) = 1
+        end if
+    end if
+end subroutine refw_mdlasd2
+
+subroutine refw_drot_all(n, x, incx, y, incy, c, s)
+    use refdrv_trace
+    implicit none
+    integer :: n, incx, incy
+    double precision :: x(*), y(*), c, s
+    external :: drot
+    rot_count = rot_count + 1
+    call drot(n, x, incx, y, incy, c, s)
+end subroutine refw_drot_all
+
+module refdrv_band
+    implicit none
+    integer :: cur_n = -1
+end module refdrv_band
+
+subroutine refw_drot_q(n, x, incx, y, incy, c, s)
+    use refdrv_trace
+    use refdrv_band
+    implicit none
+    integer :: n, incx, incy
+    double precision :: x(*), y(*), c, s
+    external :: drot
+    if (n == cur_n) rot_count = rot_count + 1
+    call drot(n, x, incx, y, incy, c, s)
+end subroutine refw_drot_q
+
+subroutine refw_band_record(n, k, hss)
+    use refdrv_trace
+    implicit none
+    integer :: n, k, hss
+    if (nmerge < tcap) then
+        nmerge = nmerge + 1
+        tr_n(nmerge) = n
+        tr_k(nmerge) = k
+        tr_rot(nmerge) = rot_count
+        tr_hss(nmerge) = hss
+    end if
+end subroutine refw_band_record
+
+subroutine refw_mdsbed2(k, n, n1, kd, d, q, ldq, indxq, rho, z, z2, dlamda, w, q2, indx, indxc, indxp, &
+                        coltyp, hflag, bsmlz, info)
+    use refdrv_trace
+    use refdrv_band
+    implicit none
+    integer :: k, n, n1, kd, ldq, bsmlz, info
+    double precision :: rho
+    logical :: hflag
+    integer :: indxq(*), indx(*), indxc(*), indxp(*), coltyp(*)
+    double precision :: d(*), q(*), z(*), z2(*), dlamda(*), w(*), q2(*)
+    external :: mdsbed2, refw_band_record
+    rot_count = 0
+    cur_n = n
+    call mdsbed2(k, n, n1, kd, d, q, ldq, indxq, rho, z, z2, dlamda, w, q2, indx, indxc, indxp, coltyp, &
+                 hflag, bsmlz, info)
+    cur_n = -1
+    if (hflag) then
+        call refw_band_record(n, k, 0)
+    else
+        call refw_band_record(n, k, 1)
+    end if
+end subroutine refw_mdsbed2
+
+subroutine refw_mdsbed22(k, n, n1, sb, d, q, ldq, rho, z, z2, dlamda, w, q2, indxq, indxp, info)
+    use refdrv_trace
+    use refdrv_band
+    implicit none
+    integer :: k, n, n1, sb, ldq, info
+    double precision :: rho
+    integer :: indxq(*), indxp(*)
+    double precision :: d(*), q(*), z(*), z2(*), dlamda(*), w(*), q2(*)
+    external :: mdsbed22, refw_band_record
+    rot_count = 0
+    cur_n = n
+    call mdsbed22(k, n, n1, sb, d, q, ldq, rho, z, z2, dlamda, w, q2, indxq, indxp, info)
+    cur_n = -1
+    call refw_band_record(n, k, -1)
+end subroutine refw_mdsbed22
+
+subroutine refd_bdsdc(n, d, e, u, vt, info) bind(c, name="refd_bdsdc")
+    use iso_c_binding
+    use refdrv_trace
+    implicit none
+    integer(c_int), value :: n
+    real(c_double) :: d(*), e(*), u(n, *), vt(n, *)
+    integer(c_int) :: info
+    double precision, allocatable :: work(:)
+    integer, allocatable :: iwork(:)
+    double precision :: q(1)
+    integer :: iq(1)
+    external :: mdbdsdc
+    nmerge = 0
+    ! 3N^2 + 4N for COMPQ = 'I' (mdbdsdc.f90:113), doubled for the HSS branch's generators
+    allocate (work(6 * n * n + 16 * n + 16), iwork(8 * n + 16))
+    call mdbdsdc('U', 'I', n, d, e, u, n, vt, n, q, iq, work, iwork, info)
+    deallocate (work, iwork)
+end subroutine refd_bdsdc
+
+subroutine refd_sbevd(n, kd, ab, ldab, w, q, ldq, info) bind(c, name="refd_sbevd")
+    use iso_c_binding
+    use refdrv_trace
+    implicit none
+    integer(c_int), value :: n, kd, ldab, ldq
+    real(c_double) :: ab(ldab, *), w(*), q(ldq, *)
+    integer(c_int) :: info
+    double precision, allocatable :: work(:)
+    integer, allocatable :: iwork(:)
+    double precision :: wq(1)
+    integer :: iq(1), lwork, liwork
+    external :: mdsbevd
+    nmerge = 0
+    call mdsbevd('I', 'U', n, kd, ab, ldab, w, q, ldq, wq, -1, iq, -1, info)
+    if (info /= 0) return
+    lwork = 4 * max(1, int(wq(1))) + (64 + 8 * kd) * n + 16 * kd * kd * n
+    liwork = 4 * max(1, iq(1)) + 16 * n
+    allocate (work(lwork), iwork(liwork))
+    call mdsbevd('I', 'U', n, kd, ab, ldab, w, q, ldq, work, lwork, iwork, liwork, info)
+    deallocate (work, iwork)
+end subroutine refd_sbevd
+
+! ---- dense -> HSS (mat2hssvd, ConstructHssd.f90:371) and the general Cauchy-like
+! construction (cauchy2hss, CauchyHss_VP2.f90:387), set up over npart's tree as DHSSMM and
+! CauchyHssMM do; per-node generator dimensions out.  And the two drivers themselves
+! (DHSSMM.f90:1, cauchyhssmm.f90:1) for their products.
+
+subroutine refd_tree_dims(ph, ltr, tr, ltr_out, tr_out, usz, vsz, bsz, dsz)
+    use aux_hss
+    implicit none
+    type(hssmm) :: ph
+    integer :: ltr, tr(*), ltr_out, tr_out(*), usz(2, *), vsz(2, *), bsz(2, *), dsz(*)
+    ltr_out = ltr
+    tr_out(1:ltr) = tr(1:ltr)
+    usz(:, 1:ltr) = ph%U(:, 1:ltr)
+    vsz(:, 1:ltr) = ph%V(:, 1:ltr)
+    bsz(:, 1:ltr) = ph%B(:, 1:ltr)
+    dsz(1:ltr) = ph%D(1:ltr)
+end subroutine refd_tree_dims
+
+subroutine refd_mat2hss(n, a, lda, ni, tol, ltr_out, tr_out, usz, vsz, bsz, dsz, prank, info) &
+        bind(c, name="refd_mat2hss")
+    use iso_c_binding
+    use aux_hss
+    use ConstructHssd
+    implicit none
+    integer(c_int), value :: n, lda, ni
+    real(c_double), value :: tol
+    real(c_double) :: a(lda, *)
+    integer(c_int) :: ltr_out, tr_out(*), usz(2, *), vsz(2, *), bsz(2, *), dsz(*), prank, info
+    integer :: ltr, lm, lvl, n1, pnh, tnk, i, nb, rk
+    type(HTinfo) :: tre
+    type(hssmm) :: ph
+    integer, allocatable :: tr(:), blk(:)
+    double precision, allocatable :: h(:), dd(:), work(:), acopy(:, :)
+    info = 0
+    nb = ni
+    allocate (blk(n / nb + 1), tr(2 * (n / nb + 1) - 1))
+    blk = 0
+    ltr = 0
+    call npart(n, nb, tr, ltr, blk, lm)
+    lvl = ceiling(log2(ltr + 1)) - 1
+    n1 = (ltr + 1) / 2
+    allocate (tre%ttr(lvl, n1), tre%lentr(lvl), tre%lfx(ltr), tre%ch(2, ltr))
+    call GetTreeInfo(tr, ltr, blk, blk, tre, lvl)
+    tnk = 0
+    do i = 1, lm
+        tnk = tnk + blk(i)**2
+    end do
+    allocate (ph%D(ltr), ph%U(2, ltr), ph%V(2, ltr), ph%B(2, ltr), ph%ph(4, ltr), ph%pd(2, ltr))
+    allocate (h(8 * nb * n), dd(tnk), work(2 * lvl * nb * n))
+    call hssexpmm0(ph, ltr)
+    allocate (acopy(n, n))
+    acopy(:, :) = a(1:n, 1:n)
+    pnh = 1
+    rk = 0
+    call mat2hssvd(acopy, n, tr, ltr, blk, lm, ph, h, dd, tol, rk, work, pnh)
+    prank = rk
+    call refd_tree_dims(ph, ltr, tr, ltr_out, tr_out, usz, vsz, bsz, dsz)
+    deallocate (tre%ttr, tre%lentr, tre%lfx, tre%ch, ph%D, ph%U, ph%V, ph%B, ph%ph, ph%pd)
+    deallocate (blk, tr, h, dd, work, acopy)
+end subroutine refd_mat2hss
+
+subroutine refd_cauchy2hss(n, ni, d, f, u, v, tol, ltr_out, tr_out, usz, vsz, bsz, dsz, info) &
+        bind(c, name="refd_cauchy2hss")
+    use iso_c_binding
+    use aux_hss
+    use CauchyHss_VP2
+    implicit none
+    integer(c_int), value :: n, ni
+    real(c_double), value :: tol
+    real(c_double) :: d(*), f(*), u(*), v(*)
+    integer(c_int) :: ltr_out, tr_out(*), usz(2, *), vsz(2, *), bsz(2, *), dsz(*), info
+    integer :: ltr, lm, lvl, n1, pnh, tnk, i, nb, n0
+    type(HTinfo) :: tre
+    type(hssmm) :: ph
+    integer, allocatable :: tr(:), m(:)
+    double precision, allocatable :: h(:), dd(:), dc(:), fc(:), uc(:), vc(:)
+    info = 0
+    nb = ni
+    allocate (m(n / nb + 1), tr(2 * (n / nb + 1) - 1))
+    m = 0
+    ltr = 0
+    call npart(n, nb, tr, ltr, m, lm)
+    lvl = ceiling(log2(ltr + 1)) - 1
+    n1 = (ltr + 1) / 2
+    allocate (tre%ttr(lvl, n1), tre%lentr(lvl), tre%lfx(ltr), tre%ch(2, ltr))
+    call GetTreeInfo(tr, ltr, m, m, tre, lvl)
+    tnk = 0
+    do i = 1, lm
+        tnk = tnk + m(i)**2
+    end do
+    allocate (ph%D(ltr), ph%U(2, ltr), ph%V(2, ltr), ph%B(2, ltr), ph%ph(4, ltr), ph%pd(2, ltr))
+    allocate (h(8 * nb * n), dd(tnk))
+    call hssexpmm0(ph, ltr)
+    allocate (dc(n), fc(n), uc(n), vc(n))
+    dc = d(1:n)
+    fc = f(1:n)
+    uc = u(1:n)
+    vc = v(1:n)
+    pnh = 1
+    n0 = n
+    call cauchy2hss(dc, fc, uc, vc, n0, tr, ltr, m, lm, ph, h, dd, tol, lvl, pnh)
+    call refd_tree_dims(ph, ltr, tr, ltr_out, tr_out, usz, vsz, bsz, dsz)
+    deallocate (tre%ttr, tre%lentr, tre%lfx, tre%ch, ph%D, ph%U, ph%V, ph%B, ph%ph, ph%pd)
+    deallocate (m, tr, h, dd, dc, fc, uc, vc)
+end subroutine refd_cauchy2hss
+
+subroutine refd_dhssmm(n, a, lda, b, ldb, k, tol, ni, ncol, info) bind(c, name="refd_dhssmm")
+    use iso_c_binding
+    implicit none
+    integer(c_int), value :: n, lda, ldb, k, ni, ncol
+    real(c_double), value :: tol
+    real(c_double) :: a(lda, *), b(ldb, *)
+    integer(c_int) :: info
+    integer :: prank, nb, nc
+    double precision :: time(2)
+    external :: dhssmm
+    prank = 0
+    nb = ni
+    nc = ncol
+    call dhssmm(a, lda, n, n, b, ldb, k, tol, prank, nb, nc, time, info)
+end subroutine refd_dhssmm
+
+subroutine refd_cauchyhssmm(n, d, f, u, v, ni, tol, ncol, b, ldb) bind(c, name="refd_cauchyhssmm")
+    use iso_c_binding
+    implicit none
+    integer(c_int), value :: n, ni, ncol, ldb
+    real(c_double), value :: tol
+    real(c_double) :: d(*), f(*), u(*), v(*), b(ldb, *)
+    integer :: nn, nb, nc
+    double precision :: time(2)
+    external :: cauchyhssmm
+    nn = n
+    nb = ni
+    nc = ncol
+    call cauchyhssmm(d, f, u, v, nn, nb, tol, nc, b, ldb, time)
+end subroutine refd_cauchyhssmm

@@ -76,6 +76,7 @@ struct M2hCopy {
 };
 __global__ void k_m2h_gather(const M2hJob *, const double *, long, double *);
 __global__ void k_m2h_svd(const M2hJob *, double *, double *, double, int *, int *);
+__global__ void k_m2h_sketch(const M2hJob *, const M2hJob *, const double *, double *, unsigned long long);
 __global__ void k_m2h_rows(const M2hBlk *, int, const M2hJob *, const double *, const double *, long, int, double *,
                            long);
 __global__ void k_m2h_cols(const M2hBlk *, int, const M2hJob *, const double *, const double *, long, int, double *);
@@ -2711,8 +2712,9 @@ int hsdc_cauchyhssmm(hsdc_ctx *c, const double *d, const double *f, const double
 // directory into Q.  Q.ok stays false for a tree of one leaf or more than 512 leaves, a
 // Jacobi sweep limit or a full arena; the caller then multiplies densely.  ranks (may be
 // null; 2 per node): the ranks of each node's U and V.
+// rsvd: randomized range finding (mat2hssvdr) with the Omega stream of seed
 static int mat2hss_build(hsdc_ctx *c, const double *dA, int n, int ni, double tol, HssPlanM &Q, int *ranks,
-                         float *ms_build)
+                         float *ms_build, bool rsvd = false, unsigned long long seed = 0)
 {
     hipStream_t s = c->stream;
     Q.ok = false;
@@ -2763,8 +2765,9 @@ static int mat2hss_build(hsdc_ctx *c, const double *dA, int n, int ni, double to
         const std::vector<int> &nds = T.lev[LL];
         const int nn = (int)nds.size();
         std::vector<int> a0(nn), a1(nn);
-        std::vector<M2hJob> jobs(2 * nn);
+        std::vector<M2hJob> jobs(2 * nn), sk(2 * nn);
         long so = 0, bo = 0;
+        bool any_sketch = false;
         for (int q = 0; q < nn; q++) {
             const int nd = nds[q];
             const int f = T.c1[nd] ? T.c1[nd] : nd, l = T.c1[nd] ? T.c2[nd] : nd;
@@ -2782,24 +2785,37 @@ static int mat2hss_build(hsdc_ctx *c, const double *dA, int n, int ni, double to
                 J.m = sd == 0 ? rn : cn;
                 J.w = sd == 0 ? nc - cn : nr - rn;
                 J.toff = so;
-                so += (long)J.w * J.m + 2L * J.m * J.m;
                 J.boff = bo;
                 bo += (long)J.m * J.m + 2L * J.m;
+                M2hJob &K = sk[2 * q + sd];
+                K = J;
+                if (rsvd && J.w > 50 && J.w > J.m + 10) {  // mat2hssvdr's sampling test
+                    K.w = J.m + 10;
+                    K.toff = so + (long)J.w * J.m;
+                    so += (long)J.w * J.m + (long)K.w * J.m + 2L * J.m * J.m;
+                    any_sketch = true;
+                } else {
+                    so += (long)J.w * J.m + 2L * J.m * J.m;
+                }
             }
         }
         CK(ensure(c->xb, sizeof(double) * (size_t)std::max(so, 1L)));
         CK(ensure(c->yb, sizeof(double) * (size_t)std::max(bo, 1L)));
         double *S = (double *)c->xb.p, *Bs = (double *)c->yb.p;
         const size_t jb = al64(sizeof(M2hJob) * jobs.size());
-        CK(ensure(c->tabs, jb + 64 + sizeof(int) * jobs.size()));
+        CK(ensure(c->tabs, 2 * jb + 64 + sizeof(int) * jobs.size()));
         char *tb = (char *)c->tabs.p;
-        M2hJob *dj = (M2hJob *)tb;
-        int *derr = (int *)(tb + jb), *drank = derr + 16;
+        M2hJob *dj = (M2hJob *)tb, *dk = (M2hJob *)(tb + jb);
+        int *derr = (int *)(tb + 2 * jb), *drank = derr + 16;
         HK(hipMemcpyAsync(dj, jobs.data(), sizeof(M2hJob) * jobs.size(), hipMemcpyHostToDevice, s));
+        HK(hipMemcpyAsync(dk, sk.data(), sizeof(M2hJob) * sk.size(), hipMemcpyHostToDevice, s));
         HK(hipMemsetAsync(derr, 0, sizeof(int), s));
         hipLaunchKernelGGL(k_m2h_gather, dim3(64, (unsigned)jobs.size()), dim3(WGS), 0, s, (const M2hJob *)dj,
                            (const double *)Cc, (long)nr, S);
-        hipLaunchKernelGGL(k_m2h_svd, dim3((unsigned)jobs.size()), dim3(WGS), 0, s, (const M2hJob *)dj, S, Bs, tol,
+        if (any_sketch)
+            hipLaunchKernelGGL(k_m2h_sketch, dim3(32, (unsigned)jobs.size()), dim3(WGS), 0, s, (const M2hJob *)dj,
+                               (const M2hJob *)dk, (const double *)S, S, seed);
+        hipLaunchKernelGGL(k_m2h_svd, dim3((unsigned)jobs.size()), dim3(WGS), 0, s, (const M2hJob *)dk, S, Bs, tol,
                            drank, derr);
         HK(hipGetLastError());
         std::vector<int> rk(jobs.size());
@@ -2936,7 +2952,7 @@ static int dhss_args(hsdc_ctx *c, const double *a, int lda, int m, int n, const 
 // blocks.  A is not modified (the reference uses it as workspace); prank is accepted and
 // unused, as in mat2hssvd.
 static int dhss_run(hsdc_ctx *c, const double *a, int lda, int n, double *b, int ldb, int k, double tol, int ni,
-                    int ncol, double *time, int *ranks, bool left)
+                    int ncol, double *time, int *ranks, bool left, bool rsvd = false, unsigned long long seed = 0)
 {
     if (time) { time[0] = 0.0; time[1] = 0.0; }
     c->evc_k = 0;
@@ -2963,7 +2979,7 @@ static int dhss_run(hsdc_ctx *c, const double *a, int lda, int n, double *b, int
                         hipMemcpyHostToDevice, s));
     std::vector<HssPlanM> P(1);
     float ms_build = 0.0f;
-    CK(mat2hss_build(c, dA, n, ni, tol, P[0], ranks, &ms_build));
+    CK(mat2hss_build(c, dA, n, ni, tol, P[0], ranks, &ms_build, rsvd, seed));
     if (!P[0].ok && ranks) ranks[0] = -1;
     const int nb = (int)std::min<size_t>(kk, (size_t)std::max(ncol, 1024));
     CK(ensure(c->tb, sizeof(double) * nn * nb));
@@ -3030,6 +3046,15 @@ int hsdc_dhssmml(hsdc_ctx *c, const double *a, int lda, int m, int n, double *b,
     const int rc = dhss_args(c, a, lda, m, n, b, ldb, k, tol, ni, ncol, false);
     if (rc) return rc;
     return dhss_run(c, a, lda, n, b, ldb, k, tol, ni, ncol, time, ranks, false);
+}
+
+int hsdc_dhssmm_rand(hsdc_ctx *c, const double *a, int lda, int m, int n, double *b, int ldb, int k, double tol,
+                     int ni, int ncol, int side, unsigned long long seed, double *time, int *ranks)
+{
+    if (side != 0 && side != 1) return -12;
+    const int rc = dhss_args(c, a, lda, m, n, b, ldb, k, tol, ni, ncol, side == 0);
+    if (rc) return rc;
+    return dhss_run(c, a, lda, n, b, ldb, k, tol, ni, ncol, time, ranks, side == 0, true, seed);
 }
 
 int hsdc_hss_dense(hsdc_ctx *c, double *a, int lda)

@@ -16,6 +16,9 @@
 //   k_m2h_cols     element per thread
 //   k_m2h_copy     generators: U (m x r), V = W^T (r x m), each sibling pair's B from C'
 //
+// With randomized range finding (mat2hssvdr), k_m2h_sketch forms Omega T for each side
+// with a long context before k_m2h_svd, which then decomposes the sketch.
+//
 // The generators land in the layout the HSS multiply reads (U, V^T, B, D per node, the
 // post-ordered tree of npart), so DHSSMM and DHSSMML run the same batched multiply as
 // the eigenvector matrices.
@@ -215,6 +218,47 @@ __global__ __launch_bounds__(MWG) void k_m2h_svd(const M2hJob *jobs, double *S, 
             }
         }
         B[e] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Randomized range finding (mat2hssvdr, ConstructHssd.f90:8-368): a side whose context
+// is longer than 50 and than m + 10 takes its basis from the sketch Omega T (pp x m,
+// pp = m + 10) instead of T.  The reference draws one flat array of uniform [0, 1)
+// numbers and reads it as a pp x w matrix (leading dimension pp) for a block column and
+// as a w x pp matrix (leading dimension w) for a block row (:104, :129, ...); entry i of
+// the flat array here is m2h_omega(seed, i), the splitmix64 stream of the seed, so the
+// oracle draws the same numbers.  sk[job] is the sketch job (w = pp, toff at its
+// scratch) or the job itself.  Grid: (chunks, jobs); one thread per element of S.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double m2h_omega(unsigned long long seed, unsigned long long i)
+{
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1p-53;
+}
+
+__global__ __launch_bounds__(MWG) void k_m2h_sketch(const M2hJob *jobs, const M2hJob *sk, const double *S0, double *S,
+                                                    unsigned long long seed)
+{
+    const M2hJob J = jobs[blockIdx.y], K = sk[blockIdx.y];
+    if (K.w == J.w) return;  // no sketch for this side
+    const int side = blockIdx.y & 1, pp = K.w, w = J.w;
+    const double *T = S0 + J.toff;
+    double *Y = S + K.toff;
+    const long tot = (long)pp * J.m;
+    for (long e = (long)blockIdx.x * MWG + threadIdx.x; e < tot; e += (long)gridDim.x * MWG) {
+        const int r = (int)(e % pp), t = (int)(e / pp);
+        const double *tc = T + (long)t * w;
+        double acc = 0.0;
+        for (int o = 0; o < w; o++) {
+            const unsigned long long idx = side ? (unsigned long long)r + (unsigned long long)o * pp
+                                                : (unsigned long long)o + (unsigned long long)r * w;
+            acc += m2h_omega(seed, idx) * tc[o];
+        }
+        Y[e] = acc;
     }
 }
 

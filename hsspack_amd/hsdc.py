@@ -35,7 +35,7 @@ EXPORTS = ["hsdc_default_params", "hsdc_ctx_create", "hsdc_ctx_destroy", "hsdc_s
            "hsdc_trace", "hsdc_trace_indxq", "hsdc_loewner_noalloc", "hsdc_cauchyhssmm", "hsdc_thread_ctx",
            "hsdc_bd_thread_ctx", "hsdc_thread_ctx_release", "hsdc_sb_default_params", "hsdc_sb_ctx_create",
            "hsdc_sb_ctx_destroy", "hsdc_sb_set_params", "hsdc_sb_get_stats", "hsdc_sb_set_stream",
-           "hsdc_sb_thread_ctx", "hsdc_sbevd", "hsdc_sbevd_dev", "hsdc_sb_trace", "hsdc_dhssmm", "hsdc_dhssmml"]
+           "hsdc_sb_thread_ctx", "hsdc_sbevd", "hsdc_sbevd_dev", "hsdc_sb_trace", "hsdc_dhssmm", "hsdc_dhssmml", "hsdc_dhssmm_rand"]
 
 # learned secular start: 34 inputs -> 64 -> 64 -> 1
 MLP_IN, MLP_H = 34, 64
@@ -155,6 +155,8 @@ def lib():
                                        _P(_i)]
         L.hsdc_dhssmm.argtypes = [_vp, _P(_d), _i, _i, _i, _P(_d), _i, _i, _d, _i, _i, _i, _P(_d), _P(_i)]
         L.hsdc_dhssmml.argtypes = [_vp, _P(_d), _i, _i, _i, _P(_d), _i, _i, _d, _i, _i, _i, _P(_d), _P(_i)]
+        L.hsdc_dhssmm_rand.argtypes = [_vp, _P(_d), _i, _i, _i, _P(_d), _i, _i, _d, _i, _i, _i, ctypes.c_ulonglong,
+                                       _P(_d), _P(_i)]
         L.hsdc_thread_ctx.restype = _vp
         L.hsdc_bd_thread_ctx.restype = _vp
         L.hsdc_sb_ctx_create.argtypes = [_P(_vp)]
@@ -462,9 +464,10 @@ class Context:
             raise RuntimeError("hsdc_cauchyhssmm: %d" % info)
         return b, t, (None if ranks[0] < 0 else ranks)
 
-    def dhssmm(self, a, b, ni, tol, ncol=100, prank=0, right=False):
+    def dhssmm(self, a, b, ni, tol, ncol=100, prank=0, right=False, seed=None):
         """DHSSMM (A b, b n x k) or, right=True, DHSSMML (b A, b k x n) with A put in HSS
-        form by truncated SVDs.  Returns (product, (construction s, multiply s),
+        form by truncated SVDs, or with seed given by randomized range finding
+        (mat2hssvdr, hsdc_dhssmm_rand).  Returns (product, (construction s, multiply s),
         ranks (2 per tree node) or None when A was applied densely)."""
         a = np.asfortranarray(np.asarray(a, np.float64))
         b = np.array(b, np.float64, order="F")
@@ -472,11 +475,18 @@ class Context:
         k = b.shape[0] if right else b.shape[1]
         t = np.zeros(2)
         ranks = np.zeros(2 * max(2 * (n // max(ni, 1)) + 2, 2), np.int32)
-        fn = lib().hsdc_dhssmml if right else lib().hsdc_dhssmm
-        rc = fn(self._h, _dp(a), max(1, n), a.shape[1], n, _dp(b), max(1, b.shape[0]), k, tol, prank, ni, ncol,
-                _dp(t), _ip(ranks))
+        if seed is not None:
+            name = "hsdc_dhssmm_rand"
+            rc = lib().hsdc_dhssmm_rand(self._h, _dp(a), max(1, n), a.shape[1], n, _dp(b), max(1, b.shape[0]), k,
+                                        tol, ni, ncol, 1 if right else 0, ctypes.c_ulonglong(seed), _dp(t),
+                                        _ip(ranks))
+        else:
+            name = "hsdc_dhssmml" if right else "hsdc_dhssmm"
+            fn = lib().hsdc_dhssmml if right else lib().hsdc_dhssmm
+            rc = fn(self._h, _dp(a), max(1, n), a.shape[1], n, _dp(b), max(1, b.shape[0]), k, tol, prank, ni, ncol,
+                    _dp(t), _ip(ranks))
         if rc != 0:
-            raise RuntimeError("%s: %d" % ("hsdc_dhssmml" if right else "hsdc_dhssmm", rc))
+            raise RuntimeError("%s: %d" % (name, rc))
         return b, t, (None if ranks[0] < 0 else ranks)
 
     def hss_dense(self, k):

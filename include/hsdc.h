@@ -231,6 +231,18 @@ int hsdc_dhssmm(hsdc_ctx *ctx, const double *a, int lda, int m, int n, double *b
 /* DHSSMML (DHSSMML.f90:2-122): b(1:k, 1:n) := b A with the same HSS form of A; ldb >= k. */
 int hsdc_dhssmml(hsdc_ctx *ctx, const double *a, int lda, int m, int n, double *b, int ldb, int k, double tol,
                  int prank, int ni, int ncol, double *time, int *ranks);
+/* DHSSMM / DHSSMML with randomized range finding (mat2hssvdr, ConstructHssd.f90:8-368;
+ * the reference calls it only from its SVD-updating code, CompSvdup.f90:97, and has no
+ * driver of its own): a block row or column whose context is longer than 50 and than
+ * its own length m + 10 takes its basis from the SVD of the sketch Omega T (m + 10 rows)
+ * instead of T, the rank = the count of the sketch's singular values above tol.  Omega
+ * is read from one flat stream of uniform [0, 1) numbers as the reference reads its
+ * random_number array (leading dimension m + 10 for a block column, the context length
+ * for a block row); entry i of the stream is splitmix64 of seed + (i + 1) 0x9E3779B97F4A7C15,
+ * top 53 bits (the reference draws it unseeded).  side 0: b(1:n, 1:k) := A b (ldb >= n);
+ * 1: b(1:k, 1:n) := b A (ldb >= k).  Other arguments and INFO as hsdc_dhssmm; -12 side. */
+int hsdc_dhssmm_rand(hsdc_ctx *ctx, const double *a, int lda, int m, int n, double *b, int ldb, int k, double tol,
+                     int ni, int ncol, int side, unsigned long long seed, double *time, int *ranks);
 
 /* Contexts owned by the calling host thread, for callers that keep no handle (the
  * Fortran drop-ins): created on first use with the default parameters (reset to them

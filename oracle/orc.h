@@ -180,6 +180,9 @@ void orc_cauchy_dense_general(int n, const double *d, const double *f, const dou
  * basis the singular vectors above tol (absolute).  Returns -1 for a tree of one leaf,
  * 1 when a Jacobi SVD did not converge. */
 int orc_mat2hss(orc_hss *H, int n, int ni, const double *a, int lda, double tol);
+/* The same with mat2hssvdr's randomized range finding (ConstructHssd.f90:8-368), the
+ * uniform array drawn from the splitmix64 stream of seed. */
+int orc_mat2hss_rand(orc_hss *H, int n, int ni, const double *a, int lda, double tol, unsigned long long seed);
 
 /* ---- bidiagonal SVD (mdbdsdc.f90 / mdlasd0-2.f90 + LAPACK DLASD3/DLASD4/DLASDQ) ---- */
 /* LAPACK DLASCL 'G' on a vector, DLARTG, DLASDT (node arrays 0-based, values 1-based). */

@@ -1392,7 +1392,59 @@ static int right_basis(int w, int m, double *T, long ldt, double tol, double *v,
     return cnt;
 }
 
+/* Entry i of mat2hssvdr's flat array of uniform [0, 1) numbers: splitmix64 of
+ * seed + (i + 1) 0x9E3779B97F4A7C15, top 53 bits (the reference draws it unseeded with
+ * random_number, ConstructHssd.f90:100). */
+static double m2h_omega(unsigned long long seed, unsigned long long i)
+{
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1p-53;
+}
+
+/* The basis of one side: from T (w x m, context x own) itself, or -- mat2hssvdr's
+ * sampling test, a context longer than 50 and than m + 10 (ConstructHssd.f90:101, 115,
+ * 144, ...) -- from the sketch S = Omega T (m + 10 rows), Omega read from the flat
+ * array with leading dimension m + 10 for a block column (side 1, :104) and as the
+ * transpose of a w x (m + 10) array for a block row (side 0, :118). */
+static int side_basis(int w, int m, double *T, double tol, double *v, double *sv, int rsvd, unsigned long long seed,
+                      int side)
+{
+    if (!(rsvd && w > 50 && w > m + 10)) return right_basis(w, m, T, w > 0 ? w : 1, tol, v, sv);
+    const int pp = m + 10;
+    double *S = (double *)malloc(sizeof(double) * (size_t)pp * m);
+    for (int t = 0; t < m; t++)
+        for (int r = 0; r < pp; r++) {
+            double acc = 0.0;
+            for (int o = 0; o < w; o++) {
+                const unsigned long long idx = side ? (unsigned long long)r + (unsigned long long)o * pp
+                                                    : (unsigned long long)o + (unsigned long long)r * w;
+                acc += m2h_omega(seed, idx) * T[o + (long)t * w];
+            }
+            S[r + (long)t * pp] = acc;
+        }
+    const int rk = right_basis(pp, m, S, pp, tol, v, sv);
+    free(S);
+    return rk;
+}
+
+static int mat2hss_impl(orc_hss *H, int n, int ni, const double *A, int lda, double tol, int rsvd,
+                        unsigned long long seed);
+
 int orc_mat2hss(orc_hss *H, int n, int ni, const double *A, int lda, double tol)
+{
+    return mat2hss_impl(H, n, ni, A, lda, tol, 0, 0);
+}
+
+int orc_mat2hss_rand(orc_hss *H, int n, int ni, const double *A, int lda, double tol, unsigned long long seed)
+{
+    return mat2hss_impl(H, n, ni, A, lda, tol, 1, seed);
+}
+
+static int mat2hss_impl(orc_hss *H, int n, int ni, const double *A, int lda, double tol, int rsvd,
+                        unsigned long long seed)
 {
     if (hss_tree_setup(H, n, ni) < 1) return -1;
     const int ltr = H->ltr, lvl = H->lvl;
@@ -1466,7 +1518,7 @@ int orc_mat2hss(orc_hss *H, int n, int ni, const double *A, int lda, double tol)
                 }
             }
             double *v = (double *)malloc(sizeof(double) * rn * rn), *sv = (double *)malloc(sizeof(double) * rn);
-            ru[q] = right_basis(w1, rn, T, w1 > 0 ? w1 : 1, tol, v, sv);
+            ru[q] = side_basis(w1, rn, T, tol, v, sv, rsvd, seed, 0);
             if (ru[q] < 0) info = 1;
             Ub[q] = v;
             free(T); free(sv);
@@ -1482,7 +1534,7 @@ int orc_mat2hss(orc_hss *H, int n, int ni, const double *A, int lda, double tol)
             }
             v = (double *)malloc(sizeof(double) * cn * cn);
             sv = (double *)malloc(sizeof(double) * cn);
-            rw[q] = right_basis(w2, cn, T, w2 > 0 ? w2 : 1, tol, v, sv);
+            rw[q] = side_basis(w2, cn, T, tol, v, sv, rsvd, seed, 1);
             if (rw[q] < 0) info = 1;
             Wb[q] = v;
             free(T); free(sv);

@@ -330,6 +330,17 @@ def mat2hss(a, ni, tol):
     return H
 
 
+def mat2hss_rand(a, ni, tol, seed):
+    """mat2hss with mat2hssvdr's randomized range finding, Omega from the seed's stream."""
+    a = np.array(a, np.float64, order="F")
+    n = a.shape[0]
+    H = Hss()
+    info = lib().orc_mat2hss_rand(ctypes.byref(H), n, ni, dp(a), n, _d(tol), ctypes.c_ulonglong(seed))
+    if info != 0:
+        raise RuntimeError("orc_mat2hss_rand failed %d" % info)
+    return H
+
+
 def cauchy_dense_general(d, f, u, v):
     n = len(d)
     a = np.zeros((n, n), order="F")
@@ -406,6 +417,7 @@ def _setup():
     L.orc_hss_build.restype = _i
     L.orc_cauchy2hss.restype = _i
     L.orc_mat2hss.restype = _i
+    L.orc_mat2hss_rand.restype = _i
     L.orc_sbevd.restype = _i
     L.orc_sym_jacobi.restype = _i
     L.orc_jacobi_svd.restype = _i

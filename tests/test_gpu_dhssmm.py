@@ -8,6 +8,10 @@ against the oracle's restatement and the dense product.
   within tol sqrt(2 n lvl) |b_j| of A b; the same for b A (DHSSMML).
 * tol = 0 reproduces A b to rounding; diagonal plus rank one gives ranks <= 1; a tree of
   one leaf falls back to the dense product.
+* The randomized construction (mat2hssvdr, hsdc_dhssmm_rand): the same case with a
+  seed, every node's ranks within one of the oracle's randomized form (at most 1 node
+  in 20 differs) and the product within 1e-11 of it (both sides); tol = 0 reproduces
+  A b; a side other than 0 or 1 is -12.
 """
 import numpy as np
 import pytest
@@ -88,3 +92,56 @@ def test_one_leaf_is_dense(ctx):
     Yl, _, rl = ctx.dhssmm(A, Bl, 64, 1e-12, right=True)
     assert rl is None
     assert np.abs(Yl - Bl @ A).max() <= 1e-13 * np.abs(Bl @ A).max()
+
+
+# ---- randomized range finding (mat2hssvdr, ConstructHssd.f90:8-368) ----
+@pytest.mark.parametrize("right", [False, True])
+def test_rand_reference_case_vs_oracle(ctx, right):
+    """hsdc_dhssmm_rand on testfmm.f90's matrix: the device forms the same sketches as
+    the oracle (one uniform stream per seed), so the product agrees with the oracle's
+    randomized HSS form to rounding.  Ranks: Omega's entries are uniform on [0, 1), and
+    their mean (1/2) makes sigma_1 of the sketch 17 to 97 times that of the block on this
+    matrix (about 1e4), so eps sigma_1(S) is 0.15 to 0.37 tol: the sketch's singular
+    values near tol carry rounding of that size, and two computations that round
+    differently can count one of them on either side.  The ranks agree to within one, at
+    no more than 1 node in 20."""
+    n, ni, tol, seed = 2000, 64, 1e-11, 2024
+    A = fmm_matrix(n, 3)
+    B = np.random.default_rng(5).uniform(0, 1, (100, n) if right else (n, 100))
+    Y, t, ranks = ctx.dhssmm(A, B, ni, tol, ncol=64, right=right, seed=seed)
+    assert ranks is not None
+    H = O.mat2hss_rand(A, ni, tol, seed)
+    lvl, ltr = H.lvl, H.ltr
+    ou, ov = node_ranks(H)
+    Hd = O.hss_dense(H)
+    O.hss_free(H)
+    du, dv = dev_ranks(ranks, ltr)
+    ref = B @ Hd if right else Hd @ B
+    exact = B @ A if right else A @ B
+    err_orc = np.abs(Y - ref).max() / np.abs(ref).max()
+    err = np.abs(Y - exact).max() / np.abs(exact).max()
+    record("dhssmm_rand%s n=%d ni=%d tol %g seed %d" % ("l" if right else "", n, ni, tol, seed),
+           rank_max=int(max(du.max(), dv.max())), rel_err_vs_oracle=err_orc, rel_err_vs_dense=err,
+           construction_s=t[0], multiply_s=t[1], nodes_rank_differs=int(np.sum(du != ou) + np.sum(dv != ov)))
+    d = np.r_[du - ou, dv - ov]
+    assert np.abs(d).max() <= 1 and np.count_nonzero(d) * 20 <= len(d)
+    assert err_orc <= 1e-11
+    assert err <= 1e-7
+
+
+def test_rand_tol_zero_and_bad_side(ctx):
+    n = 300
+    rng = np.random.default_rng(6)
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 20))
+    Y, _, ranks = ctx.dhssmm(A, B, 32, 0.0, seed=7)
+    assert ranks is not None
+    assert np.abs(Y - A @ B).max() <= 1e-11 * np.abs(A @ B).max()
+    import ctypes
+    from hsspack_amd import hsdc
+    a = np.asfortranarray(A)
+    b = np.asfortranarray(B)
+    dp = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rc = hsdc.lib().hsdc_dhssmm_rand(ctx._h, dp(a), n, n, n, dp(b), n, 20, 1e-10, 32, 100, 2,
+                                     ctypes.c_ulonglong(1), None, None)
+    assert rc == -12

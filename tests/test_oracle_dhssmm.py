@@ -94,3 +94,86 @@ def test_multiply_matches_dense_expansion():
 def test_one_leaf_tree_is_refused():
     with pytest.raises(RuntimeError):
         O.mat2hss(np.eye(20), 32, 1e-12)
+
+
+# ---- randomized range finding (mat2hssvdr, ConstructHssd.f90:8-368) ----
+def leaf_u(H, node):
+    """Node's U generator (rows x rank) from the oracle's arena."""
+    rn, ru = (int(x) for x in O.hss_ranks(H)[0][node - 1])
+    off = int(H.pu[node - 1])
+    u = np.ctypeslib.as_array(H.h, (off + rn * ru,))[off:off + rn * ru].copy()
+    return u.reshape(ru, rn).T
+
+
+def test_rand_first_leaf_basis_is_the_sketchs():
+    """The first leaf's U from the sketch the reference forms (:118, Omega read as the
+    transpose of a w x (m + 10) array), rebuilt here from the same uniform stream
+    (workloads' splitmix64, stream (0, seed)): the same rank, the same singular
+    vectors up to sign."""
+    from hsspack_amd.workloads import splitmix_uniform
+    n, ni, tol, seed = 600, 40, 1e-10, 12345
+    A = fmm_matrix(n, 8)
+    H = O.mat2hss_rand(A, ni, tol, seed)
+    try:
+        m = int(O.hss_ranks(H)[0][0][0])
+        U = leaf_u(H, 1)
+    finally:
+        O.hss_free(H)
+    T = A[:m, m:].T  # block row of the first leaf, transposed: w x m
+    w, pp = T.shape[0], m + 10
+    om = splitmix_uniform(0, w * pp, sub=seed)
+    Omega = om.reshape(pp, w)  # entry (r, o) = om[o + r w]
+    S = Omega @ T
+    _, sv, vt = np.linalg.svd(S)
+    rk = int(np.sum(sv > tol))
+    assert U.shape == (m, rk)
+    V = vt[:rk].T
+    # a singular vector is fixed to about eps sigma_1 / sigma_j: weigh by the singular values
+    dots = np.abs(np.sum(U * V, axis=0))
+    assert np.all(np.abs(dots - 1.0) <= 1e-13 * sv[0] / sv[:rk])
+    assert np.abs((V - U @ (U.T @ V)) * sv[:rk]).max() <= 1e-13 * sv[0]
+
+
+def test_rand_tol_zero_reproduces_a():
+    n, ni = 300, 32
+    A = np.random.default_rng(11).standard_normal((n, n))
+    H = O.mat2hss_rand(A, ni, 0.0, 7)
+    Hd = O.hss_dense(H)
+    O.hss_free(H)
+    assert np.abs(Hd - A).max() <= 1e-11 * np.abs(A).max()
+
+
+def test_rand_low_rank_blocks_are_captured():
+    """Off-diagonal blocks of rank 3: a sketch of m + 10 rows holds them exactly."""
+    n, ni = 500, 40
+    rng = np.random.default_rng(12)
+    A = np.diag(rng.standard_normal(n)) + rng.standard_normal((n, 3)) @ rng.standard_normal((3, n))
+    H = O.mat2hss_rand(A, ni, 1e-9, 99)
+    ru, rv = node_ranks(H)
+    Hd = O.hss_dense(H)
+    O.hss_free(H)
+    assert ru.max() <= 3 and rv.max() <= 3
+    assert np.abs(Hd - A).max() <= 1e-10 * np.abs(A).max()
+
+
+def test_rand_reference_case_product():
+    """testfmm.f90's matrix (n = 2000, ni = 64): the randomized form's product stays
+    within 1e-7 of A B at tol 1e-11, falls with tol, and one seed always gives the same
+    ranks."""
+    n, ni = 2000, 64
+    A = fmm_matrix(n, 3)
+    B = np.random.default_rng(4).uniform(0, 1, (n, 64))
+    ref = A @ B
+    errs = []
+    for tol in (1e-9, 1e-11):
+        H = O.mat2hss_rand(A, ni, tol, 2024)
+        ru, rv = node_ranks(H)
+        Hd = O.hss_dense(H)
+        O.hss_free(H)
+        errs.append(np.abs(Hd @ B - ref).max() / np.abs(ref).max())
+        H2 = O.mat2hss_rand(A, ni, tol, 2024)
+        ru2, rv2 = node_ranks(H2)
+        O.hss_free(H2)
+        assert np.array_equal(ru, ru2) and np.array_equal(rv, rv2)
+        assert 0 < ru.max() < ni
+    assert errs[1] <= errs[0] and errs[1] <= 1e-7, errs

@@ -20,18 +20,61 @@
 #define WG 256
 #define A1(a, i) (a)[(i) - 1]
 
+// Pointers typed by address space: global memory (1) and LDS (3).  Loads through an
+// untyped pointer compile to FLAT instructions, which the compiler must order against
+// every LDS access and every store; typed loads and stores let it issue the loads of
+// many columns before their results are needed.
+typedef const __attribute__((address_space(1))) double gdbl;
+typedef __attribute__((address_space(1))) double gdblw;
+typedef __attribute__((address_space(1))) int gint;
+typedef const __attribute__((address_space(1))) int gcint;
+typedef __attribute__((address_space(3))) double ldbl;
+
 // ---------------------------------------------------------------------------
 // accurate differences: f_{PL(j)} - lambda_{PU(k)} and lambda_{PU(j)} - lambda_{PU(k)}
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double fd(const HssMerge &H, int pd, int pw)
 {
-    if (pd <= pw) return (A1(H.FF, pd) - A1(H.FF, pw)) - A1(H.DIFL, pw);
-    return (A1(H.FF, pd) - A1(H.FF, pw + 1)) - A1(H.DIFR, pw);
+    gdbl *FF = (gdbl *)H.FF;
+    if (pd <= pw) return (A1(FF, pd) - A1(FF, pw)) - A1((gdbl *)H.DIFL, pw);
+    return (A1(FF, pd) - A1(FF, pw + 1)) - A1((gdbl *)H.DIFR, pw);
 }
 __device__ __forceinline__ double dd(const HssMerge &H, int pj, int pk)
 {
-    if (pj < pk) return ((A1(H.FF, pj + 1) - A1(H.FF, pk)) + A1(H.DIFR, pj)) - A1(H.DIFL, pk);
-    return ((A1(H.FF, pj) - A1(H.FF, pk + 1)) + A1(H.DIFL, pj)) - A1(H.DIFR, pk);
+    gdbl *FF = (gdbl *)H.FF;
+    if (pj < pk) return ((A1(FF, pj + 1) - A1(FF, pk)) + A1((gdbl *)H.DIFR, pj)) - A1((gdbl *)H.DIFL, pk);
+    return ((A1(FF, pj) - A1(FF, pk + 1)) + A1((gdbl *)H.DIFL, pj)) - A1((gdbl *)H.DIFR, pk);
+}
+
+// The same differences from values gathered once per root: a pass over many columns
+// loads, for each column's root p, FF(p), FF(p + 1), DIFL(p), DIFR(p) (and D0(p) for the
+// squared form) together, then forms every difference it needs from registers.  The
+// operations and their order are those of fd / dd above, so the results are identical.
+struct RootG { double f0, f1, dl, dr, s; };
+template <bool SQ>
+__device__ __forceinline__ RootG root_load(const HssMerge &H, int p)
+{
+    RootG r;
+    r.f0 = A1((gdbl *)H.FF, p);
+    r.f1 = A1((gdbl *)H.FF, p < H.K ? p + 1 : p);  // FF(p + 1) is read only for p < K
+    r.dl = A1((gdbl *)H.DIFL, p);
+    r.dr = A1((gdbl *)H.DIFR, p);
+    r.s = SQ ? A1((gdbl *)H.D0, p) : 0.0;
+    return r;
+}
+// fdq(pd, pw) with ffd = FF(pd) and w = root_load(pw)
+template <bool SQ>
+__device__ __forceinline__ double fd_r(double ffd, int pd, int pw, const RootG &w)
+{
+    const double r = (pd <= pw) ? (ffd - w.f0) - w.dl : (ffd - w.f1) - w.dr;
+    return SQ ? r * (ffd + w.s) : r;
+}
+// ddq(pj, pk) with j = root_load(pj), k = root_load(pk)
+template <bool SQ>
+__device__ __forceinline__ double dd_r(int pj, const RootG &j, int pk, const RootG &k)
+{
+    const double r = (pj < pk) ? ((j.f1 - k.f0) + j.dr) - k.dl : ((j.f0 - k.f1) + j.dl) - k.dr;
+    return SQ ? r * (j.s + k.s) : r;
 }
 // The SVD's vector matrices u_i v_j / (f_i^2 - sigma_j^2) are Cauchy-like in the
 // squared variables: with SQ every displacement is the accurate difference times the
@@ -40,13 +83,13 @@ template <bool SQ>
 __device__ __forceinline__ double fdq(const HssMerge &H, int pd, int pw)
 {
     const double r = fd(H, pd, pw);
-    return SQ ? r * (A1(H.FF, pd) + A1(H.D0, pw)) : r;
+    return SQ ? r * (A1((gdbl *)H.FF, pd) + A1((gdbl *)H.D0, pw)) : r;
 }
 template <bool SQ>
 __device__ __forceinline__ double ddq(const HssMerge &H, int pj, int pk)
 {
     const double r = dd(H, pj, pk);
-    return SQ ? r * (A1(H.D0, pj) + A1(H.D0, pk)) : r;
+    return SQ ? r * (A1((gdbl *)H.D0, pj) + A1((gdbl *)H.D0, pk)) : r;
 }
 template <bool SQ>
 __device__ __forceinline__ double ppq(double a, double b)
@@ -150,7 +193,11 @@ __device__ long scratch_take(const HssMerge &H, long n)
 // ---------------------------------------------------------------------------
 // Row compression of one node: rrluCauchyEigrow + CauchylikeEigQ (ComprcauchyEig 'r')
 // rows: the node's M row generators (LDS); columns: the N column generators
-// outside the node (private global copies).
+// outside the node (private copies: V in LDS when it fits, else global; PU global).
+// The passes over the N columns take RB columns per thread at a time (8 for 512
+// threads, 4 for 256, whose launches put several workgroups on a CU): the loads of all
+// RB (V, PU, then the roots' gathered values) are issued before any result is used,
+// and the stores of a batch come after its loads.
 // ---------------------------------------------------------------------------
 template <int HSS_CWG, bool SQ>
 __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, const HssNode *nodes, int nn,
@@ -166,6 +213,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
     __shared__ double s_nmax;
     const int tid = threadIdx.x;
+    constexpr int RB = HSS_CWG >= 512 ? 8 : 4;
     if ((int)blockIdx.x >= nn) return;
     HssNode Nd = nodes[blockIdx.x];
     const HssMerge H = hm[Nd.mrg];
@@ -192,23 +240,55 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     // private per column: the generator V (updated every step; in LDS when it fits) and
     // the root index; the poles and differences its accurate differences read are
     // shared by every node of the merge and gathered by index
-    double *V = (N <= lds_cap) ? hss_dyn : H.scratch + s_off;
-    int *PU = (int *)(H.scratch + s_off + NS);
+    const bool vl = N <= lds_cap;  // uniform over the block
+    ldbl *Vl = (ldbl *)hss_dyn;
+    gdblw *Vg = (gdblw *)(H.scratch + s_off);
+    gint *PU = (gint *)(H.scratch + s_off + NS);
+    auto vld = [&](int j) -> double { return vl ? A1(Vl, j) : A1(Vg, j); };
+    auto vst = [&](int j, double x) {
+        if (vl) A1(Vl, j) = x;
+        else A1(Vg, j) = x;
+    };
     for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < c0 - 1) ? j + 1 : j + 1 + (c1e - c0 + 1);
-        V[j] = A1(H.V, src);
-        PU[j] = A1(H.PU, src);
+        vst(j + 1, A1((gdbl *)H.V, src));
+        PU[j] = A1((gcint *)H.PU, src);
     }
-    auto fdj = [&](int pd, int j) -> double { return fdq<SQ>(H, pd, A1(PU, j)); };
-    auto ddj = [&](int j, int pk) -> double { return ddq<SQ>(H, A1(PU, j), pk); };
     auto swapc = [&](int a, int c) {
-        double t = A1(V, a); A1(V, a) = A1(V, c); A1(V, c) = t;
+        double t = vld(a); vst(a, vld(c)); vst(c, t);
         int u = A1(PU, a); A1(PU, a) = A1(PU, c); A1(PU, c) = u;
     };
+    // one batch of RB columns j0, j0 + HSS_CWG, ... (clamped to N for the loads)
+    auto cload = [&](int j0, int *pw, double *v, RootG *c) {
+        #pragma unroll
+        for (int u = 0; u < RB; u++) {
+            const int j = min(j0 + u * HSS_CWG, N);
+            pw[u] = A1(PU, j);
+            v[u] = vld(j);
+        }
+        #pragma unroll
+        for (int u = 0; u < RB; u++) c[u] = root_load<SQ>(H, pw[u]);
+    };
+    // max over j in [k, N] of |V(j) / fd(pd, PU(j))| (index j - k + 1), ffd = FF(pd)
+    auto colsearch = [&](int k, int pd, double ffd) -> VI {
+        VI y = {0.0, 0};
+        for (int j0 = k + tid; j0 <= N; j0 += RB * HSS_CWG) {
+            int pw[RB];
+            double v[RB];
+            RootG c[RB];
+            cload(j0, pw, v, c);
+            #pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int j = j0 + u * HSS_CWG;
+                if (j <= N) y = vi_better(y, VI{fabs(v[u] / fd_r<SQ>(ffd, pd, pw[u], c[u])), j - k + 1});
+            }
+        }
+        return y;
+    };
     for (int i = tid; i < M; i += HSS_CWG) {
-        PL[i] = A1(H.PL, l0 + i);
-        A[i] = A1(H.U, l0 + i);
-        F[i] = A1(H.FF, PL[i]);
+        PL[i] = A1((gcint *)H.PL, l0 + i);
+        A[i] = A1((gdbl *)H.U, l0 + i);
+        F[i] = A1((gdbl *)H.FF, PL[i]);
         Z[i] = A[i];
         PQ[i] = i + 1;
         W[i] = 0.0;
@@ -229,26 +309,19 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         fl += 4ull * (M + N - 2 * k + 3);
         {
             VI x = {0.0, 0};
-            int pwk = A1(PU, k);
-            for (int i = k + tid; i <= M; i += HSS_CWG) {
-                int pli = A1(PL, i);
-                double v = fabs(A1(Z, i) / fdq<SQ>(H, pli, pwk));
-                x = vi_better(x, VI{v, i - k + 1});
+            {
+                const int pwk = A1(PU, k);
+                const RootG cw = root_load<SQ>(H, pwk);
+                for (int i = k + tid; i <= M; i += HSS_CWG) {
+                    double v = fabs(A1(Z, i) / fd_r<SQ>(A1(F, i), A1(PL, i), pwk, cw));
+                    x = vi_better(x, VI{v, i - k + 1});
+                }
             }
             VI cl = block_argmax_t(x, red, tog);
             VI rw = pre_rw;  // computed by the previous step's update pass when pre
-            if (!pre) {
-                VI y = {0.0, 0};
-                int fk = A1(PL, k);
-                #pragma unroll 8
-                for (int j = k + tid; j <= N; j += HSS_CWG) {
-                    double v = fabs(A1(V, j) / fdj(fk, j));
-                    y = vi_better(y, VI{v, j - k + 1});
-                }
-                rw = block_argmax_t(y, red, tog);
-            }
-            double junkL = cl.v * fabs(A1(V, k)), junkU = rw.v * fabs(A1(Z, k));
-            double piv = fabs(A1(Z, k) * A1(V, k) / fdq<SQ>(H, A1(PL, k), A1(PU, k)));
+            if (!pre) rw = block_argmax_t(colsearch(k, A1(PL, k), A1(F, k)), red, tog);
+            double junkL = cl.v * fabs(vld(k)), junkU = rw.v * fabs(A1(Z, k));
+            double piv = fabs(A1(Z, k) * vld(k) / fdq<SQ>(H, A1(PL, k), A1(PU, k)));
             if (!(junkL <= piv && junkU <= piv)) {
                 int flg = (junkL > junkU) ? 1 : 0;
                 int jjL = cl.i, jjU = rw.i;
@@ -265,14 +338,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                             u = A1(PQ, k); A1(PQ, k) = A1(PQ, jj); A1(PQ, jj) = u;
                         }
                         __syncthreads();
-                        VI y2 = {0.0, 0};
-                        int fk2 = A1(PL, k);
-                        #pragma unroll 8
-                        for (int j = k + tid; j <= N; j += HSS_CWG) {
-                            double v = fabs(A1(V, j) / fdj(fk2, j));
-                            y2 = vi_better(y2, VI{v, j - k + 1});
-                        }
-                        jjU = block_argmax_t(y2, red, tog).i;
+                        jjU = block_argmax_t(colsearch(k, A1(PL, k), A1(F, k)), red, tog).i;
                         fl += 4ull * (N - k + 1);
                         if (jjU == 1) break;
                         flg = 0;
@@ -282,10 +348,13 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     if (tid == 0) swapc(k, jj);
                     __syncthreads();
                     VI x2 = {0.0, 0};
-                    int pwk2 = A1(PU, k);
-                    for (int i = k + tid; i <= M; i += HSS_CWG) {
-                        double v = fabs(A1(Z, i) / fdq<SQ>(H, A1(PL, i), pwk2));
-                        x2 = vi_better(x2, VI{v, i - k + 1});
+                    {
+                        const int pwk2 = A1(PU, k);
+                        const RootG cw = root_load<SQ>(H, pwk2);
+                        for (int i = k + tid; i <= M; i += HSS_CWG) {
+                            double v = fabs(A1(Z, i) / fd_r<SQ>(A1(F, i), A1(PL, i), pwk2, cw));
+                            x2 = vi_better(x2, VI{v, i - k + 1});
+                        }
                     }
                     jjL = block_argmax_t(x2, red, tog).i;
                     fl += 4ull * (M - k + 1);
@@ -296,25 +365,40 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         }
         // every swap of the search is followed by a barrier and the search only reads, so
         // the update pass below may start without another
-        double ukk = A1(Z, k) * A1(V, k) / fdq<SQ>(H, A1(PL, k), A1(PU, k));
+        double ukk = A1(Z, k) * vld(k) / fdq<SQ>(H, A1(PL, k), A1(PU, k));
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
             // ---- CauchyPivtEig_CP: complete pivoting over the Schur complement
             fl += 4ull * (M - k + 1) * (N - k + 1);
             VI best = {0.0, 0};
             VI bestrow = {0.0, 0};
-            for (int j = k + tid; j <= N; j += HSS_CWG) {
-                double mv = -DBL_MAX;
-                int mi = 0;
+            for (int j0 = k + tid; j0 <= N; j0 += RB * HSS_CWG) {
+                int pw[RB];
+                double v[RB], mv[RB];
+                int mi[RB];
+                RootG c[RB];
+                cload(j0, pw, v, c);
+                #pragma unroll
+                for (int u = 0; u < RB; u++) { mv[u] = -DBL_MAX; mi[u] = 0; }
                 for (int i = k; i <= M; i++) {
-                    double v = fabs(A1(Z, i) / fdj(A1(PL, i), j));
-                    if (mi == 0 || v > mv) { mv = v; mi = i - k + 1; }
+                    const double zi = A1(Z, i), fi = A1(F, i);
+                    const int pli = A1(PL, i);
+                    #pragma unroll
+                    for (int u = 0; u < RB; u++) {
+                        double vv = fabs(zi / fd_r<SQ>(fi, pli, pw[u], c[u]));
+                        if (mi[u] == 0 || vv > mv[u]) { mv[u] = vv; mi[u] = i - k + 1; }
+                    }
                 }
-                double jt = fabs(mv * A1(V, j));
-                VI c = {jt, j - k + 1};
-                VI nb = vi_better(best, c);
-                if (nb.i == c.i) bestrow.i = mi;
-                best = nb;
+                #pragma unroll
+                for (int u = 0; u < RB; u++) {
+                    const int j = j0 + u * HSS_CWG;
+                    if (j > N) continue;
+                    double jt = fabs(mv[u] * v[u]);
+                    VI cc = {jt, j - k + 1};
+                    VI nb = vi_better(best, cc);
+                    if (nb.i == cc.i) bestrow.i = mi[u];
+                    best = nb;
+                }
             }
             VI win = block_argmax_t(best, red, tog);
             __syncthreads();
@@ -332,7 +416,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 swapc(k, b);
             }
             __syncthreads();
-            ukk = A1(Z, k) * A1(V, k) / fdq<SQ>(H, A1(PL, k), A1(PU, k));
+            ukk = A1(Z, k) * vld(k) / fdq<SQ>(H, A1(PL, k), A1(PU, k));
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
         }
@@ -340,28 +424,40 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         // follows, the pass over the columns also runs step k+1's first column search
         const double Fk = A1(F, k);
         const int plk = A1(PL, k), puk = A1(PU, k);
+        const RootG ck = root_load<SQ>(H, puk);
         for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
-            double temp = fdq<SQ>(H, A1(PL, j), puk);
+            double temp = fd_r<SQ>(A1(F, j), A1(PL, j), puk, ck);
             temp = ppq<SQ>(A1(F, j), Fk) / temp;
             A1(Z, j) = A1(Z, j) * temp;
         }
         const bool fuse = k + 1 <= mn && !(k % 10 == 0 && M - k > 0);
         const int fk1 = fuse ? A1(PL, k + 1) : 0;
+        const double ff1 = fuse ? A1(F, k + 1) : 0.0;
         VI y1 = {0.0, 0};
-        #pragma unroll 8
-        for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
-            double temp = ddj(j, puk);
-            double temp2 = -1.0 * fdj(plk, j);
-            temp = temp / temp2;
-            const double vj = A1(V, j) * temp;
-            A1(V, j) = vj;
-            if (fuse) y1 = vi_better(y1, VI{fabs(vj / fdj(fk1, j)), j - k});
+        for (int j0 = k + 1 + tid; j0 <= N; j0 += RB * HSS_CWG) {
+            int pw[RB];
+            double v[RB];
+            RootG c[RB];
+            cload(j0, pw, v, c);
+            #pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int j = j0 + u * HSS_CWG;
+                double temp = dd_r<SQ>(pw[u], c[u], puk, ck);
+                double temp2 = -1.0 * fd_r<SQ>(Fk, plk, pw[u], c[u]);
+                temp = temp / temp2;
+                const double vj = v[u] * temp;
+                if (j <= N) {
+                    vst(j, vj);
+                    if (fuse) y1 = vi_better(y1, VI{fabs(vj / fd_r<SQ>(ff1, fk1, pw[u], c[u])), j - k});
+                }
+            }
         }
         for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
-            double temp = -1.0 * fdq<SQ>(H, A1(PL, j), puk);
+            double temp = -1.0 * fd_r<SQ>(A1(F, j), A1(PL, j), puk, ck);
             A1(W, j) = A1(W, j) * (temp / ppq<SQ>(Fk, A1(F, j)));
             // the factors of W(k) (their gathers in parallel; the product stays in order)
-            const double t2 = -1.0 * fdq<SQ>(H, plk, A1(PU, j));
+            const int puj = A1(PU, j);
+            const double t2 = -1.0 * fd_r<SQ>(Fk, plk, puj, root_load<SQ>(H, puj));
             A1(T1, j) = t2 / ppq<SQ>(A1(F, j), Fk);
         }
         fl += 5ull * (M - k) + 7ull * (N - k) + 12ull * (k - 1);
@@ -372,7 +468,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         if (fuse) pre_rw = block_argmax_t(y1, red, tog);
         else __syncthreads();
         if (tid == 0) {
-            double wk = fdq<SQ>(H, plk, puk) / A1(A, k);
+            double wk = fd_r<SQ>(Fk, plk, puk, ck) / A1(A, k);
             for (int j = 1; j <= k - 1; j++) wk = wk * A1(T1, j);
             A1(W, k) = wk;
         }
@@ -401,12 +497,19 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 __syncthreads();
                 const double Fii = A1(F, ii), Fjj = A1(F, jj);
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
-                #pragma unroll 8
-                for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
-                    double temp = -1.0 * fdj(plii, j);
-                    double temp2 = -1.0 * fdj(pljj, j);
-                    temp = temp / temp2;
-                    A1(V, j) = A1(V, j) * temp;
+                for (int j0 = k + 1 + tid; j0 <= N; j0 += RB * HSS_CWG) {
+                    int pw[RB];
+                    double v[RB];
+                    RootG c[RB];
+                    cload(j0, pw, v, c);
+                    #pragma unroll
+                    for (int u = 0; u < RB; u++) {
+                        const int j = j0 + u * HSS_CWG;
+                        double temp = -1.0 * fd_r<SQ>(Fii, plii, pw[u], c[u]);
+                        double temp2 = -1.0 * fd_r<SQ>(Fjj, pljj, pw[u], c[u]);
+                        temp = temp / temp2;
+                        if (j <= N) vst(j, v[u] * temp);
+                    }
                 }
                 for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
                     if (j == jj) continue;
@@ -487,7 +590,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
 // ---------------------------------------------------------------------------
 // Column compression of one node: rrluCauchyEigcol + CauchylikeEigQ2
 // (ComprcauchyEig 'c').  RRLU rows: the node's M columns (roots, alpha, LDS);
-// RRLU columns: the N row generators outside the node (private global copies).
+// RRLU columns: the N row generators outside the node (private copies, as in the row
+// compression; their passes take RB columns per thread at a time the same way).
 // ---------------------------------------------------------------------------
 template <int HSS_CWG, bool SQ>
 __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, const HssNode *nodes, int nn,
@@ -502,6 +606,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     __shared__ long s_off;
     __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
     const int tid = threadIdx.x;
+    constexpr int RB = HSS_CWG >= 512 ? 8 : 4;
     if ((int)blockIdx.x >= nn) return;
     HssNode Nd = nodes[blockIdx.x];
     const HssMerge H = hm[Nd.mrg];
@@ -521,17 +626,52 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     const int M = s_M, N = s_N, c0 = lc[2 * (node - 1)], r0 = lr[2 * (node - 1)], r1e = lr[2 * (node - 1) + 1];
     // private per outside row: its generator (updated every step; in LDS when it fits)
     // and pole index; the pole value is gathered from the merge's shared array
-    double *V = (N <= lds_cap) ? hss_dyn : H.scratch + s_off;
-    int *PU = (int *)(H.scratch + s_off + (N > 0 ? N : 1));
+    gdbl *FF = (gdbl *)H.FF;
+    const bool vl = N <= lds_cap;  // uniform over the block
+    ldbl *Vl = (ldbl *)hss_dyn;
+    gdblw *Vg = (gdblw *)(H.scratch + s_off);
+    gint *PU = (gint *)(H.scratch + s_off + (N > 0 ? N : 1));
+    auto vld = [&](int j) -> double { return vl ? A1(Vl, j) : A1(Vg, j); };
+    auto vst = [&](int j, double x) {
+        if (vl) A1(Vl, j) = x;
+        else A1(Vg, j) = x;
+    };
     for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < r0 - 1) ? j + 1 : j + 1 + (r1e - r0 + 1);
-        V[j] = A1(H.U, src);
-        PU[j] = A1(H.PL, src);
+        vst(j + 1, A1((gdbl *)H.U, src));
+        PU[j] = A1((gcint *)H.PL, src);
     }
+    // one batch of RB outside rows j0, j0 + HSS_CWG, ... (clamped to N for the loads):
+    // pole index, generator and pole value
+    auto cload = [&](int j0, int *pd, double *v, double *ffd) {
+        #pragma unroll
+        for (int u = 0; u < RB; u++) {
+            const int j = min(j0 + u * HSS_CWG, N);
+            pd[u] = A1(PU, j);
+            v[u] = vld(j);
+        }
+        #pragma unroll
+        for (int u = 0; u < RB; u++) ffd[u] = A1(FF, pd[u]);
+    };
+    // max over j in [k, N] of |V(j) / fd(PU(j), pw)| (index j - k + 1), w = root_load(pw)
+    auto colsearch = [&](int k, int pw, const RootG &w) -> VI {
+        VI y = {0.0, 0};
+        for (int j0 = k + tid; j0 <= N; j0 += RB * HSS_CWG) {
+            int pd[RB];
+            double v[RB], ffd[RB];
+            cload(j0, pd, v, ffd);
+            #pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int j = j0 + u * HSS_CWG;
+                if (j <= N) y = vi_better(y, VI{fabs(v[u] / fd_r<SQ>(ffd[u], pd[u], pw, w)), j - k + 1});
+            }
+        }
+        return y;
+    };
     for (int i = tid; i < M; i += HSS_CWG) {
-        Dr[i] = A1(H.D, c0 + i);
-        A[i] = A1(H.V, c0 + i);
-        PL[i] = A1(H.PU, c0 + i);
+        Dr[i] = A1((gdbl *)H.D, c0 + i);
+        A[i] = A1((gdbl *)H.V, c0 + i);
+        PL[i] = A1((gcint *)H.PU, c0 + i);
         Z[i] = A[i];
         PQ[i] = i + 1;
         W[i] = 0.0;
@@ -551,31 +691,29 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         fl += 4ull * (M + N - 2 * k + 3);
         {
             VI x = {0.0, 0};
-            int puk = A1(PU, k);
-            for (int i = k + tid; i <= M; i += HSS_CWG) {
-                double v = fabs(A1(Z, i) / fdq<SQ>(H, puk, A1(PL, i)));
-                x = vi_better(x, VI{v, i - k + 1});
+            {
+                const int puk = A1(PU, k);
+                const double ffk = A1(FF, puk);
+                for (int i = k + tid; i <= M; i += HSS_CWG) {
+                    const int pli = A1(PL, i);
+                    double v = fabs(A1(Z, i) / fd_r<SQ>(ffk, puk, pli, root_load<SQ>(H, pli)));
+                    x = vi_better(x, VI{v, i - k + 1});
+                }
             }
             VI cl = block_argmax_t(x, red, tog);
             VI rw = pre_rw;  // computed by the previous step's update pass when pre
             if (!pre) {
-                VI y = {0.0, 0};
-                int plk = A1(PL, k);
-                #pragma unroll 8
-                for (int j = k + tid; j <= N; j += HSS_CWG) {
-                    double v = fabs(A1(V, j) / fdq<SQ>(H, A1(PU, j), plk));
-                    y = vi_better(y, VI{v, j - k + 1});
-                }
-                rw = block_argmax_t(y, red, tog);
+                const int plk = A1(PL, k);
+                rw = block_argmax_t(colsearch(k, plk, root_load<SQ>(H, plk)), red, tog);
             }
-            double junkL = cl.v * fabs(A1(V, k)), junkU = rw.v * fabs(A1(Z, k));
+            double junkL = cl.v * fabs(vld(k)), junkU = rw.v * fabs(A1(Z, k));
             double den;
             if (A1(PL, k) < A1(PU, k))
-                den = (A1(H.FF, A1(PL, k) + 1) - A1(H.FF, A1(PU, k))) + A1(H.DIFR, A1(PL, k));
+                den = (A1(FF, A1(PL, k) + 1) - A1(FF, A1(PU, k))) + A1((gdbl *)H.DIFR, A1(PL, k));
             else
-                den = (A1(H.FF, A1(PL, k)) - A1(H.FF, A1(PU, k))) + A1(H.DIFL, A1(PL, k));
-            if (SQ) den = den * (A1(H.D0, A1(PL, k)) + A1(H.FF, A1(PU, k)));
-            double piv = fabs(A1(Z, k) * A1(V, k) / den);
+                den = (A1(FF, A1(PL, k)) - A1(FF, A1(PU, k))) + A1((gdbl *)H.DIFL, A1(PL, k));
+            if (SQ) den = den * (A1((gdbl *)H.D0, A1(PL, k)) + A1(FF, A1(PU, k)));
+            double piv = fabs(A1(Z, k) * vld(k) / den);
             if (!(junkL <= piv && junkU <= piv)) {
                 int flg = (junkL > junkU) ? 1 : 0;
                 int jjL = cl.i, jjU = rw.i;
@@ -592,14 +730,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                             u = A1(PQ, k); A1(PQ, k) = A1(PQ, jj); A1(PQ, jj) = u;
                         }
                         __syncthreads();
-                        VI y2 = {0.0, 0};
-                        int plk2 = A1(PL, k);
-                        #pragma unroll 8
-                        for (int j = k + tid; j <= N; j += HSS_CWG) {
-                            double v = fabs(A1(V, j) / fdq<SQ>(H, A1(PU, j), plk2));
-                            y2 = vi_better(y2, VI{v, j - k + 1});
-                        }
-                        jjU = block_argmax_t(y2, red, tog).i;
+                        const int plk2 = A1(PL, k);
+                        jjU = block_argmax_t(colsearch(k, plk2, root_load<SQ>(H, plk2)), red, tog).i;
                         fl += 4ull * (N - k + 1);
                         if (jjU == 1) break;
                         flg = 0;
@@ -607,16 +739,19 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     int jj = jjU + k - 1;
                     __syncthreads();
                     if (tid == 0) {
-                        double t;
-                        t = A1(V, k); A1(V, k) = A1(V, jj); A1(V, jj) = t;
+                        double t = vld(k); vst(k, vld(jj)); vst(jj, t);
                         int u = A1(PU, k); A1(PU, k) = A1(PU, jj); A1(PU, jj) = u;
                     }
                     __syncthreads();
                     VI x2 = {0.0, 0};
-                    int puk2 = A1(PU, k);
-                    for (int i = k + tid; i <= M; i += HSS_CWG) {
-                        double v = fabs(A1(Z, i) / fdq<SQ>(H, puk2, A1(PL, i)));
-                        x2 = vi_better(x2, VI{v, i - k + 1});
+                    {
+                        const int puk2 = A1(PU, k);
+                        const double ffk2 = A1(FF, puk2);
+                        for (int i = k + tid; i <= M; i += HSS_CWG) {
+                            const int pli = A1(PL, i);
+                            double v = fabs(A1(Z, i) / fd_r<SQ>(ffk2, puk2, pli, root_load<SQ>(H, pli)));
+                            x2 = vi_better(x2, VI{v, i - k + 1});
+                        }
                     }
                     jjL = block_argmax_t(x2, red, tog).i;
                     fl += 4ull * (M - k + 1);
@@ -627,26 +762,39 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         }
         // every swap of the search is followed by a barrier and the search only reads, so
         // the update pass below may start without another
-        double ukk = A1(Z, k) * A1(V, k) / (-1.0 * fdq<SQ>(H, A1(PU, k), A1(PL, k)));
+        double ukk = A1(Z, k) * vld(k) / (-1.0 * fdq<SQ>(H, A1(PU, k), A1(PL, k)));
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
             // ---- CauchyPivtEigm_CP
             fl += 4ull * (M - k + 1) * (N - k + 1);
             VI best = {0.0, 0};
             int bestrow = 0;
-            #pragma unroll 8
-            for (int j = k + tid; j <= N; j += HSS_CWG) {
-                int pu = A1(PU, j);
-                double mv = 0.0;
-                int mi = 0;
+            for (int j0 = k + tid; j0 <= N; j0 += RB * HSS_CWG) {
+                int pd[RB];
+                double v[RB], ffd[RB], mv[RB];
+                int mi[RB];
+                cload(j0, pd, v, ffd);
+                #pragma unroll
+                for (int u = 0; u < RB; u++) { mv[u] = 0.0; mi[u] = 0; }
                 for (int i = k; i <= M; i++) {
-                    double v = fabs(A1(Z, i) / fdq<SQ>(H, pu, A1(PL, i)));
-                    if (mi == 0 || v > mv) { mv = v; mi = i - k + 1; }
+                    const double zi = A1(Z, i);
+                    const int pli = A1(PL, i);
+                    const RootG ri = root_load<SQ>(H, pli);
+                    #pragma unroll
+                    for (int u = 0; u < RB; u++) {
+                        double vv = fabs(zi / fd_r<SQ>(ffd[u], pd[u], pli, ri));
+                        if (mi[u] == 0 || vv > mv[u]) { mv[u] = vv; mi[u] = i - k + 1; }
+                    }
                 }
-                VI c = {fabs(mv * A1(V, j)), j - k + 1};
-                VI nb = vi_better(best, c);
-                if (nb.i == c.i) bestrow = mi;
-                best = nb;
+                #pragma unroll
+                for (int u = 0; u < RB; u++) {
+                    const int j = j0 + u * HSS_CWG;
+                    if (j > N) continue;
+                    VI c = {fabs(mv[u] * v[u]), j - k + 1};
+                    VI nb = vi_better(best, c);
+                    if (nb.i == c.i) bestrow = mi[u];
+                    best = nb;
+                }
             }
             VI win = block_argmax_t(best, red, tog);
             if (win.i > 0 && best.i == win.i) { s_ii = bestrow; s_jj = win.i; }
@@ -659,40 +807,54 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 t = A1(A, k); A1(A, k) = A1(A, a); A1(A, a) = t;
                 int u = A1(PL, k); A1(PL, k) = A1(PL, a); A1(PL, a) = u;
                 u = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = u;
-                t = A1(V, k); A1(V, k) = A1(V, b); A1(V, b) = t;
+                t = vld(k); vst(k, vld(b)); vst(b, t);
                 u = A1(PU, k); A1(PU, k) = A1(PU, b); A1(PU, b) = u;
             }
             __syncthreads();
-            ukk = A1(Z, k) * A1(V, k) / (-1.0 * fdq<SQ>(H, A1(PU, k), A1(PL, k)));
+            ukk = A1(Z, k) * vld(k) / (-1.0 * fdq<SQ>(H, A1(PU, k), A1(PL, k)));
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
         }
-        const double Fk = A1(H.FF, A1(PU, k));
         const int plk = A1(PL, k), puk = A1(PU, k);
+        const double Fk = A1(FF, puk);
+        const RootG ck = root_load<SQ>(H, plk);
         for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
-            double temp = ddq<SQ>(H, A1(PL, j), plk);
-            double temp2 = -1.0 * fdq<SQ>(H, puk, A1(PL, j));
+            const int plj = A1(PL, j);
+            const RootG rj = root_load<SQ>(H, plj);
+            double temp = dd_r<SQ>(plj, rj, plk, ck);
+            double temp2 = -1.0 * fd_r<SQ>(Fk, puk, plj, rj);
             temp = temp / temp2;
             A1(Z, j) = A1(Z, j) * temp;
         }
         const bool fuse = k + 1 <= mn && !(k % 10 == 0 && M - k > 0);
-        const int plk1 = fuse ? A1(PL, k + 1) : 0;
+        const int plk1 = fuse ? A1(PL, k + 1) : plk;
+        const RootG ck1 = root_load<SQ>(H, plk1);
         VI y1 = {0.0, 0};
-        #pragma unroll 8
-        for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
-            double temp = fdq<SQ>(H, A1(PU, j), plk);
-            const double vj = A1(V, j) * (ppq<SQ>(A1(H.FF, A1(PU, j)), Fk) / temp);
-            A1(V, j) = vj;
-            if (fuse) y1 = vi_better(y1, VI{fabs(vj / fdq<SQ>(H, A1(PU, j), plk1)), j - k});
+        for (int j0 = k + 1 + tid; j0 <= N; j0 += RB * HSS_CWG) {
+            int pd[RB];
+            double v[RB], ffd[RB];
+            cload(j0, pd, v, ffd);
+            #pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int j = j0 + u * HSS_CWG;
+                double temp = fd_r<SQ>(ffd[u], pd[u], plk, ck);
+                const double vj = v[u] * (ppq<SQ>(ffd[u], Fk) / temp);
+                if (j <= N) {
+                    vst(j, vj);
+                    if (fuse) y1 = vi_better(y1, VI{fabs(vj / fd_r<SQ>(ffd[u], pd[u], plk1, ck1)), j - k});
+                }
+            }
         }
         for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
-            double temp = fdq<SQ>(H, puk, A1(PL, j));
-            double temp2 = ddq<SQ>(H, plk, A1(PL, j));
+            const int plj = A1(PL, j), puj = A1(PU, j);
+            const RootG rj = root_load<SQ>(H, plj);
+            double temp = fd_r<SQ>(Fk, puk, plj, rj);
+            double temp2 = dd_r<SQ>(plk, ck, plj, rj);
             temp = temp / temp2;
             A1(W, j) = A1(W, j) * temp;
             // the factors of W(k) (their gathers in parallel; the product stays in order)
-            temp = fdq<SQ>(H, A1(PU, j), plk);
-            temp2 = ddq<SQ>(H, A1(PL, j), plk);
+            temp = fd_r<SQ>(A1(FF, puj), puj, plk, ck);
+            temp2 = dd_r<SQ>(plj, rj, plk, ck);
             A1(T1, j) = temp / temp2;
         }
         fl += 6ull * (M - k) + 5ull * (N - k) + 14ull * (k - 1);
@@ -701,7 +863,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         if (fuse) pre_rw = block_argmax_t(y1, red, tog);
         else __syncthreads();
         if (tid == 0) {
-            double wk = (-1.0 * fdq<SQ>(H, puk, plk)) / A1(A, k);
+            double wk = (-1.0 * fd_r<SQ>(Fk, puk, plk, ck)) / A1(A, k);
             for (int j = 1; j <= k - 1; j++) wk = wk * A1(T1, j);
             A1(W, k) = wk;
         }
@@ -729,12 +891,19 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 fl += 10ull * (N - k) + 12ull * M + 22ull * k;
                 __syncthreads();
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
-                #pragma unroll 8
-                for (int j = k + 1 + tid; j <= N; j += HSS_CWG) {
-                    double temp = fdq<SQ>(H, A1(PU, j), plii);
-                    double temp2 = fdq<SQ>(H, A1(PU, j), pljj);
-                    temp = temp / temp2;
-                    A1(V, j) = A1(V, j) * temp;
+                const RootG cii = root_load<SQ>(H, plii), cjj = root_load<SQ>(H, pljj);
+                for (int j0 = k + 1 + tid; j0 <= N; j0 += RB * HSS_CWG) {
+                    int pd[RB];
+                    double v[RB], ffd[RB];
+                    cload(j0, pd, v, ffd);
+                    #pragma unroll
+                    for (int u = 0; u < RB; u++) {
+                        const int j = j0 + u * HSS_CWG;
+                        double temp = fd_r<SQ>(ffd[u], pd[u], plii, cii);
+                        double temp2 = fd_r<SQ>(ffd[u], pd[u], pljj, cjj);
+                        temp = temp / temp2;
+                        if (j <= N) vst(j, v[u] * temp);
+                    }
                 }
                 for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
                     if (j == jj) continue;
@@ -944,8 +1113,6 @@ typedef double d4h __attribute__((ext_vector_type(4)));
 // that they point to global memory and would use FLAT loads, which count in LGKM_CNT as
 // well as VM_CNT: every wait for the LDS operand reads would then also wait for the next
 // chunk's loads.  Global-address-space views keep the two counters apart.
-typedef const __attribute__((address_space(1))) double gdbl;
-typedef __attribute__((address_space(1))) double gdblw;
 // what a staged element outside the operands reads
 static __device__ double bg_zero = 0.0;
 #define BG_BK 32

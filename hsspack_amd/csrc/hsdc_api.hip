@@ -921,12 +921,14 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         hipLaunchKernelGGL(k_hss_init, dim3(std::min(cdiv(kmax, WGS), 256), nh), dim3(WGS), 0, s, dH);
         hipLaunchKernelGGL(k_hss_dgen, dim3(nleaf), dim3(WGS), 0, s, dH, dN + leaf0, (int)nleaf);
         // the 512-thread RRLU launches (fewer nodes than two per CU) keep each node's
-        // private generator column in LDS when it fits: 160 KB per workgroup on gfx950,
-        // 30.2 KB of it the kernels' static arrays
-        const int ldsn = std::min(kmax, HSS_LDS_V);
-        const size_t ldsb = sizeof(double) * (size_t)ldsn;
+        // private column data in LDS as far as it fits: 160 KB per workgroup on gfx950,
+        // 30.6 KB of it the kernels' static arrays.  A launch of at most one node per CU
+        // takes the whole rest; more nodes than CUs take what the largest merge's
+        // generator column needs, so that two workgroups can share a CU
         for (int st = 0; st < maxlvl; st++) {
             int nn = (int)step_cnt[st], nc = (int)comp_cnt[st];
+            const int ldsn = nn <= 256 ? HSS_LDS_V : std::min(kmax, HSS_LDS_V);
+            const size_t ldsb = sizeof(double) * (size_t)ldsn;
             const HssNode *sn = dN + step_off[st];
             const HssCompact *sc = dC + comp_off[st];
             hipLaunchKernelGGL(k_hss_reset, dim3(nh), dim3(64), 0, s, dH);

@@ -69,6 +69,17 @@ __device__ __forceinline__ double fd_r(double ffd, int pd, int pw, const RootG &
     const double r = (pd <= pw) ? (ffd - w.f0) - w.dl : (ffd - w.f1) - w.dr;
     return SQ ? r * (ffd + w.s) : r;
 }
+// Whether |z / den| can exceed the running maximum mv of a "first largest" search: the
+// hardware reciprocal gives |z| / |den| to far better than 2^-10 relative, so an entry
+// estimated below mv (1 - 2^-10) is below mv exactly and cannot replace it; every other
+// entry (and any zero, infinite or NaN operand) takes the exact division.  The searches
+// therefore choose the same entries as with a division for every element.
+__device__ __forceinline__ bool may_exceed(double z, double den, double mv)
+{
+    const double az = fabs(z);
+    const double ap = az * __builtin_amdgcn_rcp(fabs(den));
+    return !(ap < mv * (1.0 - 0x1p-10)) || (ap == 0.0 && az != 0.0);
+}
 // ddq(pj, pk) with j = root_load(pj), k = root_load(pk)
 template <bool SQ>
 __device__ __forceinline__ double dd_r(int pj, const RootG &j, int pk, const RootG &k)
@@ -134,13 +145,14 @@ __device__ VI block_argmax(VI x, VI *sh)
 }
 
 // The same with one barrier: successive calls alternate between the two halves of sh
-// (2 x blockDim / 64 entries), so a call's writes never meet the previous call's reads
+// (4 x blockDim / 64 entries; a half holds 2 x blockDim / 64 so that the two-value
+// form below shares them), so a call's writes never meet the previous call's reads
 // and the trailing barrier that protects a single buffer is not needed.  tog is
 // uniform over the block and flips on every call.
 __device__ __forceinline__ VI block_argmax_t(VI x, VI *sh, int &tog)
 {
     const int tid = threadIdx.x, nw = (int)(blockDim.x >> 6);
-    VI *b = sh + tog * nw;
+    VI *b = sh + tog * 2 * nw;
     tog ^= 1;
     x = vi_step<32>(x);
     x = vi_step<16>(x);
@@ -153,6 +165,26 @@ __device__ __forceinline__ VI block_argmax_t(VI x, VI *sh, int &tog)
     VI r = b[0];
     for (int w = 1; w < nw; w++) r = vi_better(r, b[w]);
     return r;
+}
+// Two independent (value, index) maxima under one barrier, in the same alternating
+// halves of sh.
+__device__ __forceinline__ void block_argmax2_t(VI &x, VI &y, VI *sh, int &tog)
+{
+    const int tid = threadIdx.x, nw = (int)(blockDim.x >> 6);
+    VI *b = sh + tog * 2 * nw;
+    tog ^= 1;
+    x = vi_step<32>(x); y = vi_step<32>(y);
+    x = vi_step<16>(x); y = vi_step<16>(y);
+    x = vi_step<8>(x); y = vi_step<8>(y);
+    x = vi_step<4>(x); y = vi_step<4>(y);
+    x = vi_step<2>(x); y = vi_step<2>(y);
+    x = vi_step<1>(x); y = vi_step<1>(y);
+    if ((tid & 63) == 0) { b[tid >> 6] = x; b[nw + (tid >> 6)] = y; }
+    __syncthreads();
+    VI rx = b[0], ry = b[nw];
+    for (int w = 1; w < nw; w++) { rx = vi_better(rx, b[w]); ry = vi_better(ry, b[nw + w]); }
+    x = rx;
+    y = ry;
 }
 
 // ---------------------------------------------------------------------------
@@ -193,11 +225,15 @@ __device__ long scratch_take(const HssMerge &H, long n)
 // ---------------------------------------------------------------------------
 // Row compression of one node: rrluCauchyEigrow + CauchylikeEigQ (ComprcauchyEig 'r')
 // rows: the node's M row generators (LDS); columns: the N column generators
-// outside the node (private copies: V in LDS when it fits, else global; PU global).
-// The passes over the N columns take RB columns per thread at a time (8 for 512
-// threads, 4 for 256, whose launches put several workgroups on a CU): the loads of all
-// RB (V, PU, then the roots' gathered values) are issued before any result is used,
-// and the stores of a batch come after its loads.
+// outside the node (private copies: V in LDS when it fits, else global; PU global,
+// with positions 1..min(M, N) mirrored in LDS).
+// The passes over the N columns take RB columns per thread at a time (7, or 6 in the
+// squared form, for 512 threads; 4 for 256, whose launches put several workgroups on a
+// CU): the loads of all RB (V, PU, then the roots' gathered values) are issued before
+// any result is used, the next batch's V and PU before the current batch's results,
+// and the stores of a batch after its loads.  A step the swap check does not follow runs the next step's
+// row search inside its own row update and reduces it with the next step's first
+// column search under one barrier.
 // ---------------------------------------------------------------------------
 template <int HSS_CWG, bool SQ>
 __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, const HssNode *nodes, int nn,
@@ -206,14 +242,14 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     extern __shared__ double hss_dyn[];  // lds_cap doubles: V when the node's columns fit
     __shared__ double F[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
     __shared__ double T1[HSS_MAXM], T2[HSS_MAXM];  // factors of thread 0's ordered products
-    __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
-    __shared__ VI red[2 * (HSS_CWG / 64)];
-    int tog = 0;  // which half of red the next block_argmax_t uses
+    __shared__ int PL[HSS_MAXM], PUs[HSS_MAXM];
+    __shared__ short PQ[HSS_MAXM], IP[HSS_MAXM];   // positions within the node (<= HSS_MAXM)
+    __shared__ VI red[4 * (HSS_CWG / 64)];
+    int tog = 0;  // which half of red the next reduction uses
     __shared__ long s_off;
     __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
-    __shared__ double s_nmax;
     const int tid = threadIdx.x;
-    constexpr int RB = HSS_CWG >= 512 ? 8 : 4;
+    constexpr int RB = HSS_CWG >= 512 ? (SQ ? 6 : 7) : 4;  // within 256 VGPRs
     if ((int)blockIdx.x >= nn) return;
     HssNode Nd = nodes[blockIdx.x];
     const HssMerge H = hm[Nd.mrg];
@@ -237,69 +273,135 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     if (s_flag || s_off < 0) return;
     const int M = s_M, N = s_N, l0 = lr[2 * (node - 1)], c0 = lc[2 * (node - 1)], c1e = lc[2 * (node - 1) + 1];
     const int NS = N > 0 ? N : 1;
-    // private per column: the generator V (updated every step; in LDS when it fits) and
-    // the root index; the poles and differences its accurate differences read are
-    // shared by every node of the merge and gathered by index
-    const bool vl = N <= lds_cap;  // uniform over the block
+    const int mn = M < N ? M : N;
+    // Private per column: the generator V (updated every step) and the root index PU;
+    // the values of the root that its accurate differences read (FF, DIFL, DIFR and D0 at
+    // PU and PU + 1) are shared by every node of the merge.  Kept in the dynamic LDS as
+    // far as it holds them (lds_cap doubles): V; V and PU; or V, PU and the roots'
+    // values, so that a small node's passes, swaps and pivot lookups stay in LDS.
+    // Otherwise V and PU are global and the values are gathered by index.
+    constexpr int RD = SQ ? 5 : 4;  // root values kept per column
+    const bool vl = N <= lds_cap;   // all three uniform over the block
+    const bool pl = (long)N + (N + 1) / 2 <= lds_cap;
+    const bool sm = (long)N * (1 + RD) + (N + 1) / 2 <= lds_cap;
     ldbl *Vl = (ldbl *)hss_dyn;
+    ldbl *Rl = Vl + NS;  // sm: f0, f1, dl, dr (and s) of every column, NS apart
+    __attribute__((address_space(3))) int *PUl =
+        (__attribute__((address_space(3))) int *)(sm ? Rl + RD * NS : Vl + NS);
     gdblw *Vg = (gdblw *)(H.scratch + s_off);
-    gint *PU = (gint *)(H.scratch + s_off + NS);
+    gint *PUg = (gint *)(H.scratch + s_off + NS);
     auto vld = [&](int j) -> double { return vl ? A1(Vl, j) : A1(Vg, j); };
     auto vst = [&](int j, double x) {
         if (vl) A1(Vl, j) = x;
         else A1(Vg, j) = x;
     };
+    auto pu = [&](int j) -> int { return pl ? A1(PUl, j) : A1(PUg, j); };
+    auto pu_set = [&](int j, int x) {
+        if (pl) A1(PUl, j) = x;
+        else A1(PUg, j) = x;
+    };
+    auto rl_get = [&](int j) -> RootG {
+        return RootG{Rl[j - 1], Rl[NS + j - 1], Rl[2 * NS + j - 1], Rl[3 * NS + j - 1], SQ ? Rl[4 * NS + j - 1] : 0.0};
+    };
+    // the root values of the column at position j <= min(M, N)
+    auto croot = [&](int j) -> RootG { return sm ? rl_get(j) : root_load<SQ>(H, A1(PUs, j)); };
     for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < c0 - 1) ? j + 1 : j + 1 + (c1e - c0 + 1);
         vst(j + 1, A1((gdbl *)H.V, src));
-        PU[j] = A1((gcint *)H.PU, src);
+        const int p = A1((gcint *)H.PU, src);
+        pu_set(j + 1, p);
+        if (j < mn) PUs[j] = p;
+        if (sm) {
+            const RootG r = root_load<SQ>(H, p);
+            Rl[j] = r.f0; Rl[NS + j] = r.f1; Rl[2 * NS + j] = r.dl; Rl[3 * NS + j] = r.dr;
+            if (SQ) Rl[4 * NS + j] = r.s;
+        }
     }
+    // column swap (one thread; a barrier follows)
     auto swapc = [&](int a, int c) {
         double t = vld(a); vst(a, vld(c)); vst(c, t);
-        int u = A1(PU, a); A1(PU, a) = A1(PU, c); A1(PU, c) = u;
+        int u = pu(a), w = pu(c);
+        pu_set(a, w); pu_set(c, u);
+        if (a <= mn) A1(PUs, a) = w;
+        if (c <= mn) A1(PUs, c) = u;
+        if (sm) {
+            #pragma unroll
+            for (int q = 0; q < RD; q++) {
+                t = Rl[q * NS + a - 1]; Rl[q * NS + a - 1] = Rl[q * NS + c - 1]; Rl[q * NS + c - 1] = t;
+            }
+        }
     };
-    // one batch of RB columns j0, j0 + HSS_CWG, ... (clamped to N for the loads)
-    auto cload = [&](int j0, int *pw, double *v, RootG *c) {
+    auto cload = [&](int j0, int *pw, double *v) {
         #pragma unroll
         for (int u = 0; u < RB; u++) {
             const int j = min(j0 + u * HSS_CWG, N);
-            pw[u] = A1(PU, j);
+            pw[u] = pu(j);
             v[u] = vld(j);
         }
+    };
+    // one batch with the roots' values (CP scan)
+    auto bload = [&](int j0, int *pw, double *v, RootG *c) {
+        cload(j0, pw, v);
         #pragma unroll
-        for (int u = 0; u < RB; u++) c[u] = root_load<SQ>(H, pw[u]);
+        for (int u = 0; u < RB; u++) c[u] = sm ? rl_get(min(j0 + u * HSS_CWG, N)) : root_load<SQ>(H, pw[u]);
+    };
+    // body(j, pw, v, c) for the columns j in [lo, N]; j may exceed N (loads clamped)
+    auto cpass = [&](int lo, auto body) {
+        int j0 = lo + tid;
+        if (j0 > N) return;
+        if (sm) {
+            for (int j = j0; j <= N; j += HSS_CWG) body(j, A1(PUl, j), A1(Vl, j), rl_get(j));
+            return;
+        }
+        int pw[RB];
+        double v[RB];
+        cload(j0, pw, v);
+        for (;;) {
+            RootG c[RB];
+            #pragma unroll
+            for (int u = 0; u < RB; u++) c[u] = root_load<SQ>(H, pw[u]);
+            const int jn = j0 + RB * HSS_CWG;
+            int pn[RB];
+            double vn[RB];
+            if (jn <= N) cload(jn, pn, vn);
+            #pragma unroll
+            for (int u = 0; u < RB; u++) body(j0 + u * HSS_CWG, pw[u], v[u], c[u]);
+            if (jn > N) break;
+            j0 = jn;
+            #pragma unroll
+            for (int u = 0; u < RB; u++) { pw[u] = pn[u]; v[u] = vn[u]; }
+        }
     };
     // max over j in [k, N] of |V(j) / fd(pd, PU(j))| (index j - k + 1), ffd = FF(pd)
     auto colsearch = [&](int k, int pd, double ffd) -> VI {
         VI y = {0.0, 0};
-        for (int j0 = k + tid; j0 <= N; j0 += RB * HSS_CWG) {
-            int pw[RB];
-            double v[RB];
-            RootG c[RB];
-            cload(j0, pw, v, c);
-            #pragma unroll
-            for (int u = 0; u < RB; u++) {
-                const int j = j0 + u * HSS_CWG;
-                if (j <= N) y = vi_better(y, VI{fabs(v[u] / fd_r<SQ>(ffd, pd, pw[u], c[u])), j - k + 1});
-            }
-        }
+        cpass(k, [&](int j, int pw, double v, const RootG &c) {
+            if (j <= N) y = vi_better(y, VI{fabs(v / fd_r<SQ>(ffd, pd, pw, c)), j - k + 1});
+        });
         return y;
+    };
+    // max over i in [k, M] of |Z(i) / fd(PL(i), pw)|, w = the values of root pw
+    auto rowsearch = [&](int k, int pw, const RootG &w) -> VI {
+        VI x = {0.0, 0};
+        for (int i = k + tid; i <= M; i += HSS_CWG)
+            x = vi_better(x, VI{fabs(A1(Z, i) / fd_r<SQ>(A1(F, i), A1(PL, i), pw, w)), i - k + 1});
+        return x;
     };
     for (int i = tid; i < M; i += HSS_CWG) {
         PL[i] = A1((gcint *)H.PL, l0 + i);
         A[i] = A1((gdbl *)H.U, l0 + i);
         F[i] = A1((gdbl *)H.FF, PL[i]);
         Z[i] = A[i];
-        PQ[i] = i + 1;
+        PQ[i] = (short)(i + 1);
         W[i] = 0.0;
     }
     __syncthreads();
-    const int mn = M < N ? M : N;
     double amax = 0.0;
     int Rk = mn;
     const double rho = 1.5;
-    bool pre = false;     // pre_rw holds this step's first column search
-    VI pre_rw = {0.0, 0};
+    bool pre = false;     // pre_cl, pre_rw hold this step's first row and column searches
+    VI pre_cl = {0.0, 0}, pre_rw = {0.0, 0};
+    RootG cnext = {0.0, 0.0, 0.0, 0.0, 0.0};  // root data of PU(k + 1) when pre
     // flop count of the reference's model (nflops of rrluCauchyEigrow and its pivot
     // searches, CauchyHssEig.f90:105, 147, 205, 680-703); the loop is uniform over the
     // block, thread 0 adds it to the merge's total
@@ -307,56 +409,53 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     for (int k = 1; k <= mn; k++) {
         // ---- CauchyPivtEig: rook search on the k-th Schur complement
         fl += 4ull * (M + N - 2 * k + 3);
+        RootG cw;  // root data of the column at position k (PU(k)), kept through the step
         {
-            VI x = {0.0, 0};
-            {
-                const int pwk = A1(PU, k);
-                const RootG cw = root_load<SQ>(H, pwk);
-                for (int i = k + tid; i <= M; i += HSS_CWG) {
-                    double v = fabs(A1(Z, i) / fd_r<SQ>(A1(F, i), A1(PL, i), pwk, cw));
-                    x = vi_better(x, VI{v, i - k + 1});
-                }
+            VI cl, rw;
+            if (pre) {
+                cl = pre_cl;
+                rw = pre_rw;
+                cw = cnext;
+            } else {
+                const int pwk = A1(PUs, k);
+                cw = croot(k);
+                cl = rowsearch(k, pwk, cw);
+                rw = colsearch(k, A1(PL, k), A1(F, k));
+                block_argmax2_t(cl, rw, red, tog);
             }
-            VI cl = block_argmax_t(x, red, tog);
-            VI rw = pre_rw;  // computed by the previous step's update pass when pre
-            if (!pre) rw = block_argmax_t(colsearch(k, A1(PL, k), A1(F, k)), red, tog);
             double junkL = cl.v * fabs(vld(k)), junkU = rw.v * fabs(A1(Z, k));
-            double piv = fabs(A1(Z, k) * vld(k) / fdq<SQ>(H, A1(PL, k), A1(PU, k)));
+            double piv = fabs(A1(Z, k) * vld(k) / fd_r<SQ>(A1(F, k), A1(PL, k), A1(PUs, k), cw));
             if (!(junkL <= piv && junkU <= piv)) {
                 int flg = (junkL > junkU) ? 1 : 0;
                 int jjL = cl.i, jjU = rw.i;
+                // Every thread has read position k for the tests above before any swap.
+                // A row swap is made after the column search it feeds, which reads the
+                // incoming row where it still is; every later swap follows a reduction,
+                // whose barrier comes after all reads of the previous search.
+                __syncthreads();
                 for (;;) {
                     if (flg == 1) {
-                        int jj = jjL + k - 1;
-                        __syncthreads();
+                        const int jr = jjL + k - 1;
+                        jjU = block_argmax_t(colsearch(k, A1(PL, jr), A1(F, jr)), red, tog).i;
+                        fl += 4ull * (N - k + 1);
                         if (tid == 0) {
                             double t;
-                            t = A1(F, k); A1(F, k) = A1(F, jj); A1(F, jj) = t;
-                            t = A1(Z, k); A1(Z, k) = A1(Z, jj); A1(Z, jj) = t;
-                            t = A1(A, k); A1(A, k) = A1(A, jj); A1(A, jj) = t;
-                            int u = A1(PL, k); A1(PL, k) = A1(PL, jj); A1(PL, jj) = u;
-                            u = A1(PQ, k); A1(PQ, k) = A1(PQ, jj); A1(PQ, jj) = u;
+                            t = A1(F, k); A1(F, k) = A1(F, jr); A1(F, jr) = t;
+                            t = A1(Z, k); A1(Z, k) = A1(Z, jr); A1(Z, jr) = t;
+                            t = A1(A, k); A1(A, k) = A1(A, jr); A1(A, jr) = t;
+                            int u = A1(PL, k); A1(PL, k) = A1(PL, jr); A1(PL, jr) = u;
+                            short q = A1(PQ, k); A1(PQ, k) = A1(PQ, jr); A1(PQ, jr) = q;
                         }
-                        __syncthreads();
-                        jjU = block_argmax_t(colsearch(k, A1(PL, k), A1(F, k)), red, tog).i;
-                        fl += 4ull * (N - k + 1);
-                        if (jjU == 1) break;
-                        flg = 0;
-                    }
-                    int jj = jjU + k - 1;
-                    __syncthreads();
-                    if (tid == 0) swapc(k, jj);
-                    __syncthreads();
-                    VI x2 = {0.0, 0};
-                    {
-                        const int pwk2 = A1(PU, k);
-                        const RootG cw = root_load<SQ>(H, pwk2);
-                        for (int i = k + tid; i <= M; i += HSS_CWG) {
-                            double v = fabs(A1(Z, i) / fd_r<SQ>(A1(F, i), A1(PL, i), pwk2, cw));
-                            x2 = vi_better(x2, VI{v, i - k + 1});
+                        if (jjU == 1) {
+                            __syncthreads();
+                            break;
                         }
                     }
-                    jjL = block_argmax_t(x2, red, tog).i;
+                    if (tid == 0) swapc(k, jjU + k - 1);
+                    __syncthreads();
+                    const int pwk2 = A1(PUs, k);
+                    cw = croot(k);
+                    jjL = block_argmax_t(rowsearch(k, pwk2, cw), red, tog).i;
                     fl += 4ull * (M - k + 1);
                     if (jjL == 1) break;
                     flg = 1;
@@ -365,7 +464,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         }
         // every swap of the search is followed by a barrier and the search only reads, so
         // the update pass below may start without another
-        double ukk = A1(Z, k) * vld(k) / fdq<SQ>(H, A1(PL, k), A1(PU, k));
+        double ukk = A1(Z, k) * vld(k) / fd_r<SQ>(A1(F, k), A1(PL, k), A1(PUs, k), cw);
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
             // ---- CauchyPivtEig_CP: complete pivoting over the Schur complement
@@ -377,7 +476,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 double v[RB], mv[RB];
                 int mi[RB];
                 RootG c[RB];
-                cload(j0, pw, v, c);
+                bload(j0, pw, v, c);
                 #pragma unroll
                 for (int u = 0; u < RB; u++) { mv[u] = -DBL_MAX; mi[u] = 0; }
                 for (int i = k; i <= M; i++) {
@@ -385,7 +484,9 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     const int pli = A1(PL, i);
                     #pragma unroll
                     for (int u = 0; u < RB; u++) {
-                        double vv = fabs(zi / fd_r<SQ>(fi, pli, pw[u], c[u]));
+                        const double den = fd_r<SQ>(fi, pli, pw[u], c[u]);
+                        if (mi[u] != 0 && !may_exceed(zi, den, mv[u])) continue;
+                        double vv = fabs(zi / den);
                         if (mi[u] == 0 || vv > mv[u]) { mv[u] = vv; mi[u] = i - k + 1; }
                     }
                 }
@@ -412,64 +513,75 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 t = A1(Z, k); A1(Z, k) = A1(Z, a); A1(Z, a) = t;
                 t = A1(A, k); A1(A, k) = A1(A, a); A1(A, a) = t;
                 int u = A1(PL, k); A1(PL, k) = A1(PL, a); A1(PL, a) = u;
-                u = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = u;
+                short q = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = q;
                 swapc(k, b);
             }
             __syncthreads();
-            ukk = A1(Z, k) * vld(k) / fdq<SQ>(H, A1(PL, k), A1(PU, k));
+            cw = croot(k);
+            ukk = A1(Z, k) * vld(k) / fd_r<SQ>(A1(F, k), A1(PL, k), A1(PUs, k), cw);
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
         }
         // ---- generator updates of the (k+1)-th Schur complement; unless a swap check
-        // follows, the pass over the columns also runs step k+1's first column search
+        // follows, the pass over the rows also runs step k+1's row search and the pass
+        // over the columns its first column search
         const double Fk = A1(F, k);
-        const int plk = A1(PL, k), puk = A1(PU, k);
-        const RootG ck = root_load<SQ>(H, puk);
-        for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
-            double temp = fd_r<SQ>(A1(F, j), A1(PL, j), puk, ck);
-            temp = ppq<SQ>(A1(F, j), Fk) / temp;
-            A1(Z, j) = A1(Z, j) * temp;
-        }
+        const int plk = A1(PL, k), puk = A1(PUs, k);
+        const RootG ck = cw;
         const bool fuse = k + 1 <= mn && !(k % 10 == 0 && M - k > 0);
         const int fk1 = fuse ? A1(PL, k + 1) : 0;
         const double ff1 = fuse ? A1(F, k + 1) : 0.0;
-        VI y1 = {0.0, 0};
-        for (int j0 = k + 1 + tid; j0 <= N; j0 += RB * HSS_CWG) {
-            int pw[RB];
-            double v[RB];
-            RootG c[RB];
-            cload(j0, pw, v, c);
-            #pragma unroll
-            for (int u = 0; u < RB; u++) {
-                const int j = j0 + u * HSS_CWG;
-                double temp = dd_r<SQ>(pw[u], c[u], puk, ck);
-                double temp2 = -1.0 * fd_r<SQ>(Fk, plk, pw[u], c[u]);
-                temp = temp / temp2;
-                const double vj = v[u] * temp;
-                if (j <= N) {
-                    vst(j, vj);
-                    if (fuse) y1 = vi_better(y1, VI{fabs(vj / fd_r<SQ>(ff1, fk1, pw[u], c[u])), j - k});
-                }
-            }
+        const int pun = fuse ? A1(PUs, k + 1) : puk;
+        const RootG cn = fuse ? croot(k + 1) : cw;
+        VI x1 = {0.0, 0};
+        for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
+            double temp = fd_r<SQ>(A1(F, j), A1(PL, j), puk, ck);
+            temp = ppq<SQ>(A1(F, j), Fk) / temp;
+            const double zj = A1(Z, j) * temp;
+            A1(Z, j) = zj;
+            if (fuse) x1 = vi_better(x1, VI{fabs(zj / fd_r<SQ>(A1(F, j), A1(PL, j), pun, cn)), j - k});
         }
-        for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
+        VI y1 = {0.0, 0};
+        cpass(k + 1, [&](int j, int pw, double v, const RootG &c) {
+            double temp = dd_r<SQ>(pw, c, puk, ck);
+            double temp2 = -1.0 * fd_r<SQ>(Fk, plk, pw, c);
+            temp = temp / temp2;
+            const double vj = v * temp;
+            if (j <= N) {
+                vst(j, vj);
+                if (fuse) y1 = vi_better(y1, VI{fabs(vj / fd_r<SQ>(ff1, fk1, pw, c)), j - k});
+            }
+        });
+        // j = k - 1 on thread 0, which wrote W(k - 1) at the end of the previous step: a
+        // fused step has no barrier between that write and this loop.  For the same
+        // reason the factors of W(k) go to T1 and T2 on alternate steps (thread 0 may
+        // still read the previous step's; the swap check uses both only after a barrier)
+        double *Tk = (k & 1) ? T2 : T1;
+        for (int j = k - 1 - tid; j >= 1; j -= HSS_CWG) {
             double temp = -1.0 * fd_r<SQ>(A1(F, j), A1(PL, j), puk, ck);
             A1(W, j) = A1(W, j) * (temp / ppq<SQ>(Fk, A1(F, j)));
             // the factors of W(k) (their gathers in parallel; the product stays in order)
-            const int puj = A1(PU, j);
-            const double t2 = -1.0 * fd_r<SQ>(Fk, plk, puj, root_load<SQ>(H, puj));
-            A1(T1, j) = t2 / ppq<SQ>(A1(F, j), Fk);
+            const int puj = A1(PUs, j);
+            const double t2 = -1.0 * fd_r<SQ>(Fk, plk, puj, croot(j));
+            A1(Tk, j) = t2 / ppq<SQ>(A1(F, j), Fk);
         }
         fl += 5ull * (M - k) + 7ull * (N - k) + 12ull * (k - 1);
         pre = fuse;
-        // The argmax's barrier also publishes the update pass (T1 for thread 0's product),
-        // and the next step reads W(k) only after its own first barrier, so a fused step
-        // needs no other; before a swap check, and at the last step, W(k) is published here.
-        if (fuse) pre_rw = block_argmax_t(y1, red, tog);
-        else __syncthreads();
+        // The reduction's barrier also publishes the update pass (T1 for thread 0's
+        // product), and the next step reads W(k) only after its own first barrier, so a
+        // fused step needs no other; before a swap check, and at the last step, W(k) is
+        // published here.
+        if (fuse) {
+            block_argmax2_t(x1, y1, red, tog);
+            pre_cl = x1;
+            pre_rw = y1;
+            cnext = cn;
+        } else {
+            __syncthreads();
+        }
         if (tid == 0) {
             double wk = fd_r<SQ>(Fk, plk, puk, ck) / A1(A, k);
-            for (int j = 1; j <= k - 1; j++) wk = wk * A1(T1, j);
+            for (int j = 1; j <= k - 1; j++) wk = wk * A1(Tk, j);
             A1(W, k) = wk;
         }
         if (!fuse) __syncthreads();
@@ -487,7 +599,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     double junk = mv * fabs(A1(W, jc));
                     VI c = {junk, jc};
                     VI nb = vi_better(x, c);
-                    if (nb.i == jc) IP[jc - 1] = mi;
+                    if (nb.i == jc) IP[jc - 1] = (short)mi;
                     x = nb;
                 }
                 VI win = block_argmax_t(x, red, tog);
@@ -497,20 +609,12 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 __syncthreads();
                 const double Fii = A1(F, ii), Fjj = A1(F, jj);
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
-                for (int j0 = k + 1 + tid; j0 <= N; j0 += RB * HSS_CWG) {
-                    int pw[RB];
-                    double v[RB];
-                    RootG c[RB];
-                    cload(j0, pw, v, c);
-                    #pragma unroll
-                    for (int u = 0; u < RB; u++) {
-                        const int j = j0 + u * HSS_CWG;
-                        double temp = -1.0 * fd_r<SQ>(Fii, plii, pw[u], c[u]);
-                        double temp2 = -1.0 * fd_r<SQ>(Fjj, pljj, pw[u], c[u]);
-                        temp = temp / temp2;
-                        if (j <= N) vst(j, v[u] * temp);
-                    }
-                }
+                cpass(k + 1, [&](int j, int pw, double v, const RootG &c) {
+                    double temp = -1.0 * fd_r<SQ>(Fii, plii, pw, c);
+                    double temp2 = -1.0 * fd_r<SQ>(Fjj, pljj, pw, c);
+                    temp = temp / temp2;
+                    if (j <= N) vst(j, v * temp);
+                });
                 for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
                     if (j == jj) continue;
                     A1(Z, j) = A1(Z, j) * (ppq<SQ>(A1(F, j), Fjj) / ppq<SQ>(A1(F, j), Fii));
@@ -518,15 +622,19 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 for (int j = 1 + tid; j <= k; j += HSS_CWG) {
                     if (j == ii) continue;
                     A1(W, j) = A1(W, j) * (ppq<SQ>(A1(F, j), Fii) / ppq<SQ>(A1(F, j), Fjj));
-                    double temp = fdq<SQ>(H, plii, A1(PU, j));
+                    const int puj = A1(PUs, j);
+                    const RootG cj = croot(j);
+                    double temp = fd_r<SQ>(Fii, plii, puj, cj);
                     A1(T1, j) = ppq<SQ>(Fii, A1(F, j)) / temp;
-                    temp = -1.0 * fdq<SQ>(H, pljj, A1(PU, j));
+                    temp = -1.0 * fd_r<SQ>(Fjj, pljj, puj, cj);
                     A1(T2, j) = temp / ppq<SQ>(A1(F, j), Fjj);
                 }
                 __syncthreads();
                 if (tid == 0) {
-                    double uj = A1(A, ii) * (ppq<SQ>(Fii, Fjj) / fdq<SQ>(H, plii, A1(PU, ii)));
-                    double wi = fdq<SQ>(H, pljj, A1(PU, ii)) / A1(A, jj);
+                    const int puii = A1(PUs, ii);
+                    const RootG cii = croot(ii);
+                    double uj = A1(A, ii) * (ppq<SQ>(Fii, Fjj) / fd_r<SQ>(Fii, plii, puii, cii));
+                    double wi = fd_r<SQ>(Fjj, pljj, puii, cii) / A1(A, jj);
                     for (int j = 1; j <= k; j++) {
                         if (j == ii) continue;
                         uj = uj * A1(T1, j);
@@ -535,7 +643,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     A1(Z, jj) = uj;
                     A1(W, ii) = wi;
                     int u = A1(PL, ii); A1(PL, ii) = A1(PL, jj); A1(PL, jj) = u;
-                    u = A1(PQ, ii); A1(PQ, ii) = A1(PQ, jj); A1(PQ, jj) = u;
+                    short q = A1(PQ, ii); A1(PQ, ii) = A1(PQ, jj); A1(PQ, jj) = q;
                     double t = A1(F, ii); A1(F, ii) = A1(F, jj); A1(F, jj) = t;
                     t = A1(A, ii); A1(A, ii) = A1(A, jj); A1(A, jj) = t;
                 }
@@ -591,7 +699,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
 // Column compression of one node: rrluCauchyEigcol + CauchylikeEigQ2
 // (ComprcauchyEig 'c').  RRLU rows: the node's M columns (roots, alpha, LDS);
 // RRLU columns: the N row generators outside the node (private copies, as in the row
-// compression; their passes take RB columns per thread at a time the same way).
+// compression; their passes, the mirrored positions 1..min(M, N) and the fused next
+// row search work the same way).
 // ---------------------------------------------------------------------------
 template <int HSS_CWG, bool SQ>
 __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, const HssNode *nodes, int nn,
@@ -599,10 +708,14 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
 {
     extern __shared__ double hss_dyn[];  // lds_cap doubles: V when the node's rows fit
     __shared__ double Dr[HSS_MAXM], A[HSS_MAXM], Z[HSS_MAXM], W[HSS_MAXM];
-    __shared__ double T1[HSS_MAXM], T2[HSS_MAXM];  // factors of thread 0's ordered products
-    __shared__ int PL[HSS_MAXM], PQ[HSS_MAXM], IP[HSS_MAXM];
-    __shared__ VI red[2 * (HSS_CWG / 64)];
-    int tog = 0;  // which half of red the next block_argmax_t uses
+    // factors of thread 0's ordered products; during the swap check's scan, the scanned
+    // rows' root data
+    __shared__ double T12[2 * HSS_MAXM];
+    double *const T1 = T12, *const T2 = T12 + HSS_MAXM;
+    __shared__ int PL[HSS_MAXM], PUs[HSS_MAXM];
+    __shared__ short PQ[HSS_MAXM], IP[HSS_MAXM];   // positions within the node (<= HSS_MAXM)
+    __shared__ VI red[4 * (HSS_CWG / 64)];
+    int tog = 0;  // which half of red the next reduction uses
     __shared__ long s_off;
     __shared__ int s_M, s_N, s_Rk, s_flag, s_ii, s_jj;
     const int tid = threadIdx.x;
@@ -624,136 +737,208 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     __syncthreads();
     if (s_flag || s_off < 0) return;
     const int M = s_M, N = s_N, c0 = lc[2 * (node - 1)], r0 = lr[2 * (node - 1)], r1e = lr[2 * (node - 1) + 1];
-    // private per outside row: its generator (updated every step; in LDS when it fits)
-    // and pole index; the pole value is gathered from the merge's shared array
+    const int mn = M < N ? M : N;
+    // Private per outside row: its generator V (updated every step), pole index PU and
+    // pole value FF(PU).  Kept in the dynamic LDS as far as it holds them: V; V and PU;
+    // or V, PU, the pole values and the root values of the node's own M columns (FF,
+    // DIFL, DIFR and D0 at PL and PL + 1), so that a small node's passes, swaps and row
+    // searches stay in LDS.  Otherwise V and PU are global and the values gathered.
+    constexpr int RD = SQ ? 5 : 4;
     gdbl *FF = (gdbl *)H.FF;
-    const bool vl = N <= lds_cap;  // uniform over the block
+    const int NS = N > 0 ? N : 1;
+    const bool vl = N <= lds_cap;  // all three uniform over the block
+    const bool pl = (long)N + (N + 1) / 2 <= lds_cap;
+    const bool sm = 2L * N + (N + 1) / 2 + (long)RD * M <= lds_cap;
     ldbl *Vl = (ldbl *)hss_dyn;
+    ldbl *FFl = Vl + NS;     // sm: FF(PU(j))
+    ldbl *RRl = FFl + NS;    // sm: f0, f1, dl, dr (and s) of the node's M columns, M apart
+    __attribute__((address_space(3))) int *PUl =
+        (__attribute__((address_space(3))) int *)(sm ? RRl + RD * M : Vl + NS);
     gdblw *Vg = (gdblw *)(H.scratch + s_off);
-    gint *PU = (gint *)(H.scratch + s_off + (N > 0 ? N : 1));
+    gint *PUg = (gint *)(H.scratch + s_off + NS);
     auto vld = [&](int j) -> double { return vl ? A1(Vl, j) : A1(Vg, j); };
     auto vst = [&](int j, double x) {
         if (vl) A1(Vl, j) = x;
         else A1(Vg, j) = x;
     };
+    auto pu = [&](int j) -> int { return pl ? A1(PUl, j) : A1(PUg, j); };
+    auto pu_set = [&](int j, int x) {
+        if (pl) A1(PUl, j) = x;
+        else A1(PUg, j) = x;
+    };
+    // the root values of the node's column at position i (PL(i))
+    auto rroot = [&](int i) -> RootG {
+        if (sm)
+            return RootG{RRl[i - 1], RRl[M + i - 1], RRl[2 * M + i - 1], RRl[3 * M + i - 1],
+                         SQ ? RRl[4 * M + i - 1] : 0.0};
+        return root_load<SQ>(H, A1(PL, i));
+    };
     for (int j = tid; j < N; j += HSS_CWG) {
         int src = (j < r0 - 1) ? j + 1 : j + 1 + (r1e - r0 + 1);
         vst(j + 1, A1((gdbl *)H.U, src));
-        PU[j] = A1((gcint *)H.PL, src);
+        const int p = A1((gcint *)H.PL, src);
+        pu_set(j + 1, p);
+        if (j < mn) PUs[j] = p;
+        if (sm) FFl[j] = A1(FF, p);
     }
-    // one batch of RB outside rows j0, j0 + HSS_CWG, ... (clamped to N for the loads):
-    // pole index, generator and pole value
-    auto cload = [&](int j0, int *pd, double *v, double *ffd) {
+    // outside-row swap (one thread; a barrier follows)
+    auto swapc = [&](int a, int c) {
+        double t = vld(a); vst(a, vld(c)); vst(c, t);
+        int u = pu(a), w = pu(c);
+        pu_set(a, w); pu_set(c, u);
+        if (a <= mn) A1(PUs, a) = w;
+        if (c <= mn) A1(PUs, c) = u;
+        if (sm) { t = A1(FFl, a); A1(FFl, a) = A1(FFl, c); A1(FFl, c) = t; }
+    };
+    // node-column swap of the root values (one thread, beside the swaps of Dr, Z, A, PL, PQ)
+    auto swapr = [&](int a, int c) {
+        if (sm) {
+            #pragma unroll
+            for (int q = 0; q < RD; q++) {
+                double t = RRl[q * M + a - 1]; RRl[q * M + a - 1] = RRl[q * M + c - 1]; RRl[q * M + c - 1] = t;
+            }
+        }
+    };
+    auto cload = [&](int j0, int *pd, double *v) {
         #pragma unroll
         for (int u = 0; u < RB; u++) {
             const int j = min(j0 + u * HSS_CWG, N);
-            pd[u] = A1(PU, j);
+            pd[u] = pu(j);
             v[u] = vld(j);
         }
-        #pragma unroll
-        for (int u = 0; u < RB; u++) ffd[u] = A1(FF, pd[u]);
     };
-    // max over j in [k, N] of |V(j) / fd(PU(j), pw)| (index j - k + 1), w = root_load(pw)
+    auto bload = [&](int j0, int *pd, double *v, double *ffd) {
+        cload(j0, pd, v);
+        #pragma unroll
+        for (int u = 0; u < RB; u++) ffd[u] = sm ? A1(FFl, min(j0 + u * HSS_CWG, N)) : A1(FF, pd[u]);
+    };
+    // body(j, pd, v, FF(pd)) for the outside rows j in [lo, N]; j may exceed N
+    auto cpass = [&](int lo, auto body) {
+        int j0 = lo + tid;
+        if (j0 > N) return;
+        if (sm) {
+            for (int j = j0; j <= N; j += HSS_CWG) body(j, A1(PUl, j), A1(Vl, j), A1(FFl, j));
+            return;
+        }
+        int pd[RB];
+        double v[RB];
+        cload(j0, pd, v);
+        for (;;) {
+            double ffd[RB];
+            #pragma unroll
+            for (int u = 0; u < RB; u++) ffd[u] = A1(FF, pd[u]);
+            const int jn = j0 + RB * HSS_CWG;
+            int pn[RB];
+            double vn[RB];
+            if (jn <= N) cload(jn, pn, vn);
+            #pragma unroll
+            for (int u = 0; u < RB; u++) body(j0 + u * HSS_CWG, pd[u], v[u], ffd[u]);
+            if (jn > N) break;
+            j0 = jn;
+            #pragma unroll
+            for (int u = 0; u < RB; u++) { pd[u] = pn[u]; v[u] = vn[u]; }
+        }
+    };
+    // max over j in [k, N] of |V(j) / fd(PU(j), pw)| (index j - k + 1), w = root values of pw
     auto colsearch = [&](int k, int pw, const RootG &w) -> VI {
         VI y = {0.0, 0};
-        for (int j0 = k + tid; j0 <= N; j0 += RB * HSS_CWG) {
-            int pd[RB];
-            double v[RB], ffd[RB];
-            cload(j0, pd, v, ffd);
-            #pragma unroll
-            for (int u = 0; u < RB; u++) {
-                const int j = j0 + u * HSS_CWG;
-                if (j <= N) y = vi_better(y, VI{fabs(v[u] / fd_r<SQ>(ffd[u], pd[u], pw, w)), j - k + 1});
-            }
-        }
+        cpass(k, [&](int j, int pd, double v, double ffd) {
+            if (j <= N) y = vi_better(y, VI{fabs(v / fd_r<SQ>(ffd, pd, pw, w)), j - k + 1});
+        });
         return y;
+    };
+    // max over i in [k, M] of |Z(i) / fd(pd, PL(i))|, ffd = FF(pd)
+    auto rowsearch = [&](int k, int pd, double ffd) -> VI {
+        VI x = {0.0, 0};
+        for (int i = k + tid; i <= M; i += HSS_CWG)
+            x = vi_better(x, VI{fabs(A1(Z, i) / fd_r<SQ>(ffd, pd, A1(PL, i), rroot(i))), i - k + 1});
+        return x;
     };
     for (int i = tid; i < M; i += HSS_CWG) {
         Dr[i] = A1((gdbl *)H.D, c0 + i);
         A[i] = A1((gdbl *)H.V, c0 + i);
         PL[i] = A1((gcint *)H.PU, c0 + i);
         Z[i] = A[i];
-        PQ[i] = i + 1;
+        PQ[i] = (short)(i + 1);
         W[i] = 0.0;
+        if (sm) {
+            const RootG r = root_load<SQ>(H, PL[i]);
+            RRl[i] = r.f0; RRl[M + i] = r.f1; RRl[2 * M + i] = r.dl; RRl[3 * M + i] = r.dr;
+            if (SQ) RRl[4 * M + i] = r.s;
+        }
     }
     __syncthreads();
-    const int mn = M < N ? M : N;
     double amax = 0.0;
     int Rk = mn;
     const double rho = 1.5;
-    bool pre = false;     // pre_rw holds this step's first column search
-    VI pre_rw = {0.0, 0};
+    bool pre = false;     // pre_cl, pre_rw hold this step's first row and column searches
+    VI pre_cl = {0.0, 0}, pre_rw = {0.0, 0};
+    RootG cnext = {0.0, 0.0, 0.0, 0.0, 0.0};  // root data of PL(k + 1) when pre
+    double ffnext = 0.0;                      // FF(PU(k + 1)) when pre
     // flop count of the reference's model (rrluCauchyEigcol, CauchyHssEig.f90:321, 371,
     // 460, and its pivot searches), added to the merge's total by thread 0
     unsigned long long fl = 0;
     for (int k = 1; k <= mn; k++) {
         // ---- CauchyPivtEigm
         fl += 4ull * (M + N - 2 * k + 3);
+        RootG cr;    // root data of PL(k), kept through the step
+        double ffp;  // FF(PU(k)), kept through the step
         {
-            VI x = {0.0, 0};
-            {
-                const int puk = A1(PU, k);
-                const double ffk = A1(FF, puk);
-                for (int i = k + tid; i <= M; i += HSS_CWG) {
-                    const int pli = A1(PL, i);
-                    double v = fabs(A1(Z, i) / fd_r<SQ>(ffk, puk, pli, root_load<SQ>(H, pli)));
-                    x = vi_better(x, VI{v, i - k + 1});
-                }
-            }
-            VI cl = block_argmax_t(x, red, tog);
-            VI rw = pre_rw;  // computed by the previous step's update pass when pre
-            if (!pre) {
-                const int plk = A1(PL, k);
-                rw = block_argmax_t(colsearch(k, plk, root_load<SQ>(H, plk)), red, tog);
+            VI cl, rw;
+            if (pre) {
+                cl = pre_cl;
+                rw = pre_rw;
+                cr = cnext;
+                ffp = ffnext;
+            } else {
+                const int puk = A1(PUs, k), plk = A1(PL, k);
+                ffp = A1(FF, puk);
+                cr = rroot(k);
+                cl = rowsearch(k, puk, ffp);
+                rw = colsearch(k, plk, cr);
+                block_argmax2_t(cl, rw, red, tog);
             }
             double junkL = cl.v * fabs(vld(k)), junkU = rw.v * fabs(A1(Z, k));
             double den;
-            if (A1(PL, k) < A1(PU, k))
-                den = (A1(FF, A1(PL, k) + 1) - A1(FF, A1(PU, k))) + A1((gdbl *)H.DIFR, A1(PL, k));
-            else
-                den = (A1(FF, A1(PL, k)) - A1(FF, A1(PU, k))) + A1((gdbl *)H.DIFL, A1(PL, k));
-            if (SQ) den = den * (A1((gdbl *)H.D0, A1(PL, k)) + A1(FF, A1(PU, k)));
+            {
+                const int plk = A1(PL, k), puk = A1(PUs, k);
+                if (plk < puk) den = (cr.f1 - ffp) + cr.dr;
+                else den = (cr.f0 - ffp) + cr.dl;
+                if (SQ) den = den * (cr.s + ffp);
+            }
             double piv = fabs(A1(Z, k) * vld(k) / den);
             if (!(junkL <= piv && junkU <= piv)) {
                 int flg = (junkL > junkU) ? 1 : 0;
                 int jjL = cl.i, jjU = rw.i;
+                // barriers as in the row compression: one before the first swap, then a
+                // row swap after the column search it feeds
+                __syncthreads();
                 for (;;) {
                     if (flg == 1) {
-                        int jj = jjL + k - 1;
-                        __syncthreads();
+                        const int jr = jjL + k - 1;
+                        const int plr = A1(PL, jr);
+                        cr = rroot(jr);
+                        jjU = block_argmax_t(colsearch(k, plr, cr), red, tog).i;
+                        fl += 4ull * (N - k + 1);
                         if (tid == 0) {
                             double t;
-                            t = A1(Dr, k); A1(Dr, k) = A1(Dr, jj); A1(Dr, jj) = t;
-                            t = A1(Z, k); A1(Z, k) = A1(Z, jj); A1(Z, jj) = t;
-                            t = A1(A, k); A1(A, k) = A1(A, jj); A1(A, jj) = t;
-                            int u = A1(PL, k); A1(PL, k) = A1(PL, jj); A1(PL, jj) = u;
-                            u = A1(PQ, k); A1(PQ, k) = A1(PQ, jj); A1(PQ, jj) = u;
+                            t = A1(Dr, k); A1(Dr, k) = A1(Dr, jr); A1(Dr, jr) = t;
+                            t = A1(Z, k); A1(Z, k) = A1(Z, jr); A1(Z, jr) = t;
+                            t = A1(A, k); A1(A, k) = A1(A, jr); A1(A, jr) = t;
+                            int u = A1(PL, k); A1(PL, k) = A1(PL, jr); A1(PL, jr) = u;
+                            short q = A1(PQ, k); A1(PQ, k) = A1(PQ, jr); A1(PQ, jr) = q;
+                            swapr(k, jr);
                         }
-                        __syncthreads();
-                        const int plk2 = A1(PL, k);
-                        jjU = block_argmax_t(colsearch(k, plk2, root_load<SQ>(H, plk2)), red, tog).i;
-                        fl += 4ull * (N - k + 1);
-                        if (jjU == 1) break;
-                        flg = 0;
-                    }
-                    int jj = jjU + k - 1;
-                    __syncthreads();
-                    if (tid == 0) {
-                        double t = vld(k); vst(k, vld(jj)); vst(jj, t);
-                        int u = A1(PU, k); A1(PU, k) = A1(PU, jj); A1(PU, jj) = u;
-                    }
-                    __syncthreads();
-                    VI x2 = {0.0, 0};
-                    {
-                        const int puk2 = A1(PU, k);
-                        const double ffk2 = A1(FF, puk2);
-                        for (int i = k + tid; i <= M; i += HSS_CWG) {
-                            const int pli = A1(PL, i);
-                            double v = fabs(A1(Z, i) / fd_r<SQ>(ffk2, puk2, pli, root_load<SQ>(H, pli)));
-                            x2 = vi_better(x2, VI{v, i - k + 1});
+                        if (jjU == 1) {
+                            __syncthreads();
+                            break;
                         }
                     }
-                    jjL = block_argmax_t(x2, red, tog).i;
+                    if (tid == 0) swapc(k, jjU + k - 1);
+                    __syncthreads();
+                    const int puk2 = A1(PUs, k);
+                    ffp = A1(FF, puk2);
+                    jjL = block_argmax_t(rowsearch(k, puk2, ffp), red, tog).i;
                     fl += 4ull * (M - k + 1);
                     if (jjL == 1) break;
                     flg = 1;
@@ -762,7 +947,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         }
         // every swap of the search is followed by a barrier and the search only reads, so
         // the update pass below may start without another
-        double ukk = A1(Z, k) * vld(k) / (-1.0 * fdq<SQ>(H, A1(PU, k), A1(PL, k)));
+        double ukk = A1(Z, k) * vld(k) / (-1.0 * fd_r<SQ>(ffp, A1(PUs, k), A1(PL, k), cr));
         amax = fmax(amax, fabs(ukk));
         if (fabs(ukk) < tol * amax) {
             // ---- CauchyPivtEigm_CP
@@ -773,16 +958,18 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 int pd[RB];
                 double v[RB], ffd[RB], mv[RB];
                 int mi[RB];
-                cload(j0, pd, v, ffd);
+                bload(j0, pd, v, ffd);
                 #pragma unroll
                 for (int u = 0; u < RB; u++) { mv[u] = 0.0; mi[u] = 0; }
                 for (int i = k; i <= M; i++) {
                     const double zi = A1(Z, i);
                     const int pli = A1(PL, i);
-                    const RootG ri = root_load<SQ>(H, pli);
+                    const RootG ri = rroot(i);
                     #pragma unroll
                     for (int u = 0; u < RB; u++) {
-                        double vv = fabs(zi / fd_r<SQ>(ffd[u], pd[u], pli, ri));
+                        const double den = fd_r<SQ>(ffd[u], pd[u], pli, ri);
+                        if (mi[u] != 0 && !may_exceed(zi, den, mv[u])) continue;
+                        double vv = fabs(zi / den);
                         if (mi[u] == 0 || vv > mv[u]) { mv[u] = vv; mi[u] = i - k + 1; }
                     }
                 }
@@ -806,83 +993,117 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 t = A1(Z, k); A1(Z, k) = A1(Z, a); A1(Z, a) = t;
                 t = A1(A, k); A1(A, k) = A1(A, a); A1(A, a) = t;
                 int u = A1(PL, k); A1(PL, k) = A1(PL, a); A1(PL, a) = u;
-                u = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = u;
-                t = vld(k); vst(k, vld(b)); vst(b, t);
-                u = A1(PU, k); A1(PU, k) = A1(PU, b); A1(PU, b) = u;
+                short q = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = q;
+                swapr(k, a);
+                swapc(k, b);
             }
             __syncthreads();
-            ukk = A1(Z, k) * vld(k) / (-1.0 * fdq<SQ>(H, A1(PU, k), A1(PL, k)));
+            cr = rroot(k);
+            ffp = A1(FF, A1(PUs, k));
+            ukk = A1(Z, k) * vld(k) / (-1.0 * fd_r<SQ>(ffp, A1(PUs, k), A1(PL, k), cr));
             amax = fmax(amax, fabs(ukk));
             if (fabs(ukk) < tol * amax) { Rk = k - 1; break; }
         }
-        const int plk = A1(PL, k), puk = A1(PU, k);
-        const double Fk = A1(FF, puk);
-        const RootG ck = root_load<SQ>(H, plk);
+        const int plk = A1(PL, k), puk = A1(PUs, k);
+        const double Fk = ffp;
+        const RootG ck = cr;
+        const bool fuse = k + 1 <= mn && !(k % 10 == 0 && M - k > 0);
+        const int plk1 = fuse ? A1(PL, k + 1) : plk;
+        const int pun = fuse ? A1(PUs, k + 1) : puk;
+        const RootG ck1 = fuse ? rroot(k + 1) : ck;
+        const double ffn = A1(FF, pun);
+        VI x1 = {0.0, 0};
         for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
             const int plj = A1(PL, j);
-            const RootG rj = root_load<SQ>(H, plj);
+            const RootG rj = rroot(j);
             double temp = dd_r<SQ>(plj, rj, plk, ck);
             double temp2 = -1.0 * fd_r<SQ>(Fk, puk, plj, rj);
             temp = temp / temp2;
-            A1(Z, j) = A1(Z, j) * temp;
+            const double zj = A1(Z, j) * temp;
+            A1(Z, j) = zj;
+            if (fuse) x1 = vi_better(x1, VI{fabs(zj / fd_r<SQ>(ffn, pun, plj, rj)), j - k});
         }
-        const bool fuse = k + 1 <= mn && !(k % 10 == 0 && M - k > 0);
-        const int plk1 = fuse ? A1(PL, k + 1) : plk;
-        const RootG ck1 = root_load<SQ>(H, plk1);
         VI y1 = {0.0, 0};
-        for (int j0 = k + 1 + tid; j0 <= N; j0 += RB * HSS_CWG) {
-            int pd[RB];
-            double v[RB], ffd[RB];
-            cload(j0, pd, v, ffd);
-            #pragma unroll
-            for (int u = 0; u < RB; u++) {
-                const int j = j0 + u * HSS_CWG;
-                double temp = fd_r<SQ>(ffd[u], pd[u], plk, ck);
-                const double vj = v[u] * (ppq<SQ>(ffd[u], Fk) / temp);
-                if (j <= N) {
-                    vst(j, vj);
-                    if (fuse) y1 = vi_better(y1, VI{fabs(vj / fd_r<SQ>(ffd[u], pd[u], plk1, ck1)), j - k});
-                }
+        cpass(k + 1, [&](int j, int pd, double v, double ffd) {
+            double temp = fd_r<SQ>(ffd, pd, plk, ck);
+            const double vj = v * (ppq<SQ>(ffd, Fk) / temp);
+            if (j <= N) {
+                vst(j, vj);
+                if (fuse) y1 = vi_better(y1, VI{fabs(vj / fd_r<SQ>(ffd, pd, plk1, ck1)), j - k});
             }
-        }
-        for (int j = 1 + tid; j <= k - 1; j += HSS_CWG) {
-            const int plj = A1(PL, j), puj = A1(PU, j);
-            const RootG rj = root_load<SQ>(H, plj);
+        });
+        // j = k - 1 on thread 0, and the factors of W(k) in T1 or T2 by the step's
+        // parity, as in the row compression
+        double *Tk = (k & 1) ? T2 : T1;
+        for (int j = k - 1 - tid; j >= 1; j -= HSS_CWG) {
+            const int plj = A1(PL, j), puj = A1(PUs, j);
+            const RootG rj = rroot(j);
             double temp = fd_r<SQ>(Fk, puk, plj, rj);
             double temp2 = dd_r<SQ>(plk, ck, plj, rj);
             temp = temp / temp2;
             A1(W, j) = A1(W, j) * temp;
             // the factors of W(k) (their gathers in parallel; the product stays in order)
-            temp = fd_r<SQ>(A1(FF, puj), puj, plk, ck);
+            temp = fd_r<SQ>(sm ? A1(FFl, j) : A1(FF, puj), puj, plk, ck);
             temp2 = dd_r<SQ>(plj, rj, plk, ck);
-            A1(T1, j) = temp / temp2;
+            A1(Tk, j) = temp / temp2;
         }
         fl += 6ull * (M - k) + 5ull * (N - k) + 14ull * (k - 1);
         pre = fuse;
-        // as in the row compression: the argmax's barrier publishes the update pass
-        if (fuse) pre_rw = block_argmax_t(y1, red, tog);
-        else __syncthreads();
+        // as in the row compression: the reduction's barrier publishes the update pass
+        if (fuse) {
+            block_argmax2_t(x1, y1, red, tog);
+            pre_cl = x1;
+            pre_rw = y1;
+            cnext = ck1;
+            ffnext = ffn;
+        } else {
+            __syncthreads();
+        }
         if (tid == 0) {
             double wk = (-1.0 * fd_r<SQ>(Fk, puk, plk, ck)) / A1(A, k);
-            for (int j = 1; j <= k - 1; j++) wk = wk * A1(T1, j);
+            for (int j = 1; j <= k - 1; j++) wk = wk * A1(Tk, j);
             A1(W, k) = wk;
         }
         if (!fuse) __syncthreads();
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
+                // the scanned rows' root data, gathered once into T1 / T2 when they fit
+                const int L = M - k;
+                const bool stg = !sm && L * (SQ ? 5 : 4) <= 2 * HSS_MAXM;
+                if (stg) {
+                    for (int q = tid; q < L; q += HSS_CWG) {
+                        const RootG r = root_load<SQ>(H, A1(PL, k + 1 + q));
+                        T12[q] = r.f0;
+                        T12[L + q] = r.f1;
+                        T12[2 * L + q] = r.dl;
+                        T12[3 * L + q] = r.dr;
+                        if (SQ) T12[4 * L + q] = r.s;
+                    }
+                    __syncthreads();
+                }
                 VI x = {0.0, 0};
                 for (int jc = 1 + tid; jc <= k; jc += HSS_CWG) {
                     int pljc = A1(PL, jc);
+                    const RootG rc = rroot(jc);
                     double mv = 0.0;
                     int mi = 0;
-                    for (int i = 1; i <= M - k; i++) {
-                        double tt = ddq<SQ>(H, A1(PL, i + k), pljc);
+                    for (int i = 1; i <= L; i++) {
+                        double tt;
+                        if (sm) {
+                            tt = dd_r<SQ>(A1(PL, i + k), rroot(i + k), pljc, rc);
+                        } else if (stg) {
+                            const RootG ri = {T12[i - 1], T12[L + i - 1], T12[2 * L + i - 1], T12[3 * L + i - 1],
+                                              SQ ? T12[4 * L + i - 1] : 0.0};
+                            tt = dd_r<SQ>(A1(PL, i + k), ri, pljc, rc);
+                        } else {
+                            tt = ddq<SQ>(H, A1(PL, i + k), pljc);
+                        }
                         double v = fabs(A1(Z, k + i) / tt);
                         if (mi == 0 || v > mv) { mv = v; mi = i; }
                     }
                     VI c = {mv * fabs(A1(W, jc)), jc};
                     VI nb = vi_better(x, c);
-                    if (nb.i == jc) IP[jc - 1] = mi;
+                    if (nb.i == jc) IP[jc - 1] = (short)mi;
                     x = nb;
                 }
                 VI win = block_argmax_t(x, red, tog);
@@ -891,20 +1112,13 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 fl += 10ull * (N - k) + 12ull * M + 22ull * k;
                 __syncthreads();
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
-                const RootG cii = root_load<SQ>(H, plii), cjj = root_load<SQ>(H, pljj);
-                for (int j0 = k + 1 + tid; j0 <= N; j0 += RB * HSS_CWG) {
-                    int pd[RB];
-                    double v[RB], ffd[RB];
-                    cload(j0, pd, v, ffd);
-                    #pragma unroll
-                    for (int u = 0; u < RB; u++) {
-                        const int j = j0 + u * HSS_CWG;
-                        double temp = fd_r<SQ>(ffd[u], pd[u], plii, cii);
-                        double temp2 = fd_r<SQ>(ffd[u], pd[u], pljj, cjj);
-                        temp = temp / temp2;
-                        if (j <= N) vst(j, v[u] * temp);
-                    }
-                }
+                const RootG cii = rroot(ii), cjj = rroot(jj);
+                cpass(k + 1, [&](int j, int pd, double v, double ffd) {
+                    double temp = fd_r<SQ>(ffd, pd, plii, cii);
+                    double temp2 = fd_r<SQ>(ffd, pd, pljj, cjj);
+                    temp = temp / temp2;
+                    if (j <= N) vst(j, v * temp);
+                });
                 for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
                     if (j == jj) continue;
                     double temp = ddq<SQ>(H, A1(PL, j), pljj);
@@ -914,23 +1128,25 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 }
                 for (int j = 1 + tid; j <= k; j += HSS_CWG) {
                     if (j == ii) continue;
+                    const int puj = A1(PUs, j);
                     double temp = ddq<SQ>(H, A1(PL, j), plii);
                     double temp2 = ddq<SQ>(H, A1(PL, j), pljj);
                     temp = temp / temp2;
                     A1(W, j) = A1(W, j) * temp;
                     temp = ddq<SQ>(H, plii, A1(PL, j));
-                    temp2 = -1.0 * fdq<SQ>(H, A1(PU, j), plii);
+                    temp2 = -1.0 * fdq<SQ>(H, puj, plii);
                     A1(T1, j) = temp / temp2;
-                    temp = fdq<SQ>(H, A1(PU, j), pljj);
+                    temp = fdq<SQ>(H, puj, pljj);
                     temp2 = ddq<SQ>(H, A1(PL, j), pljj);
                     A1(T2, j) = temp / temp2;
                 }
                 __syncthreads();
                 if (tid == 0) {
+                    const int puii = A1(PUs, ii);
                     double temp = ddq<SQ>(H, plii, pljj);
-                    double temp2 = -1.0 * fdq<SQ>(H, A1(PU, ii), plii);
+                    double temp2 = -1.0 * fdq<SQ>(H, puii, plii);
                     double uj = A1(A, ii) * (temp / temp2);
-                    double wi = (-1.0 * fdq<SQ>(H, A1(PU, ii), pljj)) / A1(A, jj);
+                    double wi = (-1.0 * fdq<SQ>(H, puii, pljj)) / A1(A, jj);
                     for (int j = 1; j <= k; j++) {
                         if (j == ii) continue;
                         uj = uj * A1(T1, j);
@@ -939,7 +1155,8 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     A1(Z, jj) = uj;
                     A1(W, ii) = wi;
                     int u = A1(PL, ii); A1(PL, ii) = A1(PL, jj); A1(PL, jj) = u;
-                    u = A1(PQ, ii); A1(PQ, ii) = A1(PQ, jj); A1(PQ, jj) = u;
+                    short q = A1(PQ, ii); A1(PQ, ii) = A1(PQ, jj); A1(PQ, jj) = q;
+                    swapr(ii, jj);
                     double t = A1(Dr, ii); A1(Dr, ii) = A1(Dr, jj); A1(Dr, jj) = t;
                     t = A1(A, ii); A1(A, ii) = A1(A, jj); A1(A, jj) = t;
                 }

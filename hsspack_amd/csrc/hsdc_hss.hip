@@ -149,7 +149,9 @@ __device__ VI block_argmax(VI x, VI *sh)
 // form below shares them), so a call's writes never meet the previous call's reads
 // and the trailing barrier that protects a single buffer is not needed.  tog is
 // uniform over the block and flips on every call.
-__device__ __forceinline__ VI block_argmax_t(VI x, VI *sh, int &tog)
+// With wave set the block is one wavefront: every lane holds the maximum after the
+// butterfly, and nothing goes through sh.
+__device__ __forceinline__ VI block_argmax_t(VI x, VI *sh, int &tog, bool wave = false)
 {
     const int tid = threadIdx.x, nw = (int)(blockDim.x >> 6);
     VI *b = sh + tog * 2 * nw;
@@ -160,6 +162,7 @@ __device__ __forceinline__ VI block_argmax_t(VI x, VI *sh, int &tog)
     x = vi_step<4>(x);
     x = vi_step<2>(x);
     x = vi_step<1>(x);
+    if (wave) return x;
     if ((tid & 63) == 0) b[tid >> 6] = x;
     __syncthreads();
     VI r = b[0];
@@ -168,7 +171,7 @@ __device__ __forceinline__ VI block_argmax_t(VI x, VI *sh, int &tog)
 }
 // Two independent (value, index) maxima under one barrier, in the same alternating
 // halves of sh.
-__device__ __forceinline__ void block_argmax2_t(VI &x, VI &y, VI *sh, int &tog)
+__device__ __forceinline__ void block_argmax2_t(VI &x, VI &y, VI *sh, int &tog, bool wave = false)
 {
     const int tid = threadIdx.x, nw = (int)(blockDim.x >> 6);
     VI *b = sh + tog * 2 * nw;
@@ -179,12 +182,25 @@ __device__ __forceinline__ void block_argmax2_t(VI &x, VI &y, VI *sh, int &tog)
     x = vi_step<4>(x); y = vi_step<4>(y);
     x = vi_step<2>(x); y = vi_step<2>(y);
     x = vi_step<1>(x); y = vi_step<1>(y);
+    if (wave) return;
     if ((tid & 63) == 0) { b[tid >> 6] = x; b[nw + (tid >> 6)] = y; }
     __syncthreads();
     VI rx = b[0], ry = b[nw];
     for (int w = 1; w < nw; w++) { rx = vi_better(rx, b[w]); ry = vi_better(ry, b[nw + w]); }
     x = rx;
     y = ry;
+}
+// The block's barrier, or for a one-wavefront block the ordering of its LDS accesses
+// (a wavefront executes in lockstep; the fence completes lane 0's writes before any
+// lane reads them).
+__device__ __forceinline__ void bsync(bool wave)
+{
+    if (wave) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -284,6 +300,11 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     const bool vl = N <= lds_cap;   // all three uniform over the block
     const bool pl = (long)N + (N + 1) / 2 <= lds_cap;
     const bool sm = (long)N * (1 + RD) + (N + 1) / 2 <= lds_cap;
+    // A node whose data all sit in LDS and whose outside part is small runs on one
+    // wavefront: its steps need no block barrier (the other waves leave here)
+    const bool wave = sm && N <= 384;
+    if (wave && tid >= 64) return;
+    const int NTH = wave ? 64 : HSS_CWG;
     ldbl *Vl = (ldbl *)hss_dyn;
     ldbl *Rl = Vl + NS;  // sm: f0, f1, dl, dr (and s) of every column, NS apart
     __attribute__((address_space(3))) int *PUl =
@@ -305,7 +326,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     };
     // the root values of the column at position j <= min(M, N)
     auto croot = [&](int j) -> RootG { return sm ? rl_get(j) : root_load<SQ>(H, A1(PUs, j)); };
-    for (int j = tid; j < N; j += HSS_CWG) {
+    for (int j = tid; j < N; j += NTH) {
         int src = (j < c0 - 1) ? j + 1 : j + 1 + (c1e - c0 + 1);
         vst(j + 1, A1((gdbl *)H.V, src));
         const int p = A1((gcint *)H.PU, src);
@@ -334,7 +355,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     auto cload = [&](int j0, int *pw, double *v) {
         #pragma unroll
         for (int u = 0; u < RB; u++) {
-            const int j = min(j0 + u * HSS_CWG, N);
+            const int j = min(j0 + u * NTH, N);
             pw[u] = pu(j);
             v[u] = vld(j);
         }
@@ -343,14 +364,14 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     auto bload = [&](int j0, int *pw, double *v, RootG *c) {
         cload(j0, pw, v);
         #pragma unroll
-        for (int u = 0; u < RB; u++) c[u] = sm ? rl_get(min(j0 + u * HSS_CWG, N)) : root_load<SQ>(H, pw[u]);
+        for (int u = 0; u < RB; u++) c[u] = sm ? rl_get(min(j0 + u * NTH, N)) : root_load<SQ>(H, pw[u]);
     };
     // body(j, pw, v, c) for the columns j in [lo, N]; j may exceed N (loads clamped)
     auto cpass = [&](int lo, auto body) {
         int j0 = lo + tid;
         if (j0 > N) return;
         if (sm) {
-            for (int j = j0; j <= N; j += HSS_CWG) body(j, A1(PUl, j), A1(Vl, j), rl_get(j));
+            for (int j = j0; j <= N; j += NTH) body(j, A1(PUl, j), A1(Vl, j), rl_get(j));
             return;
         }
         int pw[RB];
@@ -360,12 +381,12 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             RootG c[RB];
             #pragma unroll
             for (int u = 0; u < RB; u++) c[u] = root_load<SQ>(H, pw[u]);
-            const int jn = j0 + RB * HSS_CWG;
+            const int jn = j0 + RB * NTH;
             int pn[RB];
             double vn[RB];
             if (jn <= N) cload(jn, pn, vn);
             #pragma unroll
-            for (int u = 0; u < RB; u++) body(j0 + u * HSS_CWG, pw[u], v[u], c[u]);
+            for (int u = 0; u < RB; u++) body(j0 + u * NTH, pw[u], v[u], c[u]);
             if (jn > N) break;
             j0 = jn;
             #pragma unroll
@@ -383,11 +404,11 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
     // max over i in [k, M] of |Z(i) / fd(PL(i), pw)|, w = the values of root pw
     auto rowsearch = [&](int k, int pw, const RootG &w) -> VI {
         VI x = {0.0, 0};
-        for (int i = k + tid; i <= M; i += HSS_CWG)
+        for (int i = k + tid; i <= M; i += NTH)
             x = vi_better(x, VI{fabs(A1(Z, i) / fd_r<SQ>(A1(F, i), A1(PL, i), pw, w)), i - k + 1});
         return x;
     };
-    for (int i = tid; i < M; i += HSS_CWG) {
+    for (int i = tid; i < M; i += NTH) {
         PL[i] = A1((gcint *)H.PL, l0 + i);
         A[i] = A1((gdbl *)H.U, l0 + i);
         F[i] = A1((gdbl *)H.FF, PL[i]);
@@ -395,7 +416,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         PQ[i] = (short)(i + 1);
         W[i] = 0.0;
     }
-    __syncthreads();
+    bsync(wave);
     double amax = 0.0;
     int Rk = mn;
     const double rho = 1.5;
@@ -421,7 +442,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 cw = croot(k);
                 cl = rowsearch(k, pwk, cw);
                 rw = colsearch(k, A1(PL, k), A1(F, k));
-                block_argmax2_t(cl, rw, red, tog);
+                block_argmax2_t(cl, rw, red, tog, wave);
             }
             double junkL = cl.v * fabs(vld(k)), junkU = rw.v * fabs(A1(Z, k));
             double piv = fabs(A1(Z, k) * vld(k) / fd_r<SQ>(A1(F, k), A1(PL, k), A1(PUs, k), cw));
@@ -432,11 +453,11 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 // A row swap is made after the column search it feeds, which reads the
                 // incoming row where it still is; every later swap follows a reduction,
                 // whose barrier comes after all reads of the previous search.
-                __syncthreads();
+                bsync(wave);
                 for (;;) {
                     if (flg == 1) {
                         const int jr = jjL + k - 1;
-                        jjU = block_argmax_t(colsearch(k, A1(PL, jr), A1(F, jr)), red, tog).i;
+                        jjU = block_argmax_t(colsearch(k, A1(PL, jr), A1(F, jr)), red, tog, wave).i;
                         fl += 4ull * (N - k + 1);
                         if (tid == 0) {
                             double t;
@@ -447,15 +468,15 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                             short q = A1(PQ, k); A1(PQ, k) = A1(PQ, jr); A1(PQ, jr) = q;
                         }
                         if (jjU == 1) {
-                            __syncthreads();
+                            bsync(wave);
                             break;
                         }
                     }
                     if (tid == 0) swapc(k, jjU + k - 1);
-                    __syncthreads();
+                    bsync(wave);
                     const int pwk2 = A1(PUs, k);
                     cw = croot(k);
-                    jjL = block_argmax_t(rowsearch(k, pwk2, cw), red, tog).i;
+                    jjL = block_argmax_t(rowsearch(k, pwk2, cw), red, tog, wave).i;
                     fl += 4ull * (M - k + 1);
                     if (jjL == 1) break;
                     flg = 1;
@@ -471,7 +492,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
             fl += 4ull * (M - k + 1) * (N - k + 1);
             VI best = {0.0, 0};
             VI bestrow = {0.0, 0};
-            for (int j0 = k + tid; j0 <= N; j0 += RB * HSS_CWG) {
+            for (int j0 = k + tid; j0 <= N; j0 += RB * NTH) {
                 int pw[RB];
                 double v[RB], mv[RB];
                 int mi[RB];
@@ -492,7 +513,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 }
                 #pragma unroll
                 for (int u = 0; u < RB; u++) {
-                    const int j = j0 + u * HSS_CWG;
+                    const int j = j0 + u * NTH;
                     if (j > N) continue;
                     double jt = fabs(mv[u] * v[u]);
                     VI cc = {jt, j - k + 1};
@@ -501,10 +522,10 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     best = nb;
                 }
             }
-            VI win = block_argmax_t(best, red, tog);
-            __syncthreads();
+            VI win = block_argmax_t(best, red, tog, wave);
+            bsync(wave);
             if (win.i > 0 && best.i == win.i) { s_ii = bestrow.i; s_jj = win.i; }
-            __syncthreads();
+            bsync(wave);
             if (tid == 0) {
                 int ii = s_ii, jj = s_jj;
                 int a = ii + k - 1, b = jj + k - 1;
@@ -516,7 +537,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 short q = A1(PQ, k); A1(PQ, k) = A1(PQ, a); A1(PQ, a) = q;
                 swapc(k, b);
             }
-            __syncthreads();
+            bsync(wave);
             cw = croot(k);
             ukk = A1(Z, k) * vld(k) / fd_r<SQ>(A1(F, k), A1(PL, k), A1(PUs, k), cw);
             amax = fmax(amax, fabs(ukk));
@@ -534,7 +555,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         const int pun = fuse ? A1(PUs, k + 1) : puk;
         const RootG cn = fuse ? croot(k + 1) : cw;
         VI x1 = {0.0, 0};
-        for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
+        for (int j = k + 1 + tid; j <= M; j += NTH) {
             double temp = fd_r<SQ>(A1(F, j), A1(PL, j), puk, ck);
             temp = ppq<SQ>(A1(F, j), Fk) / temp;
             const double zj = A1(Z, j) * temp;
@@ -557,7 +578,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         // reason the factors of W(k) go to T1 and T2 on alternate steps (thread 0 may
         // still read the previous step's; the swap check uses both only after a barrier)
         double *Tk = (k & 1) ? T2 : T1;
-        for (int j = k - 1 - tid; j >= 1; j -= HSS_CWG) {
+        for (int j = k - 1 - tid; j >= 1; j -= NTH) {
             double temp = -1.0 * fd_r<SQ>(A1(F, j), A1(PL, j), puk, ck);
             A1(W, j) = A1(W, j) * (temp / ppq<SQ>(Fk, A1(F, j)));
             // the factors of W(k) (their gathers in parallel; the product stays in order)
@@ -572,24 +593,24 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         // fused step needs no other; before a swap check, and at the last step, W(k) is
         // published here.
         if (fuse) {
-            block_argmax2_t(x1, y1, red, tog);
+            block_argmax2_t(x1, y1, red, tog, wave);
             pre_cl = x1;
             pre_rw = y1;
             cnext = cn;
         } else {
-            __syncthreads();
+            bsync(wave);
         }
         if (tid == 0) {
             double wk = fd_r<SQ>(Fk, plk, puk, ck) / A1(A, k);
             for (int j = 1; j <= k - 1; j++) wk = wk * A1(Tk, j);
             A1(W, k) = wk;
         }
-        if (!fuse) __syncthreads();
+        if (!fuse) bsync(wave);
         // ---- every 10 steps: swap while max |Mat_N| > rho (searchMax2Eig)
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
                 VI x = {0.0, 0};
-                for (int jc = 1 + tid; jc <= k; jc += HSS_CWG) {
+                for (int jc = 1 + tid; jc <= k; jc += NTH) {
                     double fj = A1(F, jc), mv = 0.0;
                     int mi = 0;
                     for (int i = 1; i <= M - k; i++) {
@@ -602,11 +623,11 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     if (nb.i == jc) IP[jc - 1] = (short)mi;
                     x = nb;
                 }
-                VI win = block_argmax_t(x, red, tog);
+                VI win = block_argmax_t(x, red, tog, wave);
                 if (!(win.v > rho)) break;
                 const int ii = win.i, jj = IP[ii - 1] + k;
                 fl += 10ull * (N - k) + 8ull * M + 18ull * k;
-                __syncthreads();
+                bsync(wave);
                 const double Fii = A1(F, ii), Fjj = A1(F, jj);
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
                 cpass(k + 1, [&](int j, int pw, double v, const RootG &c) {
@@ -615,11 +636,11 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     temp = temp / temp2;
                     if (j <= N) vst(j, v * temp);
                 });
-                for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
+                for (int j = k + 1 + tid; j <= M; j += NTH) {
                     if (j == jj) continue;
                     A1(Z, j) = A1(Z, j) * (ppq<SQ>(A1(F, j), Fjj) / ppq<SQ>(A1(F, j), Fii));
                 }
-                for (int j = 1 + tid; j <= k; j += HSS_CWG) {
+                for (int j = 1 + tid; j <= k; j += NTH) {
                     if (j == ii) continue;
                     A1(W, j) = A1(W, j) * (ppq<SQ>(A1(F, j), Fii) / ppq<SQ>(A1(F, j), Fjj));
                     const int puj = A1(PUs, j);
@@ -629,7 +650,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     temp = -1.0 * fd_r<SQ>(Fjj, pljj, puj, cj);
                     A1(T2, j) = temp / ppq<SQ>(A1(F, j), Fjj);
                 }
-                __syncthreads();
+                bsync(wave);
                 if (tid == 0) {
                     const int puii = A1(PUs, ii);
                     const RootG cii = croot(ii);
@@ -647,7 +668,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                     double t = A1(F, ii); A1(F, ii) = A1(F, jj); A1(F, jj) = t;
                     t = A1(A, ii); A1(A, ii) = A1(A, jj); A1(A, jj) = t;
                 }
-                __syncthreads();
+                bsync(wave);
             }
         }
     }
@@ -667,13 +688,13 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         H.usz[2 * (node - 1) + 1] = s_Rk;
         H.pu[node - 1] = s_off;
     }
-    __syncthreads();
+    bsync(wave);
     const long off = s_off;
     if (off >= 0) {
         double *Ug = H.arena + off;
         const int r = s_Rk;
         if (Rk < mn) {
-            for (int e = tid; e < M * r; e += HSS_CWG) {
+            for (int e = tid; e < M * r; e += NTH) {
                 int i = e % M + 1, j = e / M + 1;
                 int src = IP[i - 1];  // row src of Q
                 double q;
@@ -682,14 +703,14 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
                 Ug[e] = q;
             }
         } else {
-            for (int e = tid; e < M * M; e += HSS_CWG) {
+            for (int e = tid; e < M * M; e += NTH) {
                 int i = e % M + 1, j = e / M;
                 Ug[e] = (IP[i - 1] - 1 == j) ? 1.0 : 0.0;
             }
         }
     }
     // permuted row generators and their indices back to the compacted arrays
-    for (int i = tid; i < M; i += HSS_CWG) {
+    for (int i = tid; i < M; i += NTH) {
         A1(H.U, l0 + i) = A[i];
         A1(H.PL, l0 + i) = PL[i];
     }
@@ -749,6 +770,11 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     const bool vl = N <= lds_cap;  // all three uniform over the block
     const bool pl = (long)N + (N + 1) / 2 <= lds_cap;
     const bool sm = 2L * N + (N + 1) / 2 + (long)RD * M <= lds_cap;
+    // A node whose data all sit in LDS and whose outside part is small runs on one
+    // wavefront: its steps need no block barrier (the other waves leave here)
+    const bool wave = sm && N <= 384;
+    if (wave && tid >= 64) return;
+    const int NTH = wave ? 64 : HSS_CWG;
     ldbl *Vl = (ldbl *)hss_dyn;
     ldbl *FFl = Vl + NS;     // sm: FF(PU(j))
     ldbl *RRl = FFl + NS;    // sm: f0, f1, dl, dr (and s) of the node's M columns, M apart
@@ -773,7 +799,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                          SQ ? RRl[4 * M + i - 1] : 0.0};
         return root_load<SQ>(H, A1(PL, i));
     };
-    for (int j = tid; j < N; j += HSS_CWG) {
+    for (int j = tid; j < N; j += NTH) {
         int src = (j < r0 - 1) ? j + 1 : j + 1 + (r1e - r0 + 1);
         vst(j + 1, A1((gdbl *)H.U, src));
         const int p = A1((gcint *)H.PL, src);
@@ -802,7 +828,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     auto cload = [&](int j0, int *pd, double *v) {
         #pragma unroll
         for (int u = 0; u < RB; u++) {
-            const int j = min(j0 + u * HSS_CWG, N);
+            const int j = min(j0 + u * NTH, N);
             pd[u] = pu(j);
             v[u] = vld(j);
         }
@@ -810,14 +836,14 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     auto bload = [&](int j0, int *pd, double *v, double *ffd) {
         cload(j0, pd, v);
         #pragma unroll
-        for (int u = 0; u < RB; u++) ffd[u] = sm ? A1(FFl, min(j0 + u * HSS_CWG, N)) : A1(FF, pd[u]);
+        for (int u = 0; u < RB; u++) ffd[u] = sm ? A1(FFl, min(j0 + u * NTH, N)) : A1(FF, pd[u]);
     };
     // body(j, pd, v, FF(pd)) for the outside rows j in [lo, N]; j may exceed N
     auto cpass = [&](int lo, auto body) {
         int j0 = lo + tid;
         if (j0 > N) return;
         if (sm) {
-            for (int j = j0; j <= N; j += HSS_CWG) body(j, A1(PUl, j), A1(Vl, j), A1(FFl, j));
+            for (int j = j0; j <= N; j += NTH) body(j, A1(PUl, j), A1(Vl, j), A1(FFl, j));
             return;
         }
         int pd[RB];
@@ -827,12 +853,12 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             double ffd[RB];
             #pragma unroll
             for (int u = 0; u < RB; u++) ffd[u] = A1(FF, pd[u]);
-            const int jn = j0 + RB * HSS_CWG;
+            const int jn = j0 + RB * NTH;
             int pn[RB];
             double vn[RB];
             if (jn <= N) cload(jn, pn, vn);
             #pragma unroll
-            for (int u = 0; u < RB; u++) body(j0 + u * HSS_CWG, pd[u], v[u], ffd[u]);
+            for (int u = 0; u < RB; u++) body(j0 + u * NTH, pd[u], v[u], ffd[u]);
             if (jn > N) break;
             j0 = jn;
             #pragma unroll
@@ -850,11 +876,11 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
     // max over i in [k, M] of |Z(i) / fd(pd, PL(i))|, ffd = FF(pd)
     auto rowsearch = [&](int k, int pd, double ffd) -> VI {
         VI x = {0.0, 0};
-        for (int i = k + tid; i <= M; i += HSS_CWG)
+        for (int i = k + tid; i <= M; i += NTH)
             x = vi_better(x, VI{fabs(A1(Z, i) / fd_r<SQ>(ffd, pd, A1(PL, i), rroot(i))), i - k + 1});
         return x;
     };
-    for (int i = tid; i < M; i += HSS_CWG) {
+    for (int i = tid; i < M; i += NTH) {
         Dr[i] = A1((gdbl *)H.D, c0 + i);
         A[i] = A1((gdbl *)H.V, c0 + i);
         PL[i] = A1((gcint *)H.PU, c0 + i);
@@ -867,7 +893,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             if (SQ) RRl[4 * M + i] = r.s;
         }
     }
-    __syncthreads();
+    bsync(wave);
     double amax = 0.0;
     int Rk = mn;
     const double rho = 1.5;
@@ -896,7 +922,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 cr = rroot(k);
                 cl = rowsearch(k, puk, ffp);
                 rw = colsearch(k, plk, cr);
-                block_argmax2_t(cl, rw, red, tog);
+                block_argmax2_t(cl, rw, red, tog, wave);
             }
             double junkL = cl.v * fabs(vld(k)), junkU = rw.v * fabs(A1(Z, k));
             double den;
@@ -912,13 +938,13 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 int jjL = cl.i, jjU = rw.i;
                 // barriers as in the row compression: one before the first swap, then a
                 // row swap after the column search it feeds
-                __syncthreads();
+                bsync(wave);
                 for (;;) {
                     if (flg == 1) {
                         const int jr = jjL + k - 1;
                         const int plr = A1(PL, jr);
                         cr = rroot(jr);
-                        jjU = block_argmax_t(colsearch(k, plr, cr), red, tog).i;
+                        jjU = block_argmax_t(colsearch(k, plr, cr), red, tog, wave).i;
                         fl += 4ull * (N - k + 1);
                         if (tid == 0) {
                             double t;
@@ -930,15 +956,15 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                             swapr(k, jr);
                         }
                         if (jjU == 1) {
-                            __syncthreads();
+                            bsync(wave);
                             break;
                         }
                     }
                     if (tid == 0) swapc(k, jjU + k - 1);
-                    __syncthreads();
+                    bsync(wave);
                     const int puk2 = A1(PUs, k);
                     ffp = A1(FF, puk2);
-                    jjL = block_argmax_t(rowsearch(k, puk2, ffp), red, tog).i;
+                    jjL = block_argmax_t(rowsearch(k, puk2, ffp), red, tog, wave).i;
                     fl += 4ull * (M - k + 1);
                     if (jjL == 1) break;
                     flg = 1;
@@ -954,7 +980,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
             fl += 4ull * (M - k + 1) * (N - k + 1);
             VI best = {0.0, 0};
             int bestrow = 0;
-            for (int j0 = k + tid; j0 <= N; j0 += RB * HSS_CWG) {
+            for (int j0 = k + tid; j0 <= N; j0 += RB * NTH) {
                 int pd[RB];
                 double v[RB], ffd[RB], mv[RB];
                 int mi[RB];
@@ -975,7 +1001,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 }
                 #pragma unroll
                 for (int u = 0; u < RB; u++) {
-                    const int j = j0 + u * HSS_CWG;
+                    const int j = j0 + u * NTH;
                     if (j > N) continue;
                     VI c = {fabs(mv[u] * v[u]), j - k + 1};
                     VI nb = vi_better(best, c);
@@ -983,9 +1009,9 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     best = nb;
                 }
             }
-            VI win = block_argmax_t(best, red, tog);
+            VI win = block_argmax_t(best, red, tog, wave);
             if (win.i > 0 && best.i == win.i) { s_ii = bestrow; s_jj = win.i; }
-            __syncthreads();
+            bsync(wave);
             if (tid == 0) {
                 int a = s_ii + k - 1, b = s_jj + k - 1;
                 double t;
@@ -997,7 +1023,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 swapr(k, a);
                 swapc(k, b);
             }
-            __syncthreads();
+            bsync(wave);
             cr = rroot(k);
             ffp = A1(FF, A1(PUs, k));
             ukk = A1(Z, k) * vld(k) / (-1.0 * fd_r<SQ>(ffp, A1(PUs, k), A1(PL, k), cr));
@@ -1013,7 +1039,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         const RootG ck1 = fuse ? rroot(k + 1) : ck;
         const double ffn = A1(FF, pun);
         VI x1 = {0.0, 0};
-        for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
+        for (int j = k + 1 + tid; j <= M; j += NTH) {
             const int plj = A1(PL, j);
             const RootG rj = rroot(j);
             double temp = dd_r<SQ>(plj, rj, plk, ck);
@@ -1035,7 +1061,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         // j = k - 1 on thread 0, and the factors of W(k) in T1 or T2 by the step's
         // parity, as in the row compression
         double *Tk = (k & 1) ? T2 : T1;
-        for (int j = k - 1 - tid; j >= 1; j -= HSS_CWG) {
+        for (int j = k - 1 - tid; j >= 1; j -= NTH) {
             const int plj = A1(PL, j), puj = A1(PUs, j);
             const RootG rj = rroot(j);
             double temp = fd_r<SQ>(Fk, puk, plj, rj);
@@ -1051,27 +1077,27 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         pre = fuse;
         // as in the row compression: the reduction's barrier publishes the update pass
         if (fuse) {
-            block_argmax2_t(x1, y1, red, tog);
+            block_argmax2_t(x1, y1, red, tog, wave);
             pre_cl = x1;
             pre_rw = y1;
             cnext = ck1;
             ffnext = ffn;
         } else {
-            __syncthreads();
+            bsync(wave);
         }
         if (tid == 0) {
             double wk = (-1.0 * fd_r<SQ>(Fk, puk, plk, ck)) / A1(A, k);
             for (int j = 1; j <= k - 1; j++) wk = wk * A1(Tk, j);
             A1(W, k) = wk;
         }
-        if (!fuse) __syncthreads();
+        if (!fuse) bsync(wave);
         if (k % 10 == 0 && M - k > 0) {
             for (;;) {
                 // the scanned rows' root data, gathered once into T1 / T2 when they fit
                 const int L = M - k;
                 const bool stg = !sm && L * (SQ ? 5 : 4) <= 2 * HSS_MAXM;
                 if (stg) {
-                    for (int q = tid; q < L; q += HSS_CWG) {
+                    for (int q = tid; q < L; q += NTH) {
                         const RootG r = root_load<SQ>(H, A1(PL, k + 1 + q));
                         T12[q] = r.f0;
                         T12[L + q] = r.f1;
@@ -1079,10 +1105,10 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                         T12[3 * L + q] = r.dr;
                         if (SQ) T12[4 * L + q] = r.s;
                     }
-                    __syncthreads();
+                    bsync(wave);
                 }
                 VI x = {0.0, 0};
-                for (int jc = 1 + tid; jc <= k; jc += HSS_CWG) {
+                for (int jc = 1 + tid; jc <= k; jc += NTH) {
                     int pljc = A1(PL, jc);
                     const RootG rc = rroot(jc);
                     double mv = 0.0;
@@ -1106,11 +1132,11 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     if (nb.i == jc) IP[jc - 1] = (short)mi;
                     x = nb;
                 }
-                VI win = block_argmax_t(x, red, tog);
+                VI win = block_argmax_t(x, red, tog, wave);
                 if (!(win.v > rho)) break;
                 const int ii = win.i, jj = IP[ii - 1] + k;
                 fl += 10ull * (N - k) + 12ull * M + 22ull * k;
-                __syncthreads();
+                bsync(wave);
                 const int plii = A1(PL, ii), pljj = A1(PL, jj);
                 const RootG cii = rroot(ii), cjj = rroot(jj);
                 cpass(k + 1, [&](int j, int pd, double v, double ffd) {
@@ -1119,14 +1145,14 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     temp = temp / temp2;
                     if (j <= N) vst(j, v * temp);
                 });
-                for (int j = k + 1 + tid; j <= M; j += HSS_CWG) {
+                for (int j = k + 1 + tid; j <= M; j += NTH) {
                     if (j == jj) continue;
                     double temp = ddq<SQ>(H, A1(PL, j), pljj);
                     double temp2 = ddq<SQ>(H, A1(PL, j), plii);
                     temp = temp / temp2;
                     A1(Z, j) = A1(Z, j) * temp;
                 }
-                for (int j = 1 + tid; j <= k; j += HSS_CWG) {
+                for (int j = 1 + tid; j <= k; j += NTH) {
                     if (j == ii) continue;
                     const int puj = A1(PUs, j);
                     double temp = ddq<SQ>(H, A1(PL, j), plii);
@@ -1140,7 +1166,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     temp2 = ddq<SQ>(H, A1(PL, j), pljj);
                     A1(T2, j) = temp / temp2;
                 }
-                __syncthreads();
+                bsync(wave);
                 if (tid == 0) {
                     const int puii = A1(PUs, ii);
                     double temp = ddq<SQ>(H, plii, pljj);
@@ -1160,7 +1186,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                     double t = A1(Dr, ii); A1(Dr, ii) = A1(Dr, jj); A1(Dr, jj) = t;
                     t = A1(A, ii); A1(A, ii) = A1(A, jj); A1(A, jj) = t;
                 }
-                __syncthreads();
+                bsync(wave);
             }
         }
     }
@@ -1179,13 +1205,13 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         H.vsz[2 * (node - 1) + 1] = M;
         H.pv[node - 1] = s_off;
     }
-    __syncthreads();
+    bsync(wave);
     const long off = s_off;
     if (off >= 0) {
         double *Vg = H.arena + off;
         const int r = s_Rk;
         if (Rk < mn) {
-            for (int e = tid; e < r * M; e += HSS_CWG) {
+            for (int e = tid; e < r * M; e += NTH) {
                 int i = e % r + 1, j = e / r + 1;
                 int src = IP[j - 1];  // column src of Q
                 double q;
@@ -1202,13 +1228,13 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
                 Vg[e] = q;
             }
         } else {
-            for (int e = tid; e < M * M; e += HSS_CWG) {
+            for (int e = tid; e < M * M; e += NTH) {
                 int i = e % M, j = e / M + 1;
                 Vg[e] = (IP[j - 1] - 1 == i) ? 1.0 : 0.0;
             }
         }
     }
-    for (int i = tid; i < M; i += HSS_CWG) {
+    for (int i = tid; i < M; i += NTH) {
         A1(H.D, c0 + i) = Dr[i];
         A1(H.V, c0 + i) = A[i];
         A1(H.PU, c0 + i) = PL[i];

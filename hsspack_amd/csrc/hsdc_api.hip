@@ -179,6 +179,9 @@ struct hsdc_ctx {
     int evc_k = 0, evc_ni = 0, evc_sq = 0;
     double evc_tol = 0.0;
     PinBuf pin, pin2, pin3;
+    PinBuf pinh;                        // the HSS generator directories of one batch
+    PinBuf pinb;                        // the HSS multiply's GEMM item table
+    hipEvent_t pinb_ev = nullptr;       // recorded after its upload
     PinBuf stage;                       // staging of large device-to-host copies
     hipEvent_t stev[3] = {nullptr, nullptr, nullptr};
     Work W;
@@ -717,10 +720,20 @@ static int hss_mult(hsdc_ctx *c, const std::vector<HssSpec> &sp, std::vector<Hss
     for (auto &v : steps) nitems += v.size();
     CK(ensure(c->tabs, sizeof(BG) * std::max(nitems, (size_t)1)));
     {
-        std::vector<BG> all;
-        for (auto &v : steps) all.insert(all.end(), v.begin(), v.end());
-        HK(hipMemcpyAsync(c->tabs.p, all.data(), sizeof(BG) * all.size(), hipMemcpyHostToDevice, s));
-        HK(hipStreamSynchronize(s));
+        // staged in pinned memory; the host waits only until the previous multiply's
+        // table has left it (callers that multiply column block after column block do
+        // not synchronise in between)
+        if (c->pinb_ev) HK(hipEventSynchronize(c->pinb_ev));
+        CK(ensure_pin(c->pinb, sizeof(BG) * std::max(nitems, (size_t)1)));
+        BG *all = (BG *)c->pinb.p;
+        size_t o = 0;
+        for (auto &v : steps) {
+            memcpy(all + o, v.data(), sizeof(BG) * v.size());
+            o += v.size();
+        }
+        HK(hipMemcpyAsync(c->tabs.p, all, sizeof(BG) * nitems, hipMemcpyHostToDevice, s));
+        if (!c->pinb_ev) HK(hipEventCreateWithFlags(&c->pinb_ev, hipEventDisableTiming));
+        HK(hipEventRecord(c->pinb_ev, s));
     }
     // flops of the multiply (2 nrow n k per GEMM: fastHssmmL's DGEMM calls exactly)
     for (auto &v : steps)
@@ -761,9 +774,11 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
     const int sq = sp.empty() ? 0 : sp[0].sq;
     for (const HssSpec &q : sp)
         if (q.sq != sq) return HSDC_ERR_INTERNAL;
-    // ---- device memory for every merge
-    size_t tot = 0;
-    std::vector<size_t> base(sp.size()), metasz(sp.size());
+    // ---- device memory for every merge: the generator directories (node ranges,
+    // generator shapes and offsets, counters) of all merges side by side at the front,
+    // cleared by one fill and read back by one copy; then each merge's buffers
+    size_t tot = 0, mtot = 0;
+    std::vector<size_t> base(sp.size()), mbase(sp.size());
     for (size_t h = 0; h < sp.size(); h++) {
         P[h].li = (int)h;
         int K = sp[h].K;
@@ -771,21 +786,23 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         int ltr = std::max(P[h].T.ltr, 1);
         long dds = 0;
         for (int x : P[h].T.m) dds += (long)x * x;
-        size_t o = al64(6 * sizeof(double) * K) + al64(2 * sizeof(int) * K) + al64(sizeof(int) * (11 * ltr + 8)) +
-                   al64(sizeof(long) * (4 * ltr)) + al64(sizeof(unsigned long long) * 3);
-        metasz[h] = o;
+        mbase[h] = mtot;
+        mtot += al64(sizeof(int) * (11 * ltr + 8)) + al64(sizeof(long) * (4 * ltr)) +
+                al64(sizeof(unsigned long long) * 3);
+        size_t o = al64(6 * sizeof(double) * K) + al64(2 * sizeof(int) * K);
         o += al64(sizeof(double) * std::max(dds, 1L));
         o += al64(sizeof(double) * (size_t)12 * std::max(ni, 128) * K);  // arena
         o += al64(sizeof(double) * (size_t)8 * K * std::max(P[h].T.lm, 1));  // scratch
         base[h] = tot;
         tot += o;
     }
-    CK(ensure(c->hss, tot));
+    mtot = al64(std::max(mtot, (size_t)64));
+    CK(ensure(c->hss, mtot + tot));
     char *hb = (char *)c->hss.p;
     for (size_t h = 0; h < sp.size(); h++) {
         HssPlanM &Q = P[h];
         int K = sp[h].K, ltr = std::max(Q.T.ltr, 1);
-        char *p = hb + base[h];
+        char *p = hb + mtot + base[h];
         HssMerge &H = Q.H;
         H.FF = sp[h].FF; H.D0 = sp[h].D0; H.U0 = sp[h].U0; H.V0 = sp[h].V0;
         H.DIFL = sp[h].DIFL; H.DIFR = sp[h].DIFR;
@@ -795,17 +812,15 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         int *ip = (int *)p;
         H.PL = ip; H.PU = ip + K;
         p += al64(2 * sizeof(int) * K);
-        ip = (int *)p;
-        char *meta0 = p;
+        char *mp = hb + mbase[h];
+        ip = (int *)mp;
         H.lr = ip; H.lc = ip + 2 * ltr; H.usz = ip + 4 * ltr; H.vsz = ip + 6 * ltr; H.bsz = ip + 8 * ltr;
         H.dsz = ip + 10 * ltr; H.lda = ip + 11 * ltr; H.lca = ip + 11 * ltr + 1; H.err = ip + 11 * ltr + 2;
-        p += al64(sizeof(int) * (11 * ltr + 8));
-        long *lp = (long *)p;
+        mp += al64(sizeof(int) * (11 * ltr + 8));
+        long *lp = (long *)mp;
         H.pu = lp; H.pv = lp + ltr; H.pb = lp + 2 * ltr; H.pd = lp + 3 * ltr;
-        p += al64(sizeof(long) * (4 * ltr));
-        H.arena_top = (unsigned long long *)p; H.scratch_top = H.arena_top + 1; H.flops = H.arena_top + 2;
-        p += al64(sizeof(unsigned long long) * 3);
-        HK(hipMemsetAsync(meta0, 0, p - meta0, s));
+        mp += al64(sizeof(long) * (4 * ltr));
+        H.arena_top = (unsigned long long *)mp; H.scratch_top = H.arena_top + 1; H.flops = H.arena_top + 2;
         H.dd = (double *)p;
         long dds = 0;
         for (int x : Q.T.m) dds += (long)x * x;
@@ -870,39 +885,43 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         step_cnt[st] = nodes.size() - step_off[st];
         comp_cnt[st] = comps.size() - comp_off[st];
     }
-    // initial leaf ranges and D offsets go straight into each merge's arrays
-    for (int t = 0; t < nh; t++) {
-        const HssTree &T = P[hmap[t]].T;
-        std::vector<int> lrlc(4 * T.ltr, 0);
-        std::vector<long> pd(T.ltr, 0);
-        long pdt = 0;
-        int it = 0;
-        for (int i = 1; i <= T.ltr; i++)
-            if (T.c1[i] == 0) {
-                lrlc[2 * (i - 1)] = T.leaf_r0[i] + 1;
-                lrlc[2 * (i - 1) + 1] = T.leaf_r0[i] + T.m[it];
-                pd[i - 1] = pdt;
-                pdt += (long)T.m[it] * T.m[it];
-                it++;
-            }
-        for (int i = 0; i < 2 * T.ltr; i++) lrlc[2 * T.ltr + i] = lrlc[i];
-        HK(hipMemcpyAsync(hh[t].lr, lrlc.data(), sizeof(int) * 4 * T.ltr, hipMemcpyHostToDevice, s));
-        HK(hipMemcpyAsync(hh[t].pd, pd.data(), sizeof(long) * T.ltr, hipMemcpyHostToDevice, s));
-        HK(hipStreamSynchronize(s));
-    }
+    // The directory area's first state (zeros, with every merge's leaf ranges and D
+    // offsets) and the merge / node / compaction tables, staged in pinned memory: two
+    // copies and no wait (the pinned area is next written by this call's readback, after
+    // the stream has passed both copies)
     size_t t0 = 0, t1 = al64(t0 + sizeof(HssMerge) * std::max(nh, 1)),
            t2 = al64(t1 + sizeof(HssNode) * std::max(nodes.size(), (size_t)1)),
            t3 = al64(t2 + sizeof(HssCompact) * std::max(comps.size(), (size_t)1)),
            t4 = al64(t3 + sizeof(int) * std::max(order.size(), (size_t)1));
     CK(ensure(c->htab, t4));
+    CK(ensure_pin(c->pinh, mtot + t4));
     {
-        std::vector<char> hbuf(t4);
-        memcpy(hbuf.data() + t0, hh.data(), sizeof(HssMerge) * nh);
-        memcpy(hbuf.data() + t1, nodes.data(), sizeof(HssNode) * nodes.size());
-        if (!comps.empty()) memcpy(hbuf.data() + t2, comps.data(), sizeof(HssCompact) * comps.size());
-        if (!order.empty()) memcpy(hbuf.data() + t3, order.data(), sizeof(int) * order.size());
-        HK(hipMemcpyAsync(c->htab.p, hbuf.data(), t4, hipMemcpyHostToDevice, s));
-        HK(hipStreamSynchronize(s));
+        char *img = (char *)c->pinh.p;
+        memset(img, 0, mtot);
+        for (int t = 0; t < nh; t++) {
+            const HssTree &T = P[hmap[t]].T;
+            const int ltr1 = std::max(T.ltr, 1);
+            int *lrlc = (int *)(img + mbase[hmap[t]]);
+            long *pd = (long *)(img + mbase[hmap[t]] + al64(sizeof(int) * (11 * ltr1 + 8))) + 3 * ltr1;
+            long pdt = 0;
+            int it = 0;
+            for (int i = 1; i <= T.ltr; i++)
+                if (T.c1[i] == 0) {
+                    lrlc[2 * (i - 1)] = T.leaf_r0[i] + 1;
+                    lrlc[2 * (i - 1) + 1] = T.leaf_r0[i] + T.m[it];
+                    pd[i - 1] = pdt;
+                    pdt += (long)T.m[it] * T.m[it];
+                    it++;
+                }
+            for (int i = 0; i < 2 * T.ltr; i++) lrlc[2 * T.ltr + i] = lrlc[i];
+        }
+        char *hbuf = img + mtot;
+        memcpy(hbuf + t0, hh.data(), sizeof(HssMerge) * nh);
+        memcpy(hbuf + t1, nodes.data(), sizeof(HssNode) * nodes.size());
+        if (!comps.empty()) memcpy(hbuf + t2, comps.data(), sizeof(HssCompact) * comps.size());
+        if (!order.empty()) memcpy(hbuf + t3, order.data(), sizeof(int) * order.size());
+        HK(hipMemcpyAsync(hb, img, mtot, hipMemcpyHostToDevice, s));
+        HK(hipMemcpyAsync(c->htab.p, hbuf, t4, hipMemcpyHostToDevice, s));
     }
     char *tb = (char *)c->htab.p;
     const HssMerge *dH = (const HssMerge *)(tb + t0);
@@ -964,17 +983,19 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         HK(hipGetLastError());
     }
     if (b1) HK(hipEventRecord(b1, s));
-    // ---- read back the generator directory
+    // ---- read back the generator directories (one copy for every merge)
+    CK(ensure_pin(c->pinh, mtot));
+    HK(hipMemcpyAsync(c->pinh.p, hb, mtot, hipMemcpyDeviceToHost, s));
+    HK(hipStreamSynchronize(s));
     for (int t = 0; t < nh; t++) {
         HssPlanM &Q = P[hmap[t]];
         int ltr = Q.T.ltr;
-        std::vector<int> meta(11 * ltr + 8);
-        std::vector<long> offs(4 * ltr);
-        HK(hipMemcpyAsync(meta.data(), hh[t].lr, sizeof(int) * meta.size(), hipMemcpyDeviceToHost, s));
-        HK(hipMemcpyAsync(offs.data(), hh[t].pu, sizeof(long) * offs.size(), hipMemcpyDeviceToHost, s));
-        unsigned long long nfl = 0;
-        HK(hipMemcpyAsync(&nfl, hh[t].flops, sizeof(nfl), hipMemcpyDeviceToHost, s));
-        HK(hipStreamSynchronize(s));
+        const char *mh = (const char *)c->pinh.p + mbase[hmap[t]];
+        std::vector<int> meta((const int *)mh, (const int *)mh + 11 * ltr + 8);
+        mh += al64(sizeof(int) * (11 * std::max(ltr, 1) + 8));
+        std::vector<long> offs((const long *)mh, (const long *)mh + 4 * ltr);
+        mh += al64(sizeof(long) * (4 * std::max(ltr, 1)));
+        const unsigned long long nfl = ((const unsigned long long *)mh)[2];
         c->st.hss_nflops += (long long)nfl;
         Q.usz.assign(meta.begin() + 4 * ltr, meta.begin() + 6 * ltr);
         Q.vsz.assign(meta.begin() + 6 * ltr, meta.begin() + 8 * ltr);
@@ -1000,7 +1021,7 @@ static int hss_apply(hsdc_ctx *c, const std::vector<HssSpec> &sp, int ni, double
         (void)hipEventDestroy(b1);
     }
     CK(hss_mult(c, sp, P, left));
-    if (keep) *keep = P;
+    if (keep) *keep = std::move(P);
     return 0;
 }
 
@@ -1888,6 +1909,10 @@ int hsdc_ctx_destroy(hsdc_ctx *c)
     if (c->pin.p) (void)hipHostFree(c->pin.p);
     if (c->pin2.p) (void)hipHostFree(c->pin2.p);
     if (c->pin3.p) (void)hipHostFree(c->pin3.p);
+    if (c->pinh.p) (void)hipHostFree(c->pinh.p);
+    if (c->pinb_ev) (void)hipEventSynchronize(c->pinb_ev);
+    if (c->pinb.p) (void)hipHostFree(c->pinb.p);
+    if (c->pinb_ev) (void)hipEventDestroy(c->pinb_ev);
     if (c->stage.p) (void)hipHostFree(c->stage.p);
     for (hipEvent_t e : c->stev) if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->pev) if (e) (void)hipEventDestroy(e);

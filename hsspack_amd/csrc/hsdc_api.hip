@@ -465,8 +465,17 @@ struct HssPlanM {
 // Every tile runs the k-steps in the same order, so all give the same bits.
 static void bgemm_step(const BG *items, unsigned ni, int rmax, int nmax, hipStream_t s)
 {
-    if (nmax <= 64) {
+    if (nmax <= 32) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<128, 32, 16, 1, 1, 2>), dim3(cdiv(rmax, 128), ni, 1), dim3(512), 0,
+                           s, items, 1);
+    } else if (nmax <= 48) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<128, 48, 16, 1, 1, 3>), dim3(cdiv(rmax, 128), ni, 1), dim3(512), 0,
+                           s, items, 1);
+    } else if (nmax <= 64) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<128, 64, 16, 1, 1, 4>), dim3(cdiv(rmax, 128), ni, 1), dim3(512), 0,
+                           s, items, 1);
+    } else if (nmax <= 96) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<128, 96, 16, 1, 2, 3>), dim3(cdiv(rmax, 128), ni, 1), dim3(512), 0,
                            s, items, 1);
     } else if (nmax <= 128) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgemm_w<128, 128, 16, 1, 2, 4>), dim3(cdiv(rmax, 128), ni, 1), dim3(512),

@@ -1601,6 +1601,9 @@ __global__ __launch_bounds__((BM / (16 * TA)) * (BN / (16 * TB)) * 64) void k_bg
 template __global__ void k_bgemm_w<256, 128, 16, 1, 2, 4>(const BG *, int);
 template __global__ void k_bgemm_w<128, 128, 16, 1, 2, 4>(const BG *, int);
 template __global__ void k_bgemm_w<128, 64, 16, 1, 1, 4>(const BG *, int);
+template __global__ void k_bgemm_w<128, 48, 16, 1, 1, 3>(const BG *, int);
+template __global__ void k_bgemm_w<128, 32, 16, 1, 1, 2>(const BG *, int);
+template __global__ void k_bgemm_w<128, 96, 16, 1, 2, 3>(const BG *, int);
 template __global__ void k_bgemm_w<256, 144, 16, 1, 2, 9>(const BG *, int);
 
 // X(:, t) = Q2(:, t): the K surviving old eigenvectors of each HSS merge packed

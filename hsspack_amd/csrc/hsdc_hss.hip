@@ -202,6 +202,21 @@ __device__ __forceinline__ void bsync(bool wave)
         __syncthreads();
     }
 }
+// x * t[0] * t[1] * ... * t[n - 1], multiplied in that order (one thread), with the
+// factors read from LDS eight at a time ahead of their multiplications
+__device__ __forceinline__ double ordered_prod(double x, const double *t, int n)
+{
+    int j = 0;
+    for (; j + 8 <= n; j += 8) {
+        double f[8];
+        #pragma unroll
+        for (int u = 0; u < 8; u++) f[u] = t[j + u];
+        #pragma unroll
+        for (int u = 0; u < 8; u++) x = x * f[u];
+    }
+    for (; j < n; j++) x = x * t[j];
+    return x;
+}
 
 // ---------------------------------------------------------------------------
 // leaf D blocks (CauchylikeEig on the leaf rows/cols, CauchyHssEig.f90:2089-2100)
@@ -602,7 +617,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_rowcomp(const HssMerge *hm, con
         }
         if (tid == 0) {
             double wk = fd_r<SQ>(Fk, plk, puk, ck) / A1(A, k);
-            for (int j = 1; j <= k - 1; j++) wk = wk * A1(Tk, j);
+            wk = ordered_prod(wk, Tk, k - 1);
             A1(W, k) = wk;
         }
         if (!fuse) bsync(wave);
@@ -1087,7 +1102,7 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
         }
         if (tid == 0) {
             double wk = (-1.0 * fd_r<SQ>(Fk, puk, plk, ck)) / A1(A, k);
-            for (int j = 1; j <= k - 1; j++) wk = wk * A1(Tk, j);
+            wk = ordered_prod(wk, Tk, k - 1);
             A1(W, k) = wk;
         }
         if (!fuse) bsync(wave);

@@ -413,8 +413,12 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
     constexpr int NW = NT / 64;  // wavefronts
     extern __shared__ double dsl[];  // ds_cap doubles: sorted D for the replay searches
     __shared__ int sk, sk2, scnt[4], wcnt[4][NW], sL, sfirst, sfallback, snseg;
+    __shared__ int sng;
     __shared__ int seg_a[MAXSEG], seg_b[MAXSEG], seg_nx[MAXSEG], seg_pos[MAXSEG], seg_ord[MAXSEG];
     __shared__ double seg_d[MAXSEG];
+    // survivor positions with small-z columns before them, for the replay; seg_ord is
+    // written only after the replay
+    int *const gq = seg_ord;
     __shared__ double cz[DCH], cd[DCH];
     __shared__ int ci[DCH], cty[DCH], lsn[DCH];
     __shared__ int snr, snrun;
@@ -657,6 +661,37 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
         if (snr <= DCH)
             for (int u = tid; u < snr; u += NT) { cty[u] = W.evq[s + u]; cz[u] = W.evd[s + u]; ci[u] = W.rot_pj[s + u]; }
         __syncthreads();
+        // with many survivors, the positions q (1 <= q < L) with small-z columns between
+        // survivor q-1 and q, in ascending order: the replay then visits only these and
+        // the rotation events (a walk over every survivor costs more only above a few
+        // hundred)
+        if (snr != 0 && L > 256) {
+            const int Lq = L - 1, gper = (Lq + NW - 1) / NW;
+            const int g0 = 1 + min(Lq, wv * gper), g1 = 1 + min(Lq, (wv + 1) * gper);
+            int gc = 0;
+            for (int q0 = g0; q0 < g1; q0 += 64) {
+                const int q = q0 + lane;
+                const bool gp = q < g1 && nsl[q] > nsl[q - 1] + 1;
+                gc += __popcll(__ballot(gp));
+            }
+            if (lane == 0) wcnt[1][wv] = gc;
+            __syncthreads();
+            int gb = 0, gt = 0;
+            for (int w = 0; w < NW; w++) {
+                if (w < wv) gb += wcnt[1][w];
+                gt += wcnt[1][w];
+            }
+            for (int q0 = g0; q0 < g1; q0 += 64) {
+                const int q = q0 + lane;
+                const bool gp = q < g1 && nsl[q] > nsl[q - 1] + 1;
+                const unsigned long long m = __ballot(gp);
+                const int at = gb + __popcll(m & lt);
+                if (gp && at < MAXSEG) gq[at] = q;
+                gb += __popcll(m);
+            }
+            if (tid == 0) sng = gt;
+            __syncthreads();
+        }
     }
     if (tid == 0 && !sfallback) {
         W.dlamda[s + K] = DP; W.w[s + K] = zP; indxp[K] = PJ;
@@ -677,8 +712,24 @@ __global__ __launch_bounds__(NT) void k_deflate(const Merge *mg, Work W, int cur
         const int *nsr = (L <= DCH) ? lsn : nsl;
         const bool evl = nrot <= DCH;
         push_run(0, nsr[0]);
-        for (int q = 1; q < L && nseg <= MAXSEG; q++) {
-            push_run(nsr[q - 1] + 1, nsr[q]);
+        // survivor positions in ascending order, each with the run before it, then its
+        // rotation events; with the gap list (L > 256) only the positions that have either
+        const bool sparse = L > 256;
+        const int ng = sparse ? sng : 0;
+        if (ng > MAXSEG) nseg = MAXSEG + 1;
+        int gi = 0, qn = 1;
+        while (nseg <= MAXSEG) {
+            int q = qn++;
+            if (sparse) {
+                const int qg = gi < ng ? gq[gi] : L;
+                const int qe = ev < nrot ? (evl ? cty[ev] : W.evq[s + ev]) : L;
+                q = min(qg, qe);
+            }
+            if (q >= L) break;
+            if (!sparse || (gi < ng && gq[gi] == q)) {
+                push_run(nsr[q - 1] + 1, nsr[q]);
+                gi++;
+            }
             while (ev < nrot && (evl ? cty[ev] : W.evq[s + ev]) == q && nseg <= MAXSEG) {
                 double dx = evl ? cz[ev] : W.evd[s + ev];
                 int x = evl ? ci[ev] : W.rot_pj[s + ev];

@@ -37,6 +37,7 @@
 #define BKD_MAX 64    // largest KD: the coupling block and its right vectors stay in LDS
 #define BLEAF_MAX 128 // largest leaf: its m x m block stays in LDS
 #define DCH 2048      // sorted entries staged in LDS per chunk of the deflation scan
+#define SBW 1024      // words of the deflation pre-pass's pair flags (32768 survivors)
 
 __global__ void k_secular(const Merge *, const int *, int, const MergeOut *, Work, int, int *, const SecCtl *,
                           unsigned int *);
@@ -424,6 +425,9 @@ __global__ __launch_bounds__(NT) void k_sb_deflate(const SbMerge *mg, SbDefl A)
     __shared__ int nc[DCH];
     __shared__ double red[2][NT / 64];
     __shared__ int flag, s_k, s_nrot, s_cnt, s_rot, pre[NT];
+    // bit j: the pre-pass's rotation test of survivors j-1 and j passed (survivor
+    // numbers below 32 x SBW)
+    __shared__ unsigned int pfl[SBW];
     const SbMerge M = mg[blockIdx.x];
     const int s = M.s, n = M.n, n1 = M.n1, tid = threadIdx.x, up = A.up;
     const double rho0 = A.cuts[(long)M.cut * (2L * A.kd * A.kd + A.kd) + 2 * A.kd * A.kd + up];
@@ -495,13 +499,18 @@ __global__ __launch_bounds__(NT) void k_sb_deflate(const SbMerge *mg, SbDefl A)
         }
         __syncthreads();
         const int ns = s_cnt;
+        for (int w = tid; w < SBW; w += NT) pfl[w] = 0u;
+        __syncthreads();
         int any = 0;
         for (int j = 1 + tid; j < ns; j += NT) {
             const int rp = L[j - 1], rn = L[j];
             const double zp = A.srtz[s + rp], zn = A.srtz[s + rn];
             const double tau = hypot(zn, zp), t0 = A.srtd[s + rn] - A.srtd[s + rp];
             const double c = zn / tau, sn = -zp / tau;
-            any |= fabs(t0 * c * sn) <= tol;
+            if (fabs(t0 * c * sn) <= tol) {
+                any = 1;
+                if (j < 32 * SBW) atomicOr(&pfl[j >> 5], 1u << (j & 31));
+            }
         }
         if (any) atomicOr(&s_rot, 1);
         __syncthreads();
@@ -521,7 +530,11 @@ __global__ __launch_bounds__(NT) void k_sb_deflate(const SbMerge *mg, SbDefl A)
         }
     }
     if (!(rho * az <= tol) && s_rot) {  // a rotation happens: thread 0's scan in order
-        int k = 0, k2 = n, pj = -1, nrot = 0;
+        // While the pending survivor is the previous one as staged (no rotation since it
+        // was kept), the pair test is the pre-pass's: an unflagged pair keeps it without
+        // being evaluated again
+        int k = 0, k2 = n, pj = -1, nrot = 0, qs = 0;
+        bool mod = false;
         double zp = 0.0, dp = 0.0;
         for (int c0 = 0; c0 < n; c0 += DCH) {
             const int cnt = min(DCH, n - c0);
@@ -536,7 +549,14 @@ __global__ __launch_bounds__(NT) void k_sb_deflate(const SbMerge *mg, SbDefl A)
                     const int nj = nc[t];
                     const double zn = zc[t], dn = dc[t];
                     if (rho * fabs(zn) <= tol) { ip[--k2] = nj + 1; continue; }
-                    if (pj < 0) { pj = nj; zp = zn; dp = dn; continue; }
+                    if (pj < 0) { pj = nj; zp = zn; dp = dn; qs = 1; continue; }
+                    const bool skip = !mod && qs < 32 * SBW && !((pfl[qs >> 5] >> (qs & 31)) & 1u);
+                    qs++;
+                    if (skip) {
+                        A.dlamda[s + k] = dp; A.w[s + k] = zp; ip[k] = pj + 1; k++;
+                        pj = nj; zp = zn; dp = dn;
+                        continue;
+                    }
                     const double tau = hypot(zn, zp), t0 = dn - dp;
                     const double c = zn / tau, sn = -zp / tau;
                     if (fabs(t0 * c * sn) <= tol) {
@@ -558,9 +578,11 @@ __global__ __launch_bounds__(NT) void k_sb_deflate(const SbMerge *mg, SbDefl A)
                             break;
                         }
                         pj = nj; zp = tau; dp = dnn;
+                        mod = true;
                     } else {
                         A.dlamda[s + k] = dp; A.w[s + k] = zp; ip[k] = pj + 1; k++;
                         pj = nj; zp = zn; dp = dn;
+                        mod = false;
                     }
                 }
             }

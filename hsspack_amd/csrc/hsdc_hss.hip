@@ -1264,52 +1264,64 @@ __global__ __launch_bounds__(HSS_CWG) void k_hss_colcomp(const HssMerge *hm, con
 __global__ __launch_bounds__(WG) void k_hss_compact(const HssMerge *hm, const HssCompact *cp, const int *order,
                                                     int rows)
 {
-    __shared__ int s_start[1024 + 1];
+    // Every node's count and source start read in parallel, one thread per node; the
+    // start of each node's slice by a prefix over LDS; then one pass over all elements,
+    // each finding its node by binary search over the slice starts.
+    __shared__ int s_start[1024 + 1], s_src[1024];
     HssCompact C = cp[blockIdx.x];
     const HssMerge H = hm[C.mrg];
     const int tid = threadIdx.x;
     int *rng = rows ? H.lr : H.lc;
     if (*H.err) return;
+    if (C.count > 1024) {  // more nodes than the slice table holds: the merge goes dense
+        if (tid == 0) atomicOr(H.err, 1);
+        return;
+    }
+    for (int q = tid; q < C.count; q += WG) {
+        const int nd = order[C.start + q];
+        const int r0 = rng[2 * (nd - 1)], have = rng[2 * (nd - 1) + 1] - r0 + 1;
+        int cnt = have;
+        if (q < C.nlevel) cnt = rows ? H.usz[2 * (nd - 1) + 1] : H.vsz[2 * (nd - 1)];
+        if (cnt < 0 || cnt > have) cnt = 0;
+        s_start[q] = cnt;
+        s_src[q] = r0;  // 1-based
+    }
+    __syncthreads();
     if (tid == 0) {
         int lt = 0;
         for (int q = 0; q < C.count; q++) {
-            int nd = order[C.start + q];
-            int cnt;
-            if (q < C.nlevel) cnt = rows ? H.usz[2 * (nd - 1) + 1] : H.vsz[2 * (nd - 1)];
-            else cnt = rng[2 * (nd - 1) + 1] - rng[2 * (nd - 1)] + 1;
-            int have = rng[2 * (nd - 1) + 1] - rng[2 * (nd - 1)] + 1;
-            if (cnt < 0 || cnt > have) cnt = 0;
+            const int c = s_start[q];
             s_start[q] = lt;
-            lt += cnt;
+            lt += c;
         }
         s_start[C.count] = lt;
     }
     __syncthreads();
     double *Td = H.T, *Td2 = H.T + H.K;
     int *Ti = (int *)(H.T + 2 * H.K);
-    for (int q = 0; q < C.count; q++) {
-        int nd = order[C.start + q];
-        int src0 = rng[2 * (nd - 1)];  // 1-based
-        int cnt = s_start[q + 1] - s_start[q];
-        for (int t = tid; t < cnt; t += WG) {
-            int s = src0 + t, d = s_start[q] + t;
-            if (rows) { Td[d] = A1(H.U, s); Ti[d] = A1(H.PL, s); }
-            else { Td[d] = A1(H.V, s); Td2[d] = A1(H.D, s); Ti[d] = A1(H.PU, s); }
+    const int tot = s_start[C.count];
+    for (int d = tid; d < tot; d += WG) {
+        int lo = 0, hi = C.count - 1;
+        while (lo < hi) {
+            const int md = (lo + hi + 1) >> 1;
+            if (s_start[md] <= d) lo = md;
+            else hi = md - 1;
         }
+        const int sidx = s_src[lo] + (d - s_start[lo]);
+        if (rows) { Td[d] = A1(H.U, sidx); Ti[d] = A1(H.PL, sidx); }
+        else { Td[d] = A1(H.V, sidx); Td2[d] = A1(H.D, sidx); Ti[d] = A1(H.PU, sidx); }
     }
     __syncthreads();
-    const int tot = s_start[C.count];
     for (int d = tid; d < tot; d += WG) {
         if (rows) { H.U[d] = Td[d]; H.PL[d] = Ti[d]; }
         else { H.V[d] = Td[d]; H.D[d] = Td2[d]; H.PU[d] = Ti[d]; }
     }
-    __syncthreads();
+    for (int q = tid; q < C.count; q += WG) {
+        const int nd = order[C.start + q];
+        rng[2 * (nd - 1)] = s_start[q] + 1;
+        rng[2 * (nd - 1) + 1] = s_start[q + 1];
+    }
     if (tid == 0) {
-        for (int q = 0; q < C.count; q++) {
-            int nd = order[C.start + q];
-            rng[2 * (nd - 1)] = s_start[q] + 1;
-            rng[2 * (nd - 1) + 1] = s_start[q + 1];
-        }
         if (rows) *H.lda = tot; else *H.lca = tot;
     }
 }

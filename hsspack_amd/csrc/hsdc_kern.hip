@@ -1367,6 +1367,7 @@ __device__ __forceinline__ double delta_ij(const double *d, int i, const int *or
 // poles, tau and origins of a merge with K <= LW_LDS are staged in LDS first
 // (every thread walks all K of them).
 #define LW_LDS 2048
+#define LW_U 8
 __global__ __launch_bounds__(WG) void k_loewner(const Merge *mg, const int *chunk_start, int nm,
                                                 const MergeOut *mo, Work W)
 {
@@ -1395,16 +1396,31 @@ __global__ __launch_bounds__(WG) void k_loewner(const Merge *mg, const int *chun
             }
             __syncthreads();
             if (own) {
-                if (!o.hss) {
-                    for (int jj = 0; jj < cn; jj++) {
-                        if (c0 + jj == i) continue;
-                        wv = wv * (((di - so[jj]) - st[jj]) / (di - sd[jj]));
+                // factor of pole J: the quotient, or at J = I nothing (dense: times 1.0,
+                // which leaves the product's bits as they are) or the numerator (HSS).
+                // LW_U quotients are formed independently, then multiplied into the
+                // product in J order: a level launches one thread per root, too few
+                // wavefronts to hide the divide and LDS latency of one pole at a time.
+                // The divide's operands are selected, not its result (1.0 / 1.0 and
+                // num / 1.0 are exact), so every pole runs the same straight-line code.
+                const int me = i - c0;
+                const bool hss = o.hss;
+                int jj = 0;
+                for (; jj + LW_U <= cn; jj += LW_U) {
+                    double f[LW_U];
+#pragma unroll
+                    for (int u = 0; u < LW_U; u++) {
+                        const bool self = jj + u == me;
+                        const double num = (di - so[jj + u]) - st[jj + u], den = di - sd[jj + u];
+                        f[u] = ((self && !hss) ? 1.0 : num) / (self ? 1.0 : den);
                     }
-                } else {
-                    for (int jj = 0; jj < cn; jj++) {
-                        if (c0 + jj == i) wv = wv * ((di - so[jj]) - st[jj]);
-                        else wv = wv * (((di - so[jj]) - st[jj]) / (di - sd[jj]));
-                    }
+#pragma unroll
+                    for (int u = 0; u < LW_U; u++) wv = wv * f[u];
+                }
+                for (; jj < cn; jj++) {
+                    const bool self = jj == me;
+                    const double num = (di - so[jj]) - st[jj], den = di - sd[jj];
+                    wv = wv * (((self && !hss) ? 1.0 : num) / (self ? 1.0 : den));
                 }
             }
             __syncthreads();  // the next chunk or virtual block restages
@@ -1508,15 +1524,31 @@ __global__ __launch_bounds__(WG) void k_alpha(const Merge *mg, const int *chunk_
             const int cn = min(LW_LDS, K - c0);
             for (int t = threadIdx.x; t < cn; t += WG) { sd[t] = d[c0 + t]; sw[t] = wz[c0 + t]; }
             __syncthreads();
-            if (own)
-                for (int ii = 0; ii < cn; ii++) {
+            // column entry i = j: wz_j / difl; i < j and i > j: the differences from
+            // lambda_j's two neighbouring poles.  LW_U entries are formed
+            // independently, then summed in i order (one thread per column, so few
+            // wavefronts per level that one entry at a time leaves the latency bare).
+            if (own) {
+                int ii = 0;
+                for (; ii + LW_U <= cn; ii += LW_U) {
+                    double v[LW_U];
+#pragma unroll
+                    for (int u = 0; u < LW_U; u++) {
+                        const int i = c0 + ii + u;
+                        const double dlo = (sd[ii + u] + dlamj) - difl, dhi = (sd[ii + u] + dlamjp) + difrj;
+                        v[u] = ((i == j) ? wzj : sw[ii + u]) / ((i == j) ? difl : (i < j) ? dlo : dhi);
+                    }
+#pragma unroll
+                    for (int u = 0; u < LW_U; u++) acc += v[u] * v[u];
+                }
+                for (; ii < cn; ii++) {
                     const int i = c0 + ii;
-                    double v;
-                    if (i == j) v = wzj / difl;
-                    else if (i < j) v = sw[ii] / ((sd[ii] + dlamj) - difl);
-                    else v = sw[ii] / ((sd[ii] + dlamjp) + difrj);
+                    const double den = (i == j) ? difl
+                                     : (i < j) ? (sd[ii] + dlamj) - difl : (sd[ii] + dlamjp) + difrj;
+                    const double v = ((i == j) ? wzj : sw[ii]) / den;
                     acc += v * v;
                 }
+            }
             __syncthreads();  // the next chunk or virtual block restages
         }
         if (own) W.alpha[s + j] = 1.0 / sqrt(acc);

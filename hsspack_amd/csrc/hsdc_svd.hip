@@ -969,25 +969,62 @@ __global__ __launch_bounds__(256) void k_bd_secular(const BdMerge *mg, int nmg, 
 }
 
 // ------------------------------------------------------------------------------
-// updated z (DLASD3, Loewner form): thread per index, products in DLASD3's order
+// updated z (DLASD3, Loewner form): thread per index, products in DLASD3's order.
+// DLASD3's two loops (J < I with pole D(J), J >= I with D(J+1)) are one product over
+// J = 1 .. K-1 ascending; each lane picks the pole by its own I.  The block stages
+// chunks of (D(ORG(J)), TAU(J), D(J)) in LDS and forms BL_U factors at a time before
+// multiplying them in order: a level launches one thread per index, too few
+// wavefronts to hide the latency of one factor at a time.
 // ------------------------------------------------------------------------------
+#define BL_LDS 2048
+#define BL_U 8
 __global__ __launch_bounds__(256) void k_bd_loewner(const BdMerge *mg, int nmg, BdTab tab, BdWork W, const BdOut *mo)
 {
+  __shared__ double so[BL_LDS], st[BL_LDS], sdd[BL_LDS + 1];
   const int tot = tab.low[nmg];
   for (int b = blockIdx.x; b < tot; b += gridDim.x) {
-    const int mi = bd_find(tab.low, nmg, b);
+    const int mi = bd_find(tab.low, nmg, b);  // uniform over the block
     const int s = mg[mi].s, k = mo[mi].k;
     const int i = (b - tab.low[mi]) * 256 + threadIdx.x + 1;  // 1-based
-    if (i > k) continue;
+    const bool own = i <= k;
     const double *ds = W.dsg + s;
-    const double di = ds[i - 1];
-    double zi = sv_den(di, ds[W.org[s + k - 1] - 1], W.tau[s + k - 1]);
-    for (int j = 1; j <= i - 1; j++)
-        zi = zi * (sv_den(di, ds[W.org[s + j - 1] - 1], W.tau[s + j - 1]) / (di - ds[j - 1]) / (di + ds[j - 1]));
-    for (int j = i; j <= k - 1; j++)
-        zi = zi * (sv_den(di, ds[W.org[s + j - 1] - 1], W.tau[s + j - 1]) / (di - ds[j]) / (di + ds[j]));
-    double a = sqrt(fabs(zi));
-    W.zl[s + i - 1] = W.zs[s + i - 1] >= 0.0 ? a : -a;
+    const double di = own ? ds[i - 1] : 0.0;
+    double zi = own ? sv_den(di, ds[W.org[s + k - 1] - 1], W.tau[s + k - 1]) : 1.0;
+    for (int c0 = 1; c0 <= k - 1; c0 += BL_LDS) {
+      const int cn = min(BL_LDS, k - c0);  // J = c0 .. c0 + cn - 1
+      for (int t = threadIdx.x; t < cn; t += 256) {
+        const int j = c0 + t;
+        so[t] = ds[W.org[s + j - 1] - 1];
+        st[t] = W.tau[s + j - 1];
+        sdd[t] = ds[j - 1];
+      }
+      if (threadIdx.x == 0) sdd[cn] = ds[c0 + cn - 1];
+      __syncthreads();
+      if (own) {
+        // pole of J: D(J) = sdd[t] below I, D(J+1) = sdd[t + 1] from I on
+        const int sw = i - c0;  // first t with J >= I
+        int t = 0;
+        for (; t + BL_U <= cn; t += BL_U) {
+          double f[BL_U];
+#pragma unroll
+          for (int u = 0; u < BL_U; u++) {
+            const double dd = (t + u < sw) ? sdd[t + u] : sdd[t + u + 1];
+            f[u] = sv_den(di, so[t + u], st[t + u]) / (di - dd) / (di + dd);
+          }
+#pragma unroll
+          for (int u = 0; u < BL_U; u++) zi = zi * f[u];
+        }
+        for (; t < cn; t++) {
+          const double dd = (t < sw) ? sdd[t] : sdd[t + 1];
+          zi = zi * (sv_den(di, so[t], st[t]) / (di - dd) / (di + dd));
+        }
+      }
+      __syncthreads();  // the next chunk or block restages
+    }
+    if (own) {
+      const double a = sqrt(fabs(zi));
+      W.zl[s + i - 1] = W.zs[s + i - 1] >= 0.0 ? a : -a;
+    }
   }
 }
 

@@ -21,8 +21,9 @@
 // columns through the deflation's permutation into staging), k_bd_newcol (the
 // columns formed without a product), k_bd_scatter (staging -> freed slots) and
 // k_bd_post (unscaling, DLAMRG (mdlasd1.f90:282-286), the new maps and supports).
-// U and VT are cleared on a second stream while the levels run; k_bd_fin then
-// writes the supported rows of every column in MDBDSDC's final order.
+// U and VT are cleared on a second stream while the levels run; k_bd_fin (U) and
+// k_bd_fin_vt (VT, transposed through LDS tiles) then write the supported rows of
+// every vector in MDBDSDC's final order.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <climits>
@@ -1245,14 +1246,14 @@ __global__ __launch_bounds__(256) void k_bd_xgather(const BdMerge *mg, const BdH
     const BdOut &o = mo[h.mi];
     const int s = M.s, n = M.nl + M.nr + 1, m = n + M.sqre, k = o.k;
     double *XU = X + h.xoff, *XV = XU + (long)n * k;
-    const long tot = (long)(n + m) * k;
-    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
-        const bool isV = e >= (long)n * k;
-        const long f = isV ? e - (long)n * k : e;
-        const int rows = isV ? m : n;
-        const int r = (int)(f % rows), J = (int)(f / rows);
+    // one output column per pass of the block: columns 0 .. k-1 of XU, then of XV
+    for (int cb = blockIdx.x; cb < 2 * k; cb += gridDim.x) {
+        const bool isV = cb >= k;
+        const int J = isV ? cb - k : cb, rows = isV ? m : n;
         const int c = J == 0 ? 0 : W.idxc[s + J] - 1;
-        (isV ? XV : XU)[(long)c * rows + r] = bd_acol(W, M, o, isV, isV ? V : U, ld, J, r);
+        double *dst = (isV ? XV : XU) + (long)c * rows;
+        const double *src = isV ? V : U;
+        for (int r = threadIdx.x; r < rows; r += 256) dst[r] = bd_acol(W, M, o, isV, src, ld, J, r);
     }
 }
 
@@ -1431,18 +1432,51 @@ __global__ __launch_bounds__(256) void k_bd_zero(double *A, long lda, int n)
 }
 
 __global__ __launch_bounds__(256) void k_bd_fin(const BdWork W, const double *Us, const double *Vs, long ld,
-                                                const int *perm, int n, double *U, long ldu, double *VT, long ldvt)
+                                                const int *perm, int n, double *U, long ldu)
 {
     const int i = blockIdx.x;
     if (i >= n) return;
-    const int src = perm[i];
-    if (U) {
-        const int su = W.vmU[0][src], lo = W.sulo[su], hi = W.suhi[su];
-        for (int r = lo + threadIdx.x; r < hi; r += 256) U[(long)i * ldu + r] = Us[(long)su * ld + r];
+    const int su = W.vmU[0][perm[i]], lo = W.sulo[su], hi = W.suhi[su];
+    for (int r = lo + threadIdx.x; r < hi; r += 256) U[(long)i * ldu + r] = Us[(long)su * ld + r];
+}
+
+// VT(i, r) = V(r, slot of i) for r inside the support of i (the rest of VT is already
+// zero): 64 x 64 tiles through LDS, read down V's columns and written along VT's
+// columns, both coalesced (a thread per r would store with a stride of ldvt); a tile
+// that meets no support exits after its 64 lookups.
+__global__ __launch_bounds__(256) void k_bd_fin_vt(const BdWork W, const double *Vs, long ld, const int *perm, int n,
+                                                   double *VT, long ldvt)
+{
+    __shared__ double t[64][65];
+    __shared__ long sb[64];
+    __shared__ int slo[64], shi[64], any;
+    const int i0 = blockIdx.x * 64, r0 = blockIdx.y * 64, tid = threadIdx.x;
+    if (tid == 0) any = 0;
+    __syncthreads();
+    if (tid < 64) {
+        const int i = i0 + tid;
+        int lo = 0, hi = 0;
+        long b = 0;
+        if (i < n) {
+            const int sv = W.vmV[0][perm[i]];
+            lo = max(W.svlo[sv], r0);
+            hi = min(W.svhi[sv], min(r0 + 64, n));
+            b = (long)sv * ld;
+        }
+        slo[tid] = lo; shi[tid] = hi; sb[tid] = b;
+        if (lo < hi) any = 1;
     }
-    if (VT) {
-        const int sv = W.vmV[0][src], lo = W.svlo[sv], hi = W.svhi[sv];
-        for (int r = lo + threadIdx.x; r < hi; r += 256) VT[(long)r * ldvt + i] = Vs[(long)sv * ld + r];
+    __syncthreads();
+    if (!any) return;
+    const int q = tid & 63;
+    for (int c = tid >> 6; c < 64; c += 4) {  // column i0 + c of the tile, row r0 + q
+        const int r = r0 + q;
+        if (r >= slo[c] && r < shi[c]) t[c][q] = Vs[sb[c] + r];
+    }
+    __syncthreads();
+    for (int rr = tid >> 6; rr < 64; rr += 4) {  // row r0 + rr of V, VT entry (i0 + q, r0 + rr)
+        const int r = r0 + rr;
+        if (r >= slo[q] && r < shi[q]) VT[(long)r * ldvt + i0 + q] = t[q][rr];
     }
 }
 
@@ -1727,7 +1761,7 @@ static int bd_hss_level(hsdc_bd_ctx *c, const BdMerge *mg, const BdOut *mo, cons
 {
     hipStream_t st = c->stream;
     const int nh = (int)hs.size();
-    long gtot = 0, xtot = 0, gx = 1, xmax = 1;
+    long gtot = 0, xtot = 0, gx = 1;
     int kmax = 1;
     for (const BdHss &h : hs) {
         const BdMerge &M = L[h.mi];
@@ -1737,7 +1771,6 @@ static int bd_hss_level(hsdc_bd_ctx *c, const BdMerge *mg, const BdOut *mo, cons
         kmax = std::max(kmax, k);
         const int T = sec_team(k);
         gx = std::max(gx, (long)((k + 256 / T - 1) / (256 / T)));
-        xmax = std::max(xmax, (long)(n_ + m_) * k);
     }
     BCK(bensure(c->hgen, sizeof(double) * (size_t)gtot));
     BCK(bensure(c->hx, sizeof(double) * (size_t)xtot));
@@ -1748,7 +1781,7 @@ static int bd_hss_level(hsdc_bd_ctx *c, const BdMerge *mg, const BdOut *mo, cons
     BHK(hipMemcpyAsync(dh, hs.data(), sizeof(BdHss) * nh, hipMemcpyHostToDevice, st));
     BHK(hipStreamSynchronize(st));  // hs is pageable host memory owned by the caller
     hipLaunchKernelGGL(k_bd_hssgen, dim3((unsigned)gx, nh), dim3(256), 0, st, mg, (const BdHss *)dh, c->W, mo, G);
-    hipLaunchKernelGGL(k_bd_xgather, dim3((unsigned)std::min(bcdiv(xmax, 256), 4096), nh), dim3(256), 0, st, mg,
+    hipLaunchKernelGGL(k_bd_xgather, dim3((unsigned)std::min(2L * kmax, 8192L), nh), dim3(256), 0, st, mg,
                        (const BdHss *)dh, c->W, mo, U, V, ld, X);
     BHK(hipGetLastError());
     // staging offsets of the level (k_bd_lvltab's prefix, from the K read back)
@@ -2033,8 +2066,12 @@ static int bd_solve(hsdc_bd_ctx *c, int n, const std::vector<double> &he, double
         } else {
             BHK(hipStreamWaitEvent(st, c->ev[3], 0));
         }
-        hipLaunchKernelGGL(k_bd_fin, dim3(n), dim3(256), 0, st, W, (const double *)U, (const double *)V, ld,
-                           (const int *)dperm, n, dU, ldu, dVT, ldvt);
+        if (dU)
+            hipLaunchKernelGGL(k_bd_fin, dim3(n), dim3(256), 0, st, W, (const double *)U, (const double *)V, ld,
+                               (const int *)dperm, n, dU, ldu);
+        if (dVT)
+            hipLaunchKernelGGL(k_bd_fin_vt, dim3(bcdiv(n, 64), bcdiv(n, 64)), dim3(256), 0, st, W, (const double *)V,
+                               ld, (const int *)dperm, n, dVT, ldvt);
         BHK(hipGetLastError());
         if (rc && dU) {
             BCK(bensure(c->rot, sizeof(double) * 2 * (size_t)n));

@@ -247,7 +247,9 @@ def bench_tridiag(args, np, torch, hsdc, cfg, n, seed, tol, d, e, steps, warmup,
                                     "frac_of_hbm": rate(b, acc["ms_loewner"]) / HBM_PEAK_GBS,
                                     "pairs": sp["loewner_pairs"],
                                     "unit_note": "16 B per (i, j) operand pair, K^2 per merge for the products, "
-                                                 "K^2 more per HSS merge for alpha"}
+                                                 "K^2 more per HSS merge for alpha; the operands are staged in LDS, "
+                                                 "so this is an operand rate in HBM units, not HBM traffic: the "
+                                                 "sweeps are bound by the FP64 divides of one thread per root"}
     if sp["hss_mm_flops"] and acc["ms_hss_mm"] > 0:
         tf = sp["hss_mm_flops"] / (acc["ms_hss_mm"] * 1e-3) / 1e12
         hot["k_bgemm"] = {"flops": sp["hss_mm_flops"], "ms": acc["ms_hss_mm"], "TFLOPs": tf,

@@ -240,13 +240,15 @@ __global__ __launch_bounds__(WG) void k_hss_dgen(const HssMerge *hm, const HssNo
     if (threadIdx.x == 0) H.dsz[N.node - 1] = m;
 }
 
-__device__ long arena_take(const HssMerge &H, long n)
+// inlined: a call would give the merge record an address, and the compiler would then
+// keep it in scratch memory and reload its fields there inside the pivot steps
+__device__ __forceinline__ long arena_take(const HssMerge &H, long n)
 {
     unsigned long long o = atomicAdd(H.arena_top, (unsigned long long)n);
     if ((long)(o + n) > H.arena_cap) { atomicOr(H.err, 2); return -1; }
     return (long)o;
 }
-__device__ long scratch_take(const HssMerge &H, long n)
+__device__ __forceinline__ long scratch_take(const HssMerge &H, long n)
 {
     unsigned long long o = atomicAdd(H.scratch_top, (unsigned long long)n);
     if ((long)(o + n) > H.scratch_cap) { atomicOr(H.err, 4); return -1; }

@@ -414,20 +414,20 @@ __global__ __launch_bounds__(1024) void k_bd_leaf(const BdLeaf *lv, int nleaf, B
     }
     if (rot && threadIdx.x == 0) atomicAdd(fail, 1);
     if (threadIdx.x == 0) {
-        int used[64];
-        for (int j = 0; j < pc; j++) used[j] = 0;
+        unsigned long long used = 0ULL;  // bit j: column j taken (pc <= 64); a mask, not an
+                                         // array, so nothing goes to scratch memory
         int jn = -1;
         if (L.sqre) {
             jn = 0;
             for (int j = 1; j < pc; j++)
                 if (sq[j] < sq[jn]) jn = j;
-            used[jn] = 1;
+            used |= 1ULL << jn;
         }
         for (int k = 0; k < m; k++) {
             int b = -1;
             for (int j = 0; j < pc; j++)
-                if (!used[j] && (b < 0 || sq[j] < sq[b])) b = j;
-            used[b] = 1;
+                if (!((used >> j) & 1ULL) && (b < 0 || sq[j] < sq[b])) b = j;
+            used |= 1ULL << b;
             ord[k] = b;
         }
         jn_s = jn;

@@ -1275,7 +1275,9 @@ __global__ __launch_bounds__(WG) void k_hss_compact(const HssMerge *hm, const Hs
     const int tid = threadIdx.x;
     int *rng = rows ? H.lr : H.lc;
     if (*H.err) return;
-    if (C.count > 1024) {  // more nodes than the slice table holds: the merge goes dense
+    // the host builds trees of at most 512 leaves, so a compaction lists at most 1024
+    // nodes; a larger one would not fit the slice table and sends the merge dense
+    if (C.count > 1024) {
         if (tid == 0) atomicOr(H.err, 1);
         return;
     }
